@@ -1,0 +1,505 @@
+"""Python mirror of the reference BEM operator surface over the C ABI.
+
+Mirrors modules/bem/include/hornsculpt/bem.hpp: ``AcousticMedium`` (:27-30),
+``FaceVelocity`` (:32-36), ``BemExcitation`` (:38-41), ``BemNumerics``
+(:113-123), ``FarFieldObservation`` (:125-129), ``BemRequest`` (:163-191),
+``FaceImpedance`` (:193-199), ``BemBoundaryData`` (:203-214), the per-excitation
+and per-frequency results (:231-317) and ``CudaBemSolver`` (:367-388), here
+``HipBemSolver`` (with ``CudaBemSolver`` as an alias for drop-in use).
+
+Errors follow the reference: ``ValueError`` where it throws
+std::invalid_argument, ``RuntimeError`` where it throws std::runtime_error.
+The native library is required; there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import threading
+from pathlib import Path
+from typing import Callable, List, Optional, Sequence
+
+import numpy as np
+
+_LIB_PATH = Path(__file__).resolve().parent / "libhsbem.so"
+
+HSBEM_OK, HSBEM_INVALID_ARGUMENT, HSBEM_RUNTIME_ERROR, HSBEM_DEVICE_ERROR, HSBEM_CANCELLED = range(5)
+SOLVER_LU, SOLVER_GMRES = 0, 1
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+WALL, DRIVEN = 0, 1
+
+
+class _Numerics(ctypes.Structure):
+    _fields_ = [
+        ("quadrature_order", ctypes.c_uint32),
+        ("singular_order", ctypes.c_uint32),
+        ("near_distance_ratio", ctypes.c_double),
+        ("near_relative_tolerance", ctypes.c_double),
+        ("near_maximum_depth", ctypes.c_uint32),
+        ("relative_residual_tolerance", ctypes.c_double),
+        ("maximum_refinement_steps", ctypes.c_uint32),
+        ("solver", ctypes.c_uint32),
+        ("gmres_restart", ctypes.c_uint32),
+        ("gmres_maximum_iterations", ctypes.c_uint32),
+        ("precision", ctypes.c_uint32),
+        ("plan_threads", ctypes.c_uint32),
+    ]
+
+
+class _Request(ctypes.Structure):
+    _fields_ = [
+        ("density", ctypes.c_double),
+        ("sound_speed", ctypes.c_double),
+        ("frequencies", ctypes.POINTER(ctypes.c_double)),
+        ("frequency_count", ctypes.c_uint32),
+        ("excitation_count", ctypes.c_uint32),
+        ("excitation_offsets", ctypes.POINTER(ctypes.c_uint32)),
+        ("face_ids", ctypes.POINTER(ctypes.c_uint32)),
+        ("velocities", ctypes.POINTER(ctypes.c_double)),
+        ("observers", ctypes.POINTER(ctypes.c_double)),
+        ("observer_count", ctypes.c_uint32),
+        ("far_field", ctypes.c_int32),
+        ("far_origin", ctypes.c_double * 3),
+        ("far_reference_distance", ctypes.c_double),
+        ("planning_frequencies", ctypes.POINTER(ctypes.c_double)),
+        ("planning_frequency_count", ctypes.c_uint32),
+        ("numerics", _Numerics),
+    ]
+
+
+_PROGRESS = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_int32)
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def library() -> ctypes.CDLL:
+    """Load libhsbem.so (built in-tree by hornsculpt_amd/build.py); raise if absent."""
+    global _lib
+    with _lib_lock:
+        if _lib is None:
+            if not _LIB_PATH.exists():
+                raise RuntimeError(f"native library {_LIB_PATH} is missing; run hornsculpt_amd/build.py")
+            lib = ctypes.CDLL(str(_LIB_PATH))
+            P, D, U32, U64 = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_uint64
+            sig = {
+                "hsbem_numerics_default": (None, [ctypes.POINTER(_Numerics)]),
+                "hsbem_last_error": (ctypes.c_char_p, []),
+                "hsbem_device_description": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+                "hsbem_context_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(P)]),
+                "hsbem_context_destroy": (ctypes.c_int, [P]),
+                "hsbem_mesh_create": (ctypes.c_int, [P, D, U32, ctypes.POINTER(U32), ctypes.POINTER(U32),
+                                                     ctypes.POINTER(ctypes.c_uint8), U32, ctypes.POINTER(P)]),
+                "hsbem_mesh_destroy": (ctypes.c_int, [P]),
+                "hsbem_mesh_info": (ctypes.c_int, [P, ctypes.POINTER(U64), ctypes.POINTER(U32),
+                                                   ctypes.POINTER(U32), D]),
+                "hsbem_solve": (ctypes.c_int, [P, ctypes.POINTER(_Request), _PROGRESS, P,
+                                               ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(P)]),
+                "hsbem_result_destroy": (ctypes.c_int, [P]),
+                "hsbem_result_sizes": (ctypes.c_int, [P] + [ctypes.POINTER(U32)] * 5),
+                "hsbem_result_surface_pressure": (ctypes.c_int, [P, U32, U32, D]),
+                "hsbem_result_pressure_trace": (ctypes.c_int, [P, U32, U32, D]),
+                "hsbem_result_normal_derivative": (ctypes.c_int, [P, U32, U32, D]),
+                "hsbem_result_observer_pressure": (ctypes.c_int, [P, U32, U32, D]),
+                "hsbem_result_impedances": (ctypes.c_int, [P, U32, U32, ctypes.POINTER(U32), ctypes.POINTER(U32),
+                                                           D, D, D, D]),
+                "hsbem_result_residual": (ctypes.c_int, [P, U32, U32, D, ctypes.POINTER(U32), ctypes.POINTER(U32)]),
+                "hsbem_result_timings": (ctypes.c_int, [P, U32, D]),
+                "hsbem_result_peak_device_bytes": (ctypes.c_int, [P, ctypes.POINTER(U64)]),
+                "hsbem_plan_create": (ctypes.c_int, [P, ctypes.POINTER(_Numerics), ctypes.c_double, ctypes.POINTER(P)]),
+                "hsbem_plan_build": (ctypes.c_int, [D, U32, ctypes.POINTER(U32), U32, ctypes.POINTER(_Numerics),
+                                                    ctypes.c_double, ctypes.POINTER(P)]),
+                "hsbem_plan_sizes": (ctypes.c_int, [P, ctypes.POINTER(U64)]),
+                "hsbem_plan_copy": (ctypes.c_int, [P, ctypes.POINTER(U32), ctypes.POINTER(U32), D,
+                                                   ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(U32),
+                                                   ctypes.POINTER(U32), D]),
+                "hsbem_plan_destroy": (ctypes.c_int, [P]),
+                "hsbem_assemble": (ctypes.c_int, [P, ctypes.POINTER(_Numerics), ctypes.c_double, ctypes.c_double, D, D]),
+                "hsbem_dense_gmres": (ctypes.c_int, [P, U32, D, D, D, U32, U32, ctypes.c_double, D,
+                                                     ctypes.POINTER(U32)]),
+                "hsbem_dense_lu_solve": (ctypes.c_int, [P, U32, D, U32, D]),
+            }
+            for name, (res, args) in sig.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def _check(status: int) -> None:
+    if status == HSBEM_OK:
+        return
+    message = library().hsbem_last_error().decode()
+    if status == HSBEM_INVALID_ARGUMENT:
+        raise ValueError(message)
+    if status == HSBEM_CANCELLED:
+        raise RuntimeError(message)
+    raise RuntimeError(message)
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _uptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+# ---------------------------------------------------------------- request types
+
+
+@dataclasses.dataclass
+class AcousticMedium:
+    density: float = 1.21
+    sound_speed: float = 343.0
+
+
+@dataclasses.dataclass
+class FaceVelocity:
+    face_id: int
+    normal_velocity: complex = 1.0 + 0.0j  # RMS m/s out of the solid
+
+
+@dataclasses.dataclass
+class BemExcitation:
+    name: str
+    faces: List[FaceVelocity]
+
+
+@dataclasses.dataclass
+class BemNumerics:
+    quadrature_order: int = 4
+    singular_order: int = 4
+    near_distance_ratio: float = 1.0
+    near_relative_tolerance: float = 1e-4
+    near_maximum_depth: int = 6
+    relative_residual_tolerance: float = 5e-5
+    maximum_refinement_steps: int = 3
+    solver: int = SOLVER_LU
+    gmres_restart: int = 50
+    gmres_maximum_iterations: int = 1000
+    precision: int = PRECISION_FP64
+    plan_threads: int = 0
+
+    def native(self) -> _Numerics:
+        n = _Numerics()
+        for f, _ in _Numerics._fields_:
+            setattr(n, f, getattr(self, f))
+        return n
+
+
+@dataclasses.dataclass
+class FarFieldObservation:
+    origin: Sequence[float] = (0.0, 0.0, 0.0)
+    reference_distance: float = 1.0
+
+
+@dataclasses.dataclass
+class BemRequest:
+    medium: AcousticMedium = dataclasses.field(default_factory=AcousticMedium)
+    frequencies: List[float] = dataclasses.field(default_factory=list)
+    excitations: List[BemExcitation] = dataclasses.field(default_factory=list)
+    observers: List[Sequence[float]] = dataclasses.field(default_factory=list)
+    numerics: BemNumerics = dataclasses.field(default_factory=BemNumerics)
+    far_field: Optional[FarFieldObservation] = None
+    interaction_plan_frequencies: List[float] = dataclasses.field(default_factory=list)
+
+
+# ---------------------------------------------------------------- result types
+
+
+@dataclasses.dataclass
+class FaceImpedance:
+    face_id: int
+    area: float
+    mean_pressure: complex
+    volume_velocity: complex
+    volume_impedance: complex
+
+
+@dataclasses.dataclass
+class BemBoundaryData:
+    surface_pressure: np.ndarray   # DP0 triangle means of the P1 pressure
+    pressure_trace: np.ndarray     # P1 pressure per mesh vertex
+    normal_derivative: np.ndarray  # q = -i rho omega v per triangle
+
+
+@dataclasses.dataclass
+class BemExcitationResult:
+    name: str
+    observer_pressure: np.ndarray
+    impedances: List[FaceImpedance]
+    boundary_residual: float
+    iterations: int
+    refinement_steps: int
+    boundary: BemBoundaryData
+
+
+@dataclasses.dataclass
+class BemFrequencyResult:
+    frequency: float
+    excitations: List[BemExcitationResult]
+    assembly_milliseconds: float
+    solve_milliseconds: float
+    factorization_milliseconds: float
+    triangular_solve_milliseconds: float
+    field_milliseconds: float
+    interaction_plan_milliseconds: float
+    kernel_milliseconds: dict
+
+
+@dataclasses.dataclass
+class BemResult:
+    device: str
+    triangle_count: int
+    pressure_dof_count: int
+    peak_device_bytes: int
+    frequencies: List[BemFrequencyResult]
+
+
+# ---------------------------------------------------------------- solver
+
+
+class Context:
+    """One HIP device and stream (process-wide default via ``Context.default()``)."""
+
+    _default = None
+
+    def __init__(self, device: int = 0):
+        lib = library()
+        handle = ctypes.c_void_p()
+        _check(lib.hsbem_context_create(device, ctypes.byref(handle)))
+        self.handle = handle
+
+    @classmethod
+    def default(cls) -> "Context":
+        if cls._default is None:
+            cls._default = Context(0)
+        return cls._default
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.hsbem_context_destroy(self.handle)
+            self.handle = None
+
+
+@dataclasses.dataclass
+class SurfaceMesh:
+    """Mirror of hornsculpt::SurfaceMesh (modules/core/include/hornsculpt/mesh.hpp:10-33)."""
+    vertices: np.ndarray          # (nv, 3) float64, metres
+    triangles: np.ndarray         # (nt, 3) uint32
+    face_ids: np.ndarray          # (nt,) uint32
+    boundary: np.ndarray          # (nt,) uint8, WALL or DRIVEN
+
+
+def device_description() -> str:
+    buf = ctypes.create_string_buffer(256)
+    _check(library().hsbem_device_description(buf, 256))
+    return buf.value.decode()
+
+
+class HipBemSolver:
+    """Drop-in for hornsculpt::CudaBemSolver (bem.hpp:367-388) on one MI355X."""
+
+    def __init__(self, mesh: SurfaceMesh, context: Optional[Context] = None):
+        self._lib = library()
+        self.context = context or Context.default()
+        self.mesh = mesh
+        v = np.ascontiguousarray(mesh.vertices, dtype=np.float64)
+        t = np.ascontiguousarray(mesh.triangles, dtype=np.uint32)
+        f = np.ascontiguousarray(mesh.face_ids, dtype=np.uint32)
+        b = np.ascontiguousarray(mesh.boundary, dtype=np.uint8)
+        handle = ctypes.c_void_p()
+        _check(self._lib.hsbem_mesh_create(self.context.handle, _dptr(v), len(v), _uptr(t), _uptr(f),
+                                           b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), len(t),
+                                           ctypes.byref(handle)))
+        self.handle = handle
+        h, nv, nt, area = ctypes.c_uint64(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_double()
+        _check(self._lib.hsbem_mesh_info(handle, ctypes.byref(h), ctypes.byref(nv), ctypes.byref(nt),
+                                         ctypes.byref(area)))
+        self.mesh_hash, self.vertex_count, self.triangle_count = h.value, nv.value, nt.value
+        self.surface_area = area.value
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.hsbem_mesh_destroy(self.handle)
+            self.handle = None
+
+    @staticmethod
+    def device_description() -> str:
+        return device_description()
+
+    def solve(self, request: BemRequest, cancel: Optional[ctypes.c_int32] = None,
+              progress: Optional[Callable[[int, int, float, int], None]] = None) -> BemResult:
+        lib = self._lib
+        freqs = np.ascontiguousarray(request.frequencies, dtype=np.float64)
+        plan_freqs = np.ascontiguousarray(request.interaction_plan_frequencies, dtype=np.float64)
+        offsets = [0]
+        face_ids, vel = [], []
+        for ex in request.excitations:
+            for fv in ex.faces:
+                face_ids.append(fv.face_id)
+                vel += [complex(fv.normal_velocity).real, complex(fv.normal_velocity).imag]
+            offsets.append(len(face_ids))
+        offsets = np.asarray(offsets, dtype=np.uint32)
+        face_ids = np.asarray(face_ids if face_ids else [0], dtype=np.uint32)
+        vel = np.asarray(vel if vel else [0.0, 0.0], dtype=np.float64)
+        obs = np.ascontiguousarray(np.asarray(request.observers, dtype=np.float64).reshape(-1, 3))
+        req = _Request()
+        req.density = request.medium.density
+        req.sound_speed = request.medium.sound_speed
+        req.frequencies = _dptr(freqs)
+        req.frequency_count = len(freqs)
+        req.excitation_count = len(request.excitations)
+        req.excitation_offsets = _uptr(offsets)
+        req.face_ids = _uptr(face_ids)
+        req.velocities = _dptr(vel)
+        req.observers = _dptr(obs) if len(obs) else ctypes.POINTER(ctypes.c_double)()
+        req.observer_count = len(obs)
+        req.far_field = 1 if request.far_field is not None else 0
+        if request.far_field is not None:
+            for i in range(3):
+                req.far_origin[i] = float(request.far_field.origin[i])
+            req.far_reference_distance = request.far_field.reference_distance
+        else:
+            req.far_reference_distance = 1.0
+        req.planning_frequencies = _dptr(plan_freqs) if len(plan_freqs) else ctypes.POINTER(ctypes.c_double)()
+        req.planning_frequency_count = len(plan_freqs)
+        req.numerics = request.numerics.native()
+        cb = _PROGRESS(lambda user, done, total, f, phase: progress(done, total, f, phase)) if progress else \
+            ctypes.cast(None, _PROGRESS)
+        out = ctypes.c_void_p()
+        cancel_ptr = ctypes.pointer(cancel) if cancel is not None else ctypes.POINTER(ctypes.c_int32)()
+        _check(lib.hsbem_solve(self.handle, ctypes.byref(req), cb, None, cancel_ptr, ctypes.byref(out)))
+        try:
+            return self._collect(out, request)
+        finally:
+            lib.hsbem_result_destroy(out)
+
+    def _collect(self, handle, request: BemRequest) -> BemResult:
+        lib = self._lib
+        nf, ne, nt, nv, no = (ctypes.c_uint32() for _ in range(5))
+        _check(lib.hsbem_result_sizes(handle, *(ctypes.byref(x) for x in (nf, ne, nt, nv, no))))
+        peak = ctypes.c_uint64()
+        _check(lib.hsbem_result_peak_device_bytes(handle, ctypes.byref(peak)))
+        freqs = []
+        for f in range(nf.value):
+            ms = np.zeros(9)
+            _check(lib.hsbem_result_timings(handle, f, _dptr(ms)))
+            exs = []
+            for e in range(ne.value):
+                def cvec(fn, n):
+                    a = np.zeros(2 * n)
+                    _check(fn(handle, f, e, _dptr(a)))
+                    return a.view(np.complex128)
+                surface = cvec(lib.hsbem_result_surface_pressure, nt.value)
+                trace = cvec(lib.hsbem_result_pressure_trace, nv.value)
+                deriv = cvec(lib.hsbem_result_normal_derivative, nt.value)
+                observed = cvec(lib.hsbem_result_observer_pressure, no.value)
+                count = ctypes.c_uint32()
+                _check(lib.hsbem_result_impedances(handle, f, e, ctypes.byref(count), None, None, None, None, None))
+                c = count.value
+                ids = np.zeros(max(c, 1), np.uint32)
+                area, mp, vv, z = np.zeros(max(c, 1)), np.zeros(2 * max(c, 1)), np.zeros(2 * max(c, 1)), \
+                    np.zeros(2 * max(c, 1))
+                _check(lib.hsbem_result_impedances(handle, f, e, ctypes.byref(count), _uptr(ids), _dptr(area),
+                                                   _dptr(mp), _dptr(vv), _dptr(z)))
+                imps = [FaceImpedance(int(ids[i]), float(area[i]), complex(mp[2 * i], mp[2 * i + 1]),
+                                      complex(vv[2 * i], vv[2 * i + 1]), complex(z[2 * i], z[2 * i + 1]))
+                        for i in range(c)]
+                res, its, refs = ctypes.c_double(), ctypes.c_uint32(), ctypes.c_uint32()
+                _check(lib.hsbem_result_residual(handle, f, e, ctypes.byref(res), ctypes.byref(its),
+                                                 ctypes.byref(refs)))
+                exs.append(BemExcitationResult(request.excitations[e].name, observed, imps, res.value, its.value,
+                                               refs.value, BemBoundaryData(surface, trace, deriv)))
+            freqs.append(BemFrequencyResult(request.frequencies[f], exs, ms[0], ms[1], ms[2], ms[3], ms[4], ms[5],
+                                            {"regular": ms[6], "touching": ms[7], "near": ms[8]}))
+        return BemResult("MI355X (gfx950)", nt.value, nv.value, peak.value, freqs)
+
+    # -------- internal seams (component parity tests and benchmarks)
+
+    def plan(self, numerics: BemNumerics, k_max: float) -> dict:
+        lib = self._lib
+        handle = ctypes.c_void_p()
+        n = numerics.native()
+        _check(lib.hsbem_plan_create(self.handle, ctypes.byref(n), k_max, ctypes.byref(handle)))
+        try:
+            return _copy_plan(lib.hsbem_plan_sizes, lib.hsbem_plan_copy, handle, self.triangle_count)
+        finally:
+            lib.hsbem_plan_destroy(handle)
+
+    def assemble(self, numerics: BemNumerics, k_max: float, k: float):
+        """Row-scaled A (nv x nv) and B (nv x nt) of the device assembly, complex."""
+        nv, nt = self.vertex_count, self.triangle_count
+        A = np.zeros(2 * nv * nv)
+        B = np.zeros(2 * nv * nt)
+        n = numerics.native()
+        _check(self._lib.hsbem_assemble(self.handle, ctypes.byref(n), k_max, k, _dptr(A), _dptr(B)))
+        return (A.view(np.complex128).reshape(nv, nv).T.copy(), B.view(np.complex128).reshape(nt, nv).T.copy())
+
+
+CudaBemSolver = HipBemSolver
+
+
+def _copy_plan(sizes_fn, copy_fn, handle, nt: int) -> dict:
+    sizes = np.zeros(6, np.uint64)
+    _check(sizes_fn(handle, sizes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+    nt_, nn, npatch, nwhole = (int(x) for x in sizes[:4])
+    touching = np.zeros((max(nt_, 1), 9), np.uint32)
+    near = np.zeros((max(nn, 1), 5), np.uint32)
+    patches = np.zeros((max(npatch, 1), 22))
+    whole = np.zeros(max(nwhole, 1), np.uint8)
+    offsets = np.zeros(nt + 1, np.uint32)
+    trials = np.zeros(max(nn, 1), np.uint32)
+    err = ctypes.c_double()
+    _check(copy_fn(handle, _uptr(touching), _uptr(near), _dptr(patches),
+                   whole.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _uptr(offsets), _uptr(trials),
+                   ctypes.byref(err)))
+    return {"touching": touching[:nt_], "near": near[:nn], "patches": patches[:npatch], "whole": whole[:nwhole],
+            "near_offsets": offsets, "near_trials": trials[:nn], "candidate_pairs": int(sizes[4]),
+            "maximum_depth": int(sizes[5]), "maximum_estimated_error": err.value}
+
+
+def interaction_plan(mesh: SurfaceMesh, numerics: BemNumerics, k_max: float) -> dict:
+    """Host interaction plan of a mesh (the galerkin::makeInteractionPlan seam); no device use."""
+    lib = library()
+    v = np.ascontiguousarray(mesh.vertices, dtype=np.float64)
+    t = np.ascontiguousarray(mesh.triangles, dtype=np.uint32)
+    handle = ctypes.c_void_p()
+    n = numerics.native()
+    _check(lib.hsbem_plan_build(_dptr(v), len(v), _uptr(t), len(t), ctypes.byref(n), k_max, ctypes.byref(handle)))
+    try:
+        return _copy_plan(lib.hsbem_plan_sizes, lib.hsbem_plan_copy, handle, len(t))
+    finally:
+        lib.hsbem_plan_destroy(handle)
+
+
+def dense_gmres(A: np.ndarray, b: np.ndarray, restart: int, maximum_iterations: int, tolerance: float,
+                context: Optional[Context] = None):
+    """Device GMRES on a dense complex system (the DeviceGmresSolver seam)."""
+    lib = library()
+    ctx = context or Context.default()
+    n = A.shape[0]
+    Af = np.ascontiguousarray(A.T).astype(np.complex128).view(np.float64).copy()
+    bf = np.ascontiguousarray(b, dtype=np.complex128).view(np.float64).copy()
+    x = np.zeros(2 * n)
+    res, its = ctypes.c_double(), ctypes.c_uint32()
+    _check(lib.hsbem_dense_gmres(ctx.handle, n, _dptr(Af), _dptr(bf), _dptr(x), restart, maximum_iterations,
+                                 tolerance, ctypes.byref(res), ctypes.byref(its)))
+    return x.view(np.complex128), res.value, its.value
+
+
+def dense_lu_solve(A: np.ndarray, B: np.ndarray, context: Optional[Context] = None) -> np.ndarray:
+    """Device partial-pivot complex LU solve of A X = B."""
+    lib = library()
+    ctx = context or Context.default()
+    n = A.shape[0]
+    B = np.asarray(B, dtype=np.complex128)
+    single = B.ndim == 1
+    Bm = B.reshape(n, -1)
+    Af = np.ascontiguousarray(A.T).astype(np.complex128).view(np.float64).copy()
+    Bf = np.ascontiguousarray(Bm.T).view(np.float64).copy()
+    _check(lib.hsbem_dense_lu_solve(ctx.handle, n, _dptr(Af), Bm.shape[1], _dptr(Bf)))
+    X = Bf.view(np.complex128).reshape(Bm.shape[1], n).T
+    return X[:, 0].copy() if single else X.copy()

@@ -1,0 +1,477 @@
+// C ABI (include/hsbem.h) over MeshSolver. Exceptions map to status codes the
+// way the reference throws them (std::invalid_argument / std::runtime_error).
+#include "../../include/hsbem.h"
+
+#include "solver.hpp"
+
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+
+struct hsbem_context {
+    hsbem::Context ctx;
+};
+struct hsbem_mesh {
+    hsbem_context *context;
+    std::unique_ptr<hsbem::MeshSolver> solver;
+    std::uint64_t hash;
+};
+struct hsbem_result {
+    hsbem::Result result;
+};
+struct hsbem_plan {
+    hsbem::InteractionPlan plan;
+};
+
+namespace {
+thread_local std::string last_error;
+
+template <class F> int guarded(F &&f) {
+    try {
+        f();
+        return HSBEM_OK;
+    } catch (const hsbem::Cancelled &e) {
+        last_error = e.what();
+        return HSBEM_CANCELLED;
+    } catch (const hsbem::DeviceError &e) {
+        last_error = e.what();
+        return HSBEM_DEVICE_ERROR;
+    } catch (const std::invalid_argument &e) {
+        last_error = e.what();
+        return HSBEM_INVALID_ARGUMENT;
+    } catch (const std::exception &e) {
+        last_error = e.what();
+        return HSBEM_RUNTIME_ERROR;
+    } catch (...) {
+        last_error = "unknown error";
+        return HSBEM_RUNTIME_ERROR;
+    }
+}
+
+void require(bool ok, const char *what) {
+    if (!ok)
+        throw std::invalid_argument(what);
+}
+
+hsbem::Numerics to_numerics(const hsbem_numerics &n) {
+    hsbem::Numerics o;
+    o.quadrature_order = n.quadrature_order;
+    o.singular_order = n.singular_order;
+    o.near_ratio = n.near_distance_ratio;
+    o.near_tolerance = n.near_relative_tolerance;
+    o.near_depth = n.near_maximum_depth;
+    o.residual_tolerance = n.relative_residual_tolerance;
+    o.refinement_steps = n.maximum_refinement_steps;
+    o.solver = n.solver;
+    o.gmres_restart = n.gmres_restart;
+    o.gmres_max_iterations = n.gmres_maximum_iterations;
+    o.precision = n.precision;
+    o.plan_threads = n.plan_threads;
+    return o;
+}
+
+const hsbem::ExcitationResult &excitation(const hsbem_result *r, std::uint32_t f, std::uint32_t e) {
+    require(r && f < r->result.frequencies.size() && e < r->result.frequencies[f].excitations.size(),
+            "result index out of range");
+    return r->result.frequencies[f].excitations[e];
+}
+
+void copy_complex(const std::vector<std::complex<double>> &v, double *out) {
+    for (std::size_t i = 0; i < v.size(); ++i) {
+        out[2 * i] = v[i].real();
+        out[2 * i + 1] = v[i].imag();
+    }
+}
+
+std::uint64_t fnv1a(std::uint64_t h, const void *data, std::size_t bytes) {
+    const auto *p = static_cast<const unsigned char *>(data);
+    for (std::size_t i = 0; i < bytes; ++i) {
+        h ^= p[i];
+        h *= 1099511628211ULL;
+    }
+    return h;
+}
+} // namespace
+
+extern "C" {
+
+void hsbem_numerics_default(hsbem_numerics *n) {
+    if (!n)
+        return;
+    n->quadrature_order = 4;
+    n->singular_order = 4;
+    n->near_distance_ratio = 1.0;
+    n->near_relative_tolerance = 1e-4;
+    n->near_maximum_depth = 6;
+    n->relative_residual_tolerance = 5e-5;
+    n->maximum_refinement_steps = 3;
+    n->solver = HSBEM_SOLVER_LU;
+    n->gmres_restart = 50;
+    n->gmres_maximum_iterations = 1000;
+    n->precision = HSBEM_PRECISION_FP64;
+    n->plan_threads = 0;
+}
+
+const char *hsbem_last_error(void) { return last_error.c_str(); }
+
+int hsbem_device_description(char *buffer, size_t length) {
+    return guarded([&] {
+        require(buffer && length, "null buffer");
+        int device = 0;
+        hsbem::hip_check(hipGetDevice(&device), "device query");
+        hipDeviceProp_t prop{};
+        hsbem::hip_check(hipGetDeviceProperties(&prop, device), "device properties");
+        const std::string s = std::string(prop.name) + " (" + prop.gcnArchName + ", " +
+                              std::to_string(prop.multiProcessorCount) + " CUs, " +
+                              std::to_string(prop.totalGlobalMem >> 20) + " MiB)";
+        std::strncpy(buffer, s.c_str(), length - 1);
+        buffer[length - 1] = 0;
+    });
+}
+
+int hsbem_context_create(int32_t device, hsbem_context **out) {
+    return guarded([&] {
+        require(out, "null context output");
+        auto c = std::make_unique<hsbem_context>();
+        c->ctx.device = device;
+        hsbem::hip_check(hipSetDevice(device), "set device");
+        hsbem::hip_check(hipStreamCreateWithFlags(&c->ctx.stream, hipStreamNonBlocking), "stream create");
+        *out = c.release();
+    });
+}
+
+int hsbem_context_destroy(hsbem_context *c) {
+    return guarded([&] {
+        if (!c)
+            return;
+        (void)hipStreamSynchronize(c->ctx.stream);
+        (void)hipStreamDestroy(c->ctx.stream);
+        delete c;
+    });
+}
+
+int hsbem_mesh_create(hsbem_context *c, const double *vertices, uint32_t nv, const uint32_t *triangles,
+                      const uint32_t *face_ids, const uint8_t *boundary, uint32_t nt, hsbem_mesh **out) {
+    return guarded([&] {
+        require(c && out && vertices && triangles && nv && nt, "invalid mesh arguments");
+        hsbem::SurfaceMesh m;
+        m.vertices.resize(nv);
+        for (uint32_t i = 0; i < nv; ++i)
+            m.vertices[i] = {vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2]};
+        m.triangles.resize(nt);
+        for (uint32_t t = 0; t < nt; ++t) {
+            m.triangles[t].vertices = {triangles[3 * t], triangles[3 * t + 1], triangles[3 * t + 2]};
+            m.triangles[t].face_id = face_ids ? face_ids[t] : 0;
+            m.triangles[t].boundary =
+                boundary && boundary[t] == HSBEM_BOUNDARY_DRIVEN ? hsbem::Boundary::driven : hsbem::Boundary::wall;
+        }
+        std::uint64_t h = 14695981039346656037ULL;
+        h = fnv1a(h, vertices, sizeof(double) * 3 * nv);
+        h = fnv1a(h, triangles, sizeof(uint32_t) * 3 * nt);
+        if (face_ids)
+            h = fnv1a(h, face_ids, sizeof(uint32_t) * nt);
+        if (boundary)
+            h = fnv1a(h, boundary, nt);
+        hsbem::hip_check(hipSetDevice(c->ctx.device), "set device");
+        auto mesh = std::make_unique<hsbem_mesh>();
+        mesh->context = c;
+        mesh->hash = h;
+        mesh->solver = std::make_unique<hsbem::MeshSolver>(&c->ctx, std::move(m));
+        *out = mesh.release();
+    });
+}
+
+int hsbem_mesh_destroy(hsbem_mesh *m) {
+    return guarded([&] { delete m; });
+}
+
+int hsbem_mesh_info(const hsbem_mesh *m, uint64_t *hash, uint32_t *nv, uint32_t *nt, double *area) {
+    return guarded([&] {
+        require(m, "null mesh");
+        if (hash)
+            *hash = m->hash;
+        if (nv)
+            *nv = m->solver->nv();
+        if (nt)
+            *nt = m->solver->nt();
+        if (area)
+            *area = m->solver->space().surface_area;
+    });
+}
+
+int hsbem_solve(hsbem_mesh *m, const hsbem_request *r, hsbem_progress_fn progress, void *user,
+                const volatile int32_t *cancel, hsbem_result **out) {
+    return guarded([&] {
+        require(m && r && out, "invalid solve arguments");
+        require(r->frequency_count == 0 || r->frequencies, "null frequencies");
+        require(r->excitation_count == 0 || r->excitation_offsets, "null excitation offsets");
+        require(r->observer_count == 0 || r->observers, "null observers");
+        hsbem::Request q;
+        q.density = r->density;
+        q.sound_speed = r->sound_speed;
+        q.frequencies.assign(r->frequencies, r->frequencies + r->frequency_count);
+        if (r->planning_frequency_count) {
+            require(r->planning_frequencies, "null planning frequencies");
+            q.planning_frequencies.assign(r->planning_frequencies, r->planning_frequencies + r->planning_frequency_count);
+        }
+        for (uint32_t e = 0; e < r->excitation_count; ++e) {
+            hsbem::Excitation ex;
+            for (uint32_t j = r->excitation_offsets[e]; j < r->excitation_offsets[e + 1]; ++j)
+                ex.faces.emplace_back(r->face_ids[j], std::complex<double>(r->velocities[2 * j], r->velocities[2 * j + 1]));
+            q.excitations.push_back(std::move(ex));
+        }
+        for (uint32_t i = 0; i < r->observer_count; ++i)
+            q.observers.push_back({r->observers[3 * i], r->observers[3 * i + 1], r->observers[3 * i + 2]});
+        q.far_field = r->far_field != 0;
+        q.far_origin = {r->far_origin[0], r->far_origin[1], r->far_origin[2]};
+        q.far_reference_distance = r->far_reference_distance;
+        q.numerics = to_numerics(r->numerics);
+        hsbem::hip_check(hipSetDevice(m->context->ctx.device), "set device");
+        auto res = std::make_unique<hsbem_result>();
+        res->result = m->solver->solve(q, progress, user, cancel);
+        *out = res.release();
+    });
+}
+
+int hsbem_result_destroy(hsbem_result *r) {
+    return guarded([&] { delete r; });
+}
+
+int hsbem_result_sizes(const hsbem_result *r, uint32_t *nf, uint32_t *ne, uint32_t *nt, uint32_t *nv, uint32_t *no) {
+    return guarded([&] {
+        require(r, "null result");
+        if (nf)
+            *nf = static_cast<uint32_t>(r->result.frequencies.size());
+        if (ne)
+            *ne = r->result.nexc;
+        if (nt)
+            *nt = r->result.nt;
+        if (nv)
+            *nv = r->result.nv;
+        if (no)
+            *no = r->result.nobs;
+    });
+}
+
+int hsbem_result_surface_pressure(const hsbem_result *r, uint32_t f, uint32_t e, double *out) {
+    return guarded([&] { copy_complex(excitation(r, f, e).surface_pressure, out); });
+}
+int hsbem_result_pressure_trace(const hsbem_result *r, uint32_t f, uint32_t e, double *out) {
+    return guarded([&] { copy_complex(excitation(r, f, e).pressure_trace, out); });
+}
+int hsbem_result_normal_derivative(const hsbem_result *r, uint32_t f, uint32_t e, double *out) {
+    return guarded([&] { copy_complex(excitation(r, f, e).normal_derivative, out); });
+}
+int hsbem_result_observer_pressure(const hsbem_result *r, uint32_t f, uint32_t e, double *out) {
+    return guarded([&] { copy_complex(excitation(r, f, e).observer_pressure, out); });
+}
+
+int hsbem_result_impedances(const hsbem_result *r, uint32_t f, uint32_t e, uint32_t *count, uint32_t *face_ids,
+                            double *area, double *mean_pressure, double *volume_velocity, double *impedance) {
+    return guarded([&] {
+        const auto &x = excitation(r, f, e);
+        require(count, "null count");
+        *count = static_cast<uint32_t>(x.impedances.size());
+        for (std::size_t i = 0; i < x.impedances.size(); ++i) {
+            const auto &z = x.impedances[i];
+            if (face_ids)
+                face_ids[i] = z.face_id;
+            if (area)
+                area[i] = z.area;
+            if (mean_pressure)
+                mean_pressure[2 * i] = z.mean_pressure.real(), mean_pressure[2 * i + 1] = z.mean_pressure.imag();
+            if (volume_velocity)
+                volume_velocity[2 * i] = z.volume_velocity.real(), volume_velocity[2 * i + 1] = z.volume_velocity.imag();
+            if (impedance)
+                impedance[2 * i] = z.impedance.real(), impedance[2 * i + 1] = z.impedance.imag();
+        }
+    });
+}
+
+int hsbem_result_residual(const hsbem_result *r, uint32_t f, uint32_t e, double *residual, uint32_t *iterations,
+                          uint32_t *refinements) {
+    return guarded([&] {
+        const auto &x = excitation(r, f, e);
+        if (residual)
+            *residual = x.residual;
+        if (iterations)
+            *iterations = x.iterations;
+        if (refinements)
+            *refinements = x.refinements;
+    });
+}
+
+int hsbem_result_timings(const hsbem_result *r, uint32_t f, double *ms) {
+    return guarded([&] {
+        require(r && f < r->result.frequencies.size() && ms, "result index out of range");
+        const auto &fr = r->result.frequencies[f];
+        for (int i = 0; i < 6; ++i)
+            ms[i] = fr.ms[i];
+        ms[6] = fr.regular_kernel_ms;
+        ms[7] = fr.touching_kernel_ms;
+        ms[8] = fr.near_kernel_ms;
+    });
+}
+
+int hsbem_result_peak_device_bytes(const hsbem_result *r, uint64_t *bytes) {
+    return guarded([&] {
+        require(r && bytes, "null result");
+        *bytes = r->result.peak_bytes;
+    });
+}
+
+int hsbem_plan_create(hsbem_mesh *m, const hsbem_numerics *n, double k_max, hsbem_plan **out) {
+    return guarded([&] {
+        require(m && n && out, "invalid plan arguments");
+        auto p = std::make_unique<hsbem_plan>();
+        p->plan = m->solver->plan(to_numerics(*n), k_max);
+        *out = p.release();
+    });
+}
+
+int hsbem_plan_build(const double *vertices, uint32_t nv, const uint32_t *triangles, uint32_t nt,
+                     const hsbem_numerics *n, double k_max, hsbem_plan **out) {
+    return guarded([&] {
+        require(vertices && triangles && nv && nt && n && out, "invalid plan arguments");
+        hsbem::SurfaceMesh m;
+        m.vertices.resize(nv);
+        const hsbem::Vec3 origin{vertices[0], vertices[1], vertices[2]};
+        for (uint32_t i = 0; i < nv; ++i)
+            m.vertices[i] = hsbem::vsub({vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2]}, origin);
+        m.triangles.resize(nt);
+        for (uint32_t t = 0; t < nt; ++t)
+            m.triangles[t].vertices = {triangles[3 * t], triangles[3 * t + 1], triangles[3 * t + 2]};
+        const auto space = hsbem::build_space(m);
+        const auto num = to_numerics(*n);
+        hsbem::QuadratureSettings q;
+        q.regular_order = num.quadrature_order;
+        q.singular_order = num.singular_order;
+        q.near_distance_ratio = num.near_ratio;
+        q.near_tolerance = num.near_tolerance;
+        q.near_max_depth = num.near_depth;
+        auto p = std::make_unique<hsbem_plan>();
+        p->plan = hsbem::make_interaction_plan(space, q, k_max, num.plan_threads);
+        *out = p.release();
+    });
+}
+
+int hsbem_plan_sizes(const hsbem_plan *p, uint64_t *s) {
+    return guarded([&] {
+        require(p && s, "null plan");
+        s[0] = p->plan.touching.size();
+        s[1] = p->plan.near.size();
+        s[2] = p->plan.patches.size();
+        s[3] = p->plan.whole_panel_degrees.size();
+        s[4] = p->plan.candidate_pairs;
+        s[5] = p->plan.maximum_depth;
+    });
+}
+
+int hsbem_plan_copy(const hsbem_plan *p, uint32_t *touching, uint32_t *near, double *patches, uint8_t *whole,
+                    uint32_t *offsets, uint32_t *trials, double *max_error) {
+    return guarded([&] {
+        require(p, "null plan");
+        const auto &pl = p->plan;
+        if (touching)
+            for (std::size_t i = 0; i < pl.touching.size(); ++i) {
+                const auto &j = pl.touching[i];
+                uint32_t *o = touching + 9 * i;
+                o[0] = j.test;
+                o[1] = j.trial;
+                o[2] = static_cast<uint32_t>(j.permutation.topology);
+                for (int l = 0; l < 3; ++l) {
+                    o[3 + l] = j.permutation.test[l];
+                    o[6 + l] = j.permutation.trial[l];
+                }
+            }
+        if (near)
+            for (std::size_t i = 0; i < pl.near.size(); ++i) {
+                const auto &j = pl.near[i];
+                uint32_t *o = near + 5 * i;
+                o[0] = j.test, o[1] = j.trial, o[2] = j.first_patch, o[3] = j.patch_count, o[4] = j.swap;
+            }
+        if (patches)
+            for (std::size_t i = 0; i < pl.patches.size(); ++i) {
+                const auto &q = pl.patches[i];
+                double *o = patches + 22 * i;
+                std::memcpy(o, q.test, 9 * sizeof(double));
+                std::memcpy(o + 9, q.trial, 9 * sizeof(double));
+                o[18] = q.area_fraction;
+                o[19] = q.degree;
+                o[20] = q.projection;
+                o[21] = q.projection_intervals;
+            }
+        if (whole && !pl.whole_panel_degrees.empty())
+            std::memcpy(whole, pl.whole_panel_degrees.data(), pl.whole_panel_degrees.size());
+        if (offsets)
+            std::memcpy(offsets, pl.near_offsets.data(), sizeof(uint32_t) * pl.near_offsets.size());
+        if (trials && !pl.near_trials.empty())
+            std::memcpy(trials, pl.near_trials.data(), sizeof(uint32_t) * pl.near_trials.size());
+        if (max_error)
+            *max_error = pl.maximum_estimated_relative_error;
+    });
+}
+
+int hsbem_plan_destroy(hsbem_plan *p) {
+    return guarded([&] { delete p; });
+}
+
+int hsbem_assemble(hsbem_mesh *m, const hsbem_numerics *n, double k_max, double k, double *A, double *B) {
+    return guarded([&] {
+        require(m && n && A && B && k > 0 && k_max >= k, "invalid assembly arguments");
+        hsbem::hip_check(hipSetDevice(m->context->ctx.device), "set device");
+        m->solver->assemble_dense(to_numerics(*n), k_max, k, A, B);
+    });
+}
+
+int hsbem_dense_gmres(hsbem_context *c, uint32_t n, const double *A, const double *b, double *x, uint32_t restart,
+                      uint32_t maximum_iterations, double tolerance, double *relative_residual, uint32_t *iterations) {
+    return guarded([&] {
+        require(c && A && b && x && n && restart >= 1 && maximum_iterations >= 1 && tolerance > 0,
+                "invalid GMRES arguments");
+        hsbem::hip_check(hipSetDevice(c->ctx.device), "set device");
+        hipStream_t s = c->ctx.stream;
+        const std::size_t nn = static_cast<std::size_t>(n) * n;
+        hsbem::DeviceBuffer<double> dA(2 * nn), db(2 * static_cast<std::size_t>(n)), dx(2 * static_cast<std::size_t>(n));
+        hsbem::hip_check(hipMemcpyAsync(dA.data(), A, dA.bytes(), hipMemcpyHostToDevice, s), "A upload");
+        hsbem::hip_check(hipMemcpyAsync(db.data(), b, db.bytes(), hipMemcpyHostToDevice, s), "b upload");
+        hsbem::GmresWorkspace ws;
+        double achieved = 0;
+        std::uint32_t its = 0;
+        hsbem::dense_gmres(s, dA.data(), n, db.data(), dx.data(), restart, maximum_iterations, tolerance, &achieved,
+                           &its, nullptr, ws);
+        hsbem::hip_check(hipMemcpyAsync(x, dx.data(), dx.bytes(), hipMemcpyDeviceToHost, s), "x download");
+        hsbem::hip_check(hipStreamSynchronize(s), "GMRES sync");
+        if (relative_residual)
+            *relative_residual = achieved;
+        if (iterations)
+            *iterations = its;
+    });
+}
+
+int hsbem_dense_lu_solve(hsbem_context *c, uint32_t n, const double *A, uint32_t nrhs, double *B) {
+    return guarded([&] {
+        require(c && A && B && n && nrhs, "invalid LU arguments");
+        hsbem::hip_check(hipSetDevice(c->ctx.device), "set device");
+        hipStream_t s = c->ctx.stream;
+        const std::size_t nn = static_cast<std::size_t>(n) * n;
+        hsbem::DeviceBuffer<double> dA(2 * nn), dB(2 * static_cast<std::size_t>(n) * nrhs);
+        hsbem::DeviceBuffer<std::int32_t> piv(n);
+        hsbem::DeviceBuffer<int> info(1);
+        hsbem::hip_check(hipMemcpyAsync(dA.data(), A, dA.bytes(), hipMemcpyHostToDevice, s), "A upload");
+        hsbem::hip_check(hipMemcpyAsync(dB.data(), B, dB.bytes(), hipMemcpyHostToDevice, s), "B upload");
+        hsbem::hip_check(hipMemsetAsync(info.data(), 0, sizeof(int), s), "info clear");
+        hsbem::dev::lu_factor(dA.data(), n, piv.data(), info.data(), nullptr, s);
+        int code = 0;
+        hsbem::hip_check(hipMemcpyAsync(&code, info.data(), sizeof(int), hipMemcpyDeviceToHost, s), "info download");
+        hsbem::hip_check(hipStreamSynchronize(s), "LU sync");
+        if (code)
+            throw std::runtime_error("LU factorization failed at pivot " + std::to_string(code));
+        hsbem::dev::lu_solve(dA.data(), n, piv.data(), dB.data(), nrhs, s);
+        hsbem::hip_check(hipMemcpyAsync(B, dB.data(), dB.bytes(), hipMemcpyDeviceToHost, s), "B download");
+        hsbem::hip_check(hipStreamSynchronize(s), "LU solve sync");
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,136 @@
+// Host-callable launchers of the device kernels (plain C++ declarations).
+#pragma once
+
+#include "../host/plan.hpp"
+#include "../host/rules.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+namespace hsbem::dev {
+
+// Panel SoA fields (geo[field][nt]) and bound fields (bounds[field][nt]).
+enum : int { G_V0 = 0, G_E1 = 3, G_E2 = 6, G_N = 9, G_C0 = 12, G_AREA = 21, G_COUNT = 22 };
+enum : int { B_C = 0, B_R = 3, B_H = 4, B_COUNT = 5 };
+
+// Error codes of the reference kernels (cuda_solver.cu:705-715).
+enum : int { ERR_DISTANCE = 1, ERR_OVERFLOW = 2, ERR_FIELD = 3, ERR_ESCAPED = 4 };
+
+// Rule table layout (TrianglePoint = bary[3], weight): degree 4, 6, 8, 14.
+enum : int { kRule4 = 0, kRule6 = 6, kRule8 = 18, kRule14 = 34, kRuleTableCount = 98 };
+
+struct PanelView {
+    const double *geo;
+    const std::uint32_t *nodes;
+    const double *bounds;
+    const std::uint8_t *shape;
+    std::uint32_t nt, nv;
+};
+
+struct AssemblyArgs {
+    PanelView panels;
+    const double *row_scale;        // nv, 1/lumped mass, internal node order
+    const std::int32_t *flux_column; // nt, column of BD or -1
+    double *A;                      // nv x nv complex
+    double *BD;                     // nv x nd complex
+    std::uint32_t nd;
+    double k, tolerance;
+    unsigned regular_order;
+    int *error;
+};
+
+struct ClusterView {
+    const std::uint32_t *panel_offsets, *panels; // cluster -> panel ids
+    const std::uint32_t *node_offsets, *nodes;   // cluster -> internal node ids
+    const std::uint8_t *local;                   // [3][nt] slot of each panel vertex in its cluster node list
+    std::uint32_t count;
+};
+
+struct NearView {
+    const std::uint32_t *offsets, *trials; // near CSR by test panel
+    const std::uint8_t *whole;             // all-pairs degree bytes or null
+};
+
+struct AssemblyLaunch {
+    AssemblyArgs args;
+    ClusterView clusters;
+    NearView near;
+    const double *rules;                   // kRuleTableCount x 4 doubles
+    const TouchingJob *touching;
+    std::uint32_t touching_count;
+    const double *gauss;                   // singular Gauss rule (position, weight)
+    unsigned singular_order;
+    const NearJob *near_jobs;
+    std::uint32_t near_count;
+    const PairPatch *patches;
+};
+
+void launch_assembly(const AssemblyLaunch &launch, hipStream_t stream);
+void launch_regular(const AssemblyLaunch &launch, hipStream_t stream);
+void launch_touching(const AssemblyLaunch &launch, hipStream_t stream);
+void launch_near(const AssemblyLaunch &launch, hipStream_t stream);
+void launch_mass(const AssemblyLaunch &launch, hipStream_t stream);
+void launch_flux_rhs(const double *BD, std::uint32_t nv, std::uint32_t nd, const double *q, std::uint32_t nrhs,
+                     double *rhs, hipStream_t stream);
+
+// ---- dense linear algebra (linalg.hip) ----
+
+// y[:, e] = A x[:, e] for e < nvec (A n x n complex, column-major); y may not alias x.
+void launch_gemv(const double *A, std::uint32_t n, const double *x, double *y, std::uint32_t nvec, double *scratch,
+                 hipStream_t stream);
+// Scratch doubles launch_gemv needs for n and nvec.
+std::size_t gemv_scratch_doubles(std::uint32_t n, std::uint32_t nvec);
+
+// out[j] = V[:, j]^H w for j < m (V n x m column-major); out complex m.
+void launch_project(const double *V, std::uint32_t n, std::uint32_t m, const double *w, double *out,
+                    double *scratch, hipStream_t stream);
+// w -= V h (h complex m).
+void launch_subtract_combination(const double *V, std::uint32_t n, std::uint32_t m, const double *h, double *w,
+                                 hipStream_t stream);
+// out[0] = ||x||_2^2 (one complex vector of length n, weighted by mass if given).
+void launch_norm2(const double *x, std::uint32_t n, const double *weight, double *out, double *scratch,
+                  hipStream_t stream);
+std::size_t reduction_scratch_doubles(std::uint32_t n, std::uint32_t m);
+
+// Complex LU with partial pivoting in place (A n x n), pivots n.
+void lu_factor(double *A, std::uint32_t n, std::int32_t *pivots, int *info, double *work, hipStream_t stream);
+std::size_t lu_work_doubles(std::uint32_t n);
+// Solve with the factors for nrhs right-hand sides in place (B n x nrhs).
+void lu_solve(const double *LU, std::uint32_t n, const std::int32_t *pivots, double *B, std::uint32_t nrhs,
+              hipStream_t stream);
+
+// r = b - A x, scaled by weight (lumped mass) into weighted; r unscaled into residual.
+void launch_residual(const double *A, std::uint32_t n, const double *b, const double *x, std::uint32_t nvec,
+                     double *residual, double *scratch, hipStream_t stream);
+
+// Elementwise helpers.
+void launch_axpy(std::uint32_t n, const double *alpha, const double *x, double *y, hipStream_t stream);
+// y = s * x (complex vector of length n, real s).
+void launch_scale(std::uint32_t n, double s, const double *x, double *y, hipStream_t stream);
+
+// ---- fields (field.hip) ----
+
+struct FieldArgs {
+    PanelView panels;
+    const double *pressure; // nv x nrhs complex, internal node order
+    const double *flux;     // nt x nrhs complex (DP0 q per panel)
+    std::uint32_t nrhs;
+    const double *points;   // observers (local frame) or unit directions
+    std::uint32_t count;
+    const double *rule;     // triangle rule (bary[3], w)
+    std::uint32_t rule_count;
+    double k;
+    double near_tolerance;
+    unsigned max_depth;
+    double origin[3];       // far field: phase origin (local frame)
+    double reference_distance;
+    double *out;            // count x nrhs complex
+    int *error;
+};
+
+void launch_far_field(const FieldArgs &args, double *partials, hipStream_t stream);
+void launch_observer_field(const FieldArgs &args, double *partials, hipStream_t stream);
+std::size_t field_partial_doubles(std::uint32_t nt, std::uint32_t count, std::uint32_t nrhs);
+
+} // namespace hsbem::dev

@@ -1,0 +1,582 @@
+// Dense complex FP64 linear algebra for the Burton-Miller system, gfx950.
+//
+// Replaces the vendor calls of the reference dense path
+// (cusolverDnCgetrf/Cgetrs, cuda_solver.cu:1485-1513; cublasZgemv/Zaxpy/Dznrm2
+// in device_gmres.cu:115-157) with own kernels:
+//   gemv            y = A X for up to 8 vectors: 256 rows x 128 columns per
+//                   workgroup, A streamed once (HBM bound), deterministic
+//                   split-K partials reduced in a second pass.
+//   project         CGS2 projections V^H w, one wavefront per basis column chunk.
+//   lu_factor       right-looking blocked LU (panel 32): one-workgroup panel
+//                   factorisation with pivot search, row swaps, unit-lower
+//                   TRSM, and a 64x64-tiled trailing update.
+//   lu_solve        blocked forward/backward substitution.
+#include "device.hpp"
+
+#include <algorithm>
+
+namespace hsbem::dev {
+namespace {
+
+constexpr int kGemvRows = 256;
+constexpr int kGemvCols = 128;
+constexpr int kMaxVec = 8;
+
+template <int NV>
+__global__ __launch_bounds__(kGemvRows) void gemv_partial(const double *__restrict__ A, std::uint32_t n,
+                                                           const double *__restrict__ x, std::uint32_t chunks,
+                                                           double *__restrict__ part) {
+    __shared__ double sx[NV][2 * kGemvCols];
+    const std::uint32_t row = blockIdx.x * kGemvRows + threadIdx.x;
+    const std::uint32_t c0 = blockIdx.y * kGemvCols;
+    const std::uint32_t cn = min(static_cast<std::uint32_t>(kGemvCols), n - c0);
+    for (std::uint32_t e = threadIdx.x; e < NV * 2 * kGemvCols; e += kGemvRows) {
+        const std::uint32_t v = e / (2 * kGemvCols), c = e % (2 * kGemvCols);
+        sx[v][c] = c / 2 < cn ? x[2 * (static_cast<std::size_t>(v) * n + c0) + c] : 0.0;
+    }
+    __syncthreads();
+    if (row >= n)
+        return;
+    double re[NV], im[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+        re[v] = im[v] = 0.0;
+    const double2 *col = reinterpret_cast<const double2 *>(A) + static_cast<std::size_t>(c0) * n + row;
+    std::uint32_t c = 0;
+    for (; c + 4 <= cn; c += 4) {
+        double2 a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            a[u] = col[static_cast<std::size_t>(c + u) * n];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double xr = sx[v][2 * (c + u)], xi = sx[v][2 * (c + u) + 1];
+                re[v] += a[u].x * xr - a[u].y * xi;
+                im[v] += a[u].x * xi + a[u].y * xr;
+            }
+    }
+    for (; c < cn; ++c) {
+        const double2 a = col[static_cast<std::size_t>(c) * n];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double xr = sx[v][2 * c], xi = sx[v][2 * c + 1];
+            re[v] += a.x * xr - a.y * xi;
+            im[v] += a.x * xi + a.y * xr;
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const std::size_t o = (static_cast<std::size_t>(v) * chunks + blockIdx.y) * n + row;
+        part[2 * o] = re[v];
+        part[2 * o + 1] = im[v];
+    }
+}
+
+__global__ void gemv_reduce(const double *part, std::uint32_t n, std::uint32_t chunks, std::uint32_t nvec, double *y) {
+    const std::size_t idx = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= static_cast<std::size_t>(n) * nvec)
+        return;
+    const std::uint32_t row = idx % n, v = idx / n;
+    double re = 0, im = 0;
+    for (std::uint32_t c = 0; c < chunks; ++c) {
+        const std::size_t o = (static_cast<std::size_t>(v) * chunks + c) * n + row;
+        re += part[2 * o];
+        im += part[2 * o + 1];
+    }
+    y[2 * idx] = re;
+    y[2 * idx + 1] = im;
+}
+
+// Projection partials: block b reduces rows [b*1024, (b+1)*1024) for every column
+// j < m, one wavefront per column (strided), into part[b][j].
+__global__ __launch_bounds__(256) void project_partial(const double *V, std::uint32_t n, std::uint32_t m,
+                                                      const double *w, double *part) {
+    constexpr int kRows = 1024;
+    const std::uint32_t r0 = blockIdx.x * kRows;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
+    for (std::uint32_t j = wave; j < m; j += 4) {
+        double re = 0, im = 0;
+        for (std::uint32_t r = r0 + lane; r < min(r0 + kRows, n); r += 64) {
+            const double vr = V[2 * (static_cast<std::size_t>(j) * n + r)], vi = V[2 * (static_cast<std::size_t>(j) * n + r) + 1];
+            const double xr = w[2 * r], xi = w[2 * r + 1];
+            re += vr * xr + vi * xi; // conj(v) * x
+            im += vr * xi - vi * xr;
+        }
+        re = wave_sum(re);
+        im = wave_sum(im);
+        if (lane == 0) {
+            part[2 * (static_cast<std::size_t>(blockIdx.x) * m + j)] = re;
+            part[2 * (static_cast<std::size_t>(blockIdx.x) * m + j) + 1] = im;
+        }
+    }
+}
+
+__global__ void sum_columns(const double *part, std::uint32_t blocks, std::uint32_t m, double *out) {
+    const std::uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m)
+        return;
+    double re = 0, im = 0;
+    for (std::uint32_t b = 0; b < blocks; ++b) {
+        re += part[2 * (static_cast<std::size_t>(b) * m + j)];
+        im += part[2 * (static_cast<std::size_t>(b) * m + j) + 1];
+    }
+    out[2 * j] = re;
+    out[2 * j + 1] = im;
+}
+
+__global__ void subtract_combination(const double *V, std::uint32_t n, std::uint32_t m, const double *h, double *w) {
+    const std::uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    double re = w[2 * r], im = w[2 * r + 1];
+    for (std::uint32_t j = 0; j < m; ++j) {
+        const double vr = V[2 * (static_cast<std::size_t>(j) * n + r)], vi = V[2 * (static_cast<std::size_t>(j) * n + r) + 1];
+        const double hr = h[2 * j], hi = h[2 * j + 1];
+        re -= vr * hr - vi * hi;
+        im -= vr * hi + vi * hr;
+    }
+    w[2 * r] = re;
+    w[2 * r + 1] = im;
+}
+
+__global__ __launch_bounds__(256) void norm_partial(const double *x, std::uint32_t n, const double *weight,
+                                                   double *part) {
+    __shared__ double red[4];
+    double s = 0;
+    for (std::uint32_t r = blockIdx.x * 256 + threadIdx.x; r < n; r += gridDim.x * 256) {
+        const double f = weight ? weight[r] : 1.0;
+        const double a = x[2 * r] * f, b = x[2 * r + 1] * f;
+        s += a * a + b * b;
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void norm_finish(const double *part, std::uint32_t blocks, double *out) {
+    double s = 0;
+    for (std::uint32_t b = threadIdx.x; b < blocks; b += 64)
+        s += part[b];
+    s = wave_sum(s);
+    if (threadIdx.x == 0)
+        out[0] = s;
+}
+
+__global__ void axpy_kernel(std::uint32_t n, const double *alpha, const double *x, double *y) {
+    const std::uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    const double ar = alpha[0], ai = alpha[1], xr = x[2 * r], xi = x[2 * r + 1];
+    y[2 * r] += ar * xr - ai * xi;
+    y[2 * r + 1] += ar * xi + ai * xr;
+}
+
+__global__ void scale_kernel(std::uint32_t n, double s, const double *x, double *y) {
+    const std::uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    y[2 * r] = s * x[2 * r];
+    y[2 * r + 1] = s * x[2 * r + 1];
+}
+
+__global__ void subtract_vectors(const double *b, const double *ax, std::size_t count, double *r) {
+    const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < count)
+        r[i] = b[i] - ax[i];
+}
+
+// ---------------- LU ----------------
+constexpr int kPanel = 32;
+constexpr int kPanelThreads = 1024;
+
+// Factor columns [k0, k0+nb) over rows [k0, n): pivot search, swap within the
+// panel, scale, rank-1 updates. Pivot rows are recorded as absolute indices.
+__global__ __launch_bounds__(kPanelThreads) void lu_panel(double *A, std::uint32_t n, std::uint32_t k0, std::uint32_t nb,
+                                                         std::int32_t *piv, int *info) {
+    __shared__ double best[kPanelThreads / 64];
+    __shared__ std::uint32_t where[kPanelThreads / 64];
+    __shared__ std::uint32_t pivot_row;
+    __shared__ double pr, pi;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid / 64;
+    for (std::uint32_t j = k0; j < k0 + nb; ++j) {
+        double bv = -1;
+        std::uint32_t bi = j;
+        for (std::uint32_t r = j + tid; r < n; r += kPanelThreads) {
+            // largest modulus; ties resolve to the lowest row in the reduction below
+            const double m = hypot(A[2 * (static_cast<std::size_t>(j) * n + r)], A[2 * (static_cast<std::size_t>(j) * n + r) + 1]);
+            if (m > bv) {
+                bv = m;
+                bi = r;
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(bv, off, 64);
+            const std::uint32_t oi = __shfl_xor(bi, off, 64);
+            if (ov > bv || (ov == bv && oi < bi)) {
+                bv = ov;
+                bi = oi;
+            }
+        }
+        if (lane == 0) {
+            best[wave] = bv;
+            where[wave] = bi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double v = best[0];
+            std::uint32_t w = where[0];
+            for (int q = 1; q < kPanelThreads / 64; ++q)
+                if (best[q] > v || (best[q] == v && where[q] < w)) {
+                    v = best[q];
+                    w = where[q];
+                }
+            pivot_row = w;
+            piv[j] = static_cast<std::int32_t>(w);
+            if (!(v > 0) && *info == 0)
+                *info = static_cast<int>(j) + 1;
+        }
+        __syncthreads();
+        const std::uint32_t p = pivot_row;
+        if (p != j)
+            for (std::uint32_t c = k0 + tid; c < k0 + nb; c += kPanelThreads) {
+                double *a = A + 2 * (static_cast<std::size_t>(c) * n + j), *b = A + 2 * (static_cast<std::size_t>(c) * n + p);
+                const double t0 = a[0], t1 = a[1];
+                a[0] = b[0];
+                a[1] = b[1];
+                b[0] = t0;
+                b[1] = t1;
+            }
+        __syncthreads();
+        if (tid == 0) {
+            const double ar = A[2 * (static_cast<std::size_t>(j) * n + j)], ai = A[2 * (static_cast<std::size_t>(j) * n + j) + 1];
+            const double d = ar * ar + ai * ai;
+            pr = d > 0 ? ar / d : 0.0; // 1/a = conj(a)/|a|^2
+            pi = d > 0 ? -ai / d : 0.0;
+        }
+        __syncthreads();
+        const double ir = pr, ii = pi;
+        for (std::uint32_t r = j + 1 + tid; r < n; r += kPanelThreads) {
+            double *a = A + 2 * (static_cast<std::size_t>(j) * n + r);
+            const double lr = a[0] * ir - a[1] * ii, li = a[0] * ii + a[1] * ir;
+            a[0] = lr;
+            a[1] = li;
+            for (std::uint32_t c = j + 1; c < k0 + nb; ++c) {
+                const double ur = A[2 * (static_cast<std::size_t>(c) * n + j)], ui = A[2 * (static_cast<std::size_t>(c) * n + j) + 1];
+                double *b = A + 2 * (static_cast<std::size_t>(c) * n + r);
+                b[0] -= lr * ur - li * ui;
+                b[1] -= lr * ui + li * ur;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Apply the panel's row interchanges to the columns outside it.
+__global__ void lu_swap(double *A, std::uint32_t n, std::uint32_t k0, std::uint32_t nb, const std::int32_t *piv) {
+    const std::uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n || (c >= k0 && c < k0 + nb))
+        return;
+    double *col = A + 2 * static_cast<std::size_t>(c) * n;
+    for (std::uint32_t j = k0; j < k0 + nb; ++j) {
+        const std::uint32_t p = static_cast<std::uint32_t>(piv[j]);
+        if (p != j) {
+            const double t0 = col[2 * j], t1 = col[2 * j + 1];
+            col[2 * j] = col[2 * p];
+            col[2 * j + 1] = col[2 * p + 1];
+            col[2 * p] = t0;
+            col[2 * p + 1] = t1;
+        }
+    }
+}
+
+// U12 = L11^{-1} A12 (unit lower L11 of the panel), one thread per column.
+__global__ __launch_bounds__(256) void lu_trsm(double *A, std::uint32_t n, std::uint32_t k0, std::uint32_t nb) {
+    __shared__ double L[kPanel][kPanel][2];
+    for (std::uint32_t e = threadIdx.x; e < nb * nb; e += 256) {
+        const std::uint32_t r = e % nb, c = e / nb;
+        L[r][c][0] = A[2 * ((static_cast<std::size_t>(k0) + c) * n + k0 + r)];
+        L[r][c][1] = A[2 * ((static_cast<std::size_t>(k0) + c) * n + k0 + r) + 1];
+    }
+    __syncthreads();
+    const std::uint32_t c = k0 + nb + blockIdx.x * 256 + threadIdx.x;
+    if (c >= n)
+        return;
+    double xr[kPanel], xi[kPanel];
+    double *col = A + 2 * (static_cast<std::size_t>(c) * n + k0);
+    for (std::uint32_t i = 0; i < nb; ++i) {
+        double vr = col[2 * i], vi = col[2 * i + 1];
+        for (std::uint32_t j = 0; j < i; ++j) {
+            vr -= L[i][j][0] * xr[j] - L[i][j][1] * xi[j];
+            vi -= L[i][j][0] * xi[j] + L[i][j][1] * xr[j];
+        }
+        xr[i] = vr;
+        xi[i] = vi;
+        col[2 * i] = vr;
+        col[2 * i + 1] = vi;
+    }
+}
+
+// A22 -= L21 U12 with 64x64 output tiles, K = nb (<= 32), 256 threads, 4x4 per thread.
+__global__ __launch_bounds__(256) void lu_update(double *A, std::uint32_t n, std::uint32_t k0, std::uint32_t nb) {
+    __shared__ double Ls[kPanel][64][2];
+    __shared__ double Us[kPanel][64][2];
+    const std::uint32_t base = k0 + nb;
+    const std::uint32_t r0 = base + blockIdx.x * 64, c0 = base + blockIdx.y * 64;
+    for (std::uint32_t e = threadIdx.x; e < nb * 64; e += 256) {
+        const std::uint32_t kk = e / 64, q = e % 64;
+        const std::uint32_t r = r0 + q, c = c0 + q;
+        Ls[kk][q][0] = r < n ? A[2 * ((static_cast<std::size_t>(k0) + kk) * n + r)] : 0.0;
+        Ls[kk][q][1] = r < n ? A[2 * ((static_cast<std::size_t>(k0) + kk) * n + r) + 1] : 0.0;
+        Us[kk][q][0] = c < n ? A[2 * (static_cast<std::size_t>(c) * n + k0 + kk)] : 0.0;
+        Us[kk][q][1] = c < n ? A[2 * (static_cast<std::size_t>(c) * n + k0 + kk) + 1] : 0.0;
+    }
+    __syncthreads();
+    const int tr = threadIdx.x % 16, tc = threadIdx.x / 16;
+    double cr[4][4], ci[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            cr[a][b] = ci[a][b] = 0;
+    for (std::uint32_t kk = 0; kk < nb; ++kk) {
+        double lr[4], li[4], ur[4], ui[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            lr[a] = Ls[kk][tr + 16 * a][0];
+            li[a] = Ls[kk][tr + 16 * a][1];
+            ur[a] = Us[kk][tc + 16 * a][0];
+            ui[a] = Us[kk][tc + 16 * a][1];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                cr[a][b] += lr[a] * ur[b] - li[a] * ui[b];
+                ci[a][b] += lr[a] * ui[b] + li[a] * ur[b];
+            }
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const std::uint32_t c = c0 + tc + 16 * b;
+        if (c >= n)
+            continue;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const std::uint32_t r = r0 + tr + 16 * a;
+            if (r >= n)
+                continue;
+            double *p = A + 2 * (static_cast<std::size_t>(c) * n + r);
+            p[0] -= cr[a][b];
+            p[1] -= ci[a][b];
+        }
+    }
+}
+
+// Row interchanges of every panel applied to the right-hand sides.
+__global__ void apply_pivots(double *B, std::uint32_t n, const std::int32_t *piv, std::uint32_t nrhs) {
+    const std::uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nrhs)
+        return;
+    double *x = B + 2 * static_cast<std::size_t>(e) * n;
+    for (std::uint32_t j = 0; j < n; ++j) {
+        const std::uint32_t p = static_cast<std::uint32_t>(piv[j]);
+        if (p != j) {
+            const double t0 = x[2 * j], t1 = x[2 * j + 1];
+            x[2 * j] = x[2 * p];
+            x[2 * j + 1] = x[2 * p + 1];
+            x[2 * p] = t0;
+            x[2 * p + 1] = t1;
+        }
+    }
+}
+
+constexpr int kSolveBlock = 64;
+
+// Solve the diagonal block [j0, j0+nb) (lower unit or upper) for every rhs; one
+// workgroup per rhs, the block in LDS.
+__global__ __launch_bounds__(64) void block_substitute(const double *LU, std::uint32_t n, double *B, std::uint32_t j0,
+                                                      std::uint32_t nb, int upper) {
+    __shared__ double x[kSolveBlock][2];
+    double *b = B + 2 * (static_cast<std::size_t>(blockIdx.x) * n + j0);
+    const int t = threadIdx.x;
+    if (t < static_cast<int>(nb)) {
+        x[t][0] = b[2 * t];
+        x[t][1] = b[2 * t + 1];
+    }
+    __syncthreads();
+    if (!upper) {
+        for (std::uint32_t j = 0; j < nb; ++j) {
+            if (t > static_cast<int>(j) && t < static_cast<int>(nb)) {
+                const double lr = LU[2 * ((static_cast<std::size_t>(j0) + j) * n + j0 + t)],
+                             li = LU[2 * ((static_cast<std::size_t>(j0) + j) * n + j0 + t) + 1];
+                x[t][0] -= lr * x[j][0] - li * x[j][1];
+                x[t][1] -= lr * x[j][1] + li * x[j][0];
+            }
+            __syncthreads();
+        }
+    } else {
+        for (std::uint32_t jj = nb; jj-- > 0;) {
+            if (t == static_cast<int>(jj)) {
+                const double ur = LU[2 * ((static_cast<std::size_t>(j0) + jj) * n + j0 + jj)],
+                             ui = LU[2 * ((static_cast<std::size_t>(j0) + jj) * n + j0 + jj) + 1];
+                const double d = ur * ur + ui * ui;
+                const double vr = x[jj][0], vi = x[jj][1];
+                x[jj][0] = (vr * ur + vi * ui) / d;
+                x[jj][1] = (vi * ur - vr * ui) / d;
+            }
+            __syncthreads();
+            if (t < static_cast<int>(jj)) {
+                const double ur = LU[2 * ((static_cast<std::size_t>(j0) + jj) * n + j0 + t)],
+                             ui = LU[2 * ((static_cast<std::size_t>(j0) + jj) * n + j0 + t) + 1];
+                x[t][0] -= ur * x[jj][0] - ui * x[jj][1];
+                x[t][1] -= ur * x[jj][1] + ui * x[jj][0];
+            }
+            __syncthreads();
+        }
+    }
+    if (t < static_cast<int>(nb)) {
+        b[2 * t] = x[t][0];
+        b[2 * t + 1] = x[t][1];
+    }
+}
+
+// B[rows] -= LU[rows, j0:j0+nb] * B[j0:j0+nb] for rows outside the block
+// (below for the lower sweep, above for the upper sweep).
+__global__ __launch_bounds__(256) void block_eliminate(const double *LU, std::uint32_t n, double *B, std::uint32_t j0,
+                                                      std::uint32_t nb, std::uint32_t r_begin, std::uint32_t r_end) {
+    const std::uint32_t r = r_begin + blockIdx.x * 256 + threadIdx.x;
+    const double *b = B + 2 * static_cast<std::size_t>(blockIdx.y) * n;
+    if (r >= r_end)
+        return;
+    double re = 0, im = 0;
+    for (std::uint32_t j = 0; j < nb; ++j) {
+        const double lr = LU[2 * ((static_cast<std::size_t>(j0) + j) * n + r)],
+                     li = LU[2 * ((static_cast<std::size_t>(j0) + j) * n + r) + 1];
+        const double xr = b[2 * (j0 + j)], xi = b[2 * (j0 + j) + 1];
+        re += lr * xr - li * xi;
+        im += lr * xi + li * xr;
+    }
+    double *out = B + 2 * (static_cast<std::size_t>(blockIdx.y) * n + r);
+    out[0] -= re;
+    out[1] -= im;
+}
+
+} // namespace
+
+std::size_t gemv_scratch_doubles(std::uint32_t n, std::uint32_t nvec) {
+    const std::size_t chunks = (n + kGemvCols - 1) / kGemvCols;
+    return 2 * chunks * n * std::max<std::uint32_t>(nvec, 1);
+}
+
+void launch_gemv(const double *A, std::uint32_t n, const double *x, double *y, std::uint32_t nvec, double *scratch,
+                 hipStream_t stream) {
+    const std::uint32_t chunks = (n + kGemvCols - 1) / kGemvCols;
+    for (std::uint32_t v0 = 0; v0 < nvec; v0 += kMaxVec) {
+        const std::uint32_t nv = std::min<std::uint32_t>(kMaxVec, nvec - v0);
+        const dim3 grid((n + kGemvRows - 1) / kGemvRows, chunks);
+        const double *xs = x + 2 * static_cast<std::size_t>(v0) * n;
+        switch (nv) {
+#define HS_GEMV_CASE(N)                                                                                          \
+    case N:                                                                                                      \
+        hipLaunchKernelGGL(gemv_partial<N>, grid, dim3(kGemvRows), 0, stream, A, n, xs, chunks, scratch);        \
+        break;
+            HS_GEMV_CASE(1)
+            HS_GEMV_CASE(2)
+            HS_GEMV_CASE(3)
+            HS_GEMV_CASE(4)
+            HS_GEMV_CASE(5)
+            HS_GEMV_CASE(6)
+            HS_GEMV_CASE(7)
+            HS_GEMV_CASE(8)
+#undef HS_GEMV_CASE
+        }
+        const std::size_t count = static_cast<std::size_t>(n) * nv;
+        hipLaunchKernelGGL(gemv_reduce, dim3((count + 255) / 256), dim3(256), 0, stream, scratch, n, chunks, nv,
+                           y + 2 * static_cast<std::size_t>(v0) * n);
+    }
+}
+
+std::size_t reduction_scratch_doubles(std::uint32_t n, std::uint32_t m) {
+    const std::size_t blocks = (n + 1023) / 1024;
+    return 2 * blocks * std::max<std::uint32_t>(m, 1) + 1024;
+}
+
+void launch_project(const double *V, std::uint32_t n, std::uint32_t m, const double *w, double *out, double *scratch,
+                    hipStream_t stream) {
+    const std::uint32_t blocks = (n + 1023) / 1024;
+    hipLaunchKernelGGL(project_partial, dim3(blocks), dim3(256), 0, stream, V, n, m, w, scratch);
+    hipLaunchKernelGGL(sum_columns, dim3((m + 63) / 64), dim3(64), 0, stream, scratch, blocks, m, out);
+}
+
+void launch_subtract_combination(const double *V, std::uint32_t n, std::uint32_t m, const double *h, double *w,
+                                 hipStream_t stream) {
+    hipLaunchKernelGGL(subtract_combination, dim3((n + 255) / 256), dim3(256), 0, stream, V, n, m, h, w);
+}
+
+void launch_norm2(const double *x, std::uint32_t n, const double *weight, double *out, double *scratch,
+                  hipStream_t stream) {
+    const std::uint32_t blocks = std::min<std::uint32_t>(1024, (n + 255) / 256);
+    hipLaunchKernelGGL(norm_partial, dim3(blocks), dim3(256), 0, stream, x, n, weight, scratch);
+    hipLaunchKernelGGL(norm_finish, dim3(1), dim3(64), 0, stream, scratch, blocks, out);
+}
+
+void launch_axpy(std::uint32_t n, const double *alpha, const double *x, double *y, hipStream_t stream) {
+    hipLaunchKernelGGL(axpy_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, alpha, x, y);
+}
+
+void launch_scale(std::uint32_t n, double s, const double *x, double *y, hipStream_t stream) {
+    hipLaunchKernelGGL(scale_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, s, x, y);
+}
+
+void launch_residual(const double *A, std::uint32_t n, const double *b, const double *x, std::uint32_t nvec,
+                     double *residual, double *scratch, hipStream_t stream) {
+    launch_gemv(A, n, x, residual, nvec, scratch, stream);
+    const std::size_t count = 2 * static_cast<std::size_t>(n) * nvec;
+    hipLaunchKernelGGL(subtract_vectors, dim3((count + 255) / 256), dim3(256), 0, stream, b, residual, count, residual);
+}
+
+std::size_t lu_work_doubles(std::uint32_t) { return 0; }
+
+void lu_factor(double *A, std::uint32_t n, std::int32_t *piv, int *info, double *, hipStream_t stream) {
+    for (std::uint32_t k0 = 0; k0 < n; k0 += kPanel) {
+        const std::uint32_t nb = std::min<std::uint32_t>(kPanel, n - k0);
+        hipLaunchKernelGGL(lu_panel, dim3(1), dim3(kPanelThreads), 0, stream, A, n, k0, nb, piv, info);
+        hipLaunchKernelGGL(lu_swap, dim3((n + 255) / 256), dim3(256), 0, stream, A, n, k0, nb, piv);
+        const std::uint32_t rest = n - k0 - nb;
+        if (!rest)
+            continue;
+        hipLaunchKernelGGL(lu_trsm, dim3((rest + 255) / 256), dim3(256), 0, stream, A, n, k0, nb);
+        const std::uint32_t tiles = (rest + 63) / 64;
+        hipLaunchKernelGGL(lu_update, dim3(tiles, tiles), dim3(256), 0, stream, A, n, k0, nb);
+    }
+}
+
+void lu_solve(const double *LU, std::uint32_t n, const std::int32_t *piv, double *B, std::uint32_t nrhs,
+              hipStream_t stream) {
+    hipLaunchKernelGGL(apply_pivots, dim3((nrhs + 63) / 64), dim3(64), 0, stream, B, n, piv, nrhs);
+    for (std::uint32_t j0 = 0; j0 < n; j0 += kSolveBlock) {
+        const std::uint32_t nb = std::min<std::uint32_t>(kSolveBlock, n - j0);
+        hipLaunchKernelGGL(block_substitute, dim3(nrhs), dim3(64), 0, stream, LU, n, B, j0, nb, 0);
+        const std::uint32_t rows = n - j0 - nb;
+        if (rows)
+            hipLaunchKernelGGL(block_eliminate, dim3((rows + 255) / 256, nrhs), dim3(256), 0, stream, LU, n, B, j0, nb,
+                               j0 + nb, n);
+    }
+    const std::uint32_t last = ((n - 1) / kSolveBlock) * kSolveBlock;
+    for (std::int64_t j0 = last; j0 >= 0; j0 -= kSolveBlock) {
+        const std::uint32_t nb = std::min<std::uint32_t>(kSolveBlock, n - static_cast<std::uint32_t>(j0));
+        hipLaunchKernelGGL(block_substitute, dim3(nrhs), dim3(64), 0, stream, LU, n, B, static_cast<std::uint32_t>(j0),
+                           nb, 1);
+        if (j0 > 0)
+            hipLaunchKernelGGL(block_eliminate, dim3((static_cast<std::uint32_t>(j0) + 255) / 256, nrhs), dim3(256), 0,
+                               stream, LU, n, B, static_cast<std::uint32_t>(j0), nb, 0u, static_cast<std::uint32_t>(j0));
+    }
+}
+
+} // namespace hsbem::dev

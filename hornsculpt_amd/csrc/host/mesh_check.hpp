@@ -1,0 +1,15 @@
+// Closed exterior mesh validation (the role of inspectMesh + validateExteriorComponents,
+// modules/meshing/include/hornsculpt/meshing.hpp:63 and galerkin/exterior_domain.hpp:13):
+// every undirected edge is shared by exactly two triangles with opposite
+// orientation, each connected component encloses a positive signed volume
+// (outward normals, compensated sum about a local origin), and no component
+// lies inside another (solid-angle test of one vertex).
+#pragma once
+
+#include "space.hpp"
+
+namespace hsbem {
+
+void validate_closed_exterior(const SurfaceMesh &mesh);
+
+} // namespace hsbem

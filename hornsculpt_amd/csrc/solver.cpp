@@ -1,0 +1,846 @@
+// Solve orchestration. Reference behaviour restated (cuda_solver.cu):
+//   recentring on vertex 0 and observer transform ........ :771-781, :1107-1137
+//   request validation ................................... :1054-1080, :1165-1193
+//   velocities per panel / active face columns .......... :1165-1212
+//   per-frequency q = -i rho omega v, assembly, solve .... :1292-1518
+//   FP64 residual relative to the mass-weighted rhs ...... :555-585, :1472-1481, :1519-1581
+//   observer/far-field endpoint-degree convergence loop .. :1717-1799
+//   surface outputs and port impedances ................. :1800-1846
+#include "solver.hpp"
+
+#include "host/mesh_check.hpp"
+#include "host/qualification.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numbers>
+#include <numeric>
+#include <set>
+#include <sstream>
+
+namespace hsbem {
+
+void hip_check(hipError_t status, const char *what) {
+    if (status != hipSuccess)
+        throw DeviceError(std::string(what) + ": " + hipGetErrorString(status));
+}
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double ms_since(Clock::time_point t) { return std::chrono::duration<double, std::milli>(Clock::now() - t).count(); }
+
+std::uint64_t spread21(std::uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+
+void check_error_flag(const DeviceBuffer<int> &flag, hipStream_t s) {
+    int code = 0;
+    hip_check(hipMemcpyAsync(&code, flag.data(), sizeof(int), hipMemcpyDeviceToHost, s), "error flag download");
+    hip_check(hipStreamSynchronize(s), "stream synchronize");
+    if (code == dev::ERR_DISTANCE)
+        throw std::runtime_error("BEM quadrature encountered zero or invalid source distance.");
+    if (code == dev::ERR_OVERFLOW)
+        throw std::runtime_error("BEM calculation produced a non-finite value.");
+    if (code == dev::ERR_FIELD)
+        throw std::runtime_error("BEM field integration did not reach its near-field tolerance.");
+    if (code == dev::ERR_ESCAPED)
+        throw std::runtime_error("BEM dense regular pair escaped its qualified adaptive plan.");
+}
+
+bool point_inside(const std::vector<Panel> &panels, Vec3 p) {
+    // Solid-angle sum (cuda_solver.cu:679-704): inside if |Omega| > 2 pi; points on a panel count as inside.
+    double omega = 0;
+    for (const auto &pn : panels) {
+        const Vec3 e0 = vsub(pn.vertices[1], pn.vertices[0]), e1 = vsub(pn.vertices[2], pn.vertices[0]);
+        const Vec3 rel = vsub(p, pn.vertices[0]);
+        const double scale = std::max(std::sqrt(vnorm2(p)), std::sqrt(vnorm2(pn.vertices[0])));
+        const double tol = std::max(pn.max_edge * 1e-10, scale * 32 * std::numeric_limits<double>::epsilon());
+        if (std::abs(vdot(rel, pn.normal)) <= tol) {
+            const double aa = vdot(e0, e0), ab = vdot(e0, e1), bb = vdot(e1, e1), pa = vdot(rel, e0),
+                         pb = vdot(rel, e1), det = aa * bb - ab * ab;
+            const double u = (bb * pa - ab * pb) / det, v = (aa * pb - ab * pa) / det;
+            if (u >= -1e-10 && v >= -1e-10 && u + v <= 1 + 1e-10)
+                return true;
+        }
+        const Vec3 a = vsub(pn.vertices[0], p), b = vsub(pn.vertices[1], p), c = vsub(pn.vertices[2], p);
+        const double la = std::sqrt(vnorm2(a)), lb = std::sqrt(vnorm2(b)), lc = std::sqrt(vnorm2(c));
+        const double den = la * lb * lc + vdot(a, b) * lc + vdot(b, c) * la + vdot(c, a) * lb;
+        omega += 2 * std::atan2(vdot(a, vcross(b, c)), den);
+    }
+    return std::abs(omega) > 2 * std::numbers::pi;
+}
+
+std::vector<double> rule_table() {
+    std::vector<double> t;
+    for (unsigned d : {4u, 6u, 8u, 14u})
+        for (const auto &p : triangle_rule(d)) {
+            t.push_back(p.bary[0]);
+            t.push_back(p.bary[1]);
+            t.push_back(p.bary[2]);
+            t.push_back(p.weight);
+        }
+    return t;
+}
+
+} // namespace
+
+MeshSolver::MeshSolver(Context *ctx, SurfaceMesh mesh) : ctx_(ctx) {
+    if (mesh.vertices.empty())
+        throw std::invalid_argument("BEM mesh has no vertices.");
+    origin_ = mesh.vertices.front();
+    for (auto &v : mesh.vertices) {
+        v = vsub(v, origin_);
+        if (!std::isfinite(v.x) || !std::isfinite(v.y) || !std::isfinite(v.z))
+            throw std::invalid_argument("BEM mesh coordinates are not finite.");
+    }
+    validate_closed_exterior(mesh);
+    space_ = build_space(mesh);
+    mesh_ = std::move(mesh);
+    build_clusters();
+    error_.resize(1);
+    rules_.upload(rule_table(), ctx_->stream);
+    hip_check(hipStreamSynchronize(ctx_->stream), "mesh upload");
+}
+
+void MeshSolver::build_clusters() {
+    const std::uint32_t nt = this->nt(), nv = this->nv();
+    Vec3 lo{INFINITY, INFINITY, INFINITY}, hi{-INFINITY, -INFINITY, -INFINITY};
+    for (const auto &p : space_.panels) {
+        lo = {std::min(lo.x, p.centroid.x), std::min(lo.y, p.centroid.y), std::min(lo.z, p.centroid.z)};
+        hi = {std::max(hi.x, p.centroid.x), std::max(hi.y, p.centroid.y), std::max(hi.z, p.centroid.z)};
+    }
+    const double ext = std::max({hi.x - lo.x, hi.y - lo.y, hi.z - lo.z, 1e-300});
+    std::vector<std::pair<std::uint64_t, std::uint32_t>> keyed(nt);
+    for (std::uint32_t t = 0; t < nt; ++t) {
+        const Vec3 c = space_.panels[t].centroid;
+        auto q = [&](double v, double l) {
+            return static_cast<std::uint64_t>(std::min(2097151.0, std::max(0.0, (v - l) / ext * 2097151.0)));
+        };
+        keyed[t] = {spread21(q(c.x, lo.x)) | spread21(q(c.y, lo.y)) << 1 | spread21(q(c.z, lo.z)) << 2, t};
+    }
+    std::sort(keyed.begin(), keyed.end());
+    constexpr std::size_t kPanels = 64, kNodes = 64;
+    std::vector<std::uint32_t> panel_offsets{0}, panels, node_offsets{0}, cluster_vertices;
+    std::vector<std::uint32_t> cur_nodes;
+    std::vector<std::uint32_t> cluster_of(nt);
+    auto flush = [&] {
+        cluster_vertices.insert(cluster_vertices.end(), cur_nodes.begin(), cur_nodes.end());
+        node_offsets.push_back(static_cast<std::uint32_t>(cluster_vertices.size()));
+        panel_offsets.push_back(static_cast<std::uint32_t>(panels.size()));
+        cur_nodes.clear();
+    };
+    for (const auto &[key, t] : keyed) {
+        std::size_t fresh = 0;
+        for (auto v : space_.panels[t].nodes)
+            fresh += std::find(cur_nodes.begin(), cur_nodes.end(), v) == cur_nodes.end();
+        if (panels.size() - panel_offsets.back() == kPanels || cur_nodes.size() + fresh > kNodes)
+            flush();
+        for (auto v : space_.panels[t].nodes)
+            if (std::find(cur_nodes.begin(), cur_nodes.end(), v) == cur_nodes.end())
+                cur_nodes.push_back(v);
+        cluster_of[t] = static_cast<std::uint32_t>(panel_offsets.size() - 1);
+        panels.push_back(t);
+    }
+    flush();
+    cluster_count_ = static_cast<std::uint32_t>(panel_offsets.size() - 1);
+    // internal numbering: first appearance in cluster order
+    internal_of_.assign(nv, UINT32_MAX);
+    vertex_of_.clear();
+    for (auto v : cluster_vertices)
+        if (internal_of_[v] == UINT32_MAX) {
+            internal_of_[v] = static_cast<std::uint32_t>(vertex_of_.size());
+            vertex_of_.push_back(v);
+        }
+    std::vector<std::uint32_t> cluster_nodes(cluster_vertices.size());
+    for (std::size_t i = 0; i < cluster_vertices.size(); ++i)
+        cluster_nodes[i] = internal_of_[cluster_vertices[i]];
+    std::vector<std::uint8_t> local(3 * static_cast<std::size_t>(nt));
+    std::vector<std::uint32_t> nodes(3 * static_cast<std::size_t>(nt));
+    std::vector<double> geo(dev::G_COUNT * static_cast<std::size_t>(nt)), bounds(dev::B_COUNT * static_cast<std::size_t>(nt));
+    std::vector<std::uint8_t> shape(nt);
+    for (std::uint32_t t = 0; t < nt; ++t) {
+        const Panel &p = space_.panels[t];
+        const std::uint32_t c = cluster_of[t];
+        for (int l = 0; l < 3; ++l) {
+            nodes[static_cast<std::size_t>(l) * nt + t] = internal_of_[p.nodes[l]];
+            const auto first = cluster_vertices.begin() + node_offsets[c], last = cluster_vertices.begin() + node_offsets[c + 1];
+            local[static_cast<std::size_t>(l) * nt + t] = static_cast<std::uint8_t>(std::find(first, last, p.nodes[l]) - first);
+        }
+        const Vec3 e1 = vsub(p.vertices[1], p.vertices[0]), e2 = vsub(p.vertices[2], p.vertices[0]);
+        const double g[dev::G_COUNT] = {p.vertices[0].x, p.vertices[0].y, p.vertices[0].z, e1.x, e1.y, e1.z, e2.x, e2.y,
+                                        e2.z, p.normal.x, p.normal.y, p.normal.z, p.curls[0].x, p.curls[0].y,
+                                        p.curls[0].z, p.curls[1].x, p.curls[1].y, p.curls[1].z, p.curls[2].x,
+                                        p.curls[2].y, p.curls[2].z, p.area};
+        for (int f = 0; f < dev::G_COUNT; ++f)
+            geo[static_cast<std::size_t>(f) * nt + t] = g[f];
+        const PanelBounds b = make_panel_bounds(p);
+        const double bb[dev::B_COUNT] = {b.center.x, b.center.y, b.center.z, b.radius, b.max_edge};
+        for (int f = 0; f < dev::B_COUNT; ++f)
+            bounds[static_cast<std::size_t>(f) * nt + t] = bb[f];
+        shape[t] = static_cast<std::uint8_t>(b.qualified_shape);
+    }
+    lumped_internal_.resize(nv);
+    std::vector<double> row_scale(nv);
+    for (std::uint32_t i = 0; i < nv; ++i) {
+        lumped_internal_[i] = space_.lumped_mass[vertex_of_[i]];
+        row_scale[i] = 1.0 / lumped_internal_[i];
+    }
+    hipStream_t s = ctx_->stream;
+    geo_.upload(geo, s);
+    bounds_.upload(bounds, s);
+    shape_.upload(shape, s);
+    nodes_.upload(nodes, s);
+    local_.upload(local, s);
+    row_scale_.upload(row_scale, s);
+    lumped_.upload(lumped_internal_, s);
+    cluster_panel_offsets_.upload(panel_offsets, s);
+    cluster_panels_.upload(panels, s);
+    cluster_node_offsets_.upload(node_offsets, s);
+    cluster_nodes_.upload(cluster_nodes, s);
+}
+
+InteractionPlan MeshSolver::plan(const Numerics &n, double k_max) {
+    QuadratureSettings q;
+    q.regular_order = n.quadrature_order;
+    q.singular_order = n.singular_order;
+    q.near_distance_ratio = n.near_ratio;
+    q.near_tolerance = n.near_tolerance;
+    q.near_max_depth = n.near_depth;
+    return make_interaction_plan(space_, q, k_max, n.plan_threads);
+}
+
+DevicePlan &MeshSolver::device_plan(const Numerics &n, double k_max, bool *built) {
+    const auto key = std::make_tuple(n.quadrature_order, n.singular_order, n.near_ratio, n.near_tolerance, n.near_depth,
+                                     k_max);
+    auto it = plans_.find(key);
+    if (it != plans_.end()) {
+        if (built)
+            *built = false;
+        return *it->second;
+    }
+    const auto t0 = Clock::now();
+    auto dp = std::make_unique<DevicePlan>();
+    dp->plan = plan(n, k_max);
+    hipStream_t s = ctx_->stream;
+    dp->touching.upload(dp->plan.touching, s);
+    dp->near.upload(dp->plan.near, s);
+    dp->patches.upload(dp->plan.patches, s);
+    dp->near_offsets.upload(dp->plan.near_offsets, s);
+    std::vector<std::uint32_t> trials = dp->plan.near_trials;
+    if (trials.empty())
+        trials.push_back(0);
+    dp->near_trials.upload(trials, s);
+    if (!dp->plan.whole_panel_degrees.empty())
+        dp->whole.upload(dp->plan.whole_panel_degrees, s);
+    std::vector<double> g;
+    for (const auto &p : gauss_legendre01(n.singular_order)) {
+        g.push_back(p.position);
+        g.push_back(p.weight);
+    }
+    dp->gauss.upload(g, s);
+    hip_check(hipStreamSynchronize(s), "plan upload");
+    dp->plan_ms = ms_since(t0);
+    if (built)
+        *built = true;
+    auto &ref = *dp;
+    plans_.emplace(key, std::move(dp));
+    return ref;
+}
+
+dev::AssemblyLaunch MeshSolver::assembly_launch(const DevicePlan &p, const Numerics &n, double k, double *A, double *BD,
+                                                std::uint32_t nd, const std::int32_t *flux_column) {
+    dev::AssemblyLaunch L{};
+    L.args.panels = {geo_.data(), nodes_.data(), bounds_.data(), shape_.data(), nt(), nv()};
+    L.args.row_scale = row_scale_.data();
+    L.args.flux_column = flux_column;
+    L.args.A = A;
+    L.args.BD = BD;
+    L.args.nd = nd;
+    L.args.k = k;
+    L.args.tolerance = n.near_tolerance;
+    L.args.regular_order = n.quadrature_order;
+    L.args.error = error_.data();
+    L.clusters = {cluster_panel_offsets_.data(), cluster_panels_.data(), cluster_node_offsets_.data(),
+                  cluster_nodes_.data(), local_.data(), cluster_count_};
+    L.near = {p.near_offsets.data(), p.near_trials.data(), p.whole.size() ? p.whole.data() : nullptr};
+    L.rules = rules_.data();
+    L.touching = p.touching.data();
+    L.touching_count = static_cast<std::uint32_t>(p.plan.touching.size());
+    L.gauss = p.gauss.data();
+    L.singular_order = n.singular_order;
+    L.near_jobs = p.near.data();
+    L.near_count = static_cast<std::uint32_t>(p.plan.near.size());
+    L.patches = p.patches.data();
+    return L;
+}
+
+void MeshSolver::assemble_dense(const Numerics &n, double k_max, double k, double *A_out, double *B_out) {
+    std::lock_guard lock(mutex_);
+    DevicePlan &p = device_plan(n, k_max, nullptr);
+    const std::uint32_t nv = this->nv(), nt = this->nt();
+    hipStream_t s = ctx_->stream;
+    DeviceBuffer<double> A(2 * static_cast<std::size_t>(nv) * nv), BD(2 * static_cast<std::size_t>(nv) * nt);
+    std::vector<std::int32_t> cols(nt);
+    std::iota(cols.begin(), cols.end(), 0);
+    DeviceBuffer<std::int32_t> fc;
+    fc.upload(cols, s);
+    hip_check(hipMemsetAsync(A.data(), 0, A.bytes(), s), "clear A");
+    hip_check(hipMemsetAsync(BD.data(), 0, BD.bytes(), s), "clear B");
+    hip_check(hipMemsetAsync(error_.data(), 0, sizeof(int), s), "clear error");
+    dev::launch_assembly(assembly_launch(p, n, k, A.data(), BD.data(), nt, fc.data()), s);
+    hip_check(hipGetLastError(), "assembly launch");
+    check_error_flag(error_, s);
+    std::vector<double> a(A.size()), b(BD.size());
+    hip_check(hipMemcpyAsync(a.data(), A.data(), A.bytes(), hipMemcpyDeviceToHost, s), "A download");
+    hip_check(hipMemcpyAsync(b.data(), BD.data(), BD.bytes(), hipMemcpyDeviceToHost, s), "B download");
+    hip_check(hipStreamSynchronize(s), "assembly download");
+    // internal node order -> original vertex order
+    for (std::uint32_t c = 0; c < nv; ++c)
+        for (std::uint32_t r = 0; r < nv; ++r) {
+            const std::size_t src = 2 * (static_cast<std::size_t>(c) * nv + r);
+            const std::size_t dst = 2 * (static_cast<std::size_t>(vertex_of_[c]) * nv + vertex_of_[r]);
+            A_out[dst] = a[src];
+            A_out[dst + 1] = a[src + 1];
+        }
+    for (std::uint32_t c = 0; c < nt; ++c)
+        for (std::uint32_t r = 0; r < nv; ++r) {
+            const std::size_t src = 2 * (static_cast<std::size_t>(c) * nv + r);
+            const std::size_t dst = 2 * (static_cast<std::size_t>(c) * nv + vertex_of_[r]);
+            B_out[dst] = b[src];
+            B_out[dst + 1] = b[src + 1];
+        }
+}
+
+// Restarted GMRES with twice classical Gram-Schmidt on the device; host
+// Givens rotations (device_gmres.cu:197-279, without a preconditioner: the
+// dense system is already row-equilibrated by the lumped mass).
+void dense_gmres(hipStream_t s, const double *A, std::uint32_t n, const double *b, double *x, std::uint32_t restart,
+                 std::uint32_t max_it, double tol, double *achieved, std::uint32_t *iterations,
+                 const volatile std::int32_t *cancel, GmresWorkspace &ws) {
+    using C = std::complex<double>;
+    auto &basis_ = ws.basis;
+    auto &r_ = ws.r;
+    auto &scratch_ = ws.scratch;
+    if (basis_.size() < 2 * static_cast<std::size_t>(n) * (restart + 1))
+        basis_.resize(2 * static_cast<std::size_t>(n) * (restart + 1));
+    if (r_.size() < 2 * static_cast<std::size_t>(n))
+        r_.resize(2 * static_cast<std::size_t>(n));
+    const std::size_t scratch = std::max(dev::gemv_scratch_doubles(n, 1), dev::reduction_scratch_doubles(n, restart + 1));
+    if (scratch_.size() < scratch)
+        scratch_.resize(scratch);
+    if (ws.small.size() < 4 * (restart + 2) + 8)
+        ws.small.resize(4 * (restart + 2) + 8);
+    auto &small = ws.small;
+    double *h1 = small.data(), *h2 = small.data() + 2 * (restart + 1), *nrm = small.data() + 4 * (restart + 1);
+    std::vector<double> host(4 * (restart + 1) + 1);
+    auto norm = [&](const double *v) {
+        dev::launch_norm2(v, n, nullptr, nrm, scratch_.data(), s);
+        double out = 0;
+        hip_check(hipMemcpyAsync(&out, nrm, sizeof(double), hipMemcpyDeviceToHost, s), "norm download");
+        hip_check(hipStreamSynchronize(s), "norm sync");
+        return std::sqrt(out);
+    };
+    hip_check(hipMemsetAsync(x, 0, 2 * sizeof(double) * n, s), "gmres clear");
+    const double bnorm = norm(b);
+    *iterations = 0;
+    if (bnorm == 0) {
+        *achieved = 0;
+        return;
+    }
+    const double target = tol * bnorm;
+    hip_check(hipMemcpyAsync(r_.data(), b, 2 * sizeof(double) * n, hipMemcpyDeviceToDevice, s), "gmres copy");
+    double beta = bnorm;
+    const std::size_t stride = restart + 1;
+    std::vector<C> H(stride * restart), g(stride), sn(restart);
+    std::vector<double> cs(restart);
+    std::uint32_t it = 0;
+    for (;;) {
+        if (beta <= target) {
+            *achieved = beta / bnorm;
+            *iterations = it;
+            return;
+        }
+        if (it >= max_it)
+            break;
+        dev::launch_scale(n, 1.0 / beta, r_.data(), basis_.data(), s);
+        std::fill(H.begin(), H.end(), C{});
+        std::fill(g.begin(), g.end(), C{});
+        g[0] = beta;
+        std::uint32_t used = 0;
+        for (std::uint32_t j = 0; j < restart && it < max_it; ++j) {
+            if (cancel && *cancel)
+                throw Cancelled();
+            double *w = basis_.data() + 2 * static_cast<std::size_t>(j + 1) * n;
+            dev::launch_gemv(A, n, basis_.data() + 2 * static_cast<std::size_t>(j) * n, w, 1, scratch_.data(), s);
+            dev::launch_project(basis_.data(), n, j + 1, w, h1, scratch_.data(), s);
+            dev::launch_subtract_combination(basis_.data(), n, j + 1, h1, w, s);
+            dev::launch_project(basis_.data(), n, j + 1, w, h2, scratch_.data(), s);
+            dev::launch_subtract_combination(basis_.data(), n, j + 1, h2, w, s);
+            dev::launch_norm2(w, n, nullptr, nrm, scratch_.data(), s);
+            hip_check(hipMemcpyAsync(host.data(), small.data(), sizeof(double) * host.size(), hipMemcpyDeviceToHost, s),
+                      "arnoldi download");
+            hip_check(hipStreamSynchronize(s), "arnoldi sync");
+            C *col = H.data() + static_cast<std::size_t>(j) * stride;
+            for (std::uint32_t i = 0; i <= j; ++i)
+                col[i] = C(host[2 * i], host[2 * i + 1]) + C(host[2 * (restart + 1) + 2 * i], host[2 * (restart + 1) + 2 * i + 1]);
+            const double next = std::sqrt(host[4 * (restart + 1)]);
+            col[j + 1] = next;
+            const bool breakdown = !(next > 0);
+            if (!breakdown)
+                dev::launch_scale(n, 1.0 / next, w, w, s);
+            for (std::uint32_t i = 0; i < j; ++i) {
+                const C t = cs[i] * col[i] + sn[i] * col[i + 1];
+                col[i + 1] = -std::conj(sn[i]) * col[i] + cs[i] * col[i + 1];
+                col[i] = t;
+            }
+            const C a = col[j], bb = col[j + 1];
+            const double len = std::hypot(std::abs(a), std::abs(bb));
+            if (!(len > 0) || !std::isfinite(len))
+                throw std::runtime_error("GMRES Arnoldi factorization broke down.");
+            const C phase = std::abs(a) > 0 ? a / std::abs(a) : C(1, 0);
+            cs[j] = std::abs(a) / len;
+            sn[j] = phase * std::conj(bb) / len;
+            col[j] = phase * len;
+            col[j + 1] = 0;
+            const C t = cs[j] * g[j] + sn[j] * g[j + 1];
+            g[j + 1] = -std::conj(sn[j]) * g[j] + cs[j] * g[j + 1];
+            g[j] = t;
+            ++it;
+            used = j + 1;
+            if (std::abs(g[j + 1]) <= target || breakdown)
+                break;
+        }
+        std::vector<C> y(used);
+        for (std::uint32_t row = used; row-- > 0;) {
+            C v = g[row];
+            for (std::uint32_t c = row + 1; c < used; ++c)
+                v -= H[static_cast<std::size_t>(c) * stride + row] * y[c];
+            y[row] = v / H[static_cast<std::size_t>(row) * stride + row];
+        }
+        std::vector<double> neg(2 * used);
+        for (std::uint32_t j = 0; j < used; ++j) {
+            neg[2 * j] = -y[j].real();
+            neg[2 * j + 1] = -y[j].imag();
+        }
+        hip_check(hipMemcpyAsync(h1, neg.data(), sizeof(double) * neg.size(), hipMemcpyHostToDevice, s), "y upload");
+        dev::launch_subtract_combination(basis_.data(), n, used, h1, x, s);
+        dev::launch_residual(A, n, b, x, 1, r_.data(), scratch_.data(), s);
+        beta = norm(r_.data());
+    }
+    *achieved = beta / bnorm;
+    *iterations = it;
+    std::ostringstream msg;
+    msg << "GMRES did not converge after " << it << "/" << max_it << " iterations: relative residual " << *achieved;
+    throw std::runtime_error(msg.str());
+}
+
+Result MeshSolver::solve(const Request &req, ProgressFn progress, void *user, const volatile std::int32_t *cancel) {
+    std::lock_guard lock(mutex_);
+    using C = std::complex<double>;
+    const Numerics &num = req.numerics;
+    auto cancelled = [&] {
+        if (cancel && *cancel)
+            throw Cancelled();
+    };
+    cancelled();
+    if (!std::isfinite(req.density) || req.density <= 0 || !std::isfinite(req.sound_speed) || req.sound_speed <= 0 ||
+        req.frequencies.empty() || req.excitations.empty() ||
+        (num.quadrature_order != 2 && num.quadrature_order != 4) || num.singular_order < 2 || num.singular_order > 20 ||
+        !std::isfinite(num.near_ratio) || num.near_ratio < 0 || num.near_ratio > 10 ||
+        !std::isfinite(num.near_tolerance) || num.near_tolerance < 1e-10 || num.near_tolerance > .01 ||
+        num.near_depth > 10 || !std::isfinite(num.residual_tolerance) || num.residual_tolerance < 1e-12 ||
+        num.residual_tolerance > .01 || num.refinement_steps > 10 || num.solver > 1 || num.gmres_restart < 2 ||
+        num.gmres_restart > 300 || num.gmres_max_iterations < 1)
+        throw std::invalid_argument("Invalid BEM medium, frequencies, excitation or Galerkin settings.");
+    if (req.excitations.size() > 1024 || req.observers.size() > 100000)
+        throw std::invalid_argument("BEM request exceeds indexing limits.");
+    const double k_max = planning_wavenumber(req.frequencies, req.planning_frequencies, req.sound_speed);
+    // observers (recentred, or unit directions about the far-field origin)
+    std::vector<Vec3> observers;
+    Vec3 far_origin{};
+    if (req.far_field) {
+        far_origin = vsub(req.far_origin, origin_);
+        if (!std::isfinite(far_origin.x) || !std::isfinite(far_origin.y) || !std::isfinite(far_origin.z) ||
+            !std::isfinite(req.far_reference_distance) || req.far_reference_distance <= 0 ||
+            !std::isfinite(k_max * req.far_reference_distance))
+            throw std::invalid_argument("Far-field origin and positive reference distance must be finite.");
+    }
+    for (const auto &w : req.observers) {
+        Vec3 p = vsub(w, req.far_field ? req.far_origin : origin_);
+        if (!std::isfinite(w.x) || !std::isfinite(w.y) || !std::isfinite(w.z) || !std::isfinite(p.x) ||
+            !std::isfinite(p.y) || !std::isfinite(p.z))
+            throw std::invalid_argument("BEM observation coordinates must be finite.");
+        if (req.far_field) {
+            const double d = std::hypot(p.x, p.y, p.z);
+            if (!(d > 0) || !std::isfinite(d))
+                throw std::invalid_argument("A far-field direction must differ from its origin.");
+            p = {p.x / d, p.y / d, p.z / d};
+        } else if (point_inside(space_.panels, p)) {
+            throw std::invalid_argument("BEM observation points must be finite and outside the solid.");
+        }
+        observers.push_back(p);
+    }
+    // boundary conditions per face and panel velocities per excitation
+    std::map<std::uint32_t, Boundary> conditions;
+    for (const auto &t : mesh_.triangles) {
+        const auto [it, inserted] = conditions.emplace(t.face_id, t.boundary);
+        if (!inserted && it->second != t.boundary)
+            throw std::invalid_argument("Conflicting boundary conditions on one face.");
+    }
+    const std::uint32_t nt = this->nt(), nv = this->nv(), ne = static_cast<std::uint32_t>(req.excitations.size());
+    std::set<std::uint32_t> active_faces;
+    std::vector<std::map<std::uint32_t, C>> face_velocity(ne);
+    for (std::uint32_t e = 0; e < ne; ++e) {
+        for (const auto &[face, v] : req.excitations[e].faces) {
+            if (!conditions.count(face) || conditions.at(face) != Boundary::driven || !std::isfinite(v.real()) ||
+                !std::isfinite(v.imag()) || !face_velocity[e].emplace(face, v).second)
+                throw std::invalid_argument("Excitation must assign each known Driven face at most once.");
+            active_faces.insert(face);
+        }
+        if (face_velocity[e].empty())
+            throw std::invalid_argument("Each excitation needs at least one assigned Driven face.");
+    }
+    std::vector<std::int32_t> flux_column(nt, -1);
+    std::vector<std::uint32_t> flux_panels;
+    for (std::uint32_t t = 0; t < nt; ++t)
+        if (active_faces.count(mesh_.triangles[t].face_id)) {
+            flux_column[t] = static_cast<std::int32_t>(flux_panels.size());
+            flux_panels.push_back(t);
+        }
+    const std::uint32_t nd = static_cast<std::uint32_t>(flux_panels.size());
+    hipStream_t s = ctx_->stream;
+    bool built = false;
+    DevicePlan &plan = device_plan(num, k_max, &built);
+    cancelled();
+    // workspace
+    const std::size_t nv2 = static_cast<std::size_t>(nv) * nv;
+    A_.resize(2 * nv2);
+    BD_.resize(2 * static_cast<std::size_t>(nv) * std::max<std::uint32_t>(nd, 1));
+    q_.resize(2 * static_cast<std::size_t>(std::max<std::uint32_t>(nd, 1)) * ne);
+    rhs_.resize(2 * static_cast<std::size_t>(nv) * ne);
+    x_.resize(2 * static_cast<std::size_t>(nv) * ne);
+    r_.resize(2 * static_cast<std::size_t>(nv) * ne);
+    flux_.resize(2 * static_cast<std::size_t>(nt) * ne);
+    flux_column_.upload(flux_column, s);
+    const std::size_t scratch = std::max({dev::gemv_scratch_doubles(nv, std::min<std::uint32_t>(ne, 8)),
+                                          dev::reduction_scratch_doubles(nv, num.gmres_restart + 1), std::size_t{4096}});
+    if (scratch_.size() < scratch)
+        scratch_.resize(scratch);
+    if (num.solver == 0) {
+        LU_.resize(2 * nv2);
+        pivots_.resize(nv);
+    }
+    if (!observers.empty()) {
+        std::vector<double> o;
+        for (const auto &p : observers) {
+            o.push_back(p.x);
+            o.push_back(p.y);
+            o.push_back(p.z);
+        }
+        obs_.upload(o, s);
+        field_.resize(2 * observers.size() * ne);
+        partials_.resize(dev::field_partial_doubles(nt, static_cast<std::uint32_t>(observers.size()), ne));
+    }
+    Result result;
+    result.nt = nt;
+    result.nv = nv;
+    result.nobs = static_cast<std::uint32_t>(observers.size());
+    result.nexc = ne;
+    result.peak_bytes = A_.bytes() + LU_.bytes() + BD_.bytes() + rhs_.bytes() + x_.bytes() + r_.bytes() +
+                        flux_.bytes() + scratch_.bytes() + gmres_.basis.bytes() + geo_.bytes() + plan.patches.bytes();
+    bool has_nonzero = false;
+    for (const auto &fv : face_velocity)
+        for (const auto &[f, v] : fv)
+            has_nonzero |= v != C(0);
+    hipEvent_t ev[4];
+    for (auto &e : ev)
+        hip_check(hipEventCreate(&e), "event create");
+    struct EventGuard {
+        hipEvent_t *e;
+        ~EventGuard() {
+            for (int i = 0; i < 4; ++i)
+                (void)hipEventDestroy(e[i]);
+        }
+    } guard{ev};
+    std::uint32_t completed = 0;
+    for (double f : req.frequencies) {
+        cancelled();
+        auto report = [&](std::int32_t phase) {
+            if (progress)
+                progress(user, completed, static_cast<std::uint32_t>(req.frequencies.size()), f, phase);
+        };
+        report(0);
+        FrequencyResult fr;
+        fr.frequency = f;
+        if (completed == 0)
+            fr.ms[5] = built ? plan.plan_ms : 0;
+        const double omega = 2 * std::numbers::pi * f, k = omega / req.sound_speed;
+        double max_kh = 0;
+        for (const auto &p : space_.panels)
+            max_kh = std::max(max_kh, k * p.max_edge);
+        // fluxes: q = -i rho omega v, per flux column and per panel
+        std::vector<double> qd(2 * static_cast<std::size_t>(std::max<std::uint32_t>(nd, 1)) * ne, 0.0),
+            qpanel(2 * static_cast<std::size_t>(nt) * ne, 0.0);
+        for (std::uint32_t e = 0; e < ne; ++e)
+            for (std::uint32_t t = 0; t < nt; ++t) {
+                const auto it = face_velocity[e].find(mesh_.triangles[t].face_id);
+                if (it == face_velocity[e].end())
+                    continue;
+                const C q = C(0, -req.density * omega) * it->second;
+                qpanel[2 * (static_cast<std::size_t>(e) * nt + t)] = q.real();
+                qpanel[2 * (static_cast<std::size_t>(e) * nt + t) + 1] = q.imag();
+                if (flux_column[t] >= 0) {
+                    qd[2 * (static_cast<std::size_t>(e) * nd + flux_column[t])] = q.real();
+                    qd[2 * (static_cast<std::size_t>(e) * nd + flux_column[t]) + 1] = q.imag();
+                }
+            }
+        hip_check(hipMemcpyAsync(q_.data(), qd.data(), sizeof(double) * qd.size(), hipMemcpyHostToDevice, s), "q upload");
+        hip_check(hipMemcpyAsync(flux_.data(), qpanel.data(), sizeof(double) * qpanel.size(), hipMemcpyHostToDevice, s),
+                  "flux upload");
+        const auto t_asm = Clock::now();
+        hip_check(hipMemsetAsync(A_.data(), 0, A_.bytes(), s), "clear A");
+        hip_check(hipMemsetAsync(BD_.data(), 0, BD_.bytes(), s), "clear B");
+        hip_check(hipMemsetAsync(error_.data(), 0, sizeof(int), s), "clear error");
+        const auto L = assembly_launch(plan, num, k, A_.data(), BD_.data(), nd, flux_column_.data());
+        hip_check(hipEventRecord(ev[0], s), "event");
+        dev::launch_regular(L, s);
+        hip_check(hipEventRecord(ev[1], s), "event");
+        dev::launch_touching(L, s);
+        hip_check(hipEventRecord(ev[2], s), "event");
+        dev::launch_near(L, s);
+        hip_check(hipEventRecord(ev[3], s), "event");
+        dev::launch_mass(L, s);
+        hip_check(hipGetLastError(), "assembly launch");
+        if (nd)
+            dev::launch_flux_rhs(BD_.data(), nv, nd, q_.data(), ne, rhs_.data(), s);
+        else
+            hip_check(hipMemsetAsync(rhs_.data(), 0, rhs_.bytes(), s), "clear rhs");
+        check_error_flag(error_, s);
+        float e01 = 0, e12 = 0, e23 = 0;
+        hip_check(hipEventElapsedTime(&e01, ev[0], ev[1]), "event time");
+        hip_check(hipEventElapsedTime(&e12, ev[1], ev[2]), "event time");
+        hip_check(hipEventElapsedTime(&e23, ev[2], ev[3]), "event time");
+        fr.regular_kernel_ms = e01;
+        fr.touching_kernel_ms = e12;
+        fr.near_kernel_ms = e23;
+        fr.ms[0] = ms_since(t_asm);
+        cancelled();
+        // mass-weighted rhs norms (for the unscaled relative residual)
+        std::vector<double> rhs_norm(ne);
+        for (std::uint32_t e = 0; e < ne; ++e) {
+            dev::launch_norm2(rhs_.data() + 2 * static_cast<std::size_t>(e) * nv, nv, lumped_.data(), scratch_.data() + scratch - 8,
+                              scratch_.data(), s);
+            double v = 0;
+            hip_check(hipMemcpyAsync(&v, scratch_.data() + scratch - 8, sizeof(double), hipMemcpyDeviceToHost, s), "norm");
+            hip_check(hipStreamSynchronize(s), "norm sync");
+            rhs_norm[e] = std::sqrt(v);
+            if (!std::isfinite(rhs_norm[e]))
+                throw std::runtime_error("Galerkin right-hand side is not finite.");
+        }
+        report(3);
+        const auto t_solve = Clock::now();
+        std::vector<std::uint32_t> iterations(ne, 0);
+        std::uint32_t refinements = 0;
+        if (num.solver == 0) {
+            hip_check(hipMemcpyAsync(LU_.data(), A_.data(), A_.bytes(), hipMemcpyDeviceToDevice, s), "LU copy");
+            hip_check(hipMemsetAsync(error_.data(), 0, sizeof(int), s), "clear info");
+            const auto t_f = Clock::now();
+            dev::lu_factor(LU_.data(), nv, pivots_.data(), error_.data(), nullptr, s);
+            int info = 0;
+            hip_check(hipMemcpyAsync(&info, error_.data(), sizeof(int), hipMemcpyDeviceToHost, s), "info");
+            hip_check(hipStreamSynchronize(s), "LU sync");
+            fr.ms[2] = ms_since(t_f);
+            if (info)
+                throw std::runtime_error("Burton-Miller factorization failed at pivot " + std::to_string(info));
+            const auto t_t = Clock::now();
+            hip_check(hipMemcpyAsync(x_.data(), rhs_.data(), rhs_.bytes(), hipMemcpyDeviceToDevice, s), "rhs copy");
+            dev::lu_solve(LU_.data(), nv, pivots_.data(), x_.data(), ne, s);
+            hip_check(hipStreamSynchronize(s), "solve sync");
+            fr.ms[3] = ms_since(t_t);
+        } else {
+            for (std::uint32_t e = 0; e < ne; ++e) {
+                double achieved = 0;
+                dense_gmres(s, A_.data(), nv, rhs_.data() + 2 * static_cast<std::size_t>(e) * nv,
+                            x_.data() + 2 * static_cast<std::size_t>(e) * nv, num.gmres_restart,
+                            num.gmres_max_iterations, num.residual_tolerance * 0.1, &achieved, &iterations[e], cancel,
+                            gmres_);
+            }
+        }
+        // FP64 residual of the unscaled equation; LU corrections while above tolerance
+        std::vector<double> relative(ne);
+        for (;;) {
+            dev::launch_residual(A_.data(), nv, rhs_.data(), x_.data(), ne, r_.data(), scratch_.data(), s);
+            double largest = 0;
+            for (std::uint32_t e = 0; e < ne; ++e) {
+                dev::launch_norm2(r_.data() + 2 * static_cast<std::size_t>(e) * nv, nv, lumped_.data(),
+                                  scratch_.data() + scratch - 8, scratch_.data(), s);
+                double v = 0;
+                hip_check(hipMemcpyAsync(&v, scratch_.data() + scratch - 8, sizeof(double), hipMemcpyDeviceToHost, s),
+                          "residual norm");
+                hip_check(hipStreamSynchronize(s), "residual sync");
+                relative[e] = rhs_norm[e] > 0 ? std::sqrt(v) / rhs_norm[e]
+                                               : (v == 0 ? 0 : std::numeric_limits<double>::infinity());
+                if (!std::isfinite(relative[e]))
+                    throw std::runtime_error("Burton-Miller residual is not finite.");
+                largest = std::max(largest, relative[e]);
+            }
+            if (largest <= num.residual_tolerance)
+                break;
+            if (num.solver != 0 || refinements >= num.refinement_steps) {
+                std::ostringstream msg;
+                msg.precision(9);
+                msg << "Burton-Miller residual " << largest << " exceeds tolerance " << num.residual_tolerance << " at "
+                    << f << " Hz after " << refinements << " refinement steps.";
+                throw std::runtime_error(msg.str());
+            }
+            dev::lu_solve(LU_.data(), nv, pivots_.data(), r_.data(), ne, s);
+            for (std::uint32_t e = 0; e < ne; ++e) {
+                const double one[2] = {1, 0};
+                DeviceBuffer<double> alpha;
+                alpha.upload(std::vector<double>(one, one + 2), s);
+                dev::launch_axpy(nv, alpha.data(), r_.data() + 2 * static_cast<std::size_t>(e) * nv,
+                                 x_.data() + 2 * static_cast<std::size_t>(e) * nv, s);
+                hip_check(hipStreamSynchronize(s), "refinement sync");
+            }
+            ++refinements;
+        }
+        fr.ms[1] = ms_since(t_solve);
+        // fields
+        report(5);
+        const auto t_field = Clock::now();
+        std::vector<double> observed;
+        if (!observers.empty() && has_nonzero) {
+            auto evaluate = [&](unsigned degree) {
+                cancelled();
+                const auto rule = triangle_rule(degree);
+                std::vector<double> rt;
+                for (const auto &p : rule) {
+                    rt.push_back(p.bary[0]);
+                    rt.push_back(p.bary[1]);
+                    rt.push_back(p.bary[2]);
+                    rt.push_back(p.weight);
+                }
+                DeviceBuffer<double> drule;
+                drule.upload(rt, s);
+                dev::FieldArgs fa{};
+                fa.panels = {geo_.data(), nodes_.data(), bounds_.data(), shape_.data(), nt, nv};
+                fa.pressure = x_.data();
+                fa.flux = flux_.data();
+                fa.nrhs = ne;
+                fa.points = obs_.data();
+                fa.count = static_cast<std::uint32_t>(observers.size());
+                fa.rule = drule.data();
+                fa.rule_count = static_cast<std::uint32_t>(rule.size());
+                fa.k = k;
+                fa.near_tolerance = std::min(num.near_tolerance, .025 * num.residual_tolerance);
+                fa.max_depth = num.near_depth;
+                fa.origin[0] = far_origin.x;
+                fa.origin[1] = far_origin.y;
+                fa.origin[2] = far_origin.z;
+                fa.reference_distance = req.far_reference_distance;
+                fa.out = field_.data();
+                fa.error = error_.data();
+                hip_check(hipMemsetAsync(error_.data(), 0, sizeof(int), s), "clear error");
+                if (req.far_field)
+                    dev::launch_far_field(fa, partials_.data(), s);
+                else
+                    dev::launch_observer_field(fa, partials_.data(), s);
+                hip_check(hipGetLastError(), "field launch");
+                check_error_flag(error_, s);
+                std::vector<double> out(field_.size());
+                hip_check(hipMemcpyAsync(out.data(), field_.data(), field_.bytes(), hipMemcpyDeviceToHost, s),
+                          "field download");
+                hip_check(hipStreamSynchronize(s), "field sync");
+                return out;
+            };
+            unsigned degree = minimum_endpoint_degree(max_kh, num.quadrature_order);
+            observed = evaluate(degree);
+            for (;;) {
+                const unsigned next = refined_endpoint_degree(degree);
+                auto refined = evaluate(next);
+                bool passed = true;
+                for (std::uint32_t e = 0; e < ne; ++e) {
+                    double diff = 0, norm = 0;
+                    for (std::size_t j = 0; j < observers.size(); ++j) {
+                        const std::size_t i = 2 * (e * observers.size() + j);
+                        const C hi(refined[i], refined[i + 1]), lo(observed[i], observed[i + 1]);
+                        diff += std::norm(hi - lo);
+                        norm += std::norm(hi);
+                    }
+                    const double ind = norm > 0 ? std::sqrt(diff / norm) : diff == 0 ? 0 : INFINITY;
+                    passed = passed && ind <= .025 * num.residual_tolerance;
+                }
+                observed = std::move(refined);
+                if (passed)
+                    break;
+                if (degree == 12)
+                    throw std::runtime_error("BEM complex observer-field quadrature 12/14 did not converge within "
+                                             "0.025*residualTolerance; refine the mesh or observer separation.");
+                degree = next;
+            }
+        } else if (!observers.empty()) {
+            observed.assign(2 * observers.size() * ne, 0.0);
+        }
+        // nodal pressures to host
+        std::vector<double> nodal(2 * static_cast<std::size_t>(nv) * ne);
+        hip_check(hipMemcpyAsync(nodal.data(), x_.data(), sizeof(double) * nodal.size(), hipMemcpyDeviceToHost, s),
+                  "pressure download");
+        hip_check(hipStreamSynchronize(s), "pressure sync");
+        fr.ms[4] = ms_since(t_field);
+        for (std::uint32_t e = 0; e < ne; ++e) {
+            ExcitationResult er;
+            er.residual = relative[e];
+            er.iterations = iterations[e];
+            er.refinements = refinements;
+            er.pressure_trace.resize(nv);
+            for (std::uint32_t i = 0; i < nv; ++i)
+                er.pressure_trace[vertex_of_[i]] = C(nodal[2 * (static_cast<std::size_t>(e) * nv + i)],
+                                                     nodal[2 * (static_cast<std::size_t>(e) * nv + i) + 1]);
+            er.surface_pressure.resize(nt);
+            er.normal_derivative.resize(nt);
+            for (std::uint32_t t = 0; t < nt; ++t) {
+                C avg = 0;
+                for (auto v : space_.panels[t].nodes)
+                    avg += er.pressure_trace[v] / 3.0;
+                if (!std::isfinite(avg.real()) || !std::isfinite(avg.imag()))
+                    throw std::runtime_error("Burton-Miller pressure is not finite.");
+                er.surface_pressure[t] = avg;
+                const auto it = face_velocity[e].find(mesh_.triangles[t].face_id);
+                er.normal_derivative[t] = it == face_velocity[e].end() ? C{} : C(0, -req.density * omega) * it->second;
+            }
+            for (std::size_t j = 0; j < observers.size(); ++j)
+                er.observer_pressure.emplace_back(observed[2 * (e * observers.size() + j)],
+                                                  observed[2 * (e * observers.size() + j) + 1]);
+            for (const auto &[face, v] : req.excitations[e].faces) {
+                if (std::abs(v) == 0)
+                    continue;
+                double area = 0;
+                C integrated = 0;
+                for (std::uint32_t t = 0; t < nt; ++t)
+                    if (mesh_.triangles[t].face_id == face) {
+                        area += space_.panels[t].area;
+                        integrated += er.surface_pressure[t] * space_.panels[t].area;
+                    }
+                const C mean = integrated / area, volume = v * area;
+                er.impedances.push_back({face, area, mean, volume, mean / volume});
+            }
+            fr.excitations.push_back(std::move(er));
+        }
+        result.frequencies.push_back(std::move(fr));
+        ++completed;
+        report(6);
+    }
+    return result;
+}
+
+} // namespace hsbem

@@ -1,0 +1,189 @@
+/*
+ * hsbem.h - C ABI of the MI355X Burton-Miller Galerkin BEM solver.
+ *
+ * The boundary replaces hornsculpt::CudaBemSolver
+ * (reference: modules/bem/include/hornsculpt/bem.hpp:367-388):
+ *   CudaBemSolver(SurfaceMesh)            -> hsbem_mesh_create      (bem.hpp:369)
+ *   CudaBemSolver::solve(BemRequest, ...) -> hsbem_solve            (bem.hpp:384-385)
+ *   BemResult / BemFrequencyResult / BemExcitationResult accessors
+ *                                          -> hsbem_result_*        (bem.hpp:193-332)
+ *   static deviceDescription()            -> hsbem_device_description (bem.hpp:386)
+ * and exposes the internal seams the reference tests drive directly:
+ *   galerkin::makeInteractionPlan         -> hsbem_plan_*  (galerkin/interaction_plan.hpp:68-71)
+ *   dense Galerkin assembly kernels       -> hsbem_assemble (cuda/cuda_solver.cu:266-540)
+ *   cuda_detail::DeviceGmresSolver        -> hsbem_dense_gmres (cuda/device_gmres.hpp:16-34)
+ *   cusolverDnCgetrf/Cgetrs dense solve   -> hsbem_dense_lu_solve (cuda/cuda_solver.cu:1485-1513)
+ *
+ * Conventions (bem.hpp:356): RMS phasors with exp(+i omega t), outgoing
+ * exp(-ikr), q = dp/dn = -i rho omega v, Burton-Miller alpha = -i/k.
+ * Complex numbers are (re, im) double pairs. Every matrix is column-major.
+ * Every call returns an hsbem_status; on failure hsbem_last_error() gives the
+ * message of the calling thread (std::invalid_argument -> HSBEM_INVALID_ARGUMENT,
+ * std::runtime_error -> HSBEM_RUNTIME_ERROR, as the reference throws).
+ */
+#ifndef HSBEM_H
+#define HSBEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HSBEM_API __attribute__((visibility("default")))
+
+typedef enum {
+    HSBEM_OK = 0,
+    HSBEM_INVALID_ARGUMENT = 1, /* reference: std::invalid_argument */
+    HSBEM_RUNTIME_ERROR = 2,    /* reference: std::runtime_error (numerics, budget, convergence) */
+    HSBEM_DEVICE_ERROR = 3,     /* HIP runtime failure */
+    HSBEM_CANCELLED = 4         /* reference: "BEM calculation cancelled." */
+} hsbem_status;
+
+enum { HSBEM_SOLVER_LU = 0, HSBEM_SOLVER_GMRES = 1 };
+enum { HSBEM_PRECISION_FP64 = 0, HSBEM_PRECISION_MIXED = 1 };
+enum { HSBEM_BOUNDARY_WALL = 0, HSBEM_BOUNDARY_DRIVEN = 1 };
+
+/* BemSolvePhase (bem.hpp:334-342). */
+enum {
+    HSBEM_PHASE_PREPARING = 0,
+    HSBEM_PHASE_NEAR_CONSTRUCTION = 1,
+    HSBEM_PHASE_FAR_PREPARATION = 2,
+    HSBEM_PHASE_KRYLOV_SOLVE = 3,
+    HSBEM_PHASE_RESIDUAL_AUDIT = 4,
+    HSBEM_PHASE_FIELD_EVALUATION = 5,
+    HSBEM_PHASE_FREQUENCY_COMPLETE = 6
+};
+
+typedef struct hsbem_context hsbem_context;
+typedef struct hsbem_mesh hsbem_mesh;
+typedef struct hsbem_result hsbem_result;
+typedef struct hsbem_plan hsbem_plan;
+
+/* BemNumerics (bem.hpp:113-123) plus the MI355X solver choices. */
+typedef struct {
+    uint32_t quadrature_order;           /* 2 or 4 (default 4) */
+    uint32_t singular_order;             /* 2..20 (default 4) */
+    double near_distance_ratio;          /* 0..10 (default 1) */
+    double near_relative_tolerance;      /* 1e-10..1e-2 (default 1e-4) */
+    uint32_t near_maximum_depth;         /* <= 10 (default 6) */
+    double relative_residual_tolerance;  /* 1e-12..1e-2 (default 5e-5) */
+    uint32_t maximum_refinement_steps;   /* <= 10 (default 3) */
+    uint32_t solver;                     /* HSBEM_SOLVER_LU (default) or HSBEM_SOLVER_GMRES */
+    uint32_t gmres_restart;              /* default 50 */
+    uint32_t gmres_maximum_iterations;   /* default 1000 */
+    uint32_t precision;                  /* HSBEM_PRECISION_FP64 (default) or HSBEM_PRECISION_MIXED */
+    uint32_t plan_threads;               /* host planner threads, 0 = up to 8 */
+} hsbem_numerics;
+
+/* BemRequest (bem.hpp:163-191) for a closed exterior mesh. Excitation e assigns
+ * velocities[j] (RMS m/s, complex) to faces face_ids[j] for
+ * j in [excitation_offsets[e], excitation_offsets[e+1]). */
+typedef struct {
+    double density;                      /* kg/m^3 */
+    double sound_speed;                  /* m/s */
+    const double *frequencies;           /* Hz */
+    uint32_t frequency_count;
+    uint32_t excitation_count;           /* 1..1024 */
+    const uint32_t *excitation_offsets;  /* excitation_count + 1 entries */
+    const uint32_t *face_ids;
+    const double *velocities;            /* (re, im) per face entry */
+    const double *observers;             /* xyz, world frame */
+    uint32_t observer_count;             /* <= 100000 */
+    int32_t far_field;                   /* nonzero: asymptotic far field (FarFieldObservation) */
+    double far_origin[3];
+    double far_reference_distance;       /* m, default 1 */
+    const double *planning_frequencies;  /* optional raster (interactionPlanFrequencies) */
+    uint32_t planning_frequency_count;
+    hsbem_numerics numerics;
+} hsbem_request;
+
+typedef void (*hsbem_progress_fn)(void *user, uint32_t completed_frequencies, uint32_t total_frequencies,
+                                  double frequency, int32_t phase);
+
+HSBEM_API void hsbem_numerics_default(hsbem_numerics *numerics);
+HSBEM_API const char *hsbem_last_error(void);
+HSBEM_API int hsbem_device_description(char *buffer, size_t length);
+
+HSBEM_API int hsbem_context_create(int32_t device, hsbem_context **context);
+HSBEM_API int hsbem_context_destroy(hsbem_context *context);
+
+/* Copies, recentres (vertex 0 at the origin) and validates the mesh (closed,
+ * consistently oriented, positive volume per component), builds the Galerkin
+ * space and uploads device-resident panel geometry. */
+HSBEM_API int hsbem_mesh_create(hsbem_context *context, const double *vertices, uint32_t vertex_count,
+                                const uint32_t *triangles, const uint32_t *face_ids, const uint8_t *boundary,
+                                uint32_t triangle_count, hsbem_mesh **mesh);
+HSBEM_API int hsbem_mesh_destroy(hsbem_mesh *mesh);
+/* 64-bit FNV-1a hash of the mesh arrays (state key for the serving loop). */
+HSBEM_API int hsbem_mesh_info(const hsbem_mesh *mesh, uint64_t *hash, uint32_t *vertex_count,
+                              uint32_t *triangle_count, double *surface_area);
+
+/* Solves every frequency of the request. cancel (optional) is polled between
+ * launches; a nonzero value ends the call with HSBEM_CANCELLED. */
+HSBEM_API int hsbem_solve(hsbem_mesh *mesh, const hsbem_request *request, hsbem_progress_fn progress, void *user,
+                          const volatile int32_t *cancel, hsbem_result **result);
+HSBEM_API int hsbem_result_destroy(hsbem_result *result);
+
+HSBEM_API int hsbem_result_sizes(const hsbem_result *result, uint32_t *frequency_count, uint32_t *excitation_count,
+                                 uint32_t *triangle_count, uint32_t *vertex_count, uint32_t *observer_count);
+/* DP0 triangle means of the P1 pressure (nt complex), BemBoundaryData::surfacePressure. */
+HSBEM_API int hsbem_result_surface_pressure(const hsbem_result *result, uint32_t frequency, uint32_t excitation,
+                                            double *out);
+/* P1 nodal pressure (nv complex), BemBoundaryData::pressureTrace. */
+HSBEM_API int hsbem_result_pressure_trace(const hsbem_result *result, uint32_t frequency, uint32_t excitation,
+                                          double *out);
+/* DP0 q = -i rho omega v (nt complex), BemBoundaryData::normalDerivative. */
+HSBEM_API int hsbem_result_normal_derivative(const hsbem_result *result, uint32_t frequency, uint32_t excitation,
+                                             double *out);
+HSBEM_API int hsbem_result_observer_pressure(const hsbem_result *result, uint32_t frequency, uint32_t excitation,
+                                             double *out);
+/* FaceImpedance list (bem.hpp:193-199): one entry per excited face with nonzero velocity.
+ * Pass null arrays to query *count. mean_pressure, volume_velocity, impedance are complex. */
+HSBEM_API int hsbem_result_impedances(const hsbem_result *result, uint32_t frequency, uint32_t excitation,
+                                      uint32_t *count, uint32_t *face_ids, double *area, double *mean_pressure,
+                                      double *volume_velocity, double *impedance);
+/* Relative L2 residual of the unscaled equation, Krylov iterations, refinement steps. */
+HSBEM_API int hsbem_result_residual(const hsbem_result *result, uint32_t frequency, uint32_t excitation,
+                                    double *residual, uint32_t *iterations, uint32_t *refinements);
+/* Milliseconds: assembly, solve, factorization, triangular solve, field, interaction plan. */
+HSBEM_API int hsbem_result_timings(const hsbem_result *result, uint32_t frequency, double *milliseconds);
+HSBEM_API int hsbem_result_peak_device_bytes(const hsbem_result *result, uint64_t *bytes);
+
+/* ---- internal seams (component tests, benchmarks) ---- */
+
+/* Interaction plan of the mesh for numerics at planning wavenumber k_max.
+ * sizes: touching, near jobs, patches, all-pairs bytes, candidate pairs, max depth.
+ * Copy layouts: touching 9 x uint32 (test, trial, topology, perm test[3], perm trial[3]);
+ * near 5 x uint32 (test, trial, first patch, patch count, swap); patch 22 doubles
+ * (test[9], trial[9], area fraction, degree, projection, intervals). */
+HSBEM_API int hsbem_plan_create(hsbem_mesh *mesh, const hsbem_numerics *numerics, double k_max, hsbem_plan **plan);
+/* Host-only variant on raw arrays (recentred on vertex 0 like hsbem_mesh_create); no device use. */
+HSBEM_API int hsbem_plan_build(const double *vertices, uint32_t vertex_count, const uint32_t *triangles,
+                               uint32_t triangle_count, const hsbem_numerics *numerics, double k_max,
+                               hsbem_plan **plan);
+HSBEM_API int hsbem_plan_sizes(const hsbem_plan *plan, uint64_t *sizes);
+HSBEM_API int hsbem_plan_copy(const hsbem_plan *plan, uint32_t *touching, uint32_t *near, double *patches,
+                              uint8_t *whole_panel_degrees, uint32_t *near_offsets, uint32_t *near_trials,
+                              double *maximum_estimated_error);
+HSBEM_API int hsbem_plan_destroy(hsbem_plan *plan);
+
+/* Device assembly of the row-scaled Galerkin system at wavenumber k with the
+ * plan for k_max: A (nv x nv) and B (nv x nt, every panel a flux column), both
+ * complex column-major, copied to host. */
+HSBEM_API int hsbem_assemble(hsbem_mesh *mesh, const hsbem_numerics *numerics, double k_max, double k, double *A,
+                             double *B);
+
+/* Restarted GMRES (CGS2 Arnoldi) on a dense complex n x n system on the device. */
+HSBEM_API int hsbem_dense_gmres(hsbem_context *context, uint32_t n, const double *A, const double *b, double *x,
+                                uint32_t restart, uint32_t maximum_iterations, double tolerance,
+                                double *relative_residual, uint32_t *iterations);
+/* Partial-pivot complex LU of a dense n x n matrix and solve of nrhs right-hand sides in place. */
+HSBEM_API int hsbem_dense_lu_solve(hsbem_context *context, uint32_t n, const double *A, uint32_t nrhs, double *B);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HSBEM_H */

@@ -1,0 +1,249 @@
+// Test-infrastructure only: a flat C interface over the reference planner
+// (modules/bem/galerkin/{space,pair_quadrature,interaction_plan}.cpp), compiled
+// from the reference sources by oracle/Makefile into oracle/_ref/. The product
+// never links or loads it; tests use it to pin the product planner and the
+// oracle's rules, Sauter-Schwab points and near-patch moments.
+#include "galerkin/interaction_plan.hpp"
+#include "galerkin/near_projection.hpp"
+#include "galerkin/pair_quadrature.hpp"
+#include "galerkin/regular_qualification.hpp"
+#include "galerkin/space.hpp"
+
+#include <cstdint>
+#include <cstring>
+#include <exception>
+#include <string>
+
+using namespace hornsculpt;
+using namespace hornsculpt::galerkin;
+
+namespace {
+thread_local std::string last_error;
+
+SurfaceMesh make_mesh(std::uint32_t nv, const double *xyz, std::uint32_t nt, const std::uint32_t *tri) {
+    SurfaceMesh m;
+    m.vertices.resize(nv);
+    for (std::uint32_t i = 0; i < nv; ++i)
+        m.vertices[i] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+    m.triangles.resize(nt);
+    for (std::uint32_t t = 0; t < nt; ++t)
+        m.triangles[t] = {{tri[3 * t], tri[3 * t + 1], tri[3 * t + 2]}, 1, BoundaryCondition::driven};
+    return m;
+}
+
+template <class F> int guarded(F &&f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        last_error = e.what();
+        return 1;
+    }
+}
+
+void pack_patch(const PairQuadraturePatch &p, double *out) {
+    for (int i = 0; i < 9; ++i) {
+        out[i] = p.test[i];
+        out[9 + i] = p.trial[i];
+    }
+    out[18] = p.areaFraction;
+    out[19] = p.degree;
+    out[20] = p.projection;
+    out[21] = p.projectionIntervals;
+}
+PairQuadraturePatch unpack_patch(const double *in) {
+    PairQuadraturePatch p{};
+    for (int i = 0; i < 9; ++i) {
+        p.test[i] = in[i];
+        p.trial[i] = in[9 + i];
+    }
+    p.areaFraction = in[18];
+    p.degree = static_cast<unsigned>(in[19]);
+    p.projection = static_cast<unsigned>(in[20]);
+    p.projectionIntervals = static_cast<unsigned>(in[21]);
+    return p;
+}
+
+struct PlanHandle {
+    InteractionPlan plan;
+};
+} // namespace
+
+extern "C" {
+
+const char *ref_last_error() { return last_error.c_str(); }
+
+// Panel data per triangle: area, max edge, normal[3], centroid[3], curls[9], gradients[9] (26 doubles),
+// and the lumped mass per vertex.
+int ref_space(std::uint32_t nv, const double *xyz, std::uint32_t nt, const std::uint32_t *tri, double *panel_out,
+              double *lumped_out) {
+    return guarded([&] {
+        const auto s = buildSpace(make_mesh(nv, xyz, nt, tri));
+        for (std::uint32_t t = 0; t < nt; ++t) {
+            const auto &p = s.panels[t];
+            double *o = panel_out + 26 * t;
+            o[0] = p.area;
+            o[1] = p.maximumEdgeLength;
+            o[2] = p.normal.x, o[3] = p.normal.y, o[4] = p.normal.z;
+            o[5] = p.centroid.x, o[6] = p.centroid.y, o[7] = p.centroid.z;
+            for (int l = 0; l < 3; ++l) {
+                o[8 + 3 * l] = p.curls[l].x, o[9 + 3 * l] = p.curls[l].y, o[10 + 3 * l] = p.curls[l].z;
+                o[17 + 3 * l] = p.gradients[l].x, o[18 + 3 * l] = p.gradients[l].y, o[19 + 3 * l] = p.gradients[l].z;
+            }
+        }
+        for (std::uint32_t v = 0; v < nv; ++v)
+            lumped_out[v] = s.lumpedMass[v];
+    });
+}
+
+int ref_triangle_rule(unsigned degree, double *out, unsigned *count) {
+    return guarded([&] {
+        const auto r = triangleRule(degree);
+        *count = static_cast<unsigned>(r.size());
+        if (out)
+            for (std::size_t i = 0; i < r.size(); ++i) {
+                for (int j = 0; j < 3; ++j)
+                    out[4 * i + j] = r[i].barycentric[j];
+                out[4 * i + 3] = r[i].weight;
+            }
+    });
+}
+
+int ref_gauss(unsigned order, double *out) {
+    return guarded([&] {
+        const auto g = gaussLegendre01(order);
+        for (std::size_t i = 0; i < g.size(); ++i) {
+            out[2 * i] = g[i].position;
+            out[2 * i + 1] = g[i].weight;
+        }
+    });
+}
+
+// Explicit pair rule of makePairQuadrature for the ordered pair (a, b) of a
+// mesh: touching pairs give Sauter-Schwab points in original vertex order.
+// Points: test[3], trial[3], weight (7 doubles each).
+int ref_pair_points(std::uint32_t nv, const double *xyz, std::uint32_t nt, const std::uint32_t *tri, std::uint32_t a,
+                    std::uint32_t b, unsigned singular_order, double *out, unsigned *count, int *topology) {
+    return guarded([&] {
+        const auto s = buildSpace(make_mesh(nv, xyz, nt, tri));
+        PairQuadratureSettings q;
+        q.singularOrder = singular_order;
+        q.nearDistanceRatio = 0;
+        const auto rule = makePairQuadrature(s.panels[a], s.panels[b], q, 0.0);
+        *topology = static_cast<int>(rule.topology);
+        *count = static_cast<unsigned>(rule.points.size());
+        if (out)
+            for (std::size_t i = 0; i < rule.points.size(); ++i) {
+                for (int j = 0; j < 3; ++j) {
+                    out[7 * i + j] = rule.points[i].test[j];
+                    out[7 * i + 3 + j] = rule.points[i].trial[j];
+                }
+                out[7 * i + 6] = rule.points[i].weight;
+            }
+    });
+}
+
+// estimatePairPatchCpu: 36 complex moments (re, im interleaved) and errors.
+int ref_estimate_patch(std::uint32_t nv, const double *xyz, std::uint32_t nt, const std::uint32_t *tri,
+                       std::uint32_t a, std::uint32_t b, const double *patch, double k, double *moments,
+                       double *errors) {
+    return guarded([&] {
+        const auto s = buildSpace(make_mesh(nv, xyz, nt, tri));
+        const auto e = estimatePairPatchCpu(s.panels[a], s.panels[b], unpack_patch(patch), k);
+        for (int i = 0; i < 36; ++i) {
+            moments[2 * i] = e.integral[i].real();
+            moments[2 * i + 1] = e.integral[i].imag();
+        }
+        errors[0] = e.absoluteError;
+        errors[1] = e.innerError;
+        errors[2] = e.outerError;
+    });
+}
+
+// One projection/tensor patch point as the device evaluation maps it.
+int ref_patch_point(std::uint32_t nv, const double *xyz, std::uint32_t nt, const std::uint32_t *tri, std::uint32_t a,
+                    std::uint32_t b, const double *patch, unsigned index, double *out) {
+    return guarded([&] {
+        const auto s = buildSpace(make_mesh(nv, xyz, nt, tri));
+        const auto p = unpack_patch(patch);
+        const auto rule = triangleRule(p.degree == 14 ? 14 : 8);
+        const auto pt = mapNearPatchPoint(rule.data(), static_cast<unsigned>(rule.size()), p, index, s.panels[a],
+                                          s.panels[b]);
+        for (int j = 0; j < 3; ++j) {
+            out[j] = pt.test[j];
+            out[3 + j] = pt.trial[j];
+        }
+        out[6] = pt.weight;
+    });
+}
+
+int ref_qualified_degree(std::uint32_t nv, const double *xyz, std::uint32_t nt, const std::uint32_t *tri,
+                         std::uint32_t a, std::uint32_t b, double k, double tolerance, unsigned minimum,
+                         unsigned *degree) {
+    return guarded([&] {
+        const auto s = buildSpace(make_mesh(nv, xyz, nt, tri));
+        *degree = qualifiedRegularDegree(makeRegularPanelBounds(s.panels[a]), makeRegularPanelBounds(s.panels[b]), k,
+                                         tolerance, minimum);
+    });
+}
+
+void *ref_plan_create(std::uint32_t nv, const double *xyz, std::uint32_t nt, const std::uint32_t *tri,
+                      unsigned regular_order, unsigned singular_order, double near_ratio, double tolerance,
+                      unsigned depth, double k_max, unsigned threads) {
+    PlanHandle *h = nullptr;
+    const int rc = guarded([&] {
+        const auto s = buildSpace(make_mesh(nv, xyz, nt, tri));
+        PairQuadratureSettings q;
+        q.regularOrder = regular_order;
+        q.singularOrder = singular_order;
+        q.nearDistanceRatio = near_ratio;
+        q.relativeTolerance = tolerance;
+        q.maximumSubdivisionDepth = depth;
+        h = new PlanHandle{makeInteractionPlan(s, q, k_max, {}, threads, std::size_t{16} << 30)};
+    });
+    return rc ? nullptr : h;
+}
+
+void ref_plan_sizes(void *handle, std::uint64_t *sizes) {
+    const auto &p = static_cast<PlanHandle *>(handle)->plan;
+    sizes[0] = p.touching.size();
+    sizes[1] = p.near.size();
+    sizes[2] = p.patches.size();
+    sizes[3] = p.wholePanelDegrees.size();
+    sizes[4] = p.candidatePairs;
+    sizes[5] = p.maximumDepth;
+}
+
+// touching: 8 uint32 per job (test, trial, topology, perm test[3], perm trial[3] packed as 3+3 -> 8 total)
+void ref_plan_copy(void *handle, std::uint32_t *touching, std::uint32_t *near, double *patches,
+                   std::uint8_t *whole, std::uint32_t *offsets, std::uint32_t *trials, double *max_error) {
+    const auto &p = static_cast<PlanHandle *>(handle)->plan;
+    for (std::size_t i = 0; i < p.touching.size(); ++i) {
+        const auto &j = p.touching[i];
+        std::uint32_t *o = touching + 9 * i;
+        o[0] = j.test;
+        o[1] = j.trial;
+        o[2] = static_cast<std::uint32_t>(j.permutation.topology);
+        for (int l = 0; l < 3; ++l) {
+            o[3 + l] = j.permutation.test[l];
+            o[6 + l] = j.permutation.trial[l];
+        }
+    }
+    for (std::size_t i = 0; i < p.near.size(); ++i) {
+        const auto &j = p.near[i];
+        std::uint32_t *o = near + 5 * i;
+        o[0] = j.test, o[1] = j.trial, o[2] = j.firstPatch, o[3] = j.patchCount, o[4] = j.swapBarycentric;
+    }
+    for (std::size_t i = 0; i < p.patches.size(); ++i)
+        pack_patch(p.patches[i], patches + 22 * i);
+    if (!p.wholePanelDegrees.empty())
+        std::memcpy(whole, p.wholePanelDegrees.data(), p.wholePanelDegrees.size());
+    for (std::size_t i = 0; i < p.nearOffsets.size(); ++i)
+        offsets[i] = p.nearOffsets[i];
+    for (std::size_t i = 0; i < p.nearTrials.size(); ++i)
+        trials[i] = p.nearTrials[i];
+    *max_error = p.maximumEstimatedRelativeError;
+}
+
+void ref_plan_destroy(void *handle) { delete static_cast<PlanHandle *>(handle); }
+}

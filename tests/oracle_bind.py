@@ -1,0 +1,217 @@
+"""ctypes bindings of the test oracle (oracle/liboracle.so) and of the reference
+planner built from the reference sources (oracle/_ref/libref_galerkin.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.
+"""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE = ROOT / "oracle" / "liboracle.so"
+REF = ROOT / "oracle" / "_ref" / "libref_galerkin.so"
+
+_D = ctypes.POINTER(ctypes.c_double)
+_U = ctypes.POINTER(ctypes.c_uint32)
+_U8 = ctypes.POINTER(ctypes.c_uint8)
+
+
+def _d(a):
+    return a.ctypes.data_as(_D)
+
+
+def _u(a):
+    return a.ctypes.data_as(_U)
+
+
+def oracle():
+    if not ORACLE.exists():
+        subprocess.run(["make", "-C", str(ROOT / "oracle"), "liboracle.so"], check=True, capture_output=True)
+    lib = ctypes.CDLL(str(ORACLE))
+    lib.oracle_last_error.restype = ctypes.c_char_p
+    return lib
+
+
+def reference():
+    """The reference planner library, or None where it is not built."""
+    if not REF.exists():
+        return None
+    lib = ctypes.CDLL(str(REF))
+    lib.ref_last_error.restype = ctypes.c_char_p
+    lib.ref_plan_create.restype = ctypes.c_void_p
+    lib.ref_plan_create.argtypes = [ctypes.c_uint32, _D, ctypes.c_uint32, _U, ctypes.c_uint, ctypes.c_uint,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_uint, ctypes.c_double, ctypes.c_uint]
+    lib.ref_plan_sizes.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
+    lib.ref_plan_copy.argtypes = [ctypes.c_void_p, _U, _U, _D, _U8, _U, _U, _D]
+    lib.ref_plan_destroy.argtypes = [ctypes.c_void_p]
+    return lib
+
+
+def _mesh_args(mesh):
+    V = np.ascontiguousarray(mesh.vertices, dtype=np.float64)
+    T = np.ascontiguousarray(mesh.triangles, dtype=np.uint32)
+    return V, T
+
+
+def recentred(mesh):
+    """Vertices shifted so vertex 0 is the origin (what the solver does internally)."""
+    V = np.asarray(mesh.vertices, float)
+    return V - V[0]
+
+
+def _check(lib, rc, name):
+    if rc != 0:
+        err = lib.oracle_last_error() if hasattr(lib, "oracle_last_error") else lib.ref_last_error()
+        raise RuntimeError(f"{name}: {err.decode()}")
+
+
+def ref_plan(mesh, quadrature_order=4, singular_order=4, near_ratio=1.0, tolerance=1e-4, depth=6, k_max=0.0,
+             threads=8, vertices=None):
+    lib = reference()
+    if lib is None:
+        return None
+    V = np.ascontiguousarray(recentred(mesh) if vertices is None else vertices)
+    T = np.ascontiguousarray(mesh.triangles, dtype=np.uint32)
+    h = lib.ref_plan_create(len(V), _d(V), len(T), _u(T), quadrature_order, singular_order, near_ratio, tolerance,
+                            depth, k_max, threads)
+    if not h:
+        raise RuntimeError(lib.ref_last_error().decode())
+    try:
+        sizes = np.zeros(6, np.uint64)
+        lib.ref_plan_sizes(h, sizes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+        nt_, nn, npatch, nwhole = (int(x) for x in sizes[:4])
+        touching = np.zeros((max(nt_, 1), 9), np.uint32)
+        near = np.zeros((max(nn, 1), 5), np.uint32)
+        patches = np.zeros((max(npatch, 1), 22))
+        whole = np.zeros(max(nwhole, 1), np.uint8)
+        offsets = np.zeros(len(T) + 1, np.uint32)
+        trials = np.zeros(max(nn, 1), np.uint32)
+        err = ctypes.c_double()
+        lib.ref_plan_copy(h, _u(touching), _u(near), _d(patches), whole.ctypes.data_as(_U8), _u(offsets), _u(trials),
+                          ctypes.byref(err))
+        return {"touching": touching[:nt_], "near": near[:nn], "patches": patches[:npatch], "whole": whole[:nwhole],
+                "near_offsets": offsets, "near_trials": trials[:nn], "candidate_pairs": int(sizes[4]),
+                "maximum_depth": int(sizes[5]), "maximum_estimated_error": err.value}
+    finally:
+        lib.ref_plan_destroy(h)
+
+
+def oracle_assemble(mesh, plan, k, tolerance=1e-4, regular_order=4, singular_order=4, tests=None, threads=8,
+                    vertices=None):
+    """Row-scaled A (nv x nv) and B (nv x nt) of the FP64 oracle for one wavenumber."""
+    lib = oracle()
+    V = np.ascontiguousarray(recentred(mesh) if vertices is None else vertices)
+    T = np.ascontiguousarray(mesh.triangles, dtype=np.uint32)
+    nv, nt = len(V), len(T)
+    A = np.zeros(2 * nv * nv)
+    B = np.zeros(2 * nv * nt)
+    touching = np.ascontiguousarray(plan["touching"], dtype=np.uint32)
+    near = np.ascontiguousarray(plan["near"], dtype=np.uint32)
+    patches = np.ascontiguousarray(plan["patches"], dtype=np.float64)
+    whole = plan["whole"]
+    sub = None if tests is None else np.ascontiguousarray(tests, dtype=np.uint32)
+    lib.oracle_assemble.argtypes = [ctypes.c_uint32, _D, ctypes.c_uint32, _U, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_uint, ctypes.c_uint, _U, ctypes.c_uint64, _U, ctypes.c_uint64, _D, _U8, _U,
+                                    ctypes.c_uint64, ctypes.c_uint, _D, _D]
+    rc = lib.oracle_assemble(nv, _d(V), nt, _u(T), k, tolerance, regular_order, singular_order,
+                             _u(touching), len(plan["touching"]), _u(near), len(plan["near"]),
+                             _d(patches) if len(plan["patches"]) else None,
+                             whole.ctypes.data_as(_U8) if len(whole) else None,
+                             _u(sub) if sub is not None else None, 0 if sub is None else len(sub), threads, _d(A), _d(B))
+    _check(lib, rc, "oracle_assemble")
+    return (A.view(np.complex128).reshape(nv, nv).T.copy(), B.view(np.complex128).reshape(nt, nv).T.copy())
+
+
+def oracle_lu_solve(A, R, threads=8):
+    lib = oracle()
+    n = A.shape[0]
+    Rm = np.asarray(R, np.complex128).reshape(n, -1)
+    Af = np.ascontiguousarray(A.T).astype(np.complex128).view(np.float64).copy()
+    Rf = np.ascontiguousarray(Rm.T).view(np.float64).copy()
+    lib.oracle_lu_solve.argtypes = [ctypes.c_uint64, _D, ctypes.c_uint64, _D, ctypes.c_uint]
+    _check(lib, lib.oracle_lu_solve(n, _d(Af), Rm.shape[1], _d(Rf), threads), "oracle_lu_solve")
+    X = Rf.view(np.complex128).reshape(Rm.shape[1], n).T
+    return X.reshape(np.asarray(R).shape).copy()
+
+
+def oracle_field(mesh, k, degree, pressure, flux, observers, far=False, origin=(0, 0, 0), R=1.0, near_tolerance=1e-6,
+                 depth=6, vertices=None):
+    """Observer (or far-field) pressures; pressure (nv, nrhs) and flux (nt, nrhs) complex arrays."""
+    lib = oracle()
+    V = np.ascontiguousarray(recentred(mesh) if vertices is None else vertices)
+    T = np.ascontiguousarray(mesh.triangles, dtype=np.uint32)
+    P = np.ascontiguousarray(np.asarray(pressure, np.complex128).reshape(len(V), -1).T).view(np.float64).copy()
+    Q = np.ascontiguousarray(np.asarray(flux, np.complex128).reshape(len(T), -1).T).view(np.float64).copy()
+    nrhs = P.size // (2 * len(V))
+    O = np.ascontiguousarray(np.asarray(observers, float).reshape(-1, 3))
+    out = np.zeros(2 * len(O) * nrhs)
+    org = np.asarray(origin, float)
+    lib.oracle_field.argtypes = [ctypes.c_uint32, _D, ctypes.c_uint32, _U, ctypes.c_double, ctypes.c_uint, _D, _D,
+                                 ctypes.c_uint32, _D, ctypes.c_uint32, ctypes.c_int, _D, ctypes.c_double,
+                                 ctypes.c_double, ctypes.c_uint, _D]
+    _check(lib, lib.oracle_field(len(V), _d(V), len(T), _u(T), k, degree, _d(P), _d(Q), nrhs, _d(O), len(O),
+                                 1 if far else 0, _d(org), R, near_tolerance, depth, _d(out)), "oracle_field")
+    return out.view(np.complex128).reshape(nrhs, len(O)).T.copy()
+
+
+def triangle_rule(lib, name, degree):
+    fn = getattr(lib, name)
+    count = ctypes.c_uint()
+    fn.argtypes = [ctypes.c_uint, _D, ctypes.POINTER(ctypes.c_uint)]
+    _check(lib, fn(degree, None, ctypes.byref(count)), name)
+    out = np.zeros(4 * count.value)
+    _check(lib, fn(degree, _d(out), ctypes.byref(count)), name)
+    return out.reshape(-1, 4)
+
+
+def gauss(lib, name, order):
+    fn = getattr(lib, name)
+    fn.argtypes = [ctypes.c_uint, _D]
+    out = np.zeros(2 * order)
+    _check(lib, fn(order, _d(out)), name)
+    return out.reshape(-1, 2)
+
+
+def touching_points(lib, name, mesh, a, b, order, vertices=None):
+    V = np.ascontiguousarray(recentred(mesh) if vertices is None else vertices)
+    T = np.ascontiguousarray(mesh.triangles, dtype=np.uint32)
+    fn = getattr(lib, name)
+    count = ctypes.c_uint()
+    if name == "ref_pair_points":
+        topo = ctypes.c_int()
+        fn.argtypes = [ctypes.c_uint32, _D, ctypes.c_uint32, _U, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint, _D,
+                       ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_int)]
+        _check(lib, fn(len(V), _d(V), len(T), _u(T), a, b, order, None, ctypes.byref(count), ctypes.byref(topo)), name)
+        out = np.zeros(7 * count.value)
+        _check(lib, fn(len(V), _d(V), len(T), _u(T), a, b, order, _d(out), ctypes.byref(count), ctypes.byref(topo)),
+               name)
+    else:
+        fn.argtypes = [ctypes.c_uint32, _D, ctypes.c_uint32, _U, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint, _D,
+                       ctypes.POINTER(ctypes.c_uint)]
+        _check(lib, fn(len(V), _d(V), len(T), _u(T), a, b, order, None, ctypes.byref(count)), name)
+        out = np.zeros(7 * count.value)
+        _check(lib, fn(len(V), _d(V), len(T), _u(T), a, b, order, _d(out), ctypes.byref(count)), name)
+    return out.reshape(-1, 7)
+
+
+def patch_moments(lib, name, mesh, a, b, patch, k, vertices=None):
+    V = np.ascontiguousarray(recentred(mesh) if vertices is None else vertices)
+    T = np.ascontiguousarray(mesh.triangles, dtype=np.uint32)
+    P = np.ascontiguousarray(patch, dtype=np.float64)
+    out = np.zeros(72)
+    fn = getattr(lib, name)
+    if name == "ref_estimate_patch":
+        errs = np.zeros(3)
+        fn.argtypes = [ctypes.c_uint32, _D, ctypes.c_uint32, _U, ctypes.c_uint32, ctypes.c_uint32, _D, ctypes.c_double,
+                       _D, _D]
+        _check(lib, fn(len(V), _d(V), len(T), _u(T), a, b, _d(P), k, _d(out), _d(errs)), name)
+    else:
+        fn.argtypes = [ctypes.c_uint32, _D, ctypes.c_uint32, _U, ctypes.c_uint32, ctypes.c_uint32, _D, ctypes.c_double,
+                       _D]
+        _check(lib, fn(len(V), _d(V), len(T), _u(T), a, b, _d(P), k, _d(out)), name)
+    return out.view(np.complex128)
