@@ -385,7 +385,7 @@ class HipBemSolver:
         _check(lib.hsbem_result_peak_device_bytes(handle, ctypes.byref(peak)))
         freqs = []
         for f in range(nf.value):
-            ms = np.zeros(9)
+            ms = np.zeros(12)
             _check(lib.hsbem_result_timings(handle, f, _dptr(ms)))
             exs = []
             for e in range(ne.value):
@@ -414,7 +414,9 @@ class HipBemSolver:
                 exs.append(BemExcitationResult(request.excitations[e].name, observed, imps, res.value, its.value,
                                                refs.value, BemBoundaryData(surface, trace, deriv)))
             freqs.append(BemFrequencyResult(request.frequencies[f], exs, ms[0], ms[1], ms[2], ms[3], ms[4], ms[5],
-                                            {"regular": ms[6], "touching": ms[7], "near": ms[8]}))
+                                            {"regular": ms[6], "touching": ms[7], "near": ms[8],
+                                             "regular_samples": ms[9], "touching_samples": ms[10],
+                                             "near_samples": ms[11]}))
         return BemResult("MI355X (gfx950)", nt.value, nv.value, peak.value, freqs)
 
     # -------- internal seams (component parity tests and benchmarks)

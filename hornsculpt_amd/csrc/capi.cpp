@@ -311,6 +311,9 @@ int hsbem_result_timings(const hsbem_result *r, uint32_t f, double *ms) {
         ms[6] = fr.regular_kernel_ms;
         ms[7] = fr.touching_kernel_ms;
         ms[8] = fr.near_kernel_ms;
+        ms[9] = fr.regular_samples;
+        ms[10] = fr.touching_samples;
+        ms[11] = fr.near_samples;
     });
 }
 

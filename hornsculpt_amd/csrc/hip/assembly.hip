@@ -141,6 +141,7 @@ __global__ __launch_bounds__(kTileThreads) void regular_tiles(AssemblyArgs args,
 
     const double k = args.k;
     const std::uint32_t pairs = nT * nS;
+    unsigned long long counted = 0;
     for (std::uint32_t p = tid; p < pairs; p += kTileThreads) {
         const std::uint32_t lt = p % nT, ls = p / nT;
         bool touching = false;
@@ -191,6 +192,7 @@ __global__ __launch_bounds__(kTileThreads) void regular_tiles(AssemblyArgs args,
         Moments m;
         zero(m);
         integrate_tensor(m, gt, gs, srule + 4 * offset, count, 4.0 * gt[G_AREA] * gs[G_AREA], k, nn, args.error);
+        counted += static_cast<unsigned long long>(count) * count;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             const int a = slt[i][lt];
@@ -214,6 +216,14 @@ __global__ __launch_bounds__(kTileThreads) void regular_tiles(AssemblyArgs args,
                              cscale(m.b[i], args.row_scale[row]));
             }
         }
+    }
+    if (args.samples) {
+        unsigned long long c = counted;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            c += __shfl_xor(c, off, 64);
+        if ((tid & 63) == 0 && c)
+            atomicAdd(args.samples, c);
     }
     __syncthreads();
     for (std::uint32_t e = tid; e < nA * nB; e += kTileThreads) {

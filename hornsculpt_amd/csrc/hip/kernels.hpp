@@ -38,6 +38,7 @@ struct AssemblyArgs {
     double k, tolerance;
     unsigned regular_order;
     int *error;
+    unsigned long long *samples; // optional: regular pair samples of the launch (algorithmic count)
 };
 
 struct ClusterView {

@@ -108,6 +108,7 @@ MeshSolver::MeshSolver(Context *ctx, SurfaceMesh mesh) : ctx_(ctx) {
     mesh_ = std::move(mesh);
     build_clusters();
     error_.resize(1);
+    samples_.resize(1);
     rules_.upload(rule_table(), ctx_->stream);
     hip_check(hipStreamSynchronize(ctx_->stream), "mesh upload");
 }
@@ -248,6 +249,14 @@ DevicePlan &MeshSolver::device_plan(const Numerics &n, double k_max, bool *built
         g.push_back(p.weight);
     }
     dp->gauss.upload(g, s);
+    for (const auto &j : dp->plan.touching)
+        dp->touching_samples += static_cast<double>(sector_count(j.permutation.topology)) * n.singular_order *
+                                n.singular_order * n.singular_order * n.singular_order;
+    for (const auto &j : dp->plan.near)
+        for (std::uint32_t q = 0; q < j.patch_count; ++q) {
+            const auto &patch = dp->plan.patches[j.first_patch + q];
+            dp->near_samples += patch_point_count(patch, patch.degree == 14 ? 64 : 16);
+        }
     hip_check(hipStreamSynchronize(s), "plan upload");
     dp->plan_ms = ms_since(t0);
     if (built)
@@ -611,7 +620,9 @@ Result MeshSolver::solve(const Request &req, ProgressFn progress, void *user, co
         hip_check(hipMemsetAsync(A_.data(), 0, A_.bytes(), s), "clear A");
         hip_check(hipMemsetAsync(BD_.data(), 0, BD_.bytes(), s), "clear B");
         hip_check(hipMemsetAsync(error_.data(), 0, sizeof(int), s), "clear error");
-        const auto L = assembly_launch(plan, num, k, A_.data(), BD_.data(), nd, flux_column_.data());
+        auto L = assembly_launch(plan, num, k, A_.data(), BD_.data(), nd, flux_column_.data());
+        hip_check(hipMemsetAsync(samples_.data(), 0, sizeof(unsigned long long), s), "clear samples");
+        L.args.samples = samples_.data();
         hip_check(hipEventRecord(ev[0], s), "event");
         dev::launch_regular(L, s);
         hip_check(hipEventRecord(ev[1], s), "event");
@@ -630,6 +641,13 @@ Result MeshSolver::solve(const Request &req, ProgressFn progress, void *user, co
         hip_check(hipEventElapsedTime(&e01, ev[0], ev[1]), "event time");
         hip_check(hipEventElapsedTime(&e12, ev[1], ev[2]), "event time");
         hip_check(hipEventElapsedTime(&e23, ev[2], ev[3]), "event time");
+        unsigned long long regular_samples = 0;
+        hip_check(hipMemcpyAsync(&regular_samples, samples_.data(), sizeof(regular_samples), hipMemcpyDeviceToHost, s),
+                  "samples download");
+        hip_check(hipStreamSynchronize(s), "samples sync");
+        fr.regular_samples = static_cast<double>(regular_samples);
+        fr.touching_samples = plan.touching_samples;
+        fr.near_samples = plan.near_samples;
         fr.regular_kernel_ms = e01;
         fr.touching_kernel_ms = e12;
         fr.near_kernel_ms = e23;

@@ -154,7 +154,7 @@ struct DevicePlan {
     DeviceBuffer<std::uint8_t> whole;
     DeviceBuffer<double> gauss;
     double plan_ms{};
-    double regular_samples_per_k{}; // filled lazily per k
+    double touching_samples{}, near_samples{}; // algorithmic sample counts of one assembly
 };
 
 class MeshSolver {
@@ -191,6 +191,7 @@ class MeshSolver {
     DeviceBuffer<std::uint32_t> cluster_panel_offsets_, cluster_panels_, cluster_node_offsets_, cluster_nodes_;
     std::uint32_t cluster_count_{};
     DeviceBuffer<int> error_;
+    DeviceBuffer<unsigned long long> samples_;
     std::map<std::tuple<unsigned, unsigned, double, double, unsigned, double>, std::unique_ptr<DevicePlan>> plans_;
     // workspace (reused across calls)
     DeviceBuffer<double> A_, LU_, BD_, q_, rhs_, x_, r_, flux_, scratch_, obs_, field_, partials_;

@@ -147,7 +147,9 @@ HSBEM_API int hsbem_result_impedances(const hsbem_result *result, uint32_t frequ
 /* Relative L2 residual of the unscaled equation, Krylov iterations, refinement steps. */
 HSBEM_API int hsbem_result_residual(const hsbem_result *result, uint32_t frequency, uint32_t excitation,
                                     double *residual, uint32_t *iterations, uint32_t *refinements);
-/* Milliseconds: assembly, solve, factorization, triangular solve, field, interaction plan. */
+/* ms[12]: milliseconds of assembly, solve, factorization, triangular solve, field, interaction plan;
+ * device-event milliseconds of the regular, touching and near assembly kernels; and the
+ * algorithmic sample counts of those three kernels (FP64 pair-kernel evaluations). */
 HSBEM_API int hsbem_result_timings(const hsbem_result *result, uint32_t frequency, double *milliseconds);
 HSBEM_API int hsbem_result_peak_device_bytes(const hsbem_result *result, uint64_t *bytes);
 

@@ -850,6 +850,40 @@ int oracle_patch_moments(std::uint32_t nv, const double *xyz, std::uint32_t nt, 
     });
 }
 
+// Unscaled 3x3 A block (curl term included) and 3 B moments of one ordered
+// disjoint pair integrated over one tensor patch with the FP64 pair kernel.
+int oracle_pair_block(std::uint32_t nv, const double *xyz, std::uint32_t nt, const std::uint32_t *tri,
+                      std::uint32_t a, std::uint32_t b, const double *patch, double k, double *block, double *rhs) {
+    return guarded([&] {
+        const Space s = make_space(nv, xyz, nt, tri);
+        const Panel &t = s.panels[a], &q = s.panels[b];
+        const Patch pa = unpack(patch);
+        const auto rule = tri_rule(pa.degree == 14 ? 14 : 8);
+        Moments m;
+        for (const auto &x : rule)
+            for (const auto &y : rule) {
+                PairPt p{};
+                p.w = x.w * y.w * pa.f;
+                for (int i = 0; i < 3; ++i)
+                    for (int l = 0; l < 3; ++l) {
+                        p.x[i] += x.b[l] * pa.t[3 * l + i];
+                        p.y[i] += y.b[l] * pa.s[3 * l + i];
+                    }
+                pair_sample(m, t, q, p.x, p.y, p.w, k, false);
+            }
+        for (int i = 0; i < 3; ++i) {
+            for (int j = 0; j < 3; ++j) {
+                const double cc = dot(t.curl[i], q.curl[j]);
+                const C v = m.a[3 * i + j] + C(m.g.imag() * cc / k, -m.g.real() * cc / k);
+                block[2 * (3 * i + j)] = v.real();
+                block[2 * (3 * i + j) + 1] = v.imag();
+            }
+            rhs[2 * i] = m.b[i].real();
+            rhs[2 * i + 1] = m.b[i].imag();
+        }
+    });
+}
+
 // Row-scaled A (np x np) and B (np x nt), complex interleaved, column-major.
 // test_subset (optional) restricts assembly to the listed test panels.
 int oracle_assemble(std::uint32_t nv, const double *xyz, std::uint32_t nt, const std::uint32_t *tri, double k,

@@ -1,0 +1,165 @@
+"""Host-side product logic on the CPU: the C ABI library, the interaction planner
+(bit-exact against the reference planner compiled from its sources), input
+validation and the synthetic mesh generators. No device calls."""
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from hornsculpt_amd import bem, meshes
+
+ROOT = Path(__file__).resolve().parents[1]
+needs_ref = pytest.mark.skipif(ob.reference() is None, reason="reference planner (oracle/_ref) not built")
+
+
+def test_library_exports_every_declared_symbol():
+    lib = bem.library()
+    declared = re.findall(r"HSBEM_API\s+[\w\s\*]+?\b(hsbem_\w+)\s*\(", (ROOT / "include" / "hsbem.h").read_text())
+    assert len(declared) >= 25
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_numerics_defaults_match_reference():
+    lib = bem.library()
+    n = bem._Numerics()
+    lib.hsbem_numerics_default(n)
+    # BemNumerics defaults (bem.hpp:113-123)
+    assert (n.quadrature_order, n.singular_order, n.near_maximum_depth, n.maximum_refinement_steps) == (4, 4, 6, 3)
+    assert (n.near_distance_ratio, n.near_relative_tolerance, n.relative_residual_tolerance) == (1.0, 1e-4, 5e-5)
+    py = bem.BemNumerics()
+    assert (py.quadrature_order, py.singular_order, py.near_relative_tolerance) == (4, 4, 1e-4)
+
+
+def _box(lx, ly, lz):
+    V = np.array([(x, y, z) for z in (0, lz) for y in (0, ly) for x in (0, lx)], float)
+    quads = [(0, 2, 3, 1), (4, 5, 7, 6), (0, 1, 5, 4), (2, 6, 7, 3), (0, 4, 6, 2), (1, 3, 7, 5)]
+    T = []
+    for a, b, c, d in quads:
+        T += [(a, b, c), (a, c, d)]
+    return bem.SurfaceMesh(V, np.array(T, np.uint32), np.arange(12, dtype=np.uint32), np.ones(12, np.uint8))
+
+
+PLAN_CASES = [
+    ("octa2", lambda: meshes.octa_sphere(2), 1000.0, {}),
+    ("octa3", lambda: meshes.octa_sphere(3), 1715.0, {}),
+    ("octa3-strict", lambda: meshes.octa_sphere(3), 500.0, {"tolerance": 1e-6}),
+    ("octa3-ratio0", lambda: meshes.octa_sphere(3), 1000.0, {"near_ratio": 0.0}),
+    ("ico3", lambda: meshes.icosphere(3), 1000.0, {}),
+    ("tetra", lambda: meshes.scalene_tetra(), 400.0, {}),
+    ("horn12", lambda: meshes.exponential_horn(0.012), 4000.0, {}),
+    ("thin-box", lambda: _box(0.02, 0.02, 0.0002), 3000.0, {}),          # projection fans + all-pairs mode
+    ("ico2-high-kh", lambda: meshes.icosphere(2), 2400.0, {"depth": 4}),
+]
+
+
+@needs_ref
+@pytest.mark.parametrize("name,make,frequency,opts", PLAN_CASES, ids=[c[0] for c in PLAN_CASES])
+def test_plan_bit_identical_to_reference(name, make, frequency, opts):
+    mesh = make()
+    k = 2 * math.pi * frequency / 343.0
+    tol = opts.get("tolerance", 1e-4)
+    ratio = opts.get("near_ratio", 1.0)
+    depth = opts.get("depth", 6)
+    ref = ob.ref_plan(mesh, tolerance=tol, near_ratio=ratio, depth=depth, k_max=k)
+    num = bem.BemNumerics(near_relative_tolerance=tol, near_distance_ratio=ratio, near_maximum_depth=depth)
+    mine = bem.interaction_plan(mesh, num, k)
+    for key in ("touching", "near", "patches", "whole", "near_offsets", "near_trials"):
+        assert np.array_equal(mine[key], ref[key]), key
+    for key in ("candidate_pairs", "maximum_depth", "maximum_estimated_error"):
+        assert mine[key] == ref[key], key
+
+
+@needs_ref
+def test_plan_covers_projection_and_all_pairs_paths():
+    mesh = _box(0.02, 0.02, 0.0002)
+    plan = bem.interaction_plan(mesh, bem.BemNumerics(), 2 * math.pi * 3000 / 343)
+    assert len(plan["whole"]) == 12 * 11 // 2        # all-pairs mode (sliver side panels)
+    assert np.any(plan["patches"][:, 20] == 1)       # projected fans for the 0.2 mm gap
+
+
+def test_plan_is_independent_of_thread_count():
+    mesh = meshes.icosphere(3)
+    k = 2 * math.pi * 1500 / 343
+    a = bem.interaction_plan(mesh, bem.BemNumerics(plan_threads=1), k)
+    b = bem.interaction_plan(mesh, bem.BemNumerics(plan_threads=8), k)
+    for key in ("touching", "near", "patches", "near_offsets", "near_trials"):
+        assert np.array_equal(a[key], b[key])
+
+
+@pytest.mark.parametrize("field,value", [("quadrature_order", 3), ("singular_order", 1), ("singular_order", 21),
+                                         ("near_relative_tolerance", 0.0), ("near_maximum_depth", 17)])
+def test_plan_rejects_invalid_settings(field, value):
+    num = bem.BemNumerics()
+    setattr(num, field, value)
+    with pytest.raises(ValueError):
+        bem.interaction_plan(meshes.octa_sphere(1), num, 10.0)
+
+
+def test_plan_rejects_degenerate_panels():
+    V = np.array([(0, 0, 0), (1, 0, 0), (2, 0, 0), (0, 1, 0)], float)
+    mesh = bem.SurfaceMesh(V, np.array([(0, 1, 2), (0, 1, 3)], np.uint32), np.ones(2, np.uint32), np.ones(2, np.uint8))
+    with pytest.raises(ValueError):
+        bem.interaction_plan(mesh, bem.BemNumerics(), 1.0)
+
+
+def _closed_oriented(mesh):
+    edges = {}
+    for t in mesh.triangles:
+        for l in range(3):
+            e = (int(t[l]), int(t[(l + 1) % 3]))
+            edges[e] = edges.get(e, 0) + 1
+    paired = all(c == 1 and (b, a) in edges for (a, b), c in edges.items())
+    V = mesh.vertices
+    volume = sum(np.dot(V[a], np.cross(V[b], V[c])) for a, b, c in mesh.triangles) / 6
+    return paired, volume
+
+
+def _min_angle(mesh):
+    V = mesh.vertices
+    worst = 180.0
+    for t in mesh.triangles:
+        P = V[list(t)]
+        for i in range(3):
+            u, w = P[(i + 1) % 3] - P[i], P[(i + 2) % 3] - P[i]
+            worst = min(worst, math.degrees(math.acos(np.clip(u @ w / np.linalg.norm(u) / np.linalg.norm(w), -1, 1))))
+    return worst
+
+
+@pytest.mark.parametrize("make,nv,nt", [(lambda: meshes.icosphere(3), 642, 1280), (lambda: meshes.octa_sphere(3), 258, 512),
+                                        (lambda: meshes.scalene_tetra(), 4, 4)])
+def test_reference_meshes(make, nv, nt):
+    mesh = make()
+    assert (len(mesh.vertices), len(mesh.triangles)) == (nv, nt)
+    paired, volume = _closed_oriented(mesh)
+    assert paired and volume > 0
+
+
+@pytest.mark.parametrize("edge", [0.02, 0.012, 0.0052])
+def test_horn_mesh_closed_oriented_and_shape_qualified(edge):
+    mesh = meshes.exponential_horn(edge)
+    paired, volume = _closed_oriented(mesh)
+    assert paired and volume > 0
+    assert _min_angle(mesh) >= 20.0  # keeps the planner out of its all-pairs mode
+    assert set(np.unique(mesh.face_ids).tolist()) == {1, 2}
+    assert np.all(mesh.boundary[mesh.face_ids == 1] == bem.DRIVEN)
+
+
+def test_cpp_facade_compiles_against_c_abi(tmp_path):
+    """The header-only C++ drop-in (include/hornsculpt_amd/bem.hpp) compiles and links."""
+    import subprocess
+    src = tmp_path / "caller.cpp"
+    src.write_text(
+        "#include <hornsculpt_amd/bem.hpp>\n"
+        "int main(int argc, char**) {\n"
+        "  if (argc > 5) { hornsculpt::SurfaceMesh m; hornsculpt::CudaBemSolver s(m);\n"
+        "    hornsculpt::BemRequest r; r.frequencies = {1000.0}; auto out = s.solve(r); return (int)out.triangleCount; }\n"
+        "  return 0; }\n")
+    exe = tmp_path / "caller"
+    proc = subprocess.run(["g++", "-std=c++20", "-I", str(ROOT / "include"), str(src), "-o", str(exe),
+                           str(ROOT / "hornsculpt_amd" / "libhsbem.so"), f"-Wl,-rpath,{ROOT / 'hornsculpt_amd'}"],
+                          capture_output=True, text=True)
+    assert proc.returncode == 0, proc.stderr

@@ -1,0 +1,229 @@
+"""Pins of the FP64 oracle (CPU only).
+
+Each oracle function is checked against something that fixes it independently:
+the reference's own planner/rule/moment code compiled from its sources
+(oracle/_ref), the BEAT CPU fixtures the reference ships, the analytic
+pulsating sphere, exact polynomial integration and a closed-form singular
+integral. A planted error (wrong sign, dropped term, transposed basis) fails
+at least one of them.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from fixtures import (BEAT_FREQUENCIES, SPHERE_OBSERVERS, TETRA_EXCITATIONS, analytic, beat_sphere, beat_tetra, flux,
+                      oracle_observer_field)
+from hornsculpt_amd import bem, meshes
+
+needs_ref = pytest.mark.skipif(ob.reference() is None, reason="reference planner (oracle/_ref) not built")
+
+
+def _factorial(n):
+    return math.factorial(n)
+
+
+@pytest.mark.parametrize("degree", [2, 4, 6, 8, 12, 14])
+def test_triangle_rules_integrate_polynomials_exactly(degree):
+    rule = ob.triangle_rule(ob.oracle(), "oracle_triangle_rule", degree)
+    assert abs(rule[:, 3].sum() - 0.5) < 1e-14
+    x, y = rule[:, 1], rule[:, 2]
+    for a in range(degree + 1):
+        for b in range(degree + 1 - a):
+            exact = _factorial(a) * _factorial(b) / _factorial(a + b + 2)
+            assert abs(np.sum(rule[:, 3] * x ** a * y ** b) - exact) <= 1e-14 * max(1.0, exact * 1e2)
+
+
+@needs_ref
+@pytest.mark.parametrize("degree", [2, 4, 6, 8, 12, 14])
+def test_triangle_rules_match_reference(degree):
+    a = ob.triangle_rule(ob.oracle(), "oracle_triangle_rule", degree)
+    b = ob.triangle_rule(ob.reference(), "ref_triangle_rule", degree)
+    assert a.shape == b.shape and np.max(np.abs(a - b)) <= 4.5e-16  # a few ulp (independent Newton stop)
+
+
+@needs_ref
+@pytest.mark.parametrize("order", range(2, 21))
+def test_gauss_rules_match_reference(order):
+    a = ob.gauss(ob.oracle(), "oracle_gauss", order)
+    b = ob.gauss(ob.reference(), "ref_gauss", order)
+    # Independent Newton stopping rules: the reference takes its weights from the
+    # derivative at the last pre-update iterate (weights sum to 1 within 4e-15),
+    # the oracle iterates to convergence; both agree to 2e-13 relative.
+    assert np.max(np.abs(a - b) / np.abs(b)) <= 2e-13
+    assert abs(a[:, 1].sum() - 1) < 1e-14
+
+
+def _pairs_by_topology(mesh):
+    T = mesh.triangles
+    found = {}
+    for a in range(len(T)):
+        for b in range(len(T)):
+            shared = len(set(T[a].tolist()) & set(T[b].tolist()))
+            if shared and shared not in found:
+                found[shared] = (a, b)
+        if len(found) == 3:
+            break
+    return found
+
+
+@needs_ref
+@pytest.mark.parametrize("order", [2, 4, 8])
+def test_sauter_schwab_points_match_reference(order):
+    mesh = meshes.octa_sphere(2)
+    for shared, (a, b) in _pairs_by_topology(mesh).items():
+        mine = ob.touching_points(ob.oracle(), "oracle_touching_points", mesh, a, b, order)
+        ref = ob.touching_points(ob.reference(), "ref_pair_points", mesh, a, b, order)
+        assert mine.shape == ref.shape
+        assert np.max(np.abs(mine - ref)) <= 4.5e-16, shared
+        assert abs(mine[:, 6].sum() - 0.25) < 1e-14  # the four-dimensional reference measure
+
+
+def test_sauter_schwab_closed_form_coincident():
+    """int_T int_T 1/|x-y| = (2+sqrt 2) asinh(1)/3 on the unit right triangle
+    (pair_quadrature_tests.cpp:230-244)."""
+    V = np.array([(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)], float)
+    T = np.array([(0, 1, 2), (0, 3, 1)], np.uint32)
+    mesh = bem.SurfaceMesh(V, T, np.ones(2, np.uint32), np.ones(2, np.uint8))
+    exact = (2 + math.sqrt(2)) * math.asinh(1) / 3
+    errors = []
+    for order in (4, 8, 16):
+        pts = ob.touching_points(ob.oracle(), "oracle_touching_points", mesh, 0, 0, order, vertices=V)
+        x = pts[:, 1:2] * (V[1] - V[0]) + pts[:, 2:3] * (V[2] - V[0])
+        y = pts[:, 4:5] * (V[1] - V[0]) + pts[:, 5:6] * (V[2] - V[0])
+        value = np.sum(pts[:, 6] / np.linalg.norm(x - y, axis=1))  # 4 A^2 = 1
+        errors.append(abs(value - exact))
+    assert errors[-1] < 1e-12 and errors[0] > errors[-1]
+
+
+def _box(lx, ly, lz):
+    """Closed box [0,lx]x[0,ly]x[0,lz] with two outward triangles per face."""
+    V = np.array([(x, y, z) for z in (0, lz) for y in (0, ly) for x in (0, lx)], float)
+    quads = [(0, 2, 3, 1), (4, 5, 7, 6), (0, 1, 5, 4), (2, 6, 7, 3), (0, 4, 6, 2), (1, 3, 7, 5)]
+    T = []
+    for a, b, c, d in quads:
+        T += [(a, b, c), (a, c, d)]
+    T = np.array(T, np.uint32)
+    return bem.SurfaceMesh(V, T, np.arange(len(T), dtype=np.uint32), np.ones(len(T), np.uint8))
+
+
+@needs_ref
+def test_patch_moments_match_reference_tensor_and_projection():
+    """Oracle patch point maps (tensor and projected fan) vs estimatePairPatchCpu."""
+    cases = [(meshes.octa_sphere(3), 2 * math.pi * 1715 / 343), (_box(0.02, 0.02, 0.0002), 2 * math.pi * 3000 / 343)]
+    seen_projection = False
+    for mesh, k in cases:
+        plan = ob.ref_plan(mesh, k_max=k)
+        assert len(plan["near"])
+        for job in plan["near"][:60]:
+            test, trial, first, count, swap = (int(v) for v in job)
+            if swap:
+                continue
+            for q in range(first, first + min(count, 3)):
+                patch = plan["patches"][q]
+                seen_projection |= patch[20] == 1
+                mine = ob.patch_moments(ob.oracle(), "oracle_patch_moments", mesh, test, trial, patch, k)
+                ref = ob.patch_moments(ob.reference(), "ref_estimate_patch", mesh, test, trial, patch, k)
+                assert np.max(np.abs(mine - ref)) <= 1e-12 * np.max(np.abs(ref))
+    assert seen_projection
+
+
+@needs_ref
+def test_pair_kernel_algebra_matches_reference_moments():
+    """The oracle's FP64 pair kernel (S, D, K', H with the curl term, alpha=-i/k)
+    rebuilt from the reference's independent 36-moment CPU integrals."""
+    mesh = meshes.octa_sphere(3)
+    k = 2 * math.pi * 1000 / 343
+    plan = ob.ref_plan(mesh, k_max=k)
+    space = np.zeros((len(mesh.triangles), 26))
+    lumped = np.zeros(len(mesh.vertices))
+    V = ob.recentred(mesh)
+    T = np.ascontiguousarray(mesh.triangles, dtype=np.uint32)
+    ref = ob.reference()
+    ref.ref_space.argtypes = [ctypes.c_uint32, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32,
+                              ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_double),
+                              ctypes.POINTER(ctypes.c_double)]
+    assert ref.ref_space(len(V), ob._d(V), len(T), ob._u(T), ob._d(space), ob._d(lumped)) == 0
+    lib = ob.oracle()
+    lib.oracle_pair_block.argtypes = [ctypes.c_uint32, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32,
+                                      ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.POINTER(ctypes.c_double), ctypes.c_double,
+                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    checked = 0
+    for job in plan["near"]:
+        t, s, first, count, swap = (int(v) for v in job)
+        if swap or count != 1:
+            continue
+        patch = np.ascontiguousarray(plan["patches"][first])
+        block, rhs = np.zeros(18), np.zeros(6)
+        assert lib.oracle_pair_block(len(V), ob._d(V), len(T), ob._u(T), t, s, ob._d(patch), k, ob._d(block),
+                                     ob._d(rhs)) == 0
+        block, rhs = block.view(np.complex128).reshape(3, 3), rhs.view(np.complex128)
+        M = ob.patch_moments(ref, "ref_estimate_patch", mesh, t, s, patch, k).reshape(4, 3, 3)
+        At, As = space[t, 0], space[s, 0]
+        nt_, ns_ = space[t, 2:5], space[s, 2:5]
+        scale = max(space[t, 1], space[s, 1])
+        factor = 4 * At * As / (4 * math.pi)
+        G = factor * M[0]
+        grad = factor / scale * M[1:]
+        dn = -np.einsum("c,cij->ij", ns_, grad)        # dG/dn_y moments
+        adj = np.einsum("c,cij->ij", nt_, grad)        # -dG/dn_x moments
+        a = -dn + 1j * k * float(nt_ @ ns_) * G
+        curl_t, curl_s = space[t, 8:17].reshape(3, 3), space[s, 8:17].reshape(3, 3)
+        expected = a - 1j * G.sum() * (curl_t @ curl_s.T) / k
+        expected_b = (-G + 1j * adj / k).sum(axis=1)
+        assert np.max(np.abs(block - expected)) <= 1e-12 * np.max(np.abs(expected))
+        assert np.max(np.abs(rhs - expected_b)) <= 1e-12 * np.max(np.abs(expected_b))
+        checked += 1
+        if checked == 25:
+            break
+    assert checked == 25
+
+
+@needs_ref
+def test_oracle_matches_beat_sphere_and_analytic():
+    mesh = meshes.octa_sphere(3)
+    beat = beat_sphere()
+    kmax = 2 * math.pi * max(BEAT_FREQUENCIES) / 343
+    plan = ob.ref_plan(mesh, k_max=kmax)
+    for f in BEAT_FREQUENCIES:
+        k = 2 * math.pi * f / 343
+        A, B = ob.oracle_assemble(mesh, plan, k)
+        q = flux(mesh, {1: 1.0}, f, 1.21)
+        p = ob.oracle_lu_solve(A, B @ q)
+        got = oracle_observer_field(ob, mesh, k, p, q, SPHERE_OBSERVERS)[:, 0]
+        assert np.linalg.norm(got - beat[f]) / np.linalg.norm(beat[f]) <= 1e-6
+        for j, r in enumerate((.3, 1.0, 3.0)):
+            assert abs(got[j] / analytic(f, r) - 1) < 0.035
+
+
+@needs_ref
+@pytest.mark.parametrize("order,tolerance", [(12, 1e-8), (8, 5e-5)])
+def test_oracle_matches_beat_tetra_ports(order, tolerance):
+    mesh = meshes.scalene_tetra()
+    ref = beat_tetra()
+    f = 400.0
+    k = 2 * math.pi * f / 343
+    A, B = ob.oracle_assemble(mesh, ob.ref_plan(mesh, singular_order=order, k_max=k), k, singular_order=order)
+    V = mesh.vertices
+    areas = np.array([np.linalg.norm(np.cross(V[t[1]] - V[t[0]], V[t[2]] - V[t[0]])) / 2 for t in mesh.triangles])
+    for name, vel in TETRA_EXCITATIONS.items():
+        p = ob.oracle_lu_solve(A, B @ flux(mesh, vel, f, 1.21))
+        surface = np.array([p[list(t)].mean() for t in mesh.triangles])
+        for face, v in vel.items():
+            sel = mesh.face_ids == face
+            Z = (surface[sel] * areas[sel]).sum() / areas[sel].sum() / (v * areas[sel].sum())
+            want = ref[(name, face)]["Z"]
+            assert abs(Z / want - 1) <= tolerance
+            assert abs(np.angle(Z / want, deg=True)) <= 0.02
+
+
+def test_oracle_lu_matches_numpy():
+    rng = np.random.default_rng(3)
+    for n in (1, 7, 48, 97, 130):
+        A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        B = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
+        X = ob.oracle_lu_solve(A, B)
+        assert np.max(np.abs(X - np.linalg.solve(A, B))) <= 1e-10 * np.max(np.abs(X))
