@@ -45,11 +45,12 @@ __device__ __forceinline__ double g_at(const double *geo, int field, std::uint32
 // Two-stage tensor contraction of the pair integral over rule x rule:
 // inner sums over trial points carry the trial basis, the outer sum the test basis.
 __device__ __forceinline__ void integrate_tensor(Moments &m, const double *gt, const double *gs, const double *rule,
-                                                 int n, double fourAA, double k, double nn, int *error) {
+                                                 int n, double fourAA, double k, double inv_k, double nn, int *error) {
+    const double knn = k * nn, wscale = fourAA * 0.07957747154594766788; // 4 At As / (4 pi)
     const double bx = gt[G_V0] - gs[G_V0], by = gt[G_V0 + 1] - gs[G_V0 + 1], bz = gt[G_V0 + 2] - gs[G_V0 + 2];
     for (int xi = 0; xi < n; ++xi) {
         const double *xp = rule + 4 * xi;
-        const double x1 = xp[1], x2 = xp[2], xw = xp[3] * fourAA;
+        const double x1 = xp[1], x2 = xp[2], xw = xp[3] * wscale;
         const double ltx = gt[G_E1] * x1 + gt[G_E2] * x2, lty = gt[G_E1 + 1] * x1 + gt[G_E2 + 1] * x2,
                      ltz = gt[G_E1 + 2] * x1 + gt[G_E2 + 2] * x2;
         Cplx ga[3] = {{0, 0}, {0, 0}, {0, 0}}, gg = {0, 0}, gb = {0, 0};
@@ -58,8 +59,8 @@ __device__ __forceinline__ void integrate_tensor(Moments &m, const double *gt, c
             const double y0 = yp[0], y1 = yp[1], y2 = yp[2];
             const double lsx = gs[G_E1] * y1 + gs[G_E2] * y2, lsy = gs[G_E1 + 1] * y1 + gs[G_E2 + 1] * y2,
                          lsz = gs[G_E1 + 2] * y1 + gs[G_E2 + 2] * y2;
-            const Sample s = bm_sample(bx + (ltx - lsx), by + (lty - lsy), bz + (ltz - lsz), xw * yp[3], k,
-                                       gs[G_N], gs[G_N + 1], gs[G_N + 2], gt[G_N], gt[G_N + 1], gt[G_N + 2], nn,
+            const Sample s = bm_sample(bx + (ltx - lsx), by + (lty - lsy), bz + (ltz - lsz), xw * yp[3], k, inv_k,
+                                       knn, gs[G_N], gs[G_N + 1], gs[G_N + 2], gt[G_N], gt[G_N + 1], gt[G_N + 2],
                                        false, error);
             ga[0] = cadd(ga[0], cscale(s.a, y0));
             ga[1] = cadd(ga[1], cscale(s.a, y1));
@@ -139,7 +140,7 @@ __global__ __launch_bounds__(kTileThreads) void regular_tiles(AssemblyArgs args,
     }
     __syncthreads();
 
-    const double k = args.k;
+    const double k = args.k, inv_k = 1.0 / args.k;
     const std::uint32_t pairs = nT * nS;
     unsigned long long counted = 0;
     for (std::uint32_t p = tid; p < pairs; p += kTileThreads) {
@@ -191,7 +192,8 @@ __global__ __launch_bounds__(kTileThreads) void regular_tiles(AssemblyArgs args,
         const double nn = gt[G_N] * gs[G_N] + gt[G_N + 1] * gs[G_N + 1] + gt[G_N + 2] * gs[G_N + 2];
         Moments m;
         zero(m);
-        integrate_tensor(m, gt, gs, srule + 4 * offset, count, 4.0 * gt[G_AREA] * gs[G_AREA], k, nn, args.error);
+        integrate_tensor(m, gt, gs, srule + 4 * offset, count, 4.0 * gt[G_AREA] * gs[G_AREA], k, inv_k, nn,
+                         args.error);
         counted += static_cast<unsigned long long>(count) * count;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -278,7 +280,8 @@ __global__ __launch_bounds__(256) void touching_jobs(AssemblyArgs args, const To
     const Topology topo = tj.permutation.topology;
     const bool coincident = topo == Topology::coincident;
     const double nn = gt[G_N] * gs[G_N] + gt[G_N + 1] * gs[G_N + 1] + gt[G_N + 2] * gs[G_N + 2];
-    const double fourAA = 4.0 * gt[G_AREA] * gs[G_AREA];
+    const double wscale = 4.0 * gt[G_AREA] * gs[G_AREA] * 0.07957747154594766788;
+    const double inv_k = 1.0 / args.k, knn = args.k * nn;
     const double bx = gt[G_V0] - gs[G_V0], by = gt[G_V0 + 1] - gs[G_V0 + 1], bz = gt[G_V0 + 2] - gs[G_V0 + 2];
     const unsigned per = order * order * order;
     const unsigned total = sector_count(topo) * per;
@@ -311,9 +314,9 @@ __global__ __launch_bounds__(256) void touching_jobs(AssemblyArgs args, const To
             }
             xt[at] += 1 - r;
             xs[as] += 1 - r;
-            const Sample s = bm_sample(r * dx, r * dy, r * dz, c.weight * r * r * r * gauss[2 * ri + 1] * fourAA, args.k,
-                                       gs[G_N], gs[G_N + 1], gs[G_N + 2], gt[G_N], gt[G_N + 1], gt[G_N + 2], nn,
-                                       coincident, args.error);
+            const Sample s = bm_sample(r * dx, r * dy, r * dz, c.weight * r * r * r * gauss[2 * ri + 1] * wscale, args.k,
+                                       inv_k, knn, gs[G_N], gs[G_N + 1], gs[G_N + 2], gt[G_N], gt[G_N + 1],
+                                       gt[G_N + 2], coincident, args.error);
             add_sample(m, s, xt, xs);
         }
     }
@@ -344,7 +347,8 @@ __global__ __launch_bounds__(256) void near_jobs(AssemblyArgs args, const NearJo
     load_geo(pv, nj.test, gt);
     load_geo(pv, nj.trial, gs);
     const double nn = gt[G_N] * gs[G_N] + gt[G_N + 1] * gs[G_N + 1] + gt[G_N + 2] * gs[G_N + 2];
-    const double fourAA = 4.0 * gt[G_AREA] * gs[G_AREA];
+    const double wscale = 4.0 * gt[G_AREA] * gs[G_AREA] * 0.07957747154594766788;
+    const double inv_k = 1.0 / args.k, knn = args.k * nn;
     const double bx = gt[G_V0] - gs[G_V0], by = gt[G_V0 + 1] - gs[G_V0 + 1], bz = gt[G_V0 + 2] - gs[G_V0 + 2];
     const bool swap = nj.swap != 0;
     const FanPanel ft = fan_of(gt), fs = fan_of(gs);
@@ -372,9 +376,9 @@ __global__ __launch_bounds__(256) void near_jobs(AssemblyArgs args, const NearJo
             const double lsx = gs[G_E1] * p.trial[1] + gs[G_E2] * p.trial[2],
                          lsy = gs[G_E1 + 1] * p.trial[1] + gs[G_E2 + 1] * p.trial[2],
                          lsz = gs[G_E1 + 2] * p.trial[1] + gs[G_E2 + 2] * p.trial[2];
-            const Sample s = bm_sample(bx + (ltx - lsx), by + (lty - lsy), bz + (ltz - lsz), p.weight * fourAA,
-                                       args.k, gs[G_N], gs[G_N + 1], gs[G_N + 2], gt[G_N], gt[G_N + 1], gt[G_N + 2],
-                                       nn, false, args.error);
+            const Sample s = bm_sample(bx + (ltx - lsx), by + (lty - lsy), bz + (ltz - lsz), p.weight * wscale,
+                                       args.k, inv_k, knn, gs[G_N], gs[G_N + 1], gs[G_N + 2], gt[G_N], gt[G_N + 1],
+                                       gt[G_N + 2], false, args.error);
             add_sample(m, s, p.test, p.trial);
         }
     }
