@@ -442,8 +442,8 @@ int hsbem_dense_gmres(hsbem_context *c, uint32_t n, const double *A, const doubl
         hsbem::GmresWorkspace ws;
         double achieved = 0;
         std::uint32_t its = 0;
-        hsbem::dense_gmres(s, dA.data(), n, db.data(), dx.data(), restart, maximum_iterations, tolerance, &achieved,
-                           &its, nullptr, ws);
+        hsbem::dense_gmres(s, dA.data(), false, n, db.data(), dx.data(), restart, maximum_iterations, tolerance,
+                           &achieved, &its, nullptr, ws);
         hsbem::hip_check(hipMemcpyAsync(x, dx.data(), dx.bytes(), hipMemcpyDeviceToHost, s), "x download");
         hsbem::hip_check(hipStreamSynchronize(s), "GMRES sync");
         if (relative_residual)

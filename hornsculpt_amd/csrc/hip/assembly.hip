@@ -1,30 +1,137 @@
-// Burton-Miller Galerkin assembly kernels (FP64), gfx950.
+// Burton-Miller Galerkin assembly kernels, gfx950.
 //
 // Replaces the reference kernels assembleRegular / assembleSingular /
 // assembleNear / addMass (modules/bem/cuda/cuda_solver.cu:266-540) with:
 //   regular_tiles   one workgroup per (test cluster, trial cluster) of up to 64
 //                   spatially sorted panels; panel geometry and rules in LDS; every
-//                   3x3 pair block accumulated in an LDS node tile (ds_add_f64) and
+//                   3x3 pair block accumulated in an LDS node tile (ds_add) and
 //                   flushed once per tile to HBM, so global atomics drop from 18 per
 //                   pair to about one per tile entry.
 //   touching_jobs   one wavefront per touching pair: Sauter-Schwab (u,v,w) tuples
 //                   over lanes with the radial coordinate innermost (every sector map
-//                   is linear in r), DPP/shuffle reduction, one lane scatters.
+//                   is linear in r), shuffle reduction, one lane scatters. Always FP64.
 //   near_jobs       one wavefront per near job, patch points over lanes.
 //   mass            1/2 M_pp into A, -alpha 1/2 M_pq into the flux columns.
+// Precision: the sample type R is double (plain form) or float (mixed form:
+// FP32 samples and accumulation, FP64 geometry differences, FP32 matrix).
 #include "../host/plan.hpp"
 #include "../host/qualification.hpp"
 #include "../host/rules.hpp"
 #include "device.hpp"
 #include "kernels.hpp"
 
+#include <type_traits>
+
 namespace hsbem::dev {
 namespace {
 
-constexpr int kTilePanels = 64;                // panels per cluster
-constexpr int kTileNodes = 64;                 // distinct pressure nodes per cluster
-constexpr int kTileStride = 2 * kTileNodes + 2; // doubles per trial-node row (padded)
+constexpr int kTilePanels = 64;  // panels per cluster
+constexpr int kTileNodes = 64;   // distinct pressure nodes per cluster
 constexpr int kTileThreads = 512;
+
+template <class R> struct CplxT {
+    R re, im;
+};
+
+template <class R> __device__ __forceinline__ CplxT<R> cadd_t(CplxT<R> a, CplxT<R> b) {
+    return {a.re + b.re, a.im + b.im};
+}
+template <class R> __device__ __forceinline__ CplxT<R> cscale_t(CplxT<R> a, R s) { return {a.re * s, a.im * s}; }
+
+template <class R> struct MomentsT {
+    CplxT<R> a[9], b[3], g;
+};
+
+template <class R> __device__ __forceinline__ void zero_t(MomentsT<R> &m) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+        m.a[i] = {0, 0};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        m.b[i] = {0, 0};
+    m.g = {0, 0};
+}
+
+template <class R> struct SampleT {
+    CplxT<R> g, a, b;
+};
+
+// Per-pair constants of the pair kernel in the sample precision.
+template <class R> struct PairConst {
+    R k, inv_k, k_rev, knn, nsx, nsy, nsz, ntx, nty, ntz;
+};
+
+// FP32 Burton-Miller sample (mixed form): hardware rsqrt, the phase reduced in
+// revolutions (v_fract) and hardware sin/cos of 2 pi x. Same algebra and the same
+// branch-free distance check as bm_sample.
+__device__ __forceinline__ SampleT<float> sample(float dx, float dy, float dz, float wq, const PairConst<float> &c,
+                                                 bool coincident, bool &bad) {
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    bad |= !(r2 > 0.f) || !isfinite(r2);
+    SampleT<float> s;
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * rinv;
+    const float f = __builtin_amdgcn_fractf(r * c.k_rev);
+    const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
+    const float kr = c.k * r, scale = wq * rinv;
+    s.g = {cs * scale, -sn * scale};
+    const CplxT<float> radial = {s.g.re - s.g.im * kr, s.g.im + s.g.re * kr};
+    const float ir2 = rinv * rinv;
+    const float fy = coincident ? 0.f : (dx * c.nsx + dy * c.nsy + dz * c.nsz) * ir2;
+    const float fx = coincident ? 0.f : -(dx * c.ntx + dy * c.nty + dz * c.ntz) * ir2;
+    s.a = {-radial.re * fy - c.knn * s.g.im, -radial.im * fy + c.knn * s.g.re};
+    s.b = {-s.g.re - radial.im * fx * c.inv_k, -s.g.im + radial.re * fx * c.inv_k};
+    return s;
+}
+
+__device__ __forceinline__ SampleT<double> sample(double dx, double dy, double dz, double wq,
+                                                  const PairConst<double> &c, bool coincident, bool &bad) {
+    const Sample s = bm_sample(dx, dy, dz, wq, c.k, c.inv_k, c.knn, c.nsx, c.nsy, c.nsz, c.ntx, c.nty, c.ntz,
+                               coincident, bad);
+    return {{s.g.re, s.g.im}, {s.a.re, s.a.im}, {s.b.re, s.b.im}};
+}
+
+template <class R>
+__device__ __forceinline__ PairConst<R> pair_const(const double *gt, const double *gs, double k) {
+    const double nn = gt[G_N] * gs[G_N] + gt[G_N + 1] * gs[G_N + 1] + gt[G_N + 2] * gs[G_N + 2];
+    return {static_cast<R>(k), static_cast<R>(1.0 / k), static_cast<R>(k * 0.15915494309189533577),
+            static_cast<R>(k * nn), static_cast<R>(gs[G_N]), static_cast<R>(gs[G_N + 1]), static_cast<R>(gs[G_N + 2]),
+            static_cast<R>(gt[G_N]), static_cast<R>(gt[G_N + 1]), static_cast<R>(gt[G_N + 2])};
+}
+
+template <class R>
+__device__ __forceinline__ void add_sample_t(MomentsT<R> &m, const SampleT<R> &s, const R *xt, const R *xs) {
+    m.g = cadd_t(m.g, s.g);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        m.b[i] = cadd_t(m.b[i], cscale_t(s.b, xt[i]));
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            m.a[3 * i + j] = cadd_t(m.a[3 * i + j], cscale_t(s.a, xt[i] * xs[j]));
+    }
+}
+
+template <class R> __device__ __forceinline__ R wave_sum_t(R v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <class R> __device__ __forceinline__ void wave_reduce_t(MomentsT<R> &m) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        m.a[i].re = wave_sum_t(m.a[i].re);
+        m.a[i].im = wave_sum_t(m.a[i].im);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        m.b[i].re = wave_sum_t(m.b[i].re);
+        m.b[i].im = wave_sum_t(m.b[i].im);
+    }
+    m.g.re = wave_sum_t(m.g.re);
+    m.g.im = wave_sum_t(m.g.im);
+}
 
 __device__ __forceinline__ bool contains_sorted(const std::uint32_t *v, std::uint32_t n, std::uint32_t key) {
     std::uint32_t lo = 0, hi = n;
@@ -44,52 +151,70 @@ __device__ __forceinline__ double g_at(const double *geo, int field, std::uint32
 
 // Two-stage tensor contraction of the pair integral over rule x rule:
 // inner sums over trial points carry the trial basis, the outer sum the test basis.
-__device__ __forceinline__ void integrate_tensor(Moments &m, const double *gt, const double *gs, const double *rule,
-                                                 int n, double fourAA, double k, double inv_k, double nn, int *error) {
-    const double knn = k * nn, wscale = fourAA * 0.07957747154594766788; // 4 At As / (4 pi)
-    const double bx = gt[G_V0] - gs[G_V0], by = gt[G_V0 + 1] - gs[G_V0 + 1], bz = gt[G_V0 + 2] - gs[G_V0 + 2];
+// The panel-origin difference is formed in FP64 before rounding to R; the rule
+// table (b0, b1, b2, w per point) is held in R.
+template <class R>
+__device__ __forceinline__ void integrate_tensor(MomentsT<R> &m, const double *gt, const double *gs, const R *rule,
+                                                 int n, double k, bool &bad) {
+    const PairConst<R> c = pair_const<R>(gt, gs, k);
+    const R wscale = static_cast<R>(4.0 * gt[G_AREA] * gs[G_AREA] * 0.07957747154594766788); // 4 At As / (4 pi)
+    const R bx = static_cast<R>(gt[G_V0] - gs[G_V0]), by = static_cast<R>(gt[G_V0 + 1] - gs[G_V0 + 1]),
+            bz = static_cast<R>(gt[G_V0 + 2] - gs[G_V0 + 2]);
+    const R te1x = gt[G_E1], te1y = gt[G_E1 + 1], te1z = gt[G_E1 + 2], te2x = gt[G_E2], te2y = gt[G_E2 + 1],
+            te2z = gt[G_E2 + 2];
+    const R se1x = gs[G_E1], se1y = gs[G_E1 + 1], se1z = gs[G_E1 + 2], se2x = gs[G_E2], se2y = gs[G_E2 + 1],
+            se2z = gs[G_E2 + 2];
     for (int xi = 0; xi < n; ++xi) {
-        const double *xp = rule + 4 * xi;
-        const double x1 = xp[1], x2 = xp[2], xw = xp[3] * wscale;
-        const double ltx = gt[G_E1] * x1 + gt[G_E2] * x2, lty = gt[G_E1 + 1] * x1 + gt[G_E2 + 1] * x2,
-                     ltz = gt[G_E1 + 2] * x1 + gt[G_E2 + 2] * x2;
-        Cplx ga[3] = {{0, 0}, {0, 0}, {0, 0}}, gg = {0, 0}, gb = {0, 0};
+        const R *xp = rule + 4 * xi;
+        const R x1 = xp[1], x2 = xp[2], xw = xp[3] * wscale;
+        const R ltx = te1x * x1 + te2x * x2, lty = te1y * x1 + te2y * x2, ltz = te1z * x1 + te2z * x2;
+        CplxT<R> ga[3] = {{0, 0}, {0, 0}, {0, 0}}, gg = {0, 0}, gb = {0, 0};
         for (int yi = 0; yi < n; ++yi) {
-            const double *yp = rule + 4 * yi;
-            const double y0 = yp[0], y1 = yp[1], y2 = yp[2];
-            const double lsx = gs[G_E1] * y1 + gs[G_E2] * y2, lsy = gs[G_E1 + 1] * y1 + gs[G_E2 + 1] * y2,
-                         lsz = gs[G_E1 + 2] * y1 + gs[G_E2 + 2] * y2;
-            const Sample s = bm_sample(bx + (ltx - lsx), by + (lty - lsy), bz + (ltz - lsz), xw * yp[3], k, inv_k,
-                                       knn, gs[G_N], gs[G_N + 1], gs[G_N + 2], gt[G_N], gt[G_N + 1], gt[G_N + 2],
-                                       false, error);
-            ga[0] = cadd(ga[0], cscale(s.a, y0));
-            ga[1] = cadd(ga[1], cscale(s.a, y1));
-            ga[2] = cadd(ga[2], cscale(s.a, y2));
-            gg = cadd(gg, s.g);
-            gb = cadd(gb, s.b);
+            const R *yp = rule + 4 * yi;
+            const R y0 = yp[0], y1 = yp[1], y2 = yp[2];
+            const R lsx = se1x * y1 + se2x * y2, lsy = se1y * y1 + se2y * y2, lsz = se1z * y1 + se2z * y2;
+            const SampleT<R> s = sample(bx + (ltx - lsx), by + (lty - lsy), bz + (ltz - lsz), xw * yp[3], c, false, bad);
+            ga[0] = cadd_t(ga[0], cscale_t(s.a, y0));
+            ga[1] = cadd_t(ga[1], cscale_t(s.a, y1));
+            ga[2] = cadd_t(ga[2], cscale_t(s.a, y2));
+            gg = cadd_t(gg, s.g);
+            gb = cadd_t(gb, s.b);
         }
-        m.g = cadd(m.g, gg);
+        m.g = cadd_t(m.g, gg);
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            const double xb = xp[i];
-            m.b[i] = cadd(m.b[i], cscale(gb, xb));
+            const R xb = xp[i];
+            m.b[i] = cadd_t(m.b[i], cscale_t(gb, xb));
 #pragma unroll
             for (int j = 0; j < 3; ++j)
-                m.a[3 * i + j] = cadd(m.a[3 * i + j], cscale(ga[j], xb));
+                m.a[3 * i + j] = cadd_t(m.a[3 * i + j], cscale_t(ga[j], xb));
         }
     }
 }
 
-__global__ __launch_bounds__(kTileThreads) void regular_tiles(AssemblyArgs args, ClusterView cl, NearView near,
-                                                              const double *rules) {
-    __shared__ double tile[kTileNodes * kTileStride];
+template <class M> __device__ __forceinline__ void atomic_add_entry(M *p, double re, double im) {
+    unsafeAtomicAdd(p, static_cast<M>(re));
+    unsafeAtomicAdd(p + 1, static_cast<M>(im));
+}
+
+template <class R> __device__ __forceinline__ R *matrix_of(const AssemblyArgs &a);
+template <> __device__ __forceinline__ double *matrix_of<double>(const AssemblyArgs &a) { return a.A; }
+template <> __device__ __forceinline__ float *matrix_of<float>(const AssemblyArgs &a) { return a.A32; }
+
+// R: sample/tile precision; the matrix element type is R as well.
+// Occupancy: the FP32 variant fits two 512-thread workgroups per CU (<= 128 VGPRs).
+template <class R>
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2)))
+void regular_tiles(AssemblyArgs args, ClusterView cl, NearView near, const double *rules) {
+    constexpr int stride = 2 * kTileNodes + 2; // R per trial-node row (padded)
+    __shared__ R tile[kTileNodes * stride];
     __shared__ double sgt[G_COUNT][kTilePanels], sgs[G_COUNT][kTilePanels];
     __shared__ double sbt[B_COUNT][kTilePanels], sbs[B_COUNT][kTilePanels];
     __shared__ std::uint32_t sidt[kTilePanels], sids[kTilePanels];
     __shared__ std::uint32_t snt[3][kTilePanels], sns[3][kTilePanels];
     __shared__ std::uint8_t slt[3][kTilePanels], sls[3][kTilePanels], shpt[kTilePanels], shps[kTilePanels];
     __shared__ std::int32_t sfc[kTilePanels];
-    __shared__ double srule[4 * kRuleTableCount];
+    __shared__ R srule[4 * kRuleTableCount];
 
     const std::uint32_t ct = blockIdx.y, cs = blockIdx.x;
     const std::uint32_t t0 = cl.panel_offsets[ct], nT = cl.panel_offsets[ct + 1] - t0;
@@ -99,10 +224,10 @@ __global__ __launch_bounds__(kTileThreads) void regular_tiles(AssemblyArgs args,
     const PanelView &pv = args.panels;
     const int tid = threadIdx.x;
 
-    for (int e = tid; e < kTileNodes * kTileStride; e += kTileThreads)
-        tile[e] = 0.0;
+    for (int e = tid; e < kTileNodes * stride; e += kTileThreads)
+        tile[e] = R(0);
     for (int e = tid; e < 4 * kRuleTableCount; e += kTileThreads)
-        srule[e] = rules[e];
+        srule[e] = static_cast<R>(rules[e]);
     for (int e = tid; e < static_cast<int>(G_COUNT * kTilePanels); e += kTileThreads) {
         const int f = e / kTilePanels, p = e % kTilePanels;
         if (p < static_cast<int>(nT))
@@ -140,9 +265,11 @@ __global__ __launch_bounds__(kTileThreads) void regular_tiles(AssemblyArgs args,
     }
     __syncthreads();
 
-    const double k = args.k, inv_k = 1.0 / args.k;
+    const double k = args.k;
+    const R inv_k = static_cast<R>(1.0 / k);
     const std::uint32_t pairs = nT * nS;
     unsigned long long counted = 0;
+    bool bad = false;
     for (std::uint32_t p = tid; p < pairs; p += kTileThreads) {
         const std::uint32_t lt = p % nT, ls = p / nT;
         bool touching = false;
@@ -189,11 +316,9 @@ __global__ __launch_bounds__(kTileThreads) void regular_tiles(AssemblyArgs args,
             gt[f] = sgt[f][lt];
             gs[f] = sgs[f][ls];
         }
-        const double nn = gt[G_N] * gs[G_N] + gt[G_N + 1] * gs[G_N + 1] + gt[G_N + 2] * gs[G_N + 2];
-        Moments m;
-        zero(m);
-        integrate_tensor(m, gt, gs, srule + 4 * offset, count, 4.0 * gt[G_AREA] * gs[G_AREA], k, inv_k, nn,
-                         args.error);
+        MomentsT<R> m;
+        zero_t(m);
+        integrate_tensor<R>(m, gt, gs, srule + 4 * offset, count, k, bad);
         counted += static_cast<unsigned long long>(count) * count;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -201,12 +326,12 @@ __global__ __launch_bounds__(kTileThreads) void regular_tiles(AssemblyArgs args,
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 const int b = sls[j][ls];
-                const double cc = gt[G_C0 + 3 * i] * gs[G_C0 + 3 * j] + gt[G_C0 + 3 * i + 1] * gs[G_C0 + 3 * j + 1] +
-                                  gt[G_C0 + 3 * i + 2] * gs[G_C0 + 3 * j + 2];
-                const Cplx v = {m.a[3 * i + j].re + m.g.im * cc / k, m.a[3 * i + j].im - m.g.re * cc / k};
-                double *slot = tile + b * kTileStride + 2 * a;
-                unsafeAtomicAdd(slot, v.re);
-                unsafeAtomicAdd(slot + 1, v.im);
+                const R cc = static_cast<R>(gt[G_C0 + 3 * i] * gs[G_C0 + 3 * j] +
+                                            gt[G_C0 + 3 * i + 1] * gs[G_C0 + 3 * j + 1] +
+                                            gt[G_C0 + 3 * i + 2] * gs[G_C0 + 3 * j + 2]);
+                R *slot = tile + b * stride + 2 * a;
+                unsafeAtomicAdd(slot, m.a[3 * i + j].re + m.g.im * cc * inv_k);
+                unsafeAtomicAdd(slot + 1, m.a[3 * i + j].im - m.g.re * cc * inv_k);
             }
         }
         const std::int32_t fc = sfc[ls];
@@ -214,11 +339,14 @@ __global__ __launch_bounds__(kTileThreads) void regular_tiles(AssemblyArgs args,
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 const std::uint32_t row = snt[i][lt];
-                atomic_add_c(args.BD + 2 * (static_cast<std::size_t>(fc) * pv.nv + row),
-                             cscale(m.b[i], args.row_scale[row]));
+                const double rs = args.row_scale[row];
+                atomic_add_entry(args.BD + 2 * (static_cast<std::size_t>(fc) * pv.nv + row), m.b[i].re * rs,
+                                 m.b[i].im * rs);
             }
         }
     }
+    if (bad)
+        atomicExch(args.error, ERR_DISTANCE);
     if (args.samples) {
         unsigned long long c = counted;
 #pragma unroll
@@ -228,21 +356,23 @@ __global__ __launch_bounds__(kTileThreads) void regular_tiles(AssemblyArgs args,
             atomicAdd(args.samples, c);
     }
     __syncthreads();
+    R *A = matrix_of<R>(args);
     for (std::uint32_t e = tid; e < nA * nB; e += kTileThreads) {
         const std::uint32_t a = e % nA, b = e / nA;
-        const double *slot = tile + b * kTileStride + 2 * a;
+        const R *slot = tile + b * stride + 2 * a;
         const std::uint32_t row = cl.nodes[a0 + a], col = cl.nodes[b0 + b];
         const double rs = args.row_scale[row];
-        const double re = slot[0], im = slot[1];
-        if (re != 0.0 || im != 0.0)
-            atomic_add_c(args.A + 2 * (static_cast<std::size_t>(col) * pv.nv + row), {re * rs, im * rs});
+        const R re = slot[0], im = slot[1];
+        if (re != R(0) || im != R(0))
+            atomic_add_entry(A + 2 * (static_cast<std::size_t>(col) * pv.nv + row), re * rs, im * rs);
     }
 }
 
-__device__ __forceinline__ void scatter_pair(const AssemblyArgs &args, const Moments &m, std::uint32_t ti,
+template <class M, class R>
+__device__ __forceinline__ void scatter_pair(const AssemblyArgs &args, M *A, const MomentsT<R> &m, std::uint32_t ti,
                                              std::uint32_t si) {
     const PanelView &pv = args.panels;
-    const double k = args.k;
+    const double inv_k = 1.0 / args.k;
     const std::int32_t fc = args.flux_column[si];
     for (int i = 0; i < 3; ++i) {
         const std::uint32_t row = pv.nodes[static_cast<std::size_t>(i) * pv.nt + ti];
@@ -252,11 +382,13 @@ __device__ __forceinline__ void scatter_pair(const AssemblyArgs &args, const Mom
             double cc = 0;
             for (int c = 0; c < 3; ++c)
                 cc += g_at(pv.geo, G_C0 + 3 * i + c, pv.nt, ti) * g_at(pv.geo, G_C0 + 3 * j + c, pv.nt, si);
-            const Cplx v = {m.a[3 * i + j].re + m.g.im * cc / k, m.a[3 * i + j].im - m.g.re * cc / k};
-            atomic_add_c(args.A + 2 * (static_cast<std::size_t>(col) * pv.nv + row), cscale(v, rs));
+            const double re = static_cast<double>(m.a[3 * i + j].re) + static_cast<double>(m.g.im) * cc * inv_k;
+            const double im = static_cast<double>(m.a[3 * i + j].im) - static_cast<double>(m.g.re) * cc * inv_k;
+            atomic_add_entry(A + 2 * (static_cast<std::size_t>(col) * pv.nv + row), re * rs, im * rs);
         }
         if (fc >= 0)
-            atomic_add_c(args.BD + 2 * (static_cast<std::size_t>(fc) * pv.nv + row), cscale(m.b[i], rs));
+            atomic_add_entry(args.BD + 2 * (static_cast<std::size_t>(fc) * pv.nv + row),
+                             static_cast<double>(m.b[i].re) * rs, static_cast<double>(m.b[i].im) * rs);
     }
 }
 
@@ -266,6 +398,9 @@ __device__ __forceinline__ void load_geo(const PanelView &pv, std::uint32_t t, d
         g[f] = g_at(pv.geo, f, pv.nt, t);
 }
 
+// Touching pairs are integrated in FP64 in both forms (the hypersingular weak
+// form); M is the matrix element type.
+template <class M>
 __global__ __launch_bounds__(256) void touching_jobs(AssemblyArgs args, const TouchingJob *jobs, std::uint32_t njobs,
                                                      const double *gauss, unsigned order) {
     const std::uint32_t job = blockIdx.x * 4 + threadIdx.x / 64;
@@ -279,15 +414,15 @@ __global__ __launch_bounds__(256) void touching_jobs(AssemblyArgs args, const To
     load_geo(pv, tj.trial, gs);
     const Topology topo = tj.permutation.topology;
     const bool coincident = topo == Topology::coincident;
-    const double nn = gt[G_N] * gs[G_N] + gt[G_N + 1] * gs[G_N + 1] + gt[G_N + 2] * gs[G_N + 2];
+    const PairConst<double> pc = pair_const<double>(gt, gs, args.k);
     const double wscale = 4.0 * gt[G_AREA] * gs[G_AREA] * 0.07957747154594766788;
-    const double inv_k = 1.0 / args.k, knn = args.k * nn;
     const double bx = gt[G_V0] - gs[G_V0], by = gt[G_V0 + 1] - gs[G_V0 + 1], bz = gt[G_V0 + 2] - gs[G_V0 + 2];
     const unsigned per = order * order * order;
     const unsigned total = sector_count(topo) * per;
     const int at = tj.permutation.test[0], as = tj.permutation.trial[0];
-    Moments m;
-    zero(m);
+    MomentsT<double> m;
+    zero_t(m);
+    bool bad = false;
     for (unsigned idx = lane; idx < total; idx += 64) {
         unsigned tuple = idx % per;
         const unsigned wi = tuple % order;
@@ -314,15 +449,16 @@ __global__ __launch_bounds__(256) void touching_jobs(AssemblyArgs args, const To
             }
             xt[at] += 1 - r;
             xs[as] += 1 - r;
-            const Sample s = bm_sample(r * dx, r * dy, r * dz, c.weight * r * r * r * gauss[2 * ri + 1] * wscale, args.k,
-                                       inv_k, knn, gs[G_N], gs[G_N + 1], gs[G_N + 2], gt[G_N], gt[G_N + 1],
-                                       gt[G_N + 2], coincident, args.error);
-            add_sample(m, s, xt, xs);
+            const SampleT<double> s =
+                sample(r * dx, r * dy, r * dz, c.weight * r * r * r * gauss[2 * ri + 1] * wscale, pc, coincident, bad);
+            add_sample_t(m, s, xt, xs);
         }
     }
-    wave_reduce(m);
+    if (bad)
+        atomicExch(args.error, ERR_DISTANCE);
+    wave_reduce_t(m);
     if (lane == 0)
-        scatter_pair(args, m, tj.test, tj.trial);
+        scatter_pair(args, matrix_of<M>(args), m, tj.test, tj.trial);
 }
 
 __device__ __forceinline__ FanPanel fan_of(const double *g) {
@@ -335,6 +471,8 @@ __device__ __forceinline__ FanPanel fan_of(const double *g) {
     return f;
 }
 
+// Patch maps in FP64; samples and accumulation in R; matrix element type R.
+template <class R>
 __global__ __launch_bounds__(256) void near_jobs(AssemblyArgs args, const NearJob *jobs, std::uint32_t njobs,
                                                  const PairPatch *patches, const double *rules) {
     const std::uint32_t job = blockIdx.x * 4 + threadIdx.x / 64;
@@ -346,16 +484,16 @@ __global__ __launch_bounds__(256) void near_jobs(AssemblyArgs args, const NearJo
     double gt[G_COUNT], gs[G_COUNT];
     load_geo(pv, nj.test, gt);
     load_geo(pv, nj.trial, gs);
-    const double nn = gt[G_N] * gs[G_N] + gt[G_N + 1] * gs[G_N + 1] + gt[G_N + 2] * gs[G_N + 2];
+    const PairConst<R> pc = pair_const<R>(gt, gs, args.k);
     const double wscale = 4.0 * gt[G_AREA] * gs[G_AREA] * 0.07957747154594766788;
-    const double inv_k = 1.0 / args.k, knn = args.k * nn;
     const double bx = gt[G_V0] - gs[G_V0], by = gt[G_V0 + 1] - gs[G_V0 + 1], bz = gt[G_V0 + 2] - gs[G_V0 + 2];
     const bool swap = nj.swap != 0;
     const FanPanel ft = fan_of(gt), fs = fan_of(gs);
     const TrianglePoint *r8 = reinterpret_cast<const TrianglePoint *>(rules + 4 * kRule8);
     const TrianglePoint *r14 = reinterpret_cast<const TrianglePoint *>(rules + 4 * kRule14);
-    Moments m;
-    zero(m);
+    MomentsT<R> m;
+    zero_t(m);
+    bool bad = false;
     for (std::uint32_t q = 0; q < nj.patch_count; ++q) {
         const PairPatch patch = patches[nj.first_patch + q];
         const bool high = patch.degree == 14;
@@ -376,22 +514,27 @@ __global__ __launch_bounds__(256) void near_jobs(AssemblyArgs args, const NearJo
             const double lsx = gs[G_E1] * p.trial[1] + gs[G_E2] * p.trial[2],
                          lsy = gs[G_E1 + 1] * p.trial[1] + gs[G_E2 + 1] * p.trial[2],
                          lsz = gs[G_E1 + 2] * p.trial[1] + gs[G_E2 + 2] * p.trial[2];
-            const Sample s = bm_sample(bx + (ltx - lsx), by + (lty - lsy), bz + (ltz - lsz), p.weight * wscale,
-                                       args.k, inv_k, knn, gs[G_N], gs[G_N + 1], gs[G_N + 2], gt[G_N], gt[G_N + 1],
-                                       gt[G_N + 2], false, args.error);
-            add_sample(m, s, p.test, p.trial);
+            const SampleT<R> s = sample(static_cast<R>(bx + (ltx - lsx)), static_cast<R>(by + (lty - lsy)),
+                                        static_cast<R>(bz + (ltz - lsz)), static_cast<R>(p.weight * wscale), pc,
+                                        false, bad);
+            const R xt[3] = {static_cast<R>(p.test[0]), static_cast<R>(p.test[1]), static_cast<R>(p.test[2])};
+            const R xs[3] = {static_cast<R>(p.trial[0]), static_cast<R>(p.trial[1]), static_cast<R>(p.trial[2])};
+            add_sample_t(m, s, xt, xs);
         }
     }
-    wave_reduce(m);
+    if (bad)
+        atomicExch(args.error, ERR_DISTANCE);
+    wave_reduce_t(m);
     if (lane == 0)
-        scatter_pair(args, m, nj.test, nj.trial);
+        scatter_pair(args, matrix_of<R>(args), m, nj.test, nj.trial);
 }
 
-__global__ void mass_terms(AssemblyArgs args) {
+template <class M> __global__ void mass_terms(AssemblyArgs args) {
     const std::uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const PanelView &pv = args.panels;
     if (t >= pv.nt)
         return;
+    M *A = matrix_of<M>(args);
     const double area = g_at(pv.geo, G_AREA, pv.nt, t);
     const std::int32_t fc = args.flux_column[t];
     for (int i = 0; i < 3; ++i) {
@@ -399,8 +542,8 @@ __global__ void mass_terms(AssemblyArgs args) {
         const double rs = args.row_scale[row];
         for (int j = 0; j < 3; ++j) {
             const std::uint32_t col = pv.nodes[static_cast<std::size_t>(j) * pv.nt + t];
-            unsafeAtomicAdd(args.A + 2 * (static_cast<std::size_t>(col) * pv.nv + row),
-                            rs * area * (i == j ? 2.0 : 1.0) / 24.0);
+            unsafeAtomicAdd(A + 2 * (static_cast<std::size_t>(col) * pv.nv + row),
+                            static_cast<M>(rs * area * (i == j ? 2.0 : 1.0) / 24.0));
         }
         if (fc >= 0)
             unsafeAtomicAdd(args.BD + 2 * (static_cast<std::size_t>(fc) * pv.nv + row) + 1, area / (6.0 * args.k) * rs);
@@ -429,25 +572,47 @@ __global__ void flux_rhs(const double *BD, std::uint32_t nv, std::uint32_t nd, c
 } // namespace
 
 void launch_regular(const AssemblyLaunch &L, hipStream_t stream) {
-    if (L.clusters.count)
-        hipLaunchKernelGGL(regular_tiles, dim3(L.clusters.count, L.clusters.count), dim3(kTileThreads), 0, stream,
-                           L.args, L.clusters, L.near, L.rules);
+    if (!L.clusters.count)
+        return;
+    const dim3 grid(L.clusters.count, L.clusters.count);
+    if (L.args.A32)
+        hipLaunchKernelGGL(regular_tiles<float>, grid, dim3(kTileThreads), 0, stream, L.args, L.clusters, L.near,
+                           L.rules);
+    else
+        hipLaunchKernelGGL(regular_tiles<double>, grid, dim3(kTileThreads), 0, stream, L.args, L.clusters, L.near,
+                           L.rules);
 }
 
 void launch_touching(const AssemblyLaunch &L, hipStream_t stream) {
-    if (L.touching_count)
-        hipLaunchKernelGGL(touching_jobs, dim3((L.touching_count + 3) / 4), dim3(256), 0, stream, L.args, L.touching,
-                           L.touching_count, L.gauss, L.singular_order);
+    if (!L.touching_count)
+        return;
+    const dim3 grid((L.touching_count + 3) / 4);
+    if (L.args.A32)
+        hipLaunchKernelGGL(touching_jobs<float>, grid, dim3(256), 0, stream, L.args, L.touching, L.touching_count,
+                           L.gauss, L.singular_order);
+    else
+        hipLaunchKernelGGL(touching_jobs<double>, grid, dim3(256), 0, stream, L.args, L.touching, L.touching_count,
+                           L.gauss, L.singular_order);
 }
 
 void launch_near(const AssemblyLaunch &L, hipStream_t stream) {
-    if (L.near_count)
-        hipLaunchKernelGGL(near_jobs, dim3((L.near_count + 3) / 4), dim3(256), 0, stream, L.args, L.near_jobs,
-                           L.near_count, L.patches, L.rules);
+    if (!L.near_count)
+        return;
+    const dim3 grid((L.near_count + 3) / 4);
+    if (L.args.A32)
+        hipLaunchKernelGGL(near_jobs<float>, grid, dim3(256), 0, stream, L.args, L.near_jobs, L.near_count, L.patches,
+                           L.rules);
+    else
+        hipLaunchKernelGGL(near_jobs<double>, grid, dim3(256), 0, stream, L.args, L.near_jobs, L.near_count, L.patches,
+                           L.rules);
 }
 
 void launch_mass(const AssemblyLaunch &L, hipStream_t stream) {
-    hipLaunchKernelGGL(mass_terms, dim3((L.args.panels.nt + 255) / 256), dim3(256), 0, stream, L.args);
+    const dim3 grid((L.args.panels.nt + 255) / 256);
+    if (L.args.A32)
+        hipLaunchKernelGGL(mass_terms<float>, grid, dim3(256), 0, stream, L.args);
+    else
+        hipLaunchKernelGGL(mass_terms<double>, grid, dim3(256), 0, stream, L.args);
 }
 
 void launch_assembly(const AssemblyLaunch &L, hipStream_t stream) {

@@ -76,20 +76,18 @@ __device__ __forceinline__ double rsqrt_full(double v) {
 //   b   = -G + (i/k) dG/dn_x                          (-S - alpha K')
 // with alpha = -i/k (panel_integrals.cuh:166-198). Coincident pairs have zero
 // normal derivatives on a flat panel. wq = w/(4 pi); knn = k (n_x.n_y).
+// Branch free: a zero or non-finite distance sets `bad` (the caller reports
+// ERR_DISTANCE once) and leaves a non-finite sample.
 struct Sample {
     Cplx g, a, b;
 };
 
 __device__ __forceinline__ Sample bm_sample(double dx, double dy, double dz, double wq, double k, double inv_k,
                                             double knn, double nsx, double nsy, double nsz, double ntx, double nty,
-                                            double ntz, bool coincident, int *error) {
+                                            double ntz, bool coincident, bool &bad) {
     const double r2 = dx * dx + dy * dy + dz * dz;
+    bad |= !(r2 > 0.0) || !isfinite(r2);
     Sample s;
-    if (!(r2 > 0.0) || !isfinite(r2)) {
-        atomicExch(error, ERR_DISTANCE);
-        s.g = s.a = s.b = {0, 0};
-        return s;
-    }
     const double rinv = rsqrt_full(r2);
     const double r = r2 * rinv, kr = k * r;
     double sn, cs;

@@ -32,7 +32,8 @@ struct AssemblyArgs {
     PanelView panels;
     const double *row_scale;        // nv, 1/lumped mass, internal node order
     const std::int32_t *flux_column; // nt, column of BD or -1
-    double *A;                      // nv x nv complex
+    double *A;                      // nv x nv complex (plain form) or null
+    float *A32;                     // nv x nv complex FP32 (mixed form) or null
     double *BD;                     // nv x nd complex
     std::uint32_t nd;
     double k, tolerance;
@@ -77,9 +78,10 @@ void launch_flux_rhs(const double *BD, std::uint32_t nv, std::uint32_t nd, const
 
 // ---- dense linear algebra (linalg.hip) ----
 
-// y[:, e] = A x[:, e] for e < nvec (A n x n complex, column-major); y may not alias x.
-void launch_gemv(const double *A, std::uint32_t n, const double *x, double *y, std::uint32_t nvec, double *scratch,
-                 hipStream_t stream);
+// y[:, e] = A x[:, e] for e < nvec (A n x n complex, column-major, FP64 or FP32
+// when fp32 != 0; vectors and accumulation FP64); y may not alias x.
+void launch_gemv(const void *A, bool fp32, std::uint32_t n, const double *x, double *y, std::uint32_t nvec,
+                 double *scratch, hipStream_t stream);
 // Scratch doubles launch_gemv needs for n and nvec.
 std::size_t gemv_scratch_doubles(std::uint32_t n, std::uint32_t nvec);
 
@@ -102,7 +104,7 @@ void lu_solve(const double *LU, std::uint32_t n, const std::int32_t *pivots, dou
               hipStream_t stream);
 
 // r = b - A x, scaled by weight (lumped mass) into weighted; r unscaled into residual.
-void launch_residual(const double *A, std::uint32_t n, const double *b, const double *x, std::uint32_t nvec,
+void launch_residual(const void *A, bool fp32, std::uint32_t n, const double *b, const double *x, std::uint32_t nvec,
                      double *residual, double *scratch, hipStream_t stream);
 
 // Elementwise helpers.

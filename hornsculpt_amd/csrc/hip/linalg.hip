@@ -22,8 +22,8 @@ constexpr int kGemvRows = 256;
 constexpr int kGemvCols = 128;
 constexpr int kMaxVec = 8;
 
-template <int NV>
-__global__ __launch_bounds__(kGemvRows) void gemv_partial(const double *__restrict__ A, std::uint32_t n,
+template <int NV, class T>
+__global__ __launch_bounds__(kGemvRows) void gemv_partial(const T *__restrict__ A, std::uint32_t n,
                                                            const double *__restrict__ x, std::uint32_t chunks,
                                                            double *__restrict__ part) {
     __shared__ double sx[NV][2 * kGemvCols];
@@ -41,24 +41,26 @@ __global__ __launch_bounds__(kGemvRows) void gemv_partial(const double *__restri
 #pragma unroll
     for (int v = 0; v < NV; ++v)
         re[v] = im[v] = 0.0;
-    const double2 *col = reinterpret_cast<const double2 *>(A) + static_cast<std::size_t>(c0) * n + row;
+    const T *col = A + static_cast<std::size_t>(c0) * n + row;
     std::uint32_t c = 0;
     for (; c + 4 <= cn; c += 4) {
-        double2 a[4];
+        T a[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
             a[u] = col[static_cast<std::size_t>(c + u) * n];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < 4; ++u) {
+            const double ar = a[u].x, ai = a[u].y;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 const double xr = sx[v][2 * (c + u)], xi = sx[v][2 * (c + u) + 1];
-                re[v] += a[u].x * xr - a[u].y * xi;
-                im[v] += a[u].x * xi + a[u].y * xr;
+                re[v] += ar * xr - ai * xi;
+                im[v] += ar * xi + ai * xr;
             }
+        }
     }
     for (; c < cn; ++c) {
-        const double2 a = col[static_cast<std::size_t>(c) * n];
+        const T a = col[static_cast<std::size_t>(c) * n];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const double xr = sx[v][2 * c], xi = sx[v][2 * c + 1];
@@ -474,8 +476,8 @@ std::size_t gemv_scratch_doubles(std::uint32_t n, std::uint32_t nvec) {
     return 2 * chunks * n * std::max<std::uint32_t>(nvec, 1);
 }
 
-void launch_gemv(const double *A, std::uint32_t n, const double *x, double *y, std::uint32_t nvec, double *scratch,
-                 hipStream_t stream) {
+void launch_gemv(const void *A, bool fp32, std::uint32_t n, const double *x, double *y, std::uint32_t nvec,
+                 double *scratch, hipStream_t stream) {
     const std::uint32_t chunks = (n + kGemvCols - 1) / kGemvCols;
     for (std::uint32_t v0 = 0; v0 < nvec; v0 += kMaxVec) {
         const std::uint32_t nv = std::min<std::uint32_t>(kMaxVec, nvec - v0);
@@ -484,7 +486,12 @@ void launch_gemv(const double *A, std::uint32_t n, const double *x, double *y, s
         switch (nv) {
 #define HS_GEMV_CASE(N)                                                                                          \
     case N:                                                                                                      \
-        hipLaunchKernelGGL(gemv_partial<N>, grid, dim3(kGemvRows), 0, stream, A, n, xs, chunks, scratch);        \
+        if (fp32)                                                                                                \
+            hipLaunchKernelGGL((gemv_partial<N, float2>), grid, dim3(kGemvRows), 0, stream,                      \
+                               static_cast<const float2 *>(A), n, xs, chunks, scratch);                          \
+        else                                                                                                     \
+            hipLaunchKernelGGL((gemv_partial<N, double2>), grid, dim3(kGemvRows), 0, stream,                     \
+                               static_cast<const double2 *>(A), n, xs, chunks, scratch);                         \
         break;
             HS_GEMV_CASE(1)
             HS_GEMV_CASE(2)
@@ -534,9 +541,9 @@ void launch_scale(std::uint32_t n, double s, const double *x, double *y, hipStre
     hipLaunchKernelGGL(scale_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, s, x, y);
 }
 
-void launch_residual(const double *A, std::uint32_t n, const double *b, const double *x, std::uint32_t nvec,
+void launch_residual(const void *A, bool fp32, std::uint32_t n, const double *b, const double *x, std::uint32_t nvec,
                      double *residual, double *scratch, hipStream_t stream) {
-    launch_gemv(A, n, x, residual, nvec, scratch, stream);
+    launch_gemv(A, fp32, n, x, residual, nvec, scratch, stream);
     const std::size_t count = 2 * static_cast<std::size_t>(n) * nvec;
     hipLaunchKernelGGL(subtract_vectors, dim3((count + 255) / 256), dim3(256), 0, stream, b, residual, count, residual);
 }

@@ -272,7 +272,13 @@ dev::AssemblyLaunch MeshSolver::assembly_launch(const DevicePlan &p, const Numer
     L.args.panels = {geo_.data(), nodes_.data(), bounds_.data(), shape_.data(), nt(), nv()};
     L.args.row_scale = row_scale_.data();
     L.args.flux_column = flux_column;
-    L.args.A = A;
+    if (n.precision == 1) {
+        L.args.A = nullptr;
+        L.args.A32 = reinterpret_cast<float *>(A);
+    } else {
+        L.args.A = A;
+        L.args.A32 = nullptr;
+    }
     L.args.BD = BD;
     L.args.nd = nd;
     L.args.k = k;
@@ -298,7 +304,8 @@ void MeshSolver::assemble_dense(const Numerics &n, double k_max, double k, doubl
     DevicePlan &p = device_plan(n, k_max, nullptr);
     const std::uint32_t nv = this->nv(), nt = this->nt();
     hipStream_t s = ctx_->stream;
-    DeviceBuffer<double> A(2 * static_cast<std::size_t>(nv) * nv), BD(2 * static_cast<std::size_t>(nv) * nt);
+    const bool fp32 = n.precision == 1;
+    DeviceBuffer<double> A((fp32 ? 1 : 2) * static_cast<std::size_t>(nv) * nv), BD(2 * static_cast<std::size_t>(nv) * nt);
     std::vector<std::int32_t> cols(nt);
     std::iota(cols.begin(), cols.end(), 0);
     DeviceBuffer<std::int32_t> fc;
@@ -309,8 +316,15 @@ void MeshSolver::assemble_dense(const Numerics &n, double k_max, double k, doubl
     dev::launch_assembly(assembly_launch(p, n, k, A.data(), BD.data(), nt, fc.data()), s);
     hip_check(hipGetLastError(), "assembly launch");
     check_error_flag(error_, s);
-    std::vector<double> a(A.size()), b(BD.size());
-    hip_check(hipMemcpyAsync(a.data(), A.data(), A.bytes(), hipMemcpyDeviceToHost, s), "A download");
+    std::vector<double> a(2 * static_cast<std::size_t>(nv) * nv), b(BD.size());
+    if (fp32) {
+        std::vector<float> a32(2 * static_cast<std::size_t>(nv) * nv);
+        hip_check(hipMemcpyAsync(a32.data(), A.data(), A.bytes(), hipMemcpyDeviceToHost, s), "A download");
+        hip_check(hipStreamSynchronize(s), "A sync");
+        std::copy(a32.begin(), a32.end(), a.begin());
+    } else {
+        hip_check(hipMemcpyAsync(a.data(), A.data(), A.bytes(), hipMemcpyDeviceToHost, s), "A download");
+    }
     hip_check(hipMemcpyAsync(b.data(), BD.data(), BD.bytes(), hipMemcpyDeviceToHost, s), "B download");
     hip_check(hipStreamSynchronize(s), "assembly download");
     // internal node order -> original vertex order
@@ -333,7 +347,8 @@ void MeshSolver::assemble_dense(const Numerics &n, double k_max, double k, doubl
 // Restarted GMRES with twice classical Gram-Schmidt on the device; host
 // Givens rotations (device_gmres.cu:197-279, without a preconditioner: the
 // dense system is already row-equilibrated by the lumped mass).
-void dense_gmres(hipStream_t s, const double *A, std::uint32_t n, const double *b, double *x, std::uint32_t restart,
+void dense_gmres(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const double *b, double *x,
+                 std::uint32_t restart,
                  std::uint32_t max_it, double tol, double *achieved, std::uint32_t *iterations,
                  const volatile std::int32_t *cancel, GmresWorkspace &ws) {
     using C = std::complex<double>;
@@ -390,7 +405,7 @@ void dense_gmres(hipStream_t s, const double *A, std::uint32_t n, const double *
             if (cancel && *cancel)
                 throw Cancelled();
             double *w = basis_.data() + 2 * static_cast<std::size_t>(j + 1) * n;
-            dev::launch_gemv(A, n, basis_.data() + 2 * static_cast<std::size_t>(j) * n, w, 1, scratch_.data(), s);
+            dev::launch_gemv(A, fp32, n, basis_.data() + 2 * static_cast<std::size_t>(j) * n, w, 1, scratch_.data(), s);
             dev::launch_project(basis_.data(), n, j + 1, w, h1, scratch_.data(), s);
             dev::launch_subtract_combination(basis_.data(), n, j + 1, h1, w, s);
             dev::launch_project(basis_.data(), n, j + 1, w, h2, scratch_.data(), s);
@@ -443,7 +458,7 @@ void dense_gmres(hipStream_t s, const double *A, std::uint32_t n, const double *
         }
         hip_check(hipMemcpyAsync(h1, neg.data(), sizeof(double) * neg.size(), hipMemcpyHostToDevice, s), "y upload");
         dev::launch_subtract_combination(basis_.data(), n, used, h1, x, s);
-        dev::launch_residual(A, n, b, x, 1, r_.data(), scratch_.data(), s);
+        dev::launch_residual(A, fp32, n, b, x, 1, r_.data(), scratch_.data(), s);
         beta = norm(r_.data());
     }
     *achieved = beta / bnorm;
@@ -468,7 +483,7 @@ Result MeshSolver::solve(const Request &req, ProgressFn progress, void *user, co
         !std::isfinite(num.near_ratio) || num.near_ratio < 0 || num.near_ratio > 10 ||
         !std::isfinite(num.near_tolerance) || num.near_tolerance < 1e-10 || num.near_tolerance > .01 ||
         num.near_depth > 10 || !std::isfinite(num.residual_tolerance) || num.residual_tolerance < 1e-12 ||
-        num.residual_tolerance > .01 || num.refinement_steps > 10 || num.solver > 1 || num.gmres_restart < 2 ||
+        num.residual_tolerance > .01 || num.refinement_steps > 10 || num.solver > 1 || num.precision > 1 || num.gmres_restart < 2 ||
         num.gmres_restart > 300 || num.gmres_max_iterations < 1)
         throw std::invalid_argument("Invalid BEM medium, frequencies, excitation or Galerkin settings.");
     if (req.excitations.size() > 1024 || req.observers.size() > 100000)
@@ -533,7 +548,10 @@ Result MeshSolver::solve(const Request &req, ProgressFn progress, void *user, co
     cancelled();
     // workspace
     const std::size_t nv2 = static_cast<std::size_t>(nv) * nv;
-    A_.resize(2 * nv2);
+    // Mixed form: FP32 matrix (half the bytes) with FP64 Krylov vectors; always GMRES.
+    const bool mixed = num.precision == 1;
+    const unsigned solver_kind = mixed ? 1u : num.solver;
+    A_.resize((mixed ? 1 : 2) * nv2);
     BD_.resize(2 * static_cast<std::size_t>(nv) * std::max<std::uint32_t>(nd, 1));
     q_.resize(2 * static_cast<std::size_t>(std::max<std::uint32_t>(nd, 1)) * ne);
     rhs_.resize(2 * static_cast<std::size_t>(nv) * ne);
@@ -545,7 +563,7 @@ Result MeshSolver::solve(const Request &req, ProgressFn progress, void *user, co
                                           dev::reduction_scratch_doubles(nv, num.gmres_restart + 1), std::size_t{4096}});
     if (scratch_.size() < scratch)
         scratch_.resize(scratch);
-    if (num.solver == 0) {
+    if (solver_kind == 0) {
         LU_.resize(2 * nv2);
         pivots_.resize(nv);
     }
@@ -669,7 +687,7 @@ Result MeshSolver::solve(const Request &req, ProgressFn progress, void *user, co
         const auto t_solve = Clock::now();
         std::vector<std::uint32_t> iterations(ne, 0);
         std::uint32_t refinements = 0;
-        if (num.solver == 0) {
+        if (solver_kind == 0) {
             hip_check(hipMemcpyAsync(LU_.data(), A_.data(), A_.bytes(), hipMemcpyDeviceToDevice, s), "LU copy");
             hip_check(hipMemsetAsync(error_.data(), 0, sizeof(int), s), "clear info");
             const auto t_f = Clock::now();
@@ -688,7 +706,7 @@ Result MeshSolver::solve(const Request &req, ProgressFn progress, void *user, co
         } else {
             for (std::uint32_t e = 0; e < ne; ++e) {
                 double achieved = 0;
-                dense_gmres(s, A_.data(), nv, rhs_.data() + 2 * static_cast<std::size_t>(e) * nv,
+                dense_gmres(s, A_.data(), mixed, nv, rhs_.data() + 2 * static_cast<std::size_t>(e) * nv,
                             x_.data() + 2 * static_cast<std::size_t>(e) * nv, num.gmres_restart,
                             num.gmres_max_iterations, num.residual_tolerance * 0.1, &achieved, &iterations[e], cancel,
                             gmres_);
@@ -697,7 +715,7 @@ Result MeshSolver::solve(const Request &req, ProgressFn progress, void *user, co
         // FP64 residual of the unscaled equation; LU corrections while above tolerance
         std::vector<double> relative(ne);
         for (;;) {
-            dev::launch_residual(A_.data(), nv, rhs_.data(), x_.data(), ne, r_.data(), scratch_.data(), s);
+            dev::launch_residual(A_.data(), mixed, nv, rhs_.data(), x_.data(), ne, r_.data(), scratch_.data(), s);
             double largest = 0;
             for (std::uint32_t e = 0; e < ne; ++e) {
                 dev::launch_norm2(r_.data() + 2 * static_cast<std::size_t>(e) * nv, nv, lumped_.data(),
@@ -714,7 +732,7 @@ Result MeshSolver::solve(const Request &req, ProgressFn progress, void *user, co
             }
             if (largest <= num.residual_tolerance)
                 break;
-            if (num.solver != 0 || refinements >= num.refinement_steps) {
+            if (solver_kind != 0 || refinements >= num.refinement_steps) {
                 std::ostringstream msg;
                 msg.precision(9);
                 msg << "Burton-Miller residual " << largest << " exceeds tolerance " << num.residual_tolerance << " at "

@@ -139,7 +139,7 @@ struct GmresWorkspace {
 
 // Restarted GMRES(restart) with CGS2 Arnoldi on a dense row-major-free complex
 // system (A n x n, column-major); converged when ||b - A x|| <= tol ||b||.
-void dense_gmres(hipStream_t stream, const double *A, std::uint32_t n, const double *b, double *x,
+void dense_gmres(hipStream_t stream, const void *A, bool fp32, std::uint32_t n, const double *b, double *x,
                  std::uint32_t restart, std::uint32_t max_it, double tol, double *achieved, std::uint32_t *iterations,
                  const volatile std::int32_t *cancel, GmresWorkspace &ws);
 

@@ -209,3 +209,70 @@ def test_gmres_solve_matches_lu():
     b = r_gm.frequencies[0].excitations[0].boundary.pressure_trace
     assert np.linalg.norm(a - b) / np.linalg.norm(a) <= 1e-9
     assert r_gm.frequencies[0].excitations[0].iterations > 0
+
+
+# Mixed form: FP32 regular and near samples with FP32 accumulation, FP64 geometry
+# differences, FP64 touching (Sauter-Schwab) pairs, FP32 matrix, FP64 Krylov
+# vectors and FP64 residual. Gates are against the FP64 oracle; the reference's
+# own mixed CUDA output is gated at 1 % L2 / 0.1 dB / 1 degree against BEAT
+# (bem_cuda_tests.cpp:416-459).
+MIXED = dict(precision=bem.PRECISION_MIXED)
+
+
+@pytest.mark.parametrize("level,frequency", [(3, 1715.0), (3, 4000.0)])
+def test_mixed_assembly_close_to_oracle(level, frequency):
+    mesh = meshes.octa_sphere(level)
+    k = 2 * math.pi * frequency / 343.0
+    num = bem.BemNumerics(**MIXED)
+    A, B = bem.HipBemSolver(mesh).assemble(num, k, k)
+    Ao, Bo = ob.oracle_assemble(mesh, bem.interaction_plan(mesh, num, k), k)
+    ea, eb = _rel(A, Ao), _rel(B, Bo)
+    print(f"mixed octa{level} {frequency} Hz: A {ea:.2e}  B {eb:.2e}")
+    assert ea <= 2e-6 and eb <= 2e-6
+
+
+def test_mixed_solve_close_to_fp64_icosphere():
+    mesh = meshes.icosphere(3)
+    f, rho = 1000.0, 1.205
+    k = 2 * math.pi * f / 343.0
+    req = bem.BemRequest(medium=bem.AcousticMedium(rho, 343.0), frequencies=[f],
+                         excitations=[bem.BemExcitation("unit", [bem.FaceVelocity(1, 1.0)])],
+                         numerics=bem.BemNumerics(**MIXED))
+    ex = bem.HipBemSolver(mesh).solve(req).frequencies[0].excitations[0]
+    Ao, Bo = ob.oracle_assemble(mesh, bem.interaction_plan(mesh, bem.BemNumerics(), k), k)
+    p = ob.oracle_lu_solve(Ao, Bo @ flux(mesh, {1: 1.0}, f, rho))
+    err = np.linalg.norm(ex.boundary.pressure_trace - p) / np.linalg.norm(p)
+    print(f"mixed C1: L2 vs FP64 oracle {err:.2e}, residual {ex.boundary_residual:.2e}, iterations {ex.iterations}")
+    assert err <= 1e-5
+    assert ex.boundary_residual <= bem.BemNumerics().relative_residual_tolerance
+    assert ex.iterations > 0
+
+
+def test_mixed_beat_sphere_observers():
+    beat = beat_sphere()
+    res = bem.HipBemSolver(meshes.octa_sphere(3)).solve(sphere_request(BEAT_FREQUENCIES, numerics=bem.BemNumerics(**MIXED)))
+    for fr in res.frequencies:
+        got = fr.excitations[0].observer_pressure
+        want = beat[fr.frequency]
+        l2 = np.linalg.norm(got - want) / np.linalg.norm(want)
+        print(f"mixed BEAT {fr.frequency} Hz: L2 {l2:.2e}")
+        assert l2 <= 1e-5
+
+
+@pytest.mark.parametrize("frequency", [2000.0, 5000.0])
+def test_mixed_horn_close_to_fp64(frequency):
+    mesh = meshes.exponential_horn(0.02)
+    solver = bem.HipBemSolver(mesh)
+    req = lambda num: bem.BemRequest(medium=bem.AcousticMedium(1.205, 343.0), frequencies=[frequency],
+                                     excitations=[bem.BemExcitation("throat", [bem.FaceVelocity(1, 1.0)])],
+                                     numerics=num)
+    a = solver.solve(req(bem.BemNumerics(solver=bem.SOLVER_GMRES, relative_residual_tolerance=1e-9)))
+    b = solver.solve(req(bem.BemNumerics(**MIXED)))
+    pa = a.frequencies[0].excitations[0].boundary.pressure_trace
+    pb = b.frequencies[0].excitations[0].boundary.pressure_trace
+    za = a.frequencies[0].excitations[0].impedances[0].volume_impedance
+    zb = b.frequencies[0].excitations[0].impedances[0].volume_impedance
+    err = np.linalg.norm(pb - pa) / np.linalg.norm(pa)
+    print(f"mixed horn {frequency} Hz: trace L2 {err:.2e}, throat impedance {abs(zb / za - 1):.2e}")
+    assert err <= 1e-5
+    assert abs(zb / za - 1) <= 1e-5
