@@ -14,8 +14,14 @@ request (16 frequency points); the interaction plan is built in the first
 (untimed) warmup step and reported as the cold cost. The FP64 LU cross-check
 of one frequency is untimed.
 
-Prints ONE JSON line. value = solved frequency points per second of the mixed
-form (or of the one form asked for), with the plain FP64 form and the mixed
+Forms: plain FP64 and mixed, each solved directly (one request at a time, one
+stream) and through the serving loop (slots with their own streams taking the
+frequency points of all queued requests; saturated throughput from `steps`
+sweep requests submitted together, latency from an open-loop Poisson stream of
+single-point requests at 70 % of that throughput).
+
+Prints ONE JSON line. value = solved frequency points per second of the served
+mixed form (or of the form asked for), with the plain FP64 form and the mixed
 form's surface-pressure error against it beside it.
 """
 from __future__ import annotations
@@ -47,14 +53,14 @@ def device_synchronize():
     hip.hipDeviceSynchronize()
 
 
-def horn_request(frequencies, precision, solver, tolerance):
+def horn_request(frequencies, precision, solver, tolerance, planning=()):
     directions = [(1, 0, 0)] + [(math.cos(math.radians(a)), math.sin(math.radians(a)), 0) for a in (15, 30, 45, 60, 90)] \
         + [(-1, 0, 0)]
     return bem.BemRequest(
         medium=bem.AcousticMedium(1.205, 343.0), frequencies=list(frequencies),
         excitations=[bem.BemExcitation("piston", [bem.FaceVelocity(1, 1.0)])],
         observers=[(0.3 + 10 * d[0], 10 * d[1], 10 * d[2]) for d in directions],
-        far_field=bem.FarFieldObservation((0.3, 0.0, 0.0), 1.0),
+        far_field=bem.FarFieldObservation((0.3, 0.0, 0.0), 1.0), interaction_plan_frequencies=list(planning),
         numerics=bem.BemNumerics(solver=solver, relative_residual_tolerance=tolerance, gmres_restart=50,
                                  gmres_maximum_iterations=2000, precision=precision))
 
@@ -138,6 +144,56 @@ def run_form(solver, frequencies, precision, solver_kind, tolerance, steps, warm
             "traces": [fr.excitations[0].boundary.surface_pressure for fr in res.frequencies]}
 
 
+def run_served(solver, frequencies, precision, tolerance, steps, slots, label, open_loop_points=0, seed=20261203):
+    """Serving loop: `steps` sweep requests submitted together (saturated throughput),
+    then optionally an open-loop Poisson stream of single-frequency requests at 70 %
+    of that throughput (latency from submission to completion of each point)."""
+    req = horn_request(frequencies, precision, bem.SOLVER_GMRES, tolerance)
+    server = bem.HipBemServer(slots=slots)
+    server.poll(server.submit(solver, horn_request(frequencies[:slots], precision, bem.SOLVER_GMRES, tolerance)))
+    device_synchronize()
+    start = time.perf_counter()
+    tickets = [server.submit(solver, req) for _ in range(steps)]
+    results = [server.poll(t) for t in tickets]
+    device_synchronize()
+    elapsed = time.perf_counter() - start
+    points = steps * len(frequencies)
+    kernel_ms = sum(fr.kernel_milliseconds["regular"] for r in results for fr in r.frequencies)
+    samples = sum(fr.kernel_milliseconds["regular_samples"] for r in results for fr in r.frequencies)
+    out = {"value": points / elapsed, "elapsed": elapsed, "points": points, "plan_ms": None,
+           "kernel_ms": kernel_ms, "samples": samples,
+           "iterations": [fr.excitations[0].iterations for r in results for fr in r.frequencies],
+           "traces": [fr.excitations[0].boundary.surface_pressure for fr in results[-1].frequencies],
+           "stats": server.stats()}
+    print(f"# {label}: {points / elapsed:.2f} points/s ({slots} slots, {steps} sweep requests)",
+          file=sys.stderr, flush=True)
+    if open_loop_points:
+        rng = np.random.default_rng(seed)
+        rate = 0.7 * out["value"]
+        gaps = rng.exponential(1.0 / rate, open_loop_points)
+        picks = rng.integers(0, len(frequencies), open_loop_points)
+        t0 = time.perf_counter()
+        arrival = t0
+        tickets = []
+        for gap, i in zip(gaps, picks):
+            arrival += gap
+            delay = arrival - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            # Points drawn from the design sweep share its interaction plan (planning band of the sweep).
+            tickets.append(server.submit(solver, horn_request([frequencies[i]], precision, bem.SOLVER_GMRES,
+                                                              tolerance, planning=[max(frequencies)])))
+        lat = [server.poll(t).frequencies[0].served_milliseconds for t in tickets]
+        out["p50"] = float(np.percentile(lat, 50))
+        out["p99"] = float(np.percentile(lat, 99))
+        out["open_loop"] = {"requests": open_loop_points, "rate_per_s": rate,
+                            "achieved_per_s": open_loop_points / (time.perf_counter() - t0)}
+        print(f"# {label} open loop: {open_loop_points} single-point requests at {rate:.2f}/s, latency p50 "
+              f"{out['p50']:.0f} ms p99 {out['p99']:.0f} ms", file=sys.stderr, flush=True)
+    server.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,7 +203,9 @@ def main():
     ap.add_argument("--frequencies", type=int, default=16)
     ap.add_argument("--solver", choices=["gmres", "lu"], default="gmres")
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both",
-                    help="both: time the plain FP64 form and the mixed form; value is the mixed form")
+                    help="both: time the plain FP64 form and the mixed form; value is the served mixed form")
+    ap.add_argument("--slots", type=int, default=3, help="serving-loop slots (0: no served runs)")
+    ap.add_argument("--open-loop", type=int, default=48, help="open-loop single-point requests for latency")
     ap.add_argument("--tolerance", type=float, default=1e-8)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -184,17 +242,25 @@ def main():
     if args.precision in ("both", "mixed"):
         forms["mixed"] = run_form(solver, frequencies, bem.PRECISION_MIXED, bem.SOLVER_GMRES, args.tolerance,
                                   args.steps, args.warmup, "mixed")
-    head_name = "mixed" if "mixed" in forms else "fp64"
+    if args.slots:
+        for name, prec in (("served_fp64", bem.PRECISION_FP64), ("served_mixed", bem.PRECISION_MIXED)):
+            if name.split("_")[1] in forms:
+                forms[name] = run_served(solver, frequencies, prec, args.tolerance, args.steps, args.slots,
+                                         name.replace("_", " "), args.open_loop)
+    head_name = next(n for n in ("served_mixed", "mixed", "served_fp64", "fp64") if n in forms)
     head = forms[head_name]
     mixed_error = None
-    if "mixed" in forms and "fp64" in forms:
+    if head_name in ("mixed", "served_mixed") and "fp64" in forms:
         # The plain FP64 GPU form equals the FP64 oracle to ~1e-15 (parity tests), so
         # this is the mixed form's surface-pressure error against the FP64 oracle.
         mixed_error = max(float(np.linalg.norm(m - p) / np.linalg.norm(p))
-                          for m, p in zip(forms["mixed"]["traces"], forms["fp64"]["traces"]))
-    mixed_head = head_name == "mixed"
+                          for m, p in zip(head["traces"], forms["fp64"]["traces"]))
+    mixed_head = head_name in ("mixed", "served_mixed")
     peak = FP32_PEAK_TFLOPS if mixed_head else FP64_PEAK_TFLOPS
-    achieved = FLOPS_PER_SAMPLE * head["samples"] / (head["kernel_ms"] / 1e3) / 1e12 if head["kernel_ms"] > 0 else None
+    # Kernel roofline from the direct run of the same precision: in the served runs,
+    # kernels of other slots overlap and inflate each launch's event time.
+    roof = forms.get("mixed" if mixed_head else "fp64", head)
+    achieved = FLOPS_PER_SAMPLE * roof["samples"] / (roof["kernel_ms"] / 1e3) / 1e12 if roof["kernel_ms"] > 0 else None
     line = {
         "metric": "freq-points solved/s + p50/p99 latency, served mixed vs plain FP64 @<=1e-4 err",
         "value": head["value"], "unit": "freq-points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -207,21 +273,25 @@ def main():
                                f"(restart 50, rel tol {args.tolerance:g}), 7 far-field directions",
                    "triangles": nt, "nodes": nv, "frequencies": len(frequencies), "solver": args.solver,
                    "form": head_name},
-        "latency_ms": {"p50": head["p50"], "p99": head["p99"]},
+        "latency_ms": {"p50": head.get("p50"), "p99": head.get("p99")},
         "forms": {name: {"value": f["value"], "ms_per_step": 1e3 * f["elapsed"] / args.steps,
-                         "latency_ms": {"p50": f["p50"], "p99": f["p99"]},
+                         "latency_ms": {"p50": f.get("p50"), "p99": f.get("p99")},
                          "regular_kernel_ms": f["kernel_ms"] / f["points"],
-                         "gmres_iterations": {"min": int(min(f["iterations"])), "max": int(max(f["iterations"]))}}
+                         "gmres_iterations": {"min": int(min(f["iterations"])), "max": int(max(f["iterations"]))},
+                         **({"open_loop": f["open_loop"]} if "open_loop" in f else {}),
+                         **({"server": {k: f["stats"][k] for k in ("max_running", "peak_held_bytes", "budget_bytes")}}
+                            if "stats" in f else {})}
                   for name, f in forms.items()},
-        "mixed_vs_fp64": ({"speedup": forms["mixed"]["value"] / forms["fp64"]["value"],
+        "mixed_vs_fp64": ({"form": head_name, "speedup": head["value"] / forms["fp64"]["value"],
                            "max_rel_l2_surface_pressure": mixed_error} if mixed_error is not None else None),
-        "cold": {"mesh_setup_s": construct_s, "interaction_plan_ms": head["plan_ms"]},
+        "cold": {"mesh_setup_s": construct_s,
+                 "interaction_plan_ms": next(f["plan_ms"] for f in forms.values() if f["plan_ms"] is not None)},
         "lu_cross_check_rel_l2": cross,
         "roofline": {"kernel": f"regular_tiles ({'FP32' if mixed_head else 'FP64'} pair quadrature, VALU)",
                      "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak if achieved else None, "traffic": None,
-                     "launch_ms": head["kernel_ms"] / head["points"],
-                     "flops_per_launch": FLOPS_PER_SAMPLE * head["samples"] / head["points"]},
+                     "launch_ms": roof["kernel_ms"] / roof["points"],
+                     "flops_per_launch": FLOPS_PER_SAMPLE * roof["samples"] / roof["points"]},
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(mesh, frequencies, args.cpu_threads)

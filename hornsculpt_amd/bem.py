@@ -23,7 +23,7 @@ import numpy as np
 
 _LIB_PATH = Path(__file__).resolve().parent / "libhsbem.so"
 
-HSBEM_OK, HSBEM_INVALID_ARGUMENT, HSBEM_RUNTIME_ERROR, HSBEM_DEVICE_ERROR, HSBEM_CANCELLED = range(5)
+HSBEM_OK, HSBEM_INVALID_ARGUMENT, HSBEM_RUNTIME_ERROR, HSBEM_DEVICE_ERROR, HSBEM_CANCELLED, HSBEM_PENDING = range(6)
 SOLVER_LU, SOLVER_GMRES = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 WALL, DRIVEN = 0, 1
@@ -65,6 +65,17 @@ class _Request(ctypes.Structure):
         ("planning_frequency_count", ctypes.c_uint32),
         ("numerics", _Numerics),
     ]
+
+
+class _ServerOptions(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_uint32), ("memory_budget_bytes", ctypes.c_uint64)]
+
+
+class _ServerStatistics(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+                ("points", ctypes.c_uint64), ("running", ctypes.c_uint32), ("queued", ctypes.c_uint32),
+                ("max_running", ctypes.c_uint32), ("budget_bytes", ctypes.c_uint64), ("held_bytes", ctypes.c_uint64),
+                ("peak_held_bytes", ctypes.c_uint64)]
 
 
 _PROGRESS = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_int32)
@@ -118,6 +129,13 @@ def library() -> ctypes.CDLL:
                 "hsbem_dense_gmres": (ctypes.c_int, [P, U32, D, D, D, U32, U32, ctypes.c_double, D,
                                                      ctypes.POINTER(U32)]),
                 "hsbem_dense_lu_solve": (ctypes.c_int, [P, U32, D, U32, D]),
+                "hsbem_server_create": (ctypes.c_int, [P, ctypes.POINTER(_ServerOptions), ctypes.POINTER(P)]),
+                "hsbem_server_destroy": (ctypes.c_int, [P]),
+                "hsbem_submit": (ctypes.c_int, [P, P, ctypes.POINTER(_Request), ctypes.POINTER(U64)]),
+                "hsbem_poll": (ctypes.c_int, [P, U64, ctypes.c_int32, ctypes.POINTER(P)]),
+                "hsbem_cancel": (ctypes.c_int, [P, U64]),
+                "hsbem_server_stats": (ctypes.c_int, [P, ctypes.POINTER(_ServerStatistics)]),
+                "hsbem_result_served_latency": (ctypes.c_int, [P, U32, D]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(lib, name)
@@ -247,6 +265,7 @@ class BemFrequencyResult:
     field_milliseconds: float
     interaction_plan_milliseconds: float
     kernel_milliseconds: dict
+    served_milliseconds: float = 0.0   # serving loop: submission to completion of this point
 
 
 @dataclasses.dataclass
@@ -299,6 +318,45 @@ def device_description() -> str:
     return buf.value.decode()
 
 
+def _native_request(request: BemRequest):
+    """The C request struct and the arrays it points into (keep both alive)."""
+    freqs = np.ascontiguousarray(request.frequencies, dtype=np.float64)
+    plan_freqs = np.ascontiguousarray(request.interaction_plan_frequencies, dtype=np.float64)
+    offsets = [0]
+    face_ids, vel = [], []
+    for ex in request.excitations:
+        for fv in ex.faces:
+            face_ids.append(fv.face_id)
+            vel += [complex(fv.normal_velocity).real, complex(fv.normal_velocity).imag]
+        offsets.append(len(face_ids))
+    offsets = np.asarray(offsets, dtype=np.uint32)
+    face_ids = np.asarray(face_ids if face_ids else [0], dtype=np.uint32)
+    vel = np.asarray(vel if vel else [0.0, 0.0], dtype=np.float64)
+    obs = np.ascontiguousarray(np.asarray(request.observers, dtype=np.float64).reshape(-1, 3))
+    req = _Request()
+    req.density = request.medium.density
+    req.sound_speed = request.medium.sound_speed
+    req.frequencies = _dptr(freqs)
+    req.frequency_count = len(freqs)
+    req.excitation_count = len(request.excitations)
+    req.excitation_offsets = _uptr(offsets)
+    req.face_ids = _uptr(face_ids)
+    req.velocities = _dptr(vel)
+    req.observers = _dptr(obs) if len(obs) else ctypes.POINTER(ctypes.c_double)()
+    req.observer_count = len(obs)
+    req.far_field = 1 if request.far_field is not None else 0
+    if request.far_field is not None:
+        for i in range(3):
+            req.far_origin[i] = float(request.far_field.origin[i])
+        req.far_reference_distance = request.far_field.reference_distance
+    else:
+        req.far_reference_distance = 1.0
+    req.planning_frequencies = _dptr(plan_freqs) if len(plan_freqs) else ctypes.POINTER(ctypes.c_double)()
+    req.planning_frequency_count = len(plan_freqs)
+    req.numerics = request.numerics.native()
+    return req, (freqs, plan_freqs, offsets, face_ids, vel, obs)
+
+
 class HipBemSolver:
     """Drop-in for hornsculpt::CudaBemSolver (bem.hpp:367-388) on one MI355X."""
 
@@ -333,91 +391,16 @@ class HipBemSolver:
     def solve(self, request: BemRequest, cancel: Optional[ctypes.c_int32] = None,
               progress: Optional[Callable[[int, int, float, int], None]] = None) -> BemResult:
         lib = self._lib
-        freqs = np.ascontiguousarray(request.frequencies, dtype=np.float64)
-        plan_freqs = np.ascontiguousarray(request.interaction_plan_frequencies, dtype=np.float64)
-        offsets = [0]
-        face_ids, vel = [], []
-        for ex in request.excitations:
-            for fv in ex.faces:
-                face_ids.append(fv.face_id)
-                vel += [complex(fv.normal_velocity).real, complex(fv.normal_velocity).imag]
-            offsets.append(len(face_ids))
-        offsets = np.asarray(offsets, dtype=np.uint32)
-        face_ids = np.asarray(face_ids if face_ids else [0], dtype=np.uint32)
-        vel = np.asarray(vel if vel else [0.0, 0.0], dtype=np.float64)
-        obs = np.ascontiguousarray(np.asarray(request.observers, dtype=np.float64).reshape(-1, 3))
-        req = _Request()
-        req.density = request.medium.density
-        req.sound_speed = request.medium.sound_speed
-        req.frequencies = _dptr(freqs)
-        req.frequency_count = len(freqs)
-        req.excitation_count = len(request.excitations)
-        req.excitation_offsets = _uptr(offsets)
-        req.face_ids = _uptr(face_ids)
-        req.velocities = _dptr(vel)
-        req.observers = _dptr(obs) if len(obs) else ctypes.POINTER(ctypes.c_double)()
-        req.observer_count = len(obs)
-        req.far_field = 1 if request.far_field is not None else 0
-        if request.far_field is not None:
-            for i in range(3):
-                req.far_origin[i] = float(request.far_field.origin[i])
-            req.far_reference_distance = request.far_field.reference_distance
-        else:
-            req.far_reference_distance = 1.0
-        req.planning_frequencies = _dptr(plan_freqs) if len(plan_freqs) else ctypes.POINTER(ctypes.c_double)()
-        req.planning_frequency_count = len(plan_freqs)
-        req.numerics = request.numerics.native()
+        req, _keep = _native_request(request)
         cb = _PROGRESS(lambda user, done, total, f, phase: progress(done, total, f, phase)) if progress else \
             ctypes.cast(None, _PROGRESS)
         out = ctypes.c_void_p()
         cancel_ptr = ctypes.pointer(cancel) if cancel is not None else ctypes.POINTER(ctypes.c_int32)()
         _check(lib.hsbem_solve(self.handle, ctypes.byref(req), cb, None, cancel_ptr, ctypes.byref(out)))
         try:
-            return self._collect(out, request)
+            return _collect(lib, out, request)
         finally:
             lib.hsbem_result_destroy(out)
-
-    def _collect(self, handle, request: BemRequest) -> BemResult:
-        lib = self._lib
-        nf, ne, nt, nv, no = (ctypes.c_uint32() for _ in range(5))
-        _check(lib.hsbem_result_sizes(handle, *(ctypes.byref(x) for x in (nf, ne, nt, nv, no))))
-        peak = ctypes.c_uint64()
-        _check(lib.hsbem_result_peak_device_bytes(handle, ctypes.byref(peak)))
-        freqs = []
-        for f in range(nf.value):
-            ms = np.zeros(12)
-            _check(lib.hsbem_result_timings(handle, f, _dptr(ms)))
-            exs = []
-            for e in range(ne.value):
-                def cvec(fn, n):
-                    a = np.zeros(2 * n)
-                    _check(fn(handle, f, e, _dptr(a)))
-                    return a.view(np.complex128)
-                surface = cvec(lib.hsbem_result_surface_pressure, nt.value)
-                trace = cvec(lib.hsbem_result_pressure_trace, nv.value)
-                deriv = cvec(lib.hsbem_result_normal_derivative, nt.value)
-                observed = cvec(lib.hsbem_result_observer_pressure, no.value)
-                count = ctypes.c_uint32()
-                _check(lib.hsbem_result_impedances(handle, f, e, ctypes.byref(count), None, None, None, None, None))
-                c = count.value
-                ids = np.zeros(max(c, 1), np.uint32)
-                area, mp, vv, z = np.zeros(max(c, 1)), np.zeros(2 * max(c, 1)), np.zeros(2 * max(c, 1)), \
-                    np.zeros(2 * max(c, 1))
-                _check(lib.hsbem_result_impedances(handle, f, e, ctypes.byref(count), _uptr(ids), _dptr(area),
-                                                   _dptr(mp), _dptr(vv), _dptr(z)))
-                imps = [FaceImpedance(int(ids[i]), float(area[i]), complex(mp[2 * i], mp[2 * i + 1]),
-                                      complex(vv[2 * i], vv[2 * i + 1]), complex(z[2 * i], z[2 * i + 1]))
-                        for i in range(c)]
-                res, its, refs = ctypes.c_double(), ctypes.c_uint32(), ctypes.c_uint32()
-                _check(lib.hsbem_result_residual(handle, f, e, ctypes.byref(res), ctypes.byref(its),
-                                                 ctypes.byref(refs)))
-                exs.append(BemExcitationResult(request.excitations[e].name, observed, imps, res.value, its.value,
-                                               refs.value, BemBoundaryData(surface, trace, deriv)))
-            freqs.append(BemFrequencyResult(request.frequencies[f], exs, ms[0], ms[1], ms[2], ms[3], ms[4], ms[5],
-                                            {"regular": ms[6], "touching": ms[7], "near": ms[8],
-                                             "regular_samples": ms[9], "touching_samples": ms[10],
-                                             "near_samples": ms[11]}))
-        return BemResult("MI355X (gfx950)", nt.value, nv.value, peak.value, freqs)
 
     # -------- internal seams (component parity tests and benchmarks)
 
@@ -442,6 +425,101 @@ class HipBemSolver:
 
 
 CudaBemSolver = HipBemSolver
+
+
+def _collect(lib, handle, request: BemRequest) -> BemResult:
+    nf, ne, nt, nv, no = (ctypes.c_uint32() for _ in range(5))
+    _check(lib.hsbem_result_sizes(handle, *(ctypes.byref(x) for x in (nf, ne, nt, nv, no))))
+    peak = ctypes.c_uint64()
+    _check(lib.hsbem_result_peak_device_bytes(handle, ctypes.byref(peak)))
+    freqs = []
+    for f in range(nf.value):
+        ms = np.zeros(12)
+        _check(lib.hsbem_result_timings(handle, f, _dptr(ms)))
+        exs = []
+        for e in range(ne.value):
+            def cvec(fn, n):
+                a = np.zeros(2 * n)
+                _check(fn(handle, f, e, _dptr(a)))
+                return a.view(np.complex128)
+            surface = cvec(lib.hsbem_result_surface_pressure, nt.value)
+            trace = cvec(lib.hsbem_result_pressure_trace, nv.value)
+            deriv = cvec(lib.hsbem_result_normal_derivative, nt.value)
+            observed = cvec(lib.hsbem_result_observer_pressure, no.value)
+            count = ctypes.c_uint32()
+            _check(lib.hsbem_result_impedances(handle, f, e, ctypes.byref(count), None, None, None, None, None))
+            c = count.value
+            ids = np.zeros(max(c, 1), np.uint32)
+            area, mp, vv, z = np.zeros(max(c, 1)), np.zeros(2 * max(c, 1)), np.zeros(2 * max(c, 1)), \
+                np.zeros(2 * max(c, 1))
+            _check(lib.hsbem_result_impedances(handle, f, e, ctypes.byref(count), _uptr(ids), _dptr(area),
+                                               _dptr(mp), _dptr(vv), _dptr(z)))
+            imps = [FaceImpedance(int(ids[i]), float(area[i]), complex(mp[2 * i], mp[2 * i + 1]),
+                                  complex(vv[2 * i], vv[2 * i + 1]), complex(z[2 * i], z[2 * i + 1]))
+                    for i in range(c)]
+            res, its, refs = ctypes.c_double(), ctypes.c_uint32(), ctypes.c_uint32()
+            _check(lib.hsbem_result_residual(handle, f, e, ctypes.byref(res), ctypes.byref(its),
+                                             ctypes.byref(refs)))
+            exs.append(BemExcitationResult(request.excitations[e].name, observed, imps, res.value, its.value,
+                                           refs.value, BemBoundaryData(surface, trace, deriv)))
+        served = ctypes.c_double()
+        _check(lib.hsbem_result_served_latency(handle, f, ctypes.byref(served)))
+        freqs.append(BemFrequencyResult(request.frequencies[f], exs, ms[0], ms[1], ms[2], ms[3], ms[4], ms[5],
+                                        {"regular": ms[6], "touching": ms[7], "near": ms[8],
+                                         "regular_samples": ms[9], "touching_samples": ms[10],
+                                         "near_samples": ms[11]}, served.value))
+    return BemResult("MI355X (gfx950)", nt.value, nv.value, peak.value, freqs)
+
+
+
+class HipBemServer:
+    """Serving loop over the C ABI (hsbem_server_*): submit requests on resident
+    meshes, poll for results. Frequency points of all requests share the slots."""
+
+    def __init__(self, slots: int = 3, memory_budget_bytes: int = 0, context: Optional[Context] = None):
+        self._lib = library()
+        self.context = context or Context.default()
+        opts = _ServerOptions(slots, memory_budget_bytes)
+        handle = ctypes.c_void_p()
+        _check(self._lib.hsbem_server_create(self.context.handle, ctypes.byref(opts), ctypes.byref(handle)))
+        self.handle = handle
+        self._requests = {}
+
+    def close(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.hsbem_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, solver: HipBemSolver, request: BemRequest) -> int:
+        req, _keep = _native_request(request)
+        ticket = ctypes.c_uint64()
+        _check(self._lib.hsbem_submit(self.handle, solver.handle, ctypes.byref(req), ctypes.byref(ticket)))
+        self._requests[ticket.value] = request
+        return ticket.value
+
+    def poll(self, ticket: int, wait_ms: int = -1) -> Optional[BemResult]:
+        """The result when finished, None while pending (wait_ms < 0 waits to the end)."""
+        out = ctypes.c_void_p()
+        status = self._lib.hsbem_poll(self.handle, ticket, wait_ms, ctypes.byref(out))
+        if status == HSBEM_PENDING:
+            return None
+        request = self._requests.pop(ticket, None)
+        _check(status)
+        try:
+            return _collect(self._lib, out, request)
+        finally:
+            self._lib.hsbem_result_destroy(out)
+
+    def cancel(self, ticket: int) -> None:
+        _check(self._lib.hsbem_cancel(self.handle, ticket))
+
+    def stats(self) -> dict:
+        st = _ServerStatistics()
+        _check(self._lib.hsbem_server_stats(self.handle, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in st._fields_}
 
 
 def _copy_plan(sizes_fn, copy_fn, handle, nt: int) -> dict:

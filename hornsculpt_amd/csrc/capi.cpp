@@ -2,6 +2,7 @@
 // way the reference throws them (std::invalid_argument / std::runtime_error).
 #include "../../include/hsbem.h"
 
+#include "server.hpp"
 #include "solver.hpp"
 
 #include <cstring>
@@ -14,8 +15,11 @@ struct hsbem_context {
 };
 struct hsbem_mesh {
     hsbem_context *context;
-    std::unique_ptr<hsbem::MeshSolver> solver;
+    std::shared_ptr<hsbem::MeshSolver> solver; // shared with serving tickets in flight
     std::uint64_t hash;
+};
+struct hsbem_server {
+    std::unique_ptr<hsbem::Server> server;
 };
 struct hsbem_result {
     hsbem::Result result;
@@ -69,6 +73,34 @@ hsbem::Numerics to_numerics(const hsbem_numerics &n) {
     o.precision = n.precision;
     o.plan_threads = n.plan_threads;
     return o;
+}
+
+hsbem::Request to_request(const hsbem_request &rr) {
+    const hsbem_request *r = &rr;
+    require(r->frequency_count == 0 || r->frequencies, "null frequencies");
+    require(r->excitation_count == 0 || r->excitation_offsets, "null excitation offsets");
+    require(r->observer_count == 0 || r->observers, "null observers");
+    hsbem::Request q;
+    q.density = r->density;
+    q.sound_speed = r->sound_speed;
+    q.frequencies.assign(r->frequencies, r->frequencies + r->frequency_count);
+    if (r->planning_frequency_count) {
+        require(r->planning_frequencies, "null planning frequencies");
+        q.planning_frequencies.assign(r->planning_frequencies, r->planning_frequencies + r->planning_frequency_count);
+    }
+    for (uint32_t e = 0; e < r->excitation_count; ++e) {
+        hsbem::Excitation ex;
+        for (uint32_t j = r->excitation_offsets[e]; j < r->excitation_offsets[e + 1]; ++j)
+            ex.faces.emplace_back(r->face_ids[j], std::complex<double>(r->velocities[2 * j], r->velocities[2 * j + 1]));
+        q.excitations.push_back(std::move(ex));
+    }
+    for (uint32_t i = 0; i < r->observer_count; ++i)
+        q.observers.push_back({r->observers[3 * i], r->observers[3 * i + 1], r->observers[3 * i + 2]});
+    q.far_field = r->far_field != 0;
+    q.far_origin = {r->far_origin[0], r->far_origin[1], r->far_origin[2]};
+    q.far_reference_distance = r->far_reference_distance;
+    q.numerics = to_numerics(r->numerics);
+    return q;
 }
 
 const hsbem::ExcitationResult &excitation(const hsbem_result *r, std::uint32_t f, std::uint32_t e) {
@@ -177,7 +209,7 @@ int hsbem_mesh_create(hsbem_context *c, const double *vertices, uint32_t nv, con
         auto mesh = std::make_unique<hsbem_mesh>();
         mesh->context = c;
         mesh->hash = h;
-        mesh->solver = std::make_unique<hsbem::MeshSolver>(&c->ctx, std::move(m));
+        mesh->solver = std::make_shared<hsbem::MeshSolver>(&c->ctx, std::move(m));
         *out = mesh.release();
     });
 }
@@ -204,33 +236,81 @@ int hsbem_solve(hsbem_mesh *m, const hsbem_request *r, hsbem_progress_fn progres
                 const volatile int32_t *cancel, hsbem_result **out) {
     return guarded([&] {
         require(m && r && out, "invalid solve arguments");
-        require(r->frequency_count == 0 || r->frequencies, "null frequencies");
-        require(r->excitation_count == 0 || r->excitation_offsets, "null excitation offsets");
-        require(r->observer_count == 0 || r->observers, "null observers");
-        hsbem::Request q;
-        q.density = r->density;
-        q.sound_speed = r->sound_speed;
-        q.frequencies.assign(r->frequencies, r->frequencies + r->frequency_count);
-        if (r->planning_frequency_count) {
-            require(r->planning_frequencies, "null planning frequencies");
-            q.planning_frequencies.assign(r->planning_frequencies, r->planning_frequencies + r->planning_frequency_count);
-        }
-        for (uint32_t e = 0; e < r->excitation_count; ++e) {
-            hsbem::Excitation ex;
-            for (uint32_t j = r->excitation_offsets[e]; j < r->excitation_offsets[e + 1]; ++j)
-                ex.faces.emplace_back(r->face_ids[j], std::complex<double>(r->velocities[2 * j], r->velocities[2 * j + 1]));
-            q.excitations.push_back(std::move(ex));
-        }
-        for (uint32_t i = 0; i < r->observer_count; ++i)
-            q.observers.push_back({r->observers[3 * i], r->observers[3 * i + 1], r->observers[3 * i + 2]});
-        q.far_field = r->far_field != 0;
-        q.far_origin = {r->far_origin[0], r->far_origin[1], r->far_origin[2]};
-        q.far_reference_distance = r->far_reference_distance;
-        q.numerics = to_numerics(r->numerics);
+        const hsbem::Request q = to_request(*r);
         hsbem::hip_check(hipSetDevice(m->context->ctx.device), "set device");
         auto res = std::make_unique<hsbem_result>();
         res->result = m->solver->solve(q, progress, user, cancel);
         *out = res.release();
+    });
+}
+
+int hsbem_server_create(hsbem_context *c, const hsbem_server_options *options, hsbem_server **out) {
+    return guarded([&] {
+        require(c && out, "invalid server arguments");
+        hsbem::ServerOptions o;
+        if (options) {
+            o.slots = options->slots;
+            o.memory_budget = options->memory_budget_bytes;
+        }
+        auto srv = std::make_unique<hsbem_server>();
+        srv->server = std::make_unique<hsbem::Server>(&c->ctx, o);
+        *out = srv.release();
+    });
+}
+
+int hsbem_server_destroy(hsbem_server *s) {
+    return guarded([&] { delete s; });
+}
+
+int hsbem_submit(hsbem_server *s, hsbem_mesh *m, const hsbem_request *r, uint64_t *ticket) {
+    return guarded([&] {
+        require(s && m && r && ticket, "invalid submit arguments");
+        *ticket = s->server->submit(m->solver, to_request(*r));
+    });
+}
+
+int hsbem_poll(hsbem_server *s, uint64_t ticket, int32_t wait_ms, hsbem_result **out) {
+    int pending = 0;
+    const int status = guarded([&] {
+        require(s && out, "invalid poll arguments");
+        *out = nullptr;
+        auto res = std::make_unique<hsbem_result>();
+        if (s->server->poll(ticket, wait_ms, &res->result))
+            *out = res.release();
+        else
+            pending = 1;
+    });
+    return status == HSBEM_OK && pending ? HSBEM_PENDING : status;
+}
+
+int hsbem_cancel(hsbem_server *s, uint64_t ticket) {
+    return guarded([&] {
+        require(s, "null server");
+        s->server->cancel(ticket);
+    });
+}
+
+int hsbem_server_stats(const hsbem_server *s, hsbem_server_statistics *out) {
+    return guarded([&] {
+        require(s && out, "invalid stats arguments");
+        const hsbem::ServerStats st = s->server->stats();
+        out->submitted = st.submitted;
+        out->completed = st.completed;
+        out->failed = st.failed;
+        out->points = st.points;
+        out->running = st.running;
+        out->queued = st.queued;
+        out->max_running = st.max_running;
+        out->budget_bytes = st.budget;
+        out->held_bytes = st.held_bytes;
+        out->peak_held_bytes = st.peak_held_bytes;
+    });
+}
+
+int hsbem_result_served_latency(const hsbem_result *r, uint32_t f, double *milliseconds) {
+    return guarded([&] {
+        require(r && milliseconds && f < r->result.frequencies.size(), "result index out of range");
+        *milliseconds = r->result.frequencies[f].served_ms;
     });
 }
 

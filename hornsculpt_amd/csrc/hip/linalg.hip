@@ -3,10 +3,10 @@
 // Replaces the vendor calls of the reference dense path
 // (cusolverDnCgetrf/Cgetrs, cuda_solver.cu:1485-1513; cublasZgemv/Zaxpy/Dznrm2
 // in device_gmres.cu:115-157) with own kernels:
-//   gemv            y = A X for up to 8 vectors: 256 rows x 128 columns per
-//                   workgroup, A streamed once (HBM bound), deterministic
-//                   split-K partials reduced in a second pass.
-//   project         CGS2 projections V^H w, one wavefront per basis column chunk.
+//   gemv            y = A X for up to 8 vectors (A FP64 or FP32, vectors FP64):
+//                   64 rows x 512 columns per workgroup, A streamed once (HBM
+//                   bound), deterministic split-K partials reduced in a second pass.
+//   project         CGS2 projections V^H w, one wavefront per (basis column, row block).
 //   lu_factor       right-looking blocked LU (panel 32): one-workgroup panel
 //                   factorisation with pivot search, row swaps, unit-lower
 //                   TRSM, and a 64x64-tiled trailing update.
@@ -18,62 +18,85 @@
 namespace hsbem::dev {
 namespace {
 
-constexpr int kGemvRows = 256;
-constexpr int kGemvCols = 128;
+constexpr int kGemvRows = 64;    // rows per workgroup (one per lane)
+constexpr int kGemvWaves = 4;    // waves per workgroup, splitting the column chunk
+constexpr int kGemvCols = 512;   // columns per workgroup (split-K chunk)
+constexpr int kGemvUnroll = 8;   // column loads in flight per lane
 constexpr int kMaxVec = 8;
 
+// Split-K partial products: workgroup (row block, column chunk); lane = row, the
+// four waves take interleaved columns of the chunk and are summed in LDS, so one
+// partial per (chunk, row) reaches HBM.
 template <int NV, class T>
-__global__ __launch_bounds__(kGemvRows) void gemv_partial(const T *__restrict__ A, std::uint32_t n,
-                                                           const double *__restrict__ x, std::uint32_t chunks,
-                                                           double *__restrict__ part) {
+__global__ __launch_bounds__(kGemvRows *kGemvWaves) void gemv_partial(const T *__restrict__ A, std::uint32_t n,
+                                                                     const double *__restrict__ x,
+                                                                     std::uint32_t chunks, double *__restrict__ part) {
     __shared__ double sx[NV][2 * kGemvCols];
-    const std::uint32_t row = blockIdx.x * kGemvRows + threadIdx.x;
+    __shared__ double red[kGemvWaves - 1][NV][2][kGemvRows];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
+    const std::uint32_t row = blockIdx.x * kGemvRows + lane;
     const std::uint32_t c0 = blockIdx.y * kGemvCols;
     const std::uint32_t cn = min(static_cast<std::uint32_t>(kGemvCols), n - c0);
-    for (std::uint32_t e = threadIdx.x; e < NV * 2 * kGemvCols; e += kGemvRows) {
+    for (std::uint32_t e = threadIdx.x; e < NV * 2 * kGemvCols; e += kGemvRows * kGemvWaves) {
         const std::uint32_t v = e / (2 * kGemvCols), c = e % (2 * kGemvCols);
         sx[v][c] = c / 2 < cn ? x[2 * (static_cast<std::size_t>(v) * n + c0) + c] : 0.0;
     }
     __syncthreads();
-    if (row >= n)
-        return;
     double re[NV], im[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v)
         re[v] = im[v] = 0.0;
-    const T *col = A + static_cast<std::size_t>(c0) * n + row;
-    std::uint32_t c = 0;
-    for (; c + 4 <= cn; c += 4) {
-        T a[4];
+    if (row < n) {
+        const T *col = A + static_cast<std::size_t>(c0) * n + row;
+        std::uint32_t c = wave;
+        for (; c + kGemvWaves * (kGemvUnroll - 1) < cn; c += kGemvWaves * kGemvUnroll) {
+            T a[kGemvUnroll];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            a[u] = col[static_cast<std::size_t>(c + u) * n];
+            for (int u = 0; u < kGemvUnroll; ++u)
+                a[u] = col[static_cast<std::size_t>(c + kGemvWaves * u) * n];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const double ar = a[u].x, ai = a[u].y;
+            for (int u = 0; u < kGemvUnroll; ++u) {
+                const double ar = a[u].x, ai = a[u].y;
+                const std::uint32_t cc = c + kGemvWaves * u;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    const double xr = sx[v][2 * cc], xi = sx[v][2 * cc + 1];
+                    re[v] += ar * xr - ai * xi;
+                    im[v] += ar * xi + ai * xr;
+                }
+            }
+        }
+        for (; c < cn; c += kGemvWaves) {
+            const T a = col[static_cast<std::size_t>(c) * n];
+            const double ar = a.x, ai = a.y;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                const double xr = sx[v][2 * (c + u)], xi = sx[v][2 * (c + u) + 1];
+                const double xr = sx[v][2 * c], xi = sx[v][2 * c + 1];
                 re[v] += ar * xr - ai * xi;
                 im[v] += ar * xi + ai * xr;
             }
         }
     }
-    for (; c < cn; ++c) {
-        const T a = col[static_cast<std::size_t>(c) * n];
+    if (wave > 0)
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            const double xr = sx[v][2 * c], xi = sx[v][2 * c + 1];
-            re[v] += a.x * xr - a.y * xi;
-            im[v] += a.x * xi + a.y * xr;
+            red[wave - 1][v][0][lane] = re[v];
+            red[wave - 1][v][1][lane] = im[v];
         }
-    }
+    __syncthreads();
+    if (wave == 0 && row < n)
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        const std::size_t o = (static_cast<std::size_t>(v) * chunks + blockIdx.y) * n + row;
-        part[2 * o] = re[v];
-        part[2 * o + 1] = im[v];
-    }
+        for (int v = 0; v < NV; ++v) {
+            double sr = re[v], si = im[v];
+#pragma unroll
+            for (int w = 0; w < kGemvWaves - 1; ++w) {
+                sr += red[w][v][0][lane];
+                si += red[w][v][1][lane];
+            }
+            const std::size_t o = (static_cast<std::size_t>(v) * chunks + blockIdx.y) * n + row;
+            part[2 * o] = sr;
+            part[2 * o + 1] = si;
+        }
 }
 
 __global__ void gemv_reduce(const double *part, std::uint32_t n, std::uint32_t chunks, std::uint32_t nvec, double *y) {
@@ -91,27 +114,29 @@ __global__ void gemv_reduce(const double *part, std::uint32_t n, std::uint32_t c
     y[2 * idx + 1] = im;
 }
 
-// Projection partials: block b reduces rows [b*1024, (b+1)*1024) for every column
-// j < m, one wavefront per column (strided), into part[b][j].
+// Projection partials: workgroup (b, g) reduces rows [b*kProjectRows, ...) for the
+// columns j = 4g + wave, one wavefront per column, into part[b][j].
+constexpr int kProjectRows = 512;
 __global__ __launch_bounds__(256) void project_partial(const double *V, std::uint32_t n, std::uint32_t m,
                                                       const double *w, double *part) {
-    constexpr int kRows = 1024;
+    constexpr int kRows = kProjectRows;
     const std::uint32_t r0 = blockIdx.x * kRows;
     const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
-    for (std::uint32_t j = wave; j < m; j += 4) {
-        double re = 0, im = 0;
-        for (std::uint32_t r = r0 + lane; r < min(r0 + kRows, n); r += 64) {
-            const double vr = V[2 * (static_cast<std::size_t>(j) * n + r)], vi = V[2 * (static_cast<std::size_t>(j) * n + r) + 1];
-            const double xr = w[2 * r], xi = w[2 * r + 1];
-            re += vr * xr + vi * xi; // conj(v) * x
-            im += vr * xi - vi * xr;
-        }
-        re = wave_sum(re);
-        im = wave_sum(im);
-        if (lane == 0) {
-            part[2 * (static_cast<std::size_t>(blockIdx.x) * m + j)] = re;
-            part[2 * (static_cast<std::size_t>(blockIdx.x) * m + j) + 1] = im;
-        }
+    const std::uint32_t j = blockIdx.y * 4 + wave;
+    if (j >= m)
+        return;
+    double re = 0, im = 0;
+    for (std::uint32_t r = r0 + lane; r < min(r0 + kRows, n); r += 64) {
+        const double vr = V[2 * (static_cast<std::size_t>(j) * n + r)], vi = V[2 * (static_cast<std::size_t>(j) * n + r) + 1];
+        const double xr = w[2 * r], xi = w[2 * r + 1];
+        re += vr * xr + vi * xi; // conj(v) * x
+        im += vr * xi - vi * xr;
+    }
+    re = wave_sum(re);
+    im = wave_sum(im);
+    if (lane == 0) {
+        part[2 * (static_cast<std::size_t>(blockIdx.x) * m + j)] = re;
+        part[2 * (static_cast<std::size_t>(blockIdx.x) * m + j) + 1] = im;
     }
 }
 
@@ -487,10 +512,10 @@ void launch_gemv(const void *A, bool fp32, std::uint32_t n, const double *x, dou
 #define HS_GEMV_CASE(N)                                                                                          \
     case N:                                                                                                      \
         if (fp32)                                                                                                \
-            hipLaunchKernelGGL((gemv_partial<N, float2>), grid, dim3(kGemvRows), 0, stream,                      \
+            hipLaunchKernelGGL((gemv_partial<N, float2>), grid, dim3(kGemvRows * kGemvWaves), 0, stream,                      \
                                static_cast<const float2 *>(A), n, xs, chunks, scratch);                          \
         else                                                                                                     \
-            hipLaunchKernelGGL((gemv_partial<N, double2>), grid, dim3(kGemvRows), 0, stream,                     \
+            hipLaunchKernelGGL((gemv_partial<N, double2>), grid, dim3(kGemvRows * kGemvWaves), 0, stream,                     \
                                static_cast<const double2 *>(A), n, xs, chunks, scratch);                         \
         break;
             HS_GEMV_CASE(1)
@@ -510,14 +535,14 @@ void launch_gemv(const void *A, bool fp32, std::uint32_t n, const double *x, dou
 }
 
 std::size_t reduction_scratch_doubles(std::uint32_t n, std::uint32_t m) {
-    const std::size_t blocks = (n + 1023) / 1024;
+    const std::size_t blocks = (n + kProjectRows - 1) / kProjectRows;
     return 2 * blocks * std::max<std::uint32_t>(m, 1) + 1024;
 }
 
 void launch_project(const double *V, std::uint32_t n, std::uint32_t m, const double *w, double *out, double *scratch,
                     hipStream_t stream) {
-    const std::uint32_t blocks = (n + 1023) / 1024;
-    hipLaunchKernelGGL(project_partial, dim3(blocks), dim3(256), 0, stream, V, n, m, w, scratch);
+    const std::uint32_t blocks = (n + kProjectRows - 1) / kProjectRows;
+    hipLaunchKernelGGL(project_partial, dim3(blocks, (m + 3) / 4), dim3(256), 0, stream, V, n, m, w, scratch);
     hipLaunchKernelGGL(sum_columns, dim3((m + 63) / 64), dim3(64), 0, stream, scratch, blocks, m, out);
 }
 

@@ -107,10 +107,47 @@ MeshSolver::MeshSolver(Context *ctx, SurfaceMesh mesh) : ctx_(ctx) {
     space_ = build_space(mesh);
     mesh_ = std::move(mesh);
     build_clusters();
-    error_.resize(1);
-    samples_.resize(1);
     rules_.upload(rule_table(), ctx_->stream);
     hip_check(hipStreamSynchronize(ctx_->stream), "mesh upload");
+    workspace_ = std::make_unique<Workspace>(ctx_->stream);
+}
+
+Workspace::Workspace(hipStream_t s, bool owns) : stream(s), owns_stream(owns) {
+    error.resize(1);
+    samples.resize(1);
+    for (auto &e : events)
+        hip_check(hipEventCreate(&e), "event create");
+}
+
+Workspace::~Workspace() {
+    for (auto &e : events)
+        if (e)
+            (void)hipEventDestroy(e);
+    if (owns_stream && stream) {
+        (void)hipStreamSynchronize(stream);
+        release();
+        (void)hipStreamDestroy(stream);
+    }
+}
+
+void Workspace::release() {
+    for (auto *b : {&A, &LU, &BD, &q, &rhs, &x, &r, &flux, &scratch, &obs, &field, &partials})
+        b->release();
+    flux_column.release();
+    pivots.release();
+    gmres.basis.release();
+    gmres.r.release();
+    gmres.scratch.release();
+    gmres.small.release();
+    bound = 0;
+}
+
+std::uint64_t Workspace::capacity_bytes() const {
+    std::uint64_t total = 0;
+    for (const auto *b : {&A, &LU, &BD, &q, &rhs, &x, &r, &flux, &scratch, &obs, &field, &partials})
+        total += b->capacity_bytes();
+    return total + flux_column.capacity_bytes() + pivots.capacity_bytes() + gmres.basis.capacity_bytes() +
+           gmres.r.capacity_bytes() + gmres.scratch.capacity_bytes() + gmres.small.capacity_bytes();
 }
 
 void MeshSolver::build_clusters() {
@@ -220,19 +257,20 @@ InteractionPlan MeshSolver::plan(const Numerics &n, double k_max) {
     return make_interaction_plan(space_, q, k_max, n.plan_threads);
 }
 
-DevicePlan &MeshSolver::device_plan(const Numerics &n, double k_max, bool *built) {
+std::shared_ptr<DevicePlan> MeshSolver::device_plan(const Numerics &n, double k_max, hipStream_t s, bool *built) {
     const auto key = std::make_tuple(n.quadrature_order, n.singular_order, n.near_ratio, n.near_tolerance, n.near_depth,
                                      k_max);
+    // Held while building: other streams needing the same mesh's plan wait for it.
+    std::lock_guard lock(plan_mutex_);
     auto it = plans_.find(key);
     if (it != plans_.end()) {
         if (built)
             *built = false;
-        return *it->second;
+        return it->second;
     }
     const auto t0 = Clock::now();
-    auto dp = std::make_unique<DevicePlan>();
+    auto dp = std::make_shared<DevicePlan>();
     dp->plan = plan(n, k_max);
-    hipStream_t s = ctx_->stream;
     dp->touching.upload(dp->plan.touching, s);
     dp->near.upload(dp->plan.near, s);
     dp->patches.upload(dp->plan.patches, s);
@@ -261,13 +299,12 @@ DevicePlan &MeshSolver::device_plan(const Numerics &n, double k_max, bool *built
     dp->plan_ms = ms_since(t0);
     if (built)
         *built = true;
-    auto &ref = *dp;
-    plans_.emplace(key, std::move(dp));
-    return ref;
+    plans_.emplace(key, dp);
+    return dp;
 }
 
 dev::AssemblyLaunch MeshSolver::assembly_launch(const DevicePlan &p, const Numerics &n, double k, double *A, double *BD,
-                                                std::uint32_t nd, const std::int32_t *flux_column) {
+                                                std::uint32_t nd, const std::int32_t *flux_column, int *error) {
     dev::AssemblyLaunch L{};
     L.args.panels = {geo_.data(), nodes_.data(), bounds_.data(), shape_.data(), nt(), nv()};
     L.args.row_scale = row_scale_.data();
@@ -284,7 +321,7 @@ dev::AssemblyLaunch MeshSolver::assembly_launch(const DevicePlan &p, const Numer
     L.args.k = k;
     L.args.tolerance = n.near_tolerance;
     L.args.regular_order = n.quadrature_order;
-    L.args.error = error_.data();
+    L.args.error = error;
     L.clusters = {cluster_panel_offsets_.data(), cluster_panels_.data(), cluster_node_offsets_.data(),
                   cluster_nodes_.data(), local_.data(), cluster_count_};
     L.near = {p.near_offsets.data(), p.near_trials.data(), p.whole.size() ? p.whole.data() : nullptr};
@@ -301,9 +338,11 @@ dev::AssemblyLaunch MeshSolver::assembly_launch(const DevicePlan &p, const Numer
 
 void MeshSolver::assemble_dense(const Numerics &n, double k_max, double k, double *A_out, double *B_out) {
     std::lock_guard lock(mutex_);
-    DevicePlan &p = device_plan(n, k_max, nullptr);
-    const std::uint32_t nv = this->nv(), nt = this->nt();
     hipStream_t s = ctx_->stream;
+    const std::shared_ptr<DevicePlan> plan = device_plan(n, k_max, s, nullptr);
+    const DevicePlan &p = *plan;
+    const std::uint32_t nv = this->nv(), nt = this->nt();
+    DeviceBuffer<int> error(1);
     const bool fp32 = n.precision == 1;
     DeviceBuffer<double> A((fp32 ? 1 : 2) * static_cast<std::size_t>(nv) * nv), BD(2 * static_cast<std::size_t>(nv) * nt);
     std::vector<std::int32_t> cols(nt);
@@ -312,10 +351,10 @@ void MeshSolver::assemble_dense(const Numerics &n, double k_max, double k, doubl
     fc.upload(cols, s);
     hip_check(hipMemsetAsync(A.data(), 0, A.bytes(), s), "clear A");
     hip_check(hipMemsetAsync(BD.data(), 0, BD.bytes(), s), "clear B");
-    hip_check(hipMemsetAsync(error_.data(), 0, sizeof(int), s), "clear error");
-    dev::launch_assembly(assembly_launch(p, n, k, A.data(), BD.data(), nt, fc.data()), s);
+    hip_check(hipMemsetAsync(error.data(), 0, sizeof(int), s), "clear error");
+    dev::launch_assembly(assembly_launch(p, n, k, A.data(), BD.data(), nt, fc.data(), error.data()), s);
     hip_check(hipGetLastError(), "assembly launch");
-    check_error_flag(error_, s);
+    check_error_flag(error, s);
     std::vector<double> a(2 * static_cast<std::size_t>(nv) * nv), b(BD.size());
     if (fp32) {
         std::vector<float> a32(2 * static_cast<std::size_t>(nv) * nv);
@@ -468,35 +507,32 @@ void dense_gmres(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const
     throw std::runtime_error(msg.str());
 }
 
-Result MeshSolver::solve(const Request &req, ProgressFn progress, void *user, const volatile std::int32_t *cancel) {
-    std::lock_guard lock(mutex_);
+std::shared_ptr<PreparedRequest> MeshSolver::prepare(const Request &req) const {
     using C = std::complex<double>;
     const Numerics &num = req.numerics;
-    auto cancelled = [&] {
-        if (cancel && *cancel)
-            throw Cancelled();
-    };
-    cancelled();
     if (!std::isfinite(req.density) || req.density <= 0 || !std::isfinite(req.sound_speed) || req.sound_speed <= 0 ||
         req.frequencies.empty() || req.excitations.empty() ||
         (num.quadrature_order != 2 && num.quadrature_order != 4) || num.singular_order < 2 || num.singular_order > 20 ||
         !std::isfinite(num.near_ratio) || num.near_ratio < 0 || num.near_ratio > 10 ||
         !std::isfinite(num.near_tolerance) || num.near_tolerance < 1e-10 || num.near_tolerance > .01 ||
         num.near_depth > 10 || !std::isfinite(num.residual_tolerance) || num.residual_tolerance < 1e-12 ||
-        num.residual_tolerance > .01 || num.refinement_steps > 10 || num.solver > 1 || num.precision > 1 || num.gmres_restart < 2 ||
-        num.gmres_restart > 300 || num.gmres_max_iterations < 1)
+        num.residual_tolerance > .01 || num.refinement_steps > 10 || num.solver > 1 || num.precision > 1 ||
+        num.gmres_restart < 2 || num.gmres_restart > 300 || num.gmres_max_iterations < 1)
         throw std::invalid_argument("Invalid BEM medium, frequencies, excitation or Galerkin settings.");
     if (req.excitations.size() > 1024 || req.observers.size() > 100000)
         throw std::invalid_argument("BEM request exceeds indexing limits.");
-    const double k_max = planning_wavenumber(req.frequencies, req.planning_frequencies, req.sound_speed);
+    static std::atomic<std::uint64_t> next_id{1};
+    auto P = std::make_shared<PreparedRequest>();
+    P->id = next_id++;
+    P->request = req;
+    P->k_max = planning_wavenumber(req.frequencies, req.planning_frequencies, req.sound_speed);
     // observers (recentred, or unit directions about the far-field origin)
-    std::vector<Vec3> observers;
-    Vec3 far_origin{};
     if (req.far_field) {
-        far_origin = vsub(req.far_origin, origin_);
-        if (!std::isfinite(far_origin.x) || !std::isfinite(far_origin.y) || !std::isfinite(far_origin.z) ||
+        P->far_origin = vsub(req.far_origin, origin_);
+        const Vec3 &o = P->far_origin;
+        if (!std::isfinite(o.x) || !std::isfinite(o.y) || !std::isfinite(o.z) ||
             !std::isfinite(req.far_reference_distance) || req.far_reference_distance <= 0 ||
-            !std::isfinite(k_max * req.far_reference_distance))
+            !std::isfinite(P->k_max * req.far_reference_distance))
             throw std::invalid_argument("Far-field origin and positive reference distance must be finite.");
     }
     for (const auto &w : req.observers) {
@@ -512,7 +548,7 @@ Result MeshSolver::solve(const Request &req, ProgressFn progress, void *user, co
         } else if (point_inside(space_.panels, p)) {
             throw std::invalid_argument("BEM observation points must be finite and outside the solid.");
         }
-        observers.push_back(p);
+        P->observers.push_back(p);
     }
     // boundary conditions per face and panel velocities per excitation
     std::map<std::uint32_t, Boundary> conditions;
@@ -523,360 +559,405 @@ Result MeshSolver::solve(const Request &req, ProgressFn progress, void *user, co
     }
     const std::uint32_t nt = this->nt(), nv = this->nv(), ne = static_cast<std::uint32_t>(req.excitations.size());
     std::set<std::uint32_t> active_faces;
-    std::vector<std::map<std::uint32_t, C>> face_velocity(ne);
+    P->face_velocity.resize(ne);
     for (std::uint32_t e = 0; e < ne; ++e) {
         for (const auto &[face, v] : req.excitations[e].faces) {
             if (!conditions.count(face) || conditions.at(face) != Boundary::driven || !std::isfinite(v.real()) ||
-                !std::isfinite(v.imag()) || !face_velocity[e].emplace(face, v).second)
+                !std::isfinite(v.imag()) || !P->face_velocity[e].emplace(face, v).second)
                 throw std::invalid_argument("Excitation must assign each known Driven face at most once.");
             active_faces.insert(face);
+            P->has_nonzero |= v != C(0);
         }
-        if (face_velocity[e].empty())
+        if (P->face_velocity[e].empty())
             throw std::invalid_argument("Each excitation needs at least one assigned Driven face.");
     }
-    std::vector<std::int32_t> flux_column(nt, -1);
-    std::vector<std::uint32_t> flux_panels;
+    P->flux_column.assign(nt, -1);
     for (std::uint32_t t = 0; t < nt; ++t)
-        if (active_faces.count(mesh_.triangles[t].face_id)) {
-            flux_column[t] = static_cast<std::int32_t>(flux_panels.size());
-            flux_panels.push_back(t);
-        }
-    const std::uint32_t nd = static_cast<std::uint32_t>(flux_panels.size());
-    hipStream_t s = ctx_->stream;
-    bool built = false;
-    DevicePlan &plan = device_plan(num, k_max, &built);
-    cancelled();
-    // workspace
-    const std::size_t nv2 = static_cast<std::size_t>(nv) * nv;
-    // Mixed form: FP32 matrix (half the bytes) with FP64 Krylov vectors; always GMRES.
+        if (active_faces.count(mesh_.triangles[t].face_id))
+            P->flux_column[t] = static_cast<std::int32_t>(P->nd++);
+    P->ne = ne;
+    P->nv = nv;
+    P->nt = nt;
+    // device bytes of one frequency point (buffers of bind() and dense_gmres())
     const bool mixed = num.precision == 1;
-    const unsigned solver_kind = mixed ? 1u : num.solver;
-    A_.resize((mixed ? 1 : 2) * nv2);
-    BD_.resize(2 * static_cast<std::size_t>(nv) * std::max<std::uint32_t>(nd, 1));
-    q_.resize(2 * static_cast<std::size_t>(std::max<std::uint32_t>(nd, 1)) * ne);
-    rhs_.resize(2 * static_cast<std::size_t>(nv) * ne);
-    x_.resize(2 * static_cast<std::size_t>(nv) * ne);
-    r_.resize(2 * static_cast<std::size_t>(nv) * ne);
-    flux_.resize(2 * static_cast<std::size_t>(nt) * ne);
-    flux_column_.upload(flux_column, s);
+    const bool lu = !mixed && num.solver == 0;
+    const std::uint64_t nv2 = static_cast<std::uint64_t>(nv) * nv, nd1 = std::max<std::uint32_t>(P->nd, 1);
+    const std::uint64_t nobs = P->observers.size();
+    const std::uint64_t scratch = std::max({dev::gemv_scratch_doubles(nv, std::min<std::uint32_t>(ne, 8)),
+                                            dev::reduction_scratch_doubles(nv, num.gmres_restart + 1), std::size_t{4096}});
+    std::uint64_t bytes = (mixed ? 8 : 16) * nv2 + 16 * nv * nd1 + 16 * nd1 * ne + 3 * 16ull * nv * ne + 16ull * nt * ne +
+                          8 * scratch + 4ull * nt;
+    if (lu)
+        bytes += 16 * nv2 + 4ull * nv;
+    else
+        bytes += 16ull * nv * (num.gmres_restart + 1) + 16ull * nv + 8 * scratch + 8ull * (4 * (num.gmres_restart + 2) + 8);
+    if (nobs)
+        bytes += 24 * nobs + 16 * nobs * ne + 8 * dev::field_partial_doubles(nt, static_cast<std::uint32_t>(nobs), ne);
+    P->footprint = bytes;
+    return P;
+}
+
+void MeshSolver::bind(const PreparedRequest &P, Workspace &ws) {
+    const Numerics &num = P.request.numerics;
+    const std::uint32_t nt = this->nt(), nv = this->nv(), ne = P.ne;
+    const std::size_t nv2 = static_cast<std::size_t>(nv) * nv;
+    const bool mixed = num.precision == 1;
+    ws.A.resize((mixed ? 1 : 2) * nv2);
+    ws.BD.resize(2 * static_cast<std::size_t>(nv) * std::max<std::uint32_t>(P.nd, 1));
+    ws.q.resize(2 * static_cast<std::size_t>(std::max<std::uint32_t>(P.nd, 1)) * ne);
+    ws.rhs.resize(2 * static_cast<std::size_t>(nv) * ne);
+    ws.x.resize(2 * static_cast<std::size_t>(nv) * ne);
+    ws.r.resize(2 * static_cast<std::size_t>(nv) * ne);
+    ws.flux.resize(2 * static_cast<std::size_t>(nt) * ne);
+    ws.flux_column.upload(P.flux_column, ws.stream);
     const std::size_t scratch = std::max({dev::gemv_scratch_doubles(nv, std::min<std::uint32_t>(ne, 8)),
                                           dev::reduction_scratch_doubles(nv, num.gmres_restart + 1), std::size_t{4096}});
-    if (scratch_.size() < scratch)
-        scratch_.resize(scratch);
-    if (solver_kind == 0) {
-        LU_.resize(2 * nv2);
-        pivots_.resize(nv);
+    ws.scratch.resize(scratch);
+    if (!mixed && num.solver == 0) {
+        ws.LU.resize(2 * nv2);
+        ws.pivots.resize(nv);
     }
-    if (!observers.empty()) {
+    if (!P.observers.empty()) {
         std::vector<double> o;
-        for (const auto &p : observers) {
+        for (const auto &p : P.observers) {
             o.push_back(p.x);
             o.push_back(p.y);
             o.push_back(p.z);
         }
-        obs_.upload(o, s);
-        field_.resize(2 * observers.size() * ne);
-        partials_.resize(dev::field_partial_doubles(nt, static_cast<std::uint32_t>(observers.size()), ne));
+        ws.obs.upload(o, ws.stream);
+        ws.field.resize(2 * P.observers.size() * ne);
+        ws.partials.resize(dev::field_partial_doubles(nt, static_cast<std::uint32_t>(P.observers.size()), ne));
     }
+    hip_check(hipStreamSynchronize(ws.stream), "request upload");
+    ws.bound = P.id;
+}
+
+Result MeshSolver::solve(const Request &req, ProgressFn progress, void *user, const volatile std::int32_t *cancel) {
+    std::lock_guard lock(mutex_);
+    if (cancel && *cancel)
+        throw Cancelled();
+    const auto P = prepare(req);
+    Workspace &ws = *workspace_;
     Result result;
-    result.nt = nt;
-    result.nv = nv;
-    result.nobs = static_cast<std::uint32_t>(observers.size());
-    result.nexc = ne;
-    result.peak_bytes = A_.bytes() + LU_.bytes() + BD_.bytes() + rhs_.bytes() + x_.bytes() + r_.bytes() +
-                        flux_.bytes() + scratch_.bytes() + gmres_.basis.bytes() + geo_.bytes() + plan.patches.bytes();
-    bool has_nonzero = false;
-    for (const auto &fv : face_velocity)
-        for (const auto &[f, v] : fv)
-            has_nonzero |= v != C(0);
-    hipEvent_t ev[4];
-    for (auto &e : ev)
-        hip_check(hipEventCreate(&e), "event create");
-    struct EventGuard {
-        hipEvent_t *e;
-        ~EventGuard() {
-            for (int i = 0; i < 4; ++i)
-                (void)hipEventDestroy(e[i]);
-        }
-    } guard{ev};
+    result.nt = nt();
+    result.nv = nv();
+    result.nobs = static_cast<std::uint32_t>(P->observers.size());
+    result.nexc = P->ne;
     std::uint32_t completed = 0;
-    for (double f : req.frequencies) {
-        cancelled();
+    const auto total = static_cast<std::uint32_t>(req.frequencies.size());
+    for (std::size_t i = 0; i < req.frequencies.size(); ++i) {
+        const double f = req.frequencies[i];
         auto report = [&](std::int32_t phase) {
             if (progress)
-                progress(user, completed, static_cast<std::uint32_t>(req.frequencies.size()), f, phase);
+                progress(user, completed, total, f, phase);
         };
-        report(0);
-        FrequencyResult fr;
-        fr.frequency = f;
-        if (completed == 0)
-            fr.ms[5] = built ? plan.plan_ms : 0;
-        const double omega = 2 * std::numbers::pi * f, k = omega / req.sound_speed;
-        double max_kh = 0;
-        for (const auto &p : space_.panels)
-            max_kh = std::max(max_kh, k * p.max_edge);
-        // fluxes: q = -i rho omega v, per flux column and per panel
-        std::vector<double> qd(2 * static_cast<std::size_t>(std::max<std::uint32_t>(nd, 1)) * ne, 0.0),
-            qpanel(2 * static_cast<std::size_t>(nt) * ne, 0.0);
-        for (std::uint32_t e = 0; e < ne; ++e)
-            for (std::uint32_t t = 0; t < nt; ++t) {
-                const auto it = face_velocity[e].find(mesh_.triangles[t].face_id);
-                if (it == face_velocity[e].end())
-                    continue;
-                const C q = C(0, -req.density * omega) * it->second;
-                qpanel[2 * (static_cast<std::size_t>(e) * nt + t)] = q.real();
-                qpanel[2 * (static_cast<std::size_t>(e) * nt + t) + 1] = q.imag();
-                if (flux_column[t] >= 0) {
-                    qd[2 * (static_cast<std::size_t>(e) * nd + flux_column[t])] = q.real();
-                    qd[2 * (static_cast<std::size_t>(e) * nd + flux_column[t]) + 1] = q.imag();
-                }
-            }
-        hip_check(hipMemcpyAsync(q_.data(), qd.data(), sizeof(double) * qd.size(), hipMemcpyHostToDevice, s), "q upload");
-        hip_check(hipMemcpyAsync(flux_.data(), qpanel.data(), sizeof(double) * qpanel.size(), hipMemcpyHostToDevice, s),
-                  "flux upload");
-        const auto t_asm = Clock::now();
-        hip_check(hipMemsetAsync(A_.data(), 0, A_.bytes(), s), "clear A");
-        hip_check(hipMemsetAsync(BD_.data(), 0, BD_.bytes(), s), "clear B");
-        hip_check(hipMemsetAsync(error_.data(), 0, sizeof(int), s), "clear error");
-        auto L = assembly_launch(plan, num, k, A_.data(), BD_.data(), nd, flux_column_.data());
-        hip_check(hipMemsetAsync(samples_.data(), 0, sizeof(unsigned long long), s), "clear samples");
-        L.args.samples = samples_.data();
-        hip_check(hipEventRecord(ev[0], s), "event");
-        dev::launch_regular(L, s);
-        hip_check(hipEventRecord(ev[1], s), "event");
-        dev::launch_touching(L, s);
-        hip_check(hipEventRecord(ev[2], s), "event");
-        dev::launch_near(L, s);
-        hip_check(hipEventRecord(ev[3], s), "event");
-        dev::launch_mass(L, s);
-        hip_check(hipGetLastError(), "assembly launch");
-        if (nd)
-            dev::launch_flux_rhs(BD_.data(), nv, nd, q_.data(), ne, rhs_.data(), s);
-        else
-            hip_check(hipMemsetAsync(rhs_.data(), 0, rhs_.bytes(), s), "clear rhs");
-        check_error_flag(error_, s);
-        float e01 = 0, e12 = 0, e23 = 0;
-        hip_check(hipEventElapsedTime(&e01, ev[0], ev[1]), "event time");
-        hip_check(hipEventElapsedTime(&e12, ev[1], ev[2]), "event time");
-        hip_check(hipEventElapsedTime(&e23, ev[2], ev[3]), "event time");
-        unsigned long long regular_samples = 0;
-        hip_check(hipMemcpyAsync(&regular_samples, samples_.data(), sizeof(regular_samples), hipMemcpyDeviceToHost, s),
-                  "samples download");
-        hip_check(hipStreamSynchronize(s), "samples sync");
-        fr.regular_samples = static_cast<double>(regular_samples);
-        fr.touching_samples = plan.touching_samples;
-        fr.near_samples = plan.near_samples;
-        fr.regular_kernel_ms = e01;
-        fr.touching_kernel_ms = e12;
-        fr.near_kernel_ms = e23;
-        fr.ms[0] = ms_since(t_asm);
-        cancelled();
-        // mass-weighted rhs norms (for the unscaled relative residual)
-        std::vector<double> rhs_norm(ne);
-        for (std::uint32_t e = 0; e < ne; ++e) {
-            dev::launch_norm2(rhs_.data() + 2 * static_cast<std::size_t>(e) * nv, nv, lumped_.data(), scratch_.data() + scratch - 8,
-                              scratch_.data(), s);
-            double v = 0;
-            hip_check(hipMemcpyAsync(&v, scratch_.data() + scratch - 8, sizeof(double), hipMemcpyDeviceToHost, s), "norm");
-            hip_check(hipStreamSynchronize(s), "norm sync");
-            rhs_norm[e] = std::sqrt(v);
-            if (!std::isfinite(rhs_norm[e]))
-                throw std::runtime_error("Galerkin right-hand side is not finite.");
-        }
-        report(3);
-        const auto t_solve = Clock::now();
-        std::vector<std::uint32_t> iterations(ne, 0);
-        std::uint32_t refinements = 0;
-        if (solver_kind == 0) {
-            hip_check(hipMemcpyAsync(LU_.data(), A_.data(), A_.bytes(), hipMemcpyDeviceToDevice, s), "LU copy");
-            hip_check(hipMemsetAsync(error_.data(), 0, sizeof(int), s), "clear info");
-            const auto t_f = Clock::now();
-            dev::lu_factor(LU_.data(), nv, pivots_.data(), error_.data(), nullptr, s);
-            int info = 0;
-            hip_check(hipMemcpyAsync(&info, error_.data(), sizeof(int), hipMemcpyDeviceToHost, s), "info");
-            hip_check(hipStreamSynchronize(s), "LU sync");
-            fr.ms[2] = ms_since(t_f);
-            if (info)
-                throw std::runtime_error("Burton-Miller factorization failed at pivot " + std::to_string(info));
-            const auto t_t = Clock::now();
-            hip_check(hipMemcpyAsync(x_.data(), rhs_.data(), rhs_.bytes(), hipMemcpyDeviceToDevice, s), "rhs copy");
-            dev::lu_solve(LU_.data(), nv, pivots_.data(), x_.data(), ne, s);
-            hip_check(hipStreamSynchronize(s), "solve sync");
-            fr.ms[3] = ms_since(t_t);
-        } else {
-            for (std::uint32_t e = 0; e < ne; ++e) {
-                double achieved = 0;
-                dense_gmres(s, A_.data(), mixed, nv, rhs_.data() + 2 * static_cast<std::size_t>(e) * nv,
-                            x_.data() + 2 * static_cast<std::size_t>(e) * nv, num.gmres_restart,
-                            num.gmres_max_iterations, num.residual_tolerance * 0.1, &achieved, &iterations[e], cancel,
-                            gmres_);
-            }
-        }
-        // FP64 residual of the unscaled equation; LU corrections while above tolerance
-        std::vector<double> relative(ne);
-        for (;;) {
-            dev::launch_residual(A_.data(), mixed, nv, rhs_.data(), x_.data(), ne, r_.data(), scratch_.data(), s);
-            double largest = 0;
-            for (std::uint32_t e = 0; e < ne; ++e) {
-                dev::launch_norm2(r_.data() + 2 * static_cast<std::size_t>(e) * nv, nv, lumped_.data(),
-                                  scratch_.data() + scratch - 8, scratch_.data(), s);
-                double v = 0;
-                hip_check(hipMemcpyAsync(&v, scratch_.data() + scratch - 8, sizeof(double), hipMemcpyDeviceToHost, s),
-                          "residual norm");
-                hip_check(hipStreamSynchronize(s), "residual sync");
-                relative[e] = rhs_norm[e] > 0 ? std::sqrt(v) / rhs_norm[e]
-                                               : (v == 0 ? 0 : std::numeric_limits<double>::infinity());
-                if (!std::isfinite(relative[e]))
-                    throw std::runtime_error("Burton-Miller residual is not finite.");
-                largest = std::max(largest, relative[e]);
-            }
-            if (largest <= num.residual_tolerance)
-                break;
-            if (solver_kind != 0 || refinements >= num.refinement_steps) {
-                std::ostringstream msg;
-                msg.precision(9);
-                msg << "Burton-Miller residual " << largest << " exceeds tolerance " << num.residual_tolerance << " at "
-                    << f << " Hz after " << refinements << " refinement steps.";
-                throw std::runtime_error(msg.str());
-            }
-            dev::lu_solve(LU_.data(), nv, pivots_.data(), r_.data(), ne, s);
-            for (std::uint32_t e = 0; e < ne; ++e) {
-                const double one[2] = {1, 0};
-                DeviceBuffer<double> alpha;
-                alpha.upload(std::vector<double>(one, one + 2), s);
-                dev::launch_axpy(nv, alpha.data(), r_.data() + 2 * static_cast<std::size_t>(e) * nv,
-                                 x_.data() + 2 * static_cast<std::size_t>(e) * nv, s);
-                hip_check(hipStreamSynchronize(s), "refinement sync");
-            }
-            ++refinements;
-        }
-        fr.ms[1] = ms_since(t_solve);
-        // fields
-        report(5);
-        const auto t_field = Clock::now();
-        std::vector<double> observed;
-        if (!observers.empty() && has_nonzero) {
-            auto evaluate = [&](unsigned degree) {
-                cancelled();
-                const auto rule = triangle_rule(degree);
-                std::vector<double> rt;
-                for (const auto &p : rule) {
-                    rt.push_back(p.bary[0]);
-                    rt.push_back(p.bary[1]);
-                    rt.push_back(p.bary[2]);
-                    rt.push_back(p.weight);
-                }
-                DeviceBuffer<double> drule;
-                drule.upload(rt, s);
-                dev::FieldArgs fa{};
-                fa.panels = {geo_.data(), nodes_.data(), bounds_.data(), shape_.data(), nt, nv};
-                fa.pressure = x_.data();
-                fa.flux = flux_.data();
-                fa.nrhs = ne;
-                fa.points = obs_.data();
-                fa.count = static_cast<std::uint32_t>(observers.size());
-                fa.rule = drule.data();
-                fa.rule_count = static_cast<std::uint32_t>(rule.size());
-                fa.k = k;
-                fa.near_tolerance = std::min(num.near_tolerance, .025 * num.residual_tolerance);
-                fa.max_depth = num.near_depth;
-                fa.origin[0] = far_origin.x;
-                fa.origin[1] = far_origin.y;
-                fa.origin[2] = far_origin.z;
-                fa.reference_distance = req.far_reference_distance;
-                fa.out = field_.data();
-                fa.error = error_.data();
-                hip_check(hipMemsetAsync(error_.data(), 0, sizeof(int), s), "clear error");
-                if (req.far_field)
-                    dev::launch_far_field(fa, partials_.data(), s);
-                else
-                    dev::launch_observer_field(fa, partials_.data(), s);
-                hip_check(hipGetLastError(), "field launch");
-                check_error_flag(error_, s);
-                std::vector<double> out(field_.size());
-                hip_check(hipMemcpyAsync(out.data(), field_.data(), field_.bytes(), hipMemcpyDeviceToHost, s),
-                          "field download");
-                hip_check(hipStreamSynchronize(s), "field sync");
-                return out;
-            };
-            unsigned degree = minimum_endpoint_degree(max_kh, num.quadrature_order);
-            observed = evaluate(degree);
-            for (;;) {
-                const unsigned next = refined_endpoint_degree(degree);
-                auto refined = evaluate(next);
-                bool passed = true;
-                for (std::uint32_t e = 0; e < ne; ++e) {
-                    double diff = 0, norm = 0;
-                    for (std::size_t j = 0; j < observers.size(); ++j) {
-                        const std::size_t i = 2 * (e * observers.size() + j);
-                        const C hi(refined[i], refined[i + 1]), lo(observed[i], observed[i + 1]);
-                        diff += std::norm(hi - lo);
-                        norm += std::norm(hi);
-                    }
-                    const double ind = norm > 0 ? std::sqrt(diff / norm) : diff == 0 ? 0 : INFINITY;
-                    passed = passed && ind <= .025 * num.residual_tolerance;
-                }
-                observed = std::move(refined);
-                if (passed)
-                    break;
-                if (degree == 12)
-                    throw std::runtime_error("BEM complex observer-field quadrature 12/14 did not converge within "
-                                             "0.025*residualTolerance; refine the mesh or observer separation.");
-                degree = next;
-            }
-        } else if (!observers.empty()) {
-            observed.assign(2 * observers.size() * ne, 0.0);
-        }
-        // nodal pressures to host
-        std::vector<double> nodal(2 * static_cast<std::size_t>(nv) * ne);
-        hip_check(hipMemcpyAsync(nodal.data(), x_.data(), sizeof(double) * nodal.size(), hipMemcpyDeviceToHost, s),
-                  "pressure download");
-        hip_check(hipStreamSynchronize(s), "pressure sync");
-        fr.ms[4] = ms_since(t_field);
-        for (std::uint32_t e = 0; e < ne; ++e) {
-            ExcitationResult er;
-            er.residual = relative[e];
-            er.iterations = iterations[e];
-            er.refinements = refinements;
-            er.pressure_trace.resize(nv);
-            for (std::uint32_t i = 0; i < nv; ++i)
-                er.pressure_trace[vertex_of_[i]] = C(nodal[2 * (static_cast<std::size_t>(e) * nv + i)],
-                                                     nodal[2 * (static_cast<std::size_t>(e) * nv + i) + 1]);
-            er.surface_pressure.resize(nt);
-            er.normal_derivative.resize(nt);
-            for (std::uint32_t t = 0; t < nt; ++t) {
-                C avg = 0;
-                for (auto v : space_.panels[t].nodes)
-                    avg += er.pressure_trace[v] / 3.0;
-                if (!std::isfinite(avg.real()) || !std::isfinite(avg.imag()))
-                    throw std::runtime_error("Burton-Miller pressure is not finite.");
-                er.surface_pressure[t] = avg;
-                const auto it = face_velocity[e].find(mesh_.triangles[t].face_id);
-                er.normal_derivative[t] = it == face_velocity[e].end() ? C{} : C(0, -req.density * omega) * it->second;
-            }
-            for (std::size_t j = 0; j < observers.size(); ++j)
-                er.observer_pressure.emplace_back(observed[2 * (e * observers.size() + j)],
-                                                  observed[2 * (e * observers.size() + j) + 1]);
-            for (const auto &[face, v] : req.excitations[e].faces) {
-                if (std::abs(v) == 0)
-                    continue;
-                double area = 0;
-                C integrated = 0;
-                for (std::uint32_t t = 0; t < nt; ++t)
-                    if (mesh_.triangles[t].face_id == face) {
-                        area += space_.panels[t].area;
-                        integrated += er.surface_pressure[t] * space_.panels[t].area;
-                    }
-                const C mean = integrated / area, volume = v * area;
-                er.impedances.push_back({face, area, mean, volume, mean / volume});
-            }
-            fr.excitations.push_back(std::move(er));
-        }
+        bool built = false;
+        FrequencyResult fr = solve_frequency(*P, i, ws, cancel, &built, report);
+        if (i != 0)
+            fr.ms[5] = 0;
         result.frequencies.push_back(std::move(fr));
         ++completed;
         report(6);
     }
+    result.peak_bytes = ws.capacity_bytes();
     return result;
+}
+
+FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_t index, Workspace &ws,
+                                            const volatile std::int32_t *cancel, bool *plan_built,
+                                            const std::function<void(std::int32_t)> &report) {
+    using C = std::complex<double>;
+    const Request &req = P.request;
+    const Numerics &num = req.numerics;
+    auto cancelled = [&] {
+        if (cancel && *cancel)
+            throw Cancelled();
+    };
+    auto phase = [&](std::int32_t p) {
+        if (report)
+            report(p);
+    };
+    cancelled();
+    hipStream_t s = ws.stream;
+    bool built = false;
+    const std::shared_ptr<DevicePlan> plan_ref = device_plan(num, P.k_max, s, &built);
+    const DevicePlan &plan = *plan_ref;
+    if (plan_built)
+        *plan_built = built;
+    if (ws.bound != P.id)
+        bind(P, ws);
+    cancelled();
+    const std::uint32_t nt = this->nt(), nv = this->nv(), ne = P.ne, nd = P.nd;
+    const bool mixed = num.precision == 1;
+    const unsigned solver_kind = mixed ? 1u : num.solver;
+    const std::size_t scratch = ws.scratch.size();
+    const double f = req.frequencies[index];
+    phase(0);
+    FrequencyResult fr;
+    fr.frequency = f;
+    fr.ms[5] = built ? plan.plan_ms : 0;
+    const double omega = 2 * std::numbers::pi * f, k = omega / req.sound_speed;
+    double max_kh = 0;
+    for (const auto &p : space_.panels)
+        max_kh = std::max(max_kh, k * p.max_edge);
+    // fluxes: q = -i rho omega v, per flux column and per panel
+    std::vector<double> qd(2 * static_cast<std::size_t>(std::max<std::uint32_t>(nd, 1)) * ne, 0.0),
+        qpanel(2 * static_cast<std::size_t>(nt) * ne, 0.0);
+    for (std::uint32_t e = 0; e < ne; ++e)
+        for (std::uint32_t t = 0; t < nt; ++t) {
+            const auto it = P.face_velocity[e].find(mesh_.triangles[t].face_id);
+            if (it == P.face_velocity[e].end())
+                continue;
+            const C q = C(0, -req.density * omega) * it->second;
+            qpanel[2 * (static_cast<std::size_t>(e) * nt + t)] = q.real();
+            qpanel[2 * (static_cast<std::size_t>(e) * nt + t) + 1] = q.imag();
+            if (P.flux_column[t] >= 0) {
+                qd[2 * (static_cast<std::size_t>(e) * nd + P.flux_column[t])] = q.real();
+                qd[2 * (static_cast<std::size_t>(e) * nd + P.flux_column[t]) + 1] = q.imag();
+            }
+        }
+    hip_check(hipMemcpyAsync(ws.q.data(), qd.data(), sizeof(double) * qd.size(), hipMemcpyHostToDevice, s), "q upload");
+    hip_check(hipMemcpyAsync(ws.flux.data(), qpanel.data(), sizeof(double) * qpanel.size(), hipMemcpyHostToDevice, s),
+              "flux upload");
+    const auto t_asm = Clock::now();
+    hip_check(hipMemsetAsync(ws.A.data(), 0, ws.A.bytes(), s), "clear A");
+    hip_check(hipMemsetAsync(ws.BD.data(), 0, ws.BD.bytes(), s), "clear B");
+    hip_check(hipMemsetAsync(ws.error.data(), 0, sizeof(int), s), "clear error");
+    auto L = assembly_launch(plan, num, k, ws.A.data(), ws.BD.data(), nd, ws.flux_column.data(), ws.error.data());
+    hip_check(hipMemsetAsync(ws.samples.data(), 0, sizeof(unsigned long long), s), "clear samples");
+    L.args.samples = ws.samples.data();
+    hipEvent_t *ev = ws.events;
+    hip_check(hipEventRecord(ev[0], s), "event");
+    dev::launch_regular(L, s);
+    hip_check(hipEventRecord(ev[1], s), "event");
+    dev::launch_touching(L, s);
+    hip_check(hipEventRecord(ev[2], s), "event");
+    dev::launch_near(L, s);
+    hip_check(hipEventRecord(ev[3], s), "event");
+    dev::launch_mass(L, s);
+    hip_check(hipGetLastError(), "assembly launch");
+    if (nd)
+        dev::launch_flux_rhs(ws.BD.data(), nv, nd, ws.q.data(), ne, ws.rhs.data(), s);
+    else
+        hip_check(hipMemsetAsync(ws.rhs.data(), 0, ws.rhs.bytes(), s), "clear rhs");
+    check_error_flag(ws.error, s);
+    float e01 = 0, e12 = 0, e23 = 0;
+    hip_check(hipEventElapsedTime(&e01, ev[0], ev[1]), "event time");
+    hip_check(hipEventElapsedTime(&e12, ev[1], ev[2]), "event time");
+    hip_check(hipEventElapsedTime(&e23, ev[2], ev[3]), "event time");
+    unsigned long long regular_samples = 0;
+    hip_check(hipMemcpyAsync(&regular_samples, ws.samples.data(), sizeof(regular_samples), hipMemcpyDeviceToHost, s),
+              "samples download");
+    hip_check(hipStreamSynchronize(s), "samples sync");
+    fr.regular_samples = static_cast<double>(regular_samples);
+    fr.touching_samples = plan.touching_samples;
+    fr.near_samples = plan.near_samples;
+    fr.regular_kernel_ms = e01;
+    fr.touching_kernel_ms = e12;
+    fr.near_kernel_ms = e23;
+    fr.ms[0] = ms_since(t_asm);
+    cancelled();
+    // mass-weighted rhs norms (for the unscaled relative residual)
+    std::vector<double> rhs_norm(ne);
+    for (std::uint32_t e = 0; e < ne; ++e) {
+        dev::launch_norm2(ws.rhs.data() + 2 * static_cast<std::size_t>(e) * nv, nv, lumped_.data(),
+                          ws.scratch.data() + scratch - 8, ws.scratch.data(), s);
+        double v = 0;
+        hip_check(hipMemcpyAsync(&v, ws.scratch.data() + scratch - 8, sizeof(double), hipMemcpyDeviceToHost, s), "norm");
+        hip_check(hipStreamSynchronize(s), "norm sync");
+        rhs_norm[e] = std::sqrt(v);
+        if (!std::isfinite(rhs_norm[e]))
+            throw std::runtime_error("Galerkin right-hand side is not finite.");
+    }
+    phase(3);
+    const auto t_solve = Clock::now();
+    std::vector<std::uint32_t> iterations(ne, 0);
+    std::uint32_t refinements = 0;
+    if (solver_kind == 0) {
+        hip_check(hipMemcpyAsync(ws.LU.data(), ws.A.data(), ws.A.bytes(), hipMemcpyDeviceToDevice, s), "LU copy");
+        hip_check(hipMemsetAsync(ws.error.data(), 0, sizeof(int), s), "clear info");
+        const auto t_f = Clock::now();
+        dev::lu_factor(ws.LU.data(), nv, ws.pivots.data(), ws.error.data(), nullptr, s);
+        int info = 0;
+        hip_check(hipMemcpyAsync(&info, ws.error.data(), sizeof(int), hipMemcpyDeviceToHost, s), "info");
+        hip_check(hipStreamSynchronize(s), "LU sync");
+        fr.ms[2] = ms_since(t_f);
+        if (info)
+            throw std::runtime_error("Burton-Miller factorization failed at pivot " + std::to_string(info));
+        const auto t_t = Clock::now();
+        hip_check(hipMemcpyAsync(ws.x.data(), ws.rhs.data(), ws.rhs.bytes(), hipMemcpyDeviceToDevice, s), "rhs copy");
+        dev::lu_solve(ws.LU.data(), nv, ws.pivots.data(), ws.x.data(), ne, s);
+        hip_check(hipStreamSynchronize(s), "solve sync");
+        fr.ms[3] = ms_since(t_t);
+    } else {
+        for (std::uint32_t e = 0; e < ne; ++e) {
+            double achieved = 0;
+            dense_gmres(s, ws.A.data(), mixed, nv, ws.rhs.data() + 2 * static_cast<std::size_t>(e) * nv,
+                        ws.x.data() + 2 * static_cast<std::size_t>(e) * nv, num.gmres_restart, num.gmres_max_iterations,
+                        num.residual_tolerance * 0.1, &achieved, &iterations[e], cancel, ws.gmres);
+        }
+    }
+    // FP64 residual of the unscaled equation; LU corrections while above tolerance
+    std::vector<double> relative(ne);
+    for (;;) {
+        dev::launch_residual(ws.A.data(), mixed, nv, ws.rhs.data(), ws.x.data(), ne, ws.r.data(), ws.scratch.data(), s);
+        double largest = 0;
+        for (std::uint32_t e = 0; e < ne; ++e) {
+            dev::launch_norm2(ws.r.data() + 2 * static_cast<std::size_t>(e) * nv, nv, lumped_.data(),
+                              ws.scratch.data() + scratch - 8, ws.scratch.data(), s);
+            double v = 0;
+            hip_check(hipMemcpyAsync(&v, ws.scratch.data() + scratch - 8, sizeof(double), hipMemcpyDeviceToHost, s),
+                      "residual norm");
+            hip_check(hipStreamSynchronize(s), "residual sync");
+            relative[e] = rhs_norm[e] > 0 ? std::sqrt(v) / rhs_norm[e]
+                                           : (v == 0 ? 0 : std::numeric_limits<double>::infinity());
+            if (!std::isfinite(relative[e]))
+                throw std::runtime_error("Burton-Miller residual is not finite.");
+            largest = std::max(largest, relative[e]);
+        }
+        if (largest <= num.residual_tolerance)
+            break;
+        if (solver_kind != 0 || refinements >= num.refinement_steps) {
+            std::ostringstream msg;
+            msg.precision(9);
+            msg << "Burton-Miller residual " << largest << " exceeds tolerance " << num.residual_tolerance << " at " << f
+                << " Hz after " << refinements << " refinement steps.";
+            throw std::runtime_error(msg.str());
+        }
+        dev::lu_solve(ws.LU.data(), nv, ws.pivots.data(), ws.r.data(), ne, s);
+        const double one[2] = {1, 0};
+        DeviceBuffer<double> alpha;
+        alpha.upload(std::vector<double>(one, one + 2), s);
+        for (std::uint32_t e = 0; e < ne; ++e)
+            dev::launch_axpy(nv, alpha.data(), ws.r.data() + 2 * static_cast<std::size_t>(e) * nv,
+                             ws.x.data() + 2 * static_cast<std::size_t>(e) * nv, s);
+        hip_check(hipStreamSynchronize(s), "refinement sync");
+        ++refinements;
+    }
+    fr.ms[1] = ms_since(t_solve);
+    // fields
+    phase(5);
+    const auto t_field = Clock::now();
+    const auto &observers = P.observers;
+    std::vector<double> observed;
+    if (!observers.empty() && P.has_nonzero) {
+        auto evaluate = [&](unsigned degree) {
+            cancelled();
+            const auto rule = triangle_rule(degree);
+            std::vector<double> rt;
+            for (const auto &p : rule) {
+                rt.push_back(p.bary[0]);
+                rt.push_back(p.bary[1]);
+                rt.push_back(p.bary[2]);
+                rt.push_back(p.weight);
+            }
+            DeviceBuffer<double> drule;
+            drule.upload(rt, s);
+            dev::FieldArgs fa{};
+            fa.panels = {geo_.data(), nodes_.data(), bounds_.data(), shape_.data(), nt, nv};
+            fa.pressure = ws.x.data();
+            fa.flux = ws.flux.data();
+            fa.nrhs = ne;
+            fa.points = ws.obs.data();
+            fa.count = static_cast<std::uint32_t>(observers.size());
+            fa.rule = drule.data();
+            fa.rule_count = static_cast<std::uint32_t>(rule.size());
+            fa.k = k;
+            fa.near_tolerance = std::min(num.near_tolerance, .025 * num.residual_tolerance);
+            fa.max_depth = num.near_depth;
+            fa.origin[0] = P.far_origin.x;
+            fa.origin[1] = P.far_origin.y;
+            fa.origin[2] = P.far_origin.z;
+            fa.reference_distance = req.far_reference_distance;
+            fa.out = ws.field.data();
+            fa.error = ws.error.data();
+            hip_check(hipMemsetAsync(ws.error.data(), 0, sizeof(int), s), "clear error");
+            if (req.far_field)
+                dev::launch_far_field(fa, ws.partials.data(), s);
+            else
+                dev::launch_observer_field(fa, ws.partials.data(), s);
+            hip_check(hipGetLastError(), "field launch");
+            check_error_flag(ws.error, s);
+            std::vector<double> out(ws.field.size());
+            hip_check(hipMemcpyAsync(out.data(), ws.field.data(), ws.field.bytes(), hipMemcpyDeviceToHost, s),
+                      "field download");
+            hip_check(hipStreamSynchronize(s), "field sync");
+            return out;
+        };
+        unsigned degree = minimum_endpoint_degree(max_kh, num.quadrature_order);
+        observed = evaluate(degree);
+        for (;;) {
+            const unsigned next = refined_endpoint_degree(degree);
+            auto refined = evaluate(next);
+            bool passed = true;
+            for (std::uint32_t e = 0; e < ne; ++e) {
+                double diff = 0, norm = 0;
+                for (std::size_t j = 0; j < observers.size(); ++j) {
+                    const std::size_t i = 2 * (e * observers.size() + j);
+                    const C hi(refined[i], refined[i + 1]), lo(observed[i], observed[i + 1]);
+                    diff += std::norm(hi - lo);
+                    norm += std::norm(hi);
+                }
+                const double ind = norm > 0 ? std::sqrt(diff / norm) : diff == 0 ? 0 : INFINITY;
+                passed = passed && ind <= .025 * num.residual_tolerance;
+            }
+            observed = std::move(refined);
+            if (passed)
+                break;
+            if (degree == 12)
+                throw std::runtime_error("BEM complex observer-field quadrature 12/14 did not converge within "
+                                         "0.025*residualTolerance; refine the mesh or observer separation.");
+            degree = next;
+        }
+    } else if (!observers.empty()) {
+        observed.assign(2 * observers.size() * ne, 0.0);
+    }
+    // nodal pressures to host
+    std::vector<double> nodal(2 * static_cast<std::size_t>(nv) * ne);
+    hip_check(hipMemcpyAsync(nodal.data(), ws.x.data(), sizeof(double) * nodal.size(), hipMemcpyDeviceToHost, s),
+              "pressure download");
+    hip_check(hipStreamSynchronize(s), "pressure sync");
+    fr.ms[4] = ms_since(t_field);
+    for (std::uint32_t e = 0; e < ne; ++e) {
+        ExcitationResult er;
+        er.residual = relative[e];
+        er.iterations = iterations[e];
+        er.refinements = refinements;
+        er.pressure_trace.resize(nv);
+        for (std::uint32_t i = 0; i < nv; ++i)
+            er.pressure_trace[vertex_of_[i]] = C(nodal[2 * (static_cast<std::size_t>(e) * nv + i)],
+                                                 nodal[2 * (static_cast<std::size_t>(e) * nv + i) + 1]);
+        er.surface_pressure.resize(nt);
+        er.normal_derivative.resize(nt);
+        for (std::uint32_t t = 0; t < nt; ++t) {
+            C avg = 0;
+            for (auto v : space_.panels[t].nodes)
+                avg += er.pressure_trace[v] / 3.0;
+            if (!std::isfinite(avg.real()) || !std::isfinite(avg.imag()))
+                throw std::runtime_error("Burton-Miller pressure is not finite.");
+            er.surface_pressure[t] = avg;
+            const auto it = P.face_velocity[e].find(mesh_.triangles[t].face_id);
+            er.normal_derivative[t] = it == P.face_velocity[e].end() ? C{} : C(0, -req.density * omega) * it->second;
+        }
+        for (std::size_t j = 0; j < observers.size(); ++j)
+            er.observer_pressure.emplace_back(observed[2 * (e * observers.size() + j)],
+                                              observed[2 * (e * observers.size() + j) + 1]);
+        for (const auto &[face, v] : req.excitations[e].faces) {
+            if (std::abs(v) == 0)
+                continue;
+            double area = 0;
+            C integrated = 0;
+            for (std::uint32_t t = 0; t < nt; ++t)
+                if (mesh_.triangles[t].face_id == face) {
+                    area += space_.panels[t].area;
+                    integrated += er.surface_pressure[t] * space_.panels[t].area;
+                }
+            const C mean = integrated / area, volume = v * area;
+            er.impedances.push_back({face, area, mean, volume, mean / volume});
+        }
+        fr.excitations.push_back(std::move(er));
+    }
+    return fr;
 }
 
 } // namespace hsbem

@@ -14,6 +14,7 @@
 
 #include <atomic>
 #include <complex>
+#include <functional>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -41,14 +42,14 @@ template <class T> class DeviceBuffer {
     ~DeviceBuffer() { release(); }
     DeviceBuffer(const DeviceBuffer &) = delete;
     DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    // Sets the element count; storage only grows (no device free on shrinking).
     void resize(std::size_t count) {
-        if (count == count_)
-            return;
-        release();
-        if (count) {
+        if (count > capacity_) {
+            release();
             void *p = nullptr;
             hip_check(hipMalloc(&p, count * sizeof(T)), "device allocation");
             data_ = static_cast<T *>(p);
+            capacity_ = count;
         }
         count_ = count;
     }
@@ -56,7 +57,7 @@ template <class T> class DeviceBuffer {
         if (data_)
             (void)hipFree(data_);
         data_ = nullptr;
-        count_ = 0;
+        count_ = capacity_ = 0;
     }
     void upload(const std::vector<T> &v, hipStream_t s) {
         resize(v.size());
@@ -66,10 +67,11 @@ template <class T> class DeviceBuffer {
     T *data() const { return data_; }
     std::size_t size() const { return count_; }
     std::size_t bytes() const { return count_ * sizeof(T); }
+    std::size_t capacity_bytes() const { return capacity_ * sizeof(T); }
 
   private:
     T *data_{};
-    std::size_t count_{};
+    std::size_t count_{}, capacity_{};
 };
 
 struct Context {
@@ -120,6 +122,7 @@ struct ExcitationResult {
 
 struct FrequencyResult {
     double frequency{};
+    double served_ms{}; // serving loop: submission to completion of this frequency point
     std::vector<ExcitationResult> excitations;
     double ms[6]{}; // assembly, solve, factorization, triangular, field, plan
     // device-event kernel times of this frequency (ms) and algorithmic counts
@@ -135,6 +138,27 @@ struct Result {
 
 struct GmresWorkspace {
     DeviceBuffer<double> basis, r, scratch, small;
+};
+
+// Device buffers and stream of one solve in flight. The direct solve path owns
+// one per mesh (on the context stream); the serving loop owns one per slot,
+// each on its own stream, so solves of different requests overlap.
+struct Workspace {
+    explicit Workspace(hipStream_t stream, bool owns_stream = false);
+    ~Workspace();
+    Workspace(const Workspace &) = delete;
+    Workspace &operator=(const Workspace &) = delete;
+    void release();                       // frees every buffer (device-synchronising)
+    std::uint64_t capacity_bytes() const; // device memory held
+    hipStream_t stream{};
+    bool owns_stream{};
+    DeviceBuffer<double> A, LU, BD, q, rhs, x, r, flux, scratch, obs, field, partials;
+    DeviceBuffer<std::int32_t> flux_column, pivots;
+    DeviceBuffer<int> error;
+    DeviceBuffer<unsigned long long> samples;
+    GmresWorkspace gmres;
+    hipEvent_t events[4]{};
+    std::uint64_t bound{}; // id of the PreparedRequest whose per-request data is uploaded
 };
 
 // Restarted GMRES(restart) with CGS2 Arnoldi on a dense row-major-free complex
@@ -157,10 +181,31 @@ struct DevicePlan {
     double touching_samples{}, near_samples{}; // algorithmic sample counts of one assembly
 };
 
+// A validated request with its host-side derived data (observers, flux columns,
+// per-face velocities), shared by all of its frequency points.
+struct PreparedRequest {
+    std::uint64_t id{}; // unique per preparation (workspace binding key)
+    Request request;
+    std::vector<Vec3> observers; // recentred points, or unit far-field directions
+    Vec3 far_origin{};
+    std::vector<std::map<std::uint32_t, std::complex<double>>> face_velocity;
+    std::vector<std::int32_t> flux_column;
+    std::uint32_t nd{}, ne{}, nv{}, nt{};
+    double k_max{};
+    bool has_nonzero{};
+    std::uint64_t footprint{}; // device bytes a workspace holds for one frequency point
+};
+
 class MeshSolver {
   public:
     MeshSolver(Context *ctx, SurfaceMesh mesh);
     Result solve(const Request &request, ProgressFn progress, void *user, const volatile std::int32_t *cancel);
+    // Serving seams: validation and host preparation (no device work), then one
+    // frequency point on a caller-owned workspace. Thread safe across workspaces.
+    std::shared_ptr<PreparedRequest> prepare(const Request &request) const;
+    FrequencyResult solve_frequency(const PreparedRequest &prepared, std::size_t index, Workspace &ws,
+                                    const volatile std::int32_t *cancel, bool *plan_built = nullptr,
+                                    const std::function<void(std::int32_t)> &report = {});
     // Component seams.
     InteractionPlan plan(const Numerics &numerics, double k_max);
     void assemble_dense(const Numerics &numerics, double k_max, double k, double *A, double *B);
@@ -169,11 +214,13 @@ class MeshSolver {
     std::uint32_t nv() const { return static_cast<std::uint32_t>(space_.vertices.size()); }
     std::uint32_t nt() const { return static_cast<std::uint32_t>(space_.panels.size()); }
     std::mutex &mutex() { return mutex_; }
+    Context *context() const { return ctx_; }
 
   private:
-    DevicePlan &device_plan(const Numerics &n, double k_max, bool *built);
+    std::shared_ptr<DevicePlan> device_plan(const Numerics &n, double k_max, hipStream_t stream, bool *built);
     dev::AssemblyLaunch assembly_launch(const DevicePlan &p, const Numerics &n, double k, double *A, double *BD,
-                                        std::uint32_t nd, const std::int32_t *flux_column);
+                                        std::uint32_t nd, const std::int32_t *flux_column, int *error);
+    void bind(const PreparedRequest &prepared, Workspace &ws);
     void build_clusters();
 
     Context *ctx_;
@@ -190,14 +237,10 @@ class MeshSolver {
     DeviceBuffer<std::uint8_t> shape_, local_;
     DeviceBuffer<std::uint32_t> cluster_panel_offsets_, cluster_panels_, cluster_node_offsets_, cluster_nodes_;
     std::uint32_t cluster_count_{};
-    DeviceBuffer<int> error_;
-    DeviceBuffer<unsigned long long> samples_;
-    std::map<std::tuple<unsigned, unsigned, double, double, unsigned, double>, std::unique_ptr<DevicePlan>> plans_;
-    // workspace (reused across calls)
-    DeviceBuffer<double> A_, LU_, BD_, q_, rhs_, x_, r_, flux_, scratch_, obs_, field_, partials_;
-    GmresWorkspace gmres_;
-    DeviceBuffer<std::int32_t> flux_column_, pivots_;
-    std::mutex mutex_;
+    std::map<std::tuple<unsigned, unsigned, double, double, unsigned, double>, std::shared_ptr<DevicePlan>> plans_;
+    std::mutex plan_mutex_;
+    std::unique_ptr<Workspace> workspace_; // direct solve path (context stream), reused across calls
+    std::mutex mutex_;                     // one direct solve per mesh at a time (as the reference)
 };
 
 } // namespace hsbem

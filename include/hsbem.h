@@ -38,7 +38,8 @@ typedef enum {
     HSBEM_INVALID_ARGUMENT = 1, /* reference: std::invalid_argument */
     HSBEM_RUNTIME_ERROR = 2,    /* reference: std::runtime_error (numerics, budget, convergence) */
     HSBEM_DEVICE_ERROR = 3,     /* HIP runtime failure */
-    HSBEM_CANCELLED = 4         /* reference: "BEM calculation cancelled." */
+    HSBEM_CANCELLED = 4,        /* reference: "BEM calculation cancelled." */
+    HSBEM_PENDING = 5           /* hsbem_poll: the request is still queued or running */
 } hsbem_status;
 
 enum { HSBEM_SOLVER_LU = 0, HSBEM_SOLVER_GMRES = 1 };
@@ -60,6 +61,7 @@ typedef struct hsbem_context hsbem_context;
 typedef struct hsbem_mesh hsbem_mesh;
 typedef struct hsbem_result hsbem_result;
 typedef struct hsbem_plan hsbem_plan;
+typedef struct hsbem_server hsbem_server;
 
 /* BemNumerics (bem.hpp:113-123) plus the MI355X solver choices. */
 typedef struct {
@@ -152,6 +154,41 @@ HSBEM_API int hsbem_result_residual(const hsbem_result *result, uint32_t frequen
  * algorithmic sample counts of those three kernels (FP64 pair-kernel evaluations). */
 HSBEM_API int hsbem_result_timings(const hsbem_result *result, uint32_t frequency, double *milliseconds);
 HSBEM_API int hsbem_result_peak_device_bytes(const hsbem_result *result, uint64_t *bytes);
+
+/* ---- serving loop (no counterpart class in the reference: its callers run
+ * CudaBemSolver::solve on worker threads, apps/desktop/simulation_dock.cpp:2425-2600,
+ * apps/desktop/parameter_sweep.cpp:215-250) ----
+ * Requests are split into frequency points. Each slot (own stream and workspace)
+ * takes queued points in arrival order once the point's device footprint fits
+ * the memory budget; points of requests submitted while others run start as
+ * soon as a slot frees, and their kernels overlap on the device. Meshes stay
+ * resident (geometry, clusters, interaction plans) and may be destroyed by the
+ * caller while their requests are in flight. */
+typedef struct {
+    uint32_t slots;               /* concurrent frequency points, 1..16 (default 3) */
+    uint64_t memory_budget_bytes; /* device bytes for slot workspaces; 0: 90 % of free memory */
+} hsbem_server_options;
+
+typedef struct {
+    uint64_t submitted, completed, failed, points;
+    uint32_t running, queued, max_running;
+    uint64_t budget_bytes, held_bytes, peak_held_bytes;
+} hsbem_server_statistics;
+
+HSBEM_API int hsbem_server_create(hsbem_context *context, const hsbem_server_options *options, hsbem_server **server);
+/* Cancels queued and running requests and joins the slots. */
+HSBEM_API int hsbem_server_destroy(hsbem_server *server);
+/* Validates like hsbem_solve (HSBEM_INVALID_ARGUMENT also when one frequency point
+ * exceeds the memory budget) and queues the request; returns at once. */
+HSBEM_API int hsbem_submit(hsbem_server *server, hsbem_mesh *mesh, const hsbem_request *request, uint64_t *ticket);
+/* Waits up to wait_ms (0: no wait; negative: until finished). HSBEM_OK with *result
+ * when every point is solved, HSBEM_PENDING otherwise, or the request's error
+ * status (HSBEM_CANCELLED after hsbem_cancel). A finished ticket is released. */
+HSBEM_API int hsbem_poll(hsbem_server *server, uint64_t ticket, int32_t wait_ms, hsbem_result **result);
+HSBEM_API int hsbem_cancel(hsbem_server *server, uint64_t ticket);
+HSBEM_API int hsbem_server_stats(const hsbem_server *server, hsbem_server_statistics *statistics);
+/* Milliseconds from hsbem_submit to the completion of this frequency point. */
+HSBEM_API int hsbem_result_served_latency(const hsbem_result *result, uint32_t frequency, double *milliseconds);
 
 /* ---- internal seams (component tests, benchmarks) ---- */
 

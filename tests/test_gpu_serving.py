@@ -1,0 +1,122 @@
+"""Serving loop on the device: continuous batching of frequency points from
+several requests over concurrent slots (hsbem_submit / hsbem_poll).
+
+Each served result must equal the same request solved alone. Tolerances: the
+assembly sums with device atomics, so results are reproducible to rounding
+only: plain FP64 points <= 1e-10 relative L2 (surface pressure), mixed points
+(FP32 matrix) <= 2e-5.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from fixtures import BEAT_FREQUENCIES, sphere_request
+from hornsculpt_amd import bem, meshes
+
+pytestmark = pytest.mark.gpu
+
+
+def _horn_request(frequencies, numerics):
+    return bem.BemRequest(medium=bem.AcousticMedium(1.205, 343.0), frequencies=list(frequencies),
+                          excitations=[bem.BemExcitation("throat", [bem.FaceVelocity(1, 1.0)])],
+                          observers=[(1.3, 0.0, 0.0), (0.3, 1.0, 0.0)],
+                          far_field=bem.FarFieldObservation((0.3, 0.0, 0.0), 1.0), numerics=numerics)
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _compare(alone, served, tol):
+    assert len(alone.frequencies) == len(served.frequencies)
+    worst = 0.0
+    for fa, fs in zip(alone.frequencies, served.frequencies):
+        assert fa.frequency == fs.frequency
+        assert fs.served_milliseconds > 0
+        for ea, es in zip(fa.excitations, fs.excitations):
+            worst = max(worst, _rel(es.boundary.surface_pressure, ea.boundary.surface_pressure))
+            if len(ea.observer_pressure):
+                worst = max(worst, _rel(es.observer_pressure, ea.observer_pressure))
+    assert worst <= tol, worst
+    return worst
+
+
+@pytest.fixture(scope="module")
+def solvers():
+    return {"ico": bem.HipBemSolver(meshes.icosphere(3)), "octa": bem.HipBemSolver(meshes.octa_sphere(3)),
+            "horn": bem.HipBemSolver(meshes.exponential_horn(0.02))}
+
+
+def _workload():
+    gm = bem.BemNumerics(solver=bem.SOLVER_GMRES, relative_residual_tolerance=1e-9)
+    return [
+        ("ico", sphere_request([500.0, 1000.0, 1500.0], observers=[(0, 0, 1), (1, 0, 0)]), 1e-10),
+        ("octa", sphere_request(BEAT_FREQUENCIES), 1e-10),
+        ("horn", _horn_request([1000.0, 3000.0], bem.BemNumerics(precision=bem.PRECISION_MIXED)), 2e-5),
+        ("ico", sphere_request([700.0], numerics=gm), 1e-10),
+        ("horn", _horn_request([2000.0], gm), 1e-10),
+    ]
+
+
+def test_served_results_equal_alone_results(solvers):
+    work = _workload()
+    alone = [solvers[m].solve(r) for m, r, _ in work]
+    server = bem.HipBemServer(slots=3)
+    tickets = [server.submit(solvers[m], r) for m, r, _ in work]
+    served = [server.poll(t) for t in tickets]
+    stats = server.stats()
+    server.close()
+    for (m, r, tol), a, s in zip(work, alone, served):
+        worst = _compare(a, s, tol)
+        print(f"{m} {r.frequencies}: served vs alone {worst:.2e}")
+    assert stats["points"] == sum(len(r.frequencies) for _, r, _ in work)
+    assert stats["completed"] == len(work) and stats["failed"] == 0
+    assert stats["max_running"] >= 2  # points of different requests were on the device together
+
+
+def test_admission_by_memory_footprint(solvers):
+    horn = solvers["horn"]
+    req = _horn_request([1500.0, 2500.0], bem.BemNumerics(solver=bem.SOLVER_GMRES))
+    probe = bem.HipBemServer(slots=1)
+    footprint = probe.poll(probe.submit(horn, req)).peak_device_bytes
+    probe.close()
+    assert footprint > 0
+    # Room for one point at a time: three slots, but the budget admits one footprint.
+    server = bem.HipBemServer(slots=3, memory_budget_bytes=int(1.5 * footprint))
+    tickets = [server.submit(horn, req) for _ in range(3)]
+    results = [server.poll(t) for t in tickets]
+    stats = server.stats()
+    assert stats["max_running"] == 1
+    assert stats["peak_held_bytes"] <= int(1.5 * footprint)
+    for r in results[1:]:
+        _compare(results[0], r, 1e-10)
+    # A point that can never fit is refused at submission.
+    small = bem.HipBemServer(slots=2, memory_budget_bytes=footprint // 2)
+    with pytest.raises(ValueError):
+        small.submit(horn, req)
+    small.close()
+    server.close()
+
+
+def test_invalid_request_refused_at_submission(solvers):
+    server = bem.HipBemServer(slots=2)
+    bad = sphere_request([1000.0])
+    bad.numerics.quadrature_order = 3
+    with pytest.raises(ValueError):
+        server.submit(solvers["ico"], bad)
+    server.close()
+
+
+def test_cancel_and_poll_pending(solvers):
+    server = bem.HipBemServer(slots=1)
+    freqs = [200.0 * 1.2 ** i for i in range(12)]
+    t = server.submit(solvers["horn"], _horn_request(freqs, bem.BemNumerics()))
+    assert server.poll(t, 0) is None  # twelve points cannot be done yet
+    server.cancel(t)
+    with pytest.raises(RuntimeError, match="cancelled"):
+        server.poll(t)
+    # the slot keeps serving after a cancellation
+    ok = server.poll(server.submit(solvers["ico"], sphere_request([1000.0])))
+    assert ok.frequencies[0].excitations[0].boundary_residual <= 5e-5
+    server.close()
