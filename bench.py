@@ -180,9 +180,10 @@ def run_served(solver, frequencies, precision, tolerance, steps, slots, label, o
             delay = arrival - time.perf_counter()
             if delay > 0:
                 time.sleep(delay)
-            # Points drawn from the design sweep share its interaction plan (planning band of the sweep).
+            # Points drawn from the design sweep share its interaction plan (the sweep is
+            # their planning raster, interactionPlanFrequencies in the reference request).
             tickets.append(server.submit(solver, horn_request([frequencies[i]], precision, bem.SOLVER_GMRES,
-                                                              tolerance, planning=[max(frequencies)])))
+                                                              tolerance, planning=frequencies)))
         lat = [server.poll(t).frequencies[0].served_milliseconds for t in tickets]
         out["p50"] = float(np.percentile(lat, 50))
         out["p99"] = float(np.percentile(lat, 99))

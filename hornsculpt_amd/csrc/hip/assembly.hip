@@ -10,7 +10,12 @@
 //   touching_jobs   one wavefront per touching pair: Sauter-Schwab (u,v,w) tuples
 //                   over lanes with the radial coordinate innermost (every sector map
 //                   is linear in r), shuffle reduction, one lane scatters. Always FP64.
-//   near_jobs       one wavefront per near job, patch points over lanes.
+//   near_patches    one thread per tensor near patch (16x16 or 64x64 points of the
+//                   patch's test and trial sub-triangles), two-stage contraction as
+//                   for regular pairs with the affine sub-triangle maps applied on
+//                   the fly; one scatter per patch.
+//   near_jobs       one wavefront per near job holding a projection-fan patch,
+//                   patch points over lanes.
 //   mass            1/2 M_pp into A, -alpha 1/2 M_pq into the flux columns.
 // Precision: the sample type R is double (plain form) or float (mixed form:
 // FP32 samples and accumulation, FP64 geometry differences, FP32 matrix).
@@ -188,6 +193,65 @@ __device__ __forceinline__ void integrate_tensor(MomentsT<R> &m, const double *g
 #pragma unroll
             for (int j = 0; j < 3; ++j)
                 m.a[3 * i + j] = cadd_t(m.a[3 * i + j], cscale_t(ga[j], xb));
+        }
+    }
+}
+
+// Tensor near patch: test points x of the rule mapped into the test sub-triangle
+// with barycentric vertices Tt (bary_parent = sum_j x_j Tt[j], map_patch_point),
+// trial points likewise with Ts; weight w_x w_y times the patch area fraction.
+// Same contraction as integrate_tensor; maps in FP64, rounded to R.
+template <class R>
+__device__ __forceinline__ void integrate_patch(MomentsT<R> &m, const double *gt, const double *gs, const double *Tt,
+                                                const double *Ts, double area_fraction, const R *rule, int n, double k,
+                                                bool &bad) {
+    const PairConst<R> c = pair_const<R>(gt, gs, k);
+    const R wscale = static_cast<R>(4.0 * gt[G_AREA] * gs[G_AREA] * 0.07957747154594766788 * area_fraction);
+    const R bx = static_cast<R>(gt[G_V0] - gs[G_V0]), by = static_cast<R>(gt[G_V0 + 1] - gs[G_V0 + 1]),
+            bz = static_cast<R>(gt[G_V0 + 2] - gs[G_V0 + 2]);
+    // sub-triangle maps as origin + b1 d1 + b2 d2 on the parent barycentrics
+    R to[3], t1[3], t2[3], so[3], s1[3], s2[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        to[i] = static_cast<R>(Tt[i]);
+        t1[i] = static_cast<R>(Tt[3 + i] - Tt[i]);
+        t2[i] = static_cast<R>(Tt[6 + i] - Tt[i]);
+        so[i] = static_cast<R>(Ts[i]);
+        s1[i] = static_cast<R>(Ts[3 + i] - Ts[i]);
+        s2[i] = static_cast<R>(Ts[6 + i] - Ts[i]);
+    }
+    const R te1x = gt[G_E1], te1y = gt[G_E1 + 1], te1z = gt[G_E1 + 2], te2x = gt[G_E2], te2y = gt[G_E2 + 1],
+            te2z = gt[G_E2 + 2];
+    const R se1x = gs[G_E1], se1y = gs[G_E1 + 1], se1z = gs[G_E1 + 2], se2x = gs[G_E2], se2y = gs[G_E2 + 1],
+            se2z = gs[G_E2 + 2];
+    for (int xi = 0; xi < n; ++xi) {
+        const R *xp = rule + 4 * xi;
+        R xt[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            xt[i] = to[i] + xp[1] * t1[i] + xp[2] * t2[i];
+        const R xw = xp[3] * wscale;
+        const R ltx = te1x * xt[1] + te2x * xt[2], lty = te1y * xt[1] + te2y * xt[2], ltz = te1z * xt[1] + te2z * xt[2];
+        CplxT<R> ga[3] = {{0, 0}, {0, 0}, {0, 0}}, gg = {0, 0}, gb = {0, 0};
+        for (int yi = 0; yi < n; ++yi) {
+            const R *yp = rule + 4 * yi;
+            const R y1 = so[1] + yp[1] * s1[1] + yp[2] * s2[1], y2 = so[2] + yp[1] * s1[2] + yp[2] * s2[2];
+            const R y0 = so[0] + yp[1] * s1[0] + yp[2] * s2[0];
+            const R lsx = se1x * y1 + se2x * y2, lsy = se1y * y1 + se2y * y2, lsz = se1z * y1 + se2z * y2;
+            const SampleT<R> s = sample(bx + (ltx - lsx), by + (lty - lsy), bz + (ltz - lsz), xw * yp[3], c, false, bad);
+            ga[0] = cadd_t(ga[0], cscale_t(s.a, y0));
+            ga[1] = cadd_t(ga[1], cscale_t(s.a, y1));
+            ga[2] = cadd_t(ga[2], cscale_t(s.a, y2));
+            gg = cadd_t(gg, s.g);
+            gb = cadd_t(gb, s.b);
+        }
+        m.g = cadd_t(m.g, gg);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            m.b[i] = cadd_t(m.b[i], cscale_t(gb, xt[i]));
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                m.a[3 * i + j] = cadd_t(m.a[3 * i + j], cscale_t(ga[j], xt[i]));
         }
     }
 }
@@ -529,6 +593,36 @@ __global__ __launch_bounds__(256) void near_jobs(AssemblyArgs args, const NearJo
         scatter_pair(args, matrix_of<R>(args), m, nj.test, nj.trial);
 }
 
+// Tensor near patches, one per thread (degree-8 items first, then degree 14).
+template <class R>
+__global__ __launch_bounds__(256) void near_patches(AssemblyArgs args, const NearPatchItem *items, std::uint32_t count,
+                                                    const PairPatch *patches, const double *rules) {
+    __shared__ R srule[4 * (16 + 64)];
+    for (int e = threadIdx.x; e < 4 * 16; e += blockDim.x)
+        srule[e] = static_cast<R>(rules[4 * kRule8 + e]);
+    for (int e = threadIdx.x; e < 4 * 64; e += blockDim.x)
+        srule[4 * 16 + e] = static_cast<R>(rules[4 * kRule14 + e]);
+    __syncthreads();
+    const std::uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count)
+        return;
+    const NearPatchItem it = items[idx];
+    const PairPatch &patch = patches[it.patch];
+    const PanelView &pv = args.panels;
+    double gt[G_COUNT], gs[G_COUNT];
+    load_geo(pv, it.test, gt);
+    load_geo(pv, it.trial, gs);
+    const bool high = patch.degree == 14;
+    MomentsT<R> m;
+    zero_t(m);
+    bool bad = false;
+    integrate_patch<R>(m, gt, gs, it.swap ? patch.trial : patch.test, it.swap ? patch.test : patch.trial,
+                       patch.area_fraction, srule + (high ? 4 * 16 : 0), high ? 64 : 16, args.k, bad);
+    if (bad)
+        atomicExch(args.error, ERR_DISTANCE);
+    scatter_pair(args, matrix_of<R>(args), m, it.test, it.trial);
+}
+
 template <class M> __global__ void mass_terms(AssemblyArgs args) {
     const std::uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const PanelView &pv = args.panels;
@@ -596,15 +690,24 @@ void launch_touching(const AssemblyLaunch &L, hipStream_t stream) {
 }
 
 void launch_near(const AssemblyLaunch &L, hipStream_t stream) {
-    if (!L.near_count)
-        return;
-    const dim3 grid((L.near_count + 3) / 4);
-    if (L.args.A32)
-        hipLaunchKernelGGL(near_jobs<float>, grid, dim3(256), 0, stream, L.args, L.near_jobs, L.near_count, L.patches,
-                           L.rules);
-    else
-        hipLaunchKernelGGL(near_jobs<double>, grid, dim3(256), 0, stream, L.args, L.near_jobs, L.near_count, L.patches,
-                           L.rules);
+    if (L.near_item_count) {
+        const dim3 grid((L.near_item_count + 255) / 256);
+        if (L.args.A32)
+            hipLaunchKernelGGL(near_patches<float>, grid, dim3(256), 0, stream, L.args, L.near_items,
+                               L.near_item_count, L.patches, L.rules);
+        else
+            hipLaunchKernelGGL(near_patches<double>, grid, dim3(256), 0, stream, L.args, L.near_items,
+                               L.near_item_count, L.patches, L.rules);
+    }
+    if (L.near_count) {
+        const dim3 grid((L.near_count + 3) / 4);
+        if (L.args.A32)
+            hipLaunchKernelGGL(near_jobs<float>, grid, dim3(256), 0, stream, L.args, L.near_jobs, L.near_count,
+                               L.patches, L.rules);
+        else
+            hipLaunchKernelGGL(near_jobs<double>, grid, dim3(256), 0, stream, L.args, L.near_jobs, L.near_count,
+                               L.patches, L.rules);
+    }
 }
 
 void launch_mass(const AssemblyLaunch &L, hipStream_t stream) {

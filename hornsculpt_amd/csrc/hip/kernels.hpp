@@ -54,6 +54,12 @@ struct NearView {
     const std::uint8_t *whole;             // all-pairs degree bytes or null
 };
 
+// One tensor-product near patch of a near job (no projection fan): the pair
+// (test, trial) of the job, the patch index and the job's swap flag.
+struct NearPatchItem {
+    std::uint32_t test, trial, patch, swap;
+};
+
 struct AssemblyLaunch {
     AssemblyArgs args;
     ClusterView clusters;
@@ -63,8 +69,10 @@ struct AssemblyLaunch {
     std::uint32_t touching_count;
     const double *gauss;                   // singular Gauss rule (position, weight)
     unsigned singular_order;
-    const NearJob *near_jobs;
+    const NearJob *near_jobs;              // jobs with a projection-fan patch (per-job kernel)
     std::uint32_t near_count;
+    const NearPatchItem *near_items;       // tensor patches of the other jobs (per-patch kernel)
+    std::uint32_t near_item_count;
     const PairPatch *patches;
 };
 

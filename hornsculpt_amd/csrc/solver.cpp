@@ -272,7 +272,31 @@ std::shared_ptr<DevicePlan> MeshSolver::device_plan(const Numerics &n, double k_
     auto dp = std::make_shared<DevicePlan>();
     dp->plan = plan(n, k_max);
     dp->touching.upload(dp->plan.touching, s);
-    dp->near.upload(dp->plan.near, s);
+    // Near jobs: tensor patches go to the per-patch kernel (degree 8 first, then
+    // 14, for uniform trip counts); jobs holding a projection fan stay whole.
+    std::vector<NearJob> fan_jobs;
+    std::vector<dev::NearPatchItem> items8, items14;
+    for (const auto &j : dp->plan.near) {
+        bool fan = false;
+        for (std::uint32_t q = 0; q < j.patch_count; ++q)
+            fan |= dp->plan.patches[j.first_patch + q].projection != 0;
+        if (fan) {
+            fan_jobs.push_back(j);
+            continue;
+        }
+        for (std::uint32_t q = 0; q < j.patch_count; ++q)
+            (dp->plan.patches[j.first_patch + q].degree == 14 ? items14 : items8)
+                .push_back({j.test, j.trial, j.first_patch + q, j.swap});
+    }
+    items8.insert(items8.end(), items14.begin(), items14.end());
+    dp->near_job_count = static_cast<std::uint32_t>(fan_jobs.size());
+    dp->near_item_count = static_cast<std::uint32_t>(items8.size());
+    if (fan_jobs.empty())
+        fan_jobs.push_back({});
+    if (items8.empty())
+        items8.push_back({});
+    dp->near.upload(fan_jobs, s);
+    dp->near_items.upload(items8, s);
     dp->patches.upload(dp->plan.patches, s);
     dp->near_offsets.upload(dp->plan.near_offsets, s);
     std::vector<std::uint32_t> trials = dp->plan.near_trials;
@@ -331,7 +355,9 @@ dev::AssemblyLaunch MeshSolver::assembly_launch(const DevicePlan &p, const Numer
     L.gauss = p.gauss.data();
     L.singular_order = n.singular_order;
     L.near_jobs = p.near.data();
-    L.near_count = static_cast<std::uint32_t>(p.plan.near.size());
+    L.near_count = p.near_job_count;
+    L.near_items = p.near_items.data();
+    L.near_item_count = p.near_item_count;
     L.patches = p.patches.data();
     return L;
 }

@@ -172,7 +172,9 @@ using ProgressFn = void (*)(void *, std::uint32_t, std::uint32_t, double, std::i
 struct DevicePlan {
     InteractionPlan plan;
     DeviceBuffer<TouchingJob> touching;
-    DeviceBuffer<NearJob> near;
+    DeviceBuffer<NearJob> near;                // jobs with a projection-fan patch
+    DeviceBuffer<dev::NearPatchItem> near_items; // tensor patches of all other jobs
+    std::uint32_t near_job_count{}, near_item_count{};
     DeviceBuffer<PairPatch> patches;
     DeviceBuffer<std::uint32_t> near_offsets, near_trials;
     DeviceBuffer<std::uint8_t> whole;

@@ -74,6 +74,21 @@ def test_assembly_matches_oracle_horn_rows():
     assert _rel(B[nodes], Bo[nodes]) <= 1e-12
 
 
+def test_assembly_matches_oracle_thin_box_projection_fans():
+    """0.2 mm gap: projection-fan near patches (per-job kernel) and the all-pairs mode."""
+    V = np.array([(x, y, z) for z in (0, 0.0002) for y in (0, 0.02) for x in (0, 0.02)], float)
+    quads = [(0, 2, 3, 1), (4, 5, 7, 6), (0, 1, 5, 4), (2, 6, 7, 3), (0, 4, 6, 2), (1, 3, 7, 5)]
+    T = [t for a, b, c, d in quads for t in ((a, b, c), (a, c, d))]
+    mesh = bem.SurfaceMesh(V, np.array(T, np.uint32), np.arange(12, dtype=np.uint32), np.ones(12, np.uint8))
+    k = 2 * math.pi * 3000 / 343.0
+    num = bem.BemNumerics()
+    plan = bem.interaction_plan(mesh, num, k)
+    assert np.any(plan["patches"][:, 20] == 1)
+    A, B = bem.HipBemSolver(mesh).assemble(num, k, k)
+    Ao, Bo = ob.oracle_assemble(mesh, plan, k)
+    assert _rel(A, Ao) <= 1e-12 and _rel(B, Bo) <= 1e-12
+
+
 def test_solve_matches_oracle_and_analytic_icosphere():
     """Config 1: icosphere L3 (642 v / 1280 t), 1 kHz, rho=1.205, c=343, dense FP64 + LU."""
     mesh = meshes.icosphere(3)
