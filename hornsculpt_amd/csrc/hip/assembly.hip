@@ -154,36 +154,117 @@ __device__ __forceinline__ double g_at(const double *geo, int field, std::uint32
     return geo[static_cast<std::size_t>(field) * nt + t];
 }
 
-// Two-stage tensor contraction of the pair integral over rule x rule:
-// inner sums over trial points carry the trial basis, the outer sum the test basis.
-// The panel-origin difference is formed in FP64 before rounding to R; the rule
-// table (b0, b1, b2, w per point) is held in R.
+// Sample core of a disjoint pair (regular pairs and tensor near patches), from
+// the squared distance r2, the numerators fyn = d.n_y and dntk = (d.n_x)/k
+// already formed: same algebra as bm_sample.
+__device__ __forceinline__ SampleT<float> disjoint_sample(float r2, float wq, float fyn, float dntk, float k,
+                                                          float k_rev, float knn, bool &bad) {
+    bad |= !(r2 > 0.f) || !isfinite(r2);
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * rinv;
+    const float f = __builtin_amdgcn_fractf(r * k_rev);
+    const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
+    const float kr = k * r, scale = wq * rinv;
+    SampleT<float> s;
+    s.g = {cs * scale, -sn * scale};
+    const CplxT<float> radial = {s.g.re - s.g.im * kr, s.g.im + s.g.re * kr};
+    const float ir2 = rinv * rinv, fy = fyn * ir2, fxk = -dntk * ir2;
+    s.a = {-radial.re * fy - knn * s.g.im, -radial.im * fy + knn * s.g.re};
+    s.b = {-s.g.re - radial.im * fxk, -s.g.im + radial.re * fxk};
+    return s;
+}
+
+__device__ __forceinline__ SampleT<double> disjoint_sample(double r2, double wq, double fyn, double dntk, double k,
+                                                           double, double knn, bool &bad) {
+    bad |= !(r2 > 0.0) || !isfinite(r2);
+    const double rinv = rsqrt_full(r2);
+    const double r = r2 * rinv, kr = k * r;
+    double sn, cs;
+    sincos_reduced(kr, &sn, &cs);
+    const double scale = wq * rinv;
+    SampleT<double> s;
+    s.g = {cs * scale, -sn * scale};
+    const CplxT<double> radial = {s.g.re - s.g.im * kr, s.g.im + s.g.re * kr};
+    const double ir2 = rinv * rinv, fy = fyn * ir2, fxk = -dntk * ir2;
+    s.a = {-radial.re * fy - knn * s.g.im, -radial.im * fy + knn * s.g.re};
+    s.b = {-s.g.re - radial.im * fxk, -s.g.im + radial.re * fxk};
+    return s;
+}
+
+// Flat panel frame: vertex 0, edges e1 = v1 - v0, e2 = v2 - v0, unit normal (FP64).
+struct Frame {
+    double v0[3], e1[3], e2[3], n[3];
+};
+
+__device__ __forceinline__ Frame frame_of(const double *g) {
+    Frame f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        f.v0[i] = g[G_V0 + i];
+        f.e1[i] = g[G_E1 + i];
+        f.e2[i] = g[G_E2 + i];
+        f.n[i] = g[G_N + i];
+    }
+    return f;
+}
+
+__device__ __forceinline__ double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// Two-stage tensor contraction of a disjoint pair over rule x rule (test point
+// x, trial point y, barycentrics and weight per rule row): the inner sum over
+// trial points carries the trial basis, the outer sum the test basis. With
+// d = x - y = b + l_x - l_y (b = v0_x - v0_y, l in each panel's plane):
+//   |d|^2 = |b + l_x|^2 - 2 (b + l_x).l_y + |l_y|^2, expanded in y1, y2;
+//   d.n_y = (b + l_x).n_y (l_y is orthogonal to n_y): constant in the inner loop;
+//   d.n_x = b.n_x - l_y.n_x (l_x is orthogonal to n_x).
+// Pair constants are formed in FP64 and rounded to R; wscale = 4 A_x A_y / (4 pi)
+// (times the area fraction of a near patch).
 template <class R>
-__device__ __forceinline__ void integrate_tensor(MomentsT<R> &m, const double *gt, const double *gs, const R *rule,
-                                                 int n, double k, bool &bad) {
-    const PairConst<R> c = pair_const<R>(gt, gs, k);
-    const R wscale = static_cast<R>(4.0 * gt[G_AREA] * gs[G_AREA] * 0.07957747154594766788); // 4 At As / (4 pi)
-    const R bx = static_cast<R>(gt[G_V0] - gs[G_V0]), by = static_cast<R>(gt[G_V0 + 1] - gs[G_V0 + 1]),
-            bz = static_cast<R>(gt[G_V0 + 2] - gs[G_V0 + 2]);
-    const R te1x = gt[G_E1], te1y = gt[G_E1 + 1], te1z = gt[G_E1 + 2], te2x = gt[G_E2], te2y = gt[G_E2 + 1],
-            te2z = gt[G_E2 + 2];
-    const R se1x = gs[G_E1], se1y = gs[G_E1 + 1], se1z = gs[G_E1 + 2], se2x = gs[G_E2], se2y = gs[G_E2 + 1],
-            se2z = gs[G_E2 + 2];
+__device__ __forceinline__ void contract(MomentsT<R> &m, const Frame &t, const Frame &s, double wscale_d,
+                                         const R *rule, int n, double k, bool &bad) {
+    const double b[3] = {t.v0[0] - s.v0[0], t.v0[1] - s.v0[1], t.v0[2] - s.v0[2]};
+    const double inv_k = 1.0 / k;
+    const R Q11 = static_cast<R>(dot3(s.e1, s.e1)), Q12 = static_cast<R>(2.0 * dot3(s.e1, s.e2)),
+            Q22 = static_cast<R>(dot3(s.e2, s.e2));
+    const R A1 = static_cast<R>(dot3(s.e1, t.n) * inv_k), A2 = static_cast<R>(dot3(s.e2, t.n) * inv_k),
+            C0 = static_cast<R>(dot3(b, t.n) * inv_k);
+    const R T1 = static_cast<R>(dot3(t.e1, s.n)), T2 = static_cast<R>(dot3(t.e2, s.n)),
+            Bn = static_cast<R>(dot3(b, s.n));
+    const R kk = static_cast<R>(k), k_rev = static_cast<R>(k * 0.15915494309189533577),
+            knn = static_cast<R>(k * dot3(t.n, s.n)), wscale = static_cast<R>(wscale_d);
+    R bb[3], te1[3], te2[3], se1[3], se2[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        bb[i] = static_cast<R>(b[i]);
+        te1[i] = static_cast<R>(t.e1[i]);
+        te2[i] = static_cast<R>(t.e2[i]);
+        se1[i] = static_cast<R>(s.e1[i]);
+        se2[i] = static_cast<R>(s.e2[i]);
+    }
     for (int xi = 0; xi < n; ++xi) {
         const R *xp = rule + 4 * xi;
         const R x1 = xp[1], x2 = xp[2], xw = xp[3] * wscale;
-        const R ltx = te1x * x1 + te2x * x2, lty = te1y * x1 + te2y * x2, ltz = te1z * x1 + te2z * x2;
+        R blt[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            blt[i] = bb[i] + (te1[i] * x1 + te2[i] * x2);
+        const R P1 = R(-2) * (blt[0] * se1[0] + blt[1] * se1[1] + blt[2] * se1[2]);
+        const R P2 = R(-2) * (blt[0] * se2[0] + blt[1] * se2[1] + blt[2] * se2[2]);
+        const R B = blt[0] * blt[0] + blt[1] * blt[1] + blt[2] * blt[2];
+        const R fyn = Bn + x1 * T1 + x2 * T2;
         CplxT<R> ga[3] = {{0, 0}, {0, 0}, {0, 0}}, gg = {0, 0}, gb = {0, 0};
         for (int yi = 0; yi < n; ++yi) {
             const R *yp = rule + 4 * yi;
             const R y0 = yp[0], y1 = yp[1], y2 = yp[2];
-            const R lsx = se1x * y1 + se2x * y2, lsy = se1y * y1 + se2y * y2, lsz = se1z * y1 + se2z * y2;
-            const SampleT<R> s = sample(bx + (ltx - lsx), by + (lty - lsy), bz + (ltz - lsz), xw * yp[3], c, false, bad);
-            ga[0] = cadd_t(ga[0], cscale_t(s.a, y0));
-            ga[1] = cadd_t(ga[1], cscale_t(s.a, y1));
-            ga[2] = cadd_t(ga[2], cscale_t(s.a, y2));
-            gg = cadd_t(gg, s.g);
-            gb = cadd_t(gb, s.b);
+            const R t1 = y1 * Q11 + (y2 * Q12 + P1), t2 = y2 * Q22 + P2;
+            const R r2 = y1 * t1 + (y2 * t2 + B);
+            const R dntk = C0 - (y1 * A1 + y2 * A2);
+            const SampleT<R> smp = disjoint_sample(r2, xw * yp[3], fyn, dntk, kk, k_rev, knn, bad);
+            ga[0] = cadd_t(ga[0], cscale_t(smp.a, y0));
+            ga[1] = cadd_t(ga[1], cscale_t(smp.a, y1));
+            ga[2] = cadd_t(ga[2], cscale_t(smp.a, y2));
+            gg = cadd_t(gg, smp.g);
+            gb = cadd_t(gb, smp.b);
         }
         m.g = cadd_t(m.g, gg);
 #pragma unroll
@@ -197,63 +278,46 @@ __device__ __forceinline__ void integrate_tensor(MomentsT<R> &m, const double *g
     }
 }
 
-// Tensor near patch: test points x of the rule mapped into the test sub-triangle
-// with barycentric vertices Tt (bary_parent = sum_j x_j Tt[j], map_patch_point),
-// trial points likewise with Ts; weight w_x w_y times the patch area fraction.
-// Same contraction as integrate_tensor; maps in FP64, rounded to R.
-template <class R>
-__device__ __forceinline__ void integrate_patch(MomentsT<R> &m, const double *gt, const double *gs, const double *Tt,
-                                                const double *Ts, double area_fraction, const R *rule, int n, double k,
-                                                bool &bad) {
-    const PairConst<R> c = pair_const<R>(gt, gs, k);
-    const R wscale = static_cast<R>(4.0 * gt[G_AREA] * gs[G_AREA] * 0.07957747154594766788 * area_fraction);
-    const R bx = static_cast<R>(gt[G_V0] - gs[G_V0]), by = static_cast<R>(gt[G_V0 + 1] - gs[G_V0 + 1]),
-            bz = static_cast<R>(gt[G_V0 + 2] - gs[G_V0 + 2]);
-    // sub-triangle maps as origin + b1 d1 + b2 d2 on the parent barycentrics
-    R to[3], t1[3], t2[3], so[3], s1[3], s2[3];
+// Sub-triangle of a panel with vertices at parent barycentrics T[l] (3 x 3,
+// row l = vertex l): frame of the physical sub-triangle.
+__device__ __forceinline__ Frame sub_frame(const Frame &p, const double *T) {
+    Frame f;
+    f.n[0] = p.n[0];
+    f.n[1] = p.n[1];
+    f.n[2] = p.n[2];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        to[i] = static_cast<R>(Tt[i]);
-        t1[i] = static_cast<R>(Tt[3 + i] - Tt[i]);
-        t2[i] = static_cast<R>(Tt[6 + i] - Tt[i]);
-        so[i] = static_cast<R>(Ts[i]);
-        s1[i] = static_cast<R>(Ts[3 + i] - Ts[i]);
-        s2[i] = static_cast<R>(Ts[6 + i] - Ts[i]);
+        f.v0[i] = p.v0[i] + p.e1[i] * T[1] + p.e2[i] * T[2];
+        f.e1[i] = p.e1[i] * (T[4] - T[1]) + p.e2[i] * (T[5] - T[2]);
+        f.e2[i] = p.e1[i] * (T[7] - T[1]) + p.e2[i] * (T[8] - T[2]);
     }
-    const R te1x = gt[G_E1], te1y = gt[G_E1 + 1], te1z = gt[G_E1 + 2], te2x = gt[G_E2], te2y = gt[G_E2 + 1],
-            te2z = gt[G_E2 + 2];
-    const R se1x = gs[G_E1], se1y = gs[G_E1 + 1], se1z = gs[G_E1 + 2], se2x = gs[G_E2], se2y = gs[G_E2 + 1],
-            se2z = gs[G_E2 + 2];
-    for (int xi = 0; xi < n; ++xi) {
-        const R *xp = rule + 4 * xi;
-        R xt[3];
+    return f;
+}
+
+// Moments over local sub-triangle barycentrics -> parent barycentrics:
+// a[i][j] = sum_l,q Tt[l][i] Ts[q][j] a_local[l][q], b[i] = sum_l Tt[l][i] b_local[l]
+// (map_patch_point: parent bary = sum_l local_l T[l]).
+template <class R> __device__ __forceinline__ void to_parent(MomentsT<R> &m, const double *Tt, const double *Ts) {
+    MomentsT<R> o;
+    o.g = m.g;
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
-            xt[i] = to[i] + xp[1] * t1[i] + xp[2] * t2[i];
-        const R xw = xp[3] * wscale;
-        const R ltx = te1x * xt[1] + te2x * xt[2], lty = te1y * xt[1] + te2y * xt[2], ltz = te1z * xt[1] + te2z * xt[2];
-        CplxT<R> ga[3] = {{0, 0}, {0, 0}, {0, 0}}, gg = {0, 0}, gb = {0, 0};
-        for (int yi = 0; yi < n; ++yi) {
-            const R *yp = rule + 4 * yi;
-            const R y1 = so[1] + yp[1] * s1[1] + yp[2] * s2[1], y2 = so[2] + yp[1] * s1[2] + yp[2] * s2[2];
-            const R y0 = so[0] + yp[1] * s1[0] + yp[2] * s2[0];
-            const R lsx = se1x * y1 + se2x * y2, lsy = se1y * y1 + se2y * y2, lsz = se1z * y1 + se2z * y2;
-            const SampleT<R> s = sample(bx + (ltx - lsx), by + (lty - lsy), bz + (ltz - lsz), xw * yp[3], c, false, bad);
-            ga[0] = cadd_t(ga[0], cscale_t(s.a, y0));
-            ga[1] = cadd_t(ga[1], cscale_t(s.a, y1));
-            ga[2] = cadd_t(ga[2], cscale_t(s.a, y2));
-            gg = cadd_t(gg, s.g);
-            gb = cadd_t(gb, s.b);
-        }
-        m.g = cadd_t(m.g, gg);
+    for (int i = 0; i < 3; ++i) {
+        o.b[i] = {0, 0};
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            m.b[i] = cadd_t(m.b[i], cscale_t(gb, xt[i]));
+        for (int l = 0; l < 3; ++l)
+            o.b[i] = cadd_t(o.b[i], cscale_t(m.b[l], static_cast<R>(Tt[3 * l + i])));
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
-                m.a[3 * i + j] = cadd_t(m.a[3 * i + j], cscale_t(ga[j], xt[i]));
+        for (int j = 0; j < 3; ++j) {
+            CplxT<R> v = {0, 0};
+#pragma unroll
+            for (int l = 0; l < 3; ++l)
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                    v = cadd_t(v, cscale_t(m.a[3 * l + q], static_cast<R>(Tt[3 * l + i] * Ts[3 * q + j])));
+            o.a[3 * i + j] = v;
         }
     }
+    m = o;
 }
 
 template <class M> __device__ __forceinline__ void atomic_add_entry(M *p, double re, double im) {
@@ -382,7 +446,8 @@ void regular_tiles(AssemblyArgs args, ClusterView cl, NearView near, const doubl
         }
         MomentsT<R> m;
         zero_t(m);
-        integrate_tensor<R>(m, gt, gs, srule + 4 * offset, count, k, bad);
+        contract<R>(m, frame_of(gt), frame_of(gs), 4.0 * gt[G_AREA] * gs[G_AREA] * 0.07957747154594766788,
+                    srule + 4 * offset, count, k, bad);
         counted += static_cast<unsigned long long>(count) * count;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -616,8 +681,11 @@ __global__ __launch_bounds__(256) void near_patches(AssemblyArgs args, const Nea
     MomentsT<R> m;
     zero_t(m);
     bool bad = false;
-    integrate_patch<R>(m, gt, gs, it.swap ? patch.trial : patch.test, it.swap ? patch.test : patch.trial,
-                       patch.area_fraction, srule + (high ? 4 * 16 : 0), high ? 64 : 16, args.k, bad);
+    const double *Tt = it.swap ? patch.trial : patch.test, *Ts = it.swap ? patch.test : patch.trial;
+    contract<R>(m, sub_frame(frame_of(gt), Tt), sub_frame(frame_of(gs), Ts),
+                4.0 * gt[G_AREA] * gs[G_AREA] * 0.07957747154594766788 * patch.area_fraction,
+                srule + (high ? 4 * 16 : 0), high ? 64 : 16, args.k, bad);
+    to_parent(m, Tt, Ts);
     if (bad)
         atomicExch(args.error, ERR_DISTANCE);
     scatter_pair(args, matrix_of<R>(args), m, it.test, it.trial);
