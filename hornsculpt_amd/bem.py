@@ -26,6 +26,7 @@ _LIB_PATH = Path(__file__).resolve().parent / "libhsbem.so"
 HSBEM_OK, HSBEM_INVALID_ARGUMENT, HSBEM_RUNTIME_ERROR, HSBEM_DEVICE_ERROR, HSBEM_CANCELLED, HSBEM_PENDING = range(6)
 SOLVER_LU, SOLVER_GMRES = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+PRECONDITIONER_NONE, PRECONDITIONER_BLOCK_JACOBI = 0, 1
 WALL, DRIVEN = 0, 1
 
 
@@ -43,6 +44,7 @@ class _Numerics(ctypes.Structure):
         ("gmres_maximum_iterations", ctypes.c_uint32),
         ("precision", ctypes.c_uint32),
         ("plan_threads", ctypes.c_uint32),
+        ("preconditioner", ctypes.c_uint32),
     ]
 
 
@@ -199,6 +201,7 @@ class BemNumerics:
     gmres_maximum_iterations: int = 1000
     precision: int = PRECISION_FP64
     plan_threads: int = 0
+    preconditioner: int = PRECONDITIONER_BLOCK_JACOBI   # GMRES only
 
     def native(self) -> _Numerics:
         n = _Numerics()

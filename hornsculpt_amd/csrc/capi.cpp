@@ -72,6 +72,7 @@ hsbem::Numerics to_numerics(const hsbem_numerics &n) {
     o.gmres_max_iterations = n.gmres_maximum_iterations;
     o.precision = n.precision;
     o.plan_threads = n.plan_threads;
+    o.preconditioner = n.preconditioner;
     return o;
 }
 
@@ -143,6 +144,7 @@ void hsbem_numerics_default(hsbem_numerics *n) {
     n->gmres_maximum_iterations = 1000;
     n->precision = HSBEM_PRECISION_FP64;
     n->plan_threads = 0;
+    n->preconditioner = HSBEM_PRECONDITIONER_BLOCK_JACOBI;
 }
 
 const char *hsbem_last_error(void) { return last_error.c_str(); }
@@ -523,7 +525,7 @@ int hsbem_dense_gmres(hsbem_context *c, uint32_t n, const double *A, const doubl
         double achieved = 0;
         std::uint32_t its = 0;
         hsbem::dense_gmres(s, dA.data(), false, n, db.data(), dx.data(), restart, maximum_iterations, tolerance,
-                           &achieved, &its, nullptr, ws);
+                           &achieved, &its, nullptr, ws, nullptr);
         hsbem::hip_check(hipMemcpyAsync(x, dx.data(), dx.bytes(), hipMemcpyDeviceToHost, s), "x download");
         hsbem::hip_check(hipStreamSynchronize(s), "GMRES sync");
         if (relative_residual)

@@ -112,12 +112,19 @@ void lu_solve(const double *LU, std::uint32_t n, const std::int32_t *pivots, dou
               hipStream_t stream);
 
 // r = b - A x, scaled by weight (lumped mass) into weighted; r unscaled into residual.
+// Block-Jacobi preconditioner of 64 x 64 diagonal blocks (consecutive unknowns):
+// explicit FP64 inverses per block (LU with partial pivoting), and out = M^{-1} v.
+std::size_t jacobi_factor_doubles(std::uint32_t n);
+void jacobi_factor(const void *A, bool fp32, std::uint32_t n, double *inverses, hipStream_t stream);
+void jacobi_apply(const double *inverses, std::uint32_t n, const double *v, double *out, hipStream_t stream);
 void launch_residual(const void *A, bool fp32, std::uint32_t n, const double *b, const double *x, std::uint32_t nvec,
                      double *residual, double *scratch, hipStream_t stream);
 
 // Elementwise helpers.
 void launch_axpy(std::uint32_t n, const double *alpha, const double *x, double *y, hipStream_t stream);
 // y = s * x (complex vector of length n, real s).
+// y += x (n complex)
+void launch_add(std::uint32_t n, const double *x, double *y, hipStream_t stream);
 void launch_scale(std::uint32_t n, double s, const double *x, double *y, hipStream_t stream);
 
 // ---- fields (field.hip) ----

@@ -203,6 +203,13 @@ __global__ void axpy_kernel(std::uint32_t n, const double *alpha, const double *
     y[2 * r + 1] += ar * xi + ai * xr;
 }
 
+__global__ void add_kernel(std::uint32_t n, const double *x, double *y) {
+    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * n)
+        return;
+    y[i] += x[i];
+}
+
 __global__ void scale_kernel(std::uint32_t n, double s, const double *x, double *y) {
     const std::uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n)
@@ -494,6 +501,159 @@ __global__ __launch_bounds__(256) void block_eliminate(const double *LU, std::ui
     out[1] -= im;
 }
 
+// ---- block-Jacobi preconditioner: explicit inverses of the kJacobi x kJacobi
+// diagonal blocks (consecutive internal nodes, i.e. spatial clusters) of the
+// row-scaled system, one workgroup per block: LU with partial pivoting in LDS,
+// then the inverse column by column (one thread per column of the identity,
+// forward and back substitution against the shared factors). FP64, column-major
+// per block. A block whose pivot vanishes is replaced by the identity. The apply
+// is then a block-diagonal matrix-vector product.
+constexpr int kJacobi = 64;
+
+template <class T>
+__global__ __launch_bounds__(256) void jacobi_factor(const T *__restrict__ A, std::uint32_t n, double *inverses) {
+    __shared__ double a[kJacobi][kJacobi][2]; // LU, [col][row]
+    __shared__ double x[kJacobi][kJacobi][2]; // inverse, [row][col] (thread = column: conflict free)
+    __shared__ std::int32_t p[kJacobi];
+    __shared__ int pivot_row, singular;
+    const std::uint32_t blk = blockIdx.x, r0 = blk * kJacobi;
+    const int m = static_cast<int>(min(static_cast<std::uint32_t>(kJacobi), n - r0));
+    const int tid = threadIdx.x;
+    for (int e = tid; e < kJacobi * kJacobi; e += 256) {
+        const int c = e / kJacobi, r = e % kJacobi;
+        double re = 0, im = 0;
+        if (c < m && r < m) {
+            const T v = A[static_cast<std::size_t>(r0 + c) * n + r0 + r];
+            re = v.x;
+            im = v.y;
+        } else if (c == r) {
+            re = 1;
+        }
+        a[c][r][0] = re;
+        a[c][r][1] = im;
+    }
+    if (tid < kJacobi)
+        p[tid] = tid;
+    if (tid == 0)
+        singular = 0;
+    __syncthreads();
+    for (int j = 0; j < m; ++j) {
+        if (tid < 64) { // pivot search over rows j..m-1 of column j (wavefront 0)
+            const int r = tid;
+            double mag = r >= j && r < m ? a[j][r][0] * a[j][r][0] + a[j][r][1] * a[j][r][1] : -1.0;
+            int idx = r;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double om = __shfl_xor(mag, off, 64);
+                const int oi = __shfl_xor(idx, off, 64);
+                if (om > mag || (om == mag && oi < idx)) {
+                    mag = om;
+                    idx = oi;
+                }
+            }
+            if (tid == 0) {
+                pivot_row = idx;
+                if (!(mag > 0))
+                    singular = 1;
+            }
+        }
+        __syncthreads();
+        if (singular)
+            break;
+        const int pr = pivot_row;
+        if (pr != j) {
+            for (int c = tid; c < kJacobi; c += 256) {
+                const double t0 = a[c][j][0], t1 = a[c][j][1];
+                a[c][j][0] = a[c][pr][0];
+                a[c][j][1] = a[c][pr][1];
+                a[c][pr][0] = t0;
+                a[c][pr][1] = t1;
+            }
+            if (tid == 0) {
+                const int t = p[j];
+                p[j] = p[pr];
+                p[pr] = t;
+            }
+        }
+        __syncthreads();
+        const double dr = a[j][j][0], di = a[j][j][1], inv = 1.0 / (dr * dr + di * di);
+        for (int r = j + 1 + tid; r < m; r += 256) {
+            const double xr = a[j][r][0], xi = a[j][r][1];
+            a[j][r][0] = (xr * dr + xi * di) * inv;
+            a[j][r][1] = (xi * dr - xr * di) * inv;
+        }
+        __syncthreads();
+        const int w = m - j - 1;
+        for (int e = tid; e < w * w; e += 256) {
+            const int c = j + 1 + e / w, r = j + 1 + e % w;
+            const double lr = a[j][r][0], li = a[j][r][1], ur = a[c][j][0], ui = a[c][j][1];
+            a[c][r][0] -= lr * ur - li * ui;
+            a[c][r][1] -= lr * ui + li * ur;
+        }
+        __syncthreads();
+    }
+    // inverse: column c of B^{-1} solves L U x = P e_c; thread c < m owns column c
+    if (!singular && tid < m) {
+        const int c = tid;
+        for (int i = 0; i < m; ++i) { // y = P e_c
+            x[i][c][0] = p[i] == c ? 1.0 : 0.0;
+            x[i][c][1] = 0.0;
+        }
+        for (int j = 0; j < m; ++j) { // unit lower
+            const double yr = x[j][c][0], yi = x[j][c][1];
+            for (int i = j + 1; i < m; ++i) {
+                const double lr = a[j][i][0], li = a[j][i][1];
+                x[i][c][0] -= lr * yr - li * yi;
+                x[i][c][1] -= lr * yi + li * yr;
+            }
+        }
+        for (int j = m - 1; j >= 0; --j) { // upper
+            const double dr = a[j][j][0], di = a[j][j][1], inv = 1.0 / (dr * dr + di * di);
+            const double xr = x[j][c][0], xi = x[j][c][1];
+            const double vr = (xr * dr + xi * di) * inv, vi = (xi * dr - xr * di) * inv;
+            x[j][c][0] = vr;
+            x[j][c][1] = vi;
+            for (int i = 0; i < j; ++i) {
+                const double ur = a[j][i][0], ui = a[j][i][1];
+                x[i][c][0] -= ur * vr - ui * vi;
+                x[i][c][1] -= ur * vi + ui * vr;
+            }
+        }
+    }
+    __syncthreads();
+    double *f = inverses + static_cast<std::size_t>(blk) * kJacobi * kJacobi * 2;
+    for (int e = tid; e < kJacobi * kJacobi; e += 256) {
+        const int c = e / kJacobi, r = e % kJacobi;
+        const bool id = singular != 0 || c >= m || r >= m;
+        f[2 * e] = id ? (c == r ? 1.0 : 0.0) : x[r][c][0];
+        f[2 * e + 1] = id ? 0.0 : x[r][c][1];
+    }
+}
+
+// out = M^{-1} v: one wavefront per block, lane = row, columns streamed.
+__global__ __launch_bounds__(64) void jacobi_apply(const double *inverses, std::uint32_t n, const double *v,
+                                                   double *out) {
+    __shared__ double sv[kJacobi][2];
+    const std::uint32_t blk = blockIdx.x, r0 = blk * kJacobi;
+    const int m = static_cast<int>(min(static_cast<std::uint32_t>(kJacobi), n - r0));
+    const int i = threadIdx.x;
+    sv[i][0] = i < m ? v[2 * (r0 + i)] : 0.0;
+    sv[i][1] = i < m ? v[2 * (r0 + i) + 1] : 0.0;
+    __syncthreads();
+    const double2 *f = reinterpret_cast<const double2 *>(inverses) + static_cast<std::size_t>(blk) * kJacobi * kJacobi;
+    double re = 0, im = 0;
+#pragma unroll 8
+    for (int c = 0; c < kJacobi; ++c) {
+        const double2 b = f[c * kJacobi + i];
+        re += b.x * sv[c][0] - b.y * sv[c][1];
+        im += b.x * sv[c][1] + b.y * sv[c][0];
+    }
+    if (i < m) {
+        out[2 * (r0 + i)] = re;
+        out[2 * (r0 + i) + 1] = im;
+    }
+}
+
 } // namespace
 
 std::size_t gemv_scratch_doubles(std::uint32_t n, std::uint32_t nvec) {
@@ -558,6 +718,10 @@ void launch_norm2(const double *x, std::uint32_t n, const double *weight, double
     hipLaunchKernelGGL(norm_finish, dim3(1), dim3(64), 0, stream, scratch, blocks, out);
 }
 
+void launch_add(std::uint32_t n, const double *x, double *y, hipStream_t stream) {
+    hipLaunchKernelGGL(add_kernel, dim3((2 * n + 255) / 256), dim3(256), 0, stream, n, x, y);
+}
+
 void launch_axpy(std::uint32_t n, const double *alpha, const double *x, double *y, hipStream_t stream) {
     hipLaunchKernelGGL(axpy_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, alpha, x, y);
 }
@@ -609,6 +773,24 @@ void lu_solve(const double *LU, std::uint32_t n, const std::int32_t *piv, double
             hipLaunchKernelGGL(block_eliminate, dim3((static_cast<std::uint32_t>(j0) + 255) / 256, nrhs), dim3(256), 0,
                                stream, LU, n, B, static_cast<std::uint32_t>(j0), nb, 0u, static_cast<std::uint32_t>(j0));
     }
+}
+
+std::size_t jacobi_factor_doubles(std::uint32_t n) {
+    return static_cast<std::size_t>((n + kJacobi - 1) / kJacobi) * kJacobi * kJacobi * 2;
+}
+
+void jacobi_factor(const void *A, bool fp32, std::uint32_t n, double *inverses, hipStream_t stream) {
+    const dim3 grid((n + kJacobi - 1) / kJacobi);
+    if (fp32)
+        hipLaunchKernelGGL(jacobi_factor<float2>, grid, dim3(256), 0, stream, static_cast<const float2 *>(A), n,
+                           inverses);
+    else
+        hipLaunchKernelGGL(jacobi_factor<double2>, grid, dim3(256), 0, stream, static_cast<const double2 *>(A), n,
+                           inverses);
+}
+
+void jacobi_apply(const double *inverses, std::uint32_t n, const double *v, double *out, hipStream_t stream) {
+    hipLaunchKernelGGL(jacobi_apply, dim3((n + kJacobi - 1) / kJacobi), dim3(64), 0, stream, inverses, n, v, out);
 }
 
 } // namespace hsbem::dev

@@ -135,6 +135,8 @@ void Workspace::release() {
         b->release();
     flux_column.release();
     pivots.release();
+    jacobi.release();
+    gmres.z.release();
     gmres.basis.release();
     gmres.r.release();
     gmres.scratch.release();
@@ -146,7 +148,7 @@ std::uint64_t Workspace::capacity_bytes() const {
     std::uint64_t total = 0;
     for (const auto *b : {&A, &LU, &BD, &q, &rhs, &x, &r, &flux, &scratch, &obs, &field, &partials})
         total += b->capacity_bytes();
-    return total + flux_column.capacity_bytes() + pivots.capacity_bytes() + gmres.basis.capacity_bytes() +
+    return total + flux_column.capacity_bytes() + pivots.capacity_bytes() + jacobi.capacity_bytes() + gmres.z.capacity_bytes() + gmres.basis.capacity_bytes() +
            gmres.r.capacity_bytes() + gmres.scratch.capacity_bytes() + gmres.small.capacity_bytes();
 }
 
@@ -415,7 +417,7 @@ void MeshSolver::assemble_dense(const Numerics &n, double k_max, double k, doubl
 void dense_gmres(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const double *b, double *x,
                  std::uint32_t restart,
                  std::uint32_t max_it, double tol, double *achieved, std::uint32_t *iterations,
-                 const volatile std::int32_t *cancel, GmresWorkspace &ws) {
+                 const volatile std::int32_t *cancel, GmresWorkspace &ws, const double *jacobi) {
     using C = std::complex<double>;
     auto &basis_ = ws.basis;
     auto &r_ = ws.r;
@@ -424,6 +426,8 @@ void dense_gmres(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const
         basis_.resize(2 * static_cast<std::size_t>(n) * (restart + 1));
     if (r_.size() < 2 * static_cast<std::size_t>(n))
         r_.resize(2 * static_cast<std::size_t>(n));
+    if (jacobi && ws.z.size() < 2 * static_cast<std::size_t>(n))
+        ws.z.resize(2 * static_cast<std::size_t>(n));
     const std::size_t scratch = std::max(dev::gemv_scratch_doubles(n, 1), dev::reduction_scratch_doubles(n, restart + 1));
     if (scratch_.size() < scratch)
         scratch_.resize(scratch);
@@ -470,7 +474,12 @@ void dense_gmres(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const
             if (cancel && *cancel)
                 throw Cancelled();
             double *w = basis_.data() + 2 * static_cast<std::size_t>(j + 1) * n;
-            dev::launch_gemv(A, fp32, n, basis_.data() + 2 * static_cast<std::size_t>(j) * n, w, 1, scratch_.data(), s);
+            const double *vj = basis_.data() + 2 * static_cast<std::size_t>(j) * n;
+            if (jacobi) { // w = A M^{-1} v_j
+                dev::jacobi_apply(jacobi, n, vj, ws.z.data(), s);
+                vj = ws.z.data();
+            }
+            dev::launch_gemv(A, fp32, n, vj, w, 1, scratch_.data(), s);
             dev::launch_project(basis_.data(), n, j + 1, w, h1, scratch_.data(), s);
             dev::launch_subtract_combination(basis_.data(), n, j + 1, h1, w, s);
             dev::launch_project(basis_.data(), n, j + 1, w, h2, scratch_.data(), s);
@@ -522,7 +531,14 @@ void dense_gmres(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const
             neg[2 * j + 1] = -y[j].imag();
         }
         hip_check(hipMemcpyAsync(h1, neg.data(), sizeof(double) * neg.size(), hipMemcpyHostToDevice, s), "y upload");
-        dev::launch_subtract_combination(basis_.data(), n, used, h1, x, s);
+        if (jacobi) { // x += M^{-1} V y
+            hip_check(hipMemsetAsync(r_.data(), 0, 2 * sizeof(double) * n, s), "gmres clear");
+            dev::launch_subtract_combination(basis_.data(), n, used, h1, r_.data(), s);
+            dev::jacobi_apply(jacobi, n, r_.data(), ws.z.data(), s);
+            dev::launch_add(n, ws.z.data(), x, s);
+        } else {
+            dev::launch_subtract_combination(basis_.data(), n, used, h1, x, s);
+        }
         dev::launch_residual(A, fp32, n, b, x, 1, r_.data(), scratch_.data(), s);
         beta = norm(r_.data());
     }
@@ -543,6 +559,7 @@ std::shared_ptr<PreparedRequest> MeshSolver::prepare(const Request &req) const {
         !std::isfinite(num.near_tolerance) || num.near_tolerance < 1e-10 || num.near_tolerance > .01 ||
         num.near_depth > 10 || !std::isfinite(num.residual_tolerance) || num.residual_tolerance < 1e-12 ||
         num.residual_tolerance > .01 || num.refinement_steps > 10 || num.solver > 1 || num.precision > 1 ||
+        num.preconditioner > 1 ||
         num.gmres_restart < 2 || num.gmres_restart > 300 || num.gmres_max_iterations < 1)
         throw std::invalid_argument("Invalid BEM medium, frequencies, excitation or Galerkin settings.");
     if (req.excitations.size() > 1024 || req.observers.size() > 100000)
@@ -616,7 +633,8 @@ std::shared_ptr<PreparedRequest> MeshSolver::prepare(const Request &req) const {
     if (lu)
         bytes += 16 * nv2 + 4ull * nv;
     else
-        bytes += 16ull * nv * (num.gmres_restart + 1) + 16ull * nv + 8 * scratch + 8ull * (4 * (num.gmres_restart + 2) + 8);
+        bytes += 16ull * nv * (num.gmres_restart + 2) + 16ull * nv + 8 * scratch + 8ull * (4 * (num.gmres_restart + 2) + 8) +
+                 (num.preconditioner ? 8 * dev::jacobi_factor_doubles(nv) : 0);
     if (nobs)
         bytes += 24 * nobs + 16 * nobs * ne + 8 * dev::field_partial_doubles(nt, static_cast<std::uint32_t>(nobs), ne);
     P->footprint = bytes;
@@ -816,11 +834,21 @@ FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_
         hip_check(hipStreamSynchronize(s), "solve sync");
         fr.ms[3] = ms_since(t_t);
     } else {
+        const bool jacobi = num.preconditioner == 1;
+        if (jacobi) {
+            const auto t_f = Clock::now();
+            ws.jacobi.resize(dev::jacobi_factor_doubles(nv));
+            dev::jacobi_factor(ws.A.data(), mixed, nv, ws.jacobi.data(), s);
+            hip_check(hipGetLastError(), "block-Jacobi factor");
+            hip_check(hipStreamSynchronize(s), "block-Jacobi sync");
+            fr.ms[2] = ms_since(t_f);
+        }
         for (std::uint32_t e = 0; e < ne; ++e) {
             double achieved = 0;
             dense_gmres(s, ws.A.data(), mixed, nv, ws.rhs.data() + 2 * static_cast<std::size_t>(e) * nv,
                         ws.x.data() + 2 * static_cast<std::size_t>(e) * nv, num.gmres_restart, num.gmres_max_iterations,
-                        num.residual_tolerance * 0.1, &achieved, &iterations[e], cancel, ws.gmres);
+                        num.residual_tolerance * 0.1, &achieved, &iterations[e], cancel, ws.gmres,
+                        jacobi ? ws.jacobi.data() : nullptr);
         }
     }
     // FP64 residual of the unscaled equation; LU corrections while above tolerance

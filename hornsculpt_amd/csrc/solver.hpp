@@ -90,6 +90,7 @@ struct Numerics {
     unsigned gmres_restart{50}, gmres_max_iterations{1000};
     unsigned precision{0};
     unsigned plan_threads{0};
+    unsigned preconditioner{1}; // GMRES: 0 none, 1 block-Jacobi (64 x 64 diagonal blocks)
 };
 
 struct Excitation {
@@ -137,7 +138,7 @@ struct Result {
 };
 
 struct GmresWorkspace {
-    DeviceBuffer<double> basis, r, scratch, small;
+    DeviceBuffer<double> basis, r, z, scratch, small;
 };
 
 // Device buffers and stream of one solve in flight. The direct solve path owns
@@ -154,6 +155,7 @@ struct Workspace {
     bool owns_stream{};
     DeviceBuffer<double> A, LU, BD, q, rhs, x, r, flux, scratch, obs, field, partials;
     DeviceBuffer<std::int32_t> flux_column, pivots;
+    DeviceBuffer<double> jacobi;
     DeviceBuffer<int> error;
     DeviceBuffer<unsigned long long> samples;
     GmresWorkspace gmres;
@@ -161,11 +163,12 @@ struct Workspace {
     std::uint64_t bound{}; // id of the PreparedRequest whose per-request data is uploaded
 };
 
-// Restarted GMRES(restart) with CGS2 Arnoldi on a dense row-major-free complex
-// system (A n x n, column-major); converged when ||b - A x|| <= tol ||b||.
+// Restarted GMRES(restart) with CGS2 Arnoldi on a dense complex system (A n x n,
+// column-major, FP64 or FP32), right-preconditioned by the block-Jacobi inverses
+// when given; converged when ||b - A x|| <= tol ||b||.
 void dense_gmres(hipStream_t stream, const void *A, bool fp32, std::uint32_t n, const double *b, double *x,
                  std::uint32_t restart, std::uint32_t max_it, double tol, double *achieved, std::uint32_t *iterations,
-                 const volatile std::int32_t *cancel, GmresWorkspace &ws);
+                 const volatile std::int32_t *cancel, GmresWorkspace &ws, const double *jacobi);
 
 using ProgressFn = void (*)(void *, std::uint32_t, std::uint32_t, double, std::int32_t);
 

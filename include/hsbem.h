@@ -44,6 +44,9 @@ typedef enum {
 
 enum { HSBEM_SOLVER_LU = 0, HSBEM_SOLVER_GMRES = 1 };
 enum { HSBEM_PRECISION_FP64 = 0, HSBEM_PRECISION_MIXED = 1 };
+/* Right preconditioner of the dense GMRES: LU of the 64 x 64 diagonal blocks of
+ * the row-scaled system in the spatial (cluster) node order. */
+enum { HSBEM_PRECONDITIONER_NONE = 0, HSBEM_PRECONDITIONER_BLOCK_JACOBI = 1 };
 enum { HSBEM_BOUNDARY_WALL = 0, HSBEM_BOUNDARY_DRIVEN = 1 };
 
 /* BemSolvePhase (bem.hpp:334-342). */
@@ -77,6 +80,7 @@ typedef struct {
     uint32_t gmres_maximum_iterations;   /* default 1000 */
     uint32_t precision;                  /* HSBEM_PRECISION_FP64 (default) or HSBEM_PRECISION_MIXED */
     uint32_t plan_threads;               /* host planner threads, 0 = up to 8 */
+    uint32_t preconditioner;             /* GMRES: HSBEM_PRECONDITIONER_BLOCK_JACOBI (default) or _NONE */
 } hsbem_numerics;
 
 /* BemRequest (bem.hpp:163-191) for a closed exterior mesh. Excitation e assigns

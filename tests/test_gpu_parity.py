@@ -213,6 +213,24 @@ def test_dense_lu_matches_numpy(n, nrhs):
     assert np.max(np.abs(X - Xn)) / np.max(np.abs(Xn)) <= 1e-10
 
 
+def test_block_jacobi_gmres_matches_unpreconditioned():
+    """Right block-Jacobi preconditioning changes the Krylov path, not the solution."""
+    mesh = meshes.exponential_horn(0.02)
+    solver = bem.HipBemSolver(mesh)
+    req = lambda pre: bem.BemRequest(medium=bem.AcousticMedium(1.205, 343.0), frequencies=[3000.0],
+                                     excitations=[bem.BemExcitation("throat", [bem.FaceVelocity(1, 1.0)])],
+                                     numerics=bem.BemNumerics(solver=bem.SOLVER_GMRES, relative_residual_tolerance=1e-10,
+                                                              preconditioner=pre))
+    plain = solver.solve(req(bem.PRECONDITIONER_NONE)).frequencies[0].excitations[0]
+    jac = solver.solve(req(bem.PRECONDITIONER_BLOCK_JACOBI)).frequencies[0].excitations[0]
+    err = np.linalg.norm(jac.boundary.pressure_trace - plain.boundary.pressure_trace) / \
+        np.linalg.norm(plain.boundary.pressure_trace)
+    print(f"block-Jacobi: {plain.iterations} -> {jac.iterations} iterations, difference {err:.2e}")
+    assert err <= 1e-8
+    assert jac.boundary_residual <= 1e-10 and plain.boundary_residual <= 1e-10
+    assert jac.iterations < plain.iterations
+
+
 def test_gmres_solve_matches_lu():
     mesh = meshes.icosphere(3)
     num_lu = bem.BemNumerics()

@@ -32,6 +32,10 @@ def test_numerics_defaults_match_reference():
     assert (n.near_distance_ratio, n.near_relative_tolerance, n.relative_residual_tolerance) == (1.0, 1e-4, 5e-5)
     py = bem.BemNumerics()
     assert (py.quadrature_order, py.singular_order, py.near_relative_tolerance) == (4, 4, 1e-4)
+    # MI355X additions: LU by default (reference behaviour); block-Jacobi when GMRES is chosen
+    assert (n.solver, n.precision, n.preconditioner) == (bem.SOLVER_LU, bem.PRECISION_FP64,
+                                                         bem.PRECONDITIONER_BLOCK_JACOBI)
+    assert (py.solver, py.precision, py.preconditioner) == (n.solver, n.precision, n.preconditioner)
 
 
 def _box(lx, ly, lz):
