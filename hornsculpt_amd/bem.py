@@ -123,6 +123,8 @@ def library() -> ctypes.CDLL:
                 "hsbem_plan_build": (ctypes.c_int, [D, U32, ctypes.POINTER(U32), U32, ctypes.POINTER(_Numerics),
                                                     ctypes.c_double, ctypes.POINTER(P)]),
                 "hsbem_plan_sizes": (ctypes.c_int, [P, ctypes.POINTER(U64)]),
+                "hsbem_support_tree_build": (ctypes.c_int, [D, U32, ctypes.POINTER(U32), U32, U32, U32,
+                                                            ctypes.c_double, ctypes.c_double, D, ctypes.POINTER(U64)]),
                 "hsbem_plan_copy": (ctypes.c_int, [P, ctypes.POINTER(U32), ctypes.POINTER(U32), D,
                                                    ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(U32),
                                                    ctypes.POINTER(U32), D]),
@@ -556,6 +558,56 @@ def interaction_plan(mesh: SurfaceMesh, numerics: BemNumerics, k_max: float) -> 
         return _copy_plan(lib.hsbem_plan_sizes, lib.hsbem_plan_copy, handle, len(t))
     finally:
         lib.hsbem_plan_destroy(handle)
+
+
+def support_tree(mesh: SurfaceMesh, leaf_capacity: int = 32, maximum_depth: int = 20, separation: float = 1.5,
+                 near_ratio: float = 2.0) -> np.ndarray:
+    """Wideband FMM support octree and near/far partition (host planner), serialised
+    as documented at hsbem_support_tree_build; parse with ``parse_support_tree``."""
+    lib = library()
+    v = np.ascontiguousarray(mesh.vertices, dtype=np.float64)
+    t = np.ascontiguousarray(mesh.triangles, dtype=np.uint32)
+    size = ctypes.c_uint64(0)
+    args = (_dptr(v), len(v), _uptr(t), len(t), leaf_capacity, maximum_depth, separation, near_ratio)
+    _check(lib.hsbem_support_tree_build(*args, None, ctypes.byref(size)))
+    blob = np.zeros(size.value)
+    _check(lib.hsbem_support_tree_build(*args, _dptr(blob), ctypes.byref(size)))
+    return blob
+
+
+def parse_support_tree(blob: np.ndarray) -> dict:
+    """Structured view of a serialised support tree."""
+    h = [int(x) for x in blob[:14]]
+    n_nodes, n_levels, ppn, ppf, pfn, pff, ppl, pfl, nv, nt = h[:10]
+    pos = 14
+    nodes = blob[pos:pos + 38 * n_nodes].reshape(n_nodes, 38)
+    pos += 38 * n_nodes
+    out = {"nodes": nodes, "represented": h[10:14]}
+    for name, n in (("pressure_order", nv), ("flux_order", nt), ("pressure_leaf", nv), ("flux_leaf", nt)):
+        out[name] = blob[pos:pos + n].astype(np.int64)
+        pos += n
+    for name, n in (("pp_near", ppn), ("pp_far", ppf), ("pf_near", pfn), ("pf_far", pff)):
+        out[name] = blob[pos:pos + 2 * n].reshape(n, 2).astype(np.int64)
+        pos += 2 * n
+    for name, n in (("pp_far_levels", ppl), ("pf_far_levels", pfl)):
+        out[name] = blob[pos:pos + 4 * n].reshape(n, 4).astype(np.int64)
+        pos += 4 * n
+    levels = []
+    for _ in range(n_levels):
+        c = int(blob[pos])
+        ids = blob[pos + 1:pos + 1 + c].astype(np.int64)
+        pos += 1 + c
+        radii = blob[pos:pos + 2]
+        counters = blob[pos + 2:pos + 15].astype(np.int64)
+        pos += 15
+        ns = int(blob[pos])
+        squared = blob[pos + 1:pos + 1 + ns].astype(np.int64)
+        pos += 1 + ns
+        levels.append({"nodes": ids, "max_radius": radii[0], "max_active_radius": radii[1], "counters": counters,
+                       "squared_offsets": squared})
+    out["levels"] = levels
+    assert pos == len(blob)
+    return out
 
 
 def dense_gmres(A: np.ndarray, b: np.ndarray, restart: int, maximum_iterations: int, tolerance: float,

@@ -2,6 +2,7 @@
 // way the reference throws them (std::invalid_argument / std::runtime_error).
 #include "../../include/hsbem.h"
 
+#include "host/support_tree.hpp"
 #include "server.hpp"
 #include "solver.hpp"
 
@@ -124,6 +125,55 @@ std::uint64_t fnv1a(std::uint64_t h, const void *data, std::size_t bytes) {
         h *= 1099511628211ULL;
     }
     return h;
+}
+std::vector<double> serialize_tree(const hsbem::SupportTree &t, std::uint32_t nv, std::uint32_t nt) {
+    std::vector<double> b;
+    auto put = [&](double v) { b.push_back(v); };
+    auto box = [&](const hsbem::SupportBox &x) {
+        for (double v : {x.lower.x, x.lower.y, x.lower.z, x.upper.x, x.upper.y, x.upper.z, x.max_edge})
+            put(v);
+    };
+    const auto &pp = t.pressure_pressure, &pf = t.pressure_flux;
+    for (double v : {double(t.nodes.size()), double(t.levels.size()), double(pp.near.size()), double(pp.far.size()),
+                     double(pf.near.size()), double(pf.far.size()), double(pp.far_levels.size()),
+                     double(pf.far_levels.size()), double(nv), double(nt), double(pp.represented_near),
+                     double(pp.represented_far), double(pf.represented_near), double(pf.represented_far)})
+        put(v);
+    for (const auto &n : t.nodes) {
+        put(n.center.x), put(n.center.y), put(n.center.z), put(n.half_width);
+        for (auto c : n.coords)
+            put(c);
+        put(n.level), put(n.parent);
+        for (auto c : n.children)
+            put(c);
+        put(n.first_pressure), put(n.pressure_count), put(n.first_flux), put(n.flux_count);
+        box(n.pressure_box);
+        box(n.flux_box);
+        put(n.pressure_radius), put(n.flux_radius), put(n.active);
+    }
+    for (const auto *v : {&t.pressure_order, &t.flux_order, &t.pressure_leaf, &t.flux_leaf})
+        for (auto x : *v)
+            put(x);
+    for (const auto *l : {&pp.near, &pp.far, &pf.near, &pf.far})
+        for (const auto &p : *l)
+            put(p.target), put(p.source);
+    for (const auto *l : {&pp.far_levels, &pf.far_levels})
+        for (const auto &e : *l)
+            put(e.target_level), put(e.source_level), put(double(e.interactions)), put(double(e.represented_pairs));
+    for (const auto &l : t.levels) {
+        put(double(l.nodes.size()));
+        for (auto id : l.nodes)
+            put(id);
+        put(l.max_radius), put(l.max_active_radius);
+        for (auto v : {l.pressure_outgoing, l.flux_outgoing, l.pressure_incoming, l.pp_far, l.pf_far, l.pp_offsets,
+                       l.pf_offsets, l.union_offsets, l.pp_distance_classes, l.pf_distance_classes,
+                       l.pp_polar_classes, l.pf_polar_classes, l.union_polar_classes})
+            put(double(v));
+        put(double(l.squared_offsets.size()));
+        for (auto v : l.squared_offsets)
+            put(double(v));
+    }
+    return b;
 }
 } // namespace
 
@@ -438,6 +488,32 @@ int hsbem_plan_build(const double *vertices, uint32_t nv, const uint32_t *triang
         auto p = std::make_unique<hsbem_plan>();
         p->plan = hsbem::make_interaction_plan(space, q, k_max, num.plan_threads);
         *out = p.release();
+    });
+}
+
+int hsbem_support_tree_build(const double *vertices, uint32_t nv, const uint32_t *triangles, uint32_t nt,
+                             uint32_t leaf_capacity, uint32_t maximum_depth, double separation, double near_ratio,
+                             double *blob, uint64_t *blob_size) {
+    return guarded([&] {
+        require(vertices && triangles && nv && nt && blob_size, "invalid support tree arguments");
+        hsbem::SurfaceMesh m;
+        m.vertices.resize(nv);
+        for (uint32_t i = 0; i < nv; ++i)
+            m.vertices[i] = {vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2]};
+        m.triangles.resize(nt);
+        for (uint32_t t = 0; t < nt; ++t)
+            m.triangles[t].vertices = {triangles[3 * t], triangles[3 * t + 1], triangles[3 * t + 2]};
+        hsbem::SupportTreeSettings settings;
+        settings.leaf_capacity = leaf_capacity;
+        settings.maximum_depth = maximum_depth;
+        settings.separation = separation;
+        settings.near_ratio = near_ratio;
+        const auto blob_v = serialize_tree(hsbem::make_support_tree(hsbem::build_space(m), settings), nv, nt);
+        if (blob) {
+            require(*blob_size >= blob_v.size(), "support tree buffer too small");
+            std::copy(blob_v.begin(), blob_v.end(), blob);
+        }
+        *blob_size = blob_v.size();
     });
 }
 

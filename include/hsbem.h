@@ -212,6 +212,24 @@ HSBEM_API int hsbem_plan_copy(const hsbem_plan *plan, uint32_t *touching, uint32
                               double *maximum_estimated_error);
 HSBEM_API int hsbem_plan_destroy(hsbem_plan *plan);
 
+/* Wideband FMM support octree and near/far partition (host-only; replaces
+ * makeSupportTree, modules/bem/fmm/support_tree.hpp:96-104, support_tree.cpp:395-414)
+ * on the given coordinates (no recentring). Serialised into doubles (every
+ * integer exact): pass blob = NULL to query *blob_size. Layout: header[14]
+ * (nodes, levels, pp near, pp far, pf near, pf far, pp far-level pairs, pf
+ * far-level pairs, nv, nt, represented pp near, pp far, pf near, pf far);
+ * per node 38 values (center[3], half width, coords[3], level, parent,
+ * children[8], first/count pressure, first/count flux, pressure box[7], flux
+ * box[7], pressure radius, flux radius, active channels); pressure order,
+ * flux order, pressure leaf, flux leaf; the four interaction lists (target,
+ * source); the two far-level lists (target level, source level,
+ * interactions, represented pairs); per level: node count, node ids, max
+ * radius, max active radius, 13 counters, squared-offset count and values. */
+HSBEM_API int hsbem_support_tree_build(const double *vertices, uint32_t vertex_count, const uint32_t *triangles,
+                                       uint32_t triangle_count, uint32_t leaf_capacity, uint32_t maximum_depth,
+                                       double separation_ratio, double near_distance_ratio, double *blob,
+                                       uint64_t *blob_size);
+
 /* Device assembly of the row-scaled Galerkin system at wavenumber k with the
  * plan for k_max: A (nv x nv) and B (nv x nt, every panel a flux column), both
  * complex column-major, copied to host. */

@@ -8,6 +8,7 @@
 #include "galerkin/pair_quadrature.hpp"
 #include "galerkin/regular_qualification.hpp"
 #include "galerkin/space.hpp"
+#include "fmm/support_tree.hpp"
 
 #include <cstdint>
 #include <cstring>
@@ -67,6 +68,69 @@ PairQuadraturePatch unpack_patch(const double *in) {
 struct PlanHandle {
     InteractionPlan plan;
 };
+
+// Same flat layout as the product's hsbem_support_tree_build (see include/hsbem.h).
+std::vector<double> tree_blob(const wideband::SupportTree &t, std::uint32_t nv, std::uint32_t nt) {
+    std::vector<double> b;
+    auto box = [&](const wideband::SupportBounds &x) {
+        for (double v : {x.lower.x, x.lower.y, x.lower.z, x.upper.x, x.upper.y, x.upper.z, x.maximumPanelEdge})
+            b.push_back(v);
+    };
+    const auto &pp = t.pressurePressure, &pf = t.pressureFlux;
+    for (double v : {double(t.nodes.size()), double(t.levels.size()), double(pp.nearInteractions.size()),
+                     double(pp.farInteractions.size()), double(pf.nearInteractions.size()),
+                     double(pf.farInteractions.size()), double(pp.farLevelPairs.size()), double(pf.farLevelPairs.size()),
+                     double(nv), double(nt), double(pp.representedNearPairs), double(pp.representedFarPairs),
+                     double(pf.representedNearPairs), double(pf.representedFarPairs)})
+        b.push_back(v);
+    for (const auto &n : t.nodes) {
+        for (double v : {n.center.x, n.center.y, n.center.z, n.halfWidth})
+            b.push_back(v);
+        for (auto c : n.coordinates)
+            b.push_back(c);
+        b.push_back(n.level);
+        b.push_back(n.parent);
+        for (auto c : n.children)
+            b.push_back(c);
+        for (auto v : {n.firstPressure, n.pressureCount, n.firstFlux, n.fluxCount})
+            b.push_back(v);
+        box(n.pressureBounds);
+        box(n.fluxBounds);
+        b.push_back(n.pressureRadius);
+        b.push_back(n.fluxRadius);
+        b.push_back(n.activeChannels);
+    }
+    for (const auto *v : {&t.pressureOrder, &t.fluxOrder, &t.pressureLeaf, &t.fluxLeaf})
+        for (auto x : *v)
+            b.push_back(x);
+    for (const auto *l : {&pp.nearInteractions, &pp.farInteractions, &pf.nearInteractions, &pf.farInteractions})
+        for (const auto &p : *l) {
+            b.push_back(p.target);
+            b.push_back(p.source);
+        }
+    for (const auto *l : {&pp.farLevelPairs, &pf.farLevelPairs})
+        for (const auto &e : *l)
+            for (double v : {double(e.targetLevel), double(e.sourceLevel), double(e.interactions),
+                             double(e.representedPairs)})
+                b.push_back(v);
+    for (const auto &l : t.levels) {
+        b.push_back(double(l.nodes.size()));
+        for (auto id : l.nodes)
+            b.push_back(id);
+        b.push_back(l.maximumSupportRadius);
+        b.push_back(l.maximumActiveSupportRadius);
+        for (auto v : {l.pressureOutgoingNodes, l.fluxOutgoingNodes, l.pressureIncomingNodes,
+                       l.pressurePressureFarInteractions, l.pressureFluxFarInteractions, l.pressurePressureOffsets,
+                       l.pressureFluxOffsets, l.unionOffsets, l.pressurePressureDistanceClasses,
+                       l.pressureFluxDistanceClasses, l.pressurePressurePolarClasses, l.pressureFluxPolarClasses,
+                       l.unionPolarClasses})
+            b.push_back(double(v));
+        b.push_back(double(l.squaredOffsetClasses.size()));
+        for (auto v : l.squaredOffsetClasses)
+            b.push_back(double(v));
+    }
+    return b;
+}
 } // namespace
 
 extern "C" {
@@ -246,4 +310,25 @@ void ref_plan_copy(void *handle, std::uint32_t *touching, std::uint32_t *near, d
 }
 
 void ref_plan_destroy(void *handle) { delete static_cast<PlanHandle *>(handle); }
+
+// makeSupportTree (fmm/support_tree.cpp) on the raw coordinates, serialised.
+int ref_support_tree(std::uint32_t nv, const double *xyz, std::uint32_t nt, const std::uint32_t *tri,
+                     unsigned leaf_capacity, unsigned maximum_depth, double separation, double near_ratio,
+                     double *blob, std::uint64_t *size) {
+    return guarded([&] {
+        const auto s = buildSpace(make_mesh(nv, xyz, nt, tri));
+        wideband::SupportTreeSettings settings;
+        settings.leafCapacity = leaf_capacity;
+        settings.maximumDepth = maximum_depth;
+        settings.separationRatio = separation;
+        settings.nearDistanceRatio = near_ratio;
+        const auto v = tree_blob(wideband::makeSupportTree(s, settings), nv, nt);
+        if (blob) {
+            if (*size < v.size())
+                throw std::runtime_error("support tree buffer too small");
+            std::memcpy(blob, v.data(), v.size() * sizeof(double));
+        }
+        *size = v.size();
+    });
+}
 }

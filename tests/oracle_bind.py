@@ -49,6 +49,8 @@ def reference():
     lib.ref_plan_sizes.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
     lib.ref_plan_copy.argtypes = [ctypes.c_void_p, _U, _U, _D, _U8, _U, _U, _D]
     lib.ref_plan_destroy.argtypes = [ctypes.c_void_p]
+    lib.ref_support_tree.argtypes = [ctypes.c_uint32, _D, ctypes.c_uint32, _U, ctypes.c_uint, ctypes.c_uint,
+                                     ctypes.c_double, ctypes.c_double, _D, ctypes.POINTER(ctypes.c_uint64)]
     return lib
 
 
@@ -99,6 +101,18 @@ def ref_plan(mesh, quadrature_order=4, singular_order=4, near_ratio=1.0, toleran
                 "maximum_depth": int(sizes[5]), "maximum_estimated_error": err.value}
     finally:
         lib.ref_plan_destroy(h)
+
+
+def ref_support_tree(mesh, leaf_capacity=32, maximum_depth=20, separation=1.5, near_ratio=2.0):
+    """Serialised reference makeSupportTree on the raw coordinates (layout of hsbem_support_tree_build)."""
+    lib = reference()
+    V, T = _mesh_args(mesh)
+    size = ctypes.c_uint64(0)
+    args = (len(V), _d(V), len(T), _u(T), leaf_capacity, maximum_depth, separation, near_ratio)
+    _check(lib, lib.ref_support_tree(*args, None, ctypes.byref(size)), "ref_support_tree")
+    blob = np.zeros(size.value)
+    _check(lib, lib.ref_support_tree(*args, _d(blob), ctypes.byref(size)), "ref_support_tree")
+    return blob
 
 
 def oracle_assemble(mesh, plan, k, tolerance=1e-4, regular_order=4, singular_order=4, tests=None, threads=8,
