@@ -123,6 +123,10 @@ def library() -> ctypes.CDLL:
                 "hsbem_plan_build": (ctypes.c_int, [D, U32, ctypes.POINTER(U32), U32, ctypes.POINTER(_Numerics),
                                                     ctypes.c_double, ctypes.POINTER(P)]),
                 "hsbem_plan_sizes": (ctypes.c_int, [P, ctypes.POINTER(U64)]),
+                "hsbem_support_near_plan_build": (ctypes.c_int, [D, U32, ctypes.POINTER(U32), ctypes.POINTER(U32),
+                                                                 ctypes.POINTER(ctypes.c_uint8), U32,
+                                                                 ctypes.POINTER(U32), U32, U32, U32, ctypes.c_double,
+                                                                 ctypes.c_double, D, ctypes.POINTER(U64)]),
                 "hsbem_support_tree_build": (ctypes.c_int, [D, U32, ctypes.POINTER(U32), U32, U32, U32,
                                                             ctypes.c_double, ctypes.c_double, D, ctypes.POINTER(U64)]),
                 "hsbem_plan_copy": (ctypes.c_int, [P, ctypes.POINTER(U32), ctypes.POINTER(U32), D,
@@ -573,6 +577,39 @@ def support_tree(mesh: SurfaceMesh, leaf_capacity: int = 32, maximum_depth: int 
     blob = np.zeros(size.value)
     _check(lib.hsbem_support_tree_build(*args, _dptr(blob), ctypes.byref(size)))
     return blob
+
+
+def support_near_plan(mesh: SurfaceMesh, active_faces: Sequence[int], leaf_capacity: int = 32,
+                      maximum_depth: int = 20, separation: float = 1.5, near_ratio: float = 2.0) -> np.ndarray:
+    """Near-field DOF plan of the wideband FMM path (host planner), serialised as
+    documented at hsbem_support_near_plan_build; parse with ``parse_support_near_plan``."""
+    lib = library()
+    v = np.ascontiguousarray(mesh.vertices, dtype=np.float64)
+    t = np.ascontiguousarray(mesh.triangles, dtype=np.uint32)
+    f = np.ascontiguousarray(mesh.face_ids, dtype=np.uint32)
+    b = np.ascontiguousarray(mesh.boundary, dtype=np.uint8)
+    a = np.ascontiguousarray(active_faces, dtype=np.uint32)
+    size = ctypes.c_uint64(0)
+    args = (_dptr(v), len(v), _uptr(t), _uptr(f), b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), len(t), _uptr(a),
+            len(a), leaf_capacity, maximum_depth, separation, near_ratio)
+    _check(lib.hsbem_support_near_plan_build(*args, None, ctypes.byref(size)))
+    blob = np.zeros(size.value)
+    _check(lib.hsbem_support_near_plan_build(*args, _dptr(blob), ctypes.byref(size)))
+    return blob
+
+
+def parse_support_near_plan(blob: np.ndarray, triangle_count: int) -> dict:
+    rows, pnnz, fnnz, nodes1, npanels, nblocks, njobs = (int(x) for x in blob[:7])
+    pos = 7
+    out = {}
+    for name, n in (("pressure_offsets", rows), ("pressure_columns", pnnz), ("flux_offsets", rows),
+                    ("flux_columns", fnnz), ("panel_face_column", triangle_count), ("node_panel_offsets", nodes1),
+                    ("support_panels", npanels), ("flux_block_offsets", nodes1), ("flux_block_indices", nblocks)):
+        out[name] = blob[pos:pos + n].astype(np.int64)
+        pos += n
+    out["jobs"] = blob[pos:pos + 4 * njobs].reshape(njobs, 4).astype(np.int64)
+    assert pos + 4 * njobs == len(blob)
+    return out
 
 
 def parse_support_tree(blob: np.ndarray) -> dict:

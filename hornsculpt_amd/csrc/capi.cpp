@@ -2,6 +2,7 @@
 // way the reference throws them (std::invalid_argument / std::runtime_error).
 #include "../../include/hsbem.h"
 
+#include "host/support_near_plan.hpp"
 #include "host/support_tree.hpp"
 #include "server.hpp"
 #include "solver.hpp"
@@ -514,6 +515,54 @@ int hsbem_support_tree_build(const double *vertices, uint32_t nv, const uint32_t
             std::copy(blob_v.begin(), blob_v.end(), blob);
         }
         *blob_size = blob_v.size();
+    });
+}
+
+int hsbem_support_near_plan_build(const double *vertices, uint32_t nv, const uint32_t *triangles,
+                                  const uint32_t *face_ids, const uint8_t *boundary, uint32_t nt,
+                                  const uint32_t *active_faces, uint32_t nfaces, uint32_t leaf_capacity,
+                                  uint32_t maximum_depth, double separation, double near_ratio, double *blob,
+                                  uint64_t *blob_size) {
+    return guarded([&] {
+        require(vertices && triangles && face_ids && boundary && nv && nt && blob_size && (active_faces || !nfaces),
+                "invalid near plan arguments");
+        hsbem::SurfaceMesh m;
+        m.vertices.resize(nv);
+        for (uint32_t i = 0; i < nv; ++i)
+            m.vertices[i] = {vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2]};
+        m.triangles.resize(nt);
+        for (uint32_t t = 0; t < nt; ++t) {
+            m.triangles[t].vertices = {triangles[3 * t], triangles[3 * t + 1], triangles[3 * t + 2]};
+            m.triangles[t].face_id = face_ids[t];
+            m.triangles[t].boundary = boundary[t] == HSBEM_BOUNDARY_DRIVEN ? hsbem::Boundary::driven : hsbem::Boundary::wall;
+        }
+        const auto space = hsbem::build_space(m);
+        hsbem::SupportTreeSettings settings;
+        settings.leaf_capacity = leaf_capacity;
+        settings.maximum_depth = maximum_depth;
+        settings.separation = separation;
+        settings.near_ratio = near_ratio;
+        const auto tree = hsbem::make_support_tree(space, settings);
+        const auto plan =
+            hsbem::make_support_near_plan(space, tree, std::vector<std::uint32_t>(active_faces, active_faces + nfaces));
+        const auto jobs = hsbem::support_near_jobs(space, tree, plan);
+        std::vector<double> b;
+        for (double v : {double(plan.pressure.row_offsets.size()), double(plan.pressure.columns.size()),
+                         double(plan.flux.columns.size()), double(plan.node_panel_offsets.size()),
+                         double(plan.support_panels.size()), double(plan.flux_block_indices.size()), double(jobs.size())})
+            b.push_back(v);
+        for (const auto *v : {&plan.pressure.row_offsets, &plan.pressure.columns, &plan.flux.row_offsets,
+                              &plan.flux.columns, &plan.panel_face_column, &plan.node_panel_offsets,
+                              &plan.support_panels, &plan.flux_block_offsets, &plan.flux_block_indices})
+            b.insert(b.end(), v->begin(), v->end());
+        for (const auto &j : jobs)
+            for (double v : {double(j.test), double(j.source), double(j.pressure_mask), double(j.flux_mask)})
+                b.push_back(v);
+        if (blob) {
+            require(*blob_size >= b.size(), "near plan buffer too small");
+            std::copy(b.begin(), b.end(), blob);
+        }
+        *blob_size = b.size();
     });
 }
 

@@ -230,6 +230,21 @@ HSBEM_API int hsbem_support_tree_build(const double *vertices, uint32_t vertex_c
                                        double separation_ratio, double near_distance_ratio, double *blob,
                                        uint64_t *blob_size);
 
+/* Near-field DOF plan of the wideband FMM path over the support tree of the
+ * same settings (host-only; replaces makeSupportNearPlan and
+ * SupportNearJobIterator, modules/bem/fmm/support_near_plan.hpp:13-72) for the
+ * given active face columns. Serialised into doubles: header[7] (nv + 1,
+ * pressure nnz, flux nnz, nodes + 1, support panels, flux blocks, jobs), then
+ * pressure row offsets and columns, flux row offsets and columns, panel face
+ * column (UINT32_MAX when inactive), node panel offsets, support panels, flux
+ * block offsets and indices, and per job (test panel, source panel, P1/P1
+ * mask, P1/DP0 mask). blob = NULL queries *blob_size. */
+HSBEM_API int hsbem_support_near_plan_build(const double *vertices, uint32_t vertex_count, const uint32_t *triangles,
+                                            const uint32_t *face_ids, const uint8_t *boundary, uint32_t triangle_count,
+                                            const uint32_t *active_faces, uint32_t active_face_count,
+                                            uint32_t leaf_capacity, uint32_t maximum_depth, double separation_ratio,
+                                            double near_distance_ratio, double *blob, uint64_t *blob_size);
+
 /* Device assembly of the row-scaled Galerkin system at wavenumber k with the
  * plan for k_max: A (nv x nv) and B (nv x nt, every panel a flux column), both
  * complex column-major, copied to host. */

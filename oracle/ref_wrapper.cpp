@@ -8,6 +8,7 @@
 #include "galerkin/pair_quadrature.hpp"
 #include "galerkin/regular_qualification.hpp"
 #include "galerkin/space.hpp"
+#include "fmm/support_near_plan.hpp"
 #include "fmm/support_tree.hpp"
 
 #include <cstdint>
@@ -329,6 +330,59 @@ int ref_support_tree(std::uint32_t nv, const double *xyz, std::uint32_t nt, cons
             std::memcpy(blob, v.data(), v.size() * sizeof(double));
         }
         *size = v.size();
+    });
+}
+
+// makeSupportNearPlan + SupportNearJobIterator (fmm/support_near_plan.cpp) with
+// face ids and driven flags, serialised like hsbem_support_near_plan_build.
+int ref_support_near_plan(std::uint32_t nv, const double *xyz, std::uint32_t nt, const std::uint32_t *tri,
+                          const std::uint32_t *face_ids, const std::uint8_t *driven, const std::uint32_t *faces,
+                          std::uint32_t nfaces, unsigned leaf_capacity, unsigned maximum_depth, double separation,
+                          double near_ratio, double *blob, std::uint64_t *size) {
+    return guarded([&] {
+        auto mesh = make_mesh(nv, xyz, nt, tri);
+        for (std::uint32_t t = 0; t < nt; ++t) {
+            mesh.triangles[t].faceId = face_ids[t];
+            mesh.triangles[t].boundary = driven[t] ? BoundaryCondition::driven : BoundaryCondition::wall;
+        }
+        const auto s = buildSpace(mesh);
+        wideband::SupportTreeSettings settings;
+        settings.leafCapacity = leaf_capacity;
+        settings.maximumDepth = maximum_depth;
+        settings.separationRatio = separation;
+        settings.nearDistanceRatio = near_ratio;
+        const auto tree = wideband::makeSupportTree(s, settings);
+        const auto plan = wideband::makeSupportNearPlan(s, tree, std::span<const std::uint32_t>(faces, nfaces),
+                                                        std::size_t{8} << 30);
+        std::vector<double> b;
+        for (double v : {double(plan.pressure.rowOffsets.size()), double(plan.pressure.columns.size()),
+                         double(plan.flux.columns.size()), double(plan.nodePanelOffsets.size()),
+                         double(plan.supportPanels.size()), double(plan.fluxBlockIndices.size())})
+            b.push_back(v);
+        const std::size_t job_slot = b.size();
+        b.push_back(0);
+        for (const auto *v : {&plan.pressure.rowOffsets, &plan.pressure.columns, &plan.flux.rowOffsets,
+                              &plan.flux.columns, &plan.panelToActiveFaceColumn, &plan.nodePanelOffsets,
+                              &plan.supportPanels, &plan.fluxBlockOffsets, &plan.fluxBlockIndices})
+            b.insert(b.end(), v->begin(), v->end());
+        wideband::SupportNearJobIterator it(s, tree, plan);
+        std::vector<wideband::SupportNearJob> batch(4096);
+        std::size_t jobs = 0;
+        while (!it.done()) {
+            const auto n = it.next(batch, batch.size() * sizeof(wideband::SupportNearJob));
+            for (std::size_t i = 0; i < n; ++i)
+                for (double v : {double(batch[i].testPanel), double(batch[i].sourcePanel), double(batch[i].pressureMask),
+                                 double(batch[i].fluxMask)})
+                    b.push_back(v);
+            jobs += n;
+        }
+        b[job_slot] = double(jobs);
+        if (blob) {
+            if (*size < b.size())
+                throw std::runtime_error("near plan buffer too small");
+            std::memcpy(blob, b.data(), b.size() * sizeof(double));
+        }
+        *size = b.size();
     });
 }
 }

@@ -49,6 +49,9 @@ def reference():
     lib.ref_plan_sizes.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
     lib.ref_plan_copy.argtypes = [ctypes.c_void_p, _U, _U, _D, _U8, _U, _U, _D]
     lib.ref_plan_destroy.argtypes = [ctypes.c_void_p]
+    lib.ref_support_near_plan.argtypes = [ctypes.c_uint32, _D, ctypes.c_uint32, _U, _U, _U8, _U, ctypes.c_uint32,
+                                          ctypes.c_uint, ctypes.c_uint, ctypes.c_double, ctypes.c_double, _D,
+                                          ctypes.POINTER(ctypes.c_uint64)]
     lib.ref_support_tree.argtypes = [ctypes.c_uint32, _D, ctypes.c_uint32, _U, ctypes.c_uint, ctypes.c_uint,
                                      ctypes.c_double, ctypes.c_double, _D, ctypes.POINTER(ctypes.c_uint64)]
     return lib
@@ -112,6 +115,22 @@ def ref_support_tree(mesh, leaf_capacity=32, maximum_depth=20, separation=1.5, n
     _check(lib, lib.ref_support_tree(*args, None, ctypes.byref(size)), "ref_support_tree")
     blob = np.zeros(size.value)
     _check(lib, lib.ref_support_tree(*args, _d(blob), ctypes.byref(size)), "ref_support_tree")
+    return blob
+
+
+def ref_support_near_plan(mesh, active_faces, leaf_capacity=32, maximum_depth=20, separation=1.5, near_ratio=2.0):
+    """Serialised reference makeSupportNearPlan + job iterator (layout of hsbem_support_near_plan_build)."""
+    lib = reference()
+    V, T = _mesh_args(mesh)
+    F = np.ascontiguousarray(mesh.face_ids, dtype=np.uint32)
+    D = np.ascontiguousarray(mesh.boundary, dtype=np.uint8)
+    A = np.ascontiguousarray(active_faces, dtype=np.uint32)
+    size = ctypes.c_uint64(0)
+    args = (len(V), _d(V), len(T), _u(T), _u(F), D.ctypes.data_as(_U8), _u(A), len(A), leaf_capacity, maximum_depth,
+            separation, near_ratio)
+    _check(lib, lib.ref_support_near_plan(*args, None, ctypes.byref(size)), "ref_support_near_plan")
+    blob = np.zeros(size.value)
+    _check(lib, lib.ref_support_near_plan(*args, _d(blob), ctypes.byref(size)), "ref_support_near_plan")
     return blob
 
 

@@ -59,3 +59,63 @@ def test_support_tree_rejects_bad_settings():
         bem.support_tree(meshes.octa_sphere(1), separation=1.0)
     with pytest.raises(ValueError):
         bem.support_tree(meshes.octa_sphere(1), leaf_capacity=0)
+
+
+def _banded(mesh):
+    """Three faces by centroid height: top driven (1), bottom driven (2), band wall (3)."""
+    V, T = mesh.vertices, mesh.triangles
+    z = V[T].mean(axis=1)[:, 2]
+    span = np.ptp(V[:, 2])
+    faces = np.where(z > 0.15 * span, 1, np.where(z < -0.15 * span, 2, 3)).astype(np.uint32)
+    return bem.SurfaceMesh(V, T, faces, np.where(faces == 3, bem.WALL, bem.DRIVEN).astype(np.uint8))
+
+
+NEAR_CASES = [
+    ("ico3-two-faces", lambda: _banded(meshes.icosphere(3)), [2, 1], {}),
+    ("ico3-one-face-leaf8", lambda: _banded(meshes.icosphere(3)), [1], {"leaf_capacity": 8}),
+    ("horn20-throat", lambda: meshes.exponential_horn(0.02), [1], {"leaf_capacity": 16}),
+    ("octa3-no-faces", lambda: _banded(meshes.octa_sphere(3)), [], {"near_ratio": 1.0}),
+]
+
+
+@needs_ref
+@pytest.mark.parametrize("name,make,faces,opts", NEAR_CASES, ids=[c[0] for c in NEAR_CASES])
+def test_support_near_plan_bit_identical_to_reference(name, make, faces, opts):
+    mesh = make()
+    mine = bem.support_near_plan(mesh, faces, **opts)
+    ref = ob.ref_support_near_plan(mesh, faces, **opts)
+    assert mine.shape == ref.shape
+    assert np.array_equal(mine, ref)
+
+
+def test_support_near_plan_matches_its_tree():
+    mesh = _banded(meshes.icosphere(3))
+    tree = bem.parse_support_tree(bem.support_tree(mesh, leaf_capacity=16))
+    plan = bem.parse_support_near_plan(bem.support_near_plan(mesh, [1, 2], leaf_capacity=16), len(mesh.triangles))
+    # the pressure CSR holds exactly the near DOF products of the partition
+    nodes = tree["nodes"]
+    assert plan["pressure_offsets"][-1] == tree["represented"][0]
+    order = tree["pressure_order"]
+    near = {}
+    for a, b in tree["pp_near"]:
+        fa, ca, fb, cb = (int(x) for x in (nodes[a, 17], nodes[a, 18], nodes[b, 17], nodes[b, 18]))
+        for i in order[fa:fa + ca]:
+            near.setdefault(int(i), set()).update(int(j) for j in order[fb:fb + cb])
+    for i, cols in near.items():
+        row = plan["pressure_columns"][plan["pressure_offsets"][i]:plan["pressure_offsets"][i + 1]]
+        assert sorted(cols) == row.tolist()
+    # active columns follow the caller's face order; wall panels carry none
+    col = plan["panel_face_column"]
+    assert set(col[mesh.face_ids == 1].tolist()) == {0} and set(col[mesh.face_ids == 2].tolist()) == {1}
+    assert np.all(col[mesh.face_ids == 3] == 2 ** 32 - 1)
+    # every near P1/P1 entry is produced by some job mask
+    jobs = plan["jobs"]
+    assert np.all(jobs[:, 2] < 512) and np.all(jobs[:, 3] < 8)
+
+
+def test_support_near_plan_rejects_wall_face():
+    mesh = _banded(meshes.icosphere(2))
+    with pytest.raises(ValueError):
+        bem.support_near_plan(mesh, [3])
+    with pytest.raises(ValueError):
+        bem.support_near_plan(mesh, [1, 1])
