@@ -127,6 +127,11 @@ def library() -> ctypes.CDLL:
                                                                  ctypes.POINTER(ctypes.c_uint8), U32,
                                                                  ctypes.POINTER(U32), U32, U32, U32, ctypes.c_double,
                                                                  ctypes.c_double, D, ctypes.POINTER(U64)]),
+                "hsbem_support_expansion_plan_build": (ctypes.c_int, [D, U32, ctypes.POINTER(U32),
+                                                                      ctypes.POINTER(U32), ctypes.POINTER(ctypes.c_uint8),
+                                                                      U32, ctypes.POINTER(U32), U32, U32, U32,
+                                                                      ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                                      ctypes.c_double, D, ctypes.POINTER(U64)]),
                 "hsbem_support_tree_build": (ctypes.c_int, [D, U32, ctypes.POINTER(U32), U32, U32, U32,
                                                             ctypes.c_double, ctypes.c_double, D, ctypes.POINTER(U64)]),
                 "hsbem_plan_copy": (ctypes.c_int, [P, ctypes.POINTER(U32), ctypes.POINTER(U32), D,
@@ -596,6 +601,38 @@ def support_near_plan(mesh: SurfaceMesh, active_faces: Sequence[int], leaf_capac
     blob = np.zeros(size.value)
     _check(lib.hsbem_support_near_plan_build(*args, _dptr(blob), ctypes.byref(size)))
     return blob
+
+
+LEVEL_FIELDS = ("representation", "active_radius", "electrical_radius", "accuracy_seed", "signature_order",
+                "translation_order", "quadrature_order", "spherical_coefficients", "angular_samples",
+                "source_separation", "target_separation", "policy_separation", "max_active_radius",
+                "minimum_far_distance", "diagonal_condition")
+
+
+def support_expansion_plan(mesh: SurfaceMesh, active_faces: Sequence[int], wavenumber: float, accuracy_seed: float,
+                           leaf_capacity: int = 32, maximum_depth: int = 20, separation: float = 1.5,
+                           near_ratio: float = 2.0) -> dict:
+    """Per-level wideband expansion plan (orders, LF/HF representation, convergence
+    ratios) and per-node active channels of the FMM path (host planner)."""
+    lib = library()
+    v = np.ascontiguousarray(mesh.vertices, dtype=np.float64)
+    t = np.ascontiguousarray(mesh.triangles, dtype=np.uint32)
+    f = np.ascontiguousarray(mesh.face_ids, dtype=np.uint32)
+    b = np.ascontiguousarray(mesh.boundary, dtype=np.uint8)
+    a = np.ascontiguousarray(active_faces if len(active_faces) else [0], dtype=np.uint32)
+    size = ctypes.c_uint64(0)
+    args = (_dptr(v), len(v), _uptr(t), _uptr(f), b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), len(t), _uptr(a),
+            len(active_faces), leaf_capacity, maximum_depth, separation, near_ratio, wavenumber, accuracy_seed)
+    _check(lib.hsbem_support_expansion_plan_build(*args, None, ctypes.byref(size)))
+    blob = np.zeros(size.value)
+    _check(lib.hsbem_support_expansion_plan_build(*args, _dptr(blob), ctypes.byref(size)))
+    return {"blob": blob, **parse_support_expansion_plan(blob)}
+
+
+def parse_support_expansion_plan(blob: np.ndarray) -> dict:
+    nl, nn = int(blob[0]), int(blob[1])
+    levels = [dict(zip(LEVEL_FIELDS, blob[2 + 15 * l:2 + 15 * (l + 1)])) for l in range(nl)]
+    return {"levels": levels, "active": blob[2 + 15 * nl:2 + 15 * nl + nn].astype(np.int64)}
 
 
 def parse_support_near_plan(blob: np.ndarray, triangle_count: int) -> dict:

@@ -2,6 +2,7 @@
 // way the reference throws them (std::invalid_argument / std::runtime_error).
 #include "../../include/hsbem.h"
 
+#include "host/fmm_levels.hpp"
 #include "host/support_near_plan.hpp"
 #include "host/support_tree.hpp"
 #include "server.hpp"
@@ -560,6 +561,56 @@ int hsbem_support_near_plan_build(const double *vertices, uint32_t nv, const uin
                 b.push_back(v);
         if (blob) {
             require(*blob_size >= b.size(), "near plan buffer too small");
+            std::copy(b.begin(), b.end(), blob);
+        }
+        *blob_size = b.size();
+    });
+}
+
+int hsbem_support_expansion_plan_build(const double *vertices, uint32_t nv, const uint32_t *triangles,
+                                       const uint32_t *face_ids, const uint8_t *boundary, uint32_t nt,
+                                       const uint32_t *active_faces, uint32_t nfaces, uint32_t leaf_capacity,
+                                       uint32_t maximum_depth, double separation, double near_ratio, double k,
+                                       double seed, double *blob, uint64_t *blob_size) {
+    return guarded([&] {
+        require(vertices && triangles && face_ids && boundary && nv && nt && blob_size && (active_faces || !nfaces),
+                "invalid expansion plan arguments");
+        hsbem::SurfaceMesh m;
+        m.vertices.resize(nv);
+        for (uint32_t i = 0; i < nv; ++i)
+            m.vertices[i] = {vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2]};
+        m.triangles.resize(nt);
+        for (uint32_t t = 0; t < nt; ++t)
+            m.triangles[t].vertices = {triangles[3 * t], triangles[3 * t + 1], triangles[3 * t + 2]};
+        const auto space = hsbem::build_space(m);
+        hsbem::SupportTreeSettings settings;
+        settings.leaf_capacity = leaf_capacity;
+        settings.maximum_depth = maximum_depth;
+        settings.separation = separation;
+        settings.near_ratio = near_ratio;
+        const auto tree = hsbem::make_support_tree(space, settings);
+        std::vector<std::uint32_t> column(nt, UINT32_MAX);
+        for (uint32_t t = 0; t < nt; ++t)
+            for (uint32_t f = 0; f < nfaces; ++f)
+                if (face_ids[t] == active_faces[f] && boundary[t] == HSBEM_BOUNDARY_DRIVEN) {
+                    column[t] = f;
+                    break;
+                }
+        const auto plan = hsbem::make_support_expansion_plan(tree, column, k, seed);
+        std::vector<double> b{double(plan.levels.size()), double(plan.active.size())};
+        for (const auto &l : plan.levels) {
+            const auto &e = l.expansion;
+            for (double v : {double(static_cast<std::uint32_t>(e.representation)), e.active_radius, e.electrical_radius,
+                             e.accuracy_seed, double(e.signature_order), double(e.translation_order),
+                             double(e.quadrature_order), double(e.spherical_coefficients), double(e.angular_samples),
+                             l.source_separation, l.target_separation, l.policy_separation, l.max_active_radius,
+                             l.minimum_far_distance, l.diagonal_condition})
+                b.push_back(v);
+        }
+        for (auto a : plan.active)
+            b.push_back(a);
+        if (blob) {
+            require(*blob_size >= b.size(), "expansion plan buffer too small");
             std::copy(b.begin(), b.end(), blob);
         }
         *blob_size = b.size();

@@ -245,6 +245,26 @@ HSBEM_API int hsbem_support_near_plan_build(const double *vertices, uint32_t ver
                                             uint32_t leaf_capacity, uint32_t maximum_depth, double separation_ratio,
                                             double near_distance_ratio, double *blob, uint64_t *blob_size);
 
+/* Per-level expansion plan of the wideband FMM path at wavenumber k and
+ * accuracy seed (host-only; replaces makeSupportExpansionPlan and
+ * makeWidebandLevelPlan, modules/bem/fmm/support_level_plan.hpp:8-44,
+ * wideband_transfer.hpp:57-72) over the support tree of the given settings;
+ * panels of the active faces carry flux columns. Serialised into doubles:
+ * header[2] (levels, nodes); per level 15 values (representation 0 inactive /
+ * 1 spherical / 2 plane wave, active radius, electrical radius, accuracy seed,
+ * signature, translation and quadrature orders, spherical coefficients,
+ * angular samples, source, target and policy separations, max active radius,
+ * minimum far distance, diagonal condition indicator); then per node its
+ * active channels. blob = NULL queries *blob_size. */
+HSBEM_API int hsbem_support_expansion_plan_build(const double *vertices, uint32_t vertex_count,
+                                                 const uint32_t *triangles, const uint32_t *face_ids,
+                                                 const uint8_t *boundary, uint32_t triangle_count,
+                                                 const uint32_t *active_faces, uint32_t active_face_count,
+                                                 uint32_t leaf_capacity, uint32_t maximum_depth,
+                                                 double separation_ratio, double near_distance_ratio,
+                                                 double wavenumber, double accuracy_seed, double *blob,
+                                                 uint64_t *blob_size);
+
 /* Device assembly of the row-scaled Galerkin system at wavenumber k with the
  * plan for k_max: A (nv x nv) and B (nv x nt, every panel a flux column), both
  * complex column-major, copied to host. */

@@ -8,6 +8,7 @@
 #include "galerkin/pair_quadrature.hpp"
 #include "galerkin/regular_qualification.hpp"
 #include "galerkin/space.hpp"
+#include "fmm/support_level_plan.hpp"
 #include "fmm/support_near_plan.hpp"
 #include "fmm/support_tree.hpp"
 
@@ -380,6 +381,50 @@ int ref_support_near_plan(std::uint32_t nv, const double *xyz, std::uint32_t nt,
         if (blob) {
             if (*size < b.size())
                 throw std::runtime_error("near plan buffer too small");
+            std::memcpy(blob, b.data(), b.size() * sizeof(double));
+        }
+        *size = b.size();
+    });
+}
+
+// makeSupportExpansionPlan (fmm/support_level_plan.cpp, with makeWidebandLevelPlan
+// of fmm/wideband_transfer.cpp), serialised like hsbem_support_expansion_plan_build.
+int ref_support_expansion_plan(std::uint32_t nv, const double *xyz, std::uint32_t nt, const std::uint32_t *tri,
+                               const std::uint32_t *face_ids, const std::uint8_t *driven, const std::uint32_t *faces,
+                               std::uint32_t nfaces, unsigned leaf_capacity, unsigned maximum_depth,
+                               double separation, double near_ratio, double k, double seed, double *blob,
+                               std::uint64_t *size) {
+    return guarded([&] {
+        const auto s = buildSpace(make_mesh(nv, xyz, nt, tri));
+        wideband::SupportTreeSettings settings;
+        settings.leafCapacity = leaf_capacity;
+        settings.maximumDepth = maximum_depth;
+        settings.separationRatio = separation;
+        settings.nearDistanceRatio = near_ratio;
+        const auto tree = wideband::makeSupportTree(s, settings);
+        std::vector<std::uint32_t> column(nt, UINT32_MAX);
+        for (std::uint32_t t = 0; t < nt; ++t)
+            for (std::uint32_t f = 0; f < nfaces; ++f)
+                if (face_ids[t] == faces[f] && driven[t]) {
+                    column[t] = f;
+                    break;
+                }
+        const auto plan = wideband::makeSupportExpansionPlan(tree, column, k, seed);
+        std::vector<double> b{double(plan.levels.size()), double(plan.activeChannels.size())};
+        for (const auto &l : plan.levels) {
+            const auto &e = l.expansion;
+            for (double v : {double(static_cast<int>(e.representation)), e.activeRadius, e.electricalRadius,
+                             e.accuracySeed, double(e.signatureOrder), double(e.translationOrder),
+                             double(e.quadratureOrder), double(e.sphericalCoefficients), double(e.angularSamples),
+                             l.sourceSeparation, l.targetSeparation, l.policySeparation, l.maximumActiveRadius,
+                             l.minimumFarDistance, l.diagonalConditionIndicator})
+                b.push_back(v);
+        }
+        for (auto a : plan.activeChannels)
+            b.push_back(a);
+        if (blob) {
+            if (*size < b.size())
+                throw std::runtime_error("expansion plan buffer too small");
             std::memcpy(blob, b.data(), b.size() * sizeof(double));
         }
         *size = b.size();

@@ -52,6 +52,9 @@ def reference():
     lib.ref_support_near_plan.argtypes = [ctypes.c_uint32, _D, ctypes.c_uint32, _U, _U, _U8, _U, ctypes.c_uint32,
                                           ctypes.c_uint, ctypes.c_uint, ctypes.c_double, ctypes.c_double, _D,
                                           ctypes.POINTER(ctypes.c_uint64)]
+    lib.ref_support_expansion_plan.argtypes = [ctypes.c_uint32, _D, ctypes.c_uint32, _U, _U, _U8, _U, ctypes.c_uint32,
+                                               ctypes.c_uint, ctypes.c_uint, ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_double, _D, ctypes.POINTER(ctypes.c_uint64)]
     lib.ref_support_tree.argtypes = [ctypes.c_uint32, _D, ctypes.c_uint32, _U, ctypes.c_uint, ctypes.c_uint,
                                      ctypes.c_double, ctypes.c_double, _D, ctypes.POINTER(ctypes.c_uint64)]
     return lib
@@ -131,6 +134,23 @@ def ref_support_near_plan(mesh, active_faces, leaf_capacity=32, maximum_depth=20
     _check(lib, lib.ref_support_near_plan(*args, None, ctypes.byref(size)), "ref_support_near_plan")
     blob = np.zeros(size.value)
     _check(lib, lib.ref_support_near_plan(*args, _d(blob), ctypes.byref(size)), "ref_support_near_plan")
+    return blob
+
+
+def ref_support_expansion_plan(mesh, active_faces, k, seed, leaf_capacity=32, maximum_depth=20, separation=1.5,
+                               near_ratio=2.0):
+    """Serialised reference makeSupportExpansionPlan (layout of hsbem_support_expansion_plan_build)."""
+    lib = reference()
+    V, T = _mesh_args(mesh)
+    F = np.ascontiguousarray(mesh.face_ids, dtype=np.uint32)
+    D = np.ascontiguousarray(mesh.boundary, dtype=np.uint8)
+    A = np.ascontiguousarray(active_faces if len(active_faces) else [0], dtype=np.uint32)
+    size = ctypes.c_uint64(0)
+    args = (len(V), _d(V), len(T), _u(T), _u(F), D.ctypes.data_as(_U8), _u(A), len(active_faces), leaf_capacity,
+            maximum_depth, separation, near_ratio, k, seed)
+    _check(lib, lib.ref_support_expansion_plan(*args, None, ctypes.byref(size)), "ref_support_expansion_plan")
+    blob = np.zeros(size.value)
+    _check(lib, lib.ref_support_expansion_plan(*args, _d(blob), ctypes.byref(size)), "ref_support_expansion_plan")
     return blob
 
 

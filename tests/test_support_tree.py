@@ -119,3 +119,54 @@ def test_support_near_plan_rejects_wall_face():
         bem.support_near_plan(mesh, [3])
     with pytest.raises(ValueError):
         bem.support_near_plan(mesh, [1, 1])
+
+
+EXPANSION_CASES = [
+    # (name, mesh, active faces, frequency Hz, seed, tree options)
+    ("horn12-500Hz", lambda: meshes.exponential_horn(0.012), [1], 500.0, 1e-7, {"leaf_capacity": 16}),
+    ("horn12-16kHz", lambda: meshes.exponential_horn(0.012), [1], 16000.0, 1e-7, {"leaf_capacity": 16}),
+    ("horn12-4kHz-seed4", lambda: meshes.exponential_horn(0.012), [], 4000.0, 1e-4, {"leaf_capacity": 16}),
+    ("ico3-2kHz", lambda: _banded(meshes.icosphere(3)), [1, 2], 2000.0, 1e-6, {"leaf_capacity": 8}),
+    # horn scaled x4 at 16 kHz: plane-wave (HF) coarse levels above spherical (LF) fine levels
+    ("horn12x4-16kHz", lambda: _scaled(meshes.exponential_horn(0.012), 4.0), [1], 16000.0, 1e-4,
+     {"leaf_capacity": 16}),
+]
+
+
+def _scaled(mesh, s):
+    return bem.SurfaceMesh(mesh.vertices * s, mesh.triangles, mesh.face_ids, mesh.boundary)
+
+
+@needs_ref
+@pytest.mark.parametrize("name,make,faces,f,seed,opts", EXPANSION_CASES, ids=[c[0] for c in EXPANSION_CASES])
+def test_support_expansion_plan_bit_identical_to_reference(name, make, faces, f, seed, opts):
+    mesh = make()
+    k = 2 * np.pi * f / 343.0
+    mine = bem.support_expansion_plan(mesh, faces, k, seed, **opts)
+    ref = ob.ref_support_expansion_plan(mesh, faces, k, seed, **opts)
+    assert np.array_equal(mine["blob"], ref)
+    reps = [int(l["representation"]) for l in mine["levels"]]
+    print(name, [(int(l["representation"]), int(l["signature_order"])) for l in mine["levels"]])
+    assert set(reps) <= {0, 1, 2}
+    # once a level is spherical (1), every finer level stays spherical
+    first = next((i for i, r in enumerate(reps) if r == 1), len(reps))
+    assert all(r == 1 for r in reps[first:])
+    if name == "horn12x4-16kHz":
+        assert reps.count(2) == 2
+
+
+def test_level_order_formula_closed_form():
+    """Gumerov-Duraiswami (2009) Eq. 18 at a hand-computed point."""
+    mesh = meshes.exponential_horn(0.012)
+    plan = bem.support_expansion_plan(mesh, [1], 2 * np.pi * 16000 / 343.0, 1e-7, leaf_capacity=16)
+    for l in plan["levels"]:
+        if l["representation"] == 0:
+            continue
+        eps, delta, kr = l["accuracy_seed"], l["policy_separation"], l["electrical_radius"]
+        low = 1 - (np.log(eps) + 1.5 * np.log1p(-1 / delta)) / np.log(delta)
+        high = kr + 0.5 * (3 * -np.log(eps)) ** (2 / 3) * np.cbrt(kr)
+        top = max(low, high)
+        p = top * ((low / top) ** 4 + (high / top) ** 4) ** 0.25
+        assert l["signature_order"] == np.ceil(p)
+        assert l["translation_order"] == l["quadrature_order"] == 2 * l["signature_order"]
+        assert l["angular_samples"] == 2 * (l["quadrature_order"] + 1) ** 2
