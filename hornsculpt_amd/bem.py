@@ -132,6 +132,13 @@ def library() -> ctypes.CDLL:
                                                                       U32, ctypes.POINTER(U32), U32, U32, U32,
                                                                       ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                                                       ctypes.c_double, D, ctypes.POINTER(U64)]),
+                "hsbem_near_operator_create": (ctypes.c_int, [P, ctypes.POINTER(_Numerics), ctypes.c_double,
+                                                              ctypes.c_double, U32, U32, ctypes.c_double, ctypes.c_double,
+                                                              ctypes.POINTER(U32), U32, D, ctypes.POINTER(P)]),
+                "hsbem_near_operator_sizes": (ctypes.c_int, [P, ctypes.POINTER(U64), D]),
+                "hsbem_near_operator_copy": (ctypes.c_int, [P, ctypes.POINTER(U32), ctypes.POINTER(U32), D,
+                                                            ctypes.POINTER(U32), ctypes.POINTER(U32), D, D]),
+                "hsbem_near_operator_destroy": (ctypes.c_int, [P]),
                 "hsbem_support_tree_build": (ctypes.c_int, [D, U32, ctypes.POINTER(U32), U32, U32, U32,
                                                             ctypes.c_double, ctypes.c_double, D, ctypes.POINTER(U64)]),
                 "hsbem_plan_copy": (ctypes.c_int, [P, ctypes.POINTER(U32), ctypes.POINTER(U32), D,
@@ -417,6 +424,42 @@ class HipBemSolver:
             lib.hsbem_result_destroy(out)
 
     # -------- internal seams (component parity tests and benchmarks)
+
+    def near_operator(self, numerics: BemNumerics, k_max: float, k: float, active_faces: Sequence[int],
+                      leaf_capacity: int = 32, maximum_depth: int = 20, separation: float = 1.5,
+                      near_ratio: float = 2.0, x: Optional[np.ndarray] = None) -> dict:
+        """Exact near part of the FMM operator split (device): CSR structure and
+        complex values of A_near (P1 x P1) and B_near (P1 x active faces), mesh
+        vertex order, rows scaled by 1/lumped mass; ``applied`` = A_near x."""
+        lib = self._lib
+        n = numerics.native()
+        faces = np.ascontiguousarray(active_faces if len(active_faces) else [0], dtype=np.uint32)
+        xv = None if x is None else np.ascontiguousarray(np.asarray(x, np.complex128)).view(np.float64)
+        handle = ctypes.c_void_p()
+        _check(lib.hsbem_near_operator_create(self.handle, ctypes.byref(n), k_max, k, leaf_capacity, maximum_depth,
+                                              separation, near_ratio, _uptr(faces), len(active_faces),
+                                              _dptr(xv) if xv is not None else ctypes.POINTER(ctypes.c_double)(),
+                                              ctypes.byref(handle)))
+        try:
+            sizes = np.zeros(8, np.uint64)
+            ms = ctypes.c_double()
+            _check(lib.hsbem_near_operator_sizes(handle, sizes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                                 ctypes.byref(ms)))
+            rows, nnz, fnnz = int(sizes[0]), int(sizes[1]), int(sizes[2])
+            ro, co = np.zeros(rows, np.uint32), np.zeros(max(nnz, 1), np.uint32)
+            fro, fco = np.zeros(rows, np.uint32), np.zeros(max(fnnz, 1), np.uint32)
+            va, fva = np.zeros(2 * max(nnz, 1)), np.zeros(2 * max(fnnz, 1))
+            applied = np.zeros(2 * (rows - 1)) if x is not None else None
+            _check(lib.hsbem_near_operator_copy(handle, _uptr(ro), _uptr(co), _dptr(va), _uptr(fro), _uptr(fco),
+                                                _dptr(fva), _dptr(applied) if applied is not None else
+                                                ctypes.POINTER(ctypes.c_double)()))
+            return {"row_offsets": ro, "columns": co[:nnz], "values": va.view(np.complex128)[:nnz],
+                    "flux_offsets": fro, "flux_columns": fco[:fnnz], "flux_values": fva.view(np.complex128)[:fnnz],
+                    "applied": applied.view(np.complex128) if applied is not None else None,
+                    "counts": dict(zip(("jobs", "regular", "touching", "patches", "fan_jobs"),
+                                       (int(v) for v in sizes[3:]))), "kernel_ms": ms.value}
+        finally:
+            lib.hsbem_near_operator_destroy(handle)
 
     def plan(self, numerics: BemNumerics, k_max: float) -> dict:
         lib = self._lib

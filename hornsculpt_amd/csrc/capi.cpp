@@ -8,6 +8,7 @@
 #include "server.hpp"
 #include "solver.hpp"
 
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -20,6 +21,10 @@ struct hsbem_mesh {
     hsbem_context *context;
     std::shared_ptr<hsbem::MeshSolver> solver; // shared with serving tickets in flight
     std::uint64_t hash;
+};
+struct hsbem_near_operator {
+    hsbem::NearOperator op;
+    std::uint32_t nv;
 };
 struct hsbem_server {
     std::unique_ptr<hsbem::Server> server;
@@ -615,6 +620,65 @@ int hsbem_support_expansion_plan_build(const double *vertices, uint32_t nv, cons
         }
         *blob_size = b.size();
     });
+}
+
+int hsbem_near_operator_create(hsbem_mesh *m, const hsbem_numerics *n, double k_max, double k, uint32_t leaf_capacity,
+                               uint32_t maximum_depth, double separation, double near_ratio, const uint32_t *faces,
+                               uint32_t nfaces, const double *x, hsbem_near_operator **out) {
+    return guarded([&] {
+        require(m && n && out && (faces || !nfaces), "invalid near operator arguments");
+        require(std::isfinite(k) && k > 0 && std::isfinite(k_max) && k_max > 0, "invalid wavenumber");
+        hsbem::SupportTreeSettings ts;
+        ts.leaf_capacity = leaf_capacity;
+        ts.maximum_depth = maximum_depth;
+        ts.separation = separation;
+        ts.near_ratio = near_ratio;
+        hsbem::hip_check(hipSetDevice(m->context->ctx.device), "set device");
+        auto op = std::make_unique<hsbem_near_operator>();
+        op->nv = m->solver->nv();
+        op->op = m->solver->near_operator(to_numerics(*n), k_max, k, ts, std::vector<std::uint32_t>(faces, faces + nfaces), x);
+        *out = op.release();
+    });
+}
+
+int hsbem_near_operator_sizes(const hsbem_near_operator *o, uint64_t *s, double *ms) {
+    return guarded([&] {
+        require(o && s, "null near operator");
+        const auto &op = o->op;
+        const uint64_t v[8] = {op.row_offsets.size(), op.columns.size(), op.flux_columns.size(), op.jobs,
+                               op.regular, op.touching, op.patches, op.fan_jobs};
+        std::copy(v, v + 8, s);
+        if (ms)
+            *ms = op.kernel_ms;
+    });
+}
+
+int hsbem_near_operator_copy(const hsbem_near_operator *o, uint32_t *ro, uint32_t *co, double *va, uint32_t *fro,
+                             uint32_t *fco, double *fva, double *applied) {
+    return guarded([&] {
+        require(o, "null near operator");
+        const auto &op = o->op;
+        if (ro)
+            std::copy(op.row_offsets.begin(), op.row_offsets.end(), ro);
+        if (co)
+            std::copy(op.columns.begin(), op.columns.end(), co);
+        if (va)
+            std::copy(op.values.begin(), op.values.end(), va);
+        if (fro)
+            std::copy(op.flux_offsets.begin(), op.flux_offsets.end(), fro);
+        if (fco)
+            std::copy(op.flux_columns.begin(), op.flux_columns.end(), fco);
+        if (fva)
+            std::copy(op.flux_values.begin(), op.flux_values.end(), fva);
+        if (applied) {
+            require(!op.applied.empty(), "near operator was created without a vector");
+            std::copy(op.applied.begin(), op.applied.end(), applied);
+        }
+    });
+}
+
+int hsbem_near_operator_destroy(hsbem_near_operator *o) {
+    return guarded([&] { delete o; });
 }
 
 int hsbem_plan_sizes(const hsbem_plan *p, uint64_t *s) {

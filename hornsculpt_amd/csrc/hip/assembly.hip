@@ -521,6 +521,60 @@ __device__ __forceinline__ void scatter_pair(const AssemblyArgs &args, M *A, con
     }
 }
 
+// Destination of a pair's moments in the dense path.
+template <class M> struct DenseSink {
+    template <class R>
+    __device__ __forceinline__ void operator()(const AssemblyArgs &args, const MomentsT<R> &m, std::uint32_t,
+                                               std::uint32_t ti, std::uint32_t si) const {
+        scatter_pair(args, matrix_of<M>(args), m, ti, si);
+    }
+};
+
+__device__ __forceinline__ std::uint32_t csr_slot(const std::uint32_t *offsets, const std::uint32_t *columns,
+                                                  std::uint32_t row, std::uint32_t col) {
+    std::uint32_t lo = offsets[row], hi = offsets[row + 1];
+    while (lo < hi) {
+        const std::uint32_t mid = lo + (hi - lo) / 2;
+        if (columns[mid] < col)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo; // the planned entry exists (job masks select planned entries only)
+}
+
+// Near operator: only the planned (masked) local entries of a panel pair,
+// accumulated into the pressure and flux CSR values.
+struct CsrSink {
+    NearCsrView v;
+    template <class R>
+    __device__ __forceinline__ void operator()(const AssemblyArgs &args, const MomentsT<R> &m, std::uint32_t job,
+                                               std::uint32_t ti, std::uint32_t si) const {
+        const PanelView &pv = args.panels;
+        const double inv_k = 1.0 / args.k;
+        const std::uint32_t mask = v.masks[job];
+        const std::uint32_t fc = v.face_column[si];
+        for (int i = 0; i < 3; ++i) {
+            const std::uint32_t row = pv.nodes[static_cast<std::size_t>(i) * pv.nt + ti];
+            const double rs = args.row_scale[row];
+            for (int j = 0; j < 3; ++j) {
+                if (!(mask >> (3 * i + j) & 1u))
+                    continue;
+                const std::uint32_t col = pv.nodes[static_cast<std::size_t>(j) * pv.nt + si];
+                double cc = 0;
+                for (int c = 0; c < 3; ++c)
+                    cc += g_at(pv.geo, G_C0 + 3 * i + c, pv.nt, ti) * g_at(pv.geo, G_C0 + 3 * j + c, pv.nt, si);
+                const double re = static_cast<double>(m.a[3 * i + j].re) + static_cast<double>(m.g.im) * cc * inv_k;
+                const double im = static_cast<double>(m.a[3 * i + j].im) - static_cast<double>(m.g.re) * cc * inv_k;
+                atomic_add_entry(v.values + 2 * csr_slot(v.row_offsets, v.columns, row, col), re * rs, im * rs);
+            }
+            if ((mask >> (9 + i) & 1u) && fc != UINT32_MAX)
+                atomic_add_entry(v.flux_values + 2 * csr_slot(v.flux_offsets, v.flux_columns, row, fc),
+                                 static_cast<double>(m.b[i].re) * rs, static_cast<double>(m.b[i].im) * rs);
+        }
+    }
+};
+
 __device__ __forceinline__ void load_geo(const PanelView &pv, std::uint32_t t, double *g) {
 #pragma unroll
     for (int f = 0; f < G_COUNT; ++f)
@@ -529,9 +583,9 @@ __device__ __forceinline__ void load_geo(const PanelView &pv, std::uint32_t t, d
 
 // Touching pairs are integrated in FP64 in both forms (the hypersingular weak
 // form); M is the matrix element type.
-template <class M>
+template <class Sink>
 __global__ __launch_bounds__(256) void touching_jobs(AssemblyArgs args, const TouchingJob *jobs, std::uint32_t njobs,
-                                                     const double *gauss, unsigned order) {
+                                                     const double *gauss, unsigned order, Sink sink) {
     const std::uint32_t job = blockIdx.x * 4 + threadIdx.x / 64;
     const int lane = threadIdx.x & 63;
     if (job >= njobs)
@@ -587,7 +641,7 @@ __global__ __launch_bounds__(256) void touching_jobs(AssemblyArgs args, const To
         atomicExch(args.error, ERR_DISTANCE);
     wave_reduce_t(m);
     if (lane == 0)
-        scatter_pair(args, matrix_of<M>(args), m, tj.test, tj.trial);
+        sink(args, m, job, tj.test, tj.trial);
 }
 
 __device__ __forceinline__ FanPanel fan_of(const double *g) {
@@ -601,9 +655,9 @@ __device__ __forceinline__ FanPanel fan_of(const double *g) {
 }
 
 // Patch maps in FP64; samples and accumulation in R; matrix element type R.
-template <class R>
+template <class R, class Sink>
 __global__ __launch_bounds__(256) void near_jobs(AssemblyArgs args, const NearJob *jobs, std::uint32_t njobs,
-                                                 const PairPatch *patches, const double *rules) {
+                                                 const PairPatch *patches, const double *rules, Sink sink) {
     const std::uint32_t job = blockIdx.x * 4 + threadIdx.x / 64;
     const int lane = threadIdx.x & 63;
     if (job >= njobs)
@@ -655,13 +709,13 @@ __global__ __launch_bounds__(256) void near_jobs(AssemblyArgs args, const NearJo
         atomicExch(args.error, ERR_DISTANCE);
     wave_reduce_t(m);
     if (lane == 0)
-        scatter_pair(args, matrix_of<R>(args), m, nj.test, nj.trial);
+        sink(args, m, job, nj.test, nj.trial);
 }
 
 // Tensor near patches, one per thread (degree-8 items first, then degree 14).
-template <class R>
+template <class R, class Sink>
 __global__ __launch_bounds__(256) void near_patches(AssemblyArgs args, const NearPatchItem *items, std::uint32_t count,
-                                                    const PairPatch *patches, const double *rules) {
+                                                    const PairPatch *patches, const double *rules, Sink sink) {
     __shared__ R srule[4 * (16 + 64)];
     for (int e = threadIdx.x; e < 4 * 16; e += blockDim.x)
         srule[e] = static_cast<R>(rules[4 * kRule8 + e]);
@@ -688,7 +742,7 @@ __global__ __launch_bounds__(256) void near_patches(AssemblyArgs args, const Nea
     to_parent(m, Tt, Ts);
     if (bad)
         atomicExch(args.error, ERR_DISTANCE);
-    scatter_pair(args, matrix_of<R>(args), m, it.test, it.trial);
+    sink(args, m, idx, it.test, it.trial);
 }
 
 template <class M> __global__ void mass_terms(AssemblyArgs args) {
@@ -731,7 +785,135 @@ __global__ void flux_rhs(const double *BD, std::uint32_t nv, std::uint32_t nd, c
     rhs[2 * idx + 1] = im;
 }
 
+// Regular (disjoint, not near) pairs of a list, one thread per pair: the same
+// rule selection as regular_tiles (all-pairs degree byte, else the qualified
+// degree at the solve wavenumber) and FP64 contraction.
+template <class Sink>
+__global__ __launch_bounds__(256) void regular_pairs(AssemblyArgs args, NearView near, const std::uint32_t *pairs,
+                                                     std::uint32_t count, const double *rules, Sink sink) {
+    __shared__ double srule[4 * kRuleTableCount];
+    for (int e = threadIdx.x; e < 4 * kRuleTableCount; e += blockDim.x)
+        srule[e] = rules[e];
+    __syncthreads();
+    const std::uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count)
+        return;
+    const std::uint32_t ti = pairs[2 * idx], si = pairs[2 * idx + 1];
+    const PanelView &pv = args.panels;
+    int offset = -1, n = 0;
+    if (near.whole) {
+        const unsigned d = near.whole[unordered_pair_index(ti, si)];
+        if (d) {
+            offset = d == 14 ? kRule14 : kRule8;
+            n = d == 14 ? 64 : 16;
+        }
+    }
+    if (offset < 0) {
+        PanelBounds bt, bs;
+        bt.center = {pv.bounds[static_cast<std::size_t>(B_C) * pv.nt + ti], pv.bounds[static_cast<std::size_t>(B_C + 1) * pv.nt + ti],
+                     pv.bounds[static_cast<std::size_t>(B_C + 2) * pv.nt + ti]};
+        bt.radius = pv.bounds[static_cast<std::size_t>(B_R) * pv.nt + ti];
+        bt.max_edge = pv.bounds[static_cast<std::size_t>(B_H) * pv.nt + ti];
+        bt.qualified_shape = pv.shape[ti];
+        bs.center = {pv.bounds[static_cast<std::size_t>(B_C) * pv.nt + si], pv.bounds[static_cast<std::size_t>(B_C + 1) * pv.nt + si],
+                     pv.bounds[static_cast<std::size_t>(B_C + 2) * pv.nt + si]};
+        bs.radius = pv.bounds[static_cast<std::size_t>(B_R) * pv.nt + si];
+        bs.max_edge = pv.bounds[static_cast<std::size_t>(B_H) * pv.nt + si];
+        bs.qualified_shape = pv.shape[si];
+        const unsigned d = qualified_regular_degree(bt, bs, args.k, args.tolerance, args.regular_order);
+        if (!d) {
+            atomicExch(args.error, ERR_ESCAPED);
+            return;
+        }
+        offset = d == 4 ? kRule4 : d == 6 ? kRule6 : kRule8;
+        n = d == 4 ? 6 : d == 6 ? 12 : 16;
+    }
+    double gt[G_COUNT], gs[G_COUNT];
+    load_geo(pv, ti, gt);
+    load_geo(pv, si, gs);
+    MomentsT<double> m;
+    zero_t(m);
+    bool bad = false;
+    contract<double>(m, frame_of(gt), frame_of(gs), 4.0 * gt[G_AREA] * gs[G_AREA] * 0.07957747154594766788,
+                     srule + 4 * offset, n, args.k, bad);
+    if (bad)
+        atomicExch(args.error, ERR_DISTANCE);
+    sink(args, m, idx, ti, si);
+}
+
+// Mass terms into the near CSR: 1/2 M_pp (every panel's 3 x 3 block is near)
+// and -alpha 1/2 M_pq into the active face columns.
+__global__ void mass_csr(AssemblyArgs args, NearCsrView v) {
+    const std::uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const PanelView &pv = args.panels;
+    if (t >= pv.nt)
+        return;
+    const double area = g_at(pv.geo, G_AREA, pv.nt, t);
+    const std::uint32_t fc = v.face_column[t];
+    for (int i = 0; i < 3; ++i) {
+        const std::uint32_t row = pv.nodes[static_cast<std::size_t>(i) * pv.nt + t];
+        const double rs = args.row_scale[row];
+        for (int j = 0; j < 3; ++j) {
+            const std::uint32_t col = pv.nodes[static_cast<std::size_t>(j) * pv.nt + t];
+            unsafeAtomicAdd(v.values + 2 * csr_slot(v.row_offsets, v.columns, row, col),
+                            rs * area * (i == j ? 2.0 : 1.0) / 24.0);
+        }
+        if (fc != UINT32_MAX)
+            unsafeAtomicAdd(v.flux_values + 2 * csr_slot(v.flux_offsets, v.flux_columns, row, fc) + 1,
+                            area / (6.0 * args.k) * rs);
+    }
+}
+
+// y = A x on the pressure CSR: one wavefront per row, lanes over columns.
+__global__ __launch_bounds__(256) void csr_apply(const std::uint32_t *offsets, const std::uint32_t *columns,
+                                                 const double *values, std::uint32_t rows, const double *x, double *y) {
+    const std::uint32_t row = blockIdx.x * 4 + threadIdx.x / 64;
+    const int lane = threadIdx.x & 63;
+    if (row >= rows)
+        return;
+    double re = 0, im = 0;
+    for (std::uint32_t e = offsets[row] + lane; e < offsets[row + 1]; e += 64) {
+        const double ar = values[2 * e], ai = values[2 * e + 1];
+        const std::uint32_t c = columns[e];
+        const double xr = x[2 * c], xi = x[2 * c + 1];
+        re += ar * xr - ai * xi;
+        im += ar * xi + ai * xr;
+    }
+    re = wave_sum_t(re);
+    im = wave_sum_t(im);
+    if (lane == 0) {
+        y[2 * row] = re;
+        y[2 * row + 1] = im;
+    }
+}
+
 } // namespace
+
+void launch_near_operator(const NearOperatorLaunch &L, hipStream_t stream) {
+    NearCsrView v = L.csr;
+    auto with = [&](const std::uint32_t *masks) {
+        v.masks = masks;
+        return CsrSink{v};
+    };
+    if (L.regular_count)
+        hipLaunchKernelGGL(regular_pairs<CsrSink>, dim3((L.regular_count + 255) / 256), dim3(256), 0, stream, L.args,
+                           L.near, L.regular, L.regular_count, L.rules, with(L.regular_masks));
+    if (L.touching_count)
+        hipLaunchKernelGGL(touching_jobs<CsrSink>, dim3((L.touching_count + 3) / 4), dim3(256), 0, stream, L.args,
+                           L.touching, L.touching_count, L.gauss, L.singular_order, with(L.touching_masks));
+    if (L.patch_count)
+        hipLaunchKernelGGL((near_patches<double, CsrSink>), dim3((L.patch_count + 255) / 256), dim3(256), 0, stream,
+                           L.args, L.patch_items, L.patch_count, L.patches, L.rules, with(L.patch_masks));
+    if (L.fan_count)
+        hipLaunchKernelGGL((near_jobs<double, CsrSink>), dim3((L.fan_count + 3) / 4), dim3(256), 0, stream, L.args,
+                           L.fan_jobs, L.fan_count, L.patches, L.rules, with(L.fan_masks));
+    hipLaunchKernelGGL(mass_csr, dim3((L.args.panels.nt + 255) / 256), dim3(256), 0, stream, L.args, L.csr);
+}
+
+void launch_csr_apply(const std::uint32_t *offsets, const std::uint32_t *columns, const double *values,
+                      std::uint32_t rows, const double *x, double *y, hipStream_t stream) {
+    hipLaunchKernelGGL(csr_apply, dim3((rows + 3) / 4), dim3(256), 0, stream, offsets, columns, values, rows, x, y);
+}
 
 void launch_regular(const AssemblyLaunch &L, hipStream_t stream) {
     if (!L.clusters.count)
@@ -750,31 +932,31 @@ void launch_touching(const AssemblyLaunch &L, hipStream_t stream) {
         return;
     const dim3 grid((L.touching_count + 3) / 4);
     if (L.args.A32)
-        hipLaunchKernelGGL(touching_jobs<float>, grid, dim3(256), 0, stream, L.args, L.touching, L.touching_count,
-                           L.gauss, L.singular_order);
+        hipLaunchKernelGGL(touching_jobs<DenseSink<float>>, grid, dim3(256), 0, stream, L.args, L.touching,
+                           L.touching_count, L.gauss, L.singular_order, DenseSink<float>{});
     else
-        hipLaunchKernelGGL(touching_jobs<double>, grid, dim3(256), 0, stream, L.args, L.touching, L.touching_count,
-                           L.gauss, L.singular_order);
+        hipLaunchKernelGGL(touching_jobs<DenseSink<double>>, grid, dim3(256), 0, stream, L.args, L.touching,
+                           L.touching_count, L.gauss, L.singular_order, DenseSink<double>{});
 }
 
 void launch_near(const AssemblyLaunch &L, hipStream_t stream) {
     if (L.near_item_count) {
         const dim3 grid((L.near_item_count + 255) / 256);
         if (L.args.A32)
-            hipLaunchKernelGGL(near_patches<float>, grid, dim3(256), 0, stream, L.args, L.near_items,
-                               L.near_item_count, L.patches, L.rules);
+            hipLaunchKernelGGL((near_patches<float, DenseSink<float>>), grid, dim3(256), 0, stream, L.args,
+                               L.near_items, L.near_item_count, L.patches, L.rules, DenseSink<float>{});
         else
-            hipLaunchKernelGGL(near_patches<double>, grid, dim3(256), 0, stream, L.args, L.near_items,
-                               L.near_item_count, L.patches, L.rules);
+            hipLaunchKernelGGL((near_patches<double, DenseSink<double>>), grid, dim3(256), 0, stream, L.args,
+                               L.near_items, L.near_item_count, L.patches, L.rules, DenseSink<double>{});
     }
     if (L.near_count) {
         const dim3 grid((L.near_count + 3) / 4);
         if (L.args.A32)
-            hipLaunchKernelGGL(near_jobs<float>, grid, dim3(256), 0, stream, L.args, L.near_jobs, L.near_count,
-                               L.patches, L.rules);
+            hipLaunchKernelGGL((near_jobs<float, DenseSink<float>>), grid, dim3(256), 0, stream, L.args, L.near_jobs,
+                               L.near_count, L.patches, L.rules, DenseSink<float>{});
         else
-            hipLaunchKernelGGL(near_jobs<double>, grid, dim3(256), 0, stream, L.args, L.near_jobs, L.near_count,
-                               L.patches, L.rules);
+            hipLaunchKernelGGL((near_jobs<double, DenseSink<double>>), grid, dim3(256), 0, stream, L.args,
+                               L.near_jobs, L.near_count, L.patches, L.rules, DenseSink<double>{});
     }
 }
 

@@ -60,6 +60,42 @@ struct NearPatchItem {
     std::uint32_t test, trial, patch, swap;
 };
 
+// Near operator of the FMM path: exact near entries into CSR (original node
+// numbering, rows scaled by 1/lumped mass like the dense matrix).
+struct NearCsrView {
+    const std::uint32_t *masks;                       // per job: P1/P1 bits 0-8 (3 i + j), P1/DP0 bits 9-11
+    const std::uint32_t *row_offsets, *columns;        // pressure CSR
+    double *values;                                    // complex, nnz
+    const std::uint32_t *flux_offsets, *flux_columns; // P1 x active-face CSR
+    double *flux_values;                              // complex
+    const std::uint32_t *face_column;                 // per panel, UINT32_MAX when inactive
+};
+
+struct NearOperatorLaunch {
+    AssemblyArgs args;                     // panels (original numbering), row_scale, k, tolerance, error
+    NearCsrView csr;
+    NearView near;                         // interaction plan near CSR / all-pairs degrees (qualification)
+    const double *rules;
+    const std::uint32_t *regular;          // (test, trial) pairs
+    std::uint32_t regular_count;
+    const TouchingJob *touching;
+    std::uint32_t touching_count;
+    const double *gauss;
+    unsigned singular_order;
+    const NearJob *fan_jobs;
+    std::uint32_t fan_count;
+    const NearPatchItem *patch_items;
+    std::uint32_t patch_count;
+    const PairPatch *patches;
+    // per-list mask arrays (indexed like each list)
+    const std::uint32_t *regular_masks, *touching_masks, *fan_masks, *patch_masks;
+};
+
+void launch_near_operator(const NearOperatorLaunch &launch, hipStream_t stream);
+// y[r] = sum_c values[r][c] x[c] over the pressure CSR (complex FP64).
+void launch_csr_apply(const std::uint32_t *row_offsets, const std::uint32_t *columns, const double *values,
+                      std::uint32_t rows, const double *x, double *y, hipStream_t stream);
+
 struct AssemblyLaunch {
     AssemblyArgs args;
     ClusterView clusters;

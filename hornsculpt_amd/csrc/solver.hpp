@@ -8,6 +8,7 @@
 
 #include "host/plan.hpp"
 #include "host/space.hpp"
+#include "host/support_tree.hpp"
 #include "hip/kernels.hpp"
 
 #include <hip/hip_runtime_api.h>
@@ -201,6 +202,16 @@ struct PreparedRequest {
     std::uint64_t footprint{}; // device bytes a workspace holds for one frequency point
 };
 
+// Exact near part of the FMM operator split (pressure CSR over P1 x P1 and
+// flux CSR over P1 x active faces), in mesh vertex order, rows scaled by
+// 1/lumped mass; `applied` = A_near x when x is given.
+struct NearOperator {
+    std::vector<std::uint32_t> row_offsets, columns, flux_offsets, flux_columns;
+    std::vector<double> values, flux_values, applied; // complex
+    std::size_t jobs{}, regular{}, touching{}, patches{}, fan_jobs{};
+    double kernel_ms{};
+};
+
 class MeshSolver {
   public:
     MeshSolver(Context *ctx, SurfaceMesh mesh);
@@ -213,6 +224,8 @@ class MeshSolver {
                                     const std::function<void(std::int32_t)> &report = {});
     // Component seams.
     InteractionPlan plan(const Numerics &numerics, double k_max);
+    NearOperator near_operator(const Numerics &numerics, double k_max, double k, const SupportTreeSettings &tree,
+                               const std::vector<std::uint32_t> &active_faces, const double *x);
     void assemble_dense(const Numerics &numerics, double k_max, double k, double *A, double *B);
     std::uint64_t hash() const { return hash_; }
     const Space &space() const { return space_; }
@@ -245,6 +258,8 @@ class MeshSolver {
     std::map<std::tuple<unsigned, unsigned, double, double, unsigned, double>, std::shared_ptr<DevicePlan>> plans_;
     std::mutex plan_mutex_;
     std::unique_ptr<Workspace> workspace_; // direct solve path (context stream), reused across calls
+    DeviceBuffer<std::uint32_t> original_nodes_; // 3 x nt, mesh vertex ids (near operator)
+    DeviceBuffer<double> original_row_scale_;
     std::mutex mutex_;                     // one direct solve per mesh at a time (as the reference)
 };
 

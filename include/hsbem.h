@@ -65,6 +65,7 @@ typedef struct hsbem_mesh hsbem_mesh;
 typedef struct hsbem_result hsbem_result;
 typedef struct hsbem_plan hsbem_plan;
 typedef struct hsbem_server hsbem_server;
+typedef struct hsbem_near_operator hsbem_near_operator;
 
 /* BemNumerics (bem.hpp:113-123) plus the MI355X solver choices. */
 typedef struct {
@@ -264,6 +265,25 @@ HSBEM_API int hsbem_support_expansion_plan_build(const double *vertices, uint32_
                                                  double separation_ratio, double near_distance_ratio,
                                                  double wavenumber, double accuracy_seed, double *blob,
                                                  uint64_t *blob_size);
+
+/* Near part of the FMM operator split on the device (replaces
+ * SupportNearOperator::assemble/applyPressureDevice,
+ * modules/bem/cuda/support_near_operator.hpp:13-95): the exact near entries of
+ * the row-scaled A (P1 x P1 CSR) and B (P1 x active-face CSR, per unit face
+ * flux) at wavenumber k, for the support tree of the given settings, in mesh
+ * vertex order (the mesh is recentred like hsbem_mesh_create). x (optional,
+ * nv complex) is multiplied by A_near on the device.
+ * sizes[8]: nv + 1, nnz, flux nnz, near jobs, regular, touching, tensor
+ * patches, projection-fan jobs. */
+HSBEM_API int hsbem_near_operator_create(hsbem_mesh *mesh, const hsbem_numerics *numerics, double k_max, double k,
+                                         uint32_t leaf_capacity, uint32_t maximum_depth, double separation_ratio,
+                                         double near_distance_ratio, const uint32_t *active_faces,
+                                         uint32_t active_face_count, const double *x, hsbem_near_operator **out);
+HSBEM_API int hsbem_near_operator_sizes(const hsbem_near_operator *op, uint64_t *sizes, double *kernel_milliseconds);
+HSBEM_API int hsbem_near_operator_copy(const hsbem_near_operator *op, uint32_t *row_offsets, uint32_t *columns,
+                                       double *values, uint32_t *flux_offsets, uint32_t *flux_columns,
+                                       double *flux_values, double *applied);
+HSBEM_API int hsbem_near_operator_destroy(hsbem_near_operator *op);
 
 /* Device assembly of the row-scaled Galerkin system at wavenumber k with the
  * plan for k_max: A (nv x nv) and B (nv x nt, every panel a flux column), both
