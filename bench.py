@@ -46,6 +46,10 @@ FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 (vector and matrix)
 FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 vector (published; half of it without packed FP32 instructions)
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 FLOPS_PER_SAMPLE = 150.0     # FP64 arithmetic of one Burton-Miller pair sample (sqrt/sincos/div not counted)
+# HBM bytes per regular_tiles launch on the default horn mesh (edge 5.2 mm), from rocprofv3
+# FETCH_SIZE + WRITE_SIZE runs scaled by a 1 GiB copy calibration (fetch x2.000, write x1.000):
+# most of it is the node-tile flush through L2 atomics (VALU-bound kernel).
+REGULAR_TRAFFIC_BYTES = {"fp32": (3579.8 + 11537.6) * 1e6, "fp64": (2489.1 + 15827.3) * 1e6}
 
 
 def device_synchronize():
@@ -286,11 +290,12 @@ def main():
         "mixed_vs_fp64": ({"form": head_name, "speedup": head["value"] / forms["fp64"]["value"],
                            "max_rel_l2_surface_pressure": mixed_error} if mixed_error is not None else None),
         "cold": {"mesh_setup_s": construct_s,
-                 "interaction_plan_ms": next(f["plan_ms"] for f in forms.values() if f["plan_ms"] is not None)},
+                 "interaction_plan_ms": next((f["plan_ms"] for f in forms.values() if f["plan_ms"] is not None), None)},
         "lu_cross_check_rel_l2": cross,
         "roofline": {"kernel": f"regular_tiles ({'FP32' if mixed_head else 'FP64'} pair quadrature, VALU)",
                      "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak if achieved else None, "traffic": None,
+                     "frac": achieved / peak if achieved else None,
+                     "traffic": REGULAR_TRAFFIC_BYTES["fp32" if mixed_head else "fp64"] if args.edge == 0.0052 else None,
                      "launch_ms": roof["kernel_ms"] / roof["points"],
                      "flops_per_launch": FLOPS_PER_SAMPLE * roof["samples"] / roof["points"]},
     }
