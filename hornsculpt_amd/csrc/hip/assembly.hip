@@ -156,10 +156,10 @@ __device__ __forceinline__ double g_at(const double *geo, int field, std::uint32
 
 // Sample core of a disjoint pair (regular pairs and tensor near patches), from
 // the squared distance r2, the numerators fyn = d.n_y and dntk = (d.n_x)/k
-// already formed: same algebra as bm_sample.
+// already formed: same algebra as bm_sample. A zero, negative or non-finite r2
+// yields non-finite moments, which the caller checks once per pair.
 __device__ __forceinline__ SampleT<float> disjoint_sample(float r2, float wq, float fyn, float dntk, float k,
-                                                          float k_rev, float knn, bool &bad) {
-    bad |= !(r2 > 0.f) || !isfinite(r2);
+                                                          float k_rev, float knn) {
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * rinv;
     const float f = __builtin_amdgcn_fractf(r * k_rev);
@@ -175,8 +175,7 @@ __device__ __forceinline__ SampleT<float> disjoint_sample(float r2, float wq, fl
 }
 
 __device__ __forceinline__ SampleT<double> disjoint_sample(double r2, double wq, double fyn, double dntk, double k,
-                                                           double, double knn, bool &bad) {
-    bad |= !(r2 > 0.0) || !isfinite(r2);
+                                                           double, double knn) {
     const double rinv = rsqrt_full(r2);
     const double r = r2 * rinv, kr = k * r;
     double sn, cs;
@@ -218,10 +217,13 @@ __device__ __forceinline__ double dot3(const double *a, const double *b) { retur
 //   d.n_y = (b + l_x).n_y (l_y is orthogonal to n_y): constant in the inner loop;
 //   d.n_x = b.n_x - l_y.n_x (l_x is orthogonal to n_x).
 // Pair constants are formed in FP64 and rounded to R; wscale = 4 A_x A_y / (4 pi)
-// (times the area fraction of a near patch).
-template <class R>
+// (times the area fraction of a near patch). N > 0 fixes the rule size at compile
+// time (uniform far tiles); N = 0 takes n. bad is raised when the sum of G is not
+// finite (a zero or invalid distance somewhere in the pair).
+template <class R, int N = 0>
 __device__ __forceinline__ void contract(MomentsT<R> &m, const Frame &t, const Frame &s, double wscale_d,
-                                         const R *rule, int n, double k, bool &bad) {
+                                         const R *rule, int n_runtime, double k, bool &bad) {
+    const int n = N > 0 ? N : n_runtime;
     const double b[3] = {t.v0[0] - s.v0[0], t.v0[1] - s.v0[1], t.v0[2] - s.v0[2]};
     const double inv_k = 1.0 / k;
     const R Q11 = static_cast<R>(dot3(s.e1, s.e1)), Q12 = static_cast<R>(2.0 * dot3(s.e1, s.e2)),
@@ -241,6 +243,7 @@ __device__ __forceinline__ void contract(MomentsT<R> &m, const Frame &t, const F
         se1[i] = static_cast<R>(s.e1[i]);
         se2[i] = static_cast<R>(s.e2[i]);
     }
+#pragma unroll 1
     for (int xi = 0; xi < n; ++xi) {
         const R *xp = rule + 4 * xi;
         const R x1 = xp[1], x2 = xp[2], xw = xp[3] * wscale;
@@ -253,13 +256,14 @@ __device__ __forceinline__ void contract(MomentsT<R> &m, const Frame &t, const F
         const R B = blt[0] * blt[0] + blt[1] * blt[1] + blt[2] * blt[2];
         const R fyn = Bn + x1 * T1 + x2 * T2;
         CplxT<R> ga[3] = {{0, 0}, {0, 0}, {0, 0}}, gg = {0, 0}, gb = {0, 0};
+#pragma unroll 1
         for (int yi = 0; yi < n; ++yi) {
             const R *yp = rule + 4 * yi;
             const R y0 = yp[0], y1 = yp[1], y2 = yp[2];
             const R t1 = y1 * Q11 + (y2 * Q12 + P1), t2 = y2 * Q22 + P2;
             const R r2 = y1 * t1 + (y2 * t2 + B);
             const R dntk = C0 - (y1 * A1 + y2 * A2);
-            const SampleT<R> smp = disjoint_sample(r2, xw * yp[3], fyn, dntk, kk, k_rev, knn, bad);
+            const SampleT<R> smp = disjoint_sample(r2, xw * yp[3], fyn, dntk, kk, k_rev, knn);
             ga[0] = cadd_t(ga[0], cscale_t(smp.a, y0));
             ga[1] = cadd_t(ga[1], cscale_t(smp.a, y1));
             ga[2] = cadd_t(ga[2], cscale_t(smp.a, y2));
@@ -276,6 +280,7 @@ __device__ __forceinline__ void contract(MomentsT<R> &m, const Frame &t, const F
                 m.a[3 * i + j] = cadd_t(m.a[3 * i + j], cscale_t(ga[j], xb));
         }
     }
+    bad |= !isfinite(m.g.re) || !isfinite(m.g.im);
 }
 
 // Sub-triangle of a panel with vertices at parent barycentrics T[l] (3 x 3,
@@ -331,9 +336,11 @@ template <> __device__ __forceinline__ float *matrix_of<float>(const AssemblyArg
 
 // R: sample/tile precision; the matrix element type is R as well.
 // Occupancy: the FP32 variant fits two 512-thread workgroups per CU (<= 128 VGPRs).
-template <class R>
+// FAR: the tile list holds far tiles only (every pair disjoint, not near, degree 4
+// at this wavenumber): one uniform 6 x 6 rule and no per-pair checks.
+template <class R, bool FAR>
 __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2)))
-void regular_tiles(AssemblyArgs args, ClusterView cl, NearView near, const double *rules) {
+void regular_tiles(AssemblyArgs args, ClusterView cl, NearView near, const double *rules, const std::uint32_t *tiles) {
     constexpr int stride = 2 * kTileNodes + 2; // R per trial-node row (padded)
     __shared__ R tile[kTileNodes * stride];
     __shared__ double sgt[G_COUNT][kTilePanels], sgs[G_COUNT][kTilePanels];
@@ -344,7 +351,7 @@ void regular_tiles(AssemblyArgs args, ClusterView cl, NearView near, const doubl
     __shared__ std::int32_t sfc[kTilePanels];
     __shared__ R srule[4 * kRuleTableCount];
 
-    const std::uint32_t ct = blockIdx.y, cs = blockIdx.x;
+    const std::uint32_t ct = tiles[2 * blockIdx.x], cs = tiles[2 * blockIdx.x + 1];
     const std::uint32_t t0 = cl.panel_offsets[ct], nT = cl.panel_offsets[ct + 1] - t0;
     const std::uint32_t s0 = cl.panel_offsets[cs], nS = cl.panel_offsets[cs + 1] - s0;
     const std::uint32_t a0 = cl.node_offsets[ct], nA = cl.node_offsets[ct + 1] - a0;
@@ -400,43 +407,46 @@ void regular_tiles(AssemblyArgs args, ClusterView cl, NearView near, const doubl
     bool bad = false;
     for (std::uint32_t p = tid; p < pairs; p += kTileThreads) {
         const std::uint32_t lt = p % nT, ls = p / nT;
-        bool touching = false;
+        int offset = kRule4, count = 6;
+        if constexpr (!FAR) {
+            bool touching = false;
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+            for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
-                touching |= snt[i][lt] == sns[j][ls];
-        if (touching)
-            continue;
-        const std::uint32_t ti = sidt[lt], si = sids[ls];
-        const std::uint32_t nb = near.offsets[ti];
-        if (contains_sorted(near.trials + nb, near.offsets[ti + 1] - nb, si))
-            continue;
-        int offset = -1, count = 0;
-        if (near.whole) {
-            const unsigned d = near.whole[unordered_pair_index(ti, si)];
-            if (d) {
-                offset = d == 14 ? kRule14 : kRule8;
-                count = d == 14 ? 64 : 16;
-            }
-        }
-        if (offset < 0) {
-            PanelBounds bt, bs;
-            bt.center = {sbt[B_C][lt], sbt[B_C + 1][lt], sbt[B_C + 2][lt]};
-            bt.radius = sbt[B_R][lt];
-            bt.max_edge = sbt[B_H][lt];
-            bt.qualified_shape = shpt[lt];
-            bs.center = {sbs[B_C][ls], sbs[B_C + 1][ls], sbs[B_C + 2][ls]};
-            bs.radius = sbs[B_R][ls];
-            bs.max_edge = sbs[B_H][ls];
-            bs.qualified_shape = shps[ls];
-            const unsigned d = qualified_regular_degree(bt, bs, k, args.tolerance, args.regular_order);
-            if (!d) {
-                atomicExch(args.error, ERR_ESCAPED);
+                for (int j = 0; j < 3; ++j)
+                    touching |= snt[i][lt] == sns[j][ls];
+            if (touching)
                 continue;
+            const std::uint32_t ti = sidt[lt], si = sids[ls];
+            const std::uint32_t nb = near.offsets[ti];
+            if (contains_sorted(near.trials + nb, near.offsets[ti + 1] - nb, si))
+                continue;
+            offset = -1;
+            if (near.whole) {
+                const unsigned d = near.whole[unordered_pair_index(ti, si)];
+                if (d) {
+                    offset = d == 14 ? kRule14 : kRule8;
+                    count = d == 14 ? 64 : 16;
+                }
             }
-            offset = d == 4 ? kRule4 : d == 6 ? kRule6 : kRule8;
-            count = d == 4 ? 6 : d == 6 ? 12 : 16;
+            if (offset < 0) {
+                PanelBounds bt, bs;
+                bt.center = {sbt[B_C][lt], sbt[B_C + 1][lt], sbt[B_C + 2][lt]};
+                bt.radius = sbt[B_R][lt];
+                bt.max_edge = sbt[B_H][lt];
+                bt.qualified_shape = shpt[lt];
+                bs.center = {sbs[B_C][ls], sbs[B_C + 1][ls], sbs[B_C + 2][ls]};
+                bs.radius = sbs[B_R][ls];
+                bs.max_edge = sbs[B_H][ls];
+                bs.qualified_shape = shps[ls];
+                const unsigned d = qualified_regular_degree(bt, bs, k, args.tolerance, args.regular_order);
+                if (!d) {
+                    atomicExch(args.error, ERR_ESCAPED);
+                    continue;
+                }
+                offset = d == 4 ? kRule4 : d == 6 ? kRule6 : kRule8;
+                count = d == 4 ? 6 : d == 6 ? 12 : 16;
+            }
         }
         double gt[G_COUNT], gs[G_COUNT];
 #pragma unroll
@@ -446,8 +456,11 @@ void regular_tiles(AssemblyArgs args, ClusterView cl, NearView near, const doubl
         }
         MomentsT<R> m;
         zero_t(m);
-        contract<R>(m, frame_of(gt), frame_of(gs), 4.0 * gt[G_AREA] * gs[G_AREA] * 0.07957747154594766788,
-                    srule + 4 * offset, count, k, bad);
+        const double wscale = 4.0 * gt[G_AREA] * gs[G_AREA] * 0.07957747154594766788;
+        if constexpr (FAR)
+            contract<R, 6>(m, frame_of(gt), frame_of(gs), wscale, srule + 4 * kRule4, 6, k, bad);
+        else
+            contract<R>(m, frame_of(gt), frame_of(gs), wscale, srule + 4 * offset, count, k, bad);
         counted += static_cast<unsigned long long>(count) * count;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -916,15 +929,23 @@ void launch_csr_apply(const std::uint32_t *offsets, const std::uint32_t *columns
 }
 
 void launch_regular(const AssemblyLaunch &L, hipStream_t stream) {
-    if (!L.clusters.count)
-        return;
-    const dim3 grid(L.clusters.count, L.clusters.count);
-    if (L.args.A32)
-        hipLaunchKernelGGL(regular_tiles<float>, grid, dim3(kTileThreads), 0, stream, L.args, L.clusters, L.near,
-                           L.rules);
-    else
-        hipLaunchKernelGGL(regular_tiles<double>, grid, dim3(kTileThreads), 0, stream, L.args, L.clusters, L.near,
-                           L.rules);
+    // tiles[0, far_tiles) take the uniform far path at this wavenumber, the rest the general one
+    const std::uint32_t general = L.tile_count - L.far_tiles;
+    if (L.args.A32) {
+        if (L.far_tiles)
+            hipLaunchKernelGGL((regular_tiles<float, true>), dim3(L.far_tiles), dim3(kTileThreads), 0, stream, L.args,
+                               L.clusters, L.near, L.rules, L.tiles);
+        if (general)
+            hipLaunchKernelGGL((regular_tiles<float, false>), dim3(general), dim3(kTileThreads), 0, stream, L.args,
+                               L.clusters, L.near, L.rules, L.tiles + 2 * static_cast<std::size_t>(L.far_tiles));
+    } else {
+        if (L.far_tiles)
+            hipLaunchKernelGGL((regular_tiles<double, true>), dim3(L.far_tiles), dim3(kTileThreads), 0, stream,
+                               L.args, L.clusters, L.near, L.rules, L.tiles);
+        if (general)
+            hipLaunchKernelGGL((regular_tiles<double, false>), dim3(general), dim3(kTileThreads), 0, stream, L.args,
+                               L.clusters, L.near, L.rules, L.tiles + 2 * static_cast<std::size_t>(L.far_tiles));
+    }
 }
 
 void launch_touching(const AssemblyLaunch &L, hipStream_t stream) {

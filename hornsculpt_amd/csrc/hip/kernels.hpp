@@ -99,6 +99,10 @@ void launch_csr_apply(const std::uint32_t *row_offsets, const std::uint32_t *col
 struct AssemblyLaunch {
     AssemblyArgs args;
     ClusterView clusters;
+    // (test cluster, trial cluster) tiles: the first far_tiles are far tiles whose
+    // pairs all qualify for degree 4 at this wavenumber (uniform path).
+    const std::uint32_t *tiles;
+    std::uint32_t tile_count, far_tiles;
     NearView near;
     const double *rules;                   // kRuleTableCount x 4 doubles
     const TouchingJob *touching;

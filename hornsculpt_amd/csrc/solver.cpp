@@ -243,6 +243,26 @@ void MeshSolver::build_clusters() {
     local_.upload(local, s);
     row_scale_.upload(row_scale, s);
     lumped_.upload(lumped_internal_, s);
+    // cluster bounds for the far-tile test
+    cluster_bounds_.assign(cluster_count_, {});
+    for (std::uint32_t c = 0; c < cluster_count_; ++c) {
+        ClusterBounds &cb = cluster_bounds_[c];
+        Vec3 lo{INFINITY, INFINITY, INFINITY}, hi{-INFINITY, -INFINITY, -INFINITY};
+        for (std::uint32_t q = panel_offsets[c]; q < panel_offsets[c + 1]; ++q) {
+            const Vec3 p = space_.panels[panels[q]].centroid;
+            lo = {std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z)};
+            hi = {std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z)};
+        }
+        cb.center = {(lo.x + hi.x) / 2, (lo.y + hi.y) / 2, (lo.z + hi.z) / 2};
+        cb.shapes_ok = true;
+        for (std::uint32_t q = panel_offsets[c]; q < panel_offsets[c + 1]; ++q) {
+            const std::uint32_t t = panels[q];
+            cb.spread = std::max(cb.spread, std::sqrt(vnorm2(vsub(space_.panels[t].centroid, cb.center))));
+            cb.ball = std::max(cb.ball, bounds[static_cast<std::size_t>(dev::B_R) * nt + t]);
+            cb.max_edge = std::max(cb.max_edge, space_.panels[t].max_edge);
+            cb.shapes_ok = cb.shapes_ok && shape[t];
+        }
+    }
     cluster_panel_offsets_.upload(panel_offsets, s);
     cluster_panels_.upload(panels, s);
     cluster_node_offsets_.upload(node_offsets, s);
@@ -321,6 +341,52 @@ std::shared_ptr<DevicePlan> MeshSolver::device_plan(const Numerics &n, double k_
             const auto &patch = dp->plan.patches[j.first_patch + q];
             dp->near_samples += patch_point_count(patch, patch.degree == 14 ? 64 : 16);
         }
+    // Far tiles: cluster-level bounds guarantee, with a relative margin far above
+    // rounding, that every pair of the tile is disjoint, not in the near list
+    // (ball gap above max(3, ratio) h and degree > 0 at k_max through the
+    // "k h <= 2, one-edge separation" branch) and qualifies for degree 4 at any
+    // solve k with k * max_edge <= 0.5 (the "k h <= .5, 3-edge separation" branch).
+    {
+        const std::uint32_t nc = cluster_count_;
+        std::vector<std::pair<double, std::uint32_t>> far;  // (tile edge, tile index)
+        std::vector<std::uint32_t> general;
+        const double budget = .2 * n.near_tolerance, reserve = 1e-6;
+        const bool budgets = reserve + 4 * 2.3e-6 <= budget && reserve + 4 * 6e-7 <= budget;
+        if (!dp->plan.all_pairs && budgets && n.quadrature_order <= 4) {
+            const double gap_edges = std::max(3.0, n.near_ratio);
+            for (std::uint32_t a = 0; a < nc; ++a)
+                for (std::uint32_t b = 0; b < nc; ++b) {
+                    const ClusterBounds &ca = cluster_bounds_[a], &cb = cluster_bounds_[b];
+                    const double h = std::max(ca.max_edge, cb.max_edge);
+                    const double d = std::sqrt(vnorm2(vsub(ca.center, cb.center)));
+                    const double gap = d - ca.spread - cb.spread;
+                    if (a != b && ca.shapes_ok && cb.shapes_ok &&
+                        gap >= (ca.ball + cb.ball + gap_edges * h) * (1 + 1e-9) && k_max * h <= 2 * (1 - 1e-12))
+                        far.push_back({h, a * nc + b});
+                    else
+                        general.push_back(a * nc + b);
+                }
+        } else {
+            for (std::uint32_t t = 0; t < nc * nc; ++t)
+                general.push_back(t);
+        }
+        std::stable_sort(far.begin(), far.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+        std::vector<std::uint32_t> tiles;
+        tiles.reserve(2 * static_cast<std::size_t>(nc) * nc);
+        for (const auto &[h, t] : far) {
+            dp->far_tile_edge.push_back(h);
+            tiles.push_back(t / nc);
+            tiles.push_back(t % nc);
+        }
+        for (std::uint32_t t : general) {
+            tiles.push_back(t / nc);
+            tiles.push_back(t % nc);
+        }
+        dp->tile_count = nc * nc;
+        if (tiles.empty())
+            tiles.push_back(0);
+        dp->tiles.upload(tiles, s);
+    }
     hip_check(hipStreamSynchronize(s), "plan upload");
     dp->plan_ms = ms_since(t0);
     if (built)
@@ -350,6 +416,14 @@ dev::AssemblyLaunch MeshSolver::assembly_launch(const DevicePlan &p, const Numer
     L.args.error = error;
     L.clusters = {cluster_panel_offsets_.data(), cluster_panels_.data(), cluster_node_offsets_.data(),
                   cluster_nodes_.data(), local_.data(), cluster_count_};
+    // far tiles on the uniform degree-4 path at this k: the prefix with k * edge <= 0.5
+    // (the same product and comparison as the qualification of each pair)
+    L.tiles = p.tiles.data();
+    L.tile_count = p.tile_count;
+    L.far_tiles = static_cast<std::uint32_t>(
+        std::upper_bound(p.far_tile_edge.begin(), p.far_tile_edge.end(), 0.0,
+                         [k](double, double h) { return !(k * h <= .5); }) -
+        p.far_tile_edge.begin());
     L.near = {p.near_offsets.data(), p.near_trials.data(), p.whole.size() ? p.whole.data() : nullptr};
     L.rules = rules_.data();
     L.touching = p.touching.data();

@@ -185,6 +185,20 @@ struct DevicePlan {
     DeviceBuffer<double> gauss;
     double plan_ms{};
     double touching_samples{}, near_samples{}; // algorithmic sample counts of one assembly
+    // Tile list: far tiles (no touching or near pair, degree 4 whenever
+    // k * tile_edge <= 0.5) sorted by tile_edge, then all other cluster pairs.
+    DeviceBuffer<std::uint32_t> tiles;
+    std::vector<double> far_tile_edge; // ascending, one per far tile
+    std::uint32_t tile_count{};
+};
+
+// Bounds of one spatial cluster of panels (for conservative tile tests).
+struct ClusterBounds {
+    Vec3 center{};
+    double spread{};     // max distance of a panel centroid from center
+    double ball{};       // max panel ball radius
+    double max_edge{};
+    bool shapes_ok{};
 };
 
 // A validated request with its host-side derived data (observers, flux columns,
@@ -255,6 +269,7 @@ class MeshSolver {
     DeviceBuffer<std::uint8_t> shape_, local_;
     DeviceBuffer<std::uint32_t> cluster_panel_offsets_, cluster_panels_, cluster_node_offsets_, cluster_nodes_;
     std::uint32_t cluster_count_{};
+    std::vector<ClusterBounds> cluster_bounds_;
     std::map<std::tuple<unsigned, unsigned, double, double, unsigned, double>, std::shared_ptr<DevicePlan>> plans_;
     std::mutex plan_mutex_;
     std::unique_ptr<Workspace> workspace_; // direct solve path (context stream), reused across calls
