@@ -256,7 +256,7 @@ __device__ __forceinline__ void contract(MomentsT<R> &m, const Frame &t, const F
         const R B = blt[0] * blt[0] + blt[1] * blt[1] + blt[2] * blt[2];
         const R fyn = Bn + x1 * T1 + x2 * T2;
         CplxT<R> ga[3] = {{0, 0}, {0, 0}, {0, 0}}, gg = {0, 0}, gb = {0, 0};
-#pragma unroll 1
+#pragma unroll 2
         for (int yi = 0; yi < n; ++yi) {
             const R *yp = rule + 4 * yi;
             const R y0 = yp[0], y1 = yp[1], y2 = yp[2];
