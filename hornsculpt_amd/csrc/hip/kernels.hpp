@@ -165,6 +165,8 @@ void launch_axpy(std::uint32_t n, const double *alpha, const double *x, double *
 // y = s * x (complex vector of length n, real s).
 // y += x (n complex)
 void launch_add(std::uint32_t n, const double *x, double *y, hipStream_t stream);
+// y = x / sqrt(*norm2) (device-resident squared norm; zero vector when it is 0).
+void launch_normalize(std::uint32_t n, const double *norm2, const double *x, double *y, hipStream_t stream);
 void launch_scale(std::uint32_t n, double s, const double *x, double *y, hipStream_t stream);
 
 // ---- fields (field.hip) ----

@@ -210,6 +210,15 @@ __global__ void add_kernel(std::uint32_t n, const double *x, double *y) {
     y[i] += x[i];
 }
 
+// y = x / sqrt(*norm2) with the squared norm read on the device (0 when the norm is 0).
+__global__ void normalize_kernel(std::uint32_t n, const double *norm2, const double *x, double *y) {
+    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * n)
+        return;
+    const double v = *norm2;
+    y[i] = v > 0 ? x[i] / sqrt(v) : 0.0;
+}
+
 __global__ void scale_kernel(std::uint32_t n, double s, const double *x, double *y) {
     const std::uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n)
@@ -724,6 +733,10 @@ void launch_add(std::uint32_t n, const double *x, double *y, hipStream_t stream)
 
 void launch_axpy(std::uint32_t n, const double *alpha, const double *x, double *y, hipStream_t stream) {
     hipLaunchKernelGGL(axpy_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, alpha, x, y);
+}
+
+void launch_normalize(std::uint32_t n, const double *norm2, const double *x, double *y, hipStream_t stream) {
+    hipLaunchKernelGGL(normalize_kernel, dim3((2 * n + 255) / 256), dim3(256), 0, stream, n, norm2, x, y);
 }
 
 void launch_scale(std::uint32_t n, double s, const double *x, double *y, hipStream_t stream) {

@@ -136,11 +136,7 @@ void Workspace::release() {
     flux_column.release();
     pivots.release();
     jacobi.release();
-    gmres.z.release();
-    gmres.basis.release();
-    gmres.r.release();
-    gmres.scratch.release();
-    gmres.small.release();
+    gmres.release();
     bound = 0;
 }
 
@@ -485,9 +481,29 @@ void MeshSolver::assemble_dense(const Numerics &n, double k_max, double k, doubl
         }
 }
 
+GmresWorkspace::~GmresWorkspace() { release(); }
+
+void GmresWorkspace::release() {
+    basis.release();
+    r.release();
+    z.release();
+    scratch.release();
+    small.release();
+    if (pinned)
+        (void)hipHostFree(pinned);
+    pinned = nullptr;
+    pinned_doubles = 0;
+    for (auto e : events)
+        (void)hipEventDestroy(e);
+    events.clear();
+}
+
 // Restarted GMRES with twice classical Gram-Schmidt on the device; host
-// Givens rotations (device_gmres.cu:197-279, without a preconditioner: the
-// dense system is already row-equilibrated by the lumped mass).
+// Givens rotations (device_gmres.cu:197-279). Right preconditioning by the
+// block-Jacobi inverses when given. The Arnoldi process is normalised on the
+// device, so step j + 1 is queued before the host reads step j's column (from
+// pinned memory, by event): the device never waits for the host's rotations,
+// and at most one speculative step per restart cycle is discarded.
 void dense_gmres(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const double *b, double *x,
                  std::uint32_t restart,
                  std::uint32_t max_it, double tol, double *achieved, std::uint32_t *iterations,
@@ -505,11 +521,24 @@ void dense_gmres(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const
     const std::size_t scratch = std::max(dev::gemv_scratch_doubles(n, 1), dev::reduction_scratch_doubles(n, restart + 1));
     if (scratch_.size() < scratch)
         scratch_.resize(scratch);
-    if (ws.small.size() < 4 * (restart + 2) + 8)
-        ws.small.resize(4 * (restart + 2) + 8);
+    const std::size_t column = 4 * (restart + 1) + 1; // h1, h2 (complex, restart + 1 each) and |w|^2
+    if (ws.small.size() < column + 8)
+        ws.small.resize(column + 8);
+    if (ws.pinned_doubles < column * restart) {
+        if (ws.pinned)
+            (void)hipHostFree(ws.pinned);
+        ws.pinned = nullptr;
+        hip_check(hipHostMalloc(reinterpret_cast<void **>(&ws.pinned), sizeof(double) * column * restart),
+                  "pinned GMRES columns");
+        ws.pinned_doubles = column * restart;
+    }
+    while (ws.events.size() < restart) {
+        hipEvent_t e;
+        hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event create");
+        ws.events.push_back(e);
+    }
     auto &small = ws.small;
     double *h1 = small.data(), *h2 = small.data() + 2 * (restart + 1), *nrm = small.data() + 4 * (restart + 1);
-    std::vector<double> host(4 * (restart + 1) + 1);
     auto norm = [&](const double *v) {
         dev::launch_norm2(v, n, nullptr, nrm, scratch_.data(), s);
         double out = 0;
@@ -531,6 +560,26 @@ void dense_gmres(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const
     std::vector<C> H(stride * restart), g(stride), sn(restart);
     std::vector<double> cs(restart);
     std::uint32_t it = 0;
+    // One Arnoldi step: w = A M^{-1} v_j, CGS2 against v_0..v_j, |w|^2, v_{j+1} = w/|w|,
+    // then the column (h1, h2, |w|^2) to pinned slot j.
+    auto enqueue_step = [&](std::uint32_t j) {
+        double *w = basis_.data() + 2 * static_cast<std::size_t>(j + 1) * n;
+        const double *vj = basis_.data() + 2 * static_cast<std::size_t>(j) * n;
+        if (jacobi) {
+            dev::jacobi_apply(jacobi, n, vj, ws.z.data(), s);
+            vj = ws.z.data();
+        }
+        dev::launch_gemv(A, fp32, n, vj, w, 1, scratch_.data(), s);
+        dev::launch_project(basis_.data(), n, j + 1, w, h1, scratch_.data(), s);
+        dev::launch_subtract_combination(basis_.data(), n, j + 1, h1, w, s);
+        dev::launch_project(basis_.data(), n, j + 1, w, h2, scratch_.data(), s);
+        dev::launch_subtract_combination(basis_.data(), n, j + 1, h2, w, s);
+        dev::launch_norm2(w, n, nullptr, nrm, scratch_.data(), s);
+        dev::launch_normalize(n, nrm, w, w, s);
+        hip_check(hipMemcpyAsync(ws.pinned + column * j, small.data(), sizeof(double) * column, hipMemcpyDeviceToHost, s),
+                  "arnoldi download");
+        hip_check(hipEventRecord(ws.events[j], s), "arnoldi event");
+    };
     for (;;) {
         if (beta <= target) {
             *achieved = beta / bnorm;
@@ -544,32 +593,22 @@ void dense_gmres(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const
         std::fill(g.begin(), g.end(), C{});
         g[0] = beta;
         std::uint32_t used = 0;
+        enqueue_step(0);
         for (std::uint32_t j = 0; j < restart && it < max_it; ++j) {
-            if (cancel && *cancel)
+            if (cancel && *cancel) {
+                (void)hipStreamSynchronize(s);
                 throw Cancelled();
-            double *w = basis_.data() + 2 * static_cast<std::size_t>(j + 1) * n;
-            const double *vj = basis_.data() + 2 * static_cast<std::size_t>(j) * n;
-            if (jacobi) { // w = A M^{-1} v_j
-                dev::jacobi_apply(jacobi, n, vj, ws.z.data(), s);
-                vj = ws.z.data();
             }
-            dev::launch_gemv(A, fp32, n, vj, w, 1, scratch_.data(), s);
-            dev::launch_project(basis_.data(), n, j + 1, w, h1, scratch_.data(), s);
-            dev::launch_subtract_combination(basis_.data(), n, j + 1, h1, w, s);
-            dev::launch_project(basis_.data(), n, j + 1, w, h2, scratch_.data(), s);
-            dev::launch_subtract_combination(basis_.data(), n, j + 1, h2, w, s);
-            dev::launch_norm2(w, n, nullptr, nrm, scratch_.data(), s);
-            hip_check(hipMemcpyAsync(host.data(), small.data(), sizeof(double) * host.size(), hipMemcpyDeviceToHost, s),
-                      "arnoldi download");
-            hip_check(hipStreamSynchronize(s), "arnoldi sync");
+            if (j + 1 < restart && it + 1 < max_it)
+                enqueue_step(j + 1); // speculative: keeps the device busy while the host rotates column j
+            hip_check(hipEventSynchronize(ws.events[j]), "arnoldi sync");
+            const double *host = ws.pinned + column * j;
             C *col = H.data() + static_cast<std::size_t>(j) * stride;
             for (std::uint32_t i = 0; i <= j; ++i)
                 col[i] = C(host[2 * i], host[2 * i + 1]) + C(host[2 * (restart + 1) + 2 * i], host[2 * (restart + 1) + 2 * i + 1]);
             const double next = std::sqrt(host[4 * (restart + 1)]);
             col[j + 1] = next;
             const bool breakdown = !(next > 0);
-            if (!breakdown)
-                dev::launch_scale(n, 1.0 / next, w, w, s);
             for (std::uint32_t i = 0; i < j; ++i) {
                 const C t = cs[i] * col[i] + sn[i] * col[i + 1];
                 col[i + 1] = -std::conj(sn[i]) * col[i] + cs[i] * col[i + 1];
@@ -577,8 +616,10 @@ void dense_gmres(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const
             }
             const C a = col[j], bb = col[j + 1];
             const double len = std::hypot(std::abs(a), std::abs(bb));
-            if (!(len > 0) || !std::isfinite(len))
+            if (!(len > 0) || !std::isfinite(len)) {
+                (void)hipStreamSynchronize(s);
                 throw std::runtime_error("GMRES Arnoldi factorization broke down.");
+            }
             const C phase = std::abs(a) > 0 ? a / std::abs(a) : C(1, 0);
             cs[j] = std::abs(a) / len;
             sn[j] = phase * std::conj(bb) / len;
@@ -604,6 +645,7 @@ void dense_gmres(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const
             neg[2 * j] = -y[j].real();
             neg[2 * j + 1] = -y[j].imag();
         }
+        // h1 is free once every queued step has read it (stream order).
         hip_check(hipMemcpyAsync(h1, neg.data(), sizeof(double) * neg.size(), hipMemcpyHostToDevice, s), "y upload");
         if (jacobi) { // x += M^{-1} V y
             hip_check(hipMemsetAsync(r_.data(), 0, 2 * sizeof(double) * n, s), "gmres clear");

@@ -139,7 +139,16 @@ struct Result {
 };
 
 struct GmresWorkspace {
+    GmresWorkspace() = default;
+    GmresWorkspace(const GmresWorkspace &) = delete;
+    GmresWorkspace &operator=(const GmresWorkspace &) = delete;
+    ~GmresWorkspace();
+    void release();
     DeviceBuffer<double> basis, r, z, scratch, small;
+    // pinned per-step copies of the Arnoldi column and its events (pipelined host Givens)
+    double *pinned{};
+    std::size_t pinned_doubles{};
+    std::vector<hipEvent_t> events;
 };
 
 // Device buffers and stream of one solve in flight. The direct solve path owns
