@@ -139,6 +139,8 @@ def library() -> ctypes.CDLL:
                 "hsbem_near_operator_copy": (ctypes.c_int, [P, ctypes.POINTER(U32), ctypes.POINTER(U32), D,
                                                             ctypes.POINTER(U32), ctypes.POINTER(U32), D, D]),
                 "hsbem_near_operator_destroy": (ctypes.c_int, [P]),
+                "hsbem_rotation_plan": (ctypes.c_int, [U32, D, D, D, D, ctypes.c_int32, D]),
+                "hsbem_rcr_queue_policy": (ctypes.c_int, [U32, U32, U32, ctypes.POINTER(U64)]),
                 "hsbem_support_tree_build": (ctypes.c_int, [D, U32, ctypes.POINTER(U32), U32, U32, U32,
                                                             ctypes.c_double, ctypes.c_double, D, ctypes.POINTER(U64)]),
                 "hsbem_plan_copy": (ctypes.c_int, [P, ctypes.POINTER(U32), ctypes.POINTER(U32), D,
@@ -676,6 +678,32 @@ def parse_support_expansion_plan(blob: np.ndarray) -> dict:
     nl, nn = int(blob[0]), int(blob[1])
     levels = [dict(zip(LEVEL_FIELDS, blob[2 + 15 * l:2 + 15 * (l + 1)])) for l in range(nl)]
     return {"levels": levels, "active": blob[2 + 15 * nl:2 + 15 * nl + nn].astype(np.int64)}
+
+
+def rotation_plan(order: int, axis: Sequence[float], coefficients: Optional[np.ndarray] = None,
+                  direction: int = 1):
+    """Wigner small-d blocks (per degree, row-major), azimuth phases exp(+i m phi) and,
+    with coefficients ((order+1)^2 complex, index l*l+l+m), the rotated coefficients:
+    direction > 0 pulls f back to the axis frame, direction < 0 restores it."""
+    lib = library()
+    nd = sum((2 * l + 1) ** 2 for l in range(order + 1))
+    d, az = np.zeros(nd), np.zeros(2 * (2 * order + 1))
+    ax = np.ascontiguousarray(axis, dtype=np.float64)
+    out = None
+    if coefficients is not None:
+        c = np.ascontiguousarray(np.asarray(coefficients, np.complex128)).view(np.float64)
+        out = np.zeros_like(c)
+        _check(lib.hsbem_rotation_plan(order, _dptr(ax), _dptr(d), _dptr(az), _dptr(c), direction, _dptr(out)))
+    else:
+        _check(lib.hsbem_rotation_plan(order, _dptr(ax), _dptr(d), _dptr(az), None, 0, None))
+    return d, az.view(np.complex128), (out.view(np.complex128) if out is not None else None)
+
+
+def rcr_queue_policy(source_order: int, target_order: int, reserved_batches: int) -> tuple:
+    out = np.zeros(3, np.uint64)
+    _check(library().hsbem_rcr_queue_policy(source_order, target_order, reserved_batches,
+                                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+    return tuple(int(v) for v in out)
 
 
 def parse_support_near_plan(blob: np.ndarray, triangle_count: int) -> dict:

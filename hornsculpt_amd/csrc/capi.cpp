@@ -3,6 +3,7 @@
 #include "../../include/hsbem.h"
 
 #include "host/fmm_levels.hpp"
+#include "host/fmm_rotation.hpp"
 #include "host/support_near_plan.hpp"
 #include "host/support_tree.hpp"
 #include "server.hpp"
@@ -679,6 +680,43 @@ int hsbem_near_operator_copy(const hsbem_near_operator *o, uint32_t *ro, uint32_
 
 int hsbem_near_operator_destroy(hsbem_near_operator *o) {
     return guarded([&] { delete o; });
+}
+
+int hsbem_rotation_plan(uint32_t order, const double *axis, double *small_d, double *azimuth, const double *coefficients,
+                        int32_t direction, double *rotated) {
+    return guarded([&] {
+        require(axis, "null rotation axis");
+        const auto p = hsbem::make_rotation_plan(order, {axis[0], axis[1], axis[2]});
+        if (small_d)
+            std::copy(p.small_d.begin(), p.small_d.end(), small_d);
+        if (azimuth)
+            for (std::size_t i = 0; i < p.azimuth.size(); ++i) {
+                azimuth[2 * i] = p.azimuth[i].real();
+                azimuth[2 * i + 1] = p.azimuth[i].imag();
+            }
+        if (coefficients) {
+            require(rotated && direction != 0, "rotation needs an output and a direction");
+            const std::size_t nc = static_cast<std::size_t>(order + 1) * (order + 1);
+            std::vector<std::complex<double>> c(nc);
+            for (std::size_t i = 0; i < nc; ++i)
+                c[i] = {coefficients[2 * i], coefficients[2 * i + 1]};
+            const auto r = direction > 0 ? hsbem::rotate_to_axis(p, c) : hsbem::rotate_from_axis(p, c);
+            for (std::size_t i = 0; i < nc; ++i) {
+                rotated[2 * i] = r[i].real();
+                rotated[2 * i + 1] = r[i].imag();
+            }
+        }
+    });
+}
+
+int hsbem_rcr_queue_policy(uint32_t source_order, uint32_t target_order, uint32_t reserved, uint64_t *out) {
+    return guarded([&] {
+        require(out, "null output");
+        const auto p = hsbem::make_rcr_queue_policy(source_order, target_order, reserved);
+        out[0] = p.batch_capacity;
+        out[1] = p.maximum_chunk_batches;
+        out[2] = p.work_per_link;
+    });
 }
 
 int hsbem_plan_sizes(const hsbem_plan *p, uint64_t *s) {

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <complex>
 #include <limits>
+#include <math.h>
 #include <stdexcept>
 
 namespace hsbem {
@@ -62,7 +63,9 @@ double enforce_diagonal_stability(WidebandLevelPlan &p, double k, double distanc
         throw std::invalid_argument("Support diagonal electrical distance is invalid.");
     // h_l^(2)(z) by upward recurrence from h_0 = (sin z + i cos z)/z, h_1 = h_0 (1/z + i).
     using C = std::complex<double>;
-    C h_prev{std::sin(z) / z, std::cos(z) / z};
+    double sz, cz;
+    ::sincos(z, &sz, &cz); // joint routine, as the reference build evaluates this pair
+    C h_prev{sz / z, cz / z};
     C h = h_prev * C{1 / z, 1};
     double indicator = z * std::abs(h_prev);
     for (unsigned l = 1; l <= p.translation_order; ++l) {

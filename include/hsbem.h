@@ -285,6 +285,20 @@ HSBEM_API int hsbem_near_operator_copy(const hsbem_near_operator *op, uint32_t *
                                        double *flux_values, double *applied);
 HSBEM_API int hsbem_near_operator_destroy(hsbem_near_operator *op);
 
+/* Spherical-harmonic rotation plan of the LF FMM translations (host-only;
+ * replaces makeSphericalRotationPlan / rotateToAxis / rotateFromAxis,
+ * modules/bem/fmm/spherical_rotation.hpp:8-27): Wigner small-d blocks of
+ * degrees 0..order (small_d: sum (2l+1)^2 doubles, row-major per degree) and
+ * azimuth phases exp(+i m phi) (2 order + 1 complex). When coefficients
+ * (order+1)^2 complex, indexed l*l+l+m) are given, they are rotated to the axis
+ * (direction > 0) or back (direction < 0) into rotated. */
+HSBEM_API int hsbem_rotation_plan(uint32_t order, const double *axis, double *small_d, double *azimuth,
+                                  const double *coefficients, int32_t direction, double *rotated);
+/* RCR work model of the LF queue (makeRcrQueuePolicy, modules/bem/fmm/rcr_work.hpp:23-34):
+ * out = (batch capacity, chunk batches, work per link). */
+HSBEM_API int hsbem_rcr_queue_policy(uint32_t source_order, uint32_t target_order, uint32_t reserved_batches,
+                                     uint64_t *out);
+
 /* Device assembly of the row-scaled Galerkin system at wavenumber k with the
  * plan for k_max: A (nv x nv) and B (nv x nt, every panel a flux column), both
  * complex column-major, copied to host. */

@@ -8,6 +8,8 @@
 #include "galerkin/pair_quadrature.hpp"
 #include "galerkin/regular_qualification.hpp"
 #include "galerkin/space.hpp"
+#include "fmm/rcr_work.hpp"
+#include "fmm/spherical_rotation.hpp"
 #include "fmm/support_level_plan.hpp"
 #include "fmm/support_near_plan.hpp"
 #include "fmm/support_tree.hpp"
@@ -428,6 +430,39 @@ int ref_support_expansion_plan(std::uint32_t nv, const double *xyz, std::uint32_
             std::memcpy(blob, b.data(), b.size() * sizeof(double));
         }
         *size = b.size();
+    });
+}
+
+// makeSphericalRotationPlan / rotateToAxis / rotateFromAxis (fmm/spherical_rotation.cpp).
+int ref_rotation_plan(unsigned order, const double *axis, double *small_d, double *azimuth, const double *coefficients,
+                      int direction, double *rotated) {
+    return guarded([&] {
+        const auto p = wideband::makeSphericalRotationPlan(order, {axis[0], axis[1], axis[2]});
+        std::memcpy(small_d, p.smallD.data(), p.smallD.size() * sizeof(double));
+        for (std::size_t i = 0; i < p.azimuthPhases.size(); ++i) {
+            azimuth[2 * i] = p.azimuthPhases[i].real();
+            azimuth[2 * i + 1] = p.azimuthPhases[i].imag();
+        }
+        if (coefficients) {
+            const std::size_t nc = static_cast<std::size_t>(order + 1) * (order + 1);
+            std::vector<wideband::Complex> c(nc);
+            for (std::size_t i = 0; i < nc; ++i)
+                c[i] = {coefficients[2 * i], coefficients[2 * i + 1]};
+            const auto r = direction > 0 ? wideband::rotateToAxis(p, c) : wideband::rotateFromAxis(p, c);
+            for (std::size_t i = 0; i < nc; ++i) {
+                rotated[2 * i] = r[i].real();
+                rotated[2 * i + 1] = r[i].imag();
+            }
+        }
+    });
+}
+
+int ref_rcr_queue_policy(unsigned source_order, unsigned target_order, unsigned reserved, std::uint64_t *out) {
+    return guarded([&] {
+        const auto p = wideband::makeRcrQueuePolicy(source_order, target_order, reserved);
+        out[0] = p.batchCapacity;
+        out[1] = p.maximumChunkBatches;
+        out[2] = p.workPerLink;
     });
 }
 }

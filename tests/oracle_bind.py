@@ -55,6 +55,8 @@ def reference():
     lib.ref_support_expansion_plan.argtypes = [ctypes.c_uint32, _D, ctypes.c_uint32, _U, _U, _U8, _U, ctypes.c_uint32,
                                                ctypes.c_uint, ctypes.c_uint, ctypes.c_double, ctypes.c_double,
                                                ctypes.c_double, ctypes.c_double, _D, ctypes.POINTER(ctypes.c_uint64)]
+    lib.ref_rotation_plan.argtypes = [ctypes.c_uint, _D, _D, _D, _D, ctypes.c_int, _D]
+    lib.ref_rcr_queue_policy.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.POINTER(ctypes.c_uint64)]
     lib.ref_support_tree.argtypes = [ctypes.c_uint32, _D, ctypes.c_uint32, _U, ctypes.c_uint, ctypes.c_uint,
                                      ctypes.c_double, ctypes.c_double, _D, ctypes.POINTER(ctypes.c_uint64)]
     return lib
@@ -152,6 +154,31 @@ def ref_support_expansion_plan(mesh, active_faces, k, seed, leaf_capacity=32, ma
     blob = np.zeros(size.value)
     _check(lib, lib.ref_support_expansion_plan(*args, _d(blob), ctypes.byref(size)), "ref_support_expansion_plan")
     return blob
+
+
+def ref_rotation(order, axis, coefficients=None, direction=1):
+    """Reference Wigner small-d blocks, azimuth phases and (optionally) rotated coefficients."""
+    lib = reference()
+    nd = sum((2 * l + 1) ** 2 for l in range(order + 1))
+    d, az = np.zeros(nd), np.zeros(2 * (2 * order + 1))
+    ax = np.ascontiguousarray(axis, dtype=np.float64)
+    out = None
+    if coefficients is not None:
+        c = np.ascontiguousarray(np.asarray(coefficients, np.complex128)).view(np.float64)
+        out = np.zeros_like(c)
+        rc = lib.ref_rotation_plan(order, _d(ax), _d(d), _d(az), _d(c), direction, _d(out))
+    else:
+        rc = lib.ref_rotation_plan(order, _d(ax), _d(d), _d(az), None, 0, None)
+    _check(lib, rc, "ref_rotation_plan")
+    return d, az.view(np.complex128), (out.view(np.complex128) if out is not None else None)
+
+
+def ref_rcr_queue_policy(source_order, target_order, reserved):
+    lib = reference()
+    out = np.zeros(3, np.uint64)
+    _check(lib, lib.ref_rcr_queue_policy(source_order, target_order, reserved,
+                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))), "ref_rcr_queue_policy")
+    return out
 
 
 def oracle_assemble(mesh, plan, k, tolerance=1e-4, regular_order=4, singular_order=4, tests=None, threads=8,
