@@ -141,6 +141,12 @@ def library() -> ctypes.CDLL:
                 "hsbem_near_operator_destroy": (ctypes.c_int, [P]),
                 "hsbem_rotation_plan": (ctypes.c_int, [U32, D, D, D, D, ctypes.c_int32, D]),
                 "hsbem_rcr_queue_policy": (ctypes.c_int, [U32, U32, U32, ctypes.POINTER(U64)]),
+                "hsbem_sphere_rule": (ctypes.c_int, [U32, D, D]),
+                "hsbem_spherical_harmonics": (ctypes.c_int, [U32, D, D]),
+                "hsbem_regular_radial_table": (ctypes.c_int, [U32, ctypes.c_double, D, D]),
+                "hsbem_angular_transfer": (ctypes.c_int, [U32, U32, U32, ctypes.c_int32, ctypes.c_double, D, D, D]),
+                "hsbem_harmonic_spectrum": (ctypes.c_int, [U32, U32, ctypes.c_int32, D, D]),
+                "hsbem_diagonal_translation": (ctypes.c_int, [U32, U32, ctypes.c_double, D, D]),
                 "hsbem_support_tree_build": (ctypes.c_int, [D, U32, ctypes.POINTER(U32), U32, U32, U32,
                                                             ctypes.c_double, ctypes.c_double, D, ctypes.POINTER(U64)]),
                 "hsbem_plan_copy": (ctypes.c_int, [P, ctypes.POINTER(U32), ctypes.POINTER(U32), D,
@@ -704,6 +710,80 @@ def rcr_queue_policy(source_order: int, target_order: int, reserved_batches: int
     _check(library().hsbem_rcr_queue_policy(source_order, target_order, reserved_batches,
                                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
     return tuple(int(v) for v in out)
+
+
+TRANSFER_RESAMPLE, TRANSFER_OUTGOING_TO_PARENT, TRANSFER_INCOMING_TO_CHILD = 0, 1, 2
+
+
+def _rule_size(order: int) -> int:
+    return 2 * (order + 1) ** 2
+
+
+def _complex_arg(values, n: int, what: str) -> np.ndarray:
+    v = np.ascontiguousarray(np.asarray(values, np.complex128).ravel())
+    if v.size != n:
+        raise ValueError(f"{what} needs {n} complex values, got {v.size}")
+    return v.view(np.float64)
+
+
+def sphere_rule(order: int):
+    """Gauss-Legendre in cos(theta) x 2(order+1) equispaced azimuths, polar-major:
+    (directions [2(order+1)^2, 3], weights)."""
+    n = _rule_size(order)
+    d, w = np.zeros(3 * n), np.zeros(n)
+    _check(library().hsbem_sphere_rule(order, _dptr(d), _dptr(w)))
+    return d.reshape(n, 3), w
+
+
+def spherical_harmonics(order: int, direction: Sequence[float]) -> np.ndarray:
+    """Orthonormal Y_lm (Condon-Shortley), index l*l + l + m."""
+    out = np.zeros(2 * (order + 1) ** 2)
+    d = np.ascontiguousarray(direction, dtype=np.float64)
+    _check(library().hsbem_spherical_harmonics(order, _dptr(d), _dptr(out)))
+    return out.view(np.complex128)
+
+
+def regular_radial_table(order: int, x: float):
+    """(j_l(x), j_l'(x)) for l = 0..order."""
+    v, dv = np.zeros(order + 1), np.zeros(order + 1)
+    _check(library().hsbem_regular_radial_table(order, float(x), _dptr(v), _dptr(dv)))
+    return v, dv
+
+
+def angular_transfer(values, source_order: int, target_order: int, bandlimit: int,
+                     mode: int = TRANSFER_RESAMPLE, k: float = 0.0, shift=None) -> np.ndarray:
+    """Bandlimited resampling between canonical sphere rules; modes 1/2 add the
+    parent/child plane-wave shift (shift = child centre - parent centre)."""
+    v = _complex_arg(values, _rule_size(source_order), "source spectrum")
+    out = np.zeros(2 * _rule_size(target_order))
+    sh = None if shift is None else np.ascontiguousarray(shift, dtype=np.float64)
+    _check(library().hsbem_angular_transfer(source_order, target_order, bandlimit, mode, float(k),
+                                            None if sh is None else _dptr(sh), _dptr(v), _dptr(out)))
+    return out.view(np.complex128)
+
+
+def multipole_to_spectrum(multipoles, order: int, rule_order: int) -> np.ndarray:
+    """F(d_q) = 4 pi sum i^l M_lm Y_lm(d_q) on the canonical rule of rule_order."""
+    v = _complex_arg(multipoles, (order + 1) ** 2, "multipoles")
+    out = np.zeros(2 * _rule_size(rule_order))
+    _check(library().hsbem_harmonic_spectrum(order, rule_order, 1, _dptr(v), _dptr(out)))
+    return out.view(np.complex128)
+
+
+def spectrum_to_local(incoming, rule_order: int, order: int) -> np.ndarray:
+    """L_lm = 4 pi (-i)^l sum_q w_q a(d_q) conj(Y_lm(d_q))."""
+    v = _complex_arg(incoming, _rule_size(rule_order), "incoming spectrum")
+    out = np.zeros(2 * (order + 1) ** 2)
+    _check(library().hsbem_harmonic_spectrum(order, rule_order, -1, _dptr(v), _dptr(out)))
+    return out.view(np.complex128)
+
+
+def diagonal_translation(rule_order: int, expansion_order: int, k: float, target_minus_source) -> np.ndarray:
+    """(-ik)/(16 pi^2) sum_l (2l+1) (-i)^l h_l^(2)(kR) P_l(d_q . R^) per rule sample."""
+    r = np.ascontiguousarray(target_minus_source, dtype=np.float64)
+    out = np.zeros(2 * _rule_size(rule_order))
+    _check(library().hsbem_diagonal_translation(rule_order, expansion_order, float(k), _dptr(r), _dptr(out)))
+    return out.view(np.complex128)
 
 
 def parse_support_near_plan(blob: np.ndarray, triangle_count: int) -> dict:

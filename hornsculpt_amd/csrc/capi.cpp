@@ -4,6 +4,7 @@
 
 #include "host/fmm_levels.hpp"
 #include "host/fmm_rotation.hpp"
+#include "host/fmm_transfer.hpp"
 #include "host/support_near_plan.hpp"
 #include "host/support_tree.hpp"
 #include "server.hpp"
@@ -706,6 +707,104 @@ int hsbem_rotation_plan(uint32_t order, const double *axis, double *small_d, dou
                 rotated[2 * i + 1] = r[i].imag();
             }
         }
+    });
+}
+
+namespace {
+
+std::vector<std::complex<double>> complex_in(const double *v, std::size_t n) {
+    std::vector<std::complex<double>> out(n);
+    for (std::size_t i = 0; i < n; ++i)
+        out[i] = {v[2 * i], v[2 * i + 1]};
+    return out;
+}
+
+void complex_out(const std::vector<std::complex<double>> &v, double *out) {
+    for (std::size_t i = 0; i < v.size(); ++i) {
+        out[2 * i] = v[i].real();
+        out[2 * i + 1] = v[i].imag();
+    }
+}
+
+std::size_t rule_size(uint32_t order) { return 2 * static_cast<std::size_t>(order + 1) * (order + 1); }
+
+} // namespace
+
+int hsbem_sphere_rule(uint32_t order, double *directions, double *weights) {
+    return guarded([&] {
+        require(directions && weights, "null sphere rule output");
+        const auto r = hsbem::make_sphere_rule(order);
+        for (std::size_t q = 0; q < r.samples.size(); ++q) {
+            directions[3 * q] = r.samples[q].direction.x;
+            directions[3 * q + 1] = r.samples[q].direction.y;
+            directions[3 * q + 2] = r.samples[q].direction.z;
+            weights[q] = r.samples[q].weight;
+        }
+    });
+}
+
+int hsbem_spherical_harmonics(uint32_t order, const double *direction, double *values) {
+    return guarded([&] {
+        require(direction && values, "null harmonic argument");
+        complex_out(hsbem::spherical_harmonics(order, {direction[0], direction[1], direction[2]}), values);
+    });
+}
+
+int hsbem_regular_radial_table(uint32_t order, double x, double *values, double *derivatives) {
+    return guarded([&] {
+        require(values && derivatives, "null radial output");
+        const auto t = hsbem::regular_radial_table(order, x);
+        std::copy(t.values.begin(), t.values.end(), values);
+        std::copy(t.derivatives.begin(), t.derivatives.end(), derivatives);
+    });
+}
+
+int hsbem_angular_transfer(uint32_t source_order, uint32_t target_order, uint32_t bandlimit, int32_t mode, double k,
+                           const double *shift, const double *values, double *out) {
+    return guarded([&] {
+        require(values && out, "null spectrum");
+        require(mode == HSBEM_TRANSFER_RESAMPLE || shift, "null shift");
+        const auto source = hsbem::make_sphere_rule(source_order), target = hsbem::make_sphere_rule(target_order);
+        const auto v = complex_in(values, rule_size(source_order));
+        const hsbem::Vec3 d = shift ? hsbem::Vec3{shift[0], shift[1], shift[2]} : hsbem::Vec3{};
+        switch (mode) {
+        case HSBEM_TRANSFER_RESAMPLE:
+            complex_out(hsbem::resample_spectrum(v, source, target, bandlimit), out);
+            break;
+        case HSBEM_TRANSFER_OUTGOING_TO_PARENT:
+            complex_out(hsbem::outgoing_to_parent(v, source, target, bandlimit, k, d), out);
+            break;
+        case HSBEM_TRANSFER_INCOMING_TO_CHILD:
+            complex_out(hsbem::incoming_to_child(v, source, target, bandlimit, k, d), out);
+            break;
+        default:
+            throw std::invalid_argument("unknown angular transfer mode");
+        }
+    });
+}
+
+int hsbem_harmonic_spectrum(uint32_t order, uint32_t rule_order, int32_t direction, const double *in, double *out) {
+    return guarded([&] {
+        require(in && out && direction != 0, "null spectrum or direction");
+        const auto rule = hsbem::make_sphere_rule(rule_order);
+        if (direction > 0)
+            complex_out(hsbem::multipole_to_spectrum(complex_in(in, static_cast<std::size_t>(order + 1) * (order + 1)),
+                                                     order, rule),
+                        out);
+        else
+            complex_out(hsbem::spectrum_to_local(complex_in(in, rule_size(rule_order)), rule, order), out);
+    });
+}
+
+int hsbem_diagonal_translation(uint32_t rule_order, uint32_t expansion_order, double k,
+                               const double *target_minus_source, double *out) {
+    return guarded([&] {
+        require(target_minus_source && out, "null translation argument");
+        const auto rule = hsbem::make_sphere_rule(rule_order);
+        complex_out(hsbem::diagonal_translation(
+                        rule, expansion_order, k,
+                        {target_minus_source[0], target_minus_source[1], target_minus_source[2]}),
+                    out);
     });
 }
 

@@ -294,6 +294,34 @@ HSBEM_API int hsbem_near_operator_destroy(hsbem_near_operator *op);
  * (direction > 0) or back (direction < 0) into rotated. */
 HSBEM_API int hsbem_rotation_plan(uint32_t order, const double *axis, double *small_d, double *azimuth,
                                   const double *coefficients, int32_t direction, double *rotated);
+/* Angular and radial numerics of the wideband translations (host-only;
+ * modules/bem/fmm/wideband_transfer.hpp:14-104, wideband_transfer.cpp:1-498).
+ * Complex arrays are interleaved (re, im). Sphere rules are the canonical ones
+ * of makeSphereRule: order L has 2 (L+1)^2 samples, polar-major, L+1 rings of
+ * 2 (L+1) azimuths. */
+/* makeSphereRule (wideband_transfer.hpp:28-30): directions 3 per sample, weights. */
+HSBEM_API int hsbem_sphere_rule(uint32_t order, double *directions, double *weights);
+/* sphericalHarmonics (wideband_transfer.hpp:32-34): (order+1)^2 complex values,
+ * index l*l + l + m, for a finite unit direction. */
+HSBEM_API int hsbem_spherical_harmonics(uint32_t order, const double *direction, double *values);
+/* regularRadialTable (wideband_transfer.hpp:36-42): j_l(x) and j_l'(x), l = 0..order. */
+HSBEM_API int hsbem_regular_radial_table(uint32_t order, double x, double *values, double *derivatives);
+#define HSBEM_TRANSFER_RESAMPLE 0
+#define HSBEM_TRANSFER_OUTGOING_TO_PARENT 1
+#define HSBEM_TRANSFER_INCOMING_TO_CHILD 2
+/* resampleSpectrum / outgoingToParent / incomingToChild (wideband_transfer.hpp:82-92)
+ * from the rule of source_order to the rule of target_order at the given
+ * bandlimit; shift = child centre - parent centre (unused for a resample). */
+HSBEM_API int hsbem_angular_transfer(uint32_t source_order, uint32_t target_order, uint32_t bandlimit, int32_t mode,
+                                     double k, const double *shift, const double *values, double *out);
+/* multipoleToSpectrum (direction > 0: (order+1)^2 multipoles in, rule samples out)
+ * or spectrumToLocal (direction < 0: rule samples in, (order+1)^2 locals out),
+ * wideband_transfer.hpp:72-78. */
+HSBEM_API int hsbem_harmonic_spectrum(uint32_t order, uint32_t rule_order, int32_t direction, const double *in,
+                                      double *out);
+/* diagonalTranslation (wideband_transfer.hpp:94-101): one complex factor per rule sample. */
+HSBEM_API int hsbem_diagonal_translation(uint32_t rule_order, uint32_t expansion_order, double k,
+                                         const double *target_minus_source, double *out);
 /* RCR work model of the LF queue (makeRcrQueuePolicy, modules/bem/fmm/rcr_work.hpp:23-34):
  * out = (batch capacity, chunk batches, work per link). */
 HSBEM_API int hsbem_rcr_queue_policy(uint32_t source_order, uint32_t target_order, uint32_t reserved_batches,

@@ -13,6 +13,7 @@
 #include "fmm/support_level_plan.hpp"
 #include "fmm/support_near_plan.hpp"
 #include "fmm/support_tree.hpp"
+#include "fmm/wideband_transfer.hpp"
 
 #include <cstdint>
 #include <cstring>
@@ -463,6 +464,82 @@ int ref_rcr_queue_policy(unsigned source_order, unsigned target_order, unsigned 
         out[0] = p.batchCapacity;
         out[1] = p.maximumChunkBatches;
         out[2] = p.workPerLink;
+    });
+}
+}
+
+namespace {
+std::vector<wideband::Complex> ref_complex_in(const double *v, std::size_t n) {
+    std::vector<wideband::Complex> out(n);
+    for (std::size_t i = 0; i < n; ++i)
+        out[i] = {v[2 * i], v[2 * i + 1]};
+    return out;
+}
+void ref_complex_out(const std::vector<wideband::Complex> &v, double *out) {
+    for (std::size_t i = 0; i < v.size(); ++i) {
+        out[2 * i] = v[i].real();
+        out[2 * i + 1] = v[i].imag();
+    }
+}
+} // namespace
+
+extern "C" {
+int ref_sphere_rule(unsigned order, double *directions, double *weights) {
+    return guarded([&] {
+        const auto r = wideband::makeSphereRule(order);
+        for (std::size_t q = 0; q < r.samples.size(); ++q) {
+            directions[3 * q] = r.samples[q].direction.x;
+            directions[3 * q + 1] = r.samples[q].direction.y;
+            directions[3 * q + 2] = r.samples[q].direction.z;
+            weights[q] = r.samples[q].weight;
+        }
+    });
+}
+
+int ref_spherical_harmonics(unsigned order, const double *d, double *values) {
+    return guarded([&] { ref_complex_out(wideband::sphericalHarmonics(order, {d[0], d[1], d[2]}), values); });
+}
+
+int ref_regular_radial_table(unsigned order, double x, double *values, double *derivatives) {
+    return guarded([&] {
+        const auto t = wideband::regularRadialTable(order, x);
+        std::memcpy(values, t.values.data(), t.values.size() * sizeof(double));
+        std::memcpy(derivatives, t.derivatives.data(), t.derivatives.size() * sizeof(double));
+    });
+}
+
+int ref_angular_transfer(unsigned source_order, unsigned target_order, unsigned bandlimit, int mode, double k,
+                         const double *shift, const double *values, double *out) {
+    return guarded([&] {
+        const auto s = wideband::makeSphereRule(source_order), t = wideband::makeSphereRule(target_order);
+        const auto v = ref_complex_in(values, s.samples.size());
+        const Vec3 d = shift ? Vec3{shift[0], shift[1], shift[2]} : Vec3{};
+        if (mode == 0)
+            ref_complex_out(wideband::resampleSpectrum(v, s, t, bandlimit), out);
+        else if (mode == 1)
+            ref_complex_out(wideband::outgoingToParent(v, s, t, bandlimit, k, d), out);
+        else
+            ref_complex_out(wideband::incomingToChild(v, s, t, bandlimit, k, d), out);
+    });
+}
+
+int ref_harmonic_spectrum(unsigned order, unsigned rule_order, int direction, const double *in, double *out) {
+    return guarded([&] {
+        const auto r = wideband::makeSphereRule(rule_order);
+        if (direction > 0)
+            ref_complex_out(wideband::multipoleToSpectrum(
+                                ref_complex_in(in, static_cast<std::size_t>(order + 1) * (order + 1)), order, r),
+                            out);
+        else
+            ref_complex_out(wideband::spectrumToLocal(ref_complex_in(in, r.samples.size()), r, order), out);
+    });
+}
+
+int ref_diagonal_translation(unsigned rule_order, unsigned expansion_order, double k, const double *r, double *out) {
+    return guarded([&] {
+        ref_complex_out(
+            wideband::diagonalTranslation(wideband::makeSphereRule(rule_order), expansion_order, k, {r[0], r[1], r[2]}),
+            out);
     });
 }
 }

@@ -57,6 +57,13 @@ def reference():
                                                ctypes.c_double, ctypes.c_double, _D, ctypes.POINTER(ctypes.c_uint64)]
     lib.ref_rotation_plan.argtypes = [ctypes.c_uint, _D, _D, _D, _D, ctypes.c_int, _D]
     lib.ref_rcr_queue_policy.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.POINTER(ctypes.c_uint64)]
+    lib.ref_sphere_rule.argtypes = [ctypes.c_uint, _D, _D]
+    lib.ref_spherical_harmonics.argtypes = [ctypes.c_uint, _D, _D]
+    lib.ref_regular_radial_table.argtypes = [ctypes.c_uint, ctypes.c_double, _D, _D]
+    lib.ref_angular_transfer.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_int, ctypes.c_double,
+                                         _D, _D, _D]
+    lib.ref_harmonic_spectrum.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.c_int, _D, _D]
+    lib.ref_diagonal_translation.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.c_double, _D, _D]
     lib.ref_support_tree.argtypes = [ctypes.c_uint32, _D, ctypes.c_uint32, _U, ctypes.c_uint, ctypes.c_uint,
                                      ctypes.c_double, ctypes.c_double, _D, ctypes.POINTER(ctypes.c_uint64)]
     return lib
@@ -171,6 +178,57 @@ def ref_rotation(order, axis, coefficients=None, direction=1):
         rc = lib.ref_rotation_plan(order, _d(ax), _d(d), _d(az), None, 0, None)
     _check(lib, rc, "ref_rotation_plan")
     return d, az.view(np.complex128), (out.view(np.complex128) if out is not None else None)
+
+
+def ref_sphere_rule(order):
+    lib = reference()
+    n = 2 * (order + 1) ** 2
+    d, w = np.zeros(3 * n), np.zeros(n)
+    _check(lib, lib.ref_sphere_rule(order, _d(d), _d(w)), "ref_sphere_rule")
+    return d.reshape(n, 3), w
+
+
+def ref_spherical_harmonics(order, direction):
+    lib = reference()
+    out = np.zeros(2 * (order + 1) ** 2)
+    dd = np.ascontiguousarray(direction, dtype=np.float64)
+    _check(lib, lib.ref_spherical_harmonics(order, _d(dd), _d(out)), "ref_spherical_harmonics")
+    return out.view(np.complex128)
+
+
+def ref_regular_radial_table(order, x):
+    lib = reference()
+    v, dv = np.zeros(order + 1), np.zeros(order + 1)
+    _check(lib, lib.ref_regular_radial_table(order, float(x), _d(v), _d(dv)), "ref_regular_radial_table")
+    return v, dv
+
+
+def ref_angular_transfer(values, source_order, target_order, bandlimit, mode=0, k=0.0, shift=None):
+    lib = reference()
+    v = np.ascontiguousarray(np.asarray(values, np.complex128)).view(np.float64)
+    out = np.zeros(4 * (target_order + 1) ** 2)
+    sh = None if shift is None else np.ascontiguousarray(shift, dtype=np.float64)
+    _check(lib, lib.ref_angular_transfer(source_order, target_order, bandlimit, mode, float(k),
+                                         None if sh is None else _d(sh), _d(v), _d(out)), "ref_angular_transfer")
+    return out.view(np.complex128)
+
+
+def ref_harmonic_spectrum(values, order, rule_order, direction):
+    lib = reference()
+    v = np.ascontiguousarray(np.asarray(values, np.complex128)).view(np.float64)
+    n = 2 * (rule_order + 1) ** 2 if direction > 0 else (order + 1) ** 2
+    out = np.zeros(2 * n)
+    _check(lib, lib.ref_harmonic_spectrum(order, rule_order, direction, _d(v), _d(out)), "ref_harmonic_spectrum")
+    return out.view(np.complex128)
+
+
+def ref_diagonal_translation(rule_order, expansion_order, k, r):
+    lib = reference()
+    rr = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.zeros(4 * (rule_order + 1) ** 2)
+    _check(lib, lib.ref_diagonal_translation(rule_order, expansion_order, float(k), _d(rr), _d(out)),
+           "ref_diagonal_translation")
+    return out.view(np.complex128)
 
 
 def ref_rcr_queue_policy(source_order, target_order, reserved):

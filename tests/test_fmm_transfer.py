@@ -1,0 +1,241 @@
+"""Angular and radial numerics of the wideband FMM translations: sphere rules,
+Y_lm, j_l tables, bandlimited resampling with parent/child shifts, multipole and
+local spectra, and the diagonal Helmholtz translation.
+
+Each function is bit-identical to the reference (fmm/wideband_transfer.cpp
+compiled in oracle/_ref) and pinned by the mathematics: numpy's Gauss-Legendre
+nodes, exact integration of Y_lm products (orthonormality), scipy's Y_lm and
+spherical Bessel functions, plane-wave shift identities and the Gegenbauer
+addition theorem that the diagonal translation must reproduce for
+e^{-ikr}/(4 pi r)."""
+import numpy as np
+import pytest
+from scipy.special import spherical_jn, sph_harm
+
+import oracle_bind as ob
+from hornsculpt_amd import bem
+
+needs_ref = pytest.mark.skipif(ob.reference() is None, reason="reference planner (oracle/_ref) not built")
+pytestmark = pytest.mark.filterwarnings("ignore::DeprecationWarning")
+
+DIRECTIONS = [(0.0, 0.0, 1.0), (0.0, 0.0, -1.0), (1.0, 0.0, 0.0), (0.36, -0.48, 0.8),
+              (-0.6, 0.0, -0.8), (0.0, -1.0, 0.0)]
+
+
+def _unit(v):
+    v = np.asarray(v, float)
+    return v / np.linalg.norm(v)
+
+
+def _harmonic_matrix(order, directions):
+    """Y[q, l*l+l+m] by the product's own routine (pinned separately against scipy)."""
+    return np.array([bem.spherical_harmonics(order, d) for d in directions])
+
+
+def _spectrum(order, rule_order, seed):
+    """Samples of a random degree-`order` harmonic expansion on a canonical rule."""
+    rng = np.random.default_rng(seed)
+    n = (order + 1) ** 2
+    c = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    dirs, _ = bem.sphere_rule(rule_order)
+    return c, _harmonic_matrix(order, dirs) @ c
+
+
+# ---------------------------------------------------------------- bit parity
+
+
+@needs_ref
+@pytest.mark.parametrize("order", [0, 1, 2, 5, 32, 200])
+def test_sphere_rule_bit_identical(order):
+    d, w = bem.sphere_rule(order)
+    rd, rw = ob.ref_sphere_rule(order)
+    assert np.array_equal(d, rd) and np.array_equal(w, rw)
+
+
+@needs_ref
+@pytest.mark.parametrize("order", [0, 1, 12, 100])
+@pytest.mark.parametrize("direction", DIRECTIONS)
+def test_spherical_harmonics_bit_identical(order, direction):
+    assert np.array_equal(bem.spherical_harmonics(order, direction), ob.ref_spherical_harmonics(order, direction))
+
+
+@needs_ref
+@pytest.mark.parametrize("order", [0, 1, 3, 20, 80])
+@pytest.mark.parametrize("x", [0.0, 1e-3, 0.5, 1.0, 1.0000001, 1.5, 10.0, 50.0, 113.0, 150.0, 400.0])
+def test_regular_radial_table_bit_identical(order, x):
+    v, dv = bem.regular_radial_table(order, x)
+    rv, rdv = ob.ref_regular_radial_table(order, x)
+    assert np.array_equal(v, rv) and np.array_equal(dv, rdv)
+
+
+@needs_ref
+@pytest.mark.parametrize("source,target,bandlimit", [(4, 6, 4), (6, 4, 4), (10, 10, 8), (3, 9, 2)])
+@pytest.mark.parametrize("mode", [bem.TRANSFER_RESAMPLE, bem.TRANSFER_OUTGOING_TO_PARENT,
+                                  bem.TRANSFER_INCOMING_TO_CHILD])
+def test_angular_transfer_bit_identical(source, target, bandlimit, mode):
+    rng = np.random.default_rng(source * 100 + target)
+    n = 2 * (source + 1) ** 2
+    v = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    shift = (0.03, -0.02, 0.05)
+    got = bem.angular_transfer(v, source, target, bandlimit, mode, 17.0, shift)
+    want = ob.ref_angular_transfer(v, source, target, bandlimit, mode, 17.0, shift)
+    assert np.array_equal(got, want)
+
+
+@needs_ref
+@pytest.mark.parametrize("order,rule_order", [(0, 0), (3, 5), (8, 8), (12, 20)])
+def test_harmonic_spectra_bit_identical(order, rule_order):
+    rng = np.random.default_rng(order)
+    m = rng.standard_normal((order + 1) ** 2) + 1j * rng.standard_normal((order + 1) ** 2)
+    assert np.array_equal(bem.multipole_to_spectrum(m, order, rule_order),
+                          ob.ref_harmonic_spectrum(m, order, rule_order, 1))
+    n = 2 * (rule_order + 1) ** 2
+    a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert np.array_equal(bem.spectrum_to_local(a, rule_order, order), ob.ref_harmonic_spectrum(a, order, rule_order, -1))
+
+
+@needs_ref
+@pytest.mark.parametrize("rule_order,order,k,r", [(10, 10, 30.0, (0.3, 0.1, -0.2)), (0, 0, 1.0, (0, 0, 1)),
+                                                  (40, 20, 20.0, (0.5, 0.2, 0.1)), (16, 30, 5.0, (-1, 2, 0.5))])
+def test_diagonal_translation_bit_identical(rule_order, order, k, r):
+    assert np.array_equal(bem.diagonal_translation(rule_order, order, k, r),
+                          ob.ref_diagonal_translation(rule_order, order, k, r))
+
+
+# ---------------------------------------------------------------- mathematics
+
+
+@pytest.mark.parametrize("order", [0, 1, 4, 15, 60])
+def test_sphere_rule_is_gauss_legendre_times_trapezoid(order):
+    d, w = bem.sphere_rule(order)
+    na = 2 * (order + 1)
+    x, gw = np.polynomial.legendre.leggauss(order + 1)
+    mu = d[::na, 2]
+    # descending nodes; weights 2 pi / na of the Gauss weight per azimuth
+    assert np.allclose(mu, x[::-1], atol=1e-14, rtol=0)
+    # numpy's own weights carry a few 1e-13 of relative error at 61 nodes
+    assert np.allclose(w.reshape(order + 1, na), (gw[::-1] * 2 * np.pi / na)[:, None], rtol=1e-12, atol=0)
+    phi = np.arctan2(d[:, 1], d[:, 0]) % (2 * np.pi)
+    assert np.allclose(phi.reshape(order + 1, na)[:, 1:], (2 * np.pi * np.arange(1, na) / na)[None, :], atol=1e-12)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1, atol=1e-15)
+    assert abs(w.sum() - 4 * np.pi) < 1e-12
+
+
+@pytest.mark.parametrize("order", [1, 6, 20])
+def test_sphere_rule_integrates_harmonic_products_exactly(order):
+    d, w = bem.sphere_rule(order)
+    Y = _harmonic_matrix(order, d)
+    gram = (Y.conj().T * w) @ Y
+    assert np.allclose(gram, np.eye(Y.shape[1]), atol=1e-12)
+
+
+@pytest.mark.parametrize("order", [0, 1, 5, 30])
+@pytest.mark.parametrize("direction", DIRECTIONS + [(0.2, 0.3, 0.5), (-0.7, -0.1, 0.05)])
+def test_spherical_harmonics_match_scipy(order, direction):
+    u = _unit(direction)
+    theta, phi = np.arccos(np.clip(u[2], -1, 1)), np.arctan2(u[1], u[0])
+    y = bem.spherical_harmonics(order, u)
+    for l in range(order + 1):
+        for m in range(-l, l + 1):
+            assert abs(y[l * l + l + m] - sph_harm(m, l, phi, theta)) < 1e-12 * max(1.0, np.sqrt(2 * l + 1))
+
+
+def test_spherical_harmonics_rejects_non_unit_direction():
+    with pytest.raises(ValueError, match="unit vector"):
+        bem.spherical_harmonics(3, (1.0, 1.0, 0.0))
+
+
+@pytest.mark.parametrize("order", [0, 1, 7, 40, 90])
+@pytest.mark.parametrize("x", [1e-4, 0.3, 1.0, 2.5, 9.0, 33.0, 61.0, 130.0, 500.0])
+def test_regular_radial_table_matches_scipy(order, x):
+    v, dv = bem.regular_radial_table(order, x)
+    ls = np.arange(order + 1)
+    jv, jd = spherical_jn(ls, x), spherical_jn(ls, x, derivative=True)
+    scale = np.maximum(np.abs(jv), 1e-300)
+    mask = np.abs(jv) > 1e-250
+    assert np.all(np.abs(v - jv)[mask] <= 1e-11 * scale[mask] + 1e-15)
+    dmask = np.abs(jd) > 1e-250
+    assert np.all(np.abs(dv - jd)[dmask] <= 1e-10 * np.abs(jd)[dmask] + 1e-15)
+
+
+def test_regular_radial_table_at_zero():
+    v, dv = bem.regular_radial_table(4, 0.0)
+    assert v.tolist() == [1, 0, 0, 0, 0] and dv.tolist() == [0, 1 / 3, 0, 0, 0]
+    with pytest.raises(ValueError, match="nonnegative"):
+        bem.regular_radial_table(4, -1.0)
+
+
+@pytest.mark.parametrize("source,target,degree,bandlimit", [(6, 9, 6, 6), (9, 6, 5, 6), (12, 12, 12, 7)])
+def test_resample_is_bandlimited_interpolation(source, target, degree, bandlimit):
+    c, v = _spectrum(degree, source, 3)
+    td, _ = bem.sphere_rule(target)
+    got = bem.angular_transfer(v, source, target, bandlimit)
+    keep = np.array([l <= bandlimit for l in range(degree + 1) for _ in range(2 * l + 1)])
+    want = _harmonic_matrix(degree, td) @ np.where(keep, c, 0)
+    assert np.allclose(got, want, atol=1e-11 * np.abs(c).max())
+
+
+def test_parent_child_shifts_are_plane_wave_translations():
+    k, child, parent = 9.0, np.array([0.1, -0.05, 0.08]), np.array([0.0, 0.0, 0.0])
+    y = child + np.array([0.04, 0.03, -0.05])  # a source point inside the child
+    cd, _ = bem.sphere_rule(18)
+    pd, _ = bem.sphere_rule(26)
+    # outgoing signature e^{+ik d.(y - c)} on the child rule, shifted to the parent
+    sig = np.exp(1j * k * cd @ (y - child))
+    up = bem.angular_transfer(sig, 18, 26, 18, bem.TRANSFER_OUTGOING_TO_PARENT, k, child - parent)
+    assert np.allclose(up, np.exp(1j * k * pd @ (y - parent)), atol=1e-10)
+    # incoming spectra: int a_P(d) e^{-ik d.(x-P)} = int a_c(d) e^{-ik d.(x-c)} needs
+    # a_c(d) = a_P(d) e^{-ik d.(c-P)}; a_P = e^{+ik d.(c-P+w)} leaves the bandlimited e^{+ik d.w}
+    w = np.array([-0.03, 0.02, 0.04])
+    loc = np.exp(1j * k * pd @ (child - parent + w))
+    down = bem.angular_transfer(loc, 26, 18, 18, bem.TRANSFER_INCOMING_TO_CHILD, k, child - parent)
+    assert np.allclose(down, np.exp(1j * k * cd @ w), atol=1e-10)
+
+
+@pytest.mark.parametrize("order,rule_order", [(0, 0), (4, 4), (9, 13)])
+def test_multipole_local_round_trip(order, rule_order):
+    rng = np.random.default_rng(11)
+    m = rng.standard_normal((order + 1) ** 2) + 1j * rng.standard_normal((order + 1) ** 2)
+    back = bem.spectrum_to_local(bem.multipole_to_spectrum(m, order, rule_order), rule_order, order)
+    # 4 pi i^l then 4 pi (-i)^l against an exact quadrature
+    assert np.allclose(back, 16 * np.pi ** 2 * m, atol=1e-11 * np.abs(m).max() * 16 * np.pi ** 2)
+
+
+def test_local_expansion_reproduces_the_field():
+    """u(r) = int a(d) e^{-ik d.r} dOmega equals sum L_lm j_l(kr) Y_lm(r^)."""
+    k, degree, rule = 3.0, 6, 30
+    _, a = _spectrum(degree, rule, 21)
+    d, w = bem.sphere_rule(rule)
+    L = bem.spectrum_to_local(a, rule, degree)
+    for r in ([0.1, 0.2, -0.3], [0.0, 0.0, 0.4], [-0.25, 0.05, 0.0]):
+        r = np.asarray(r)
+        direct = np.sum(w * a * np.exp(-1j * k * d @ r))
+        j, _ = bem.regular_radial_table(degree, k * np.linalg.norm(r))
+        y = bem.spherical_harmonics(degree, _unit(r))
+        series = sum(L[l * l + l + m] * j[l] * y[l * l + l + m] for l in range(degree + 1) for m in range(-l, l + 1))
+        assert abs(direct - series) < 1e-12 * max(1.0, abs(direct))
+
+
+@pytest.mark.parametrize("k,R", [(20.0, (0.5, 0.2, 0.1)), (6.0, (-0.9, 0.4, 1.1)), (40.0, (0.0, 0.0, -0.45))])
+def test_diagonal_translation_reproduces_the_greens_function(k, R):
+    """sum_q w_q T_q(R) e^{+ik d.(y-cs)} e^{-ik d.(x-ct)} = e^{-ik|x-y|}/(4 pi |x-y|)."""
+    R = np.asarray(R)
+    p = int(np.ceil(k * np.linalg.norm(R))) + 12
+    rule = 2 * p
+    d, w = bem.sphere_rule(rule)
+    T = bem.diagonal_translation(rule, p, k, R)
+    cs, ct = np.array([0.3, -0.1, 0.2]), np.array([0.3, -0.1, 0.2]) + R
+    rng = np.random.default_rng(2)
+    h = 0.5 / k
+    for _ in range(4):
+        y = cs + rng.uniform(-h, h, 3)
+        x = ct + rng.uniform(-h, h, 3)
+        approx = np.sum(w * T * np.exp(1j * k * d @ (y - cs)) * np.exp(-1j * k * d @ (x - ct)))
+        r = np.linalg.norm(x - y)
+        exact = np.exp(-1j * k * r) / (4 * np.pi * r)
+        assert abs(approx - exact) < 1e-9 * abs(exact)
+
+
+def test_diagonal_translation_rejects_degenerate_geometry():
+    with pytest.raises(ValueError, match="positive kR"):
+        bem.diagonal_translation(4, 4, 10.0, (0.0, 0.0, 0.0))
