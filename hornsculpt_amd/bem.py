@@ -147,6 +147,14 @@ def library() -> ctypes.CDLL:
                 "hsbem_angular_transfer": (ctypes.c_int, [U32, U32, U32, ctypes.c_int32, ctypes.c_double, D, D, D]),
                 "hsbem_harmonic_spectrum": (ctypes.c_int, [U32, U32, ctypes.c_int32, D, D]),
                 "hsbem_diagonal_translation": (ctypes.c_int, [U32, U32, ctypes.c_double, D, D]),
+                "hsbem_scaled_waves": (ctypes.c_int, [U32, ctypes.c_double, ctypes.c_double, D, ctypes.c_int32, D]),
+                "hsbem_scaled_expansion_evaluate": (ctypes.c_int, [U32, ctypes.c_double, ctypes.c_double, D,
+                                                                   ctypes.c_int32, D, D]),
+                "hsbem_scaled_spectrum": (ctypes.c_int, [U32, ctypes.c_double, ctypes.c_double, U32, ctypes.c_int32,
+                                                         D, D]),
+                "hsbem_axial_plan": (ctypes.c_int, [ctypes.c_int32, U32, U32, ctypes.c_double, ctypes.c_double,
+                                                    ctypes.c_double, ctypes.c_double, U32, ctypes.c_double,
+                                                    ctypes.POINTER(U64), D, D, D, D]),
                 "hsbem_support_tree_build": (ctypes.c_int, [D, U32, ctypes.POINTER(U32), U32, U32, U32,
                                                             ctypes.c_double, ctypes.c_double, D, ctypes.POINTER(U64)]),
                 "hsbem_plan_copy": (ctypes.c_int, [P, ctypes.POINTER(U32), ctypes.POINTER(U32), D,
@@ -784,6 +792,78 @@ def diagonal_translation(rule_order: int, expansion_order: int, k: float, target
     out = np.zeros(2 * _rule_size(rule_order))
     _check(library().hsbem_diagonal_translation(rule_order, expansion_order, float(k), _dptr(r), _dptr(out)))
     return out.view(np.complex128)
+
+
+AXIAL_MULTIPOLE_TO_PARENT, AXIAL_MULTIPOLE_TO_LOCAL, AXIAL_LOCAL_TO_CHILD = 0, 1, 2
+
+
+def _waves(raw: np.ndarray):
+    """(values, gradients[..., 3]) from 8-double wave records."""
+    c = raw.view(np.complex128).reshape(-1, 4)
+    return c[:, 0].copy(), c[:, 1:].copy()
+
+
+def scaled_waves(order: int, k: float, radius: float, point, outgoing: bool = False):
+    """Scaled regular (or outgoing) waves at point - centre: values[(order+1)^2] and
+    gradients[(order+1)^2, 3]."""
+    out = np.zeros(8 * (order + 1) ** 2)
+    p = np.ascontiguousarray(point, dtype=np.float64)
+    _check(library().hsbem_scaled_waves(order, float(k), float(radius), _dptr(p), int(bool(outgoing)), _dptr(out)))
+    return _waves(out)
+
+
+def scaled_expansion_evaluate(coefficients, order: int, radius: float, k: float, point, local: bool):
+    """Field value and gradient of a scaled multipole (local=False) or local expansion."""
+    c = _complex_arg(coefficients, (order + 1) ** 2, "coefficients")
+    out = np.zeros(8)
+    p = np.ascontiguousarray(point, dtype=np.float64)
+    _check(library().hsbem_scaled_expansion_evaluate(order, float(radius), float(k), _dptr(c), int(bool(local)),
+                                                     _dptr(p), _dptr(out)))
+    v, g = _waves(out)
+    return v[0], g[0]
+
+
+def scaled_multipole_to_spectrum(coefficients, order: int, radius: float, k: float, rule_order: int) -> np.ndarray:
+    c = _complex_arg(coefficients, (order + 1) ** 2, "multipoles")
+    out = np.zeros(2 * _rule_size(rule_order))
+    _check(library().hsbem_scaled_spectrum(order, float(radius), float(k), rule_order, 1, _dptr(c), _dptr(out)))
+    return out.view(np.complex128)
+
+
+def spectrum_to_scaled_local(incoming, rule_order: int, order: int, k: float, radius: float) -> np.ndarray:
+    c = _complex_arg(incoming, _rule_size(rule_order), "incoming spectrum")
+    out = np.zeros(2 * (order + 1) ** 2)
+    _check(library().hsbem_scaled_spectrum(order, float(radius), float(k), rule_order, -1, _dptr(c), _dptr(out)))
+    return out.view(np.complex128)
+
+
+def axial_plan(kind: int, source_order: int, target_order: int, k: float, source_radius: float,
+               target_radius: float, distance: float, polar_count: int, projection_radius: float = 0.0,
+               source=None):
+    """Coaxial m-block translation plan: (blocks, probe radius, translated source or None).
+    blocks[m] is the (target_order-m+1) x (source_order-m+1) complex block of order |m|."""
+    lib = library()
+    count = ctypes.c_uint64(0)
+    args = (kind, source_order, target_order, float(k), float(source_radius), float(target_radius), float(distance),
+            polar_count, float(projection_radius))
+    _check(lib.hsbem_axial_plan(*args, ctypes.byref(count), None, None, None, None))
+    coeffs, probe = np.zeros(2 * count.value), ctypes.c_double(0)
+    applied = None
+    if source is not None:
+        src = _complex_arg(source, (source_order + 1) ** 2, "axial source")
+        applied = np.zeros(2 * (target_order + 1) ** 2)
+        _check(lib.hsbem_axial_plan(*args, ctypes.byref(count), _dptr(coeffs), ctypes.byref(probe), _dptr(src),
+                                    _dptr(applied)))
+        applied = applied.view(np.complex128)
+    else:
+        _check(lib.hsbem_axial_plan(*args, ctypes.byref(count), _dptr(coeffs), ctypes.byref(probe), None, None))
+    flat = coeffs.view(np.complex128)
+    blocks, pos = [], 0
+    for m in range(min(source_order, target_order) + 1):
+        r, c = target_order - m + 1, source_order - m + 1
+        blocks.append(flat[pos:pos + r * c].reshape(r, c))
+        pos += r * c
+    return blocks, probe.value, applied
 
 
 def parse_support_near_plan(blob: np.ndarray, triangle_count: int) -> dict:

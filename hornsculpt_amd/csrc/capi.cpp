@@ -4,6 +4,7 @@
 
 #include "host/fmm_levels.hpp"
 #include "host/fmm_rotation.hpp"
+#include "host/fmm_scaled.hpp"
 #include "host/fmm_transfer.hpp"
 #include "host/support_near_plan.hpp"
 #include "host/support_tree.hpp"
@@ -805,6 +806,81 @@ int hsbem_diagonal_translation(uint32_t rule_order, uint32_t expansion_order, do
                         rule, expansion_order, k,
                         {target_minus_source[0], target_minus_source[1], target_minus_source[2]}),
                     out);
+    });
+}
+
+namespace {
+
+void wave_out(const hsbem::WaveValue &w, double *out) {
+    out[0] = w.value.real();
+    out[1] = w.value.imag();
+    for (int i = 0; i < 3; ++i) {
+        out[2 + 2 * i] = w.gradient[i].real();
+        out[3 + 2 * i] = w.gradient[i].imag();
+    }
+}
+
+} // namespace
+
+int hsbem_scaled_waves(uint32_t order, double k, double radius, const double *point, int32_t outgoing,
+                       double *waves) {
+    return guarded([&] {
+        require(point && waves, "null wave argument");
+        const hsbem::Vec3 p{point[0], point[1], point[2]};
+        const auto w = outgoing ? hsbem::scaled_outgoing_waves(order, k, radius, p)
+                                : hsbem::scaled_regular_waves(order, k, radius, p);
+        for (std::size_t j = 0; j < w.size(); ++j)
+            wave_out(w[j], waves + 8 * j);
+    });
+}
+
+int hsbem_scaled_expansion_evaluate(uint32_t order, double radius, double k, const double *coefficients,
+                                    int32_t local, const double *point, double *wave) {
+    return guarded([&] {
+        require(coefficients && point && wave, "null expansion argument");
+        const hsbem::ScaledExpansion e{order, radius,
+                                       complex_in(coefficients, static_cast<std::size_t>(order + 1) * (order + 1))};
+        const hsbem::Vec3 p{point[0], point[1], point[2]};
+        wave_out(local ? hsbem::evaluate_scaled_local(e, k, p) : hsbem::evaluate_scaled_multipole(e, k, p), wave);
+    });
+}
+
+int hsbem_scaled_spectrum(uint32_t order, double radius, double k, uint32_t rule_order, int32_t direction,
+                          const double *in, double *out) {
+    return guarded([&] {
+        require(in && out && direction != 0, "null spectrum or direction");
+        const auto rule = hsbem::make_sphere_rule(rule_order);
+        if (direction > 0)
+            complex_out(hsbem::scaled_multipole_to_spectrum(
+                            {order, radius, complex_in(in, static_cast<std::size_t>(order + 1) * (order + 1))}, k, rule),
+                        out);
+        else
+            complex_out(hsbem::spectrum_to_scaled_local(complex_in(in, rule_size(rule_order)), rule, order, k, radius)
+                            .coefficients,
+                        out);
+    });
+}
+
+int hsbem_axial_plan(int32_t kind, uint32_t source_order, uint32_t target_order, double k, double source_radius,
+                     double target_radius, double distance, uint32_t polar_count, double projection_radius,
+                     uint64_t *coefficient_count, double *coefficients, double *probe, const double *source,
+                     double *applied) {
+    return guarded([&] {
+        require(coefficient_count, "null coefficient count");
+        const auto plan = hsbem::make_axial_plan(static_cast<hsbem::AxialKind>(kind), source_order, target_order, k,
+                                                 source_radius, target_radius, distance, polar_count,
+                                                 projection_radius);
+        *coefficient_count = plan.coefficients.size();
+        if (coefficients)
+            complex_out(plan.coefficients, coefficients);
+        if (probe)
+            *probe = plan.projection_radius;
+        if (source) {
+            require(applied, "null applied output");
+            complex_out(hsbem::apply_axial_plan(
+                            plan, complex_in(source, static_cast<std::size_t>(source_order + 1) * (source_order + 1))),
+                        applied);
+        }
     });
 }
 

@@ -322,6 +322,34 @@ HSBEM_API int hsbem_harmonic_spectrum(uint32_t order, uint32_t rule_order, int32
 /* diagonalTranslation (wideband_transfer.hpp:94-101): one complex factor per rule sample. */
 HSBEM_API int hsbem_diagonal_translation(uint32_t rule_order, uint32_t expansion_order, double k,
                                          const double *target_minus_source, double *out);
+/* Scaled LF spherical waves and coaxial translation plans (host-only;
+ * modules/bem/fmm/scaled_translation.hpp:8-69, scaled_translation.cpp:1-399).
+ * A wave is 8 doubles: value (re, im) then the gradient's x, y, z (re, im). */
+/* scaledRegularWaves (outgoing == 0) / scaledOutgoingWaves (outgoing != 0),
+ * scaled_translation.hpp:19-20: (order+1)^2 waves at point - centre. */
+HSBEM_API int hsbem_scaled_waves(uint32_t order, double k, double radius, const double *point, int32_t outgoing,
+                                 double *waves);
+/* evaluateScaledMultipole (local == 0) / evaluateScaledLocal (local != 0),
+ * scaled_translation.hpp:35-36: one wave. */
+HSBEM_API int hsbem_scaled_expansion_evaluate(uint32_t order, double radius, double k, const double *coefficients,
+                                              int32_t local, const double *point, double *wave);
+/* scaledMultipoleToSpectrum (direction > 0) / spectrumToScaledLocal (direction < 0),
+ * scaled_translation.hpp:37-40, on the canonical rule of rule_order. */
+HSBEM_API int hsbem_scaled_spectrum(uint32_t order, double radius, double k, uint32_t rule_order, int32_t direction,
+                                    const double *in, double *out);
+#define HSBEM_AXIAL_MULTIPOLE_TO_PARENT 0
+#define HSBEM_AXIAL_MULTIPOLE_TO_LOCAL 1
+#define HSBEM_AXIAL_LOCAL_TO_CHILD 2
+/* makeScaledAxialPlan / applyScaledAxialPlan (scaled_translation.hpp:42-67): the
+ * per-|m| blocks (rows l = m..target_order, columns n = m..source_order,
+ * row-major, blocks in m order) into coefficients when given, their count, the
+ * probe radius used, and with source ((source_order+1)^2 complex) the translated
+ * coefficients into applied. distance: child - parent (M2M, L2L) or target -
+ * source (M2L) along +z; projection_radius 0 selects the default probe. */
+HSBEM_API int hsbem_axial_plan(int32_t kind, uint32_t source_order, uint32_t target_order, double k,
+                               double source_radius, double target_radius, double distance, uint32_t polar_count,
+                               double projection_radius, uint64_t *coefficient_count, double *coefficients,
+                               double *probe, const double *source, double *applied);
 /* RCR work model of the LF queue (makeRcrQueuePolicy, modules/bem/fmm/rcr_work.hpp:23-34):
  * out = (batch capacity, chunk batches, work per link). */
 HSBEM_API int hsbem_rcr_queue_policy(uint32_t source_order, uint32_t target_order, uint32_t reserved_batches,

@@ -14,6 +14,7 @@
 #include "fmm/support_near_plan.hpp"
 #include "fmm/support_tree.hpp"
 #include "fmm/wideband_transfer.hpp"
+#include "fmm/scaled_translation.hpp"
 
 #include <cstdint>
 #include <cstring>
@@ -540,6 +541,75 @@ int ref_diagonal_translation(unsigned rule_order, unsigned expansion_order, doub
         ref_complex_out(
             wideband::diagonalTranslation(wideband::makeSphereRule(rule_order), expansion_order, k, {r[0], r[1], r[2]}),
             out);
+    });
+}
+}
+
+namespace {
+void ref_wave_out(const wideband::ScalarWave &w, double *out) {
+    out[0] = w.value.real();
+    out[1] = w.value.imag();
+    for (int i = 0; i < 3; ++i) {
+        out[2 + 2 * i] = w.gradient[i].real();
+        out[3 + 2 * i] = w.gradient[i].imag();
+    }
+}
+} // namespace
+
+extern "C" {
+int ref_scaled_waves(unsigned order, double k, double radius, const double *p, int outgoing, double *waves) {
+    return guarded([&] {
+        const auto b = outgoing ? wideband::scaledOutgoingWaves(order, k, radius, {p[0], p[1], p[2]})
+                                : wideband::scaledRegularWaves(order, k, radius, {p[0], p[1], p[2]});
+        for (std::size_t j = 0; j < b.waves.size(); ++j)
+            ref_wave_out(b.waves[j], waves + 8 * j);
+    });
+}
+
+int ref_scaled_expansion_evaluate(unsigned order, double radius, double k, const double *c, int local,
+                                  const double *p, double *wave) {
+    return guarded([&] {
+        const auto coefficients = ref_complex_in(c, static_cast<std::size_t>(order + 1) * (order + 1));
+        if (local)
+            ref_wave_out(wideband::evaluateScaledLocal({order, radius, coefficients}, k, {p[0], p[1], p[2]}), wave);
+        else
+            ref_wave_out(wideband::evaluateScaledMultipole({order, radius, coefficients}, k, {p[0], p[1], p[2]}),
+                         wave);
+    });
+}
+
+int ref_scaled_spectrum(unsigned order, double radius, double k, unsigned rule_order, int direction, const double *in,
+                        double *out) {
+    return guarded([&] {
+        const auto rule = wideband::makeSphereRule(rule_order);
+        if (direction > 0)
+            ref_complex_out(wideband::scaledMultipoleToSpectrum(
+                                {order, radius, ref_complex_in(in, static_cast<std::size_t>(order + 1) * (order + 1))},
+                                k, rule),
+                            out);
+        else
+            ref_complex_out(
+                wideband::spectrumToScaledLocal(ref_complex_in(in, rule.samples.size()), rule, order, k, radius)
+                    .coefficients,
+                out);
+    });
+}
+
+int ref_axial_plan(int kind, unsigned ps, unsigned pt, double k, double as, double at, double d, unsigned polar,
+                   double projection, std::uint64_t *count, double *coefficients, double *probe, const double *source,
+                   double *applied) {
+    return guarded([&] {
+        const auto plan = wideband::makeScaledAxialPlan(static_cast<wideband::ScaledAxialKind>(kind), ps, pt, k, as,
+                                                        at, d, polar, projection);
+        *count = plan.coefficients.size();
+        if (coefficients)
+            ref_complex_out(plan.coefficients, coefficients);
+        if (probe)
+            *probe = plan.projectionRadius;
+        if (source)
+            ref_complex_out(wideband::applyScaledAxialPlan(
+                                plan, ref_complex_in(source, static_cast<std::size_t>(ps + 1) * (ps + 1))),
+                            applied);
     });
 }
 }

@@ -64,6 +64,14 @@ def reference():
                                          _D, _D, _D]
     lib.ref_harmonic_spectrum.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.c_int, _D, _D]
     lib.ref_diagonal_translation.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.c_double, _D, _D]
+    lib.ref_scaled_waves.argtypes = [ctypes.c_uint, ctypes.c_double, ctypes.c_double, _D, ctypes.c_int, _D]
+    lib.ref_scaled_expansion_evaluate.argtypes = [ctypes.c_uint, ctypes.c_double, ctypes.c_double, _D, ctypes.c_int,
+                                                  _D, _D]
+    lib.ref_scaled_spectrum.argtypes = [ctypes.c_uint, ctypes.c_double, ctypes.c_double, ctypes.c_uint, ctypes.c_int,
+                                        _D, _D]
+    lib.ref_axial_plan.argtypes = [ctypes.c_int, ctypes.c_uint, ctypes.c_uint, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_uint, ctypes.c_double,
+                                   ctypes.POINTER(ctypes.c_uint64), _D, _D, _D, _D]
     lib.ref_support_tree.argtypes = [ctypes.c_uint32, _D, ctypes.c_uint32, _U, ctypes.c_uint, ctypes.c_uint,
                                      ctypes.c_double, ctypes.c_double, _D, ctypes.POINTER(ctypes.c_uint64)]
     return lib
@@ -229,6 +237,48 @@ def ref_diagonal_translation(rule_order, expansion_order, k, r):
     _check(lib, lib.ref_diagonal_translation(rule_order, expansion_order, float(k), _d(rr), _d(out)),
            "ref_diagonal_translation")
     return out.view(np.complex128)
+
+
+def ref_scaled_waves(order, k, radius, point, outgoing=False):
+    lib = reference()
+    out = np.zeros(8 * (order + 1) ** 2)
+    p = np.ascontiguousarray(point, dtype=np.float64)
+    _check(lib, lib.ref_scaled_waves(order, float(k), float(radius), _d(p), int(bool(outgoing)), _d(out)),
+           "ref_scaled_waves")
+    return out
+
+
+def ref_scaled_expansion_evaluate(coefficients, order, radius, k, point, local):
+    lib = reference()
+    c = np.ascontiguousarray(np.asarray(coefficients, np.complex128)).view(np.float64)
+    out = np.zeros(8)
+    p = np.ascontiguousarray(point, dtype=np.float64)
+    _check(lib, lib.ref_scaled_expansion_evaluate(order, float(radius), float(k), _d(c), int(bool(local)), _d(p),
+                                                  _d(out)), "ref_scaled_expansion_evaluate")
+    return out
+
+
+def ref_scaled_spectrum(values, order, radius, k, rule_order, direction):
+    lib = reference()
+    v = np.ascontiguousarray(np.asarray(values, np.complex128)).view(np.float64)
+    n = 2 * (rule_order + 1) ** 2 if direction > 0 else (order + 1) ** 2
+    out = np.zeros(2 * n)
+    _check(lib, lib.ref_scaled_spectrum(order, float(radius), float(k), rule_order, direction, _d(v), _d(out)),
+           "ref_scaled_spectrum")
+    return out.view(np.complex128)
+
+
+def ref_axial_plan(kind, ps, pt, k, a_s, a_t, d, polar, projection=0.0, source=None):
+    lib = reference()
+    count = ctypes.c_uint64(0)
+    args = (kind, ps, pt, float(k), float(a_s), float(a_t), float(d), polar, float(projection))
+    _check(lib, lib.ref_axial_plan(*args, ctypes.byref(count), None, None, None, None), "ref_axial_plan")
+    coeffs, probe = np.zeros(2 * count.value), np.zeros(1)
+    src = None if source is None else np.ascontiguousarray(np.asarray(source, np.complex128)).view(np.float64)
+    applied = np.zeros(2 * (pt + 1) ** 2)
+    _check(lib, lib.ref_axial_plan(*args, ctypes.byref(count), _d(coeffs), _d(probe),
+                                   None if src is None else _d(src), _d(applied)), "ref_axial_plan")
+    return coeffs.view(np.complex128), probe[0], (applied.view(np.complex128) if src is not None else None)
 
 
 def ref_rcr_queue_policy(source_order, target_order, reserved):

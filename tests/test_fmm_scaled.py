@@ -1,0 +1,215 @@
+"""Scaled LF spherical waves, expansions and coaxial m-block translation plans of
+the wideband FMM.
+
+Bit-identical to the reference (fmm/scaled_translation.cpp compiled in
+oracle/_ref) and pinned by the mathematics: the closed forms
+R_lm = (2l+1)!! j_l(kr) / (ka)^l Y_lm and O_lm = -ik (ka)^l / (2l+1)!! h_l^(2)(kr) Y_lm
+against scipy, gradients against central differences, and the M2M / M2L / L2L
+plans against the fields they must reproduce (a monopole's Green's function and
+random expansions evaluated directly)."""
+import numpy as np
+import pytest
+from scipy.special import factorial2, sph_harm, spherical_jn, spherical_yn
+
+import oracle_bind as ob
+from hornsculpt_amd import bem
+
+needs_ref = pytest.mark.skipif(ob.reference() is None, reason="reference planner (oracle/_ref) not built")
+pytestmark = pytest.mark.filterwarnings("ignore::DeprecationWarning")
+
+POINTS = [(0.01, -0.02, 0.015), (0.0, 0.0, 0.03), (0.0, 0.0, -0.02), (0.1, 0.05, -0.2), (-0.003, 0.004, 0.0)]
+PARENT, M2L, L2L = bem.AXIAL_MULTIPOLE_TO_PARENT, bem.AXIAL_MULTIPOLE_TO_LOCAL, bem.AXIAL_LOCAL_TO_CHILD
+
+
+def _coefficients(order, seed, decay=0.2):
+    rng = np.random.default_rng(seed)
+    deg = np.floor(np.sqrt(np.arange((order + 1) ** 2)))
+    return decay ** deg * (rng.standard_normal(deg.size) + 1j * rng.standard_normal(deg.size))
+
+
+def _green(k, x):
+    r = np.linalg.norm(x)
+    return np.exp(-1j * k * r) / (4 * np.pi * r)
+
+
+# ---------------------------------------------------------------- bit parity
+
+
+@needs_ref
+@pytest.mark.parametrize("order", [0, 1, 6, 24])
+@pytest.mark.parametrize("k", [0.0, 3.0, 50.0])
+@pytest.mark.parametrize("point", POINTS + [(0.0, 0.0, 0.0)])
+@pytest.mark.parametrize("outgoing", [False, True])
+def test_scaled_waves_bit_identical(order, k, point, outgoing):
+    if outgoing and not any(point):
+        pytest.skip("the outgoing basis is singular at its centre")
+    v, g = bem.scaled_waves(order, k, 0.04, point, outgoing)
+    ref = ob.ref_scaled_waves(order, k, 0.04, point, outgoing).view(np.complex128).reshape(-1, 4)
+    assert np.array_equal(v, ref[:, 0]) and np.array_equal(g, ref[:, 1:])
+
+
+@needs_ref
+@pytest.mark.parametrize("local", [False, True])
+@pytest.mark.parametrize("point", POINTS)
+def test_scaled_expansions_bit_identical(local, point):
+    c = _coefficients(8, 4)
+    v, g = bem.scaled_expansion_evaluate(c, 8, 0.06, 10.0, point, local)
+    ref = ob.ref_scaled_expansion_evaluate(c, 8, 0.06, 10.0, point, local).view(np.complex128)
+    assert v == ref[0] and np.array_equal(g, ref[1:])
+
+
+@needs_ref
+@pytest.mark.parametrize("order,radius,k,rule", [(6, 0.05, 20.0, 10), (0, 0.1, 1.0, 2), (12, 0.02, 300.0, 14)])
+def test_scaled_spectra_bit_identical(order, radius, k, rule):
+    c = _coefficients(order, 9)
+    assert np.array_equal(bem.scaled_multipole_to_spectrum(c, order, radius, k, rule),
+                          ob.ref_scaled_spectrum(c, order, radius, k, rule, 1))
+    rng = np.random.default_rng(3)
+    n = 2 * (rule + 1) ** 2
+    a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert np.array_equal(bem.spectrum_to_scaled_local(a, rule, order, k, radius),
+                          ob.ref_scaled_spectrum(a, order, radius, k, rule, -1))
+
+
+AXIAL_CASES = [
+    (PARENT, 8, 8, 10.0, 0.02, 0.06, 0.015, 25, 0.0),
+    (M2L, 8, 8, 10.0, 0.02, 0.04, 0.2, 25, 0.0),
+    (L2L, 8, 8, 10.0, 0.06, 0.02, 0.015, 25, 0.0),
+    (M2L, 0, 30, 250.0, 0.04, 0.04, 0.2, 69, 0.0),
+    (PARENT, 5, 9, 40.0, 0.01, 0.03, 0.02, 19, 0.0),
+    (M2L, 9, 5, 40.0, 0.03, 0.03, 0.12, 19, 0.025),
+    (L2L, 7, 4, 0.0, 0.05, 0.02, 0.02, 15, 0.0),
+]
+
+
+@needs_ref
+@pytest.mark.parametrize("case", AXIAL_CASES)
+def test_axial_plans_bit_identical(case):
+    kind, ps, pt, k, a_s, a_t, d, polar, probe = case
+    c = _coefficients(ps, ps + pt)
+    blocks, got_probe, applied = bem.axial_plan(kind, ps, pt, k, a_s, a_t, d, polar, probe, c)
+    ref, ref_probe, ref_applied = ob.ref_axial_plan(kind, ps, pt, k, a_s, a_t, d, polar, probe, c)
+    assert np.array_equal(np.concatenate([b.ravel() for b in blocks]), ref)
+    assert got_probe == ref_probe and np.array_equal(applied, ref_applied)
+
+
+# ---------------------------------------------------------------- mathematics
+
+
+@pytest.mark.parametrize("k", [0.0, 7.0, 30.0, 900.0])
+@pytest.mark.parametrize("point", POINTS)
+def test_scaled_waves_closed_forms(k, point):
+    a, order = 0.04, 7
+    p = np.asarray(point)
+    r = np.linalg.norm(p)
+    u = p / r
+    th, ph = np.arccos(np.clip(u[2], -1, 1)), np.arctan2(u[1], u[0])
+    v, _ = bem.scaled_waves(order, k, a, p)
+    vo = bem.scaled_waves(order, k, a, p, outgoing=True)[0] if k * a <= 128 else None
+    for l in range(order + 1):
+        df = factorial2(2 * l + 1)
+        Y = np.array([sph_harm(m, l, ph, th) for m in range(-l, l + 1)])
+        got = v[l * l:(l + 1) ** 2]
+        want = (r / a) ** l * Y if k == 0 else df * spherical_jn(l, k * r) / (k * a) ** l * Y
+        # per degree: on the axis scipy leaves ~1e-17 where m != 0 vanishes exactly
+        assert np.all(np.abs(got - want) <= 1e-12 * np.abs(want).max() + 1e-300)
+        if vo is None:
+            continue
+        # k -> 0: h_l^(2)(kr) ~ i (2l-1)!! / (kr)^(l+1) leaves a^l / ((2l+1) r^(l+1)) Y_lm
+        h = spherical_jn(l, k * r) - 1j * spherical_yn(l, k * r) if k else 0
+        wo = a ** l / ((2 * l + 1) * r ** (l + 1)) * Y if k == 0 else -1j * k * (k * a) ** l / df * h * Y
+        assert np.all(np.abs(vo[l * l:(l + 1) ** 2] - wo) <= 1e-11 * np.abs(wo).max())
+
+
+@pytest.mark.parametrize("outgoing", [False, True])
+@pytest.mark.parametrize("point", POINTS)
+def test_scaled_wave_gradients_match_differences(outgoing, point):
+    k, a, order, h = 25.0, 0.05, 6, 1e-7
+    v, g = bem.scaled_waves(order, k, a, point, outgoing)
+    for i in range(3):
+        e = np.zeros(3)
+        e[i] = h
+        fd = (bem.scaled_waves(order, k, a, np.asarray(point) + e, outgoing)[0] -
+              bem.scaled_waves(order, k, a, np.asarray(point) - e, outgoing)[0]) / (2 * h)
+        for l in range(order + 1):
+            s = slice(l * l, (l + 1) ** 2)
+            assert np.all(np.abs(g[s, i] - fd[s]) <= 1e-6 * np.abs(g[s]).max())
+
+
+def test_regular_waves_at_the_centre_are_finite_polynomial_limits():
+    v, g = bem.scaled_waves(3, 12.0, 0.04, (0.0, 0.0, 0.0))
+    assert v[0] == pytest.approx(1 / np.sqrt(4 * np.pi)) and np.all(v[1:] == 0)
+    # r Y_1m is linear: only degree 1 has a nonzero gradient at the centre
+    assert np.all(g[0] == 0) and np.all(g[4:] == 0) and np.all(np.isfinite(g))
+    assert abs(g[2, 2]) == pytest.approx(np.sqrt(3 / (4 * np.pi)) / 0.04)
+
+
+@pytest.mark.parametrize("ka", [1.0, np.pi * (1 - 1e-9), 4.493409457909064 * (1 + 1e-9), 10.0, 30.0])
+def test_monopole_translations_reproduce_the_greens_function(ka):
+    """M2L of a monopole gives G and grad G near the target; M2M moves it to a parent."""
+    radius = 0.04
+    k = ka / radius
+    order = int(np.ceil(ka)) + 20
+    polar = 2 * order + 9
+    strength = 0.7 - 0.4j
+    mono = np.array([strength / np.sqrt(4 * np.pi)])
+    _, _, local = bem.axial_plan(M2L, 0, order, k, radius, radius, 0.2, polar, source=mono)
+    point = np.array([0.019, -0.015, 0.021])
+    x = np.array([0, 0, 0.2]) + point
+    exact = strength * _green(k, x)
+    grad = -exact * (1 + 1j * k * np.linalg.norm(x)) / np.linalg.norm(x) ** 2 * x
+    v, g = bem.scaled_expansion_evaluate(local, order, radius, k, point, local=True)
+    assert abs(v - exact) < 3e-10 * abs(exact)
+    assert np.all(np.abs(g - grad) < 3e-10 * (k + 1 / np.linalg.norm(x)) * abs(exact))
+    _, _, parent = bem.axial_plan(PARENT, 0, order, k, 0.01, radius, 0.015, polar, source=mono)
+    v, _ = bem.scaled_expansion_evaluate(parent, order, radius, k, x, local=False)
+    # the child (the monopole) sits 0.015 above the parent centre
+    want = strength * _green(k, x - np.array([0, 0, 0.015]))
+    assert abs(v - want) < 3e-10 * abs(want)
+
+
+def test_axial_plans_translate_random_expansions():
+    k, p = 10.0, 8
+    src = _coefficients(p, 1)
+    # M2M: child multipole (radius .02) at +0.015 z of the parent (radius .06)
+    _, _, up = bem.axial_plan(PARENT, p, 16, k, 0.02, 0.06, 0.015, 41, source=src)
+    for x in ([0.2, 0.1, -0.3], [0.0, 0.0, 0.25], [-0.15, 0.2, 0.1]):
+        x = np.asarray(x)
+        direct = bem.scaled_expansion_evaluate(src, p, 0.02, k, x - [0, 0, 0.015], local=False)[0]
+        moved = bem.scaled_expansion_evaluate(up, 16, 0.06, k, x, local=False)[0]
+        assert abs(moved - direct) < 1e-9 * abs(direct)
+    # M2L: target 0.2 above the source; L2L: child 0.015 above the parent
+    _, _, loc = bem.axial_plan(M2L, p, 24, k, 0.02, 0.04, 0.2, 49, source=src)
+    _, _, child = bem.axial_plan(L2L, 24, 24, k, 0.04, 0.02, 0.015, 49, source=loc)
+    for y in ([0.01, -0.005, 0.008], [0.0, 0.0, -0.01]):
+        y = np.asarray(y)
+        direct = bem.scaled_expansion_evaluate(src, p, 0.02, k, y + [0, 0, 0.2], local=False)[0]
+        across = bem.scaled_expansion_evaluate(loc, 24, 0.04, k, y, local=True)[0]
+        assert abs(across - direct) < 1e-9 * abs(direct)
+        down = bem.scaled_expansion_evaluate(child, 24, 0.02, k, y - [0, 0, 0.015], local=True)[0]
+        assert abs(down - direct) < 1e-9 * abs(direct)
+
+
+def test_axial_blocks_shape_and_probe_defaults():
+    blocks, probe, _ = bem.axial_plan(M2L, 5, 9, 40.0, 0.03, 0.03, 0.12, 19)
+    assert [b.shape for b in blocks] == [(10 - m, 6 - m) for m in range(6)]
+    assert probe == 0.03
+    assert bem.axial_plan(PARENT, 3, 3, 5.0, 0.02, 0.05, 0.01, 7)[1] == 0.02 * 1.01
+    assert bem.axial_plan(M2L, 3, 3, 5.0, 0.02, 0.3, 0.2, 7)[1] == 0.1
+    with pytest.raises(ValueError, match="polar"):
+        bem.axial_plan(M2L, 3, 3, 5.0, 0.02, 0.05, 0.2, 6)
+    with pytest.raises(ValueError, match="probe"):
+        bem.axial_plan(M2L, 3, 3, 5.0, 0.02, 0.05, 0.04, 7, projection_radius=0.05)
+
+
+def test_scaled_spectra_match_the_unscaled_conventions():
+    """scaled multipole -> spectrum multiplies degree l by (ka)^l/(2l+1)!!, and the
+    scaled local the same, around the unscaled transforms."""
+    order, a, k, rule = 5, 0.05, 20.0, 9
+    c = _coefficients(order, 2)
+    f = np.array([(k * a) ** l / factorial2(2 * l + 1) for l in range(order + 1) for _ in range(2 * l + 1)])
+    assert np.allclose(bem.scaled_multipole_to_spectrum(c, order, a, k, rule),
+                       bem.multipole_to_spectrum(c * f, order, rule), rtol=1e-13, atol=1e-15)
+    spec = bem.multipole_to_spectrum(c, order, rule)
+    assert np.allclose(bem.spectrum_to_scaled_local(spec, rule, order, k, a),
+                       bem.spectrum_to_local(spec, rule, order) * f, rtol=1e-13, atol=1e-13)
