@@ -30,6 +30,11 @@ PRECONDITIONER_NONE, PRECONDITIONER_BLOCK_JACOBI = 0, 1
 WALL, DRIVEN = 0, 1
 
 
+class _FmmSettings(ctypes.Structure):
+    _fields_ = [("leaf_capacity", ctypes.c_uint32), ("maximum_depth", ctypes.c_uint32), ("separation", ctypes.c_double),
+                ("near_ratio", ctypes.c_double), ("accuracy_seed", ctypes.c_double)]
+
+
 class _Numerics(ctypes.Structure):
     _fields_ = [
         ("quadrature_order", ctypes.c_uint32),
@@ -139,6 +144,18 @@ def library() -> ctypes.CDLL:
                 "hsbem_near_operator_copy": (ctypes.c_int, [P, ctypes.POINTER(U32), ctypes.POINTER(U32), D,
                                                             ctypes.POINTER(U32), ctypes.POINTER(U32), D, D]),
                 "hsbem_near_operator_destroy": (ctypes.c_int, [P]),
+                "hsbem_fmm_settings_default": (None, [ctypes.POINTER(_FmmSettings)]),
+                "hsbem_fmm_operator_create": (ctypes.c_int, [P, ctypes.POINTER(_Numerics), ctypes.POINTER(_FmmSettings),
+                                                             ctypes.c_double, ctypes.c_double, ctypes.POINTER(U32), U32,
+                                                             ctypes.POINTER(P)]),
+                "hsbem_fmm_operator_apply": (ctypes.c_int, [P, D, D]),
+                "hsbem_fmm_operator_precondition": (ctypes.c_int, [P, D, D]),
+                "hsbem_fmm_operator_solve": (ctypes.c_int, [P, D, D, U32, U32, ctypes.c_double, ctypes.c_int32, D,
+                                                            ctypes.POINTER(U32)]),
+                "hsbem_fmm_operator_time_apply": (ctypes.c_int, [P, U32, D]),
+                "hsbem_fmm_operator_statistics": (ctypes.c_int, [P, ctypes.POINTER(U64), D, ctypes.POINTER(U32),
+                                                                 ctypes.POINTER(U32)]),
+                "hsbem_fmm_operator_destroy": (ctypes.c_int, [P]),
                 "hsbem_rotation_plan": (ctypes.c_int, [U32, D, D, D, D, ctypes.c_int32, D]),
                 "hsbem_rcr_queue_policy": (ctypes.c_int, [U32, U32, U32, ctypes.POINTER(U64)]),
                 "hsbem_sphere_rule": (ctypes.c_int, [U32, D, D]),
@@ -477,6 +494,13 @@ class HipBemSolver:
         finally:
             lib.hsbem_near_operator_destroy(handle)
 
+    def fmm_operator(self, numerics: BemNumerics, k_max: float, k: float, active_faces: Sequence[int] = (),
+                     leaf_capacity: int = 64, maximum_depth: int = 20, separation: float = 1.5,
+                     near_ratio: float = 2.0, accuracy_seed: float = 5e-7) -> "HipFmmOperator":
+        """Matrix-free FMM pressure operator (device) of this mesh at wavenumber k."""
+        return HipFmmOperator(self, numerics, k_max, k, active_faces, leaf_capacity, maximum_depth, separation,
+                              near_ratio, accuracy_seed)
+
     def plan(self, numerics: BemNumerics, k_max: float) -> dict:
         lib = self._lib
         handle = ctypes.c_void_p()
@@ -626,6 +650,79 @@ def interaction_plan(mesh: SurfaceMesh, numerics: BemNumerics, k_max: float) -> 
         return _copy_plan(lib.hsbem_plan_sizes, lib.hsbem_plan_copy, handle, len(t))
     finally:
         lib.hsbem_plan_destroy(handle)
+
+
+class HipFmmOperator:
+    """y = A x of the row-scaled Burton-Miller pressure operator: exact near CSR plus
+    the low-frequency FMM far field (mirror of SupportFmmOperator::applyPressureDevice,
+    modules/bem/cuda/support_fmm_operator.hpp:84-91). Vectors: nv complex, mesh vertex order."""
+
+    STAT_FIELDS = ("nodes", "levels", "leaves", "top_level", "rule_points", "near_nnz", "far_pairs", "m2m_links",
+                   "m2l_links", "l2l_links", "rotations", "axial_plans", "expansion_values", "moment_values",
+                   "device_bytes", "size")
+
+    def __init__(self, solver: "HipBemSolver", numerics: BemNumerics, k_max: float, k: float,
+                 active_faces: Sequence[int], leaf_capacity: int, maximum_depth: int, separation: float,
+                 near_ratio: float, accuracy_seed: float):
+        self._lib = library()
+        self._solver = solver  # keeps the mesh alive
+        st = _FmmSettings(leaf_capacity, maximum_depth, separation, near_ratio, accuracy_seed)
+        faces = np.ascontiguousarray(active_faces if len(active_faces) else [0], dtype=np.uint32)
+        n = numerics.native()
+        self.handle = ctypes.c_void_p()
+        _check(self._lib.hsbem_fmm_operator_create(solver.handle, ctypes.byref(n), ctypes.byref(st), float(k_max),
+                                                   float(k), _uptr(faces), len(active_faces),
+                                                   ctypes.byref(self.handle)))
+        self.size = int(self.statistics()["size"])
+
+    def close(self):
+        if self.handle:
+            self._lib.hsbem_fmm_operator_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _vector(self, v) -> np.ndarray:
+        return _complex_arg(v, self.size, "FMM vector")
+
+    def apply(self, x) -> np.ndarray:
+        xv, y = self._vector(x), np.zeros(2 * self.size)
+        _check(self._lib.hsbem_fmm_operator_apply(self.handle, _dptr(xv), _dptr(y)))
+        return y.view(np.complex128)
+
+    def precondition(self, v) -> np.ndarray:
+        vv, z = self._vector(v), np.zeros(2 * self.size)
+        _check(self._lib.hsbem_fmm_operator_precondition(self.handle, _dptr(vv), _dptr(z)))
+        return z.view(np.complex128)
+
+    def solve(self, b, restart: int = 50, maximum_iterations: int = 1000, tolerance: float = 1e-8,
+              precondition: bool = True):
+        """(x, achieved relative residual, iterations) of GMRES on A x = b."""
+        bv, x = self._vector(b), np.zeros(2 * self.size)
+        achieved, its = ctypes.c_double(), ctypes.c_uint32()
+        _check(self._lib.hsbem_fmm_operator_solve(self.handle, _dptr(bv), _dptr(x), restart, maximum_iterations,
+                                                  tolerance, int(bool(precondition)), ctypes.byref(achieved),
+                                                  ctypes.byref(its)))
+        return x.view(np.complex128), achieved.value, its.value
+
+    def time_apply(self, repetitions: int = 10) -> float:
+        ms = ctypes.c_double()
+        _check(self._lib.hsbem_fmm_operator_time_apply(self.handle, repetitions, ctypes.byref(ms)))
+        return ms.value
+
+    def statistics(self) -> dict:
+        counts, times = np.zeros(16, np.uint64), np.zeros(4)
+        orders, nl = np.zeros(64, np.uint32), ctypes.c_uint32()
+        _check(self._lib.hsbem_fmm_operator_statistics(self.handle, counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                                       _dptr(times), _uptr(orders), ctypes.byref(nl)))
+        out = {k: int(v) for k, v in zip(self.STAT_FIELDS, counts)}
+        out.update(dict(zip(("tree_ms", "near_ms", "plans_ms", "moments_ms"), times.tolist())))
+        out["level_orders"] = orders[:nl.value].tolist()
+        return out
 
 
 def support_tree(mesh: SurfaceMesh, leaf_capacity: int = 32, maximum_depth: int = 20, separation: float = 1.5,

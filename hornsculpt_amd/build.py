@@ -29,9 +29,9 @@ HOST_SOURCES = [
     "host/space.cpp", "host/rules.cpp", "host/adaptive.cpp", "host/plan.cpp", "host/mesh_check.cpp",
     "host/support_tree.cpp", "host/support_near_plan.cpp", "host/fmm_levels.cpp", "host/fmm_rotation.cpp", "host/fmm_transfer.cpp",
     "host/fmm_scaled.cpp",
-    "solver.cpp", "near_operator.cpp", "server.cpp", "capi.cpp",
+    "solver.cpp", "near_operator.cpp", "fmm_operator.cpp", "server.cpp", "capi.cpp",
 ]
-DEVICE_SOURCES = ["hip/assembly.hip", "hip/linalg.hip", "hip/field.hip"]
+DEVICE_SOURCES = ["hip/assembly.hip", "hip/linalg.hip", "hip/field.hip", "hip/fmm.hip"]
 
 
 def _compile(source: str, device: bool) -> Path:

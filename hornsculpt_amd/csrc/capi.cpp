@@ -2,6 +2,7 @@
 // way the reference throws them (std::invalid_argument / std::runtime_error).
 #include "../../include/hsbem.h"
 
+#include "fmm_operator.hpp"
 #include "host/fmm_levels.hpp"
 #include "host/fmm_rotation.hpp"
 #include "host/fmm_scaled.hpp"
@@ -28,6 +29,13 @@ struct hsbem_mesh {
 struct hsbem_near_operator {
     hsbem::NearOperator op;
     std::uint32_t nv;
+};
+struct hsbem_fmm_operator {
+    std::shared_ptr<hsbem::MeshSolver> solver; // keeps the mesh's device data alive
+    hsbem::Context *ctx;
+    std::unique_ptr<hsbem::FmmOperator> op;
+    hsbem::GmresWorkspace gmres;
+    hsbem::DeviceBuffer<double> x, y;
 };
 struct hsbem_server {
     std::unique_ptr<hsbem::Server> server;
@@ -185,6 +193,23 @@ std::vector<double> serialize_tree(const hsbem::SupportTree &t, std::uint32_t nv
     }
     return b;
 }
+} // namespace
+
+namespace {
+
+// Host vector in, device operator, host vector out (the context stream).
+template <class F> void fmm_host_call(hsbem_fmm_operator *o, const double *in, double *out, F &&f) {
+    require(o && in && out, "null FMM operator argument");
+    hsbem::hip_check(hipSetDevice(o->ctx->device), "set device");
+    const std::size_t n = 2 * static_cast<std::size_t>(o->op->size());
+    hipStream_t s = o->ctx->stream;
+    o->x.upload(std::vector<double>(in, in + n), s);
+    o->y.resize(n);
+    f(o->x.data(), o->y.data(), s);
+    hsbem::hip_check(hipMemcpyAsync(out, o->y.data(), n * sizeof(double), hipMemcpyDeviceToHost, s), "FMM download");
+    hsbem::hip_check(hipStreamSynchronize(s), "FMM sync");
+}
+
 } // namespace
 
 extern "C" {
@@ -641,6 +666,120 @@ int hsbem_near_operator_create(hsbem_mesh *m, const hsbem_numerics *n, double k_
         op->nv = m->solver->nv();
         op->op = m->solver->near_operator(to_numerics(*n), k_max, k, ts, std::vector<std::uint32_t>(faces, faces + nfaces), x);
         *out = op.release();
+    });
+}
+
+void hsbem_fmm_settings_default(hsbem_fmm_settings *s) {
+    if (!s)
+        return;
+    const hsbem::FmmSettings d;
+    s->leaf_capacity = d.tree.leaf_capacity;
+    s->maximum_depth = d.tree.maximum_depth;
+    s->separation = d.tree.separation;
+    s->near_ratio = d.tree.near_ratio;
+    s->accuracy_seed = d.accuracy_seed;
+}
+
+int hsbem_fmm_operator_create(hsbem_mesh *m, const hsbem_numerics *n, const hsbem_fmm_settings *settings, double k_max,
+                              double k, const uint32_t *faces, uint32_t nfaces, hsbem_fmm_operator **out) {
+    return guarded([&] {
+        require(m && n && settings && out && (faces || !nfaces), "invalid FMM operator arguments");
+        require(std::isfinite(k) && k > 0 && std::isfinite(k_max) && k_max > 0, "invalid wavenumber");
+        hsbem::FmmSettings fs;
+        fs.tree.leaf_capacity = settings->leaf_capacity;
+        fs.tree.maximum_depth = settings->maximum_depth;
+        fs.tree.separation = settings->separation;
+        fs.tree.near_ratio = settings->near_ratio;
+        fs.accuracy_seed = settings->accuracy_seed;
+        hsbem::hip_check(hipSetDevice(m->context->ctx.device), "set device");
+        auto op = std::make_unique<hsbem_fmm_operator>();
+        op->solver = m->solver;
+        op->ctx = &m->context->ctx;
+        op->op = m->solver->fmm_operator(to_numerics(*n), k_max, k, fs, std::vector<std::uint32_t>(faces, faces + nfaces));
+        *out = op.release();
+    });
+}
+
+
+int hsbem_fmm_operator_apply(hsbem_fmm_operator *o, const double *x, double *y) {
+    return guarded([&] {
+        fmm_host_call(o, x, y, [&](const double *dx, double *dy, hipStream_t s) { o->op->apply(dx, dy, s); });
+    });
+}
+
+int hsbem_fmm_operator_precondition(hsbem_fmm_operator *o, const double *v, double *z) {
+    return guarded([&] {
+        fmm_host_call(o, v, z, [&](const double *dv, double *dz, hipStream_t s) { o->op->precondition(dv, dz, s); });
+    });
+}
+
+int hsbem_fmm_operator_solve(hsbem_fmm_operator *o, const double *b, double *x, uint32_t restart,
+                             uint32_t maximum_iterations, double tolerance, int32_t precondition, double *achieved,
+                             uint32_t *iterations) {
+    return guarded([&] {
+        require(achieved && iterations && restart >= 2 && maximum_iterations >= 1 && tolerance > 0,
+                "invalid GMRES arguments");
+        fmm_host_call(o, b, x, [&](const double *db, double *dx, hipStream_t s) {
+            hsbem::KrylovOperator kop;
+            const hsbem::FmmOperator *op = o->op.get();
+            kop.apply = [op](const double *v, double *w, hipStream_t st) { op->apply(v, w, st); };
+            if (precondition)
+                kop.precondition = [op](const double *v, double *z, hipStream_t st) { op->precondition(v, z, st); };
+            hsbem::gmres(s, kop, op->size(), db, dx, restart, maximum_iterations, tolerance, achieved, iterations,
+                         nullptr, o->gmres);
+        });
+    });
+}
+
+int hsbem_fmm_operator_time_apply(hsbem_fmm_operator *o, uint32_t repetitions, double *milliseconds) {
+    return guarded([&] {
+        require(o && milliseconds && repetitions, "invalid FMM timing arguments");
+        hsbem::hip_check(hipSetDevice(o->ctx->device), "set device");
+        const std::size_t n = 2 * static_cast<std::size_t>(o->op->size());
+        hipStream_t s = o->ctx->stream;
+        o->x.upload(std::vector<double>(n, 1.0), s);
+        o->y.resize(n);
+        o->op->apply(o->x.data(), o->y.data(), s); // warm
+        hipEvent_t e0, e1;
+        hsbem::hip_check(hipEventCreate(&e0), "event");
+        hsbem::hip_check(hipEventCreate(&e1), "event");
+        hsbem::hip_check(hipEventRecord(e0, s), "event");
+        for (uint32_t r = 0; r < repetitions; ++r)
+            o->op->apply(o->x.data(), o->y.data(), s);
+        hsbem::hip_check(hipEventRecord(e1, s), "event");
+        hsbem::hip_check(hipEventSynchronize(e1), "event sync");
+        float ms = 0;
+        hsbem::hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *milliseconds = ms / repetitions;
+    });
+}
+
+int hsbem_fmm_operator_statistics(const hsbem_fmm_operator *o, uint64_t *counts, double *times, uint32_t *orders,
+                                  uint32_t *level_count) {
+    return guarded([&] {
+        require(o && counts && times && level_count, "null FMM statistics argument");
+        const auto &st = o->op->statistics();
+        const uint64_t v[16] = {st.nodes, st.levels, st.leaves, st.top_level, st.rule_points, st.near_nnz,
+                                st.far_pairs, st.m2m_links, st.m2l_links, st.l2l_links, st.rotations, st.axial_plans,
+                                st.expansion_values, st.moment_values, st.device_bytes, o->op->size()};
+        std::copy(v, v + 16, counts);
+        times[0] = st.tree_ms;
+        times[1] = st.near_ms;
+        times[2] = st.plans_ms;
+        times[3] = st.moments_ms;
+        if (orders)
+            std::copy(st.level_orders.begin(), st.level_orders.end(), orders);
+        *level_count = static_cast<uint32_t>(st.level_orders.size());
+    });
+}
+
+int hsbem_fmm_operator_destroy(hsbem_fmm_operator *o) {
+    return guarded([&] {
+        if (o)
+            hsbem::hip_check(hipSetDevice(o->ctx->device), "set device");
+        delete o;
     });
 }
 

@@ -1,0 +1,455 @@
+#include "fmm_operator.hpp"
+
+#include "host/fmm_levels.hpp"
+#include "host/fmm_rotation.hpp"
+#include "host/fmm_scaled.hpp"
+#include "host/qualification.hpp"
+#include "host/rules.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <future>
+#include <map>
+#include <numeric>
+#include <stdexcept>
+#include <thread>
+
+namespace hsbem {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double ms_since(Clock::time_point t) { return std::chrono::duration<double, std::milli>(Clock::now() - t).count(); }
+
+using Triple = std::array<std::int64_t, 3>;
+
+Triple reduced(Triple t) {
+    std::int64_t g = 0;
+    for (auto v : t)
+        g = std::gcd(g, v < 0 ? -v : v);
+    if (g > 1)
+        for (auto &v : t)
+            v /= g;
+    return t;
+}
+
+std::int64_t squared(const Triple &t) { return t[0] * t[0] + t[1] * t[1] + t[2] * t[2]; }
+
+// Runs f(i) for i < count on up to hardware_concurrency threads.
+template <class F> void parallel_for(std::size_t count, F &&f) {
+    const unsigned workers =
+        static_cast<unsigned>(std::min<std::size_t>(count, std::max(1u, std::thread::hardware_concurrency())));
+    if (workers <= 1) {
+        for (std::size_t i = 0; i < count; ++i)
+            f(i);
+        return;
+    }
+    std::atomic<std::size_t> next{0};
+    std::vector<std::future<void>> jobs;
+    for (unsigned w = 0; w < workers; ++w)
+        jobs.push_back(std::async(std::launch::async, [&] {
+            for (std::size_t i = next++; i < count; i = next++)
+                f(i);
+        }));
+    for (auto &j : jobs)
+        j.get(); // rethrows the first worker exception
+}
+
+template <class T> void upload_nonempty(DeviceBuffer<T> &d, const std::vector<T> &v, hipStream_t s) {
+    d.upload(v.empty() ? std::vector<T>(1) : v, s);
+}
+
+} // namespace
+
+std::unique_ptr<FmmOperator> MeshSolver::fmm_operator(const Numerics &n, double k_max, double k, const FmmSettings &fs,
+                                                      const std::vector<std::uint32_t> &faces) {
+    if (!std::isfinite(k) || k <= 0 || !std::isfinite(fs.accuracy_seed) || fs.accuracy_seed <= 0 ||
+        fs.accuracy_seed >= 1)
+        throw std::invalid_argument("FMM operator needs a positive wavenumber and an accuracy seed in (0, 1).");
+    std::lock_guard lock(mutex_);
+    hipStream_t s = ctx_->stream;
+    auto op = std::unique_ptr<FmmOperator>(new FmmOperator());
+    FmmStatistics &st = op->stats_;
+    const std::uint32_t nv = this->nv(), nt = this->nt();
+    op->n_ = nv;
+
+    auto t0 = Clock::now();
+    const SupportTree tree = make_support_tree(space_, fs.tree);
+    const SupportNearPlan plan = make_support_near_plan(space_, tree, faces);
+    const SupportExpansionPlan ep = make_support_expansion_plan(tree, plan.panel_face_column, k, fs.accuracy_seed);
+    st.tree_ms = ms_since(t0);
+    st.nodes = static_cast<std::uint32_t>(tree.nodes.size());
+    st.levels = static_cast<std::uint32_t>(tree.levels.size());
+    st.far_pairs = tree.pressure_pressure.far.size();
+
+    t0 = Clock::now();
+    build_near(n, k_max, k, tree, plan, s, op->near_);
+    st.near_ms = ms_since(t0);
+    st.near_nnz = op->near_.nnz;
+
+    // ---- levels: LF order and basis radius of every active level
+    const std::uint32_t nl = st.levels;
+    std::vector<int> order(nl, -1);
+    std::vector<double> radius(nl, 0);
+    std::uint32_t top = nl;
+    for (std::uint32_t l = 0; l < nl; ++l) {
+        const WidebandLevelPlan &lp = ep.levels[l].expansion;
+        if (lp.representation == LevelRepresentation::inactive)
+            continue;
+        if (lp.signature_order > static_cast<unsigned>(dev::kMaxFmmOrder) ||
+            k * lp.active_radius > kMaximumSphericalElectricalRadius)
+            throw std::runtime_error("FMM level " + std::to_string(l) + " needs order " +
+                                     std::to_string(lp.signature_order) +
+                                     " beyond the low-frequency translations built here (at most " +
+                                     std::to_string(dev::kMaxFmmOrder) + ").");
+        order[l] = static_cast<int>(lp.signature_order);
+        radius[l] = lp.active_radius;
+        top = std::min(top, l);
+    }
+    st.top_level = top;
+    for (std::uint32_t l = 0; l < nl; ++l)
+        st.level_orders.push_back(order[l] < 0 ? 0u : static_cast<std::uint32_t>(order[l]));
+    for (std::uint32_t l = top; l < nl; ++l)
+        if (order[l] < 0)
+            throw std::runtime_error("FMM active levels are not contiguous.");
+    auto coeffs = [&](std::uint32_t l) { return static_cast<std::uint64_t>(order[l] + 1) * (order[l] + 1); };
+
+    // ---- expansion pool: multipole and local of every node with P1 DOFs on an active level
+    const std::uint64_t kNone = UINT64_MAX;
+    std::vector<std::uint64_t> mult(tree.nodes.size(), kNone), loc(tree.nodes.size(), kNone);
+    std::uint64_t pool = 0;
+    for (std::uint32_t id = 0; id < tree.nodes.size(); ++id) {
+        const SupportNode &nd = tree.nodes[id];
+        if (!nd.pressure_count || order[nd.level] < 0)
+            continue;
+        mult[id] = pool;
+        pool += coeffs(nd.level);
+        loc[id] = pool;
+        pool += coeffs(nd.level);
+    }
+    st.expansion_values = pool;
+
+    // ---- leaves, their DOFs and (leaf, support panel) items
+    std::vector<dev::FmmLeaf> leaves;
+    std::vector<std::uint32_t> leaf_dofs;
+    std::vector<dev::FmmLeafItem> items;
+    std::vector<std::int32_t> local(nv, -1);
+    std::uint64_t moments = 0;
+    for (std::uint32_t id = 0; id < tree.nodes.size(); ++id) {
+        const SupportNode &nd = tree.nodes[id];
+        if (!nd.leaf() || !nd.pressure_count || order[nd.level] < 0)
+            continue;
+        if (nd.pressure_count > static_cast<std::uint32_t>(dev::kMaxLeafDofs) || order[nd.level] > dev::kMaxLeafOrder)
+            throw std::runtime_error("FMM leaf exceeds the supported DOF count or order.");
+        dev::FmmLeaf lf{};
+        lf.center[0] = nd.center.x;
+        lf.center[1] = nd.center.y;
+        lf.center[2] = nd.center.z;
+        lf.radius = radius[nd.level];
+        lf.order = static_cast<std::uint32_t>(order[nd.level]);
+        lf.first_dof = static_cast<std::uint32_t>(leaf_dofs.size());
+        lf.dof_count = nd.pressure_count;
+        lf.emission = moments;
+        lf.multipole = mult[id];
+        lf.local = loc[id];
+        moments += static_cast<std::uint64_t>(nd.pressure_count) * coeffs(nd.level);
+        for (std::uint32_t j = 0; j < nd.pressure_count; ++j) {
+            const std::uint32_t v = tree.pressure_order[nd.first_pressure + j];
+            local[v] = static_cast<std::int32_t>(j);
+            leaf_dofs.push_back(v);
+        }
+        const auto leaf_index = static_cast<std::uint32_t>(leaves.size());
+        for (std::uint32_t q = plan.node_panel_offsets[id]; q < plan.node_panel_offsets[id + 1]; ++q) {
+            const std::uint32_t t = plan.support_panels[q];
+            dev::FmmLeafItem it{leaf_index, t, {-1, -1, -1}, 0};
+            for (int c = 0; c < 3; ++c) {
+                const std::uint32_t v = space_.panels[t].nodes[c];
+                if (tree.pressure_leaf[v] == id)
+                    it.corner[c] = local[v];
+            }
+            items.push_back(it);
+        }
+        leaves.push_back(lf);
+    }
+    op->leaf_count_ = static_cast<std::uint32_t>(leaves.size());
+    st.leaves = op->leaf_count_;
+    st.moment_values = moments;
+
+    // ---- translation links
+    t0 = Clock::now();
+    std::map<Triple, std::uint32_t> direction_index;
+    std::vector<Triple> directions;
+    std::vector<int> direction_order;
+    auto direction = [&](Triple t, int ord) {
+        t = reduced(t);
+        auto [it, fresh] = direction_index.emplace(t, static_cast<std::uint32_t>(directions.size()));
+        if (fresh) {
+            directions.push_back(t);
+            direction_order.push_back(ord);
+        } else {
+            direction_order[it->second] = std::max(direction_order[it->second], ord);
+        }
+        return it->second;
+    };
+    struct AxialKey {
+        int kind;
+        std::uint32_t source_level, target_level;
+        std::int64_t length2;
+        auto operator<=>(const AxialKey &) const = default;
+    };
+    std::map<AxialKey, std::uint32_t> axial_index;
+    std::vector<AxialKey> axial_keys;
+    auto axial = [&](AxialKey key) {
+        auto [it, fresh] = axial_index.emplace(key, static_cast<std::uint32_t>(axial_keys.size()));
+        if (fresh)
+            axial_keys.push_back(key);
+        return it->second;
+    };
+    auto half = [&](std::uint32_t l) { return tree.nodes[tree.levels[l].nodes.front()].half_width; };
+    for (std::uint32_t l = 0; l < nl; ++l)
+        for (std::uint32_t id : tree.levels[l].nodes)
+            if (std::abs(tree.nodes[id].half_width - tree.nodes[tree.levels[l].nodes.front()].half_width) >
+                1e-12 * tree.nodes[id].half_width)
+                throw std::runtime_error("FMM level cells are not uniform.");
+    // child - parent in units of half the child's cell width
+    auto child_offset = [&](const SupportNode &c, const SupportNode &p) {
+        Triple t;
+        for (int a = 0; a < 3; ++a)
+            t[a] = 2 * static_cast<std::int64_t>(c.coords[a]) + 1 - 2 * (2 * static_cast<std::int64_t>(p.coords[a]) + 1);
+        return t;
+    };
+    std::vector<dev::FmmDestination> destinations;
+    std::vector<dev::FmmLink> links;
+    auto begin_destination = [&](std::uint64_t target, int ord) {
+        destinations.push_back({target, static_cast<std::uint32_t>(ord), static_cast<std::uint32_t>(links.size()), 0, 0});
+    };
+    auto add_link = [&](std::uint64_t source, int source_order, std::uint32_t rot, std::uint32_t ax) {
+        links.push_back({source, static_cast<std::uint32_t>(source_order), rot, ax, 0});
+        ++destinations.back().link_count;
+    };
+    auto end_launch = [&](std::vector<FmmOperator::Launch> &list, std::uint32_t first) {
+        FmmOperator::Launch L{first, static_cast<std::uint32_t>(destinations.size()) - first, 0};
+        for (std::uint32_t d = first; d < destinations.size(); ++d) {
+            L.max_coefficients =
+                std::max(L.max_coefficients, (destinations[d].order + 1) * (destinations[d].order + 1));
+            for (std::uint32_t i = 0; i < destinations[d].link_count; ++i) {
+                const auto &lk = links[destinations[d].first_link + i];
+                L.max_coefficients = std::max(L.max_coefficients, (lk.source_order + 1) * (lk.source_order + 1));
+            }
+        }
+        if (L.destination_count)
+            list.push_back(L);
+    };
+    // upward: parents at level l from children at l + 1
+    for (std::uint32_t l = nl - 1; l-- > top;) {
+        const std::uint32_t first = static_cast<std::uint32_t>(destinations.size());
+        for (std::uint32_t id : tree.levels[l].nodes) {
+            const SupportNode &p = tree.nodes[id];
+            if (p.leaf() || mult[id] == kNone)
+                continue;
+            begin_destination(mult[id], order[l]);
+            for (std::uint32_t c : p.children) {
+                if (c == kNoSupportNode || mult[c] == kNone)
+                    continue;
+                const Triple off = child_offset(tree.nodes[c], p);
+                add_link(mult[c], order[l + 1], direction(off, std::max(order[l], order[l + 1])),
+                         axial({static_cast<int>(AxialKind::multipole_to_parent), l + 1, l, squared(off)}));
+                ++st.m2m_links;
+            }
+        }
+        end_launch(op->upward_, first);
+    }
+    // downward: M2L from the far sources and L2L from the parent
+    std::vector<std::vector<std::uint32_t>> far_sources(tree.nodes.size());
+    for (const SupportPair &pr : tree.pressure_pressure.far)
+        far_sources[pr.target].push_back(pr.source);
+    for (std::uint32_t l = top; l < nl; ++l) {
+        const std::uint32_t first = static_cast<std::uint32_t>(destinations.size());
+        for (std::uint32_t id : tree.levels[l].nodes) {
+            if (loc[id] == kNone)
+                continue;
+            const SupportNode &t = tree.nodes[id];
+            begin_destination(loc[id], order[l]);
+            for (std::uint32_t sid : far_sources[id]) {
+                if (mult[sid] == kNone)
+                    continue;
+                const SupportNode &sn = tree.nodes[sid];
+                Triple off;
+                for (int a = 0; a < 3; ++a)
+                    off[a] = static_cast<std::int64_t>(t.coords[a]) - static_cast<std::int64_t>(sn.coords[a]);
+                add_link(mult[sid], order[l], direction(off, order[l]),
+                         axial({static_cast<int>(AxialKind::multipole_to_local), l, l, squared(off)}));
+                ++st.m2l_links;
+            }
+            if (l > top && t.parent != kNoSupportNode && loc[t.parent] != kNone) {
+                const Triple off = child_offset(t, tree.nodes[t.parent]);
+                add_link(loc[t.parent], order[l - 1], direction(off, std::max(order[l], order[l - 1])),
+                         axial({static_cast<int>(AxialKind::local_to_child), l - 1, l, squared(off)}));
+                ++st.l2l_links;
+            }
+        }
+        end_launch(op->downward_, first);
+    }
+
+    // ---- rotation and coaxial plans (host, in parallel)
+    std::vector<RotationPlan> rotations(directions.size());
+    parallel_for(directions.size(), [&](std::size_t i) {
+        const Triple &d = directions[i];
+        rotations[i] = make_rotation_plan(static_cast<unsigned>(direction_order[i]),
+                                          {static_cast<double>(d[0]), static_cast<double>(d[1]), static_cast<double>(d[2])});
+    });
+    std::vector<AxialPlan> plans(axial_keys.size());
+    parallel_for(axial_keys.size(), [&](std::size_t i) {
+        const AxialKey &key = axial_keys[i];
+        const int ps = order[key.source_level], pt = order[key.target_level];
+        // M2L offsets count whole cells (2 half widths) of the level, parent links
+        // half widths of the child's level
+        const double unit = key.kind == static_cast<int>(AxialKind::multipole_to_local)
+                                ? 2 * half(key.source_level)
+                                : half(std::max(key.source_level, key.target_level));
+        const double distance = std::sqrt(static_cast<double>(key.length2)) * unit;
+        plans[i] = make_axial_plan(static_cast<AxialKind>(key.kind), static_cast<unsigned>(ps), static_cast<unsigned>(pt), k,
+                                   radius[key.source_level], radius[key.target_level], distance,
+                                   2 * static_cast<unsigned>(std::max(ps, pt)) + 9);
+    });
+    st.rotations = rotations.size();
+    st.axial_plans = plans.size();
+    std::vector<dev::FmmRotation> rot_table;
+    std::vector<double> small_d, azimuth;
+    for (const RotationPlan &r : rotations) {
+        rot_table.push_back({small_d.size(), azimuth.size() / 2, r.order, 0});
+        small_d.insert(small_d.end(), r.small_d.begin(), r.small_d.end());
+        for (const auto &z : r.azimuth) {
+            azimuth.push_back(z.real());
+            azimuth.push_back(z.imag());
+        }
+    }
+    std::vector<std::uint64_t> axial_offsets;
+    std::vector<double> axial_values;
+    for (const AxialPlan &p : plans) {
+        axial_offsets.push_back(axial_values.size() / 2);
+        for (const auto &z : p.coefficients) {
+            axial_values.push_back(z.real());
+            axial_values.push_back(z.imag());
+        }
+    }
+    st.plans_ms = ms_since(t0);
+
+    // ---- device data
+    std::vector<double> rule;
+    const double max_kh = [&] {
+        double h = 0;
+        for (const Panel &p : space_.panels)
+            h = std::max(h, p.max_edge);
+        return k * h;
+    }();
+    const unsigned degree = minimum_endpoint_degree(max_kh, n.quadrature_order);
+    if (degree > 8)
+        throw std::runtime_error("FMM leaf quadrature above degree 8 (k h > 2) is not built here.");
+    for (const auto &q : triangle_rule(degree)) {
+        rule.insert(rule.end(), {q.bary[0], q.bary[1], q.bary[2], q.weight});
+    }
+    st.rule_points = static_cast<std::uint32_t>(rule.size() / 4);
+    upload_nonempty(op->leaves_, leaves, s);
+    upload_nonempty(op->leaf_dofs_, leaf_dofs, s);
+    op->order_.upload(tree.pressure_order, s);
+    DeviceBuffer<dev::FmmLeafItem> d_items;
+    upload_nonempty(d_items, items, s);
+    DeviceBuffer<double> d_rule;
+    d_rule.upload(rule, s);
+    op->emission_.resize(2 * std::max<std::uint64_t>(moments, 1));
+    op->testing_.resize(2 * std::max<std::uint64_t>(moments, 1));
+    op->pool_.resize(2 * std::max<std::uint64_t>(pool, 1));
+    hip_check(hipMemsetAsync(op->emission_.data(), 0, op->emission_.bytes(), s), "clear emission");
+    hip_check(hipMemsetAsync(op->testing_.data(), 0, op->testing_.bytes(), s), "clear testing");
+    hip_check(hipMemsetAsync(op->pool_.data(), 0, op->pool_.bytes(), s), "clear expansions");
+    upload_nonempty(op->destinations_, destinations, s);
+    upload_nonempty(op->links_, links, s);
+    upload_nonempty(op->rotations_, rot_table, s);
+    upload_nonempty(op->small_d_, small_d, s);
+    upload_nonempty(op->azimuth_, azimuth, s);
+    upload_nonempty(op->axial_, axial_values, s);
+    upload_nonempty(op->axial_offsets_, axial_offsets, s);
+    DeviceBuffer<int> d_err(1);
+    hip_check(hipMemsetAsync(d_err.data(), 0, sizeof(int), s), "clear error");
+
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventRecord(e0, s), "event");
+    dev::FmmLeafMomentsLaunch M{};
+    M.geo = geo_.data();
+    M.nt = nt;
+    M.rule = d_rule.data();
+    M.rule_points = st.rule_points;
+    M.k = k;
+    M.alpha_re = 0;
+    M.alpha_im = -1 / k;
+    M.row_scale = original_row_scale_.data();
+    M.leaves = op->leaves_.data();
+    M.leaf_dofs = op->leaf_dofs_.data();
+    M.items = d_items.data();
+    M.item_count = static_cast<std::uint32_t>(items.size());
+    M.emission = op->emission_.data();
+    M.testing = op->testing_.data();
+    M.error = d_err.data();
+    dev::launch_leaf_moments(M, s);
+    hip_check(hipGetLastError(), "leaf moments launch");
+
+    // near block-Jacobi in tree order
+    const std::uint32_t nb = (nv + dev::kFmmJacobi - 1) / dev::kFmmJacobi;
+    std::vector<std::uint32_t> position(nv);
+    for (std::uint32_t i = 0; i < nv; ++i)
+        position[tree.pressure_order[i]] = i;
+    DeviceBuffer<std::uint32_t> d_pos;
+    d_pos.upload(position, s);
+    DeviceBuffer<double> blocks(2 * static_cast<std::size_t>(nb) * dev::kFmmJacobi * dev::kFmmJacobi);
+    hip_check(hipMemsetAsync(blocks.data(), 0, blocks.bytes(), s), "clear blocks");
+    dev::launch_gather_blocks(op->near_.row_offsets.data(), op->near_.columns.data(), op->near_.values.data(),
+                              op->order_.data(), d_pos.data(), nv, blocks.data(), s);
+    op->jacobi_.resize(dev::jacobi_factor_doubles(nv));
+    dev::jacobi_factor_gathered(blocks.data(), nv, op->jacobi_.data(), s);
+    hip_check(hipEventRecord(e1, s), "event");
+    hip_check(hipGetLastError(), "jacobi launch");
+    int code = 0;
+    hip_check(hipMemcpyAsync(&code, d_err.data(), sizeof(int), hipMemcpyDeviceToHost, s), "error download");
+    hip_check(hipStreamSynchronize(s), "fmm build sync");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    st.moments_ms = ms;
+    if (code)
+        throw std::runtime_error("FMM leaf moments left the bounded radial-table domain.");
+    st.device_bytes = op->near_.values.bytes() + op->near_.columns.bytes() + op->emission_.bytes() +
+                      op->testing_.bytes() + op->pool_.bytes() + op->small_d_.bytes() + op->azimuth_.bytes() +
+                      op->axial_.bytes() + op->jacobi_.bytes() + op->links_.bytes() + op->destinations_.bytes();
+    return op;
+}
+
+void FmmOperator::apply(const double *x, double *y, hipStream_t s) const {
+    dev::launch_csr_apply(near_.row_offsets.data(), near_.columns.data(), near_.values.data(), n_, x, y, s);
+    dev::launch_leaf_emit(leaves_.data(), leaf_count_, leaf_dofs_.data(), emission_.data(), x, pool_.data(), s);
+    dev::FmmTranslateLaunch T{};
+    T.links = links_.data();
+    T.rotations = rotations_.data();
+    T.small_d = small_d_.data();
+    T.azimuth = azimuth_.data();
+    T.axial = axial_.data();
+    T.axial_offsets = axial_offsets_.data();
+    T.source_expansions = pool_.data();
+    T.target_expansions = pool_.data();
+    for (const auto *list : {&upward_, &downward_})
+        for (const Launch &L : *list) {
+            T.destinations = destinations_.data() + L.first_destination;
+            T.max_coefficients = L.max_coefficients;
+            dev::launch_translate(T, L.destination_count, s);
+        }
+    dev::launch_leaf_test(leaves_.data(), leaf_count_, leaf_dofs_.data(), testing_.data(), pool_.data(), y, s);
+}
+
+void FmmOperator::precondition(const double *v, double *z, hipStream_t s) const {
+    dev::jacobi_apply(jacobi_.data(), n_, v, z, s, order_.data());
+}
+
+} // namespace hsbem

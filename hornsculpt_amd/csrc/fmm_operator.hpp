@@ -1,0 +1,77 @@
+// Matrix-free Burton-Miller pressure operator of the wideband FMM path on one GPU.
+//
+// Replaces SupportFmmOperator::applyPressureDevice and preconditionDevice
+// (modules/bem/cuda/support_fmm_operator.hpp:84-90, support_fmm_operator.cu:1220-1296,
+// support_preconditioner.cu:33-190): A p = A_near p + far field, where
+//   A_near   exact near CSR of the support near plan (the near operator's entries);
+//   far      leaf emission E p -> upward M2M -> M2L over the tree's far pairs and
+//            L2L downward -> leaf testing T L, all in the low-frequency scaled
+//            spherical basis of the level plan (orders from Eq. 18 at the accuracy
+//            seed, basis radius = the level's active radius), every translation as
+//            rotate / coaxial m-block plan / rotate back;
+//   M^{-1}   block-Jacobi: explicit inverses of the 64 x 64 diagonal blocks of the
+//            near operator in tree order.
+// Numbering: original mesh vertices (global P1 numbering), complex FP64 vectors.
+#pragma once
+
+#include "hip/fmm_kernels.hpp"
+#include "host/support_near_plan.hpp"
+#include "host/support_tree.hpp"
+#include "solver.hpp"
+
+#include <cstdint>
+#include <vector>
+
+namespace hsbem {
+
+struct FmmSettings {
+    SupportTreeSettings tree{64, 20, 1.5, 2.0};
+    double accuracy_seed{5e-7}; // Eq. 18 accuracy of the level orders
+};
+
+struct FmmStatistics {
+    std::uint32_t nodes{}, levels{}, leaves{}, top_level{}, rule_points{};
+    std::uint64_t near_nnz{}, far_pairs{}, m2m_links{}, m2l_links{}, l2l_links{}, rotations{}, axial_plans{};
+    std::uint64_t expansion_values{}, moment_values{}, device_bytes{};
+    std::vector<std::uint32_t> level_orders;
+    double tree_ms{}, near_ms{}, plans_ms{}, moments_ms{};
+};
+
+struct DeviceNear {
+    DeviceBuffer<std::uint32_t> row_offsets, columns, flux_offsets, flux_columns;
+    DeviceBuffer<double> values, flux_values;
+    std::size_t nnz{}, fnnz{}, jobs{}, regular{}, touching{}, patches{}, fan_jobs{};
+    float kernel_ms{};
+};
+
+class FmmOperator {
+  public:
+    // y = A x (device pointers, nv complex each, y may not alias x).
+    void apply(const double *x, double *y, hipStream_t stream) const;
+    // z = M^{-1} v (block-Jacobi of the near operator).
+    void precondition(const double *v, double *z, hipStream_t stream) const;
+    std::uint32_t size() const { return n_; }
+    const FmmStatistics &statistics() const { return stats_; }
+
+  private:
+    friend class MeshSolver;
+    struct Launch {
+        std::uint32_t first_destination{}, destination_count{}, max_coefficients{};
+    };
+    std::uint32_t n_{};
+    FmmStatistics stats_;
+    DeviceNear near_;
+    DeviceBuffer<dev::FmmLeaf> leaves_;
+    DeviceBuffer<std::uint32_t> leaf_dofs_, order_;
+    DeviceBuffer<double> emission_, testing_, pool_;
+    DeviceBuffer<dev::FmmDestination> destinations_;
+    DeviceBuffer<dev::FmmLink> links_;
+    DeviceBuffer<dev::FmmRotation> rotations_;
+    DeviceBuffer<double> small_d_, azimuth_, axial_;
+    DeviceBuffer<std::uint64_t> axial_offsets_;
+    DeviceBuffer<double> jacobi_;
+    std::vector<Launch> upward_, downward_;
+    std::uint32_t leaf_count_{};
+};
+
+} // namespace hsbem

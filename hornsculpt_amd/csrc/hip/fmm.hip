@@ -1,0 +1,414 @@
+// Far field of the wideband FMM operator (low-frequency spherical form), FP64.
+//
+// Replaces the device side of SupportLeafOperator and the LF translations of
+// SupportFmmOperator (modules/bem/cuda/wideband_leaf.cu:14-903,
+// wideband_rotation.cu, wideband_axial.cu, support_fmm_operator.cu:1103-1296):
+//   * leaf moments   per (leaf, support panel): the panel rule's samples of the
+//                    regular waves R_lm(y - c) = Jhat_l(k r) (r/a)^l Y_lm and their
+//                    gradients, reduced over a 16-lane group per panel, give the
+//                    emission rows conj(n . grad R_lm) phi_j and the testing rows
+//                    (-R_lm - alpha n . grad R_lm) phi_i / m_i of the leaf's P1 DOFs;
+//   * emission       multipole of a leaf = E_leaf p (one block per leaf);
+//   * translation    every M2M / M2L / L2L link is rotate-to-axis, coaxial m-block
+//                    translation, rotate-back (O(p^3)); one block per destination
+//                    expansion accumulates its links in LDS (no atomics);
+//   * testing        y_i += T_leaf L_leaf (one wavefront per DOF).
+#include "fmm_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+namespace hsbem::dev {
+namespace {
+
+constexpr int kGroup = 16;      // lanes per (leaf, panel) item: rules of at most 16 points
+constexpr int kMillerMax = 256; // Miller recurrence start bound (z <= 96 at the leaves)
+
+struct C2 {
+    double re, im;
+};
+__device__ __forceinline__ C2 operator+(C2 a, C2 b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ C2 operator-(C2 a, C2 b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ C2 operator*(C2 a, C2 b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ C2 operator*(double s, C2 a) { return {s * a.re, s * a.im}; }
+__device__ __forceinline__ C2 conj(C2 a) { return {a.re, -a.im}; }
+
+// value and gradient of one solid harmonic
+struct Wave {
+    C2 v, g[3];
+};
+
+__device__ __forceinline__ Wave product(const Wave &a, const Wave &b) {
+    Wave r;
+    r.v = a.v * b.v;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        r.g[i] = a.g[i] * b.v + a.v * b.g[i];
+    return r;
+}
+
+__device__ __forceinline__ Wave scaled(Wave a, double s) {
+    a.v = s * a.v;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        a.g[i] = s * a.g[i];
+    return a;
+}
+
+// Jhat_l(z) = (2l+1)!! j_l(z) / z^l for l = 0..top (top <= kMaxLeafOrder + 1): the
+// power series while its first correction z^2 / (2 (2l+3)) is at most one, else
+// j_l by normalised Miller recursion (downward from past the turning point,
+// normalised by the larger of j0 and j1) divided by z^l / (2l+1)!!.
+__device__ bool normalized_regular(int top, double z, double *J) {
+    int split = 0; // l < split use the j_l table
+    if (z > 1)
+        while (split <= top && z * z > 2 * (2.0 * split + 3))
+            ++split;
+    if (split) {
+        const int start = max(split + 48, static_cast<int>(ceil(z + 48 + 8 * cbrt(z))));
+        if (start + 2 > kMillerMax)
+            return false;
+        double mil[kMillerMax];
+        mil[start + 1] = 0;
+        mil[start] = 1;
+        for (int l = start; l > 0; --l) {
+            mil[l - 1] = ((2.0 * l + 1) / z) * mil[l] - mil[l + 1];
+            if (fabs(mil[l - 1]) > 1e200)
+                for (int j = l - 1; j <= start + 1; ++j)
+                    mil[j] *= 1e-200;
+        }
+        double s, c;
+        sincos(z, &s, &c);
+        const double j0 = s / z, j1 = (j0 - c) / z;
+        const double norm = fabs(j0) > fabs(j1) ? j0 / mil[0] : j1 / mil[1];
+        double lead = 1;
+        for (int l = 0; l < split; ++l) {
+            if (l)
+                lead *= z / (2.0 * l + 1);
+            J[l] = norm * mil[l] / lead;
+        }
+    }
+    for (int l = split; l <= top; ++l) {
+        double term = 1, sum = 1;
+        for (int n = 1; n < 128; ++n) {
+            term *= -z * z / (2.0 * n * (2.0 * l + 2 * n + 1));
+            sum += term;
+            if (fabs(term) < 2.220446049250313e-16 * fabs(sum))
+                break;
+        }
+        J[l] = sum;
+    }
+    return true;
+}
+
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+    for (int off = kGroup / 2; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, kGroup);
+    return v;
+}
+
+__device__ __forceinline__ void atomic_add_c(double *p, C2 v) {
+    atomicAdd(p, v.re);
+    atomicAdd(p + 1, v.im);
+}
+
+// One 16-lane group per (leaf, support panel) item, lane = rule point.
+__global__ __launch_bounds__(64) void leaf_moments(FmmLeafMomentsLaunch L) {
+    const int lane = threadIdx.x % kGroup;
+    const std::uint32_t item = blockIdx.x * (64 / kGroup) + threadIdx.x / kGroup;
+    const bool live_item = item < L.item_count;
+    const FmmLeafItem it = live_item ? L.items[item] : FmmLeafItem{};
+    const FmmLeaf leaf = L.leaves[live_item ? it.leaf : 0];
+    const std::uint32_t nt = L.nt, t = it.panel;
+    const bool live = live_item && lane < L.rule_points;
+    const double *rp = L.rule + 4 * (live ? lane : 0);
+    const double b0 = rp[0], b1 = rp[1], b2 = rp[2];
+    const double *g = L.geo;
+    const double area = g[static_cast<std::size_t>(G_AREA) * nt + t];
+    const double w = live ? rp[3] * 2 * area : 0.0; // rule weights sum to 1/2
+    double y[3], n[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double v0 = g[static_cast<std::size_t>(G_V0 + i) * nt + t];
+        y[i] = v0 + b1 * g[static_cast<std::size_t>(G_E1 + i) * nt + t] + b2 * g[static_cast<std::size_t>(G_E2 + i) * nt + t];
+        n[i] = g[static_cast<std::size_t>(G_N + i) * nt + t];
+    }
+    const double a = leaf.radius, k = L.k;
+    const double x[3] = {y[0] - leaf.center[0], y[1] - leaf.center[1], y[2] - leaf.center[2]};
+    const double u[3] = {x[0] / a, x[1] / a, x[2] / a};
+    const int p = static_cast<int>(leaf.order);
+    double J[kMaxLeafOrder + 2];
+    const bool ok = normalized_regular(p + 1, k * sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]), J);
+    if (!ok && live && L.error)
+        atomicMax(L.error, 1);
+    const double bc[3] = {b0, b1, b2};
+    const C2 alpha{L.alpha_re, L.alpha_im};
+    const std::size_t nc = static_cast<std::size_t>(p + 1) * (p + 1);
+    // solid harmonics |u|^l Y_lm(u) as polynomials in x + iy, z, r^2 with gradients in u
+    const Wave xy{{u[0], u[1]}, {{1, 0}, {0, 1}, {0, 0}}}, zz{{u[2], 0}, {{0, 0}, {0, 0}, {1, 0}}},
+        r2{{u[0] * u[0] + u[1] * u[1] + u[2] * u[2], 0}, {{2 * u[0], 0}, {2 * u[1], 0}, {2 * u[2], 0}}};
+    Wave diag{{0.28209479177387814, 0}, {{0, 0}, {0, 0}, {0, 0}}}; // 1 / sqrt(4 pi)
+    for (int m = 0; m <= p; ++m) {
+        if (m)
+            diag = scaled(product(xy, diag), -sqrt((2.0 * m + 1) / (2.0 * m)));
+        Wave prev{}, cur = diag;
+        for (int l = m; l <= p; ++l) {
+            if (l == m + 1) {
+                prev = cur;
+                cur = scaled(product(zz, diag), sqrt(2.0 * m + 3));
+            } else if (l > m + 1) {
+                const double den = static_cast<double>(l) * l - static_cast<double>(m) * m;
+                const double ca = sqrt((4.0 * l * l - 1) / den),
+                             cb = sqrt((2.0 * l + 1) * ((l - 1.0) * (l - 1.0) - static_cast<double>(m) * m) /
+                                       ((2.0 * l - 3) * den));
+                const Wave za = scaled(product(zz, cur), ca), rb = scaled(product(r2, prev), cb);
+                Wave nx;
+                nx.v = za.v - rb.v;
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    nx.g[i] = za.g[i] - rb.g[i];
+                prev = cur;
+                cur = nx;
+            }
+            // R_lm and n . grad R_lm
+            const double jl = J[l], jl1 = J[l + 1];
+            const C2 R = jl * cur.v;
+            C2 dn{0, 0};
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                dn = dn + n[i] * ((jl / a) * cur.g[i] - (k * k * x[i] / (2.0 * l + 3) * jl1) * cur.v);
+            // emission conj(dn) w, testing (-R - alpha dn) w; the -m partners follow from
+            // R_{l,-m} = (-1)^m conj(R_lm) (real n and Jhat)
+            const C2 e = w * conj(dn), tt = w * (C2{-R.re, -R.im} - alpha * dn);
+            const C2 em = (m % 2 ? -w : w) * dn,
+                     tm = (m % 2 ? -w : w) * (C2{-R.re, R.im} - alpha * conj(dn));
+            const std::size_t jp = static_cast<std::size_t>(l) * l + l + m, jm = static_cast<std::size_t>(l) * l + l - m;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double s = bc[c];
+                const C2 sums[4] = {{group_sum(s * e.re), group_sum(s * e.im)},
+                                    {group_sum(s * tt.re), group_sum(s * tt.im)},
+                                    {group_sum(s * em.re), group_sum(s * em.im)},
+                                    {group_sum(s * tm.re), group_sum(s * tm.im)}};
+                const std::int32_t j = it.corner[c];
+                if (live_item && lane == 0 && j >= 0) {
+                    const std::size_t row = static_cast<std::size_t>(j) * nc;
+                    const double scale = L.row_scale[L.leaf_dofs[leaf.first_dof + j]];
+                    atomic_add_c(L.emission + 2 * (leaf.emission + row + jp), sums[0]);
+                    atomic_add_c(L.testing + 2 * (leaf.emission + row + jp), scale * sums[1]);
+                    if (m) {
+                        atomic_add_c(L.emission + 2 * (leaf.emission + row + jm), sums[2]);
+                        atomic_add_c(L.testing + 2 * (leaf.emission + row + jm), scale * sums[3]);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// M_leaf = E_leaf p: one block per leaf, thread per coefficient.
+__global__ __launch_bounds__(256) void leaf_emit(const FmmLeaf *leaves, const std::uint32_t *leaf_dofs,
+                                                 const double *emission, const double *p, double *expansions) {
+    __shared__ double sp[2 * kMaxLeafDofs];
+    const FmmLeaf leaf = leaves[blockIdx.x];
+    for (std::uint32_t j = threadIdx.x; j < leaf.dof_count; j += blockDim.x) {
+        const std::uint32_t d = leaf_dofs[leaf.first_dof + j];
+        sp[2 * j] = p[2 * d];
+        sp[2 * j + 1] = p[2 * d + 1];
+    }
+    __syncthreads();
+    const std::size_t nc = static_cast<std::size_t>(leaf.order + 1) * (leaf.order + 1);
+    const double2 *E = reinterpret_cast<const double2 *>(emission) + leaf.emission;
+    for (std::size_t c = threadIdx.x; c < nc; c += blockDim.x) {
+        double re = 0, im = 0;
+        for (std::uint32_t j = 0; j < leaf.dof_count; ++j) {
+            const double2 e = E[j * nc + c];
+            re += e.x * sp[2 * j] - e.y * sp[2 * j + 1];
+            im += e.x * sp[2 * j + 1] + e.y * sp[2 * j];
+        }
+        expansions[2 * (leaf.multipole + c)] = re;
+        expansions[2 * (leaf.multipole + c) + 1] = im;
+    }
+}
+
+// y_i += T_leaf[i] . L_leaf: one wavefront per DOF.
+__global__ __launch_bounds__(256) void leaf_test(const FmmLeaf *leaves, const std::uint32_t *leaf_dofs,
+                                                 const double *testing, const double *expansions, double *y) {
+    const FmmLeaf leaf = leaves[blockIdx.x];
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const std::size_t nc = static_cast<std::size_t>(leaf.order + 1) * (leaf.order + 1);
+    const double2 *T = reinterpret_cast<const double2 *>(testing) + leaf.emission;
+    const double2 *Lc = reinterpret_cast<const double2 *>(expansions) + leaf.local;
+    for (std::uint32_t j = wave; j < leaf.dof_count; j += blockDim.x / 64) {
+        double re = 0, im = 0;
+        for (std::size_t c = lane; c < nc; c += 64) {
+            const double2 t = T[j * nc + c], l = Lc[c];
+            re += t.x * l.x - t.y * l.y;
+            im += t.x * l.y + t.y * l.x;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            re += __shfl_xor(re, off, 64);
+            im += __shfl_xor(im, off, 64);
+        }
+        if (lane == 0) {
+            const std::uint32_t d = leaf_dofs[leaf.first_dof + j];
+            y[2 * d] += re;
+            y[2 * d + 1] += im;
+        }
+    }
+}
+
+__device__ __forceinline__ void split_index(int e, int &l, int &t) {
+    l = static_cast<int>(sqrt(static_cast<double>(e)));
+    while (l * l > e)
+        --l;
+    while ((l + 1) * (l + 1) <= e)
+        ++l;
+    t = e - l * l;
+}
+
+// Offset of the |m| block of a coaxial plan: sum_{j<m} (pt+1-j)(ps+1-j).
+__device__ __forceinline__ std::size_t axial_block(int m, int ps, int pt) {
+    std::size_t off = 0;
+    for (int j = 0; j < m; ++j)
+        off += static_cast<std::size_t>(pt + 1 - j) * (ps + 1 - j);
+    return off;
+}
+
+// Offset of degree l in a rotation's small-d table: sum_{j<l} (2j+1)^2 = l(2l-1)(2l+1)/3.
+__device__ __forceinline__ std::size_t degree_block(int l) {
+    return static_cast<std::size_t>(l) * (2 * l - 1) * (2 * l + 1) / 3;
+}
+
+// One block per destination expansion: for each link, rotate the source to the
+// link axis, translate along it, rotate back and accumulate.
+__global__ __launch_bounds__(256) void translate(FmmTranslateLaunch L) {
+    extern __shared__ double lds[];
+    const FmmDestination dst = L.destinations[blockIdx.x];
+    const int pt = static_cast<int>(dst.order);
+    const int nct = (pt + 1) * (pt + 1);
+    C2 *acc = reinterpret_cast<C2 *>(lds);
+    C2 *b1 = acc + L.max_coefficients, *b2 = b1 + L.max_coefficients;
+    for (int e = threadIdx.x; e < nct; e += blockDim.x)
+        acc[e] = {0, 0};
+    const C2 *ex = reinterpret_cast<const C2 *>(L.source_expansions);
+    const C2 *az = reinterpret_cast<const C2 *>(L.azimuth);
+    const C2 *ax = reinterpret_cast<const C2 *>(L.axial);
+    for (std::uint32_t li = dst.first_link; li < dst.first_link + dst.link_count; ++li) {
+        const FmmLink link = L.links[li];
+        const int ps = static_cast<int>(link.source_order);
+        const int ncs = (ps + 1) * (ps + 1);
+        const FmmRotation rot = L.rotations[link.rotation];
+        const double *d = L.small_d + rot.small_d;
+        const C2 *phase = az + rot.azimuth + rot.order; // phase[m] = exp(i m phi)
+        const C2 *T = ax + L.axial_offsets[link.axial];
+        __syncthreads();
+        for (int e = threadIdx.x; e < ncs; e += blockDim.x)
+            b1[e] = ex[link.source + e];
+        __syncthreads();
+        // rotate to the axis: b2[l,t] = sum_s d_l[s][t] e^{i(s-l)phi} b1[l,s]
+        for (int e = threadIdx.x; e < ncs; e += blockDim.x) {
+            int l, t;
+            split_index(e, l, t);
+            const int n = 2 * l + 1;
+            const double *dl = d + degree_block(l);
+            C2 s{0, 0};
+            for (int q = 0; q < n; ++q)
+                s = s + dl[q * n + t] * (phase[q - l] * b1[l * l + q]);
+            b2[e] = s;
+        }
+        __syncthreads();
+        // coaxial translation, one block per |m| shared by +-m
+        const int mm = min(ps, pt);
+        for (int e = threadIdx.x; e < nct; e += blockDim.x) {
+            int l, t;
+            split_index(e, l, t);
+            const int m = t - l, am = abs(m);
+            C2 s{0, 0};
+            if (am <= mm) {
+                const C2 *blk = T + axial_block(am, ps, pt) + static_cast<std::size_t>(l - am) * (ps - am + 1);
+                for (int n = am; n <= ps; ++n)
+                    s = s + blk[n - am] * b2[n * n + n + m];
+            }
+            b1[e] = s;
+        }
+        __syncthreads();
+        // rotate back: acc[l,t] += e^{-i(t-l)phi} sum_s d_l[t][s] b1[l,s]
+        for (int e = threadIdx.x; e < nct; e += blockDim.x) {
+            int l, t;
+            split_index(e, l, t);
+            const int n = 2 * l + 1;
+            const double *dl = d + degree_block(l) + static_cast<std::size_t>(t) * n;
+            C2 s{0, 0};
+            for (int q = 0; q < n; ++q)
+                s = s + dl[q] * b1[l * l + q];
+            acc[e] = acc[e] + conj(phase[t - l]) * s;
+        }
+    }
+    __syncthreads();
+    C2 *out = reinterpret_cast<C2 *>(L.target_expansions) + dst.target;
+    for (int e = threadIdx.x; e < nct; e += blockDim.x)
+        out[e] = acc[e];
+}
+
+// Block-Jacobi blocks of the near operator: block b holds rows and columns
+// order[b*W .. b*W+W) (tree order); entries absent from the CSR are zero.
+__global__ void gather_blocks(const std::uint32_t *row_offsets, const std::uint32_t *columns, const double *values,
+                              const std::uint32_t *order, const std::uint32_t *position, std::uint32_t n,
+                              double *blocks) {
+    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const std::uint32_t row = order[i], blk = i / kFmmJacobi, r = i % kFmmJacobi;
+    double *B = blocks + static_cast<std::size_t>(blk) * kFmmJacobi * kFmmJacobi * 2;
+    for (std::uint32_t e = row_offsets[row]; e < row_offsets[row + 1]; ++e) {
+        const std::uint32_t pc = position[columns[e]];
+        if (pc / kFmmJacobi != blk)
+            continue;
+        const std::uint32_t c = pc % kFmmJacobi;
+        B[2 * (static_cast<std::size_t>(c) * kFmmJacobi + r)] = values[2 * e];
+        B[2 * (static_cast<std::size_t>(c) * kFmmJacobi + r) + 1] = values[2 * e + 1];
+    }
+}
+
+} // namespace
+
+void launch_leaf_moments(const FmmLeafMomentsLaunch &launch, hipStream_t stream) {
+    if (!launch.item_count)
+        return;
+    const std::uint32_t per_block = 64 / kGroup;
+    hipLaunchKernelGGL(leaf_moments, dim3((launch.item_count + per_block - 1) / per_block), dim3(64), 0, stream,
+                       launch);
+}
+
+void launch_leaf_emit(const FmmLeaf *leaves, std::uint32_t leaf_count, const std::uint32_t *leaf_dofs,
+                      const double *emission, const double *p, double *expansions, hipStream_t stream) {
+    if (leaf_count)
+        hipLaunchKernelGGL(leaf_emit, dim3(leaf_count), dim3(256), 0, stream, leaves, leaf_dofs, emission, p, expansions);
+}
+
+void launch_leaf_test(const FmmLeaf *leaves, std::uint32_t leaf_count, const std::uint32_t *leaf_dofs,
+                      const double *testing, const double *expansions, double *y, hipStream_t stream) {
+    if (leaf_count)
+        hipLaunchKernelGGL(leaf_test, dim3(leaf_count), dim3(256), 0, stream, leaves, leaf_dofs, testing, expansions, y);
+}
+
+void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destination_count, hipStream_t stream) {
+    if (!destination_count)
+        return;
+    const std::size_t lds = 3 * sizeof(double) * 2 * launch.max_coefficients;
+    hipLaunchKernelGGL(translate, dim3(destination_count), dim3(256), lds, stream, launch);
+}
+
+void launch_gather_blocks(const std::uint32_t *row_offsets, const std::uint32_t *columns, const double *values,
+                          const std::uint32_t *order, const std::uint32_t *position, std::uint32_t n, double *blocks,
+                          hipStream_t stream) {
+    hipLaunchKernelGGL(gather_blocks, dim3((n + 255) / 256), dim3(256), 0, stream, row_offsets, columns, values, order,
+                       position, n, blocks);
+}
+
+} // namespace hsbem::dev

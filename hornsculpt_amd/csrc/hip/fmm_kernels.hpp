@@ -1,0 +1,96 @@
+// Host-callable launchers of the FMM far-field kernels (fmm.hip).
+#pragma once
+
+#include "kernels.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+namespace hsbem::dev {
+
+inline constexpr int kMaxLeafOrder = 64;   // expansion order at the leaves
+inline constexpr int kMaxLeafDofs = 512;   // P1 DOFs of one leaf
+inline constexpr int kMaxFmmOrder = 35;    // any translated expansion (LDS: 3 x (p+1)^2 complex <= 64 KiB)
+inline constexpr int kFmmJacobi = 64;      // block width of the near block-Jacobi blocks
+
+// One leaf of the support tree with P1 DOFs. Offsets count complex values.
+struct FmmLeaf {
+    double center[3];
+    double radius;               // basis normalisation scale of the leaf's level
+    std::uint32_t order;         // expansion order of the leaf's level
+    std::uint32_t first_dof, dof_count; // into the leaf DOF list (tree order)
+    std::uint32_t pad;
+    std::uint64_t emission;      // E and T rows of this leaf: dof_count x (order+1)^2
+    std::uint64_t multipole, local; // into the expansion pool
+};
+
+// (leaf, support panel): local DOF index of each panel corner in the leaf, or -1.
+struct FmmLeafItem {
+    std::uint32_t leaf, panel;
+    std::int32_t corner[3];
+    std::uint32_t pad;
+};
+
+struct FmmLeafMomentsLaunch {
+    const double *geo;            // panel SoA (original numbering)
+    std::uint32_t nt;
+    const double *rule;           // rule_points x (b0, b1, b2, w), weights sum to 1/2
+    std::uint32_t rule_points;    // <= 16
+    double k, alpha_re, alpha_im; // Burton-Miller alpha = -i/k
+    const double *row_scale;      // 1 / lumped mass, original numbering
+    const FmmLeaf *leaves;
+    const std::uint32_t *leaf_dofs;
+    const FmmLeafItem *items;
+    std::uint32_t item_count;
+    double *emission, *testing;   // complex, zeroed before the launch
+    int *error;                   // set when a radial table leaves its bounded domain
+};
+
+// A direction of translation: Wigner small-d blocks of degrees 0..order and the
+// azimuth phases exp(i m phi), m = -order..order.
+struct FmmRotation {
+    std::uint64_t small_d; // doubles
+    std::uint64_t azimuth; // complex values
+    std::uint32_t order, pad;
+};
+
+// One translation into a destination: source expansion (pool offset), its
+// order, the rotation of the link axis and the coaxial plan.
+struct FmmLink {
+    std::uint64_t source;
+    std::uint32_t source_order, rotation, axial, pad;
+};
+
+struct FmmDestination {
+    std::uint64_t target;     // pool offset of the destination expansion (overwritten)
+    std::uint32_t order;      // destination order
+    std::uint32_t first_link, link_count, pad;
+};
+
+struct FmmTranslateLaunch {
+    const FmmDestination *destinations;
+    const FmmLink *links;
+    const FmmRotation *rotations;
+    const double *small_d;
+    const double *azimuth;          // complex
+    const double *axial;            // complex m-block coefficients of every plan
+    const std::uint64_t *axial_offsets; // complex offset of each plan
+    const double *source_expansions;
+    double *target_expansions;
+    std::uint32_t max_coefficients; // (p+1)^2 of the largest order in the launch
+};
+
+void launch_leaf_moments(const FmmLeafMomentsLaunch &launch, hipStream_t stream);
+void launch_leaf_emit(const FmmLeaf *leaves, std::uint32_t leaf_count, const std::uint32_t *leaf_dofs,
+                      const double *emission, const double *p, double *expansions, hipStream_t stream);
+void launch_leaf_test(const FmmLeaf *leaves, std::uint32_t leaf_count, const std::uint32_t *leaf_dofs,
+                      const double *testing, const double *expansions, double *y, hipStream_t stream);
+void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destination_count, hipStream_t stream);
+// blocks (zeroed, n rounded up to kFmmJacobi) from the near CSR in tree order;
+// position[dof] = index of dof in order.
+void launch_gather_blocks(const std::uint32_t *row_offsets, const std::uint32_t *columns, const double *values,
+                          const std::uint32_t *order, const std::uint32_t *position, std::uint32_t n, double *blocks,
+                          hipStream_t stream);
+
+} // namespace hsbem::dev

@@ -156,7 +156,11 @@ void lu_solve(const double *LU, std::uint32_t n, const std::int32_t *pivots, dou
 // explicit FP64 inverses per block (LU with partial pivoting), and out = M^{-1} v.
 std::size_t jacobi_factor_doubles(std::uint32_t n);
 void jacobi_factor(const void *A, bool fp32, std::uint32_t n, double *inverses, hipStream_t stream);
-void jacobi_apply(const double *inverses, std::uint32_t n, const double *v, double *out, hipStream_t stream);
+// Inverses of gathered blocks (complex FP64, column-major per 64 x 64 block).
+void jacobi_factor_gathered(const double *blocks, std::uint32_t n, double *inverses, hipStream_t stream);
+// out = M^{-1} v; block rows are order[64 b .. 64 b + 64) when order is given.
+void jacobi_apply(const double *inverses, std::uint32_t n, const double *v, double *out, hipStream_t stream,
+                  const std::uint32_t *order = nullptr);
 void launch_residual(const void *A, bool fp32, std::uint32_t n, const double *b, const double *x, std::uint32_t nvec,
                      double *residual, double *scratch, hipStream_t stream);
 
@@ -165,6 +169,8 @@ void launch_axpy(std::uint32_t n, const double *alpha, const double *x, double *
 // y = s * x (complex vector of length n, real s).
 // y += x (n complex)
 void launch_add(std::uint32_t n, const double *x, double *y, hipStream_t stream);
+// r = b - r (complex vectors of length n).
+void launch_subtract_from(std::uint32_t n, const double *b, double *r, hipStream_t stream);
 // y = x / sqrt(*norm2) (device-resident squared norm; zero vector when it is 0).
 void launch_normalize(std::uint32_t n, const double *norm2, const double *x, double *y, hipStream_t stream);
 void launch_scale(std::uint32_t n, double s, const double *x, double *y, hipStream_t stream);

@@ -210,6 +210,13 @@ __global__ void add_kernel(std::uint32_t n, const double *x, double *y) {
     y[i] += x[i];
 }
 
+__global__ void subtract_from_kernel(std::uint32_t n, const double *b, double *r) {
+    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * n)
+        return;
+    r[i] = b[i] - r[i];
+}
+
 // y = x / sqrt(*norm2) with the squared norm read on the device (0 when the norm is 0).
 __global__ void normalize_kernel(std::uint32_t n, const double *norm2, const double *x, double *y) {
     const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -519,8 +526,27 @@ __global__ __launch_bounds__(256) void block_eliminate(const double *LU, std::ui
 // is then a block-diagonal matrix-vector product.
 constexpr int kJacobi = 64;
 
-template <class T>
-__global__ __launch_bounds__(256) void jacobi_factor(const T *__restrict__ A, std::uint32_t n, double *inverses) {
+// Diagonal block entry (row r, column c of block blk) of a dense column-major matrix.
+template <class T> struct DenseBlocks {
+    const T *A;
+    std::uint32_t n;
+    __device__ double2 operator()(std::uint32_t blk, int r, int c) const {
+        const T v = A[static_cast<std::size_t>(blk * kJacobi + c) * n + blk * kJacobi + r];
+        return {static_cast<double>(v.x), static_cast<double>(v.y)};
+    }
+};
+
+// ... of gathered blocks (complex FP64, column-major per block).
+struct GatheredBlocks {
+    const double *B;
+    __device__ double2 operator()(std::uint32_t blk, int r, int c) const {
+        const double *e = B + 2 * ((static_cast<std::size_t>(blk) * kJacobi + c) * kJacobi + r);
+        return {e[0], e[1]};
+    }
+};
+
+template <class Load>
+__global__ __launch_bounds__(256) void jacobi_factor(Load load, std::uint32_t n, double *inverses) {
     __shared__ double a[kJacobi][kJacobi][2]; // LU, [col][row]
     __shared__ double x[kJacobi][kJacobi][2]; // inverse, [row][col] (thread = column: conflict free)
     __shared__ std::int32_t p[kJacobi];
@@ -532,7 +558,7 @@ __global__ __launch_bounds__(256) void jacobi_factor(const T *__restrict__ A, st
         const int c = e / kJacobi, r = e % kJacobi;
         double re = 0, im = 0;
         if (c < m && r < m) {
-            const T v = A[static_cast<std::size_t>(r0 + c) * n + r0 + r];
+            const double2 v = load(blk, r, c);
             re = v.x;
             im = v.y;
         } else if (c == r) {
@@ -639,15 +665,17 @@ __global__ __launch_bounds__(256) void jacobi_factor(const T *__restrict__ A, st
     }
 }
 
-// out = M^{-1} v: one wavefront per block, lane = row, columns streamed.
+// out = M^{-1} v: one wavefront per block, lane = row, columns streamed. Block
+// rows are order[r0 .. r0 + 64) when order is given, else consecutive.
 __global__ __launch_bounds__(64) void jacobi_apply(const double *inverses, std::uint32_t n, const double *v,
-                                                   double *out) {
+                                                   double *out, const std::uint32_t *order) {
     __shared__ double sv[kJacobi][2];
     const std::uint32_t blk = blockIdx.x, r0 = blk * kJacobi;
     const int m = static_cast<int>(min(static_cast<std::uint32_t>(kJacobi), n - r0));
     const int i = threadIdx.x;
-    sv[i][0] = i < m ? v[2 * (r0 + i)] : 0.0;
-    sv[i][1] = i < m ? v[2 * (r0 + i) + 1] : 0.0;
+    const std::uint32_t row = i < m ? (order ? order[r0 + i] : r0 + i) : 0;
+    sv[i][0] = i < m ? v[2 * row] : 0.0;
+    sv[i][1] = i < m ? v[2 * row + 1] : 0.0;
     __syncthreads();
     const double2 *f = reinterpret_cast<const double2 *>(inverses) + static_cast<std::size_t>(blk) * kJacobi * kJacobi;
     double re = 0, im = 0;
@@ -658,8 +686,8 @@ __global__ __launch_bounds__(64) void jacobi_apply(const double *inverses, std::
         im += b.x * sv[c][1] + b.y * sv[c][0];
     }
     if (i < m) {
-        out[2 * (r0 + i)] = re;
-        out[2 * (r0 + i) + 1] = im;
+        out[2 * row] = re;
+        out[2 * row + 1] = im;
     }
 }
 
@@ -731,6 +759,10 @@ void launch_add(std::uint32_t n, const double *x, double *y, hipStream_t stream)
     hipLaunchKernelGGL(add_kernel, dim3((2 * n + 255) / 256), dim3(256), 0, stream, n, x, y);
 }
 
+void launch_subtract_from(std::uint32_t n, const double *b, double *r, hipStream_t stream) {
+    hipLaunchKernelGGL(subtract_from_kernel, dim3((2 * n + 255) / 256), dim3(256), 0, stream, n, b, r);
+}
+
 void launch_axpy(std::uint32_t n, const double *alpha, const double *x, double *y, hipStream_t stream) {
     hipLaunchKernelGGL(axpy_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, alpha, x, y);
 }
@@ -795,15 +827,24 @@ std::size_t jacobi_factor_doubles(std::uint32_t n) {
 void jacobi_factor(const void *A, bool fp32, std::uint32_t n, double *inverses, hipStream_t stream) {
     const dim3 grid((n + kJacobi - 1) / kJacobi);
     if (fp32)
-        hipLaunchKernelGGL(jacobi_factor<float2>, grid, dim3(256), 0, stream, static_cast<const float2 *>(A), n,
+        hipLaunchKernelGGL(jacobi_factor<DenseBlocks<float2>>, grid, dim3(256), 0, stream,
+                           DenseBlocks<float2>{static_cast<const float2 *>(A), n}, n,
                            inverses);
     else
-        hipLaunchKernelGGL(jacobi_factor<double2>, grid, dim3(256), 0, stream, static_cast<const double2 *>(A), n,
+        hipLaunchKernelGGL(jacobi_factor<DenseBlocks<double2>>, grid, dim3(256), 0, stream,
+                           DenseBlocks<double2>{static_cast<const double2 *>(A), n}, n,
                            inverses);
 }
 
-void jacobi_apply(const double *inverses, std::uint32_t n, const double *v, double *out, hipStream_t stream) {
-    hipLaunchKernelGGL(jacobi_apply, dim3((n + kJacobi - 1) / kJacobi), dim3(64), 0, stream, inverses, n, v, out);
+void jacobi_factor_gathered(const double *blocks, std::uint32_t n, double *inverses, hipStream_t stream) {
+    const dim3 grid((n + kJacobi - 1) / kJacobi);
+    hipLaunchKernelGGL(jacobi_factor<GatheredBlocks>, grid, dim3(256), 0, stream, GatheredBlocks{blocks}, n, inverses);
+}
+
+void jacobi_apply(const double *inverses, std::uint32_t n, const double *v, double *out, hipStream_t stream,
+                  const std::uint32_t *order) {
+    hipLaunchKernelGGL(jacobi_apply, dim3((n + kJacobi - 1) / kJacobi), dim3(64), 0, stream, inverses, n, v, out,
+                       order);
 }
 
 } // namespace hsbem::dev

@@ -7,6 +7,7 @@
 // regular pairs (qualified degree), integrated in FP64 by the dense path's
 // kernels and accumulated into the CSR entries their masks select; the mass
 // terms close the near blocks. The SpMV applies A_near to a pressure vector.
+#include "fmm_operator.hpp"
 #include "host/support_near_plan.hpp"
 #include "solver.hpp"
 
@@ -15,13 +16,9 @@
 
 namespace hsbem {
 
-NearOperator MeshSolver::near_operator(const Numerics &n, double k_max, double k, const SupportTreeSettings &ts,
-                                       const std::vector<std::uint32_t> &faces, const double *x) {
-    std::lock_guard lock(mutex_);
-    hipStream_t s = ctx_->stream;
+void MeshSolver::build_near(const Numerics &n, double k_max, double k, const SupportTree &tree,
+                            const SupportNearPlan &plan, hipStream_t s, DeviceNear &out) {
     const std::uint32_t nt = this->nt(), nv = this->nv();
-    const SupportTree tree = make_support_tree(space_, ts);
-    const SupportNearPlan plan = make_support_near_plan(space_, tree, faces);
     const std::vector<SupportNearJob> jobs = support_near_jobs(space_, tree, plan);
     const std::shared_ptr<DevicePlan> dp_ref = device_plan(n, k_max, s, nullptr);
     const InteractionPlan &ip = dp_ref->plan;
@@ -77,7 +74,9 @@ NearOperator MeshSolver::near_operator(const Numerics &n, double k_max, double k
     auto up32 = [&](DeviceBuffer<std::uint32_t> &d, const std::vector<std::uint32_t> &v) {
         d.upload(v.empty() ? std::vector<std::uint32_t>{0} : v, s);
     };
-    DeviceBuffer<std::uint32_t> d_ro, d_co, d_fo, d_fc, d_face, d_reg, d_rm, d_tm, d_fm, d_pm;
+    DeviceBuffer<std::uint32_t> &d_ro = out.row_offsets, &d_co = out.columns, &d_fo = out.flux_offsets,
+                                &d_fc = out.flux_columns;
+    DeviceBuffer<std::uint32_t> d_face, d_reg, d_rm, d_tm, d_fm, d_pm;
     up32(d_ro, plan.pressure.row_offsets);
     up32(d_co, plan.pressure.columns);
     up32(d_fo, plan.flux.row_offsets);
@@ -95,7 +94,9 @@ NearOperator MeshSolver::near_operator(const Numerics &n, double k_max, double k
     DeviceBuffer<dev::NearPatchItem> d_items;
     d_items.upload(items.empty() ? std::vector<dev::NearPatchItem>(1) : items, s);
     const std::size_t nnz = plan.pressure.columns.size(), fnnz = plan.flux.columns.size();
-    DeviceBuffer<double> d_val(2 * std::max<std::size_t>(nnz, 1)), d_fval(2 * std::max<std::size_t>(fnnz, 1));
+    DeviceBuffer<double> &d_val = out.values, &d_fval = out.flux_values;
+    d_val.resize(2 * std::max<std::size_t>(nnz, 1));
+    d_fval.resize(2 * std::max<std::size_t>(fnnz, 1));
     DeviceBuffer<int> d_err(1);
     hip_check(hipMemsetAsync(d_val.data(), 0, d_val.bytes(), s), "clear near values");
     hip_check(hipMemsetAsync(d_fval.data(), 0, d_fval.bytes(), s), "clear near flux values");
@@ -143,6 +144,28 @@ NearOperator MeshSolver::near_operator(const Numerics &n, double k_max, double k
     (void)hipEventDestroy(e1);
     if (code)
         throw std::runtime_error("BEM near operator quadrature failed (device error " + std::to_string(code) + ").");
+    out.nnz = nnz;
+    out.fnnz = fnnz;
+    out.jobs = jobs.size();
+    out.regular = regular_masks.size();
+    out.touching = touching.size();
+    out.patches = items.size();
+    out.fan_jobs = fans.size();
+    out.kernel_ms = ms;
+}
+
+NearOperator MeshSolver::near_operator(const Numerics &n, double k_max, double k, const SupportTreeSettings &ts,
+                                       const std::vector<std::uint32_t> &faces, const double *x) {
+    std::lock_guard lock(mutex_);
+    hipStream_t s = ctx_->stream;
+    const std::uint32_t nv = this->nv();
+    const SupportTree tree = make_support_tree(space_, ts);
+    const SupportNearPlan plan = make_support_near_plan(space_, tree, faces);
+    DeviceNear dn;
+    build_near(n, k_max, k, tree, plan, s, dn);
+    const std::size_t nnz = dn.nnz, fnnz = dn.fnnz;
+    DeviceBuffer<double> &d_val = dn.values, &d_fval = dn.flux_values;
+    DeviceBuffer<std::uint32_t> &d_ro = dn.row_offsets, &d_co = dn.columns;
     NearOperator out;
     out.row_offsets = plan.pressure.row_offsets;
     out.columns = plan.pressure.columns;
@@ -163,12 +186,12 @@ NearOperator MeshSolver::near_operator(const Numerics &n, double k_max, double k
         hip_check(hipMemcpyAsync(out.applied.data(), dy.data(), dy.bytes(), hipMemcpyDeviceToHost, s), "apply download");
     }
     hip_check(hipStreamSynchronize(s), "near operator download");
-    out.jobs = jobs.size();
-    out.regular = regular_masks.size();
-    out.touching = touching.size();
-    out.patches = items.size();
-    out.fan_jobs = fans.size();
-    out.kernel_ms = ms;
+    out.jobs = dn.jobs;
+    out.regular = dn.regular;
+    out.touching = dn.touching;
+    out.patches = dn.patches;
+    out.fan_jobs = dn.fan_jobs;
+    out.kernel_ms = dn.kernel_ms;
     return out;
 }
 

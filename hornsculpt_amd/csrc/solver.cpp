@@ -505,10 +505,28 @@ void GmresWorkspace::release() {
 // pinned memory, by event): the device never waits for the host's rotations,
 // and at most one speculative step per restart cycle is discarded.
 void dense_gmres(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const double *b, double *x,
-                 std::uint32_t restart,
-                 std::uint32_t max_it, double tol, double *achieved, std::uint32_t *iterations,
+                 std::uint32_t restart, std::uint32_t max_it, double tol, double *achieved, std::uint32_t *iterations,
                  const volatile std::int32_t *cancel, GmresWorkspace &ws, const double *jacobi) {
+    const std::size_t scratch = dev::gemv_scratch_doubles(n, 1);
+    if (ws.scratch.size() < scratch)
+        ws.scratch.resize(scratch);
+    KrylovOperator op;
+    op.apply = [&, A, fp32, n](const double *v, double *w, hipStream_t st) {
+        dev::launch_gemv(A, fp32, n, v, w, 1, ws.scratch.data(), st);
+    };
+    op.residual = [&, A, fp32, n](const double *bb, const double *xx, double *r, hipStream_t st) {
+        dev::launch_residual(A, fp32, n, bb, xx, 1, r, ws.scratch.data(), st);
+    };
+    if (jacobi)
+        op.precondition = [jacobi, n](const double *v, double *z, hipStream_t st) { dev::jacobi_apply(jacobi, n, v, z, st); };
+    gmres(s, op, n, b, x, restart, max_it, tol, achieved, iterations, cancel, ws);
+}
+
+void gmres(hipStream_t s, const KrylovOperator &op, std::uint32_t n, const double *b, double *x, std::uint32_t restart,
+           std::uint32_t max_it, double tol, double *achieved, std::uint32_t *iterations,
+           const volatile std::int32_t *cancel, GmresWorkspace &ws) {
     using C = std::complex<double>;
+    const bool jacobi = static_cast<bool>(op.precondition);
     auto &basis_ = ws.basis;
     auto &r_ = ws.r;
     auto &scratch_ = ws.scratch;
@@ -566,10 +584,10 @@ void dense_gmres(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const
         double *w = basis_.data() + 2 * static_cast<std::size_t>(j + 1) * n;
         const double *vj = basis_.data() + 2 * static_cast<std::size_t>(j) * n;
         if (jacobi) {
-            dev::jacobi_apply(jacobi, n, vj, ws.z.data(), s);
+            op.precondition(vj, ws.z.data(), s);
             vj = ws.z.data();
         }
-        dev::launch_gemv(A, fp32, n, vj, w, 1, scratch_.data(), s);
+        op.apply(vj, w, s);
         dev::launch_project(basis_.data(), n, j + 1, w, h1, scratch_.data(), s);
         dev::launch_subtract_combination(basis_.data(), n, j + 1, h1, w, s);
         dev::launch_project(basis_.data(), n, j + 1, w, h2, scratch_.data(), s);
@@ -650,12 +668,17 @@ void dense_gmres(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const
         if (jacobi) { // x += M^{-1} V y
             hip_check(hipMemsetAsync(r_.data(), 0, 2 * sizeof(double) * n, s), "gmres clear");
             dev::launch_subtract_combination(basis_.data(), n, used, h1, r_.data(), s);
-            dev::jacobi_apply(jacobi, n, r_.data(), ws.z.data(), s);
+            op.precondition(r_.data(), ws.z.data(), s);
             dev::launch_add(n, ws.z.data(), x, s);
         } else {
             dev::launch_subtract_combination(basis_.data(), n, used, h1, x, s);
         }
-        dev::launch_residual(A, fp32, n, b, x, 1, r_.data(), scratch_.data(), s);
+        if (op.residual) {
+            op.residual(b, x, r_.data(), s);
+        } else {
+            op.apply(x, r_.data(), s);
+            dev::launch_subtract_from(n, b, r_.data(), s);
+        }
         beta = norm(r_.data());
     }
     *achieved = beta / bnorm;

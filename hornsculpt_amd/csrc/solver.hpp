@@ -180,6 +180,20 @@ void dense_gmres(hipStream_t stream, const void *A, bool fp32, std::uint32_t n, 
                  std::uint32_t restart, std::uint32_t max_it, double tol, double *achieved, std::uint32_t *iterations,
                  const volatile std::int32_t *cancel, GmresWorkspace &ws, const double *jacobi);
 
+// A matrix-free operator for gmres(): y = A x and z = M^{-1} v (optional), both
+// enqueued on the given stream; residual (optional) r = b - A x in one pass.
+struct KrylovOperator {
+    std::function<void(const double *x, double *y, hipStream_t)> apply;
+    std::function<void(const double *v, double *z, hipStream_t)> precondition;
+    std::function<void(const double *b, const double *x, double *r, hipStream_t)> residual;
+};
+
+// Restarted, right-preconditioned GMRES(restart) with CGS2 Arnoldi over any
+// device operator; converged when ||b - A x|| <= tol ||b||.
+void gmres(hipStream_t stream, const KrylovOperator &op, std::uint32_t n, const double *b, double *x,
+           std::uint32_t restart, std::uint32_t max_it, double tol, double *achieved, std::uint32_t *iterations,
+           const volatile std::int32_t *cancel, GmresWorkspace &ws);
+
 using ProgressFn = void (*)(void *, std::uint32_t, std::uint32_t, double, std::int32_t);
 
 struct DevicePlan {
@@ -228,6 +242,11 @@ struct PreparedRequest {
 // Exact near part of the FMM operator split (pressure CSR over P1 x P1 and
 // flux CSR over P1 x active faces), in mesh vertex order, rows scaled by
 // 1/lumped mass; `applied` = A_near x when x is given.
+struct DeviceNear;
+struct FmmSettings;
+class FmmOperator;
+struct SupportNearPlan;
+
 struct NearOperator {
     std::vector<std::uint32_t> row_offsets, columns, flux_offsets, flux_columns;
     std::vector<double> values, flux_values, applied; // complex
@@ -250,6 +269,9 @@ class MeshSolver {
     NearOperator near_operator(const Numerics &numerics, double k_max, double k, const SupportTreeSettings &tree,
                                const std::vector<std::uint32_t> &active_faces, const double *x);
     void assemble_dense(const Numerics &numerics, double k_max, double k, double *A, double *B);
+    // Matrix-free FMM pressure operator at wavenumber k (near entries planned for k_max).
+    std::unique_ptr<FmmOperator> fmm_operator(const Numerics &numerics, double k_max, double k,
+                                              const FmmSettings &settings, const std::vector<std::uint32_t> &active_faces);
     std::uint64_t hash() const { return hash_; }
     const Space &space() const { return space_; }
     std::uint32_t nv() const { return static_cast<std::uint32_t>(space_.vertices.size()); }
@@ -263,6 +285,9 @@ class MeshSolver {
                                         std::uint32_t nd, const std::int32_t *flux_column, int *error);
     void bind(const PreparedRequest &prepared, Workspace &ws);
     void build_clusters();
+    // Exact near entries of the support near plan into device CSR (original numbering).
+    void build_near(const Numerics &n, double k_max, double k, const SupportTree &tree, const SupportNearPlan &plan,
+                    hipStream_t stream, DeviceNear &out);
 
     Context *ctx_;
     SurfaceMesh mesh_;

@@ -285,6 +285,45 @@ HSBEM_API int hsbem_near_operator_copy(const hsbem_near_operator *op, uint32_t *
                                        double *flux_values, double *applied);
 HSBEM_API int hsbem_near_operator_destroy(hsbem_near_operator *op);
 
+/* Matrix-free FMM pressure operator (replaces SupportFmmOperator,
+ * modules/bem/cuda/support_fmm_operator.hpp:66-136, with its near part
+ * SupportNearOperator and its preconditioner SupportBlockJacobi,
+ * support_preconditioner.hpp:9-25): y = A x for the row-scaled Burton-Miller
+ * pressure operator of the mesh at wavenumber k (near entries planned for
+ * k_max), A x = exact near CSR x + far field through the support octree in the
+ * low-frequency spherical basis (orders from the level plan at accuracy_seed).
+ * Vectors are nv complex (interleaved), global P1 numbering. */
+typedef struct hsbem_fmm_settings {
+    uint32_t leaf_capacity, maximum_depth;
+    double separation, near_ratio;
+    double accuracy_seed;
+} hsbem_fmm_settings;
+typedef struct hsbem_fmm_operator hsbem_fmm_operator;
+HSBEM_API void hsbem_fmm_settings_default(hsbem_fmm_settings *settings);
+HSBEM_API int hsbem_fmm_operator_create(hsbem_mesh *mesh, const hsbem_numerics *numerics,
+                                        const hsbem_fmm_settings *settings, double k_max, double k,
+                                        const uint32_t *active_faces, uint32_t active_face_count,
+                                        hsbem_fmm_operator **out);
+/* applyPressureDevice (support_fmm_operator.hpp:86-88): y = A x. */
+HSBEM_API int hsbem_fmm_operator_apply(hsbem_fmm_operator *op, const double *x, double *y);
+/* preconditionDevice (support_fmm_operator.hpp:91): z = M^{-1} v, 64 x 64 near
+ * diagonal blocks in tree order. */
+HSBEM_API int hsbem_fmm_operator_precondition(hsbem_fmm_operator *op, const double *v, double *z);
+/* Restarted GMRES on A x = b with the FMM operator (DeviceGmresSolver::solve,
+ * modules/bem/cuda/device_gmres.hpp:11-35), right block-Jacobi when precondition != 0. */
+HSBEM_API int hsbem_fmm_operator_solve(hsbem_fmm_operator *op, const double *b, double *x, uint32_t restart,
+                                       uint32_t maximum_iterations, double tolerance, int32_t precondition,
+                                       double *achieved, uint32_t *iterations);
+/* Average device time of one apply over repetitions (HIP events, context stream). */
+HSBEM_API int hsbem_fmm_operator_time_apply(hsbem_fmm_operator *op, uint32_t repetitions, double *milliseconds);
+/* counts[16] = nodes, levels, leaves, top level, rule points, near nnz, far pairs,
+ * M2M links, M2L links, L2L links, rotations, coaxial plans, expansion values,
+ * moment values, device bytes, nv; times[4] = tree, near, plans, moments (ms);
+ * orders[level_count] = expansion order per level (0: inactive). */
+HSBEM_API int hsbem_fmm_operator_statistics(const hsbem_fmm_operator *op, uint64_t *counts, double *times,
+                                            uint32_t *orders, uint32_t *level_count);
+HSBEM_API int hsbem_fmm_operator_destroy(hsbem_fmm_operator *op);
+
 /* Spherical-harmonic rotation plan of the LF FMM translations (host-only;
  * replaces makeSphericalRotationPlan / rotateToAxis / rotateFromAxis,
  * modules/bem/fmm/spherical_rotation.hpp:8-27): Wigner small-d blocks of
