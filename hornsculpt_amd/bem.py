@@ -715,12 +715,12 @@ class HipFmmOperator:
         return ms.value
 
     def statistics(self) -> dict:
-        counts, times = np.zeros(16, np.uint64), np.zeros(4)
+        counts, times = np.zeros(16, np.uint64), np.zeros(5)
         orders, nl = np.zeros(64, np.uint32), ctypes.c_uint32()
         _check(self._lib.hsbem_fmm_operator_statistics(self.handle, counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
                                                        _dptr(times), _uptr(orders), ctypes.byref(nl)))
         out = {k: int(v) for k, v in zip(self.STAT_FIELDS, counts)}
-        out.update(dict(zip(("tree_ms", "near_ms", "plans_ms", "moments_ms"), times.tolist())))
+        out.update(dict(zip(("tree_ms", "near_ms", "plans_ms", "moments_ms", "near_kernel_ms"), times.tolist())))
         out["level_orders"] = orders[:nl.value].tolist()
         return out
 

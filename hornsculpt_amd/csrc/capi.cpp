@@ -769,6 +769,7 @@ int hsbem_fmm_operator_statistics(const hsbem_fmm_operator *o, uint64_t *counts,
         times[1] = st.near_ms;
         times[2] = st.plans_ms;
         times[3] = st.moments_ms;
+        times[4] = st.near_kernel_ms;
         if (orders)
             std::copy(st.level_orders.begin(), st.level_orders.end(), orders);
         *level_count = static_cast<uint32_t>(st.level_orders.size());

@@ -86,6 +86,7 @@ std::unique_ptr<FmmOperator> MeshSolver::fmm_operator(const Numerics &n, double 
     build_near(n, k_max, k, tree, plan, s, op->near_);
     st.near_ms = ms_since(t0);
     st.near_nnz = op->near_.nnz;
+    st.near_kernel_ms = op->near_.kernel_ms;
 
     // ---- levels: LF order and basis radius of every active level
     const std::uint32_t nl = st.levels;
@@ -218,25 +219,61 @@ std::unique_ptr<FmmOperator> MeshSolver::fmm_operator(const Numerics &n, double 
             t[a] = 2 * static_cast<std::int64_t>(c.coords[a]) + 1 - 2 * (2 * static_cast<std::int64_t>(p.coords[a]) + 1);
         return t;
     };
+    // Links of one launch are stored in destination order with launch-relative
+    // indices; each gets a workspace slot for the larger of its two orders.
     std::vector<dev::FmmDestination> destinations;
     std::vector<dev::FmmLink> links;
+    std::vector<std::uint32_t> link_order, axial_order;
+    std::vector<dev::FmmChunk> chunks, axial_chunks;
+    std::uint32_t launch_link_base = 0;
+    std::uint64_t slot = 0, workspace = 0;
     auto begin_destination = [&](std::uint64_t target, int ord) {
-        destinations.push_back({target, static_cast<std::uint32_t>(ord), static_cast<std::uint32_t>(links.size()), 0, 0});
+        destinations.push_back({target, static_cast<std::uint32_t>(ord),
+                                static_cast<std::uint32_t>(links.size()) - launch_link_base, 0, 0});
     };
     auto add_link = [&](std::uint64_t source, int source_order, std::uint32_t rot, std::uint32_t ax) {
-        links.push_back({source, static_cast<std::uint32_t>(source_order), rot, ax, 0});
+        const int target_order = static_cast<int>(destinations.back().order);
+        const int big = std::max(source_order, target_order);
+        links.push_back({source, slot, static_cast<std::uint32_t>(source_order), static_cast<std::uint32_t>(target_order),
+                         rot, ax});
+        slot += static_cast<std::uint64_t>(big + 1) * (big + 1);
         ++destinations.back().link_count;
     };
-    auto end_launch = [&](std::vector<FmmOperator::Launch> &list, std::uint32_t first) {
-        FmmOperator::Launch L{first, static_cast<std::uint32_t>(destinations.size()) - first, 0};
-        for (std::uint32_t d = first; d < destinations.size(); ++d) {
-            L.max_coefficients =
-                std::max(L.max_coefficients, (destinations[d].order + 1) * (destinations[d].order + 1));
-            for (std::uint32_t i = 0; i < destinations[d].link_count; ++i) {
-                const auto &lk = links[destinations[d].first_link + i];
-                L.max_coefficients = std::max(L.max_coefficients, (lk.source_order + 1) * (lk.source_order + 1));
+    // Chunks of at most kFmmChunk launch links sharing key(link), in key order;
+    // chunk.first indexes the launch's own slice of the order array.
+    auto make_chunks = [&](std::uint32_t count, auto key, std::vector<std::uint32_t> &order_out,
+                           std::vector<dev::FmmChunk> &chunks_out) {
+        std::vector<std::uint32_t> idx(count);
+        std::iota(idx.begin(), idx.end(), 0u);
+        std::stable_sort(idx.begin(), idx.end(), [&](std::uint32_t a, std::uint32_t b) {
+            return key(links[launch_link_base + a]) < key(links[launch_link_base + b]);
+        });
+        for (std::uint32_t i = 0; i < idx.size();) {
+            const std::uint32_t kv = key(links[launch_link_base + idx[i]]);
+            dev::FmmChunk ch{kv, i, 0, 0};
+            while (i < idx.size() && ch.count < static_cast<std::uint32_t>(dev::kFmmChunk) &&
+                   key(links[launch_link_base + idx[i]]) == kv) {
+                const auto &lk = links[launch_link_base + idx[i]];
+                ch.max_order = std::max({ch.max_order, lk.source_order, lk.target_order});
+                ++ch.count;
+                ++i;
             }
+            chunks_out.push_back(ch);
         }
+        order_out.insert(order_out.end(), idx.begin(), idx.end());
+    };
+    auto end_launch = [&](std::vector<FmmOperator::Launch> &list, std::uint32_t first) {
+        FmmOperator::Launch L{first, static_cast<std::uint32_t>(destinations.size()) - first, launch_link_base,
+                              static_cast<std::uint32_t>(links.size()) - launch_link_base,
+                              static_cast<std::uint32_t>(chunks.size()), 0,
+                              static_cast<std::uint32_t>(axial_chunks.size()), 0};
+        make_chunks(L.link_count, [](const dev::FmmLink &lk) { return lk.rotation; }, link_order, chunks);
+        make_chunks(L.link_count, [](const dev::FmmLink &lk) { return lk.axial; }, axial_order, axial_chunks);
+        L.chunk_count = static_cast<std::uint32_t>(chunks.size()) - L.first_chunk;
+        L.axial_chunk_count = static_cast<std::uint32_t>(axial_chunks.size()) - L.first_axial_chunk;
+        workspace = std::max(workspace, slot);
+        slot = 0;
+        launch_link_base = static_cast<std::uint32_t>(links.size());
         if (L.destination_count)
             list.push_back(L);
     };
@@ -365,6 +402,12 @@ std::unique_ptr<FmmOperator> MeshSolver::fmm_operator(const Numerics &n, double 
     hip_check(hipMemsetAsync(op->pool_.data(), 0, op->pool_.bytes(), s), "clear expansions");
     upload_nonempty(op->destinations_, destinations, s);
     upload_nonempty(op->links_, links, s);
+    upload_nonempty(op->chunks_, chunks, s);
+    upload_nonempty(op->link_order_, link_order, s);
+    upload_nonempty(op->axial_chunks_, axial_chunks, s);
+    upload_nonempty(op->axial_order_, axial_order, s);
+    op->work_y_.resize(2 * std::max<std::uint64_t>(workspace, 1));
+    op->work_z_.resize(2 * std::max<std::uint64_t>(workspace, 1));
     upload_nonempty(op->rotations_, rot_table, s);
     upload_nonempty(op->small_d_, small_d, s);
     upload_nonempty(op->azimuth_, azimuth, s);
@@ -423,7 +466,8 @@ std::unique_ptr<FmmOperator> MeshSolver::fmm_operator(const Numerics &n, double 
         throw std::runtime_error("FMM leaf moments left the bounded radial-table domain.");
     st.device_bytes = op->near_.values.bytes() + op->near_.columns.bytes() + op->emission_.bytes() +
                       op->testing_.bytes() + op->pool_.bytes() + op->small_d_.bytes() + op->azimuth_.bytes() +
-                      op->axial_.bytes() + op->jacobi_.bytes() + op->links_.bytes() + op->destinations_.bytes();
+                      op->axial_.bytes() + op->jacobi_.bytes() + op->links_.bytes() + op->destinations_.bytes() +
+                      op->work_y_.bytes() + op->work_z_.bytes();
     return op;
 }
 
@@ -431,8 +475,9 @@ void FmmOperator::apply(const double *x, double *y, hipStream_t s) const {
     dev::launch_csr_apply(near_.row_offsets.data(), near_.columns.data(), near_.values.data(), n_, x, y, s);
     dev::launch_leaf_emit(leaves_.data(), leaf_count_, leaf_dofs_.data(), emission_.data(), x, pool_.data(), s);
     dev::FmmTranslateLaunch T{};
-    T.links = links_.data();
     T.rotations = rotations_.data();
+    T.work_y = work_y_.data();
+    T.work_z = work_z_.data();
     T.small_d = small_d_.data();
     T.azimuth = azimuth_.data();
     T.axial = axial_.data();
@@ -442,8 +487,12 @@ void FmmOperator::apply(const double *x, double *y, hipStream_t s) const {
     for (const auto *list : {&upward_, &downward_})
         for (const Launch &L : *list) {
             T.destinations = destinations_.data() + L.first_destination;
-            T.max_coefficients = L.max_coefficients;
-            dev::launch_translate(T, L.destination_count, s);
+            T.links = links_.data() + L.first_link;
+            T.order = link_order_.data() + L.first_link;
+            T.chunks = chunks_.data() + L.first_chunk;
+            T.axial_order = axial_order_.data() + L.first_link;
+            T.axial_chunks = axial_chunks_.data() + L.first_axial_chunk;
+            dev::launch_translate(T, L.destination_count, L.link_count, L.chunk_count, L.axial_chunk_count, s);
         }
     dev::launch_leaf_test(leaves_.data(), leaf_count_, leaf_dofs_.data(), testing_.data(), pool_.data(), y, s);
 }

@@ -34,7 +34,7 @@ struct FmmStatistics {
     std::uint64_t near_nnz{}, far_pairs{}, m2m_links{}, m2l_links{}, l2l_links{}, rotations{}, axial_plans{};
     std::uint64_t expansion_values{}, moment_values{}, device_bytes{};
     std::vector<std::uint32_t> level_orders;
-    double tree_ms{}, near_ms{}, plans_ms{}, moments_ms{};
+    double tree_ms{}, near_ms{}, plans_ms{}, moments_ms{}, near_kernel_ms{};
 };
 
 struct DeviceNear {
@@ -56,7 +56,8 @@ class FmmOperator {
   private:
     friend class MeshSolver;
     struct Launch {
-        std::uint32_t first_destination{}, destination_count{}, max_coefficients{};
+        std::uint32_t first_destination{}, destination_count{}, first_link{}, link_count{}, first_chunk{},
+            chunk_count{}, first_axial_chunk{}, axial_chunk_count{};
     };
     std::uint32_t n_{};
     FmmStatistics stats_;
@@ -66,6 +67,10 @@ class FmmOperator {
     DeviceBuffer<double> emission_, testing_, pool_;
     DeviceBuffer<dev::FmmDestination> destinations_;
     DeviceBuffer<dev::FmmLink> links_;
+    DeviceBuffer<dev::FmmChunk> chunks_;
+    DeviceBuffer<std::uint32_t> link_order_, axial_order_;
+    DeviceBuffer<dev::FmmChunk> axial_chunks_;
+    mutable DeviceBuffer<double> work_y_, work_z_;
     DeviceBuffer<dev::FmmRotation> rotations_;
     DeviceBuffer<double> small_d_, azimuth_, axial_;
     DeviceBuffer<std::uint64_t> axial_offsets_;

@@ -10,8 +10,10 @@
 //                    (-R_lm - alpha n . grad R_lm) phi_i / m_i of the leaf's P1 DOFs;
 //   * emission       multipole of a leaf = E_leaf p (one block per leaf);
 //   * translation    every M2M / M2L / L2L link is rotate-to-axis, coaxial m-block
-//                    translation, rotate-back (O(p^3)); one block per destination
-//                    expansion accumulates its links in LDS (no atomics);
+//                    translation, rotate-back (O(p^3)): the rotations run per chunk
+//                    of links sharing a direction with the Wigner blocks staged in
+//                    LDS, the coaxial products per link, and one block per
+//                    destination sums its links (no atomics);
 //   * testing        y_i += T_leaf L_leaf (one wavefront per DOF).
 #include "fmm_kernels.hpp"
 
@@ -284,75 +286,137 @@ __device__ __forceinline__ std::size_t degree_block(int l) {
     return static_cast<std::size_t>(l) * (2 * l - 1) * (2 * l + 1) / 3;
 }
 
-// One block per destination expansion: for each link, rotate the source to the
-// link axis, translate along it, rotate back and accumulate.
-__global__ __launch_bounds__(256) void translate(FmmTranslateLaunch L) {
-    extern __shared__ double lds[];
-    const FmmDestination dst = L.destinations[blockIdx.x];
-    const int pt = static_cast<int>(dst.order);
-    const int nct = (pt + 1) * (pt + 1);
-    C2 *acc = reinterpret_cast<C2 *>(lds);
-    C2 *b1 = acc + L.max_coefficients, *b2 = b1 + L.max_coefficients;
-    for (int e = threadIdx.x; e < nct; e += blockDim.x)
-        acc[e] = {0, 0};
-    const C2 *ex = reinterpret_cast<const C2 *>(L.source_expansions);
-    const C2 *az = reinterpret_cast<const C2 *>(L.azimuth);
-    const C2 *ax = reinterpret_cast<const C2 *>(L.axial);
-    for (std::uint32_t li = dst.first_link; li < dst.first_link + dst.link_count; ++li) {
-        const FmmLink link = L.links[li];
-        const int ps = static_cast<int>(link.source_order);
-        const int ncs = (ps + 1) * (ps + 1);
-        const FmmRotation rot = L.rotations[link.rotation];
-        const double *d = L.small_d + rot.small_d;
-        const C2 *phase = az + rot.azimuth + rot.order; // phase[m] = exp(i m phi)
-        const C2 *T = ax + L.axial_offsets[link.axial];
+// Translations run as four passes over the links of one level:
+//   rotate    per chunk of links sharing a direction: the Wigner blocks d_l are
+//             staged in LDS degree by degree and applied to every source of the
+//             chunk (Y = d^T e^{i m phi} X);
+//   coaxial   one wavefront per link: per-|m| block product Z = T Y;
+//   back      per chunk again: W = e^{-i m phi} d Z;
+//   gather    one block per destination: the sum of its links' W.
+// Y and W share one workspace slot per link, Z uses a second.
+constexpr int kStage = (2 * kMaxFmmOrder + 1) * (2 * kMaxFmmOrder + 1);
+
+// Degree by degree: d_l and the chunk's degree-l slices staged in LDS.
+__global__ __launch_bounds__(256) void rotate_links(FmmTranslateLaunch L, bool back) {
+    __shared__ double dl[kStage];
+    __shared__ C2 xs[kFmmChunk][2 * kMaxFmmOrder + 1];
+    __shared__ C2 ph[2 * kMaxFmmOrder + 1];
+    __shared__ std::uint64_t src_off[kFmmChunk], dst_off[kFmmChunk];
+    __shared__ int ord[kFmmChunk];
+    const FmmChunk ch = L.chunks[blockIdx.x];
+    const FmmRotation rot = L.rotations[ch.rotation];
+    const C2 *az = reinterpret_cast<const C2 *>(L.azimuth) + rot.azimuth;
+    for (int m = threadIdx.x; m <= 2 * static_cast<int>(rot.order); m += blockDim.x)
+        ph[m] = az[m];
+    if (threadIdx.x < ch.count) {
+        const FmmLink lk = L.links[L.order[ch.first + threadIdx.x]];
+        src_off[threadIdx.x] = back ? lk.slot : lk.source;
+        dst_off[threadIdx.x] = lk.slot;
+        ord[threadIdx.x] = static_cast<int>(back ? lk.target_order : lk.source_order);
+    }
+    const C2 *in = reinterpret_cast<const C2 *>(back ? L.work_z : L.source_expansions);
+    C2 *out = reinterpret_cast<C2 *>(L.work_y);
+    const int count = static_cast<int>(ch.count), P = static_cast<int>(rot.order);
+    for (int l = 0; l <= static_cast<int>(ch.max_order); ++l) {
+        const int n = 2 * l + 1;
+        const std::size_t base = static_cast<std::size_t>(l) * l;
         __syncthreads();
-        for (int e = threadIdx.x; e < ncs; e += blockDim.x)
-            b1[e] = ex[link.source + e];
-        __syncthreads();
-        // rotate to the axis: b2[l,t] = sum_s d_l[s][t] e^{i(s-l)phi} b1[l,s]
-        for (int e = threadIdx.x; e < ncs; e += blockDim.x) {
-            int l, t;
-            split_index(e, l, t);
-            const int n = 2 * l + 1;
-            const double *dl = d + degree_block(l);
-            C2 s{0, 0};
-            for (int q = 0; q < n; ++q)
-                s = s + dl[q * n + t] * (phase[q - l] * b1[l * l + q]);
-            b2[e] = s;
-        }
-        __syncthreads();
-        // coaxial translation, one block per |m| shared by +-m
-        const int mm = min(ps, pt);
-        for (int e = threadIdx.x; e < nct; e += blockDim.x) {
-            int l, t;
-            split_index(e, l, t);
-            const int m = t - l, am = abs(m);
-            C2 s{0, 0};
-            if (am <= mm) {
-                const C2 *blk = T + axial_block(am, ps, pt) + static_cast<std::size_t>(l - am) * (ps - am + 1);
-                for (int n = am; n <= ps; ++n)
-                    s = s + blk[n - am] * b2[n * n + n + m];
+        const double *src = L.small_d + rot.small_d + static_cast<std::size_t>(l) * (2 * l - 1) * (2 * l + 1) / 3;
+        for (int e = threadIdx.x; e < n * n; e += blockDim.x)
+            dl[e] = src[e];
+        for (int e = threadIdx.x; e < count * n; e += blockDim.x) {
+            const int j = e / n, q = e % n;
+            C2 v{0, 0};
+            if (l <= ord[j]) {
+                v = in[src_off[j] + base + q];
+                if (!back)
+                    v = ph[P + q - l] * v;
             }
-            b1[e] = s;
+            xs[j][q] = v;
         }
         __syncthreads();
-        // rotate back: acc[l,t] += e^{-i(t-l)phi} sum_s d_l[t][s] b1[l,s]
-        for (int e = threadIdx.x; e < nct; e += blockDim.x) {
-            int l, t;
-            split_index(e, l, t);
-            const int n = 2 * l + 1;
-            const double *dl = d + degree_block(l) + static_cast<std::size_t>(t) * n;
+        for (int e = threadIdx.x; e < count * n; e += blockDim.x) {
+            const int j = e / n, t = e % n;
+            if (l > ord[j])
+                continue;
             C2 s{0, 0};
-            for (int q = 0; q < n; ++q)
-                s = s + dl[q] * b1[l * l + q];
-            acc[e] = acc[e] + conj(phase[t - l]) * s;
+            if (!back) {
+                for (int q = 0; q < n; ++q)
+                    s = s + dl[q * n + t] * xs[j][q];
+                out[dst_off[j] + base + t] = s;
+            } else {
+                for (int q = 0; q < n; ++q)
+                    s = s + dl[t * n + q] * xs[j][q];
+                out[dst_off[j] + base + t] = conj(ph[P + t - l]) * s;
+            }
         }
     }
+}
+
+// Z = T Y per |m| block for a chunk of links sharing one coaxial plan: the
+// block T_m and the chunk's order-m columns of Y are staged in LDS.
+constexpr int kAxialRows = kMaxFmmOrder + 1;
+
+__global__ __launch_bounds__(256) void coaxial_links(FmmTranslateLaunch L) {
+    __shared__ C2 tm[kAxialRows * kAxialRows];
+    __shared__ C2 ys[kFmmChunk][2][kAxialRows];
+    __shared__ std::uint64_t slot[kFmmChunk];
+    const FmmChunk ch = L.axial_chunks[blockIdx.x];
+    const FmmLink first = L.links[L.axial_order[ch.first]];
+    const int ps = static_cast<int>(first.source_order), pt = static_cast<int>(first.target_order), mm = min(ps, pt);
+    const C2 *T = reinterpret_cast<const C2 *>(L.axial) + L.axial_offsets[first.axial];
+    const C2 *y = reinterpret_cast<const C2 *>(L.work_y);
+    C2 *z = reinterpret_cast<C2 *>(L.work_z);
+    const int count = static_cast<int>(ch.count);
+    if (threadIdx.x < count)
+        slot[threadIdx.x] = L.links[L.axial_order[ch.first + threadIdx.x]].slot;
+    // degrees above min(ps, pt) in |m| have no source: those outputs are zero
+    const int nct = (pt + 1) * (pt + 1);
     __syncthreads();
+    for (int e = threadIdx.x; e < count * nct; e += blockDim.x) {
+        const int j = e / nct, c = e % nct;
+        int l, t;
+        split_index(c, l, t);
+        if (abs(t - l) > mm)
+            z[slot[j] + c] = {0, 0};
+    }
+    std::size_t off = 0;
+    for (int m = 0; m <= mm; ++m) {
+        const int rows = pt - m + 1, cols = ps - m + 1;
+        __syncthreads();
+        for (int e = threadIdx.x; e < rows * cols; e += blockDim.x)
+            tm[e] = T[off + e];
+        for (int e = threadIdx.x; e < count * 2 * cols; e += blockDim.x) {
+            const int j = e / (2 * cols), r = e % (2 * cols), sgn = r / cols, n = m + r % cols;
+            ys[j][sgn][n - m] = y[slot[j] + static_cast<std::size_t>(n) * n + n + (sgn ? -m : m)];
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < count * 2 * rows; e += blockDim.x) {
+            const int j = e / (2 * rows), r = e % (2 * rows), sgn = r / rows, l = m + r % rows;
+            if (sgn && !m)
+                continue;
+            C2 s{0, 0};
+            const C2 *trow = tm + (l - m) * cols;
+            for (int c = 0; c < cols; ++c)
+                s = s + trow[c] * ys[j][sgn][c];
+            z[slot[j] + static_cast<std::size_t>(l) * l + l + (sgn ? -m : m)] = s;
+        }
+        off += static_cast<std::size_t>(rows) * cols;
+    }
+}
+
+// target = sum of the destination's link results.
+__global__ __launch_bounds__(256) void gather_links(FmmTranslateLaunch L) {
+    const FmmDestination dst = L.destinations[blockIdx.x];
+    const int nct = static_cast<int>((dst.order + 1) * (dst.order + 1));
+    const C2 *w = reinterpret_cast<const C2 *>(L.work_y);
     C2 *out = reinterpret_cast<C2 *>(L.target_expansions) + dst.target;
-    for (int e = threadIdx.x; e < nct; e += blockDim.x)
-        out[e] = acc[e];
+    for (int e = threadIdx.x; e < nct; e += blockDim.x) {
+        C2 s{0, 0};
+        for (std::uint32_t i = 0; i < dst.link_count; ++i)
+            s = s + w[L.links[dst.first_link + i].slot + e];
+        out[e] = s;
+    }
 }
 
 // Block-Jacobi blocks of the near operator: block b holds rows and columns
@@ -397,11 +461,16 @@ void launch_leaf_test(const FmmLeaf *leaves, std::uint32_t leaf_count, const std
         hipLaunchKernelGGL(leaf_test, dim3(leaf_count), dim3(256), 0, stream, leaves, leaf_dofs, testing, expansions, y);
 }
 
-void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destination_count, hipStream_t stream) {
+void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destination_count, std::uint32_t link_count,
+                      std::uint32_t chunk_count, std::uint32_t axial_chunk_count, hipStream_t stream) {
     if (!destination_count)
         return;
-    const std::size_t lds = 3 * sizeof(double) * 2 * launch.max_coefficients;
-    hipLaunchKernelGGL(translate, dim3(destination_count), dim3(256), lds, stream, launch);
+    if (link_count) {
+        hipLaunchKernelGGL(rotate_links, dim3(chunk_count), dim3(256), 0, stream, launch, false);
+        hipLaunchKernelGGL(coaxial_links, dim3(axial_chunk_count), dim3(256), 0, stream, launch);
+        hipLaunchKernelGGL(rotate_links, dim3(chunk_count), dim3(256), 0, stream, launch, true);
+    }
+    hipLaunchKernelGGL(gather_links, dim3(destination_count), dim3(256), 0, stream, launch);
 }
 
 void launch_gather_blocks(const std::uint32_t *row_offsets, const std::uint32_t *columns, const double *values,

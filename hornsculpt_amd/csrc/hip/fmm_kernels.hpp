@@ -11,7 +11,8 @@ namespace hsbem::dev {
 
 inline constexpr int kMaxLeafOrder = 64;   // expansion order at the leaves
 inline constexpr int kMaxLeafDofs = 512;   // P1 DOFs of one leaf
-inline constexpr int kMaxFmmOrder = 35;    // any translated expansion (LDS: 3 x (p+1)^2 complex <= 64 KiB)
+inline constexpr int kMaxFmmOrder = 40;    // any translated expansion (LDS stage of d_l: (2p+1)^2 doubles)
+inline constexpr int kFmmChunk = 16;       // links per rotation chunk
 inline constexpr int kFmmJacobi = 64;      // block width of the near block-Jacobi blocks
 
 // One leaf of the support tree with P1 DOFs. Offsets count complex values.
@@ -55,11 +56,17 @@ struct FmmRotation {
     std::uint32_t order, pad;
 };
 
-// One translation into a destination: source expansion (pool offset), its
-// order, the rotation of the link axis and the coaxial plan.
+// One translation into a destination: source expansion (pool offset), the
+// orders, the rotation of the link axis, the coaxial plan and the link's
+// workspace slot (complex offset, room for the larger of both orders).
 struct FmmLink {
-    std::uint64_t source;
-    std::uint32_t source_order, rotation, axial, pad;
+    std::uint64_t source, slot;
+    std::uint32_t source_order, target_order, rotation, axial;
+};
+
+// Consecutive links (in the direction-sorted order) sharing one rotation.
+struct FmmChunk {
+    std::uint32_t rotation, first, count, max_order;
 };
 
 struct FmmDestination {
@@ -69,8 +76,13 @@ struct FmmDestination {
 };
 
 struct FmmTranslateLaunch {
-    const FmmDestination *destinations;
-    const FmmLink *links;
+    const FmmDestination *destinations; // of this launch
+    const FmmLink *links;               // of this launch (destination order)
+    const std::uint32_t *order;         // link indices sorted by rotation
+    const FmmChunk *chunks;             // rotation chunks
+    const std::uint32_t *axial_order;   // link indices sorted by coaxial plan
+    const FmmChunk *axial_chunks;       // chunks sharing a coaxial plan (rotation field = plan)
+    double *work_y, *work_z;            // link workspaces (complex)
     const FmmRotation *rotations;
     const double *small_d;
     const double *azimuth;          // complex
@@ -78,7 +90,6 @@ struct FmmTranslateLaunch {
     const std::uint64_t *axial_offsets; // complex offset of each plan
     const double *source_expansions;
     double *target_expansions;
-    std::uint32_t max_coefficients; // (p+1)^2 of the largest order in the launch
 };
 
 void launch_leaf_moments(const FmmLeafMomentsLaunch &launch, hipStream_t stream);
@@ -86,7 +97,8 @@ void launch_leaf_emit(const FmmLeaf *leaves, std::uint32_t leaf_count, const std
                       const double *emission, const double *p, double *expansions, hipStream_t stream);
 void launch_leaf_test(const FmmLeaf *leaves, std::uint32_t leaf_count, const std::uint32_t *leaf_dofs,
                       const double *testing, const double *expansions, double *y, hipStream_t stream);
-void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destination_count, hipStream_t stream);
+void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destination_count, std::uint32_t link_count,
+                      std::uint32_t chunk_count, std::uint32_t axial_chunk_count, hipStream_t stream);
 // blocks (zeroed, n rounded up to kFmmJacobi) from the near CSR in tree order;
 // position[dof] = index of dof in order.
 void launch_gather_blocks(const std::uint32_t *row_offsets, const std::uint32_t *columns, const double *values,

@@ -318,7 +318,8 @@ HSBEM_API int hsbem_fmm_operator_solve(hsbem_fmm_operator *op, const double *b, 
 HSBEM_API int hsbem_fmm_operator_time_apply(hsbem_fmm_operator *op, uint32_t repetitions, double *milliseconds);
 /* counts[16] = nodes, levels, leaves, top level, rule points, near nnz, far pairs,
  * M2M links, M2L links, L2L links, rotations, coaxial plans, expansion values,
- * moment values, device bytes, nv; times[4] = tree, near, plans, moments (ms);
+ * moment values, device bytes, nv; times[5] = tree, near, plans, moments, near
+ * kernels (ms);
  * orders[level_count] = expansion order per level (0: inactive). */
 HSBEM_API int hsbem_fmm_operator_statistics(const hsbem_fmm_operator *op, uint64_t *counts, double *times,
                                             uint32_t *orders, uint32_t *level_count);
