@@ -28,6 +28,7 @@ SOLVER_LU, SOLVER_GMRES = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 PRECONDITIONER_NONE, PRECONDITIONER_BLOCK_JACOBI = 0, 1
 WALL, DRIVEN = 0, 1
+ACCELERATION_DENSE, ACCELERATION_FAST_MULTIPOLE = 0, 1
 
 
 class _FmmSettings(ctypes.Structure):
@@ -50,6 +51,11 @@ class _Numerics(ctypes.Structure):
         ("precision", ctypes.c_uint32),
         ("plan_threads", ctypes.c_uint32),
         ("preconditioner", ctypes.c_uint32),
+        ("acceleration", ctypes.c_uint32),
+        ("fmm_leaf_capacity", ctypes.c_uint32),
+        ("fmm_separation_ratio", ctypes.c_double),
+        ("fmm_maximum_iterations", ctypes.c_uint32),
+        ("fmm_restart_length", ctypes.c_uint32),
     ]
 
 
@@ -253,6 +259,11 @@ class BemNumerics:
     precision: int = PRECISION_FP64
     plan_threads: int = 0
     preconditioner: int = PRECONDITIONER_BLOCK_JACOBI   # GMRES only
+    acceleration: int = ACCELERATION_DENSE
+    fmm_leaf_capacity: int = 16          # FastMultipoleSettings (bem.hpp:60-71)
+    fmm_separation_ratio: float = 1.5
+    fmm_maximum_iterations: int = 300
+    fmm_restart_length: int = 40
 
     def native(self) -> _Numerics:
         n = _Numerics()

@@ -92,6 +92,11 @@ hsbem::Numerics to_numerics(const hsbem_numerics &n) {
     o.precision = n.precision;
     o.plan_threads = n.plan_threads;
     o.preconditioner = n.preconditioner;
+    o.acceleration = n.acceleration;
+    o.fmm_leaf_capacity = n.fmm_leaf_capacity;
+    o.fmm_separation = n.fmm_separation_ratio;
+    o.fmm_maximum_iterations = n.fmm_maximum_iterations;
+    o.fmm_restart = n.fmm_restart_length;
     return o;
 }
 
@@ -230,6 +235,11 @@ void hsbem_numerics_default(hsbem_numerics *n) {
     n->precision = HSBEM_PRECISION_FP64;
     n->plan_threads = 0;
     n->preconditioner = HSBEM_PRECONDITIONER_BLOCK_JACOBI;
+    n->acceleration = HSBEM_ACCELERATION_DENSE;
+    n->fmm_leaf_capacity = 16;
+    n->fmm_separation_ratio = 1.5;
+    n->fmm_maximum_iterations = 300;
+    n->fmm_restart_length = 40;
 }
 
 const char *hsbem_last_error(void) { return last_error.c_str(); }

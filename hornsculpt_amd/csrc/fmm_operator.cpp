@@ -14,6 +14,9 @@
 #include <numeric>
 #include <stdexcept>
 #include <thread>
+#include <sstream>
+#include <numbers>
+#include <complex>
 
 namespace hsbem {
 namespace {
@@ -63,19 +66,32 @@ template <class T> void upload_nonempty(DeviceBuffer<T> &d, const std::vector<T>
 
 std::unique_ptr<FmmOperator> MeshSolver::fmm_operator(const Numerics &n, double k_max, double k, const FmmSettings &fs,
                                                       const std::vector<std::uint32_t> &faces) {
+    std::lock_guard lock(mutex_);
+    return build_fmm(n, k_max, k, fs, faces, ctx_->stream, nullptr);
+}
+
+std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_max, double k, const FmmSettings &fs,
+                                                   const std::vector<std::uint32_t> &faces, hipStream_t s,
+                                                   const FmmOperator *share) {
     if (!std::isfinite(k) || k <= 0 || !std::isfinite(fs.accuracy_seed) || fs.accuracy_seed <= 0 ||
         fs.accuracy_seed >= 1)
         throw std::invalid_argument("FMM operator needs a positive wavenumber and an accuracy seed in (0, 1).");
-    std::lock_guard lock(mutex_);
-    hipStream_t s = ctx_->stream;
     auto op = std::unique_ptr<FmmOperator>(new FmmOperator());
     FmmStatistics &st = op->stats_;
     const std::uint32_t nv = this->nv(), nt = this->nt();
     op->n_ = nv;
 
     auto t0 = Clock::now();
-    const SupportTree tree = make_support_tree(space_, fs.tree);
-    const SupportNearPlan plan = make_support_near_plan(space_, tree, faces);
+    std::shared_ptr<const NearStructure> ns;
+    if (share) {
+        op->near_ = share->near_;
+        op->jacobi_ = share->jacobi_;
+        ns = op->near_->structure;
+    } else {
+        ns = near_structure(n, k_max, fs.tree, faces, s);
+    }
+    const SupportTree &tree = ns->tree;
+    const SupportNearPlan &plan = ns->plan;
     const SupportExpansionPlan ep = make_support_expansion_plan(tree, plan.panel_face_column, k, fs.accuracy_seed);
     st.tree_ms = ms_since(t0);
     st.nodes = static_cast<std::uint32_t>(tree.nodes.size());
@@ -83,10 +99,14 @@ std::unique_ptr<FmmOperator> MeshSolver::fmm_operator(const Numerics &n, double 
     st.far_pairs = tree.pressure_pressure.far.size();
 
     t0 = Clock::now();
-    build_near(n, k_max, k, tree, plan, s, op->near_);
+    if (!share) {
+        auto dn = std::make_shared<DeviceNear>();
+        integrate_near(n, k, ns, s, *dn);
+        op->near_ = dn;
+    }
     st.near_ms = ms_since(t0);
-    st.near_nnz = op->near_.nnz;
-    st.near_kernel_ms = op->near_.kernel_ms;
+    st.near_nnz = op->near_->nnz;
+    st.near_kernel_ms = op->near_->kernel_ms;
 
     // ---- levels: LF order and basis radius of every active level
     const std::uint32_t nl = st.levels;
@@ -440,18 +460,22 @@ std::unique_ptr<FmmOperator> MeshSolver::fmm_operator(const Numerics &n, double 
     hip_check(hipGetLastError(), "leaf moments launch");
 
     // near block-Jacobi in tree order
-    const std::uint32_t nb = (nv + dev::kFmmJacobi - 1) / dev::kFmmJacobi;
-    std::vector<std::uint32_t> position(nv);
-    for (std::uint32_t i = 0; i < nv; ++i)
-        position[tree.pressure_order[i]] = i;
     DeviceBuffer<std::uint32_t> d_pos;
-    d_pos.upload(position, s);
-    DeviceBuffer<double> blocks(2 * static_cast<std::size_t>(nb) * dev::kFmmJacobi * dev::kFmmJacobi);
-    hip_check(hipMemsetAsync(blocks.data(), 0, blocks.bytes(), s), "clear blocks");
-    dev::launch_gather_blocks(op->near_.row_offsets.data(), op->near_.columns.data(), op->near_.values.data(),
-                              op->order_.data(), d_pos.data(), nv, blocks.data(), s);
-    op->jacobi_.resize(dev::jacobi_factor_doubles(nv));
-    dev::jacobi_factor_gathered(blocks.data(), nv, op->jacobi_.data(), s);
+    DeviceBuffer<double> blocks;
+    if (!share) {
+        const std::uint32_t nb = (nv + dev::kFmmJacobi - 1) / dev::kFmmJacobi;
+        std::vector<std::uint32_t> position(nv);
+        for (std::uint32_t i = 0; i < nv; ++i)
+            position[tree.pressure_order[i]] = i;
+        d_pos.upload(position, s);
+        blocks.resize(2 * static_cast<std::size_t>(nb) * dev::kFmmJacobi * dev::kFmmJacobi);
+        hip_check(hipMemsetAsync(blocks.data(), 0, blocks.bytes(), s), "clear blocks");
+        dev::launch_gather_blocks(ns->row_offsets.data(), ns->columns.data(), op->near_->values.data(),
+                                  op->order_.data(), d_pos.data(), nv, blocks.data(), s);
+        auto jac = std::make_shared<DeviceBuffer<double>>(dev::jacobi_factor_doubles(nv));
+        dev::jacobi_factor_gathered(blocks.data(), nv, jac->data(), s);
+        op->jacobi_ = jac;
+    }
     hip_check(hipEventRecord(e1, s), "event");
     hip_check(hipGetLastError(), "jacobi launch");
     int code = 0;
@@ -464,15 +488,16 @@ std::unique_ptr<FmmOperator> MeshSolver::fmm_operator(const Numerics &n, double 
     st.moments_ms = ms;
     if (code)
         throw std::runtime_error("FMM leaf moments left the bounded radial-table domain.");
-    st.device_bytes = op->near_.values.bytes() + op->near_.columns.bytes() + op->emission_.bytes() +
+    st.device_bytes = op->near_->values.bytes() + ns->columns.bytes() + op->emission_.bytes() +
                       op->testing_.bytes() + op->pool_.bytes() + op->small_d_.bytes() + op->azimuth_.bytes() +
-                      op->axial_.bytes() + op->jacobi_.bytes() + op->links_.bytes() + op->destinations_.bytes() +
+                      op->axial_.bytes() + op->jacobi_->bytes() + op->links_.bytes() + op->destinations_.bytes() +
                       op->work_y_.bytes() + op->work_z_.bytes();
     return op;
 }
 
 void FmmOperator::apply(const double *x, double *y, hipStream_t s) const {
-    dev::launch_csr_apply(near_.row_offsets.data(), near_.columns.data(), near_.values.data(), n_, x, y, s);
+    dev::launch_csr_apply(near_->structure->row_offsets.data(), near_->structure->columns.data(), near_->values.data(),
+                          n_, x, y, s);
     dev::launch_leaf_emit(leaves_.data(), leaf_count_, leaf_dofs_.data(), emission_.data(), x, pool_.data(), s);
     dev::FmmTranslateLaunch T{};
     T.rotations = rotations_.data();
@@ -498,7 +523,213 @@ void FmmOperator::apply(const double *x, double *y, hipStream_t s) const {
 }
 
 void FmmOperator::precondition(const double *v, double *z, hipStream_t s) const {
-    dev::jacobi_apply(jacobi_.data(), n_, v, z, s, order_.data());
+    dev::jacobi_apply(jacobi_->data(), n_, v, z, s, order_.data());
+}
+
+namespace {
+
+double weighted_norm(const double *v, std::uint32_t n, const double *weights, Workspace &ws, hipStream_t s) {
+    double *out = ws.scratch.data() + ws.scratch.size() - 8;
+    dev::launch_norm2(v, n, weights, out, ws.scratch.data(), s);
+    double value = 0;
+    hip_check(hipMemcpyAsync(&value, out, sizeof(double), hipMemcpyDeviceToHost, s), "norm download");
+    hip_check(hipStreamSynchronize(s), "norm sync");
+    return std::sqrt(value);
+}
+
+} // namespace
+
+void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, double omega,
+                           std::vector<double> &relative, std::vector<std::uint32_t> &iterations, FrequencyResult &fr,
+                           const volatile std::int32_t *cancel) {
+    using C = std::complex<double>;
+    const Request &req = P.request;
+    const Numerics &num = req.numerics;
+    hipStream_t s = ws.stream;
+    const std::uint32_t nv = this->nv(), nt = this->nt(), ne = P.ne;
+    const double tol = num.residual_tolerance;
+    // active faces: every face with a nonzero velocity in some excitation, ascending
+    std::vector<std::uint32_t> faces;
+    for (const auto &fv : P.face_velocity)
+        for (const auto &[face, v] : fv)
+            if (std::abs(v) > 0)
+                faces.push_back(face);
+    std::sort(faces.begin(), faces.end());
+    faces.erase(std::unique(faces.begin(), faces.end()), faces.end());
+    FmmSettings fs;
+    fs.tree.leaf_capacity = num.fmm_leaf_capacity;
+    fs.tree.separation = num.fmm_separation;
+    fs.tree.near_ratio = num.near_ratio;
+    fs.accuracy_seed = 0.1 * tol;
+    const auto t_build = Clock::now();
+    const std::unique_ptr<FmmOperator> op = build_fmm(num, P.k_max, k, fs, faces, s, nullptr);
+    fr.ms[0] = ms_since(t_build);
+    fr.near_kernel_ms = op->statistics().near_kernel_ms;
+    {
+        std::lock_guard lock(plan_mutex_);
+        if (!internal_of_device_.size())
+            internal_of_device_.upload(internal_of_, s);
+    }
+    // right-hand sides b = B q, mesh vertex order: near flux CSR per face column + far pairs
+    const std::size_t nvec = 2 * static_cast<std::size_t>(nv) * ne;
+    ws.fmm_b.resize(nvec);
+    ws.fmm_x.resize(nvec);
+    ws.fmm_r.resize(nvec);
+    hip_check(hipMemsetAsync(ws.fmm_b.data(), 0, ws.fmm_b.bytes(), s), "clear FMM rhs");
+    std::vector<double> qf(2 * std::max<std::size_t>(faces.size(), 1) * ne, 0.0);
+    for (std::uint32_t e = 0; e < ne; ++e)
+        for (std::size_t f = 0; f < faces.size(); ++f) {
+            const auto it = P.face_velocity[e].find(faces[f]);
+            if (it == P.face_velocity[e].end())
+                continue;
+            const C q = C(0, -req.density * omega) * it->second;
+            qf[2 * (e * faces.size() + f)] = q.real();
+            qf[2 * (e * faces.size() + f) + 1] = q.imag();
+        }
+    ws.fmm_q.upload(qf, s);
+    const DeviceNear &dn = op->near();
+    const NearStructure &ns = *dn.structure;
+    dev::launch_flux_csr_apply(ns.flux_offsets.data(), ns.flux_columns.data(), dn.flux_values.data(), nv,
+                               ws.fmm_q.data(), static_cast<std::uint32_t>(faces.size()), ne, ws.fmm_b.data(), s);
+    {
+        const SupportTree &tree = ns.tree;
+        std::vector<std::uint32_t> flux_panels;
+        for (std::uint32_t t = 0; t < nt; ++t)
+            if (ns.plan.panel_face_column[t] != UINT32_MAX)
+                flux_panels.push_back(t);
+        std::vector<std::uint32_t> near_offsets(tree.nodes.size() + 1, 0), near_sources;
+        for (const SupportPair &pr : tree.pressure_flux.near)
+            ++near_offsets[pr.target + 1];
+        for (std::size_t i = 0; i < tree.nodes.size(); ++i)
+            near_offsets[i + 1] += near_offsets[i];
+        near_sources.resize(near_offsets.back());
+        std::vector<std::uint32_t> fill(near_offsets.begin(), near_offsets.end() - 1);
+        for (const SupportPair &pr : tree.pressure_flux.near)
+            near_sources[fill[pr.target]++] = pr.source;
+        double max_kh = 0;
+        for (const Panel &p : space_.panels)
+            max_kh = std::max(max_kh, k * p.max_edge);
+        std::vector<double> rule;
+        for (const auto &q : triangle_rule(minimum_endpoint_degree(max_kh, num.quadrature_order)))
+            rule.insert(rule.end(), {q.bary[0], q.bary[1], q.bary[2], q.weight});
+        DeviceBuffer<std::uint32_t> d_panels, d_pl, d_fl, d_no, d_ns;
+        DeviceBuffer<double> d_rule;
+        d_panels.upload(flux_panels.empty() ? std::vector<std::uint32_t>{0} : flux_panels, s);
+        d_pl.upload(tree.pressure_leaf, s);
+        d_fl.upload(tree.flux_leaf, s);
+        d_no.upload(near_offsets, s);
+        d_ns.upload(near_sources.empty() ? std::vector<std::uint32_t>{0} : near_sources, s);
+        d_rule.upload(rule, s);
+        dev::FmmFarFluxLaunch F{};
+        F.geo = geo_.data();
+        F.nodes = original_nodes_.data();
+        F.nt = nt;
+        F.nv = nv;
+        F.nt_all = nt;
+        F.flux_panels = d_panels.data();
+        F.flux_count = static_cast<std::uint32_t>(flux_panels.size());
+        F.pressure_leaf = d_pl.data();
+        F.flux_leaf = d_fl.data();
+        F.near_offsets = d_no.data();
+        F.near_sources = d_ns.data();
+        F.rule = d_rule.data();
+        F.rule_points = static_cast<std::uint32_t>(rule.size() / 4);
+        F.k = k;
+        F.row_scale = original_row_scale_.data();
+        F.q = ws.flux.data();
+        F.excitations = ne;
+        F.out = ws.fmm_b.data();
+        dev::launch_far_flux(F, s);
+        hip_check(hipGetLastError(), "far flux launch");
+        hip_check(hipStreamSynchronize(s), "far flux sync");
+    }
+    // GMRES per excitation on the FMM operator, then the FP64 residual of the unscaled equation
+    const auto t_solve = Clock::now();
+    auto krylov = [&](const FmmOperator &A) {
+        KrylovOperator kop;
+        kop.apply = [&A](const double *v, double *w, hipStream_t st) { A.apply(v, w, st); };
+        if (num.preconditioner == 1)
+            kop.precondition = [&A](const double *v, double *z, hipStream_t st) { A.precondition(v, z, st); };
+        return kop;
+    };
+    const KrylovOperator kop = krylov(*op);
+    auto vec = [&](DeviceBuffer<double> &b, std::uint32_t e) { return b.data() + 2 * static_cast<std::size_t>(e) * nv; };
+    std::vector<double> bnorm(ne);
+    for (std::uint32_t e = 0; e < ne; ++e) {
+        bnorm[e] = weighted_norm(vec(ws.fmm_b, e), nv, original_lumped_.data(), ws, s);
+        if (!std::isfinite(bnorm[e]))
+            throw std::runtime_error("Galerkin right-hand side is not finite.");
+        double achieved = 0;
+        gmres(s, kop, nv, vec(ws.fmm_b, e), vec(ws.fmm_x, e), num.fmm_restart, num.fmm_maximum_iterations, 0.1 * tol,
+              &achieved, &iterations[e], cancel, ws.gmres);
+    }
+    auto residuals = [&](const FmmOperator &A) {
+        for (std::uint32_t e = 0; e < ne; ++e) {
+            A.apply(vec(ws.fmm_x, e), vec(ws.fmm_r, e), s);
+            dev::launch_subtract_from(nv, vec(ws.fmm_b, e), vec(ws.fmm_r, e), s);
+            const double rn = weighted_norm(vec(ws.fmm_r, e), nv, original_lumped_.data(), ws, s);
+            relative[e] = bnorm[e] > 0 ? rn / bnorm[e] : (rn == 0 ? 0 : std::numeric_limits<double>::infinity());
+        }
+    };
+    residuals(*op);
+    // seed audit: the same action with a ten times tighter accuracy seed
+    FmmSettings fine = fs;
+    fine.accuracy_seed = 0.1 * fs.accuracy_seed;
+    const std::unique_ptr<FmmOperator> audit = build_fmm(num, P.k_max, k, fine, faces, s, op.get());
+    double discrepancy = 0;
+    DeviceBuffer<double> ax(2 * static_cast<std::size_t>(nv));
+    for (std::uint32_t e = 0; e < ne; ++e) {
+        // (A_seed - A_fine) x = (b - r) - A_fine x
+        audit->apply(vec(ws.fmm_x, e), ax.data(), s);
+        dev::launch_subtract_from(nv, vec(ws.fmm_b, e), ax.data(), s); // b - A_fine x
+        dev::launch_subtract_from(nv, vec(ws.fmm_r, e), ax.data(), s); // r - (b - A_fine x)
+        const double d = weighted_norm(ax.data(), nv, original_lumped_.data(), ws, s);
+        discrepancy = std::max(discrepancy, bnorm[e] > 0 ? d / bnorm[e] : 0.0);
+    }
+    fr.fmm_audit = discrepancy;
+    if (discrepancy > 0.25 * tol) {
+        // defect correction with the finer operator
+        const KrylovOperator fine_op = krylov(*audit);
+        for (std::uint32_t e = 0; e < ne; ++e) {
+            audit->apply(vec(ws.fmm_x, e), vec(ws.fmm_r, e), s);
+            dev::launch_subtract_from(nv, vec(ws.fmm_b, e), vec(ws.fmm_r, e), s);
+            double achieved = 0;
+            std::uint32_t its = 0;
+            gmres(s, fine_op, nv, vec(ws.fmm_r, e), ax.data(), num.fmm_restart, num.fmm_maximum_iterations, 0.1 * tol,
+                  &achieved, &its, cancel, ws.gmres);
+            dev::launch_add(nv, ax.data(), vec(ws.fmm_x, e), s);
+            iterations[e] += its;
+        }
+        residuals(*audit);
+        fr.fmm_corrected = true;
+    }
+    fr.ms[1] = ms_since(t_solve);
+    {
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "event");
+        hip_check(hipEventCreate(&e1), "event");
+        hip_check(hipEventRecord(e0, s), "event");
+        op->apply(ws.fmm_x.data(), ax.data(), s);
+        hip_check(hipEventRecord(e1, s), "event");
+        hip_check(hipEventSynchronize(e1), "event sync");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        fr.fmm_apply_ms = ms;
+    }
+    for (std::uint32_t e = 0; e < ne; ++e)
+        if (!(relative[e] <= tol)) {
+            std::ostringstream msg;
+            msg.precision(9);
+            msg << "Burton-Miller residual " << relative[e] << " exceeds tolerance " << tol << " at "
+                << omega / (2 * std::numbers::pi) << " Hz (fast multipole form).";
+            throw std::runtime_error(msg.str());
+        }
+    // pressures (and right-hand sides) in the internal order of the result path
+    dev::launch_permute(ws.fmm_x.data(), ws.x.data(), internal_of_device_.data(), nv, ne, true, s);
+    dev::launch_permute(ws.fmm_b.data(), ws.rhs.data(), internal_of_device_.data(), nv, ne, true, s);
+    hip_check(hipStreamSynchronize(s), "FMM point sync");
 }
 
 } // namespace hsbem

@@ -37,10 +37,25 @@ struct FmmStatistics {
     double tree_ms{}, near_ms{}, plans_ms{}, moments_ms{}, near_kernel_ms{};
 };
 
+// Frequency-independent part of the near operator: support tree, near plan,
+// the near jobs split by integration route and the CSR pattern, device resident.
+struct NearStructure {
+    SupportTree tree;
+    SupportNearPlan plan;
+    std::shared_ptr<DevicePlan> interaction; // Galerkin plan that classified the jobs
+    DeviceBuffer<std::uint32_t> row_offsets, columns, flux_offsets, flux_columns, face_column;
+    DeviceBuffer<std::uint32_t> regular, regular_masks, touching_masks, fan_masks, patch_masks;
+    DeviceBuffer<TouchingJob> touching;
+    DeviceBuffer<NearJob> fans;
+    DeviceBuffer<dev::NearPatchItem> items;
+    std::size_t nnz{}, fnnz{}, jobs{}, regular_count{}, touching_count{}, patch_count{}, fan_count{};
+};
+
+// Near entries at one wavenumber (values in the structure's CSR pattern).
 struct DeviceNear {
-    DeviceBuffer<std::uint32_t> row_offsets, columns, flux_offsets, flux_columns;
+    std::shared_ptr<const NearStructure> structure;
     DeviceBuffer<double> values, flux_values;
-    std::size_t nnz{}, fnnz{}, jobs{}, regular{}, touching{}, patches{}, fan_jobs{};
+    std::size_t nnz{}, fnnz{};
     float kernel_ms{};
 };
 
@@ -51,6 +66,7 @@ class FmmOperator {
     // z = M^{-1} v (block-Jacobi of the near operator).
     void precondition(const double *v, double *z, hipStream_t stream) const;
     std::uint32_t size() const { return n_; }
+    const DeviceNear &near() const { return *near_; }
     const FmmStatistics &statistics() const { return stats_; }
 
   private:
@@ -61,7 +77,8 @@ class FmmOperator {
     };
     std::uint32_t n_{};
     FmmStatistics stats_;
-    DeviceNear near_;
+    std::shared_ptr<const DeviceNear> near_;
+    std::shared_ptr<const DeviceBuffer<double>> jacobi_;
     DeviceBuffer<dev::FmmLeaf> leaves_;
     DeviceBuffer<std::uint32_t> leaf_dofs_, order_;
     DeviceBuffer<double> emission_, testing_, pool_;
@@ -74,7 +91,6 @@ class FmmOperator {
     DeviceBuffer<dev::FmmRotation> rotations_;
     DeviceBuffer<double> small_d_, azimuth_, axial_;
     DeviceBuffer<std::uint64_t> axial_offsets_;
-    DeviceBuffer<double> jacobi_;
     std::vector<Launch> upward_, downward_;
     std::uint32_t leaf_count_{};
 };

@@ -439,6 +439,113 @@ __global__ void gather_blocks(const std::uint32_t *row_offsets, const std::uint3
     }
 }
 
+// Far part of B q for the FMM path, by direct pair quadrature (the flux panels
+// are the few driven ones): for test panel s and flux panel t, every corner i of
+// s whose pressure leaf is not a near leaf pair of t's flux leaf receives
+//   (1/m_i) q_t sum_a sum_b w_a w_b phi_i(x_a) (-G - alpha dG/dn_x)(x_a, y_b).
+__global__ __launch_bounds__(256) void far_flux(FmmFarFluxLaunch L) {
+    const std::uint64_t pair = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (pair >= static_cast<std::uint64_t>(L.nt) * L.flux_count)
+        return;
+    const std::uint32_t s = static_cast<std::uint32_t>(pair / L.flux_count);
+    const std::uint32_t t = L.flux_panels[pair % L.flux_count];
+    const std::uint32_t fleaf = L.flux_leaf[t];
+    bool far[3];
+    std::uint32_t node[3];
+    bool any = false;
+    for (int c = 0; c < 3; ++c) {
+        node[c] = L.nodes[static_cast<std::size_t>(c) * L.nt + s];
+        const std::uint32_t pl = L.pressure_leaf[node[c]];
+        bool near = false;
+        for (std::uint32_t e = L.near_offsets[pl]; e < L.near_offsets[pl + 1]; ++e)
+            near |= L.near_sources[e] == fleaf;
+        far[c] = !near;
+        any |= far[c];
+    }
+    if (!any)
+        return;
+    const double *g = L.geo;
+    const std::size_t nt = L.nt;
+    double vs[3], e1s[3], e2s[3], ns[3], vt[3], e1t[3], e2t[3];
+    for (int i = 0; i < 3; ++i) {
+        vs[i] = g[(G_V0 + i) * nt + s];
+        e1s[i] = g[(G_E1 + i) * nt + s];
+        e2s[i] = g[(G_E2 + i) * nt + s];
+        ns[i] = g[(G_N + i) * nt + s];
+        vt[i] = g[(G_V0 + i) * nt + t];
+        e1t[i] = g[(G_E1 + i) * nt + t];
+        e2t[i] = g[(G_E2 + i) * nt + t];
+    }
+    const double ws = 2 * g[G_AREA * nt + s], wt = 2 * g[G_AREA * nt + t], k = L.k;
+    const C2 alpha{0, -1 / k};
+    C2 acc[3] = {{0, 0}, {0, 0}, {0, 0}};
+    for (std::uint32_t a = 0; a < L.rule_points; ++a) {
+        const double *ra = L.rule + 4 * a;
+        double x[3];
+        for (int i = 0; i < 3; ++i)
+            x[i] = vs[i] + ra[1] * e1s[i] + ra[2] * e2s[i];
+        C2 kern{0, 0};
+        for (std::uint32_t b = 0; b < L.rule_points; ++b) {
+            const double *rb = L.rule + 4 * b;
+            double d[3], r2 = 0, nd = 0;
+            for (int i = 0; i < 3; ++i) {
+                d[i] = x[i] - (vt[i] + rb[1] * e1t[i] + rb[2] * e2t[i]);
+                r2 += d[i] * d[i];
+                nd += ns[i] * d[i];
+            }
+            const double r = sqrt(r2);
+            double sn, cs;
+            sincos(k * r, &sn, &cs);
+            const C2 G = (1.0 / (4 * 3.141592653589793 * r)) * C2{cs, -sn};
+            const C2 dG = (nd / r) * (C2{-1 / r, -k} * G);
+            kern = kern + (rb[3] * wt) * (C2{-G.re, -G.im} - alpha * dG);
+        }
+        for (int c = 0; c < 3; ++c)
+            acc[c] = acc[c] + (ra[3] * ws * ra[c]) * kern;
+    }
+    for (std::uint32_t e = 0; e < L.excitations; ++e) {
+        const C2 q{L.q[2 * (static_cast<std::size_t>(e) * L.nt_all + t)], L.q[2 * (static_cast<std::size_t>(e) * L.nt_all + t) + 1]};
+        if (q.re == 0 && q.im == 0)
+            continue;
+        for (int c = 0; c < 3; ++c)
+            if (far[c]) {
+                const C2 v = L.row_scale[node[c]] * (q * acc[c]);
+                atomic_add_c(L.out + 2 * (static_cast<std::size_t>(e) * L.nv + node[c]), v);
+            }
+    }
+}
+
+// out[:, e] += B_near q_e with B_near the P1 x face-column near CSR.
+__global__ void flux_csr_apply(const std::uint32_t *offsets, const std::uint32_t *columns, const double *values,
+                               std::uint32_t rows, const double *q, std::uint32_t faces, std::uint32_t excitations,
+                               double *out) {
+    const std::uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows)
+        return;
+    for (std::uint32_t e = 0; e < excitations; ++e) {
+        C2 s{0, 0};
+        for (std::uint32_t j = offsets[r]; j < offsets[r + 1]; ++j) {
+            const C2 v{values[2 * j], values[2 * j + 1]};
+            const std::size_t f = static_cast<std::size_t>(e) * faces + columns[j];
+            s = s + v * C2{q[2 * f], q[2 * f + 1]};
+        }
+        out[2 * (static_cast<std::size_t>(e) * rows + r)] += s.re;
+        out[2 * (static_cast<std::size_t>(e) * rows + r) + 1] += s.im;
+    }
+}
+
+// out[map[i]] = in[i] (forward) or out[i] = in[map[i]] (backward), per vector.
+__global__ void permute_vectors(const double *in, double *out, const std::uint32_t *map, std::uint32_t n,
+                                std::uint32_t count, bool forward) {
+    const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= static_cast<std::uint64_t>(n) * count)
+        return;
+    const std::uint64_t v = i / n, j = i % n, base = v * n;
+    const std::uint64_t src = base + (forward ? j : map[j]), dst = base + (forward ? map[j] : j);
+    out[2 * dst] = in[2 * src];
+    out[2 * dst + 1] = in[2 * src + 1];
+}
+
 } // namespace
 
 void launch_leaf_moments(const FmmLeafMomentsLaunch &launch, hipStream_t stream) {
@@ -478,6 +585,28 @@ void launch_gather_blocks(const std::uint32_t *row_offsets, const std::uint32_t 
                           hipStream_t stream) {
     hipLaunchKernelGGL(gather_blocks, dim3((n + 255) / 256), dim3(256), 0, stream, row_offsets, columns, values, order,
                        position, n, blocks);
+}
+
+void launch_far_flux(const FmmFarFluxLaunch &launch, hipStream_t stream) {
+    const std::uint64_t pairs = static_cast<std::uint64_t>(launch.nt) * launch.flux_count;
+    if (pairs)
+        hipLaunchKernelGGL(far_flux, dim3(static_cast<std::uint32_t>((pairs + 255) / 256)), dim3(256), 0, stream, launch);
+}
+
+void launch_flux_csr_apply(const std::uint32_t *offsets, const std::uint32_t *columns, const double *values,
+                           std::uint32_t rows, const double *q, std::uint32_t faces, std::uint32_t excitations,
+                           double *out, hipStream_t stream) {
+    if (rows && faces)
+        hipLaunchKernelGGL(flux_csr_apply, dim3((rows + 255) / 256), dim3(256), 0, stream, offsets, columns, values,
+                           rows, q, faces, excitations, out);
+}
+
+void launch_permute(const double *in, double *out, const std::uint32_t *map, std::uint32_t n, std::uint32_t count,
+                    bool forward, hipStream_t stream) {
+    const std::uint64_t total = static_cast<std::uint64_t>(n) * count;
+    if (total)
+        hipLaunchKernelGGL(permute_vectors, dim3(static_cast<std::uint32_t>((total + 255) / 256)), dim3(256), 0,
+                           stream, in, out, map, n, count, forward);
 }
 
 } // namespace hsbem::dev

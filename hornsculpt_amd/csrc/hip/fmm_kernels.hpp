@@ -92,7 +92,34 @@ struct FmmTranslateLaunch {
     double *target_expansions;
 };
 
+// Far part of B q by direct pair quadrature over the flux panels (original numbering).
+struct FmmFarFluxLaunch {
+    const double *geo;
+    const std::uint32_t *nodes;          // 3 x nt, mesh vertex ids
+    std::uint32_t nt, nv, nt_all;        // nt_all: panels per excitation block of q (= nt)
+    const std::uint32_t *flux_panels;    // active driven panels
+    std::uint32_t flux_count;
+    const std::uint32_t *pressure_leaf;  // per vertex
+    const std::uint32_t *flux_leaf;      // per panel
+    const std::uint32_t *near_offsets, *near_sources; // per pressure leaf: near flux leaves
+    const double *rule;                  // (b0, b1, b2, w), weights sum to 1/2
+    std::uint32_t rule_points;
+    double k;
+    const double *row_scale;             // 1 / lumped mass
+    const double *q;                     // per excitation, per panel (complex)
+    std::uint32_t excitations;
+    double *out;                         // nv x excitations (complex), accumulated
+};
+
 void launch_leaf_moments(const FmmLeafMomentsLaunch &launch, hipStream_t stream);
+void launch_far_flux(const FmmFarFluxLaunch &launch, hipStream_t stream);
+// out[:, e] += B_near q_e (q: faces x excitations complex).
+void launch_flux_csr_apply(const std::uint32_t *offsets, const std::uint32_t *columns, const double *values,
+                           std::uint32_t rows, const double *q, std::uint32_t faces, std::uint32_t excitations,
+                           double *out, hipStream_t stream);
+// Per vector of length n: out[map[i]] = in[i] (forward) or out[i] = in[map[i]].
+void launch_permute(const double *in, double *out, const std::uint32_t *map, std::uint32_t n, std::uint32_t count,
+                    bool forward, hipStream_t stream);
 void launch_leaf_emit(const FmmLeaf *leaves, std::uint32_t leaf_count, const std::uint32_t *leaf_dofs,
                       const double *emission, const double *p, double *expansions, hipStream_t stream);
 void launch_leaf_test(const FmmLeaf *leaves, std::uint32_t leaf_count, const std::uint32_t *leaf_dofs,

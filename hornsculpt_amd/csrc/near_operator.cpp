@@ -7,6 +7,10 @@
 // regular pairs (qualified degree), integrated in FP64 by the dense path's
 // kernels and accumulated into the CSR entries their masks select; the mass
 // terms close the near blocks. The SpMV applies A_near to a pressure vector.
+//
+// The split job lists and the CSR pattern depend on the mesh, the tree settings,
+// the active faces and the planning wavenumber only: they are built once
+// (NearStructure, device resident) and integrated per frequency.
 #include "fmm_operator.hpp"
 #include "host/support_near_plan.hpp"
 #include "solver.hpp"
@@ -16,12 +20,26 @@
 
 namespace hsbem {
 
-void MeshSolver::build_near(const Numerics &n, double k_max, double k, const SupportTree &tree,
-                            const SupportNearPlan &plan, hipStream_t s, DeviceNear &out) {
+std::shared_ptr<const NearStructure> MeshSolver::near_structure(const Numerics &n, double k_max,
+                                                                const SupportTreeSettings &ts,
+                                                                const std::vector<std::uint32_t> &faces,
+                                                                hipStream_t s) {
+    const auto key = std::make_tuple(ts.leaf_capacity, ts.maximum_depth, ts.separation, ts.near_ratio, faces,
+                                     n.quadrature_order, n.singular_order, n.near_ratio, n.near_tolerance, n.near_depth,
+                                     k_max);
+    {
+        std::lock_guard lock(plan_mutex_);
+        if (near_cache_ && near_cache_key_ == key)
+            return near_cache_;
+    }
     const std::uint32_t nt = this->nt(), nv = this->nv();
-    const std::vector<SupportNearJob> jobs = support_near_jobs(space_, tree, plan);
+    auto ns = std::make_shared<NearStructure>();
+    ns->tree = make_support_tree(space_, ts);
+    ns->plan = make_support_near_plan(space_, ns->tree, faces);
+    const std::vector<SupportNearJob> jobs = support_near_jobs(space_, ns->tree, ns->plan);
     const std::shared_ptr<DevicePlan> dp_ref = device_plan(n, k_max, s, nullptr);
     const InteractionPlan &ip = dp_ref->plan;
+    ns->interaction = dp_ref;
     // split the jobs by integration route
     std::vector<std::uint32_t> regular, regular_masks, touching_masks, fan_masks, patch_masks;
     std::vector<TouchingJob> touching;
@@ -59,49 +77,68 @@ void MeshSolver::build_near(const Numerics &n, double k_max, double k, const Sup
         regular.push_back(j.source);
         regular_masks.push_back(mask);
     }
-    // mesh-order node ids and row scales (the dense path uses a cluster order)
-    if (!original_nodes_.size()) {
-        std::vector<std::uint32_t> nodes(3 * static_cast<std::size_t>(nt));
-        for (std::uint32_t t = 0; t < nt; ++t)
-            for (int l = 0; l < 3; ++l)
-                nodes[static_cast<std::size_t>(l) * nt + t] = space_.panels[t].nodes[l];
-        std::vector<double> scale(nv);
-        for (std::uint32_t i = 0; i < nv; ++i)
-            scale[i] = 1.0 / space_.lumped_mass[i];
-        original_nodes_.upload(nodes, s);
-        original_row_scale_.upload(scale, s);
+    {
+        // mesh-order node ids and row scales (the dense path uses a cluster order)
+        std::lock_guard lock(plan_mutex_);
+        if (!original_nodes_.size()) {
+            std::vector<std::uint32_t> nodes(3 * static_cast<std::size_t>(nt));
+            for (std::uint32_t t = 0; t < nt; ++t)
+                for (int l = 0; l < 3; ++l)
+                    nodes[static_cast<std::size_t>(l) * nt + t] = space_.panels[t].nodes[l];
+            std::vector<double> scale(nv), mass(nv);
+            for (std::uint32_t i = 0; i < nv; ++i) {
+                scale[i] = 1.0 / space_.lumped_mass[i];
+                mass[i] = space_.lumped_mass[i];
+            }
+            original_nodes_.upload(nodes, s);
+            original_row_scale_.upload(scale, s);
+            original_lumped_.upload(mass, s);
+        }
     }
     auto up32 = [&](DeviceBuffer<std::uint32_t> &d, const std::vector<std::uint32_t> &v) {
         d.upload(v.empty() ? std::vector<std::uint32_t>{0} : v, s);
     };
-    DeviceBuffer<std::uint32_t> &d_ro = out.row_offsets, &d_co = out.columns, &d_fo = out.flux_offsets,
-                                &d_fc = out.flux_columns;
-    DeviceBuffer<std::uint32_t> d_face, d_reg, d_rm, d_tm, d_fm, d_pm;
-    up32(d_ro, plan.pressure.row_offsets);
-    up32(d_co, plan.pressure.columns);
-    up32(d_fo, plan.flux.row_offsets);
-    up32(d_fc, plan.flux.columns);
-    up32(d_face, plan.panel_face_column);
-    up32(d_reg, regular);
-    up32(d_rm, regular_masks);
-    up32(d_tm, touching_masks);
-    up32(d_fm, fan_masks);
-    up32(d_pm, patch_masks);
-    DeviceBuffer<TouchingJob> d_touch;
-    d_touch.upload(touching.empty() ? std::vector<TouchingJob>(1) : touching, s);
-    DeviceBuffer<NearJob> d_fans;
-    d_fans.upload(fans.empty() ? std::vector<NearJob>(1) : fans, s);
-    DeviceBuffer<dev::NearPatchItem> d_items;
-    d_items.upload(items.empty() ? std::vector<dev::NearPatchItem>(1) : items, s);
-    const std::size_t nnz = plan.pressure.columns.size(), fnnz = plan.flux.columns.size();
-    DeviceBuffer<double> &d_val = out.values, &d_fval = out.flux_values;
-    d_val.resize(2 * std::max<std::size_t>(nnz, 1));
-    d_fval.resize(2 * std::max<std::size_t>(fnnz, 1));
+    up32(ns->row_offsets, ns->plan.pressure.row_offsets);
+    up32(ns->columns, ns->plan.pressure.columns);
+    up32(ns->flux_offsets, ns->plan.flux.row_offsets);
+    up32(ns->flux_columns, ns->plan.flux.columns);
+    up32(ns->face_column, ns->plan.panel_face_column);
+    up32(ns->regular, regular);
+    up32(ns->regular_masks, regular_masks);
+    up32(ns->touching_masks, touching_masks);
+    up32(ns->fan_masks, fan_masks);
+    up32(ns->patch_masks, patch_masks);
+    ns->touching.upload(touching.empty() ? std::vector<TouchingJob>(1) : touching, s);
+    ns->fans.upload(fans.empty() ? std::vector<NearJob>(1) : fans, s);
+    ns->items.upload(items.empty() ? std::vector<dev::NearPatchItem>(1) : items, s);
+    ns->nnz = ns->plan.pressure.columns.size();
+    ns->fnnz = ns->plan.flux.columns.size();
+    ns->jobs = jobs.size();
+    ns->regular_count = regular_masks.size();
+    ns->touching_count = touching.size();
+    ns->patch_count = items.size();
+    ns->fan_count = fans.size();
+    hip_check(hipStreamSynchronize(s), "near structure upload");
+    std::lock_guard lock(plan_mutex_);
+    near_cache_ = ns;
+    near_cache_key_ = key;
+    return ns;
+}
+
+void MeshSolver::integrate_near(const Numerics &n, double k, const std::shared_ptr<const NearStructure> &ns,
+                                hipStream_t s, DeviceNear &out) {
+    const std::uint32_t nt = this->nt(), nv = this->nv();
+    out.structure = ns;
+    out.nnz = ns->nnz;
+    out.fnnz = ns->fnnz;
+    out.values.resize(2 * std::max<std::size_t>(ns->nnz, 1));
+    out.flux_values.resize(2 * std::max<std::size_t>(ns->fnnz, 1));
     DeviceBuffer<int> d_err(1);
-    hip_check(hipMemsetAsync(d_val.data(), 0, d_val.bytes(), s), "clear near values");
-    hip_check(hipMemsetAsync(d_fval.data(), 0, d_fval.bytes(), s), "clear near flux values");
+    hip_check(hipMemsetAsync(out.values.data(), 0, out.values.bytes(), s), "clear near values");
+    hip_check(hipMemsetAsync(out.flux_values.data(), 0, out.flux_values.bytes(), s), "clear near flux values");
     hip_check(hipMemsetAsync(d_err.data(), 0, sizeof(int), s), "clear error");
 
+    const DevicePlan &dp = *ns->interaction;
     dev::NearOperatorLaunch L{};
     L.args.panels = {geo_.data(), original_nodes_.data(), bounds_.data(), shape_.data(), nt, nv};
     L.args.row_scale = original_row_scale_.data();
@@ -109,25 +146,25 @@ void MeshSolver::build_near(const Numerics &n, double k_max, double k, const Sup
     L.args.tolerance = n.near_tolerance;
     L.args.regular_order = n.quadrature_order;
     L.args.error = d_err.data();
-    L.csr = {nullptr, d_ro.data(), d_co.data(), d_val.data(), d_fo.data(), d_fc.data(), d_fval.data(), d_face.data()};
-    L.near = {dp_ref->near_offsets.data(), dp_ref->near_trials.data(),
-              dp_ref->whole.size() ? dp_ref->whole.data() : nullptr};
+    L.csr = {nullptr,          ns->row_offsets.data(),  ns->columns.data(),      out.values.data(),
+             ns->flux_offsets.data(), ns->flux_columns.data(), out.flux_values.data(), ns->face_column.data()};
+    L.near = {dp.near_offsets.data(), dp.near_trials.data(), dp.whole.size() ? dp.whole.data() : nullptr};
     L.rules = rules_.data();
-    L.regular = d_reg.data();
-    L.regular_count = static_cast<std::uint32_t>(regular_masks.size());
-    L.touching = d_touch.data();
-    L.touching_count = static_cast<std::uint32_t>(touching.size());
-    L.gauss = dp_ref->gauss.data();
+    L.regular = ns->regular.data();
+    L.regular_count = static_cast<std::uint32_t>(ns->regular_count);
+    L.touching = ns->touching.data();
+    L.touching_count = static_cast<std::uint32_t>(ns->touching_count);
+    L.gauss = dp.gauss.data();
     L.singular_order = n.singular_order;
-    L.fan_jobs = d_fans.data();
-    L.fan_count = static_cast<std::uint32_t>(fans.size());
-    L.patch_items = d_items.data();
-    L.patch_count = static_cast<std::uint32_t>(items.size());
-    L.patches = dp_ref->patches.data();
-    L.regular_masks = d_rm.data();
-    L.touching_masks = d_tm.data();
-    L.fan_masks = d_fm.data();
-    L.patch_masks = d_pm.data();
+    L.fan_jobs = ns->fans.data();
+    L.fan_count = static_cast<std::uint32_t>(ns->fan_count);
+    L.patch_items = ns->items.data();
+    L.patch_count = static_cast<std::uint32_t>(ns->patch_count);
+    L.patches = dp.patches.data();
+    L.regular_masks = ns->regular_masks.data();
+    L.touching_masks = ns->touching_masks.data();
+    L.fan_masks = ns->fan_masks.data();
+    L.patch_masks = ns->patch_masks.data();
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "event");
     hip_check(hipEventCreate(&e1), "event");
@@ -144,13 +181,6 @@ void MeshSolver::build_near(const Numerics &n, double k_max, double k, const Sup
     (void)hipEventDestroy(e1);
     if (code)
         throw std::runtime_error("BEM near operator quadrature failed (device error " + std::to_string(code) + ").");
-    out.nnz = nnz;
-    out.fnnz = fnnz;
-    out.jobs = jobs.size();
-    out.regular = regular_masks.size();
-    out.touching = touching.size();
-    out.patches = items.size();
-    out.fan_jobs = fans.size();
     out.kernel_ms = ms;
 }
 
@@ -159,38 +189,35 @@ NearOperator MeshSolver::near_operator(const Numerics &n, double k_max, double k
     std::lock_guard lock(mutex_);
     hipStream_t s = ctx_->stream;
     const std::uint32_t nv = this->nv();
-    const SupportTree tree = make_support_tree(space_, ts);
-    const SupportNearPlan plan = make_support_near_plan(space_, tree, faces);
     DeviceNear dn;
-    build_near(n, k_max, k, tree, plan, s, dn);
-    const std::size_t nnz = dn.nnz, fnnz = dn.fnnz;
-    DeviceBuffer<double> &d_val = dn.values, &d_fval = dn.flux_values;
-    DeviceBuffer<std::uint32_t> &d_ro = dn.row_offsets, &d_co = dn.columns;
+    integrate_near(n, k, near_structure(n, k_max, ts, faces, s), s, dn);
+    const NearStructure &ns = *dn.structure;
     NearOperator out;
-    out.row_offsets = plan.pressure.row_offsets;
-    out.columns = plan.pressure.columns;
-    out.flux_offsets = plan.flux.row_offsets;
-    out.flux_columns = plan.flux.columns;
-    out.values.resize(2 * nnz);
-    out.flux_values.resize(2 * fnnz);
-    hip_check(hipMemcpyAsync(out.values.data(), d_val.data(), sizeof(double) * out.values.size(), hipMemcpyDeviceToHost, s),
+    out.row_offsets = ns.plan.pressure.row_offsets;
+    out.columns = ns.plan.pressure.columns;
+    out.flux_offsets = ns.plan.flux.row_offsets;
+    out.flux_columns = ns.plan.flux.columns;
+    out.values.resize(2 * dn.nnz);
+    out.flux_values.resize(2 * dn.fnnz);
+    hip_check(hipMemcpyAsync(out.values.data(), dn.values.data(), sizeof(double) * out.values.size(),
+                             hipMemcpyDeviceToHost, s),
               "near values download");
-    hip_check(hipMemcpyAsync(out.flux_values.data(), d_fval.data(), sizeof(double) * out.flux_values.size(),
+    hip_check(hipMemcpyAsync(out.flux_values.data(), dn.flux_values.data(), sizeof(double) * out.flux_values.size(),
                              hipMemcpyDeviceToHost, s),
               "near flux values download");
     if (x) {
         DeviceBuffer<double> dx, dy(2 * static_cast<std::size_t>(nv));
         dx.upload(std::vector<double>(x, x + 2 * static_cast<std::size_t>(nv)), s);
-        dev::launch_csr_apply(d_ro.data(), d_co.data(), d_val.data(), nv, dx.data(), dy.data(), s);
+        dev::launch_csr_apply(ns.row_offsets.data(), ns.columns.data(), dn.values.data(), nv, dx.data(), dy.data(), s);
         out.applied.resize(2 * static_cast<std::size_t>(nv));
         hip_check(hipMemcpyAsync(out.applied.data(), dy.data(), dy.bytes(), hipMemcpyDeviceToHost, s), "apply download");
     }
     hip_check(hipStreamSynchronize(s), "near operator download");
-    out.jobs = dn.jobs;
-    out.regular = dn.regular;
-    out.touching = dn.touching;
-    out.patches = dn.patches;
-    out.fan_jobs = dn.fan_jobs;
+    out.jobs = ns.jobs;
+    out.regular = ns.regular_count;
+    out.touching = ns.touching_count;
+    out.patches = ns.patch_count;
+    out.fan_jobs = ns.fan_count;
     out.kernel_ms = dn.kernel_ms;
     return out;
 }

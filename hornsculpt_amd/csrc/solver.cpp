@@ -131,7 +131,8 @@ Workspace::~Workspace() {
 }
 
 void Workspace::release() {
-    for (auto *b : {&A, &LU, &BD, &q, &rhs, &x, &r, &flux, &scratch, &obs, &field, &partials})
+    for (auto *b : {&A, &LU, &BD, &q, &rhs, &x, &r, &flux, &scratch, &obs, &field, &partials, &fmm_b, &fmm_x, &fmm_r,
+                     &fmm_q})
         b->release();
     flux_column.release();
     pivots.release();
@@ -142,7 +143,8 @@ void Workspace::release() {
 
 std::uint64_t Workspace::capacity_bytes() const {
     std::uint64_t total = 0;
-    for (const auto *b : {&A, &LU, &BD, &q, &rhs, &x, &r, &flux, &scratch, &obs, &field, &partials})
+    for (const auto *b : {&A, &LU, &BD, &q, &rhs, &x, &r, &flux, &scratch, &obs, &field, &partials, &fmm_b, &fmm_x,
+                           &fmm_r, &fmm_q})
         total += b->capacity_bytes();
     return total + flux_column.capacity_bytes() + pivots.capacity_bytes() + jacobi.capacity_bytes() + gmres.z.capacity_bytes() + gmres.basis.capacity_bytes() +
            gmres.r.capacity_bytes() + gmres.scratch.capacity_bytes() + gmres.small.capacity_bytes();
@@ -699,8 +701,16 @@ std::shared_ptr<PreparedRequest> MeshSolver::prepare(const Request &req) const {
         num.near_depth > 10 || !std::isfinite(num.residual_tolerance) || num.residual_tolerance < 1e-12 ||
         num.residual_tolerance > .01 || num.refinement_steps > 10 || num.solver > 1 || num.precision > 1 ||
         num.preconditioner > 1 ||
-        num.gmres_restart < 2 || num.gmres_restart > 300 || num.gmres_max_iterations < 1)
+        num.gmres_restart < 2 || num.gmres_restart > 300 || num.gmres_max_iterations < 1 || num.acceleration > 1)
         throw std::invalid_argument("Invalid BEM medium, frequencies, excitation or Galerkin settings.");
+    // fast multipole settings (cuda_solver.cu:1057-1067)
+    if (num.acceleration == 1 &&
+        (num.fmm_leaf_capacity < 4 || num.fmm_leaf_capacity > 256 || !std::isfinite(num.fmm_separation) ||
+         num.fmm_separation < 1.5 || num.fmm_separation > 10 || num.fmm_maximum_iterations < 1 ||
+         num.fmm_maximum_iterations > 10000 || num.fmm_restart < 2 || num.fmm_restart > 300))
+        throw std::invalid_argument("Invalid Fast Multipole or GMRES settings.");
+    if (num.acceleration == 1 && num.precision != 0)
+        throw std::invalid_argument("The fast multipole form runs in FP64 (precision must be FP64).");
     if (req.excitations.size() > 1024 || req.observers.size() > 100000)
         throw std::invalid_argument("BEM request exceeds indexing limits.");
     static std::atomic<std::uint64_t> next_id{1};
@@ -767,9 +777,14 @@ std::shared_ptr<PreparedRequest> MeshSolver::prepare(const Request &req) const {
     const std::uint64_t nobs = P->observers.size();
     const std::uint64_t scratch = std::max({dev::gemv_scratch_doubles(nv, std::min<std::uint32_t>(ne, 8)),
                                             dev::reduction_scratch_doubles(nv, num.gmres_restart + 1), std::size_t{4096}});
-    std::uint64_t bytes = (mixed ? 8 : 16) * nv2 + 16 * nv * nd1 + 16 * nd1 * ne + 3 * 16ull * nv * ne + 16ull * nt * ne +
-                          8 * scratch + 4ull * nt;
-    if (lu)
+    const bool fmm = num.acceleration == 1;
+    std::uint64_t bytes = (fmm ? 0 : (mixed ? 8 : 16) * nv2) + 16 * nv * nd1 + 16 * nd1 * ne + 3 * 16ull * nv * ne +
+                          16ull * nt * ne + 8 * scratch + 4ull * nt;
+    if (fmm) // operator estimate: near entries, leaf moments, expansions and link workspaces
+        bytes += 3 * 16ull * nv * ne + 16ull * nv * 2048 + (64ull << 20) +
+                 16ull * nv * (num.fmm_restart + 2) + 16ull * nv + 8ull * (4 * (num.fmm_restart + 2) + 8) +
+                 8 * dev::jacobi_factor_doubles(nv);
+    else if (lu)
         bytes += 16 * nv2 + 4ull * nv;
     else
         bytes += 16ull * nv * (num.gmres_restart + 2) + 16ull * nv + 8 * scratch + 8ull * (4 * (num.gmres_restart + 2) + 8) +
@@ -785,7 +800,8 @@ void MeshSolver::bind(const PreparedRequest &P, Workspace &ws) {
     const std::uint32_t nt = this->nt(), nv = this->nv(), ne = P.ne;
     const std::size_t nv2 = static_cast<std::size_t>(nv) * nv;
     const bool mixed = num.precision == 1;
-    ws.A.resize((mixed ? 1 : 2) * nv2);
+    const bool fmm = num.acceleration == 1;
+    ws.A.resize(fmm ? 0 : (mixed ? 1 : 2) * nv2);
     ws.BD.resize(2 * static_cast<std::size_t>(nv) * std::max<std::uint32_t>(P.nd, 1));
     ws.q.resize(2 * static_cast<std::size_t>(std::max<std::uint32_t>(P.nd, 1)) * ne);
     ws.rhs.resize(2 * static_cast<std::size_t>(nv) * ne);
@@ -796,7 +812,7 @@ void MeshSolver::bind(const PreparedRequest &P, Workspace &ws) {
     const std::size_t scratch = std::max({dev::gemv_scratch_doubles(nv, std::min<std::uint32_t>(ne, 8)),
                                           dev::reduction_scratch_doubles(nv, num.gmres_restart + 1), std::size_t{4096}});
     ws.scratch.resize(scratch);
-    if (!mixed && num.solver == 0) {
+    if (!fmm && !mixed && num.solver == 0) {
         ws.LU.resize(2 * nv2);
         ws.pivots.resize(nv);
     }
@@ -902,6 +918,13 @@ FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_
     hip_check(hipMemcpyAsync(ws.q.data(), qd.data(), sizeof(double) * qd.size(), hipMemcpyHostToDevice, s), "q upload");
     hip_check(hipMemcpyAsync(ws.flux.data(), qpanel.data(), sizeof(double) * qpanel.size(), hipMemcpyHostToDevice, s),
               "flux upload");
+    std::vector<std::uint32_t> iterations(ne, 0);
+    std::uint32_t refinements = 0;
+    std::vector<double> relative(ne);
+    if (num.acceleration == 1) {
+        phase(3);
+        fmm_point(P, ws, k, omega, relative, iterations, fr, cancel);
+    } else {
     const auto t_asm = Clock::now();
     hip_check(hipMemsetAsync(ws.A.data(), 0, ws.A.bytes(), s), "clear A");
     hip_check(hipMemsetAsync(ws.BD.data(), 0, ws.BD.bytes(), s), "clear B");
@@ -954,8 +977,6 @@ FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_
     }
     phase(3);
     const auto t_solve = Clock::now();
-    std::vector<std::uint32_t> iterations(ne, 0);
-    std::uint32_t refinements = 0;
     if (solver_kind == 0) {
         hip_check(hipMemcpyAsync(ws.LU.data(), ws.A.data(), ws.A.bytes(), hipMemcpyDeviceToDevice, s), "LU copy");
         hip_check(hipMemsetAsync(ws.error.data(), 0, sizeof(int), s), "clear info");
@@ -991,7 +1012,6 @@ FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_
         }
     }
     // FP64 residual of the unscaled equation; LU corrections while above tolerance
-    std::vector<double> relative(ne);
     for (;;) {
         dev::launch_residual(ws.A.data(), mixed, nv, ws.rhs.data(), ws.x.data(), ne, ws.r.data(), ws.scratch.data(), s);
         double largest = 0;
@@ -1028,6 +1048,7 @@ FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_
         ++refinements;
     }
     fr.ms[1] = ms_since(t_solve);
+    } // dense form
     // fields
     phase(5);
     const auto t_field = Clock::now();

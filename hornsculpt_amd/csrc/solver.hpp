@@ -92,6 +92,10 @@ struct Numerics {
     unsigned precision{0};
     unsigned plan_threads{0};
     unsigned preconditioner{1}; // GMRES: 0 none, 1 block-Jacobi (64 x 64 diagonal blocks)
+    unsigned acceleration{0};   // 0 dense, 1 fast multipole
+    unsigned fmm_leaf_capacity{16};
+    double fmm_separation{1.5};
+    unsigned fmm_maximum_iterations{300}, fmm_restart{40};
 };
 
 struct Excitation {
@@ -130,6 +134,10 @@ struct FrequencyResult {
     // device-event kernel times of this frequency (ms) and algorithmic counts
     double regular_kernel_ms{}, touching_kernel_ms{}, near_kernel_ms{};
     double regular_samples{}, touching_samples{}, near_samples{};
+    // FMM form: seed-audit discrepancy ||(A_seed - A_seed/10) x|| / ||b|| (largest excitation),
+    // whether the defect correction with the finer operator ran, and the operator's apply time
+    double fmm_audit{}, fmm_apply_ms{};
+    bool fmm_corrected{};
 };
 
 struct Result {
@@ -164,6 +172,7 @@ struct Workspace {
     hipStream_t stream{};
     bool owns_stream{};
     DeviceBuffer<double> A, LU, BD, q, rhs, x, r, flux, scratch, obs, field, partials;
+    DeviceBuffer<double> fmm_b, fmm_x, fmm_r, fmm_q; // FMM form, mesh vertex order
     DeviceBuffer<std::int32_t> flux_column, pivots;
     DeviceBuffer<double> jacobi;
     DeviceBuffer<int> error;
@@ -243,6 +252,7 @@ struct PreparedRequest {
 // flux CSR over P1 x active faces), in mesh vertex order, rows scaled by
 // 1/lumped mass; `applied` = A_near x when x is given.
 struct DeviceNear;
+struct NearStructure;
 struct FmmSettings;
 class FmmOperator;
 struct SupportNearPlan;
@@ -285,9 +295,20 @@ class MeshSolver {
                                         std::uint32_t nd, const std::int32_t *flux_column, int *error);
     void bind(const PreparedRequest &prepared, Workspace &ws);
     void build_clusters();
-    // Exact near entries of the support near plan into device CSR (original numbering).
-    void build_near(const Numerics &n, double k_max, double k, const SupportTree &tree, const SupportNearPlan &plan,
-                    hipStream_t stream, DeviceNear &out);
+    // Near structure (tree, near plan, split jobs, CSR pattern; cached per settings)
+    // and its entries at wavenumber k (original numbering, rows scaled by 1/mass).
+    std::shared_ptr<const NearStructure> near_structure(const Numerics &n, double k_max, const SupportTreeSettings &ts,
+                                                        const std::vector<std::uint32_t> &faces, hipStream_t stream);
+    void integrate_near(const Numerics &n, double k, const std::shared_ptr<const NearStructure> &structure,
+                        hipStream_t stream, DeviceNear &out);
+    // FMM operator on a caller stream; share: reuse its near entries and preconditioner.
+    std::unique_ptr<FmmOperator> build_fmm(const Numerics &n, double k_max, double k, const FmmSettings &settings,
+                                           const std::vector<std::uint32_t> &faces, hipStream_t stream,
+                                           const FmmOperator *share);
+    // One frequency point of the FMM form: right-hand sides, GMRES, residuals and the
+    // seed audit; leaves the pressures in ws.x (internal order).
+    void fmm_point(const PreparedRequest &P, Workspace &ws, double k, double omega, std::vector<double> &relative,
+                   std::vector<std::uint32_t> &iterations, FrequencyResult &fr, const volatile std::int32_t *cancel);
 
     Context *ctx_;
     SurfaceMesh mesh_;
@@ -308,7 +329,12 @@ class MeshSolver {
     std::mutex plan_mutex_;
     std::unique_ptr<Workspace> workspace_; // direct solve path (context stream), reused across calls
     DeviceBuffer<std::uint32_t> original_nodes_; // 3 x nt, mesh vertex ids (near operator)
-    DeviceBuffer<double> original_row_scale_;
+    DeviceBuffer<double> original_row_scale_, original_lumped_;
+    DeviceBuffer<std::uint32_t> internal_of_device_; // mesh vertex -> internal node
+    using NearKey = std::tuple<unsigned, unsigned, double, double, std::vector<std::uint32_t>, unsigned, unsigned,
+                               double, double, unsigned, double>;
+    std::shared_ptr<const NearStructure> near_cache_;
+    NearKey near_cache_key_;
     std::mutex mutex_;                     // one direct solve per mesh at a time (as the reference)
 };
 

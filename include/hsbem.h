@@ -60,6 +60,9 @@ enum {
     HSBEM_PHASE_FREQUENCY_COMPLETE = 6
 };
 
+#define HSBEM_ACCELERATION_DENSE 0
+#define HSBEM_ACCELERATION_FAST_MULTIPOLE 1
+
 typedef struct hsbem_context hsbem_context;
 typedef struct hsbem_mesh hsbem_mesh;
 typedef struct hsbem_result hsbem_result;
@@ -82,6 +85,12 @@ typedef struct {
     uint32_t precision;                  /* HSBEM_PRECISION_FP64 (default) or HSBEM_PRECISION_MIXED */
     uint32_t plan_threads;               /* host planner threads, 0 = up to 8 */
     uint32_t preconditioner;             /* GMRES: HSBEM_PRECONDITIONER_BLOCK_JACOBI (default) or _NONE */
+    /* BemNumerics::acceleration and FastMultipoleSettings (bem.hpp:60-71, 121): */
+    uint32_t acceleration;               /* HSBEM_ACCELERATION_DENSE (default) or _FAST_MULTIPOLE (FP64) */
+    uint32_t fmm_leaf_capacity;          /* leafSupportCapacity, 4..256 (default 16) */
+    double fmm_separation_ratio;         /* separationRatio, 1.5..10 (default 1.5) */
+    uint32_t fmm_maximum_iterations;     /* maximumIterations (default 300) */
+    uint32_t fmm_restart_length;         /* restartLength (default 40) */
 } hsbem_numerics;
 
 /* BemRequest (bem.hpp:163-191) for a closed exterior mesh. Excitation e assigns

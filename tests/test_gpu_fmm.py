@@ -95,3 +95,48 @@ def test_fmm_gmres_reproduces_dense_solution():
     assert _rel(x_plain, x_ref) <= 5e-6
     assert its <= its_plain
     op.close()
+
+
+def _horn_request(frequencies, numerics, observers=()):
+    mesh = meshes.exponential_horn(0.012)
+    driven = sorted(set(int(f) for f, b in zip(mesh.face_ids, mesh.boundary) if b == bem.DRIVEN))
+    req = bem.BemRequest(medium=bem.AcousticMedium(1.205, 343.0), frequencies=list(frequencies),
+                         excitations=[bem.BemExcitation("piston", [bem.FaceVelocity(f, 1.0) for f in driven])],
+                         observers=list(observers), numerics=numerics)
+    return mesh, req
+
+
+def test_fmm_request_matches_dense_request():
+    """hsbem_solve with the fast multipole form reproduces the dense FP64 solve (same
+    residual tolerance) at low and high frequency, observers included."""
+    freqs = [400.0, 2500.0]
+    observers = [(0.35, 0.0, 0.0), (0.2, 0.15, 0.05)]
+    dense = bem.BemNumerics(solver=bem.SOLVER_GMRES, relative_residual_tolerance=1e-6)
+    fmm = bem.BemNumerics(solver=bem.SOLVER_GMRES, relative_residual_tolerance=1e-6,
+                          acceleration=bem.ACCELERATION_FAST_MULTIPOLE)
+    mesh, rd = _horn_request(freqs, dense, observers)
+    _, rf = _horn_request(freqs, fmm, observers)
+    solver = bem.HipBemSolver(mesh)
+    res_d, res_f = solver.solve(rd), solver.solve(rf)
+    for fd, ff in zip(res_d.frequencies, res_f.frequencies):
+        ed, ef = fd.excitations[0], ff.excitations[0]
+        err = _rel(ef.boundary.pressure_trace, ed.boundary.pressure_trace)
+        obs = _rel(ef.observer_pressure, ed.observer_pressure)
+        print(f"{ff.frequency:.0f} Hz: FMM vs dense trace {err:.2e}, observers {obs:.2e}, residual "
+              f"{ef.boundary_residual:.1e}, iterations {ef.iterations} (dense {ed.iterations}), "
+              f"build {ff.assembly_milliseconds:.0f} ms, solve {ff.solve_milliseconds:.0f} ms")
+        assert ef.boundary_residual <= 1e-6
+        assert err <= 1e-5 and obs <= 1e-5
+        assert ef.iterations > 0
+
+
+def test_fmm_request_validation():
+    mesh, req = _horn_request([500.0], bem.BemNumerics(acceleration=bem.ACCELERATION_FAST_MULTIPOLE,
+                                                       precision=bem.PRECISION_MIXED))
+    solver = bem.HipBemSolver(mesh)
+    with pytest.raises(ValueError, match="FP64"):
+        solver.solve(req)
+    _, req = _horn_request([500.0], bem.BemNumerics(acceleration=bem.ACCELERATION_FAST_MULTIPOLE,
+                                                    fmm_leaf_capacity=2))
+    with pytest.raises(ValueError, match="Fast Multipole"):
+        solver.solve(req)
