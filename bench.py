@@ -199,6 +199,36 @@ def run_served(solver, frequencies, precision, tolerance, steps, slots, label, o
     return out
 
 
+def run_fmm(solver, frequencies, leaf_capacity, repetitions, check=True):
+    """FMM pressure operator on the same mesh: build time, HIP-event apply time and
+    the agreement of its action with the dense FP64 device assembly (a random vector)."""
+    num = bem.BemNumerics()
+    rng = np.random.default_rng(11)
+    out = []
+    for f in frequencies:
+        k = 2 * math.pi * f / 343.0
+        t0 = time.perf_counter()
+        op = solver.fmm_operator(num, k, k, leaf_capacity=leaf_capacity)
+        build_ms = 1e3 * (time.perf_counter() - t0)
+        st = op.statistics()
+        apply_ms = op.time_apply(repetitions)
+        err = None
+        if check:
+            A, _ = solver.assemble(num, k, k)
+            x = rng.standard_normal(A.shape[0]) + 1j * rng.standard_normal(A.shape[0])
+            y = A @ x
+            err = float(np.linalg.norm(op.apply(x) - y) / np.linalg.norm(y))
+            del A
+        out.append({"frequency_hz": f, "build_ms": round(build_ms, 1), "apply_ms": round(apply_ms, 3),
+                    "apply_rel_err_vs_dense": err, "level_orders": st["level_orders"], "leaves": st["leaves"],
+                    "m2l_links": st["m2l_links"], "near_nnz": st["near_nnz"],
+                    "device_mb": round(st["device_bytes"] / 1e6, 1)})
+        print(f"# FMM {f:.0f} Hz: build {build_ms:.0f} ms, apply {apply_ms:.3f} ms, vs dense {err}, orders "
+              f"{st['level_orders']}, {st['m2l_links']} M2L links", file=sys.stderr, flush=True)
+        op.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +245,11 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cross-check", action="store_true")
+    ap.add_argument("--fmm", action="store_true",
+                    help="also measure the FMM pressure operator (lowest and highest frequency) on the same mesh")
+    ap.add_argument("--fmm-only", action="store_true", help="only the FMM operator measurement (one JSON line)")
+    ap.add_argument("--fmm-leaf", type=int, default=64)
+    ap.add_argument("--fmm-repetitions", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: every GPU box here has one MI355X and the solver has no multi-GPU path "
@@ -229,6 +264,14 @@ def main():
     t0 = time.perf_counter()
     solver = bem.HipBemSolver(mesh)
     construct_s = time.perf_counter() - t0
+    if args.fmm_only:
+        fmm = run_fmm(solver, [frequencies[0], frequencies[-1]], args.fmm_leaf, args.fmm_repetitions,
+                      check=not args.no_cross_check)
+        print(json.dumps({"metric": "FMM pressure operator apply time", "unit": "ms", "value": fmm[-1]["apply_ms"],
+                          "higher_is_better": False, "n_gpus": 1, "dtype": "f64", "data": "synthetic",
+                          "config": {"workload": f"exponential horn {nt} triangles / {nv} P1 nodes, leaf "
+                                                 f"{args.fmm_leaf}, accuracy seed 5e-7"}, "fmm": fmm}), flush=True)
+        return
     cross = None
     if not args.no_cross_check:
         # Dense FP64 LU cross-check of the FP64 GMRES solve at the highest frequency (untimed).
@@ -299,6 +342,9 @@ def main():
                      "launch_ms": roof["kernel_ms"] / roof["points"],
                      "flops_per_launch": FLOPS_PER_SAMPLE * roof["samples"] / roof["points"]},
     }
+    if args.fmm:
+        line["fmm"] = run_fmm(solver, [frequencies[0], frequencies[-1]], args.fmm_leaf, args.fmm_repetitions,
+                              check=not args.no_cross_check)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(mesh, frequencies, args.cpu_threads)
     print(json.dumps(line), flush=True)

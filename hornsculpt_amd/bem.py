@@ -161,6 +161,7 @@ def library() -> ctypes.CDLL:
                 "hsbem_fmm_operator_time_apply": (ctypes.c_int, [P, U32, D]),
                 "hsbem_fmm_operator_statistics": (ctypes.c_int, [P, ctypes.POINTER(U64), D, ctypes.POINTER(U32),
                                                                  ctypes.POINTER(U32)]),
+                "hsbem_fmm_operator_set_graph": (ctypes.c_int, [P, ctypes.c_int32]),
                 "hsbem_fmm_operator_destroy": (ctypes.c_int, [P]),
                 "hsbem_rotation_plan": (ctypes.c_int, [U32, D, D, D, D, ctypes.c_int32, D]),
                 "hsbem_rcr_queue_policy": (ctypes.c_int, [U32, U32, U32, ctypes.POINTER(U64)]),
@@ -719,6 +720,10 @@ class HipFmmOperator:
                                                   tolerance, int(bool(precondition)), ctypes.byref(achieved),
                                                   ctypes.byref(its)))
         return x.view(np.complex128), achieved.value, its.value
+
+    def set_graph(self, enable: bool = True):
+        """Replay the whole apply from one captured HIP graph (captured on first use)."""
+        _check(self._lib.hsbem_fmm_operator_set_graph(self.handle, int(bool(enable))))
 
     def time_apply(self, repetitions: int = 10) -> float:
         ms = ctypes.c_double()

@@ -33,6 +33,7 @@ struct hsbem_near_operator {
 struct hsbem_fmm_operator {
     std::shared_ptr<hsbem::MeshSolver> solver; // keeps the mesh's device data alive
     hsbem::Context *ctx;
+    bool graph{};
     std::unique_ptr<hsbem::FmmOperator> op;
     hsbem::GmresWorkspace gmres;
     hsbem::DeviceBuffer<double> x, y;
@@ -713,7 +714,12 @@ int hsbem_fmm_operator_create(hsbem_mesh *m, const hsbem_numerics *n, const hsbe
 
 int hsbem_fmm_operator_apply(hsbem_fmm_operator *o, const double *x, double *y) {
     return guarded([&] {
-        fmm_host_call(o, x, y, [&](const double *dx, double *dy, hipStream_t s) { o->op->apply(dx, dy, s); });
+        fmm_host_call(o, x, y, [&](const double *dx, double *dy, hipStream_t s) {
+            if (o->graph)
+                o->op->apply_graph(dx, dy, s);
+            else
+                o->op->apply(dx, dy, s);
+        });
     });
 }
 
@@ -732,7 +738,10 @@ int hsbem_fmm_operator_solve(hsbem_fmm_operator *o, const double *b, double *x, 
         fmm_host_call(o, b, x, [&](const double *db, double *dx, hipStream_t s) {
             hsbem::KrylovOperator kop;
             const hsbem::FmmOperator *op = o->op.get();
-            kop.apply = [op](const double *v, double *w, hipStream_t st) { op->apply(v, w, st); };
+            if (o->graph)
+                kop.apply = [op](const double *v, double *w, hipStream_t st) { op->apply_graph(v, w, st); };
+            else
+                kop.apply = [op](const double *v, double *w, hipStream_t st) { op->apply(v, w, st); };
             if (precondition)
                 kop.precondition = [op](const double *v, double *z, hipStream_t st) { op->precondition(v, z, st); };
             hsbem::gmres(s, kop, op->size(), db, dx, restart, maximum_iterations, tolerance, achieved, iterations,
@@ -749,13 +758,19 @@ int hsbem_fmm_operator_time_apply(hsbem_fmm_operator *o, uint32_t repetitions, d
         hipStream_t s = o->ctx->stream;
         o->x.upload(std::vector<double>(n, 1.0), s);
         o->y.resize(n);
-        o->op->apply(o->x.data(), o->y.data(), s); // warm
+        auto run = [&] {
+            if (o->graph)
+                o->op->apply_graph(o->x.data(), o->y.data(), s);
+            else
+                o->op->apply(o->x.data(), o->y.data(), s);
+        };
+        run(); // warm (and capture)
         hipEvent_t e0, e1;
         hsbem::hip_check(hipEventCreate(&e0), "event");
         hsbem::hip_check(hipEventCreate(&e1), "event");
         hsbem::hip_check(hipEventRecord(e0, s), "event");
         for (uint32_t r = 0; r < repetitions; ++r)
-            o->op->apply(o->x.data(), o->y.data(), s);
+            run();
         hsbem::hip_check(hipEventRecord(e1, s), "event");
         hsbem::hip_check(hipEventSynchronize(e1), "event sync");
         float ms = 0;
@@ -783,6 +798,13 @@ int hsbem_fmm_operator_statistics(const hsbem_fmm_operator *o, uint64_t *counts,
         if (orders)
             std::copy(st.level_orders.begin(), st.level_orders.end(), orders);
         *level_count = static_cast<uint32_t>(st.level_orders.size());
+    });
+}
+
+int hsbem_fmm_operator_set_graph(hsbem_fmm_operator *o, int32_t enable) {
+    return guarded([&] {
+        require(o, "null FMM operator");
+        o->graph = enable != 0;
     });
 }
 

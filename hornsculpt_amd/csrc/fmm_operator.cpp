@@ -522,6 +522,28 @@ void FmmOperator::apply(const double *x, double *y, hipStream_t s) const {
     dev::launch_leaf_test(leaves_.data(), leaf_count_, leaf_dofs_.data(), testing_.data(), pool_.data(), y, s);
 }
 
+FmmOperator::~FmmOperator() {
+    if (graph_exec_)
+        (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_)
+        (void)hipGraphDestroy(graph_);
+}
+
+void FmmOperator::apply_graph(const double *x, double *y, hipStream_t s) const {
+    const std::size_t bytes = 2 * sizeof(double) * n_;
+    if (!graph_exec_) {
+        graph_x_.resize(2 * static_cast<std::size_t>(n_));
+        graph_y_.resize(2 * static_cast<std::size_t>(n_));
+        hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "FMM graph capture");
+        apply(graph_x_.data(), graph_y_.data(), s);
+        hip_check(hipStreamEndCapture(s, &graph_), "FMM graph capture end");
+        hip_check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "FMM graph instantiate");
+    }
+    hip_check(hipMemcpyAsync(graph_x_.data(), x, bytes, hipMemcpyDeviceToDevice, s), "FMM graph input");
+    hip_check(hipGraphLaunch(graph_exec_, s), "FMM graph launch");
+    hip_check(hipMemcpyAsync(y, graph_y_.data(), bytes, hipMemcpyDeviceToDevice, s), "FMM graph output");
+}
+
 void FmmOperator::precondition(const double *v, double *z, hipStream_t s) const {
     dev::jacobi_apply(jacobi_->data(), n_, v, z, s, order_.data());
 }

@@ -63,8 +63,13 @@ class FmmOperator {
   public:
     // y = A x (device pointers, nv complex each, y may not alias x).
     void apply(const double *x, double *y, hipStream_t stream) const;
+    // y = A x through a captured graph of the whole apply (M2L graph replay): the
+    // graph runs on operator-owned input/output buffers, captured on first use on
+    // the given (non-default) stream; x and y are copied in and out on the device.
+    void apply_graph(const double *x, double *y, hipStream_t stream) const;
     // z = M^{-1} v (block-Jacobi of the near operator).
     void precondition(const double *v, double *z, hipStream_t stream) const;
+    ~FmmOperator();
     std::uint32_t size() const { return n_; }
     const DeviceNear &near() const { return *near_; }
     const FmmStatistics &statistics() const { return stats_; }
@@ -93,6 +98,9 @@ class FmmOperator {
     DeviceBuffer<std::uint64_t> axial_offsets_;
     std::vector<Launch> upward_, downward_;
     std::uint32_t leaf_count_{};
+    mutable DeviceBuffer<double> graph_x_, graph_y_;
+    mutable hipGraph_t graph_{};
+    mutable hipGraphExec_t graph_exec_{};
 };
 
 } // namespace hsbem

@@ -296,10 +296,14 @@ __device__ __forceinline__ std::size_t degree_block(int l) {
 // Y and W share one workspace slot per link, Z uses a second.
 constexpr int kStage = (2 * kMaxFmmOrder + 1) * (2 * kMaxFmmOrder + 1);
 
-// Degree by degree: d_l and the chunk's degree-l slices staged in LDS.
+// Degree by degree: d_l and the chunk's degree-l slices staged in LDS; the next
+// degree's values are loaded into registers while the current one is computed.
+constexpr int kStagePerThread = (kStage + 255) / 256;
+constexpr int kSlicePerThread = (kFmmChunk * (2 * kMaxFmmOrder + 1) + 255) / 256;
+
 __global__ __launch_bounds__(256) void rotate_links(FmmTranslateLaunch L, bool back) {
     __shared__ double dl[kStage];
-    __shared__ C2 xs[kFmmChunk][2 * kMaxFmmOrder + 1];
+    __shared__ C2 xs[kFmmChunk][2 * kMaxFmmOrder + 1]; // rows >= count: stale, their results are dropped
     __shared__ C2 ph[2 * kMaxFmmOrder + 1];
     __shared__ std::uint64_t src_off[kFmmChunk], dst_off[kFmmChunk];
     __shared__ int ord[kFmmChunk];
@@ -314,48 +318,86 @@ __global__ __launch_bounds__(256) void rotate_links(FmmTranslateLaunch L, bool b
         dst_off[threadIdx.x] = lk.slot;
         ord[threadIdx.x] = static_cast<int>(back ? lk.target_order : lk.source_order);
     }
+    __syncthreads();
     const C2 *in = reinterpret_cast<const C2 *>(back ? L.work_z : L.source_expansions);
     C2 *out = reinterpret_cast<C2 *>(L.work_y);
-    const int count = static_cast<int>(ch.count), P = static_cast<int>(rot.order);
-    for (int l = 0; l <= static_cast<int>(ch.max_order); ++l) {
+    const int count = static_cast<int>(ch.count), P = static_cast<int>(rot.order), top = static_cast<int>(ch.max_order);
+    double pd[kStagePerThread];
+    C2 px[kSlicePerThread];
+    auto fetch = [&](int l) {
+        const int n = 2 * l + 1;
+        const double *src = L.small_d + rot.small_d + static_cast<std::size_t>(l) * (2 * l - 1) * (2 * l + 1) / 3;
+#pragma unroll
+        for (int i = 0; i < kStagePerThread; ++i) {
+            const int e = threadIdx.x + 256 * i;
+            pd[i] = e < n * n ? src[e] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < kSlicePerThread; ++i) {
+            const int e = threadIdx.x + 256 * i;
+            C2 v{0, 0};
+            if (e < count * n) {
+                const int j = e / n, q = e % n;
+                if (l <= ord[j])
+                    v = in[src_off[j] + static_cast<std::size_t>(l) * l + q];
+            }
+            px[i] = v;
+        }
+    };
+    fetch(0);
+    for (int l = 0; l <= top; ++l) {
         const int n = 2 * l + 1;
         const std::size_t base = static_cast<std::size_t>(l) * l;
         __syncthreads();
-        const double *src = L.small_d + rot.small_d + static_cast<std::size_t>(l) * (2 * l - 1) * (2 * l + 1) / 3;
-        for (int e = threadIdx.x; e < n * n; e += blockDim.x)
-            dl[e] = src[e];
-        for (int e = threadIdx.x; e < count * n; e += blockDim.x) {
-            const int j = e / n, q = e % n;
-            C2 v{0, 0};
-            if (l <= ord[j]) {
-                v = in[src_off[j] + base + q];
-                if (!back)
-                    v = ph[P + q - l] * v;
+#pragma unroll
+        for (int i = 0; i < kStagePerThread; ++i) {
+            const int e = threadIdx.x + 256 * i;
+            if (e < n * n)
+                dl[e] = pd[i];
+        }
+#pragma unroll
+        for (int i = 0; i < kSlicePerThread; ++i) {
+            const int e = threadIdx.x + 256 * i;
+            if (e < count * n) {
+                const int j = e / n, q = e % n;
+                xs[j][q] = back ? px[i] : ph[P + q - l] * px[i];
             }
-            xs[j][q] = v;
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < count * n; e += blockDim.x) {
-            const int j = e / n, t = e % n;
-            if (l > ord[j])
-                continue;
-            C2 s{0, 0};
-            if (!back) {
-                for (int q = 0; q < n; ++q)
-                    s = s + dl[q * n + t] * xs[j][q];
-                out[dst_off[j] + base + t] = s;
-            } else {
-                for (int q = 0; q < n; ++q)
-                    s = s + dl[t * n + q] * xs[j][q];
-                out[dst_off[j] + base + t] = conj(ph[P + t - l]) * s;
+        if (l < top)
+            fetch(l + 1);
+        // thread = (output t, group of 4 links): one Wigner load feeds 4 accumulators
+        const int groups = (count + 3) / 4;
+        for (int e = threadIdx.x; e < groups * n; e += blockDim.x) {
+            const int g = e / n, t = e % n, j0 = 4 * g;
+            C2 s0{0, 0}, s1{0, 0}, s2{0, 0}, s3{0, 0};
+            const int stride = back ? 1 : n;
+            const double *drow = back ? dl + t * n : dl + t;
+            for (int q = 0; q < n; ++q) {
+                const double d = drow[q * stride];
+                s0 = s0 + d * xs[j0][q];
+                s1 = s1 + d * xs[j0 + 1][q];
+                s2 = s2 + d * xs[j0 + 2][q];
+                s3 = s3 + d * xs[j0 + 3][q];
+            }
+            const C2 sv[4] = {s0, s1, s2, s3};
+            const C2 rot_back = back ? conj(ph[P + t - l]) : C2{1, 0};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int j = j0 + i;
+                if (j < count && l <= ord[j])
+                    out[dst_off[j] + base + t] = back ? rot_back * sv[i] : sv[i];
             }
         }
     }
 }
 
 // Z = T Y per |m| block for a chunk of links sharing one coaxial plan: the
-// block T_m and the chunk's order-m columns of Y are staged in LDS.
+// block T_m and the chunk's order-m columns of Y are staged in LDS, the next
+// block prefetched into registers during the current product.
 constexpr int kAxialRows = kMaxFmmOrder + 1;
+constexpr int kBlockPerThread = (kAxialRows * kAxialRows + 255) / 256;
+constexpr int kColumnPerThread = (kFmmChunk * 2 * kAxialRows + 255) / 256;
 
 __global__ __launch_bounds__(256) void coaxial_links(FmmTranslateLaunch L) {
     __shared__ C2 tm[kAxialRows * kAxialRows];
@@ -380,28 +422,70 @@ __global__ __launch_bounds__(256) void coaxial_links(FmmTranslateLaunch L) {
         if (abs(t - l) > mm)
             z[slot[j] + c] = {0, 0};
     }
+    C2 pt_[kBlockPerThread], py[kColumnPerThread];
     std::size_t off = 0;
+    auto fetch = [&](int m, std::size_t o) {
+        const int rows = pt - m + 1, cols = ps - m + 1;
+#pragma unroll
+        for (int i = 0; i < kBlockPerThread; ++i) {
+            const int e = threadIdx.x + 256 * i;
+            pt_[i] = e < rows * cols ? T[o + e] : C2{0, 0};
+        }
+#pragma unroll
+        for (int i = 0; i < kColumnPerThread; ++i) {
+            const int e = threadIdx.x + 256 * i;
+            C2 v{0, 0};
+            if (e < count * 2 * cols) {
+                const int j = e / (2 * cols), r = e % (2 * cols), sgn = r / cols, n = m + r % cols;
+                v = y[slot[j] + static_cast<std::size_t>(n) * n + n + (sgn ? -m : m)];
+            }
+            py[i] = v;
+        }
+    };
+    fetch(0, 0);
     for (int m = 0; m <= mm; ++m) {
         const int rows = pt - m + 1, cols = ps - m + 1;
         __syncthreads();
-        for (int e = threadIdx.x; e < rows * cols; e += blockDim.x)
-            tm[e] = T[off + e];
-        for (int e = threadIdx.x; e < count * 2 * cols; e += blockDim.x) {
-            const int j = e / (2 * cols), r = e % (2 * cols), sgn = r / cols, n = m + r % cols;
-            ys[j][sgn][n - m] = y[slot[j] + static_cast<std::size_t>(n) * n + n + (sgn ? -m : m)];
+#pragma unroll
+        for (int i = 0; i < kBlockPerThread; ++i) {
+            const int e = threadIdx.x + 256 * i;
+            if (e < rows * cols)
+                tm[e] = pt_[i];
+        }
+#pragma unroll
+        for (int i = 0; i < kColumnPerThread; ++i) {
+            const int e = threadIdx.x + 256 * i;
+            if (e < count * 2 * cols) {
+                const int j = e / (2 * cols), r = e % (2 * cols), sgn = r / cols;
+                ys[j][sgn][r % cols] = py[i];
+            }
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < count * 2 * rows; e += blockDim.x) {
-            const int j = e / (2 * rows), r = e % (2 * rows), sgn = r / rows, l = m + r % rows;
+        off += static_cast<std::size_t>(rows) * cols;
+        if (m < mm)
+            fetch(m + 1, off);
+        // thread = (row, sign, group of 4 links): one T load feeds 4 accumulators
+        const int groups = (count + 3) / 4;
+        for (int e = threadIdx.x; e < groups * 2 * rows; e += blockDim.x) {
+            const int g = e / (2 * rows), r = e % (2 * rows), sgn = r / rows, l = m + r % rows, j0 = 4 * g;
             if (sgn && !m)
                 continue;
-            C2 s{0, 0};
+            C2 s0{0, 0}, s1{0, 0}, s2{0, 0}, s3{0, 0};
             const C2 *trow = tm + (l - m) * cols;
-            for (int c = 0; c < cols; ++c)
-                s = s + trow[c] * ys[j][sgn][c];
-            z[slot[j] + static_cast<std::size_t>(l) * l + l + (sgn ? -m : m)] = s;
+            for (int c = 0; c < cols; ++c) {
+                const C2 tv = trow[c];
+                s0 = s0 + tv * ys[j0][sgn][c];
+                s1 = s1 + tv * ys[j0 + 1][sgn][c];
+                s2 = s2 + tv * ys[j0 + 2][sgn][c];
+                s3 = s3 + tv * ys[j0 + 3][sgn][c];
+            }
+            const C2 sv[4] = {s0, s1, s2, s3};
+            const std::size_t idx = static_cast<std::size_t>(l) * l + l + (sgn ? -m : m);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (j0 + i < count)
+                    z[slot[j0 + i] + idx] = sv[i];
         }
-        off += static_cast<std::size_t>(rows) * cols;
     }
 }
 

@@ -332,6 +332,10 @@ HSBEM_API int hsbem_fmm_operator_time_apply(hsbem_fmm_operator *op, uint32_t rep
  * orders[level_count] = expansion order per level (0: inactive). */
 HSBEM_API int hsbem_fmm_operator_statistics(const hsbem_fmm_operator *op, uint64_t *counts, double *times,
                                             uint32_t *orders, uint32_t *level_count);
+/* Route later applies and solves through one captured graph of the whole apply
+ * (replaces the M2L chunk graph replay, support_m2l_graph.hpp:13-91, and
+ * support_fmm_operator.cu:865-1027): captured on first use, replayed after. */
+HSBEM_API int hsbem_fmm_operator_set_graph(hsbem_fmm_operator *op, int32_t enable);
 HSBEM_API int hsbem_fmm_operator_destroy(hsbem_fmm_operator *op);
 
 /* Spherical-harmonic rotation plan of the LF FMM translations (host-only;

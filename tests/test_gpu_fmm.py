@@ -140,3 +140,33 @@ def test_fmm_request_validation():
                                                     fmm_leaf_capacity=2))
     with pytest.raises(ValueError, match="Fast Multipole"):
         solver.solve(req)
+
+
+def test_fmm_graph_replays_match_eager_applies():
+    """The captured apply, replayed several times with eager applies and host
+    synchronisations in between, equals the eager apply every time; GMRES through
+    the graph equals GMRES without it."""
+    mesh = meshes.exponential_horn(0.012)
+    k = 2 * math.pi * 1200.0 / 343.0
+    solver = bem.HipBemSolver(mesh)
+    op = solver.fmm_operator(bem.BemNumerics(), k, k, leaf_capacity=32)
+    rng = np.random.default_rng(7)
+    xs = [rng.standard_normal(op.size) + 1j * rng.standard_normal(op.size) for _ in range(4)]
+    eager = [op.apply(x) for x in xs]
+    eager_ms = op.time_apply(5)
+    op.set_graph(True)
+    for rep in range(3):
+        for x, ye in zip(xs, eager):
+            yg = op.apply(x)
+            assert np.array_equal(yg, ye), f"replay {rep} differs"
+            op.set_graph(False)
+            assert np.array_equal(op.apply(x), ye)  # eager work between replays
+            op.set_graph(True)
+    graph_ms = op.time_apply(5)
+    b = eager[0]
+    xg, resg, itg = op.solve(b, tolerance=1e-9)
+    op.set_graph(False)
+    xe, rese, ite = op.solve(b, tolerance=1e-9)
+    print(f"FMM apply eager {eager_ms:.3f} ms, graph {graph_ms:.3f} ms; GMRES {itg} / {ite} iterations")
+    assert itg == ite and np.array_equal(xg, xe)
+    op.close()
