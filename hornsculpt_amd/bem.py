@@ -171,6 +171,18 @@ def library() -> ctypes.CDLL:
                 "hsbem_angular_transfer": (ctypes.c_int, [U32, U32, U32, ctypes.c_int32, ctypes.c_double, D, D, D]),
                 "hsbem_harmonic_spectrum": (ctypes.c_int, [U32, U32, ctypes.c_int32, D, D]),
                 "hsbem_diagonal_translation": (ctypes.c_int, [U32, U32, ctypes.c_double, D, D]),
+                "hsbem_spl_db": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_double, D]),
+                "hsbem_normalized_volume_impedance": (ctypes.c_int, [ctypes.c_double] * 5 + [D]),
+                "hsbem_angular_level_db": (ctypes.c_int, [D, D, U32, ctypes.c_double, D,
+                                                          ctypes.POINTER(ctypes.c_int32)]),
+                "hsbem_beamwidth": (ctypes.c_int, [D, D, U32, ctypes.c_double, D, ctypes.POINTER(ctypes.c_int32)]),
+                "hsbem_cta2034_cuts": (ctypes.c_int, [D, D, U32, D, D, U32, D, ctypes.POINTER(ctypes.c_int32)]),
+                "hsbem_angular_frequency_response": (ctypes.c_int, [D, ctypes.POINTER(U32), D, D, U32,
+                                                                    ctypes.c_double, ctypes.c_int32,
+                                                                    ctypes.c_double, D,
+                                                                    ctypes.POINTER(ctypes.c_int32)]),
+                "hsbem_acoustic_frame": (ctypes.c_int, [D, D, D, D]),
+                "hsbem_polar_directions": (ctypes.c_int, [D, D, D, ctypes.c_int32, D, U32, D]),
                 "hsbem_scaled_waves": (ctypes.c_int, [U32, ctypes.c_double, ctypes.c_double, D, ctypes.c_int32, D]),
                 "hsbem_scaled_expansion_evaluate": (ctypes.c_int, [U32, ctypes.c_double, ctypes.c_double, D,
                                                                    ctypes.c_int32, D, D]),
@@ -977,6 +989,100 @@ def axial_plan(kind: int, source_order: int, target_order: int, k: float, source
         blocks.append(flat[pos:pos + r * c].reshape(r, c))
         pos += r * c
     return blocks, probe.value, applied
+
+
+# ---------------------------------------------------------------- result post-processing
+# Mirror of modules/analysis/include/hornsculpt/acoustic_analysis.hpp:11-78 and
+# acoustic_frame.hpp:8-26; angular samples are sequences of (angle_degrees, level_db).
+
+CTA_FIELDS = ("on_axis", "listening_window", "early_reflections", "early_reflections_di", "sound_power",
+              "sound_power_di", "predicted_in_room")
+
+
+def _samples(samples):
+    a = np.ascontiguousarray([float(x[0]) for x in samples] or [0.0])
+    l = np.ascontiguousarray([float(x[1]) for x in samples] or [0.0])
+    return a, l, len(samples)
+
+
+def spl_db(pressure: complex, reference: float = 20e-6) -> float:
+    out = ctypes.c_double()
+    _check(library().hsbem_spl_db(complex(pressure).real, complex(pressure).imag, reference, ctypes.byref(out)))
+    return out.value
+
+
+def normalized_volume_impedance(impedance: complex, area: float, density: float, sound_speed: float) -> complex:
+    out = np.zeros(2)
+    z = complex(impedance)
+    _check(library().hsbem_normalized_volume_impedance(z.real, z.imag, area, density, sound_speed, _dptr(out)))
+    return complex(out[0], out[1])
+
+
+def angular_level_db(samples, angle: float) -> Optional[float]:
+    a, l, n = _samples(samples)
+    level, present = ctypes.c_double(), ctypes.c_int32()
+    _check(library().hsbem_angular_level_db(_dptr(a), _dptr(l), n, float(angle), ctypes.byref(level),
+                                            ctypes.byref(present)))
+    return level.value if present.value else None
+
+
+def beamwidth(samples, attenuation_db: float = 6.0) -> Optional[tuple]:
+    """(left, right, width) in degrees of the lobe containing 0 degrees, or None."""
+    a, l, n = _samples(samples)
+    out, present = np.zeros(3), ctypes.c_int32()
+    _check(library().hsbem_beamwidth(_dptr(a), _dptr(l), n, float(attenuation_db), _dptr(out), ctypes.byref(present)))
+    return tuple(out.tolist()) if present.value else None
+
+
+def cta2034_cuts(horizontal, vertical) -> dict:
+    ha, hl, hn = _samples(horizontal)
+    va, vl, vn = _samples(vertical)
+    out, present = np.zeros(7), np.zeros(7, np.int32)
+    _check(library().hsbem_cta2034_cuts(_dptr(ha), _dptr(hl), hn, _dptr(va), _dptr(vl), vn, _dptr(out),
+                                        present.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+    return {k: (float(v) if p else None) for k, v, p in zip(CTA_FIELDS, out, present)}
+
+
+def angular_frequency_response(rows, angle: float, relative: bool = False, reference_angle: float = 0.0) -> list:
+    """rows: sequence of (frequency_hz, samples); one level (or None) per row."""
+    freqs = np.ascontiguousarray([float(r[0]) for r in rows] or [0.0])
+    offsets = [0]
+    angles, levels = [], []
+    for _, smp in rows:
+        angles += [float(x[0]) for x in smp]
+        levels += [float(x[1]) for x in smp]
+        offsets.append(len(angles))
+    off = np.ascontiguousarray(offsets, dtype=np.uint32)
+    a = np.ascontiguousarray(angles or [0.0])
+    l = np.ascontiguousarray(levels or [0.0])
+    out, present = np.zeros(max(len(rows), 1)), np.zeros(max(len(rows), 1), np.int32)
+    _check(library().hsbem_angular_frequency_response(_dptr(freqs), _uptr(off), _dptr(a), _dptr(l), len(rows),
+                                                      float(angle), int(bool(relative)), float(reference_angle),
+                                                      _dptr(out), present.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+    return [float(v) if p else None for v, p in zip(out[:len(rows)], present[:len(rows)])]
+
+
+def acoustic_frame(origin=(0.0, 0.0, 0.0), forward=(0.0, 0.0, 1.0), horizontal=None) -> dict:
+    """Normalised H/V frame (default H from world X, or Y near the X axis, when horizontal is None)."""
+    o = np.ascontiguousarray(origin, dtype=np.float64)
+    f = np.ascontiguousarray(forward, dtype=np.float64)
+    h = None if horizontal is None else np.ascontiguousarray(horizontal, dtype=np.float64)
+    out = np.zeros(12)
+    _check(library().hsbem_acoustic_frame(_dptr(o), _dptr(f), None if h is None else _dptr(h), _dptr(out)))
+    return {"origin": out[0:3], "forward": out[3:6], "horizontal": out[6:9], "vertical": out[9:12]}
+
+
+def polar_directions(degrees, vertical: bool = False, origin=(0.0, 0.0, 0.0), forward=(0.0, 0.0, 1.0),
+                     horizontal=None) -> np.ndarray:
+    """Unit far-field directions cos(a) forward + sin(a) H (or V) of a polar sweep."""
+    o = np.ascontiguousarray(origin, dtype=np.float64)
+    f = np.ascontiguousarray(forward, dtype=np.float64)
+    h = None if horizontal is None else np.ascontiguousarray(horizontal, dtype=np.float64)
+    d = np.ascontiguousarray(degrees, dtype=np.float64)
+    out = np.zeros(3 * len(d))
+    _check(library().hsbem_polar_directions(_dptr(o), _dptr(f), None if h is None else _dptr(h), int(bool(vertical)),
+                                            _dptr(d) if len(d) else None, len(d), _dptr(out)))
+    return out.reshape(-1, 3)
 
 
 def parse_support_near_plan(blob: np.ndarray, triangle_count: int) -> dict:

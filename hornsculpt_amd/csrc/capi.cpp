@@ -3,6 +3,7 @@
 #include "../../include/hsbem.h"
 
 #include "fmm_operator.hpp"
+#include "host/acoustic_analysis.hpp"
 #include "host/fmm_levels.hpp"
 #include "host/fmm_rotation.hpp"
 #include "host/fmm_scaled.hpp"
@@ -16,6 +17,7 @@
 #include <cstring>
 #include <exception>
 #include <memory>
+#include <optional>
 #include <string>
 
 struct hsbem_context {
@@ -1052,6 +1054,127 @@ int hsbem_axial_plan(int32_t kind, uint32_t source_order, uint32_t target_order,
             complex_out(hsbem::apply_axial_plan(
                             plan, complex_in(source, static_cast<std::size_t>(source_order + 1) * (source_order + 1))),
                         applied);
+        }
+    });
+}
+
+namespace {
+
+std::vector<hsbem::AngularSample> samples_in(const double *angles, const double *levels, uint32_t n) {
+    require(n == 0 || (angles && levels), "null angular samples");
+    std::vector<hsbem::AngularSample> s(n);
+    for (uint32_t i = 0; i < n; ++i)
+        s[i] = {angles[i], levels[i]};
+    return s;
+}
+
+hsbem::AcousticFrame frame_in(const double *origin, const double *forward, const double *horizontal) {
+    require(origin && forward, "null acoustic frame");
+    const hsbem::Vec3 o{origin[0], origin[1], origin[2]}, f{forward[0], forward[1], forward[2]};
+    if (!horizontal)
+        return hsbem::default_acoustic_frame(o, f);
+    return hsbem::normalized_acoustic_frame({o, f, {horizontal[0], horizontal[1], horizontal[2]}});
+}
+
+} // namespace
+
+int hsbem_spl_db(double re, double im, double reference, double *out) {
+    return guarded([&] {
+        require(out, "null SPL output");
+        *out = hsbem::spl_db({re, im}, reference);
+    });
+}
+
+int hsbem_normalized_volume_impedance(double re, double im, double area, double density, double sound_speed,
+                                      double *out) {
+    return guarded([&] {
+        require(out, "null impedance output");
+        const auto z = hsbem::normalized_volume_impedance({re, im}, area, density, sound_speed);
+        out[0] = z.real();
+        out[1] = z.imag();
+    });
+}
+
+int hsbem_angular_level_db(const double *angles, const double *levels, uint32_t count, double angle, double *level,
+                           int32_t *present) {
+    return guarded([&] {
+        require(level && present, "null angular level output");
+        const auto v = hsbem::angular_level_db(samples_in(angles, levels, count), angle);
+        *present = v.has_value();
+        *level = v.value_or(0.0);
+    });
+}
+
+int hsbem_beamwidth(const double *angles, const double *levels, uint32_t count, double attenuation, double *out,
+                    int32_t *present) {
+    return guarded([&] {
+        require(out && present, "null beamwidth output");
+        const auto b = hsbem::beamwidth(samples_in(angles, levels, count), attenuation);
+        *present = b.has_value();
+        out[0] = b ? b->left : 0.0;
+        out[1] = b ? b->right : 0.0;
+        out[2] = b ? b->width : 0.0;
+    });
+}
+
+int hsbem_cta2034_cuts(const double *h_angles, const double *h_levels, uint32_t h_count, const double *v_angles,
+                       const double *v_levels, uint32_t v_count, double *out, int32_t *present) {
+    return guarded([&] {
+        require(out && present, "null CTA output");
+        const auto c = hsbem::cta2034_cuts(samples_in(h_angles, h_levels, h_count),
+                                           samples_in(v_angles, v_levels, v_count));
+        const std::optional<double> v[7] = {c.on_axis, c.listening_window, c.early_reflections,
+                                            c.early_reflections_di, c.sound_power, c.sound_power_di,
+                                            c.predicted_in_room};
+        for (int i = 0; i < 7; ++i) {
+            present[i] = v[i].has_value();
+            out[i] = v[i].value_or(0.0);
+        }
+    });
+}
+
+int hsbem_angular_frequency_response(const double *frequencies, const uint32_t *row_offsets, const double *angles,
+                                     const double *levels, uint32_t rows, double angle, int32_t relative,
+                                     double reference_angle, double *levels_out, int32_t *present) {
+    return guarded([&] {
+        require(rows == 0 || (frequencies && row_offsets && levels_out && present), "null response arguments");
+        std::vector<hsbem::AngularRow> r(rows);
+        for (uint32_t i = 0; i < rows; ++i) {
+            require(row_offsets[i + 1] >= row_offsets[i], "invalid row offsets");
+            r[i].frequency = frequencies[i];
+            r[i].samples = samples_in(angles ? angles + row_offsets[i] : nullptr,
+                                      levels ? levels + row_offsets[i] : nullptr, row_offsets[i + 1] - row_offsets[i]);
+        }
+        const auto v = hsbem::angular_frequency_response(r, angle, relative != 0, reference_angle);
+        for (uint32_t i = 0; i < rows; ++i) {
+            present[i] = v[i].has_value();
+            levels_out[i] = v[i].value_or(0.0);
+        }
+    });
+}
+
+int hsbem_acoustic_frame(const double *origin, const double *forward, const double *horizontal, double *out) {
+    return guarded([&] {
+        require(out, "null frame output");
+        const auto f = frame_in(origin, forward, horizontal);
+        const hsbem::Vec3 v = hsbem::vertical_direction(f);
+        const double w[12] = {f.origin.x,     f.origin.y,     f.origin.z,     f.forward.x,
+                              f.forward.y,    f.forward.z,    f.horizontal.x, f.horizontal.y,
+                              f.horizontal.z, v.x,            v.y,            v.z};
+        std::copy(w, w + 12, out);
+    });
+}
+
+int hsbem_polar_directions(const double *origin, const double *forward, const double *horizontal, int32_t vertical,
+                           const double *degrees, uint32_t count, double *directions) {
+    return guarded([&] {
+        require((degrees && directions) || count == 0, "null polar arguments");
+        const auto d = hsbem::polar_directions(frame_in(origin, forward, horizontal), vertical != 0,
+                                               std::vector<double>(degrees, degrees + count));
+        for (uint32_t i = 0; i < count; ++i) {
+            directions[3 * i] = d[i].x;
+            directions[3 * i + 1] = d[i].y;
+            directions[3 * i + 2] = d[i].z;
         }
     });
 }

@@ -338,6 +338,42 @@ HSBEM_API int hsbem_fmm_operator_statistics(const hsbem_fmm_operator *op, uint64
 HSBEM_API int hsbem_fmm_operator_set_graph(hsbem_fmm_operator *op, int32_t enable);
 HSBEM_API int hsbem_fmm_operator_destroy(hsbem_fmm_operator *op);
 
+/* Result post-processing (modules/analysis/include/hornsculpt/acoustic_analysis.hpp:11-78,
+ * acoustic_frame.hpp:8-26). Angular samples: angles (degrees, strictly increasing)
+ * and levels (dB) arrays; optional results set *present (0: no value). */
+/* soundPressureLevelDecibels: 20 log10(|p| / reference); a zero pressure gives -inf. */
+HSBEM_API int hsbem_spl_db(double re, double im, double reference, double *out);
+/* normalizedVolumeImpedance: Z S / (rho c). out[2]. */
+HSBEM_API int hsbem_normalized_volume_impedance(double re, double im, double area, double density,
+                                                double sound_speed, double *out);
+/* angularLevelDecibels: linear dB interpolation between finite samples. */
+HSBEM_API int hsbem_angular_level_db(const double *angles, const double *levels, uint32_t count, double angle,
+                                     double *level, int32_t *present);
+/* beamwidthDegrees: out = left, right, width (degrees). */
+HSBEM_API int hsbem_beamwidth(const double *angles, const double *levels, uint32_t count, double attenuation,
+                              double *out, int32_t *present);
+/* cta2034Cuts: out[7] = on axis, listening window, early reflections, early
+ * reflections DI, sound power, sound power DI, predicted in-room (dB). */
+HSBEM_API int hsbem_cta2034_cuts(const double *h_angles, const double *h_levels, uint32_t h_count,
+                                 const double *v_angles, const double *v_levels, uint32_t v_count, double *out,
+                                 int32_t *present);
+/* angularFrequencyResponse: rows i = frequencies[i] with samples
+ * [row_offsets[i], row_offsets[i+1]); relative != 0 subtracts the level at
+ * reference_angle per row. */
+HSBEM_API int hsbem_angular_frequency_response(const double *frequencies, const uint32_t *row_offsets,
+                                               const double *angles, const double *levels, uint32_t rows,
+                                               double angle, int32_t relative, double reference_angle,
+                                               double *levels_out, int32_t *present);
+/* normalizedAcousticFrame (horizontal given) or defaultAcousticFrame (horizontal
+ * null) and verticalDirection: out[12] = origin, forward, horizontal, vertical. */
+HSBEM_API int hsbem_acoustic_frame(const double *origin, const double *forward, const double *horizontal,
+                                   double *out);
+/* Far-field directions of an H (vertical == 0) or V polar sweep in that frame:
+ * cos(a) forward + sin(a) axis; directions[3 count]. No reference counterpart:
+ * the observer directions a directivity request hands to hsbem_solve. */
+HSBEM_API int hsbem_polar_directions(const double *origin, const double *forward, const double *horizontal,
+                                     int32_t vertical, const double *degrees, uint32_t count, double *directions);
+
 /* Spherical-harmonic rotation plan of the LF FMM translations (host-only;
  * replaces makeSphericalRotationPlan / rotateToAxis / rotateFromAxis,
  * modules/bem/fmm/spherical_rotation.hpp:8-27): Wigner small-d blocks of

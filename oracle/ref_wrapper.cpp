@@ -15,6 +15,8 @@
 #include "fmm/support_tree.hpp"
 #include "fmm/wideband_transfer.hpp"
 #include "fmm/scaled_translation.hpp"
+#include <hornsculpt/acoustic_analysis.hpp>
+#include <hornsculpt/acoustic_frame.hpp>
 
 #include <cstdint>
 #include <cstring>
@@ -610,6 +612,96 @@ int ref_axial_plan(int kind, unsigned ps, unsigned pt, double k, double as, doub
             ref_complex_out(wideband::applyScaledAxialPlan(
                                 plan, ref_complex_in(source, static_cast<std::size_t>(ps + 1) * (ps + 1))),
                             applied);
+    });
+}
+}
+
+namespace {
+std::vector<AcousticsAngularSample> ref_samples(const double *a, const double *l, unsigned n) {
+    std::vector<AcousticsAngularSample> s(n);
+    for (unsigned i = 0; i < n; ++i)
+        s[i] = {a[i], l[i]};
+    return s;
+}
+} // namespace
+
+extern "C" {
+int ref_spl_db(double re, double im, double reference, double *out) {
+    return guarded([&] { *out = soundPressureLevelDecibels({re, im}, reference); });
+}
+
+int ref_normalized_volume_impedance(double re, double im, double area, double rho, double c, double *out) {
+    return guarded([&] {
+        const auto z = normalizedVolumeImpedance({re, im}, area, rho, c);
+        out[0] = z.real();
+        out[1] = z.imag();
+    });
+}
+
+int ref_angular_level_db(const double *a, const double *l, unsigned n, double angle, double *level, int *present) {
+    return guarded([&] {
+        const auto s = ref_samples(a, l, n);
+        const auto v = angularLevelDecibels(s, angle);
+        *present = v.has_value();
+        *level = v.value_or(0.0);
+    });
+}
+
+int ref_beamwidth(const double *a, const double *l, unsigned n, double attenuation, double *out, int *present) {
+    return guarded([&] {
+        const auto s = ref_samples(a, l, n);
+        const auto b = beamwidthDegrees(s, attenuation);
+        *present = b.has_value();
+        out[0] = b ? b->leftDegrees : 0.0;
+        out[1] = b ? b->rightDegrees : 0.0;
+        out[2] = b ? b->widthDegrees : 0.0;
+    });
+}
+
+int ref_cta2034_cuts(const double *ha, const double *hl, unsigned hn, const double *va, const double *vl, unsigned vn,
+                     double *out, int *present) {
+    return guarded([&] {
+        const auto h = ref_samples(ha, hl, hn), v = ref_samples(va, vl, vn);
+        const auto c = cta2034Cuts(h, v);
+        const std::optional<double> x[7] = {c.onAxisDb,    c.listeningWindowDb, c.earlyReflectionsDb,
+                                            c.earlyReflectionsDiDb, c.soundPowerDb, c.soundPowerDiDb,
+                                            c.predictedInRoomDb};
+        for (int i = 0; i < 7; ++i) {
+            present[i] = x[i].has_value();
+            out[i] = x[i].value_or(0.0);
+        }
+    });
+}
+
+int ref_angular_frequency_response(const double *f, const unsigned *offsets, const double *a, const double *l,
+                                   unsigned rows, double angle, int relative, double reference, double *out,
+                                   int *present) {
+    return guarded([&] {
+        std::vector<std::vector<AcousticsAngularSample>> data(rows);
+        std::vector<AngularFrequencyRow> r(rows);
+        for (unsigned i = 0; i < rows; ++i) {
+            data[i] = ref_samples(a + offsets[i], l + offsets[i], offsets[i + 1] - offsets[i]);
+            r[i] = {f[i], data[i]};
+        }
+        const auto v = angularFrequencyResponse(
+            r, angle, relative ? AngularResponseNormalization::referenceAngle : AngularResponseNormalization::absolute,
+            reference);
+        for (unsigned i = 0; i < rows; ++i) {
+            present[i] = v[i].levelDecibels.has_value();
+            out[i] = v[i].levelDecibels.value_or(0.0);
+        }
+    });
+}
+
+int ref_acoustic_frame(const double *o, const double *f, const double *h, double *out) {
+    return guarded([&] {
+        const AcousticFrame fr = h ? normalizedAcousticFrame({{o[0], o[1], o[2]}, {f[0], f[1], f[2]}, {h[0], h[1], h[2]}})
+                                   : defaultAcousticFrame({o[0], o[1], o[2]}, {f[0], f[1], f[2]});
+        const Vec3 v = verticalDirection(fr);
+        const double w[12] = {fr.origin.x,     fr.origin.y,     fr.origin.z,     fr.forward.x,
+                              fr.forward.y,    fr.forward.z,    fr.horizontal.x, fr.horizontal.y,
+                              fr.horizontal.z, v.x,             v.y,             v.z};
+        std::memcpy(out, w, sizeof(w));
     });
 }
 }

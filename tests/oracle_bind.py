@@ -72,6 +72,15 @@ def reference():
     lib.ref_axial_plan.argtypes = [ctypes.c_int, ctypes.c_uint, ctypes.c_uint, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_uint, ctypes.c_double,
                                    ctypes.POINTER(ctypes.c_uint64), _D, _D, _D, _D]
+    _I = ctypes.POINTER(ctypes.c_int)
+    lib.ref_spl_db.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _D]
+    lib.ref_normalized_volume_impedance.argtypes = [ctypes.c_double] * 5 + [_D]
+    lib.ref_angular_level_db.argtypes = [_D, _D, ctypes.c_uint, ctypes.c_double, _D, _I]
+    lib.ref_beamwidth.argtypes = [_D, _D, ctypes.c_uint, ctypes.c_double, _D, _I]
+    lib.ref_cta2034_cuts.argtypes = [_D, _D, ctypes.c_uint, _D, _D, ctypes.c_uint, _D, _I]
+    lib.ref_angular_frequency_response.argtypes = [_D, _U, _D, _D, ctypes.c_uint, ctypes.c_double, ctypes.c_int,
+                                                   ctypes.c_double, _D, _I]
+    lib.ref_acoustic_frame.argtypes = [_D, _D, _D, _D]
     lib.ref_support_tree.argtypes = [ctypes.c_uint32, _D, ctypes.c_uint32, _U, ctypes.c_uint, ctypes.c_uint,
                                      ctypes.c_double, ctypes.c_double, _D, ctypes.POINTER(ctypes.c_uint64)]
     return lib
@@ -279,6 +288,80 @@ def ref_axial_plan(kind, ps, pt, k, a_s, a_t, d, polar, projection=0.0, source=N
     _check(lib, lib.ref_axial_plan(*args, ctypes.byref(count), _d(coeffs), _d(probe),
                                    None if src is None else _d(src), _d(applied)), "ref_axial_plan")
     return coeffs.view(np.complex128), probe[0], (applied.view(np.complex128) if src is not None else None)
+
+
+def _ref_samples(samples):
+    a = np.ascontiguousarray([float(x[0]) for x in samples] or [0.0])
+    l = np.ascontiguousarray([float(x[1]) for x in samples] or [0.0])
+    return a, l, len(samples)
+
+
+def ref_spl_db(p, reference_pressure=20e-6):
+    lib, out = reference(), np.zeros(1)
+    _check(lib, lib.ref_spl_db(complex(p).real, complex(p).imag, reference_pressure, _d(out)), "ref_spl_db")
+    return out[0]
+
+
+def ref_normalized_volume_impedance(z, area, rho, c):
+    lib, out = reference(), np.zeros(2)
+    _check(lib, lib.ref_normalized_volume_impedance(complex(z).real, complex(z).imag, area, rho, c, _d(out)),
+           "ref_normalized_volume_impedance")
+    return complex(out[0], out[1])
+
+
+def ref_angular_level_db(samples, angle):
+    lib = reference()
+    a, l, n = _ref_samples(samples)
+    out, present = np.zeros(1), np.zeros(1, np.int32)
+    _check(lib, lib.ref_angular_level_db(_d(a), _d(l), n, float(angle), _d(out),
+                                         present.ctypes.data_as(ctypes.POINTER(ctypes.c_int))), "ref_angular_level_db")
+    return float(out[0]) if present[0] else None
+
+
+def ref_beamwidth(samples, attenuation=6.0):
+    lib = reference()
+    a, l, n = _ref_samples(samples)
+    out, present = np.zeros(3), np.zeros(1, np.int32)
+    _check(lib, lib.ref_beamwidth(_d(a), _d(l), n, float(attenuation), _d(out),
+                                  present.ctypes.data_as(ctypes.POINTER(ctypes.c_int))), "ref_beamwidth")
+    return tuple(out.tolist()) if present[0] else None
+
+
+def ref_cta2034_cuts(h, v):
+    lib = reference()
+    ha, hl, hn = _ref_samples(h)
+    va, vl, vn = _ref_samples(v)
+    out, present = np.zeros(7), np.zeros(7, np.int32)
+    _check(lib, lib.ref_cta2034_cuts(_d(ha), _d(hl), hn, _d(va), _d(vl), vn, _d(out),
+                                     present.ctypes.data_as(ctypes.POINTER(ctypes.c_int))), "ref_cta2034_cuts")
+    return [float(x) if p else None for x, p in zip(out, present)]
+
+
+def ref_angular_frequency_response(rows, angle, relative=False, reference_angle=0.0):
+    lib = reference()
+    freqs = np.ascontiguousarray([float(r[0]) for r in rows] or [0.0])
+    offsets, angles, levels = [0], [], []
+    for _, smp in rows:
+        angles += [float(x[0]) for x in smp]
+        levels += [float(x[1]) for x in smp]
+        offsets.append(len(angles))
+    off = np.ascontiguousarray(offsets, dtype=np.uint32)
+    a, l = np.ascontiguousarray(angles or [0.0]), np.ascontiguousarray(levels or [0.0])
+    out, present = np.zeros(max(len(rows), 1)), np.zeros(max(len(rows), 1), np.int32)
+    _check(lib, lib.ref_angular_frequency_response(_d(freqs), _u(off), _d(a), _d(l), len(rows), float(angle),
+                                                   int(bool(relative)), float(reference_angle), _d(out),
+                                                   present.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
+           "ref_angular_frequency_response")
+    return [float(x) if p else None for x, p in zip(out[:len(rows)], present[:len(rows)])]
+
+
+def ref_acoustic_frame(origin, forward, horizontal=None):
+    lib = reference()
+    o, f = np.ascontiguousarray(origin, dtype=np.float64), np.ascontiguousarray(forward, dtype=np.float64)
+    h = None if horizontal is None else np.ascontiguousarray(horizontal, dtype=np.float64)
+    out = np.zeros(12)
+    _check(lib, lib.ref_acoustic_frame(_d(o), _d(f), None if h is None else _d(h), _d(out)), "ref_acoustic_frame")
+    return out
 
 
 def ref_rcr_queue_policy(source_order, target_order, reserved):
