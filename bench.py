@@ -245,8 +245,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cross-check", action="store_true")
-    ap.add_argument("--fmm", action="store_true",
-                    help="also measure the FMM pressure operator (lowest and highest frequency) on the same mesh")
+    ap.add_argument("--no-fmm", action="store_true",
+                    help="skip the FMM pressure operator measurement (lowest and highest frequency, same mesh)")
     ap.add_argument("--fmm-only", action="store_true", help="only the FMM operator measurement (one JSON line)")
     ap.add_argument("--fmm-leaf", type=int, default=64)
     ap.add_argument("--fmm-repetitions", type=int, default=20)
@@ -342,9 +342,12 @@ def main():
                      "launch_ms": roof["kernel_ms"] / roof["points"],
                      "flops_per_launch": FLOPS_PER_SAMPLE * roof["samples"] / roof["points"]},
     }
-    if args.fmm:
-        line["fmm"] = run_fmm(solver, [frequencies[0], frequencies[-1]], args.fmm_leaf, args.fmm_repetitions,
-                              check=not args.no_cross_check)
+    if not args.no_fmm:
+        try:
+            line["fmm"] = run_fmm(solver, [frequencies[0], frequencies[-1]], args.fmm_leaf, args.fmm_repetitions,
+                                  check=not args.no_cross_check)
+        except Exception as exc:  # reported, never fatal for the headline line
+            line["fmm"] = {"error": f"{type(exc).__name__}: {exc}"}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(mesh, frequencies, args.cpu_threads)
     print(json.dumps(line), flush=True)

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <limits>
 #include <stdexcept>
+#include <thread>
 
 namespace hsbem {
 namespace {
@@ -177,38 +178,63 @@ std::vector<SupportNearJob> support_near_jobs(const Space &space, const SupportT
         plan.flux_block_offsets.size() != tree.nodes.size() + 1 ||
         plan.flux_block_indices.size() != tree.pressure_flux.near.size())
         throw std::invalid_argument("Support near iterator plan and geometry disagree.");
-    std::vector<SupportNearJob> jobs;
-    std::vector<std::uint32_t> stamp(nt, 0), order;
-    std::vector<std::uint16_t> mask(nt, 0);
-    for (std::uint32_t t = 0; t < nt; ++t) {
-        order.clear();
-        auto touch = [&](std::uint32_t p) -> std::uint16_t & {
-            if (stamp[p] != t + 1) {
-                stamp[p] = t + 1;
-                mask[p] = 0;
-                order.push_back(p);
-            }
-            return mask[p];
-        };
-        for (unsigned i = 0; i < 3; ++i) {
-            const std::uint32_t v = space.panels[t].nodes[i];
-            for (std::uint32_t j = plan.pressure.row_offsets[v]; j < plan.pressure.row_offsets[v + 1]; ++j) {
-                const std::uint32_t u = plan.pressure.columns[j];
-                for (std::size_t k = space.incidence_offsets[u]; k < space.incidence_offsets[u + 1]; ++k)
-                    touch(space.incidences[k].panel) |= static_cast<std::uint16_t>(1u << (3 * i + space.incidences[k].local));
-            }
-            const std::uint32_t leaf = tree.pressure_leaf[v];
-            for (std::uint32_t b = plan.flux_block_offsets[leaf]; b < plan.flux_block_offsets[leaf + 1]; ++b) {
-                const SupportNode &src = tree.nodes[tree.pressure_flux.near[plan.flux_block_indices[b]].source];
-                for (std::uint32_t k = src.first_flux; k < src.first_flux + src.flux_count; ++k) {
-                    const std::uint32_t p = tree.flux_order[k];
-                    if (plan.panel_face_column[p] != kNone)
-                        touch(p) |= static_cast<std::uint16_t>(1u << (9 + i));
+    // test panels in contiguous ranges per thread, concatenated in order: the
+    // result equals the single-threaded iterator order
+    const unsigned workers = static_cast<unsigned>(
+        std::max<std::size_t>(1, std::min<std::size_t>(std::thread::hardware_concurrency(), nt / 256 + 1)));
+    std::vector<std::vector<SupportNearJob>> parts(workers);
+    auto run = [&](unsigned w) {
+        const std::uint32_t first = static_cast<std::uint32_t>(nt * w / workers),
+                            last = static_cast<std::uint32_t>(nt * (w + 1) / workers);
+        std::vector<SupportNearJob> &jobs = parts[w];
+        std::vector<std::uint32_t> stamp(nt, 0), order;
+        std::vector<std::uint16_t> mask(nt, 0);
+        for (std::uint32_t t = first; t < last; ++t) {
+            order.clear();
+            auto touch = [&](std::uint32_t p) -> std::uint16_t & {
+                if (stamp[p] != t + 1) {
+                    stamp[p] = t + 1;
+                    mask[p] = 0;
+                    order.push_back(p);
+                }
+                return mask[p];
+            };
+            for (unsigned i = 0; i < 3; ++i) {
+                const std::uint32_t v = space.panels[t].nodes[i];
+                for (std::uint32_t j = plan.pressure.row_offsets[v]; j < plan.pressure.row_offsets[v + 1]; ++j) {
+                    const std::uint32_t u = plan.pressure.columns[j];
+                    for (std::size_t k = space.incidence_offsets[u]; k < space.incidence_offsets[u + 1]; ++k)
+                        touch(space.incidences[k].panel) |=
+                            static_cast<std::uint16_t>(1u << (3 * i + space.incidences[k].local));
+                }
+                const std::uint32_t leaf = tree.pressure_leaf[v];
+                for (std::uint32_t b = plan.flux_block_offsets[leaf]; b < plan.flux_block_offsets[leaf + 1]; ++b) {
+                    const SupportNode &src = tree.nodes[tree.pressure_flux.near[plan.flux_block_indices[b]].source];
+                    for (std::uint32_t k = src.first_flux; k < src.first_flux + src.flux_count; ++k) {
+                        const std::uint32_t p = tree.flux_order[k];
+                        if (plan.panel_face_column[p] != kNone)
+                            touch(p) |= static_cast<std::uint16_t>(1u << (9 + i));
+                    }
                 }
             }
+            for (std::uint32_t p : order)
+                jobs.push_back({t, p, static_cast<std::uint16_t>(mask[p] & 0x1ffu), static_cast<std::uint8_t>(mask[p] >> 9)});
         }
-        for (std::uint32_t p : order)
-            jobs.push_back({t, p, static_cast<std::uint16_t>(mask[p] & 0x1ffu), static_cast<std::uint8_t>(mask[p] >> 9)});
+    };
+    std::vector<std::thread> threads;
+    for (unsigned w = 1; w < workers; ++w)
+        threads.emplace_back(run, w);
+    run(0);
+    for (auto &th : threads)
+        th.join();
+    std::size_t total = 0;
+    for (const auto &p : parts)
+        total += p.size();
+    std::vector<SupportNearJob> jobs;
+    jobs.reserve(total);
+    for (auto &p : parts) {
+        jobs.insert(jobs.end(), p.begin(), p.end());
+        std::vector<SupportNearJob>().swap(p);
     }
     return jobs;
 }

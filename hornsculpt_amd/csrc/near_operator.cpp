@@ -17,6 +17,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <thread>
+#include <type_traits>
 
 namespace hsbem {
 
@@ -40,43 +42,80 @@ std::shared_ptr<const NearStructure> MeshSolver::near_structure(const Numerics &
     const std::shared_ptr<DevicePlan> dp_ref = device_plan(n, k_max, s, nullptr);
     const InteractionPlan &ip = dp_ref->plan;
     ns->interaction = dp_ref;
-    // split the jobs by integration route
-    std::vector<std::uint32_t> regular, regular_masks, touching_masks, fan_masks, patch_masks;
-    std::vector<TouchingJob> touching;
-    std::vector<NearJob> fans;
-    std::vector<dev::NearPatchItem> items;
-    for (const SupportNearJob &j : jobs) {
-        const std::uint32_t mask = j.pressure_mask | static_cast<std::uint32_t>(j.flux_mask) << 9;
-        const Panel &a = space_.panels[j.test], &b = space_.panels[j.source];
-        const Permutation perm = classify_pair_ids(a.nodes.data(), b.nodes.data());
-        if (perm.topology != Topology::disjoint) {
-            touching.push_back({j.test, j.source, perm});
-            touching_masks.push_back(mask);
-            continue;
-        }
-        const auto first = ip.near_trials.begin() + ip.near_offsets[j.test],
-                   last = ip.near_trials.begin() + ip.near_offsets[j.test + 1];
-        const auto it = std::lower_bound(first, last, j.source);
-        if (it != last && *it == j.source) {
-            const NearJob &nj = ip.near[static_cast<std::size_t>(it - ip.near_trials.begin())];
-            bool fan = false;
-            for (std::uint32_t q = 0; q < nj.patch_count; ++q)
-                fan |= ip.patches[nj.first_patch + q].projection != 0;
-            if (fan) {
-                fans.push_back(nj);
-                fan_masks.push_back(mask);
-            } else {
-                for (std::uint32_t q = 0; q < nj.patch_count; ++q) {
-                    items.push_back({nj.test, nj.trial, nj.first_patch + q, nj.swap});
-                    patch_masks.push_back(mask);
-                }
+    // split the jobs by integration route: contiguous job ranges per thread, each
+    // list concatenated in job order
+    struct Split {
+        std::vector<std::uint32_t> regular, regular_masks, touching_masks, fan_masks, patch_masks;
+        std::vector<TouchingJob> touching;
+        std::vector<NearJob> fans;
+        std::vector<dev::NearPatchItem> items;
+    };
+    const unsigned workers = static_cast<unsigned>(std::max<std::size_t>(
+        1, std::min<std::size_t>(std::thread::hardware_concurrency(), jobs.size() / 65536 + 1)));
+    std::vector<Split> parts(workers);
+    auto classify = [&](unsigned w) {
+        Split &out = parts[w];
+        const std::size_t first = jobs.size() * w / workers, last = jobs.size() * (w + 1) / workers;
+        for (std::size_t q = first; q < last; ++q) {
+            const SupportNearJob &j = jobs[q];
+            const std::uint32_t mask = j.pressure_mask | static_cast<std::uint32_t>(j.flux_mask) << 9;
+            const Panel &a = space_.panels[j.test], &b = space_.panels[j.source];
+            const Permutation perm = classify_pair_ids(a.nodes.data(), b.nodes.data());
+            if (perm.topology != Topology::disjoint) {
+                out.touching.push_back({j.test, j.source, perm});
+                out.touching_masks.push_back(mask);
+                continue;
             }
-            continue;
+            const auto lo = ip.near_trials.begin() + ip.near_offsets[j.test],
+                       hi = ip.near_trials.begin() + ip.near_offsets[j.test + 1];
+            const auto it = std::lower_bound(lo, hi, j.source);
+            if (it != hi && *it == j.source) {
+                const NearJob &nj = ip.near[static_cast<std::size_t>(it - ip.near_trials.begin())];
+                bool fan = false;
+                for (std::uint32_t p = 0; p < nj.patch_count; ++p)
+                    fan |= ip.patches[nj.first_patch + p].projection != 0;
+                if (fan) {
+                    out.fans.push_back(nj);
+                    out.fan_masks.push_back(mask);
+                } else {
+                    for (std::uint32_t p = 0; p < nj.patch_count; ++p) {
+                        out.items.push_back({nj.test, nj.trial, nj.first_patch + p, nj.swap});
+                        out.patch_masks.push_back(mask);
+                    }
+                }
+                continue;
+            }
+            out.regular.push_back(j.test);
+            out.regular.push_back(j.source);
+            out.regular_masks.push_back(mask);
         }
-        regular.push_back(j.test);
-        regular.push_back(j.source);
-        regular_masks.push_back(mask);
+    };
+    {
+        std::vector<std::thread> threads;
+        for (unsigned w = 1; w < workers; ++w)
+            threads.emplace_back(classify, w);
+        classify(0);
+        for (auto &th : threads)
+            th.join();
     }
+    auto gather = [&](auto member) {
+        std::remove_reference_t<decltype(parts[0].*member)> all;
+        std::size_t total = 0;
+        for (auto &p : parts)
+            total += (p.*member).size();
+        all.reserve(total);
+        for (auto &p : parts) {
+            all.insert(all.end(), (p.*member).begin(), (p.*member).end());
+            std::remove_reference_t<decltype(p.*member)>().swap(p.*member);
+        }
+        return all;
+    };
+    const std::vector<std::uint32_t> regular = gather(&Split::regular), regular_masks = gather(&Split::regular_masks),
+                                     touching_masks = gather(&Split::touching_masks),
+                                     fan_masks = gather(&Split::fan_masks), patch_masks = gather(&Split::patch_masks);
+    const std::vector<TouchingJob> touching = gather(&Split::touching);
+    const std::vector<NearJob> fans = gather(&Split::fans);
+    const std::vector<dev::NearPatchItem> items = gather(&Split::items);
     {
         // mesh-order node ids and row scales (the dense path uses a cluster order)
         std::lock_guard lock(plan_mutex_);
