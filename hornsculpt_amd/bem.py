@@ -161,7 +161,11 @@ def library() -> ctypes.CDLL:
                 "hsbem_fmm_operator_time_apply": (ctypes.c_int, [P, U32, D]),
                 "hsbem_fmm_operator_statistics": (ctypes.c_int, [P, ctypes.POINTER(U64), D, ctypes.POINTER(U32),
                                                                  ctypes.POINTER(U32)]),
+                "hsbem_fmm_operator_levels": (ctypes.c_int, [P, ctypes.POINTER(U64), ctypes.POINTER(U32),
+                                                             ctypes.POINTER(U32), ctypes.POINTER(U32)]),
                 "hsbem_fmm_operator_set_graph": (ctypes.c_int, [P, ctypes.c_int32]),
+                "hsbem_angular_transfer_device": (ctypes.c_int, [P, ctypes.c_int32, U32, U32, U32, ctypes.c_double,
+                                                                 ctypes.c_double, D, D, D]),
                 "hsbem_fmm_operator_destroy": (ctypes.c_int, [P]),
                 "hsbem_rotation_plan": (ctypes.c_int, [U32, D, D, D, D, ctypes.c_int32, D]),
                 "hsbem_rcr_queue_policy": (ctypes.c_int, [U32, U32, U32, ctypes.POINTER(U64)]),
@@ -750,6 +754,11 @@ class HipFmmOperator:
         out = {k: int(v) for k, v in zip(self.STAT_FIELDS, counts)}
         out.update(dict(zip(("tree_ms", "near_ms", "plans_ms", "moments_ms", "near_kernel_ms"), times.tolist())))
         out["level_orders"] = orders[:nl.value].tolist()
+        counts4, rep, rules = np.zeros(4, np.uint64), np.zeros(64, np.uint32), np.zeros(64, np.uint32)
+        _check(self._lib.hsbem_fmm_operator_levels(self.handle, counts4.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                                   _uptr(rep), _uptr(rules), ctypes.byref(nl)))
+        out.update(plane_wave_levels=int(counts4[0]), angular_links=int(counts4[1]), diagonal_classes=int(counts4[2]),
+                   level_representation=rep[:nl.value].tolist(), level_rule_orders=rules[:nl.value].tolist())
         return out
 
 
@@ -892,6 +901,27 @@ def angular_transfer(values, source_order: int, target_order: int, bandlimit: in
     sh = None if shift is None else np.ascontiguousarray(shift, dtype=np.float64)
     _check(library().hsbem_angular_transfer(source_order, target_order, bandlimit, mode, float(k),
                                             None if sh is None else _dptr(sh), _dptr(v), _dptr(out)))
+    return out.view(np.complex128)
+
+
+DEVICE_SCALED_TO_PARENT, DEVICE_OUTGOING_TO_PARENT, DEVICE_INCOMING_TO_SCALED_CHILD, DEVICE_INCOMING_TO_CHILD = 0, 1, 2, 3
+
+
+def angular_transfer_device(values, kind: int, source_order: int, target_order: int, bandlimit: int, k: float,
+                            radius: float, shift, context: Optional["Context"] = None) -> np.ndarray:
+    """One plane-wave transfer through the FMM operator's device kernels: scaled
+    multipole -> parent signature (kind 0), child -> parent signature (1), parent
+    local signature -> scaled child local (2), parent -> child local signature (3).
+    Orders are rule orders of signatures or the spherical order (= bandlimit) of a
+    scaled expansion; shift = child centre - parent centre."""
+    ctx = context or Context.default()
+    n_in = (source_order + 1) ** 2 if kind == DEVICE_SCALED_TO_PARENT else _rule_size(source_order)
+    n_out = (target_order + 1) ** 2 if kind == DEVICE_INCOMING_TO_SCALED_CHILD else _rule_size(target_order)
+    v = _complex_arg(values, n_in, "transfer input")
+    out = np.zeros(2 * n_out)
+    sh = np.ascontiguousarray(shift, dtype=np.float64)
+    _check(library().hsbem_angular_transfer_device(ctx.handle, kind, source_order, target_order, bandlimit, float(k),
+                                                   float(radius), _dptr(sh), _dptr(v), _dptr(out)))
     return out.view(np.complex128)
 
 

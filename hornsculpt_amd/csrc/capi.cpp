@@ -803,6 +803,23 @@ int hsbem_fmm_operator_statistics(const hsbem_fmm_operator *o, uint64_t *counts,
     });
 }
 
+int hsbem_fmm_operator_levels(const hsbem_fmm_operator *o, uint64_t *counts, uint32_t *representation,
+                              uint32_t *rule_orders, uint32_t *level_count) {
+    return guarded([&] {
+        require(o && counts && level_count, "null FMM level argument");
+        const auto &st = o->op->statistics();
+        counts[0] = st.plane_wave_levels;
+        counts[1] = st.angular_links;
+        counts[2] = st.diagonal_classes;
+        counts[3] = st.level_representation.size();
+        if (representation)
+            std::copy(st.level_representation.begin(), st.level_representation.end(), representation);
+        if (rule_orders)
+            std::copy(st.level_rule_orders.begin(), st.level_rule_orders.end(), rule_orders);
+        *level_count = static_cast<uint32_t>(st.level_representation.size());
+    });
+}
+
 int hsbem_fmm_operator_set_graph(hsbem_fmm_operator *o, int32_t enable) {
     return guarded([&] {
         require(o, "null FMM operator");
@@ -955,6 +972,26 @@ int hsbem_angular_transfer(uint32_t source_order, uint32_t target_order, uint32_
         default:
             throw std::invalid_argument("unknown angular transfer mode");
         }
+    });
+}
+
+int hsbem_angular_transfer_device(hsbem_context *c, int32_t kind, uint32_t source_order, uint32_t target_order,
+                                  uint32_t bandlimit, double k, double radius, const double *shift, const double *in,
+                                  double *out) {
+    return guarded([&] {
+        require(c && shift && in && out, "null device angular transfer argument");
+        require(kind >= HSBEM_DEVICE_SCALED_TO_PARENT && kind <= HSBEM_DEVICE_INCOMING_TO_CHILD,
+                "unknown device angular transfer kind");
+        hsbem::hip_check(hipSetDevice(c->ctx.device), "set device");
+        using K = hsbem::dev::HfTransferKind;
+        const K kinds[] = {K::up_handoff, K::up_resample, K::down_handoff, K::down_resample};
+        const K kd = kinds[kind];
+        const std::size_t n_in = kd == K::up_handoff ? static_cast<std::size_t>(source_order + 1) * (source_order + 1)
+                                                     : rule_size(source_order);
+        const auto v = complex_in(in, n_in);
+        complex_out(hsbem::device_angular_transfer(kd, source_order, target_order, bandlimit, k, radius,
+                                                   {shift[0], shift[1], shift[2]}, v, c->ctx.stream),
+                    out);
     });
 }
 

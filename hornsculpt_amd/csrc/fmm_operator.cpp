@@ -58,6 +58,111 @@ template <class F> void parallel_for(std::size_t count, F &&f) {
         j.get(); // rethrows the first worker exception
 }
 
+// Appends a plane-wave rule's ring table (cos theta, sin theta, sample weight) and
+// its orthonormal Legendre rows P_lm(cos theta) = Y_lm(theta, 0), l <= bmax.
+void append_hf_rule(const SphereRule &R, std::uint32_t bmax, std::vector<dev::HfRule> &rules, std::vector<double> &polar,
+                    std::vector<double> &legendre) {
+    const std::uint32_t rings = R.order + 1, na = 2 * rings;
+    const std::size_t row = static_cast<std::size_t>(bmax + 1) * (bmax + 2) / 2;
+    rules.push_back({legendre.size(), polar.size(), rings, na, bmax, 0});
+    std::vector<double> ring(3 * rings), table(row * rings);
+    for (std::uint32_t i = 0; i < rings; ++i) {
+        const SphereSample &x = R.samples[static_cast<std::size_t>(i) * na];
+        const double mu = x.direction.z;
+        ring[3 * i] = mu;
+        ring[3 * i + 1] = std::sqrt(std::max(0.0, 1 - mu * mu));
+        ring[3 * i + 2] = x.weight;
+    }
+    parallel_for(rings, [&](std::size_t i) {
+        const auto y = spherical_harmonics(bmax, {ring[3 * i + 1], 0.0, ring[3 * i]});
+        for (std::uint32_t l = 0; l <= bmax; ++l)
+            for (std::uint32_t m = 0; m <= l; ++m)
+                table[i * row + static_cast<std::size_t>(l) * (l + 1) / 2 + m] = y[static_cast<std::size_t>(l) * l + l + m].real();
+    });
+    polar.insert(polar.end(), ring.begin(), ring.end());
+    legendre.insert(legendre.end(), table.begin(), table.end());
+}
+
+} // namespace
+
+std::vector<std::complex<double>> device_angular_transfer(dev::HfTransferKind kind, unsigned source_order,
+                                                          unsigned target_order, unsigned bandlimit, double k,
+                                                          double radius, Vec3 child_minus_parent,
+                                                          const std::vector<std::complex<double>> &input,
+                                                          hipStream_t s) {
+    using C = std::complex<double>;
+    const bool up = kind == dev::HfTransferKind::up_handoff || kind == dev::HfTransferKind::up_resample;
+    const bool in_spherical = kind == dev::HfTransferKind::up_handoff,
+               out_spherical = kind == dev::HfTransferKind::down_handoff;
+    auto rule_samples = [](unsigned q) { return static_cast<std::size_t>(q + 1) * 2 * (q + 1); };
+    if (!std::isfinite(k) || k <= 0 || !std::isfinite(radius) || radius <= 0 ||
+        (in_spherical && source_order != bandlimit) || (out_spherical && target_order != bandlimit) ||
+        (!in_spherical && (bandlimit > source_order || 2 * (source_order + 1) > unsigned(dev::kMaxHfAzimuths))) ||
+        (!out_spherical && (bandlimit > target_order || 2 * (target_order + 1) > unsigned(dev::kMaxHfAzimuths))) ||
+        bandlimit > 1023)
+        throw std::invalid_argument("Invalid device angular transfer orders, wavenumber or radius.");
+    const std::size_t n_in = in_spherical ? static_cast<std::size_t>(source_order + 1) * (source_order + 1)
+                                          : rule_samples(source_order);
+    const std::size_t n_out = out_spherical ? static_cast<std::size_t>(target_order + 1) * (target_order + 1)
+                                            : rule_samples(target_order);
+    if (input.size() != n_in)
+        throw std::invalid_argument("Device angular transfer input has the wrong size.");
+    std::vector<dev::HfRule> rules;
+    std::vector<double> polar, legendre;
+    dev::HfLink lk{};
+    if (!in_spherical) {
+        append_hf_rule(make_sphere_rule(source_order), bandlimit, rules, polar, legendre);
+        lk.in_rule = 0;
+    }
+    if (!out_spherical) {
+        append_hf_rule(make_sphere_rule(target_order), bandlimit, rules, polar, legendre);
+        lk.out_rule = static_cast<std::uint32_t>(rules.size() - 1);
+    }
+    if (in_spherical)
+        lk.in_rule = lk.out_rule;
+    if (out_spherical)
+        lk.out_rule = lk.in_rule;
+    const double sign = up ? k : -k;
+    lk.shift[0] = sign * child_minus_parent.x;
+    lk.shift[1] = sign * child_minus_parent.y;
+    lk.shift[2] = sign * child_minus_parent.z;
+    lk.ka = k * radius;
+    lk.order = bandlimit;
+    lk.source = 0;
+    lk.target = n_in;
+    const std::uint32_t in_rings = in_spherical ? 0 : source_order + 1, out_rings = out_spherical ? 0 : target_order + 1;
+    std::vector<double> pool(2 * (n_in + n_out), 0.0);
+    for (std::size_t i = 0; i < n_in; ++i) {
+        pool[2 * i] = input[i].real();
+        pool[2 * i + 1] = input[i].imag();
+    }
+    DeviceBuffer<dev::HfRule> d_rules;
+    DeviceBuffer<double> d_polar, d_legendre, d_pool, d_fourier, d_harmonic;
+    DeviceBuffer<dev::HfLink> d_link;
+    DeviceBuffer<dev::FmmDestination> d_dst;
+    d_rules.upload(rules, s);
+    d_polar.upload(polar, s);
+    d_legendre.upload(legendre, s);
+    d_pool.upload(pool, s);
+    d_fourier.resize(2 * static_cast<std::size_t>(std::max(in_rings, out_rings)) * (2 * bandlimit + 1));
+    d_harmonic.resize(2 * static_cast<std::size_t>(bandlimit + 1) * (bandlimit + 1));
+    d_link.upload(std::vector<dev::HfLink>{lk}, s);
+    d_dst.upload(std::vector<dev::FmmDestination>{{n_in, 0, 0, 1, 0}}, s);
+    dev::HfTransferLaunch H{d_rules.data(), d_legendre.data(), d_polar.data(), d_link.data(), d_dst.data(),
+                            d_fourier.data(), d_harmonic.data(), d_pool.data(), d_pool.data()};
+    dev::launch_hf_transfer(H, kind, 1, 1, in_rings, out_rings, static_cast<std::uint32_t>(out_spherical ? 0 : n_out), s);
+    hip_check(hipGetLastError(), "angular transfer launch");
+    std::vector<double> back(pool.size());
+    hip_check(hipMemcpyAsync(back.data(), d_pool.data(), d_pool.bytes(), hipMemcpyDeviceToHost, s), "angular download");
+    hip_check(hipStreamSynchronize(s), "angular transfer sync");
+    std::vector<C> out(n_out);
+    for (std::size_t i = 0; i < n_out; ++i)
+        out[i] = {back[2 * (n_in + i)], back[2 * (n_in + i) + 1]};
+    return out;
+}
+
+namespace {
+
 template <class T> void upload_nonempty(DeviceBuffer<T> &d, const std::vector<T> &v, hipStream_t s) {
     d.upload(v.empty() ? std::vector<T>(1) : v, s);
 }
@@ -108,32 +213,62 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
     st.near_nnz = op->near_->nnz;
     st.near_kernel_ms = op->near_->kernel_ms;
 
-    // ---- levels: LF order and basis radius of every active level
+    // ---- levels: order, basis radius and representation of every active level
     const std::uint32_t nl = st.levels;
     std::vector<int> order(nl, -1);
     std::vector<double> radius(nl, 0);
+    std::vector<char> plane(nl, 0);
+    std::vector<unsigned> rule_order(nl, 0), translation_order(nl, 0);
     std::uint32_t top = nl;
     for (std::uint32_t l = 0; l < nl; ++l) {
         const WidebandLevelPlan &lp = ep.levels[l].expansion;
         if (lp.representation == LevelRepresentation::inactive)
             continue;
-        if (lp.signature_order > static_cast<unsigned>(dev::kMaxFmmOrder) ||
-            k * lp.active_radius > kMaximumSphericalElectricalRadius)
-            throw std::runtime_error("FMM level " + std::to_string(l) + " needs order " +
-                                     std::to_string(lp.signature_order) +
-                                     " beyond the low-frequency translations built here (at most " +
-                                     std::to_string(dev::kMaxFmmOrder) + ").");
+        if (lp.representation == LevelRepresentation::plane_wave) {
+            if (2 * (lp.quadrature_order + 1) > static_cast<unsigned>(dev::kMaxHfAzimuths) ||
+                lp.signature_order > lp.quadrature_order)
+                throw std::runtime_error("FMM level " + std::to_string(l) + " needs a plane-wave rule of order " +
+                                         std::to_string(lp.quadrature_order) + " (at most " +
+                                         std::to_string(dev::kMaxHfAzimuths / 2 - 1) + ").");
+            plane[l] = 1;
+            rule_order[l] = lp.quadrature_order;
+            translation_order[l] = lp.translation_order;
+        } else if (lp.signature_order > static_cast<unsigned>(dev::kMaxWideOrder) ||
+                   k * lp.active_radius > kMaximumSphericalElectricalRadius) {
+            throw std::runtime_error("FMM level " + std::to_string(l) + " needs spherical order " +
+                                     std::to_string(lp.signature_order) + " beyond the translations built here (at most " +
+                                     std::to_string(dev::kMaxWideOrder) + ").");
+        }
         order[l] = static_cast<int>(lp.signature_order);
         radius[l] = lp.active_radius;
         top = std::min(top, l);
     }
     st.top_level = top;
-    for (std::uint32_t l = 0; l < nl; ++l)
+    for (std::uint32_t l = 0; l < nl; ++l) {
         st.level_orders.push_back(order[l] < 0 ? 0u : static_cast<std::uint32_t>(order[l]));
-    for (std::uint32_t l = top; l < nl; ++l)
+        st.level_representation.push_back(order[l] < 0 ? 0u : plane[l] ? 2u : 1u);
+        st.level_rule_orders.push_back(rule_order[l]);
+        st.plane_wave_levels += plane[l] ? 1 : 0;
+    }
+    for (std::uint32_t l = top; l < nl; ++l) {
         if (order[l] < 0)
             throw std::runtime_error("FMM active levels are not contiguous.");
-    auto coeffs = [&](std::uint32_t l) { return static_cast<std::uint64_t>(order[l] + 1) * (order[l] + 1); };
+        if (l > top && plane[l] && !plane[l - 1])
+            throw std::runtime_error("FMM plan has a plane-wave level below a spherical one.");
+    }
+    std::vector<std::uint32_t> rule_index(nl, 0);
+    std::vector<SphereRule> rules;
+    for (std::uint32_t l = 0; l < nl; ++l)
+        if (plane[l]) {
+            rule_index[l] = static_cast<std::uint32_t>(rules.size());
+            rules.push_back(make_sphere_rule(rule_order[l]));
+        }
+    auto samples = [&](std::uint32_t l) {
+        return static_cast<std::uint64_t>(rule_order[l] + 1) * 2 * (rule_order[l] + 1);
+    };
+    auto coeffs = [&](std::uint32_t l) {
+        return plane[l] ? samples(l) : static_cast<std::uint64_t>(order[l] + 1) * (order[l] + 1);
+    };
 
     // ---- expansion pool: multipole and local of every node with P1 DOFs on an active level
     const std::uint64_t kNone = UINT64_MAX;
@@ -160,6 +295,8 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
         const SupportNode &nd = tree.nodes[id];
         if (!nd.leaf() || !nd.pressure_count || order[nd.level] < 0)
             continue;
+        if (plane[nd.level])
+            throw std::runtime_error("FMM leaves on plane-wave levels are not built here.");
         if (nd.pressure_count > static_cast<std::uint32_t>(dev::kMaxLeafDofs) || order[nd.level] > dev::kMaxLeafOrder)
             throw std::runtime_error("FMM leaf exceeds the supported DOF count or order.");
         dev::FmmLeaf lf{};
@@ -282,7 +419,7 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
         }
         order_out.insert(order_out.end(), idx.begin(), idx.end());
     };
-    auto end_launch = [&](std::vector<FmmOperator::Launch> &list, std::uint32_t first) {
+    auto end_launch = [&](std::uint32_t first) {
         FmmOperator::Launch L{first, static_cast<std::uint32_t>(destinations.size()) - first, launch_link_base,
                               static_cast<std::uint32_t>(links.size()) - launch_link_base,
                               static_cast<std::uint32_t>(chunks.size()), 0,
@@ -294,11 +431,100 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
         workspace = std::max(workspace, slot);
         slot = 0;
         launch_link_base = static_cast<std::uint32_t>(links.size());
-        if (L.destination_count)
-            list.push_back(L);
+        if (L.destination_count) {
+            op->stages_.push_back({FmmOperator::StageKind::translate, static_cast<std::uint32_t>(op->translate_.size())});
+            op->translate_.push_back(L);
+        }
     };
+    // plane-wave transfers: links in destination order, launch-relative indices and
+    // workspace offsets
+    std::vector<dev::HfLink> hf_links;
+    std::vector<dev::FmmDestination> hf_destinations;
+    std::uint64_t hf_fourier = 0, hf_harmonic = 0, hf_fourier_size = 0, hf_harmonic_size = 0;
+    std::vector<std::uint32_t> rule_bandlimit(rules.size(), 0);
+    FmmOperator::HfLaunch hl{};
+    auto hf_begin = [&](dev::HfTransferKind kind) {
+        hl = {};
+        hl.kind = kind;
+        hl.first_link = static_cast<std::uint32_t>(hf_links.size());
+        hl.first_destination = static_cast<std::uint32_t>(hf_destinations.size());
+        hf_fourier = hf_harmonic = 0;
+    };
+    auto hf_destination = [&](std::uint64_t target) {
+        hf_destinations.push_back({target, 0, static_cast<std::uint32_t>(hf_links.size()) - hl.first_link, 0, 0});
+    };
+    // source: the child (upward) or the parent (downward) node; spherical levels pass no rule
+    auto hf_link = [&](std::uint64_t source, std::uint64_t target, const SupportNode &child, const SupportNode &parent,
+                       std::uint32_t child_level) {
+        const bool up = hl.kind == dev::HfTransferKind::up_handoff || hl.kind == dev::HfTransferKind::up_resample;
+        const std::uint32_t parent_level = child_level - 1;
+        dev::HfLink lk{};
+        lk.source = source;
+        lk.target = target;
+        const double sign = up ? k : -k;
+        lk.shift[0] = sign * (child.center.x - parent.center.x);
+        lk.shift[1] = sign * (child.center.y - parent.center.y);
+        lk.shift[2] = sign * (child.center.z - parent.center.z);
+        lk.ka = k * radius[child_level];
+        lk.order = static_cast<std::uint32_t>(plane[child_level] ? std::min(order[child_level], order[parent_level])
+                                                                 : order[child_level]);
+        const std::uint32_t pr = rule_index[parent_level], cr = plane[child_level] ? rule_index[child_level] : pr;
+        lk.in_rule = up ? cr : pr;
+        lk.out_rule = up ? pr : cr;
+        const std::uint64_t in_rings = hl.kind == dev::HfTransferKind::up_handoff ? 0 : rules[lk.in_rule].order + 1;
+        const std::uint64_t out_rings = hl.kind == dev::HfTransferKind::down_handoff ? 0 : rules[lk.out_rule].order + 1;
+        hl.in_rings = std::max(hl.in_rings, static_cast<std::uint32_t>(in_rings));
+        hl.out_rings = std::max(hl.out_rings, static_cast<std::uint32_t>(out_rings));
+        if (out_rings)
+            hl.out_samples = std::max(hl.out_samples, static_cast<std::uint32_t>(samples(up ? parent_level : child_level)));
+        lk.fourier = hf_fourier;
+        lk.harmonic = hf_harmonic;
+        hf_fourier += std::max(in_rings, out_rings) * (2 * lk.order + 1);
+        hf_harmonic += static_cast<std::uint64_t>(lk.order + 1) * (lk.order + 1);
+        rule_bandlimit[pr] = std::max(rule_bandlimit[pr], lk.order);
+        rule_bandlimit[cr] = std::max(rule_bandlimit[cr], lk.order);
+        hf_links.push_back(lk);
+        ++hf_destinations.back().link_count;
+        ++st.angular_links;
+    };
+    auto hf_end = [&] {
+        hl.link_count = static_cast<std::uint32_t>(hf_links.size()) - hl.first_link;
+        hl.destination_count = static_cast<std::uint32_t>(hf_destinations.size()) - hl.first_destination;
+        if (hl.link_count > 65535 || hl.destination_count > 65535)
+            throw std::runtime_error("FMM plane-wave level exceeds 65535 transfer links.");
+        hf_fourier_size = std::max(hf_fourier_size, hf_fourier);
+        hf_harmonic_size = std::max(hf_harmonic_size, hf_harmonic);
+        if (hl.link_count) {
+            op->stages_.push_back({FmmOperator::StageKind::angular, static_cast<std::uint32_t>(op->angular_.size())});
+            op->angular_.push_back(hl);
+        }
+    };
+    // plane-wave M2L: one diagonal translation per (level, cell offset)
+    std::vector<dev::HfDiagonalLink> diag_links;
+    std::vector<dev::FmmDestination> diag_destinations;
+    std::map<std::pair<std::uint32_t, Triple>, std::uint64_t> diag_index;
+    std::vector<std::pair<std::uint32_t, Triple>> diag_keys;
+    std::uint64_t diag_values = 0;
     // upward: parents at level l from children at l + 1
     for (std::uint32_t l = nl - 1; l-- > top;) {
+        if (plane[l]) {
+            hf_begin(plane[l + 1] ? dev::HfTransferKind::up_resample : dev::HfTransferKind::up_handoff);
+            for (std::uint32_t id : tree.levels[l].nodes) {
+                const SupportNode &p = tree.nodes[id];
+                if (p.leaf() || mult[id] == kNone ||
+                    std::none_of(p.children.begin(), p.children.end(),
+                                 [&](std::uint32_t c) { return c != kNoSupportNode && mult[c] != kNone; }))
+                    continue; // no source below: the signature keeps its zero fill
+                hf_destination(mult[id]);
+                for (std::uint32_t c : p.children)
+                    if (c != kNoSupportNode && mult[c] != kNone) {
+                        hf_link(mult[c], 0, tree.nodes[c], p, l + 1);
+                        ++st.m2m_links;
+                    }
+            }
+            hf_end();
+            continue;
+        }
         const std::uint32_t first = static_cast<std::uint32_t>(destinations.size());
         for (std::uint32_t id : tree.levels[l].nodes) {
             const SupportNode &p = tree.nodes[id];
@@ -314,39 +540,95 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
                 ++st.m2m_links;
             }
         }
-        end_launch(op->upward_, first);
+        end_launch(first);
     }
     // downward: M2L from the far sources and L2L from the parent
     std::vector<std::vector<std::uint32_t>> far_sources(tree.nodes.size());
     for (const SupportPair &pr : tree.pressure_pressure.far)
         far_sources[pr.target].push_back(pr.source);
+    auto cell_offset = [&](const SupportNode &t, const SupportNode &sn) {
+        Triple off;
+        for (int a = 0; a < 3; ++a)
+            off[a] = static_cast<std::int64_t>(t.coords[a]) - static_cast<std::int64_t>(sn.coords[a]);
+        return off;
+    };
     for (std::uint32_t l = top; l < nl; ++l) {
-        const std::uint32_t first = static_cast<std::uint32_t>(destinations.size());
-        for (std::uint32_t id : tree.levels[l].nodes) {
-            if (loc[id] == kNone)
-                continue;
-            const SupportNode &t = tree.nodes[id];
-            begin_destination(loc[id], order[l]);
-            for (std::uint32_t sid : far_sources[id]) {
-                if (mult[sid] == kNone)
+        const bool parent_plane = l > top && plane[l - 1];
+        if (plane[l]) {
+            FmmOperator::DiagonalLaunch dl{static_cast<std::uint32_t>(diag_destinations.size()), 0,
+                                           static_cast<std::uint32_t>(diag_links.size()), samples(l)};
+            for (std::uint32_t id : tree.levels[l].nodes) {
+                if (loc[id] == kNone)
                     continue;
-                const SupportNode &sn = tree.nodes[sid];
-                Triple off;
-                for (int a = 0; a < 3; ++a)
-                    off[a] = static_cast<std::int64_t>(t.coords[a]) - static_cast<std::int64_t>(sn.coords[a]);
-                add_link(mult[sid], order[l], direction(off, order[l]),
-                         axial({static_cast<int>(AxialKind::multipole_to_local), l, l, squared(off)}));
-                ++st.m2l_links;
+                const SupportNode &t = tree.nodes[id];
+                diag_destinations.push_back(
+                    {loc[id], 0, static_cast<std::uint32_t>(diag_links.size()) - dl.first_link, 0, 0});
+                for (std::uint32_t sid : far_sources[id]) {
+                    if (mult[sid] == kNone)
+                        continue;
+                    const auto key = std::make_pair(l, cell_offset(t, tree.nodes[sid]));
+                    auto [it, fresh] = diag_index.emplace(key, diag_values);
+                    if (fresh) {
+                        diag_keys.push_back(key);
+                        diag_values += samples(l);
+                    }
+                    diag_links.push_back({mult[sid], it->second});
+                    ++diag_destinations.back().link_count;
+                    ++st.m2l_links;
+                }
             }
-            if (l > top && t.parent != kNoSupportNode && loc[t.parent] != kNone) {
-                const Triple off = child_offset(t, tree.nodes[t.parent]);
-                add_link(loc[t.parent], order[l - 1], direction(off, std::max(order[l], order[l - 1])),
-                         axial({static_cast<int>(AxialKind::local_to_child), l - 1, l, squared(off)}));
+            dl.destination_count = static_cast<std::uint32_t>(diag_destinations.size()) - dl.first_destination;
+            if (dl.destination_count > 65535)
+                throw std::runtime_error("FMM plane-wave level exceeds 65535 destinations.");
+            if (dl.destination_count) {
+                op->stages_.push_back({FmmOperator::StageKind::diagonal, static_cast<std::uint32_t>(op->diagonal_.size())});
+                op->diagonal_.push_back(dl);
+            }
+        } else {
+            const std::uint32_t first = static_cast<std::uint32_t>(destinations.size());
+            for (std::uint32_t id : tree.levels[l].nodes) {
+                if (loc[id] == kNone)
+                    continue;
+                const SupportNode &t = tree.nodes[id];
+                begin_destination(loc[id], order[l]);
+                for (std::uint32_t sid : far_sources[id]) {
+                    if (mult[sid] == kNone)
+                        continue;
+                    const Triple off = cell_offset(t, tree.nodes[sid]);
+                    add_link(mult[sid], order[l], direction(off, order[l]),
+                             axial({static_cast<int>(AxialKind::multipole_to_local), l, l, squared(off)}));
+                    ++st.m2l_links;
+                }
+                if (l > top && !parent_plane && t.parent != kNoSupportNode && loc[t.parent] != kNone) {
+                    const Triple off = child_offset(t, tree.nodes[t.parent]);
+                    add_link(loc[t.parent], order[l - 1], direction(off, std::max(order[l], order[l - 1])),
+                             axial({static_cast<int>(AxialKind::local_to_child), l - 1, l, squared(off)}));
+                    ++st.l2l_links;
+                }
+            }
+            end_launch(first);
+        }
+        if (parent_plane) {
+            // added after the level's own M2L sums
+            hf_begin(plane[l] ? dev::HfTransferKind::down_resample : dev::HfTransferKind::down_handoff);
+            for (std::uint32_t id : tree.levels[l].nodes) {
+                const SupportNode &t = tree.nodes[id];
+                if (loc[id] == kNone || t.parent == kNoSupportNode || loc[t.parent] == kNone)
+                    continue;
+                hf_destination(loc[id]);
+                hf_link(loc[t.parent], loc[id], t, tree.nodes[t.parent], l);
                 ++st.l2l_links;
             }
+            hf_end();
         }
-        end_launch(op->downward_, first);
     }
+
+    // the kernels of a launch are chosen by the largest order it touches: its links'
+    // orders and the orders of the (level-shared) rotation plans it applies
+    for (FmmOperator::Launch &L : op->translate_)
+        for (std::uint32_t c = L.first_chunk; c < L.first_chunk + L.chunk_count; ++c)
+            L.max_order = std::max({L.max_order, chunks[c].max_order,
+                                    static_cast<std::uint32_t>(direction_order[chunks[c].rotation])});
 
     // ---- rotation and coaxial plans (host, in parallel)
     std::vector<RotationPlan> rotations(directions.size());
@@ -390,6 +672,25 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
             axial_values.push_back(z.imag());
         }
     }
+    // plane-wave rules (ring cosines, weights, Legendre rows up to each rule's largest
+    // bandlimit) and the diagonal translations of every (level, cell offset)
+    std::vector<dev::HfRule> hf_rules;
+    std::vector<double> hf_polar, hf_legendre;
+    for (std::size_t r = 0; r < rules.size(); ++r)
+        append_hf_rule(rules[r], rule_bandlimit[r], hf_rules, hf_polar, hf_legendre);
+    std::vector<double> diag_table(2 * diag_values);
+    parallel_for(diag_keys.size(), [&](std::size_t i) {
+        const auto &[l, off] = diag_keys[i];
+        const double unit = 2 * half(l);
+        const auto T = diagonal_translation(rules[rule_index[l]], translation_order[l], k,
+                                            {off[0] * unit, off[1] * unit, off[2] * unit});
+        const std::uint64_t base = diag_index.at(diag_keys[i]);
+        for (std::size_t q = 0; q < T.size(); ++q) {
+            diag_table[2 * (base + q)] = T[q].real();
+            diag_table[2 * (base + q) + 1] = T[q].imag();
+        }
+    });
+    st.diagonal_classes = diag_keys.size();
     st.plans_ms = ms_since(t0);
 
     // ---- device data
@@ -433,6 +734,16 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
     upload_nonempty(op->azimuth_, azimuth, s);
     upload_nonempty(op->axial_, axial_values, s);
     upload_nonempty(op->axial_offsets_, axial_offsets, s);
+    upload_nonempty(op->hf_rules_, hf_rules, s);
+    upload_nonempty(op->hf_polar_, hf_polar, s);
+    upload_nonempty(op->hf_legendre_, hf_legendre, s);
+    upload_nonempty(op->hf_links_, hf_links, s);
+    upload_nonempty(op->hf_destinations_, hf_destinations, s);
+    upload_nonempty(op->diag_links_, diag_links, s);
+    upload_nonempty(op->diag_destinations_, diag_destinations, s);
+    upload_nonempty(op->diag_table_, diag_table, s);
+    op->hf_fourier_.resize(2 * std::max<std::uint64_t>(hf_fourier_size, 1));
+    op->hf_harmonic_.resize(2 * std::max<std::uint64_t>(hf_harmonic_size, 1));
     DeviceBuffer<int> d_err(1);
     hip_check(hipMemsetAsync(d_err.data(), 0, sizeof(int), s), "clear error");
 
@@ -491,7 +802,8 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
     st.device_bytes = op->near_->values.bytes() + ns->columns.bytes() + op->emission_.bytes() +
                       op->testing_.bytes() + op->pool_.bytes() + op->small_d_.bytes() + op->azimuth_.bytes() +
                       op->axial_.bytes() + op->jacobi_->bytes() + op->links_.bytes() + op->destinations_.bytes() +
-                      op->work_y_.bytes() + op->work_z_.bytes();
+                      op->work_y_.bytes() + op->work_z_.bytes() + op->hf_legendre_.bytes() + op->diag_table_.bytes() +
+                      op->hf_fourier_.bytes() + op->hf_harmonic_.bytes();
     return op;
 }
 
@@ -509,16 +821,43 @@ void FmmOperator::apply(const double *x, double *y, hipStream_t s) const {
     T.axial_offsets = axial_offsets_.data();
     T.source_expansions = pool_.data();
     T.target_expansions = pool_.data();
-    for (const auto *list : {&upward_, &downward_})
-        for (const Launch &L : *list) {
+    dev::HfTransferLaunch H{};
+    H.rules = hf_rules_.data();
+    H.legendre = hf_legendre_.data();
+    H.polar = hf_polar_.data();
+    H.fourier_ws = hf_fourier_.data();
+    H.harmonic_ws = hf_harmonic_.data();
+    H.source_expansions = pool_.data();
+    H.target_expansions = pool_.data();
+    dev::HfDiagonalLaunch D{};
+    D.coefficients = diag_table_.data();
+    D.source_expansions = pool_.data();
+    D.target_expansions = pool_.data();
+    for (const Stage &stage : stages_) {
+        if (stage.kind == StageKind::translate) {
+            const Launch &L = translate_[stage.index];
             T.destinations = destinations_.data() + L.first_destination;
             T.links = links_.data() + L.first_link;
             T.order = link_order_.data() + L.first_link;
             T.chunks = chunks_.data() + L.first_chunk;
             T.axial_order = axial_order_.data() + L.first_link;
             T.axial_chunks = axial_chunks_.data() + L.first_axial_chunk;
-            dev::launch_translate(T, L.destination_count, L.link_count, L.chunk_count, L.axial_chunk_count, s);
+            dev::launch_translate(T, L.destination_count, L.link_count, L.chunk_count, L.axial_chunk_count, L.max_order,
+                                  s);
+        } else if (stage.kind == StageKind::angular) {
+            const HfLaunch &A = angular_[stage.index];
+            H.links = hf_links_.data() + A.first_link;
+            H.destinations = hf_destinations_.data() + A.first_destination;
+            dev::launch_hf_transfer(H, A.kind, A.link_count, A.destination_count, A.in_rings, A.out_rings, A.out_samples,
+                                    s);
+        } else {
+            const DiagonalLaunch &G = diagonal_[stage.index];
+            D.destinations = diag_destinations_.data() + G.first_destination;
+            D.links = diag_links_.data() + G.first_link;
+            D.samples = G.samples;
+            dev::launch_hf_diagonal(D, G.destination_count, s);
         }
+    }
     dev::launch_leaf_test(leaves_.data(), leaf_count_, leaf_dofs_.data(), testing_.data(), pool_.data(), y, s);
 }
 

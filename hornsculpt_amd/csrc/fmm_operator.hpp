@@ -19,6 +19,7 @@
 #include "host/support_tree.hpp"
 #include "solver.hpp"
 
+#include <complex>
 #include <cstdint>
 #include <vector>
 
@@ -33,7 +34,8 @@ struct FmmStatistics {
     std::uint32_t nodes{}, levels{}, leaves{}, top_level{}, rule_points{};
     std::uint64_t near_nnz{}, far_pairs{}, m2m_links{}, m2l_links{}, l2l_links{}, rotations{}, axial_plans{};
     std::uint64_t expansion_values{}, moment_values{}, device_bytes{};
-    std::vector<std::uint32_t> level_orders;
+    std::uint64_t plane_wave_levels{}, angular_links{}, diagonal_classes{};
+    std::vector<std::uint32_t> level_orders, level_representation, level_rule_orders; // representation 1 LF, 2 HF
     double tree_ms{}, near_ms{}, plans_ms{}, moments_ms{}, near_kernel_ms{};
 };
 
@@ -78,7 +80,7 @@ class FmmOperator {
     friend class MeshSolver;
     struct Launch {
         std::uint32_t first_destination{}, destination_count{}, first_link{}, link_count{}, first_chunk{},
-            chunk_count{}, first_axial_chunk{}, axial_chunk_count{};
+            chunk_count{}, first_axial_chunk{}, axial_chunk_count{}, max_order{};
     };
     std::uint32_t n_{};
     FmmStatistics stats_;
@@ -96,11 +98,46 @@ class FmmOperator {
     DeviceBuffer<dev::FmmRotation> rotations_;
     DeviceBuffer<double> small_d_, azimuth_, axial_;
     DeviceBuffer<std::uint64_t> axial_offsets_;
-    std::vector<Launch> upward_, downward_;
+    struct HfLaunch {
+        dev::HfTransferKind kind{};
+        std::uint32_t first_link{}, link_count{}, first_destination{}, destination_count{}, in_rings{}, out_rings{},
+            out_samples{};
+    };
+    struct DiagonalLaunch {
+        std::uint32_t first_destination{}, destination_count{}, first_link{};
+        std::uint64_t samples{};
+    };
+    enum class StageKind : std::uint32_t { translate, angular, diagonal };
+    struct Stage {
+        StageKind kind;
+        std::uint32_t index;
+    };
+    std::vector<Launch> translate_;
+    std::vector<HfLaunch> angular_;
+    std::vector<DiagonalLaunch> diagonal_;
+    std::vector<Stage> stages_; // upward levels coarseward, then downward levels leafward
+    DeviceBuffer<dev::HfRule> hf_rules_;
+    DeviceBuffer<double> hf_polar_, hf_legendre_, diag_table_;
+    DeviceBuffer<dev::HfLink> hf_links_;
+    DeviceBuffer<dev::FmmDestination> hf_destinations_, diag_destinations_;
+    DeviceBuffer<dev::HfDiagonalLink> diag_links_;
+    mutable DeviceBuffer<double> hf_fourier_, hf_harmonic_;
     std::uint32_t leaf_count_{};
     mutable DeviceBuffer<double> graph_x_, graph_y_;
     mutable hipGraph_t graph_{};
     mutable hipGraphExec_t graph_exec_{};
 };
+
+// One plane-wave transfer on the device through the operator's angular kernels
+// (AngularTransferOperator::outgoingToParentDevice / incomingToChildDevice /
+// scaledOutgoingToParentDevice / incomingToScaledChildDevice,
+// modules/bem/cuda/wideband_operator.hpp:39-51). Orders: rule orders of the source
+// and target signatures, or the spherical order (= bandlimit) of a hand-off's
+// scaled expansion; radius: its basis radius.
+std::vector<std::complex<double>> device_angular_transfer(dev::HfTransferKind kind, unsigned source_order,
+                                                          unsigned target_order, unsigned bandlimit, double k,
+                                                          double radius, Vec3 child_minus_parent,
+                                                          const std::vector<std::complex<double>> &input,
+                                                          hipStream_t stream);
 
 } // namespace hsbem

@@ -489,6 +489,149 @@ __global__ __launch_bounds__(256) void coaxial_links(FmmTranslateLaunch L) {
     }
 }
 
+// Orders above kMaxFmmOrder (up to kMaxWideOrder): the same two passes with the
+// Wigner block staged in column (forward) or row (back) tiles of kWideTile and no
+// register prefetch; one workgroup per chunk holds ~100 KB of LDS.
+constexpr int kWideN = 2 * kMaxWideOrder + 1;
+constexpr int kWideTile = kMaxWideOrder + 1;
+
+__global__ __launch_bounds__(256) void rotate_links_wide(FmmTranslateLaunch L, bool back) {
+    __shared__ double dt[kWideN * kWideTile];
+    __shared__ C2 xs[kFmmChunk][kWideN];
+    __shared__ C2 ph[kWideN];
+    __shared__ std::uint64_t src_off[kFmmChunk], dst_off[kFmmChunk];
+    __shared__ int ord[kFmmChunk];
+    const FmmChunk ch = L.chunks[blockIdx.x];
+    const FmmRotation rot = L.rotations[ch.rotation];
+    const C2 *az = reinterpret_cast<const C2 *>(L.azimuth) + rot.azimuth;
+    for (int m = threadIdx.x; m <= 2 * static_cast<int>(rot.order); m += blockDim.x)
+        ph[m] = az[m];
+    if (threadIdx.x < ch.count) {
+        const FmmLink lk = L.links[L.order[ch.first + threadIdx.x]];
+        src_off[threadIdx.x] = back ? lk.slot : lk.source;
+        dst_off[threadIdx.x] = lk.slot;
+        ord[threadIdx.x] = static_cast<int>(back ? lk.target_order : lk.source_order);
+    }
+    __syncthreads();
+    const C2 *in = reinterpret_cast<const C2 *>(back ? L.work_z : L.source_expansions);
+    C2 *out = reinterpret_cast<C2 *>(L.work_y);
+    const int count = static_cast<int>(ch.count), P = static_cast<int>(rot.order), top = static_cast<int>(ch.max_order);
+    const int groups = (count + 3) / 4;
+    for (int l = 0; l <= top; ++l) {
+        const int n = 2 * l + 1;
+        const std::size_t base = static_cast<std::size_t>(l) * l;
+        const double *dl = L.small_d + rot.small_d + static_cast<std::size_t>(l) * (2 * l - 1) * (2 * l + 1) / 3;
+        __syncthreads();
+        for (int e = threadIdx.x; e < count * n; e += blockDim.x) {
+            const int j = e / n, q = e % n;
+            C2 v{0, 0};
+            if (l <= ord[j]) {
+                v = in[src_off[j] + base + q];
+                if (!back)
+                    v = ph[P + q - l] * v;
+            }
+            xs[j][q] = v;
+        }
+        for (int t0 = 0; t0 < n; t0 += kWideTile) {
+            const int tw = min(kWideTile, n - t0);
+            __syncthreads();
+            // forward: dt[q][t - t0] = d[q][t]; back: dt[t - t0][q] = d[t][q]
+            for (int e = threadIdx.x; e < n * tw; e += blockDim.x) {
+                if (back) {
+                    const int t = e / n, q = e % n;
+                    dt[t * n + q] = dl[static_cast<std::size_t>(t0 + t) * n + q];
+                } else {
+                    const int q = e / tw, t = e % tw;
+                    dt[q * tw + t] = dl[static_cast<std::size_t>(q) * n + t0 + t];
+                }
+            }
+            __syncthreads();
+            for (int e = threadIdx.x; e < groups * tw; e += blockDim.x) {
+                const int g = e / tw, tt = e % tw, t = t0 + tt, j0 = 4 * g;
+                C2 s0{0, 0}, s1{0, 0}, s2{0, 0}, s3{0, 0};
+                const int stride = back ? 1 : tw;
+                const double *drow = back ? dt + tt * n : dt + tt;
+                for (int q = 0; q < n; ++q) {
+                    const double d = drow[q * stride];
+                    s0 = s0 + d * xs[j0][q];
+                    s1 = s1 + d * xs[min(j0 + 1, kFmmChunk - 1)][q];
+                    s2 = s2 + d * xs[min(j0 + 2, kFmmChunk - 1)][q];
+                    s3 = s3 + d * xs[min(j0 + 3, kFmmChunk - 1)][q];
+                }
+                const C2 sv[4] = {s0, s1, s2, s3};
+                const C2 rot_back = back ? conj(ph[P + t - l]) : C2{1, 0};
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int j = j0 + i;
+                    if (j < count && l <= ord[j])
+                        out[dst_off[j] + base + t] = back ? rot_back * sv[i] : sv[i];
+                }
+            }
+        }
+    }
+}
+
+constexpr int kWideRows = kMaxWideOrder + 1;
+
+__global__ __launch_bounds__(256) void coaxial_links_wide(FmmTranslateLaunch L) {
+    __shared__ C2 tm[kWideRows * kWideRows];
+    __shared__ C2 ys[kFmmChunk][2][kWideRows];
+    __shared__ std::uint64_t slot[kFmmChunk];
+    const FmmChunk ch = L.axial_chunks[blockIdx.x];
+    const FmmLink first = L.links[L.axial_order[ch.first]];
+    const int ps = static_cast<int>(first.source_order), pt = static_cast<int>(first.target_order), mm = min(ps, pt);
+    const C2 *T = reinterpret_cast<const C2 *>(L.axial) + L.axial_offsets[first.axial];
+    const C2 *y = reinterpret_cast<const C2 *>(L.work_y);
+    C2 *z = reinterpret_cast<C2 *>(L.work_z);
+    const int count = static_cast<int>(ch.count);
+    if (threadIdx.x < count)
+        slot[threadIdx.x] = L.links[L.axial_order[ch.first + threadIdx.x]].slot;
+    const int nct = (pt + 1) * (pt + 1);
+    __syncthreads();
+    for (int e = threadIdx.x; e < count * nct; e += blockDim.x) {
+        const int j = e / nct, c = e % nct;
+        int l, t;
+        split_index(c, l, t);
+        if (abs(t - l) > mm)
+            z[slot[j] + c] = {0, 0};
+    }
+    std::size_t off = 0;
+    const int groups = (count + 3) / 4;
+    for (int m = 0; m <= mm; ++m) {
+        const int rows = pt - m + 1, cols = ps - m + 1;
+        __syncthreads();
+        for (int e = threadIdx.x; e < rows * cols; e += blockDim.x)
+            tm[e] = T[off + e];
+        for (int e = threadIdx.x; e < kFmmChunk * 2 * cols; e += blockDim.x) {
+            const int j = e / (2 * cols), r = e % (2 * cols), sgn = r / cols, n = m + r % cols;
+            ys[j][sgn][r % cols] = j < count ? y[slot[j] + static_cast<std::size_t>(n) * n + n + (sgn ? -m : m)]
+                                             : C2{0, 0};
+        }
+        __syncthreads();
+        off += static_cast<std::size_t>(rows) * cols;
+        for (int e = threadIdx.x; e < groups * 2 * rows; e += blockDim.x) {
+            const int g = e / (2 * rows), r = e % (2 * rows), sgn = r / rows, l = m + r % rows, j0 = 4 * g;
+            if (sgn && !m)
+                continue;
+            C2 s0{0, 0}, s1{0, 0}, s2{0, 0}, s3{0, 0};
+            const C2 *trow = tm + (l - m) * cols;
+            for (int c = 0; c < cols; ++c) {
+                const C2 tv = trow[c];
+                s0 = s0 + tv * ys[j0][sgn][c];
+                s1 = s1 + tv * ys[j0 + 1][sgn][c];
+                s2 = s2 + tv * ys[j0 + 2][sgn][c];
+                s3 = s3 + tv * ys[j0 + 3][sgn][c];
+            }
+            const C2 sv[4] = {s0, s1, s2, s3};
+            const std::size_t idx = static_cast<std::size_t>(l) * l + l + (sgn ? -m : m);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (j0 + i < count)
+                    z[slot[j0 + i] + idx] = sv[i];
+        }
+    }
+}
+
 // target = sum of the destination's link results.
 __global__ __launch_bounds__(256) void gather_links(FmmTranslateLaunch L) {
     const FmmDestination dst = L.destinations[blockIdx.x];
@@ -653,10 +796,15 @@ void launch_leaf_test(const FmmLeaf *leaves, std::uint32_t leaf_count, const std
 }
 
 void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destination_count, std::uint32_t link_count,
-                      std::uint32_t chunk_count, std::uint32_t axial_chunk_count, hipStream_t stream) {
+                      std::uint32_t chunk_count, std::uint32_t axial_chunk_count, std::uint32_t max_order,
+                      hipStream_t stream) {
     if (!destination_count)
         return;
-    if (link_count) {
+    if (link_count && max_order > kMaxFmmOrder) {
+        hipLaunchKernelGGL(rotate_links_wide, dim3(chunk_count), dim3(256), 0, stream, launch, false);
+        hipLaunchKernelGGL(coaxial_links_wide, dim3(axial_chunk_count), dim3(256), 0, stream, launch);
+        hipLaunchKernelGGL(rotate_links_wide, dim3(chunk_count), dim3(256), 0, stream, launch, true);
+    } else if (link_count) {
         hipLaunchKernelGGL(rotate_links, dim3(chunk_count), dim3(256), 0, stream, launch, false);
         hipLaunchKernelGGL(coaxial_links, dim3(axial_chunk_count), dim3(256), 0, stream, launch);
         hipLaunchKernelGGL(rotate_links, dim3(chunk_count), dim3(256), 0, stream, launch, true);

@@ -11,7 +11,8 @@ namespace hsbem::dev {
 
 inline constexpr int kMaxLeafOrder = 64;   // expansion order at the leaves
 inline constexpr int kMaxLeafDofs = 512;   // P1 DOFs of one leaf
-inline constexpr int kMaxFmmOrder = 40;    // any translated expansion (LDS stage of d_l: (2p+1)^2 doubles)
+inline constexpr int kMaxFmmOrder = 40;    // translations with the whole d_l block staged: (2p+1)^2 doubles
+inline constexpr int kMaxWideOrder = 64;   // any translated expansion (tiled d_l stage above kMaxFmmOrder)
 inline constexpr int kFmmChunk = 16;       // links per rotation chunk
 inline constexpr int kFmmJacobi = 64;      // block width of the near block-Jacobi blocks
 
@@ -111,6 +112,69 @@ struct FmmFarFluxLaunch {
     double *out;                         // nv x excitations (complex), accumulated
 };
 
+// ---- plane-wave levels (fmm_hf.hip)
+
+inline constexpr int kMaxHfAzimuths = 2048; // azimuths of a plane-wave rule: 2 (order + 1), order <= 1023
+
+// Sphere rule of a plane-wave level: rings x azimuths samples, polar-major.
+struct HfRule {
+    std::uint64_t legendre;          // doubles: ring r, degree l, order m >= 0 at r * T + l(l+1)/2 + m,
+                                     // T = (bmax+1)(bmax+2)/2: orthonormal P_lm(cos theta) (Y_lm at phi = 0)
+    std::uint64_t polar;             // doubles: per ring (cos theta, sin theta, sample weight)
+    std::uint32_t rings, azimuths, bmax, pad;
+};
+
+// One angular transfer into a destination. Offsets count complex values.
+struct HfLink {
+    std::uint64_t source;            // pool offset of the source signature or scaled multipole
+    std::uint64_t target;            // pool offset of the scaled local (down hand-off), else unused
+    std::uint64_t fourier;           // workspace: max(in, out rings) x (2 order + 1)
+    std::uint64_t harmonic;          // workspace: (order + 1)^2
+    double shift[3];                 // +k (c_child - c_parent) upward, -k (c_child - c_parent) downward
+    double ka;                       // k a of the scaled spherical level of a hand-off
+    std::uint32_t in_rule, out_rule; // rules of the source and destination signatures
+    std::uint32_t order;             // bandlimit (the spherical order of a hand-off)
+    std::uint32_t pad;
+};
+
+enum class HfTransferKind : std::uint32_t {
+    up_handoff = 0,    // scaled multipole of a spherical child -> plane-wave parent signature
+    up_resample = 1,   // plane-wave child -> plane-wave parent signature
+    down_handoff = 2,  // plane-wave parent local -> scaled local of a spherical child (added)
+    down_resample = 3, // plane-wave parent local -> plane-wave child local (added)
+};
+
+struct HfTransferLaunch {
+    const HfRule *rules;
+    const double *legendre, *polar;
+    const HfLink *links;               // of this launch, in destination order
+    const FmmDestination *destinations; // target = pool offset of the destination signature
+    double *fourier_ws, *harmonic_ws;
+    const double *source_expansions;
+    double *target_expansions;
+};
+
+struct HfDiagonalLink {
+    std::uint64_t source;       // pool offset of the source signature
+    std::uint64_t coefficients; // offset of the direction class's diagonal translation
+};
+
+struct HfDiagonalLaunch {
+    const FmmDestination *destinations; // target: pool offset of the destination local signature (overwritten)
+    const HfDiagonalLink *links;
+    const double *coefficients;         // complex
+    std::uint64_t samples;              // samples of the level's rule
+    const double *source_expansions;
+    double *target_expansions;
+};
+
+// in_rings / out_rings: the largest ring counts of the links' input / output rules;
+// out_samples: samples of the destinations' rule.
+void launch_hf_transfer(const HfTransferLaunch &launch, HfTransferKind kind, std::uint32_t link_count,
+                        std::uint32_t destination_count, std::uint32_t in_rings, std::uint32_t out_rings,
+                        std::uint32_t out_samples, hipStream_t stream);
+void launch_hf_diagonal(const HfDiagonalLaunch &launch, std::uint32_t destination_count, hipStream_t stream);
+
 void launch_leaf_moments(const FmmLeafMomentsLaunch &launch, hipStream_t stream);
 void launch_far_flux(const FmmFarFluxLaunch &launch, hipStream_t stream);
 // out[:, e] += B_near q_e (q: faces x excitations complex).
@@ -124,8 +188,11 @@ void launch_leaf_emit(const FmmLeaf *leaves, std::uint32_t leaf_count, const std
                       const double *emission, const double *p, double *expansions, hipStream_t stream);
 void launch_leaf_test(const FmmLeaf *leaves, std::uint32_t leaf_count, const std::uint32_t *leaf_dofs,
                       const double *testing, const double *expansions, double *y, hipStream_t stream);
+// max_order: largest source or target order of the launch's links (selects the tiled
+// kernels above kMaxFmmOrder; at most kMaxWideOrder).
 void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destination_count, std::uint32_t link_count,
-                      std::uint32_t chunk_count, std::uint32_t axial_chunk_count, hipStream_t stream);
+                      std::uint32_t chunk_count, std::uint32_t axial_chunk_count, std::uint32_t max_order,
+                      hipStream_t stream);
 // blocks (zeroed, n rounded up to kFmmJacobi) from the near CSR in tree order;
 // position[dof] = index of dof in order.
 void launch_gather_blocks(const std::uint32_t *row_offsets, const std::uint32_t *columns, const double *values,

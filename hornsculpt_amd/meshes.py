@@ -93,6 +93,19 @@ class _Builder:
             self.vertices.append((x, r * math.cos(a), r * math.sin(a)))
         return list(range(start, start + n))
 
+    def curve_ring(self, x, radius_of, n, phase=0.0):
+        """Ring of n points at axial x, equally spaced in arc length along the
+        star-shaped closed curve r = radius_of(angle) in the (y, z) plane."""
+        a = np.linspace(0.0, 2 * math.pi, 4097)
+        r = radius_of(a)
+        y, z = r * np.cos(a), r * np.sin(a)
+        s = np.concatenate([[0.0], np.cumsum(np.hypot(np.diff(y), np.diff(z)))])
+        at = (np.arange(n) + phase) / n * s[-1]
+        ys, zs = np.interp(at, s, y), np.interp(at, s, z)
+        start = len(self.vertices)
+        self.vertices.extend((x, float(u), float(w)) for u, w in zip(ys, zs))
+        return list(range(start, start + n))
+
     def point(self, p):
         self.vertices.append(tuple(p))
         return len(self.vertices) - 1
@@ -174,15 +187,23 @@ def exponential_horn(h: float, throat_radius: float = 0.0127, mouth_radius: floa
     def rin(x):
         return throat_radius * math.exp(m * x)
 
-    # inner flare rings placed by arc length ~h
-    xs = [0.0]
-    while xs[-1] < length:
-        x = xs[-1]
-        slope = m * rin(x)
-        dx = h / math.sqrt(1 + slope * slope)
-        xs.append(min(length, x + dx))
-    if len(xs) > 2 and xs[-1] - xs[-2] < 0.3 * h:
-        xs.pop(-2)
+    # inner flare rings placed by arc length ~h (the flanged variant: equal arc
+    # lengths, so no interval exceeds h)
+    if flange > 0:
+        grid = np.linspace(0.0, length, 20001)
+        arc = np.concatenate([[0.0], np.cumsum(np.hypot(np.diff(grid), np.diff(throat_radius * np.exp(m * grid))))])
+        xs = [float(v) for v in np.interp(np.linspace(0.0, arc[-1], max(1, int(math.ceil(arc[-1] / h))) + 1),
+                                          arc, grid)]
+        xs[-1] = length
+    else:
+        xs = [0.0]
+        while xs[-1] < length:
+            x = xs[-1]
+            slope = m * rin(x)
+            dx = h / math.sqrt(1 + slope * slope)
+            xs.append(min(length, x + dx))
+        if len(xs) > 2 and xs[-1] - xs[-2] < 0.3 * h:
+            xs.pop(-2)
     inner = []
     for k, x in enumerate(xs):
         inner.append(b.ring(x, rin(x), _ring_count(rin(x), h), phase=0.5 * (k % 2)))
@@ -192,18 +213,31 @@ def exponential_horn(h: float, throat_radius: float = 0.0127, mouth_radius: floa
     _disk(b, 0.0, inner[0], rin(0.0), h, throat_face, DRIVEN, facing_negative=False)
     # outer surface: offset radius r + wall (approximately normal offset), from mouth back to the rear
     rout = lambda x: rin(x) + wall * math.sqrt(1 + (m * rin(x)) ** 2)
-    mouth_out = b.ring(length, rout(length), _ring_count(rout(length), h), phase=0.25)
-    # rim annulus between inner mouth ring and outer mouth ring (facing +x)
-    ann = [inner[-1]]
-    layers = max(1, int(round((rout(length) - rin(length)) / h)))
-    for k in range(1, layers):
-        rr = rin(length) + (rout(length) - rin(length)) * k / layers
-        ann.append(b.ring(length, rr, _ring_count(rr, h), phase=0.5 * (k % 2)))
-    ann.append(mouth_out)
-    for k in range(len(ann) - 1):
-        b.strip(ann[k], ann[k + 1], wall_face, WALL, flip=True)
-    outer = [mouth_out]
-    for k, x in enumerate(reversed(xs[:-1])):
+    if flange > 0:
+        if 0.5 * flange <= rout(length) + 2 * h:
+            raise ValueError("The flange must extend beyond the horn's outer wall.")
+        xb = length - wall
+        back = b.ring(xb, rout(xb), _ring_count(rout(xb), h), phase=0.25)
+        _flange(b, inner[-1], rin(length), back, rout(xb), length, wall, 0.5 * flange, h, wall_face)
+        outer = [back]
+        grid = np.linspace(0.0, xb, 20001)
+        arc = np.concatenate([[0.0], np.cumsum(np.hypot(np.diff(grid), np.diff([rout(v) for v in grid])))])
+        count = max(1, int(math.ceil(arc[-1] / h)))
+        outer_x = [float(v) for v in np.interp(np.linspace(0.0, arc[-1], count + 1), arc, grid)][count - 1::-1]
+    else:
+        mouth_out = b.ring(length, rout(length), _ring_count(rout(length), h), phase=0.25)
+        # rim annulus between inner mouth ring and outer mouth ring (facing +x)
+        ann = [inner[-1]]
+        layers = max(1, int(round((rout(length) - rin(length)) / h)))
+        for k in range(1, layers):
+            rr = rin(length) + (rout(length) - rin(length)) * k / layers
+            ann.append(b.ring(length, rr, _ring_count(rr, h), phase=0.5 * (k % 2)))
+        ann.append(mouth_out)
+        for k in range(len(ann) - 1):
+            b.strip(ann[k], ann[k + 1], wall_face, WALL, flip=True)
+        outer = [mouth_out]
+        outer_x = list(reversed(xs[:-1]))
+    for k, x in enumerate(outer_x):
         outer.append(b.ring(x, rout(x), _ring_count(rout(x), h), phase=0.25 + 0.5 * (k % 2)))
     # rear extension of the outer wall to the closure
     xr = -rear
@@ -214,10 +248,48 @@ def exponential_horn(h: float, throat_radius: float = 0.0127, mouth_radius: floa
     for k in range(len(outer) - 1):
         b.strip(outer[k], outer[k + 1], wall_face, WALL, flip=True)
     _disk(b, xr, outer[-1], rout(0.0), h, wall_face, WALL, facing_negative=True)
-    mesh = b.mesh()
-    if flange > 0:
-        raise NotImplementedError("flanged horn is built by exponential_horn_flanged")
-    return mesh
+    return b.mesh()
+
+
+def _flange(b: _Builder, mouth, r_mouth, back, r_back, x, wall, half, h, face):
+    """Square rigid baffle (side 2*half, thickness wall) in the mouth plane x: the
+    front face (facing +x) from the inner mouth ring out to the square, the four
+    side faces, and the back face (facing -x, at x - wall) from the square in to
+    the outer horn wall's ring ``back``. Face rings morph from the circle to the
+    square, r(t, a) = (1 - t) r0 + t half / max(|cos a|, |sin a|), spaced ~h in arc
+    length; the square rings hold 8q points, so the four corners are vertices."""
+
+    def square(a):
+        return half / np.maximum(np.abs(np.cos(a)), np.abs(np.sin(a)))
+
+    def morph(r0, t):
+        return lambda a: (1 - t) * r0 + t * square(a)
+
+    def count(r0, t):
+        return max(8, int(round((2 * math.pi * r0 * (1 - t) + 8 * half * t) / h)))
+
+    n_square = 8 * max(1, int(round(half / h)))
+
+    def annulus(x_face, circle, r0, edge, outward):
+        # rings between the circle (t = 0) and the square edge (t = 1)
+        layers = max(1, int(round((math.sqrt(2) * half - r0) / h)))  # ~h at the corners
+        rings = [circle]
+        for k in range(1, layers):
+            t = k / layers
+            rings.append(b.curve_ring(x_face, morph(r0, t), count(r0, t), phase=0.5 * (k % 2)))
+        rings.append(edge)
+        if not outward:
+            rings.reverse()
+        for k in range(len(rings) - 1):
+            b.strip(rings[k], rings[k + 1], face, WALL, flip=True)
+
+    edge_front = b.curve_ring(x, square, n_square)
+    annulus(x, mouth, r_mouth, edge_front, outward=True)
+    steps = max(1, int(round(wall / h)))
+    side = [edge_front] + [b.curve_ring(x - wall * k / steps, square, n_square) for k in range(1, steps + 1)]
+    for k in range(steps):
+        b.strip(side[k], side[k + 1], face, WALL, flip=True)
+    annulus(x - wall, back, r_back, side[-1], outward=False)
 
 
 def analytic_sphere_pressure(frequency: float, radius_observer: float, density: float = 1.21, c: float = 343.0,

@@ -332,6 +332,12 @@ HSBEM_API int hsbem_fmm_operator_time_apply(hsbem_fmm_operator *op, uint32_t rep
  * orders[level_count] = expansion order per level (0: inactive). */
 HSBEM_API int hsbem_fmm_operator_statistics(const hsbem_fmm_operator *op, uint64_t *counts, double *times,
                                             uint32_t *orders, uint32_t *level_count);
+/* Plane-wave levels of an FMM operator: counts[4] = plane-wave levels, angular
+ * transfer links, diagonal translation classes, total levels; per level
+ * representation[level_count] (0 inactive, 1 scaled spherical, 2 plane wave) and
+ * rule_orders[level_count] (sphere rule order of a plane-wave level, else 0). */
+HSBEM_API int hsbem_fmm_operator_levels(const hsbem_fmm_operator *op, uint64_t *counts, uint32_t *representation,
+                                        uint32_t *rule_orders, uint32_t *level_count);
 /* Route later applies and solves through one captured graph of the whole apply
  * (replaces the M2L chunk graph replay, support_m2l_graph.hpp:13-91, and
  * support_fmm_operator.cu:865-1027): captured on first use, replayed after. */
@@ -426,6 +432,20 @@ HSBEM_API int hsbem_scaled_expansion_evaluate(uint32_t order, double radius, dou
  * scaled_translation.hpp:37-40, on the canonical rule of rule_order. */
 HSBEM_API int hsbem_scaled_spectrum(uint32_t order, double radius, double k, uint32_t rule_order, int32_t direction,
                                     const double *in, double *out);
+#define HSBEM_DEVICE_SCALED_TO_PARENT 0
+#define HSBEM_DEVICE_OUTGOING_TO_PARENT 1
+#define HSBEM_DEVICE_INCOMING_TO_SCALED_CHILD 2
+#define HSBEM_DEVICE_INCOMING_TO_CHILD 3
+/* One plane-wave transfer on the device, through the FMM operator's angular kernels
+ * (AngularTransferOperator::scaledOutgoingToParentDevice / outgoingToParentDevice /
+ * incomingToScaledChildDevice / incomingToChildDevice, wideband_operator.hpp:39-51,
+ * wideband_angular.cu:479-505). source_order / target_order: rule orders of the
+ * signatures, or the spherical order (= bandlimit) of a hand-off's scaled
+ * expansion of basis radius `radius`; shift = child centre - parent centre.
+ * in: rule samples or (order+1)^2 coefficients; out likewise. */
+HSBEM_API int hsbem_angular_transfer_device(hsbem_context *ctx, int32_t kind, uint32_t source_order,
+                                            uint32_t target_order, uint32_t bandlimit, double k, double radius,
+                                            const double *shift, const double *in, double *out);
 #define HSBEM_AXIAL_MULTIPOLE_TO_PARENT 0
 #define HSBEM_AXIAL_MULTIPOLE_TO_LOCAL 1
 #define HSBEM_AXIAL_LOCAL_TO_CHILD 2

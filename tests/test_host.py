@@ -156,6 +156,34 @@ def test_horn_mesh_closed_oriented_and_shape_qualified(edge):
     assert np.all(mesh.boundary[mesh.face_ids == 1] == bem.DRIVEN)
 
 
+@pytest.mark.parametrize("edge", [0.01, 0.0045])
+def test_flanged_horn_mesh_closed_oriented_and_shape_qualified(edge):
+    """The horn on a 0.6 m square flange (5 mm slab in the mouth plane): closed,
+    consistently oriented outward, angles >= 20 degrees and, at the 4.5 mm size
+    (~118k triangles / ~59k nodes), every edge short enough for k h <= 2 at 16 kHz."""
+    mesh = meshes.exponential_horn(edge, flange=0.6)
+    T = mesh.triangles.astype(np.int64)
+    V = mesh.vertices
+    directed = np.concatenate([T[:, [0, 1]], T[:, [1, 2]], T[:, [2, 0]]])
+    keys = directed[:, 0] * len(V) + directed[:, 1]
+    assert len(np.unique(keys)) == len(keys)  # no directed edge twice
+    assert np.all(np.isin(directed[:, 1] * len(V) + directed[:, 0], keys))  # every edge paired, opposite direction
+    volume = np.einsum("ij,ij->i", V[T[:, 0]], np.cross(V[T[:, 1]], V[T[:, 2]])).sum() / 6
+    assert volume > 0
+    P = V[T]
+    cos = []
+    for i in range(3):
+        u, w = P[:, (i + 1) % 3] - P[:, i], P[:, (i + 2) % 3] - P[:, i]
+        cos.append((u * w).sum(1) / np.linalg.norm(u, axis=1) / np.linalg.norm(w, axis=1))
+    assert np.degrees(np.arccos(np.clip(np.max(cos), -1, 1))) >= 20.0
+    assert np.abs(V[:, 1]).max() == pytest.approx(0.3) and np.abs(V[:, 2]).max() == pytest.approx(0.3)
+    assert np.all(mesh.boundary[mesh.face_ids == 1] == bem.DRIVEN)
+    if edge == 0.0045:
+        longest = max(np.linalg.norm(P[:, i] - P[:, (i + 1) % 3], axis=1).max() for i in range(3))
+        assert 2 * math.pi * 16000 / 343 * longest <= 2.0
+        assert 110_000 <= len(T) <= 125_000
+
+
 def test_cpp_facade_compiles_against_c_abi(tmp_path):
     """The header-only C++ drop-in (include/hornsculpt_amd/bem.hpp) compiles and links."""
     import subprocess
