@@ -212,6 +212,7 @@ def library() -> ctypes.CDLL:
                 "hsbem_cancel": (ctypes.c_int, [P, U64]),
                 "hsbem_server_stats": (ctypes.c_int, [P, ctypes.POINTER(_ServerStatistics)]),
                 "hsbem_result_served_latency": (ctypes.c_int, [P, U32, D]),
+                "hsbem_result_fmm": (ctypes.c_int, [P, U32, D]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(lib, name)
@@ -348,6 +349,7 @@ class BemFrequencyResult:
     interaction_plan_milliseconds: float
     kernel_milliseconds: dict
     served_milliseconds: float = 0.0   # serving loop: submission to completion of this point
+    fmm: Optional[dict] = None         # fast multipole form: audit, apply time and operator sizes
 
 
 @dataclasses.dataclass
@@ -589,10 +591,16 @@ def _collect(lib, handle, request: BemRequest) -> BemResult:
                                            refs.value, BemBoundaryData(surface, trace, deriv)))
         served = ctypes.c_double()
         _check(lib.hsbem_result_served_latency(handle, f, ctypes.byref(served)))
+        fm = np.zeros(7)
+        _check(lib.hsbem_result_fmm(handle, f, _dptr(fm)))
+        fmm = None if request.numerics.acceleration != ACCELERATION_FAST_MULTIPOLE else {
+            "audit_discrepancy": float(fm[0]), "apply_milliseconds": float(fm[1]), "corrected": bool(fm[2]),
+            "plane_wave_levels": int(fm[3]), "m2l_links": int(fm[4]), "near_nnz": int(fm[5]),
+            "device_bytes": int(fm[6])}
         freqs.append(BemFrequencyResult(request.frequencies[f], exs, ms[0], ms[1], ms[2], ms[3], ms[4], ms[5],
                                         {"regular": ms[6], "touching": ms[7], "near": ms[8],
                                          "regular_samples": ms[9], "touching_samples": ms[10],
-                                         "near_samples": ms[11]}, served.value))
+                                         "near_samples": ms[11]}, served.value, fmm))
     return BemResult("MI355X (gfx950)", nt.value, nv.value, peak.value, freqs)
 
 
@@ -758,6 +766,7 @@ class HipFmmOperator:
         _check(self._lib.hsbem_fmm_operator_levels(self.handle, counts4.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
                                                    _uptr(rep), _uptr(rules), ctypes.byref(nl)))
         out.update(plane_wave_levels=int(counts4[0]), angular_links=int(counts4[1]), diagonal_classes=int(counts4[2]),
+                   plane_wave_leaves=int(counts4[3]),
                    level_representation=rep[:nl.value].tolist(), level_rule_orders=rules[:nl.value].tolist())
         return out
 

@@ -497,6 +497,20 @@ int hsbem_result_timings(const hsbem_result *r, uint32_t f, double *ms) {
     });
 }
 
+int hsbem_result_fmm(const hsbem_result *r, uint32_t f, double *out) {
+    return guarded([&] {
+        require(r && f < r->result.frequencies.size() && out, "result index out of range");
+        const auto &fr = r->result.frequencies[f];
+        out[0] = fr.fmm_audit;
+        out[1] = fr.fmm_apply_ms;
+        out[2] = fr.fmm_corrected ? 1 : 0;
+        out[3] = static_cast<double>(fr.fmm_plane_wave_levels);
+        out[4] = static_cast<double>(fr.fmm_m2l_links);
+        out[5] = static_cast<double>(fr.fmm_near_nnz);
+        out[6] = static_cast<double>(fr.fmm_device_bytes);
+    });
+}
+
 int hsbem_result_peak_device_bytes(const hsbem_result *r, uint64_t *bytes) {
     return guarded([&] {
         require(r && bytes, "null result");
@@ -811,7 +825,7 @@ int hsbem_fmm_operator_levels(const hsbem_fmm_operator *o, uint64_t *counts, uin
         counts[0] = st.plane_wave_levels;
         counts[1] = st.angular_links;
         counts[2] = st.diagonal_classes;
-        counts[3] = st.level_representation.size();
+        counts[3] = st.plane_wave_leaves;
         if (representation)
             std::copy(st.level_representation.begin(), st.level_representation.end(), representation);
         if (rule_orders)

@@ -182,6 +182,7 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
         fs.accuracy_seed >= 1)
         throw std::invalid_argument("FMM operator needs a positive wavenumber and an accuracy seed in (0, 1).");
     auto op = std::unique_ptr<FmmOperator>(new FmmOperator());
+    op->k_ = k;
     FmmStatistics &st = op->stats_;
     const std::uint32_t nv = this->nv(), nt = this->nt();
     op->n_ = nv;
@@ -285,52 +286,99 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
     }
     st.expansion_values = pool;
 
-    // ---- leaves, their DOFs and (leaf, support panel) items
+    // ---- leaves, their DOFs and (leaf, support panel) items; leaves on plane-wave
+    // levels emit and test their signatures directly
     std::vector<dev::FmmLeaf> leaves;
     std::vector<std::uint32_t> leaf_dofs;
     std::vector<dev::FmmLeafItem> items;
+    std::vector<dev::HfLeaf> hf_leaves;
+    std::vector<dev::HfLeafItem> hf_items;
+    std::vector<double> leaf_scale;
+    std::uint32_t hf_max_samples = 0;
     std::vector<std::int32_t> local(nv, -1);
     std::uint64_t moments = 0;
     for (std::uint32_t id = 0; id < tree.nodes.size(); ++id) {
         const SupportNode &nd = tree.nodes[id];
         if (!nd.leaf() || !nd.pressure_count || order[nd.level] < 0)
             continue;
-        if (plane[nd.level])
-            throw std::runtime_error("FMM leaves on plane-wave levels are not built here.");
-        if (nd.pressure_count > static_cast<std::uint32_t>(dev::kMaxLeafDofs) || order[nd.level] > dev::kMaxLeafOrder)
+        const bool hf = plane[nd.level] != 0;
+        if (nd.pressure_count > static_cast<std::uint32_t>(dev::kMaxLeafDofs) ||
+            (!hf && order[nd.level] > dev::kMaxLeafOrder))
             throw std::runtime_error("FMM leaf exceeds the supported DOF count or order.");
+        const auto first_dof = static_cast<std::uint32_t>(leaf_dofs.size());
+        for (std::uint32_t j = 0; j < nd.pressure_count; ++j) {
+            const std::uint32_t v = tree.pressure_order[nd.first_pressure + j];
+            local[v] = static_cast<std::int32_t>(j);
+            leaf_dofs.push_back(v);
+            leaf_scale.push_back(1.0 / space_.lumped_mass[v]);
+        }
+        auto corners = [&](std::uint32_t t, std::int32_t out[3]) {
+            for (int c = 0; c < 3; ++c) {
+                const std::uint32_t v = space_.panels[t].nodes[c];
+                out[c] = tree.pressure_leaf[v] == id ? local[v] : -1;
+            }
+        };
+        if (hf) {
+            dev::HfLeaf hl{};
+            hl.center[0] = nd.center.x;
+            hl.center[1] = nd.center.y;
+            hl.center[2] = nd.center.z;
+            hl.multipole = mult[id];
+            hl.local = loc[id];
+            hl.rule = rule_index[nd.level];
+            hl.first_item = static_cast<std::uint32_t>(hf_items.size());
+            hl.first_dof = first_dof;
+            hl.dof_count = nd.pressure_count;
+            const auto leaf_index = static_cast<std::uint32_t>(hf_leaves.size());
+            for (std::uint32_t q = plan.node_panel_offsets[id]; q < plan.node_panel_offsets[id + 1]; ++q) {
+                const std::uint32_t t = plan.support_panels[q];
+                const Panel &pn = space_.panels[t];
+                dev::HfLeafItem it{};
+                for (int x = 0; x < 3; ++x) {
+                    const double v0 = x == 0 ? pn.vertices[0].x : x == 1 ? pn.vertices[0].y : pn.vertices[0].z;
+                    const double v1 = x == 0 ? pn.vertices[1].x : x == 1 ? pn.vertices[1].y : pn.vertices[1].z;
+                    const double v2 = x == 0 ? pn.vertices[2].x : x == 1 ? pn.vertices[2].y : pn.vertices[2].z;
+                    it.v0[x] = v0;
+                    it.e1[x] = v1 - v0;
+                    it.e2[x] = v2 - v0;
+                }
+                it.normal[0] = pn.normal.x;
+                it.normal[1] = pn.normal.y;
+                it.normal[2] = pn.normal.z;
+                it.area2 = 2 * pn.area;
+                corners(t, it.corner);
+                it.leaf = leaf_index;
+                hf_items.push_back(it);
+            }
+            hl.item_count = static_cast<std::uint32_t>(hf_items.size()) - hl.first_item;
+            hf_max_samples = std::max(hf_max_samples, static_cast<std::uint32_t>(samples(nd.level)));
+            hf_leaves.push_back(hl);
+            continue;
+        }
         dev::FmmLeaf lf{};
         lf.center[0] = nd.center.x;
         lf.center[1] = nd.center.y;
         lf.center[2] = nd.center.z;
         lf.radius = radius[nd.level];
         lf.order = static_cast<std::uint32_t>(order[nd.level]);
-        lf.first_dof = static_cast<std::uint32_t>(leaf_dofs.size());
+        lf.first_dof = first_dof;
         lf.dof_count = nd.pressure_count;
         lf.emission = moments;
         lf.multipole = mult[id];
         lf.local = loc[id];
         moments += static_cast<std::uint64_t>(nd.pressure_count) * coeffs(nd.level);
-        for (std::uint32_t j = 0; j < nd.pressure_count; ++j) {
-            const std::uint32_t v = tree.pressure_order[nd.first_pressure + j];
-            local[v] = static_cast<std::int32_t>(j);
-            leaf_dofs.push_back(v);
-        }
         const auto leaf_index = static_cast<std::uint32_t>(leaves.size());
         for (std::uint32_t q = plan.node_panel_offsets[id]; q < plan.node_panel_offsets[id + 1]; ++q) {
             const std::uint32_t t = plan.support_panels[q];
             dev::FmmLeafItem it{leaf_index, t, {-1, -1, -1}, 0};
-            for (int c = 0; c < 3; ++c) {
-                const std::uint32_t v = space_.panels[t].nodes[c];
-                if (tree.pressure_leaf[v] == id)
-                    it.corner[c] = local[v];
-            }
+            corners(t, it.corner);
             items.push_back(it);
         }
         leaves.push_back(lf);
     }
     op->leaf_count_ = static_cast<std::uint32_t>(leaves.size());
-    st.leaves = op->leaf_count_;
+    st.leaves = op->leaf_count_ + static_cast<std::uint32_t>(hf_leaves.size());
+    st.plane_wave_leaves = hf_leaves.size();
     st.moment_values = moments;
 
     // ---- translation links
@@ -713,8 +761,15 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
     op->order_.upload(tree.pressure_order, s);
     DeviceBuffer<dev::FmmLeafItem> d_items;
     upload_nonempty(d_items, items, s);
-    DeviceBuffer<double> d_rule;
+    DeviceBuffer<double> &d_rule = op->leaf_rule_;
     d_rule.upload(rule, s);
+    op->hf_leaf_count_ = static_cast<std::uint32_t>(hf_leaves.size());
+    op->hf_leaf_item_count_ = static_cast<std::uint32_t>(hf_items.size());
+    op->hf_max_samples_ = hf_max_samples;
+    upload_nonempty(op->hf_leaves_, hf_leaves, s);
+    upload_nonempty(op->hf_leaf_items_, hf_items, s);
+    upload_nonempty(op->leaf_scale_, leaf_scale, s);
+    op->hf_field_.resize(2 * std::max<std::size_t>(hf_items.size() * st.rule_points, 1));
     op->emission_.resize(2 * std::max<std::uint64_t>(moments, 1));
     op->testing_.resize(2 * std::max<std::uint64_t>(moments, 1));
     op->pool_.resize(2 * std::max<std::uint64_t>(pool, 1));
@@ -811,6 +866,10 @@ void FmmOperator::apply(const double *x, double *y, hipStream_t s) const {
     dev::launch_csr_apply(near_->structure->row_offsets.data(), near_->structure->columns.data(), near_->values.data(),
                           n_, x, y, s);
     dev::launch_leaf_emit(leaves_.data(), leaf_count_, leaf_dofs_.data(), emission_.data(), x, pool_.data(), s);
+    dev::HfLeafLaunch HL{hf_leaves_.data(), hf_leaf_count_, hf_leaf_items_.data(), hf_leaf_item_count_,
+                         leaf_dofs_.data(), leaf_scale_.data(), hf_rules_.data(), hf_polar_.data(), leaf_rule_.data(),
+                         stats_.rule_points, hf_max_samples_, k_, hf_field_.data(), pool_.data()};
+    dev::launch_hf_leaf_emit(HL, x, s);
     dev::FmmTranslateLaunch T{};
     T.rotations = rotations_.data();
     T.work_y = work_y_.data();
@@ -859,6 +918,7 @@ void FmmOperator::apply(const double *x, double *y, hipStream_t s) const {
         }
     }
     dev::launch_leaf_test(leaves_.data(), leaf_count_, leaf_dofs_.data(), testing_.data(), pool_.data(), y, s);
+    dev::launch_hf_leaf_test(HL, y, s);
 }
 
 FmmOperator::~FmmOperator() {
@@ -926,6 +986,10 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
     const std::unique_ptr<FmmOperator> op = build_fmm(num, P.k_max, k, fs, faces, s, nullptr);
     fr.ms[0] = ms_since(t_build);
     fr.near_kernel_ms = op->statistics().near_kernel_ms;
+    fr.fmm_plane_wave_levels = op->statistics().plane_wave_levels;
+    fr.fmm_m2l_links = op->statistics().m2l_links;
+    fr.fmm_near_nnz = op->statistics().near_nnz;
+    fr.fmm_device_bytes = op->statistics().device_bytes;
     {
         std::lock_guard lock(plan_mutex_);
         if (!internal_of_device_.size())

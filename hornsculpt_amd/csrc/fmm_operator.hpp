@@ -34,7 +34,7 @@ struct FmmStatistics {
     std::uint32_t nodes{}, levels{}, leaves{}, top_level{}, rule_points{};
     std::uint64_t near_nnz{}, far_pairs{}, m2m_links{}, m2l_links{}, l2l_links{}, rotations{}, axial_plans{};
     std::uint64_t expansion_values{}, moment_values{}, device_bytes{};
-    std::uint64_t plane_wave_levels{}, angular_links{}, diagonal_classes{};
+    std::uint64_t plane_wave_levels{}, plane_wave_leaves{}, angular_links{}, diagonal_classes{};
     std::vector<std::uint32_t> level_orders, level_representation, level_rule_orders; // representation 1 LF, 2 HF
     double tree_ms{}, near_ms{}, plans_ms{}, moments_ms{}, near_kernel_ms{};
 };
@@ -121,7 +121,12 @@ class FmmOperator {
     DeviceBuffer<dev::HfLink> hf_links_;
     DeviceBuffer<dev::FmmDestination> hf_destinations_, diag_destinations_;
     DeviceBuffer<dev::HfDiagonalLink> diag_links_;
-    mutable DeviceBuffer<double> hf_fourier_, hf_harmonic_;
+    mutable DeviceBuffer<double> hf_fourier_, hf_harmonic_, hf_field_;
+    DeviceBuffer<dev::HfLeaf> hf_leaves_;
+    DeviceBuffer<dev::HfLeafItem> hf_leaf_items_;
+    DeviceBuffer<double> leaf_scale_, leaf_rule_;
+    std::uint32_t hf_leaf_count_{}, hf_leaf_item_count_{}, hf_max_samples_{};
+    double k_{};
     std::uint32_t leaf_count_{};
     mutable DeviceBuffer<double> graph_x_, graph_y_;
     mutable hipGraph_t graph_{};

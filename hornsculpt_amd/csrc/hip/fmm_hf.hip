@@ -230,7 +230,165 @@ __global__ __launch_bounds__(256) void hf_diagonal(HfDiagonalLaunch L) {
     reinterpret_cast<C2 *>(L.target_expansions)[dst.target + s] = total;
 }
 
+// direction of sample s of a rule (ring r = s / n, azimuth a = s % n) and its weight
+__device__ __forceinline__ void sample_direction(const double *polar, const HfRule &R, int s, double d[3], double &w) {
+    const int n = static_cast<int>(R.azimuths), r = s / n, a = s % n;
+    const double mu = polar[R.polar + 3 * r], rho = polar[R.polar + 3 * r + 1];
+    double sn, cs;
+    sincospi(2.0 * a / n, &sn, &cs);
+    d[0] = rho * cs;
+    d[1] = rho * sn;
+    d[2] = mu;
+    w = polar[R.polar + 3 * r + 2];
+}
+
+// Block (sample tile, leaf): thread per sample of the leaf's rule.
+__global__ __launch_bounds__(256) void hf_leaf_emit(HfLeafLaunch L, const double *p) {
+    const HfLeaf leaf = L.leaves[blockIdx.y];
+    const HfRule R = L.rules[leaf.rule];
+    const int samples = static_cast<int>(R.rings * R.azimuths);
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= samples)
+        return;
+    double d[3], w;
+    sample_direction(L.polar, R, s, d, w);
+    const double k = L.k;
+    double re = 0, im = 0;
+    for (std::uint32_t i = 0; i < leaf.item_count; ++i) {
+        const HfLeafItem it = L.items[leaf.first_item + i];
+        C2 pc[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            pc[c] = {0, 0};
+            if (it.corner[c] >= 0) {
+                const std::uint32_t dof = L.leaf_dofs[leaf.first_dof + it.corner[c]];
+                pc[c] = {p[2 * dof], p[2 * dof + 1]};
+            }
+        }
+        const double nd = it.normal[0] * d[0] + it.normal[1] * d[1] + it.normal[2] * d[2];
+        for (std::uint32_t q = 0; q < L.rule_points; ++q) {
+            const double b0 = L.rule[4 * q], b1 = L.rule[4 * q + 1], b2 = L.rule[4 * q + 2];
+            const double wq = L.rule[4 * q + 3] * it.area2 * nd;
+            double arg = 0;
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+                arg += d[x] * (it.v0[x] + b1 * it.e1[x] + b2 * it.e2[x] - leaf.center[x]);
+            double sn, cs;
+            sincos(k * arg, &sn, &cs);
+            const C2 rho = b0 * pc[0] + b1 * pc[1] + b2 * pc[2];
+            const C2 v = C2{cs, sn} * rho;
+            re += wq * v.re;
+            im += wq * v.im;
+        }
+    }
+    // times i k
+    reinterpret_cast<C2 *>(L.expansions)[leaf.multipole + s] = {-k * im, k * re};
+}
+
+// Block per item: g_q = sum_d w_d (n.d - 1) exp(-i k d.(y_q - c)) L(d) for every
+// rule point q of the panel, reduced over the block.
+__global__ __launch_bounds__(256) void hf_leaf_field(HfLeafLaunch L) {
+    constexpr int kPoints = 16;
+    __shared__ double part[2][kPoints][4]; // per point, per wavefront partial (re, im)
+    const HfLeafItem it = L.items[blockIdx.x];
+    const HfLeaf leaf = L.leaves[it.leaf];
+    const HfRule R = L.rules[leaf.rule];
+    const int samples = static_cast<int>(R.rings * R.azimuths), Q = static_cast<int>(L.rule_points);
+    double y[kPoints][3];
+#pragma unroll
+    for (int q = 0; q < kPoints; ++q) {
+        const int qq = q < Q ? q : 0;
+        const double b1 = L.rule[4 * qq + 1], b2 = L.rule[4 * qq + 2];
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+            y[q][x] = it.v0[x] + b1 * it.e1[x] + b2 * it.e2[x] - leaf.center[x];
+    }
+    C2 acc[kPoints];
+#pragma unroll
+    for (int q = 0; q < kPoints; ++q)
+        acc[q] = {0, 0};
+    const C2 *Lc = reinterpret_cast<const C2 *>(L.expansions) + leaf.local;
+    const double k = L.k;
+    for (int s = threadIdx.x; s < samples; s += blockDim.x) {
+        double d[3], w;
+        sample_direction(L.polar, R, s, d, w);
+        const double nd = it.normal[0] * d[0] + it.normal[1] * d[1] + it.normal[2] * d[2];
+        const C2 a = (w * (nd - 1)) * Lc[s];
+#pragma unroll
+        for (int q = 0; q < kPoints; ++q) {
+            if (q < Q) {
+                double sn, cs;
+                sincos(-k * (d[0] * y[q][0] + d[1] * y[q][1] + d[2] * y[q][2]), &sn, &cs);
+                acc[q] = acc[q] + C2{cs, sn} * a;
+            }
+        }
+    }
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+#pragma unroll
+    for (int q = 0; q < kPoints; ++q) {
+        double re = acc[q].re, im = acc[q].im;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            re += __shfl_xor(re, off, 64);
+            im += __shfl_xor(im, off, 64);
+        }
+        if (lane == 0) {
+            part[0][q][wave] = re;
+            part[1][q][wave] = im;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < Q) {
+        const int q = threadIdx.x;
+        double re = 0, im = 0;
+        for (int v = 0; v < static_cast<int>(blockDim.x) / 64; ++v) {
+            re += part[0][q][v];
+            im += part[1][q][v];
+        }
+        reinterpret_cast<C2 *>(L.field)[static_cast<std::size_t>(blockIdx.x) * Q + q] = {re, im};
+    }
+}
+
+// Block per leaf, thread per DOF: y_i += (1/m_i) sum over the leaf's panels holding i
+// of sum_q 2A w_q phi_i(y_q) g_q (panels in item order: deterministic).
+__global__ __launch_bounds__(256) void hf_leaf_gather(HfLeafLaunch L, double *y) {
+    const HfLeaf leaf = L.leaves[blockIdx.x];
+    const C2 *g = reinterpret_cast<const C2 *>(L.field);
+    const int Q = static_cast<int>(L.rule_points);
+    for (std::uint32_t j = threadIdx.x; j < leaf.dof_count; j += blockDim.x) {
+        C2 sum{0, 0};
+        for (std::uint32_t i = 0; i < leaf.item_count; ++i) {
+            const std::uint32_t item = leaf.first_item + i;
+            const HfLeafItem &it = L.items[item];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (it.corner[c] != static_cast<std::int32_t>(j))
+                    continue;
+                for (int q = 0; q < Q; ++q)
+                    sum = sum + (L.rule[4 * q + 3] * it.area2 * L.rule[4 * q + c]) * g[static_cast<std::size_t>(item) * Q + q];
+            }
+        }
+        const std::uint32_t dof = L.leaf_dofs[leaf.first_dof + j];
+        const double scale = L.row_scale[leaf.first_dof + j];
+        y[2 * dof] += scale * sum.re;
+        y[2 * dof + 1] += scale * sum.im;
+    }
+}
+
 } // namespace
+
+void launch_hf_leaf_emit(const HfLeafLaunch &launch, const double *p, hipStream_t stream) {
+    if (launch.leaf_count)
+        hipLaunchKernelGGL(hf_leaf_emit, dim3((launch.max_samples + 255) / 256, launch.leaf_count), dim3(256), 0, stream,
+                           launch, p);
+}
+
+void launch_hf_leaf_test(const HfLeafLaunch &launch, double *y, hipStream_t stream) {
+    if (!launch.leaf_count)
+        return;
+    hipLaunchKernelGGL(hf_leaf_field, dim3(launch.item_count), dim3(256), 0, stream, launch);
+    hipLaunchKernelGGL(hf_leaf_gather, dim3(launch.leaf_count), dim3(256), 0, stream, launch, y);
+}
 
 void launch_hf_transfer(const HfTransferLaunch &launch, HfTransferKind kind, std::uint32_t link_count,
                         std::uint32_t destination_count, std::uint32_t in_rings, std::uint32_t out_rings,

@@ -168,6 +168,47 @@ struct HfDiagonalLaunch {
     double *target_expansions;
 };
 
+// A leaf on a plane-wave level: its signature is emitted and tested directly on the
+// level's sphere rule (no moment tables).
+struct HfLeaf {
+    double center[3];
+    std::uint64_t multipole, local;  // pool offsets of the outgoing and incoming signatures
+    std::uint32_t rule;              // HfRule of the level
+    std::uint32_t first_item, item_count; // support panels of the leaf in the HF item list
+    std::uint32_t first_dof, dof_count;   // into the leaf DOF list
+    std::uint32_t pad;
+};
+
+// A support panel of a plane-wave leaf: geometry, corner DOFs (local index or -1).
+struct HfLeafItem {
+    double v0[3], e1[3], e2[3], normal[3];
+    double area2;                    // twice the panel area (rule weights sum to 1/2)
+    std::int32_t corner[3];
+    std::uint32_t leaf;
+};
+
+struct HfLeafLaunch {
+    const HfLeaf *leaves;
+    std::uint32_t leaf_count;
+    const HfLeafItem *items;
+    std::uint32_t item_count;
+    const std::uint32_t *leaf_dofs;
+    const double *row_scale;         // per leaf DOF entry (1 / lumped mass)
+    const HfRule *rules;
+    const double *polar;
+    const double *rule;              // panel rule: rule_points x (b0, b1, b2, w)
+    std::uint32_t rule_points;       // <= 16
+    std::uint32_t max_samples;       // largest rule sample count of the leaves
+    double k;
+    double *field;                   // workspace: item_count x rule_points complex
+    double *expansions;              // pool
+};
+
+// Emission: F(d) = i k sum_panels sum_q 2A w_q (n.d) p(y_q) exp(i k d.(y_q - c)).
+void launch_hf_leaf_emit(const HfLeafLaunch &launch, const double *p, hipStream_t stream);
+// Testing: y_i += (1/m_i) sum_panels sum_q 2A w_q phi_i(y_q) sum_d w_d (n.d - 1) exp(-i k d.(y_q - c)) L(d).
+void launch_hf_leaf_test(const HfLeafLaunch &launch, double *y, hipStream_t stream);
+
 // in_rings / out_rings: the largest ring counts of the links' input / output rules;
 // out_samples: samples of the destinations' rule.
 void launch_hf_transfer(const HfTransferLaunch &launch, HfTransferKind kind, std::uint32_t link_count,

@@ -138,6 +138,7 @@ struct FrequencyResult {
     // whether the defect correction with the finer operator ran, and the operator's apply time
     double fmm_audit{}, fmm_apply_ms{};
     bool fmm_corrected{};
+    std::uint64_t fmm_plane_wave_levels{}, fmm_m2l_links{}, fmm_near_nnz{}, fmm_device_bytes{};
 };
 
 struct Result {

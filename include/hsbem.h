@@ -168,6 +168,11 @@ HSBEM_API int hsbem_result_residual(const hsbem_result *result, uint32_t frequen
  * algorithmic sample counts of those three kernels (FP64 pair-kernel evaluations). */
 HSBEM_API int hsbem_result_timings(const hsbem_result *result, uint32_t frequency, double *milliseconds);
 HSBEM_API int hsbem_result_peak_device_bytes(const hsbem_result *result, uint64_t *bytes);
+/* Fast multipole evidence of a frequency point (BemFrequencyResult's FMM statistics,
+ * bem.hpp:193-332; zeros for the dense form): out[7] = seed-audit discrepancy
+ * (relative, against the seed/10 operator), apply milliseconds, defect correction
+ * applied (0/1), plane-wave levels, M2L links, near CSR entries, operator device bytes. */
+HSBEM_API int hsbem_result_fmm(const hsbem_result *result, uint32_t frequency, double *out);
 
 /* ---- serving loop (no counterpart class in the reference: its callers run
  * CudaBemSolver::solve on worker threads, apps/desktop/simulation_dock.cpp:2425-2600,
@@ -333,7 +338,7 @@ HSBEM_API int hsbem_fmm_operator_time_apply(hsbem_fmm_operator *op, uint32_t rep
 HSBEM_API int hsbem_fmm_operator_statistics(const hsbem_fmm_operator *op, uint64_t *counts, double *times,
                                             uint32_t *orders, uint32_t *level_count);
 /* Plane-wave levels of an FMM operator: counts[4] = plane-wave levels, angular
- * transfer links, diagonal translation classes, total levels; per level
+ * transfer links, diagonal translation classes, leaves on plane-wave levels; per level
  * representation[level_count] (0 inactive, 1 scaled spherical, 2 plane wave) and
  * rule_orders[level_count] (sphere rule order of a plane-wave level, else 0). */
 HSBEM_API int hsbem_fmm_operator_levels(const hsbem_fmm_operator *op, uint64_t *counts, uint32_t *representation,

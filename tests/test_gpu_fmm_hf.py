@@ -95,7 +95,7 @@ def test_fmm_apply_with_plane_wave_level_matches_dense(sphere_hf):
     op = solver.fmm_operator(num, k, k, leaf_capacity=32)
     st = op.statistics()
     print("plane-wave FMM", {key: st[key] for key in ("levels", "leaves", "far_pairs", "m2l_links", "angular_links",
-                                                       "diagonal_classes")},
+                                                       "diagonal_classes", "plane_wave_leaves")},
           "representation", st["level_representation"], "orders", st["level_orders"], "rules", st["level_rule_orders"])
     assert st["plane_wave_levels"] >= 1 and st["diagonal_classes"] > 0 and st["angular_links"] > 0
     A, _ = solver.assemble(num, k, k)
@@ -104,6 +104,32 @@ def test_fmm_apply_with_plane_wave_level_matches_dense(sphere_hf):
         err = _rel(op.apply(x), A @ x)
         print(f"plane-wave FMM apply rel err {err:.2e}, apply {op.time_apply(5):.3f} ms")
         assert err <= 2e-6
+    op.close()
+
+
+def test_fmm_apply_with_plane_wave_leaf_matches_dense():
+    """A small sphere (42 nodes) 0.25 m from the 10242-node sphere: with leaf
+    capacity 64 its leaf sits on the plane-wave level 2 at 20 kHz, so its signature
+    is emitted and tested directly on the level's sphere rule."""
+    big, small = meshes.icosphere(5), meshes.icosphere(1, radius=0.008)
+    V = np.vstack([big.vertices, small.vertices + np.array([0.0, 0.0, 0.25])])
+    T = np.vstack([big.triangles, small.triangles + len(big.vertices)]).astype(np.uint32)
+    mesh = bem.SurfaceMesh(V, T, np.ones(len(T), np.uint32), np.full(len(T), bem.DRIVEN, np.uint8))
+    k = 2 * math.pi * 20000.0 / 343.0
+    num = bem.BemNumerics()
+    solver = bem.HipBemSolver(mesh)
+    op = solver.fmm_operator(num, k, k, leaf_capacity=64)
+    st = op.statistics()
+    print("plane-wave leaves", st["plane_wave_leaves"], "representation", st["level_representation"],
+          "rules", st["level_rule_orders"])
+    assert st["plane_wave_leaves"] >= 1
+    A, _ = solver.assemble(num, k, k)
+    x = _random(len(V), 7)
+    err = _rel(op.apply(x), A @ x)
+    small_rows = slice(len(big.vertices), None)
+    err_small = _rel(op.apply(x)[small_rows], (A @ x)[small_rows])
+    print(f"apply rel err {err:.2e} (small sphere rows {err_small:.2e}), apply {op.time_apply(5):.3f} ms")
+    assert err <= 2e-6 and err_small <= 2e-6
     op.close()
 
 
