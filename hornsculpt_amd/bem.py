@@ -156,6 +156,7 @@ def library() -> ctypes.CDLL:
                                                              ctypes.POINTER(P)]),
                 "hsbem_fmm_operator_apply": (ctypes.c_int, [P, D, D]),
                 "hsbem_fmm_operator_precondition": (ctypes.c_int, [P, D, D]),
+                "hsbem_fmm_operator_apply_direct": (ctypes.c_int, [P, D, D]),
                 "hsbem_fmm_operator_solve": (ctypes.c_int, [P, D, D, U32, U32, ctypes.c_double, ctypes.c_int32, D,
                                                             ctypes.POINTER(U32)]),
                 "hsbem_fmm_operator_time_apply": (ctypes.c_int, [P, U32, D]),
@@ -591,12 +592,12 @@ def _collect(lib, handle, request: BemRequest) -> BemResult:
                                            refs.value, BemBoundaryData(surface, trace, deriv)))
         served = ctypes.c_double()
         _check(lib.hsbem_result_served_latency(handle, f, ctypes.byref(served)))
-        fm = np.zeros(7)
+        fm = np.zeros(8)
         _check(lib.hsbem_result_fmm(handle, f, _dptr(fm)))
         fmm = None if request.numerics.acceleration != ACCELERATION_FAST_MULTIPOLE else {
             "audit_discrepancy": float(fm[0]), "apply_milliseconds": float(fm[1]), "corrected": bool(fm[2]),
             "plane_wave_levels": int(fm[3]), "m2l_links": int(fm[4]), "near_nnz": int(fm[5]),
-            "device_bytes": int(fm[6])}
+            "device_bytes": int(fm[6]), "direct_audit": bool(fm[7])}
         freqs.append(BemFrequencyResult(request.frequencies[f], exs, ms[0], ms[1], ms[2], ms[3], ms[4], ms[5],
                                         {"regular": ms[6], "touching": ms[7], "near": ms[8],
                                          "regular_samples": ms[9], "touching_samples": ms[10],
@@ -728,6 +729,13 @@ class HipFmmOperator:
     def apply(self, x) -> np.ndarray:
         xv, y = self._vector(x), np.zeros(2 * self.size)
         _check(self._lib.hsbem_fmm_operator_apply(self.handle, _dptr(xv), _dptr(y)))
+        return y.view(np.complex128)
+
+    def apply_direct(self, x) -> np.ndarray:
+        """A x with the far part integrated directly over every far DOF pair (the
+        direct Galerkin audit; meshes of at most 20000 panels)."""
+        xv, y = self._vector(x), np.zeros(2 * self.size)
+        _check(self._lib.hsbem_fmm_operator_apply_direct(self.handle, _dptr(xv), _dptr(y)))
         return y.view(np.complex128)
 
     def precondition(self, v) -> np.ndarray:

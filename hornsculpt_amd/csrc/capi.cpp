@@ -508,6 +508,7 @@ int hsbem_result_fmm(const hsbem_result *r, uint32_t f, double *out) {
         out[4] = static_cast<double>(fr.fmm_m2l_links);
         out[5] = static_cast<double>(fr.fmm_near_nnz);
         out[6] = static_cast<double>(fr.fmm_device_bytes);
+        out[7] = fr.fmm_direct_audit ? 1 : 0;
     });
 }
 
@@ -736,6 +737,12 @@ int hsbem_fmm_operator_apply(hsbem_fmm_operator *o, const double *x, double *y) 
             else
                 o->op->apply(dx, dy, s);
         });
+    });
+}
+
+int hsbem_fmm_operator_apply_direct(hsbem_fmm_operator *o, const double *x, double *y) {
+    return guarded([&] {
+        fmm_host_call(o, x, y, [&](const double *dx, double *dy, hipStream_t s) { o->op->apply_direct(dx, dy, s); });
     });
 }
 

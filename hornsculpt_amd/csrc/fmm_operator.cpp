@@ -383,12 +383,15 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
 
     // ---- translation links
     t0 = Clock::now();
-    std::map<Triple, std::uint32_t> direction_index;
+    // one rotation plan per (direction, order class): links up to kMaxFmmOrder never
+    // share a plan (and so the tiled kernels) with a wider level
+    std::map<std::pair<Triple, bool>, std::uint32_t> direction_index;
     std::vector<Triple> directions;
     std::vector<int> direction_order;
     auto direction = [&](Triple t, int ord) {
         t = reduced(t);
-        auto [it, fresh] = direction_index.emplace(t, static_cast<std::uint32_t>(directions.size()));
+        auto [it, fresh] = direction_index.emplace(std::make_pair(t, ord > dev::kMaxFmmOrder),
+                                                   static_cast<std::uint32_t>(directions.size()));
         if (fresh) {
             directions.push_back(t);
             direction_order.push_back(ord);
@@ -854,6 +857,48 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
     st.moments_ms = ms;
     if (code)
         throw std::runtime_error("FMM leaf moments left the bounded radial-table domain.");
+    // direct audit data (small meshes) and the work model deciding between the audits
+    op->geo_ = geo_.data();
+    op->row_scale_ = original_row_scale_.data();
+    op->nodes_ = original_nodes_.data();
+    op->nt_ = nt;
+    {
+        const long double panels = nt, q = st.rule_points, ratio = 2; // FP32 : FP64 throughput of MI355X
+        op->direct_work_ = static_cast<double>(panels * panels * (9.L + q * q * (13.L + 1.L / ratio)));
+        long double multipole = 0;
+        for (std::uint32_t l = 0; l < nl; ++l) {
+            if (order[l] < 0)
+                continue;
+            const long double pairs = static_cast<long double>(tree.levels[l].pp_far) + tree.levels[l].pf_far;
+            const long double p = order[l], n = p + 1;
+            if (!plane[l]) {
+                const long double rotation = n * (4 * p * p + 8 * p + 3) / 3, axial_work = n * n + p * n * (2 * p + 1) / 3;
+                multipole += pairs * (2 * rotation + axial_work) +
+                             static_cast<long double>(tree.levels[l].squared_offsets.size()) * n * n * n * n;
+            } else {
+                multipole += pairs * 2.L * (rule_order[l] + 1.L) * (rule_order[l] + 1.L);
+            }
+        }
+        op->multipole_work_ = static_cast<double>(multipole);
+    }
+    if (nt <= kMaximumDirectAuditPanels) {
+        std::vector<std::uint32_t> dense(tree.nodes.size(), UINT32_MAX), vertex_leaf(nv);
+        std::uint32_t count = 0;
+        for (std::uint32_t v = 0; v < nv; ++v) {
+            const std::uint32_t leaf = tree.pressure_leaf[v];
+            if (dense[leaf] == UINT32_MAX)
+                dense[leaf] = count++;
+            vertex_leaf[v] = dense[leaf];
+        }
+        std::vector<std::uint8_t> near(static_cast<std::size_t>(count) * count, 0);
+        for (const SupportPair &pr : tree.pressure_pressure.near)
+            if (dense[pr.target] != UINT32_MAX && dense[pr.source] != UINT32_MAX)
+                near[static_cast<std::size_t>(dense[pr.target]) * count + dense[pr.source]] = 1;
+        op->vertex_leaf_.upload(vertex_leaf, s);
+        op->near_leaves_.upload(near, s);
+        op->direct_leaf_count_ = count;
+        hip_check(hipStreamSynchronize(s), "direct audit upload");
+    }
     st.device_bytes = op->near_->values.bytes() + ns->columns.bytes() + op->emission_.bytes() +
                       op->testing_.bytes() + op->pool_.bytes() + op->small_d_.bytes() + op->azimuth_.bytes() +
                       op->axial_.bytes() + op->jacobi_->bytes() + op->links_.bytes() + op->destinations_.bytes() +
@@ -941,6 +986,20 @@ void FmmOperator::apply_graph(const double *x, double *y, hipStream_t s) const {
     hip_check(hipMemcpyAsync(graph_x_.data(), x, bytes, hipMemcpyDeviceToDevice, s), "FMM graph input");
     hip_check(hipGraphLaunch(graph_exec_, s), "FMM graph launch");
     hip_check(hipMemcpyAsync(y, graph_y_.data(), bytes, hipMemcpyDeviceToDevice, s), "FMM graph output");
+}
+
+void FmmOperator::apply_direct(const double *x, double *y, hipStream_t s) const {
+    if (!direct_available())
+        throw std::runtime_error("The direct Galerkin audit is built for meshes of at most 20000 panels.");
+    dev::launch_csr_apply(near_->structure->row_offsets.data(), near_->structure->columns.data(), near_->values.data(),
+                          n_, x, y, s);
+    dev::FmmDirectLaunch D{geo_, nodes_, nt_, n_, vertex_leaf_.data(), near_leaves_.data(), direct_leaf_count_,
+                           leaf_rule_.data(), stats_.rule_points, k_, row_scale_, x, y};
+    dev::launch_direct_far(D, s);
+}
+
+bool FmmOperator::prefer_direct() const {
+    return direct_available() && stats_.levels && direct_work_ <= 2 * multipole_work_;
 }
 
 void FmmOperator::precondition(const double *v, double *z, hipStream_t s) const {
@@ -1097,12 +1156,43 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
         }
     };
     residuals(*op);
+    DeviceBuffer<double> ax(2 * static_cast<std::size_t>(nv));
+    if (op->prefer_direct()) {
+        // direct Galerkin audit: residual of the original operator with the far part
+        // integrated directly, defect corrections on the FMM operator until it holds
+        auto direct_residuals = [&] {
+            for (std::uint32_t e = 0; e < ne; ++e) {
+                op->apply_direct(vec(ws.fmm_x, e), vec(ws.fmm_r, e), s);
+                dev::launch_subtract_from(nv, vec(ws.fmm_b, e), vec(ws.fmm_r, e), s);
+                const double rn = weighted_norm(vec(ws.fmm_r, e), nv, original_lumped_.data(), ws, s);
+                relative[e] = bnorm[e] > 0 ? rn / bnorm[e] : (rn == 0 ? 0 : std::numeric_limits<double>::infinity());
+            }
+        };
+        direct_residuals();
+        for (unsigned step = 0; *std::max_element(relative.begin(), relative.end()) > tol; ++step) {
+            if (step == num.refinement_steps)
+                break; // reported by the residual check below
+            for (std::uint32_t e = 0; e < ne; ++e) {
+                if (!(relative[e] > tol))
+                    continue;
+                double achieved = 0;
+                std::uint32_t its = 0;
+                gmres(s, kop, nv, vec(ws.fmm_r, e), ax.data(), num.fmm_restart, num.fmm_maximum_iterations, 0.1 * tol,
+                      &achieved, &its, cancel, ws.gmres);
+                dev::launch_add(nv, ax.data(), vec(ws.fmm_x, e), s);
+                iterations[e] += its;
+            }
+            direct_residuals();
+            fr.fmm_corrected = true;
+        }
+        fr.fmm_direct_audit = true;
+        fr.fmm_audit = 0;
+    } else {
     // seed audit: the same action with a ten times tighter accuracy seed
     FmmSettings fine = fs;
     fine.accuracy_seed = 0.1 * fs.accuracy_seed;
     const std::unique_ptr<FmmOperator> audit = build_fmm(num, P.k_max, k, fine, faces, s, op.get());
     double discrepancy = 0;
-    DeviceBuffer<double> ax(2 * static_cast<std::size_t>(nv));
     for (std::uint32_t e = 0; e < ne; ++e) {
         // (A_seed - A_fine) x = (b - r) - A_fine x
         audit->apply(vec(ws.fmm_x, e), ax.data(), s);
@@ -1127,6 +1217,7 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
         }
         residuals(*audit);
         fr.fmm_corrected = true;
+    }
     }
     fr.ms[1] = ms_since(t_solve);
     {

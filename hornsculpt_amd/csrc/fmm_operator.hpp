@@ -25,6 +25,8 @@
 
 namespace hsbem {
 
+inline constexpr std::size_t kMaximumDirectAuditPanels = 20000;
+
 struct FmmSettings {
     SupportTreeSettings tree{64, 20, 1.5, 2.0};
     double accuracy_seed{5e-7}; // Eq. 18 accuracy of the level orders
@@ -71,6 +73,14 @@ class FmmOperator {
     void apply_graph(const double *x, double *y, hipStream_t stream) const;
     // z = M^{-1} v (block-Jacobi of the near operator).
     void precondition(const double *v, double *z, hipStream_t stream) const;
+    // y = A x with the far part integrated directly over every far DOF pair (the
+    // direct Galerkin audit, SupportDirectAudit::applyPressureDevice,
+    // support_direct_audit.hpp:11-22): meshes of at most kMaximumDirectAuditPanels panels.
+    void apply_direct(const double *x, double *y, hipStream_t stream) const;
+    bool direct_available() const { return direct_leaf_count_ > 0; }
+    // preferSupportDirectAudit (support_fmm_solve.cu:79-115): the direct audit when its
+    // pair work is at most twice the multipole work of this operator's levels.
+    bool prefer_direct() const;
     ~FmmOperator();
     std::uint32_t size() const { return n_; }
     const DeviceNear &near() const { return *near_; }
@@ -126,6 +136,13 @@ class FmmOperator {
     DeviceBuffer<dev::HfLeafItem> hf_leaf_items_;
     DeviceBuffer<double> leaf_scale_, leaf_rule_;
     std::uint32_t hf_leaf_count_{}, hf_leaf_item_count_{}, hf_max_samples_{};
+    // direct audit: solver-owned geometry (alive while the operator is), leaf map, near matrix
+    const double *geo_{}, *row_scale_{};
+    const std::uint32_t *nodes_{};
+    std::uint32_t nt_{}, direct_leaf_count_{};
+    DeviceBuffer<std::uint32_t> vertex_leaf_;
+    DeviceBuffer<std::uint8_t> near_leaves_;
+    double direct_work_{}, multipole_work_{};
     double k_{};
     std::uint32_t leaf_count_{};
     mutable DeviceBuffer<double> graph_x_, graph_y_;

@@ -742,6 +742,119 @@ __global__ __launch_bounds__(256) void far_flux(FmmFarFluxLaunch L) {
     }
 }
 
+// Direct far action of the Burton-Miller pressure operator. For a test panel s and a
+// trial panel t the far corner pairs (ci, cj) contribute
+//   (1/m_i) x_j int_s int_t phi_i(x) phi_j(y) [-dG/dn_y - alpha d2G/dn_x dn_y] dy dx,
+// G = exp(-ikr)/(4 pi r), d = x - y, g' = -(ik + 1/r) G, g'' = G/r^2 + (ik + 1/r)^2 G:
+//   dG/dn_y = -g' (d.n_y)/r,
+//   d2G/dn_x dn_y = -(d.n_x)(d.n_y)(g''/r^2 - g'/r^3) - (g'/r)(n_x.n_y).
+// Block per test panel, threads over trial panels; per-thread row sums reduced in
+// the block, one atomic add per test corner.
+__global__ __launch_bounds__(256) void direct_far(FmmDirectLaunch L) {
+    __shared__ double red[2][3][4];
+    const std::uint32_t s = blockIdx.x;
+    const std::size_t nt = L.nt;
+    const double *g = L.geo;
+    double vs[3], e1s[3], e2s[3], ns[3];
+    std::uint32_t node_s[3], leaf_s[3];
+    for (int i = 0; i < 3; ++i) {
+        vs[i] = g[(G_V0 + i) * nt + s];
+        e1s[i] = g[(G_E1 + i) * nt + s];
+        e2s[i] = g[(G_E2 + i) * nt + s];
+        ns[i] = g[(G_N + i) * nt + s];
+        node_s[i] = L.nodes[static_cast<std::size_t>(i) * nt + s];
+        leaf_s[i] = L.vertex_leaf[node_s[i]];
+    }
+    const double ws = 2 * g[G_AREA * nt + s], k = L.k;
+    const int Q = static_cast<int>(L.rule_points);
+    const C2 alpha{0, -1 / k};
+    C2 acc[3] = {{0, 0}, {0, 0}, {0, 0}};
+    for (std::uint32_t t = threadIdx.x; t < L.nt; t += blockDim.x) {
+        std::uint32_t node_t[3];
+        bool far[3][3], any = false;
+        for (int c = 0; c < 3; ++c)
+            node_t[c] = L.nodes[static_cast<std::size_t>(c) * nt + t];
+        for (int ci = 0; ci < 3; ++ci)
+            for (int cj = 0; cj < 3; ++cj) {
+                far[ci][cj] = !L.near[static_cast<std::size_t>(leaf_s[ci]) * L.leaf_count + L.vertex_leaf[node_t[cj]]];
+                any |= far[ci][cj];
+            }
+        if (!any)
+            continue;
+        double vt[3], e1t[3], e2t[3], nn[3];
+        for (int i = 0; i < 3; ++i) {
+            vt[i] = g[(G_V0 + i) * nt + t];
+            e1t[i] = g[(G_E1 + i) * nt + t];
+            e2t[i] = g[(G_E2 + i) * nt + t];
+            nn[i] = g[(G_N + i) * nt + t];
+        }
+        const double wt = 2 * g[G_AREA * nt + t];
+        const double nxny = ns[0] * nn[0] + ns[1] * nn[1] + ns[2] * nn[2];
+        C2 M[3][3] = {};
+        for (int a = 0; a < Q; ++a) {
+            const double *ra = L.rule + 4 * a;
+            double x[3];
+            for (int i = 0; i < 3; ++i)
+                x[i] = vs[i] + ra[1] * e1s[i] + ra[2] * e2s[i];
+            C2 T[3] = {{0, 0}, {0, 0}, {0, 0}};
+            for (int b = 0; b < Q; ++b) {
+                const double *rb = L.rule + 4 * b;
+                double d[3], r2 = 0, dnx = 0, dny = 0;
+                for (int i = 0; i < 3; ++i) {
+                    d[i] = x[i] - (vt[i] + rb[1] * e1t[i] + rb[2] * e2t[i]);
+                    r2 += d[i] * d[i];
+                    dnx += ns[i] * d[i];
+                    dny += nn[i] * d[i];
+                }
+                const double r = sqrt(r2), ir = 1 / r;
+                double sn, cs;
+                sincos(k * r, &sn, &cs);
+                const C2 G = (ir / (4 * 3.141592653589793)) * C2{cs, -sn};
+                const C2 ikr{ir, k};                       // ik + 1/r
+                const C2 g1 = C2{-1, 0} * (ikr * G);        // g'
+                const C2 g2 = (ir * ir) * G + (ikr * ikr) * G; // g''
+                const C2 dGy = (-dny * ir) * g1;
+                const C2 H = (-dnx * dny) * ((ir * ir) * g2 - (ir * ir * ir) * g1) - (nxny * ir) * g1;
+                const C2 K = (rb[3] * wt) * (C2{-dGy.re, -dGy.im} - alpha * H);
+                for (int c = 0; c < 3; ++c)
+                    T[c] = T[c] + rb[c] * K;
+            }
+            for (int ci = 0; ci < 3; ++ci)
+                for (int cj = 0; cj < 3; ++cj)
+                    M[ci][cj] = M[ci][cj] + (ra[3] * ws * ra[ci]) * T[cj];
+        }
+        for (int cj = 0; cj < 3; ++cj) {
+            const C2 xj{L.x[2 * node_t[cj]], L.x[2 * node_t[cj] + 1]};
+            for (int ci = 0; ci < 3; ++ci)
+                if (far[ci][cj])
+                    acc[ci] = acc[ci] + M[ci][cj] * xj;
+        }
+    }
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    for (int c = 0; c < 3; ++c) {
+        double re = acc[c].re, im = acc[c].im;
+        for (int off = 32; off > 0; off >>= 1) {
+            re += __shfl_xor(re, off, 64);
+            im += __shfl_xor(im, off, 64);
+        }
+        if (lane == 0) {
+            red[0][c][wave] = re;
+            red[1][c][wave] = im;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int c = threadIdx.x;
+        double re = 0, im = 0;
+        for (int w = 0; w < static_cast<int>(blockDim.x) / 64; ++w) {
+            re += red[0][c][w];
+            im += red[1][c][w];
+        }
+        const double sc = L.row_scale[node_s[c]];
+        atomic_add_c(L.y + 2 * static_cast<std::size_t>(node_s[c]), C2{sc * re, sc * im});
+    }
+}
+
 // out[:, e] += B_near q_e with B_near the P1 x face-column near CSR.
 __global__ void flux_csr_apply(const std::uint32_t *offsets, const std::uint32_t *columns, const double *values,
                                std::uint32_t rows, const double *q, std::uint32_t faces, std::uint32_t excitations,
@@ -817,6 +930,11 @@ void launch_gather_blocks(const std::uint32_t *row_offsets, const std::uint32_t 
                           hipStream_t stream) {
     hipLaunchKernelGGL(gather_blocks, dim3((n + 255) / 256), dim3(256), 0, stream, row_offsets, columns, values, order,
                        position, n, blocks);
+}
+
+void launch_direct_far(const FmmDirectLaunch &launch, hipStream_t stream) {
+    if (launch.nt)
+        hipLaunchKernelGGL(direct_far, dim3(launch.nt), dim3(256), 0, stream, launch);
 }
 
 void launch_far_flux(const FmmFarFluxLaunch &launch, hipStream_t stream) {

@@ -216,6 +216,25 @@ void launch_hf_transfer(const HfTransferLaunch &launch, HfTransferKind kind, std
                         std::uint32_t out_samples, hipStream_t stream);
 void launch_hf_diagonal(const HfDiagonalLaunch &launch, std::uint32_t destination_count, hipStream_t stream);
 
+// Direct far part of A x for the direct Galerkin audit: every P1 DOF pair whose
+// pressure leaves are not near, integrated with the panel rule (original numbering).
+struct FmmDirectLaunch {
+    const double *geo;                 // panel SoA
+    const std::uint32_t *nodes;        // 3 x nt mesh vertex ids
+    std::uint32_t nt, nv;
+    const std::uint32_t *vertex_leaf;  // dense pressure-leaf index per vertex
+    const std::uint8_t *near;          // leaf_count x leaf_count: near[target * leaf_count + source]
+    std::uint32_t leaf_count;
+    const double *rule;                // rule_points x (b0, b1, b2, w), weights sum to 1/2
+    std::uint32_t rule_points;         // <= 16
+    double k;
+    const double *row_scale;           // 1 / lumped mass
+    const double *x;                   // complex, nv
+    double *y;                         // complex, nv, accumulated
+};
+// y += A_far x (block per test panel, atomics per test DOF).
+void launch_direct_far(const FmmDirectLaunch &launch, hipStream_t stream);
+
 void launch_leaf_moments(const FmmLeafMomentsLaunch &launch, hipStream_t stream);
 void launch_far_flux(const FmmFarFluxLaunch &launch, hipStream_t stream);
 // out[:, e] += B_near q_e (q: faces x excitations complex).

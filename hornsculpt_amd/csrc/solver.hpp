@@ -137,7 +137,7 @@ struct FrequencyResult {
     // FMM form: seed-audit discrepancy ||(A_seed - A_seed/10) x|| / ||b|| (largest excitation),
     // whether the defect correction with the finer operator ran, and the operator's apply time
     double fmm_audit{}, fmm_apply_ms{};
-    bool fmm_corrected{};
+    bool fmm_corrected{}, fmm_direct_audit{};
     std::uint64_t fmm_plane_wave_levels{}, fmm_m2l_links{}, fmm_near_nnz{}, fmm_device_bytes{};
 };
 

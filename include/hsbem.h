@@ -169,9 +169,11 @@ HSBEM_API int hsbem_result_residual(const hsbem_result *result, uint32_t frequen
 HSBEM_API int hsbem_result_timings(const hsbem_result *result, uint32_t frequency, double *milliseconds);
 HSBEM_API int hsbem_result_peak_device_bytes(const hsbem_result *result, uint64_t *bytes);
 /* Fast multipole evidence of a frequency point (BemFrequencyResult's FMM statistics,
- * bem.hpp:193-332; zeros for the dense form): out[7] = seed-audit discrepancy
- * (relative, against the seed/10 operator), apply milliseconds, defect correction
- * applied (0/1), plane-wave levels, M2L links, near CSR entries, operator device bytes. */
+ * bem.hpp:193-332; zeros for the dense form): out[8] = seed-audit discrepancy
+ * (relative, against the seed/10 operator; 0 under the direct audit), apply
+ * milliseconds, defect correction applied (0/1), plane-wave levels, M2L links, near
+ * CSR entries, operator device bytes, residual evaluated by the direct Galerkin
+ * audit (0/1). */
 HSBEM_API int hsbem_result_fmm(const hsbem_result *result, uint32_t frequency, double *out);
 
 /* ---- serving loop (no counterpart class in the reference: its callers run
@@ -322,6 +324,10 @@ HSBEM_API int hsbem_fmm_operator_create(hsbem_mesh *mesh, const hsbem_numerics *
 HSBEM_API int hsbem_fmm_operator_apply(hsbem_fmm_operator *op, const double *x, double *y);
 /* preconditionDevice (support_fmm_operator.hpp:91): z = M^{-1} v, 64 x 64 near
  * diagonal blocks in tree order. */
+/* y = A x with the far part integrated directly over every far DOF pair
+ * (SupportDirectAudit::applyPressureDevice, support_direct_audit.hpp:11-22); meshes
+ * of at most 20000 panels (HSBEM_RUNTIME_ERROR above). */
+HSBEM_API int hsbem_fmm_operator_apply_direct(hsbem_fmm_operator *op, const double *x, double *y);
 HSBEM_API int hsbem_fmm_operator_precondition(hsbem_fmm_operator *op, const double *v, double *z);
 /* Restarted GMRES on A x = b with the FMM operator (DeviceGmresSolver::solve,
  * modules/bem/cuda/device_gmres.hpp:11-35), right block-Jacobi when precondition != 0. */

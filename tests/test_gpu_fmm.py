@@ -74,6 +74,23 @@ def test_fmm_apply_matches_dense_horn(frequency):
     op.close()
 
 
+def test_direct_audit_action_matches_dense_horn():
+    """The direct Galerkin audit (exact near CSR + far part integrated directly over
+    every far DOF pair of the same leaf partition) equals the dense matrix and the
+    FMM action to quadrature accuracy."""
+    mesh = meshes.exponential_horn(0.012)
+    k = 2 * math.pi * 2000.0 / 343.0
+    num = bem.BemNumerics()
+    solver = bem.HipBemSolver(mesh)
+    A, _ = solver.assemble(num, k, k)
+    op = solver.fmm_operator(num, k, k, leaf_capacity=32)
+    x = np.random.default_rng(5).standard_normal(A.shape[0]) + 1j * np.random.default_rng(6).standard_normal(A.shape[0])
+    yd, yf, yx = A @ x, op.apply(x), op.apply_direct(x)
+    print(f"direct audit vs dense {_rel(yx, yd):.2e}, vs FMM {_rel(yx, yf):.2e}, FMM vs dense {_rel(yf, yd):.2e}")
+    assert _rel(yx, yd) <= 1e-6 and _rel(yx, yf) <= 1e-6
+    op.close()
+
+
 def test_fmm_gmres_reproduces_dense_solution():
     mesh = meshes.exponential_horn(0.012)
     k = 2 * math.pi * 1500.0 / 343.0
@@ -128,6 +145,8 @@ def test_fmm_request_matches_dense_request():
         assert ef.boundary_residual <= 1e-6
         assert err <= 1e-5 and obs <= 1e-5
         assert ef.iterations > 0
+        print("  FMM evidence", ff.fmm)
+        assert ff.fmm is not None and ff.fmm["m2l_links"] > 0
 
 
 def test_fmm_request_validation():
