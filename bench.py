@@ -262,7 +262,9 @@ def run_flange(edge, frequencies, leaf_capacity, tolerance, repeats):
             print(f"# flange {'cold' if r == 0 else 'warm'} {fr.frequency:.0f} Hz: build {fr.assembly_milliseconds:.0f} ms, "
                   f"solve {fr.solve_milliseconds:.0f} ms, {ex.iterations} iterations, residual "
                   f"{ex.boundary_residual:.1e}, audit {fr.fmm['audit_discrepancy']:.1e}, apply "
-                  f"{fr.fmm['apply_milliseconds']:.1f} ms, plane-wave levels {fr.fmm['plane_wave_levels']}",
+                  f"{fr.fmm['apply_milliseconds']:.1f} ms, plane-wave levels {fr.fmm['plane_wave_levels']}, endpoint "
+                  f"degree {fr.fmm['endpoint_degree']} (comparison {fr.fmm['quadrature_discrepancy']:.1e}, "
+                  f"{fr.fmm['promotions']} promotions), direct audit {fr.fmm['direct_audit']}",
                   file=sys.stderr, flush=True)
         runs.append({"wall_s": round(wall, 3), "points": points})
     warm = runs[1:] or runs

@@ -33,7 +33,8 @@ ACCELERATION_DENSE, ACCELERATION_FAST_MULTIPOLE = 0, 1
 
 class _FmmSettings(ctypes.Structure):
     _fields_ = [("leaf_capacity", ctypes.c_uint32), ("maximum_depth", ctypes.c_uint32), ("separation", ctypes.c_double),
-                ("near_ratio", ctypes.c_double), ("accuracy_seed", ctypes.c_double)]
+                ("near_ratio", ctypes.c_double), ("accuracy_seed", ctypes.c_double),
+                ("endpoint_degree", ctypes.c_uint32)]
 
 
 class _Numerics(ctypes.Structure):
@@ -527,10 +528,12 @@ class HipBemSolver:
 
     def fmm_operator(self, numerics: BemNumerics, k_max: float, k: float, active_faces: Sequence[int] = (),
                      leaf_capacity: int = 64, maximum_depth: int = 20, separation: float = 1.5,
-                     near_ratio: float = 2.0, accuracy_seed: float = 5e-7) -> "HipFmmOperator":
-        """Matrix-free FMM pressure operator (device) of this mesh at wavenumber k."""
+                     near_ratio: float = 2.0, accuracy_seed: float = 5e-7,
+                     endpoint_degree: int = 0) -> "HipFmmOperator":
+        """Matrix-free FMM pressure operator (device) of this mesh at wavenumber k;
+        endpoint_degree: leaf panel rule (0: the electrical minimum)."""
         return HipFmmOperator(self, numerics, k_max, k, active_faces, leaf_capacity, maximum_depth, separation,
-                              near_ratio, accuracy_seed)
+                              near_ratio, accuracy_seed, endpoint_degree)
 
     def plan(self, numerics: BemNumerics, k_max: float) -> dict:
         lib = self._lib
@@ -592,12 +595,13 @@ def _collect(lib, handle, request: BemRequest) -> BemResult:
                                            refs.value, BemBoundaryData(surface, trace, deriv)))
         served = ctypes.c_double()
         _check(lib.hsbem_result_served_latency(handle, f, ctypes.byref(served)))
-        fm = np.zeros(8)
+        fm = np.zeros(11)
         _check(lib.hsbem_result_fmm(handle, f, _dptr(fm)))
         fmm = None if request.numerics.acceleration != ACCELERATION_FAST_MULTIPOLE else {
             "audit_discrepancy": float(fm[0]), "apply_milliseconds": float(fm[1]), "corrected": bool(fm[2]),
             "plane_wave_levels": int(fm[3]), "m2l_links": int(fm[4]), "near_nnz": int(fm[5]),
-            "device_bytes": int(fm[6]), "direct_audit": bool(fm[7])}
+            "device_bytes": int(fm[6]), "direct_audit": bool(fm[7]), "quadrature_discrepancy": float(fm[8]),
+            "endpoint_degree": int(fm[9]), "promotions": int(fm[10])}
         freqs.append(BemFrequencyResult(request.frequencies[f], exs, ms[0], ms[1], ms[2], ms[3], ms[4], ms[5],
                                         {"regular": ms[6], "touching": ms[7], "near": ms[8],
                                          "regular_samples": ms[9], "touching_samples": ms[10],
@@ -700,10 +704,10 @@ class HipFmmOperator:
 
     def __init__(self, solver: "HipBemSolver", numerics: BemNumerics, k_max: float, k: float,
                  active_faces: Sequence[int], leaf_capacity: int, maximum_depth: int, separation: float,
-                 near_ratio: float, accuracy_seed: float):
+                 near_ratio: float, accuracy_seed: float, endpoint_degree: int = 0):
         self._lib = library()
         self._solver = solver  # keeps the mesh alive
-        st = _FmmSettings(leaf_capacity, maximum_depth, separation, near_ratio, accuracy_seed)
+        st = _FmmSettings(leaf_capacity, maximum_depth, separation, near_ratio, accuracy_seed, endpoint_degree)
         faces = np.ascontiguousarray(active_faces if len(active_faces) else [0], dtype=np.uint32)
         n = numerics.native()
         self.handle = ctypes.c_void_p()

@@ -509,6 +509,9 @@ int hsbem_result_fmm(const hsbem_result *r, uint32_t f, double *out) {
         out[5] = static_cast<double>(fr.fmm_near_nnz);
         out[6] = static_cast<double>(fr.fmm_device_bytes);
         out[7] = fr.fmm_direct_audit ? 1 : 0;
+        out[8] = fr.fmm_quadrature_discrepancy;
+        out[9] = fr.fmm_endpoint_degree;
+        out[10] = fr.fmm_promotions;
     });
 }
 
@@ -706,6 +709,7 @@ void hsbem_fmm_settings_default(hsbem_fmm_settings *s) {
     s->separation = d.tree.separation;
     s->near_ratio = d.tree.near_ratio;
     s->accuracy_seed = d.accuracy_seed;
+    s->endpoint_degree = d.endpoint_degree;
 }
 
 int hsbem_fmm_operator_create(hsbem_mesh *m, const hsbem_numerics *n, const hsbem_fmm_settings *settings, double k_max,
@@ -719,6 +723,10 @@ int hsbem_fmm_operator_create(hsbem_mesh *m, const hsbem_numerics *n, const hsbe
         fs.tree.separation = settings->separation;
         fs.tree.near_ratio = settings->near_ratio;
         fs.accuracy_seed = settings->accuracy_seed;
+        fs.endpoint_degree = settings->endpoint_degree;
+        require(fs.endpoint_degree == 0 || fs.endpoint_degree == 4 || fs.endpoint_degree == 6 ||
+                    fs.endpoint_degree == 8 || fs.endpoint_degree == 12 || fs.endpoint_degree == 14,
+                "FMM endpoint degree must be 0, 4, 6, 8, 12 or 14");
         hsbem::hip_check(hipSetDevice(m->context->ctx.device), "set device");
         auto op = std::make_unique<hsbem_fmm_operator>();
         op->solver = m->solver;

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <functional>
 #include <future>
 #include <map>
 #include <numeric>
@@ -177,7 +178,7 @@ std::unique_ptr<FmmOperator> MeshSolver::fmm_operator(const Numerics &n, double 
 
 std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_max, double k, const FmmSettings &fs,
                                                    const std::vector<std::uint32_t> &faces, hipStream_t s,
-                                                   const FmmOperator *share) {
+                                                   const FmmOperator *share, unsigned endpoint_degree) {
     if (!std::isfinite(k) || k <= 0 || !std::isfinite(fs.accuracy_seed) || fs.accuracy_seed <= 0 ||
         fs.accuracy_seed >= 1)
         throw std::invalid_argument("FMM operator needs a positive wavenumber and an accuracy seed in (0, 1).");
@@ -752,9 +753,11 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
             h = std::max(h, p.max_edge);
         return k * h;
     }();
-    const unsigned degree = minimum_endpoint_degree(max_kh, n.quadrature_order);
-    if (degree > 8)
-        throw std::runtime_error("FMM leaf quadrature above degree 8 (k h > 2) is not built here.");
+    const unsigned degree = std::max(minimum_endpoint_degree(max_kh, n.quadrature_order),
+                                     endpoint_degree ? endpoint_degree : fs.endpoint_degree);
+    if (degree > 14)
+        throw std::runtime_error("FMM endpoint quadrature above degree 14 is not available.");
+    st.endpoint_degree = degree;
     for (const auto &q : triangle_rule(degree)) {
         rule.insert(rule.end(), {q.bary[0], q.bary[1], q.bary[2], q.weight});
     }
@@ -1054,12 +1057,12 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
         if (!internal_of_device_.size())
             internal_of_device_.upload(internal_of_, s);
     }
-    // right-hand sides b = B q, mesh vertex order: near flux CSR per face column + far pairs
+    // right-hand sides b = B q, mesh vertex order: near flux CSR per face column + far
+    // pairs integrated with the operator's endpoint rule
     const std::size_t nvec = 2 * static_cast<std::size_t>(nv) * ne;
     ws.fmm_b.resize(nvec);
     ws.fmm_x.resize(nvec);
     ws.fmm_r.resize(nvec);
-    hip_check(hipMemsetAsync(ws.fmm_b.data(), 0, ws.fmm_b.bytes(), s), "clear FMM rhs");
     std::vector<double> qf(2 * std::max<std::size_t>(faces.size(), 1) * ne, 0.0);
     for (std::uint32_t e = 0; e < ne; ++e)
         for (std::size_t f = 0; f < faces.size(); ++f) {
@@ -1071,16 +1074,16 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
             qf[2 * (e * faces.size() + f) + 1] = q.imag();
         }
     ws.fmm_q.upload(qf, s);
-    const DeviceNear &dn = op->near();
-    const NearStructure &ns = *dn.structure;
-    dev::launch_flux_csr_apply(ns.flux_offsets.data(), ns.flux_columns.data(), dn.flux_values.data(), nv,
-                               ws.fmm_q.data(), static_cast<std::uint32_t>(faces.size()), ne, ws.fmm_b.data(), s);
+    const NearStructure &ns = *op->near().structure;
+    DeviceBuffer<std::uint32_t> d_panels, d_pl, d_fl, d_no, d_ns;
+    std::uint32_t flux_count = 0;
     {
         const SupportTree &tree = ns.tree;
         std::vector<std::uint32_t> flux_panels;
         for (std::uint32_t t = 0; t < nt; ++t)
             if (ns.plan.panel_face_column[t] != UINT32_MAX)
                 flux_panels.push_back(t);
+        flux_count = static_cast<std::uint32_t>(flux_panels.size());
         std::vector<std::uint32_t> near_offsets(tree.nodes.size() + 1, 0), near_sources;
         for (const SupportPair &pr : tree.pressure_flux.near)
             ++near_offsets[pr.target + 1];
@@ -1090,20 +1093,18 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
         std::vector<std::uint32_t> fill(near_offsets.begin(), near_offsets.end() - 1);
         for (const SupportPair &pr : tree.pressure_flux.near)
             near_sources[fill[pr.target]++] = pr.source;
-        double max_kh = 0;
-        for (const Panel &p : space_.panels)
-            max_kh = std::max(max_kh, k * p.max_edge);
-        std::vector<double> rule;
-        for (const auto &q : triangle_rule(minimum_endpoint_degree(max_kh, num.quadrature_order)))
-            rule.insert(rule.end(), {q.bary[0], q.bary[1], q.bary[2], q.weight});
-        DeviceBuffer<std::uint32_t> d_panels, d_pl, d_fl, d_no, d_ns;
-        DeviceBuffer<double> d_rule;
         d_panels.upload(flux_panels.empty() ? std::vector<std::uint32_t>{0} : flux_panels, s);
         d_pl.upload(tree.pressure_leaf, s);
         d_fl.upload(tree.flux_leaf, s);
         d_no.upload(near_offsets, s);
         d_ns.upload(near_sources.empty() ? std::vector<std::uint32_t>{0} : near_sources, s);
-        d_rule.upload(rule, s);
+    }
+    auto right_hand_sides = [&](const FmmOperator &A, DeviceBuffer<double> &out) {
+        out.resize(nvec);
+        hip_check(hipMemsetAsync(out.data(), 0, out.bytes(), s), "clear FMM rhs");
+        const DeviceNear &dn = A.near();
+        dev::launch_flux_csr_apply(ns.flux_offsets.data(), ns.flux_columns.data(), dn.flux_values.data(), nv,
+                                   ws.fmm_q.data(), static_cast<std::uint32_t>(faces.size()), ne, out.data(), s);
         dev::FmmFarFluxLaunch F{};
         F.geo = geo_.data();
         F.nodes = original_nodes_.data();
@@ -1111,23 +1112,27 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
         F.nv = nv;
         F.nt_all = nt;
         F.flux_panels = d_panels.data();
-        F.flux_count = static_cast<std::uint32_t>(flux_panels.size());
+        F.flux_count = flux_count;
         F.pressure_leaf = d_pl.data();
         F.flux_leaf = d_fl.data();
         F.near_offsets = d_no.data();
         F.near_sources = d_ns.data();
-        F.rule = d_rule.data();
-        F.rule_points = static_cast<std::uint32_t>(rule.size() / 4);
+        F.rule = A.leaf_rule_.data();
+        F.rule_points = A.statistics().rule_points;
         F.k = k;
         F.row_scale = original_row_scale_.data();
         F.q = ws.flux.data();
         F.excitations = ne;
-        F.out = ws.fmm_b.data();
+        F.out = out.data();
         dev::launch_far_flux(F, s);
         hip_check(hipGetLastError(), "far flux launch");
-        hip_check(hipStreamSynchronize(s), "far flux sync");
-    }
-    // GMRES per excitation on the FMM operator, then the FP64 residual of the unscaled equation
+    };
+    right_hand_sides(*op, ws.fmm_b);
+
+    // GMRES per excitation on the FMM operator, then the audits of the solution: the
+    // direct Galerkin audit or the seed/10 operator (with defect corrections), and
+    // the endpoint-quadrature comparison against the next panel rule, promoting the
+    // operator's rule until the comparison holds (support_fmm_solve.cu:526-636)
     const auto t_solve = Clock::now();
     auto krylov = [&](const FmmOperator &A) {
         KrylovOperator kop;
@@ -1136,39 +1141,39 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
             kop.precondition = [&A](const double *v, double *z, hipStream_t st) { A.precondition(v, z, st); };
         return kop;
     };
-    const KrylovOperator kop = krylov(*op);
     auto vec = [&](DeviceBuffer<double> &b, std::uint32_t e) { return b.data() + 2 * static_cast<std::size_t>(e) * nv; };
+    auto wnorm = [&](const double *v) { return weighted_norm(v, nv, original_lumped_.data(), ws, s); };
+    auto ratio = [](double a, double b) { return b > 0 ? a / b : (a == 0 ? 0 : std::numeric_limits<double>::infinity()); };
     std::vector<double> bnorm(ne);
-    for (std::uint32_t e = 0; e < ne; ++e) {
-        bnorm[e] = weighted_norm(vec(ws.fmm_b, e), nv, original_lumped_.data(), ws, s);
-        if (!std::isfinite(bnorm[e]))
-            throw std::runtime_error("Galerkin right-hand side is not finite.");
-        double achieved = 0;
-        gmres(s, kop, nv, vec(ws.fmm_b, e), vec(ws.fmm_x, e), num.fmm_restart, num.fmm_maximum_iterations, 0.1 * tol,
-              &achieved, &iterations[e], cancel, ws.gmres);
-    }
-    auto residuals = [&](const FmmOperator &A) {
+    auto norms_of_b = [&] {
         for (std::uint32_t e = 0; e < ne; ++e) {
-            A.apply(vec(ws.fmm_x, e), vec(ws.fmm_r, e), s);
-            dev::launch_subtract_from(nv, vec(ws.fmm_b, e), vec(ws.fmm_r, e), s);
-            const double rn = weighted_norm(vec(ws.fmm_r, e), nv, original_lumped_.data(), ws, s);
-            relative[e] = bnorm[e] > 0 ? rn / bnorm[e] : (rn == 0 ? 0 : std::numeric_limits<double>::infinity());
+            bnorm[e] = wnorm(vec(ws.fmm_b, e));
+            if (!std::isfinite(bnorm[e]))
+                throw std::runtime_error("Galerkin right-hand side is not finite.");
         }
     };
-    residuals(*op);
-    DeviceBuffer<double> ax(2 * static_cast<std::size_t>(nv));
-    if (op->prefer_direct()) {
-        // direct Galerkin audit: residual of the original operator with the far part
-        // integrated directly, defect corrections on the FMM operator until it holds
-        auto direct_residuals = [&] {
-            for (std::uint32_t e = 0; e < ne; ++e) {
-                op->apply_direct(vec(ws.fmm_x, e), vec(ws.fmm_r, e), s);
-                dev::launch_subtract_from(nv, vec(ws.fmm_b, e), vec(ws.fmm_r, e), s);
-                const double rn = weighted_norm(vec(ws.fmm_r, e), nv, original_lumped_.data(), ws, s);
-                relative[e] = bnorm[e] > 0 ? rn / bnorm[e] : (rn == 0 ? 0 : std::numeric_limits<double>::infinity());
-            }
-        };
-        direct_residuals();
+    norms_of_b();
+    {
+        const KrylovOperator kop = krylov(*op);
+        for (std::uint32_t e = 0; e < ne; ++e) {
+            double achieved = 0;
+            gmres(s, kop, nv, vec(ws.fmm_b, e), vec(ws.fmm_x, e), num.fmm_restart, num.fmm_maximum_iterations,
+                  0.1 * tol, &achieved, &iterations[e], cancel, ws.gmres);
+        }
+    }
+    // r = b - A x per excitation with a given action; relative residuals
+    auto residuals = [&](const auto &action) {
+        for (std::uint32_t e = 0; e < ne; ++e) {
+            action(vec(ws.fmm_x, e), vec(ws.fmm_r, e));
+            dev::launch_subtract_from(nv, vec(ws.fmm_b, e), vec(ws.fmm_r, e), s);
+            relative[e] = ratio(wnorm(vec(ws.fmm_r, e)), bnorm[e]);
+        }
+    };
+    DeviceBuffer<double> ax(2 * static_cast<std::size_t>(nv)), t1(2 * static_cast<std::size_t>(nv)),
+        t2(2 * static_cast<std::size_t>(nv)), b_next;
+    // defect corrections x += A_op^{-1} r while the audited residual exceeds tol
+    auto correct = [&](const FmmOperator &krylov_op, const auto &action) {
+        const KrylovOperator kop = krylov(krylov_op);
         for (unsigned step = 0; *std::max_element(relative.begin(), relative.end()) > tol; ++step) {
             if (step == num.refinement_steps)
                 break; // reported by the residual check below
@@ -1182,50 +1187,95 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
                 dev::launch_add(nv, ax.data(), vec(ws.fmm_x, e), s);
                 iterations[e] += its;
             }
-            direct_residuals();
+            residuals(action);
             fr.fmm_corrected = true;
         }
-        fr.fmm_direct_audit = true;
-        fr.fmm_audit = 0;
-    } else {
-    // seed audit: the same action with a ten times tighter accuracy seed
+    };
     FmmSettings fine = fs;
     fine.accuracy_seed = 0.1 * fs.accuracy_seed;
-    const std::unique_ptr<FmmOperator> audit = build_fmm(num, P.k_max, k, fine, faces, s, op.get());
-    double discrepancy = 0;
-    for (std::uint32_t e = 0; e < ne; ++e) {
-        // (A_seed - A_fine) x = (b - r) - A_fine x
-        audit->apply(vec(ws.fmm_x, e), ax.data(), s);
-        dev::launch_subtract_from(nv, vec(ws.fmm_b, e), ax.data(), s); // b - A_fine x
-        dev::launch_subtract_from(nv, vec(ws.fmm_r, e), ax.data(), s); // r - (b - A_fine x)
-        const double d = weighted_norm(ax.data(), nv, original_lumped_.data(), ws, s);
-        discrepancy = std::max(discrepancy, bnorm[e] > 0 ? d / bnorm[e] : 0.0);
-    }
-    fr.fmm_audit = discrepancy;
-    if (discrepancy > 0.25 * tol) {
-        // defect correction with the finer operator
-        const KrylovOperator fine_op = krylov(*audit);
-        for (std::uint32_t e = 0; e < ne; ++e) {
-            audit->apply(vec(ws.fmm_x, e), vec(ws.fmm_r, e), s);
-            dev::launch_subtract_from(nv, vec(ws.fmm_b, e), vec(ws.fmm_r, e), s);
-            double achieved = 0;
-            std::uint32_t its = 0;
-            gmres(s, fine_op, nv, vec(ws.fmm_r, e), ax.data(), num.fmm_restart, num.fmm_maximum_iterations, 0.1 * tol,
-                  &achieved, &its, cancel, ws.gmres);
-            dev::launch_add(nv, ax.data(), vec(ws.fmm_x, e), s);
-            iterations[e] += its;
+    std::unique_ptr<FmmOperator> promoted; // the operator after endpoint-rule promotions
+    for (;;) {
+        const FmmOperator &A = promoted ? *promoted : *op;
+        const bool direct = A.prefer_direct();
+        std::unique_ptr<FmmOperator> audit;
+        std::function<void(const double *, double *)> audited;
+        if (direct) {
+            audited = [&](const double *x, double *y) { A.apply_direct(x, y, s); };
+            residuals(audited);
+            correct(A, audited);
+            fr.fmm_audit = 0;
+        } else {
+            // seed audit: the same action with a ten times tighter accuracy seed
+            residuals([&](const double *x, double *y) { A.apply(x, y, s); });
+            audit = build_fmm(num, P.k_max, k, fine, faces, s, &A, A.statistics().endpoint_degree);
+            double discrepancy = 0;
+            for (std::uint32_t e = 0; e < ne; ++e) {
+                // (A_seed - A_fine) x = (b - r) - A_fine x
+                audit->apply(vec(ws.fmm_x, e), ax.data(), s);
+                dev::launch_subtract_from(nv, vec(ws.fmm_b, e), ax.data(), s);
+                dev::launch_subtract_from(nv, vec(ws.fmm_r, e), ax.data(), s);
+                discrepancy = std::max(discrepancy, ratio(wnorm(ax.data()), bnorm[e]));
+            }
+            fr.fmm_audit = std::max(fr.fmm_audit, discrepancy);
+            audited = [&](const double *x, double *y) { audit->apply(x, y, s); };
+            if (discrepancy > 0.25 * tol) {
+                residuals(audited);
+                correct(*audit, audited);
+            }
         }
-        residuals(*audit);
-        fr.fmm_corrected = true;
+        fr.fmm_direct_audit = direct;
+        // endpoint-quadrature comparison: the audited action and B at the next rule
+        const unsigned degree = A.statistics().endpoint_degree, next = refined_endpoint_degree(degree);
+        if (!next)
+            throw std::runtime_error("FMM endpoint quadrature has no independent refinement rule.");
+        const std::unique_ptr<FmmOperator> cmp =
+            build_fmm(num, P.k_max, k, direct ? fs : fine, faces, s, &A, next);
+        right_hand_sides(*cmp, b_next);
+        double quad = 0, next_residual = 0;
+        for (std::uint32_t e = 0; e < ne; ++e) {
+            if (direct)
+                cmp->apply_direct(vec(ws.fmm_x, e), t1.data(), s);
+            else
+                cmp->apply(vec(ws.fmm_x, e), t1.data(), s);
+            audited(vec(ws.fmm_x, e), t2.data());
+            dev::launch_subtract_from(nv, t1.data(), t2.data(), s); // A_next x - A x
+            const double da = wnorm(t2.data());
+            hip_check(hipMemcpyAsync(t2.data(), vec(b_next, e), t2.bytes(), hipMemcpyDeviceToDevice, s), "copy");
+            dev::launch_subtract_from(nv, vec(ws.fmm_b, e), t2.data(), s); // b_next - b
+            const double db = wnorm(t2.data());
+            const double bn = wnorm(vec(b_next, e));
+            dev::launch_subtract_from(nv, vec(b_next, e), t1.data(), s); // b_next - A_next x
+            quad = std::max(quad, ratio(da + db, bn));
+            next_residual = std::max(next_residual, ratio(wnorm(t1.data()), bn));
+        }
+        fr.fmm_quadrature_discrepancy = quad;
+        fr.fmm_endpoint_degree = degree;
+        if (quad <= 0.025 * tol && next_residual <= tol)
+            break;
+        if (degree >= 12) {
+            std::ostringstream msg;
+            msg.precision(6);
+            msg << "FMM endpoint quadrature " << degree << "/" << next << " did not converge: A/B action difference "
+                << quad << " exceeds " << 0.025 * tol << " or the higher-rule residual " << next_residual
+                << " exceeds " << tol << "; refine the mesh.";
+            throw std::runtime_error(msg.str());
+        }
+        // promote the operator's rule; the right-hand side follows the new rule
+        promoted = build_fmm(num, P.k_max, k, fs, faces, s, &A, next);
+        ++fr.fmm_promotions;
+        right_hand_sides(*promoted, ws.fmm_b);
+        norms_of_b();
+        residuals([&](const double *x, double *y) { promoted->apply(x, y, s); });
+        correct(*promoted, [&](const double *x, double *y) { promoted->apply(x, y, s); });
     }
-    }
+    const FmmOperator &final_op = promoted ? *promoted : *op;
     fr.ms[1] = ms_since(t_solve);
     {
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "event");
         hip_check(hipEventCreate(&e1), "event");
         hip_check(hipEventRecord(e0, s), "event");
-        op->apply(ws.fmm_x.data(), ax.data(), s);
+        final_op.apply(ws.fmm_x.data(), ax.data(), s);
         hip_check(hipEventRecord(e1, s), "event");
         hip_check(hipEventSynchronize(e1), "event sync");
         float ms = 0;

@@ -30,10 +30,11 @@ inline constexpr std::size_t kMaximumDirectAuditPanels = 20000;
 struct FmmSettings {
     SupportTreeSettings tree{64, 20, 1.5, 2.0};
     double accuracy_seed{5e-7}; // Eq. 18 accuracy of the level orders
+    unsigned endpoint_degree{};  // leaf panel rule degree; 0: the electrical minimum
 };
 
 struct FmmStatistics {
-    std::uint32_t nodes{}, levels{}, leaves{}, top_level{}, rule_points{};
+    std::uint32_t nodes{}, levels{}, leaves{}, top_level{}, rule_points{}, endpoint_degree{};
     std::uint64_t near_nnz{}, far_pairs{}, m2m_links{}, m2l_links{}, l2l_links{}, rotations{}, axial_plans{};
     std::uint64_t expansion_values{}, moment_values{}, device_bytes{};
     std::uint64_t plane_wave_levels{}, plane_wave_leaves{}, angular_links{}, diagonal_classes{};

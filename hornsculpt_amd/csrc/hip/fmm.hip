@@ -25,7 +25,8 @@
 namespace hsbem::dev {
 namespace {
 
-constexpr int kGroup = 16;      // lanes per (leaf, panel) item: rules of at most 16 points
+constexpr int kGroupSmall = 16; // lanes per (leaf, panel) item: rules of at most 16 points
+constexpr int kGroupLarge = 64; // one wavefront per item: rules of at most 64 points (degrees 12, 14)
 constexpr int kMillerMax = 256; // Miller recurrence start bound (z <= 96 at the leaves)
 
 struct C2 {
@@ -105,7 +106,7 @@ __device__ bool normalized_regular(int top, double z, double *J) {
     return true;
 }
 
-__device__ __forceinline__ double group_sum(double v) {
+template <int kGroup> __device__ __forceinline__ double group_sum(double v) {
 #pragma unroll
     for (int off = kGroup / 2; off > 0; off >>= 1)
         v += __shfl_xor(v, off, kGroup);
@@ -117,8 +118,8 @@ __device__ __forceinline__ void atomic_add_c(double *p, C2 v) {
     atomicAdd(p + 1, v.im);
 }
 
-// One 16-lane group per (leaf, support panel) item, lane = rule point.
-__global__ __launch_bounds__(64) void leaf_moments(FmmLeafMomentsLaunch L) {
+// One kGroup-lane group per (leaf, support panel) item, lane = rule point.
+template <int kGroup> __global__ __launch_bounds__(64) void leaf_moments(FmmLeafMomentsLaunch L) {
     const int lane = threadIdx.x % kGroup;
     const std::uint32_t item = blockIdx.x * (64 / kGroup) + threadIdx.x / kGroup;
     const bool live_item = item < L.item_count;
@@ -191,10 +192,10 @@ __global__ __launch_bounds__(64) void leaf_moments(FmmLeafMomentsLaunch L) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const double s = bc[c];
-                const C2 sums[4] = {{group_sum(s * e.re), group_sum(s * e.im)},
-                                    {group_sum(s * tt.re), group_sum(s * tt.im)},
-                                    {group_sum(s * em.re), group_sum(s * em.im)},
-                                    {group_sum(s * tm.re), group_sum(s * tm.im)}};
+                const C2 sums[4] = {{group_sum<kGroup>(s * e.re), group_sum<kGroup>(s * e.im)},
+                                    {group_sum<kGroup>(s * tt.re), group_sum<kGroup>(s * tt.im)},
+                                    {group_sum<kGroup>(s * em.re), group_sum<kGroup>(s * em.im)},
+                                    {group_sum<kGroup>(s * tm.re), group_sum<kGroup>(s * tm.im)}};
                 const std::int32_t j = it.corner[c];
                 if (live_item && lane == 0 && j >= 0) {
                     const std::size_t row = static_cast<std::size_t>(j) * nc;
@@ -891,9 +892,13 @@ __global__ void permute_vectors(const double *in, double *out, const std::uint32
 void launch_leaf_moments(const FmmLeafMomentsLaunch &launch, hipStream_t stream) {
     if (!launch.item_count)
         return;
-    const std::uint32_t per_block = 64 / kGroup;
-    hipLaunchKernelGGL(leaf_moments, dim3((launch.item_count + per_block - 1) / per_block), dim3(64), 0, stream,
-                       launch);
+    if (launch.rule_points <= static_cast<std::uint32_t>(kGroupSmall)) {
+        const std::uint32_t per_block = 64 / kGroupSmall;
+        hipLaunchKernelGGL(leaf_moments<kGroupSmall>, dim3((launch.item_count + per_block - 1) / per_block), dim3(64),
+                           0, stream, launch);
+    } else {
+        hipLaunchKernelGGL(leaf_moments<kGroupLarge>, dim3(launch.item_count), dim3(64), 0, stream, launch);
+    }
 }
 
 void launch_leaf_emit(const FmmLeaf *leaves, std::uint32_t leaf_count, const std::uint32_t *leaf_dofs,

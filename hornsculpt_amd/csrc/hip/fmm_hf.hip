@@ -286,7 +286,7 @@ __global__ __launch_bounds__(256) void hf_leaf_emit(HfLeafLaunch L, const double
 }
 
 // Block per item: g_q = sum_d w_d (n.d - 1) exp(-i k d.(y_q - c)) L(d) for every
-// rule point q of the panel, reduced over the block.
+// rule point q of the panel (16 points per pass), reduced over the block.
 __global__ __launch_bounds__(256) void hf_leaf_field(HfLeafLaunch L) {
     constexpr int kPoints = 16;
     __shared__ double part[2][kPoints][4]; // per point, per wavefront partial (re, im)
@@ -294,58 +294,61 @@ __global__ __launch_bounds__(256) void hf_leaf_field(HfLeafLaunch L) {
     const HfLeaf leaf = L.leaves[it.leaf];
     const HfRule R = L.rules[leaf.rule];
     const int samples = static_cast<int>(R.rings * R.azimuths), Q = static_cast<int>(L.rule_points);
-    double y[kPoints][3];
-#pragma unroll
-    for (int q = 0; q < kPoints; ++q) {
-        const int qq = q < Q ? q : 0;
-        const double b1 = L.rule[4 * qq + 1], b2 = L.rule[4 * qq + 2];
-#pragma unroll
-        for (int x = 0; x < 3; ++x)
-            y[q][x] = it.v0[x] + b1 * it.e1[x] + b2 * it.e2[x] - leaf.center[x];
-    }
-    C2 acc[kPoints];
-#pragma unroll
-    for (int q = 0; q < kPoints; ++q)
-        acc[q] = {0, 0};
     const C2 *Lc = reinterpret_cast<const C2 *>(L.expansions) + leaf.local;
     const double k = L.k;
-    for (int s = threadIdx.x; s < samples; s += blockDim.x) {
-        double d[3], w;
-        sample_direction(L.polar, R, s, d, w);
-        const double nd = it.normal[0] * d[0] + it.normal[1] * d[1] + it.normal[2] * d[2];
-        const C2 a = (w * (nd - 1)) * Lc[s];
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    for (int q0 = 0; q0 < Q; q0 += kPoints) {
+        double y[kPoints][3];
 #pragma unroll
         for (int q = 0; q < kPoints; ++q) {
-            if (q < Q) {
-                double sn, cs;
-                sincos(-k * (d[0] * y[q][0] + d[1] * y[q][1] + d[2] * y[q][2]), &sn, &cs);
-                acc[q] = acc[q] + C2{cs, sn} * a;
+            const int qq = q0 + q < Q ? q0 + q : 0;
+            const double b1 = L.rule[4 * qq + 1], b2 = L.rule[4 * qq + 2];
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+                y[q][x] = it.v0[x] + b1 * it.e1[x] + b2 * it.e2[x] - leaf.center[x];
+        }
+        C2 acc[kPoints];
+#pragma unroll
+        for (int q = 0; q < kPoints; ++q)
+            acc[q] = {0, 0};
+        for (int s = threadIdx.x; s < samples; s += blockDim.x) {
+            double d[3], w;
+            sample_direction(L.polar, R, s, d, w);
+            const double nd = it.normal[0] * d[0] + it.normal[1] * d[1] + it.normal[2] * d[2];
+            const C2 a = (w * (nd - 1)) * Lc[s];
+#pragma unroll
+            for (int q = 0; q < kPoints; ++q) {
+                if (q0 + q < Q) {
+                    double sn, cs;
+                    sincos(-k * (d[0] * y[q][0] + d[1] * y[q][1] + d[2] * y[q][2]), &sn, &cs);
+                    acc[q] = acc[q] + C2{cs, sn} * a;
+                }
             }
         }
-    }
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+        __syncthreads(); // part[] of the previous pass consumed
 #pragma unroll
-    for (int q = 0; q < kPoints; ++q) {
-        double re = acc[q].re, im = acc[q].im;
+        for (int q = 0; q < kPoints; ++q) {
+            double re = acc[q].re, im = acc[q].im;
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            re += __shfl_xor(re, off, 64);
-            im += __shfl_xor(im, off, 64);
+            for (int off = 32; off > 0; off >>= 1) {
+                re += __shfl_xor(re, off, 64);
+                im += __shfl_xor(im, off, 64);
+            }
+            if (lane == 0) {
+                part[0][q][wave] = re;
+                part[1][q][wave] = im;
+            }
         }
-        if (lane == 0) {
-            part[0][q][wave] = re;
-            part[1][q][wave] = im;
+        __syncthreads();
+        if (threadIdx.x < kPoints && q0 + static_cast<int>(threadIdx.x) < Q) {
+            const int q = threadIdx.x;
+            double re = 0, im = 0;
+            for (int v = 0; v < static_cast<int>(blockDim.x) / 64; ++v) {
+                re += part[0][q][v];
+                im += part[1][q][v];
+            }
+            reinterpret_cast<C2 *>(L.field)[static_cast<std::size_t>(blockIdx.x) * Q + q0 + q] = {re, im};
         }
-    }
-    __syncthreads();
-    if (threadIdx.x < Q) {
-        const int q = threadIdx.x;
-        double re = 0, im = 0;
-        for (int v = 0; v < static_cast<int>(blockDim.x) / 64; ++v) {
-            re += part[0][q][v];
-            im += part[1][q][v];
-        }
-        reinterpret_cast<C2 *>(L.field)[static_cast<std::size_t>(blockIdx.x) * Q + q] = {re, im};
     }
 }
 

@@ -38,7 +38,7 @@ struct FmmLeafMomentsLaunch {
     const double *geo;            // panel SoA (original numbering)
     std::uint32_t nt;
     const double *rule;           // rule_points x (b0, b1, b2, w), weights sum to 1/2
-    std::uint32_t rule_points;    // <= 16
+    std::uint32_t rule_points;    // <= 64
     double k, alpha_re, alpha_im; // Burton-Miller alpha = -i/k
     const double *row_scale;      // 1 / lumped mass, original numbering
     const FmmLeaf *leaves;
@@ -197,7 +197,7 @@ struct HfLeafLaunch {
     const HfRule *rules;
     const double *polar;
     const double *rule;              // panel rule: rule_points x (b0, b1, b2, w)
-    std::uint32_t rule_points;       // <= 16
+    std::uint32_t rule_points;       // <= 64
     std::uint32_t max_samples;       // largest rule sample count of the leaves
     double k;
     double *field;                   // workspace: item_count x rule_points complex
@@ -226,7 +226,7 @@ struct FmmDirectLaunch {
     const std::uint8_t *near;          // leaf_count x leaf_count: near[target * leaf_count + source]
     std::uint32_t leaf_count;
     const double *rule;                // rule_points x (b0, b1, b2, w), weights sum to 1/2
-    std::uint32_t rule_points;         // <= 16
+    std::uint32_t rule_points;         // <= 64
     double k;
     const double *row_scale;           // 1 / lumped mass
     const double *x;                   // complex, nv

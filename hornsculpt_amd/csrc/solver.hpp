@@ -138,6 +138,8 @@ struct FrequencyResult {
     // whether the defect correction with the finer operator ran, and the operator's apply time
     double fmm_audit{}, fmm_apply_ms{};
     bool fmm_corrected{}, fmm_direct_audit{};
+    double fmm_quadrature_discrepancy{};
+    unsigned fmm_endpoint_degree{}, fmm_promotions{};
     std::uint64_t fmm_plane_wave_levels{}, fmm_m2l_links{}, fmm_near_nnz{}, fmm_device_bytes{};
 };
 
@@ -305,7 +307,7 @@ class MeshSolver {
     // FMM operator on a caller stream; share: reuse its near entries and preconditioner.
     std::unique_ptr<FmmOperator> build_fmm(const Numerics &n, double k_max, double k, const FmmSettings &settings,
                                            const std::vector<std::uint32_t> &faces, hipStream_t stream,
-                                           const FmmOperator *share);
+                                           const FmmOperator *share, unsigned endpoint_degree = 0);
     // One frequency point of the FMM form: right-hand sides, GMRES, residuals and the
     // seed audit; leaves the pressures in ws.x (internal order).
     void fmm_point(const PreparedRequest &P, Workspace &ws, double k, double omega, std::vector<double> &relative,

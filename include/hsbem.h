@@ -169,11 +169,12 @@ HSBEM_API int hsbem_result_residual(const hsbem_result *result, uint32_t frequen
 HSBEM_API int hsbem_result_timings(const hsbem_result *result, uint32_t frequency, double *milliseconds);
 HSBEM_API int hsbem_result_peak_device_bytes(const hsbem_result *result, uint64_t *bytes);
 /* Fast multipole evidence of a frequency point (BemFrequencyResult's FMM statistics,
- * bem.hpp:193-332; zeros for the dense form): out[8] = seed-audit discrepancy
+ * bem.hpp:193-332; zeros for the dense form): out[11] = seed-audit discrepancy
  * (relative, against the seed/10 operator; 0 under the direct audit), apply
  * milliseconds, defect correction applied (0/1), plane-wave levels, M2L links, near
  * CSR entries, operator device bytes, residual evaluated by the direct Galerkin
- * audit (0/1). */
+ * audit (0/1), endpoint-quadrature discrepancy against the next panel rule (A and B
+ * actions, relative), endpoint rule degree, endpoint-rule promotions. */
 HSBEM_API int hsbem_result_fmm(const hsbem_result *result, uint32_t frequency, double *out);
 
 /* ---- serving loop (no counterpart class in the reference: its callers run
@@ -313,6 +314,7 @@ typedef struct hsbem_fmm_settings {
     uint32_t leaf_capacity, maximum_depth;
     double separation, near_ratio;
     double accuracy_seed;
+    uint32_t endpoint_degree; /* leaf panel rule degree (4, 6, 8, 12, 14); 0: minimumEndpointDegree */
 } hsbem_fmm_settings;
 typedef struct hsbem_fmm_operator hsbem_fmm_operator;
 HSBEM_API void hsbem_fmm_settings_default(hsbem_fmm_settings *settings);

@@ -91,6 +91,30 @@ def test_direct_audit_action_matches_dense_horn():
     op.close()
 
 
+@pytest.mark.parametrize("degree", [12, 14])
+def test_fmm_apply_with_promoted_endpoint_rule(degree):
+    """Leaf panel rules of degree 12 and 14 (49 and 64 points: one wavefront per
+    (leaf, panel) item) give the same action as the minimum rule to the FMM accuracy
+    and match the dense matrix; the direct audit at the same rule agrees."""
+    mesh = meshes.exponential_horn(0.012)
+    k = 2 * math.pi * 2500.0 / 343.0
+    num = bem.BemNumerics()
+    solver = bem.HipBemSolver(mesh)
+    A, _ = solver.assemble(num, k, k)
+    base = solver.fmm_operator(num, k, k, leaf_capacity=32)
+    high = solver.fmm_operator(num, k, k, leaf_capacity=32, endpoint_degree=degree)
+    assert high.statistics()["rule_points"] > 16
+    x = np.random.default_rng(9).standard_normal(A.shape[0]) + 1j * np.random.default_rng(10).standard_normal(A.shape[0])
+    yd, yb, yh, yx = A @ x, base.apply(x), high.apply(x), high.apply_direct(x)
+    print(f"degree {degree} ({high.statistics()['rule_points']} points): vs dense {_rel(yh, yd):.2e}, vs minimum rule "
+          f"{_rel(yh, yb):.2e}, direct audit {_rel(yx, yh):.2e}")
+    assert _rel(yh, yd) <= 2e-6 and _rel(yh, yb) <= 1e-6 and _rel(yx, yh) <= 1e-6
+    with pytest.raises(ValueError, match="endpoint degree"):
+        solver.fmm_operator(num, k, k, endpoint_degree=10)
+    base.close()
+    high.close()
+
+
 def test_fmm_gmres_reproduces_dense_solution():
     mesh = meshes.exponential_horn(0.012)
     k = 2 * math.pi * 1500.0 / 343.0
@@ -147,6 +171,7 @@ def test_fmm_request_matches_dense_request():
         assert ef.iterations > 0
         print("  FMM evidence", ff.fmm)
         assert ff.fmm is not None and ff.fmm["m2l_links"] > 0
+        assert ff.fmm["quadrature_discrepancy"] <= 0.025 * 1e-6 and ff.fmm["endpoint_degree"] >= 4
 
 
 def test_fmm_request_validation():
