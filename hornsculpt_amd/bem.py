@@ -166,6 +166,11 @@ def library() -> ctypes.CDLL:
                 "hsbem_fmm_operator_levels": (ctypes.c_int, [P, ctypes.POINTER(U64), ctypes.POINTER(U32),
                                                              ctypes.POINTER(U32), ctypes.POINTER(U32)]),
                 "hsbem_fmm_operator_set_graph": (ctypes.c_int, [P, ctypes.c_int32]),
+                "hsbem_plan_cache_create": (ctypes.c_int, [U64, ctypes.POINTER(P)]),
+                "hsbem_plan_cache_axial": (ctypes.c_int, [P, ctypes.c_int32, U32, U32] + [ctypes.c_double] * 4 +
+                                           [U32, ctypes.c_double, ctypes.POINTER(U64), D]),
+                "hsbem_plan_cache_statistics": (ctypes.c_int, [P, ctypes.POINTER(U64), D]),
+                "hsbem_plan_cache_destroy": (ctypes.c_int, [P]),
                 "hsbem_angular_transfer_device": (ctypes.c_int, [P, ctypes.c_int32, U32, U32, U32, ctypes.c_double,
                                                                  ctypes.c_double, D, D, D]),
                 "hsbem_fmm_operator_destroy": (ctypes.c_int, [P]),
@@ -774,11 +779,11 @@ class HipFmmOperator:
         out = {k: int(v) for k, v in zip(self.STAT_FIELDS, counts)}
         out.update(dict(zip(("tree_ms", "near_ms", "plans_ms", "moments_ms", "near_kernel_ms"), times.tolist())))
         out["level_orders"] = orders[:nl.value].tolist()
-        counts4, rep, rules = np.zeros(4, np.uint64), np.zeros(64, np.uint32), np.zeros(64, np.uint32)
+        counts4, rep, rules = np.zeros(6, np.uint64), np.zeros(64, np.uint32), np.zeros(64, np.uint32)
         _check(self._lib.hsbem_fmm_operator_levels(self.handle, counts4.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
                                                    _uptr(rep), _uptr(rules), ctypes.byref(nl)))
         out.update(plane_wave_levels=int(counts4[0]), angular_links=int(counts4[1]), diagonal_classes=int(counts4[2]),
-                   plane_wave_leaves=int(counts4[3]),
+                   plane_wave_leaves=int(counts4[3]), plan_cache_hits=int(counts4[4]), plans_built=int(counts4[5]),
                    level_representation=rep[:nl.value].tolist(), level_rule_orders=rules[:nl.value].tolist())
         return out
 
@@ -971,6 +976,43 @@ def diagonal_translation(rule_order: int, expansion_order: int, k: float, target
 
 
 AXIAL_MULTIPOLE_TO_PARENT, AXIAL_MULTIPOLE_TO_LOCAL, AXIAL_LOCAL_TO_CHILD = 0, 1, 2
+
+
+class TranslationPlanCache:
+    """Bounded LRU of coaxial translation plans keyed by the exact request (host)."""
+
+    def __init__(self, maximum_bytes: int = 512 << 20):
+        self._lib = library()
+        self.handle = ctypes.c_void_p()
+        _check(self._lib.hsbem_plan_cache_create(int(maximum_bytes), ctypes.byref(self.handle)))
+
+    def axial(self, kind: int, source_order: int, target_order: int, k: float, source_radius: float,
+              target_radius: float, distance: float, polar_count: int, projection_radius: float = 0.0) -> np.ndarray:
+        # one call per request: the block sizes follow from the orders
+        n = sum((target_order - m + 1) * (source_order - m + 1) for m in range(min(source_order, target_order) + 1))
+        count, out = ctypes.c_uint64(), np.zeros(2 * n)
+        _check(self._lib.hsbem_plan_cache_axial(self.handle, kind, source_order, target_order, float(k),
+                                                float(source_radius), float(target_radius), float(distance),
+                                                polar_count, float(projection_radius), ctypes.byref(count),
+                                                _dptr(out)))
+        assert count.value == n
+        return out.view(np.complex128)
+
+    def statistics(self) -> dict:
+        out, ms = np.zeros(5, np.uint64), np.zeros(1)
+        _check(self._lib.hsbem_plan_cache_statistics(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                                     _dptr(ms)))
+        return dict(zip(("hits", "plans_built", "evictions", "live_bytes", "peak_bytes"), map(int, out)),
+                    builder_ms=float(ms[0]))
+
+    def close(self):
+        if self.handle:
+            self._lib.hsbem_plan_cache_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            self.close()
 
 
 def _waves(raw: np.ndarray):

@@ -40,6 +40,10 @@ struct hsbem_fmm_operator {
     hsbem::GmresWorkspace gmres;
     hsbem::DeviceBuffer<double> x, y;
 };
+struct hsbem_plan_cache {
+    hsbem::TranslationPlanCache cache;
+    explicit hsbem_plan_cache(std::size_t bytes) : cache(bytes) {}
+};
 struct hsbem_server {
     std::unique_ptr<hsbem::Server> server;
 };
@@ -841,6 +845,8 @@ int hsbem_fmm_operator_levels(const hsbem_fmm_operator *o, uint64_t *counts, uin
         counts[1] = st.angular_links;
         counts[2] = st.diagonal_classes;
         counts[3] = st.plane_wave_leaves;
+        counts[4] = st.plan_cache_hits;
+        counts[5] = st.plans_built;
         if (representation)
             std::copy(st.level_representation.begin(), st.level_representation.end(), representation);
         if (rule_orders)
@@ -1022,6 +1028,45 @@ int hsbem_angular_transfer_device(hsbem_context *c, int32_t kind, uint32_t sourc
                                                    {shift[0], shift[1], shift[2]}, v, c->ctx.stream),
                     out);
     });
+}
+
+int hsbem_plan_cache_create(uint64_t maximum_bytes, hsbem_plan_cache **out) {
+    return guarded([&] {
+        require(out && maximum_bytes, "invalid plan cache arguments");
+        *out = new hsbem_plan_cache(static_cast<std::size_t>(maximum_bytes));
+    });
+}
+
+int hsbem_plan_cache_axial(hsbem_plan_cache *c, int32_t kind, uint32_t ps, uint32_t pt, double k, double as, double at,
+                           double distance, uint32_t polar_count, double projection_radius, uint64_t *count,
+                           double *coefficients) {
+    return guarded([&] {
+        require(c && count, "null plan cache argument");
+        require(kind >= HSBEM_AXIAL_MULTIPOLE_TO_PARENT && kind <= HSBEM_AXIAL_LOCAL_TO_CHILD, "unknown axial kind");
+        const auto plan = c->cache.axial({static_cast<hsbem::AxialKind>(kind), ps, pt, polar_count, k, as, at,
+                                          distance, projection_radius});
+        *count = plan->coefficients.size();
+        if (coefficients)
+            complex_out(plan->coefficients, coefficients);
+    });
+}
+
+int hsbem_plan_cache_statistics(const hsbem_plan_cache *c, uint64_t *out, double *builder_ms) {
+    return guarded([&] {
+        require(c && out, "null plan cache argument");
+        const auto st = c->cache.statistics();
+        out[0] = st.hits;
+        out[1] = st.plans_built;
+        out[2] = st.evictions;
+        out[3] = st.live_bytes;
+        out[4] = st.peak_bytes;
+        if (builder_ms)
+            *builder_ms = st.builder_ms;
+    });
+}
+
+int hsbem_plan_cache_destroy(hsbem_plan_cache *c) {
+    return guarded([&] { delete c; });
 }
 
 int hsbem_harmonic_spectrum(uint32_t order, uint32_t rule_order, int32_t direction, const double *in, double *out) {

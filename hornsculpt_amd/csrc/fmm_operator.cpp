@@ -683,13 +683,15 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
                                     static_cast<std::uint32_t>(direction_order[chunks[c].rotation])});
 
     // ---- rotation and coaxial plans (host, in parallel)
-    std::vector<RotationPlan> rotations(directions.size());
+    const TranslationPlanCacheStatistics cache_before = translation_plans_.statistics();
+    std::vector<std::shared_ptr<const RotationPlan>> rotations(directions.size());
     parallel_for(directions.size(), [&](std::size_t i) {
         const Triple &d = directions[i];
-        rotations[i] = make_rotation_plan(static_cast<unsigned>(direction_order[i]),
-                                          {static_cast<double>(d[0]), static_cast<double>(d[1]), static_cast<double>(d[2])});
+        rotations[i] = translation_plans_.rotation(
+            static_cast<unsigned>(direction_order[i]),
+            {static_cast<double>(d[0]), static_cast<double>(d[1]), static_cast<double>(d[2])});
     });
-    std::vector<AxialPlan> plans(axial_keys.size());
+    std::vector<std::shared_ptr<const AxialPlan>> plans(axial_keys.size());
     parallel_for(axial_keys.size(), [&](std::size_t i) {
         const AxialKey &key = axial_keys[i];
         const int ps = order[key.source_level], pt = order[key.target_level];
@@ -699,15 +701,21 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
                                 ? 2 * half(key.source_level)
                                 : half(std::max(key.source_level, key.target_level));
         const double distance = std::sqrt(static_cast<double>(key.length2)) * unit;
-        plans[i] = make_axial_plan(static_cast<AxialKind>(key.kind), static_cast<unsigned>(ps), static_cast<unsigned>(pt), k,
-                                   radius[key.source_level], radius[key.target_level], distance,
-                                   2 * static_cast<unsigned>(std::max(ps, pt)) + 9);
+        plans[i] = translation_plans_.axial({static_cast<AxialKind>(key.kind), static_cast<unsigned>(ps),
+                                             static_cast<unsigned>(pt), 2 * static_cast<unsigned>(std::max(ps, pt)) + 9,
+                                             k, radius[key.source_level], radius[key.target_level], distance, 0.0});
     });
+    {
+        const TranslationPlanCacheStatistics after = translation_plans_.statistics();
+        st.plan_cache_hits = after.hits - cache_before.hits;
+        st.plans_built = after.plans_built - cache_before.plans_built;
+    }
     st.rotations = rotations.size();
     st.axial_plans = plans.size();
     std::vector<dev::FmmRotation> rot_table;
     std::vector<double> small_d, azimuth;
-    for (const RotationPlan &r : rotations) {
+    for (const auto &rp : rotations) {
+        const RotationPlan &r = *rp;
         rot_table.push_back({small_d.size(), azimuth.size() / 2, r.order, 0});
         small_d.insert(small_d.end(), r.small_d.begin(), r.small_d.end());
         for (const auto &z : r.azimuth) {
@@ -717,9 +725,9 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
     }
     std::vector<std::uint64_t> axial_offsets;
     std::vector<double> axial_values;
-    for (const AxialPlan &p : plans) {
+    for (const auto &pp : plans) {
         axial_offsets.push_back(axial_values.size() / 2);
-        for (const auto &z : p.coefficients) {
+        for (const auto &z : pp->coefficients) {
             axial_values.push_back(z.real());
             axial_values.push_back(z.imag());
         }

@@ -38,6 +38,7 @@ struct FmmStatistics {
     std::uint64_t near_nnz{}, far_pairs{}, m2m_links{}, m2l_links{}, l2l_links{}, rotations{}, axial_plans{};
     std::uint64_t expansion_values{}, moment_values{}, device_bytes{};
     std::uint64_t plane_wave_levels{}, plane_wave_leaves{}, angular_links{}, diagonal_classes{};
+    std::uint64_t plan_cache_hits{}, plans_built{}; // translation plans reused / built by this operator
     std::vector<std::uint32_t> level_orders, level_representation, level_rule_orders; // representation 1 LF, 2 HF
     double tree_ms{}, near_ms{}, plans_ms{}, moments_ms{}, near_kernel_ms{};
 };

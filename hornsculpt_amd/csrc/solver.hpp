@@ -9,6 +9,7 @@
 #include "host/plan.hpp"
 #include "host/space.hpp"
 #include "host/support_tree.hpp"
+#include "host/translation_plan_cache.hpp"
 #include "hip/kernels.hpp"
 
 #include <hip/hip_runtime_api.h>
@@ -337,6 +338,7 @@ class MeshSolver {
     using NearKey = std::tuple<unsigned, unsigned, double, double, std::vector<std::uint32_t>, unsigned, unsigned,
                                double, double, unsigned, double>;
     std::shared_ptr<const NearStructure> near_cache_;
+    TranslationPlanCache translation_plans_; // FMM rotation and coaxial plans of this mesh (thread safe)
     NearKey near_cache_key_;
     std::mutex mutex_;                     // one direct solve per mesh at a time (as the reference)
 };

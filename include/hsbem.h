@@ -345,8 +345,9 @@ HSBEM_API int hsbem_fmm_operator_time_apply(hsbem_fmm_operator *op, uint32_t rep
  * orders[level_count] = expansion order per level (0: inactive). */
 HSBEM_API int hsbem_fmm_operator_statistics(const hsbem_fmm_operator *op, uint64_t *counts, double *times,
                                             uint32_t *orders, uint32_t *level_count);
-/* Plane-wave levels of an FMM operator: counts[4] = plane-wave levels, angular
- * transfer links, diagonal translation classes, leaves on plane-wave levels; per level
+/* Plane-wave levels of an FMM operator: counts[6] = plane-wave levels, angular
+ * transfer links, diagonal translation classes, leaves on plane-wave levels,
+ * translation plans taken from the mesh's plan cache, plans built; per level
  * representation[level_count] (0 inactive, 1 scaled spherical, 2 plane wave) and
  * rule_orders[level_count] (sphere rule order of a plane-wave level, else 0). */
 HSBEM_API int hsbem_fmm_operator_levels(const hsbem_fmm_operator *op, uint64_t *counts, uint32_t *representation,
@@ -459,6 +460,19 @@ HSBEM_API int hsbem_scaled_spectrum(uint32_t order, double radius, double k, uin
 HSBEM_API int hsbem_angular_transfer_device(hsbem_context *ctx, int32_t kind, uint32_t source_order,
                                             uint32_t target_order, uint32_t bandlimit, double k, double radius,
                                             const double *shift, const double *in, double *out);
+/* TranslationPlanCache (modules/bem/fmm/translation_plan_cache.hpp:23-42): a bounded
+ * LRU of coaxial plans keyed by the exact request. `axial` returns the plan of
+ * hsbem_axial_plan's arguments (coefficients when given, their count); statistics
+ * out[5] = hits, plans built, evictions, live bytes, peak bytes; *builder_ms the
+ * summed build time. */
+typedef struct hsbem_plan_cache hsbem_plan_cache;
+HSBEM_API int hsbem_plan_cache_create(uint64_t maximum_bytes, hsbem_plan_cache **out);
+HSBEM_API int hsbem_plan_cache_axial(hsbem_plan_cache *cache, int32_t kind, uint32_t source_order,
+                                     uint32_t target_order, double k, double source_radius, double target_radius,
+                                     double distance, uint32_t polar_count, double projection_radius,
+                                     uint64_t *coefficient_count, double *coefficients);
+HSBEM_API int hsbem_plan_cache_statistics(const hsbem_plan_cache *cache, uint64_t *out, double *builder_ms);
+HSBEM_API int hsbem_plan_cache_destroy(hsbem_plan_cache *cache);
 #define HSBEM_AXIAL_MULTIPOLE_TO_PARENT 0
 #define HSBEM_AXIAL_MULTIPOLE_TO_LOCAL 1
 #define HSBEM_AXIAL_LOCAL_TO_CHILD 2

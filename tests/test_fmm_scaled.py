@@ -213,3 +213,33 @@ def test_scaled_spectra_match_the_unscaled_conventions():
     spec = bem.multipole_to_spectrum(c, order, rule)
     assert np.allclose(bem.spectrum_to_scaled_local(spec, rule, order, k, a),
                        bem.spectrum_to_local(spec, rule, order) * f, rtol=1e-13, atol=1e-13)
+
+
+def test_translation_plan_cache_hits_evicts_and_matches_builder():
+    """The mesh-level plan cache (TranslationPlanCache) returns exactly the plan the
+    builder makes for the same request, counts hits, and evicts least recently used
+    plans above its byte budget while keeping the newest one."""
+    req = [(bem.AXIAL_MULTIPOLE_TO_LOCAL, 12, 12, 30.0, 0.05, 0.05, 0.2, 33),
+           (bem.AXIAL_MULTIPOLE_TO_PARENT, 10, 12, 30.0, 0.025, 0.05, 0.03, 33),
+           (bem.AXIAL_LOCAL_TO_CHILD, 12, 10, 30.0, 0.05, 0.025, 0.03, 33)]
+    cache = bem.TranslationPlanCache()
+    for r in req:
+        blocks, _, _ = bem.axial_plan(*r)
+        flat = np.concatenate([b.ravel() for b in blocks])
+        assert np.array_equal(cache.axial(*r), flat)
+    assert np.array_equal(cache.axial(*req[0]), np.concatenate([b.ravel() for b in bem.axial_plan(*req[0])[0]]))
+    st = cache.statistics()
+    assert st["plans_built"] == 3 and st["hits"] == 1 and st["evictions"] == 0 and st["live_bytes"] > 0
+    # a budget below one plan keeps only the newest: every request is rebuilt
+    small = bem.TranslationPlanCache(maximum_bytes=1)
+    for r in req + req:
+        small.axial(*r)
+    st = small.statistics()
+    assert st["plans_built"] == 6 and st["hits"] == 0 and st["evictions"] == 5
+    # a different wavenumber bit pattern is a different plan
+    cache.axial(*req[0][:3], 30.0 * (1 + 1e-15), *req[0][4:])
+    assert cache.statistics()["plans_built"] == 4
+    with pytest.raises(ValueError):
+        bem.TranslationPlanCache(maximum_bytes=0)
+    cache.close()
+    small.close()

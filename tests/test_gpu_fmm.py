@@ -104,6 +104,8 @@ def test_fmm_apply_with_promoted_endpoint_rule(degree):
     base = solver.fmm_operator(num, k, k, leaf_capacity=32)
     high = solver.fmm_operator(num, k, k, leaf_capacity=32, endpoint_degree=degree)
     assert high.statistics()["rule_points"] > 16
+    # same mesh, wavenumber and seed: every translation plan comes from the mesh's plan cache
+    assert high.statistics()["plans_built"] == 0 and high.statistics()["plan_cache_hits"] > 0
     x = np.random.default_rng(9).standard_normal(A.shape[0]) + 1j * np.random.default_rng(10).standard_normal(A.shape[0])
     yd, yb, yh, yx = A @ x, base.apply(x), high.apply(x), high.apply_direct(x)
     print(f"degree {degree} ({high.statistics()['rule_points']} points): vs dense {_rel(yh, yd):.2e}, vs minimum rule "
