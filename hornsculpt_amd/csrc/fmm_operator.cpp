@@ -684,12 +684,15 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
 
     // ---- rotation and coaxial plans (host, in parallel)
     const TranslationPlanCacheStatistics cache_before = translation_plans_.statistics();
+    // Wigner blocks per (order, polar class) from the mesh's cache, phases per direction
     std::vector<std::shared_ptr<const RotationPlan>> rotations(directions.size());
+    std::vector<std::vector<std::complex<double>>> phases(directions.size());
     parallel_for(directions.size(), [&](std::size_t i) {
         const Triple &d = directions[i];
-        rotations[i] = translation_plans_.rotation(
-            static_cast<unsigned>(direction_order[i]),
-            {static_cast<double>(d[0]), static_cast<double>(d[1]), static_cast<double>(d[2])});
+        const Vec3 axis{static_cast<double>(d[0]), static_cast<double>(d[1]), static_cast<double>(d[2])};
+        const auto ord = static_cast<unsigned>(direction_order[i]);
+        rotations[i] = translation_plans_.wigner(ord, rotation_polar_angle(axis));
+        phases[i] = rotation_azimuth(ord, axis);
     });
     std::vector<std::shared_ptr<const AxialPlan>> plans(axial_keys.size());
     parallel_for(axial_keys.size(), [&](std::size_t i) {
@@ -714,11 +717,14 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
     st.axial_plans = plans.size();
     std::vector<dev::FmmRotation> rot_table;
     std::vector<double> small_d, azimuth;
-    for (const auto &rp : rotations) {
-        const RotationPlan &r = *rp;
-        rot_table.push_back({small_d.size(), azimuth.size() / 2, r.order, 0});
-        small_d.insert(small_d.end(), r.small_d.begin(), r.small_d.end());
-        for (const auto &z : r.azimuth) {
+    std::map<const RotationPlan *, std::uint64_t> block_offset; // one copy per polar class
+    for (std::size_t i = 0; i < rotations.size(); ++i) {
+        const RotationPlan &r = *rotations[i];
+        auto [it, fresh] = block_offset.emplace(&r, small_d.size());
+        if (fresh)
+            small_d.insert(small_d.end(), r.small_d.begin(), r.small_d.end());
+        rot_table.push_back({it->second, azimuth.size() / 2, r.order, 0});
+        for (const auto &z : phases[i]) {
             azimuth.push_back(z.real());
             azimuth.push_back(z.imag());
         }

@@ -35,24 +35,40 @@ void check(const RotationPlan &p, const std::vector<std::complex<double>> &c) {
 
 } // namespace
 
+double rotation_polar_angle(Vec3 axis) { return std::atan2(std::hypot(axis.x, axis.y), axis.z); }
+
+std::vector<std::complex<double>> rotation_azimuth(unsigned order, Vec3 axis) {
+    const double phi = std::atan2(axis.y, axis.x);
+    std::vector<std::complex<double>> out;
+    out.reserve(2 * order + 1);
+    for (int m = -static_cast<int>(order); m <= static_cast<int>(order); ++m) {
+        double sm, cm;
+        sin_cos(m * phi, &sm, &cm);
+        out.emplace_back(cm, sm);
+    }
+    return out;
+}
+
 RotationPlan make_rotation_plan(unsigned order, Vec3 axis) {
     if (order > kMaximumRotationOrder)
         throw std::invalid_argument("Spherical rotation order exceeds the supported contract.");
     const double len = std::hypot(axis.x, axis.y, axis.z);
     if (!std::isfinite(len) || len <= 0)
         throw std::invalid_argument("Spherical rotation axis must be finite and nonzero.");
-    const double theta = std::atan2(std::hypot(axis.x, axis.y), axis.z), phi = std::atan2(axis.y, axis.x);
+    RotationPlan p = wigner_blocks(order, rotation_polar_angle(axis));
+    p.azimuth = rotation_azimuth(order, axis);
+    return p;
+}
+
+RotationPlan wigner_blocks(unsigned order, double theta) {
+    if (order > kMaximumRotationOrder || !std::isfinite(theta))
+        throw std::invalid_argument("Spherical rotation order or polar angle is invalid.");
     double sh, ch;
     sin_cos(theta / 2, &sh, &ch);
     RotationPlan p;
     p.order = order;
     p.degree_offsets = {0, 1};
     p.small_d = {1};
-    for (int m = -static_cast<int>(order); m <= static_cast<int>(order); ++m) {
-        double sm, cm;
-        sin_cos(m * phi, &sm, &cm);
-        p.azimuth.emplace_back(cm, sm);
-    }
     // d^{q/2} from d^{(q-1)/2} and the spin-1/2 rotation [[c, -s], [s, c]]:
     // d_q[i][j] = sum over the four neighbours with coupling weights
     // u_i = sqrt(i/q) (from index i-1) and w_i = sqrt((q-i)/q) (from index i).

@@ -31,6 +31,12 @@ struct RotationPlan {
 };
 
 RotationPlan make_rotation_plan(unsigned order, Vec3 axis);
+// The two factors of a plan: the Wigner blocks depend on the polar angle
+// theta = atan2(|axis_xy|, axis_z) only (no azimuth in the result), the phases
+// exp(+i m phi) on the azimuth phi = atan2(axis_y, axis_x).
+double rotation_polar_angle(Vec3 axis);
+RotationPlan wigner_blocks(unsigned order, double theta);
+std::vector<std::complex<double>> rotation_azimuth(unsigned order, Vec3 axis);
 // Coefficients indexed l*l + l + m.
 std::vector<std::complex<double>> rotate_to_axis(const RotationPlan &plan, const std::vector<std::complex<double>> &c);
 std::vector<std::complex<double>> rotate_from_axis(const RotationPlan &plan,

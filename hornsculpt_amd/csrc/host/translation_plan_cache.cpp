@@ -86,8 +86,8 @@ std::shared_ptr<const AxialPlan> TranslationPlanCache::axial(const AxialPlanRequ
     return plan;
 }
 
-std::shared_ptr<const RotationPlan> TranslationPlanCache::rotation(unsigned order, Vec3 axis) {
-    const Key key{1, order, bits(axis.x), bits(axis.y), bits(axis.z), 0, 0, 0, 0, 0};
+std::shared_ptr<const RotationPlan> TranslationPlanCache::wigner(unsigned order, double theta) {
+    const Key key{1, order, bits(theta), 0, 0, 0, 0, 0, 0, 0};
     {
         std::lock_guard lock(mutex_);
         const auto it = entries_.find(key);
@@ -98,7 +98,7 @@ std::shared_ptr<const RotationPlan> TranslationPlanCache::rotation(unsigned orde
         }
     }
     const auto t0 = std::chrono::steady_clock::now();
-    auto plan = std::make_shared<const RotationPlan>(make_rotation_plan(order, axis));
+    auto plan = std::make_shared<const RotationPlan>(wigner_blocks(order, theta));
     const double built = ms_since(t0);
     std::lock_guard lock(mutex_);
     ++stats_.plans_built;

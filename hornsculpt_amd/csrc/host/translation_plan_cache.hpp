@@ -5,9 +5,9 @@
 // translation_plan_cache.cpp): coaxial plans keyed by the exact request (kind,
 // orders, polar count and the bit patterns of k, radii, distance and probe radius),
 // least-recently-used eviction above a byte budget (512 MB by default), hit and
-// build statistics. Rotation plans (Wigner blocks and azimuth phases) depend on the
-// direction and order only, so they share the repository under their own keys and
-// are reused across wavenumbers. Plans are built outside the lock; two threads
+// build statistics. The Wigner blocks of rotations depend on the order and the polar
+// angle only (every direction of one polar class shares them), so they share the
+// repository under their own keys and are reused across wavenumbers. Plans are built outside the lock; two threads
 // asking for the same missing plan may both build it, the first stored one wins.
 #pragma once
 
@@ -35,9 +35,10 @@ struct TranslationPlanCacheStatistics {
 
 class TranslationPlanCache {
   public:
-    explicit TranslationPlanCache(std::size_t maximum_bytes = std::size_t{512} << 20);
+    explicit TranslationPlanCache(std::size_t maximum_bytes = std::size_t{1} << 30);
     std::shared_ptr<const AxialPlan> axial(const AxialPlanRequest &request);
-    std::shared_ptr<const RotationPlan> rotation(unsigned order, Vec3 axis);
+    // Wigner blocks of one polar class (the plan's azimuth is left empty).
+    std::shared_ptr<const RotationPlan> wigner(unsigned order, double theta);
     TranslationPlanCacheStatistics statistics() const;
 
   private:
