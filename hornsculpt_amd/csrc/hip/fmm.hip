@@ -282,6 +282,16 @@ __device__ __forceinline__ std::size_t axial_block(int m, int ps, int pt) {
     return off;
 }
 
+// Position of coefficient (l, m) of an order-p expansion in |m|-major order: blocks
+// |m| = 0..p, each holding l = |m|..p for +|m| and then (|m| > 0) for -|m|; block |m|
+// starts at (2|m|-1)(p+1) - (|m|-1)|m| (0 for |m| = 0). The forward rotation writes
+// the rotated source in this order so the coaxial products read contiguous columns.
+__device__ __forceinline__ std::size_t m_major(int p, int l, int m) {
+    const int a = m < 0 ? -m : m;
+    const std::size_t block = a ? static_cast<std::size_t>(2 * a - 1) * (p + 1) - static_cast<std::size_t>(a - 1) * a : 0;
+    return block + (m < 0 ? p - a + 1 : 0) + (l - a);
+}
+
 // Offset of degree l in a rotation's small-d table: sum_{j<l} (2j+1)^2 = l(2l-1)(2l+1)/3.
 __device__ __forceinline__ std::size_t degree_block(int l) {
     return static_cast<std::size_t>(l) * (2 * l - 1) * (2 * l + 1) / 3;
@@ -290,7 +300,7 @@ __device__ __forceinline__ std::size_t degree_block(int l) {
 // Translations run as four passes over the links of one level:
 //   rotate    per chunk of links sharing a direction: the Wigner blocks d_l are
 //             staged in LDS degree by degree and applied to every source of the
-//             chunk (Y = d^T e^{i m phi} X);
+//             chunk (Y = d^T e^{i m phi} X, stored |m|-major);
 //   coaxial   one wavefront per link: per-|m| block product Z = T Y;
 //   back      per chunk again: W = e^{-i m phi} d Z;
 //   gather    one block per destination: the sum of its links' W.
@@ -387,7 +397,7 @@ __global__ __launch_bounds__(256) void rotate_links(FmmTranslateLaunch L, bool b
             for (int i = 0; i < 4; ++i) {
                 const int j = j0 + i;
                 if (j < count && l <= ord[j])
-                    out[dst_off[j] + base + t] = back ? rot_back * sv[i] : sv[i];
+                    out[dst_off[j] + (back ? base + t : m_major(ord[j], l, t - l))] = back ? rot_back * sv[i] : sv[i];
             }
         }
     }
@@ -438,7 +448,7 @@ __global__ __launch_bounds__(256) void coaxial_links(FmmTranslateLaunch L) {
             C2 v{0, 0};
             if (e < count * 2 * cols) {
                 const int j = e / (2 * cols), r = e % (2 * cols), sgn = r / cols, n = m + r % cols;
-                v = y[slot[j] + static_cast<std::size_t>(n) * n + n + (sgn ? -m : m)];
+                v = y[slot[j] + m_major(ps, n, sgn ? -m : m)];
             }
             py[i] = v;
         }
@@ -565,7 +575,7 @@ __global__ __launch_bounds__(256) void rotate_links_wide(FmmTranslateLaunch L, b
                 for (int i = 0; i < 4; ++i) {
                     const int j = j0 + i;
                     if (j < count && l <= ord[j])
-                        out[dst_off[j] + base + t] = back ? rot_back * sv[i] : sv[i];
+                        out[dst_off[j] + (back ? base + t : m_major(ord[j], l, t - l))] = back ? rot_back * sv[i] : sv[i];
                 }
             }
         }
@@ -605,7 +615,7 @@ __global__ __launch_bounds__(256) void coaxial_links_wide(FmmTranslateLaunch L) 
             tm[e] = T[off + e];
         for (int e = threadIdx.x; e < kFmmChunk * 2 * cols; e += blockDim.x) {
             const int j = e / (2 * cols), r = e % (2 * cols), sgn = r / cols, n = m + r % cols;
-            ys[j][sgn][r % cols] = j < count ? y[slot[j] + static_cast<std::size_t>(n) * n + n + (sgn ? -m : m)]
+            ys[j][sgn][r % cols] = j < count ? y[slot[j] + m_major(ps, n, sgn ? -m : m)]
                                              : C2{0, 0};
         }
         __syncthreads();
