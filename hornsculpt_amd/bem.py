@@ -600,13 +600,13 @@ def _collect(lib, handle, request: BemRequest) -> BemResult:
                                            refs.value, BemBoundaryData(surface, trace, deriv)))
         served = ctypes.c_double()
         _check(lib.hsbem_result_served_latency(handle, f, ctypes.byref(served)))
-        fm = np.zeros(11)
+        fm = np.zeros(12)
         _check(lib.hsbem_result_fmm(handle, f, _dptr(fm)))
         fmm = None if request.numerics.acceleration != ACCELERATION_FAST_MULTIPOLE else {
             "audit_discrepancy": float(fm[0]), "apply_milliseconds": float(fm[1]), "corrected": bool(fm[2]),
             "plane_wave_levels": int(fm[3]), "m2l_links": int(fm[4]), "near_nnz": int(fm[5]),
             "device_bytes": int(fm[6]), "direct_audit": bool(fm[7]), "quadrature_discrepancy": float(fm[8]),
-            "endpoint_degree": int(fm[9]), "promotions": int(fm[10])}
+            "endpoint_degree": int(fm[9]), "promotions": int(fm[10]), "verification_seed": float(fm[11])}
         freqs.append(BemFrequencyResult(request.frequencies[f], exs, ms[0], ms[1], ms[2], ms[3], ms[4], ms[5],
                                         {"regular": ms[6], "touching": ms[7], "near": ms[8],
                                          "regular_samples": ms[9], "touching_samples": ms[10],

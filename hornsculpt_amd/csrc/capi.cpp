@@ -516,6 +516,7 @@ int hsbem_result_fmm(const hsbem_result *r, uint32_t f, double *out) {
         out[8] = fr.fmm_quadrature_discrepancy;
         out[9] = fr.fmm_endpoint_degree;
         out[10] = fr.fmm_promotions;
+        out[11] = fr.fmm_verification_seed;
     });
 }
 

@@ -235,11 +235,11 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
             plane[l] = 1;
             rule_order[l] = lp.quadrature_order;
             translation_order[l] = lp.translation_order;
-        } else if (lp.signature_order > static_cast<unsigned>(dev::kMaxWideOrder) ||
+        } else if (lp.signature_order > static_cast<unsigned>(dev::kMaxBigOrder) ||
                    k * lp.active_radius > kMaximumSphericalElectricalRadius) {
             throw std::runtime_error("FMM level " + std::to_string(l) + " needs spherical order " +
                                      std::to_string(lp.signature_order) + " beyond the translations built here (at most " +
-                                     std::to_string(dev::kMaxWideOrder) + ").");
+                                     std::to_string(dev::kMaxBigOrder) + ").");
         }
         order[l] = static_cast<int>(lp.signature_order);
         radius[l] = lp.active_radius;
@@ -304,7 +304,7 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
             continue;
         const bool hf = plane[nd.level] != 0;
         if (nd.pressure_count > static_cast<std::uint32_t>(dev::kMaxLeafDofs) ||
-            (!hf && order[nd.level] > dev::kMaxLeafOrder))
+            (!hf && order[nd.level] > dev::kMaxBigOrder))
             throw std::runtime_error("FMM leaf exceeds the supported DOF count or order.");
         const auto first_dof = static_cast<std::uint32_t>(leaf_dofs.size());
         for (std::uint32_t j = 0; j < nd.pressure_count; ++j) {
@@ -384,14 +384,16 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
 
     // ---- translation links
     t0 = Clock::now();
-    // one rotation plan per (direction, order class): links up to kMaxFmmOrder never
-    // share a plan (and so the tiled kernels) with a wider level
-    std::map<std::pair<Triple, bool>, std::uint32_t> direction_index;
+    // one rotation plan per (direction, order class): a launch's kernels are chosen by
+    // its largest order (<= 40, <= 64, <= 128), so narrower levels never share a plan
+    // with a wider one
+    std::map<std::pair<Triple, int>, std::uint32_t> direction_index;
     std::vector<Triple> directions;
     std::vector<int> direction_order;
     auto direction = [&](Triple t, int ord) {
         t = reduced(t);
-        auto [it, fresh] = direction_index.emplace(std::make_pair(t, ord > dev::kMaxFmmOrder),
+        const int order_class = ord > dev::kMaxWideOrder ? 2 : ord > dev::kMaxFmmOrder ? 1 : 0;
+        auto [it, fresh] = direction_index.emplace(std::make_pair(t, order_class),
                                                    static_cast<std::uint32_t>(directions.size()));
         if (fresh) {
             directions.push_back(t);
@@ -839,6 +841,8 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
     M.leaf_dofs = op->leaf_dofs_.data();
     M.items = d_items.data();
     M.item_count = static_cast<std::uint32_t>(items.size());
+    for (const dev::FmmLeaf &lf : leaves)
+        M.max_order = std::max(M.max_order, lf.order);
     M.emission = op->emission_.data();
     M.testing = op->testing_.data();
     M.error = d_err.data();
@@ -1036,6 +1040,14 @@ double weighted_norm(const double *v, std::uint32_t n, const double *weights, Wo
 
 } // namespace
 
+namespace {
+// GMRES target of the FMM solve as a fraction of the residual tolerance: the
+// reference's Krylov settings scale 0.05 times its device GMRES fraction 0.1
+// (support_fmm_solve.hpp:11, device_gmres.cu:191-192).
+constexpr double kKrylovFraction = 0.05 * 0.1;
+constexpr unsigned kMaximumSeedRefinements = 4; // support_fmm_solve.hpp:8
+} // namespace
+
 void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, double omega,
                            std::vector<double> &relative, std::vector<std::uint32_t> &iterations, FrequencyResult &fr,
                            const volatile std::int32_t *cancel) {
@@ -1172,7 +1184,7 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
         for (std::uint32_t e = 0; e < ne; ++e) {
             double achieved = 0;
             gmres(s, kop, nv, vec(ws.fmm_b, e), vec(ws.fmm_x, e), num.fmm_restart, num.fmm_maximum_iterations,
-                  0.1 * tol, &achieved, &iterations[e], cancel, ws.gmres);
+                  kKrylovFraction * tol, &achieved, &iterations[e], cancel, ws.gmres);
         }
     }
     // r = b - A x per excitation with a given action; relative residuals
@@ -1196,8 +1208,8 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
                     continue;
                 double achieved = 0;
                 std::uint32_t its = 0;
-                gmres(s, kop, nv, vec(ws.fmm_r, e), ax.data(), num.fmm_restart, num.fmm_maximum_iterations, 0.1 * tol,
-                      &achieved, &its, cancel, ws.gmres);
+                gmres(s, kop, nv, vec(ws.fmm_r, e), ax.data(), num.fmm_restart, num.fmm_maximum_iterations,
+                      kKrylovFraction * tol, &achieved, &its, cancel, ws.gmres);
                 dev::launch_add(nv, ax.data(), vec(ws.fmm_x, e), s);
                 iterations[e] += its;
             }
@@ -1205,8 +1217,6 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
             fr.fmm_corrected = true;
         }
     };
-    FmmSettings fine = fs;
-    fine.accuracy_seed = 0.1 * fs.accuracy_seed;
     std::unique_ptr<FmmOperator> promoted; // the operator after endpoint-rule promotions
     for (;;) {
         const FmmOperator &A = promoted ? *promoted : *op;
@@ -1219,31 +1229,73 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
             correct(A, audited);
             fr.fmm_audit = 0;
         } else {
-            // seed audit: the same action with a ten times tighter accuracy seed
-            residuals([&](const double *x, double *y) { A.apply(x, y, s); });
-            audit = build_fmm(num, P.k_max, k, fine, faces, s, &A, A.statistics().endpoint_degree);
-            double discrepancy = 0;
-            for (std::uint32_t e = 0; e < ne; ++e) {
-                // (A_seed - A_fine) x = (b - r) - A_fine x
-                audit->apply(vec(ws.fmm_x, e), ax.data(), s);
-                dev::launch_subtract_from(nv, vec(ws.fmm_b, e), ax.data(), s);
-                dev::launch_subtract_from(nv, vec(ws.fmm_r, e), ax.data(), s);
-                discrepancy = std::max(discrepancy, ratio(wnorm(ax.data()), bnorm[e]));
+            // seed audits (support_fmm_solve.cu:393-524): the solution is corrected with
+            // the seed/10 operator until its residual holds, then checked against the
+            // next operator with a distinct level plan (seed/100, ...): their actions at
+            // the solution must agree to 0.25 tol; otherwise the finer one takes over
+            auto action = [&](const FmmOperator &O) {
+                return [&O, &s](const double *x, double *y) { O.apply(x, y, s); };
+            };
+            auto same_levels = [](const FmmOperator &a, const FmmOperator &b) {
+                const FmmStatistics &x = a.statistics(), &y = b.statistics();
+                return x.level_orders == y.level_orders && x.level_representation == y.level_representation &&
+                       x.level_rule_orders == y.level_rule_orders;
+            };
+            FmmSettings aset = fs;
+            aset.accuracy_seed = fs.accuracy_seed / 10;
+            audit = build_fmm(num, P.k_max, k, aset, faces, s, &A, A.statistics().endpoint_degree);
+            residuals(action(*audit));
+            DeviceBuffer<double> prod(nvec);
+            for (unsigned refinements = 0;;) {
+                correct(*audit, action(*audit));
+                for (std::uint32_t e = 0; e < ne; ++e)
+                    audit->apply(vec(ws.fmm_x, e), vec(prod, e), s);
+                std::unique_ptr<FmmOperator> verify;
+                for (;;) {
+                    aset.accuracy_seed /= 10;
+                    verify = build_fmm(num, P.k_max, k, aset, faces, s, &A, A.statistics().endpoint_degree);
+                    if (!same_levels(*verify, *audit))
+                        break;
+                    if (refinements == kMaximumSeedRefinements)
+                        throw std::runtime_error("FMM seed refinement exhausted without a distinct level plan.");
+                    ++refinements;
+                }
+                residuals(action(*verify));
+                double discrepancy = 0;
+                for (std::uint32_t e = 0; e < ne; ++e) {
+                    // A_audit x - A_verify x = prod - (b - r)
+                    hip_check(hipMemcpyAsync(t1.data(), vec(ws.fmm_r, e), t1.bytes(), hipMemcpyDeviceToDevice, s), "copy");
+                    dev::launch_subtract_from(nv, vec(ws.fmm_b, e), t1.data(), s); // b - r = A_verify x
+                    dev::launch_subtract_from(nv, vec(prod, e), t1.data(), s);     // prod - A_verify x
+                    discrepancy = std::max(discrepancy, ratio(wnorm(t1.data()), bnorm[e]));
+                }
+                fr.fmm_audit = discrepancy;
+                fr.fmm_verification_seed = aset.accuracy_seed;
+                audit = std::move(verify);
+                if (*std::max_element(relative.begin(), relative.end()) <= tol && discrepancy <= 0.25 * tol)
+                    break;
+                if (refinements == kMaximumSeedRefinements) {
+                    std::ostringstream msg;
+                    msg.precision(6);
+                    msg << "FMM accuracy-seed refinement exhausted: residual "
+                        << *std::max_element(relative.begin(), relative.end()) << ", operator discrepancy "
+                        << discrepancy << "; tolerance " << tol << ".";
+                    throw std::runtime_error(msg.str());
+                }
+                ++refinements;
+                fr.fmm_corrected = true;
             }
-            fr.fmm_audit = std::max(fr.fmm_audit, discrepancy);
             audited = [&](const double *x, double *y) { audit->apply(x, y, s); };
-            if (discrepancy > 0.25 * tol) {
-                residuals(audited);
-                correct(*audit, audited);
-            }
         }
         fr.fmm_direct_audit = direct;
         // endpoint-quadrature comparison: the audited action and B at the next rule
         const unsigned degree = A.statistics().endpoint_degree, next = refined_endpoint_degree(degree);
         if (!next)
             throw std::runtime_error("FMM endpoint quadrature has no independent refinement rule.");
-        const std::unique_ptr<FmmOperator> cmp =
-            build_fmm(num, P.k_max, k, direct ? fs : fine, faces, s, &A, next);
+        FmmSettings cset = fs;
+        if (!direct)
+            cset.accuracy_seed = fr.fmm_verification_seed; // the verified seed, the next rule
+        const std::unique_ptr<FmmOperator> cmp = build_fmm(num, P.k_max, k, cset, faces, s, &A, next);
         right_hand_sides(*cmp, b_next);
         double quad = 0, next_residual = 0;
         for (std::uint32_t e = 0; e < ne; ++e) {

@@ -60,7 +60,7 @@ __device__ __forceinline__ Wave scaled(Wave a, double s) {
     return a;
 }
 
-// Jhat_l(z) = (2l+1)!! j_l(z) / z^l for l = 0..top (top <= kMaxLeafOrder + 1): the
+// Jhat_l(z) = (2l+1)!! j_l(z) / z^l for l = 0..top (top <= kMaxBigOrder + 1): the
 // power series while its first correction z^2 / (2 (2l+3)) is at most one, else
 // j_l by normalised Miller recursion (downward from past the turning point,
 // normalised by the larger of j0 and j1) divided by z^l / (2l+1)!!.
@@ -119,7 +119,7 @@ __device__ __forceinline__ void atomic_add_c(double *p, C2 v) {
 }
 
 // One kGroup-lane group per (leaf, support panel) item, lane = rule point.
-template <int kGroup> __global__ __launch_bounds__(64) void leaf_moments(FmmLeafMomentsLaunch L) {
+template <int kGroup, int kTop> __global__ __launch_bounds__(64) void leaf_moments(FmmLeafMomentsLaunch L) {
     const int lane = threadIdx.x % kGroup;
     const std::uint32_t item = blockIdx.x * (64 / kGroup) + threadIdx.x / kGroup;
     const bool live_item = item < L.item_count;
@@ -143,7 +143,7 @@ template <int kGroup> __global__ __launch_bounds__(64) void leaf_moments(FmmLeaf
     const double x[3] = {y[0] - leaf.center[0], y[1] - leaf.center[1], y[2] - leaf.center[2]};
     const double u[3] = {x[0] / a, x[1] / a, x[2] / a};
     const int p = static_cast<int>(leaf.order);
-    double J[kMaxLeafOrder + 2];
+    double J[kTop + 2];
     const bool ok = normalized_regular(p + 1, k * sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]), J);
     if (!ok && live && L.error)
         atomicMax(L.error, 1);
@@ -643,6 +643,193 @@ __global__ __launch_bounds__(256) void coaxial_links_wide(FmmTranslateLaunch L) 
     }
 }
 
+// Orders above kMaxWideOrder (up to kMaxBigOrder): no staging of a whole Wigner or
+// coaxial block fits in LDS, so the reduction index is tiled. Rotations: per degree,
+// all of the chunk's inputs (one half of the chunk, 8 links, at a time) stay in LDS
+// while 32-row slabs of d_l pass through; every thread keeps the running sums of
+// its (t, 4-link group) items in registers. Coaxial products: per |m| block the
+// chunk's columns stay in LDS while 16-column slabs of T_m pass through.
+constexpr int kBigN = 2 * kMaxBigOrder + 1;
+constexpr int kBigHalf = kFmmChunk / 2;
+constexpr int kBigSlab = 32;
+constexpr int kBigItems = (kBigN * (kBigHalf / 4) + 255) / 256; // (t, group) items per thread
+
+__global__ __launch_bounds__(256) void rotate_links_big(FmmTranslateLaunch L, bool back) {
+    __shared__ C2 xs[kBigHalf][kBigN];
+    __shared__ double dt[kBigSlab * kBigN];
+    __shared__ C2 ph[kBigN];
+    __shared__ std::uint64_t src_off[kFmmChunk], dst_off[kFmmChunk];
+    __shared__ int ord[kFmmChunk];
+    const FmmChunk ch = L.chunks[blockIdx.x];
+    const FmmRotation rot = L.rotations[ch.rotation];
+    const C2 *az = reinterpret_cast<const C2 *>(L.azimuth) + rot.azimuth;
+    for (int m = threadIdx.x; m <= 2 * static_cast<int>(rot.order); m += blockDim.x)
+        ph[m] = az[m];
+    if (threadIdx.x < ch.count) {
+        const FmmLink lk = L.links[L.order[ch.first + threadIdx.x]];
+        src_off[threadIdx.x] = back ? lk.slot : lk.source;
+        dst_off[threadIdx.x] = lk.slot;
+        ord[threadIdx.x] = static_cast<int>(back ? lk.target_order : lk.source_order);
+    }
+    __syncthreads();
+    const C2 *in = reinterpret_cast<const C2 *>(back ? L.work_z : L.source_expansions);
+    C2 *out = reinterpret_cast<C2 *>(L.work_y);
+    const int count = static_cast<int>(ch.count), P = static_cast<int>(rot.order), top = static_cast<int>(ch.max_order);
+    for (int h = 0; h * kBigHalf < count; ++h) {
+        const int j_first = h * kBigHalf, hc = min(kBigHalf, count - j_first);
+        for (int l = 0; l <= top; ++l) {
+            const int n = 2 * l + 1;
+            const std::size_t base = static_cast<std::size_t>(l) * l;
+            const double *dl = L.small_d + rot.small_d + static_cast<std::size_t>(l) * (2 * l - 1) * (2 * l + 1) / 3;
+            __syncthreads();
+            for (int e = threadIdx.x; e < kBigHalf * n; e += blockDim.x) {
+                const int jj = e / n, q = e % n, j = j_first + jj;
+                C2 v{0, 0};
+                if (jj < hc && l <= ord[j]) {
+                    v = in[src_off[j] + base + q];
+                    if (!back)
+                        v = ph[P + q - l] * v;
+                }
+                xs[jj][q] = v;
+            }
+            C2 acc[kBigItems][4];
+#pragma unroll
+            for (int it = 0; it < kBigItems; ++it)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    acc[it][i] = {0, 0};
+            const int items = n * (kBigHalf / 4);
+            for (int q0 = 0; q0 < n; q0 += kBigSlab) {
+                const int qw = min(kBigSlab, n - q0);
+                __syncthreads();
+                // dt[qq][t] = d[q][t] (forward) or d[t][q] (back), q = q0 + qq
+                for (int e = threadIdx.x; e < qw * n; e += blockDim.x) {
+                    if (back) {
+                        const int t = e / qw, qq = e % qw;
+                        dt[qq * n + t] = dl[static_cast<std::size_t>(t) * n + q0 + qq];
+                    } else {
+                        const int qq = e / n, t = e % n;
+                        dt[qq * n + t] = dl[static_cast<std::size_t>(q0 + qq) * n + t];
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int it = 0; it < kBigItems; ++it) {
+                    const int e = threadIdx.x + 256 * it;
+                    if (e >= items)
+                        continue;
+                    const int g = e / n, t = e % n, j0 = 4 * g;
+                    for (int qq = 0; qq < qw; ++qq) {
+                        const double d = dt[qq * n + t];
+                        const int q = q0 + qq;
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            acc[it][i] = acc[it][i] + d * xs[j0 + i][q];
+                    }
+                }
+            }
+#pragma unroll
+            for (int it = 0; it < kBigItems; ++it) {
+                const int e = threadIdx.x + 256 * it;
+                if (e >= items)
+                    continue;
+                const int g = e / n, t = e % n;
+                const C2 rot_back = back ? conj(ph[P + t - l]) : C2{1, 0};
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int jj = 4 * g + i, j = j_first + jj;
+                    if (jj < hc && l <= ord[j])
+                        out[dst_off[j] + (back ? base + t : m_major(ord[j], l, t - l))] =
+                            back ? rot_back * acc[it][i] : acc[it][i];
+                }
+            }
+        }
+    }
+}
+
+constexpr int kBigRows = kMaxBigOrder + 1;
+constexpr int kBigCols = 16;
+constexpr int kBigCoaxItems = (kBigRows * 2 * (kFmmChunk / 4) + 255) / 256; // (row, sign, group) per thread
+
+__global__ __launch_bounds__(256) void coaxial_links_big(FmmTranslateLaunch L) {
+    __shared__ C2 ys[kFmmChunk][2][kBigRows];
+    __shared__ C2 tm[kBigRows * kBigCols];
+    __shared__ std::uint64_t slot[kFmmChunk];
+    const FmmChunk ch = L.axial_chunks[blockIdx.x];
+    const FmmLink first = L.links[L.axial_order[ch.first]];
+    const int ps = static_cast<int>(first.source_order), pt = static_cast<int>(first.target_order), mm = min(ps, pt);
+    const C2 *T = reinterpret_cast<const C2 *>(L.axial) + L.axial_offsets[first.axial];
+    const C2 *y = reinterpret_cast<const C2 *>(L.work_y);
+    C2 *z = reinterpret_cast<C2 *>(L.work_z);
+    const int count = static_cast<int>(ch.count);
+    if (threadIdx.x < count)
+        slot[threadIdx.x] = L.links[L.axial_order[ch.first + threadIdx.x]].slot;
+    const int nct = (pt + 1) * (pt + 1);
+    __syncthreads();
+    for (int e = threadIdx.x; e < count * nct; e += blockDim.x) {
+        const int j = e / nct, c = e % nct;
+        int l, t;
+        split_index(c, l, t);
+        if (abs(t - l) > mm)
+            z[slot[j] + c] = {0, 0};
+    }
+    std::size_t off = 0;
+    for (int m = 0; m <= mm; ++m) {
+        const int rows = pt - m + 1, cols = ps - m + 1;
+        __syncthreads();
+        for (int e = threadIdx.x; e < kFmmChunk * 2 * cols; e += blockDim.x) {
+            const int j = e / (2 * cols), r = e % (2 * cols), sgn = r / cols, n = m + r % cols;
+            ys[j][sgn][r % cols] = j < count ? y[slot[j] + m_major(ps, n, sgn ? -m : m)] : C2{0, 0};
+        }
+        C2 acc[kBigCoaxItems][4];
+#pragma unroll
+        for (int it = 0; it < kBigCoaxItems; ++it)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                acc[it][i] = {0, 0};
+        const int items = rows * 2 * (kFmmChunk / 4);
+        for (int c0 = 0; c0 < cols; c0 += kBigCols) {
+            const int cw = min(kBigCols, cols - c0);
+            __syncthreads();
+            for (int e = threadIdx.x; e < rows * cw; e += blockDim.x) {
+                const int r = e / cw, cc = e % cw;
+                tm[r * kBigCols + cc] = T[off + static_cast<std::size_t>(r) * cols + c0 + cc];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int it = 0; it < kBigCoaxItems; ++it) {
+                const int e = threadIdx.x + 256 * it;
+                if (e >= items)
+                    continue;
+                const int g = e / (2 * rows), r = e % (2 * rows), sgn = r / rows, row = r % rows, j0 = 4 * g;
+                if (sgn && !m)
+                    continue;
+                for (int cc = 0; cc < cw; ++cc) {
+                    const C2 tv = tm[row * kBigCols + cc];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        acc[it][i] = acc[it][i] + tv * ys[j0 + i][sgn][c0 + cc];
+                }
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < kBigCoaxItems; ++it) {
+            const int e = threadIdx.x + 256 * it;
+            if (e >= items)
+                continue;
+            const int g = e / (2 * rows), r = e % (2 * rows), sgn = r / rows, l = m + r % rows, j0 = 4 * g;
+            if (sgn && !m)
+                continue;
+            const std::size_t idx = static_cast<std::size_t>(l) * l + l + (sgn ? -m : m);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (j0 + i < count)
+                    z[slot[j0 + i] + idx] = acc[it][i];
+        }
+        off += static_cast<std::size_t>(rows) * cols;
+    }
+}
+
 // target = sum of the destination's link results.
 __global__ __launch_bounds__(256) void gather_links(FmmTranslateLaunch L) {
     const FmmDestination dst = L.destinations[blockIdx.x];
@@ -902,12 +1089,21 @@ __global__ void permute_vectors(const double *in, double *out, const std::uint32
 void launch_leaf_moments(const FmmLeafMomentsLaunch &launch, hipStream_t stream) {
     if (!launch.item_count)
         return;
+    // register table of the radial factors up to the leaves' largest order
+    const bool wide = launch.max_order > static_cast<std::uint32_t>(kMaxLeafOrder);
     if (launch.rule_points <= static_cast<std::uint32_t>(kGroupSmall)) {
         const std::uint32_t per_block = 64 / kGroupSmall;
-        hipLaunchKernelGGL(leaf_moments<kGroupSmall>, dim3((launch.item_count + per_block - 1) / per_block), dim3(64),
-                           0, stream, launch);
+        const dim3 grid((launch.item_count + per_block - 1) / per_block);
+        if (wide)
+            hipLaunchKernelGGL((leaf_moments<kGroupSmall, kMaxBigOrder>), grid, dim3(64), 0, stream, launch);
+        else
+            hipLaunchKernelGGL((leaf_moments<kGroupSmall, kMaxLeafOrder>), grid, dim3(64), 0, stream, launch);
+    } else if (wide) {
+        hipLaunchKernelGGL((leaf_moments<kGroupLarge, kMaxBigOrder>), dim3(launch.item_count), dim3(64), 0, stream,
+                           launch);
     } else {
-        hipLaunchKernelGGL(leaf_moments<kGroupLarge>, dim3(launch.item_count), dim3(64), 0, stream, launch);
+        hipLaunchKernelGGL((leaf_moments<kGroupLarge, kMaxLeafOrder>), dim3(launch.item_count), dim3(64), 0, stream,
+                           launch);
     }
 }
 
@@ -928,7 +1124,11 @@ void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destinatio
                       hipStream_t stream) {
     if (!destination_count)
         return;
-    if (link_count && max_order > kMaxFmmOrder) {
+    if (link_count && max_order > kMaxWideOrder) {
+        hipLaunchKernelGGL(rotate_links_big, dim3(chunk_count), dim3(256), 0, stream, launch, false);
+        hipLaunchKernelGGL(coaxial_links_big, dim3(axial_chunk_count), dim3(256), 0, stream, launch);
+        hipLaunchKernelGGL(rotate_links_big, dim3(chunk_count), dim3(256), 0, stream, launch, true);
+    } else if (link_count && max_order > kMaxFmmOrder) {
         hipLaunchKernelGGL(rotate_links_wide, dim3(chunk_count), dim3(256), 0, stream, launch, false);
         hipLaunchKernelGGL(coaxial_links_wide, dim3(axial_chunk_count), dim3(256), 0, stream, launch);
         hipLaunchKernelGGL(rotate_links_wide, dim3(chunk_count), dim3(256), 0, stream, launch, true);

@@ -9,10 +9,11 @@
 
 namespace hsbem::dev {
 
-inline constexpr int kMaxLeafOrder = 64;   // expansion order at the leaves
+inline constexpr int kMaxLeafOrder = 64;   // leaf orders with the smaller radial register table (up to kMaxBigOrder)
 inline constexpr int kMaxLeafDofs = 512;   // P1 DOFs of one leaf
 inline constexpr int kMaxFmmOrder = 40;    // translations with the whole d_l block staged: (2p+1)^2 doubles
-inline constexpr int kMaxWideOrder = 64;   // any translated expansion (tiled d_l stage above kMaxFmmOrder)
+inline constexpr int kMaxWideOrder = 64;   // tiled d_l stage above kMaxFmmOrder
+inline constexpr int kMaxBigOrder = 128;   // any translated expansion (reduction-tiled kernels above kMaxWideOrder)
 inline constexpr int kFmmChunk = 16;       // links per rotation chunk
 inline constexpr int kFmmJacobi = 64;      // block width of the near block-Jacobi blocks
 
@@ -45,6 +46,7 @@ struct FmmLeafMomentsLaunch {
     const std::uint32_t *leaf_dofs;
     const FmmLeafItem *items;
     std::uint32_t item_count;
+    std::uint32_t max_order;      // largest leaf order
     double *emission, *testing;   // complex, zeroed before the launch
     int *error;                   // set when a radial table leaves its bounded domain
 };
@@ -248,8 +250,9 @@ void launch_leaf_emit(const FmmLeaf *leaves, std::uint32_t leaf_count, const std
                       const double *emission, const double *p, double *expansions, hipStream_t stream);
 void launch_leaf_test(const FmmLeaf *leaves, std::uint32_t leaf_count, const std::uint32_t *leaf_dofs,
                       const double *testing, const double *expansions, double *y, hipStream_t stream);
-// max_order: largest source or target order of the launch's links (selects the tiled
-// kernels above kMaxFmmOrder; at most kMaxWideOrder).
+// max_order: largest source or target order of the launch's links and rotation plans
+// (selects the tiled kernels above kMaxFmmOrder and the reduction-tiled ones above
+// kMaxWideOrder; at most kMaxBigOrder).
 void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destination_count, std::uint32_t link_count,
                       std::uint32_t chunk_count, std::uint32_t axial_chunk_count, std::uint32_t max_order,
                       hipStream_t stream);

@@ -169,12 +169,14 @@ HSBEM_API int hsbem_result_residual(const hsbem_result *result, uint32_t frequen
 HSBEM_API int hsbem_result_timings(const hsbem_result *result, uint32_t frequency, double *milliseconds);
 HSBEM_API int hsbem_result_peak_device_bytes(const hsbem_result *result, uint64_t *bytes);
 /* Fast multipole evidence of a frequency point (BemFrequencyResult's FMM statistics,
- * bem.hpp:193-332; zeros for the dense form): out[11] = seed-audit discrepancy
- * (relative, against the seed/10 operator; 0 under the direct audit), apply
+ * bem.hpp:193-332; zeros for the dense form): out[12] = seed-audit discrepancy
+ * (relative; 0 under the direct audit), apply
  * milliseconds, defect correction applied (0/1), plane-wave levels, M2L links, near
  * CSR entries, operator device bytes, residual evaluated by the direct Galerkin
  * audit (0/1), endpoint-quadrature discrepancy against the next panel rule (A and B
- * actions, relative), endpoint rule degree, endpoint-rule promotions. */
+ * actions, relative), endpoint rule degree, endpoint-rule promotions, and (out[11])
+ * the accuracy seed of the verification operator (0 under the direct audit). The seed
+ * audit discrepancy is that of the verification operator against the audit one. */
 HSBEM_API int hsbem_result_fmm(const hsbem_result *result, uint32_t frequency, double *out);
 
 /* ---- serving loop (no counterpart class in the reference: its callers run

@@ -174,6 +174,7 @@ def test_fmm_request_matches_dense_request():
         print("  FMM evidence", ff.fmm)
         assert ff.fmm is not None and ff.fmm["m2l_links"] > 0
         assert ff.fmm["quadrature_discrepancy"] <= 0.025 * 1e-6 and ff.fmm["endpoint_degree"] >= 4
+        assert ff.fmm["direct_audit"] or (ff.fmm["verification_seed"] <= 1e-9 and ff.fmm["audit_discrepancy"] <= 0.25e-6)
 
 
 def test_fmm_request_validation():

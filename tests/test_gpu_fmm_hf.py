@@ -182,3 +182,26 @@ def test_fmm_wide_spherical_orders_match_dense():
     assert _rel(yw, yn) <= 1e-7
     wide.close()
     narrow.close()
+
+
+def test_fmm_orders_above_64_match_dense():
+    """Accuracy seed 1e-16 on a 2562-node sphere at 14 kHz puts the leaf level at
+    order 65 (reduction-tiled rotations and coaxial products, the larger radial
+    table of the leaf moments, the 49-point degree-12 rule for k h = 2.1): the apply
+    still equals the dense matrix and the seed-1e-13 operator (orders <= 56)."""
+    mesh = meshes.icosphere(4)
+    k = 2 * math.pi * 14000.0 / 343.0
+    num = bem.BemNumerics()
+    solver = bem.HipBemSolver(mesh)
+    A, _ = solver.assemble(num, k, k)
+    big = solver.fmm_operator(num, k, k, leaf_capacity=32, accuracy_seed=1e-16)
+    ref = solver.fmm_operator(num, k, k, leaf_capacity=32, accuracy_seed=1e-13)
+    orders = big.statistics()["level_orders"]
+    print("orders", orders, "reference orders", ref.statistics()["level_orders"])
+    assert max(orders) > 64
+    x = _random(A.shape[0], 4)
+    yb, yr, yd = big.apply(x), ref.apply(x), A @ x
+    print(f"big vs dense {_rel(yb, yd):.2e}, vs seed-1e-13 operator {_rel(yb, yr):.2e}, apply {big.time_apply(3):.2f} ms")
+    assert _rel(yb, yd) <= 2e-6 and _rel(yb, yr) <= 1e-9
+    big.close()
+    ref.close()
