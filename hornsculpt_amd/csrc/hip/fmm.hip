@@ -305,17 +305,17 @@ __device__ __forceinline__ std::size_t degree_block(int l) {
 //   back      per chunk again: W = e^{-i m phi} d Z;
 //   gather    one block per destination: the sum of its links' W.
 // Y and W share one workspace slot per link, Z uses a second.
-constexpr int kStage = (2 * kMaxFmmOrder + 1) * (2 * kMaxFmmOrder + 1);
-
 // Degree by degree: d_l and the chunk's degree-l slices staged in LDS; the next
 // degree's values are loaded into registers while the current one is computed.
-constexpr int kStagePerThread = (kStage + 255) / 256;
-constexpr int kSlicePerThread = (kFmmChunk * (2 * kMaxFmmOrder + 1) + 255) / 256;
-
-__global__ __launch_bounds__(256) void rotate_links(FmmTranslateLaunch L, bool back) {
+// Instantiated for orders up to kP = 32 and 40 (LDS and registers sized to kP, so the
+// common orders 24-32 keep three workgroups per CU).
+template <int kP> __global__ __launch_bounds__(256) void rotate_links(FmmTranslateLaunch L, bool back) {
+    constexpr int kStage = (2 * kP + 1) * (2 * kP + 1);
+    constexpr int kStagePerThread = (kStage + 255) / 256;
+    constexpr int kSlicePerThread = (kFmmChunk * (2 * kP + 1) + 255) / 256;
     __shared__ double dl[kStage];
-    __shared__ C2 xs[kFmmChunk][2 * kMaxFmmOrder + 1]; // rows >= count: stale, their results are dropped
-    __shared__ C2 ph[2 * kMaxFmmOrder + 1];
+    __shared__ C2 xs[kFmmChunk][2 * kP + 1]; // rows >= count: stale, their results are dropped
+    __shared__ C2 ph[2 * kP + 1];
     __shared__ std::uint64_t src_off[kFmmChunk], dst_off[kFmmChunk];
     __shared__ int ord[kFmmChunk];
     const FmmChunk ch = L.chunks[blockIdx.x];
@@ -406,11 +406,12 @@ __global__ __launch_bounds__(256) void rotate_links(FmmTranslateLaunch L, bool b
 // Z = T Y per |m| block for a chunk of links sharing one coaxial plan: the
 // block T_m and the chunk's order-m columns of Y are staged in LDS, the next
 // block prefetched into registers during the current product.
-constexpr int kAxialRows = kMaxFmmOrder + 1;
-constexpr int kBlockPerThread = (kAxialRows * kAxialRows + 255) / 256;
-constexpr int kColumnPerThread = (kFmmChunk * 2 * kAxialRows + 255) / 256;
-
-__global__ __launch_bounds__(256) void coaxial_links(FmmTranslateLaunch L) {
+// T_m rows are stored with an odd stride (cols | 1) of 16-byte values, so the
+// consecutive rows read by consecutive lanes fall in distinct bank groups.
+template <int kP> __global__ __launch_bounds__(256) void coaxial_links(FmmTranslateLaunch L) {
+    constexpr int kAxialRows = kP + 1;
+    constexpr int kBlockPerThread = (kAxialRows * kAxialRows + 255) / 256;
+    constexpr int kColumnPerThread = (kFmmChunk * 2 * kAxialRows + 255) / 256;
     __shared__ C2 tm[kAxialRows * kAxialRows];
     __shared__ C2 ys[kFmmChunk][2][kAxialRows];
     __shared__ std::uint64_t slot[kFmmChunk];
@@ -457,11 +458,12 @@ __global__ __launch_bounds__(256) void coaxial_links(FmmTranslateLaunch L) {
     for (int m = 0; m <= mm; ++m) {
         const int rows = pt - m + 1, cols = ps - m + 1;
         __syncthreads();
+        const int stride = cols | 1;
 #pragma unroll
         for (int i = 0; i < kBlockPerThread; ++i) {
             const int e = threadIdx.x + 256 * i;
             if (e < rows * cols)
-                tm[e] = pt_[i];
+                tm[(e / cols) * stride + e % cols] = pt_[i];
         }
 #pragma unroll
         for (int i = 0; i < kColumnPerThread; ++i) {
@@ -482,7 +484,7 @@ __global__ __launch_bounds__(256) void coaxial_links(FmmTranslateLaunch L) {
             if (sgn && !m)
                 continue;
             C2 s0{0, 0}, s1{0, 0}, s2{0, 0}, s3{0, 0};
-            const C2 *trow = tm + (l - m) * cols;
+            const C2 *trow = tm + (l - m) * stride;
             for (int c = 0; c < cols; ++c) {
                 const C2 tv = trow[c];
                 s0 = s0 + tv * ys[j0][sgn][c];
@@ -611,8 +613,9 @@ __global__ __launch_bounds__(256) void coaxial_links_wide(FmmTranslateLaunch L) 
     for (int m = 0; m <= mm; ++m) {
         const int rows = pt - m + 1, cols = ps - m + 1;
         __syncthreads();
+        const int stride = cols | 1;
         for (int e = threadIdx.x; e < rows * cols; e += blockDim.x)
-            tm[e] = T[off + e];
+            tm[(e / cols) * stride + e % cols] = T[off + e];
         for (int e = threadIdx.x; e < kFmmChunk * 2 * cols; e += blockDim.x) {
             const int j = e / (2 * cols), r = e % (2 * cols), sgn = r / cols, n = m + r % cols;
             ys[j][sgn][r % cols] = j < count ? y[slot[j] + m_major(ps, n, sgn ? -m : m)]
@@ -625,7 +628,7 @@ __global__ __launch_bounds__(256) void coaxial_links_wide(FmmTranslateLaunch L) 
             if (sgn && !m)
                 continue;
             C2 s0{0, 0}, s1{0, 0}, s2{0, 0}, s3{0, 0};
-            const C2 *trow = tm + (l - m) * cols;
+            const C2 *trow = tm + (l - m) * stride;
             for (int c = 0; c < cols; ++c) {
                 const C2 tv = trow[c];
                 s0 = s0 + tv * ys[j0][sgn][c];
@@ -1132,10 +1135,14 @@ void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destinatio
         hipLaunchKernelGGL(rotate_links_wide, dim3(chunk_count), dim3(256), 0, stream, launch, false);
         hipLaunchKernelGGL(coaxial_links_wide, dim3(axial_chunk_count), dim3(256), 0, stream, launch);
         hipLaunchKernelGGL(rotate_links_wide, dim3(chunk_count), dim3(256), 0, stream, launch, true);
+    } else if (link_count && max_order > kSmallFmmOrder) {
+        hipLaunchKernelGGL(rotate_links<kMaxFmmOrder>, dim3(chunk_count), dim3(256), 0, stream, launch, false);
+        hipLaunchKernelGGL(coaxial_links<kMaxFmmOrder>, dim3(axial_chunk_count), dim3(256), 0, stream, launch);
+        hipLaunchKernelGGL(rotate_links<kMaxFmmOrder>, dim3(chunk_count), dim3(256), 0, stream, launch, true);
     } else if (link_count) {
-        hipLaunchKernelGGL(rotate_links, dim3(chunk_count), dim3(256), 0, stream, launch, false);
-        hipLaunchKernelGGL(coaxial_links, dim3(axial_chunk_count), dim3(256), 0, stream, launch);
-        hipLaunchKernelGGL(rotate_links, dim3(chunk_count), dim3(256), 0, stream, launch, true);
+        hipLaunchKernelGGL(rotate_links<kSmallFmmOrder>, dim3(chunk_count), dim3(256), 0, stream, launch, false);
+        hipLaunchKernelGGL(coaxial_links<kSmallFmmOrder>, dim3(axial_chunk_count), dim3(256), 0, stream, launch);
+        hipLaunchKernelGGL(rotate_links<kSmallFmmOrder>, dim3(chunk_count), dim3(256), 0, stream, launch, true);
     }
     hipLaunchKernelGGL(gather_links, dim3(destination_count), dim3(256), 0, stream, launch);
 }

@@ -11,6 +11,7 @@ namespace hsbem::dev {
 
 inline constexpr int kMaxLeafOrder = 64;   // leaf orders with the smaller radial register table (up to kMaxBigOrder)
 inline constexpr int kMaxLeafDofs = 512;   // P1 DOFs of one leaf
+inline constexpr int kSmallFmmOrder = 32;  // smaller instantiation of the staged translations
 inline constexpr int kMaxFmmOrder = 40;    // translations with the whole d_l block staged: (2p+1)^2 doubles
 inline constexpr int kMaxWideOrder = 64;   // tiled d_l stage above kMaxFmmOrder
 inline constexpr int kMaxBigOrder = 128;   // any translated expansion (reduction-tiled kernels above kMaxWideOrder)
