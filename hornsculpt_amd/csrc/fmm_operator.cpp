@@ -385,15 +385,16 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
     // ---- translation links
     t0 = Clock::now();
     // one rotation plan per (direction, order class): a launch's kernels are chosen by
-    // its largest order (<= 32, <= 40, <= 64, <= 128), so narrower levels never share a
-    // plan with a wider one
+    // its largest order (<= 32, <= 40, <= 48, <= 64, <= 128), so narrower levels never
+    // share a plan with a wider one
     std::map<std::pair<Triple, int>, std::uint32_t> direction_index;
     std::vector<Triple> directions;
     std::vector<int> direction_order;
     auto direction = [&](Triple t, int ord) {
         t = reduced(t);
-        const int order_class = ord > dev::kMaxWideOrder  ? 3
-                                : ord > dev::kMaxFmmOrder ? 2
+        const int order_class = ord > dev::kMaxWideOrder    ? 4
+                                : ord > dev::kMidFmmOrder   ? 3
+                                : ord > dev::kMaxFmmOrder   ? 2
                                 : ord > dev::kSmallFmmOrder ? 1
                                                             : 0;
         auto [it, fresh] = direction_index.emplace(std::make_pair(t, order_class),

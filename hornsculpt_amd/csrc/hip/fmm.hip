@@ -307,8 +307,8 @@ __device__ __forceinline__ std::size_t degree_block(int l) {
 // Y and W share one workspace slot per link, Z uses a second.
 // Degree by degree: d_l and the chunk's degree-l slices staged in LDS; the next
 // degree's values are loaded into registers while the current one is computed.
-// Instantiated for orders up to kP = 32 and 40 (LDS and registers sized to kP, so the
-// common orders 24-32 keep three workgroups per CU).
+// Instantiated for orders up to kP = 32, 40 and 48 (LDS and registers sized to kP, so
+// the common orders 24-32 keep three workgroups per CU).
 template <int kP> __global__ __launch_bounds__(256) void rotate_links(FmmTranslateLaunch L, bool back) {
     constexpr int kStage = (2 * kP + 1) * (2 * kP + 1);
     constexpr int kStagePerThread = (kStage + 255) / 256;
@@ -1131,10 +1131,14 @@ void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destinatio
         hipLaunchKernelGGL(rotate_links_big, dim3(chunk_count), dim3(256), 0, stream, launch, false);
         hipLaunchKernelGGL(coaxial_links_big, dim3(axial_chunk_count), dim3(256), 0, stream, launch);
         hipLaunchKernelGGL(rotate_links_big, dim3(chunk_count), dim3(256), 0, stream, launch, true);
-    } else if (link_count && max_order > kMaxFmmOrder) {
+    } else if (link_count && max_order > kMidFmmOrder) {
         hipLaunchKernelGGL(rotate_links_wide, dim3(chunk_count), dim3(256), 0, stream, launch, false);
         hipLaunchKernelGGL(coaxial_links_wide, dim3(axial_chunk_count), dim3(256), 0, stream, launch);
         hipLaunchKernelGGL(rotate_links_wide, dim3(chunk_count), dim3(256), 0, stream, launch, true);
+    } else if (link_count && max_order > kMaxFmmOrder) {
+        hipLaunchKernelGGL(rotate_links<kMidFmmOrder>, dim3(chunk_count), dim3(256), 0, stream, launch, false);
+        hipLaunchKernelGGL(coaxial_links<kMidFmmOrder>, dim3(axial_chunk_count), dim3(256), 0, stream, launch);
+        hipLaunchKernelGGL(rotate_links<kMidFmmOrder>, dim3(chunk_count), dim3(256), 0, stream, launch, true);
     } else if (link_count && max_order > kSmallFmmOrder) {
         hipLaunchKernelGGL(rotate_links<kMaxFmmOrder>, dim3(chunk_count), dim3(256), 0, stream, launch, false);
         hipLaunchKernelGGL(coaxial_links<kMaxFmmOrder>, dim3(axial_chunk_count), dim3(256), 0, stream, launch);

@@ -13,7 +13,8 @@ inline constexpr int kMaxLeafOrder = 64;   // leaf orders with the smaller radia
 inline constexpr int kMaxLeafDofs = 512;   // P1 DOFs of one leaf
 inline constexpr int kSmallFmmOrder = 32;  // smaller instantiation of the staged translations
 inline constexpr int kMaxFmmOrder = 40;    // translations with the whole d_l block staged: (2p+1)^2 doubles
-inline constexpr int kMaxWideOrder = 64;   // tiled d_l stage above kMaxFmmOrder
+inline constexpr int kMidFmmOrder = 48;    // largest whole-block instantiation (one workgroup per CU)
+inline constexpr int kMaxWideOrder = 64;   // tiled d_l stage above kMidFmmOrder
 inline constexpr int kMaxBigOrder = 128;   // any translated expansion (reduction-tiled kernels above kMaxWideOrder)
 inline constexpr int kFmmChunk = 16;       // links per rotation chunk
 inline constexpr int kFmmJacobi = 64;      // block width of the near block-Jacobi blocks
