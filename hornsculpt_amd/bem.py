@@ -57,6 +57,7 @@ class _Numerics(ctypes.Structure):
         ("fmm_separation_ratio", ctypes.c_double),
         ("fmm_maximum_iterations", ctypes.c_uint32),
         ("fmm_restart_length", ctypes.c_uint32),
+        ("fmm_block_width", ctypes.c_uint32),
     ]
 
 
@@ -289,6 +290,7 @@ class BemNumerics:
     fmm_separation_ratio: float = 1.5
     fmm_maximum_iterations: int = 300
     fmm_restart_length: int = 40
+    fmm_block_width: int = 32            # preconditionerBlockWidth (bem.hpp:70), power of two 32..2048
 
     def native(self) -> _Numerics:
         n = _Numerics()

@@ -104,6 +104,7 @@ hsbem::Numerics to_numerics(const hsbem_numerics &n) {
     o.fmm_separation = n.fmm_separation_ratio;
     o.fmm_maximum_iterations = n.fmm_maximum_iterations;
     o.fmm_restart = n.fmm_restart_length;
+    o.fmm_block_width = n.fmm_block_width;
     return o;
 }
 
@@ -247,6 +248,7 @@ void hsbem_numerics_default(hsbem_numerics *n) {
     n->fmm_separation_ratio = 1.5;
     n->fmm_maximum_iterations = 300;
     n->fmm_restart_length = 40;
+    n->fmm_block_width = 32;
 }
 
 const char *hsbem_last_error(void) { return last_error.c_str(); }
@@ -732,6 +734,7 @@ int hsbem_fmm_operator_create(hsbem_mesh *m, const hsbem_numerics *n, const hsbe
         require(fs.endpoint_degree == 0 || fs.endpoint_degree == 4 || fs.endpoint_degree == 6 ||
                     fs.endpoint_degree == 8 || fs.endpoint_degree == 12 || fs.endpoint_degree == 14,
                 "FMM endpoint degree must be 0, 4, 6, 8, 12 or 14");
+        fs.block_width = n->fmm_block_width; // the preconditioner's blocks follow the numerics
         hsbem::hip_check(hipSetDevice(m->context->ctx.device), "set device");
         auto op = std::make_unique<hsbem_fmm_operator>();
         op->solver = m->solver;

@@ -182,8 +182,11 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
     if (!std::isfinite(k) || k <= 0 || !std::isfinite(fs.accuracy_seed) || fs.accuracy_seed <= 0 ||
         fs.accuracy_seed >= 1)
         throw std::invalid_argument("FMM operator needs a positive wavenumber and an accuracy seed in (0, 1).");
+    if (fs.block_width < 32 || fs.block_width > dev::kMaxJacobiWidth || (fs.block_width & (fs.block_width - 1)))
+        throw std::invalid_argument("FMM preconditioner block width must be a power of two in [32, 2048].");
     auto op = std::unique_ptr<FmmOperator>(new FmmOperator());
     op->k_ = k;
+    op->block_width_ = share ? share->block_width_ : fs.block_width;
     FmmStatistics &st = op->stats_;
     const std::uint32_t nv = this->nv(), nt = this->nt();
     op->n_ = nv;
@@ -857,17 +860,25 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
     DeviceBuffer<std::uint32_t> d_pos;
     DeviceBuffer<double> blocks;
     if (!share) {
-        const std::uint32_t nb = (nv + dev::kFmmJacobi - 1) / dev::kFmmJacobi;
+        const std::uint32_t W = fs.block_width, nb = (nv + W - 1) / W;
         std::vector<std::uint32_t> position(nv);
         for (std::uint32_t i = 0; i < nv; ++i)
             position[tree.pressure_order[i]] = i;
         d_pos.upload(position, s);
-        blocks.resize(2 * static_cast<std::size_t>(nb) * dev::kFmmJacobi * dev::kFmmJacobi);
+        blocks.resize(2 * static_cast<std::size_t>(nb) * W * W);
         hip_check(hipMemsetAsync(blocks.data(), 0, blocks.bytes(), s), "clear blocks");
         dev::launch_gather_blocks(ns->row_offsets.data(), ns->columns.data(), op->near_->values.data(),
-                                  op->order_.data(), d_pos.data(), nv, blocks.data(), s);
-        auto jac = std::make_shared<DeviceBuffer<double>>(dev::jacobi_factor_doubles(nv));
-        dev::jacobi_factor_gathered(blocks.data(), nv, jac->data(), s);
+                                  op->order_.data(), d_pos.data(), nv, W, blocks.data(), s);
+        std::shared_ptr<DeviceBuffer<double>> jac;
+        if (W == static_cast<std::uint32_t>(dev::kFmmJacobi)) {
+            jac = std::make_shared<DeviceBuffer<double>>(dev::jacobi_factor_doubles(nv));
+            dev::jacobi_factor_gathered(blocks.data(), nv, jac->data(), s);
+        } else {
+            jac = std::make_shared<DeviceBuffer<double>>(dev::jacobi_wide_doubles(nv, W));
+            DeviceBuffer<std::int32_t> pivots(static_cast<std::size_t>(nb) * W);
+            dev::jacobi_factor_wide(blocks.data(), nv, W, jac->data(), pivots.data(), s);
+            hip_check(hipStreamSynchronize(s), "block-Jacobi factor");
+        }
         op->jacobi_ = jac;
     }
     hip_check(hipEventRecord(e1, s), "event");
@@ -1028,7 +1039,10 @@ bool FmmOperator::prefer_direct() const {
 }
 
 void FmmOperator::precondition(const double *v, double *z, hipStream_t s) const {
-    dev::jacobi_apply(jacobi_->data(), n_, v, z, s, order_.data());
+    if (block_width_ == static_cast<std::uint32_t>(dev::kFmmJacobi))
+        dev::jacobi_apply(jacobi_->data(), n_, v, z, s, order_.data());
+    else
+        dev::jacobi_apply_wide(jacobi_->data(), n_, block_width_, v, z, s, order_.data());
 }
 
 namespace {
@@ -1073,6 +1087,7 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
     fs.tree.leaf_capacity = num.fmm_leaf_capacity;
     fs.tree.separation = num.fmm_separation;
     fs.tree.near_ratio = num.near_ratio;
+    fs.block_width = num.fmm_block_width;
     fs.accuracy_seed = 0.1 * tol;
     const auto t_build = Clock::now();
     const std::unique_ptr<FmmOperator> op = build_fmm(num, P.k_max, k, fs, faces, s, nullptr);

@@ -31,6 +31,7 @@ struct FmmSettings {
     SupportTreeSettings tree{64, 20, 1.5, 2.0};
     double accuracy_seed{5e-7}; // Eq. 18 accuracy of the level orders
     unsigned endpoint_degree{};  // leaf panel rule degree; 0: the electrical minimum
+    unsigned block_width{32};    // block-Jacobi width (preconditionerBlockWidth), power of two 32..2048
 };
 
 struct FmmStatistics {
@@ -73,7 +74,7 @@ class FmmOperator {
     // graph runs on operator-owned input/output buffers, captured on first use on
     // the given (non-default) stream; x and y are copied in and out on the device.
     void apply_graph(const double *x, double *y, hipStream_t stream) const;
-    // z = M^{-1} v (block-Jacobi of the near operator).
+    // z = M^{-1} v (block-Jacobi of the near operator, blocks of the configured width).
     void precondition(const double *v, double *z, hipStream_t stream) const;
     // y = A x with the far part integrated directly over every far DOF pair (the
     // direct Galerkin audit, SupportDirectAudit::applyPressureDevice,
@@ -137,7 +138,7 @@ class FmmOperator {
     DeviceBuffer<dev::HfLeaf> hf_leaves_;
     DeviceBuffer<dev::HfLeafItem> hf_leaf_items_;
     DeviceBuffer<double> leaf_scale_, leaf_rule_;
-    std::uint32_t hf_leaf_count_{}, hf_leaf_item_count_{}, hf_max_samples_{};
+    std::uint32_t hf_leaf_count_{}, hf_leaf_item_count_{}, hf_max_samples_{}, block_width_{64};
     // direct audit: solver-owned geometry (alive while the operator is), leaf map, near matrix
     const double *geo_{}, *row_scale_{};
     const std::uint32_t *nodes_{};

@@ -851,19 +851,19 @@ __global__ __launch_bounds__(256) void gather_links(FmmTranslateLaunch L) {
 // order[b*W .. b*W+W) (tree order); entries absent from the CSR are zero.
 __global__ void gather_blocks(const std::uint32_t *row_offsets, const std::uint32_t *columns, const double *values,
                               const std::uint32_t *order, const std::uint32_t *position, std::uint32_t n,
-                              double *blocks) {
+                              std::uint32_t width, double *blocks) {
     const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    const std::uint32_t row = order[i], blk = i / kFmmJacobi, r = i % kFmmJacobi;
-    double *B = blocks + static_cast<std::size_t>(blk) * kFmmJacobi * kFmmJacobi * 2;
+    const std::uint32_t W = width, row = order[i], blk = i / W, r = i % W;
+    double *B = blocks + static_cast<std::size_t>(blk) * W * W * 2;
     for (std::uint32_t e = row_offsets[row]; e < row_offsets[row + 1]; ++e) {
         const std::uint32_t pc = position[columns[e]];
-        if (pc / kFmmJacobi != blk)
+        if (pc / W != blk)
             continue;
-        const std::uint32_t c = pc % kFmmJacobi;
-        B[2 * (static_cast<std::size_t>(c) * kFmmJacobi + r)] = values[2 * e];
-        B[2 * (static_cast<std::size_t>(c) * kFmmJacobi + r) + 1] = values[2 * e + 1];
+        const std::uint32_t c = pc % W;
+        B[2 * (static_cast<std::size_t>(c) * W + r)] = values[2 * e];
+        B[2 * (static_cast<std::size_t>(c) * W + r) + 1] = values[2 * e + 1];
     }
 }
 
@@ -1152,10 +1152,10 @@ void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destinatio
 }
 
 void launch_gather_blocks(const std::uint32_t *row_offsets, const std::uint32_t *columns, const double *values,
-                          const std::uint32_t *order, const std::uint32_t *position, std::uint32_t n, double *blocks,
-                          hipStream_t stream) {
+                          const std::uint32_t *order, const std::uint32_t *position, std::uint32_t n,
+                          std::uint32_t width, double *blocks, hipStream_t stream) {
     hipLaunchKernelGGL(gather_blocks, dim3((n + 255) / 256), dim3(256), 0, stream, row_offsets, columns, values, order,
-                       position, n, blocks);
+                       position, n, width, blocks);
 }
 
 void launch_direct_far(const FmmDirectLaunch &launch, hipStream_t stream) {

@@ -17,7 +17,7 @@ inline constexpr int kMidFmmOrder = 48;    // largest whole-block instantiation 
 inline constexpr int kMaxWideOrder = 64;   // tiled d_l stage above kMidFmmOrder
 inline constexpr int kMaxBigOrder = 128;   // any translated expansion (reduction-tiled kernels above kMaxWideOrder)
 inline constexpr int kFmmChunk = 16;       // links per rotation chunk
-inline constexpr int kFmmJacobi = 64;      // block width of the near block-Jacobi blocks
+inline constexpr int kFmmJacobi = 64;      // block width served by the LDS factorisation (others: the wide path)
 
 // One leaf of the support tree with P1 DOFs. Offsets count complex values.
 struct FmmLeaf {
@@ -258,10 +258,10 @@ void launch_leaf_test(const FmmLeaf *leaves, std::uint32_t leaf_count, const std
 void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destination_count, std::uint32_t link_count,
                       std::uint32_t chunk_count, std::uint32_t axial_chunk_count, std::uint32_t max_order,
                       hipStream_t stream);
-// blocks (zeroed, n rounded up to kFmmJacobi) from the near CSR in tree order;
+// blocks (zeroed, n rounded up to width) from the near CSR in tree order;
 // position[dof] = index of dof in order.
 void launch_gather_blocks(const std::uint32_t *row_offsets, const std::uint32_t *columns, const double *values,
-                          const std::uint32_t *order, const std::uint32_t *position, std::uint32_t n, double *blocks,
-                          hipStream_t stream);
+                          const std::uint32_t *order, const std::uint32_t *position, std::uint32_t n,
+                          std::uint32_t width, double *blocks, hipStream_t stream);
 
 } // namespace hsbem::dev

@@ -152,6 +152,16 @@ void lu_solve(const double *LU, std::uint32_t n, const std::int32_t *pivots, dou
               hipStream_t stream);
 
 // r = b - A x, scaled by weight (lumped mass) into weighted; r unscaled into residual.
+// Block-Jacobi of any width W <= kMaxJacobiWidth (a power of two) over gathered
+// blocks (nb x W x W complex, column-major per block, factored in place by a panel
+// LU; pivots nb x W); the inverses are column-major W x W per block and
+// out = M^{-1} v with block rows order[W b .. W b + W) when order is given.
+inline constexpr std::uint32_t kMaxJacobiWidth = 2048;
+std::size_t jacobi_wide_doubles(std::uint32_t n, std::uint32_t width);
+void jacobi_factor_wide(double *blocks, std::uint32_t n, std::uint32_t width, double *inverses, std::int32_t *pivots,
+                        hipStream_t stream);
+void jacobi_apply_wide(const double *inverses, std::uint32_t n, std::uint32_t width, const double *v, double *out,
+                       hipStream_t stream, const std::uint32_t *order = nullptr);
 // Block-Jacobi preconditioner of 64 x 64 diagonal blocks (consecutive unknowns):
 // explicit FP64 inverses per block (LU with partial pivoting), and out = M^{-1} v.
 std::size_t jacobi_factor_doubles(std::uint32_t n);

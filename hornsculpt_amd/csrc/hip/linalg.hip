@@ -691,7 +691,237 @@ __global__ __launch_bounds__(64) void jacobi_apply(const double *inverses, std::
     }
 }
 
+// Block-Jacobi of any block width W (the FMM preconditioner's preconditionerBlockWidth).
+// One workgroup per gathered W x W block (complex FP64, column-major a[c * W + r]),
+// factored in place in global memory by a right-looking LU with partial pivoting in
+// panels of kWidePanel columns: the panel is factored column by column (pivot search,
+// full-row interchange, scaling, update of the remaining panel columns), then the
+// panel's rows of U are solved against the unit L11 and the trailing matrix takes
+// one rank-kWidePanel update, so the trailing matrix is read and written once per panel
+// instead of once per column. The inverse X = U^{-1} L^{-1} P is formed the same
+// way: X starts as P, a blocked forward sweep with L and a blocked backward sweep
+// with U, each a small per-column triangular solve plus a rank-kWidePanel update of the
+// rows below (above). A vanishing pivot gives the identity block, as in the LDS kernel.
+constexpr int kWidePanel = 16;
+
+// A[r][c] -= sum_t L(r, t) U(t, c) for r in [rb, re), c in [cb, ce), t < kb, with
+// L(r, t) = lp[t * W + r], U(t, c) = up[c * W + t]; all column-major with leading
+// dimension W. A thread keeps its row of L in registers and streams columns; when
+// there are fewer rows than threads, the threads split the columns into groups.
+__device__ void rank_update(double2 *A, std::uint32_t W, int rb, int re, int cb, int ce, const double2 *lp,
+                            const double2 *up, int kb) {
+    const int rows = re - rb;
+    if (rows <= 0 || ce <= cb)
+        return;
+    const int threads = static_cast<int>(blockDim.x);
+    const int groups = rows >= threads ? 1 : threads / rows;
+    for (int e = static_cast<int>(threadIdx.x); e < rows * groups; e += threads) {
+        const int r = rb + e % rows, g = e / rows;
+        double2 l[kWidePanel];
+#pragma unroll
+        for (int t = 0; t < kWidePanel; ++t)
+            l[t] = t < kb ? lp[static_cast<std::size_t>(t) * W + r] : double2{0, 0};
+        for (int c = cb + g; c < ce; c += groups) {
+            double2 acc = A[static_cast<std::size_t>(c) * W + r];
+            const double2 *u = up + static_cast<std::size_t>(c) * W;
+#pragma unroll
+            for (int t = 0; t < kWidePanel; ++t)
+                if (t < kb) {
+                    const double2 v = u[t];
+                    acc.x -= l[t].x * v.x - l[t].y * v.y;
+                    acc.y -= l[t].x * v.y + l[t].y * v.x;
+                }
+            A[static_cast<std::size_t>(c) * W + r] = acc;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void jacobi_factor_wide(double *blocks, std::uint32_t n, std::uint32_t width,
+                                                          double *inverses, std::int32_t *pivots) {
+    __shared__ double red_mag[256];
+    __shared__ int red_idx[256];
+    __shared__ int singular;
+    const std::uint32_t blk = blockIdx.x, W = width, r0 = blk * W;
+    const int m = static_cast<int>(min(W, n - r0));
+    double2 *a = reinterpret_cast<double2 *>(blocks) + static_cast<std::size_t>(blk) * W * W;
+    std::int32_t *p = pivots + static_cast<std::size_t>(blk) * W;
+    const int tid = threadIdx.x;
+    auto at = [&](int c, int r) -> double2 & { return a[static_cast<std::size_t>(c) * W + r]; };
+    for (int r = tid; r < static_cast<int>(W); r += 256)
+        p[r] = r;
+    if (tid == 0)
+        singular = 0;
+    __syncthreads();
+    for (int j0 = 0; j0 < m; j0 += kWidePanel) {
+        const int je = min(j0 + kWidePanel, m);
+        for (int j = j0; j < je; ++j) {
+            double best = -1.0;
+            int bi = j;
+            for (int r = j + tid; r < m; r += 256) {
+                const double2 v = at(j, r);
+                const double mag = v.x * v.x + v.y * v.y;
+                if (mag > best) {
+                    best = mag;
+                    bi = r;
+                }
+            }
+            red_mag[tid] = best;
+            red_idx[tid] = bi;
+            __syncthreads();
+            for (int off = 128; off > 0; off >>= 1) {
+                if (tid < off) {
+                    const double om = red_mag[tid + off];
+                    const int oi = red_idx[tid + off];
+                    if (om > red_mag[tid] || (om == red_mag[tid] && oi < red_idx[tid])) {
+                        red_mag[tid] = om;
+                        red_idx[tid] = oi;
+                    }
+                }
+                __syncthreads();
+            }
+            if (!(red_mag[0] > 0)) { // uniform: every thread reads the reduced value
+                if (tid == 0)
+                    singular = 1;
+                break;
+            }
+            const int pr = red_idx[0];
+            if (pr != j) {
+                for (int c = tid; c < m; c += 256) {
+                    const double2 t = at(c, j);
+                    at(c, j) = at(c, pr);
+                    at(c, pr) = t;
+                }
+                if (tid == 0) {
+                    const int t = p[j];
+                    p[j] = p[pr];
+                    p[pr] = t;
+                }
+            }
+            __syncthreads();
+            const double2 d = at(j, j);
+            const double inv = 1.0 / (d.x * d.x + d.y * d.y);
+            for (int r = j + 1 + tid; r < m; r += 256) {
+                const double2 x = at(j, r);
+                at(j, r) = {(x.x * d.x + x.y * d.y) * inv, (x.y * d.x - x.x * d.y) * inv};
+            }
+            __syncthreads();
+            const int rows = m - j - 1, cols = je - j - 1;
+            for (int e = tid; e < rows * cols; e += 256) {
+                const int c = j + 1 + e / rows, r = j + 1 + e % rows;
+                const double2 l = at(j, r), u = at(c, j);
+                double2 &t = at(c, r);
+                t.x -= l.x * u.x - l.y * u.y;
+                t.y -= l.x * u.y + l.y * u.x;
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+        if (singular)
+            break;
+        for (int c = je + tid; c < m; c += 256) // U12 = L11^{-1} A12
+            for (int j = j0; j < je; ++j) {
+                const double2 u = at(c, j);
+                for (int i = j + 1; i < je; ++i) {
+                    const double2 l = at(j, i);
+                    at(c, i).x -= l.x * u.x - l.y * u.y;
+                    at(c, i).y -= l.x * u.y + l.y * u.x;
+                }
+            }
+        __syncthreads();
+        rank_update(a, W, je, m, je, m, a + static_cast<std::size_t>(j0) * W, a + j0, je - j0);
+        __syncthreads();
+    }
+    double2 *x = reinterpret_cast<double2 *>(inverses) + static_cast<std::size_t>(blk) * W * W; // x[c * W + r]
+    const bool identity = singular != 0;
+    for (std::size_t e = tid; e < static_cast<std::size_t>(W) * W; e += 256) {
+        const int c = static_cast<int>(e / W), r = static_cast<int>(e % W);
+        const bool inside = !identity && r < m && c < m;
+        x[e] = {inside ? (p[r] == c ? 1.0 : 0.0) : (r == c ? 1.0 : 0.0), 0.0};
+    }
+    __syncthreads();
+    if (identity)
+        return;
+    auto xt = [&](int c, int r) -> double2 & { return x[static_cast<std::size_t>(c) * W + r]; };
+    for (int j0 = 0; j0 < m; j0 += kWidePanel) { // L X = P
+        const int je = min(j0 + kWidePanel, m);
+        for (int c = tid; c < m; c += 256)
+            for (int j = j0; j < je; ++j) {
+                const double2 y = xt(c, j);
+                for (int i = j + 1; i < je; ++i) {
+                    const double2 l = at(j, i);
+                    xt(c, i).x -= l.x * y.x - l.y * y.y;
+                    xt(c, i).y -= l.x * y.y + l.y * y.x;
+                }
+            }
+        __syncthreads();
+        rank_update(x, W, je, m, 0, m, a + static_cast<std::size_t>(j0) * W, x + j0, je - j0);
+        __syncthreads();
+    }
+    const int last = ((m - 1) / kWidePanel) * kWidePanel;
+    for (int j0 = last; j0 >= 0; j0 -= kWidePanel) { // U X = Y
+        const int je = min(j0 + kWidePanel, m);
+        for (int c = tid; c < m; c += 256)
+            for (int j = je - 1; j >= j0; --j) {
+                const double2 d = at(j, j), y = xt(c, j);
+                const double inv = 1.0 / (d.x * d.x + d.y * d.y);
+                const double2 v = {(y.x * d.x + y.y * d.y) * inv, (y.y * d.x - y.x * d.y) * inv};
+                xt(c, j) = v;
+                for (int i = j0; i < j; ++i) {
+                    const double2 u = at(j, i);
+                    xt(c, i).x -= u.x * v.x - u.y * v.y;
+                    xt(c, i).y -= u.x * v.y + u.y * v.x;
+                }
+            }
+        __syncthreads();
+        rank_update(x, W, 0, j0, 0, m, a + static_cast<std::size_t>(j0) * W, x + j0, je - j0);
+        __syncthreads();
+    }
+}
+
+// out = M^{-1} v for width W: one workgroup of min(W, 256) threads per block, the
+// block's v in LDS, threads stride over the rows (column-major inverse: coalesced
+// per column).
+__global__ __launch_bounds__(256) void jacobi_apply_wide(const double *inverses, std::uint32_t n, std::uint32_t width,
+                                                         const double *v, double *out, const std::uint32_t *order) {
+    __shared__ double2 sv[kMaxJacobiWidth];
+    const std::uint32_t blk = blockIdx.x, W = width, r0 = blk * W;
+    const int m = static_cast<int>(min(W, n - r0)), threads = static_cast<int>(blockDim.x);
+    for (int i = threadIdx.x; i < m; i += threads) {
+        const std::uint32_t row = order ? order[r0 + i] : r0 + i;
+        sv[i] = double2{v[2 * row], v[2 * row + 1]};
+    }
+    __syncthreads();
+    const double2 *f = reinterpret_cast<const double2 *>(inverses) + static_cast<std::size_t>(blk) * W * W;
+    for (int r = threadIdx.x; r < m; r += threads) {
+        double re = 0, im = 0;
+        for (int c = 0; c < m; ++c) {
+            const double2 b = f[static_cast<std::size_t>(c) * W + r];
+            re += b.x * sv[c].x - b.y * sv[c].y;
+            im += b.x * sv[c].y + b.y * sv[c].x;
+        }
+        const std::uint32_t row = order ? order[r0 + r] : r0 + r;
+        out[2 * row] = re;
+        out[2 * row + 1] = im;
+    }
+}
+
 } // namespace
+
+std::size_t jacobi_wide_doubles(std::uint32_t n, std::uint32_t width) {
+    return static_cast<std::size_t>((n + width - 1) / width) * width * width * 2;
+}
+
+void jacobi_factor_wide(double *blocks, std::uint32_t n, std::uint32_t width, double *inverses, std::int32_t *pivots,
+                        hipStream_t stream) {
+    hipLaunchKernelGGL(jacobi_factor_wide, dim3((n + width - 1) / width), dim3(256), 0, stream, blocks, n, width,
+                       inverses, pivots);
+}
+
+void jacobi_apply_wide(const double *inverses, std::uint32_t n, std::uint32_t width, const double *v, double *out,
+                       hipStream_t stream, const std::uint32_t *order) {
+    hipLaunchKernelGGL(jacobi_apply_wide, dim3((n + width - 1) / width), dim3(std::min(width, 256u)), 0, stream,
+                       inverses, n, width, v, out, order);
+}
 
 std::size_t gemv_scratch_doubles(std::uint32_t n, std::uint32_t nvec) {
     const std::size_t chunks = (n + kGemvCols - 1) / kGemvCols;

@@ -707,7 +707,9 @@ std::shared_ptr<PreparedRequest> MeshSolver::prepare(const Request &req) const {
     if (num.acceleration == 1 &&
         (num.fmm_leaf_capacity < 4 || num.fmm_leaf_capacity > 256 || !std::isfinite(num.fmm_separation) ||
          num.fmm_separation < 1.5 || num.fmm_separation > 10 || num.fmm_maximum_iterations < 1 ||
-         num.fmm_maximum_iterations > 10000 || num.fmm_restart < 2 || num.fmm_restart > 300))
+         num.fmm_maximum_iterations > 10000 || num.fmm_restart < 2 || num.fmm_restart > 300 ||
+         num.fmm_block_width < 32 || num.fmm_block_width > 2048 ||
+         (num.fmm_block_width & (num.fmm_block_width - 1)) != 0))
         throw std::invalid_argument("Invalid Fast Multipole or GMRES settings.");
     if (num.acceleration == 1 && num.precision != 0)
         throw std::invalid_argument("The fast multipole form runs in FP64 (precision must be FP64).");
@@ -783,7 +785,8 @@ std::shared_ptr<PreparedRequest> MeshSolver::prepare(const Request &req) const {
     if (fmm) // operator estimate: near entries, leaf moments, expansions and link workspaces
         bytes += 3 * 16ull * nv * ne + 16ull * nv * 2048 + (64ull << 20) +
                  16ull * nv * (num.fmm_restart + 2) + 16ull * nv + 8ull * (4 * (num.fmm_restart + 2) + 8) +
-                 8 * dev::jacobi_factor_doubles(nv);
+                 8 * (num.fmm_block_width == 64 ? dev::jacobi_factor_doubles(nv)
+                                                : 2 * dev::jacobi_wide_doubles(nv, num.fmm_block_width));
     else if (lu)
         bytes += 16 * nv2 + 4ull * nv;
     else

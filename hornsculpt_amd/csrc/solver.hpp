@@ -97,6 +97,7 @@ struct Numerics {
     unsigned fmm_leaf_capacity{16};
     double fmm_separation{1.5};
     unsigned fmm_maximum_iterations{300}, fmm_restart{40};
+    unsigned fmm_block_width{32}; // preconditionerBlockWidth: power of two 32..2048
 };
 
 struct Excitation {

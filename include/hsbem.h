@@ -91,6 +91,7 @@ typedef struct {
     double fmm_separation_ratio;         /* separationRatio, 1.5..10 (default 1.5) */
     uint32_t fmm_maximum_iterations;     /* maximumIterations (default 300) */
     uint32_t fmm_restart_length;         /* restartLength (default 40) */
+    uint32_t fmm_block_width;            /* preconditionerBlockWidth (bem.hpp:70): 32 (default), 64, ..., 2048 */
 } hsbem_numerics;
 
 /* BemRequest (bem.hpp:163-191) for a closed exterior mesh. Excitation e assigns

@@ -5,6 +5,7 @@ Parity: the operator's action equals the dense row-scaled Burton-Miller matrix
 (FP64 oracle on the CPU for the sphere, the device dense assembly, itself
 parity-tested against the oracle, for the horn) to the FMM accuracy, and GMRES
 on the FMM operator reproduces the dense solution."""
+import dataclasses
 import math
 
 import numpy as np
@@ -140,6 +141,56 @@ def test_fmm_gmres_reproduces_dense_solution():
     op.close()
 
 
+@pytest.fixture(scope="module")
+def horn_near():
+    """1950-node horn at 1.5 kHz: the near CSR of the FMM split (mesh order) and the
+    support tree's pressure order for leaf capacity 32."""
+    mesh = meshes.exponential_horn(0.012)
+    k = 2 * math.pi * 1500.0 / 343.0
+    num = bem.BemNumerics()
+    solver = bem.HipBemSolver(mesh)
+    near = solver.near_operator(num, k, k, [1], leaf_capacity=32)
+    n = len(mesh.vertices)
+    rows = np.repeat(np.arange(n), np.diff(near["row_offsets"]))
+    N = np.zeros((n, n), complex)
+    N[rows, near["columns"]] = near["values"]
+    shifted = bem.SurfaceMesh(mesh.vertices - mesh.vertices[0], mesh.triangles, mesh.face_ids, mesh.boundary)
+    order = bem.parse_support_tree(bem.support_tree(shifted, leaf_capacity=32))["pressure_order"]
+    return mesh, k, solver, N, order
+
+
+@pytest.mark.parametrize("width", [32, 64, 128, 256, 1024, 2048])
+def test_fmm_block_jacobi_widths(horn_near, width):
+    """preconditionerBlockWidth 32..2048 (support_preconditioner.cu:33-190): z = M^{-1} v
+    equals, block by block, the numpy solve with the diagonal W x W blocks of the near
+    matrix in tree order (blocks with tails: 1950 = 60 x 32 + 30; at 2048 the single
+    block is the whole near matrix, independent of the order), and GMRES converges."""
+    mesh, k, solver, N, order = horn_near
+    num = dataclasses.replace(bem.BemNumerics(), fmm_block_width=width)
+    op = solver.fmm_operator(num, k, k, leaf_capacity=32)
+    rng = np.random.default_rng(width)
+    v = rng.standard_normal(len(order)) + 1j * rng.standard_normal(len(order))
+    z = op.precondition(v)
+    want = np.zeros_like(v)
+    for b0 in range(0, len(order), width):
+        idx = order[b0:b0 + width]
+        want[idx] = np.linalg.solve(N[np.ix_(idx, idx)], v[idx])
+    err = _rel(z, want)
+    b = N @ np.ones(len(order))
+    _, achieved, its = op.solve(b, tolerance=1e-9)
+    print(f"block width {width}: M^-1 v rel err {err:.2e}, GMRES {its} iterations, residual {achieved:.1e}")
+    assert err <= 1e-10
+    assert achieved <= 1e-9
+    op.close()
+
+
+def test_fmm_block_width_validation(horn_near):
+    mesh, k, solver, _, _ = horn_near
+    for width in (16, 48, 4096):
+        with pytest.raises(ValueError):
+            solver.fmm_operator(dataclasses.replace(bem.BemNumerics(), fmm_block_width=width), k, k, leaf_capacity=32)
+
+
 def _horn_request(frequencies, numerics, observers=()):
     mesh = meshes.exponential_horn(0.012)
     driven = sorted(set(int(f) for f, b in zip(mesh.face_ids, mesh.boundary) if b == bem.DRIVEN))
@@ -187,6 +238,11 @@ def test_fmm_request_validation():
                                                     fmm_leaf_capacity=2))
     with pytest.raises(ValueError, match="Fast Multipole"):
         solver.solve(req)
+    for width in (16, 96, 4096):  # preconditionerBlockWidth: 32, 64, ..., 2048 (cuda_solver.cu:1063-1066)
+        _, req = _horn_request([500.0], bem.BemNumerics(acceleration=bem.ACCELERATION_FAST_MULTIPOLE,
+                                                        fmm_block_width=width))
+        with pytest.raises(ValueError, match="Fast Multipole"):
+            solver.solve(req)
 
 
 def test_fmm_graph_replays_match_eager_applies():

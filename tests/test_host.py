@@ -37,8 +37,9 @@ def test_numerics_defaults_match_reference():
                                                          bem.PRECONDITIONER_BLOCK_JACOBI)
     assert (py.solver, py.precision, py.preconditioner) == (n.solver, n.precision, n.preconditioner)
     # acceleration and FastMultipoleSettings defaults (bem.hpp:60-71, 121)
-    fmm = ("acceleration", "fmm_leaf_capacity", "fmm_separation_ratio", "fmm_maximum_iterations", "fmm_restart_length")
-    assert tuple(getattr(n, f) for f in fmm) == (bem.ACCELERATION_DENSE, 16, 1.5, 300, 40)
+    fmm = ("acceleration", "fmm_leaf_capacity", "fmm_separation_ratio", "fmm_maximum_iterations", "fmm_restart_length",
+           "fmm_block_width")
+    assert tuple(getattr(n, f) for f in fmm) == (bem.ACCELERATION_DENSE, 16, 1.5, 300, 40, 32)
     assert tuple(getattr(py, f) for f in fmm) == tuple(getattr(n, f) for f in fmm)
 
 
