@@ -80,11 +80,16 @@ def scalene_tetra() -> SurfaceMesh:
 
 
 class _Builder:
-    def __init__(self):
+    def __init__(self, multiple: int = 1):
         self.vertices = []
         self.triangles = []
         self.faces = []
         self.bc = []
+        self.multiple = multiple  # ring point counts are multiples of this; strips repeat per sector
+
+    def count(self, r, h):
+        n = max(8, int(round(2 * math.pi * r / h)))
+        return self.multiple * -(-n // self.multiple)
 
     def ring(self, x, r, n, phase=0.0):
         start = len(self.vertices)
@@ -117,8 +122,13 @@ class _Builder:
 
     def strip(self, r0, r1, face, bc, flip=False):
         """Triangulate between two closed rings (counts may differ), always
-        advancing along the shorter new diagonal (keeps the triangles near isotropic)."""
+        advancing along the shorter new diagonal (keeps the triangles near isotropic).
+        When both counts are multiples of ``multiple`` = q, the moves of the first
+        1/q sector are replayed in every sector, so the strip is invariant under a
+        rotation by 2 pi / q about the axis (no rounding-decided ties)."""
         n0, n1 = len(r0), len(r1)
+        q = self.multiple if self.multiple > 1 and n0 % self.multiple == 0 and n1 % self.multiple == 0 else 1
+        n0s, n1s = n0 // q, n1 // q
         V = self.vertices
         # start r1 at the vertex angularly closest to r0[0]
         p0 = V[r0[0]]
@@ -130,25 +140,27 @@ class _Builder:
             pa, pb = V[a], V[b]
             return (pa[0] - pb[0]) ** 2 + (pa[1] - pb[1]) ** 2 + (pa[2] - pb[2]) ** 2
 
-        while i < n0 or j < n1:
-            top = i < n0 and (j >= n1 or d2(r0[(i + 1) % n0], r1[j % n1]) <= d2(r0[i % n0], r1[(j + 1) % n1]))
-            if top:
-                t = (r0[i % n0], r0[(i + 1) % n0], r1[j % n1])
-                i += 1
-            else:
-                t = (r0[i % n0], r1[(j + 1) % n1], r1[j % n1])
-                j += 1
-            if flip:
-                t = (t[0], t[2], t[1])
-            self.tri(*t, face, bc)
+        moves = []
+        while i < n0s or j < n1s:
+            top = i < n0s and (j >= n1s or d2(r0[(i + 1) % n0], r1[j % n1]) <= d2(r0[i % n0], r1[(j + 1) % n1]))
+            moves.append(top)
+            i, j = (i + 1, j) if top else (i, j + 1)
+        for sector in range(q):
+            i, j = sector * n0s, sector * n1s
+            for top in moves:
+                if top:
+                    t = (r0[i % n0], r0[(i + 1) % n0], r1[j % n1])
+                    i += 1
+                else:
+                    t = (r0[i % n0], r1[(j + 1) % n1], r1[j % n1])
+                    j += 1
+                if flip:
+                    t = (t[0], t[2], t[1])
+                self.tri(*t, face, bc)
 
     def mesh(self) -> SurfaceMesh:
         return SurfaceMesh(np.array(self.vertices, float), np.array(self.triangles, np.uint32),
                            np.array(self.faces, np.uint32), np.array(self.bc, np.uint8))
-
-
-def _ring_count(r, h):
-    return max(8, int(round(2 * math.pi * r / h)))
 
 
 def _disk(b: _Builder, x, ring, r, h, face, bc, facing_negative):
@@ -157,7 +169,7 @@ def _disk(b: _Builder, x, ring, r, h, face, bc, facing_negative):
     outer = ring
     for k in range(layers - 1, 0, -1):
         rr = r * k / layers
-        inner = b.ring(x, rr, _ring_count(rr, h), phase=0.5 * (k % 2))
+        inner = b.ring(x, rr, b.count(rr, h), phase=0.5 * (k % 2))
         b.strip(outer, inner, face, bc, flip=facing_negative)
         outer = inner
     c = b.point((x, 0.0, 0.0))
@@ -171,7 +183,7 @@ def _disk(b: _Builder, x, ring, r, h, face, bc, facing_negative):
 
 def exponential_horn(h: float, throat_radius: float = 0.0127, mouth_radius: float = 0.150, length: float = 0.300,
                      wall: float = 0.005, throat_face: int = 1, wall_face: int = 2, flange: float = 0.0,
-                     rear_length: float = None) -> SurfaceMesh:
+                     rear_length: float = None, ring_multiple: int = 1) -> SurfaceMesh:
     """Closed exponential horn along +x: r(x) = r_t exp(m x), m = ln(r_m/r_t)/L.
 
     Surfaces (outward normals into the air): inner flare (face ``wall_face``),
@@ -179,10 +191,14 @@ def exponential_horn(h: float, throat_radius: float = 0.0127, mouth_radius: floa
     the mouth, outer flare offset by ``wall`` along the normal, and a rear closure
     disk at x=-rear_length. ``flange`` > 0 adds a square rigid baffle of that
     side length in the mouth plane (closed slab of thickness ``wall``).
+    ``ring_multiple`` = q rounds every ring's point count up to a multiple of q
+    and repeats the strips per sector: the unflanged mesh is then invariant under
+    rotations by 2 pi / q about the axis (q = 4: the rocking modes cos / sin of
+    the azimuth map onto each other).
     """
     m = math.log(mouth_radius / throat_radius) / length
     rear = wall if rear_length is None else rear_length
-    b = _Builder()
+    b = _Builder(ring_multiple)
 
     def rin(x):
         return throat_radius * math.exp(m * x)
@@ -206,7 +222,7 @@ def exponential_horn(h: float, throat_radius: float = 0.0127, mouth_radius: floa
             xs.pop(-2)
     inner = []
     for k, x in enumerate(xs):
-        inner.append(b.ring(x, rin(x), _ring_count(rin(x), h), phase=0.5 * (k % 2)))
+        inner.append(b.ring(x, rin(x), b.count(rin(x), h), phase=0.5 * (k % 2)))
     for k in range(len(inner) - 1):
         b.strip(inner[k], inner[k + 1], wall_face, WALL, flip=True)
     # driven throat disk facing +x (into the horn air)
@@ -217,7 +233,7 @@ def exponential_horn(h: float, throat_radius: float = 0.0127, mouth_radius: floa
         if 0.5 * flange <= rout(length) + 2 * h:
             raise ValueError("The flange must extend beyond the horn's outer wall.")
         xb = length - wall
-        back = b.ring(xb, rout(xb), _ring_count(rout(xb), h), phase=0.25)
+        back = b.ring(xb, rout(xb), b.count(rout(xb), h), phase=0.25)
         _flange(b, inner[-1], rin(length), back, rout(xb), length, wall, 0.5 * flange, h, wall_face)
         outer = [back]
         grid = np.linspace(0.0, xb, 20001)
@@ -225,26 +241,26 @@ def exponential_horn(h: float, throat_radius: float = 0.0127, mouth_radius: floa
         count = max(1, int(math.ceil(arc[-1] / h)))
         outer_x = [float(v) for v in np.interp(np.linspace(0.0, arc[-1], count + 1), arc, grid)][count - 1::-1]
     else:
-        mouth_out = b.ring(length, rout(length), _ring_count(rout(length), h), phase=0.25)
+        mouth_out = b.ring(length, rout(length), b.count(rout(length), h), phase=0.25)
         # rim annulus between inner mouth ring and outer mouth ring (facing +x)
         ann = [inner[-1]]
         layers = max(1, int(round((rout(length) - rin(length)) / h)))
         for k in range(1, layers):
             rr = rin(length) + (rout(length) - rin(length)) * k / layers
-            ann.append(b.ring(length, rr, _ring_count(rr, h), phase=0.5 * (k % 2)))
+            ann.append(b.ring(length, rr, b.count(rr, h), phase=0.5 * (k % 2)))
         ann.append(mouth_out)
         for k in range(len(ann) - 1):
             b.strip(ann[k], ann[k + 1], wall_face, WALL, flip=True)
         outer = [mouth_out]
         outer_x = list(reversed(xs[:-1]))
     for k, x in enumerate(outer_x):
-        outer.append(b.ring(x, rout(x), _ring_count(rout(x), h), phase=0.25 + 0.5 * (k % 2)))
+        outer.append(b.ring(x, rout(x), b.count(rout(x), h), phase=0.25 + 0.5 * (k % 2)))
     # rear extension of the outer wall to the closure
     xr = -rear
     steps = max(1, int(round(rear / h)))
     for k in range(1, steps + 1):
         x = -rear * k / steps
-        outer.append(b.ring(x, rout(0.0), _ring_count(rout(0.0), h), phase=0.5 * (k % 2)))
+        outer.append(b.ring(x, rout(0.0), b.count(rout(0.0), h), phase=0.5 * (k % 2)))
     for k in range(len(outer) - 1):
         b.strip(outer[k], outer[k + 1], wall_face, WALL, flip=True)
     _disk(b, xr, outer[-1], rout(0.0), h, wall_face, WALL, facing_negative=True)
