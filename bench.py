@@ -40,7 +40,7 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-from hornsculpt_amd import bem, meshes  # noqa: E402
+from hornsculpt_amd import bem, meshes, variants  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 (vector and matrix)
 FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 vector (published; half of it without packed FP32 instructions)
@@ -199,6 +199,88 @@ def run_served(solver, frequencies, precision, tolerance, steps, slots, label, o
     return out
 
 
+def _area_weighted_error(mesh, a, b):
+    V, T = mesh.vertices, mesh.triangles
+    area = 0.5 * np.linalg.norm(np.cross(V[T[:, 1]] - V[T[:, 0]], V[T[:, 2]] - V[T[:, 0]]), axis=1)
+    return float(math.sqrt(np.sum(area * np.abs(a - b) ** 2) / np.sum(area * np.abs(b) ** 2)))
+
+
+def run_sweep(variant_count, frequency_count, slots, tolerance):
+    """Batched sweep: the first `variant_count` seeded horn variants (of the 32
+    drawn from mt19937_64(20261202)), `frequency_count` frequencies spread over the
+    64-point 100 Hz - 20 kHz raster (the raster is every request's planning raster),
+    four throat right-hand sides per frequency. Every (variant, frequency) request
+    is submitted to the serving loop at once, first in the plain FP64 form, then in
+    the mixed form; a warm pass per variant builds the interaction plans (cold cost,
+    reported apart). Error: area-weighted surface-pressure error of the mixed form
+    against the FP64 form, worst over variants, frequencies and right-hand sides."""
+    raster = variants.sweep_frequencies()
+    picks = sorted({round(i * (len(raster) - 1) / max(frequency_count - 1, 1)) for i in range(frequency_count)})
+    chosen = [raster[i] for i in picks]
+    drawn = variants.horn_variants()
+    sizes_all = [len(v.mesh().triangles) for v in drawn]
+    work = []
+    for v in drawn[:variant_count]:
+        mesh, excitations = variants.throat_excitations(v.mesh())
+        work.append((v, mesh, bem.HipBemSolver(mesh), excitations))
+        print(f"# variant {v.index}: r_m {v.mouth_radius * 1e3:.1f} mm, L {v.length * 1e3:.1f} mm, edge "
+              f"{v.edge * 1e3:.2f} mm, {len(mesh.triangles)} triangles", file=sys.stderr, flush=True)
+
+    def request(excitations, frequencies, precision):
+        return bem.BemRequest(medium=bem.AcousticMedium(1.205, 343.0), frequencies=list(frequencies),
+                              excitations=excitations, interaction_plan_frequencies=raster,
+                              observers=[(1.0, 0.0, 0.0), (0.866, 0.5, 0.0), (0.866, 0.0, 0.5)],
+                              far_field=bem.FarFieldObservation((0.0, 0.0, 0.0), 1.0),
+                              numerics=bem.BemNumerics(solver=bem.SOLVER_GMRES, relative_residual_tolerance=tolerance,
+                                                       gmres_restart=50, gmres_maximum_iterations=2000,
+                                                       precision=precision))
+
+    t0 = time.perf_counter()
+    for v, _, solver, excitations in work:  # cold: interaction plans at the raster's planning wavenumber
+        r = solver.solve(request(excitations, [chosen[0]], bem.PRECISION_MIXED))
+        print(f"# variant {v.index}: interaction plan {r.frequencies[0].interaction_plan_milliseconds / 1e3:.1f} s "
+              f"(cold, {time.perf_counter() - t0:.0f} s so far)", file=sys.stderr, flush=True)
+    cold_s = time.perf_counter() - t0
+    out = {"variants": [{"index": v.index, "mouth_radius": v.mouth_radius, "length": v.length, "edge": v.edge,
+                         "triangles": len(m.triangles), "nodes": len(m.vertices)} for v, m, _, _ in work],
+           "frequencies": chosen, "right_hand_sides": 4, "cold_s": cold_s,
+           "triangles_all_32_variants": [min(sizes_all), max(sizes_all)]}
+    results = {}
+    for label, precision in (("fp64", bem.PRECISION_FP64), ("mixed", bem.PRECISION_MIXED)):
+        server = bem.HipBemServer(slots=slots)
+        device_synchronize()
+        start = time.perf_counter()
+        tickets = [server.submit(solver, request(excitations, [f], precision))
+                   for _, _, solver, excitations in work for f in chosen]
+        done = []
+        for i, t in enumerate(tickets):
+            done.append(server.poll(t))
+            if (i + 1) % len(chosen) == 0:
+                print(f"# {label}: variant {(i + 1) // len(chosen)} of {len(work)} done", file=sys.stderr, flush=True)
+        device_synchronize()
+        elapsed = time.perf_counter() - start
+        stats = server.stats()
+        server.close()
+        results[label] = done
+        points = len(tickets)
+        out[label] = {"points_per_s": points / elapsed, "elapsed_s": elapsed, "points": points,
+                      "p50_ms": float(np.percentile([r.frequencies[0].served_milliseconds for r in done], 50)),
+                      "p99_ms": float(np.percentile([r.frequencies[0].served_milliseconds for r in done], 99)),
+                      "iterations": [min(e.iterations for r in done for e in r.frequencies[0].excitations),
+                                     max(e.iterations for r in done for e in r.frequencies[0].excitations)],
+                      "max_running": stats["max_running"]}
+        print(f"# sweep {label}: {points / elapsed:.2f} points/s ({points} points x 4 right-hand sides)",
+              file=sys.stderr, flush=True)
+    worst = 0.0
+    meshes_per_ticket = [m for _, m, _, _ in work for _ in chosen]
+    for mesh, rf, rm in zip(meshes_per_ticket, results["fp64"], results["mixed"]):
+        for ef, em in zip(rf.frequencies[0].excitations, rm.frequencies[0].excitations):
+            worst = max(worst, _area_weighted_error(mesh, em.boundary.surface_pressure, ef.boundary.surface_pressure))
+    out["mixed_vs_fp64_surface_pressure"] = worst
+    out["mixed_over_fp64"] = out["mixed"]["points_per_s"] / out["fp64"]["points_per_s"]
+    return out
+
+
 def run_fmm(solver, frequencies, leaf_capacity, repetitions, check=True):
     """FMM pressure operator on the same mesh: build time, HIP-event apply time and
     the agreement of its action with the dense FP64 device assembly (a random vector)."""
@@ -298,6 +380,10 @@ def main():
                     help="only the flanged-horn fast multipole solve (118k triangles, 500 Hz and 16 kHz; one JSON line)")
     ap.add_argument("--flange-edge", type=float, default=0.0045)
     ap.add_argument("--flange-repeats", type=int, default=1)
+    ap.add_argument("--sweep", action="store_true",
+                    help="only the batched horn sweep (seeded variants x frequencies x 4 right-hand sides; one JSON line)")
+    ap.add_argument("--sweep-variants", type=int, default=4)
+    ap.add_argument("--sweep-frequencies", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: every GPU box here has one MI355X and the solver has no multi-GPU path "
@@ -313,6 +399,19 @@ def main():
                                                  f"multipole form (leaf {args.fmm_leaf}, separation 1.5, GMRES restart "
                                                  "50, rel tol 1e-6, accuracy seed 1e-7)"},
                           "flange": fl}), flush=True)
+        return
+    if args.sweep:
+        sw = run_sweep(args.sweep_variants, args.sweep_frequencies, max(args.slots, 1), args.tolerance)
+        sizes = [v["triangles"] for v in sw["variants"]]
+        print(json.dumps({"metric": "batched-sweep freq-points solved/s (4 right-hand sides per point), served mixed "
+                                    "vs plain FP64", "unit": "freq-points/s", "value": sw["mixed"]["points_per_s"],
+                          "higher_is_better": True, "n_gpus": 1, "dtype": "f32+f64",
+                          "data": "synthetic (seeded horn variants generated in-process)",
+                          "config": {"workload": f"first {len(sizes)} of 32 seeded horn variants ({min(sizes)}-"
+                                                 f"{max(sizes)} triangles), {len(sw['frequencies'])} of 64 log-spaced "
+                                                 f"frequencies 100 Hz-20 kHz, 4 throat right-hand sides, dense "
+                                                 f"assembly + GMRES (restart 50, rel tol {args.tolerance:g})"},
+                          "sweep": sw}), flush=True)
         return
     frequencies = [200.0 * 40.0 ** (i / (args.frequencies - 1)) for i in range(args.frequencies)]
     mesh = meshes.exponential_horn(args.edge)

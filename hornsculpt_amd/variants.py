@@ -5,8 +5,10 @@ libstdc++'s ``uniform_real_distribution<double>``: one 64-bit draw x per value,
 u = double(x) / 2^64 clamped below 1, a + (b - a) u): per variant the mouth
 radius r_m ~ U[80, 200] mm, the length L ~ U[150, 450] mm and a target triangle
 count N ~ log-uniform [4k, 60k]; the edge length h reaching N (from the area
-scaling of a coarse mesh) is capped at 5.4 mm (k h <= 2 at 20 kHz). Meshes have
-4-fold rotational symmetry about the axis.
+scaling of a coarse mesh) is lowered until the mesh's longest edge is at most
+5.4 mm, so k h_max <= 2 at 20 kHz and the interaction planner stays out of its
+all-pairs mode (interaction_plan.cpp:185-190). Meshes have 4-fold rotational
+symmetry about the axis.
 
 Each frequency carries four right-hand sides on the driven throat disk, whose
 triangles get one face id each: the uniform piston (v = 1), the two rocking
@@ -95,9 +97,19 @@ def horn_variants(count: int = 32, seed: int = 20261202, triangles=(4000.0, 6000
         target = math.exp(uniform(g, math.log(triangles[0]), math.log(triangles[1])))
         coarse = 0.02
         n0 = len(meshes.exponential_horn(coarse, mouth_radius=r_m, length=length, ring_multiple=4).triangles)
-        edge = min(max_edge, coarse * math.sqrt(n0 / target))
+        edge = coarse * math.sqrt(n0 / target)
+        for _ in range(8):
+            longest = maximum_edge(meshes.exponential_horn(edge, mouth_radius=r_m, length=length, ring_multiple=4))
+            if longest <= max_edge:
+                break
+            edge *= 0.995 * max_edge / longest
         out.append(HornVariant(i, r_m, length, target, edge))
     return out
+
+
+def maximum_edge(mesh: bem.SurfaceMesh) -> float:
+    V, T = mesh.vertices, mesh.triangles
+    return float(max(np.linalg.norm(V[T[:, a]] - V[T[:, b]], axis=1).max() for a, b in ((0, 1), (1, 2), (2, 0))))
 
 
 def sweep_frequencies(count: int = 64, low: float = 100.0, high: float = 20000.0) -> List[float]:
