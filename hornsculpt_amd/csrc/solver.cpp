@@ -147,7 +147,9 @@ std::uint64_t Workspace::capacity_bytes() const {
                            &fmm_r, &fmm_q})
         total += b->capacity_bytes();
     return total + flux_column.capacity_bytes() + pivots.capacity_bytes() + jacobi.capacity_bytes() + gmres.z.capacity_bytes() + gmres.basis.capacity_bytes() +
-           gmres.r.capacity_bytes() + gmres.scratch.capacity_bytes() + gmres.small.capacity_bytes();
+           gmres.r.capacity_bytes() + gmres.scratch.capacity_bytes() + gmres.small.capacity_bytes() +
+           gmres.multi_basis.capacity_bytes() + gmres.multi_in.capacity_bytes() + gmres.multi_out.capacity_bytes() +
+           gmres.multi_small.capacity_bytes();
 }
 
 void MeshSolver::build_clusters() {
@@ -486,6 +488,14 @@ void MeshSolver::assemble_dense(const Numerics &n, double k_max, double k, doubl
 GmresWorkspace::~GmresWorkspace() { release(); }
 
 void GmresWorkspace::release() {
+    multi_basis.release();
+    multi_in.release();
+    multi_out.release();
+    multi_small.release();
+    if (multi_pinned)
+        (void)hipHostFree(multi_pinned);
+    multi_pinned = nullptr;
+    multi_pinned_doubles = 0;
     basis.release();
     r.release();
     z.release();
@@ -522,6 +532,193 @@ void dense_gmres(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const
     if (jacobi)
         op.precondition = [jacobi, n](const double *v, double *z, hipStream_t st) { dev::jacobi_apply(jacobi, n, v, z, st); };
     gmres(s, op, n, b, x, restart, max_it, tol, achieved, iterations, cancel, ws);
+}
+
+void dense_gmres_multi(hipStream_t s, const void *A, bool fp32, std::uint32_t n, std::uint32_t nrhs, const double *b,
+                       double *x, std::uint32_t restart, std::uint32_t max_it, double tol, double *achieved,
+                       std::uint32_t *iterations, const volatile std::int32_t *cancel, GmresWorkspace &ws,
+                       const double *jacobi) {
+    using C = std::complex<double>;
+    if (nrhs == 0 || nrhs > kGmresBatch)
+        throw std::invalid_argument("dense_gmres_multi takes 1 to 8 right-hand sides");
+    const std::size_t vec = 2 * static_cast<std::size_t>(n), stride = restart + 1;
+    const std::size_t column = 4 * stride + 1, slot = column + 8; // per system: h1, h2, |w|^2 (+ a norm cell)
+    if (ws.multi_basis.size() < nrhs * stride * vec)
+        ws.multi_basis.resize(nrhs * stride * vec);
+    if (ws.multi_in.size() < nrhs * vec)
+        ws.multi_in.resize(nrhs * vec);
+    if (ws.multi_out.size() < nrhs * vec)
+        ws.multi_out.resize(nrhs * vec);
+    if (ws.multi_small.size() < nrhs * slot)
+        ws.multi_small.resize(nrhs * slot);
+    const std::size_t scratch = std::max(dev::gemv_scratch_doubles(n, nrhs), dev::reduction_scratch_doubles(n, stride));
+    if (ws.scratch.size() < scratch)
+        ws.scratch.resize(scratch);
+    if (ws.multi_pinned_doubles < nrhs * slot) {
+        if (ws.multi_pinned)
+            (void)hipHostFree(ws.multi_pinned);
+        ws.multi_pinned = nullptr;
+        hip_check(hipHostMalloc(reinterpret_cast<void **>(&ws.multi_pinned), sizeof(double) * nrhs * slot),
+                  "pinned GMRES columns");
+        ws.multi_pinned_doubles = nrhs * slot;
+    }
+    double *sc = ws.scratch.data(), *basis0 = ws.multi_basis.data(), *in = ws.multi_in.data(),
+           *out = ws.multi_out.data(), *small0 = ws.multi_small.data(), *pinned = ws.multi_pinned;
+    auto basis = [&](std::uint32_t e, std::uint32_t j) { return basis0 + (e * stride + j) * vec; };
+    auto h1 = [&](std::uint32_t e) { return small0 + e * slot; };
+    auto h2 = [&](std::uint32_t e) { return small0 + e * slot + 2 * stride; };
+    auto nrm = [&](std::uint32_t e) { return small0 + e * slot + 4 * stride; };
+    auto cell = [&](std::uint32_t e) { return small0 + e * slot + column; };
+    auto xe = [&](std::uint32_t e) { return x + e * vec; };
+    auto re = [&](std::uint32_t e) { return out + e * vec; }; // residuals live in the output pack between cycles
+    // |v_e|^2 of the given vectors to the host (one synchronisation for all)
+    auto norms = [&](auto vector_of, const std::vector<std::uint32_t> &which) {
+        for (auto e : which)
+            dev::launch_norm2(vector_of(e), n, nullptr, cell(e), sc, s);
+        hip_check(hipMemcpyAsync(pinned, small0, sizeof(double) * nrhs * slot, hipMemcpyDeviceToHost, s), "norms");
+        hip_check(hipStreamSynchronize(s), "norm sync");
+        std::vector<double> v(nrhs, 0.0);
+        for (auto e : which)
+            v[e] = std::sqrt(pinned[e * slot + column]);
+        return v;
+    };
+    std::vector<std::uint32_t> all(nrhs);
+    for (std::uint32_t e = 0; e < nrhs; ++e)
+        all[e] = e;
+    hip_check(hipMemsetAsync(x, 0, sizeof(double) * nrhs * vec, s), "gmres clear");
+    const auto bnorm = norms([&](std::uint32_t e) { return b + e * vec; }, all);
+    hip_check(hipMemcpyAsync(out, b, sizeof(double) * nrhs * vec, hipMemcpyDeviceToDevice, s), "gmres copy");
+    std::vector<double> beta = bnorm, target(nrhs);
+    std::vector<std::uint32_t> it(nrhs, 0), open;
+    for (std::uint32_t e = 0; e < nrhs; ++e) {
+        target[e] = tol * bnorm[e];
+        if (bnorm[e] == 0) {
+            achieved[e] = 0;
+            iterations[e] = 0;
+        } else {
+            open.push_back(e);
+        }
+    }
+    std::vector<std::vector<C>> H(nrhs, std::vector<C>(stride * restart)), g(nrhs, std::vector<C>(stride)),
+        sn(nrhs, std::vector<C>(restart));
+    std::vector<std::vector<double>> cs(nrhs, std::vector<double>(restart));
+    std::vector<std::uint32_t> used(nrhs);
+    while (!open.empty()) {
+        // systems whose true residual meets the target are done; the rest start a cycle
+        std::vector<std::uint32_t> cycle;
+        for (auto e : open) {
+            if (beta[e] <= target[e]) {
+                achieved[e] = beta[e] / bnorm[e];
+                iterations[e] = it[e];
+            } else if (it[e] >= max_it) {
+                std::ostringstream msg;
+                msg << "GMRES did not converge after " << it[e] << "/" << max_it
+                    << " iterations: relative residual " << beta[e] / bnorm[e];
+                throw std::runtime_error(msg.str());
+            } else {
+                cycle.push_back(e);
+            }
+        }
+        open = cycle;
+        if (open.empty())
+            break;
+        for (auto e : cycle) {
+            dev::launch_scale(n, 1.0 / beta[e], re(e), basis(e, 0), s);
+            std::fill(H[e].begin(), H[e].end(), C{});
+            std::fill(g[e].begin(), g[e].end(), C{});
+            g[e][0] = beta[e];
+            used[e] = 0;
+        }
+        std::vector<std::uint32_t> active = cycle;
+        for (std::uint32_t j = 0; j < restart && !active.empty(); ++j) {
+            if (cancel && *cancel) {
+                (void)hipStreamSynchronize(s);
+                throw Cancelled();
+            }
+            const auto c = static_cast<std::uint32_t>(active.size());
+            for (std::uint32_t a = 0; a < c; ++a) { // pack M^{-1} v_j of the active systems
+                if (jacobi)
+                    dev::jacobi_apply(jacobi, n, basis(active[a], j), in + a * vec, s);
+                else
+                    hip_check(hipMemcpyAsync(in + a * vec, basis(active[a], j), sizeof(double) * vec,
+                                             hipMemcpyDeviceToDevice, s), "pack");
+            }
+            dev::launch_gemv(A, fp32, n, in, out, c, sc, s);
+            for (std::uint32_t a = 0; a < c; ++a) {
+                const std::uint32_t e = active[a];
+                double *w = basis(e, j + 1);
+                hip_check(hipMemcpyAsync(w, out + a * vec, sizeof(double) * vec, hipMemcpyDeviceToDevice, s), "unpack");
+                dev::launch_project(basis(e, 0), n, j + 1, w, h1(e), sc, s);
+                dev::launch_subtract_combination(basis(e, 0), n, j + 1, h1(e), w, s);
+                dev::launch_project(basis(e, 0), n, j + 1, w, h2(e), sc, s);
+                dev::launch_subtract_combination(basis(e, 0), n, j + 1, h2(e), w, s);
+                dev::launch_norm2(w, n, nullptr, nrm(e), sc, s);
+                dev::launch_normalize(n, nrm(e), w, w, s);
+            }
+            hip_check(hipMemcpyAsync(pinned, small0, sizeof(double) * nrhs * slot, hipMemcpyDeviceToHost, s),
+                      "arnoldi download");
+            hip_check(hipStreamSynchronize(s), "arnoldi sync");
+            std::vector<std::uint32_t> still;
+            for (auto e : active) { // Givens rotations of system e's column j (as gmres)
+                const double *host = pinned + e * slot;
+                C *col = H[e].data() + static_cast<std::size_t>(j) * stride;
+                for (std::uint32_t i = 0; i <= j; ++i)
+                    col[i] = C(host[2 * i], host[2 * i + 1]) + C(host[2 * stride + 2 * i], host[2 * stride + 2 * i + 1]);
+                const double next = std::sqrt(host[4 * stride]);
+                col[j + 1] = next;
+                const bool breakdown = !(next > 0);
+                for (std::uint32_t i = 0; i < j; ++i) {
+                    const C t = cs[e][i] * col[i] + sn[e][i] * col[i + 1];
+                    col[i + 1] = -std::conj(sn[e][i]) * col[i] + cs[e][i] * col[i + 1];
+                    col[i] = t;
+                }
+                const C a = col[j], bb = col[j + 1];
+                const double len = std::hypot(std::abs(a), std::abs(bb));
+                if (!(len > 0) || !std::isfinite(len))
+                    throw std::runtime_error("GMRES Arnoldi factorization broke down.");
+                const C phase = std::abs(a) > 0 ? a / std::abs(a) : C(1, 0);
+                cs[e][j] = std::abs(a) / len;
+                sn[e][j] = phase * std::conj(bb) / len;
+                col[j] = phase * len;
+                col[j + 1] = 0;
+                const C t = cs[e][j] * g[e][j] + sn[e][j] * g[e][j + 1];
+                g[e][j + 1] = -std::conj(sn[e][j]) * g[e][j] + cs[e][j] * g[e][j + 1];
+                g[e][j] = t;
+                ++it[e];
+                used[e] = j + 1;
+                if (!(std::abs(g[e][j + 1]) <= target[e] || breakdown || it[e] >= max_it))
+                    still.push_back(e);
+            }
+            active = still;
+        }
+        // x_e += M^{-1} V_e y_e, then the true residuals of the cycle's systems
+        for (auto e : cycle) {
+            const std::uint32_t m = used[e];
+            std::vector<C> y(m);
+            for (std::uint32_t row = m; row-- > 0;) {
+                C v = g[e][row];
+                for (std::uint32_t col = row + 1; col < m; ++col)
+                    v -= H[e][static_cast<std::size_t>(col) * stride + row] * y[col];
+                y[row] = v / H[e][static_cast<std::size_t>(row) * stride + row];
+            }
+            double *neg = pinned + e * slot; // host staging: system e's own slot (the stream is idle here)
+            for (std::uint32_t j = 0; j < m; ++j) {
+                neg[2 * j] = -y[j].real();
+                neg[2 * j + 1] = -y[j].imag();
+            }
+            hip_check(hipMemcpyAsync(h1(e), neg, sizeof(double) * 2 * m, hipMemcpyHostToDevice, s), "y upload");
+            if (jacobi) {
+                hip_check(hipMemsetAsync(in, 0, sizeof(double) * vec, s), "gmres clear");
+                dev::launch_subtract_combination(basis(e, 0), n, m, h1(e), in, s);
+                dev::jacobi_apply(jacobi, n, in, in + vec * (nrhs > 1 ? 1 : 0), s);
+                dev::launch_add(n, in + vec * (nrhs > 1 ? 1 : 0), xe(e), s);
+            } else {
+                dev::launch_subtract_combination(basis(e, 0), n, m, h1(e), xe(e), s);
+            }
+        }
+        dev::launch_residual(A, fp32, n, b, x, nrhs, out, sc, s);
+        beta = norms(re, cycle);
+    }
 }
 
 void gmres(hipStream_t s, const KrylovOperator &op, std::uint32_t n, const double *b, double *x, std::uint32_t restart,
@@ -791,6 +988,10 @@ std::shared_ptr<PreparedRequest> MeshSolver::prepare(const Request &req) const {
         bytes += 16 * nv2 + 4ull * nv;
     else
         bytes += 16ull * nv * (num.gmres_restart + 2) + 16ull * nv + 8 * scratch + 8ull * (4 * (num.gmres_restart + 2) + 8) +
+                 (ne > 1 ? 16ull * nv * std::min<std::uint32_t>(ne, kGmresBatch) * (num.gmres_restart + 3) +
+                               8ull * std::min<std::uint32_t>(ne, kGmresBatch) * (4 * (num.gmres_restart + 2) + 16) +
+                               8 * dev::gemv_scratch_doubles(nv, std::min<std::uint32_t>(ne, kGmresBatch))
+                         : 0) +
                  (num.preconditioner ? 8 * dev::jacobi_factor_doubles(nv) : 0);
     if (nobs)
         bytes += 24 * nobs + 16 * nobs * ne + 8 * dev::field_partial_doubles(nt, static_cast<std::uint32_t>(nobs), ne);
@@ -1006,12 +1207,20 @@ FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_
             hip_check(hipStreamSynchronize(s), "block-Jacobi sync");
             fr.ms[2] = ms_since(t_f);
         }
-        for (std::uint32_t e = 0; e < ne; ++e) {
+        if (ne == 1) {
             double achieved = 0;
-            dense_gmres(s, ws.A.data(), mixed, nv, ws.rhs.data() + 2 * static_cast<std::size_t>(e) * nv,
-                        ws.x.data() + 2 * static_cast<std::size_t>(e) * nv, num.gmres_restart, num.gmres_max_iterations,
-                        num.residual_tolerance * 0.1, &achieved, &iterations[e], cancel, ws.gmres,
-                        jacobi ? ws.jacobi.data() : nullptr);
+            dense_gmres(s, ws.A.data(), mixed, nv, ws.rhs.data(), ws.x.data(), num.gmres_restart,
+                        num.gmres_max_iterations, num.residual_tolerance * 0.1, &achieved, &iterations[0], cancel,
+                        ws.gmres, jacobi ? ws.jacobi.data() : nullptr);
+        } else { // right-hand sides in lockstep batches sharing each product with A
+            std::vector<double> achieved(kGmresBatch);
+            for (std::uint32_t e0 = 0; e0 < ne; e0 += kGmresBatch) {
+                const std::uint32_t m = std::min(kGmresBatch, ne - e0);
+                dense_gmres_multi(s, ws.A.data(), mixed, nv, m, ws.rhs.data() + 2 * static_cast<std::size_t>(e0) * nv,
+                                  ws.x.data() + 2 * static_cast<std::size_t>(e0) * nv, num.gmres_restart,
+                                  num.gmres_max_iterations, num.residual_tolerance * 0.1, achieved.data(),
+                                  &iterations[e0], cancel, ws.gmres, jacobi ? ws.jacobi.data() : nullptr);
+            }
         }
     }
     // FP64 residual of the unscaled equation; LU corrections while above tolerance

@@ -162,7 +162,16 @@ struct GmresWorkspace {
     double *pinned{};
     std::size_t pinned_doubles{};
     std::vector<hipEvent_t> events;
+    // several right-hand sides in lockstep: one basis per system, packed operator
+    // inputs / outputs, per-system Arnoldi columns
+    DeviceBuffer<double> multi_basis, multi_in, multi_out, multi_small;
+    double *multi_pinned{};
+    std::size_t multi_pinned_doubles{};
 };
+
+// Right-hand sides solved together by dense_gmres_multi (one multi-vector
+// product per Arnoldi step reads the matrix once for all of them).
+inline constexpr std::uint32_t kGmresBatch = 8;
 
 // Device buffers and stream of one solve in flight. The direct solve path owns
 // one per mesh (on the context stream); the serving loop owns one per slot,
@@ -193,6 +202,16 @@ struct Workspace {
 void dense_gmres(hipStream_t stream, const void *A, bool fp32, std::uint32_t n, const double *b, double *x,
                  std::uint32_t restart, std::uint32_t max_it, double tol, double *achieved, std::uint32_t *iterations,
                  const volatile std::int32_t *cancel, GmresWorkspace &ws, const double *jacobi);
+
+// dense_gmres for nrhs systems A x_e = b_e (b, x: nrhs vectors of n complex,
+// consecutive) in lockstep: every Arnoldi step applies A to the active systems'
+// preconditioned vectors in one multi-vector product; each system keeps its own
+// basis, Hessenberg and rotations, leaves the cycle when its residual estimate
+// reaches tol ||b_e||, and is finished when its true residual does.
+void dense_gmres_multi(hipStream_t stream, const void *A, bool fp32, std::uint32_t n, std::uint32_t nrhs,
+                       const double *b, double *x, std::uint32_t restart, std::uint32_t max_it, double tol,
+                       double *achieved, std::uint32_t *iterations, const volatile std::int32_t *cancel,
+                       GmresWorkspace &ws, const double *jacobi);
 
 // A matrix-free operator for gmres(): y = A x and z = M^{-1} v (optional), both
 // enqueued on the given stream; residual (optional) r = b - A x in one pass.

@@ -132,3 +132,32 @@ def test_served_variants_equal_alone():
     print(f"variants of {sizes} triangles: served vs alone {worst:.1e}, max running {stats['max_running']}")
     assert worst <= 2e-5
     assert stats["failed"] == 0 and stats["max_running"] >= 2
+
+
+def test_multi_rhs_gmres_matches_single_and_lu(small_variant):
+    """Ten right-hand sides (random complex throat velocities) through GMRES in
+    lockstep batches (8 + 2) sharing each product with A: every solution equals the
+    LU solution and the same right-hand side solved alone by the single-system
+    GMRES, with the same iteration count (+-1: the residual estimate may cross the
+    target one step apart under a different rounding)."""
+    mesh, excitations = small_variant
+    ids = [fv.face_id for fv in excitations[0].faces]
+    rng = np.random.default_rng(11)
+    many = [bem.BemExcitation(f"random{e}", [bem.FaceVelocity(f, complex(rng.standard_normal(), rng.standard_normal()))
+                                             for f in ids]) for e in range(10)]
+    solver = bem.HipBemSolver(mesh)
+    freq = [3000.0]
+    gm = lambda exc: bem.BemRequest(medium=bem.AcousticMedium(1.205, 343.0), frequencies=freq, excitations=exc,
+                                    numerics=bem.BemNumerics(solver=bem.SOLVER_GMRES, relative_residual_tolerance=1e-10))
+    multi = solver.solve(gm(many)).frequencies[0].excitations
+    lu = solver.solve(bem.BemRequest(medium=bem.AcousticMedium(1.205, 343.0), frequencies=freq, excitations=many,
+                                     numerics=bem.BemNumerics(relative_residual_tolerance=1e-10))).frequencies[0].excitations
+    worst_lu = max(_rel(m.boundary.pressure_trace, l.boundary.pressure_trace) for m, l in zip(multi, lu))
+    for e in (0, 8, 9):
+        single = solver.solve(gm([many[e]])).frequencies[0].excitations[0]
+        err = _rel(multi[e].boundary.pressure_trace, single.boundary.pressure_trace)
+        print(f"rhs {e}: multi vs single {err:.1e}, iterations {multi[e].iterations} / {single.iterations}")
+        assert err <= 1e-8 and abs(multi[e].iterations - single.iterations) <= 1
+    print(f"multi vs LU worst {worst_lu:.1e}, residuals {max(m.boundary_residual for m in multi):.1e}")
+    assert worst_lu <= 1e-8
+    assert max(m.boundary_residual for m in multi) <= 1e-10
