@@ -311,7 +311,7 @@ def run_fmm(solver, frequencies, leaf_capacity, repetitions, check=True):
     return out
 
 
-def run_flange(edge, frequencies, leaf_capacity, tolerance, repeats):
+def run_flange(edge, frequencies, leaf_capacity, tolerance, repeats, block_width=32):
     """Horn on a 0.6 m square flange (5 mm slab in the mouth plane) through the fast
     multipole form of the request solve (FP64, GMRES restart 50, relative residual
     tolerance, accuracy seed tolerance/10, seed/10 audit): the first request builds
@@ -326,7 +326,8 @@ def run_flange(edge, frequencies, leaf_capacity, tolerance, repeats):
         observers=[(0.3 + 10.0, 0.0, 0.0), (0.3 + 10 * math.cos(math.radians(30)), 10 * math.sin(math.radians(30)), 0.0)],
         numerics=bem.BemNumerics(solver=bem.SOLVER_GMRES, relative_residual_tolerance=tolerance,
                                  acceleration=bem.ACCELERATION_FAST_MULTIPOLE, fmm_leaf_capacity=leaf_capacity,
-                                 fmm_separation_ratio=1.5, fmm_restart_length=50, fmm_maximum_iterations=500))
+                                 fmm_separation_ratio=1.5, fmm_restart_length=50, fmm_maximum_iterations=500,
+                                 fmm_block_width=block_width))
     runs = []
     for r in range(1 + repeats):
         t0 = time.perf_counter()
@@ -351,7 +352,7 @@ def run_flange(edge, frequencies, leaf_capacity, tolerance, repeats):
         runs.append({"wall_s": round(wall, 3), "points": points})
     warm = runs[1:] or runs
     wall = min(run["wall_s"] for run in warm)
-    return {"triangles": nt, "nodes": nv, "cold_wall_s": runs[0]["wall_s"], "warm_wall_s": wall,
+    return {"triangles": nt, "nodes": nv, "block_width": block_width, "cold_wall_s": runs[0]["wall_s"], "warm_wall_s": wall,
             "points_per_s_warm": len(frequencies) / wall, "runs": runs}
 
 
@@ -380,6 +381,7 @@ def main():
                     help="only the flanged-horn fast multipole solve (118k triangles, 500 Hz and 16 kHz; one JSON line)")
     ap.add_argument("--flange-edge", type=float, default=0.0045)
     ap.add_argument("--flange-repeats", type=int, default=1)
+    ap.add_argument("--flange-block-width", type=int, default=32, help="FMM block-Jacobi width (32, 64, ..., 2048)")
     ap.add_argument("--sweep", action="store_true",
                     help="only the batched horn sweep (seeded variants x frequencies x 4 right-hand sides; one JSON line)")
     ap.add_argument("--sweep-variants", type=int, default=4)
@@ -390,14 +392,15 @@ def main():
               f"(requested --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
     if args.flange:
-        fl = run_flange(args.flange_edge, [500.0, 16000.0], args.fmm_leaf, 1e-6, args.flange_repeats)
+        fl = run_flange(args.flange_edge, [500.0, 16000.0], args.fmm_leaf, 1e-6, args.flange_repeats,
+                        args.flange_block_width)
         print(json.dumps({"metric": "FMM flanged-horn freq-points solved/s (warm)", "unit": "freq-points/s",
                           "value": fl["points_per_s_warm"], "higher_is_better": True, "n_gpus": 1, "dtype": "f64",
                           "data": "synthetic (parametric flanged horn mesh generated in-process)",
                           "config": {"workload": f"exponential horn on a 0.6 m square flange, {fl['triangles']} "
                                                  f"triangles / {fl['nodes']} P1 nodes, 500 Hz and 16 kHz, fast "
                                                  f"multipole form (leaf {args.fmm_leaf}, separation 1.5, GMRES restart "
-                                                 "50, rel tol 1e-6, accuracy seed 1e-7)"},
+                                                 f"50, rel tol 1e-6, accuracy seed 1e-7, block-Jacobi width {args.flange_block_width})"},
                           "flange": fl}), flush=True)
         return
     if args.sweep:
