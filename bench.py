@@ -281,6 +281,81 @@ def run_sweep(variant_count, frequency_count, slots, tolerance):
     return out
 
 
+def run_stream(duration_s, rate, pool_size, triangles, slots, tolerance, precision, seed=20261203):
+    """Open-loop served stream: Poisson arrivals at `rate` requests/s for
+    `duration_s`, drawn from mt19937_64(seed): per request a horn variant of a pool
+    of `pool_size` resident meshes (seeded variants with target triangle counts
+    log-uniform over `triangles`, state keyed by mesh hash), a frequency count
+    round(log-uniform[1, 32]) and that many frequencies log-uniform in
+    [100 Hz, 20 kHz], sorted; a draw with k h_max > 2 at its highest frequency is
+    redrawn. Each mesh's interaction plan is built once, at the highest frequency
+    its longest edge admits, and serves every request on it. Latency of a request =
+    submission to completion of its last point."""
+    g = variants.Mt19937_64(seed)
+    pool = variants.horn_variants(pool_size, seed=seed, triangles=triangles, max_edge=math.inf)
+    solvers, caps, meshes_of = {}, {}, []
+    piston = [bem.BemExcitation("piston", [bem.FaceVelocity(1, 1.0)])]
+    for v in pool:
+        mesh = v.mesh()
+        key = hash((mesh.vertices.tobytes(), mesh.triangles.tobytes()))
+        if key not in solvers:
+            solvers[key] = (bem.HipBemSolver(mesh), piston)
+            # highest frequency with k h_max <= 2, kept 0.1 % inside so rounding cannot
+            # put the planner into its all-pairs mode (k h_max > 2)
+            caps[key] = 0.999 * 2.0 * 343.0 / (2 * math.pi * variants.maximum_edge(mesh))
+        meshes_of.append((key, len(mesh.triangles)))
+    draws, redraws, t = [], 0, 0.0
+    while True:
+        t += -math.log(1.0 - variants.uniform(g, 0.0, 1.0)) / rate
+        if t > duration_s:
+            break
+        key, nt = meshes_of[min(pool_size - 1, int(variants.uniform(g, 0.0, pool_size)))]
+        while True:
+            count = int(round(math.exp(variants.uniform(g, 0.0, math.log(32.0)))))
+            freqs = sorted(math.exp(variants.uniform(g, math.log(100.0), math.log(20000.0))) for _ in range(count))
+            if freqs[-1] <= caps[key]:
+                break
+            redraws += 1
+        draws.append((t, key, nt, freqs))
+
+    def request(key, freqs):
+        solver, excitations = solvers[key]
+        return bem.BemRequest(medium=bem.AcousticMedium(1.205, 343.0), frequencies=freqs,
+                              excitations=excitations, interaction_plan_frequencies=sorted(freqs + [caps[key]]),
+                              numerics=bem.BemNumerics(solver=bem.SOLVER_GMRES, relative_residual_tolerance=tolerance,
+                                                       gmres_restart=50, gmres_maximum_iterations=2000,
+                                                       precision=precision))
+
+    t0 = time.perf_counter()
+    for key in solvers:  # cold: one plan per resident mesh
+        solvers[key][0].solve(request(key, [100.0]))
+        print(f"# stream pool mesh planned ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+    cold_s = time.perf_counter() - t0
+    server = bem.HipBemServer(slots=slots)
+    start = time.perf_counter()
+    tickets = []
+    for i, (at, key, nt, freqs) in enumerate(draws):
+        delay = start + at - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        tickets.append((server.submit(solvers[key][0], request(key, freqs)), len(freqs), nt))
+        if (i + 1) % 50 == 0:
+            print(f"# stream: {i + 1} of {len(draws)} requests submitted", file=sys.stderr, flush=True)
+    latencies, points = [], 0
+    for ticket, count, _ in tickets:
+        res = server.poll(ticket)
+        latencies.append(max(fr.served_milliseconds for fr in res.frequencies))
+        points += count
+    elapsed = time.perf_counter() - start
+    stats = server.stats()
+    server.close()
+    sizes = sorted({nt for _, nt in meshes_of})
+    return {"requests": len(draws), "points": points, "elapsed_s": elapsed, "points_per_s": points / elapsed,
+            "offered_rate_per_s": rate, "duration_s": duration_s, "p50_ms": float(np.percentile(latencies, 50)),
+            "p99_ms": float(np.percentile(latencies, 99)), "redraw_rate": redraws / max(1, redraws + len(draws)),
+            "pool_triangles": sizes, "cold_s": cold_s, "max_running": stats["max_running"]}
+
+
 def run_fmm(solver, frequencies, leaf_capacity, repetitions, check=True):
     """FMM pressure operator on the same mesh: build time, HIP-event apply time and
     the agreement of its action with the dense FP64 device assembly (a random vector)."""
@@ -386,6 +461,11 @@ def main():
                     help="only the batched horn sweep (seeded variants x frequencies x 4 right-hand sides; one JSON line)")
     ap.add_argument("--sweep-variants", type=int, default=4)
     ap.add_argument("--sweep-frequencies", type=int, default=8)
+    ap.add_argument("--stream", action="store_true",
+                    help="only the open-loop served stream of multi-frequency requests on a pool of meshes (one JSON line)")
+    ap.add_argument("--stream-rate", type=float, default=2.0, help="Poisson arrivals per second")
+    ap.add_argument("--stream-seconds", type=float, default=60.0)
+    ap.add_argument("--stream-pool", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: every GPU box here has one MI355X and the solver has no multi-GPU path "
@@ -402,6 +482,23 @@ def main():
                                                  f"multipole form (leaf {args.fmm_leaf}, separation 1.5, GMRES restart "
                                                  f"50, rel tol 1e-6, accuracy seed 1e-7, block-Jacobi width {args.flange_block_width})"},
                           "flange": fl}), flush=True)
+        return
+    if args.stream:
+        out = {}
+        for label, precision in (("mixed", bem.PRECISION_MIXED), ("fp64", bem.PRECISION_FP64)):
+            out[label] = run_stream(args.stream_seconds, args.stream_rate, args.stream_pool, (2000.0, 20000.0),
+                                    max(args.slots, 1), args.tolerance, precision)
+            print(f"# stream {label}: {out[label]['points_per_s']:.2f} points/s, p50 {out[label]['p50_ms']:.0f} ms, "
+                  f"p99 {out[label]['p99_ms']:.0f} ms", file=sys.stderr, flush=True)
+        sizes = out["mixed"]["pool_triangles"]
+        print(json.dumps({"metric": "served stream freq-points solved/s + p50/p99 request latency, mixed vs plain FP64",
+                          "unit": "freq-points/s", "value": out["mixed"]["points_per_s"], "higher_is_better": True,
+                          "n_gpus": 1, "dtype": "f32+f64", "data": "synthetic (seeded stream of horn-variant requests)",
+                          "config": {"workload": f"open-loop Poisson stream, {args.stream_rate:g} requests/s for "
+                                                 f"{args.stream_seconds:g} s, {len(sizes)} resident horn meshes "
+                                                 f"({min(sizes)}-{max(sizes)} triangles), 1-32 frequencies per request "
+                                                 f"in 100 Hz-20 kHz, dense assembly + GMRES (rel tol {args.tolerance:g})"},
+                          "stream": out}), flush=True)
         return
     if args.sweep:
         sw = run_sweep(args.sweep_variants, args.sweep_frequencies, max(args.slots, 1), args.tolerance)
