@@ -244,21 +244,27 @@ __global__ void subtract_vectors(const double *b, const double *ax, std::size_t 
 constexpr int kPanel = 32;
 constexpr int kPanelThreads = 1024;
 
-// Factor columns [k0, k0+nb) over rows [k0, n): pivot search, swap within the
-// panel, scale, rank-1 updates. Pivot rows are recorded as absolute indices.
-__global__ __launch_bounds__(kPanelThreads) void lu_panel(double *A, std::uint32_t n, std::uint32_t k0, std::uint32_t nb,
-                                                         std::int32_t *piv, int *info) {
+// Factor the leaf columns [j0, j0 + w) of the panel [k0, pe) over rows [j0, n):
+// pivot search (largest squared modulus, ties to the lowest row), interchange of
+// rows j and p across every panel column [k0, pe) (the factored L columns to the
+// left and the pending columns to the right alike), scaling, and rank-1 updates of
+// the leaf's remaining columns only. One workgroup; the leaf is a few columns wide,
+// so the sweeps it streams through this CU stay small. Pivot rows are recorded as
+// absolute indices.
+constexpr int kLeaf = 4;
+__global__ __launch_bounds__(kPanelThreads) void lu_leaf(double *A, std::uint32_t n, std::uint32_t k0, std::uint32_t j0,
+                                                        std::uint32_t w, std::uint32_t pe, std::int32_t *piv, int *info) {
     __shared__ double best[kPanelThreads / 64];
     __shared__ std::uint32_t where[kPanelThreads / 64];
     __shared__ std::uint32_t pivot_row;
     __shared__ double pr, pi;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid / 64;
-    for (std::uint32_t j = k0; j < k0 + nb; ++j) {
+    for (std::uint32_t j = j0; j < j0 + w; ++j) {
         double bv = -1;
         std::uint32_t bi = j;
         for (std::uint32_t r = j + tid; r < n; r += kPanelThreads) {
-            // largest modulus; ties resolve to the lowest row in the reduction below
-            const double m = hypot(A[2 * (static_cast<std::size_t>(j) * n + r)], A[2 * (static_cast<std::size_t>(j) * n + r) + 1]);
+            const double ar = A[2 * (static_cast<std::size_t>(j) * n + r)], ai = A[2 * (static_cast<std::size_t>(j) * n + r) + 1];
+            const double m = ar * ar + ai * ai;
             if (m > bv) {
                 bv = m;
                 bi = r;
@@ -280,21 +286,21 @@ __global__ __launch_bounds__(kPanelThreads) void lu_panel(double *A, std::uint32
         __syncthreads();
         if (tid == 0) {
             double v = best[0];
-            std::uint32_t w = where[0];
+            std::uint32_t at = where[0];
             for (int q = 1; q < kPanelThreads / 64; ++q)
-                if (best[q] > v || (best[q] == v && where[q] < w)) {
+                if (best[q] > v || (best[q] == v && where[q] < at)) {
                     v = best[q];
-                    w = where[q];
+                    at = where[q];
                 }
-            pivot_row = w;
-            piv[j] = static_cast<std::int32_t>(w);
+            pivot_row = at;
+            piv[j] = static_cast<std::int32_t>(at);
             if (!(v > 0) && *info == 0)
                 *info = static_cast<int>(j) + 1;
         }
         __syncthreads();
         const std::uint32_t p = pivot_row;
         if (p != j)
-            for (std::uint32_t c = k0 + tid; c < k0 + nb; c += kPanelThreads) {
+            for (std::uint32_t c = k0 + tid; c < pe; c += kPanelThreads) {
                 double *a = A + 2 * (static_cast<std::size_t>(c) * n + j), *b = A + 2 * (static_cast<std::size_t>(c) * n + p);
                 const double t0 = a[0], t1 = a[1];
                 a[0] = b[0];
@@ -316,7 +322,7 @@ __global__ __launch_bounds__(kPanelThreads) void lu_panel(double *A, std::uint32
             const double lr = a[0] * ir - a[1] * ii, li = a[0] * ii + a[1] * ir;
             a[0] = lr;
             a[1] = li;
-            for (std::uint32_t c = j + 1; c < k0 + nb; ++c) {
+            for (std::uint32_t c = j + 1; c < j0 + w; ++c) {
                 const double ur = A[2 * (static_cast<std::size_t>(c) * n + j)], ui = A[2 * (static_cast<std::size_t>(c) * n + j) + 1];
                 double *b = A + 2 * (static_cast<std::size_t>(c) * n + r);
                 b[0] -= lr * ur - li * ui;
@@ -345,17 +351,19 @@ __global__ void lu_swap(double *A, std::uint32_t n, std::uint32_t k0, std::uint3
     }
 }
 
-// U12 = L11^{-1} A12 (unit lower L11 of the panel), one thread per column.
-__global__ __launch_bounds__(256) void lu_trsm(double *A, std::uint32_t n, std::uint32_t k0, std::uint32_t nb) {
+// U12 = L11^{-1} A12 (unit lower nb x nb L11 at [k0, k0 + nb)) for the columns
+// [k0 + nb, c_end), one thread per column (small workgroups: many CUs share the columns).
+__global__ __launch_bounds__(256) void lu_trsm(double *A, std::uint32_t n, std::uint32_t k0, std::uint32_t nb,
+                                               std::uint32_t c_end) {
     __shared__ double L[kPanel][kPanel][2];
-    for (std::uint32_t e = threadIdx.x; e < nb * nb; e += 256) {
+    for (std::uint32_t e = threadIdx.x; e < nb * nb; e += blockDim.x) {
         const std::uint32_t r = e % nb, c = e / nb;
         L[r][c][0] = A[2 * ((static_cast<std::size_t>(k0) + c) * n + k0 + r)];
         L[r][c][1] = A[2 * ((static_cast<std::size_t>(k0) + c) * n + k0 + r) + 1];
     }
     __syncthreads();
-    const std::uint32_t c = k0 + nb + blockIdx.x * 256 + threadIdx.x;
-    if (c >= n)
+    const std::uint32_t c = k0 + nb + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= c_end)
         return;
     double xr[kPanel], xi[kPanel];
     double *col = A + 2 * (static_cast<std::size_t>(c) * n + k0);
@@ -372,60 +380,89 @@ __global__ __launch_bounds__(256) void lu_trsm(double *A, std::uint32_t n, std::
     }
 }
 
-// A22 -= L21 U12 with 64x64 output tiles, K = nb (<= 32), 256 threads, 4x4 per thread.
-__global__ __launch_bounds__(256) void lu_update(double *A, std::uint32_t n, std::uint32_t k0, std::uint32_t nb) {
-    __shared__ double Ls[kPanel][64][2];
-    __shared__ double Us[kPanel][64][2];
-    const std::uint32_t base = k0 + nb;
-    const std::uint32_t r0 = base + blockIdx.x * 64, c0 = base + blockIdx.y * 64;
-    for (std::uint32_t e = threadIdx.x; e < nb * 64; e += 256) {
-        const std::uint32_t kk = e / 64, q = e % 64;
-        const std::uint32_t r = r0 + q, c = c0 + q;
-        Ls[kk][q][0] = r < n ? A[2 * ((static_cast<std::size_t>(k0) + kk) * n + r)] : 0.0;
-        Ls[kk][q][1] = r < n ? A[2 * ((static_cast<std::size_t>(k0) + kk) * n + r) + 1] : 0.0;
-        Us[kk][q][0] = c < n ? A[2 * (static_cast<std::size_t>(c) * n + k0 + kk)] : 0.0;
-        Us[kk][q][1] = c < n ? A[2 * (static_cast<std::size_t>(c) * n + k0 + kk) + 1] : 0.0;
+// A22 -= L21 U12 over rows [k0 + nb, n) and columns [k0 + nb, c_end), K = nb <= 32,
+// on the FP64 matrix cores: 256 threads per 64 x 64 complex output
+// tile, each wavefront a 32 x 32 quarter as 2 x 2 blocks of 16 x 16. The product is
+// formed transposed, D = U^T L^T (v_mfma_f64_16x16x4f64: A[i][k] from lane i + 16 k,
+// B[k][j] from lane j + 16 k, D[i][j] in lane j + 16 (i mod 4), register i / 4), so
+// D's index j runs over rows and each 16-lane group writes 16 consecutive rows of one
+// column. A complex MAC is four real MFMAs: Re += Ur Lr - Ui Li, Im += Ui Lr + Ur Li.
+// K is padded with zeros to a multiple of 4.
+typedef double lu_v4d __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void lu_update_mfma(double *A, std::uint32_t n, std::uint32_t k0, std::uint32_t nb,
+                                                      std::uint32_t c_end) {
+    __shared__ double2 Ls[kPanel][65]; // Ls[k][q] = A[r0 + q][k0 + k] (padded row: no bank conflicts across k)
+    __shared__ double2 Us[kPanel][65]; // Us[k][q] = A[k0 + k][c0 + q]
+    const std::uint32_t base = k0 + nb, r0 = base + blockIdx.x * 64, c0 = base + blockIdx.y * 64;
+    const std::uint32_t kpad = (nb + 3) & ~3u;
+    const double2 *A2 = reinterpret_cast<const double2 *>(A);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, i = lane & 15, kq = lane >> 4;
+    const int rh = wave & 1, ch = wave >> 1;
+    // this lane's old C values, loaded first so their latency hides behind the staging and the MFMAs
+    double2 cold[2][2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const std::uint32_t r = r0 + rh * 32 + a * 16 + i, c = c0 + ch * 32 + b * 16 + 4 * v + kq;
+                cold[a][b][v] = r < n && c < c_end ? A2[static_cast<std::size_t>(c) * n + r] : double2{0.0, 0.0};
+            }
+    for (std::uint32_t e = threadIdx.x; e < kpad * 64; e += 256) {
+        const std::uint32_t kk = e / 64, q = e % 64, r = r0 + q, c = c0 + q;
+        Ls[kk][q] = kk < nb && r < n ? A2[(static_cast<std::size_t>(k0) + kk) * n + r] : double2{0.0, 0.0};
+        Us[kk][q] = kk < nb && c < c_end ? A2[static_cast<std::size_t>(c) * n + k0 + kk] : double2{0.0, 0.0};
     }
     __syncthreads();
-    const int tr = threadIdx.x % 16, tc = threadIdx.x / 16;
-    double cr[4][4], ci[4][4];
+    lu_v4d accr[2][2], acci[2][2];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
-            cr[a][b] = ci[a][b] = 0;
-    for (std::uint32_t kk = 0; kk < nb; ++kk) {
-        double lr[4], li[4], ur[4], ui[4];
+        for (int b = 0; b < 2; ++b)
+            accr[a][b] = acci[a][b] = lu_v4d{0.0, 0.0, 0.0, 0.0};
+    for (std::uint32_t ks = 0; ks < kpad / 4; ++ks) {
+        const std::uint32_t kk = 4 * ks + kq;
+        double lr[2], li[2], ur[2], ui[2];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            lr[a] = Ls[kk][tr + 16 * a][0];
-            li[a] = Ls[kk][tr + 16 * a][1];
-            ur[a] = Us[kk][tc + 16 * a][0];
-            ui[a] = Us[kk][tc + 16 * a][1];
+        for (int a = 0; a < 2; ++a) {
+            const double2 l = Ls[kk][rh * 32 + a * 16 + i];
+            lr[a] = l.x;
+            li[a] = l.y;
         }
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 2; ++b) {
+            const double2 u = Us[kk][ch * 32 + b * 16 + i];
+            ur[b] = u.x;
+            ui[b] = u.y;
+        }
+        // four passes over the eight independent accumulators: no MFMA waits on the one before it
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                cr[a][b] += lr[a] * ur[b] - li[a] * ui[b];
-                ci[a][b] += lr[a] * ui[b] + li[a] * ur[b];
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[b], lr[a], accr[a][b], 0, 0, 0);
+                acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ui[b], lr[a], acci[a][b], 0, 0, 0);
+            }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ui[b], li[a], accr[a][b], 0, 0, 0);
+                acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[b], li[a], acci[a][b], 0, 0, 0);
             }
     }
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const std::uint32_t c = c0 + tc + 16 * b;
-        if (c >= n)
-            continue;
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const std::uint32_t r = r0 + tr + 16 * a;
-            if (r >= n)
-                continue;
-            double *p = A + 2 * (static_cast<std::size_t>(c) * n + r);
-            p[0] -= cr[a][b];
-            p[1] -= ci[a][b];
-        }
-    }
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const std::uint32_t r = r0 + rh * 32 + a * 16 + i, c = c0 + ch * 32 + b * 16 + 4 * v + kq;
+                if (r < n && c < c_end)
+                    reinterpret_cast<double2 *>(A)[static_cast<std::size_t>(c) * n + r] =
+                        double2{cold[a][b][v].x - accr[a][b][v], cold[a][b][v].y - acci[a][b][v]};
+            }
 }
 
 // Row interchanges of every panel applied to the right-hand sides.
@@ -1014,17 +1051,30 @@ void launch_residual(const void *A, bool fp32, std::uint32_t n, const double *b,
 
 std::size_t lu_work_doubles(std::uint32_t) { return 0; }
 
+// Right-looking blocked LU with partial pivoting: panels of kPanel columns, each
+// factored in leaves of kLeaf columns (lu_leaf on one workgroup, then the leaf's
+// rows of U for the rest of the panel and a rank-kLeaf update of the panel below
+// it on every CU), then the panel's interchanges outside it, U12 and the
+// rank-kPanel trailing update on the matrix cores.
 void lu_factor(double *A, std::uint32_t n, std::int32_t *piv, int *info, double *, hipStream_t stream) {
     for (std::uint32_t k0 = 0; k0 < n; k0 += kPanel) {
-        const std::uint32_t nb = std::min<std::uint32_t>(kPanel, n - k0);
-        hipLaunchKernelGGL(lu_panel, dim3(1), dim3(kPanelThreads), 0, stream, A, n, k0, nb, piv, info);
-        hipLaunchKernelGGL(lu_swap, dim3((n + 255) / 256), dim3(256), 0, stream, A, n, k0, nb, piv);
-        const std::uint32_t rest = n - k0 - nb;
+        const std::uint32_t nb = std::min<std::uint32_t>(kPanel, n - k0), pe = k0 + nb;
+        for (std::uint32_t j0 = k0; j0 < pe; j0 += kLeaf) {
+            const std::uint32_t w = std::min<std::uint32_t>(kLeaf, pe - j0);
+            hipLaunchKernelGGL(lu_leaf, dim3(1), dim3(kPanelThreads), 0, stream, A, n, k0, j0, w, pe, piv, info);
+            if (j0 + w >= pe)
+                continue;
+            hipLaunchKernelGGL(lu_trsm, dim3(1), dim3(64), 0, stream, A, n, j0, w, pe);
+            const std::uint32_t below = n - j0 - w;
+            hipLaunchKernelGGL(lu_update_mfma, dim3((below + 63) / 64, 1), dim3(256), 0, stream, A, n, j0, w, pe);
+        }
+        hipLaunchKernelGGL(lu_swap, dim3((n + 63) / 64), dim3(64), 0, stream, A, n, k0, nb, piv);
+        const std::uint32_t rest = n - pe;
         if (!rest)
             continue;
-        hipLaunchKernelGGL(lu_trsm, dim3((rest + 255) / 256), dim3(256), 0, stream, A, n, k0, nb);
+        hipLaunchKernelGGL(lu_trsm, dim3((rest + 63) / 64), dim3(64), 0, stream, A, n, k0, nb, n);
         const std::uint32_t tiles = (rest + 63) / 64;
-        hipLaunchKernelGGL(lu_update, dim3(tiles, tiles), dim3(256), 0, stream, A, n, k0, nb);
+        hipLaunchKernelGGL(lu_update_mfma, dim3(tiles, tiles), dim3(256), 0, stream, A, n, k0, nb, n);
     }
 }
 
