@@ -461,6 +461,8 @@ def main():
                     help="only the batched horn sweep (seeded variants x frequencies x 4 right-hand sides; one JSON line)")
     ap.add_argument("--sweep-variants", type=int, default=4)
     ap.add_argument("--sweep-frequencies", type=int, default=8)
+    ap.add_argument("--lu", type=int, default=0, metavar="N",
+                    help="only the dense complex LU solve of a seeded random N x N system (one JSON line)")
     ap.add_argument("--stream", action="store_true",
                     help="only the open-loop served stream of multi-frequency requests on a pool of meshes (one JSON line)")
     ap.add_argument("--stream-rate", type=float, default=2.0, help="Poisson arrivals per second")
@@ -482,6 +484,25 @@ def main():
                                                  f"multipole form (leaf {args.fmm_leaf}, separation 1.5, GMRES restart "
                                                  f"50, rel tol 1e-6, accuracy seed 1e-7, block-Jacobi width {args.flange_block_width})"},
                           "flange": fl}), flush=True)
+        return
+    if args.lu:
+        n = args.lu
+        rng = np.random.default_rng(1)
+        A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        b = rng.standard_normal((n, 1)) + 1j * rng.standard_normal((n, 1))
+        walls = []
+        for rep in range(2):
+            t0 = time.perf_counter()
+            x = bem.dense_lu_solve(A, b)
+            walls.append(time.perf_counter() - t0)
+            print(f"# dense LU n={n} run {rep}: {walls[-1]:.3f} s including the host transfers", file=sys.stderr,
+                  flush=True)
+        residual = float(np.linalg.norm(A @ x - b) / np.linalg.norm(b))
+        print(json.dumps({"metric": "dense complex LU solve wall time (incl. host transfers)", "unit": "s",
+                          "value": min(walls), "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
+                          "data": "synthetic (seeded random complex matrix)",
+                          "config": {"workload": f"dense complex FP64 LU with partial pivoting, n = {n}, 1 RHS"},
+                          "flops": 8.0 / 3.0 * n ** 3, "relative_residual": residual}), flush=True)
         return
     if args.stream:
         out = {}

@@ -352,31 +352,45 @@ __global__ void lu_swap(double *A, std::uint32_t n, std::uint32_t k0, std::uint3
 }
 
 // U12 = L11^{-1} A12 (unit lower nb x nb L11 at [k0, k0 + nb)) for the columns
-// [k0 + nb, c_end), one thread per column (small workgroups: many CUs share the columns).
-__global__ __launch_bounds__(256) void lu_trsm(double *A, std::uint32_t n, std::uint32_t k0, std::uint32_t nb,
-                                               std::uint32_t c_end) {
+// [k0 + nb, c_end): 64 threads per 64 columns. The block's nb x 64 slice of A12 is
+// staged through LDS with consecutive threads on consecutive rows of a column
+// (coalesced), each thread solves one column from LDS, and the slice is written back
+// the same way. Rows of the staged slice are padded against LDS bank conflicts.
+__global__ __launch_bounds__(64) void lu_trsm(double *A, std::uint32_t n, std::uint32_t k0, std::uint32_t nb,
+                                              std::uint32_t c_end) {
     __shared__ double L[kPanel][kPanel][2];
-    for (std::uint32_t e = threadIdx.x; e < nb * nb; e += blockDim.x) {
+    __shared__ double2 X[64][kPanel + 1];
+    for (std::uint32_t e = threadIdx.x; e < nb * nb; e += 64) {
         const std::uint32_t r = e % nb, c = e / nb;
         L[r][c][0] = A[2 * ((static_cast<std::size_t>(k0) + c) * n + k0 + r)];
         L[r][c][1] = A[2 * ((static_cast<std::size_t>(k0) + c) * n + k0 + r) + 1];
     }
+    const std::uint32_t c_first = k0 + nb + blockIdx.x * 64;
+    const std::uint32_t cols = min(64u, c_end - c_first);
+    double2 *A2 = reinterpret_cast<double2 *>(A);
+    for (std::uint32_t e = threadIdx.x; e < cols * nb; e += 64) {
+        const std::uint32_t q = e / nb, r = e % nb;
+        X[q][r] = A2[static_cast<std::size_t>(c_first + q) * n + k0 + r];
+    }
     __syncthreads();
-    const std::uint32_t c = k0 + nb + blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= c_end)
-        return;
-    double xr[kPanel], xi[kPanel];
-    double *col = A + 2 * (static_cast<std::size_t>(c) * n + k0);
-    for (std::uint32_t i = 0; i < nb; ++i) {
-        double vr = col[2 * i], vi = col[2 * i + 1];
-        for (std::uint32_t j = 0; j < i; ++j) {
-            vr -= L[i][j][0] * xr[j] - L[i][j][1] * xi[j];
-            vi -= L[i][j][0] * xi[j] + L[i][j][1] * xr[j];
+    const std::uint32_t t = threadIdx.x;
+    if (t < cols) {
+        double xr[kPanel], xi[kPanel];
+        for (std::uint32_t i = 0; i < nb; ++i) {
+            double vr = X[t][i].x, vi = X[t][i].y;
+            for (std::uint32_t j = 0; j < i; ++j) {
+                vr -= L[i][j][0] * xr[j] - L[i][j][1] * xi[j];
+                vi -= L[i][j][0] * xi[j] + L[i][j][1] * xr[j];
+            }
+            xr[i] = vr;
+            xi[i] = vi;
+            X[t][i] = double2{vr, vi};
         }
-        xr[i] = vr;
-        xi[i] = vi;
-        col[2 * i] = vr;
-        col[2 * i + 1] = vi;
+    }
+    __syncthreads();
+    for (std::uint32_t e = threadIdx.x; e < cols * nb; e += 64) {
+        const std::uint32_t q = e / nb, r = e % nb;
+        A2[static_cast<std::size_t>(c_first + q) * n + k0 + r] = X[q][r];
     }
 }
 
