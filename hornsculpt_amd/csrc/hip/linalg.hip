@@ -244,27 +244,32 @@ __global__ void subtract_vectors(const double *b, const double *ax, std::size_t 
 constexpr int kPanel = 32;
 constexpr int kPanelThreads = 1024;
 
-// Factor the leaf columns [j0, j0 + w) of the panel [k0, pe) over rows [j0, n):
-// pivot search (largest squared modulus, ties to the lowest row), interchange of
-// rows j and p across every panel column [k0, pe) (the factored L columns to the
-// left and the pending columns to the right alike), scaling, and rank-1 updates of
-// the leaf's remaining columns only. One workgroup; the leaf is a few columns wide,
-// so the sweeps it streams through this CU stay small. Pivot rows are recorded as
-// absolute indices.
+// Factor the leaf columns [j0, j0 + w) of the panel [k0, pe) over rows [j0, n).
+// Per column j, three barriers: (1) pivot search (largest squared modulus, ties to
+// the lowest row) reduced per wavefront into LDS; (2) every thread finishes the
+// reduction itself, a few threads stage the pivot row p's leaf entries in LDS and
+// one records the pivot; (3) one pass over the rows r > j writes
+// l = a[r][j] / a[p][j] and a[r][c] - l u[c] for the leaf columns c > j, reading
+// row j in place of row p (the interchange folded in), while the thread of row p
+// also writes the pivot row into row j, and the other panel columns (the leaf's
+// factored columns before j included) swap rows j and p. One workgroup; pivot rows
+// are recorded as absolute indices.
 constexpr int kLeaf = 4;
 __global__ __launch_bounds__(kPanelThreads) void lu_leaf(double *A, std::uint32_t n, std::uint32_t k0, std::uint32_t j0,
                                                         std::uint32_t w, std::uint32_t pe, std::int32_t *piv, int *info) {
     __shared__ double best[kPanelThreads / 64];
     __shared__ std::uint32_t where[kPanelThreads / 64];
-    __shared__ std::uint32_t pivot_row;
-    __shared__ double pr, pi;
+    __shared__ double2 u[kLeaf];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid / 64;
-    for (std::uint32_t j = j0; j < j0 + w; ++j) {
+    double2 *A2 = reinterpret_cast<double2 *>(A);
+    auto at = [&](std::uint32_t c, std::uint32_t r) -> double2 & { return A2[static_cast<std::size_t>(c) * n + r]; };
+    const std::uint32_t je = j0 + w;
+    for (std::uint32_t j = j0; j < je; ++j) {
         double bv = -1;
         std::uint32_t bi = j;
         for (std::uint32_t r = j + tid; r < n; r += kPanelThreads) {
-            const double ar = A[2 * (static_cast<std::size_t>(j) * n + r)], ai = A[2 * (static_cast<std::size_t>(j) * n + r) + 1];
-            const double m = ar * ar + ai * ai;
+            const double2 v = at(j, r);
+            const double m = v.x * v.x + v.y * v.y;
             if (m > bv) {
                 bv = m;
                 bi = r;
@@ -283,53 +288,46 @@ __global__ __launch_bounds__(kPanelThreads) void lu_leaf(double *A, std::uint32_
             best[wave] = bv;
             where[wave] = bi;
         }
-        __syncthreads();
+        __syncthreads(); // (1)
+        double v = best[0];
+        std::uint32_t p = where[0];
+        for (int q = 1; q < kPanelThreads / 64; ++q)
+            if (best[q] > v || (best[q] == v && where[q] < p)) {
+                v = best[q];
+                p = where[q];
+            }
+        if (tid < static_cast<int>(je - j))
+            u[tid] = at(j + tid, p);
         if (tid == 0) {
-            double v = best[0];
-            std::uint32_t at = where[0];
-            for (int q = 1; q < kPanelThreads / 64; ++q)
-                if (best[q] > v || (best[q] == v && where[q] < at)) {
-                    v = best[q];
-                    at = where[q];
-                }
-            pivot_row = at;
-            piv[j] = static_cast<std::int32_t>(at);
+            piv[j] = static_cast<std::int32_t>(p);
             if (!(v > 0) && *info == 0)
                 *info = static_cast<int>(j) + 1;
         }
-        __syncthreads();
-        const std::uint32_t p = pivot_row;
-        if (p != j)
-            for (std::uint32_t c = k0 + tid; c < pe; c += kPanelThreads) {
-                double *a = A + 2 * (static_cast<std::size_t>(c) * n + j), *b = A + 2 * (static_cast<std::size_t>(c) * n + p);
-                const double t0 = a[0], t1 = a[1];
-                a[0] = b[0];
-                a[1] = b[1];
-                b[0] = t0;
-                b[1] = t1;
-            }
-        __syncthreads();
-        if (tid == 0) {
-            const double ar = A[2 * (static_cast<std::size_t>(j) * n + j)], ai = A[2 * (static_cast<std::size_t>(j) * n + j) + 1];
-            const double d = ar * ar + ai * ai;
-            pr = d > 0 ? ar / d : 0.0; // 1/a = conj(a)/|a|^2
-            pi = d > 0 ? -ai / d : 0.0;
-        }
-        __syncthreads();
-        const double ir = pr, ii = pi;
+        __syncthreads(); // (2): every thread has p and the pivot row's leaf entries
+        const double2 d = u[0];
+        const double dd = d.x * d.x + d.y * d.y;
+        const double ir = dd > 0 ? d.x / dd : 0.0, ii = dd > 0 ? -d.y / dd : 0.0; // 1/a = conj(a)/|a|^2
         for (std::uint32_t r = j + 1 + tid; r < n; r += kPanelThreads) {
-            double *a = A + 2 * (static_cast<std::size_t>(j) * n + r);
-            const double lr = a[0] * ir - a[1] * ii, li = a[0] * ii + a[1] * ir;
-            a[0] = lr;
-            a[1] = li;
-            for (std::uint32_t c = j + 1; c < j0 + w; ++c) {
-                const double ur = A[2 * (static_cast<std::size_t>(c) * n + j)], ui = A[2 * (static_cast<std::size_t>(c) * n + j) + 1];
-                double *b = A + 2 * (static_cast<std::size_t>(c) * n + r);
-                b[0] -= lr * ur - li * ui;
-                b[1] -= lr * ui + li * ur;
+            const std::uint32_t src = r == p ? j : r; // row p receives the old row j
+            const double2 a = at(j, src);
+            const double lr = a.x * ir - a.y * ii, li = a.x * ii + a.y * ir;
+            for (std::uint32_t c = j + 1; c < je; ++c) {
+                const double2 b = at(c, src), uc = u[c - j];
+                at(c, r) = double2{b.x - (lr * uc.x - li * uc.y), b.y - (lr * uc.y + li * uc.x)};
             }
+            at(j, r) = double2{lr, li};
+            if (r == p) // this thread read the old row j; the pivot row takes its place
+                for (std::uint32_t c = j; c < je; ++c)
+                    at(c, j) = u[c - j];
         }
-        __syncthreads();
+        if (p != j) // the other panel columns (incl. this leaf's L columns before j): interchange rows j and p
+            for (std::uint32_t c = k0 + tid; c < pe; c += kPanelThreads)
+                if (c < j || c >= je) {
+                    const double2 t = at(c, j);
+                    at(c, j) = at(c, p);
+                    at(c, p) = t;
+                }
+        __syncthreads(); // (3)
     }
 }
 
