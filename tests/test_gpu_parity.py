@@ -214,6 +214,16 @@ def test_dense_lu_matches_numpy(n, nrhs):
     assert np.max(np.abs(X - Xn)) / np.max(np.abs(Xn)) <= 1e-10
 
 
+def test_dense_lu_reports_singular_pivot():
+    """A zero column stays zero through every update, so its pivot search finds no
+    nonzero entry: the factorization reports the first singular pivot (1-based)."""
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((300, 300)) + 1j * rng.standard_normal((300, 300))
+    A[:, 40] = 0
+    with pytest.raises(Exception, match="pivot 41"):
+        bem.dense_lu_solve(A, np.ones((300, 1), complex))
+
+
 def test_block_jacobi_gmres_matches_unpreconditioned():
     """Right block-Jacobi preconditioning changes the Krylov path, not the solution."""
     mesh = meshes.exponential_horn(0.02)
