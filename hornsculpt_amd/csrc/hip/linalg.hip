@@ -5,15 +5,21 @@
 // in device_gmres.cu:115-157) with own kernels:
 //   gemv            y = A X for up to 8 vectors (A FP64 or FP32, vectors FP64):
 //                   64 rows x 512 columns per workgroup, A streamed once (HBM
-//                   bound), deterministic split-K partials reduced in a second pass.
+//                   bound), deterministic split-K partials reduced in a second pass;
+//                   for one vector the column chunk (512, 1024 or 2048) is chosen by
+//                   timing the candidates once per (n, precision) on the live stream.
 //   project         CGS2 projections V^H w, one wavefront per (basis column, row block).
-//   lu_factor       right-looking blocked LU (panel 32): one-workgroup panel
-//                   factorisation with pivot search, row swaps, unit-lower
-//                   TRSM, and a 64x64-tiled trailing update.
+//   lu_factor       right-looking blocked LU (panel 32, 4-column leaves): pivot
+//                   search, row interchanges, unit-lower TRSM, and trailing
+//                   updates on the FP64 matrix cores.
 //   lu_solve        blocked forward/backward substitution.
 #include "device.hpp"
 
 #include <algorithm>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <utility>
 
 namespace hsbem::dev {
 namespace {
@@ -27,18 +33,18 @@ constexpr int kMaxVec = 8;
 // Split-K partial products: workgroup (row block, column chunk); lane = row, the
 // four waves take interleaved columns of the chunk and are summed in LDS, so one
 // partial per (chunk, row) reaches HBM.
-template <int NV, class T>
+template <int NV, class T, int COLS = kGemvCols>
 __global__ __launch_bounds__(kGemvRows *kGemvWaves) void gemv_partial(const T *__restrict__ A, std::uint32_t n,
                                                                      const double *__restrict__ x,
                                                                      std::uint32_t chunks, double *__restrict__ part) {
-    __shared__ double sx[NV][2 * kGemvCols];
+    __shared__ double sx[NV][2 * COLS];
     __shared__ double red[kGemvWaves - 1][NV][2][kGemvRows];
     const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
     const std::uint32_t row = blockIdx.x * kGemvRows + lane;
-    const std::uint32_t c0 = blockIdx.y * kGemvCols;
-    const std::uint32_t cn = min(static_cast<std::uint32_t>(kGemvCols), n - c0);
-    for (std::uint32_t e = threadIdx.x; e < NV * 2 * kGemvCols; e += kGemvRows * kGemvWaves) {
-        const std::uint32_t v = e / (2 * kGemvCols), c = e % (2 * kGemvCols);
+    const std::uint32_t c0 = blockIdx.y * COLS;
+    const std::uint32_t cn = min(static_cast<std::uint32_t>(COLS), n - c0);
+    for (std::uint32_t e = threadIdx.x; e < NV * 2 * COLS; e += kGemvRows * kGemvWaves) {
+        const std::uint32_t v = e / (2 * COLS), c = e % (2 * COLS);
         sx[v][c] = c / 2 < cn ? x[2 * (static_cast<std::size_t>(v) * n + c0) + c] : 0.0;
     }
     __syncthreads();
@@ -977,8 +983,90 @@ std::size_t gemv_scratch_doubles(std::uint32_t n, std::uint32_t nvec) {
     return 2 * chunks * n * std::max<std::uint32_t>(nvec, 1);
 }
 
+namespace {
+
+template <int COLS>
+void launch_gemv_one(const void *A, bool fp32, std::uint32_t n, const double *x, double *y, double *scratch,
+                     hipStream_t stream) {
+    const std::uint32_t chunks = (n + COLS - 1) / COLS;
+    const dim3 grid((n + kGemvRows - 1) / kGemvRows, chunks);
+    if (fp32)
+        hipLaunchKernelGGL((gemv_partial<1, float2, COLS>), grid, dim3(kGemvRows * kGemvWaves), 0, stream,
+                           static_cast<const float2 *>(A), n, x, chunks, scratch);
+    else
+        hipLaunchKernelGGL((gemv_partial<1, double2, COLS>), grid, dim3(kGemvRows * kGemvWaves), 0, stream,
+                           static_cast<const double2 *>(A), n, x, chunks, scratch);
+    hipLaunchKernelGGL(gemv_reduce, dim3((n + 255) / 256), dim3(256), 0, stream, scratch, n, chunks, 1u, y);
+}
+
+void launch_gemv_cols(std::uint32_t cols, const void *A, bool fp32, std::uint32_t n, const double *x, double *y,
+                      double *scratch, hipStream_t stream) {
+    if (cols == 2048)
+        launch_gemv_one<2048>(A, fp32, n, x, y, scratch, stream);
+    else if (cols == 1024)
+        launch_gemv_one<1024>(A, fp32, n, x, y, scratch, stream);
+    else
+        launch_gemv_one<kGemvCols>(A, fp32, n, x, y, scratch, stream);
+}
+
+// Tile autotuner of the one-vector product: per (n, precision) the column chunk
+// with the least time over three timed launches (after one warm launch each) on
+// the caller's stream and data; the product it leaves in y is the last candidate's,
+// and the caller's own launch follows. Not tuned (the default chunk) while the
+// stream is being captured; HSBEM_GEMV_COLS = 512 / 1024 / 2048 pins the chunk.
+std::uint32_t gemv_columns(const void *A, bool fp32, std::uint32_t n, const double *x, double *y, double *scratch,
+                           hipStream_t stream) {
+    if (const char *pin = std::getenv("HSBEM_GEMV_COLS")) {
+        const unsigned long v = std::strtoul(pin, nullptr, 10);
+        if (v == 512 || v == 1024 || v == 2048)
+            return static_cast<std::uint32_t>(v);
+    }
+    static std::mutex mutex;
+    static std::map<std::pair<std::uint32_t, bool>, std::uint32_t> chosen;
+    std::lock_guard lock(mutex);
+    const auto key = std::make_pair(n, fp32);
+    if (auto it = chosen.find(key); it != chosen.end())
+        return it->second;
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &capture) != hipSuccess || capture != hipStreamCaptureStatusNone)
+        return kGemvCols;
+    hipEvent_t start, stop;
+    if (hipEventCreate(&start) != hipSuccess)
+        return kGemvCols;
+    if (hipEventCreate(&stop) != hipSuccess) {
+        (void)hipEventDestroy(start);
+        return kGemvCols;
+    }
+    std::uint32_t best = kGemvCols;
+    float best_ms = 0;
+    for (const std::uint32_t cols : {512u, 1024u, 2048u}) {
+        launch_gemv_cols(cols, A, fp32, n, x, y, scratch, stream);
+        (void)hipEventRecord(start, stream);
+        for (int r = 0; r < 3; ++r)
+            launch_gemv_cols(cols, A, fp32, n, x, y, scratch, stream);
+        (void)hipEventRecord(stop, stream);
+        float ms = 0;
+        if (hipEventSynchronize(stop) != hipSuccess || hipEventElapsedTime(&ms, start, stop) != hipSuccess)
+            continue;
+        if (cols == 512u || ms < best_ms) {
+            best = cols;
+            best_ms = ms;
+        }
+    }
+    (void)hipEventDestroy(start);
+    (void)hipEventDestroy(stop);
+    chosen.emplace(key, best);
+    return best;
+}
+
+} // namespace
+
 void launch_gemv(const void *A, bool fp32, std::uint32_t n, const double *x, double *y, std::uint32_t nvec,
                  double *scratch, hipStream_t stream) {
+    if (nvec == 1) {
+        launch_gemv_cols(gemv_columns(A, fp32, n, x, y, scratch, stream), A, fp32, n, x, y, scratch, stream);
+        return;
+    }
     const std::uint32_t chunks = (n + kGemvCols - 1) / kGemvCols;
     for (std::uint32_t v0 = 0; v0 < nvec; v0 += kMaxVec) {
         const std::uint32_t nv = std::min<std::uint32_t>(kMaxVec, nvec - v0);

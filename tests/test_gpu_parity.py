@@ -224,6 +224,28 @@ def test_dense_lu_reports_singular_pivot():
         bem.dense_lu_solve(A, np.ones((300, 1), complex))
 
 
+@pytest.mark.parametrize("n", [700, 3000])
+def test_gemv_column_chunks_agree(monkeypatch, n):
+    """The one-vector product's column chunk (512, 1024, 2048: the autotuner's
+    candidates, pinned by HSBEM_GEMV_COLS) changes only the split-K summation order:
+    GMRES reaches the same solution, which solves the system."""
+    rng = np.random.default_rng(n)
+    A = 4 * np.eye(n) + (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    xs = {}
+    for cols in ("512", "1024", "2048"):
+        monkeypatch.setenv("HSBEM_GEMV_COLS", cols)
+        x, achieved, its = bem.dense_gmres(A, b, 50, 500, 1e-12)
+        assert achieved <= 1e-12
+        xs[cols] = x
+    monkeypatch.delenv("HSBEM_GEMV_COLS")
+    x_auto, _, _ = bem.dense_gmres(A, b, 50, 500, 1e-12)  # the tuned chunk
+    ref = np.linalg.solve(A, b)
+    for x in list(xs.values()) + [x_auto]:
+        assert np.linalg.norm(x - ref) / np.linalg.norm(ref) <= 1e-10
+        assert np.linalg.norm(x - xs["512"]) / np.linalg.norm(ref) <= 1e-11
+
+
 def test_block_jacobi_gmres_matches_unpreconditioned():
     """Right block-Jacobi preconditioning changes the Krylov path, not the solution."""
     mesh = meshes.exponential_horn(0.02)
