@@ -653,7 +653,7 @@ __global__ __launch_bounds__(256) void coaxial_links_wide(FmmTranslateLaunch L) 
 // its (t, 4-link group) items in registers. Coaxial products: per |m| block the
 // chunk's columns stay in LDS while 16-column slabs of T_m pass through.
 constexpr int kBigN = 2 * kMaxBigOrder + 1;
-constexpr int kBigHalf = kFmmChunk / 2;
+constexpr int kBigHalf = 8; // links per pass (a chunk of kFmmChunk links takes kFmmChunk / 8 passes)
 constexpr int kBigSlab = 32;
 constexpr int kBigItems = (kBigN * (kBigHalf / 4) + 255) / 256; // (t, group) items per thread
 
@@ -755,7 +755,7 @@ constexpr int kBigCols = 16;
 constexpr int kBigCoaxItems = (kBigRows * 2 * (kFmmChunk / 4) + 255) / 256; // (row, sign, group) per thread
 
 __global__ __launch_bounds__(256) void coaxial_links_big(FmmTranslateLaunch L) {
-    __shared__ C2 ys[kFmmChunk][2][kBigRows];
+    __shared__ C2 ys[kFmmChunk][2][kBigCols]; // the current column tile of every link's +m / -m sources
     __shared__ C2 tm[kBigRows * kBigCols];
     __shared__ std::uint64_t slot[kFmmChunk];
     const FmmChunk ch = L.axial_chunks[blockIdx.x];
@@ -779,11 +779,6 @@ __global__ __launch_bounds__(256) void coaxial_links_big(FmmTranslateLaunch L) {
     std::size_t off = 0;
     for (int m = 0; m <= mm; ++m) {
         const int rows = pt - m + 1, cols = ps - m + 1;
-        __syncthreads();
-        for (int e = threadIdx.x; e < kFmmChunk * 2 * cols; e += blockDim.x) {
-            const int j = e / (2 * cols), r = e % (2 * cols), sgn = r / cols, n = m + r % cols;
-            ys[j][sgn][r % cols] = j < count ? y[slot[j] + m_major(ps, n, sgn ? -m : m)] : C2{0, 0};
-        }
         C2 acc[kBigCoaxItems][4];
 #pragma unroll
         for (int it = 0; it < kBigCoaxItems; ++it)
@@ -798,6 +793,10 @@ __global__ __launch_bounds__(256) void coaxial_links_big(FmmTranslateLaunch L) {
                 const int r = e / cw, cc = e % cw;
                 tm[r * kBigCols + cc] = T[off + static_cast<std::size_t>(r) * cols + c0 + cc];
             }
+            for (int e = threadIdx.x; e < kFmmChunk * 2 * cw; e += blockDim.x) {
+                const int j = e / (2 * cw), r = e % (2 * cw), sgn = r / cw, cc = r % cw, n = m + c0 + cc;
+                ys[j][sgn][cc] = j < count ? y[slot[j] + m_major(ps, n, sgn ? -m : m)] : C2{0, 0};
+            }
             __syncthreads();
 #pragma unroll
             for (int it = 0; it < kBigCoaxItems; ++it) {
@@ -811,7 +810,7 @@ __global__ __launch_bounds__(256) void coaxial_links_big(FmmTranslateLaunch L) {
                     const C2 tv = tm[row * kBigCols + cc];
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
-                        acc[it][i] = acc[it][i] + tv * ys[j0 + i][sgn][c0 + cc];
+                        acc[it][i] = acc[it][i] + tv * ys[j0 + i][sgn][cc];
                 }
             }
         }
