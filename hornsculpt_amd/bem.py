@@ -195,6 +195,7 @@ def library() -> ctypes.CDLL:
                                                                     ctypes.POINTER(ctypes.c_int32)]),
                 "hsbem_acoustic_frame": (ctypes.c_int, [D, D, D, D]),
                 "hsbem_polar_directions": (ctypes.c_int, [D, D, D, ctypes.c_int32, D, U32, D]),
+                "hsbem_frequency_schedule": (ctypes.c_int, [U64, ctypes.POINTER(ctypes.c_uint64)]),
                 "hsbem_scaled_waves": (ctypes.c_int, [U32, ctypes.c_double, ctypes.c_double, D, ctypes.c_int32, D]),
                 "hsbem_scaled_expansion_evaluate": (ctypes.c_int, [U32, ctypes.c_double, ctypes.c_double, D,
                                                                    ctypes.c_int32, D, D]),
@@ -1178,6 +1179,28 @@ def polar_directions(degrees, vertical: bool = False, origin=(0.0, 0.0, 0.0), fo
     _check(library().hsbem_polar_directions(_dptr(o), _dptr(f), None if h is None else _dptr(h), int(bool(vertical)),
                                             _dptr(d) if len(d) else None, len(d), _dptr(out)))
     return out.reshape(-1, 3)
+
+
+def frequency_schedule(count: int) -> List[int]:
+    """Progressive solve order of a raster of `count` frequencies (frequencySchedule,
+    apps/desktop/frequency_schedule.hpp:11-32)."""
+    out = np.zeros(max(count, 1), np.uint64)
+    _check(library().hsbem_frequency_schedule(count, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+    return [int(i) for i in out[:count]]
+
+
+def progressive_requests(request: BemRequest) -> List[tuple]:
+    """A sweep request as single-frequency requests in frequency_schedule order, as the
+    reference's realtime solve issues them (simulation_dock.cpp:2568-2576): each keeps
+    the sweep's planning raster (its interaction_plan_frequencies, else its frequencies),
+    so all points share one interaction plan. Returns [(raster index, request)]."""
+    raster = list(request.interaction_plan_frequencies) or list(request.frequencies)
+    out = []
+    for i in frequency_schedule(len(request.frequencies)):
+        single = dataclasses.replace(request, frequencies=[request.frequencies[i]],
+                                     interaction_plan_frequencies=raster)
+        out.append((i, single))
+    return out
 
 
 def parse_support_near_plan(blob: np.ndarray, triangle_count: int) -> dict:

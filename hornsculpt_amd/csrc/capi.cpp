@@ -4,6 +4,7 @@
 
 #include "fmm_operator.hpp"
 #include "host/acoustic_analysis.hpp"
+#include "host/frequency_schedule.hpp"
 #include "host/fmm_levels.hpp"
 #include "host/fmm_rotation.hpp"
 #include "host/fmm_scaled.hpp"
@@ -1291,6 +1292,14 @@ int hsbem_polar_directions(const double *origin, const double *forward, const do
             directions[3 * i + 1] = d[i].y;
             directions[3 * i + 2] = d[i].z;
         }
+    });
+}
+
+int hsbem_frequency_schedule(uint64_t count, uint64_t *order) {
+    return guarded([&] {
+        require(order || count == 0, "null schedule output");
+        const auto s = hsbem::frequency_schedule(static_cast<std::size_t>(count));
+        std::copy(s.begin(), s.end(), order);
     });
 }
 

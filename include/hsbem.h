@@ -397,6 +397,13 @@ HSBEM_API int hsbem_acoustic_frame(const double *origin, const double *forward, 
 HSBEM_API int hsbem_polar_directions(const double *origin, const double *forward, const double *horizontal,
                                      int32_t vertical, const double *degrees, uint32_t count, double *directions);
 
+/* Progressive solve order of a raster of count frequencies for realtime sweeps
+ * (replaces frequencySchedule, apps/desktop/frequency_schedule.hpp:11-32): the
+ * highest index, the lowest, then the midpoint of the widest remaining index gap,
+ * repeatedly, equal gaps higher in the raster first; order[count] is a permutation
+ * of 0..count-1. */
+HSBEM_API int hsbem_frequency_schedule(uint64_t count, uint64_t *order);
+
 /* Spherical-harmonic rotation plan of the LF FMM translations (host-only;
  * replaces makeSphericalRotationPlan / rotateToAxis / rotateFromAxis,
  * modules/bem/fmm/spherical_rotation.hpp:8-27): Wigner small-d blocks of

@@ -120,3 +120,28 @@ def test_cancel_and_poll_pending(solvers):
     ok = server.poll(server.submit(solvers["ico"], sphere_request([1000.0])))
     assert ok.frequencies[0].excitations[0].boundary_residual <= 5e-5
     server.close()
+
+
+def test_progressive_sweep_equals_sweep(solvers):
+    """The realtime order (frequency_schedule, simulation_dock.cpp:2568-2576): the
+    sweep's points as single-frequency requests submitted in schedule order share the
+    sweep's planning raster, so each point equals the same point of the sweep solved
+    at once."""
+    horn = solvers["horn"]
+    freqs = [300.0 * 1.6 ** i for i in range(6)]
+    req = _horn_request(freqs, bem.BemNumerics(solver=bem.SOLVER_GMRES, relative_residual_tolerance=1e-9))
+    req.interaction_plan_frequencies = list(freqs)
+    whole = horn.solve(req)
+    server = bem.HipBemServer(slots=1)
+    tickets = [(i, server.submit(horn, r)) for i, r in bem.progressive_requests(req)]
+    done = [(i, server.poll(t)) for i, t in tickets]
+    server.close()
+    worst = 0.0
+    for i, res in done:
+        assert res.frequencies[0].frequency == freqs[i]
+        got = res.frequencies[0].excitations[0].boundary.pressure_trace
+        want = whole.frequencies[i].excitations[0].boundary.pressure_trace
+        worst = max(worst, _rel(got, want))
+    print(f"progressive order {[i for i, _ in done]}: worst vs sweep {worst:.1e}")
+    assert [i for i, _ in done] == [5, 0, 2, 3, 4, 1]
+    assert worst <= 1e-10

@@ -200,3 +200,24 @@ def test_cpp_facade_compiles_against_c_abi(tmp_path):
                            str(ROOT / "hornsculpt_amd" / "libhsbem.so"), f"-Wl,-rpath,{ROOT / 'hornsculpt_amd'}"],
                           capture_output=True, text=True)
     assert proc.returncode == 0, proc.stderr
+
+
+def test_frequency_schedule_matches_reference_tests():
+    # realtime_simulation_tests.cpp:3639-3647: coarse-to-fine with high-gap priority, a permutation
+    assert bem.frequency_schedule(9) == [8, 0, 4, 6, 2, 7, 5, 3, 1]
+    for count in range(83):
+        assert sorted(bem.frequency_schedule(count)) == list(range(count))
+    # by hand: (0,4) -> 2; equal gaps (0,2), (2,4): the higher first -> 3, then 1
+    assert bem.frequency_schedule(5) == [4, 0, 2, 3, 1]
+    assert bem.frequency_schedule(1) == [0] and bem.frequency_schedule(2) == [1, 0]
+
+
+def test_progressive_requests_share_the_sweep_raster():
+    freqs = [200.0 * 1.5 ** i for i in range(6)]
+    req = bem.BemRequest(frequencies=freqs, excitations=[bem.BemExcitation("p", [bem.FaceVelocity(1, 1.0)])])
+    prog = bem.progressive_requests(req)
+    assert [i for i, _ in prog] == bem.frequency_schedule(6)
+    for i, r in prog:
+        assert r.frequencies == [freqs[i]] and r.interaction_plan_frequencies == freqs
+    req.interaction_plan_frequencies = freqs + [9000.0]
+    assert all(r.interaction_plan_frequencies == freqs + [9000.0] for _, r in bem.progressive_requests(req))
