@@ -461,6 +461,8 @@ def main():
                     help="only the batched horn sweep (seeded variants x frequencies x 4 right-hand sides; one JSON line)")
     ap.add_argument("--sweep-variants", type=int, default=4)
     ap.add_argument("--sweep-frequencies", type=int, default=8)
+    ap.add_argument("--fixture", default="", metavar="JSON",
+                    help="only a benchmark fixture file (the reference's solver_benchmark input format; one JSON line)")
     ap.add_argument("--lu", type=int, default=0, metavar="N",
                     help="only the dense complex LU solve of a seeded random N x N system (one JSON line)")
     ap.add_argument("--stream", action="store_true",
@@ -484,6 +486,27 @@ def main():
                                                  f"multipole form (leaf {args.fmm_leaf}, separation 1.5, GMRES restart "
                                                  f"50, rel tol 1e-6, accuracy seed 1e-7, block-Jacobi width {args.flange_block_width})"},
                           "flange": fl}), flush=True)
+        return
+    if args.fixture:
+        mesh, req, digest = bem.read_benchmark_fixture(args.fixture)
+        req.numerics = bem.BemNumerics(precision=bem.PRECISION_MIXED if args.precision == "mixed" else
+                                       bem.PRECISION_FP64,
+                                       solver=bem.SOLVER_GMRES if args.precision == "mixed" else bem.SOLVER_LU)
+        solver = bem.HipBemSolver(mesh)
+        walls = []
+        for rep in range(2):  # cold (plans) and warm
+            t0 = time.perf_counter()
+            res = solver.solve(req)
+            walls.append(time.perf_counter() - t0)
+        print(json.dumps({"metric": "benchmark fixture freq-points solved/s (warm)", "unit": "freq-points/s",
+                          "value": len(req.frequencies) / walls[1], "higher_is_better": True, "n_gpus": 1,
+                          "dtype": "f32+f64" if args.precision == "mixed" else "f64", "data": "fixture",
+                          "config": {"workload": f"fixture {os.path.basename(args.fixture)} (sha256 {digest[:16]}), "
+                                                 f"{len(mesh.triangles)} triangles, {len(req.frequencies)} frequencies, "
+                                                 f"{len(req.excitations)} excitations"},
+                          "cold_s": walls[0], "warm_s": walls[1],
+                          "residual": max(e.boundary_residual for f in res.frequencies for e in f.excitations)}),
+              flush=True)
         return
     if args.lu:
         n = args.lu

@@ -1203,6 +1203,94 @@ def progressive_requests(request: BemRequest) -> List[tuple]:
     return out
 
 
+def read_benchmark_fixture(source) -> tuple:
+    """(mesh, request, sha256 of the bytes) of a benchmark fixture JSON (readFixture,
+    tests/solver_benchmark.cpp:81-175): `source` is a path, bytes or str. Keys: phasor
+    "exp(+iwt)", rho, c, vertices_m, triangles0, tags, optional driven_tags (default:
+    every tag), observations_m (field points), frequencies_hz, and either excitations
+    [{name, face_velocities: [{face_id, real, imag}]}] or the legacy driven_velocity
+    [re, im] (two excitations on the driven tags: "uniform" and "proportional-phase"
+    = 0.6 e^{0.71 i} times it); optional normal_velocities_m_s (per excitation, the DP0
+    velocity of every triangle) must be reproduced by the excitations."""
+    import hashlib
+    import json
+    import os
+    if isinstance(source, (str, os.PathLike)) and not str(source).lstrip().startswith("{"):
+        with open(source, "rb") as fh:
+            raw = fh.read()
+    else:
+        raw = source.encode() if isinstance(source, str) else bytes(source)
+
+    def fail(message):
+        raise ValueError(message)
+
+    try:
+        root = json.loads(raw)
+    except ValueError:
+        fail("Invalid fixture JSON.")
+    if not isinstance(root, dict):
+        fail("Invalid fixture JSON.")
+    if root.get("phasor") != "exp(+iwt)":
+        fail("Expected positive-time phasor fixture.")
+    V = np.asarray(root["vertices_m"], dtype=np.float64).reshape(-1, 3)
+    tags = [int(t) for t in root["tags"]]
+    tris = root["triangles0"]
+    if len(tags) != len(tris):
+        fail("Triangle and tag counts differ.")
+    driven = {int(t) for t in root.get("driven_tags", [])} or set(tags)
+    T = np.zeros((len(tris), 3), np.uint32)
+    for i, t in enumerate(tris):
+        if len(t) != 3:
+            fail("Triangle must contain three indices.")
+        if any(int(v) < 0 or int(v) >= len(V) for v in t):
+            fail("Triangle index is out of range.")
+        T[i] = [int(v) for v in t]
+    faces = np.asarray(tags, np.uint32)
+    bc = np.array([DRIVEN if t in driven else WALL for t in tags], np.uint8)
+    mesh = SurfaceMesh(V, T, faces, bc)
+    freqs = []
+    for f in root["frequencies_hz"]:
+        if not isinstance(f, (int, float)) or not f > 0:
+            fail("Frequency must be positive.")
+        freqs.append(float(f))
+    excitations = []
+    if "excitations" in root:
+        for d in root["excitations"]:
+            excitations.append(BemExcitation(str(d.get("name", "")), [
+                FaceVelocity(int(fv["face_id"]), complex(float(fv["real"]), float(fv["imag"])))
+                for fv in d["face_velocities"]]))
+    else:
+        v = root["driven_velocity"]
+        if len(v) != 2:
+            fail("Invalid legacy complex velocity.")
+        unit = complex(float(v[0]), float(v[1]))
+        for name, factor in (("uniform", 1.0), ("proportional-phase", 0.6 * np.exp(0.71j))):
+            excitations.append(BemExcitation(name, [FaceVelocity(t, unit * factor) for t in sorted(driven)]))
+    if not excitations:
+        fail("Fixture has no excitation.")
+    if "normal_velocities_m_s" in root:
+        columns = root["normal_velocities_m_s"]
+        if len(columns) != len(excitations):
+            fail("DP0 RHS count mismatch.")
+        for column, exc in zip(columns, excitations):
+            if len(column) != len(tags):
+                fail("DP0 column length mismatch.")
+            by_face = {}
+            for fv in exc.faces:
+                by_face.setdefault(fv.face_id, fv.normal_velocity)
+            for row, value in enumerate(column):
+                if len(value) != 2:
+                    fail("Invalid DP0 velocity.")
+                expected = complex(float(value[0]), float(value[1]))
+                if abs(expected - by_face.get(tags[row], 0j)) >= 1e-12:
+                    fail("Face excitation does not reproduce fixture DP0 data.")
+                if bc[row] != DRIVEN and abs(expected) != 0:
+                    fail("Nonzero Wall velocity.")
+    request = BemRequest(medium=AcousticMedium(float(root["rho"]), float(root["c"])), frequencies=freqs,
+                         excitations=excitations, observers=[tuple(p) for p in root.get("observations_m", [])])
+    return mesh, request, hashlib.sha256(raw).hexdigest()
+
+
 def parse_support_near_plan(blob: np.ndarray, triangle_count: int) -> dict:
     rows, pnnz, fnnz, nodes1, npanels, nblocks, njobs = (int(x) for x in blob[:7])
     pos = 7
