@@ -504,51 +504,50 @@ __global__ void apply_pivots(double *B, std::uint32_t n, const std::int32_t *piv
 constexpr int kSolveBlock = 64;
 
 // Solve the diagonal block [j0, j0+nb) (lower unit or upper) for every rhs; one
-// workgroup per rhs, the block in LDS.
+// workgroup per rhs. The block is staged in LDS first (column by column, rows over
+// the threads: coalesced, independent loads), so the nb sequential steps read LDS
+// only; the arithmetic and its order are those of a column-by-column substitution.
 __global__ __launch_bounds__(64) void block_substitute(const double *LU, std::uint32_t n, double *B, std::uint32_t j0,
                                                       std::uint32_t nb, int upper) {
-    __shared__ double x[kSolveBlock][2];
-    double *b = B + 2 * (static_cast<std::size_t>(blockIdx.x) * n + j0);
-    const int t = threadIdx.x;
-    if (t < static_cast<int>(nb)) {
-        x[t][0] = b[2 * t];
-        x[t][1] = b[2 * t + 1];
+    __shared__ double2 blk[kSolveBlock][kSolveBlock + 1]; // blk[c][r] = LU[j0 + r][j0 + c]
+    __shared__ double2 x[kSolveBlock];
+    const double2 *LU2 = reinterpret_cast<const double2 *>(LU);
+    double2 *b = reinterpret_cast<double2 *>(B) + static_cast<std::size_t>(blockIdx.x) * n + j0;
+    const int t = threadIdx.x, m = static_cast<int>(nb);
+    if (t < m) {
+        for (int c = 0; c < m; ++c)
+            blk[c][t] = LU2[(static_cast<std::size_t>(j0) + c) * n + j0 + t];
+        x[t] = b[t];
     }
     __syncthreads();
     if (!upper) {
-        for (std::uint32_t j = 0; j < nb; ++j) {
-            if (t > static_cast<int>(j) && t < static_cast<int>(nb)) {
-                const double lr = LU[2 * ((static_cast<std::size_t>(j0) + j) * n + j0 + t)],
-                             li = LU[2 * ((static_cast<std::size_t>(j0) + j) * n + j0 + t) + 1];
-                x[t][0] -= lr * x[j][0] - li * x[j][1];
-                x[t][1] -= lr * x[j][1] + li * x[j][0];
+        for (int j = 0; j < m; ++j) {
+            if (t > j && t < m) {
+                const double2 l = blk[j][t], xj = x[j];
+                x[t].x -= l.x * xj.x - l.y * xj.y;
+                x[t].y -= l.x * xj.y + l.y * xj.x;
             }
             __syncthreads();
         }
     } else {
-        for (std::uint32_t jj = nb; jj-- > 0;) {
-            if (t == static_cast<int>(jj)) {
-                const double ur = LU[2 * ((static_cast<std::size_t>(j0) + jj) * n + j0 + jj)],
-                             ui = LU[2 * ((static_cast<std::size_t>(j0) + jj) * n + j0 + jj) + 1];
-                const double d = ur * ur + ui * ui;
-                const double vr = x[jj][0], vi = x[jj][1];
-                x[jj][0] = (vr * ur + vi * ui) / d;
-                x[jj][1] = (vi * ur - vr * ui) / d;
+        for (int jj = m - 1; jj >= 0; --jj) {
+            if (t == jj) {
+                const double2 u = blk[jj][jj];
+                const double d = u.x * u.x + u.y * u.y;
+                const double vr = x[jj].x, vi = x[jj].y;
+                x[jj] = double2{(vr * u.x + vi * u.y) / d, (vi * u.x - vr * u.y) / d};
             }
             __syncthreads();
-            if (t < static_cast<int>(jj)) {
-                const double ur = LU[2 * ((static_cast<std::size_t>(j0) + jj) * n + j0 + t)],
-                             ui = LU[2 * ((static_cast<std::size_t>(j0) + jj) * n + j0 + t) + 1];
-                x[t][0] -= ur * x[jj][0] - ui * x[jj][1];
-                x[t][1] -= ur * x[jj][1] + ui * x[jj][0];
+            if (t < jj) {
+                const double2 u = blk[jj][t], xj = x[jj];
+                x[t].x -= u.x * xj.x - u.y * xj.y;
+                x[t].y -= u.x * xj.y + u.y * xj.x;
             }
             __syncthreads();
         }
     }
-    if (t < static_cast<int>(nb)) {
-        b[2 * t] = x[t][0];
-        b[2 * t + 1] = x[t][1];
-    }
+    if (t < m)
+        b[t] = x[t];
 }
 
 // B[rows] -= LU[rows, j0:j0+nb] * B[j0:j0+nb] for rows outside the block
