@@ -14,6 +14,7 @@ The native library is required; there is no fallback path.
 from __future__ import annotations
 
 import ctypes
+import math
 import dataclasses
 import threading
 from pathlib import Path
@@ -90,7 +91,8 @@ class _ServerStatistics(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64),
                 ("points", ctypes.c_uint64), ("running", ctypes.c_uint32), ("queued", ctypes.c_uint32),
                 ("max_running", ctypes.c_uint32), ("budget_bytes", ctypes.c_uint64), ("held_bytes", ctypes.c_uint64),
-                ("peak_held_bytes", ctypes.c_uint64)]
+                ("peak_held_bytes", ctypes.c_uint64), ("resident_bytes", ctypes.c_uint64),
+                ("peak_total_bytes", ctypes.c_uint64)]
 
 
 _PROGRESS = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_int32)
@@ -119,6 +121,8 @@ def library() -> ctypes.CDLL:
                 "hsbem_mesh_destroy": (ctypes.c_int, [P]),
                 "hsbem_mesh_info": (ctypes.c_int, [P, ctypes.POINTER(U64), ctypes.POINTER(U32),
                                                    ctypes.POINTER(U32), D]),
+                "hsbem_mesh_cache_info": (ctypes.c_int, [P, ctypes.POINTER(U64), ctypes.POINTER(U64)]),
+                "hsbem_mesh_validate": (ctypes.c_int, [D, U32, ctypes.POINTER(U32), U32]),
                 "hsbem_solve": (ctypes.c_int, [P, ctypes.POINTER(_Request), _PROGRESS, P,
                                                ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(P)]),
                 "hsbem_result_destroy": (ctypes.c_int, [P]),
@@ -473,6 +477,12 @@ class HipBemSolver:
         self.mesh_hash, self.vertex_count, self.triangle_count = h.value, nv.value, nt.value
         self.surface_area = area.value
 
+    def cache_info(self) -> dict:
+        """Cached interaction plans and resident device bytes of this mesh."""
+        plans, resident = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(self._lib.hsbem_mesh_cache_info(self.handle, ctypes.byref(plans), ctypes.byref(resident)))
+        return {"cached_plans": plans.value, "resident_bytes": resident.value}
+
     def __del__(self):
         if getattr(self, "handle", None) and _lib is not None:
             _lib.hsbem_mesh_destroy(self.handle)
@@ -685,6 +695,15 @@ def _copy_plan(sizes_fn, copy_fn, handle, nt: int) -> dict:
     return {"touching": touching[:nt_], "near": near[:nn], "patches": patches[:npatch], "whole": whole[:nwhole],
             "near_offsets": offsets, "near_trials": trials[:nn], "candidate_pairs": int(sizes[4]),
             "maximum_depth": int(sizes[5]), "maximum_estimated_error": err.value}
+
+
+def validate_mesh(mesh: SurfaceMesh) -> None:
+    """The mesh checks of HipBemSolver(mesh) on the host only (hsbem_mesh_validate):
+    closed, consistently outward oriented, no self-intersection, no nested
+    component. Raises ValueError with the reason."""
+    v = np.ascontiguousarray(mesh.vertices, dtype=np.float64)
+    t = np.ascontiguousarray(mesh.triangles, dtype=np.uint32)
+    _check(library().hsbem_mesh_validate(_dptr(v), len(v), _uptr(t), len(t)))
 
 
 def interaction_plan(mesh: SurfaceMesh, numerics: BemNumerics, k_max: float) -> dict:
@@ -1224,6 +1243,33 @@ def read_benchmark_fixture(source) -> tuple:
     def fail(message):
         raise ValueError(message)
 
+    def is_number(v):  # a JSON number (bool is not one)
+        return isinstance(v, (int, float)) and not isinstance(v, bool)
+
+    def number(obj, name):
+        v = obj.get(name) if isinstance(obj, dict) else None
+        if not is_number(v) or not math.isfinite(v):
+            fail("Missing or invalid scalar fixture value.")
+        return float(v)
+
+    def array(obj, name):
+        v = obj.get(name) if isinstance(obj, dict) else None
+        if not isinstance(v, list):
+            fail("Missing fixture array.")
+        return v
+
+    def position(v):
+        if not isinstance(v, list) or len(v) != 3:
+            fail("Position must contain three coordinates.")
+        if not all(is_number(x) and math.isfinite(x) for x in v):
+            fail("Invalid position coordinate.")
+        return tuple(float(x) for x in v)
+
+    def index(v):
+        if not is_number(v) or not (0 <= v <= 4294967295) or math.floor(v) != v:
+            fail("Invalid unsigned fixture index.")
+        return int(v)
+
     try:
         root = json.loads(raw)
     except ValueError:
@@ -1232,35 +1278,42 @@ def read_benchmark_fixture(source) -> tuple:
         fail("Invalid fixture JSON.")
     if root.get("phasor") != "exp(+iwt)":
         fail("Expected positive-time phasor fixture.")
-    V = np.asarray(root["vertices_m"], dtype=np.float64).reshape(-1, 3)
-    tags = [int(t) for t in root["tags"]]
-    tris = root["triangles0"]
-    if len(tags) != len(tris):
+    medium = AcousticMedium(number(root, "rho"), number(root, "c"))
+    V = np.asarray([position(v) for v in array(root, "vertices_m")], dtype=np.float64).reshape(-1, 3)
+    driven = {index(t) for t in array(root, "driven_tags")} if "driven_tags" in root else set()
+    tags_json, tris = array(root, "tags"), array(root, "triangles0")
+    if len(tags_json) != len(tris):
         fail("Triangle and tag counts differ.")
-    driven = {int(t) for t in root.get("driven_tags", [])} or set(tags)
+    tags = [index(t) for t in tags_json]
+    driven = driven or set(tags)
     T = np.zeros((len(tris), 3), np.uint32)
     for i, t in enumerate(tris):
-        if len(t) != 3:
+        if not isinstance(t, list) or len(t) != 3:
             fail("Triangle must contain three indices.")
-        if any(int(v) < 0 or int(v) >= len(V) for v in t):
+        ids = [index(v) for v in t]
+        if any(v >= len(V) for v in ids):
             fail("Triangle index is out of range.")
-        T[i] = [int(v) for v in t]
+        T[i] = ids
     faces = np.asarray(tags, np.uint32)
     bc = np.array([DRIVEN if t in driven else WALL for t in tags], np.uint8)
     mesh = SurfaceMesh(V, T, faces, bc)
+    observers = [position(p) for p in array(root, "observations_m")]
     freqs = []
-    for f in root["frequencies_hz"]:
-        if not isinstance(f, (int, float)) or not f > 0:
+    for f in array(root, "frequencies_hz"):
+        if not is_number(f) or not f > 0:
             fail("Frequency must be positive.")
         freqs.append(float(f))
     excitations = []
     if "excitations" in root:
-        for d in root["excitations"]:
-            excitations.append(BemExcitation(str(d.get("name", "")), [
-                FaceVelocity(int(fv["face_id"]), complex(float(fv["real"]), float(fv["imag"])))
-                for fv in d["face_velocities"]]))
+        for d in array(root, "excitations"):
+            d = d if isinstance(d, dict) else {}
+            name = d.get("name", "")
+            excitations.append(BemExcitation(name if isinstance(name, str) else "", [
+                FaceVelocity(index(fv.get("face_id") if isinstance(fv, dict) else None),
+                             complex(number(fv, "real"), number(fv, "imag")))
+                for fv in array(d, "face_velocities")]))
     else:
-        v = root["driven_velocity"]
+        v = array(root, "driven_velocity")
         if len(v) != 2:
             fail("Invalid legacy complex velocity.")
         unit = complex(float(v[0]), float(v[1]))
@@ -1269,25 +1322,24 @@ def read_benchmark_fixture(source) -> tuple:
     if not excitations:
         fail("Fixture has no excitation.")
     if "normal_velocities_m_s" in root:
-        columns = root["normal_velocities_m_s"]
+        columns = array(root, "normal_velocities_m_s")
         if len(columns) != len(excitations):
             fail("DP0 RHS count mismatch.")
         for column, exc in zip(columns, excitations):
-            if len(column) != len(tags):
+            if not isinstance(column, list) or len(column) != len(tags):
                 fail("DP0 column length mismatch.")
             by_face = {}
             for fv in exc.faces:
                 by_face.setdefault(fv.face_id, fv.normal_velocity)
             for row, value in enumerate(column):
-                if len(value) != 2:
+                if not isinstance(value, list) or len(value) != 2:
                     fail("Invalid DP0 velocity.")
                 expected = complex(float(value[0]), float(value[1]))
-                if abs(expected - by_face.get(tags[row], 0j)) >= 1e-12:
+                if not abs(expected - by_face.get(tags[row], 0j)) < 1e-12:
                     fail("Face excitation does not reproduce fixture DP0 data.")
                 if bc[row] != DRIVEN and abs(expected) != 0:
                     fail("Nonzero Wall velocity.")
-    request = BemRequest(medium=AcousticMedium(float(root["rho"]), float(root["c"])), frequencies=freqs,
-                         excitations=excitations, observers=[tuple(p) for p in root.get("observations_m", [])])
+    request = BemRequest(medium=medium, frequencies=freqs, excitations=excitations, observers=observers)
     return mesh, request, hashlib.sha256(raw).hexdigest()
 
 

@@ -13,6 +13,7 @@
 #include "host/support_tree.hpp"
 #include "server.hpp"
 #include "solver.hpp"
+#include "host/mesh_check.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -321,6 +322,24 @@ int hsbem_mesh_create(hsbem_context *c, const double *vertices, uint32_t nv, con
     });
 }
 
+int hsbem_mesh_validate(const double *vertices, uint32_t nv, const uint32_t *triangles, uint32_t nt) {
+    return guarded([&] {
+        require(vertices && triangles && nv && nt, "invalid mesh arguments");
+        hsbem::SurfaceMesh m;
+        m.vertices.resize(nv);
+        const hsbem::Vec3 origin{vertices[0], vertices[1], vertices[2]};
+        for (uint32_t i = 0; i < nv; ++i) {
+            m.vertices[i] = hsbem::vsub({vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2]}, origin);
+            if (!std::isfinite(m.vertices[i].x) || !std::isfinite(m.vertices[i].y) || !std::isfinite(m.vertices[i].z))
+                throw std::invalid_argument("BEM mesh coordinates are not finite.");
+        }
+        m.triangles.resize(nt);
+        for (uint32_t t = 0; t < nt; ++t)
+            m.triangles[t].vertices = {triangles[3 * t], triangles[3 * t + 1], triangles[3 * t + 2]};
+        hsbem::validate_closed_exterior(m);
+    });
+}
+
 int hsbem_mesh_destroy(hsbem_mesh *m) {
     return guarded([&] { delete m; });
 }
@@ -336,6 +355,16 @@ int hsbem_mesh_info(const hsbem_mesh *m, uint64_t *hash, uint32_t *nv, uint32_t 
             *nt = m->solver->nt();
         if (area)
             *area = m->solver->space().surface_area;
+    });
+}
+
+int hsbem_mesh_cache_info(hsbem_mesh *m, uint64_t *cached_plans, uint64_t *resident_bytes) {
+    return guarded([&] {
+        require(m, "null mesh");
+        if (cached_plans)
+            *cached_plans = m->solver->cached_plans();
+        if (resident_bytes)
+            *resident_bytes = m->solver->resident_bytes();
     });
 }
 
@@ -411,6 +440,8 @@ int hsbem_server_stats(const hsbem_server *s, hsbem_server_statistics *out) {
         out->budget_bytes = st.budget;
         out->held_bytes = st.held_bytes;
         out->peak_held_bytes = st.peak_held_bytes;
+        out->resident_bytes = st.resident_bytes;
+        out->peak_total_bytes = st.peak_total_bytes;
     });
 }
 

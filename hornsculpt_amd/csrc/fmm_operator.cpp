@@ -940,6 +940,7 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
                       op->axial_.bytes() + op->jacobi_->bytes() + op->links_.bytes() + op->destinations_.bytes() +
                       op->work_y_.bytes() + op->work_z_.bytes() + op->hf_legendre_.bytes() + op->diag_table_.bytes() +
                       op->hf_fourier_.bytes() + op->hf_harmonic_.bytes();
+    st.shared_bytes = op->near_->values.bytes() + ns->columns.bytes() + op->jacobi_->bytes();
     return op;
 }
 
@@ -1096,7 +1097,13 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
     fr.fmm_plane_wave_levels = op->statistics().plane_wave_levels;
     fr.fmm_m2l_links = op->statistics().m2l_links;
     fr.fmm_near_nnz = op->statistics().near_nnz;
-    fr.fmm_device_bytes = op->statistics().device_bytes;
+    // device bytes of the point: the first operator whole, later ones (audit,
+    // verification, comparison, promotion; all share the first one's near entries
+    // and preconditioner) without the shared part -- an upper bound of the peak
+    std::uint64_t point_bytes = op->statistics().device_bytes;
+    auto count_operator = [&point_bytes](const FmmOperator &o) {
+        point_bytes += o.statistics().device_bytes - o.statistics().shared_bytes;
+    };
     {
         std::lock_guard lock(plan_mutex_);
         if (!internal_of_device_.size())
@@ -1263,6 +1270,7 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
             FmmSettings aset = fs;
             aset.accuracy_seed = fs.accuracy_seed / 10;
             audit = build_fmm(num, P.k_max, k, aset, faces, s, &A, A.statistics().endpoint_degree);
+            count_operator(*audit);
             residuals(action(*audit));
             DeviceBuffer<double> prod(nvec);
             for (unsigned refinements = 0;;) {
@@ -1273,6 +1281,7 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
                 for (;;) {
                     aset.accuracy_seed /= 10;
                     verify = build_fmm(num, P.k_max, k, aset, faces, s, &A, A.statistics().endpoint_degree);
+                    count_operator(*verify);
                     if (!same_levels(*verify, *audit))
                         break;
                     if (refinements == kMaximumSeedRefinements)
@@ -1315,6 +1324,7 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
         if (!direct)
             cset.accuracy_seed = fr.fmm_verification_seed; // the verified seed, the next rule
         const std::unique_ptr<FmmOperator> cmp = build_fmm(num, P.k_max, k, cset, faces, s, &A, next);
+        count_operator(*cmp);
         right_hand_sides(*cmp, b_next);
         double quad = 0, next_residual = 0;
         for (std::uint32_t e = 0; e < ne; ++e) {
@@ -1347,6 +1357,7 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
         }
         // promote the operator's rule; the right-hand side follows the new rule
         promoted = build_fmm(num, P.k_max, k, fs, faces, s, &A, next);
+        count_operator(*promoted);
         ++fr.fmm_promotions;
         right_hand_sides(*promoted, ws.fmm_b);
         norms_of_b();
@@ -1355,6 +1366,11 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
     }
     const FmmOperator &final_op = promoted ? *promoted : *op;
     fr.ms[1] = ms_since(t_solve);
+    point_bytes += ws.fmm_b.capacity_bytes() + ws.fmm_x.capacity_bytes() + ws.fmm_r.capacity_bytes() +
+                   ax.capacity_bytes() + t1.capacity_bytes() + t2.capacity_bytes() + b_next.capacity_bytes() +
+                   2 * static_cast<std::uint64_t>(nvec) * sizeof(double); // the seed audit's product vectors
+    fr.fmm_device_bytes = point_bytes;
+    note_fmm_point_bytes(num, point_bytes);
     {
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "event");

@@ -38,6 +38,7 @@ struct FmmStatistics {
     std::uint32_t nodes{}, levels{}, leaves{}, top_level{}, rule_points{}, endpoint_degree{};
     std::uint64_t near_nnz{}, far_pairs{}, m2m_links{}, m2l_links{}, l2l_links{}, rotations{}, axial_plans{};
     std::uint64_t expansion_values{}, moment_values{}, device_bytes{};
+    std::uint64_t shared_bytes{}; // of device_bytes: near entries and preconditioner shared with later operators
     std::uint64_t plane_wave_levels{}, plane_wave_leaves{}, angular_links{}, diagonal_classes{};
     std::uint64_t plan_cache_hits{}, plans_built{}; // translation plans reused / built by this operator
     std::vector<std::uint32_t> level_orders, level_representation, level_rule_orders; // representation 1 LF, 2 HF
@@ -56,6 +57,13 @@ struct NearStructure {
     DeviceBuffer<NearJob> fans;
     DeviceBuffer<dev::NearPatchItem> items;
     std::size_t nnz{}, fnnz{}, jobs{}, regular_count{}, touching_count{}, patch_count{}, fan_count{};
+    std::uint64_t device_bytes() const {
+        return row_offsets.capacity_bytes() + columns.capacity_bytes() + flux_offsets.capacity_bytes() +
+               flux_columns.capacity_bytes() + face_column.capacity_bytes() + regular.capacity_bytes() +
+               regular_masks.capacity_bytes() + touching_masks.capacity_bytes() + fan_masks.capacity_bytes() +
+               patch_masks.capacity_bytes() + touching.capacity_bytes() + fans.capacity_bytes() +
+               items.capacity_bytes();
+    }
 };
 
 // Near entries at one wavenumber (values in the structure's CSR pattern).

@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <map>
 #include <mutex>
+#include <shared_mutex>
 #include <utility>
 
 namespace hsbem::dev {
@@ -1020,10 +1021,18 @@ std::uint32_t gemv_columns(const void *A, bool fp32, std::uint32_t n, const doub
         if (v == 512 || v == 1024 || v == 2048)
             return static_cast<std::uint32_t>(v);
     }
-    static std::mutex mutex;
+    // Lookups take a shared lock, so slots on other streams never wait on each
+    // other once a size is tuned; only the first use of an (n, precision) pair
+    // tunes, under the exclusive lock (other slots needing that pair wait for it).
+    static std::shared_mutex mutex;
     static std::map<std::pair<std::uint32_t, bool>, std::uint32_t> chosen;
-    std::lock_guard lock(mutex);
     const auto key = std::make_pair(n, fp32);
+    {
+        std::shared_lock lookup(mutex);
+        if (auto it = chosen.find(key); it != chosen.end())
+            return it->second;
+    }
+    std::unique_lock lock(mutex);
     if (auto it = chosen.find(key); it != chosen.end())
         return it->second;
     hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;

@@ -161,6 +161,7 @@ std::shared_ptr<const NearStructure> MeshSolver::near_structure(const Numerics &
     std::lock_guard lock(plan_mutex_);
     near_cache_ = ns;
     near_cache_key_ = key;
+    near_bytes_ = ns->device_bytes();
     return ns;
 }
 

@@ -8,12 +8,19 @@ namespace {
 
 // True when a workspace sized for `held` has room for every buffer of `next`,
 // so the slot can switch requests without freeing (hipFree synchronises the device).
+// Buffers only grow, so `next` must need no buffer kind `held` lacks and no more
+// bytes than `held` reserved: the same solver, form and preconditioner, Krylov
+// bases no longer than held's, and every dimension within held's.
 bool covers(const PreparedRequest &held, const PreparedRequest &next) {
     const Numerics &a = held.request.numerics, &b = next.request.numerics;
-    const bool lu_a = a.precision == 0 && a.solver == 0, lu_b = b.precision == 0 && b.solver == 0;
-    return next.nv <= held.nv && next.nt <= held.nt && std::max(next.nd, 1u) <= std::max(held.nd, 1u) &&
-           next.ne <= held.ne && next.observers.size() <= held.observers.size() &&
-           lu_a == lu_b && (lu_b || b.gmres_restart <= a.gmres_restart) && (b.precision == 1 || a.precision == 0);
+    const bool fmm = b.acceleration == 1;
+    const bool same_kind = a.acceleration == b.acceleration && a.precision == b.precision &&
+                           (fmm || a.solver == b.solver) && a.preconditioner == b.preconditioner &&
+                           (!fmm || a.fmm_block_width == b.fmm_block_width);
+    const bool krylov = fmm ? b.fmm_restart <= a.fmm_restart : b.gmres_restart <= a.gmres_restart;
+    return same_kind && krylov && next.footprint <= held.footprint && next.nv <= held.nv && next.nt <= held.nt &&
+           std::max(next.nd, 1u) <= std::max(held.nd, 1u) && next.ne <= held.ne &&
+           next.observers.size() <= held.observers.size();
 }
 
 } // namespace
@@ -83,6 +90,11 @@ std::uint64_t Server::submit(std::shared_ptr<MeshSolver> mesh, const Request &re
         std::lock_guard lock(mutex_);
         if (!start_error_.empty())
             throw DeviceError("BEM serving slots failed to start: " + start_error_);
+        bool known = false;
+        for (const auto &w : meshes_)
+            known = known || w.lock() == t->mesh;
+        if (!known)
+            meshes_.push_back(t->mesh);
         id = t->id = next_ticket_++;
         tickets_.emplace(id, t);
         for (std::uint32_t i = 0; i < n; ++i)
@@ -92,6 +104,20 @@ std::uint64_t Server::submit(std::shared_ptr<MeshSolver> mesh, const Request &re
     }
     work_cv_.notify_all();
     return id;
+}
+
+std::uint64_t Server::resident_locked() {
+    std::uint64_t total = 0;
+    for (auto it = meshes_.begin(); it != meshes_.end();) {
+        if (const auto m = it->lock()) {
+            total += m->resident_bytes();
+            ++it;
+        } else {
+            it = meshes_.erase(it);
+        }
+    }
+    stats_.resident_bytes = total;
+    return total;
 }
 
 void Server::fail(Ticket &t, std::exception_ptr e) {
@@ -114,6 +140,7 @@ void Server::worker_loop(unsigned slot, Workspace &ws) {
                 // that fits the budget (arrival order, smaller points may pass a
                 // blocked larger one).
                 bool found = false;
+                const std::uint64_t resident = resident_locked();
                 for (auto it = queue_.begin(); it != queue_.end();) {
                     Ticket &t = *it->ticket;
                     if (t.error || t.cancel) {
@@ -124,9 +151,14 @@ void Server::worker_loop(unsigned slot, Workspace &ws) {
                         done_cv_.notify_all();
                         continue;
                     }
+                    // the slot's workspace, the other slots' and the meshes' resident state
+                    // (plus this point's plan if it is not built yet) must fit the budget
                     const std::uint64_t need = t.prepared->footprint;
-                    if ((held && covers(*held, *t.prepared)) ||
-                        held_ - slot_bytes_[slot] + need <= options_.memory_budget) {
+                    const std::uint64_t state = resident + t.mesh->pending_plan_bytes(*t.prepared);
+                    const bool reuse = held && covers(*held, *t.prepared);
+                    const std::uint64_t total = (reuse ? held_ : held_ - slot_bytes_[slot] + need) + state;
+                    if (total <= options_.memory_budget) {
+                        stats_.peak_total_bytes = std::max(stats_.peak_total_bytes, total);
                         item = *it;
                         queue_.erase(it);
                         found = true;
@@ -136,6 +168,24 @@ void Server::worker_loop(unsigned slot, Workspace &ws) {
                 }
                 if (found)
                     break;
+                if (!queue_.empty() && !stats_.running && !held_) {
+                    // Nothing runs and no slot holds memory, yet no point fits: drop the
+                    // meshes' unused cached plans; if that frees nothing, the front
+                    // request cannot be served under this budget.
+                    std::uint64_t released = 0;
+                    for (const auto &w : meshes_)
+                        if (const auto m = w.lock())
+                            released += m->trim_plans(0);
+                    if (released)
+                        continue;
+                    std::ostringstream msg;
+                    msg << "BEM serving budget of " << options_.memory_budget
+                        << " device bytes cannot hold the resident mesh state (" << resident
+                        << " bytes) and one frequency point of this request.";
+                    Ticket &t = *queue_.front().ticket;
+                    fail(t, std::make_exception_ptr(std::runtime_error(msg.str())));
+                    continue;
+                }
                 if (!queue_.empty() && slot_bytes_[slot]) {
                     release = true; // an idle slot gives its memory back to blocked points
                     break;

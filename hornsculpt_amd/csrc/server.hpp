@@ -9,8 +9,8 @@
 // order as soon as a slot is free and the point's device footprint fits the
 // memory budget, so points of requests that arrive while others run are
 // admitted immediately and their kernels overlap on the device. Mesh state
-// (geometry, clusters, interaction plans) stays resident in the MeshSolver and
-// is shared by every slot.
+// (geometry, clusters, interaction plans, FMM near structures) stays resident in
+// the MeshSolver, is shared by every slot and counts against the same budget.
 #pragma once
 
 #include "solver.hpp"
@@ -32,6 +32,8 @@ struct ServerStats {
     std::uint64_t submitted{}, completed{}, failed{}, points{};
     std::uint32_t running{}, queued{}, max_running{};
     std::uint64_t budget{}, held_bytes{}, peak_held_bytes{};
+    std::uint64_t resident_bytes{}; // device bytes meshes keep between points (geometry, plans, near structures)
+    std::uint64_t peak_total_bytes{}; // largest held_bytes + resident_bytes seen at an admission
 };
 
 class Server {
@@ -69,12 +71,15 @@ class Server {
     };
     void worker_loop(unsigned slot, Workspace &ws);
     void fail(Ticket &t, std::exception_ptr e);
+    // Device bytes resident in the meshes this server has seen (mutex_ held).
+    std::uint64_t resident_locked();
 
     Context *ctx_;
     ServerOptions options_;
     mutable std::mutex mutex_;
     std::condition_variable work_cv_, done_cv_;
     std::deque<Item> queue_;
+    std::vector<std::weak_ptr<MeshSolver>> meshes_; // meshes whose resident state counts against the budget
     std::map<std::uint64_t, std::shared_ptr<Ticket>> tickets_;
     std::vector<std::uint64_t> slot_bytes_; // footprint reserved by each slot's workspace
     std::uint64_t held_{};

@@ -109,6 +109,11 @@ MeshSolver::MeshSolver(Context *ctx, SurfaceMesh mesh) : ctx_(ctx) {
     build_clusters();
     rules_.upload(rule_table(), ctx_->stream);
     hip_check(hipStreamSynchronize(ctx_->stream), "mesh upload");
+    geometry_bytes_ = geo_.capacity_bytes() + bounds_.capacity_bytes() + row_scale_.capacity_bytes() +
+                      lumped_.capacity_bytes() + rules_.capacity_bytes() + nodes_.capacity_bytes() +
+                      shape_.capacity_bytes() + local_.capacity_bytes() + cluster_panel_offsets_.capacity_bytes() +
+                      cluster_panels_.capacity_bytes() + cluster_node_offsets_.capacity_bytes() +
+                      cluster_nodes_.capacity_bytes();
     workspace_ = std::make_unique<Workspace>(ctx_->stream);
 }
 
@@ -284,11 +289,15 @@ std::shared_ptr<DevicePlan> MeshSolver::device_plan(const Numerics &n, double k_
                                      k_max);
     // Held while building: other streams needing the same mesh's plan wait for it.
     std::lock_guard lock(plan_mutex_);
-    auto it = plans_.find(key);
-    if (it != plans_.end()) {
-        if (built)
-            *built = false;
-        return it->second;
+    {
+        std::lock_guard index(plan_index_mutex_);
+        auto it = plans_.find(key);
+        if (it != plans_.end()) {
+            it->second->last_use = ++plan_clock_;
+            if (built)
+                *built = false;
+            return it->second;
+        }
     }
     const auto t0 = Clock::now();
     auto dp = std::make_shared<DevicePlan>();
@@ -391,8 +400,65 @@ std::shared_ptr<DevicePlan> MeshSolver::device_plan(const Numerics &n, double k_
     dp->plan_ms = ms_since(t0);
     if (built)
         *built = true;
-    plans_.emplace(key, dp);
+    {
+        std::lock_guard index(plan_index_mutex_);
+        dp->last_use = ++plan_clock_;
+        plans_.emplace(key, dp);
+        plan_bytes_ += dp->device_bytes();
+    }
+    trim_plans(kPlanCacheEntries);
     return dp;
+}
+
+std::uint64_t MeshSolver::trim_plans(std::size_t keep) {
+    std::lock_guard index(plan_index_mutex_);
+    std::uint64_t released = 0;
+    while (plans_.size() > keep) {
+        auto victim = plans_.end();
+        for (auto it = plans_.begin(); it != plans_.end(); ++it)
+            if (it->second.use_count() == 1 && (victim == plans_.end() || it->second->last_use < victim->second->last_use))
+                victim = it;
+        if (victim == plans_.end())
+            break; // every remaining plan is held by a solve or the near structure
+        const std::uint64_t bytes = victim->second->device_bytes();
+        plans_.erase(victim); // the last reference: device buffers are freed here
+        plan_bytes_ -= bytes;
+        released += bytes;
+    }
+    return released;
+}
+
+std::size_t MeshSolver::cached_plans() {
+    std::lock_guard index(plan_index_mutex_);
+    return plans_.size();
+}
+
+std::uint64_t MeshSolver::resident_bytes() { return geometry_bytes_ + plan_bytes_.load() + near_bytes_.load(); }
+
+std::uint64_t MeshSolver::pending_plan_bytes(const PreparedRequest &P) {
+    const Numerics &n = P.request.numerics;
+    const auto key = std::make_tuple(n.quadrature_order, n.singular_order, n.near_ratio, n.near_tolerance, n.near_depth,
+                                     P.k_max);
+    std::lock_guard index(plan_index_mutex_);
+    return plans_.count(key) && (n.acceleration == 0 || near_bytes_.load() > 0) ? 0 : P.plan_estimate;
+}
+
+namespace {
+std::tuple<unsigned, double, unsigned, unsigned, unsigned> fmm_bytes_key(const Numerics &n) {
+    return {n.fmm_leaf_capacity, n.fmm_separation, n.fmm_block_width, n.fmm_restart, n.quadrature_order};
+}
+} // namespace
+
+std::uint64_t MeshSolver::fmm_point_bytes(const Numerics &n) {
+    std::lock_guard index(plan_index_mutex_);
+    const auto it = fmm_point_bytes_.find(fmm_bytes_key(n));
+    return it == fmm_point_bytes_.end() ? 0 : it->second;
+}
+
+void MeshSolver::note_fmm_point_bytes(const Numerics &n, std::uint64_t bytes) {
+    std::lock_guard index(plan_index_mutex_);
+    auto &v = fmm_point_bytes_[fmm_bytes_key(n)];
+    v = std::max(v, bytes);
 }
 
 dev::AssemblyLaunch MeshSolver::assembly_launch(const DevicePlan &p, const Numerics &n, double k, double *A, double *BD,
@@ -979,11 +1045,17 @@ std::shared_ptr<PreparedRequest> MeshSolver::prepare(const Request &req) const {
     const bool fmm = num.acceleration == 1;
     std::uint64_t bytes = (fmm ? 0 : (mixed ? 8 : 16) * nv2) + 16 * nv * nd1 + 16 * nd1 * ne + 3 * 16ull * nv * ne +
                           16ull * nt * ne + 8 * scratch + 4ull * nt;
-    if (fmm) // operator estimate: near entries, leaf moments, expansions and link workspaces
-        bytes += 3 * 16ull * nv * ne + 16ull * nv * 2048 + (64ull << 20) +
-                 16ull * nv * (num.fmm_restart + 2) + 16ull * nv + 8ull * (4 * (num.fmm_restart + 2) + 8) +
+    if (fmm) {
+        // operators of the point (near entries, leaf moments, expansions, link workspaces,
+        // audit / verification / comparison operators): the largest measured on this mesh
+        // with these settings, else an estimate of about 200 KB per node (measured: 166 KB
+        // per node on a 10k-node horn, 203 KB on a 59k-node flanged horn at 16 kHz)
+        const std::uint64_t measured = const_cast<MeshSolver *>(this)->fmm_point_bytes(num);
+        bytes += measured ? measured + measured / 10 : 16ull * nv * 12288 + (64ull << 20);
+        bytes += 16ull * nv * (num.fmm_restart + 2) + 16ull * nv + 8ull * (4 * (num.fmm_restart + 2) + 8) +
                  8 * (num.fmm_block_width == 64 ? dev::jacobi_factor_doubles(nv)
                                                 : 2 * dev::jacobi_wide_doubles(nv, num.fmm_block_width));
+    }
     else if (lu)
         bytes += 16 * nv2 + 4ull * nv;
     else
@@ -996,6 +1068,13 @@ std::shared_ptr<PreparedRequest> MeshSolver::prepare(const Request &req) const {
     if (nobs)
         bytes += 24 * nobs + 16 * nobs * ne + 8 * dev::field_partial_doubles(nt, static_cast<std::uint32_t>(nobs), ne);
     P->footprint = bytes;
+    // resident plan: touching jobs (~13 per panel, 16 B), near CSR and patches, cluster
+    // tiles (8 B per cluster pair); the FMM near structure adds its job lists (about
+    // 27 leaves x leaf capacity near panels per panel, 16 B each)
+    const std::uint64_t clusters = nt / 48 + 1;
+    P->plan_estimate = 16ull * 16 * nt + 8 * clusters * clusters + 8ull * nt * 16;
+    if (fmm)
+        P->plan_estimate += 16ull * nt * 27 * num.fmm_leaf_capacity;
     return P;
 }
 

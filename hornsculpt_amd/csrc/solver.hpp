@@ -246,7 +246,16 @@ struct DevicePlan {
     DeviceBuffer<std::uint32_t> tiles;
     std::vector<double> far_tile_edge; // ascending, one per far tile
     std::uint32_t tile_count{};
+    std::uint64_t last_use{}; // plan cache recency (MeshSolver::plans_)
+    std::uint64_t device_bytes() const {
+        return touching.capacity_bytes() + near.capacity_bytes() + near_items.capacity_bytes() +
+               patches.capacity_bytes() + near_offsets.capacity_bytes() + near_trials.capacity_bytes() +
+               whole.capacity_bytes() + gauss.capacity_bytes() + tiles.capacity_bytes();
+    }
 };
+
+// Plans kept per mesh (least recently used evicted beyond this, unless a solve holds it).
+inline constexpr std::size_t kPlanCacheEntries = 4;
 
 // Bounds of one spatial cluster of panels (for conservative tile tests).
 struct ClusterBounds {
@@ -269,7 +278,8 @@ struct PreparedRequest {
     std::uint32_t nd{}, ne{}, nv{}, nt{};
     double k_max{};
     bool has_nonzero{};
-    std::uint64_t footprint{}; // device bytes a workspace holds for one frequency point
+    std::uint64_t footprint{};     // device bytes a workspace holds for one frequency point
+    std::uint64_t plan_estimate{}; // device bytes of this request's interaction plan when first built
 };
 
 // Exact near part of the FMM operator split (pressure CSR over P1 x P1 and
@@ -307,6 +317,18 @@ class MeshSolver {
     std::unique_ptr<FmmOperator> fmm_operator(const Numerics &numerics, double k_max, double k,
                                               const FmmSettings &settings, const std::vector<std::uint32_t> &active_faces);
     std::uint64_t hash() const { return hash_; }
+    // Device bytes this mesh keeps resident between solves: geometry, clusters,
+    // cached interaction plans and the cached FMM near structure.
+    std::uint64_t resident_bytes();
+    // Device bytes the plan of a request will add when it is not cached yet (0 when cached).
+    std::uint64_t pending_plan_bytes(const PreparedRequest &prepared);
+    std::size_t cached_plans();
+    // Drops cached interaction plans no solve holds, least recently used first, until at
+    // most `keep` remain; returns the device bytes released.
+    std::uint64_t trim_plans(std::size_t keep);
+    // Largest device footprint measured for one FMM point of this mesh with these settings (0: none yet).
+    std::uint64_t fmm_point_bytes(const Numerics &numerics);
+    void note_fmm_point_bytes(const Numerics &numerics, std::uint64_t bytes);
     const Space &space() const { return space_; }
     std::uint32_t nv() const { return static_cast<std::uint32_t>(space_.vertices.size()); }
     std::uint32_t nt() const { return static_cast<std::uint32_t>(space_.panels.size()); }
@@ -350,7 +372,12 @@ class MeshSolver {
     std::uint32_t cluster_count_{};
     std::vector<ClusterBounds> cluster_bounds_;
     std::map<std::tuple<unsigned, unsigned, double, double, unsigned, double>, std::shared_ptr<DevicePlan>> plans_;
-    std::mutex plan_mutex_;
+    std::mutex plan_mutex_;       // serialises plan and near-structure builds
+    std::mutex plan_index_mutex_; // guards plans_ lookups, insertion and eviction only
+    std::uint64_t plan_clock_{};
+    std::atomic<std::uint64_t> plan_bytes_{0}, near_bytes_{0};
+    std::uint64_t geometry_bytes_{};
+    std::map<std::tuple<unsigned, double, unsigned, unsigned, unsigned>, std::uint64_t> fmm_point_bytes_;
     std::unique_ptr<Workspace> workspace_; // direct solve path (context stream), reused across calls
     DeviceBuffer<std::uint32_t> original_nodes_; // 3 x nt, mesh vertex ids (near operator)
     DeviceBuffer<double> original_row_scale_, original_lumped_;

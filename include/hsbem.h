@@ -127,18 +127,28 @@ HSBEM_API int hsbem_context_create(int32_t device, hsbem_context **context);
 HSBEM_API int hsbem_context_destroy(hsbem_context *context);
 
 /* Copies, recentres (vertex 0 at the origin) and validates the mesh (closed,
- * consistently oriented, positive volume per component), builds the Galerkin
- * space and uploads device-resident panel geometry. */
+ * consistently oriented, positive volume per component, no self-intersection,
+ * no nested component), builds the Galerkin space and uploads device-resident
+ * panel geometry. Replaces CudaBemSolver(SurfaceMesh) (bem.hpp:369) with the
+ * checks of validatedSpace (cuda_solver.cu:782-789). */
 HSBEM_API int hsbem_mesh_create(hsbem_context *context, const double *vertices, uint32_t vertex_count,
                                 const uint32_t *triangles, const uint32_t *face_ids, const uint8_t *boundary,
                                 uint32_t triangle_count, hsbem_mesh **mesh);
 HSBEM_API int hsbem_mesh_destroy(hsbem_mesh *mesh);
+/* The validation of hsbem_mesh_create on the host only (no device, no context):
+ * HSBEM_OK, or HSBEM_INVALID_ARGUMENT with the reason in hsbem_last_error(). */
+HSBEM_API int hsbem_mesh_validate(const double *vertices, uint32_t vertex_count, const uint32_t *triangles,
+                                  uint32_t triangle_count);
 /* 64-bit FNV-1a hash of the mesh arrays (state key for the serving loop). */
 HSBEM_API int hsbem_mesh_info(const hsbem_mesh *mesh, uint64_t *hash, uint32_t *vertex_count,
                               uint32_t *triangle_count, double *surface_area);
 
 /* Solves every frequency of the request. cancel (optional) is polled between
  * launches; a nonzero value ends the call with HSBEM_CANCELLED. */
+/* Device state the mesh keeps between solves: the number of cached interaction
+ * plans (at most 4 not held by a running solve; least recently used evicted) and
+ * the resident device bytes (geometry, cached plans, FMM near structure). */
+HSBEM_API int hsbem_mesh_cache_info(hsbem_mesh *mesh, uint64_t *cached_plans, uint64_t *resident_bytes);
 HSBEM_API int hsbem_solve(hsbem_mesh *mesh, const hsbem_request *request, hsbem_progress_fn progress, void *user,
                           const volatile int32_t *cancel, hsbem_result **result);
 HSBEM_API int hsbem_result_destroy(hsbem_result *result);
@@ -198,6 +208,10 @@ typedef struct {
     uint64_t submitted, completed, failed, points;
     uint32_t running, queued, max_running;
     uint64_t budget_bytes, held_bytes, peak_held_bytes;
+    /* device bytes the served meshes keep between points (geometry, cached plans,
+     * FMM near structures) and the largest held + resident total at an admission;
+     * both count against budget_bytes */
+    uint64_t resident_bytes, peak_total_bytes;
 } hsbem_server_statistics;
 
 HSBEM_API int hsbem_server_create(hsbem_context *context, const hsbem_server_options *options, hsbem_server **server);

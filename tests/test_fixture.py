@@ -75,6 +75,49 @@ def test_refusals(change, message):
         bem.read_benchmark_fixture(json.dumps(root))
 
 
+def _without(key):
+    _, root = _fixture()
+    del root[key]
+    return root
+
+
+@pytest.mark.parametrize("root,message", [
+    (_fixture(rho=float("nan"))[1], "invalid scalar"),
+    (_fixture(c=True)[1], "invalid scalar"),  # a JSON bool is not a number
+    (_without("rho"), "invalid scalar"),
+    (_without("observations_m"), "Missing fixture array"),  # the reference requires the array
+    (_without("vertices_m"), "Missing fixture array"),
+    (_fixture(tags="1")[1], "Missing fixture array"),
+    (_fixture(observations_m=[[0.0, 0.0]])[1], "three coordinates"),
+    (_fixture(observations_m=[[0.0, float("inf"), 1.0]])[1], "Invalid position coordinate"),
+    (_fixture(vertices_m=[[0.0, 0.0, "x"]])[1], "Invalid position coordinate"),
+    (_fixture(frequencies_hz=[True])[1], "Frequency must be positive"),
+    (_fixture(frequencies_hz=[-5.0])[1], "Frequency must be positive"),
+    (_fixture(driven_tags=[1.5])[1], "Invalid unsigned fixture index"),
+    (_fixture(driven_tags=[-1])[1], "Invalid unsigned fixture index"),
+    (_fixture(excitations=[{"name": "x", "face_velocities": [{"face_id": 1, "real": 1.0}]}])[1],
+     "invalid scalar"),
+    (_fixture(excitations=[{"name": "x", "face_velocities": [{"face_id": 2.5, "real": 1.0, "imag": 0.0}]}])[1],
+     "Invalid unsigned fixture index"),
+    (_fixture(excitations=[{"name": "x"}])[1], "Missing fixture array"),
+])
+def test_field_refusals(root, message):
+    with pytest.raises(ValueError, match=message):
+        bem.read_benchmark_fixture(json.dumps(root))
+
+
+def test_fractional_and_negative_tags_refused():
+    for bad in (1.5, -2):
+        _, root = _fixture()
+        root["tags"][0] = bad
+        with pytest.raises(ValueError, match="Invalid unsigned fixture index"):
+            bem.read_benchmark_fixture(json.dumps(root))
+    _, root = _fixture()
+    root["triangles0"][2] = [0, 1.25, 2]
+    with pytest.raises(ValueError, match="Invalid unsigned fixture index"):
+        bem.read_benchmark_fixture(json.dumps(root))
+
+
 def test_out_of_range_index_refused():
     _, root = _fixture()
     root["triangles0"][3] = [0, 1, 10 ** 6]

@@ -145,3 +145,63 @@ def test_progressive_sweep_equals_sweep(solvers):
     print(f"progressive order {[i for i, _ in done]}: worst vs sweep {worst:.1e}")
     assert [i for i, _ in done] == [5, 0, 2, 3, 4, 1]
     assert worst <= 1e-10
+
+
+def test_fmm_and_dense_lu_under_one_point_budget():
+    """A fast-multipole request and a dense LU request on the same mesh, served under
+    a budget that holds the resident mesh state and one point at a time: the slot
+    that served the FMM point must not reuse its workspace for the LU point (its A
+    and LU factors are larger), held + resident bytes stay within the budget, and
+    both results equal the requests solved alone."""
+    mesh = meshes.exponential_horn(0.012)
+    driven = sorted(set(int(f) for f, b in zip(mesh.face_ids, mesh.boundary) if b == bem.DRIVEN))
+
+    def request(numerics):
+        return bem.BemRequest(medium=bem.AcousticMedium(1.205, 343.0), frequencies=[800.0],
+                              excitations=[bem.BemExcitation("piston", [bem.FaceVelocity(f, 1.0) for f in driven])],
+                              numerics=numerics)
+
+    fmm = request(bem.BemNumerics(solver=bem.SOLVER_GMRES, relative_residual_tolerance=1e-6,
+                                  acceleration=bem.ACCELERATION_FAST_MULTIPOLE))
+    lu = request(bem.BemNumerics(relative_residual_tolerance=1e-6))
+    solver = bem.HipBemSolver(mesh)
+    alone = [solver.solve(fmm), solver.solve(lu)]
+    probe = bem.HipBemServer(slots=1)  # the served footprint of each point (FMM: as measured by the solves)
+    footprints = [probe.poll(probe.submit(solver, r)).peak_device_bytes for r in (fmm, lu)]
+    probe.close()
+    resident = solver.cache_info()["resident_bytes"]
+    budget = resident + max(footprints) + min(footprints) // 2
+    server = bem.HipBemServer(slots=2, memory_budget_bytes=budget)
+    tickets = [server.submit(solver, fmm), server.submit(solver, lu), server.submit(solver, fmm)]
+    served = [server.poll(t) for t in tickets]
+    stats = server.stats()
+    server.close()
+    print(f"budget {budget}, resident {stats['resident_bytes']}, peak held {stats['peak_held_bytes']}, "
+          f"peak total {stats['peak_total_bytes']}, footprints {footprints}")
+    assert stats["failed"] == 0 and stats["max_running"] == 1
+    assert stats["peak_total_bytes"] <= budget
+    _compare(alone[0], served[0], 1e-8)
+    _compare(alone[1], served[1], 1e-10)
+    _compare(alone[0], served[2], 1e-8)
+
+
+def test_plan_cache_stays_bounded_over_many_rasters(solvers):
+    """Each distinct planning raster builds an interaction plan; the mesh keeps at
+    most four that no solve holds, so serving many rasters keeps the plan count and
+    the resident bytes bounded."""
+    ico = solvers["ico"]
+    server = bem.HipBemServer(slots=2)
+    counts, resident = [], []
+    for i in range(7):
+        req = sphere_request([400.0 + 10.0 * i])
+        req.interaction_plan_frequencies = [400.0 + 10.0 * i, 1500.0 + 100.0 * i]
+        server.poll(server.submit(ico, req))
+        info = ico.cache_info()
+        counts.append(info["cached_plans"])
+        resident.append(info["resident_bytes"])
+    stats = server.stats()
+    server.close()
+    print("cached plans", counts, "resident bytes", resident)
+    assert max(counts) <= 4 and counts[-1] == 4
+    assert resident[-1] <= max(resident[3:5]) * 1.05  # no growth once the cache is full
+    assert stats["peak_total_bytes"] <= stats["budget_bytes"]
