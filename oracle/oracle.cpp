@@ -363,19 +363,30 @@ struct Problem {
     std::vector<double> row_scale;
 };
 
-void scatter(const Problem &pr, const Moments &m, std::uint32_t ti, std::uint32_t si, double k, C *A, C *B) {
+// Output rows: the whole matrix (map == nullptr, leading dimension np) or a
+// compact set of rows (map[row] = position or -1, leading dimension ld).
+struct RowSink {
+    const std::int64_t *map;
+    std::size_t ld;
+    std::int64_t at(std::uint32_t row) const { return map ? map[row] : std::int64_t(row); }
+};
+
+void scatter(const Problem &pr, const Moments &m, std::uint32_t ti, std::uint32_t si, double k, C *A, C *B,
+             const RowSink &sink) {
     const Space &sp = pr.space;
     const Panel &t = sp.panels[ti], &s = sp.panels[si];
-    const std::size_t np = sp.vertices.size();
     for (int i = 0; i < 3; ++i) {
         const std::uint32_t row = t.n[i];
+        const std::int64_t at = sink.at(row);
+        if (at < 0)
+            continue;
         const double rs = pr.row_scale[row];
         for (int j = 0; j < 3; ++j) {
             const double cc = dot(t.curl[i], s.curl[j]);
             const C val = m.a[3 * i + j] + C(m.g.imag() * cc / k, -m.g.real() * cc / k); // -i G curl.curl / k
-            A[row + std::size_t(s.n[j]) * np] += val * rs;
+            A[at + std::size_t(s.n[j]) * sink.ld] += val * rs;
         }
-        B[row + std::size_t(si) * np] += m.b[i] * rs;
+        B[at + std::size_t(si) * sink.ld] += m.b[i] * rs;
     }
 }
 
@@ -418,7 +429,7 @@ template <class F> void parallel_for(std::size_t n, F &&f, unsigned threads) {
 // shared between panels, so accumulate per test panel into private buffers.
 void assemble(const Problem &pr, const PlanView &plan, double k, double tol, unsigned regular_order,
               unsigned singular_order, C *A, C *B, const std::uint32_t *test_subset, std::uint64_t n_subset,
-              unsigned threads) {
+              unsigned threads, const RowSink &sink) {
     const Space &sp = pr.space;
     const std::size_t np = sp.vertices.size(), nt = sp.panels.size();
     const auto gs = gauss01(singular_order);
@@ -558,16 +569,19 @@ void assemble(const Problem &pr, const PlanView &plan, double k, double tol, uns
         threads);
     for (std::size_t idx = 0; idx < tests.size(); ++idx)
         for (const auto &blk : results[idx])
-            scatter(pr, blk.m, tests[idx], blk.s, k, A, B);
+            scatter(pr, blk.m, tests[idx], blk.s, k, A, B, sink);
     // Galerkin mass: 1/2 M_pp in A and -alpha 1/2 M_pq = i A/(6k) in B.
     for (std::uint32_t ti : tests) {
         const Panel &t = sp.panels[ti];
         for (int i = 0; i < 3; ++i) {
             const std::uint32_t row = t.n[i];
+            const std::int64_t at = sink.at(row);
+            if (at < 0)
+                continue;
             const double rs = pr.row_scale[row];
             for (int j = 0; j < 3; ++j)
-                A[row + std::size_t(t.n[j]) * np] += rs * t.area * (i == j ? 2.0 : 1.0) / 24.0;
-            B[row + std::size_t(ti) * np] += C(0, t.area / (6 * k) * rs);
+                A[at + std::size_t(t.n[j]) * sink.ld] += rs * t.area * (i == j ? 2.0 : 1.0) / 24.0;
+            B[at + std::size_t(ti) * sink.ld] += C(0, t.area / (6 * k) * rs);
         }
     }
 }
@@ -895,7 +909,34 @@ int oracle_assemble(std::uint32_t nv, const double *xyz, std::uint32_t nt, const
         const Problem pr = make_problem(nv, xyz, nt, tri);
         const PlanView plan{touching, n_touching, near, n_near, patches, whole, nullptr, nullptr};
         assemble(pr, plan, k, tolerance, regular_order, singular_order, reinterpret_cast<C *>(A),
-                 reinterpret_cast<C *>(B), test_subset, n_subset, threads);
+                 reinterpret_cast<C *>(B), test_subset, n_subset, threads, RowSink{nullptr, nv});
+    });
+}
+
+// Rows `rows[0..n_rows)` of the same A and B, compact (column-major with leading
+// dimension n_rows): only the test panels that touch one of the rows are
+// integrated, and nothing of the full matrices is formed, so rows of meshes far
+// beyond a dense host matrix stay cheap.
+int oracle_assemble_rows(std::uint32_t nv, const double *xyz, std::uint32_t nt, const std::uint32_t *tri, double k,
+                         double tolerance, unsigned regular_order, unsigned singular_order,
+                         const std::uint32_t *touching, std::uint64_t n_touching, const std::uint32_t *near,
+                         std::uint64_t n_near, const double *patches, const std::uint8_t *whole,
+                         const std::uint32_t *rows, std::uint32_t n_rows, unsigned threads, double *A, double *B) {
+    return guarded([&] {
+        std::vector<std::int64_t> map(nv, -1);
+        for (std::uint32_t r = 0; r < n_rows; ++r) {
+            if (rows[r] >= nv || map[rows[r]] >= 0)
+                throw std::invalid_argument("rows must be distinct vertex indices");
+            map[rows[r]] = r;
+        }
+        std::vector<std::uint32_t> tests;
+        for (std::uint32_t t = 0; t < nt; ++t)
+            if (map[tri[3 * t]] >= 0 || map[tri[3 * t + 1]] >= 0 || map[tri[3 * t + 2]] >= 0)
+                tests.push_back(t);
+        const Problem pr = make_problem(nv, xyz, nt, tri);
+        const PlanView plan{touching, n_touching, near, n_near, patches, whole, nullptr, nullptr};
+        assemble(pr, plan, k, tolerance, regular_order, singular_order, reinterpret_cast<C *>(A),
+                 reinterpret_cast<C *>(B), tests.data(), tests.size(), threads, RowSink{map.data(), n_rows});
     });
 }
 

@@ -398,6 +398,33 @@ def oracle_assemble(mesh, plan, k, tolerance=1e-4, regular_order=4, singular_ord
     return (A.view(np.complex128).reshape(nv, nv).T.copy(), B.view(np.complex128).reshape(nt, nv).T.copy())
 
 
+def oracle_assemble_rows(mesh, plan, k, rows, tolerance=1e-4, regular_order=4, singular_order=4, threads=8,
+                         vertices=None):
+    """Rows ``rows`` of the oracle's row-scaled A (len(rows) x nv) and B (len(rows) x nt),
+    integrating only the test panels that touch those rows."""
+    lib = oracle()
+    V = np.ascontiguousarray(recentred(mesh) if vertices is None else vertices)
+    T = np.ascontiguousarray(mesh.triangles, dtype=np.uint32)
+    nv, nt = len(V), len(T)
+    R = np.ascontiguousarray(rows, dtype=np.uint32)
+    m = len(R)
+    A = np.zeros(2 * m * nv)
+    B = np.zeros(2 * m * nt)
+    touching = np.ascontiguousarray(plan["touching"], dtype=np.uint32)
+    near = np.ascontiguousarray(plan["near"], dtype=np.uint32)
+    patches = np.ascontiguousarray(plan["patches"], dtype=np.float64)
+    whole = plan["whole"]
+    lib.oracle_assemble_rows.argtypes = [ctypes.c_uint32, _D, ctypes.c_uint32, _U, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_uint, ctypes.c_uint, _U, ctypes.c_uint64, _U, ctypes.c_uint64, _D,
+                                         _U8, _U, ctypes.c_uint32, ctypes.c_uint, _D, _D]
+    rc = lib.oracle_assemble_rows(nv, _d(V), nt, _u(T), k, tolerance, regular_order, singular_order,
+                                  _u(touching), len(plan["touching"]), _u(near), len(plan["near"]),
+                                  _d(patches) if len(plan["patches"]) else None,
+                                  whole.ctypes.data_as(_U8) if len(whole) else None, _u(R), m, threads, _d(A), _d(B))
+    _check(lib, rc, "oracle_assemble_rows")
+    return (A.view(np.complex128).reshape(nv, m).T.copy(), B.view(np.complex128).reshape(nt, m).T.copy())
+
+
 def oracle_lu_solve(A, R, threads=8):
     lib = oracle()
     n = A.shape[0]

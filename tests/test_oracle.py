@@ -227,3 +227,19 @@ def test_oracle_lu_matches_numpy():
         B = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
         X = ob.oracle_lu_solve(A, B)
         assert np.max(np.abs(X - np.linalg.solve(A, B))) <= 1e-10 * np.max(np.abs(X))
+
+
+@pytest.mark.parametrize("frequency", [500.0, 2000.0])
+def test_row_mode_equals_full_assembly_rows(frequency):
+    """The compact row mode integrates the same pairs and scatters them in the same
+    order as the full assembly, so its rows are the full matrix's rows exactly."""
+    mesh = meshes.exponential_horn(0.02)
+    k = 2 * math.pi * frequency / 343.0
+    plan = bem.interaction_plan(mesh, bem.BemNumerics(), k)
+    rows = np.random.default_rng(int(frequency)).choice(len(mesh.vertices), 25, replace=False)
+    tests = np.unique([t for t, tri in enumerate(mesh.triangles) if set(tri) & set(rows.tolist())])
+    Ao, Bo = ob.oracle_assemble(mesh, plan, k, tests=tests)
+    Ar, Br = ob.oracle_assemble_rows(mesh, plan, k, rows)
+    assert np.array_equal(Ar, Ao[rows]) and np.array_equal(Br, Bo[rows])
+    with pytest.raises(RuntimeError):
+        ob.oracle_assemble_rows(mesh, plan, k, [3, 3])
