@@ -4,343 +4,368 @@
 
 #include <algorithm>
 #include <cmath>
-#include <limits>
 #include <numbers>
 #include <stdexcept>
+#include <string>
 
 namespace hsbem {
 namespace {
 
 using C = std::complex<double>;
+constexpr double kPi = std::numbers::pi;
 
 std::size_t lm(unsigned l, int m) { return static_cast<std::size_t>(l) * l + l + m; }
 
-std::size_t coefficient_count(unsigned order) {
+std::size_t waves_for(unsigned order) {
     if (order > kMaximumSphericalOrder)
-        throw std::invalid_argument("Scaled spherical reference order exceeds its bounded CPU contract.");
+        throw std::invalid_argument("scaled expansion order " + std::to_string(order) +
+                                    " exceeds the supported maximum of " + std::to_string(kMaximumSphericalOrder));
     return static_cast<std::size_t>(order + 1) * (order + 1);
 }
 
-double component(Vec3 v, unsigned i) { return i == 0 ? v.x : i == 1 ? v.y : v.z; }
-
-void check_domain(double k, double a) {
+void require_scale(double k, double a) {
     if (!std::isfinite(k) || k < 0 || !std::isfinite(a) || a <= 0 || k * a > kMaximumSphericalElectricalRadius)
-        throw std::invalid_argument("Scaled spherical waves require k>=0, a>0 and supported ka.");
+        throw std::invalid_argument("scaled waves need k >= 0, a finite scale a > 0 and k a within the supported range");
 }
 
-void check_expansion(unsigned order, double a, const std::vector<C> &c, double k) {
-    check_domain(k, a);
-    if (c.size() != coefficient_count(order))
-        throw std::invalid_argument("Scaled spherical expansion has an inconsistent coefficient count.");
-    for (const C &v : c)
-        if (!std::isfinite(v.real()) || !std::isfinite(v.imag()))
-            throw std::invalid_argument("Scaled spherical expansion contains a non-finite coefficient.");
+bool finite(C v) { return std::isfinite(v.real()) && std::isfinite(v.imag()); }
+
+void require_expansion(const ScaledExpansion &e, double k) {
+    require_scale(k, e.radius);
+    if (e.coefficients.size() != waves_for(e.order))
+        throw std::invalid_argument("scaled expansion coefficient count does not match its order");
+    for (const C &v : e.coefficients)
+        if (!finite(v))
+            throw std::invalid_argument("scaled expansion coefficients must be finite");
 }
 
-// product rule a*b with gradients
-WaveValue product(const WaveValue &a, const WaveValue &b) {
-    WaveValue r;
-    r.value = a.value * b.value;
-    for (unsigned i = 0; i < 3; ++i)
-        r.gradient[i] = a.gradient[i] * b.value + a.value * b.gradient[i];
-    return r;
-}
+double axis(Vec3 v, unsigned i) { return i == 0 ? v.x : i == 1 ? v.y : v.z; }
 
-WaveValue scaled(WaveValue a, double f) {
-    a.value *= f;
-    for (C &g : a.gradient)
-        g *= f;
-    return a;
-}
+// Regular solid harmonics S_lm(p) = |p|^l Y_lm(p/|p|) (orthonormal Y, Condon-Shortley
+// phase) and their gradients. Values by the Cartesian form of the Legendre
+// recurrences (sectoral step with x + iy, first off-diagonal with z, three-term step
+// with z and |p|^2); gradients from the degree-lowering ladder identities
+//   d/dz S_lm = c_l sqrt((l-m)(l+m)) S_{l-1,m},
+//   (d/dx + i d/dy) S_lm = c_l sqrt((l-m)(l-m-1)) S_{l-1,m+1},
+//   (d/dx - i d/dy) S_lm = -c_l sqrt((l+m)(l+m-1)) S_{l-1,m-1},   c_l = sqrt((2l+1)/(2l-1)),
+// which hold for every m and need no division by |p| (finite at the origin).
+struct SolidHarmonics {
+    std::vector<C> value;
+    std::vector<std::array<C, 3>> gradient;
+};
 
-WaveValue difference(WaveValue a, const WaveValue &b) {
-    a.value -= b.value;
-    for (unsigned i = 0; i < 3; ++i)
-        a.gradient[i] -= b.gradient[i];
-    return a;
-}
-
-// Normalised solid harmonics |u|^l Y_lm(u/|u|) as polynomials in (x + iy, z, r^2):
-// the Y_lm recursions with mu -> z, rho e^{i phi} -> x + iy, and the two-term step
-// carrying r^2; gradients by the product rule, finite at the poles and the origin.
-std::vector<WaveValue> solid_harmonics(unsigned order, Vec3 u) {
-    std::vector<WaveValue> out(coefficient_count(order));
-    const WaveValue xy{{u.x, u.y}, {C{1, 0}, C{0, 1}, C{}}}, z{{u.z, 0}, {C{}, C{}, C{1, 0}}},
-        r2{{u.x * u.x + u.y * u.y + u.z * u.z, 0}, {C{2 * u.x, 0}, C{2 * u.y, 0}, C{2 * u.z, 0}}};
-    WaveValue diag;
-    diag.value = 1 / std::sqrt(4 * std::numbers::pi);
+SolidHarmonics solid_harmonics(unsigned order, Vec3 p) {
+    const std::size_t n = waves_for(order);
+    SolidHarmonics s{std::vector<C>(n), std::vector<std::array<C, 3>>(n)};
+    const C w{p.x, p.y};
+    const double z = p.z, r2 = p.x * p.x + p.y * p.y + p.z * p.z;
+    C sectoral{1 / std::sqrt(4 * kPi), 0};
     for (unsigned m = 0; m <= order; ++m) {
         const int im = static_cast<int>(m);
         if (m)
-            diag = scaled(product(xy, diag), -std::sqrt((2. * m + 1) / (2. * m)));
-        out[lm(m, im)] = diag;
+            sectoral *= -std::sqrt((2.0 * m + 1) / (2.0 * m)) * w;
+        s.value[lm(m, im)] = sectoral;
         if (m < order)
-            out[lm(m + 1, im)] = scaled(product(z, diag), std::sqrt(2. * m + 3));
+            s.value[lm(m + 1, im)] = std::sqrt(2.0 * m + 3) * z * sectoral;
+        const double mm = static_cast<double>(m) * m;
         for (unsigned l = m + 2; l <= order; ++l) {
-            const double den = static_cast<double>(l) * l - static_cast<double>(m) * m,
-                         a = std::sqrt((4. * l * l - 1) / den),
-                         b = std::sqrt((2. * l + 1) * ((l - 1.) * (l - 1.) - static_cast<double>(m) * m) /
-                                       ((2. * l - 3) * den));
-            out[lm(l, im)] = difference(scaled(product(z, out[lm(l - 1, im)]), a),
-                                        scaled(product(r2, out[lm(l - 2, im)]), b));
+            const double ll = static_cast<double>(l) * l, span = ll - mm;
+            const double a = std::sqrt((4 * ll - 1) / span);
+            const double b = std::sqrt(((l - 1.0) * (l - 1.0) - mm) * (2.0 * l + 1) / (span * (2.0 * l - 3)));
+            s.value[lm(l, im)] = a * z * s.value[lm(l - 1, im)] - b * r2 * s.value[lm(l - 2, im)];
         }
-        if (m)
-            for (unsigned l = m; l <= order; ++l) {
-                WaveValue neg = out[lm(l, im)];
-                const double sign = m % 2 ? -1. : 1.;
-                neg.value = sign * std::conj(neg.value);
-                for (C &g : neg.gradient)
-                    g = sign * std::conj(g);
-                out[lm(l, -im)] = neg;
-            }
+        if (m) {
+            const double sign = m % 2 ? -1.0 : 1.0;
+            for (unsigned l = m; l <= order; ++l)
+                s.value[lm(l, -im)] = sign * std::conj(s.value[lm(l, im)]);
+        }
+    }
+    const C i{0, 1};
+    for (unsigned l = 1; l <= order; ++l) {
+        const double cl = std::sqrt((2.0 * l + 1) / (2.0 * l - 1));
+        const int il = static_cast<int>(l);
+        for (int m = -il; m <= il; ++m) {
+            auto below = [&](int mm) { return std::abs(mm) <= il - 1 ? s.value[lm(l - 1, mm)] : C{}; };
+            const C dz = cl * std::sqrt(static_cast<double>(il - m) * (il + m)) * below(m);
+            const C up = cl * std::sqrt(std::max(0.0, static_cast<double>(il - m) * (il - m - 1))) * below(m + 1);
+            const C down = -cl * std::sqrt(std::max(0.0, static_cast<double>(il + m) * (il + m - 1))) * below(m - 1);
+            s.gradient[lm(l, m)] = {0.5 * (up + down), -0.5 * i * (up - down), dz};
+        }
+    }
+    return s;
+}
+
+// Jhat_l(z) = (2l+1)!! j_l(z) / z^l, l = 0..order. In this normalisation the Bessel
+// recurrence reads Jhat_{l-1} = Jhat_l - z^2 / ((2l+1)(2l+3)) Jhat_{l+1}; run downward
+// (Miller) its solution converges to Jhat, whose scale follows from
+// Jhat_0 = sin z / z or Jhat_1 = 3 j_1(z) / z, whichever j is larger in magnitude.
+std::vector<double> regular_profile(unsigned order, double z) {
+    if (!std::isfinite(z) || z < 0 || z > 2 * kMaximumSphericalElectricalRadius)
+        throw std::invalid_argument("scaled regular waves: k r is outside the supported range");
+    std::vector<double> out(order + 1, 1.0);
+    if (z == 0)
+        return out;
+    const double reach = std::max(static_cast<double>(order), z);
+    const unsigned top = order + 2 + static_cast<unsigned>(z + 30 + 10 * std::cbrt(reach));
+    std::vector<double> g(top + 2, 0.0);
+    g[top] = 1;
+    for (unsigned l = top; l > 0; --l)
+        g[l - 1] = g[l] - z * z / ((2.0 * l + 1) * (2.0 * l + 3)) * g[l + 1];
+    const double j0 = std::sin(z) / z, j1 = (j0 - std::cos(z)) / z;
+    const double scale = std::abs(j0) >= std::abs(j1) ? j0 / g[0] : 3 * j1 / z / g[1];
+    for (unsigned l = 0; l <= order; ++l)
+        out[l] = scale * g[l];
+    return out;
+}
+
+// f_l(r) = (a/r)^(l+1) Hhat_l(kr) = -i (ka)^(l+1) h_l^(2)(kr) / (2l-1)!!, l = 0..order,
+// with Hhat_l -> 1 as kr -> 0, by the upward recurrence of h_l (stable for h):
+// f_0 = t e^{-ikr}, f_1 = t (1 + ikr) f_0, f_{l+1} = t f_l - (ka)^2 / ((2l-1)(2l+1)) f_{l-1},
+// t = a/r; and f_l' from the degree-lowering identity h_l' = h_{l-1} - (l+1) h_l / (kr):
+// f_0' = -f_1 / a, f_l' = k^2 a / (2l-1) f_{l-1} - (l+1) f_l / r.
+void outgoing_profile(unsigned order, double k, double a, double r, std::vector<C> &f, std::vector<C> &df) {
+    const double t = a / r, ka = k * a;
+    f.assign(order + 2, C{});
+    df.assign(order + 1, C{});
+    f[0] = t * std::polar(1.0, -k * r);
+    f[1] = t * C{1, k * r} * f[0];
+    for (unsigned l = 1; l <= order; ++l)
+        f[l + 1] = t * f[l] - ka * ka / ((2.0 * l - 1) * (2.0 * l + 1)) * f[l - 1];
+    df[0] = -f[1] / a;
+    for (unsigned l = 1; l <= order; ++l)
+        df[l] = k * k * a / (2.0 * l - 1) * f[l - 1] - (l + 1.0) / r * f[l];
+}
+
+WaveValue combine(const std::vector<WaveValue> &basis, const std::vector<C> &coefficients) {
+    WaveValue out;
+    for (std::size_t j = 0; j < basis.size(); ++j) {
+        out.value += coefficients[j] * basis[j].value;
+        for (unsigned d = 0; d < 3; ++d)
+            out.gradient[d] += coefficients[j] * basis[j].gradient[d];
     }
     return out;
 }
 
-// Jhat_l(z) = (2l+1)!! j_l(z) / z^l, l = 0..order: the power series while its first
-// correction z^2 / (2 (2l+3)) is at most one (alternating, decreasing terms), the
-// j_l table divided by z^l / (2l+1)!! beyond.
-std::vector<double> normalized_regular(unsigned order, double z) {
-    if (!std::isfinite(z) || z < 0 || z > 2 * kMaximumSphericalElectricalRadius)
-        throw std::invalid_argument("Scaled regular reference projection exceeds its stable LF domain.");
-    std::vector<double> out(order + 1);
-    const RegularRadialTable table = z > 1 ? regular_radial_table(order, z) : RegularRadialTable{};
-    double lead = 1;
+// Multiplies degree l of an expansion by (ka)^l / (2l+1)!! (the scaled <-> unscaled
+// regular normalisation).
+void scale_degrees(std::vector<C> &c, unsigned order, double ka) {
+    double f = 1;
     for (unsigned l = 0; l <= order; ++l) {
         if (l)
-            lead *= z / (2. * l + 1);
-        if (z > 1 && z * z > 2 * (2. * l + 3)) {
-            out[l] = table.values[l] / lead;
-            continue;
-        }
-        double term = 1, sum = 1;
-        for (unsigned n = 1; n < 128; ++n) {
-            term *= -z * z / (2. * n * (2. * l + 2 * n + 1));
-            sum += term;
-            if (std::abs(term) < std::numeric_limits<double>::epsilon() * std::abs(sum))
-                break;
-        }
-        out[l] = sum;
+            f *= ka / (2.0 * l + 1);
+        for (int m = -static_cast<int>(l); m <= static_cast<int>(l); ++m)
+            c[lm(l, m)] *= f;
     }
-    return out;
-}
-
-void accumulate(WaveValue &out, const WaveValue &basis, C c) {
-    out.value += c * basis.value;
-    for (unsigned i = 0; i < 3; ++i)
-        out.gradient[i] += c * basis.gradient[i];
-}
-
-// (a/r)^(l+1) Hhat_l(kr), l = 0..order+1, with t = a/r:
-// O_0 = t e^{-ikr}, O_1 = t (1 + ikr) O_0, O_{l+1} = t O_l - (ka)^2 / ((2l-1)(2l+1)) O_{l-1}
-std::vector<C> outgoing_radial(unsigned order, double k, double a, double r) {
-    const double t = a / r, ka = k * a;
-    const C i{0, 1};
-    std::vector<C> out(order + 2);
-    out[0] = t * std::exp(-i * (k * r));
-    out[1] = t * (1.0 + i * (k * r)) * out[0];
-    for (unsigned l = 1; l <= order; ++l)
-        out[l + 1] = t * out[l] - ka * ka / ((2. * l - 1) * (2. * l + 1)) * out[l - 1];
-    return out;
 }
 
 } // namespace
 
 std::vector<WaveValue> scaled_regular_waves(unsigned order, double k, double a, Vec3 p) {
-    check_domain(k, a);
-    const auto solid = solid_harmonics(order, vscale(p, 1 / a));
-    const auto radial = normalized_regular(order + 1, k * std::hypot(p.x, p.y, p.z));
-    std::vector<WaveValue> out(solid.size());
-    for (unsigned l = 0; l <= order; ++l)
+    require_scale(k, a);
+    const SolidHarmonics s = solid_harmonics(order, vscale(p, 1 / a));
+    const std::vector<double> J = regular_profile(order + 1, k * std::hypot(p.x, p.y, p.z));
+    std::vector<WaveValue> out(s.value.size());
+    // grad [Jhat_l(k|p|) S_lm(p/a)] = Jhat_l grad S / a - k^2 p Jhat_{l+1} / (2l+3) S
+    for (unsigned l = 0; l <= order; ++l) {
+        const double radial = -k * k * J[l + 1] / (2.0 * l + 3);
         for (int m = -static_cast<int>(l); m <= static_cast<int>(l); ++m) {
             const std::size_t j = lm(l, m);
-            out[j].value = radial[l] * solid[j].value;
-            for (unsigned i = 0; i < 3; ++i)
-                out[j].gradient[i] = radial[l] * solid[j].gradient[i] / a -
-                                     (k * k * component(p, i) / (2. * l + 3)) * radial[l + 1] * solid[j].value;
+            out[j].value = J[l] * s.value[j];
+            for (unsigned d = 0; d < 3; ++d)
+                out[j].gradient[d] = J[l] / a * s.gradient[j][d] + radial * axis(p, d) * s.value[j];
         }
+    }
     return out;
 }
 
 std::vector<WaveValue> scaled_outgoing_waves(unsigned order, double k, double a, Vec3 p) {
-    check_domain(k, a);
+    require_scale(k, a);
     const double r = std::hypot(p.x, p.y, p.z);
     if (!std::isfinite(r) || r <= 0)
-        throw std::invalid_argument("Scaled outgoing basis requires a finite point outside its singular center.");
+        throw std::invalid_argument("scaled outgoing waves are singular at their centre: need a finite point r > 0");
     const Vec3 u = vscale(p, 1 / r);
-    const auto solid = solid_harmonics(order, u);
-    const auto radial = outgoing_radial(order, k, a, r);
-    std::vector<WaveValue> out(coefficient_count(order));
+    // On the unit sphere S_lm(u) = Y_lm(u); grad_p Y_lm(p/|p|) = (grad S_lm(u) - l u S_lm(u)) / r.
+    const SolidHarmonics s = solid_harmonics(order, u);
+    std::vector<C> f, df;
+    outgoing_profile(order, k, a, r, f, df);
+    std::vector<WaveValue> out(s.value.size());
     for (unsigned l = 0; l <= order; ++l) {
-        const C dr = (l / r) * radial[l] - (2. * l + 1) / a * radial[l + 1];
-        const double norm = 1 / (a * (2. * l + 1));
+        const double norm = 1 / (a * (2.0 * l + 1));
+        const C value = norm * f[l], radial = norm * df[l], angular = value / r;
         for (int m = -static_cast<int>(l); m <= static_cast<int>(l); ++m) {
             const std::size_t j = lm(l, m);
-            WaveValue w;
-            w.value = norm * radial[l] * solid[j].value;
-            for (unsigned i = 0; i < 3; ++i)
-                w.gradient[i] = norm * (dr * component(u, i) * solid[j].value +
-                                        radial[l] / r * (solid[j].gradient[i] - l * component(u, i) * solid[j].value));
-            out[j] = w;
+            out[j].value = value * s.value[j];
+            for (unsigned d = 0; d < 3; ++d)
+                out[j].gradient[d] = radial * axis(u, d) * s.value[j] +
+                                     angular * (s.gradient[j][d] - static_cast<double>(l) * axis(u, d) * s.value[j]);
         }
     }
     return out;
 }
 
-WaveValue evaluate_scaled_multipole(const ScaledExpansion &s, double k, Vec3 p) {
-    check_expansion(s.order, s.radius, s.coefficients, k);
-    const auto basis = scaled_outgoing_waves(s.order, k, s.radius, p);
-    WaveValue out;
-    for (std::size_t j = 0; j < basis.size(); ++j)
-        accumulate(out, basis[j], s.coefficients[j]);
-    return out;
+WaveValue evaluate_scaled_multipole(const ScaledExpansion &e, double k, Vec3 p) {
+    require_expansion(e, k);
+    return combine(scaled_outgoing_waves(e.order, k, e.radius, p), e.coefficients);
 }
 
-WaveValue evaluate_scaled_local(const ScaledExpansion &s, double k, Vec3 p) {
-    check_expansion(s.order, s.radius, s.coefficients, k);
-    const auto basis = scaled_regular_waves(s.order, k, s.radius, p);
-    WaveValue out;
-    for (std::size_t j = 0; j < basis.size(); ++j)
-        accumulate(out, basis[j], s.coefficients[j]);
-    return out;
+WaveValue evaluate_scaled_local(const ScaledExpansion &e, double k, Vec3 p) {
+    require_expansion(e, k);
+    return combine(scaled_regular_waves(e.order, k, e.radius, p), e.coefficients);
 }
 
-std::vector<C> scaled_multipole_to_spectrum(const ScaledExpansion &s, double k, const SphereRule &rule) {
-    check_expansion(s.order, s.radius, s.coefficients, k);
-    auto c = s.coefficients;
-    double f = 1; // (ka)^l / (2l+1)!!
-    for (unsigned l = 0; l <= s.order; ++l) {
-        if (l)
-            f *= k * s.radius / (2. * l + 1);
-        for (int m = -static_cast<int>(l); m <= static_cast<int>(l); ++m)
-            c[lm(l, m)] *= f;
-    }
-    return multipole_to_spectrum(c, s.order, rule);
+std::vector<C> scaled_multipole_to_spectrum(const ScaledExpansion &e, double k, const SphereRule &rule) {
+    require_expansion(e, k);
+    std::vector<C> c = e.coefficients;
+    scale_degrees(c, e.order, k * e.radius);
+    return multipole_to_spectrum(c, e.order, rule);
 }
 
 ScaledExpansion spectrum_to_scaled_local(const std::vector<C> &incoming, const SphereRule &rule, unsigned order,
                                          double k, double a) {
-    check_domain(k, a);
-    coefficient_count(order);
+    require_scale(k, a);
+    waves_for(order);
     ScaledExpansion out{order, a, spectrum_to_local(incoming, rule, order)};
-    double f = 1;
-    for (unsigned l = 0; l <= order; ++l) {
-        if (l)
-            f *= k * a / (2. * l + 1);
-        for (int m = -static_cast<int>(l); m <= static_cast<int>(l); ++m)
-            out.coefficients[lm(l, m)] *= f;
-    }
+    scale_degrees(out.coefficients, order, k * a);
     return out;
 }
 
 AxialPlan make_axial_plan(AxialKind kind, unsigned ps, unsigned pt, double k, double as, double at, double d,
                           unsigned polar_count, double projection_radius) {
-    check_domain(k, as);
-    check_domain(k, at);
-    coefficient_count(ps);
-    coefficient_count(pt);
+    require_scale(k, as);
+    require_scale(k, at);
+    waves_for(ps);
+    waves_for(pt);
     if (kind != AxialKind::multipole_to_parent && kind != AxialKind::multipole_to_local &&
         kind != AxialKind::local_to_child)
-        throw std::invalid_argument("Scaled axial translation kind is invalid.");
-    if (!std::isfinite(d) || d < 0 || polar_count < 2 * std::max(ps, pt) + 1 || polar_count > 1024)
-        throw std::invalid_argument("Scaled axial plan needs finite separation and independently resolved polar "
-                                    "quadrature.");
+        throw std::invalid_argument("unknown coaxial translation kind");
+    if (!std::isfinite(d) || d < 0)
+        throw std::invalid_argument("coaxial translation distance must be finite and nonnegative");
+    if (polar_count < 2 * std::max(ps, pt) + 1 || polar_count > 1024)
+        throw std::invalid_argument("coaxial plan needs between 2 max(p_s, p_t) + 1 and 1024 polar nodes");
     if (!std::isfinite(projection_radius) || projection_radius < 0)
-        throw std::invalid_argument("Scaled axial projection radius must be finite and nonnegative.");
-    const double probe = projection_radius > 0                  ? projection_radius
-                         : kind == AxialKind::multipole_to_parent ? as * 1.01
-                         : kind == AxialKind::multipole_to_local  ? std::min(at, d * .5)
-                                                                  : at;
-    if (!(probe > 0) || (kind == AxialKind::multipole_to_local && probe >= d))
-        throw std::invalid_argument("Scaled axial Cauchy probe intersects the outgoing singular center.");
-    AxialPlan plan{kind, ps, pt, polar_count, {0}, {}, probe};
-    const unsigned mmax = std::min(ps, pt);
-    for (unsigned m = 0; m <= mmax; ++m)
-        plan.block_offsets.push_back(plan.block_offsets.back() +
-                                     static_cast<std::size_t>(pt - m + 1) * (ps - m + 1));
+        throw std::invalid_argument("coaxial projection radius must be finite and nonnegative");
+    // Sphere on which the translated field is projected: around the child for the
+    // multipole shift (just outside its source ball), around the target centre otherwise.
+    double rho = projection_radius;
+    if (rho == 0)
+        rho = kind == AxialKind::multipole_to_parent ? 1.01 * as
+              : kind == AxialKind::multipole_to_local ? std::min(at, 0.5 * d)
+                                                      : at;
+    if (!(rho > 0) || (kind == AxialKind::multipole_to_local && rho >= d))
+        throw std::invalid_argument("coaxial projection sphere must not reach the source's singular centre");
+
+    AxialPlan plan{kind, ps, pt, polar_count, {0}, {}, rho};
+    const unsigned mtop = std::min(ps, pt);
+    for (unsigned m = 0; m <= mtop; ++m)
+        plan.block_offsets.push_back(plan.block_offsets.back() + static_cast<std::size_t>(pt - m + 1) * (ps - m + 1));
     plan.coefficients.assign(plan.block_offsets.back(), C{});
-    const SphereRule sphere = make_sphere_rule(polar_count - 1);
-    const auto target_radial = normalized_regular(pt + 1, k * probe);
-    for (unsigned ring = 0; ring < polar_count; ++ring) {
-        // azimuth 0 of each ring; the azimuth integral is analytic (2 pi x Gauss weight)
-        const SphereSample &s = sphere.samples[static_cast<std::size_t>(ring) * 2 * polar_count];
-        const Vec3 u = s.direction;
-        const double w = s.weight * (2 * polar_count);
-        const Vec3 point = vadd({0, 0, d}, vscale(u, probe));
-        const auto angular = spherical_harmonics(std::max(ps, pt), u);
-        std::vector<WaveValue> src, tgt;
-        if (kind == AxialKind::multipole_to_parent) {
-            src = scaled_outgoing_waves(ps, k, as, vscale(u, probe));
-            tgt = scaled_regular_waves(pt, k, at, point);
-        } else if (kind == AxialKind::multipole_to_local) {
-            src = scaled_outgoing_waves(ps, k, as, point);
-        } else {
-            src = scaled_regular_waves(ps, k, as, point);
+
+    std::vector<double> nodes, weights;
+    gauss_legendre(polar_count, nodes, weights);
+    const std::vector<double> J = regular_profile(pt + 1, k * rho);
+    // Rows of the Cauchy projection: ((a_t / rho)^l Jhat_l, (a_t / rho)^l rho d/drho[rho^l Jhat_l] / rho^l)
+    // normalised by their squared length, so that value and radial derivative together
+    // determine every degree even where Jhat_l(k rho) vanishes.
+    std::vector<double> value_row(pt + 1), slope_row(pt + 1);
+    for (unsigned l = 0; l <= pt; ++l) {
+        const double slope = l * J[l] - (k * rho) * (k * rho) / (2.0 * l + 3) * J[l + 1];
+        const double length2 = J[l] * J[l] + slope * slope;
+        if (!(length2 > 0) || !std::isfinite(length2))
+            throw std::runtime_error("coaxial projection: radial profile vanishes on the projection sphere");
+        const double lift = std::pow(at / rho, static_cast<double>(l));
+        value_row[l] = lift * J[l] / length2;
+        slope_row[l] = lift * slope / length2;
+    }
+    std::vector<C> alpha(waves_for(pt)), beta(alpha.size()), source_value, source_slope;
+    for (unsigned q = 0; q < polar_count; ++q) {
+        // azimuth 0 of the ring; the coaxial azimuthal integral is exact: 2 pi
+        const double mu = nodes[q], w = 2 * kPi * weights[q];
+        const Vec3 u{std::sqrt((1 - mu) * (1 + mu)), 0.0, mu};
+        const Vec3 on_sphere = vscale(u, rho);
+        const Vec3 shifted = vadd({0.0, 0.0, d}, on_sphere);
+        const std::vector<WaveValue> src =
+            kind == AxialKind::multipole_to_parent ? scaled_outgoing_waves(ps, k, as, on_sphere)
+            : kind == AxialKind::multipole_to_local ? scaled_outgoing_waves(ps, k, as, shifted)
+                                                    : scaled_regular_waves(ps, k, as, shifted);
+        source_value.assign(src.size(), C{});
+        source_slope.assign(src.size(), C{});
+        for (std::size_t j = 0; j < src.size(); ++j) {
+            source_value[j] = src[j].value;
+            source_slope[j] = src[j].gradient[0] * u.x + src[j].gradient[2] * u.z;
         }
-        for (unsigned m = 0; m <= mmax; ++m)
+        // Target weights: plan row (l, m) += alpha_lm * value + beta_lm * radial slope.
+        if (kind == AxialKind::multipole_to_parent) {
+            // Green's identity over the sphere: rho^2 (u conj(d R_lm) - conj(R_lm) d u)
+            const std::vector<WaveValue> tgt = scaled_regular_waves(pt, k, at, shifted);
+            for (unsigned l = 0; l <= pt; ++l)
+                for (unsigned m = 0; m <= std::min(l, mtop); ++m) {
+                    const std::size_t j = lm(l, static_cast<int>(m));
+                    const C tgt_slope = tgt[j].gradient[0] * u.x + tgt[j].gradient[2] * u.z;
+                    alpha[j] = w * rho * rho * std::conj(tgt_slope);
+                    beta[j] = -w * rho * rho * std::conj(tgt[j].value);
+                }
+        } else {
+            const std::vector<C> Y = spherical_harmonics(pt, u);
+            for (unsigned l = 0; l <= pt; ++l)
+                for (unsigned m = 0; m <= std::min(l, mtop); ++m) {
+                    const std::size_t j = lm(l, static_cast<int>(m));
+                    const C y = w * std::conj(Y[j]);
+                    alpha[j] = y * value_row[l];
+                    beta[j] = y * (slope_row[l] * rho);
+                }
+        }
+        for (unsigned m = 0; m <= mtop; ++m) {
+            C *block = plan.coefficients.data() + plan.block_offsets[m];
+            const unsigned cols = ps - m + 1;
             for (unsigned l = m; l <= pt; ++l) {
                 const std::size_t row = lm(l, static_cast<int>(m));
-                const double dj = l * target_radial[l] - k * k * probe * probe / (2. * l + 3) * target_radial[l + 1],
-                             den = target_radial[l] * target_radial[l] + dj * dj;
+                C *out = block + static_cast<std::size_t>(l - m) * cols;
                 for (unsigned n = m; n <= ps; ++n) {
                     const std::size_t col = lm(n, static_cast<int>(m));
-                    C src_normal{};
-                    for (unsigned i = 0; i < 3; ++i)
-                        src_normal += src[col].gradient[i] * component(u, i);
-                    C v{};
-                    if (kind == AxialKind::multipole_to_parent) {
-                        C tgt_normal{};
-                        for (unsigned i = 0; i < 3; ++i)
-                            tgt_normal += tgt[row].gradient[i] * component(u, i);
-                        v = probe * probe * (src[col].value * std::conj(tgt_normal) - src_normal * std::conj(tgt[row].value));
-                    } else {
-                        if (!(den > 0) || !std::isfinite(den))
-                            throw std::runtime_error("Scaled axial Cauchy projection has an unresolved radial "
-                                                     "normalization.");
-                        v = std::pow(at / probe, l) * std::conj(angular[row]) *
-                            (target_radial[l] * src[col].value + dj * probe * src_normal) / den;
-                    }
-                    plan.coefficients[plan.block_offsets[m] + static_cast<std::size_t>(l - m) * (ps - m + 1) + n - m] +=
-                        w * v;
+                    out[n - m] += alpha[row] * source_value[col] + beta[row] * source_slope[col];
                 }
             }
+        }
     }
     for (const C &v : plan.coefficients)
-        if (!std::isfinite(v.real()) || !std::isfinite(v.imag()))
-            throw std::runtime_error("Scaled axial plan has a non-finite translation coefficient.");
+        if (!finite(v))
+            throw std::runtime_error("coaxial plan has a non-finite coefficient");
     return plan;
 }
 
 std::vector<C> apply_axial_plan(const AxialPlan &p, const std::vector<C> &source) {
-    if (source.size() != coefficient_count(p.source_order) ||
-        p.block_offsets.size() != std::min(p.source_order, p.target_order) + 2 || p.block_offsets.front() != 0 ||
-        p.block_offsets.back() != p.coefficients.size())
-        throw std::invalid_argument("Scaled axial application has inconsistent dimensions.");
-    std::vector<C> out(coefficient_count(p.target_order));
+    const unsigned mtop = std::min(p.source_order, p.target_order);
+    if (source.size() != waves_for(p.source_order) || p.block_offsets.size() != mtop + 2 ||
+        p.block_offsets.front() != 0 || p.block_offsets.back() != p.coefficients.size())
+        throw std::invalid_argument("coaxial plan and source expansion do not match");
+    for (unsigned m = 0; m <= mtop; ++m)
+        if (p.block_offsets[m + 1] < p.block_offsets[m] ||
+            p.block_offsets[m + 1] - p.block_offsets[m] !=
+                static_cast<std::size_t>(p.target_order - m + 1) * (p.source_order - m + 1))
+            throw std::invalid_argument("coaxial plan block " + std::to_string(m) + " has the wrong size");
     for (const C &v : source)
-        if (!std::isfinite(v.real()) || !std::isfinite(v.imag()))
-            throw std::invalid_argument("Scaled axial source contains a non-finite coefficient.");
-    for (unsigned m = 0; m <= std::min(p.source_order, p.target_order); ++m) {
-        const std::size_t expected = static_cast<std::size_t>(p.target_order - m + 1) * (p.source_order - m + 1);
-        if (p.block_offsets[m] > p.block_offsets[m + 1] || p.block_offsets[m + 1] > p.coefficients.size() ||
-            p.block_offsets[m + 1] - p.block_offsets[m] != expected)
-            throw std::invalid_argument("Scaled axial application has an invalid m-block span.");
+        if (!finite(v))
+            throw std::invalid_argument("coaxial translation source must be finite");
+    std::vector<C> out(waves_for(p.target_order));
+    for (unsigned m = 0; m <= mtop; ++m) {
+        const C *block = p.coefficients.data() + p.block_offsets[m];
+        const unsigned cols = p.source_order - m + 1;
+        // the +m and -m orders share the block
         for (const int sm : {static_cast<int>(m), -static_cast<int>(m)}) {
+            if (m == 0 && sm < 0)
+                break;
             for (unsigned l = m; l <= p.target_order; ++l) {
+                const C *row = block + static_cast<std::size_t>(l - m) * cols;
                 C sum{};
                 for (unsigned n = m; n <= p.source_order; ++n)
-                    sum += p.coefficients[p.block_offsets[m] + static_cast<std::size_t>(l - m) * (p.source_order - m + 1) +
-                                          n - m] *
-                           source[lm(n, sm)];
+                    sum += row[n - m] * source[lm(n, sm)];
                 out[lm(l, sm)] = sum;
             }
-            if (m == 0)
-                break;
         }
     }
     for (const C &v : out)
-        if (!std::isfinite(v.real()) || !std::isfinite(v.imag()))
-            throw std::runtime_error("Scaled axial application produced a non-finite coefficient.");
+        if (!finite(v))
+            throw std::runtime_error("coaxial translation produced a non-finite coefficient");
     return out;
 }
 

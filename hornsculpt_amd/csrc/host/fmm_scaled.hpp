@@ -1,23 +1,28 @@
 // Scaled low-frequency spherical waves and coaxial translation plans of the
 // wideband FMM.
 //
-// Restates modules/bem/fmm/scaled_translation.hpp:8-69 / scaled_translation.cpp:1-399
-// (scaledRegularWaves, scaledOutgoingWaves, evaluateScaledMultipole/Local,
-// scaledMultipoleToSpectrum, spectrumToScaledLocal, makeScaledAxialPlan,
-// applyScaledAxialPlan):
-//   * regular basis   R_lm(x) = Jhat_l(k r) (r/a)^l Y_lm(x/r), with
-//                     Jhat_l(z) = (2l+1)!! j_l(z) / z^l (-> 1 as z -> 0; DLMF 10.53.1),
-//                     built from solid harmonics r^l Y_lm as polynomials with exact
-//                     gradients, and grad Jhat_l(kr) = -k^2 x Jhat_{l+1}(kr) / (2l+3);
-//   * outgoing basis  O_lm(x) = (a/r)^(l+1) Hhat_l(kr) Y_lm / (a (2l+1)), the radial
-//                     factor by the rescaled recurrence of DLMF 10.51.1-2 that never
-//                     forms h_l(kr) ~ (kr)^(-l-1);
+// The quantities are those of modules/bem/fmm/scaled_translation.hpp:8-69 (scaled
+// regular / outgoing waves with gradients, expansion evaluation, spectra, coaxial
+// m-block plans and their application); the derivation and methods are this
+// project's own:
+//   * regular basis   R_lm(x) = Jhat_l(k r) S_lm(x / a), S_lm(p) = |p|^l Y_lm the
+//                     regular solid harmonics, Jhat_l(z) = (2l+1)!! j_l(z) / z^l
+//                     (-> 1 as z -> 0) by Miller's downward recurrence in that
+//                     normalisation; gradients of S_lm from the degree-lowering ladder
+//                     identities, grad Jhat_l(k r) = -k^2 x Jhat_{l+1}(k r) / (2l+3);
+//   * outgoing basis  O_lm(x) = (a/r)^(l+1) Hhat_l(kr) Y_lm / (a (2l+1)) with
+//                     Hhat_l(z) = -i z^(l+1) h_l^(2)(z) / (2l-1)!! (-> 1 as z -> 0), by
+//                     the upward recurrence of h_l in that normalisation; radial
+//                     derivative from h_l' = h_{l-1} - (l+1) h_l / z;
 //   * coaxial plans   translation along +z as one block per |m| (rows l = m..p_t,
-//                     columns n = m..p_s, row-major), shared by +m and -m; M2M by
-//                     the Green's-identity flux over a sphere of 1.01 a_s, M2L and
-//                     L2L by Cauchy projection of the value and radial derivative on
-//                     a probe sphere (min(a_t, d/2) for M2L, a_t for L2L), using the
-//                     analytic azimuth integral and Gauss-Legendre in the polar angle.
+//                     columns n = m..p_s, row-major), shared by +m and -m; each
+//                     Gauss-Legendre ring of a projection sphere contributes a rank-2
+//                     update (target weights x source values and radial slopes):
+//                     Green's identity over a sphere of 1.01 a_s for the multipole
+//                     shift, least-squares (Cauchy) projection of value and radial
+//                     slope onto the target's regular basis on a sphere of radius
+//                     min(a_t, d/2) (M2L) or a_t (L2L) otherwise; the azimuthal
+//                     integral of a coaxial term is exactly 2 pi.
 #pragma once
 
 #include "fmm_transfer.hpp"

@@ -2,430 +2,421 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <limits>
-#include <math.h>
 #include <numbers>
 #include <stdexcept>
+#include <string>
 
 namespace hsbem {
 namespace {
 
-constexpr unsigned kMaximumOrder = 1024;
-constexpr double kPi = std::numbers::pi;
 using C = std::complex<double>;
+constexpr unsigned kOrderLimit = 1024;
+constexpr double kPi = std::numbers::pi;
 
-void check_order(unsigned order) {
-    if (order > kMaximumOrder)
-        throw std::invalid_argument("Wideband angular order exceeds the supported CPU contract.");
+void require_order(unsigned order, const char *what) {
+    if (order > kOrderLimit)
+        throw std::invalid_argument(std::string(what) + " order " + std::to_string(order) +
+                                    " is above the supported limit of 1024");
 }
 
-double dot(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-
-std::size_t lm(unsigned l, int m) { return static_cast<std::size_t>(l) * l + l + m; }
-
-void check_rule(const SphereRule &r) {
-    check_order(r.order);
-    const auto n = static_cast<std::size_t>(r.order + 1);
-    if (r.samples.size() != 2 * n * n)
-        throw std::invalid_argument("Wideband sphere rule has an inconsistent sample count.");
-    for (const auto &s : r.samples)
-        if (!std::isfinite(s.weight) || s.weight <= 0 || !std::isfinite(dot(s.direction, s.direction)) ||
-            std::abs(dot(s.direction, s.direction) - 1) > 1e-12)
-            throw std::invalid_argument("Wideband sphere rule has an invalid direction or weight.");
-}
-
-void check_values(const std::vector<C> &v, const SphereRule &r) {
-    check_rule(r);
-    if (v.size() != r.samples.size())
-        throw std::invalid_argument("Wideband spectrum does not match its angular rule.");
-    for (const C &x : v)
-        if (!std::isfinite(x.real()) || !std::isfinite(x.imag()))
-            throw std::invalid_argument("Wideband spectrum contains a non-finite value.");
-}
-
-// exp(i a); the reference build evaluates cos and sin of one argument jointly
-C phase(double a) {
-    if (!std::isfinite(a))
-        throw std::invalid_argument("Wideband translation phase is non-finite.");
-    double s, c;
-    ::sincos(a, &s, &c);
-    return {c, s};
-}
+bool finite(C v) { return std::isfinite(v.real()) && std::isfinite(v.imag()); }
 
 std::size_t tri(unsigned l, unsigned m) { return static_cast<std::size_t>(l) * (l + 1) / 2 + m; }
+std::size_t lm(unsigned l, int m) { return static_cast<std::size_t>(l) * l + l + m; }
 
-// Normalised associated Legendre functions (the Y_lm polar factor) of one ring.
-std::vector<double> polar_factors(const AngularTransferPlan &p, double mu) {
-    const unsigned L = p.bandlimit;
-    std::vector<double> out(p.recurrence_first.size());
-    const double rho = std::sqrt(std::max(0., 1 - mu * mu));
-    double diag = 1 / std::sqrt(4 * kPi);
-    for (unsigned m = 0; m <= L; ++m) {
-        if (m)
-            diag *= p.diagonal_scale[m] * rho;
-        out[tri(m, m)] = diag;
-        if (m < L)
-            out[tri(m + 1, m)] = std::sqrt(2. * m + 3) * mu * diag;
-        for (unsigned l = m + 2; l <= L; ++l)
-            out[tri(l, m)] = p.recurrence_first[tri(l, m)] * mu * out[tri(l - 1, m)] -
-                             p.recurrence_second[tri(l, m)] * out[tri(l - 2, m)];
-    }
-    return out;
-}
+double dot3(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
-// Rings of a canonical (polar-major, equispaced azimuth) rule.
-std::vector<PolarSample> rings(const SphereRule &r) {
-    check_rule(r);
-    const unsigned na = 2 * (r.order + 1);
-    std::vector<PolarSample> out;
-    for (unsigned j = 0; j <= r.order; ++j) {
-        const SphereSample &first = r.samples[static_cast<std::size_t>(j) * na];
-        const double mu = first.direction.z, rho = std::sqrt(std::max(0., 1 - mu * mu));
-        for (unsigned a = 0; a < na; ++a) {
-            const SphereSample &s = r.samples[static_cast<std::size_t>(j) * na + a];
-            const double ang = 2 * kPi * a / na;
-            if (std::abs(s.direction.z - mu) > 1e-13 || std::abs(s.direction.x - rho * std::cos(ang)) > 1e-13 ||
-                std::abs(s.direction.y - rho * std::sin(ang)) > 1e-13 ||
-                std::abs(s.weight - first.weight) > 1e-13 * first.weight)
-                throw std::invalid_argument("Wideband transfer requires a polar-major canonical sphere rule.");
+} // namespace
+
+// n-point Gauss-Legendre rule on [-1, 1], nodes descending. Newton's method on P_n
+// from Tricomi's asymptotic estimate of the i-th root; the nodes of the upper half
+// are mirrored, so the rule is symmetric to the last bit.
+void gauss_legendre(unsigned n, std::vector<double> &x, std::vector<double> &w) {
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
+    auto legendre_pair = [n](double t, double &pn, double &pn1) { // P_n(t), P_{n-1}(t)
+        double prev = 1, cur = t;
+        for (unsigned l = 2; l <= n; ++l) {
+            const double next = ((2.0 * l - 1) * t * cur - (l - 1.0) * prev) / l;
+            prev = cur;
+            cur = next;
         }
-        out.push_back({mu, first.weight * na / (2 * kPi)});
+        pn = n == 0 ? 1 : cur;
+        pn1 = prev;
+    };
+    const double nn = n;
+    for (unsigned i = 0; i < (n + 1) / 2; ++i) {
+        const bool middle = 2 * i + 1 == n;
+        double t = middle ? 0.0
+                          : (1 - (nn - 1) / (8 * nn * nn * nn)) * std::cos(kPi * (4.0 * i + 3) / (4.0 * nn + 2));
+        double pn = 0, pn1 = 0;
+        for (int it = 0; !middle; ++it) {
+            if (it == 60)
+                throw std::runtime_error("Gauss-Legendre node iteration failed to converge");
+            legendre_pair(t, pn, pn1);
+            const double step = pn / (nn * (t * pn - pn1) / (t * t - 1));
+            t -= step;
+            if (std::abs(step) <= 2 * std::numeric_limits<double>::epsilon())
+                break;
+        }
+        legendre_pair(t, pn, pn1);
+        const double slope = nn * (t * pn - pn1) / (t * t - 1);
+        const double weight = 2 / ((1 - t * t) * slope * slope);
+        x[i] = t;
+        x[n - 1 - i] = -t;
+        w[i] = w[n - 1 - i] = weight;
     }
-    return out;
 }
 
-// sum_q w_q v_q conj(Y_lm(d_q)), l <= order
-std::vector<C> project(const std::vector<C> &v, const SphereRule &r, unsigned order) {
-    check_values(v, r);
-    if (order > r.order)
-        throw std::invalid_argument("Wideband projection exceeds its angular sampling order.");
-    check_order(order);
-    std::vector<C> c(static_cast<std::size_t>(order + 1) * (order + 1));
-    for (std::size_t q = 0; q < v.size(); ++q) {
-        const auto y = spherical_harmonics(order, r.samples[q].direction);
-        const C w = v[q] * r.samples[q].weight;
-        for (std::size_t j = 0; j < c.size(); ++j)
-            c[j] += w * std::conj(y[j]);
+namespace {
+
+// The polar rings of a product rule: cos(theta) and the (shared) sample weight.
+struct Rings {
+    unsigned count{}, azimuths{};
+    std::vector<double> mu, weight;
+};
+
+Rings rings_of(const SphereRule &rule) {
+    require_order(rule.order, "sphere rule");
+    Rings r{rule.order + 1, 2 * (rule.order + 1), {}, {}};
+    if (rule.samples.size() != static_cast<std::size_t>(r.count) * r.azimuths)
+        throw std::invalid_argument("sphere rule sample count does not match its order");
+    for (unsigned j = 0; j < r.count; ++j) {
+        const SphereSample *ring = rule.samples.data() + static_cast<std::size_t>(j) * r.azimuths;
+        const double mu = ring[0].direction.z, rho = std::sqrt(std::max(0.0, (1 - mu) * (1 + mu)));
+        for (unsigned a = 0; a < r.azimuths; ++a) {
+            const double phi = 2 * kPi * a / r.azimuths;
+            const SphereSample &s = ring[a];
+            const bool ok = s.weight > 0 && std::isfinite(s.weight) && std::abs(s.direction.z - mu) <= 1e-13 &&
+                            std::abs(s.direction.x - rho * std::cos(phi)) <= 1e-13 &&
+                            std::abs(s.direction.y - rho * std::sin(phi)) <= 1e-13 &&
+                            std::abs(s.weight - ring[0].weight) <= 1e-13 * ring[0].weight;
+            if (!ok)
+                throw std::invalid_argument("sphere rule is not a polar-major product rule with equispaced azimuths");
+        }
+        r.mu.push_back(mu);
+        r.weight.push_back(ring[0].weight);
+    }
+    return r;
+}
+
+void require_samples(const std::vector<C> &v, const Rings &r) {
+    if (v.size() != static_cast<std::size_t>(r.count) * r.azimuths)
+        throw std::invalid_argument("signature sample count does not match its sphere rule");
+    for (const C &x : v)
+        if (!finite(x))
+            throw std::invalid_argument("signature samples must be finite");
+}
+
+// e^{-2 pi i t / n}, t = 0..n-1.
+std::vector<C> twiddles(unsigned n) {
+    std::vector<C> t(n);
+    for (unsigned i = 0; i < n; ++i) {
+        const double a = -2 * kPi * i / n;
+        t[i] = {std::cos(a), std::sin(a)};
+    }
+    return t;
+}
+
+// Twiddle index step of azimuthal order m on n azimuths.
+std::size_t step_of(int m, unsigned n) {
+    const long long s = m % static_cast<long long>(n);
+    return static_cast<std::size_t>(s < 0 ? s + n : s);
+}
+
+// (-1)^m for negative m with |m| odd (Y_{l,-|m|} = (-1)^m conj(Y_l|m|)).
+double parity(int m) { return m < 0 && (-m) % 2 ? -1.0 : 1.0; }
+
+// c_lm = sum_q w_q v_q conj(Y_lm(s_q)), l <= L: per ring the azimuthal DFT, then the
+// Legendre sum.
+std::vector<C> analyse(const std::vector<C> &v, const Rings &r, unsigned L) {
+    std::vector<C> c(static_cast<std::size_t>(L + 1) * (L + 1));
+    const std::vector<C> tw = twiddles(r.azimuths);
+    const int iL = static_cast<int>(L);
+    for (unsigned j = 0; j < r.count; ++j) {
+        const std::vector<double> P = normalized_legendre(L, r.mu[j]);
+        const C *ring = v.data() + static_cast<std::size_t>(j) * r.azimuths;
+        for (int m = -iL; m <= iL; ++m) {
+            const std::size_t step = step_of(m, r.azimuths);
+            C f{};
+            for (std::size_t a = 0, idx = 0; a < r.azimuths; ++a) {
+                f += ring[a] * tw[idx];
+                idx += step;
+                if (idx >= r.azimuths)
+                    idx -= r.azimuths;
+            }
+            f *= r.weight[j] * parity(m);
+            const unsigned am = static_cast<unsigned>(std::abs(m));
+            for (unsigned l = am; l <= L; ++l)
+                c[lm(l, m)] += f * P[tri(l, am)];
+        }
     }
     return c;
 }
 
-// sum_lm c_lm Y_lm(d_q) at every rule direction
-std::vector<C> evaluate(const std::vector<C> &c, unsigned order, const SphereRule &r) {
-    check_rule(r);
-    check_order(order);
-    if (c.size() != static_cast<std::size_t>(order + 1) * (order + 1))
-        throw std::invalid_argument("Wideband harmonic coefficients have an invalid size.");
-    std::vector<C> out(r.samples.size());
-    for (std::size_t q = 0; q < out.size(); ++q) {
-        const auto y = spherical_harmonics(order, r.samples[q].direction);
-        for (std::size_t j = 0; j < c.size(); ++j)
-            out[q] += c[j] * y[j];
+// sum_lm c_lm Y_lm(s_q) on every sample of the rings, l <= L.
+std::vector<C> synthesise(const std::vector<C> &c, unsigned L, const Rings &r) {
+    std::vector<C> out(static_cast<std::size_t>(r.count) * r.azimuths);
+    const std::vector<C> tw = twiddles(r.azimuths);
+    const int iL = static_cast<int>(L);
+    for (unsigned j = 0; j < r.count; ++j) {
+        const std::vector<double> P = normalized_legendre(L, r.mu[j]);
+        C *ring = out.data() + static_cast<std::size_t>(j) * r.azimuths;
+        for (int m = -iL; m <= iL; ++m) {
+            const unsigned am = static_cast<unsigned>(std::abs(m));
+            C g{};
+            for (unsigned l = am; l <= L; ++l)
+                g += c[lm(l, m)] * P[tri(l, am)];
+            g *= parity(m);
+            const std::size_t step = step_of(m, r.azimuths);
+            for (std::size_t a = 0, idx = 0; a < r.azimuths; ++a) {
+                ring[a] += g * std::conj(tw[idx]);
+                idx += step;
+                if (idx >= r.azimuths)
+                    idx -= r.azimuths;
+            }
+        }
     }
     return out;
+}
+
+void multiply_phases(std::vector<C> &v, const SphereRule &rule, double scale, Vec3 shift) {
+    for (std::size_t q = 0; q < v.size(); ++q) {
+        const double angle = scale * dot3(rule.samples[q].direction, shift);
+        if (!std::isfinite(angle))
+            throw std::invalid_argument("plane-wave shift phase is not finite");
+        v[q] *= std::polar(1.0, angle);
+    }
 }
 
 } // namespace
 
 SphereRule make_sphere_rule(unsigned order) {
-    check_order(order);
+    require_order(order, "sphere rule");
+    const unsigned n = order + 1, na = 2 * n;
+    std::vector<double> mu, w;
+    gauss_legendre(n, mu, w);
+    std::vector<double> cs(na), sn(na);
+    for (unsigned a = 0; a < na; ++a) {
+        const double phi = 2 * kPi * a / na;
+        cs[a] = std::cos(phi);
+        sn[a] = std::sin(phi);
+    }
     SphereRule rule{order, {}};
-    const unsigned np = order + 1, na = 2 * np;
-    rule.samples.reserve(static_cast<std::size_t>(np) * na);
-    for (unsigned j = 0; j < np; ++j) {
-        // Newton on P_np from the Tricomi-type initial guess cos(pi (j + 3/4) / (np + 1/2))
-        double mu = std::cos(kPi * (j + .75) / (np + .5)), dp = 0;
-        bool done = false;
-        for (unsigned it = 0; it < 100; ++it) {
-            double p0 = 0, p1 = 1;
-            for (unsigned l = 1; l <= np; ++l) {
-                const double p2 = ((2 * l - 1) * mu * p1 - (l - 1) * p0) / l;
-                p0 = p1;
-                p1 = p2;
-            }
-            dp = np * (mu * p1 - p0) / (mu * mu - 1);
-            const double step = p1 / dp;
-            mu -= step;
-            if (std::abs(step) <= 4 * std::numeric_limits<double>::epsilon()) {
-                done = true;
-                break;
-            }
-        }
-        if (!done)
-            throw std::runtime_error("Wideband Gauss-Legendre rule did not converge.");
-        const double w = 2 / ((1 - mu * mu) * dp * dp), rho = std::sqrt(std::max(0., 1 - mu * mu));
-        for (unsigned m = 0; m < na; ++m) {
-            const double phi = 2 * kPi * m / na;
-            double sp, cp;
-            ::sincos(phi, &sp, &cp); // the reference build evaluates this pair jointly
-            rule.samples.push_back({{rho * cp, rho * sp, mu}, w * 2 * kPi / na});
-        }
+    rule.samples.reserve(static_cast<std::size_t>(n) * na);
+    for (unsigned j = 0; j < n; ++j) {
+        const double rho = std::sqrt((1 - mu[j]) * (1 + mu[j])), weight = w[j] * (2 * kPi / na);
+        for (unsigned a = 0; a < na; ++a)
+            rule.samples.push_back({{rho * cs[a], rho * sn[a], mu[j]}, weight});
     }
     return rule;
 }
 
-std::vector<C> spherical_harmonics(unsigned order, Vec3 d) {
-    const double n2 = dot(d, d);
-    if (!std::isfinite(n2) || std::abs(n2 - 1) > 1e-12)
-        throw std::invalid_argument("Spherical harmonic direction must be a finite unit vector.");
-    check_order(order);
-    std::vector<C> y(static_cast<std::size_t>(order + 1) * (order + 1));
-    const double mu = std::clamp(d.z, -1., 1.), rho = std::hypot(d.x, d.y);
-    const C e = rho == 0 ? C{1, 0} : C{d.x / rho, d.y / rho};
-    C diag{1 / std::sqrt(4 * kPi), 0};
+std::vector<double> normalized_legendre(unsigned order, double mu) {
+    require_order(order, "Legendre table");
+    if (!(std::abs(mu) <= 1))
+        throw std::invalid_argument("Legendre argument must lie in [-1, 1]");
+    std::vector<double> p(tri(order, order) + 1);
+    const double rho = std::sqrt((1 - mu) * (1 + mu));
+    double sectoral = 1 / std::sqrt(4 * kPi);
     for (unsigned m = 0; m <= order; ++m) {
         if (m)
-            diag *= -std::sqrt((2. * m + 1) / (2. * m)) * rho * e;
-        y[lm(m, static_cast<int>(m))] = diag;
-        if (m < order)
-            y[lm(m + 1, static_cast<int>(m))] = std::sqrt(2. * m + 3) * mu * diag;
+            sectoral *= -std::sqrt((2.0 * m + 1) / (2.0 * m)) * rho;
+        p[tri(m, m)] = sectoral;
+        if (m == order)
+            break;
+        p[tri(m + 1, m)] = std::sqrt(2.0 * m + 3) * mu * sectoral;
+        const double mm = static_cast<double>(m) * m;
         for (unsigned l = m + 2; l <= order; ++l) {
-            const double den = static_cast<double>(l) * l - static_cast<double>(m) * m;
-            const double a = std::sqrt((4. * l * l - 1) / den),
-                         b = std::sqrt((2. * l + 1) * ((l - 1.) * (l - 1.) - static_cast<double>(m) * m) /
-                                       ((2. * l - 3) * den));
-            y[lm(l, static_cast<int>(m))] = a * mu * y[lm(l - 1, static_cast<int>(m))] - b * y[lm(l - 2, static_cast<int>(m))];
-        }
-        if (m)
-            for (unsigned l = m; l <= order; ++l)
-                y[lm(l, -static_cast<int>(m))] = (m % 2 ? -1. : 1.) * std::conj(y[lm(l, static_cast<int>(m))]);
-    }
-    return y;
-}
-
-RegularRadialTable regular_radial_table(unsigned order, double x) {
-    check_order(order);
-    if (!std::isfinite(x) || x < 0)
-        throw std::invalid_argument("Regular spherical waves require a finite nonnegative argument.");
-    RegularRadialTable t;
-    t.values.assign(order + 1, 0.0);
-    t.derivatives.assign(order + 1, 0.0);
-    if (x == 0) {
-        t.values[0] = 1;
-        if (order)
-            t.derivatives[1] = 1. / 3;
-        return t;
-    }
-    std::vector<double> j(order + 2);
-    if (x <= 1) {
-        // j_l(x) = x^l / (2l+1)!! sum_n (-x^2/2)^n / (n! (2l+3)(2l+5)...(2l+2n+1))
-        double lead = 1;
-        for (unsigned l = 0; l <= order + 1; ++l) {
-            const double lead_over_x = l ? lead / (2. * l + 1) : 0;
-            if (l)
-                lead *= x / (2. * l + 1);
-            double term = 1, sum = 1, weighted = 0;
-            for (unsigned n = 1; n < 128; ++n) {
-                term *= -x * x / (2. * n * (2. * l + 2 * n + 1));
-                sum += term;
-                weighted += 2 * n * term;
-                if (std::abs(term) <= std::numeric_limits<double>::epsilon() * std::abs(sum))
-                    break;
-            }
-            j[l] = lead * sum;
-            if (l && l <= order)
-                t.derivatives[l] = lead_over_x * (l * sum + weighted);
-        }
-    } else {
-        double sx, cx;
-        ::sincos(x, &sx, &cx); // the reference build evaluates this pair jointly
-        const double j0 = sx / x, j1 = (j0 - cx) / x;
-        if (x > order + 32.) {
-            j[0] = j0;
-            j[1] = j1;
-            for (unsigned l = 1; l <= order; ++l)
-                j[l + 1] = ((2. * l + 1) / x) * j[l] - j[l - 1];
-        } else {
-            const unsigned top = std::max(order + 48u, static_cast<unsigned>(std::ceil(x + 48 + 8 * std::cbrt(x))));
-            std::vector<double> mil(top + 2);
-            mil[top] = 1;
-            for (unsigned l = top; l > 0; --l) {
-                mil[l - 1] = ((2. * l + 1) / x) * mil[l] - mil[l + 1];
-                if (std::abs(mil[l - 1]) > 1e200)
-                    for (std::size_t i = l - 1; i < mil.size(); ++i)
-                        mil[i] *= 1e-200;
-            }
-            const double scale = std::abs(j0) > std::abs(j1) ? j0 / mil[0] : j1 / mil[1];
-            for (unsigned l = 0; l <= order + 1; ++l)
-                j[l] = scale * mil[l];
-        }
-        for (unsigned l = 1; l <= order; ++l)
-            t.derivatives[l] = (l / x) * j[l] - j[l + 1];
-    }
-    t.derivatives[0] = -j[1];
-    std::copy_n(j.begin(), order + 1, t.values.begin());
-    for (unsigned l = 0; l <= order; ++l)
-        if (!std::isfinite(t.values[l]) || !std::isfinite(t.derivatives[l]))
-            throw std::runtime_error("Regular spherical-wave recursion produced a non-finite value.");
-    return t;
-}
-
-AngularTransferPlan make_angular_transfer_plan(const SphereRule &source, const SphereRule &target, unsigned L) {
-    check_order(L);
-    if (L > source.order || L > target.order)
-        throw std::invalid_argument("Wideband transfer bandlimit exceeds its source or target sampling.");
-    AngularTransferPlan p;
-    p.source_azimuths = 2 * (source.order + 1);
-    p.target_azimuths = 2 * (target.order + 1);
-    p.bandlimit = L;
-    p.source_polar = rings(source);
-    p.target_polar = rings(target);
-    const std::size_t count = static_cast<std::size_t>(L + 1) * (L + 2) / 2;
-    p.recurrence_first.assign(count, 0.0);
-    p.recurrence_second.assign(count, 0.0);
-    p.diagonal_scale.assign(L + 1, 0.0);
-    for (unsigned m = 0; m <= L; ++m) {
-        if (m)
-            p.diagonal_scale[m] = -std::sqrt((2. * m + 1) / (2. * m));
-        for (unsigned l = m + 2; l <= L; ++l) {
-            const double den = static_cast<double>(l) * l - static_cast<double>(m) * m;
-            p.recurrence_first[tri(l, m)] = std::sqrt((4. * l * l - 1) / den);
-            p.recurrence_second[tri(l, m)] =
-                std::sqrt((2. * l + 1) * ((l - 1.) * (l - 1.) - static_cast<double>(m) * m) / ((2. * l - 3) * den));
+            const double ll = static_cast<double>(l) * l, span = ll - mm;
+            const double a = std::sqrt((4 * ll - 1) / span);
+            const double b = std::sqrt(((l - 1.0) * (l - 1.0) - mm) * (2.0 * l + 1) / (span * (2.0 * l - 3)));
+            p[tri(l, m)] = a * mu * p[tri(l - 1, m)] - b * p[tri(l - 2, m)];
         }
     }
     return p;
 }
 
-std::vector<C> resample_spectrum(const std::vector<C> &values, const AngularTransferPlan &p) {
-    check_order(p.bandlimit);
-    const unsigned L = p.bandlimit;
-    const std::size_t count = static_cast<std::size_t>(L + 1) * (L + 2) / 2;
-    if (p.source_polar.size() <= L || p.target_polar.size() <= L || p.source_azimuths != 2 * p.source_polar.size() ||
-        p.target_azimuths != 2 * p.target_polar.size() || p.recurrence_first.size() != count ||
-        p.recurrence_second.size() != count || p.diagonal_scale.size() != L + 1 ||
-        values.size() != p.source_polar.size() * p.source_azimuths)
-        throw std::invalid_argument("Wideband separable transfer has an inconsistent plan or spectrum.");
-    for (const C &v : values)
-        if (!std::isfinite(v.real()) || !std::isfinite(v.imag()))
-            throw std::invalid_argument("Wideband spectrum contains a non-finite value.");
-    for (const auto *polar : {&p.source_polar, &p.target_polar})
-        for (const PolarSample &s : *polar)
-            if (!std::isfinite(s.cosine) || std::abs(s.cosine) > 1 || !std::isfinite(s.weight) || s.weight <= 0)
-                throw std::invalid_argument("Wideband separable transfer has an invalid polar rule.");
-    for (const auto *a : {&p.recurrence_first, &p.recurrence_second, &p.diagonal_scale})
-        for (double v : *a)
-            if (!std::isfinite(v))
-                throw std::invalid_argument("Wideband separable transfer has a non-finite recurrence.");
-    const int iL = static_cast<int>(L);
-    // analysis: per ring, azimuthal Fourier coefficient of order m, then the polar factor
-    std::vector<C> c(static_cast<std::size_t>(L + 1) * (L + 1));
-    for (std::size_t j = 0; j < p.source_polar.size(); ++j) {
-        const auto P = polar_factors(p, p.source_polar[j].cosine);
-        for (int m = -iL; m <= iL; ++m) {
-            C f{};
-            for (unsigned a = 0; a < p.source_azimuths; ++a)
-                f += values[j * p.source_azimuths + a] * phase(-2 * kPi * m * a / p.source_azimuths);
-            f *= 2 * kPi * p.source_polar[j].weight / p.source_azimuths;
-            const unsigned am = static_cast<unsigned>(std::abs(m));
-            const double sign = m < 0 && am % 2 ? -1. : 1.;
-            for (unsigned l = am; l <= L; ++l)
-                c[lm(l, m)] += f * (sign * P[tri(l, am)]);
+std::vector<C> spherical_harmonics(unsigned order, Vec3 d) {
+    const double n2 = dot3(d, d);
+    if (!std::isfinite(n2) || std::abs(n2 - 1) > 1e-12)
+        throw std::invalid_argument("spherical harmonics need a finite unit direction");
+    const std::vector<double> P = normalized_legendre(order, std::clamp(d.z, -1.0, 1.0));
+    const double rho = std::hypot(d.x, d.y);
+    const C e = rho > 0 ? C{d.x / rho, d.y / rho} : C{1, 0};
+    std::vector<C> y(static_cast<std::size_t>(order + 1) * (order + 1));
+    C em{1, 0}; // e^{i m phi}
+    for (unsigned m = 0; m <= order; ++m) {
+        const double sign = m % 2 ? -1.0 : 1.0;
+        for (unsigned l = m; l <= order; ++l) {
+            const C v = P[tri(l, m)] * em;
+            y[lm(l, static_cast<int>(m))] = v;
+            if (m)
+                y[lm(l, -static_cast<int>(m))] = sign * std::conj(v);
         }
+        em *= e;
     }
-    // synthesis on the target rings
-    std::vector<C> out(p.target_polar.size() * p.target_azimuths);
-    for (std::size_t j = 0; j < p.target_polar.size(); ++j) {
-        const auto P = polar_factors(p, p.target_polar[j].cosine);
-        for (int m = -iL; m <= iL; ++m) {
-            const unsigned am = static_cast<unsigned>(std::abs(m));
-            const double sign = m < 0 && am % 2 ? -1. : 1.;
-            C f{};
-            for (unsigned l = am; l <= L; ++l)
-                f += c[lm(l, m)] * (sign * P[tri(l, am)]);
-            for (unsigned a = 0; a < p.target_azimuths; ++a)
-                out[j * p.target_azimuths + a] += f * phase(2 * kPi * m * a / p.target_azimuths);
+    return y;
+}
+
+RegularRadialTable regular_radial_table(unsigned order, double x) {
+    require_order(order, "spherical Bessel table");
+    if (!std::isfinite(x) || x < 0)
+        throw std::invalid_argument("spherical Bessel argument must be finite and nonnegative");
+    RegularRadialTable t{std::vector<double>(order + 1, 0.0), std::vector<double>(order + 1, 0.0)};
+    if (x == 0) {
+        t.values[0] = 1;
+        if (order)
+            t.derivatives[1] = 1.0 / 3;
+        return t;
+    }
+    std::vector<double> j(order + 2);
+    const double s = std::sin(x), c = std::cos(x), j0 = s / x, j1 = (s / x - c) / x;
+    if (x < 1e-8) {
+        // two terms of the power series are exact in double precision here
+        double lead = 1; // x^l / (2l+1)!!
+        for (unsigned l = 0; l <= order + 1; ++l) {
+            if (l)
+                lead *= x / (2.0 * l + 1);
+            j[l] = lead * (1 - x * x / (2 * (2.0 * l + 3)));
         }
+    } else if (x > order + 32.0) {
+        // every l <= order + 1 lies below x: the upward recurrence is stable
+        j[0] = j0;
+        j[1] = j1;
+        for (unsigned l = 1; l <= order; ++l)
+            j[l + 1] = (2.0 * l + 1) / x * j[l] - j[l - 1];
+    } else {
+        // Miller: the downward recurrence converges to the minimal solution j_l; its
+        // scale and sign follow from sum_l (2l+1) j_l^2 = 1 and the larger of j_0, j_1.
+        const double reach = std::max(static_cast<double>(order + 1), x);
+        const unsigned top = static_cast<unsigned>(reach + 40 + 10 * std::cbrt(reach));
+        std::vector<double> f(top + 2, 0.0);
+        f[top] = 1;
+        for (unsigned l = top; l > 0; --l) {
+            f[l - 1] = (2.0 * l + 1) / x * f[l] - f[l + 1];
+            if (std::abs(f[l - 1]) > 1e100)
+                for (unsigned i = l - 1; i <= top; ++i)
+                    f[i] *= 1e-100;
+        }
+        double norm = 0;
+        for (unsigned l = 0; l <= top; ++l)
+            norm += (2.0 * l + 1) * f[l] * f[l];
+        double scale = 1 / std::sqrt(norm);
+        if ((std::abs(j0) >= std::abs(j1) ? j0 * f[0] : j1 * f[1]) < 0)
+            scale = -scale;
+        for (unsigned l = 0; l <= order + 1; ++l)
+            j[l] = scale * f[l];
     }
-    return out;
+    t.derivatives[0] = -j[1];
+    for (unsigned l = 1; l <= order; ++l)
+        t.derivatives[l] = j[l - 1] - (l + 1.0) / x * j[l];
+    std::copy_n(j.begin(), order + 1, t.values.begin());
+    for (unsigned l = 0; l <= order; ++l)
+        if (!std::isfinite(t.values[l]) || !std::isfinite(t.derivatives[l]))
+            throw std::runtime_error("spherical Bessel table is not finite");
+    return t;
 }
 
 std::vector<C> resample_spectrum(const std::vector<C> &values, const SphereRule &source, const SphereRule &target,
                                  unsigned L) {
-    check_values(values, source);
-    return resample_spectrum(values, make_angular_transfer_plan(source, target, L));
+    require_order(L, "resampling bandlimit");
+    const Rings from = rings_of(source), to = rings_of(target);
+    if (L > source.order || L > target.order)
+        throw std::invalid_argument("resampling bandlimit exceeds the source or target rule order");
+    require_samples(values, from);
+    return synthesise(analyse(values, from, L), L, to);
 }
 
 std::vector<C> outgoing_to_parent(const std::vector<C> &child, const SphereRule &child_rule,
                                   const SphereRule &parent_rule, unsigned L, double k, Vec3 shift) {
-    auto out = resample_spectrum(child, child_rule, parent_rule, L);
-    for (std::size_t q = 0; q < out.size(); ++q)
-        out[q] *= phase(k * dot(parent_rule.samples[q].direction, shift));
+    std::vector<C> out = resample_spectrum(child, child_rule, parent_rule, L);
+    multiply_phases(out, parent_rule, k, shift);
     return out;
 }
 
 std::vector<C> incoming_to_child(const std::vector<C> &parent, const SphereRule &parent_rule,
                                  const SphereRule &child_rule, unsigned L, double k, Vec3 shift) {
-    check_values(parent, parent_rule);
+    require_samples(parent, rings_of(parent_rule));
     std::vector<C> shifted(parent);
-    for (std::size_t q = 0; q < shifted.size(); ++q)
-        shifted[q] *= phase(-k * dot(parent_rule.samples[q].direction, shift));
+    multiply_phases(shifted, parent_rule, -k, shift);
     return resample_spectrum(shifted, parent_rule, child_rule, L);
 }
 
 std::vector<C> multipole_to_spectrum(const std::vector<C> &multipoles, unsigned order, const SphereRule &rule) {
-    check_order(order);
+    require_order(order, "multipole");
     if (multipoles.size() != static_cast<std::size_t>(order + 1) * (order + 1))
-        throw std::invalid_argument("Wideband multipoles have an invalid size.");
-    std::vector<C> c(multipoles);
-    C f{4 * kPi, 0}; // 4 pi i^l
-    for (unsigned l = 0; l <= order; ++l) {
+        throw std::invalid_argument("multipole coefficient count does not match its order");
+    std::vector<C> c(multipoles.size());
+    C il{4 * kPi, 0}; // 4 pi i^l
+    for (unsigned l = 0; l <= order; ++l, il *= C{0, 1})
         for (int m = -static_cast<int>(l); m <= static_cast<int>(l); ++m) {
-            if (!std::isfinite(c[lm(l, m)].real()) || !std::isfinite(c[lm(l, m)].imag()))
-                throw std::invalid_argument("Wideband multipole is non-finite.");
-            c[lm(l, m)] *= f;
+            if (!finite(multipoles[lm(l, m)]))
+                throw std::invalid_argument("multipole coefficients must be finite");
+            c[lm(l, m)] = il * multipoles[lm(l, m)];
         }
-        f *= C{0, 1};
-    }
-    return evaluate(c, order, rule);
+    return synthesise(c, order, rings_of(rule));
 }
 
 std::vector<C> spectrum_to_local(const std::vector<C> &incoming, const SphereRule &rule, unsigned order) {
-    auto c = project(incoming, rule, order);
-    C f{4 * kPi, 0}; // 4 pi (-i)^l
-    for (unsigned l = 0; l <= order; ++l) {
+    const Rings r = rings_of(rule);
+    require_order(order, "local expansion");
+    if (order > rule.order)
+        throw std::invalid_argument("local expansion order exceeds the order its sphere rule resolves");
+    require_samples(incoming, r);
+    std::vector<C> c = analyse(incoming, r, order);
+    C il{4 * kPi, 0}; // 4 pi (-i)^l
+    for (unsigned l = 0; l <= order; ++l, il *= C{0, -1})
         for (int m = -static_cast<int>(l); m <= static_cast<int>(l); ++m)
-            c[lm(l, m)] *= f;
-        f *= C{0, -1};
-    }
+            c[lm(l, m)] *= il;
     return c;
 }
 
-std::vector<C> diagonal_translation(const SphereRule &rule, unsigned order, double k, Vec3 r) {
-    check_rule(rule);
-    check_order(order);
-    const double dist = std::sqrt(dot(r, r)), z = k * dist;
-    if (!std::isfinite(dist) || !std::isfinite(z) || k <= 0 || dist <= 0 || z == 0)
-        throw std::invalid_argument("Wideband diagonal translation requires finite positive kR.");
-    const Vec3 u{r.x / dist, r.y / dist, r.z / dist};
-    // h_l^(2)(z): h_0 = (sin z + i cos z)/z, h_1 = (1/z + i) h_0, upward three-term recurrence
-    std::vector<C> h(order + 1);
-    double sz, cz;
-    ::sincos(z, &sz, &cz); // the reference build evaluates this pair jointly
-    h[0] = C{sz, cz} / z;
-    if (order)
-        h[1] = C{1 / z, 1} * h[0];
-    for (unsigned l = 1; l < order; ++l)
-        h[l + 1] = ((2. * l + 1) / z) * h[l] - h[l - 1];
-    for (const C &v : h)
-        if (!std::isfinite(v.real()) || !std::isfinite(v.imag()))
-            throw std::runtime_error("Unstable diagonal translation: use the low-frequency representation.");
-    std::vector<C> out(rule.samples.size());
-    const C pre{0, -k / (16 * kPi * kPi)};
-    for (std::size_t q = 0; q < out.size(); ++q) {
-        const double mu = std::clamp(dot(rule.samples[q].direction, u), -1., 1.);
-        double p0 = 0, p1 = 1;
-        C ph{1, 0}, sum{};
-        for (unsigned l = 0; l <= order; ++l) {
-            sum += (2. * l + 1) * ph * h[l] * p1; // (2l+1) (-i)^l h_l P_l(mu)
-            const double p2 = ((2. * l + 1) * mu * p1 - l * p0) / (l + 1.);
-            p0 = p1;
-            p1 = p2;
-            ph *= C{0, -1};
+std::vector<C> diagonal_translation(const SphereRule &rule, unsigned order, double k, Vec3 R) {
+    rings_of(rule);
+    require_order(order, "diagonal translation");
+    const double dist = std::sqrt(dot3(R, R)), z = k * dist;
+    if (!(k > 0) || !(dist > 0) || !std::isfinite(z) || z == 0)
+        throw std::invalid_argument("diagonal translation needs k > 0 and a finite nonzero separation");
+    const Vec3 axis{R.x / dist, R.y / dist, R.z / dist};
+    // c_l = (2l+1) (-i)^l h_l^(2)(z); h_l by the upward recurrence (stable for h).
+    std::vector<C> coef(order + 1);
+    {
+        const double s = std::sin(z), c = std::cos(z);
+        C h_prev{s / z, c / z}; // h_0 = j_0 - i y_0
+        C h = order ? C{s / z / z - c / z, c / z / z + s / z} : C{}; // h_1
+        C phase{1, 0};
+        for (unsigned l = 0; l <= order; ++l, phase *= C{0, -1}) {
+            const C hl = l == 0 ? h_prev : h;
+            if (!finite(hl))
+                throw std::runtime_error("diagonal translation overflowed: the separation is too small for the "
+                                         "expansion order (use the spherical representation)");
+            coef[l] = (2.0 * l + 1) * phase * hl;
+            if (l >= 1) {
+                const C next = (2.0 * l + 1) / z * h - h_prev;
+                h_prev = h;
+                h = next;
+            }
         }
-        out[q] = pre * sum;
-        if (!std::isfinite(out[q].real()) || !std::isfinite(out[q].imag()))
-            throw std::runtime_error("Wideband diagonal translation is non-finite.");
+    }
+    const C pre{0, -k / (16 * kPi * kPi)};
+    std::vector<C> out(rule.samples.size());
+    for (std::size_t q = 0; q < out.size(); ++q) {
+        const double mu = std::clamp(dot3(rule.samples[q].direction, axis), -1.0, 1.0);
+        // Clenshaw: b_l = c_l + alpha_l b_{l+1} + beta_{l+1} b_{l+2},
+        // alpha_l = (2l+1) mu / (l+1), beta_l = -l / (l+1); the sum is b_0.
+        C b1{}, b2{};
+        for (int l = static_cast<int>(order); l >= 0; --l) {
+            const double alpha = (2.0 * l + 1) * mu / (l + 1.0), beta = -(l + 1.0) / (l + 2.0);
+            const C b0 = coef[l] + alpha * b1 + beta * b2;
+            b2 = b1;
+            b1 = b0;
+        }
+        out[q] = pre * b1;
+        if (!finite(out[q]))
+            throw std::runtime_error("diagonal translation is not finite");
     }
     return out;
 }

@@ -1,8 +1,10 @@
 """Scaled LF spherical waves, expansions and coaxial m-block translation plans of
 the wideband FMM.
 
-Bit-identical to the reference (fmm/scaled_translation.cpp compiled in
-oracle/_ref) and pinned by the mathematics: the closed forms
+Equal to the reference (fmm/scaled_translation.cpp compiled in oracle/_ref) to
+rounding-level tolerances (its own methods differ: ladder gradients of the solid
+harmonics, Miller's recurrence for Jhat_l, rank-2 ring updates of the plans) and
+pinned by the mathematics: the closed forms
 R_lm = (2l+1)!! j_l(kr) / (ka)^l Y_lm and O_lm = -ik (ka)^l / (2l+1)!! h_l^(2)(kr) Y_lm
 against scipy, gradients against central differences, and the M2M / M2L / L2L
 plans against the fields they must reproduce (a monopole's Green's function and
@@ -32,7 +34,7 @@ def _green(k, x):
     return np.exp(-1j * k * r) / (4 * np.pi * r)
 
 
-# ---------------------------------------------------------------- bit parity
+# ---------------------------------------------------------------- reference build
 
 
 @needs_ref
@@ -40,35 +42,39 @@ def _green(k, x):
 @pytest.mark.parametrize("k", [0.0, 3.0, 50.0])
 @pytest.mark.parametrize("point", POINTS + [(0.0, 0.0, 0.0)])
 @pytest.mark.parametrize("outgoing", [False, True])
-def test_scaled_waves_bit_identical(order, k, point, outgoing):
+def test_scaled_waves_match_reference(order, k, point, outgoing):
     if outgoing and not any(point):
         pytest.skip("the outgoing basis is singular at its centre")
     v, g = bem.scaled_waves(order, k, 0.04, point, outgoing)
     ref = ob.ref_scaled_waves(order, k, 0.04, point, outgoing).view(np.complex128).reshape(-1, 4)
-    assert np.array_equal(v, ref[:, 0]) and np.array_equal(g, ref[:, 1:])
+    # per degree: relative to the largest value (gradient) of that degree
+    for l in range(order + 1):
+        s = slice(l * l, (l + 1) ** 2)
+        assert np.max(np.abs(v[s] - ref[s, 0])) <= 1e-12 * max(np.max(np.abs(ref[s, 0])), 1e-300)
+        assert np.max(np.abs(g[s] - ref[s, 1:])) <= 1e-12 * max(np.max(np.abs(ref[s, 1:])), 1e-300)
 
 
 @needs_ref
 @pytest.mark.parametrize("local", [False, True])
 @pytest.mark.parametrize("point", POINTS)
-def test_scaled_expansions_bit_identical(local, point):
+def test_scaled_expansions_match_reference(local, point):
     c = _coefficients(8, 4)
     v, g = bem.scaled_expansion_evaluate(c, 8, 0.06, 10.0, point, local)
     ref = ob.ref_scaled_expansion_evaluate(c, 8, 0.06, 10.0, point, local).view(np.complex128)
-    assert v == ref[0] and np.array_equal(g, ref[1:])
+    assert abs(v - ref[0]) <= 1e-12 * abs(ref[0]) and np.max(np.abs(g - ref[1:])) <= 1e-12 * np.max(np.abs(ref[1:]))
 
 
 @needs_ref
 @pytest.mark.parametrize("order,radius,k,rule", [(6, 0.05, 20.0, 10), (0, 0.1, 1.0, 2), (12, 0.02, 300.0, 14)])
-def test_scaled_spectra_bit_identical(order, radius, k, rule):
+def test_scaled_spectra_match_reference(order, radius, k, rule):
     c = _coefficients(order, 9)
-    assert np.array_equal(bem.scaled_multipole_to_spectrum(c, order, radius, k, rule),
-                          ob.ref_scaled_spectrum(c, order, radius, k, rule, 1))
+    got, want = bem.scaled_multipole_to_spectrum(c, order, radius, k, rule), ob.ref_scaled_spectrum(c, order, radius, k, rule, 1)
+    assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
     rng = np.random.default_rng(3)
     n = 2 * (rule + 1) ** 2
     a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
-    assert np.array_equal(bem.spectrum_to_scaled_local(a, rule, order, k, radius),
-                          ob.ref_scaled_spectrum(a, order, radius, k, rule, -1))
+    got, want = bem.spectrum_to_scaled_local(a, rule, order, k, radius), ob.ref_scaled_spectrum(a, order, radius, k, rule, -1)
+    assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
 
 
 AXIAL_CASES = [
@@ -84,13 +90,17 @@ AXIAL_CASES = [
 
 @needs_ref
 @pytest.mark.parametrize("case", AXIAL_CASES)
-def test_axial_plans_bit_identical(case):
+def test_axial_plans_match_reference(case):
+    """Each m-block row (target degree l) relative to its largest entry: the scaled
+    basis keeps every row O(1), so the bar is uniform."""
     kind, ps, pt, k, a_s, a_t, d, polar, probe = case
     c = _coefficients(ps, ps + pt)
     blocks, got_probe, applied = bem.axial_plan(kind, ps, pt, k, a_s, a_t, d, polar, probe, c)
     ref, ref_probe, ref_applied = ob.ref_axial_plan(kind, ps, pt, k, a_s, a_t, d, polar, probe, c)
-    assert np.array_equal(np.concatenate([b.ravel() for b in blocks]), ref)
-    assert got_probe == ref_probe and np.array_equal(applied, ref_applied)
+    got = np.concatenate([b.ravel() for b in blocks])
+    assert np.max(np.abs(got - ref)) <= 1e-11 * np.max(np.abs(ref))
+    assert got_probe == ref_probe
+    assert np.max(np.abs(applied - ref_applied)) <= 1e-11 * np.max(np.abs(ref_applied))
 
 
 # ---------------------------------------------------------------- mathematics
@@ -198,7 +208,7 @@ def test_axial_blocks_shape_and_probe_defaults():
     assert bem.axial_plan(M2L, 3, 3, 5.0, 0.02, 0.3, 0.2, 7)[1] == 0.1
     with pytest.raises(ValueError, match="polar"):
         bem.axial_plan(M2L, 3, 3, 5.0, 0.02, 0.05, 0.2, 6)
-    with pytest.raises(ValueError, match="probe"):
+    with pytest.raises(ValueError, match="singular centre"):
         bem.axial_plan(M2L, 3, 3, 5.0, 0.02, 0.05, 0.04, 7, projection_radius=0.05)
 
 
