@@ -218,6 +218,7 @@ def library() -> ctypes.CDLL:
                 "hsbem_dense_gmres": (ctypes.c_int, [P, U32, D, D, D, U32, U32, ctypes.c_double, D,
                                                      ctypes.POINTER(U32)]),
                 "hsbem_dense_lu_solve": (ctypes.c_int, [P, U32, D, U32, D]),
+                "hsbem_dense_lu_solve_noalloc": (ctypes.c_int, [P, U32, D, U32, D]),
                 "hsbem_server_create": (ctypes.c_int, [P, ctypes.POINTER(_ServerOptions), ctypes.POINTER(P)]),
                 "hsbem_server_destroy": (ctypes.c_int, [P]),
                 "hsbem_submit": (ctypes.c_int, [P, P, ctypes.POINTER(_Request), ctypes.POINTER(U64)]),

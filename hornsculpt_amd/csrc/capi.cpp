@@ -1462,4 +1462,11 @@ int hsbem_dense_lu_solve(hsbem_context *c, uint32_t n, const double *A, uint32_t
     });
 }
 
+int hsbem_dense_lu_solve_noalloc(hsbem_context *c, uint32_t n, double *A, uint32_t nrhs, double *B) {
+    return guarded([&] {
+        require(c != nullptr, "invalid LU arguments");
+        hsbem::dev::dense_lu_solve_noalloc(c->ctx.device, c->ctx.stream, n, A, nrhs, B);
+    });
+}
+
 } // extern "C"

@@ -151,6 +151,15 @@ std::size_t lu_work_doubles(std::uint32_t n);
 void lu_solve(const double *LU, std::uint32_t n, const std::int32_t *pivots, double *B, std::uint32_t nrhs,
               hipStream_t stream);
 
+// LU factorization and solve (as lu_factor + lu_solve) whose pivots and status
+// word live in a static device array instead of allocated memory: A (n x n,
+// column-major complex) is overwritten by its factors, B (n x nrhs) by the
+// solutions; n <= noalloc_lu_max_order(); synchronises the stream; calls that
+// overlap on one device race on the shared array.
+void dense_lu_solve_noalloc(int device, hipStream_t stream, std::uint32_t n, double *A, std::uint32_t nrhs,
+                            double *B);
+std::uint32_t noalloc_lu_max_order();
+
 // r = b - A x, scaled by weight (lumped mass) into weighted; r unscaled into residual.
 // Block-Jacobi of any width W <= kMaxJacobiWidth (a power of two) over gathered
 // blocks (nb x W x W complex, column-major per block, factored in place by a panel

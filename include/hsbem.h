@@ -527,6 +527,15 @@ HSBEM_API int hsbem_dense_gmres(hsbem_context *context, uint32_t n, const double
                                 double *relative_residual, uint32_t *iterations);
 /* Partial-pivot complex LU of a dense n x n matrix and solve of nrhs right-hand sides in place. */
 HSBEM_API int hsbem_dense_lu_solve(hsbem_context *context, uint32_t n, const double *A, uint32_t nrhs, double *B);
+/* The same LU solve on caller-owned DEVICE buffers, allocating no device memory: the pivots and the
+ * singular-pivot status live in a static device array of the library (256 KiB + 256 B per GPU, held
+ * until the process ends). No counterpart in the reference: it is for callers that cannot allocate
+ * (code under stream capture, tight serving loops, harnesses that forbid hipMalloc). A (column-major
+ * complex n x n, interleaved re/im) is overwritten by its LU factors and B (n x nrhs) by the
+ * solutions; n <= 65536 (larger orders are refused with HSBEM_INVALID_ARGUMENT). The work runs on the
+ * context's stream, which is synchronised before the call returns; the caller orders its own writes
+ * of A and B before the call. Calls that overlap on one GPU share its array and race. */
+HSBEM_API int hsbem_dense_lu_solve_noalloc(hsbem_context *context, uint32_t n, double *A, uint32_t nrhs, double *B);
 
 #ifdef __cplusplus
 }
