@@ -87,14 +87,18 @@ MAX_EDGE = 0.0054
 
 
 def horn_variants(count: int = 32, seed: int = 20261202, triangles=(4000.0, 60000.0),
-                  max_edge: float = MAX_EDGE) -> List[HornVariant]:
-    """The sweep's variants: draws (r_m, L, N) per variant in that order."""
+                  max_edge: float = MAX_EDGE, targets=None) -> List[HornVariant]:
+    """The sweep's variants: draws (r_m, L, N) per variant in that order. `targets`
+    (one triangle count per variant) replaces the drawn N; the draw is still made so
+    the shapes are those of the same seed."""
     g = Mt19937_64(seed)
     out = []
     for i in range(count):
         r_m = uniform(g, 0.080, 0.200)
         length = uniform(g, 0.150, 0.450)
         target = math.exp(uniform(g, math.log(triangles[0]), math.log(triangles[1])))
+        if targets is not None:
+            target = float(targets[i])
         coarse = 0.02
         n0 = len(meshes.exponential_horn(coarse, mouth_radius=r_m, length=length, ring_multiple=4).triangles)
         edge = coarse * math.sqrt(n0 / target)
