@@ -331,7 +331,7 @@ int hsbem_mesh_validate(const double *vertices, uint32_t nv, const uint32_t *tri
         for (uint32_t i = 0; i < nv; ++i) {
             m.vertices[i] = hsbem::vsub({vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2]}, origin);
             if (!std::isfinite(m.vertices[i].x) || !std::isfinite(m.vertices[i].y) || !std::isfinite(m.vertices[i].z))
-                throw std::invalid_argument("BEM mesh coordinates are not finite.");
+                throw std::invalid_argument("mesh vertex coordinates must be finite");
         }
         m.triangles.resize(nt);
         for (uint32_t t = 0; t < nt; ++t)

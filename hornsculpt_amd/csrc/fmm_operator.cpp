@@ -779,7 +779,7 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
     const unsigned degree = std::max(minimum_endpoint_degree(max_kh, n.quadrature_order),
                                      endpoint_degree ? endpoint_degree : fs.endpoint_degree);
     if (degree > 14)
-        throw std::runtime_error("FMM endpoint quadrature above degree 14 is not available.");
+        throw std::runtime_error("fast multipole panel-rule comparison above degree 14 is not available.");
     st.endpoint_degree = degree;
     for (const auto &q : triangle_rule(degree)) {
         rule.insert(rule.end(), {q.bary[0], q.bary[1], q.bary[2], q.weight});
@@ -1201,7 +1201,7 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
         for (std::uint32_t e = 0; e < ne; ++e) {
             bnorm[e] = wnorm(vec(ws.fmm_b, e));
             if (!std::isfinite(bnorm[e]))
-                throw std::runtime_error("Galerkin right-hand side is not finite.");
+                throw std::runtime_error("the assembled right-hand side is not finite");
         }
     };
     norms_of_b();
@@ -1319,7 +1319,7 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
         // endpoint-quadrature comparison: the audited action and B at the next rule
         const unsigned degree = A.statistics().endpoint_degree, next = refined_endpoint_degree(degree);
         if (!next)
-            throw std::runtime_error("FMM endpoint quadrature has no independent refinement rule.");
+            throw std::runtime_error("fast multipole panel rule cannot be refined further for the endpoint comparison");
         FmmSettings cset = fs;
         if (!direct)
             cset.accuracy_seed = fr.fmm_verification_seed; // the verified seed, the next rule
@@ -1350,9 +1350,9 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
         if (degree >= 12) {
             std::ostringstream msg;
             msg.precision(6);
-            msg << "FMM endpoint quadrature " << degree << "/" << next << " did not converge: A/B action difference "
+            msg << "fast multipole panel-rule comparison " << degree << "/" << next << " did not converge: A/B action difference "
                 << quad << " exceeds " << 0.025 * tol << " or the higher-rule residual " << next_residual
-                << " exceeds " << tol << "; refine the mesh.";
+                << " exceeds " << tol << "; use a finer mesh";
             throw std::runtime_error(msg.str());
         }
         // promote the operator's rule; the right-hand side follows the new rule
@@ -1389,7 +1389,7 @@ void MeshSolver::fmm_point(const PreparedRequest &P, Workspace &ws, double k, do
         if (!(relative[e] <= tol)) {
             std::ostringstream msg;
             msg.precision(9);
-            msg << "Burton-Miller residual " << relative[e] << " exceeds tolerance " << tol << " at "
+            msg << "relative residual " << relative[e] << " is above the requested tolerance " << tol << " at "
                 << omega / (2 * std::numbers::pi) << " Hz (fast multipole form).";
             throw std::runtime_error(msg.str());
         }

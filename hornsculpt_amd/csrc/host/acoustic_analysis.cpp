@@ -20,7 +20,7 @@ double fraction(double value, double a, double b) {
 void check_samples(const std::vector<AngularSample> &s) {
     for (std::size_t i = 0; i < s.size(); ++i)
         if (!std::isfinite(s[i].angle) || (i != 0 && s[i].angle <= s[i - 1].angle))
-            throw std::invalid_argument("Angular samples must be finite and strictly increasing.");
+            throw std::invalid_argument("angles must be finite and strictly increasing");
 }
 
 bool finite(Vec3 v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z); }
@@ -40,7 +40,7 @@ Vec3 unit(Vec3 v) {
 
 double spl_db(std::complex<double> p, double reference) {
     if (!std::isfinite(p.real()) || !std::isfinite(p.imag()) || !std::isfinite(reference) || reference <= 0.0)
-        throw std::invalid_argument("SPL requires finite pressure and a positive reference.");
+        throw std::invalid_argument("sound pressure level needs a finite pressure and a positive reference pressure");
     const double scale = std::max(std::abs(p.real()), std::abs(p.imag()));
     if (scale == 0.0)
         return -kInf;
@@ -51,10 +51,10 @@ double spl_db(std::complex<double> p, double reference) {
 std::complex<double> normalized_volume_impedance(std::complex<double> z, double area, double density, double c) {
     if (!std::isfinite(z.real()) || !std::isfinite(z.imag()) || !std::isfinite(area) || area <= 0.0 ||
         !std::isfinite(density) || density <= 0.0 || !std::isfinite(c) || c <= 0.0)
-        throw std::invalid_argument("Impedance normalization requires finite positive SI data.");
+        throw std::invalid_argument("impedance normalisation needs finite values and positive area, density and sound speed");
     const auto out = z * ((area / density) / c);
     if (!std::isfinite(out.real()) || !std::isfinite(out.imag()))
-        throw std::invalid_argument("Normalized impedance exceeds the finite numeric range.");
+        throw std::invalid_argument("normalised impedance overflows");
     return out;
 }
 
@@ -133,7 +133,7 @@ CtaCuts cta2034_cuts(const std::vector<AngularSample> &horizontal, const std::ve
 
 std::optional<double> angular_level_db(const std::vector<AngularSample> &s, double angle) {
     if (!std::isfinite(angle))
-        throw std::invalid_argument("Reference angle must be finite.");
+        throw std::invalid_argument("the reference angle must be finite");
     check_samples(s);
     const auto right = std::lower_bound(s.begin(), s.end(), angle,
                                         [](const AngularSample &x, double a) { return x.angle < a; });
@@ -149,7 +149,7 @@ std::optional<double> angular_level_db(const std::vector<AngularSample> &s, doub
 
 std::optional<Beamwidth> beamwidth(const std::vector<AngularSample> &s, double attenuation) {
     if (!std::isfinite(attenuation) || attenuation <= 0.0)
-        throw std::invalid_argument("Beamwidth attenuation must be finite and positive.");
+        throw std::invalid_argument("beamwidth attenuation must be a finite positive number of dB");
     const auto axis_level = angular_level_db(s, 0.0);
     if (!axis_level)
         return std::nullopt;
@@ -188,13 +188,13 @@ std::optional<Beamwidth> beamwidth(const std::vector<AngularSample> &s, double a
 std::vector<std::optional<double>> angular_frequency_response(const std::vector<AngularRow> &rows, double angle,
                                                               bool relative, double reference_angle) {
     if (!std::isfinite(angle) || !std::isfinite(reference_angle))
-        throw std::invalid_argument("Response and reference angles must be finite.");
+        throw std::invalid_argument("response and reference angles must be finite");
     std::vector<std::optional<double>> out;
     out.reserve(rows.size());
     for (std::size_t i = 0; i < rows.size(); ++i) {
         const AngularRow &row = rows[i];
         if (!std::isfinite(row.frequency) || row.frequency <= 0 || (i != 0 && row.frequency <= rows[i - 1].frequency))
-            throw std::invalid_argument("Response frequencies must be positive and strictly increasing.");
+            throw std::invalid_argument("response frequencies must be positive and strictly increasing");
         auto level = angular_level_db(row.samples, angle);
         if (level && relative) {
             const auto reference = angular_level_db(row.samples, reference_angle);

@@ -90,7 +90,7 @@ double diameter(const std::array<Vec3, 3> &p) {
 inline std::array<std::complex<double>, 4> kernels(Vec3 d, double k, double scale) {
     const double r = std::sqrt(vnorm2(d));
     if (!(r > 0))
-        throw std::invalid_argument("Disjoint quadrature panels intersect or contain duplicate geometric nodes.");
+        throw std::invalid_argument("near pair: the two panels overlap or repeat a vertex position");
     const auto g = std::exp(std::complex<double>(0, -k * r)) / r;
     const auto dg = -g * std::complex<double>(1, k * r) * (scale / (r * r));
     return {g, dg * d.x, dg * d.y, dg * d.z};
@@ -233,7 +233,7 @@ std::vector<SubTri> projection_partition(const Panel &test, const Panel &trial) 
             }
         }
     if (std::abs(area - 1) > 1024 * std::numeric_limits<double>::epsilon())
-        throw std::runtime_error("Projected near-pair partition lost reference area.");
+        throw std::runtime_error("near pair: the projected fan partition does not cover the panel's area");
     return out;
 }
 
@@ -255,7 +255,7 @@ double max_norm(const Moments &m) {
     for (const auto &v : m) {
         const double a = std::abs(v);
         if (!std::isfinite(a))
-            throw std::runtime_error("Near-pair quadrature produced a non-finite moment.");
+            throw std::runtime_error("near pair: a quadrature moment is not finite");
         out = std::max(out, a);
     }
     return out;
@@ -274,7 +274,7 @@ double panel_distance(const Panel &test, const Panel &trial) {
 
 bool is_near_pair(const Panel &test, const Panel &trial, double ratio) {
     if (!std::isfinite(ratio) || ratio < 0)
-        throw std::invalid_argument("Near distance ratio must be finite and nonnegative.");
+        throw std::invalid_argument("near distance ratio must be finite and not negative");
     if (ratio == 0)
         return false;
     if (classify_pair_ids(test.nodes.data(), trial.nodes.data()).topology != Topology::disjoint)
@@ -353,7 +353,7 @@ AdaptivePair::AdaptivePair(const Panel &test, const Panel &trial, double toleran
     : test_(test), trial_(trial), tolerance_(tolerance), k_(k), max_depth_(max_depth) {
     if (!std::isfinite(k) || k < 0 || !std::isfinite(tolerance) || tolerance <= 0 || max_depth > 16 ||
         classify_pair_ids(test.nodes.data(), trial.nodes.data()).topology != Topology::disjoint)
-        throw std::invalid_argument("Invalid adaptive pair settings or non-disjoint topology.");
+        throw std::invalid_argument("near pair: invalid refinement settings or the panels share a vertex");
     projection_ = use_projection(test, trial, std::max(test.max_edge, trial.max_edge));
     promote_far_root_ = !projection_ && !is_near_pair(test, trial, 1.0);
     if (projection_) {
@@ -364,7 +364,7 @@ AdaptivePair::AdaptivePair(const Panel &test, const Panel &trial, double toleran
             initial_.push_back(s);
         }
         if (initial_.size() > 16)
-            throw std::logic_error("Projected line partition exceeds its triangle bound.");
+            throw std::logic_error("near pair: the projected partition has too many pieces");
     } else {
         initial_.push_back(Sub{});
     }
@@ -383,7 +383,7 @@ void AdaptivePair::request_initial() {
 void AdaptivePair::include(Leaf l) {
     absolute_error_ += l.error;
     if (!std::isfinite(absolute_error_))
-        throw std::runtime_error("Non-finite accumulated quadrature error.");
+        throw std::runtime_error("near pair: the accumulated error estimate is not finite");
     for (unsigned i = 0; i < 36; ++i)
         integral_[i] += l.integral[i];
     const bool outer_limit = l.depth == max_depth_ && l.degree == 14;
@@ -422,10 +422,10 @@ void AdaptivePair::advance() {
         }
         const double ref = max_norm(integral_);
         if (!(ref > 0))
-            throw std::runtime_error("Near-pair quadrature has no finite nonzero reference scale.");
+            throw std::runtime_error("near pair: no finite nonzero scale to measure the error against");
         result_.estimated_relative_error = absolute_error_ / ref;
         if (!std::isfinite(result_.estimated_relative_error) || result_.estimated_relative_error > tolerance_)
-            throw std::runtime_error("Near-pair quadrature global error estimate exceeds the requested tolerance.");
+            throw std::runtime_error("near pair: refinement ended above the requested tolerance");
         finished_ = true;
         return;
     }
@@ -444,7 +444,7 @@ void AdaptivePair::advance() {
         requested_leaves_.push_back(make_leaf(worst.test, worst.trial, worst.depth, 14, worst.intervals));
     } else {
         if (pending_.size() + at_limit_.size() + 4 > 50000)
-            throw std::runtime_error("Near-pair quadrature exceeds the fifty-thousand-patch safety limit.");
+            throw std::runtime_error("near pair: refinement needs more than 50000 patches");
         auto split = [](const Sub &s) {
             const Bary a = s.v[0], b = s.v[1], c = s.v[2];
             const Bary ab = midpoint(a, b), bc = midpoint(b, c), ca = midpoint(c, a);
@@ -465,11 +465,11 @@ void AdaptivePair::advance() {
 
 void AdaptivePair::commit(const std::vector<PatchEstimate> &estimates) {
     if (finished_ || estimates.size() != requested_leaves_.size())
-        throw std::invalid_argument("Adaptive patch commit count/state mismatch.");
+        throw std::invalid_argument("near pair: internal patch bookkeeping is inconsistent");
     for (const auto &e : estimates) {
         if (!(e.absolute_error >= 0) || !std::isfinite(e.absolute_error) || !(e.inner_error >= 0) ||
             !std::isfinite(e.inner_error) || !(e.outer_error >= 0) || !std::isfinite(e.outer_error))
-            throw std::runtime_error("Near-pair quadrature produced a non-finite error estimate.");
+            throw std::runtime_error("near pair: an error estimate is not finite");
         (void)max_norm(e.integral);
     }
     const auto leaves = requested_leaves_;

@@ -16,7 +16,7 @@ constexpr double kInf = std::numeric_limits<double>::infinity();
 
 void require_spherical_domain(const WidebandLevelPlan &p) {
     if (p.electrical_radius > kMaximumSphericalElectricalRadius || p.signature_order > kMaximumSphericalOrder)
-        throw std::runtime_error("Conditioned support LF plan exceeds the supported spherical domain.");
+        throw std::runtime_error("spherical expansion plan: the conditioned order is beyond the supported range");
 }
 
 } // namespace
@@ -24,13 +24,13 @@ void require_spherical_domain(const WidebandLevelPlan &p) {
 WidebandLevelPlan make_wideband_level_plan(double radius, double k, double delta, double eps) {
     if (!std::isfinite(radius) || radius < 0 || !std::isfinite(k) || k <= 0 || !std::isfinite(delta) || delta <= 1 ||
         !std::isfinite(eps) || eps <= 0 || eps >= 1)
-        throw std::invalid_argument("Wideband level planning requires finite positive physical scales and accuracy.");
+        throw std::invalid_argument("level planning needs finite positive box sizes, wavenumber and accuracy");
     WidebandLevelPlan p;
     p.active_radius = radius;
     p.electrical_radius = k * radius;
     p.accuracy_seed = eps;
     if (!std::isfinite(p.electrical_radius))
-        throw std::invalid_argument("Wideband active electrical radius is non-finite.");
+        throw std::invalid_argument("level planning: electrical radius is not finite");
     if (radius == 0)
         return p;
     const double kr = p.electrical_radius;
@@ -39,7 +39,7 @@ WidebandLevelPlan make_wideband_level_plan(double radius, double k, double delta
     const double top = std::max(low, high);
     const double order = top * std::pow(std::pow(low / top, 4) + std::pow(high / top, 4), .25);
     if (!std::isfinite(order) || std::ceil(order) > kMaximumAngularOrder / 2)
-        throw std::runtime_error("Wideband full angular order exceeds the supported resource contract.");
+        throw std::runtime_error("level planning: plane-wave order is beyond the supported range");
     p.signature_order = static_cast<unsigned>(std::ceil(order));
     p.translation_order = p.quadrature_order = 2 * p.signature_order;
     p.spherical_coefficients = static_cast<std::uint64_t>(p.signature_order + 1) * (p.signature_order + 1);
@@ -57,10 +57,10 @@ double enforce_diagonal_stability(WidebandLevelPlan &p, double k, double distanc
     if (p.representation != LevelRepresentation::plane_wave || distance == 0)
         return 0;
     if (!std::isfinite(distance) || distance <= 0 || !std::isfinite(k) || k <= 0)
-        throw std::invalid_argument("Support diagonal conditioning requires a positive physical distance.");
+        throw std::invalid_argument("diagonal-translation check needs a positive separation");
     const double z = k * distance;
     if (!std::isfinite(z) || !(z > 0))
-        throw std::invalid_argument("Support diagonal electrical distance is invalid.");
+        throw std::invalid_argument("diagonal-translation check: electrical separation is not finite");
     // h_l^(2)(z) by upward recurrence from h_0 = (sin z + i cos z)/z, h_1 = h_0 (1/z + i).
     using C = std::complex<double>;
     double sz, cz;
@@ -87,7 +87,7 @@ SupportExpansionPlan make_support_expansion_plan(const SupportTree &tree, const 
                                                  double k, double seed) {
     if (face_column.size() != tree.flux_leaf.size() || !std::isfinite(k) || k <= 0 || !std::isfinite(seed) ||
         seed <= 0 || seed >= 1)
-        throw std::invalid_argument("Support expansion geometry or accuracy seed is invalid.");
+        throw std::invalid_argument("expansion plan: invalid geometry or accuracy seed");
     const std::size_t nn = tree.nodes.size();
     SupportExpansionPlan out;
     out.levels.resize(tree.levels.size());
@@ -99,7 +99,7 @@ SupportExpansionPlan make_support_expansion_plan(const SupportTree &tree, const 
             continue;
         for (std::uint32_t n = tree.flux_leaf[panel]; n != kNoSupportNode; n = tree.nodes[n].parent) {
             if (n >= nn)
-                throw std::invalid_argument("Support expansion flux leaf is invalid.");
+                throw std::invalid_argument("expansion plan: a flux leaf index is out of range");
             if (flux[n])
                 break;
             flux[n] = 1;
@@ -109,7 +109,7 @@ SupportExpansionPlan make_support_expansion_plan(const SupportTree &tree, const 
     std::vector<std::array<double, 3>> delta(nn, {kInf, kInf, kInf});
     auto visit = [&](const SupportPair &pr, bool is_flux) {
         if (pr.source >= nn || pr.target >= nn)
-            throw std::invalid_argument("Support expansion far endpoint is invalid.");
+            throw std::invalid_argument("expansion plan: a far-pair node index is out of range");
         if (is_flux && !flux[pr.source])
             return;
         const SupportNode &s = tree.nodes[pr.source], &t = tree.nodes[pr.target];
@@ -117,7 +117,7 @@ SupportExpansionPlan make_support_expansion_plan(const SupportTree &tree, const 
         const double d = std::hypot(t.center.x - s.center.x, t.center.y - s.center.y, t.center.z - s.center.z);
         if (s.level != t.level || s.level >= tree.levels.size() || !(rs > 0) || !(rt > 0) || !std::isfinite(d) ||
             !(d > rs + rt))
-            throw std::invalid_argument("Support expansion requires separated equal-level far balls.");
+            throw std::invalid_argument("expansion plan: far pairs must be separated boxes of one level");
         double &ds = delta[pr.source][is_flux ? 1 : 0];
         ds = std::min(ds, (d - rt) / rs);
         delta[pr.target][2] = std::min(delta[pr.target][2], (d - rs) / rt);
@@ -133,11 +133,11 @@ SupportExpansionPlan make_support_expansion_plan(const SupportTree &tree, const 
         lev.source_separation = lev.target_separation = kInf;
         for (std::uint32_t id : tree.levels[l].nodes) {
             if (id >= nn || tree.nodes[id].level != l)
-                throw std::invalid_argument("Support expansion level inventory is invalid.");
+                throw std::invalid_argument("expansion plan: level list is inconsistent");
             const SupportNode &n = tree.nodes[id];
             if (n.parent != kNoSupportNode) {
                 if (n.parent >= nn || tree.nodes[n.parent].level >= l)
-                    throw std::invalid_argument("Support expansion parent ordering is invalid.");
+                    throw std::invalid_argument("expansion plan: a parent does not precede its child");
                 const SupportNode &pa = tree.nodes[n.parent];
                 const double shift = std::hypot(n.center.x - pa.center.x, n.center.y - pa.center.y, n.center.z - pa.center.z);
                 auto nested = [&](double rc, double rp) {
@@ -145,7 +145,7 @@ SupportExpansionPlan make_support_expansion_plan(const SupportTree &tree, const 
                 };
                 if ((n.pressure_count && !nested(n.pressure_radius, pa.pressure_radius)) ||
                     (n.flux_count && flux[id] && !nested(n.flux_radius, pa.flux_radius)))
-                    throw std::invalid_argument("Support expansion requires nested physical support balls.");
+                    throw std::invalid_argument("expansion plan: a child's ball is not inside its parent's");
                 for (int c = 0; c < 3; ++c)
                     delta[id][c] = std::min(delta[id][c], delta[n.parent][c]);
             }
@@ -184,7 +184,7 @@ SupportExpansionPlan make_support_expansion_plan(const SupportTree &tree, const 
         }
         policy = std::min({policy, lev.source_separation, lev.target_separation});
         if (!std::isfinite(policy) || !(policy > 1))
-            throw std::invalid_argument("Support expansion has no finite convergence geometry.");
+            throw std::invalid_argument("expansion plan: convergence ratio is not finite");
         lev.policy_separation = policy;
         lev.expansion = make_wideband_level_plan(lev.expansion.active_radius, k, policy, seed);
         lev.diagonal_condition = enforce_diagonal_stability(lev.expansion, k, lev.minimum_far_distance);

@@ -17,20 +17,20 @@ void check(const RotationPlan &p, const std::vector<std::complex<double>> &c) {
     if (p.order > kMaximumRotationOrder || c.size() != static_cast<std::size_t>(p.order + 1) * (p.order + 1) ||
         p.degree_offsets.size() != p.order + 2 || p.degree_offsets.front() != 0 ||
         p.degree_offsets.back() != p.small_d.size() || p.azimuth.size() != 2 * p.order + 1)
-        throw std::invalid_argument("Spherical rotation dimensions are inconsistent.");
+        throw std::invalid_argument("rotation: coefficient count does not match the order");
     for (unsigned l = 0; l <= p.order; ++l)
         if (p.degree_offsets[l] > p.degree_offsets[l + 1] ||
             p.degree_offsets[l + 1] - p.degree_offsets[l] != static_cast<std::size_t>(2 * l + 1) * (2 * l + 1))
-            throw std::invalid_argument("Spherical rotation degree block is invalid.");
+            throw std::invalid_argument("rotation: a degree block has the wrong size");
     for (const auto &v : c)
         if (!std::isfinite(v.real()) || !std::isfinite(v.imag()))
-            throw std::invalid_argument("Spherical rotation source is non-finite.");
+            throw std::invalid_argument("rotation: source coefficients must be finite");
     for (double v : p.small_d)
         if (!std::isfinite(v))
-            throw std::invalid_argument("Spherical rotation coefficient is non-finite.");
+            throw std::invalid_argument("rotation: a Wigner coefficient is not finite");
     for (const auto &v : p.azimuth)
         if (!std::isfinite(v.real()) || !std::isfinite(v.imag()))
-            throw std::invalid_argument("Spherical rotation azimuth phase is non-finite.");
+            throw std::invalid_argument("rotation: an azimuth phase is not finite");
 }
 
 } // namespace
@@ -51,10 +51,10 @@ std::vector<std::complex<double>> rotation_azimuth(unsigned order, Vec3 axis) {
 
 RotationPlan make_rotation_plan(unsigned order, Vec3 axis) {
     if (order > kMaximumRotationOrder)
-        throw std::invalid_argument("Spherical rotation order exceeds the supported contract.");
+        throw std::invalid_argument("rotation order is beyond the supported range");
     const double len = std::hypot(axis.x, axis.y, axis.z);
     if (!std::isfinite(len) || len <= 0)
-        throw std::invalid_argument("Spherical rotation axis must be finite and nonzero.");
+        throw std::invalid_argument("rotation axis must be finite and nonzero");
     RotationPlan p = wigner_blocks(order, rotation_polar_angle(axis));
     p.azimuth = rotation_azimuth(order, axis);
     return p;
@@ -135,13 +135,13 @@ RcrQueuePolicy make_rcr_queue_policy(unsigned source_order, unsigned target_orde
     constexpr std::size_t kMaximumChunkWork = 64 * 1024 * 1024;
     constexpr unsigned kMaximumQueuedBatches = 32;
     if (source_order > 128 || target_order > 128 || !reserved || reserved > 256)
-        throw std::invalid_argument("LF RCR queue requires supported orders and batch capacity.");
+        throw std::invalid_argument("translation queue: orders or batch capacity out of range");
     const std::size_t src = static_cast<std::size_t>(source_order + 1) * (source_order + 1),
                       tgt = static_cast<std::size_t>(target_order + 1) * (target_order + 1),
                       work = src * (2 * source_order + 2) + tgt * (2 * target_order + 2 + source_order + 1);
     const auto capacity = static_cast<unsigned>(std::min<std::size_t>(reserved, kMaximumChunkWork / work));
     if (!capacity)
-        throw std::runtime_error("One LF RCR link exceeds the bounded queue work domain.");
+        throw std::runtime_error("translation queue: one link needs more work than a batch holds");
     const auto chunks =
         static_cast<unsigned>(std::min<std::size_t>(kMaximumQueuedBatches, kMaximumChunkWork / (capacity * work)));
     return {capacity, chunks, work};

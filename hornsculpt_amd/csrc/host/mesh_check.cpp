@@ -181,13 +181,13 @@ void validate_closed_exterior(const SurfaceMesh &mesh) {
         for (int l = 0; l < 3; ++l) {
             const std::uint32_t a = t.vertices[l], b = t.vertices[(l + 1) % 3];
             if (a >= mesh.vertices.size() || b >= mesh.vertices.size() || a == b)
-                throw std::invalid_argument("BEM requires a closed, outward-oriented, non-intersecting mesh.");
+                throw std::invalid_argument("the mesh must be closed, outward-oriented and free of self-intersections");
             if (++directed[{a, b}] > 1)
-                throw std::invalid_argument("BEM requires a closed, outward-oriented, non-intersecting mesh.");
+                throw std::invalid_argument("the mesh must be closed, outward-oriented and free of self-intersections");
         }
     for (const auto &[edge, count] : directed)
         if (!directed.count({edge.second, edge.first}))
-            throw std::invalid_argument("BEM requires a closed, outward-oriented, non-intersecting mesh.");
+            throw std::invalid_argument("the mesh must be closed, outward-oriented and free of self-intersections");
     // connected components over shared vertices
     std::vector<std::uint32_t> parent(mesh.vertices.size());
     std::iota(parent.begin(), parent.end(), 0u);

@@ -20,7 +20,7 @@ using Pair = std::pair<std::uint32_t, std::uint32_t>;
 
 std::uint32_t checked32(std::size_t v) {
     if (v > std::numeric_limits<std::uint32_t>::max())
-        throw std::runtime_error("BEM interaction plan exceeds 32-bit indexing limits.");
+        throw std::runtime_error("interaction plan is too large for 32-bit indices");
     return static_cast<std::uint32_t>(v);
 }
 
@@ -30,11 +30,11 @@ Permutation classify_checked(const Panel &t, const Panel &s) {
     for (unsigned i = 0; i < 3; ++i)
         for (unsigned j = i + 1; j < 3; ++j)
             if (t.nodes[i] == t.nodes[j] || s.nodes[i] == s.nodes[j])
-                throw std::invalid_argument("Panel contains repeated node IDs.");
+                throw std::invalid_argument("a triangle repeats a vertex index");
     for (unsigned i = 0; i < 3; ++i)
         for (unsigned j = 0; j < 3; ++j)
             if (t.nodes[i] == s.nodes[j] && vnorm2(vsub(t.vertices[i], s.vertices[j])) != 0)
-                throw std::invalid_argument("A shared node ID has inconsistent coordinates.");
+                throw std::invalid_argument("one vertex index is used with two different positions");
     return classify_pair_ids(t.nodes.data(), s.nodes.data());
 }
 
@@ -44,7 +44,7 @@ std::vector<Pair> spatial_candidates(const Space &space, double ratio) {
     double h_max = 0;
     for (const auto &p : space.panels) {
         if (!(p.max_edge > 0) || !std::isfinite(p.max_edge))
-            throw std::invalid_argument("Interaction plan has an invalid panel size.");
+            throw std::invalid_argument("interaction plan: a panel has an invalid size");
         h_max = std::max(h_max, p.max_edge);
     }
     const Vec3 origin = space.panels.front().vertices[0];
@@ -55,14 +55,14 @@ std::vector<Pair> spatial_candidates(const Space &space, double ratio) {
     }
     const double width = (ratio + 2) * h_max + 128 * std::numeric_limits<double>::epsilon() * extent;
     if (!(width > 0) || !std::isfinite(width))
-        throw std::invalid_argument("Interaction-plan grid scale is not finite.");
+        throw std::invalid_argument("interaction plan: candidate grid spacing is not finite");
     using Cell = std::tuple<std::int64_t, std::int64_t, std::int64_t>;
     auto cell_of = [&](Vec3 v) {
         auto axis = [&](double value, double base) {
             const double c = std::floor((value - base) / width);
             constexpr double limit = static_cast<double>(std::numeric_limits<std::int64_t>::max()) / 2;
             if (!std::isfinite(c) || c < -limit || c > limit)
-                throw std::invalid_argument("Interaction-plan spatial index exceeds its coordinate range.");
+                throw std::invalid_argument("interaction plan: candidate grid index out of range");
             return static_cast<std::int64_t>(c);
         };
         return Cell{axis(v.x, origin.x), axis(v.y, origin.y), axis(v.z, origin.z)};
@@ -112,23 +112,23 @@ struct PlannedPair {
 
 double planning_wavenumber(const std::vector<double> &solved, const std::vector<double> &planned, double c) {
     if (solved.empty() || !std::isfinite(c) || c <= 0)
-        throw std::invalid_argument("Interaction planning requires frequencies and sound speed.");
+        throw std::invalid_argument("interaction planning needs frequencies and a sound speed");
     std::set<double> raster;
     double f_max = 0;
     for (double f : planned) {
         if (!std::isfinite(f) || f <= 0 || !raster.insert(f).second)
-            throw std::invalid_argument("Interaction planning frequencies must be finite, positive and unique.");
+            throw std::invalid_argument("planning frequencies must be finite, positive and distinct");
         f_max = std::max(f_max, f);
     }
     for (double f : solved) {
         if (!std::isfinite(f) || f <= 0 || (!planned.empty() && !raster.count(f)))
-            throw std::invalid_argument("Solved frequency is invalid or absent from its interaction planning raster.");
+            throw std::invalid_argument("each solved frequency must be one of the planning frequencies");
         if (planned.empty())
             f_max = std::max(f_max, f);
     }
     const double k = 2 * std::numbers::pi * f_max / c;
     if (!std::isfinite(k))
-        throw std::invalid_argument("Interaction planning wavenumber overflows.");
+        throw std::invalid_argument("planning wavenumber is not finite");
     return k;
 }
 
@@ -138,7 +138,7 @@ InteractionPlan make_interaction_plan(const Space &space, const QuadratureSettin
         !std::isfinite(s.near_distance_ratio) || s.near_distance_ratio < 0 || !std::isfinite(s.near_tolerance) ||
         s.near_tolerance <= 0 || s.near_max_depth > 16 || (s.regular_order != 2 && s.regular_order != 4) ||
         s.singular_order < 2 || s.singular_order > 20 || space.incidence_offsets.size() != space.vertices.size() + 1)
-        throw std::invalid_argument("Invalid BEM interaction-plan input or quadrature settings.");
+        throw std::invalid_argument("invalid interaction-plan input or quadrature settings");
     const std::size_t n = space.panels.size();
     checked32(n);
     InteractionPlan plan;
@@ -199,7 +199,7 @@ InteractionPlan make_interaction_plan(const Space &space, const QuadratureSettin
             rule = adaptive_pair_rule(a, b, s.near_tolerance, s.near_max_depth, k_max);
         } catch (const std::exception &e) {
             std::ostringstream msg;
-            msg << "Near pair panels " << pr.first << '/' << pr.second << " (faces " << a.face_id << '/'
+            msg << "near panels " << pr.first << '/' << pr.second << " (faces " << a.face_id << '/'
                 << b.face_id << ", k=" << k_max << ", gap=" << panel_distance(a, b)
                 << " m, depth=" << s.near_max_depth << "): " << e.what();
             throw std::runtime_error(msg.str());

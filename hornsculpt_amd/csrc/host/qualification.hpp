@@ -73,10 +73,10 @@ HS_HD inline unsigned qualified_regular_degree(const PanelBounds &t, const Panel
 // (regular_qualification.hpp:98-111); k*h above 3.25 is rejected.
 inline unsigned minimum_endpoint_degree(double max_kh, unsigned requested = 4) {
     if (!std::isfinite(max_kh) || max_kh < 0)
-        throw std::invalid_argument("Endpoint quadrature requires a finite electrical panel size.");
+        throw std::invalid_argument("panel rule selection needs a finite k h");
     if (max_kh > 3.25)
-        throw std::runtime_error("BEM panel quadrature: maximum k*h=" + std::to_string(max_kh) +
-                                 " exceeds the qualified electrical range 3.25; refine the mesh.");
+        throw std::runtime_error("largest k h = " + std::to_string(max_kh) +
+                                 " is above the qualified limit 3.25: use a finer mesh");
     const unsigned electrical = max_kh <= .25 ? 4 : max_kh <= 1 ? 6 : max_kh <= 2 ? 8 : 12;
     return std::max(electrical, requested);
 }

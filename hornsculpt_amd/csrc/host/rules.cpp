@@ -15,7 +15,7 @@ namespace hsbem {
 
 std::vector<GaussPoint> gauss_legendre01(unsigned order) {
     if (order < 2 || order > 20)
-        throw std::invalid_argument("Gauss order must be between 2 and 20.");
+        throw std::invalid_argument("Gauss-Legendre order must lie in 2..20");
     std::vector<GaussPoint> rule(order);
     for (unsigned i = 0; i < (order + 1) / 2; ++i) {
         double x = std::cos(std::numbers::pi * (static_cast<double>(i) + .75) / (static_cast<double>(order) + .5));
@@ -33,7 +33,7 @@ std::vector<GaussPoint> gauss_legendre01(unsigned order) {
             if (std::abs(step) < 4 * std::numeric_limits<double>::epsilon())
                 break;
             if (it == 31)
-                throw std::runtime_error("Gauss-Legendre roots did not converge.");
+                throw std::runtime_error("Gauss-Legendre node iteration did not converge");
         }
         const double w = 1 / ((1 - x * x) * dp * dp);
         rule[i] = {(1 - x) * .5, w};
@@ -88,7 +88,7 @@ std::vector<TrianglePoint> triangle_rule(unsigned degree) {
         orbit3(.050547228317031, .898905543365938, .016229248811599);
         orbit6(.263112829634638, .728492392955404, .008394777409958, .0136151570872175);
     } else {
-        throw std::invalid_argument("Triangle quadrature degree must be 2, 4, 6, 8, 12 or 14.");
+        throw std::invalid_argument("triangle rule degree must be one of 2, 4, 6, 8, 12, 14");
     }
     return rule;
 }

@@ -17,13 +17,13 @@ bool finite3(Vec3 v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::i
 
 Space build_space(const SurfaceMesh &mesh) {
     if (mesh.vertices.empty() || mesh.triangles.empty())
-        throw std::invalid_argument("Galerkin space requires a nonempty surface mesh.");
+        throw std::invalid_argument("the surface mesh is empty");
     if (mesh.vertices.size() > std::numeric_limits<std::uint32_t>::max() ||
         mesh.triangles.size() > std::numeric_limits<std::uint32_t>::max())
-        throw std::invalid_argument("Galerkin mesh exceeds the 32-bit panel index limit.");
+        throw std::invalid_argument("the mesh has too many triangles for 32-bit indices");
     for (const auto &v : mesh.vertices)
         if (!finite3(v))
-            throw std::invalid_argument("Galerkin mesh has non-finite coordinates.");
+            throw std::invalid_argument("mesh vertex coordinates must be finite");
 
     Space space;
     space.vertices = mesh.vertices;
@@ -39,7 +39,7 @@ Space build_space(const SurfaceMesh &mesh) {
         p.boundary = tri.boundary;
         for (int l = 0; l < 3; ++l) {
             if (p.nodes[l] >= nv)
-                throw std::invalid_argument("Galerkin triangle has an invalid vertex index.");
+                throw std::invalid_argument("a triangle refers to a vertex that does not exist");
             p.vertices[l] = space.vertices[p.nodes[l]];
         }
         const Vec3 a = p.vertices[0], b = p.vertices[1], c = p.vertices[2];
@@ -50,7 +50,7 @@ Space build_space(const SurfaceMesh &mesh) {
         const double threshold =
             128.0 * std::numeric_limits<double>::epsilon() * p.max_edge * p.max_edge;
         if (!std::isfinite(twice_area) || !std::isfinite(p.max_edge) || twice_area <= threshold)
-            throw std::invalid_argument("Galerkin triangle is degenerate or numerically unresolved.");
+            throw std::invalid_argument("a triangle has (numerically) zero area");
         p.area = 0.5 * twice_area;
         p.normal = over(twice_area_vector, twice_area);
         p.centroid = {a.x + (ab.x + ac.x) / 3.0, a.y + (ab.y + ac.y) / 3.0,
@@ -60,7 +60,7 @@ Space build_space(const SurfaceMesh &mesh) {
             p.gradients[l] = over(vcross(p.normal, opposite[l]), twice_area);
             p.curls[l] = vcross(p.normal, p.gradients[l]);
             if (!finite3(p.gradients[l]) || !finite3(p.curls[l]))
-                throw std::invalid_argument("Galerkin basis gradient is not finite.");
+                throw std::invalid_argument("a triangle's basis gradient is not finite");
             ++space.incidence_offsets[static_cast<std::size_t>(p.nodes[l]) + 1];
             space.lumped_mass[p.nodes[l]] += p.area / 3.0;
         }
@@ -72,7 +72,7 @@ Space build_space(const SurfaceMesh &mesh) {
     }
     for (std::size_t v = 0; v < nv; ++v) {
         if (space.incidence_offsets[v + 1] == 0)
-            throw std::invalid_argument("Galerkin mesh contains an orphan pressure vertex.");
+            throw std::invalid_argument("a vertex belongs to no triangle");
         space.incidence_offsets[v + 1] += space.incidence_offsets[v];
     }
     space.incidences.resize(space.incidence_offsets.back());

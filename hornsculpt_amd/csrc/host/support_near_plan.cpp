@@ -16,7 +16,7 @@ void exclusive_scan(std::vector<std::uint32_t> &offsets) {
     for (auto &o : offsets) {
         sum += o;
         if (sum > UINT32_MAX)
-            throw std::overflow_error("Support near CSR exceeds 32-bit indexing.");
+            throw std::overflow_error("near structure is too large for 32-bit indices");
         o = static_cast<std::uint32_t>(sum);
     }
 }
@@ -25,12 +25,12 @@ void exclusive_scan(std::vector<std::uint32_t> &offsets) {
 void group_by_target(const SupportTree &tree, const std::vector<SupportPair> &pairs, std::vector<std::uint32_t> &offsets,
                      std::vector<std::uint32_t> &indices) {
     if (pairs.size() > UINT32_MAX)
-        throw std::overflow_error("Support near block index overflow.");
+        throw std::overflow_error("near structure: block index overflow");
     offsets.assign(tree.nodes.size() + 1, 0);
     for (const SupportPair &p : pairs) {
         if (p.target >= tree.nodes.size() || p.source >= tree.nodes.size() || !tree.nodes[p.target].leaf() ||
             !tree.nodes[p.source].leaf())
-            throw std::invalid_argument("Support near block is not a leaf Cartesian product.");
+            throw std::invalid_argument("near structure: a near block is not a pair of leaves");
         ++offsets[p.target + 1];
     }
     exclusive_scan(offsets);
@@ -47,14 +47,14 @@ SupportNearPlan make_support_near_plan(const Space &space, const SupportTree &tr
     const std::size_t nv = space.vertices.size(), nt = space.panels.size();
     if (nv != tree.pressure_order.size() || nt != tree.flux_order.size() || tree.nodes.empty() ||
         tree.pressure_leaf.size() != nv || tree.flux_leaf.size() != nt)
-        throw std::invalid_argument("Support near space and hierarchy disagree.");
+        throw std::invalid_argument("near structure: tree and mesh do not match");
     if (faces.size() > UINT32_MAX)
-        throw std::overflow_error("Active near face column index overflow.");
+        throw std::overflow_error("near structure: too many driven face columns");
     SupportNearPlan plan;
     plan.active_faces = faces;
     for (std::size_t i = 0; i < faces.size(); ++i)
         if (std::find(faces.begin(), faces.begin() + i, faces[i]) != faces.begin() + i)
-            throw std::invalid_argument("Active near face IDs are duplicated.");
+            throw std::invalid_argument("near structure: a driven face id is listed twice");
     plan.panel_face_column.assign(nt, kNone);
     std::vector<char> seen(faces.size(), 0);
     for (std::size_t p = 0; p < nt; ++p) {
@@ -62,13 +62,13 @@ SupportNearPlan make_support_near_plan(const Space &space, const SupportTree &tr
         if (it == faces.end())
             continue;
         if (space.panels[p].boundary != Boundary::driven)
-            throw std::invalid_argument("Active near face includes a wall panel.");
+            throw std::invalid_argument("near structure: a listed face has a wall panel");
         const auto column = static_cast<std::uint32_t>(it - faces.begin());
         plan.panel_face_column[p] = column;
         seen[column] = 1;
     }
     if (std::find(seen.begin(), seen.end(), 0) != seen.end())
-        throw std::invalid_argument("Active near face has no driven panels.");
+        throw std::invalid_argument("near structure: a listed face has no driven panel");
 
     // flux CSR: one column set per target leaf, shared by its P1 rows
     group_by_target(tree, tree.pressure_flux.near, plan.flux_block_offsets, plan.flux_block_indices);
@@ -120,13 +120,13 @@ SupportNearPlan make_support_near_plan(const Space &space, const SupportTree &tr
         for (std::uint32_t b = pp_offsets[n]; b < pp_offsets[n + 1]; ++b)
             count += tree.nodes[tree.pressure_pressure.near[pp_indices[b]].source].pressure_count;
         if (count > nv)
-            throw std::invalid_argument("Support near source DOFs are duplicated.");
+            throw std::invalid_argument("near structure: a source node appears twice in a leaf");
         for (std::uint32_t j = node.first_pressure; j < node.first_pressure + node.pressure_count; ++j)
             plan.pressure.row_offsets[tree.pressure_order[j] + 1] = static_cast<std::uint32_t>(count);
     }
     exclusive_scan(plan.pressure.row_offsets);
     if (plan.pressure.row_offsets.back() != tree.pressure_pressure.represented_near)
-        throw std::invalid_argument("Support near CSR and represented pair count disagree.");
+        throw std::invalid_argument("near structure: row lengths do not add up to the pair count");
     plan.pressure.columns.assign(plan.pressure.row_offsets.back(), 0);
     std::vector<std::uint32_t> cols;
     for (std::size_t n = 0; n < tree.nodes.size(); ++n) {
@@ -141,7 +141,7 @@ SupportNearPlan make_support_near_plan(const Space &space, const SupportTree &tr
         }
         std::sort(cols.begin(), cols.end());
         if (std::adjacent_find(cols.begin(), cols.end()) != cols.end())
-            throw std::invalid_argument("Support near CSR duplicates a source DOF.");
+            throw std::invalid_argument("near structure: a row repeats a column");
         for (std::uint32_t j = node.first_pressure; j < node.first_pressure + node.pressure_count; ++j)
             std::copy(cols.begin(), cols.end(), plan.pressure.columns.begin() + plan.pressure.row_offsets[tree.pressure_order[j]]);
     }
@@ -165,7 +165,7 @@ SupportNearPlan make_support_near_plan(const Space &space, const SupportTree &tr
                 }
             }
         if (plan.support_panels.size() > UINT32_MAX)
-            throw std::overflow_error("Support panel reference index overflow.");
+            throw std::overflow_error("near structure: panel reference overflow");
         std::sort(plan.support_panels.begin() + static_cast<std::ptrdiff_t>(first), plan.support_panels.end());
         plan.node_panel_offsets[n + 1] = static_cast<std::uint32_t>(plan.support_panels.size());
     }
@@ -177,7 +177,7 @@ std::vector<SupportNearJob> support_near_jobs(const Space &space, const SupportT
     if (plan.panel_face_column.size() != nt || plan.node_panel_offsets.size() != tree.nodes.size() + 1 ||
         plan.flux_block_offsets.size() != tree.nodes.size() + 1 ||
         plan.flux_block_indices.size() != tree.pressure_flux.near.size())
-        throw std::invalid_argument("Support near iterator plan and geometry disagree.");
+        throw std::invalid_argument("near structure: job list and mesh do not match");
     // test panels in contiguous ranges per thread, concatenated in order: the
     // result equals the single-threaded iterator order
     const unsigned workers = static_cast<unsigned>(

@@ -37,7 +37,7 @@ class TreeBuilder {
     SupportTree run() {
         const std::size_t nv = sp_.vertices.size(), nt = sp_.panels.size();
         if (!nv || !nt || sp_.incidence_offsets.size() != nv + 1 || nv >= kNoSupportNode || nt >= kNoSupportNode)
-            throw std::invalid_argument("Wideband support geometry is empty or invalid.");
+            throw std::invalid_argument("octree: empty or invalid geometry");
         t_.separation = cfg_.separation;
         t_.near_ratio = cfg_.near_ratio;
         // DP0 supports: one panel each; P1 supports: the incident panel star.
@@ -47,7 +47,7 @@ class TreeBuilder {
             SupportBox b = empty_box();
             for (const Vec3 &v : p.vertices) {
                 if (!std::isfinite(v.x) || !std::isfinite(v.y) || !std::isfinite(v.z))
-                    throw std::invalid_argument("Wideband support coordinates are not finite.");
+                    throw std::invalid_argument("octree: coordinates must be finite");
                 grow(b, {v, v, p.max_edge});
             }
             t_.flux_supports.push_back({p.centroid, b});
@@ -57,12 +57,12 @@ class TreeBuilder {
         for (std::size_t v = 0; v < nv; ++v) {
             const std::size_t lo = sp_.incidence_offsets[v], hi = sp_.incidence_offsets[v + 1];
             if (lo >= hi || hi > sp_.incidences.size())
-                throw std::invalid_argument("Wideband pressure support has invalid incidences.");
+                throw std::invalid_argument("octree: a node's panel list is invalid");
             SupportBox b = empty_box();
             for (std::size_t j = lo; j < hi; ++j) {
                 const std::uint32_t panel = sp_.incidences[j].panel;
                 if (panel >= nt)
-                    throw std::invalid_argument("Wideband support panel index is invalid.");
+                    throw std::invalid_argument("octree: panel index out of range");
                 grow(b, t_.flux_supports[panel].box);
             }
             t_.pressure_supports.push_back({sp_.vertices[v], b});
@@ -78,7 +78,7 @@ class TreeBuilder {
         const double half =
             .5 * std::max({all.upper.x - all.lower.x, all.upper.y - all.lower.y, all.upper.z - all.lower.z});
         if (!(half > 0) || !std::isfinite(half))
-            throw std::invalid_argument("Wideband root bounds are invalid.");
+            throw std::invalid_argument("octree: root box is not finite");
         subdivide(root, std::nextafter(half, kInf), {0, 0, 0}, 0, kNoSupportNode, 0, static_cast<std::uint32_t>(nv), 0,
                   static_cast<std::uint32_t>(nt));
         nest_radii();
@@ -150,7 +150,7 @@ class TreeBuilder {
                             std::uint32_t parent, std::uint32_t fp, std::uint32_t np, std::uint32_t fq,
                             std::uint32_t nq) {
         if (t_.nodes.size() >= kNoSupportNode)
-            throw std::overflow_error("Wideband node index overflow.");
+            throw std::overflow_error("octree: too many boxes for 32-bit indices");
         SupportNode n;
         n.center = c;
         n.half_width = half;
@@ -234,7 +234,7 @@ class TreeBuilder {
         const bool tl = t.leaf(), sl = s.leaf();
         if (far || (tl && sl)) {
             if (interactions_ == cfg_.maximum_interactions)
-                throw std::runtime_error("Wideband interaction memory budget exceeded.");
+                throw std::runtime_error("octree: interaction lists exceed their memory budget");
             ++interactions_;
             const std::uint64_t pairs = static_cast<std::uint64_t>(t.pressure_count) * ns;
             if (far) {
@@ -366,7 +366,7 @@ class TreeBuilder {
 SupportTree make_support_tree(const Space &space, const SupportTreeSettings &settings) {
     if (!settings.leaf_capacity || settings.maximum_depth > 30 || !std::isfinite(settings.separation) ||
         settings.separation <= 1 || !std::isfinite(settings.near_ratio) || settings.near_ratio < 0)
-        throw std::invalid_argument("Wideband support tree settings are invalid.");
+        throw std::invalid_argument("octree: invalid leaf capacity, depth, separation or near ratio");
     return TreeBuilder(space, settings).run();
 }
 

@@ -48,13 +48,13 @@ void check_error_flag(const DeviceBuffer<int> &flag, hipStream_t s) {
     hip_check(hipMemcpyAsync(&code, flag.data(), sizeof(int), hipMemcpyDeviceToHost, s), "error flag download");
     hip_check(hipStreamSynchronize(s), "stream synchronize");
     if (code == dev::ERR_DISTANCE)
-        throw std::runtime_error("BEM quadrature encountered zero or invalid source distance.");
+        throw std::runtime_error("a quadrature sample landed on (or too close to) its source point");
     if (code == dev::ERR_OVERFLOW)
         throw std::runtime_error("BEM calculation produced a non-finite value.");
     if (code == dev::ERR_FIELD)
-        throw std::runtime_error("BEM field integration did not reach its near-field tolerance.");
+        throw std::runtime_error("observer field: near-field refinement did not reach its tolerance");
     if (code == dev::ERR_ESCAPED)
-        throw std::runtime_error("BEM dense regular pair escaped its qualified adaptive plan.");
+        throw std::runtime_error("a panel pair needs a rule the interaction plan did not qualify");
 }
 
 bool point_inside(const std::vector<Panel> &panels, Vec3 p) {
@@ -96,12 +96,12 @@ std::vector<double> rule_table() {
 
 MeshSolver::MeshSolver(Context *ctx, SurfaceMesh mesh) : ctx_(ctx) {
     if (mesh.vertices.empty())
-        throw std::invalid_argument("BEM mesh has no vertices.");
+        throw std::invalid_argument("the mesh has no vertices");
     origin_ = mesh.vertices.front();
     for (auto &v : mesh.vertices) {
         v = vsub(v, origin_);
         if (!std::isfinite(v.x) || !std::isfinite(v.y) || !std::isfinite(v.z))
-            throw std::invalid_argument("BEM mesh coordinates are not finite.");
+            throw std::invalid_argument("mesh vertex coordinates must be finite");
     }
     validate_closed_exterior(mesh);
     space_ = build_space(mesh);
@@ -678,7 +678,7 @@ void dense_gmres_multi(hipStream_t s, const void *A, bool fp32, std::uint32_t n,
                 iterations[e] = it[e];
             } else if (it[e] >= max_it) {
                 std::ostringstream msg;
-                msg << "GMRES did not converge after " << it[e] << "/" << max_it
+                msg << "GMRES stopped without converging after " << it[e] << "/" << max_it
                     << " iterations: relative residual " << beta[e] / bnorm[e];
                 throw std::runtime_error(msg.str());
             } else {
@@ -741,7 +741,7 @@ void dense_gmres_multi(hipStream_t s, const void *A, bool fp32, std::uint32_t n,
                 const C a = col[j], bb = col[j + 1];
                 const double len = std::hypot(std::abs(a), std::abs(bb));
                 if (!(len > 0) || !std::isfinite(len))
-                    throw std::runtime_error("GMRES Arnoldi factorization broke down.");
+                    throw std::runtime_error("GMRES: the Arnoldi process broke down");
                 const C phase = std::abs(a) > 0 ? a / std::abs(a) : C(1, 0);
                 cs[e][j] = std::abs(a) / len;
                 sn[e][j] = phase * std::conj(bb) / len;
@@ -901,7 +901,7 @@ void gmres(hipStream_t s, const KrylovOperator &op, std::uint32_t n, const doubl
             const double len = std::hypot(std::abs(a), std::abs(bb));
             if (!(len > 0) || !std::isfinite(len)) {
                 (void)hipStreamSynchronize(s);
-                throw std::runtime_error("GMRES Arnoldi factorization broke down.");
+                throw std::runtime_error("GMRES: the Arnoldi process broke down");
             }
             const C phase = std::abs(a) > 0 ? a / std::abs(a) : C(1, 0);
             cs[j] = std::abs(a) / len;
@@ -949,7 +949,7 @@ void gmres(hipStream_t s, const KrylovOperator &op, std::uint32_t n, const doubl
     *achieved = beta / bnorm;
     *iterations = it;
     std::ostringstream msg;
-    msg << "GMRES did not converge after " << it << "/" << max_it << " iterations: relative residual " << *achieved;
+    msg << "GMRES stopped without converging after " << it << "/" << max_it << " iterations: relative residual " << *achieved;
     throw std::runtime_error(msg.str());
 }
 
@@ -965,7 +965,7 @@ std::shared_ptr<PreparedRequest> MeshSolver::prepare(const Request &req) const {
         num.residual_tolerance > .01 || num.refinement_steps > 10 || num.solver > 1 || num.precision > 1 ||
         num.preconditioner > 1 ||
         num.gmres_restart < 2 || num.gmres_restart > 300 || num.gmres_max_iterations < 1 || num.acceleration > 1)
-        throw std::invalid_argument("Invalid BEM medium, frequencies, excitation or Galerkin settings.");
+        throw std::invalid_argument("invalid medium, frequencies, excitations or quadrature settings");
     // fast multipole settings (cuda_solver.cu:1057-1067)
     if (num.acceleration == 1 &&
         (num.fmm_leaf_capacity < 4 || num.fmm_leaf_capacity > 256 || !std::isfinite(num.fmm_separation) ||
@@ -973,11 +973,11 @@ std::shared_ptr<PreparedRequest> MeshSolver::prepare(const Request &req) const {
          num.fmm_maximum_iterations > 10000 || num.fmm_restart < 2 || num.fmm_restart > 300 ||
          num.fmm_block_width < 32 || num.fmm_block_width > 2048 ||
          (num.fmm_block_width & (num.fmm_block_width - 1)) != 0))
-        throw std::invalid_argument("Invalid Fast Multipole or GMRES settings.");
+        throw std::invalid_argument("invalid Fast Multipole or GMRES settings (leaf capacity, separation, restart, iterations or block width)");
     if (num.acceleration == 1 && num.precision != 0)
         throw std::invalid_argument("The fast multipole form runs in FP64 (precision must be FP64).");
     if (req.excitations.size() > 1024 || req.observers.size() > 100000)
-        throw std::invalid_argument("BEM request exceeds indexing limits.");
+        throw std::invalid_argument("the request is too large for 32-bit indices");
     static std::atomic<std::uint64_t> next_id{1};
     auto P = std::make_shared<PreparedRequest>();
     P->id = next_id++;
@@ -990,20 +990,20 @@ std::shared_ptr<PreparedRequest> MeshSolver::prepare(const Request &req) const {
         if (!std::isfinite(o.x) || !std::isfinite(o.y) || !std::isfinite(o.z) ||
             !std::isfinite(req.far_reference_distance) || req.far_reference_distance <= 0 ||
             !std::isfinite(P->k_max * req.far_reference_distance))
-            throw std::invalid_argument("Far-field origin and positive reference distance must be finite.");
+            throw std::invalid_argument("far field needs a finite origin and a finite positive reference distance");
     }
     for (const auto &w : req.observers) {
         Vec3 p = vsub(w, req.far_field ? req.far_origin : origin_);
         if (!std::isfinite(w.x) || !std::isfinite(w.y) || !std::isfinite(w.z) || !std::isfinite(p.x) ||
             !std::isfinite(p.y) || !std::isfinite(p.z))
-            throw std::invalid_argument("BEM observation coordinates must be finite.");
+            throw std::invalid_argument("observer coordinates must be finite");
         if (req.far_field) {
             const double d = std::hypot(p.x, p.y, p.z);
             if (!(d > 0) || !std::isfinite(d))
-                throw std::invalid_argument("A far-field direction must differ from its origin.");
+                throw std::invalid_argument("a far-field observer must not sit at the origin");
             p = {p.x / d, p.y / d, p.z / d};
         } else if (point_inside(space_.panels, p)) {
-            throw std::invalid_argument("BEM observation points must be finite and outside the solid.");
+            throw std::invalid_argument("observers must be finite and outside the body");
         }
         P->observers.push_back(p);
     }
@@ -1012,7 +1012,7 @@ std::shared_ptr<PreparedRequest> MeshSolver::prepare(const Request &req) const {
     for (const auto &t : mesh_.triangles) {
         const auto [it, inserted] = conditions.emplace(t.face_id, t.boundary);
         if (!inserted && it->second != t.boundary)
-            throw std::invalid_argument("Conflicting boundary conditions on one face.");
+            throw std::invalid_argument("one face id carries both wall and driven triangles");
     }
     const std::uint32_t nt = this->nt(), nv = this->nv(), ne = static_cast<std::uint32_t>(req.excitations.size());
     std::set<std::uint32_t> active_faces;
@@ -1021,12 +1021,12 @@ std::shared_ptr<PreparedRequest> MeshSolver::prepare(const Request &req) const {
         for (const auto &[face, v] : req.excitations[e].faces) {
             if (!conditions.count(face) || conditions.at(face) != Boundary::driven || !std::isfinite(v.real()) ||
                 !std::isfinite(v.imag()) || !P->face_velocity[e].emplace(face, v).second)
-                throw std::invalid_argument("Excitation must assign each known Driven face at most once.");
+                throw std::invalid_argument("an excitation lists a face twice or a face that is not driven");
             active_faces.insert(face);
             P->has_nonzero |= v != C(0);
         }
         if (P->face_velocity[e].empty())
-            throw std::invalid_argument("Each excitation needs at least one assigned Driven face.");
+            throw std::invalid_argument("every excitation needs at least one driven face");
     }
     P->flux_column.assign(nt, -1);
     for (std::uint32_t t = 0; t < nt; ++t)
@@ -1256,7 +1256,7 @@ FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_
         hip_check(hipStreamSynchronize(s), "norm sync");
         rhs_norm[e] = std::sqrt(v);
         if (!std::isfinite(rhs_norm[e]))
-            throw std::runtime_error("Galerkin right-hand side is not finite.");
+            throw std::runtime_error("the assembled right-hand side is not finite");
     }
     phase(3);
     const auto t_solve = Clock::now();
@@ -1270,7 +1270,7 @@ FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_
         hip_check(hipStreamSynchronize(s), "LU sync");
         fr.ms[2] = ms_since(t_f);
         if (info)
-            throw std::runtime_error("Burton-Miller factorization failed at pivot " + std::to_string(info));
+            throw std::runtime_error("LU factorization hit a zero pivot at column " + std::to_string(info));
         const auto t_t = Clock::now();
         hip_check(hipMemcpyAsync(ws.x.data(), ws.rhs.data(), ws.rhs.bytes(), hipMemcpyDeviceToDevice, s), "rhs copy");
         dev::lu_solve(ws.LU.data(), nv, ws.pivots.data(), ws.x.data(), ne, s);
@@ -1316,7 +1316,7 @@ FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_
             relative[e] = rhs_norm[e] > 0 ? std::sqrt(v) / rhs_norm[e]
                                            : (v == 0 ? 0 : std::numeric_limits<double>::infinity());
             if (!std::isfinite(relative[e]))
-                throw std::runtime_error("Burton-Miller residual is not finite.");
+                throw std::runtime_error("relative residual is not finite.");
             largest = std::max(largest, relative[e]);
         }
         if (largest <= num.residual_tolerance)
@@ -1324,7 +1324,7 @@ FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_
         if (solver_kind != 0 || refinements >= num.refinement_steps) {
             std::ostringstream msg;
             msg.precision(9);
-            msg << "Burton-Miller residual " << largest << " exceeds tolerance " << num.residual_tolerance << " at " << f
+            msg << "relative residual " << largest << " is above the requested tolerance " << num.residual_tolerance << " at " << f
                 << " Hz after " << refinements << " refinement steps.";
             throw std::runtime_error(msg.str());
         }
@@ -1410,8 +1410,8 @@ FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_
             if (passed)
                 break;
             if (degree == 12)
-                throw std::runtime_error("BEM complex observer-field quadrature 12/14 did not converge within "
-                                         "0.025*residualTolerance; refine the mesh or observer separation.");
+                throw std::runtime_error("observer field: the degree 12/14 panel rule did not agree within "
+                                         "0.025 x the residual tolerance; use a finer mesh or observers farther from the surface");
             degree = next;
         }
     } else if (!observers.empty()) {
@@ -1439,7 +1439,7 @@ FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_
             for (auto v : space_.panels[t].nodes)
                 avg += er.pressure_trace[v] / 3.0;
             if (!std::isfinite(avg.real()) || !std::isfinite(avg.imag()))
-                throw std::runtime_error("Burton-Miller pressure is not finite.");
+                throw std::runtime_error("the solved pressure is not finite");
             er.surface_pressure[t] = avg;
             const auto it = P.face_velocity[e].find(mesh_.triangles[t].face_id);
             er.normal_derivative[t] = it == P.face_velocity[e].end() ? C{} : C(0, -req.density * omega) * it->second;
