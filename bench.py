@@ -281,79 +281,212 @@ def run_sweep(variant_count, frequency_count, slots, tolerance):
     return out
 
 
-def run_stream(duration_s, rate, pool_size, triangles, slots, tolerance, precision, seed=20261203):
-    """Open-loop served stream: Poisson arrivals at `rate` requests/s for
-    `duration_s`, drawn from mt19937_64(seed): per request a horn variant of a pool
-    of `pool_size` resident meshes (seeded variants with target triangle counts
-    log-uniform over `triangles`, state keyed by mesh hash), a frequency count
-    round(log-uniform[1, 32]) and that many frequencies log-uniform in
-    [100 Hz, 20 kHz], sorted; a draw with k h_max > 2 at its highest frequency is
-    redrawn. Each mesh's interaction plan is built once, at the highest frequency
-    its longest edge admits, and serves every request on it. Latency of a request =
-    submission to completion of its last point."""
-    g = variants.Mt19937_64(seed)
-    pool = variants.horn_variants(pool_size, seed=seed, triangles=triangles, max_edge=math.inf)
-    solvers, caps, meshes_of = {}, {}, []
-    piston = [bem.BemExcitation("piston", [bem.FaceVelocity(1, 1.0)])]
-    for v in pool:
-        mesh = v.mesh()
-        key = hash((mesh.vertices.tobytes(), mesh.triangles.tobytes()))
-        if key not in solvers:
-            solvers[key] = (bem.HipBemSolver(mesh), piston)
-            # highest frequency with k h_max <= 2, kept 0.1 % inside so rounding cannot
-            # put the planner into its all-pairs mode (k h_max > 2)
-            caps[key] = 0.999 * 2.0 * 343.0 / (2 * math.pi * variants.maximum_edge(mesh))
-        meshes_of.append((key, len(mesh.triangles)))
+STREAM_SEED = 20261203   # arrivals and request draws (mt19937_64)
+POOL_SEED = 20261204     # horn shapes of the resident mesh pool
+DIRECTIVITY_DEG = (0.0, 15.0, 30.0, 45.0, 60.0, 90.0)
+
+
+def cpu_description():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((line.split(":", 1)[1].strip() for line in f if line.startswith("model name")), model)
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"model": model, "nproc": usable}
+
+
+class StreamPool:
+    """Resident horn meshes of the served stream: `size` seeded horn shapes
+    (mt19937_64(POOL_SEED): mouth radius U[80, 200] mm, length U[150, 450] mm) meshed
+    at triangle counts spread log-uniformly over [lo, hi] (stratified, so the pool
+    spans the whole range), one solver per mesh hash, each with the highest frequency
+    its longest edge admits (k h_max <= 2, kept 0.1 % inside)."""
+
+    def __init__(self, size, lo, hi):
+        targets = [lo * (hi / lo) ** (i / max(size - 1, 1)) for i in range(size)]
+        self.entries = []
+        for v in variants.horn_variants(size, seed=POOL_SEED, triangles=(lo, hi), max_edge=math.inf, targets=targets):
+            mesh = v.mesh()
+            cap = 0.999 * 2.0 * 343.0 / (2 * math.pi * variants.maximum_edge(mesh))
+            self.entries.append({"mesh": mesh, "solver": bem.HipBemSolver(mesh), "cap": cap, "variant": v,
+                                 "triangles": len(mesh.triangles), "nodes": len(mesh.vertices)})
+
+    def request(self, index, frequencies, precision, tolerance):
+        e = self.entries[index]
+        length = e["variant"].length
+        # far field about the mouth centre; the horn axis is +x
+        directions = [(math.cos(math.radians(a)), math.sin(math.radians(a)), 0.0) for a in DIRECTIVITY_DEG]
+        return bem.BemRequest(
+            medium=bem.AcousticMedium(1.205, 343.0), frequencies=list(frequencies),
+            excitations=[bem.BemExcitation("piston", [bem.FaceVelocity(1, 1.0)])],
+            observers=[(length + 10 * d[0], 10 * d[1], 10 * d[2]) for d in directions],
+            far_field=bem.FarFieldObservation((length, 0.0, 0.0), 1.0),
+            interaction_plan_frequencies=sorted(list(frequencies) + [e["cap"]]),
+            numerics=bem.BemNumerics(solver=bem.SOLVER_GMRES, relative_residual_tolerance=tolerance, gmres_restart=50,
+                                     gmres_maximum_iterations=3000, precision=precision))
+
+
+def draw_stream(pool, rate, duration_s):
+    """Open-loop Poisson arrivals at `rate` per second for `duration_s`: per request a
+    pool mesh (uniform over the log-spread pool, so log-uniform in size), a frequency
+    count round(log-uniform[1, 32]) and that many frequencies log-uniform in
+    [100 Hz, 20 kHz], sorted; a draw above the mesh's k h_max = 2 limit is redrawn."""
+    g = variants.Mt19937_64(STREAM_SEED)
     draws, redraws, t = [], 0, 0.0
+    n = len(pool.entries)
     while True:
         t += -math.log(1.0 - variants.uniform(g, 0.0, 1.0)) / rate
         if t > duration_s:
             break
-        key, nt = meshes_of[min(pool_size - 1, int(variants.uniform(g, 0.0, pool_size)))]
+        index = min(n - 1, int(variants.uniform(g, 0.0, n)))
         while True:
             count = int(round(math.exp(variants.uniform(g, 0.0, math.log(32.0)))))
             freqs = sorted(math.exp(variants.uniform(g, math.log(100.0), math.log(20000.0))) for _ in range(count))
-            if freqs[-1] <= caps[key]:
+            if freqs[-1] <= pool.entries[index]["cap"]:
                 break
             redraws += 1
-        draws.append((t, key, nt, freqs))
+        draws.append((t, index, freqs))
+    return draws, redraws / max(1, redraws + len(draws))
 
-    def request(key, freqs):
-        solver, excitations = solvers[key]
-        return bem.BemRequest(medium=bem.AcousticMedium(1.205, 343.0), frequencies=freqs,
-                              excitations=excitations, interaction_plan_frequencies=sorted(freqs + [caps[key]]),
-                              numerics=bem.BemNumerics(solver=bem.SOLVER_GMRES, relative_residual_tolerance=tolerance,
-                                                       gmres_restart=50, gmres_maximum_iterations=2000,
-                                                       precision=precision))
 
-    t0 = time.perf_counter()
-    for key in solvers:  # cold: one plan per resident mesh
-        solvers[key][0].solve(request(key, [100.0]))
-        print(f"# stream pool mesh planned ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
-    cold_s = time.perf_counter() - t0
+def serve_stream(pool, draws, precision, tolerance, slots, window_s, warmup_steps, steps, label):
+    """One form over the stream: arrivals submitted on their schedule for the whole
+    run; the first `warmup_steps` windows are untimed, the next `steps` windows are
+    timed (points completed inside them / their duration). Requests still open at
+    the end are cancelled; their latency is censored at the end of the run."""
     server = bem.HipBemServer(slots=slots)
     start = time.perf_counter()
-    tickets = []
-    for i, (at, key, nt, freqs) in enumerate(draws):
-        delay = start + at - time.perf_counter()
-        if delay > 0:
-            time.sleep(delay)
-        tickets.append((server.submit(solvers[key][0], request(key, freqs)), len(freqs), nt))
-        if (i + 1) % 50 == 0:
-            print(f"# stream: {i + 1} of {len(draws)} requests submitted", file=sys.stderr, flush=True)
-    latencies, points = [], 0
-    for ticket, count, _ in tickets:
-        res = server.poll(ticket)
+    timed_from, end = start + warmup_steps * window_s, start + (warmup_steps + steps) * window_s
+    tickets, base_points, base_time, i = [], None, None, 0
+    next_report = start + 15.0
+    while True:
+        now = time.perf_counter()
+        if base_points is None and now >= timed_from:
+            base_points, base_time = server.stats()["points"], now
+        if now >= end:
+            break
+        if now >= next_report:
+            st = server.stats()
+            print(f"# {label}: t={now - start:.0f} s, {i} requests submitted, {st['points']} points done, "
+                  f"{st['queued']} queued", file=sys.stderr, flush=True)
+            next_report += 15.0
+        if i < len(draws) and start + draws[i][0] <= now:
+            tickets.append(server.submit(pool.entries[draws[i][1]]["solver"],
+                                         pool.request(draws[i][1], draws[i][2], precision, tolerance)))
+            i += 1
+            continue
+        wake = min(t for t in (timed_from if base_points is None else end, end,
+                               start + draws[i][0] if i < len(draws) else end, next_report))
+        time.sleep(max(0.0, min(wake - time.perf_counter(), 0.05)))
+    end_points, end_time = server.stats()["points"], time.perf_counter()
+    results, latencies, censored = {}, [], 0
+    kernel_ms, samples = 0.0, 0.0
+    for n, ticket in enumerate(tickets):
+        res = server.poll(ticket, 0)
+        if res is None:
+            server.cancel(ticket)
+            censored += 1
+            latencies.append(1e3 * (end_time - (start + draws[n][0])))
+            continue
+        results[n] = res
         latencies.append(max(fr.served_milliseconds for fr in res.frequencies))
-        points += count
-    elapsed = time.perf_counter() - start
+        for fr in res.frequencies:
+            kernel_ms += fr.kernel_milliseconds["regular"]
+            samples += fr.kernel_milliseconds["regular_samples"]
     stats = server.stats()
     server.close()
-    sizes = sorted({nt for _, nt in meshes_of})
-    return {"requests": len(draws), "points": points, "elapsed_s": elapsed, "points_per_s": points / elapsed,
-            "offered_rate_per_s": rate, "duration_s": duration_s, "p50_ms": float(np.percentile(latencies, 50)),
-            "p99_ms": float(np.percentile(latencies, 99)), "redraw_rate": redraws / max(1, redraws + len(draws)),
-            "pool_triangles": sizes, "cold_s": cold_s, "max_running": stats["max_running"]}
+    timed = end_time - base_time
+    out = {"points_per_s": (end_points - base_points) / timed, "timed_s": timed,
+           "points_timed": end_points - base_points, "requests_submitted": len(tickets),
+           "requests_completed": len(results), "requests_censored": censored,
+           "p50_ms": float(np.percentile(latencies, 50)), "p99_ms": float(np.percentile(latencies, 99)),
+           "latency_censored_at_end": censored > 0, "max_running": stats["max_running"],
+           "peak_held_bytes": stats["peak_held_bytes"], "budget_bytes": stats["budget_bytes"],
+           "regular_kernel_ms_served": kernel_ms, "regular_samples_served": samples}
+    print(f"# {label}: {out['points_per_s']:.2f} points/s over {timed:.0f} s timed, {len(results)}/{len(tickets)} "
+          f"requests done, latency p50 {out['p50_ms']:.0f} ms p99 {out['p99_ms']:.0f} ms", file=sys.stderr, flush=True)
+    return out, results
+
+
+def _directivity_db(observers):
+    p = np.abs(np.asarray(observers))
+    return 20 * np.log10(np.maximum(p, 1e-300) / max(p[0], 1e-300))
+
+
+def compare_forms(pool, draws, exact, mixed):
+    """Per-result error of the mixed form against the plain FP64 form (which equals
+    the FP64 oracle to ~1e-15 in the parity tests) on every request both completed:
+    area-weighted relative L2 of the surface pressure, on-axis SPL difference (dB) and
+    the largest directivity difference (dB, levels relative to on-axis, 0-90 deg)."""
+    worst = {"surface_pressure": 0.0, "on_axis_spl_db": 0.0, "directivity_db": 0.0}
+    points = 0
+    for n in sorted(set(exact) & set(mixed)):
+        mesh = pool.entries[draws[n][1]]["mesh"]
+        for fe, fm in zip(exact[n].frequencies, mixed[n].frequencies):
+            ee, em = fe.excitations[0], fm.excitations[0]
+            worst["surface_pressure"] = max(worst["surface_pressure"], _area_weighted_error(
+                mesh, em.boundary.surface_pressure, ee.boundary.surface_pressure))
+            worst["on_axis_spl_db"] = max(worst["on_axis_spl_db"], abs(
+                bem.spl_db(complex(em.observer_pressure[0])) - bem.spl_db(complex(ee.observer_pressure[0]))))
+            worst["directivity_db"] = max(worst["directivity_db"], float(np.max(np.abs(
+                _directivity_db(em.observer_pressure) - _directivity_db(ee.observer_pressure)))))
+            points += 1
+    worst["points_compared"] = points
+    return worst
+
+
+def roofline_leg(pool, precision, tolerance):
+    """The dominant kernel alone on the GPU: one direct solve (no other slot) of the
+    largest pool mesh at two frequencies of its raster (its limit / 8 and its limit);
+    FP32/FP64 sample flops over the HIP-event time of regular_tiles on the solver's
+    stream."""
+    e = pool.entries[-1]
+    freqs = [min(e["cap"], 20000.0) / 8, min(e["cap"], 20000.0)]
+    res = e["solver"].solve(pool.request(len(pool.entries) - 1, freqs, precision, tolerance))
+    ms = sum(fr.kernel_milliseconds["regular"] for fr in res.frequencies)
+    samples = sum(fr.kernel_milliseconds["regular_samples"] for fr in res.frequencies)
+    launches = len(res.frequencies)
+    return {"mesh_triangles": e["triangles"], "frequencies_hz": freqs, "launch_ms": ms / launches,
+            "flops_per_launch": FLOPS_PER_SAMPLE * samples / launches,
+            "achieved_tflops": FLOPS_PER_SAMPLE * samples / (ms / 1e3) / 1e12}
+
+
+def cpu_stream_baseline(pool, threads):
+    """FP64 oracle on host cores for one point of the smallest pool mesh (full dense
+    assembly + LU) and C1 (icosphere L3, 1 kHz) in full; the per-point time on the
+    stream's mix is scaled from the small mesh by the mix's mean squared node count
+    (assembly cost grows with nv^2; the LU's nv^3 is reported apart)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_bind as ob
+
+    def full_point(mesh, f):
+        k = 2 * math.pi * f / 343.0
+        t0 = time.perf_counter()
+        plan = bem.interaction_plan(mesh, bem.BemNumerics(), k)
+        t1 = time.perf_counter()
+        A, B = ob.oracle_assemble(mesh, plan, k, threads=threads)
+        t2 = time.perf_counter()
+        q = np.array([-1j * 1.205 * 2 * math.pi * f * (1.0 if fid == 1 else 0.0) for fid in mesh.face_ids])
+        ob.oracle_lu_solve(A, B @ q, threads=threads)
+        t3 = time.perf_counter()
+        return t1 - t0, t2 - t1, t3 - t2
+
+    small = pool.entries[0]
+    f_small = min(small["cap"], 20000.0) / 2
+    plan_s, asm_s, lu_s = full_point(small["mesh"], f_small)
+    nv2 = np.mean([e["nodes"] ** 2 for e in pool.entries])
+    nv3 = np.mean([e["nodes"] ** 3 for e in pool.entries])
+    per_point = asm_s * nv2 / small["nodes"] ** 2 + lu_s * nv3 / small["nodes"] ** 3
+    c1_plan, c1_asm, c1_lu = full_point(meshes.icosphere(3), 1000.0)
+    return {"value": 1.0 / per_point, "unit": "freq-points/s", "cores": threads, "kind": "port",
+            "cpu": cpu_description(),
+            "sample": f"FP64 oracle: one full point of the smallest pool mesh ({small['triangles']} triangles, "
+                      f"{f_small:.0f} Hz: assembly {asm_s:.1f} s, LU {lu_s:.2f} s, host plan {plan_s:.1f} s) scaled "
+                      f"to the stream's mesh mix by mean nv^2 (assembly) and nv^3 (LU): {per_point:.0f} s per point",
+            "c1_full_solve_s": c1_plan + c1_asm + c1_lu,
+            "c1_detail": {"plan_s": c1_plan, "assembly_s": c1_asm, "lu_s": c1_lu}}
 
 
 def run_fmm(solver, frequencies, leaf_capacity, repetitions, check=True):
@@ -431,11 +564,110 @@ def run_flange(edge, frequencies, leaf_capacity, tolerance, repeats, block_width
             "points_per_s_warm": len(frequencies) / wall, "runs": runs}
 
 
+def main_stream(args):
+    """The headline: the served stream, plain FP64 then mixed, on the same arrivals."""
+    steps = 3 if args.steps is None else args.steps
+    warmup = 1 if args.warmup is None else args.warmup
+    t0 = time.perf_counter()
+    pool = StreamPool(args.pool, args.pool_min, args.pool_max)
+    sizes = [e["triangles"] for e in pool.entries]
+    print(f"# stream pool: {sizes} triangles ({time.perf_counter() - t0:.1f} s to mesh)", file=sys.stderr, flush=True)
+    if args.roofline_leg:
+        leg = roofline_leg(pool, bem.PRECISION_MIXED, args.mixed_tolerance)  # first solve: plan + kernel
+        leg = roofline_leg(pool, bem.PRECISION_MIXED, args.mixed_tolerance)
+        print(json.dumps({"metric": "regular_tiles FP32 TFLOP/s (isolated leg of the served stream)",
+                          "unit": "TFLOP/s", "value": leg["achieved_tflops"], "higher_is_better": True, "n_gpus": 1,
+                          "dtype": "f32", "data": "synthetic", "config": {"workload": "roofline leg", **leg}}),
+              flush=True)
+        return
+    # Cold: one interaction plan per resident mesh, at its frequency limit, through a
+    # throwaway server so no direct-solve workspace stays allocated.
+    t0 = time.perf_counter()
+    server = bem.HipBemServer(slots=1)
+    for n, e in enumerate(pool.entries):
+        server.poll(server.submit(e["solver"], pool.request(n, [100.0], bem.PRECISION_FP64, args.tolerance)))
+        print(f"# stream pool mesh {n} ({e['triangles']} triangles) planned, {time.perf_counter() - t0:.0f} s",
+              file=sys.stderr, flush=True)
+    server.close()
+    cold_s = time.perf_counter() - t0
+    duration = (warmup + steps) * args.window
+    draws, redraw_rate = draw_stream(pool, args.rate, duration)
+    offered_points = sum(len(d[2]) for d in draws)
+    print(f"# stream: {len(draws)} requests / {offered_points} points offered over {duration:.0f} s", file=sys.stderr,
+          flush=True)
+    forms, results = {}, {}
+    for label, precision, tol in (("served_fp64", bem.PRECISION_FP64, args.tolerance),
+                                  ("served_mixed", bem.PRECISION_MIXED, args.mixed_tolerance)):
+        device_synchronize()
+        forms[label], results[label] = serve_stream(pool, draws, precision, tol, args.slots, args.window, warmup,
+                                                    steps, label)
+        forms[label]["gmres_tolerance"] = tol
+    errors = compare_forms(pool, draws, results["served_fp64"], results["served_mixed"])
+    print(f"# mixed vs FP64 on {errors['points_compared']} points: surface pressure {errors['surface_pressure']:.2e}, "
+          f"on-axis SPL {errors['on_axis_spl_db']:.2e} dB, directivity {errors['directivity_db']:.2e} dB",
+          file=sys.stderr, flush=True)
+    leg = roofline_leg(pool, bem.PRECISION_MIXED, args.mixed_tolerance)
+    print(f"# roofline leg: regular_tiles FP32 {leg['achieved_tflops']:.1f} TF/s, {leg['launch_ms']:.1f} ms per launch",
+          file=sys.stderr, flush=True)
+    head = forms["served_mixed"]
+    line = {
+        "metric": "freq-points solved/s + p50/p99 latency, served mixed vs plain FP64 @<=1e-4 err",
+        "value": head["points_per_s"], "unit": "freq-points/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": 1e3 * head["timed_s"] / steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic (seeded stream of horn-variant requests, meshes generated in-process)",
+        "config": {"workload": f"served stream: open-loop Poisson {args.rate:g} requests/s, {len(sizes)} resident horn "
+                               f"meshes {min(sizes)}-{max(sizes)} triangles (log-spread; request mesh uniform over the "
+                               f"pool), round(log-uniform[1, 32]) frequencies per request log-uniform in 100 Hz-20 kHz "
+                               f"(k h_max <= 2, redraw rate {redraw_rate:.2f}), dense Galerkin assembly + GMRES "
+                               f"(restart 50; FP64 form rel tol {args.tolerance:g}, mixed form {args.mixed_tolerance:g}), "
+                               f"{args.slots} slots, {warmup} untimed + {steps} timed windows of {args.window:g} s",
+                   "pool_triangles": sizes, "rate_per_s": args.rate, "requests_offered": len(draws),
+                   "points_offered": offered_points, "window_s": args.window, "slots": args.slots},
+        "latency_ms": {"p50": head["p50_ms"], "p99": head["p99_ms"]},
+        "forms": forms,
+        "mixed_vs_fp64": {"speedup": head["points_per_s"] / forms["served_fp64"]["points_per_s"],
+                          "max_rel_l2_surface_pressure": errors["surface_pressure"],
+                          "max_on_axis_spl_db": errors["on_axis_spl_db"],
+                          "max_directivity_db": errors["directivity_db"],
+                          "points_compared": errors["points_compared"]},
+        "cold": {"plans_s": cold_s},
+        "roofline": {"kernel": "regular_tiles (FP32 pair quadrature of the mixed form, VALU; isolated leg)",
+                     "bound": "valu", "achieved": leg["achieved_tflops"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": leg["achieved_tflops"] / FP32_PEAK_TFLOPS, "traffic": None,
+                     "launch_ms": leg["launch_ms"], "flops_per_launch": leg["flops_per_launch"],
+                     "leg": {"mesh_triangles": leg["mesh_triangles"], "frequencies_hz": leg["frequencies_hz"]},
+                     "served_achieved": (FLOPS_PER_SAMPLE * head["regular_samples_served"] /
+                                         (head["regular_kernel_ms_served"] / 1e3) / 1e12
+                                         if head["regular_kernel_ms_served"] > 0 else None)},
+    }
+    if not args.no_c2:
+        mesh = meshes.exponential_horn(args.edge)
+        solver = bem.HipBemSolver(mesh)
+        frequencies = [200.0 * 40.0 ** (i / (args.frequencies - 1)) for i in range(args.frequencies)]
+        c2 = {}
+        for name, prec, kind, tol in (("fp64", bem.PRECISION_FP64, bem.SOLVER_GMRES, args.tolerance),
+                                      ("mixed", bem.PRECISION_MIXED, bem.SOLVER_GMRES, args.mixed_tolerance)):
+            f = run_form(solver, frequencies, prec, kind, tol, 1, 1, f"dense horn sweep {name}")
+            c2[name] = {"value": f["value"], "regular_kernel_ms": f["kernel_ms"] / f["points"],
+                        "gmres_iterations": [int(min(f["iterations"])), int(max(f["iterations"]))],
+                        "traces": f["traces"]}
+        err = max(_area_weighted_error(mesh, m, p) for m, p in zip(c2["mixed"].pop("traces"), c2["fp64"].pop("traces")))
+        line["forms"]["dense_horn_sweep"] = {"triangles": len(mesh.triangles), "frequencies": len(frequencies), **c2,
+                                             "mixed_over_fp64": c2["mixed"]["value"] / c2["fp64"]["value"],
+                                             "mixed_vs_fp64_surface_pressure": err}
+        del solver
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_stream_baseline(pool, args.cpu_threads)
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps: stream windows (default 3), or sweeps with --horn (default 2)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 1)")
     ap.add_argument("--edge", type=float, default=0.0052)
     ap.add_argument("--frequencies", type=int, default=16)
     ap.add_argument("--solver", choices=["gmres", "lu"], default="gmres")
@@ -465,11 +697,18 @@ def main():
                     help="only a benchmark fixture file (the reference's solver_benchmark input format; one JSON line)")
     ap.add_argument("--lu", type=int, default=0, metavar="N",
                     help="only the dense complex LU solve of a seeded random N x N system (one JSON line)")
-    ap.add_argument("--stream", action="store_true",
-                    help="only the open-loop served stream of multi-frequency requests on a pool of meshes (one JSON line)")
-    ap.add_argument("--stream-rate", type=float, default=2.0, help="Poisson arrivals per second")
-    ap.add_argument("--stream-seconds", type=float, default=60.0)
-    ap.add_argument("--stream-pool", type=int, default=8)
+    ap.add_argument("--horn", action="store_true",
+                    help="only the dense horn sweep (20.7k triangles, 16 frequencies; one JSON line)")
+    ap.add_argument("--rate", type=float, default=20.0, help="served stream: Poisson arrivals per second")
+    ap.add_argument("--window", type=float, default=30.0, help="served stream: seconds per step (window)")
+    ap.add_argument("--pool", type=int, default=8, help="served stream: resident horn meshes")
+    ap.add_argument("--pool-min", type=float, default=2000.0)
+    ap.add_argument("--pool-max", type=float, default=120000.0)
+    ap.add_argument("--mixed-tolerance", type=float, default=1e-6,
+                    help="served stream: GMRES relative residual of the mixed form (FP64 form: --tolerance)")
+    ap.add_argument("--no-c2", action="store_true", help="served stream: skip the dense horn sweep forms entry")
+    ap.add_argument("--roofline-leg", action="store_true",
+                    help="only the isolated roofline leg of the served stream (for kernel profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: every GPU box here has one MI355X and the solver has no multi-GPU path "
@@ -527,23 +766,6 @@ def main():
                           "config": {"workload": f"dense complex FP64 LU with partial pivoting, n = {n}, 1 RHS"},
                           "flops": 8.0 / 3.0 * n ** 3, "relative_residual": residual}), flush=True)
         return
-    if args.stream:
-        out = {}
-        for label, precision in (("mixed", bem.PRECISION_MIXED), ("fp64", bem.PRECISION_FP64)):
-            out[label] = run_stream(args.stream_seconds, args.stream_rate, args.stream_pool, (2000.0, 20000.0),
-                                    max(args.slots, 1), args.tolerance, precision)
-            print(f"# stream {label}: {out[label]['points_per_s']:.2f} points/s, p50 {out[label]['p50_ms']:.0f} ms, "
-                  f"p99 {out[label]['p99_ms']:.0f} ms", file=sys.stderr, flush=True)
-        sizes = out["mixed"]["pool_triangles"]
-        print(json.dumps({"metric": "served stream freq-points solved/s + p50/p99 request latency, mixed vs plain FP64",
-                          "unit": "freq-points/s", "value": out["mixed"]["points_per_s"], "higher_is_better": True,
-                          "n_gpus": 1, "dtype": "f32+f64", "data": "synthetic (seeded stream of horn-variant requests)",
-                          "config": {"workload": f"open-loop Poisson stream, {args.stream_rate:g} requests/s for "
-                                                 f"{args.stream_seconds:g} s, {len(sizes)} resident horn meshes "
-                                                 f"({min(sizes)}-{max(sizes)} triangles), 1-32 frequencies per request "
-                                                 f"in 100 Hz-20 kHz, dense assembly + GMRES (rel tol {args.tolerance:g})"},
-                          "stream": out}), flush=True)
-        return
     if args.sweep:
         sw = run_sweep(args.sweep_variants, args.sweep_frequencies, max(args.slots, 1), args.tolerance)
         sizes = [v["triangles"] for v in sw["variants"]]
@@ -557,6 +779,13 @@ def main():
                                                  f"assembly + GMRES (restart 50, rel tol {args.tolerance:g})"},
                           "sweep": sw}), flush=True)
         return
+    if not args.horn:
+        main_stream(args)
+        return
+    if args.steps is None:
+        args.steps = 2
+    if args.warmup is None:
+        args.warmup = 1
     frequencies = [200.0 * 40.0 ** (i / (args.frequencies - 1)) for i in range(args.frequencies)]
     mesh = meshes.exponential_horn(args.edge)
     nt, nv = len(mesh.triangles), len(mesh.vertices)
