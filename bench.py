@@ -381,7 +381,7 @@ def serve_stream(pool, draws, precision, tolerance, slots, window_s, warmup_step
                                start + draws[i][0] if i < len(draws) else end, next_report))
         time.sleep(max(0.0, min(wake - time.perf_counter(), 0.05)))
     end_points, end_time = server.stats()["points"], time.perf_counter()
-    results, latencies, censored = {}, [], 0
+    results, latencies, censored, finished = {}, [], 0, {}
     kernel_ms, samples = 0.0, 0.0
     for n, ticket in enumerate(tickets):
         res = server.poll(ticket, 0)
@@ -392,6 +392,7 @@ def serve_stream(pool, draws, precision, tolerance, slots, window_s, warmup_step
             continue
         results[n] = res
         latencies.append(max(fr.served_milliseconds for fr in res.frequencies))
+        finished[n] = draws[n][0] + latencies[-1] / 1e3  # completion, seconds after the stream's start
         for fr in res.frequencies:
             kernel_ms += fr.kernel_milliseconds["regular"]
             samples += fr.kernel_milliseconds["regular_samples"]
@@ -404,10 +405,26 @@ def serve_stream(pool, draws, precision, tolerance, slots, window_s, warmup_step
            "p50_ms": float(np.percentile(latencies, 50)), "p99_ms": float(np.percentile(latencies, 99)),
            "latency_censored_at_end": censored > 0, "max_running": stats["max_running"],
            "peak_held_bytes": stats["peak_held_bytes"], "budget_bytes": stats["budget_bytes"],
-           "regular_kernel_ms_served": kernel_ms, "regular_samples_served": samples}
+           "regular_kernel_ms_served": kernel_ms, "regular_samples_served": samples, "finished_s": finished}
     print(f"# {label}: {out['points_per_s']:.2f} points/s over {timed:.0f} s timed, {len(results)}/{len(tickets)} "
           f"requests done, latency p50 {out['p50_ms']:.0f} ms p99 {out['p99_ms']:.0f} ms", file=sys.stderr, flush=True)
     return out, results
+
+
+def common_prefix(draws, forms):
+    """Both forms on identical work: the longest arrival-order prefix of requests that
+    every form completed, its points, and per form the time from the stream's start
+    to the last completion among them (the forms see the same arrivals and serve
+    them in arrival order, so this is each form's time for the same work)."""
+    k = 0
+    while all(k in f["finished_s"] for f in forms.values()):
+        k += 1
+    if k == 0:
+        return None
+    points = sum(len(draws[n][2]) for n in range(k))
+    times = {name: max(f["finished_s"][n] for n in range(k)) for name, f in forms.items()}
+    return {"requests": k, "points": points, "seconds": times,
+            "points_per_s": {name: points / t for name, t in times.items()}}
 
 
 def _directivity_db(observers):
@@ -602,6 +619,13 @@ def main_stream(args):
         forms[label], results[label] = serve_stream(pool, draws, precision, tol, args.slots, args.window, warmup,
                                                     steps, label)
         forms[label]["gmres_tolerance"] = tol
+    prefix = common_prefix(draws, forms)
+    for f in forms.values():
+        f.pop("finished_s")
+    if prefix:
+        print(f"# common prefix: {prefix['requests']} requests / {prefix['points']} points: FP64 "
+              f"{prefix['seconds']['served_fp64']:.1f} s, mixed {prefix['seconds']['served_mixed']:.1f} s",
+              file=sys.stderr, flush=True)
     errors = compare_forms(pool, draws, results["served_fp64"], results["served_mixed"])
     print(f"# mixed vs FP64 on {errors['points_compared']} points: surface pressure {errors['surface_pressure']:.2e}, "
           f"on-axis SPL {errors['on_axis_spl_db']:.2e} dB, directivity {errors['directivity_db']:.2e} dB",
@@ -626,7 +650,10 @@ def main_stream(args):
                    "points_offered": offered_points, "window_s": args.window, "slots": args.slots},
         "latency_ms": {"p50": head["p50_ms"], "p99": head["p99_ms"]},
         "forms": forms,
-        "mixed_vs_fp64": {"speedup": head["points_per_s"] / forms["served_fp64"]["points_per_s"],
+        "mixed_vs_fp64": {"speedup_same_work": (prefix["seconds"]["served_fp64"] / prefix["seconds"]["served_mixed"]
+                                                if prefix else None),
+                          "same_work": prefix,
+                          "speedup_timed_windows": head["points_per_s"] / forms["served_fp64"]["points_per_s"],
                           "max_rel_l2_surface_pressure": errors["surface_pressure"],
                           "max_on_axis_spl_db": errors["on_axis_spl_db"],
                           "max_directivity_db": errors["directivity_db"],

@@ -128,8 +128,10 @@ void launch_flux_rhs(const double *BD, std::uint32_t nv, std::uint32_t nd, const
 
 // y[:, e] = A x[:, e] for e < nvec (A n x n complex, column-major, FP64 or FP32
 // when fp32 != 0; vectors and accumulation FP64); y may not alias x.
+// skip (device int, single-vector products only): when nonzero at run time the
+// kernels return at once (a captured solver step after its loop has finished).
 void launch_gemv(const void *A, bool fp32, std::uint32_t n, const double *x, double *y, std::uint32_t nvec,
-                 double *scratch, hipStream_t stream);
+                 double *scratch, hipStream_t stream, const int *skip = nullptr);
 // Scratch doubles launch_gemv needs for n and nvec.
 std::size_t gemv_scratch_doubles(std::uint32_t n, std::uint32_t nvec);
 
@@ -150,6 +152,33 @@ std::size_t lu_work_doubles(std::uint32_t n);
 // Solve with the factors for nrhs right-hand sides in place (B n x nrhs).
 void lu_solve(const double *LU, std::uint32_t n, const std::int32_t *pivots, double *B, std::uint32_t nrhs,
               hipStream_t stream);
+
+// Device-resident GMRES restart cycle (one right-hand side): the step index, the
+// Hessenberg columns, the Givens rotations, g and the stop flag live in `small` /
+// `state`, so one Arnoldi step (load, operator, orthogonalize) is a fixed launch
+// sequence a HIP graph can replay; every kernel of a step returns at once after the
+// cycle has stopped. state: [j, done, used, limit, error]; small layout: h1 at 0
+// ((restart+1) complex, reused for the cycle's y), g and H at the offsets below
+// (H column-major, stride restart+1).
+struct GmresDeviceCycle {
+    double *basis;    // (restart + 1) x n complex
+    double *current;  // n complex: v_j
+    double *w;        // n complex: the operator's output being orthogonalized
+    double *scratch;  // >= gmres_cycle_scratch_doubles(n, restart)
+    double *small;    // gmres_cycle_small_doubles(restart)
+    int *state;       // 5 ints
+    std::uint32_t n, restart;
+};
+std::size_t gmres_cycle_small_doubles(std::uint32_t restart);
+std::size_t gmres_cycle_scratch_doubles(std::uint32_t n, std::uint32_t restart);
+std::size_t gmres_cycle_h_offset(std::uint32_t restart);
+std::size_t gmres_cycle_g_offset(std::uint32_t restart);
+// Starts a cycle: g = (beta, 0, ...), the absolute target, at most `limit` steps.
+void gmres_cycle_begin(const GmresDeviceCycle &c, double beta, double target, std::uint32_t limit, hipStream_t s);
+// current = v_j.
+void gmres_cycle_load(const GmresDeviceCycle &c, hipStream_t s);
+// CGS2 of w against v_0..v_j, v_{j+1} = w / |w|, column j rotated, j += 1.
+void gmres_cycle_orthogonalize(const GmresDeviceCycle &c, hipStream_t s);
 
 // LU factorization and solve (as lu_factor + lu_solve) whose pivots and status
 // word live in a static device array instead of allocated memory: A (n x n,

@@ -37,7 +37,10 @@ constexpr int kMaxVec = 8;
 template <int NV, class T, int COLS = kGemvCols>
 __global__ __launch_bounds__(kGemvRows *kGemvWaves) void gemv_partial(const T *__restrict__ A, std::uint32_t n,
                                                                      const double *__restrict__ x,
-                                                                     std::uint32_t chunks, double *__restrict__ part) {
+                                                                     std::uint32_t chunks, double *__restrict__ part,
+                                                                     const int *skip) {
+    if (skip && *skip) // a device-resident solver loop has already finished
+        return;
     __shared__ double sx[NV][2 * COLS];
     __shared__ double red[kGemvWaves - 1][NV][2][kGemvRows];
     const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
@@ -106,7 +109,10 @@ __global__ __launch_bounds__(kGemvRows *kGemvWaves) void gemv_partial(const T *_
         }
 }
 
-__global__ void gemv_reduce(const double *part, std::uint32_t n, std::uint32_t chunks, std::uint32_t nvec, double *y) {
+__global__ void gemv_reduce(const double *part, std::uint32_t n, std::uint32_t chunks, std::uint32_t nvec, double *y,
+                            const int *skip) {
+    if (skip && *skip)
+        return;
     const std::size_t idx = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= static_cast<std::size_t>(n) * nvec)
         return;
@@ -987,26 +993,26 @@ namespace {
 
 template <int COLS>
 void launch_gemv_one(const void *A, bool fp32, std::uint32_t n, const double *x, double *y, double *scratch,
-                     hipStream_t stream) {
+                     hipStream_t stream, const int *skip) {
     const std::uint32_t chunks = (n + COLS - 1) / COLS;
     const dim3 grid((n + kGemvRows - 1) / kGemvRows, chunks);
     if (fp32)
         hipLaunchKernelGGL((gemv_partial<1, float2, COLS>), grid, dim3(kGemvRows * kGemvWaves), 0, stream,
-                           static_cast<const float2 *>(A), n, x, chunks, scratch);
+                           static_cast<const float2 *>(A), n, x, chunks, scratch, skip);
     else
         hipLaunchKernelGGL((gemv_partial<1, double2, COLS>), grid, dim3(kGemvRows * kGemvWaves), 0, stream,
-                           static_cast<const double2 *>(A), n, x, chunks, scratch);
-    hipLaunchKernelGGL(gemv_reduce, dim3((n + 255) / 256), dim3(256), 0, stream, scratch, n, chunks, 1u, y);
+                           static_cast<const double2 *>(A), n, x, chunks, scratch, skip);
+    hipLaunchKernelGGL(gemv_reduce, dim3((n + 255) / 256), dim3(256), 0, stream, scratch, n, chunks, 1u, y, skip);
 }
 
 void launch_gemv_cols(std::uint32_t cols, const void *A, bool fp32, std::uint32_t n, const double *x, double *y,
-                      double *scratch, hipStream_t stream) {
+                      double *scratch, hipStream_t stream, const int *skip = nullptr) {
     if (cols == 2048)
-        launch_gemv_one<2048>(A, fp32, n, x, y, scratch, stream);
+        launch_gemv_one<2048>(A, fp32, n, x, y, scratch, stream, skip);
     else if (cols == 1024)
-        launch_gemv_one<1024>(A, fp32, n, x, y, scratch, stream);
+        launch_gemv_one<1024>(A, fp32, n, x, y, scratch, stream, skip);
     else
-        launch_gemv_one<kGemvCols>(A, fp32, n, x, y, scratch, stream);
+        launch_gemv_one<kGemvCols>(A, fp32, n, x, y, scratch, stream, skip);
 }
 
 // Tile autotuner of the one-vector product: per (n, precision) the column chunk
@@ -1070,9 +1076,9 @@ std::uint32_t gemv_columns(const void *A, bool fp32, std::uint32_t n, const doub
 } // namespace
 
 void launch_gemv(const void *A, bool fp32, std::uint32_t n, const double *x, double *y, std::uint32_t nvec,
-                 double *scratch, hipStream_t stream) {
+                 double *scratch, hipStream_t stream, const int *skip) {
     if (nvec == 1) {
-        launch_gemv_cols(gemv_columns(A, fp32, n, x, y, scratch, stream), A, fp32, n, x, y, scratch, stream);
+        launch_gemv_cols(gemv_columns(A, fp32, n, x, y, scratch, stream), A, fp32, n, x, y, scratch, stream, skip);
         return;
     }
     const std::uint32_t chunks = (n + kGemvCols - 1) / kGemvCols;
@@ -1085,10 +1091,10 @@ void launch_gemv(const void *A, bool fp32, std::uint32_t n, const double *x, dou
     case N:                                                                                                      \
         if (fp32)                                                                                                \
             hipLaunchKernelGGL((gemv_partial<N, float2>), grid, dim3(kGemvRows * kGemvWaves), 0, stream,                      \
-                               static_cast<const float2 *>(A), n, xs, chunks, scratch);                          \
+                               static_cast<const float2 *>(A), n, xs, chunks, scratch, nullptr);                 \
         else                                                                                                     \
             hipLaunchKernelGGL((gemv_partial<N, double2>), grid, dim3(kGemvRows * kGemvWaves), 0, stream,                     \
-                               static_cast<const double2 *>(A), n, xs, chunks, scratch);                         \
+                               static_cast<const double2 *>(A), n, xs, chunks, scratch, nullptr);                \
         break;
             HS_GEMV_CASE(1)
             HS_GEMV_CASE(2)
@@ -1102,7 +1108,7 @@ void launch_gemv(const void *A, bool fp32, std::uint32_t n, const double *x, dou
         }
         const std::size_t count = static_cast<std::size_t>(n) * nv;
         hipLaunchKernelGGL(gemv_reduce, dim3((count + 255) / 256), dim3(256), 0, stream, scratch, n, chunks, nv,
-                           y + 2 * static_cast<std::size_t>(v0) * n);
+                           y + 2 * static_cast<std::size_t>(v0) * n, nullptr);
     }
 }
 

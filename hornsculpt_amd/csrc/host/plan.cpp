@@ -167,8 +167,10 @@ InteractionPlan make_interaction_plan(const Space &space, const QuadratureSettin
     const std::vector<Pair> pairs =
         all_pairs ? std::vector<Pair>{} : spatial_candidates(space, std::max(2., s.near_distance_ratio));
     plan.candidate_pairs = pairs.size();
+    // Up to 16 host threads (the reference caps its planner at 8): one MI355X's share of a
+    // host's cores; the result is sorted, so it does not depend on the thread count.
     unsigned workers = threads ? threads : std::max(1u, std::thread::hardware_concurrency());
-    workers = static_cast<unsigned>(std::min<std::size_t>({workers, 8, all_pairs ? n : pairs.size()}));
+    workers = static_cast<unsigned>(std::min<std::size_t>({workers, 16, all_pairs ? n : pairs.size()}));
     if (!workers)
         return plan;
 

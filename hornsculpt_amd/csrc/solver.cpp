@@ -12,6 +12,7 @@
 #include "host/qualification.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -554,6 +555,22 @@ void MeshSolver::assemble_dense(const Numerics &n, double k_max, double k, doubl
 GmresWorkspace::~GmresWorkspace() { release(); }
 
 void GmresWorkspace::release() {
+    if (step_graph)
+        (void)hipGraphExecDestroy(step_graph);
+    step_graph = nullptr;
+    step_key = {};
+    cycle_current.release();
+    cycle_w.release();
+    cycle_z.release();
+    cycle_small.release();
+    cycle_state.release();
+    if (cycle_pinned)
+        (void)hipHostFree(cycle_pinned);
+    cycle_pinned = nullptr;
+    cycle_pinned_ints = 0;
+    for (hipEvent_t e : cycle_events)
+        (void)hipEventDestroy(e);
+    cycle_events.clear();
     multi_basis.release();
     multi_in.release();
     multi_out.release();
@@ -582,9 +599,200 @@ void GmresWorkspace::release() {
 // device, so step j + 1 is queued before the host reads step j's column (from
 // pinned memory, by event): the device never waits for the host's rotations,
 // and at most one speculative step per restart cycle is discarded.
+namespace {
+
+// Restarted GMRES on the dense operator with device-resident cycles: one Arnoldi
+// step (v_j, block-Jacobi, the product with A, CGS2, the rotation) is captured once
+// into a HIP graph and replayed once per iteration; the host reads the cycle state
+// every kPoll steps, one poll behind the queue, so it never waits for a step to
+// finish before queuing the next, and solves the small triangular system once per
+// cycle. Same arithmetic as gmres(): the same reductions in the same order and the
+// same rotation formulas (evaluated on the device).
+constexpr std::uint32_t kPoll = 4;
+
+void dense_gmres_graph(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const double *b, double *x,
+                       std::uint32_t restart, std::uint32_t max_it, double tol, double *achieved,
+                       std::uint32_t *iterations, const volatile std::int32_t *cancel, GmresWorkspace &ws,
+                       const double *jacobi) {
+    using C = std::complex<double>;
+    const std::size_t vec = 2 * static_cast<std::size_t>(n), stride = restart + 1;
+    if (ws.basis.size() < vec * stride)
+        ws.basis.resize(vec * stride);
+    if (ws.r.size() < vec)
+        ws.r.resize(vec);
+    for (auto *buf : {&ws.cycle_current, &ws.cycle_w, &ws.cycle_z})
+        if (buf->size() < vec)
+            buf->resize(vec);
+    const std::size_t scratch =
+        std::max({dev::gemv_scratch_doubles(n, 1), dev::gmres_cycle_scratch_doubles(n, restart),
+                  dev::reduction_scratch_doubles(n, 1)});
+    if (ws.scratch.size() < scratch)
+        ws.scratch.resize(scratch);
+    if (ws.cycle_small.size() < dev::gmres_cycle_small_doubles(restart))
+        ws.cycle_small.resize(dev::gmres_cycle_small_doubles(restart));
+    if (ws.cycle_state.size() < 8)
+        ws.cycle_state.resize(8);
+    if (ws.small.size() < 16)
+        ws.small.resize(16);
+    const std::size_t polls = restart / kPoll + 2;
+    if (ws.cycle_pinned_ints < 8 * polls) {
+        if (ws.cycle_pinned)
+            (void)hipHostFree(ws.cycle_pinned);
+        ws.cycle_pinned = nullptr;
+        hip_check(hipHostMalloc(reinterpret_cast<void **>(&ws.cycle_pinned), sizeof(int) * 8 * polls),
+                  "pinned GMRES cycle state");
+        ws.cycle_pinned_ints = 8 * polls;
+    }
+    while (ws.cycle_events.size() < polls) {
+        hipEvent_t e;
+        hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event create");
+        ws.cycle_events.push_back(e);
+    }
+    const dev::GmresDeviceCycle cyc{ws.basis.data(), ws.cycle_current.data(), ws.cycle_w.data(), ws.scratch.data(),
+                                    ws.cycle_small.data(), ws.cycle_state.data(), n, restart};
+    double *nrm = ws.small.data();
+    auto norm = [&](const double *v) {
+        dev::launch_norm2(v, n, nullptr, nrm, ws.scratch.data(), s);
+        double out = 0;
+        hip_check(hipMemcpyAsync(&out, nrm, sizeof(double), hipMemcpyDeviceToHost, s), "norm download");
+        hip_check(hipStreamSynchronize(s), "norm sync");
+        return std::sqrt(out);
+    };
+    // The captured step: its arguments (buffers, A, sizes) are fixed; recapture when they change.
+    const auto key = std::make_tuple(A, fp32, n, restart, jacobi, static_cast<const double *>(ws.basis.data()),
+                                     static_cast<const double *>(ws.scratch.data()));
+    if (!ws.step_graph || ws.step_key != key) {
+        if (ws.step_graph)
+            (void)hipGraphExecDestroy(ws.step_graph);
+        ws.step_graph = nullptr;
+        // one eager product first: the column chunk is tuned outside the capture
+        dev::launch_gemv(A, fp32, n, b, ws.cycle_w.data(), 1, ws.scratch.data(), s);
+        hip_check(hipStreamSynchronize(s), "product warm-up");
+        hipGraph_t graph = nullptr;
+        hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture begin");
+        dev::gmres_cycle_load(cyc, s);
+        const double *in = ws.cycle_current.data();
+        if (jacobi) {
+            dev::jacobi_apply(jacobi, n, in, ws.cycle_z.data(), s);
+            in = ws.cycle_z.data();
+        }
+        dev::launch_gemv(A, fp32, n, in, ws.cycle_w.data(), 1, ws.scratch.data(), s, ws.cycle_state.data() + 1);
+        dev::gmres_cycle_orthogonalize(cyc, s);
+        const hipError_t captured = hipStreamEndCapture(s, &graph);
+        hip_check(captured, "capture end");
+        const hipError_t made = hipGraphInstantiate(&ws.step_graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        hip_check(made, "graph instantiate");
+        ws.step_key = key;
+    }
+    hip_check(hipMemsetAsync(x, 0, vec * sizeof(double), s), "gmres clear");
+    const double bnorm = norm(b);
+    *iterations = 0;
+    if (bnorm == 0) {
+        *achieved = 0;
+        return;
+    }
+    const double target = tol * bnorm;
+    hip_check(hipMemcpyAsync(ws.r.data(), b, vec * sizeof(double), hipMemcpyDeviceToDevice, s), "gmres copy");
+    double beta = bnorm;
+    std::uint32_t it = 0;
+    std::vector<double> host_small(dev::gmres_cycle_small_doubles(restart));
+    const std::size_t hoff = dev::gmres_cycle_h_offset(restart), goff = dev::gmres_cycle_g_offset(restart);
+    for (;;) {
+        if (beta <= target) {
+            *achieved = beta / bnorm;
+            *iterations = it;
+            return;
+        }
+        if (it >= max_it)
+            break;
+        dev::launch_scale(n, 1.0 / beta, ws.r.data(), ws.basis.data(), s);
+        const std::uint32_t limit = std::min(restart, max_it - it);
+        dev::gmres_cycle_begin(cyc, beta, target, limit, s);
+        // queue the steps; every kPoll steps copy the state out and look at the
+        // previous copy (one poll behind), stopping once a copy shows the cycle done
+        std::uint32_t launched = 0, poll = 0;
+        bool done = false;
+        while (launched < limit && !done) {
+            if (cancel && *cancel) {
+                (void)hipStreamSynchronize(s);
+                throw Cancelled();
+            }
+            hip_check(hipGraphLaunch(ws.step_graph, s), "step launch");
+            ++launched;
+            if (launched % kPoll == 0 || launched == limit) {
+                int *slot = ws.cycle_pinned + 8 * poll;
+                hip_check(hipMemcpyAsync(slot, ws.cycle_state.data(), 8 * sizeof(int), hipMemcpyDeviceToHost, s),
+                          "cycle state");
+                hip_check(hipEventRecord(ws.cycle_events[poll], s), "cycle event");
+                if (poll > 0) {
+                    hip_check(hipEventSynchronize(ws.cycle_events[poll - 1]), "cycle poll");
+                    done = ws.cycle_pinned[8 * (poll - 1) + 1] != 0;
+                }
+                ++poll;
+            }
+        }
+        hip_check(hipMemcpyAsync(host_small.data(), ws.cycle_small.data(), sizeof(double) * host_small.size(),
+                                 hipMemcpyDeviceToHost, s),
+                  "cycle download");
+        int state[8] = {};
+        hip_check(hipMemcpyAsync(state, ws.cycle_state.data(), sizeof(state), hipMemcpyDeviceToHost, s), "state");
+        hip_check(hipStreamSynchronize(s), "cycle sync");
+        if (state[4])
+            throw std::runtime_error("GMRES: the Arnoldi process broke down");
+        const std::uint32_t used = static_cast<std::uint32_t>(state[2]);
+        it += used;
+        const C *H = reinterpret_cast<const C *>(host_small.data() + hoff);
+        const C *g = reinterpret_cast<const C *>(host_small.data() + goff);
+        std::vector<C> y(used);
+        for (std::uint32_t row = used; row-- > 0;) {
+            C v = g[row];
+            for (std::uint32_t c = row + 1; c < used; ++c)
+                v -= H[static_cast<std::size_t>(c) * stride + row] * y[c];
+            y[row] = v / H[static_cast<std::size_t>(row) * stride + row];
+        }
+        std::vector<double> neg(2 * used);
+        for (std::uint32_t j = 0; j < used; ++j) {
+            neg[2 * j] = -y[j].real();
+            neg[2 * j + 1] = -y[j].imag();
+        }
+        double *ybuf = ws.cycle_small.data(); // h1 slot: free between cycles
+        if (used) {
+            hip_check(hipMemcpyAsync(ybuf, neg.data(), sizeof(double) * neg.size(), hipMemcpyHostToDevice, s),
+                      "y upload");
+            if (jacobi) { // x += M^{-1} V y
+                hip_check(hipMemsetAsync(ws.r.data(), 0, vec * sizeof(double), s), "gmres clear");
+                dev::launch_subtract_combination(ws.basis.data(), n, used, ybuf, ws.r.data(), s);
+                dev::jacobi_apply(jacobi, n, ws.r.data(), ws.cycle_z.data(), s);
+                dev::launch_add(n, ws.cycle_z.data(), x, s);
+            } else {
+                dev::launch_subtract_combination(ws.basis.data(), n, used, ybuf, x, s);
+            }
+        }
+        dev::launch_residual(A, fp32, n, b, x, 1, ws.r.data(), ws.scratch.data(), s);
+        beta = norm(ws.r.data());
+    }
+    *achieved = beta / bnorm;
+    *iterations = it;
+    std::ostringstream msg;
+    msg << "GMRES stopped without converging after " << it << "/" << max_it << " iterations: relative residual " << *achieved;
+    throw std::runtime_error(msg.str());
+}
+
+bool gmres_graph_enabled() {
+    const char *v = std::getenv("HSBEM_GMRES_GRAPH");
+    return !(v && v[0] == '0');
+}
+
+} // namespace
+
 void dense_gmres(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const double *b, double *x,
                  std::uint32_t restart, std::uint32_t max_it, double tol, double *achieved, std::uint32_t *iterations,
                  const volatile std::int32_t *cancel, GmresWorkspace &ws, const double *jacobi) {
+    if (gmres_graph_enabled()) {
+        dense_gmres_graph(s, A, fp32, n, b, x, restart, max_it, tol, achieved, iterations, cancel, ws, jacobi);
+        return;
+    }
     const std::size_t scratch = dev::gemv_scratch_doubles(n, 1);
     if (ws.scratch.size() < scratch)
         ws.scratch.resize(scratch);

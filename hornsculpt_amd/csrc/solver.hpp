@@ -167,6 +167,16 @@ struct GmresWorkspace {
     DeviceBuffer<double> multi_basis, multi_in, multi_out, multi_small;
     double *multi_pinned{};
     std::size_t multi_pinned_doubles{};
+    // device-resident cycles (dense_gmres): v_j and the operator output, the cycle
+    // state, the captured Arnoldi step and what it was captured for, pinned state reads
+    DeviceBuffer<double> cycle_current, cycle_w, cycle_z, cycle_small;
+    DeviceBuffer<int> cycle_state;
+    hipGraphExec_t step_graph{};
+    std::tuple<const void *, bool, std::uint32_t, std::uint32_t, const double *, const double *, const double *>
+        step_key{};
+    int *cycle_pinned{};
+    std::size_t cycle_pinned_ints{};
+    std::vector<hipEvent_t> cycle_events;
 };
 
 // Right-hand sides solved together by dense_gmres_multi (one multi-vector

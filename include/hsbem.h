@@ -83,7 +83,7 @@ typedef struct {
     uint32_t gmres_restart;              /* default 50 */
     uint32_t gmres_maximum_iterations;   /* default 1000 */
     uint32_t precision;                  /* HSBEM_PRECISION_FP64 (default) or HSBEM_PRECISION_MIXED */
-    uint32_t plan_threads;               /* host planner threads, 0 = up to 8 */
+    uint32_t plan_threads;               /* host planner threads, 0 = up to 16 */
     uint32_t preconditioner;             /* GMRES: HSBEM_PRECONDITIONER_BLOCK_JACOBI (default) or _NONE */
     /* BemNumerics::acceleration and FastMultipoleSettings (bem.hpp:60-71, 121): */
     uint32_t acceleration;               /* HSBEM_ACCELERATION_DENSE (default) or _FAST_MULTIPOLE (FP64) */
