@@ -896,7 +896,7 @@ def main():
                  next((f["plan_ms"] for f in forms.values() if f["plan_ms"] is not None), None)},
         "lu_cross_check_rel_l2": cross,
         "roofline": {"kernel": f"regular_tiles ({'FP32' if mixed_head else 'FP64'} pair quadrature, VALU)",
-                     "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak if achieved else None,
                      "traffic": REGULAR_TRAFFIC_BYTES["fp32" if mixed_head else "fp64"] if args.edge == 0.0052 else None,
                      "launch_ms": roof["kernel_ms"] / roof["points"],

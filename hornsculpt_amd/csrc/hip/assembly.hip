@@ -25,6 +25,8 @@
 #include "device.hpp"
 #include "kernels.hpp"
 
+#include <stdexcept>
+#include <string>
 #include <type_traits>
 
 namespace hsbem::dev {
@@ -154,41 +156,55 @@ __device__ __forceinline__ double g_at(const double *geo, int field, std::uint32
     return geo[static_cast<std::size_t>(field) * nt + t];
 }
 
-// Sample core of a disjoint pair (regular pairs and tensor near patches), from
-// the squared distance r2, the numerators fyn = d.n_y and dntk = (d.n_x)/k
-// already formed: same algebra as bm_sample. A zero, negative or non-finite r2
-// yields non-finite moments, which the caller checks once per pair.
-__device__ __forceinline__ SampleT<float> disjoint_sample(float r2, float wq, float fyn, float dntk, float k,
-                                                          float k_rev, float knn) {
+// Sample core of a disjoint pair (regular pairs and tensor near patches), from the
+// squared distance r2, the numerators fyn = d.n_y and dntk = (d.n_x)/k already
+// formed: the same algebra as bm_sample for g and a; of b = -g + i radial fxk only
+// radial fxk is returned, the caller sums it and forms sum b = -sum g + i sum(radial
+// fxk) once per test point. A zero, negative or non-finite r2 yields non-finite
+// moments, which the caller checks once per pair.
+template <class R> struct CoreT {
+    CplxT<R> g, a, rb;
+};
+
+__device__ __forceinline__ CoreT<float> pair_core(float r2, float wq, float fyn, float dntk, float k, float k_rev,
+                                                  float knn) {
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * rinv;
     const float f = __builtin_amdgcn_fractf(r * k_rev);
     const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
     const float kr = k * r, scale = wq * rinv;
-    SampleT<float> s;
+    CoreT<float> s;
     s.g = {cs * scale, -sn * scale};
     const CplxT<float> radial = {s.g.re - s.g.im * kr, s.g.im + s.g.re * kr};
     const float ir2 = rinv * rinv, fy = fyn * ir2, fxk = -dntk * ir2;
     s.a = {-radial.re * fy - knn * s.g.im, -radial.im * fy + knn * s.g.re};
-    s.b = {-s.g.re - radial.im * fxk, -s.g.im + radial.re * fxk};
+    s.rb = {radial.re * fxk, radial.im * fxk};
     return s;
 }
 
-__device__ __forceinline__ SampleT<double> disjoint_sample(double r2, double wq, double fyn, double dntk, double k,
-                                                           double, double knn) {
+__device__ __forceinline__ CoreT<double> pair_core(double r2, double wq, double fyn, double dntk, double k, double,
+                                                   double knn) {
     const double rinv = rsqrt_full(r2);
     const double r = r2 * rinv, kr = k * r;
     double sn, cs;
     sincos_reduced(kr, &sn, &cs);
     const double scale = wq * rinv;
-    SampleT<double> s;
+    CoreT<double> s;
     s.g = {cs * scale, -sn * scale};
     const CplxT<double> radial = {s.g.re - s.g.im * kr, s.g.im + s.g.re * kr};
     const double ir2 = rinv * rinv, fy = fyn * ir2, fxk = -dntk * ir2;
     s.a = {-radial.re * fy - knn * s.g.im, -radial.im * fy + knn * s.g.re};
-    s.b = {-s.g.re - radial.im * fxk, -s.g.im + radial.re * fxk};
+    s.rb = {radial.re * fxk, radial.im * fxk};
     return s;
 }
+
+// The regular rules of degree 4, 6 and 8 (6, 12 and 16 points: barycentrics and
+// weight per point), passed by value as kernel arguments: with the trial loop
+// unrolled over a compile-time count its points are scalar operands, read once per
+// wavefront instead of once per lane and sample.
+template <class R> struct TileRules {
+    R d4[6][4], d6[12][4], d8[16][4];
+};
 
 // Flat panel frame: vertex 0, edges e1 = v1 - v0, e2 = v2 - v0, unit normal (FP64).
 struct Frame {
@@ -217,12 +233,14 @@ __device__ __forceinline__ double dot3(const double *a, const double *b) { retur
 //   d.n_y = (b + l_x).n_y (l_y is orthogonal to n_y): constant in the inner loop;
 //   d.n_x = b.n_x - l_y.n_x (l_x is orthogonal to n_x).
 // Pair constants are formed in FP64 and rounded to R; wscale = 4 A_x A_y / (4 pi)
-// (times the area fraction of a near patch). N > 0 fixes the rule size at compile
-// time (uniform far tiles); N = 0 takes n. bad is raised when the sum of G is not
-// finite (a zero or invalid distance somewhere in the pair).
+// (times the area fraction of a near patch). N = 0 takes n points from `rule` for
+// both panels; N > 0 takes the test points from `rule` and the trial points from
+// `yrule` (N of them, unrolled: a kernel-argument array). bad is raised when the
+// sum of G is not finite (a zero or invalid distance somewhere in the pair).
 template <class R, int N = 0>
 __device__ __forceinline__ void contract(MomentsT<R> &m, const Frame &t, const Frame &s, double wscale_d,
-                                         const R *rule, int n_runtime, double k, bool &bad) {
+                                         const R *rule, int n_runtime, double k, bool &bad,
+                                         const R (*yrule)[4] = nullptr) {
     const int n = N > 0 ? N : n_runtime;
     const double b[3] = {t.v0[0] - s.v0[0], t.v0[1] - s.v0[1], t.v0[2] - s.v0[2]};
     const double inv_k = 1.0 / k;
@@ -255,21 +273,33 @@ __device__ __forceinline__ void contract(MomentsT<R> &m, const Frame &t, const F
         const R P2 = R(-2) * (blt[0] * se2[0] + blt[1] * se2[1] + blt[2] * se2[2]);
         const R B = blt[0] * blt[0] + blt[1] * blt[1] + blt[2] * blt[2];
         const R fyn = Bn + x1 * T1 + x2 * T2;
-        CplxT<R> ga[3] = {{0, 0}, {0, 0}, {0, 0}}, gg = {0, 0}, gb = {0, 0};
-#pragma unroll 2
-        for (int yi = 0; yi < n; ++yi) {
-            const R *yp = rule + 4 * yi;
-            const R y0 = yp[0], y1 = yp[1], y2 = yp[2];
+        CplxT<R> ga[3] = {{0, 0}, {0, 0}, {0, 0}}, gg = {0, 0}, rb = {0, 0};
+        auto sample_at = [&](R y0, R y1, R y2, R yw) {
             const R t1 = y1 * Q11 + (y2 * Q12 + P1), t2 = y2 * Q22 + P2;
             const R r2 = y1 * t1 + (y2 * t2 + B);
             const R dntk = C0 - (y1 * A1 + y2 * A2);
-            const SampleT<R> smp = disjoint_sample(r2, xw * yp[3], fyn, dntk, kk, k_rev, knn);
+            const CoreT<R> smp = pair_core(r2, xw * yw, fyn, dntk, kk, k_rev, knn);
             ga[0] = cadd_t(ga[0], cscale_t(smp.a, y0));
             ga[1] = cadd_t(ga[1], cscale_t(smp.a, y1));
             ga[2] = cadd_t(ga[2], cscale_t(smp.a, y2));
             gg = cadd_t(gg, smp.g);
-            gb = cadd_t(gb, smp.b);
+            rb = cadd_t(rb, smp.rb);
+        };
+        if constexpr (N > 0) {
+            // the 6-point rule fully unrolled (its x-invariant per-point terms stay in
+            // registers); longer rules four points at a time, which keeps them in the loop
+            constexpr int kUnroll = N <= 6 ? N : 4;
+#pragma unroll kUnroll
+            for (int yi = 0; yi < N; ++yi)
+                sample_at(yrule[yi][0], yrule[yi][1], yrule[yi][2], yrule[yi][3]);
+        } else {
+#pragma unroll 2
+            for (int yi = 0; yi < n; ++yi) {
+                const R *yp = rule + 4 * yi;
+                sample_at(yp[0], yp[1], yp[2], yp[3]);
+            }
         }
+        const CplxT<R> gb = {-gg.re - rb.im, -gg.im + rb.re};
         m.g = cadd_t(m.g, gg);
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -340,7 +370,8 @@ template <> __device__ __forceinline__ float *matrix_of<float>(const AssemblyArg
 // at this wavenumber): one uniform 6 x 6 rule and no per-pair checks.
 template <class R, bool FAR>
 __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2)))
-void regular_tiles(AssemblyArgs args, ClusterView cl, NearView near, const double *rules, const std::uint32_t *tiles) {
+void regular_tiles(AssemblyArgs args, ClusterView cl, NearView near, const double *rules, const std::uint32_t *tiles,
+                   TileRules<R> fixed) {
     constexpr int stride = 2 * kTileNodes + 2; // R per trial-node row (padded)
     __shared__ R tile[kTileNodes * stride];
     __shared__ double sgt[G_COUNT][kTilePanels], sgs[G_COUNT][kTilePanels];
@@ -349,6 +380,8 @@ void regular_tiles(AssemblyArgs args, ClusterView cl, NearView near, const doubl
     __shared__ std::uint32_t snt[3][kTilePanels], sns[3][kTilePanels];
     __shared__ std::uint8_t slt[3][kTilePanels], sls[3][kTilePanels], shpt[kTilePanels], shps[kTilePanels];
     __shared__ std::int32_t sfc[kTilePanels];
+    __shared__ std::uint32_t srow[kTileNodes], scol[kTileNodes];
+    __shared__ double srs[kTileNodes];
     __shared__ R srule[4 * kRuleTableCount];
 
     const std::uint32_t ct = tiles[2 * blockIdx.x], cs = tiles[2 * blockIdx.x + 1];
@@ -378,6 +411,13 @@ void regular_tiles(AssemblyArgs args, ClusterView cl, NearView near, const doubl
             sbs[f][p] = pv.bounds[static_cast<std::size_t>(f) * pv.nt + cl.panels[s0 + p]];
     }
     for (int p = tid; p < kTilePanels; p += kTileThreads) {
+        if (p < static_cast<int>(nA)) {
+            const std::uint32_t row = cl.nodes[a0 + p];
+            srow[p] = row;
+            srs[p] = args.row_scale[row];
+        }
+        if (p < static_cast<int>(nB))
+            scol[p] = cl.nodes[b0 + p];
         if (p < static_cast<int>(nT)) {
             const std::uint32_t id = cl.panels[t0 + p];
             sidt[p] = id;
@@ -457,10 +497,22 @@ void regular_tiles(AssemblyArgs args, ClusterView cl, NearView near, const doubl
         MomentsT<R> m;
         zero_t(m);
         const double wscale = 4.0 * gt[G_AREA] * gs[G_AREA] * 0.07957747154594766788;
-        if constexpr (FAR)
-            contract<R, 6>(m, frame_of(gt), frame_of(gs), wscale, srule + 4 * kRule4, 6, k, bad);
-        else
-            contract<R>(m, frame_of(gt), frame_of(gs), wscale, srule + 4 * offset, count, k, bad);
+        if constexpr (FAR) {
+            contract<R, 6>(m, frame_of(gt), frame_of(gs), wscale, srule + 4 * kRule4, 6, k, bad, fixed.d4);
+        } else {
+            // the unrolled fixed-rule forms when every active lane of the wavefront
+            // takes the same rule (a uniform branch), else the counted loop
+            const int first = __builtin_amdgcn_readfirstlane(offset);
+            const bool uniform = __all(offset == first);
+            if (uniform && first == kRule6)
+                contract<R, 12>(m, frame_of(gt), frame_of(gs), wscale, srule + 4 * kRule6, 12, k, bad, fixed.d6);
+            else if (uniform && first == kRule4)
+                contract<R, 6>(m, frame_of(gt), frame_of(gs), wscale, srule + 4 * kRule4, 6, k, bad, fixed.d4);
+            else if (uniform && first == kRule8)
+                contract<R, 16>(m, frame_of(gt), frame_of(gs), wscale, srule + 4 * kRule8, 16, k, bad, fixed.d8);
+            else
+                contract<R>(m, frame_of(gt), frame_of(gs), wscale, srule + 4 * offset, count, k, bad);
+        }
         counted += static_cast<unsigned long long>(count) * count;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -502,8 +554,8 @@ void regular_tiles(AssemblyArgs args, ClusterView cl, NearView near, const doubl
     for (std::uint32_t e = tid; e < nA * nB; e += kTileThreads) {
         const std::uint32_t a = e % nA, b = e / nA;
         const R *slot = tile + b * stride + 2 * a;
-        const std::uint32_t row = cl.nodes[a0 + a], col = cl.nodes[b0 + b];
-        const double rs = args.row_scale[row];
+        const std::uint32_t row = srow[a], col = scol[b];
+        const double rs = srs[a];
         const R re = slot[0], im = slot[1];
         if (re != R(0) || im != R(0))
             atomic_add_entry(A + 2 * (static_cast<std::size_t>(col) * pv.nv + row), re * rs, im * rs);
@@ -928,24 +980,49 @@ void launch_csr_apply(const std::uint32_t *offsets, const std::uint32_t *columns
     hipLaunchKernelGGL(csr_apply, dim3((rows + 3) / 4), dim3(256), 0, stream, offsets, columns, values, rows, x, y);
 }
 
-void launch_regular(const AssemblyLaunch &L, hipStream_t stream) {
+namespace {
+
+// The degree 4, 6 and 8 rules in the kernel-argument form (same points and order
+// as the device rule table).
+template <class R> TileRules<R> make_tile_rules() {
+    TileRules<R> t{};
+    auto fill = [](R(*dst)[4], unsigned degree, std::size_t count) {
+        const auto rule = triangle_rule(degree);
+        if (rule.size() != count)
+            throw std::logic_error("regular rule of degree " + std::to_string(degree) + " has " +
+                                   std::to_string(rule.size()) + " points, the tile kernels expect " +
+                                   std::to_string(count));
+        for (std::size_t i = 0; i < count; ++i) {
+            for (int c = 0; c < 3; ++c)
+                dst[i][c] = static_cast<R>(rule[i].bary[c]);
+            dst[i][3] = static_cast<R>(rule[i].weight);
+        }
+    };
+    fill(t.d4, 4, 6);
+    fill(t.d6, 6, 12);
+    fill(t.d8, 8, 16);
+    return t;
+}
+
+template <class R> void launch_tiles(const AssemblyLaunch &L, hipStream_t stream) {
+    static const TileRules<R> fixed = make_tile_rules<R>();
     // tiles[0, far_tiles) take the uniform far path at this wavenumber, the rest the general one
     const std::uint32_t general = L.tile_count - L.far_tiles;
-    if (L.args.A32) {
-        if (L.far_tiles)
-            hipLaunchKernelGGL((regular_tiles<float, true>), dim3(L.far_tiles), dim3(kTileThreads), 0, stream, L.args,
-                               L.clusters, L.near, L.rules, L.tiles);
-        if (general)
-            hipLaunchKernelGGL((regular_tiles<float, false>), dim3(general), dim3(kTileThreads), 0, stream, L.args,
-                               L.clusters, L.near, L.rules, L.tiles + 2 * static_cast<std::size_t>(L.far_tiles));
-    } else {
-        if (L.far_tiles)
-            hipLaunchKernelGGL((regular_tiles<double, true>), dim3(L.far_tiles), dim3(kTileThreads), 0, stream,
-                               L.args, L.clusters, L.near, L.rules, L.tiles);
-        if (general)
-            hipLaunchKernelGGL((regular_tiles<double, false>), dim3(general), dim3(kTileThreads), 0, stream, L.args,
-                               L.clusters, L.near, L.rules, L.tiles + 2 * static_cast<std::size_t>(L.far_tiles));
-    }
+    if (L.far_tiles)
+        hipLaunchKernelGGL((regular_tiles<R, true>), dim3(L.far_tiles), dim3(kTileThreads), 0, stream, L.args,
+                           L.clusters, L.near, L.rules, L.tiles, fixed);
+    if (general)
+        hipLaunchKernelGGL((regular_tiles<R, false>), dim3(general), dim3(kTileThreads), 0, stream, L.args,
+                           L.clusters, L.near, L.rules, L.tiles + 2 * static_cast<std::size_t>(L.far_tiles), fixed);
+}
+
+} // namespace
+
+void launch_regular(const AssemblyLaunch &L, hipStream_t stream) {
+    if (L.args.A32)
+        launch_tiles<float>(L, stream);
+    else
+        launch_tiles<double>(L, stream);
 }
 
 void launch_touching(const AssemblyLaunch &L, hipStream_t stream) {
