@@ -371,7 +371,7 @@ template <> __device__ __forceinline__ float *matrix_of<float>(const AssemblyArg
 template <class R, bool FAR>
 __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2)))
 void regular_tiles(AssemblyArgs args, ClusterView cl, NearView near, const double *rules, const std::uint32_t *tiles,
-                   TileRules<R> fixed) {
+                   std::uint32_t count, TileRules<R> fixed) {
     constexpr int stride = 2 * kTileNodes + 2; // R per trial-node row (padded)
     __shared__ R tile[kTileNodes * stride];
     __shared__ double sgt[G_COUNT][kTilePanels], sgs[G_COUNT][kTilePanels];
@@ -384,7 +384,18 @@ void regular_tiles(AssemblyArgs args, ClusterView cl, NearView near, const doubl
     __shared__ double srs[kTileNodes];
     __shared__ R srule[4 * kRuleTableCount];
 
-    const std::uint32_t ct = tiles[2 * blockIdx.x], cs = tiles[2 * blockIdx.x + 1];
+    // blocks [0, count) take the listed (unordered) tiles, blocks [count, 2 count) their
+    // mirrors; the mirror of a diagonal tile is the tile itself and has no work
+    const bool mirror = blockIdx.x >= count;
+    const std::uint32_t listed = mirror ? blockIdx.x - count : blockIdx.x;
+    std::uint32_t ct = tiles[2 * listed], cs = tiles[2 * listed + 1];
+    if (mirror) {
+        if (ct == cs)
+            return;
+        const std::uint32_t swap = ct;
+        ct = cs;
+        cs = swap;
+    }
     const std::uint32_t t0 = cl.panel_offsets[ct], nT = cl.panel_offsets[ct + 1] - t0;
     const std::uint32_t s0 = cl.panel_offsets[cs], nS = cl.panel_offsets[cs + 1] - s0;
     const std::uint32_t a0 = cl.node_offsets[ct], nA = cl.node_offsets[ct + 1] - a0;
@@ -560,6 +571,374 @@ void regular_tiles(AssemblyArgs args, ClusterView cl, NearView near, const doubl
         if (re != R(0) || im != R(0))
             atomic_add_entry(A + 2 * (static_cast<std::size_t>(col) * pv.nv + row), re * rs, im * rs);
     }
+}
+
+// ---- symmetric regular tiles (mixed form) ----
+//
+// One workgroup per unordered cluster pair (A, B), A <= B. Each regular panel pair
+// (t in A, s in B) is integrated once on its rule x rule samples and both ordered
+// pairs (t, s) and (s, t) are formed from them: distance, phase, g and the radial
+// factor are shared, only the normal projections differ. With d = x - y (x on t, y
+// on s), P = radial (d.n_s)/r^2 and Q = -radial (d.n_t)/(k r^2):
+//   (t, s):  a = -P + i knn g,     b = -g + i Q
+//   (s, t):  a = -k Q + i knn g,   b = -g + i P / k
+// (for (s, t) the displacement is -d, its trial normal n_t and its test normal n_s).
+// The sums TG, TP, TQ of g, P, Q weighted by the test basis x_i and the trial basis
+// y_j (3 x 3 complex each) give all 2 x 9 A moments and 2 x 3 B moments. The (t, s)
+// entries go to node tile AB ([B node][A node]), the (s, t) entries to node tile BA
+// ([A node][B node]); on a diagonal tile (A = B) only pairs t < s are taken and both
+// orientations go to tile AB. Geometry: vertex 0 in FP64 (the origin difference is
+// formed in FP64 and rounded), edges, normals, curls and area in FP32.
+constexpr int kSymFields = G_COUNT - G_E1;     // FP32 fields: e1, e2, n, curls, area
+constexpr int kStrideAB = 2 * kTileNodes;      // lanes run along A nodes (the column offset)
+constexpr int kStrideBA = 2 * kTileNodes + 1;  // lanes run along A nodes (the row index): odd stride
+enum : int { F_E1 = 0, F_E2 = 3, F_N = 6, F_C0 = 9, F_AREA = 18 };
+
+struct SymSums {
+    CplxT<float> g[3][3], p[3][3], q[3][3]; // [test basis i][trial basis j]
+};
+
+struct SymPair {
+    float b[3], te1[3], te2[3], se1[3], se2[3];
+    float Q11, Q12, Q22, A1, A2, C0, T1, T2, Bn, wscale;
+};
+
+// Sums of one pair over rule x rule: N > 0 takes both rules' points from the
+// kernel-argument array frule (N points, trial loop unrolled), N = 0 takes n points
+// from the FP64 table drule.
+template <int N>
+__device__ __forceinline__ void contract_sym(SymSums &T, const SymPair &c, float k, float k_rev,
+                                             const float (*frule)[4], const double *drule, int n_runtime) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            T.g[i][j] = T.p[i][j] = T.q[i][j] = {0.f, 0.f};
+    const int n = N > 0 ? N : n_runtime;
+#pragma unroll 1
+    for (int xi = 0; xi < n; ++xi) {
+        float x0, x1, x2, xw;
+        if constexpr (N > 0) {
+            x0 = frule[xi][0];
+            x1 = frule[xi][1];
+            x2 = frule[xi][2];
+            xw = frule[xi][3];
+        } else {
+            x0 = static_cast<float>(drule[4 * xi]);
+            x1 = static_cast<float>(drule[4 * xi + 1]);
+            x2 = static_cast<float>(drule[4 * xi + 2]);
+            xw = static_cast<float>(drule[4 * xi + 3]);
+        }
+        xw *= c.wscale;
+        float blt[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            blt[i] = c.b[i] + (c.te1[i] * x1 + c.te2[i] * x2);
+        const float P1 = -2.f * (blt[0] * c.se1[0] + blt[1] * c.se1[1] + blt[2] * c.se1[2]);
+        const float P2 = -2.f * (blt[0] * c.se2[0] + blt[1] * c.se2[1] + blt[2] * c.se2[2]);
+        const float B = blt[0] * blt[0] + blt[1] * blt[1] + blt[2] * blt[2];
+        const float fyn = c.Bn + x1 * c.T1 + x2 * c.T2;
+        CplxT<float> sg[3] = {{0, 0}, {0, 0}, {0, 0}}, sp[3] = {{0, 0}, {0, 0}, {0, 0}},
+                     sq[3] = {{0, 0}, {0, 0}, {0, 0}};
+        auto sample_at = [&](float y0, float y1, float y2, float yw) {
+            const float t1 = y1 * c.Q11 + (y2 * c.Q12 + P1), t2 = y2 * c.Q22 + P2;
+            const float r2 = y1 * t1 + (y2 * t2 + B);
+            const float dntk = c.C0 - (y1 * c.A1 + y2 * c.A2);
+            const float rinv = __builtin_amdgcn_rsqf(r2);
+            const float r = r2 * rinv;
+            const float f = __builtin_amdgcn_fractf(r * k_rev);
+            const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
+            const float kr = k * r, scale = (xw * yw) * rinv;
+            const CplxT<float> g = {cs * scale, -sn * scale};
+            const CplxT<float> radial = {g.re - g.im * kr, g.im + g.re * kr};
+            const float ir2 = rinv * rinv, tau = fyn * ir2, fxk = -dntk * ir2;
+            const CplxT<float> P = {radial.re * tau, radial.im * tau}, Q = {radial.re * fxk, radial.im * fxk};
+            const float yb[3] = {y0, y1, y2};
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                sg[j] = cadd_t(sg[j], cscale_t(g, yb[j]));
+                sp[j] = cadd_t(sp[j], cscale_t(P, yb[j]));
+                sq[j] = cadd_t(sq[j], cscale_t(Q, yb[j]));
+            }
+        };
+        if constexpr (N > 0) {
+            constexpr int kUnroll = N <= 6 ? N : 4;
+#pragma unroll kUnroll
+            for (int yi = 0; yi < N; ++yi)
+                sample_at(frule[yi][0], frule[yi][1], frule[yi][2], frule[yi][3]);
+        } else {
+#pragma unroll 2
+            for (int yi = 0; yi < n; ++yi)
+                sample_at(static_cast<float>(drule[4 * yi]), static_cast<float>(drule[4 * yi + 1]),
+                          static_cast<float>(drule[4 * yi + 2]), static_cast<float>(drule[4 * yi + 3]));
+        }
+        const float xb[3] = {x0, x1, x2};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                T.g[i][j] = cadd_t(T.g[i][j], cscale_t(sg[j], xb[i]));
+                T.p[i][j] = cadd_t(T.p[i][j], cscale_t(sp[j], xb[i]));
+                T.q[i][j] = cadd_t(T.q[i][j], cscale_t(sq[j], xb[i]));
+            }
+    }
+}
+
+// Rule of a regular pair of a general tile (the same decisions as regular_tiles):
+// -1 for a touching pair, a pair of the near list or an escaped pair (which raises
+// ERR_ESCAPED), else the rule table offset; count receives the point count.
+__device__ __forceinline__ int general_pair_rule(const AssemblyArgs &args, const NearView &near, std::uint32_t ti,
+                                                 std::uint32_t si, double k, int &count) {
+    const std::uint32_t nb = near.offsets[ti];
+    if (contains_sorted(near.trials + nb, near.offsets[ti + 1] - nb, si))
+        return -1;
+    if (near.whole) {
+        const unsigned d = near.whole[unordered_pair_index(ti, si)];
+        if (d) {
+            count = d == 14 ? 64 : 16;
+            return d == 14 ? kRule14 : kRule8;
+        }
+    }
+    const PanelView &pv = args.panels;
+    auto bounds_of = [&](std::uint32_t id) {
+        PanelBounds b;
+        b.center = {pv.bounds[static_cast<std::size_t>(B_C) * pv.nt + id],
+                    pv.bounds[static_cast<std::size_t>(B_C + 1) * pv.nt + id],
+                    pv.bounds[static_cast<std::size_t>(B_C + 2) * pv.nt + id]};
+        b.radius = pv.bounds[static_cast<std::size_t>(B_R) * pv.nt + id];
+        b.max_edge = pv.bounds[static_cast<std::size_t>(B_H) * pv.nt + id];
+        b.qualified_shape = pv.shape[id];
+        return b;
+    };
+    const unsigned d = qualified_regular_degree(bounds_of(ti), bounds_of(si), k, args.tolerance, args.regular_order);
+    if (!d) {
+        atomicExch(args.error, ERR_ESCAPED);
+        return -1;
+    }
+    count = d == 4 ? 6 : d == 6 ? 12 : 16;
+    return d == 4 ? kRule4 : d == 6 ? kRule6 : kRule8;
+}
+
+// FAR: the tile list holds far tiles only (A < B, every pair disjoint, not near,
+// degree 4 at this wavenumber). Occupancy: two 512-thread workgroups per CU
+// (LDS <= 80 KiB, VGPRs <= 128).
+template <bool FAR>
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(4)))
+void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const double *rules,
+                       const std::uint32_t *tiles, TileRules<float> fixed) {
+    __shared__ float tab[kTileNodes * kStrideAB];
+    __shared__ float tba[kTileNodes * kStrideBA];
+    __shared__ double sv0[2][3][kTilePanels];
+    __shared__ float sgf[2][kSymFields][kTilePanels];
+    __shared__ std::uint32_t snode[2][3][kTilePanels];
+    __shared__ std::uint8_t slocal[2][3][kTilePanels];
+    __shared__ std::int32_t sfc[2][kTilePanels];
+    __shared__ float srs[2][kTileNodes];
+
+    const std::uint32_t ca = tiles[2 * blockIdx.x], cb = tiles[2 * blockIdx.x + 1];
+    const bool diag = !FAR && ca == cb;
+    const std::uint32_t t0 = cl.panel_offsets[ca], nT = cl.panel_offsets[ca + 1] - t0;
+    const std::uint32_t s0 = cl.panel_offsets[cb], nS = cl.panel_offsets[cb + 1] - s0;
+    const std::uint32_t a0 = cl.node_offsets[ca], nA = cl.node_offsets[ca + 1] - a0;
+    const std::uint32_t b0 = cl.node_offsets[cb], nB = cl.node_offsets[cb + 1] - b0;
+    const PanelView &pv = args.panels;
+    const int tid = threadIdx.x;
+
+    for (int e = tid; e < kTileNodes * kStrideAB; e += kTileThreads)
+        tab[e] = 0.f;
+    if (!diag)
+        for (int e = tid; e < kTileNodes * kStrideBA; e += kTileThreads)
+            tba[e] = 0.f;
+    for (int e = tid; e < 2 * G_COUNT * kTilePanels; e += kTileThreads) {
+        const int c = e / (G_COUNT * kTilePanels), f = (e / kTilePanels) % G_COUNT, p = e % kTilePanels;
+        if (p < static_cast<int>(c ? nS : nT)) {
+            const double v = g_at(pv.geo, f, pv.nt, cl.panels[(c ? s0 : t0) + p]);
+            if (f < G_E1)
+                sv0[c][f][p] = v;
+            else
+                sgf[c][f - G_E1][p] = static_cast<float>(v);
+        }
+    }
+    for (int e = tid; e < 2 * kTilePanels; e += kTileThreads) {
+        const int c = e / kTilePanels, p = e % kTilePanels;
+        if (p < static_cast<int>(c ? nS : nT)) {
+            const std::uint32_t id = cl.panels[(c ? s0 : t0) + p];
+            sfc[c][p] = args.flux_column[id];
+            for (int l = 0; l < 3; ++l) {
+                snode[c][l][p] = pv.nodes[static_cast<std::size_t>(l) * pv.nt + id];
+                slocal[c][l][p] = cl.local[static_cast<std::size_t>(l) * pv.nt + id];
+            }
+        }
+        if (p < static_cast<int>(c ? nB : nA))
+            srs[c][p] = static_cast<float>(args.row_scale[cl.nodes[(c ? b0 : a0) + p]]);
+    }
+    __syncthreads();
+
+    const double k = args.k;
+    const float kf = static_cast<float>(k), inv_k = static_cast<float>(1.0 / k),
+                k_rev = static_cast<float>(k * 0.15915494309189533577);
+    const std::uint32_t pairs = nT * nS;
+    unsigned long long counted = 0;
+    bool bad = false;
+    for (std::uint32_t p = tid; p < pairs; p += kTileThreads) {
+        const std::uint32_t lt = p % nT, ls = p / nT;
+        int offset = kRule4, count = 6;
+        if constexpr (!FAR) {
+            if (diag && lt >= ls)
+                continue;
+            bool touching = false;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    touching |= snode[0][i][lt] == snode[1][j][ls];
+            if (touching)
+                continue;
+            offset = general_pair_rule(args, near, cl.panels[t0 + lt], cl.panels[s0 + ls], k, count);
+            if (offset < 0)
+                continue;
+        }
+        SymPair c;
+        {
+            const double bd[3] = {sv0[0][0][lt] - sv0[1][0][ls], sv0[0][1][lt] - sv0[1][1][ls],
+                                  sv0[0][2][lt] - sv0[1][2][ls]};
+            float tn[3], sn[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                c.b[i] = static_cast<float>(bd[i]);
+                c.te1[i] = sgf[0][F_E1 + i][lt];
+                c.te2[i] = sgf[0][F_E2 + i][lt];
+                c.se1[i] = sgf[1][F_E1 + i][ls];
+                c.se2[i] = sgf[1][F_E2 + i][ls];
+                tn[i] = sgf[0][F_N + i][lt];
+                sn[i] = sgf[1][F_N + i][ls];
+            }
+            auto dot = [](const float *u, const float *v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+            c.Q11 = dot(c.se1, c.se1);
+            c.Q12 = 2.f * dot(c.se1, c.se2);
+            c.Q22 = dot(c.se2, c.se2);
+            c.A1 = dot(c.se1, tn) * inv_k;
+            c.A2 = dot(c.se2, tn) * inv_k;
+            c.C0 = static_cast<float>((bd[0] * tn[0] + bd[1] * tn[1] + bd[2] * tn[2]) * (1.0 / k));
+            c.T1 = dot(c.te1, sn);
+            c.T2 = dot(c.te2, sn);
+            c.Bn = static_cast<float>(bd[0] * sn[0] + bd[1] * sn[1] + bd[2] * sn[2]);
+            c.wscale = 4.f * sgf[0][F_AREA][lt] * sgf[1][F_AREA][ls] * 0.07957747154594766788f;
+        }
+        SymSums T;
+        if constexpr (FAR) {
+            contract_sym<6>(T, c, kf, k_rev, fixed.d4, nullptr, 6);
+        } else {
+            const int first = __builtin_amdgcn_readfirstlane(offset);
+            const bool uniform = __all(offset == first);
+            if (uniform && first == kRule6)
+                contract_sym<12>(T, c, kf, k_rev, fixed.d6, nullptr, 12);
+            else if (uniform && first == kRule4)
+                contract_sym<6>(T, c, kf, k_rev, fixed.d4, nullptr, 6);
+            else if (uniform && first == kRule8)
+                contract_sym<16>(T, c, kf, k_rev, fixed.d8, nullptr, 16);
+            else
+                contract_sym<0>(T, c, kf, k_rev, nullptr, rules + 4 * offset, count);
+        }
+        counted += 2ull * static_cast<unsigned long long>(count) * count;
+        CplxT<float> G = {0.f, 0.f};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                G = cadd_t(G, T.g[i][j]);
+        bad |= !isfinite(G.re) || !isfinite(G.im);
+        float tn[3], sn[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            tn[i] = sgf[0][F_N + i][lt];
+            sn[i] = sgf[1][F_N + i][ls];
+        }
+        const float knn = kf * (tn[0] * sn[0] + tn[1] * sn[1] + tn[2] * sn[2]);
+        const CplxT<float> Gk = {G.re * inv_k, G.im * inv_k};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int a = slocal[0][i][lt];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int b = slocal[1][j][ls];
+                const float cc = sgf[0][F_C0 + 3 * i][lt] * sgf[1][F_C0 + 3 * j][ls] +
+                                 sgf[0][F_C0 + 3 * i + 1][lt] * sgf[1][F_C0 + 3 * j + 1][ls] +
+                                 sgf[0][F_C0 + 3 * i + 2][lt] * sgf[1][F_C0 + 3 * j + 2][ls];
+                const CplxT<float> tg = T.g[i][j], tp = T.p[i][j], tq = T.q[i][j];
+                // (t, s): row = t's node i (A), column = s's node j (B)
+                float *ts = tab + b * kStrideAB + 2 * a;
+                unsafeAtomicAdd(ts, -tp.re - knn * tg.im + Gk.im * cc);
+                unsafeAtomicAdd(ts + 1, -tp.im + knn * tg.re - Gk.re * cc);
+                // (s, t): row = s's node j (B), column = t's node i (A)
+                float *st = diag ? tab + a * kStrideAB + 2 * b : tba + a * kStrideBA + 2 * b;
+                unsafeAtomicAdd(st, -kf * tq.re - knn * tg.im + Gk.im * cc);
+                unsafeAtomicAdd(st + 1, -kf * tq.im + knn * tg.re - Gk.re * cc);
+            }
+        }
+        const std::int32_t fs = sfc[1][ls], ft = sfc[0][lt];
+        if (fs >= 0) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                float re = 0.f, im = 0.f;
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    re -= T.g[i][j].re + T.q[i][j].im;
+                    im += T.q[i][j].re - T.g[i][j].im;
+                }
+                const std::uint32_t row = snode[0][i][lt];
+                const double rs = args.row_scale[row];
+                atomic_add_entry(args.BD + 2 * (static_cast<std::size_t>(fs) * pv.nv + row), re * rs, im * rs);
+            }
+        }
+        if (ft >= 0) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                float re = 0.f, im = 0.f;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    re -= T.g[i][j].re + T.p[i][j].im * inv_k;
+                    im += T.p[i][j].re * inv_k - T.g[i][j].im;
+                }
+                const std::uint32_t row = snode[1][j][ls];
+                const double rs = args.row_scale[row];
+                atomic_add_entry(args.BD + 2 * (static_cast<std::size_t>(ft) * pv.nv + row), re * rs, im * rs);
+            }
+        }
+    }
+    if (bad)
+        atomicExch(args.error, ERR_DISTANCE);
+    if (args.samples) {
+        unsigned long long v = counted;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            v += __shfl_xor(v, off, 64);
+        if ((tid & 63) == 0 && v)
+            atomicAdd(args.samples, v);
+    }
+    __syncthreads();
+    float *A = args.A32;
+    for (std::uint32_t e = tid; e < nA * nB; e += kTileThreads) {
+        const std::uint32_t a = e % nA, b = e / nA;
+        const float re = tab[b * kStrideAB + 2 * a], im = tab[b * kStrideAB + 2 * a + 1];
+        if (re != 0.f || im != 0.f) {
+            const std::uint32_t row = cl.nodes[a0 + a], col = cl.nodes[b0 + b];
+            const float rs = srs[0][a];
+            unsafeAtomicAdd(A + 2 * (static_cast<std::size_t>(col) * pv.nv + row), re * rs);
+            unsafeAtomicAdd(A + 2 * (static_cast<std::size_t>(col) * pv.nv + row) + 1, im * rs);
+        }
+    }
+    if (!diag)
+        for (std::uint32_t e = tid; e < nA * nB; e += kTileThreads) {
+            const std::uint32_t b = e % nB, a = e / nB;
+            const float re = tba[a * kStrideBA + 2 * b], im = tba[a * kStrideBA + 2 * b + 1];
+            if (re != 0.f || im != 0.f) {
+                const std::uint32_t row = cl.nodes[b0 + b], col = cl.nodes[a0 + a];
+                const float rs = srs[1][b];
+                unsafeAtomicAdd(A + 2 * (static_cast<std::size_t>(col) * pv.nv + row), re * rs);
+                unsafeAtomicAdd(A + 2 * (static_cast<std::size_t>(col) * pv.nv + row) + 1, im * rs);
+            }
+        }
 }
 
 template <class M, class R>
@@ -1004,25 +1383,32 @@ template <class R> TileRules<R> make_tile_rules() {
     return t;
 }
 
-template <class R> void launch_tiles(const AssemblyLaunch &L, hipStream_t stream) {
-    static const TileRules<R> fixed = make_tile_rules<R>();
-    // tiles[0, far_tiles) take the uniform far path at this wavenumber, the rest the general one
-    const std::uint32_t general = L.tile_count - L.far_tiles;
-    if (L.far_tiles)
-        hipLaunchKernelGGL((regular_tiles<R, true>), dim3(L.far_tiles), dim3(kTileThreads), 0, stream, L.args,
-                           L.clusters, L.near, L.rules, L.tiles, fixed);
-    if (general)
-        hipLaunchKernelGGL((regular_tiles<R, false>), dim3(general), dim3(kTileThreads), 0, stream, L.args,
-                           L.clusters, L.near, L.rules, L.tiles + 2 * static_cast<std::size_t>(L.far_tiles), fixed);
-}
-
 } // namespace
 
+// The tile list holds unordered cluster pairs (a <= b): tiles[0, far_tiles) take the
+// uniform far path at this wavenumber, the rest the general one. The mixed form forms
+// both orientations of a pair in one workgroup; the FP64 form launches every listed
+// tile and its mirror.
 void launch_regular(const AssemblyLaunch &L, hipStream_t stream) {
-    if (L.args.A32)
-        launch_tiles<float>(L, stream);
-    else
-        launch_tiles<double>(L, stream);
+    const std::uint32_t general = L.tile_count - L.far_tiles;
+    const std::uint32_t *general_tiles = L.tiles + 2 * static_cast<std::size_t>(L.far_tiles);
+    if (L.args.A32) {
+        static const TileRules<float> fixed = make_tile_rules<float>();
+        if (L.far_tiles)
+            hipLaunchKernelGGL((regular_tiles_sym<true>), dim3(L.far_tiles), dim3(kTileThreads), 0, stream, L.args,
+                               L.clusters, L.near, L.rules, L.tiles, fixed);
+        if (general)
+            hipLaunchKernelGGL((regular_tiles_sym<false>), dim3(general), dim3(kTileThreads), 0, stream, L.args,
+                               L.clusters, L.near, L.rules, general_tiles, fixed);
+    } else {
+        static const TileRules<double> fixed = make_tile_rules<double>();
+        if (L.far_tiles)
+            hipLaunchKernelGGL((regular_tiles<double, true>), dim3(2 * L.far_tiles), dim3(kTileThreads), 0, stream,
+                               L.args, L.clusters, L.near, L.rules, L.tiles, L.far_tiles, fixed);
+        if (general)
+            hipLaunchKernelGGL((regular_tiles<double, false>), dim3(2 * general), dim3(kTileThreads), 0, stream,
+                               L.args, L.clusters, L.near, L.rules, general_tiles, general, fixed);
+    }
 }
 
 void launch_touching(const AssemblyLaunch &L, hipStream_t stream) {

@@ -362,10 +362,11 @@ std::shared_ptr<DevicePlan> MeshSolver::device_plan(const Numerics &n, double k_
         std::vector<std::uint32_t> general;
         const double budget = .2 * n.near_tolerance, reserve = 1e-6;
         const bool budgets = reserve + 4 * 2.3e-6 <= budget && reserve + 4 * 6e-7 <= budget;
+        // unordered cluster pairs a <= b: the kernels form both orientations of each
         if (!dp->plan.all_pairs && budgets && n.quadrature_order <= 4) {
             const double gap_edges = std::max(3.0, n.near_ratio);
             for (std::uint32_t a = 0; a < nc; ++a)
-                for (std::uint32_t b = 0; b < nc; ++b) {
+                for (std::uint32_t b = a; b < nc; ++b) {
                     const ClusterBounds &ca = cluster_bounds_[a], &cb = cluster_bounds_[b];
                     const double h = std::max(ca.max_edge, cb.max_edge);
                     const double d = std::sqrt(vnorm2(vsub(ca.center, cb.center)));
@@ -377,12 +378,13 @@ std::shared_ptr<DevicePlan> MeshSolver::device_plan(const Numerics &n, double k_
                         general.push_back(a * nc + b);
                 }
         } else {
-            for (std::uint32_t t = 0; t < nc * nc; ++t)
-                general.push_back(t);
+            for (std::uint32_t a = 0; a < nc; ++a)
+                for (std::uint32_t b = a; b < nc; ++b)
+                    general.push_back(a * nc + b);
         }
         std::stable_sort(far.begin(), far.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
         std::vector<std::uint32_t> tiles;
-        tiles.reserve(2 * static_cast<std::size_t>(nc) * nc);
+        tiles.reserve(static_cast<std::size_t>(nc) * (nc + 1));
         for (const auto &[h, t] : far) {
             dp->far_tile_edge.push_back(h);
             tiles.push_back(t / nc);
@@ -392,7 +394,7 @@ std::shared_ptr<DevicePlan> MeshSolver::device_plan(const Numerics &n, double k_
             tiles.push_back(t / nc);
             tiles.push_back(t % nc);
         }
-        dp->tile_count = nc * nc;
+        dp->tile_count = nc * (nc + 1) / 2;
         if (tiles.empty())
             tiles.push_back(0);
         dp->tiles.upload(tiles, s);
