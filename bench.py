@@ -383,6 +383,7 @@ def serve_stream(pool, draws, precision, tolerance, slots, window_s, warmup_step
     end_points, end_time = server.stats()["points"], time.perf_counter()
     results, latencies, censored, finished = {}, [], 0, {}
     kernel_ms, samples = 0.0, 0.0
+    phases = {"assembly": 0.0, "solve": 0.0, "factorization": 0.0, "field": 0.0, "interaction_plan": 0.0}
     for n, ticket in enumerate(tickets):
         res = server.poll(ticket, 0)
         if res is None:
@@ -396,6 +397,8 @@ def serve_stream(pool, draws, precision, tolerance, slots, window_s, warmup_step
         for fr in res.frequencies:
             kernel_ms += fr.kernel_milliseconds["regular"]
             samples += fr.kernel_milliseconds["regular_samples"]
+            for phase in phases:
+                phases[phase] += getattr(fr, f"{phase}_milliseconds")
     stats = server.stats()
     server.close()
     timed = end_time - base_time
@@ -405,7 +408,8 @@ def serve_stream(pool, draws, precision, tolerance, slots, window_s, warmup_step
            "p50_ms": float(np.percentile(latencies, 50)), "p99_ms": float(np.percentile(latencies, 99)),
            "latency_censored_at_end": censored > 0, "max_running": stats["max_running"],
            "peak_held_bytes": stats["peak_held_bytes"], "budget_bytes": stats["budget_bytes"],
-           "regular_kernel_ms_served": kernel_ms, "regular_samples_served": samples, "finished_s": finished}
+           "regular_kernel_ms_served": kernel_ms, "regular_samples_served": samples,
+           "phase_ms_completed": phases, "finished_s": finished}
     print(f"# {label}: {out['points_per_s']:.2f} points/s over {timed:.0f} s timed, {len(results)}/{len(tickets)} "
           f"requests done, latency p50 {out['p50_ms']:.0f} ms p99 {out['p99_ms']:.0f} ms", file=sys.stderr, flush=True)
     return out, results

@@ -2,11 +2,12 @@
 //
 // Replaces the reference kernels assembleRegular / assembleSingular /
 // assembleNear / addMass (modules/bem/cuda/cuda_solver.cu:266-540) with:
-//   regular_tiles   one workgroup per (test cluster, trial cluster) of up to 64
-//                   spatially sorted panels; panel geometry and rules in LDS; every
-//                   3x3 pair block accumulated in an LDS node tile (ds_add) and
-//                   flushed once per tile to HBM, so global atomics drop from 18 per
-//                   pair to about one per tile entry.
+//   regular_tiles_sym  one workgroup per unordered pair of clusters of up to 64
+//                   spatially sorted panels; panel geometry in LDS, the rules as
+//                   kernel arguments; each panel pair integrated once for both
+//                   orientations, every 3x3 pair block accumulated in two LDS node
+//                   tiles (ds_add) flushed once per tile to HBM, so global atomics
+//                   drop from 18 per ordered pair to about one per tile entry.
 //   touching_jobs   one wavefront per touching pair: Sauter-Schwab (u,v,w) tuples
 //                   over lanes with the radial coordinate innermost (every sector map
 //                   is linear in r), shuffle reduction, one lane scatters. Always FP64.
@@ -364,216 +365,7 @@ template <class R> __device__ __forceinline__ R *matrix_of(const AssemblyArgs &a
 template <> __device__ __forceinline__ double *matrix_of<double>(const AssemblyArgs &a) { return a.A; }
 template <> __device__ __forceinline__ float *matrix_of<float>(const AssemblyArgs &a) { return a.A32; }
 
-// R: sample/tile precision; the matrix element type is R as well.
-// Occupancy: the FP32 variant fits two 512-thread workgroups per CU (<= 128 VGPRs).
-// FAR: the tile list holds far tiles only (every pair disjoint, not near, degree 4
-// at this wavenumber): one uniform 6 x 6 rule and no per-pair checks.
-template <class R, bool FAR>
-__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2)))
-void regular_tiles(AssemblyArgs args, ClusterView cl, NearView near, const double *rules, const std::uint32_t *tiles,
-                   std::uint32_t count, TileRules<R> fixed) {
-    constexpr int stride = 2 * kTileNodes + 2; // R per trial-node row (padded)
-    __shared__ R tile[kTileNodes * stride];
-    __shared__ double sgt[G_COUNT][kTilePanels], sgs[G_COUNT][kTilePanels];
-    __shared__ double sbt[B_COUNT][kTilePanels], sbs[B_COUNT][kTilePanels];
-    __shared__ std::uint32_t sidt[kTilePanels], sids[kTilePanels];
-    __shared__ std::uint32_t snt[3][kTilePanels], sns[3][kTilePanels];
-    __shared__ std::uint8_t slt[3][kTilePanels], sls[3][kTilePanels], shpt[kTilePanels], shps[kTilePanels];
-    __shared__ std::int32_t sfc[kTilePanels];
-    __shared__ std::uint32_t srow[kTileNodes], scol[kTileNodes];
-    __shared__ double srs[kTileNodes];
-    __shared__ R srule[4 * kRuleTableCount];
-
-    // blocks [0, count) take the listed (unordered) tiles, blocks [count, 2 count) their
-    // mirrors; the mirror of a diagonal tile is the tile itself and has no work
-    const bool mirror = blockIdx.x >= count;
-    const std::uint32_t listed = mirror ? blockIdx.x - count : blockIdx.x;
-    std::uint32_t ct = tiles[2 * listed], cs = tiles[2 * listed + 1];
-    if (mirror) {
-        if (ct == cs)
-            return;
-        const std::uint32_t swap = ct;
-        ct = cs;
-        cs = swap;
-    }
-    const std::uint32_t t0 = cl.panel_offsets[ct], nT = cl.panel_offsets[ct + 1] - t0;
-    const std::uint32_t s0 = cl.panel_offsets[cs], nS = cl.panel_offsets[cs + 1] - s0;
-    const std::uint32_t a0 = cl.node_offsets[ct], nA = cl.node_offsets[ct + 1] - a0;
-    const std::uint32_t b0 = cl.node_offsets[cs], nB = cl.node_offsets[cs + 1] - b0;
-    const PanelView &pv = args.panels;
-    const int tid = threadIdx.x;
-
-    for (int e = tid; e < kTileNodes * stride; e += kTileThreads)
-        tile[e] = R(0);
-    for (int e = tid; e < 4 * kRuleTableCount; e += kTileThreads)
-        srule[e] = static_cast<R>(rules[e]);
-    for (int e = tid; e < static_cast<int>(G_COUNT * kTilePanels); e += kTileThreads) {
-        const int f = e / kTilePanels, p = e % kTilePanels;
-        if (p < static_cast<int>(nT))
-            sgt[f][p] = g_at(pv.geo, f, pv.nt, cl.panels[t0 + p]);
-        if (p < static_cast<int>(nS))
-            sgs[f][p] = g_at(pv.geo, f, pv.nt, cl.panels[s0 + p]);
-    }
-    for (int e = tid; e < static_cast<int>(B_COUNT * kTilePanels); e += kTileThreads) {
-        const int f = e / kTilePanels, p = e % kTilePanels;
-        if (p < static_cast<int>(nT))
-            sbt[f][p] = pv.bounds[static_cast<std::size_t>(f) * pv.nt + cl.panels[t0 + p]];
-        if (p < static_cast<int>(nS))
-            sbs[f][p] = pv.bounds[static_cast<std::size_t>(f) * pv.nt + cl.panels[s0 + p]];
-    }
-    for (int p = tid; p < kTilePanels; p += kTileThreads) {
-        if (p < static_cast<int>(nA)) {
-            const std::uint32_t row = cl.nodes[a0 + p];
-            srow[p] = row;
-            srs[p] = args.row_scale[row];
-        }
-        if (p < static_cast<int>(nB))
-            scol[p] = cl.nodes[b0 + p];
-        if (p < static_cast<int>(nT)) {
-            const std::uint32_t id = cl.panels[t0 + p];
-            sidt[p] = id;
-            shpt[p] = pv.shape[id];
-            for (int l = 0; l < 3; ++l) {
-                snt[l][p] = pv.nodes[static_cast<std::size_t>(l) * pv.nt + id];
-                slt[l][p] = cl.local[static_cast<std::size_t>(l) * pv.nt + id];
-            }
-        }
-        if (p < static_cast<int>(nS)) {
-            const std::uint32_t id = cl.panels[s0 + p];
-            sids[p] = id;
-            shps[p] = pv.shape[id];
-            sfc[p] = args.flux_column[id];
-            for (int l = 0; l < 3; ++l) {
-                sns[l][p] = pv.nodes[static_cast<std::size_t>(l) * pv.nt + id];
-                sls[l][p] = cl.local[static_cast<std::size_t>(l) * pv.nt + id];
-            }
-        }
-    }
-    __syncthreads();
-
-    const double k = args.k;
-    const R inv_k = static_cast<R>(1.0 / k);
-    const std::uint32_t pairs = nT * nS;
-    unsigned long long counted = 0;
-    bool bad = false;
-    for (std::uint32_t p = tid; p < pairs; p += kTileThreads) {
-        const std::uint32_t lt = p % nT, ls = p / nT;
-        int offset = kRule4, count = 6;
-        if constexpr (!FAR) {
-            bool touching = false;
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    touching |= snt[i][lt] == sns[j][ls];
-            if (touching)
-                continue;
-            const std::uint32_t ti = sidt[lt], si = sids[ls];
-            const std::uint32_t nb = near.offsets[ti];
-            if (contains_sorted(near.trials + nb, near.offsets[ti + 1] - nb, si))
-                continue;
-            offset = -1;
-            if (near.whole) {
-                const unsigned d = near.whole[unordered_pair_index(ti, si)];
-                if (d) {
-                    offset = d == 14 ? kRule14 : kRule8;
-                    count = d == 14 ? 64 : 16;
-                }
-            }
-            if (offset < 0) {
-                PanelBounds bt, bs;
-                bt.center = {sbt[B_C][lt], sbt[B_C + 1][lt], sbt[B_C + 2][lt]};
-                bt.radius = sbt[B_R][lt];
-                bt.max_edge = sbt[B_H][lt];
-                bt.qualified_shape = shpt[lt];
-                bs.center = {sbs[B_C][ls], sbs[B_C + 1][ls], sbs[B_C + 2][ls]};
-                bs.radius = sbs[B_R][ls];
-                bs.max_edge = sbs[B_H][ls];
-                bs.qualified_shape = shps[ls];
-                const unsigned d = qualified_regular_degree(bt, bs, k, args.tolerance, args.regular_order);
-                if (!d) {
-                    atomicExch(args.error, ERR_ESCAPED);
-                    continue;
-                }
-                offset = d == 4 ? kRule4 : d == 6 ? kRule6 : kRule8;
-                count = d == 4 ? 6 : d == 6 ? 12 : 16;
-            }
-        }
-        double gt[G_COUNT], gs[G_COUNT];
-#pragma unroll
-        for (int f = 0; f < G_COUNT; ++f) {
-            gt[f] = sgt[f][lt];
-            gs[f] = sgs[f][ls];
-        }
-        MomentsT<R> m;
-        zero_t(m);
-        const double wscale = 4.0 * gt[G_AREA] * gs[G_AREA] * 0.07957747154594766788;
-        if constexpr (FAR) {
-            contract<R, 6>(m, frame_of(gt), frame_of(gs), wscale, srule + 4 * kRule4, 6, k, bad, fixed.d4);
-        } else {
-            // the unrolled fixed-rule forms when every active lane of the wavefront
-            // takes the same rule (a uniform branch), else the counted loop
-            const int first = __builtin_amdgcn_readfirstlane(offset);
-            const bool uniform = __all(offset == first);
-            if (uniform && first == kRule6)
-                contract<R, 12>(m, frame_of(gt), frame_of(gs), wscale, srule + 4 * kRule6, 12, k, bad, fixed.d6);
-            else if (uniform && first == kRule4)
-                contract<R, 6>(m, frame_of(gt), frame_of(gs), wscale, srule + 4 * kRule4, 6, k, bad, fixed.d4);
-            else if (uniform && first == kRule8)
-                contract<R, 16>(m, frame_of(gt), frame_of(gs), wscale, srule + 4 * kRule8, 16, k, bad, fixed.d8);
-            else
-                contract<R>(m, frame_of(gt), frame_of(gs), wscale, srule + 4 * offset, count, k, bad);
-        }
-        counted += static_cast<unsigned long long>(count) * count;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            const int a = slt[i][lt];
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const int b = sls[j][ls];
-                const R cc = static_cast<R>(gt[G_C0 + 3 * i] * gs[G_C0 + 3 * j] +
-                                            gt[G_C0 + 3 * i + 1] * gs[G_C0 + 3 * j + 1] +
-                                            gt[G_C0 + 3 * i + 2] * gs[G_C0 + 3 * j + 2]);
-                R *slot = tile + b * stride + 2 * a;
-                unsafeAtomicAdd(slot, m.a[3 * i + j].re + m.g.im * cc * inv_k);
-                unsafeAtomicAdd(slot + 1, m.a[3 * i + j].im - m.g.re * cc * inv_k);
-            }
-        }
-        const std::int32_t fc = sfc[ls];
-        if (fc >= 0) {
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                const std::uint32_t row = snt[i][lt];
-                const double rs = args.row_scale[row];
-                atomic_add_entry(args.BD + 2 * (static_cast<std::size_t>(fc) * pv.nv + row), m.b[i].re * rs,
-                                 m.b[i].im * rs);
-            }
-        }
-    }
-    if (bad)
-        atomicExch(args.error, ERR_DISTANCE);
-    if (args.samples) {
-        unsigned long long c = counted;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1)
-            c += __shfl_xor(c, off, 64);
-        if ((tid & 63) == 0 && c)
-            atomicAdd(args.samples, c);
-    }
-    __syncthreads();
-    R *A = matrix_of<R>(args);
-    for (std::uint32_t e = tid; e < nA * nB; e += kTileThreads) {
-        const std::uint32_t a = e % nA, b = e / nA;
-        const R *slot = tile + b * stride + 2 * a;
-        const std::uint32_t row = srow[a], col = scol[b];
-        const double rs = srs[a];
-        const R re = slot[0], im = slot[1];
-        if (re != R(0) || im != R(0))
-            atomic_add_entry(A + 2 * (static_cast<std::size_t>(col) * pv.nv + row), re * rs, im * rs);
-    }
-}
-
-// ---- symmetric regular tiles (mixed form) ----
+// ---- symmetric regular tiles ----
 //
 // One workgroup per unordered cluster pair (A, B), A <= B. Each regular panel pair
 // (t in A, s in B) is integrated once on its rule x rule samples and both ordered
@@ -588,72 +380,100 @@ void regular_tiles(AssemblyArgs args, ClusterView cl, NearView near, const doubl
 // entries go to node tile AB ([B node][A node]), the (s, t) entries to node tile BA
 // ([A node][B node]); on a diagonal tile (A = B) only pairs t < s are taken and both
 // orientations go to tile AB. Geometry: vertex 0 in FP64 (the origin difference is
-// formed in FP64 and rounded), edges, normals, curls and area in FP32.
-constexpr int kSymFields = G_COUNT - G_E1;     // FP32 fields: e1, e2, n, curls, area
+// formed in FP64 and rounded to R), edges, normals, curls and area in R.
+constexpr int kSymFields = G_COUNT - G_E1;     // fields held in R: e1, e2, n, curls, area
 constexpr int kStrideAB = 2 * kTileNodes;      // lanes run along A nodes (the column offset)
 constexpr int kStrideBA = 2 * kTileNodes + 1;  // lanes run along A nodes (the row index): odd stride
 enum : int { F_E1 = 0, F_E2 = 3, F_N = 6, F_C0 = 9, F_AREA = 18 };
 
-struct SymSums {
-    CplxT<float> g[3][3], p[3][3], q[3][3]; // [test basis i][trial basis j]
+template <class R> struct SymSums {
+    CplxT<R> g[3][3], p[3][3], q[3][3]; // [test basis i][trial basis j]
 };
 
-struct SymPair {
-    float b[3], te1[3], te2[3], se1[3], se2[3];
-    float Q11, Q12, Q22, A1, A2, C0, T1, T2, Bn, wscale;
+template <class R> struct SymPair {
+    R b[3], te1[3], te2[3], se1[3], se2[3];
+    R Q11, Q12, Q22, A1, A2, C0, T1, T2, Bn, wscale;
 };
+
+// Shared part of a sample: g = wq exp(-ikr)/r and radial = g (1 + i k r), and 1/r^2.
+// FP32: hardware rsqrt, the phase reduced in revolutions, hardware sin/cos; FP64:
+// full-accuracy rsqrt and the Cody-Waite sin/cos of k r.
+template <class R> struct SymCore {
+    CplxT<R> g, radial;
+    R ir2;
+};
+
+__device__ __forceinline__ SymCore<float> sym_core(float r2, float wq, float k, float k_rev) {
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * rinv;
+    const float f = __builtin_amdgcn_fractf(r * k_rev);
+    const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
+    const float kr = k * r, scale = wq * rinv;
+    SymCore<float> o;
+    o.g = {cs * scale, -sn * scale};
+    o.radial = {o.g.re - o.g.im * kr, o.g.im + o.g.re * kr};
+    o.ir2 = rinv * rinv;
+    return o;
+}
+
+__device__ __forceinline__ SymCore<double> sym_core(double r2, double wq, double k, double) {
+    const double rinv = rsqrt_full(r2);
+    const double r = r2 * rinv, kr = k * r;
+    double sn, cs;
+    sincos_reduced(kr, &sn, &cs);
+    const double scale = wq * rinv;
+    SymCore<double> o;
+    o.g = {cs * scale, -sn * scale};
+    o.radial = {o.g.re - o.g.im * kr, o.g.im + o.g.re * kr};
+    o.ir2 = rinv * rinv;
+    return o;
+}
 
 // Sums of one pair over rule x rule: N > 0 takes both rules' points from the
 // kernel-argument array frule (N points, trial loop unrolled), N = 0 takes n points
 // from the FP64 table drule.
-template <int N>
-__device__ __forceinline__ void contract_sym(SymSums &T, const SymPair &c, float k, float k_rev,
-                                             const float (*frule)[4], const double *drule, int n_runtime) {
+template <class R, int N>
+__device__ __forceinline__ void contract_sym(SymSums<R> &T, const SymPair<R> &c, R k, R k_rev,
+                                             const R (*frule)[4], const double *drule, int n_runtime) {
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            T.g[i][j] = T.p[i][j] = T.q[i][j] = {0.f, 0.f};
+            T.g[i][j] = T.p[i][j] = T.q[i][j] = {R(0), R(0)};
     const int n = N > 0 ? N : n_runtime;
 #pragma unroll 1
     for (int xi = 0; xi < n; ++xi) {
-        float x0, x1, x2, xw;
+        R x0, x1, x2, xw;
         if constexpr (N > 0) {
             x0 = frule[xi][0];
             x1 = frule[xi][1];
             x2 = frule[xi][2];
             xw = frule[xi][3];
         } else {
-            x0 = static_cast<float>(drule[4 * xi]);
-            x1 = static_cast<float>(drule[4 * xi + 1]);
-            x2 = static_cast<float>(drule[4 * xi + 2]);
-            xw = static_cast<float>(drule[4 * xi + 3]);
+            x0 = static_cast<R>(drule[4 * xi]);
+            x1 = static_cast<R>(drule[4 * xi + 1]);
+            x2 = static_cast<R>(drule[4 * xi + 2]);
+            xw = static_cast<R>(drule[4 * xi + 3]);
         }
         xw *= c.wscale;
-        float blt[3];
+        R blt[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
             blt[i] = c.b[i] + (c.te1[i] * x1 + c.te2[i] * x2);
-        const float P1 = -2.f * (blt[0] * c.se1[0] + blt[1] * c.se1[1] + blt[2] * c.se1[2]);
-        const float P2 = -2.f * (blt[0] * c.se2[0] + blt[1] * c.se2[1] + blt[2] * c.se2[2]);
-        const float B = blt[0] * blt[0] + blt[1] * blt[1] + blt[2] * blt[2];
-        const float fyn = c.Bn + x1 * c.T1 + x2 * c.T2;
-        CplxT<float> sg[3] = {{0, 0}, {0, 0}, {0, 0}}, sp[3] = {{0, 0}, {0, 0}, {0, 0}},
-                     sq[3] = {{0, 0}, {0, 0}, {0, 0}};
-        auto sample_at = [&](float y0, float y1, float y2, float yw) {
-            const float t1 = y1 * c.Q11 + (y2 * c.Q12 + P1), t2 = y2 * c.Q22 + P2;
-            const float r2 = y1 * t1 + (y2 * t2 + B);
-            const float dntk = c.C0 - (y1 * c.A1 + y2 * c.A2);
-            const float rinv = __builtin_amdgcn_rsqf(r2);
-            const float r = r2 * rinv;
-            const float f = __builtin_amdgcn_fractf(r * k_rev);
-            const float sn = __builtin_amdgcn_sinf(f), cs = __builtin_amdgcn_cosf(f);
-            const float kr = k * r, scale = (xw * yw) * rinv;
-            const CplxT<float> g = {cs * scale, -sn * scale};
-            const CplxT<float> radial = {g.re - g.im * kr, g.im + g.re * kr};
-            const float ir2 = rinv * rinv, tau = fyn * ir2, fxk = -dntk * ir2;
-            const CplxT<float> P = {radial.re * tau, radial.im * tau}, Q = {radial.re * fxk, radial.im * fxk};
-            const float yb[3] = {y0, y1, y2};
+        const R P1 = R(-2) * (blt[0] * c.se1[0] + blt[1] * c.se1[1] + blt[2] * c.se1[2]);
+        const R P2 = R(-2) * (blt[0] * c.se2[0] + blt[1] * c.se2[1] + blt[2] * c.se2[2]);
+        const R B = blt[0] * blt[0] + blt[1] * blt[1] + blt[2] * blt[2];
+        const R fyn = c.Bn + x1 * c.T1 + x2 * c.T2;
+        CplxT<R> sg[3] = {{0, 0}, {0, 0}, {0, 0}}, sp[3] = {{0, 0}, {0, 0}, {0, 0}}, sq[3] = {{0, 0}, {0, 0}, {0, 0}};
+        auto sample_at = [&](R y0, R y1, R y2, R yw) {
+            const R t1 = y1 * c.Q11 + (y2 * c.Q12 + P1), t2 = y2 * c.Q22 + P2;
+            const R r2 = y1 * t1 + (y2 * t2 + B);
+            const R dntk = c.C0 - (y1 * c.A1 + y2 * c.A2);
+            const SymCore<R> o = sym_core(r2, xw * yw, k, k_rev);
+            const CplxT<R> g = o.g, radial = o.radial;
+            const R tau = fyn * o.ir2, fxk = -dntk * o.ir2;
+            const CplxT<R> P = {radial.re * tau, radial.im * tau}, Q = {radial.re * fxk, radial.im * fxk};
+            const R yb[3] = {y0, y1, y2};
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 sg[j] = cadd_t(sg[j], cscale_t(g, yb[j]));
@@ -669,10 +489,10 @@ __device__ __forceinline__ void contract_sym(SymSums &T, const SymPair &c, float
         } else {
 #pragma unroll 2
             for (int yi = 0; yi < n; ++yi)
-                sample_at(static_cast<float>(drule[4 * yi]), static_cast<float>(drule[4 * yi + 1]),
-                          static_cast<float>(drule[4 * yi + 2]), static_cast<float>(drule[4 * yi + 3]));
+                sample_at(static_cast<R>(drule[4 * yi]), static_cast<R>(drule[4 * yi + 1]),
+                          static_cast<R>(drule[4 * yi + 2]), static_cast<R>(drule[4 * yi + 3]));
         }
-        const float xb[3] = {x0, x1, x2};
+        const R xb[3] = {x0, x1, x2};
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -684,7 +504,7 @@ __device__ __forceinline__ void contract_sym(SymSums &T, const SymPair &c, float
     }
 }
 
-// Rule of a regular pair of a general tile (the same decisions as regular_tiles):
+// Rule of a regular pair of a general tile:
 // -1 for a touching pair, a pair of the near list or an escaped pair (which raises
 // ERR_ESCAPED), else the rule table offset; count receives the point count.
 __device__ __forceinline__ int general_pair_rule(const AssemblyArgs &args, const NearView &near, std::uint32_t ti,
@@ -720,20 +540,20 @@ __device__ __forceinline__ int general_pair_rule(const AssemblyArgs &args, const
 }
 
 // FAR: the tile list holds far tiles only (A < B, every pair disjoint, not near,
-// degree 4 at this wavenumber). Occupancy: two 512-thread workgroups per CU
-// (LDS <= 80 KiB, VGPRs <= 128).
-template <bool FAR>
-__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(4)))
+// degree 4 at this wavenumber). Occupancy: FP32, two 512-thread workgroups per CU
+// (LDS <= 80 KiB, VGPRs <= 128); FP64, one (LDS <= 160 KiB, VGPRs <= 256).
+template <class R, bool FAR>
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2)))
 void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const double *rules,
-                       const std::uint32_t *tiles, TileRules<float> fixed) {
-    __shared__ float tab[kTileNodes * kStrideAB];
-    __shared__ float tba[kTileNodes * kStrideBA];
+                       const std::uint32_t *tiles, TileRules<R> fixed) {
+    __shared__ R tab[kTileNodes * kStrideAB];
+    __shared__ R tba[kTileNodes * kStrideBA];
     __shared__ double sv0[2][3][kTilePanels];
-    __shared__ float sgf[2][kSymFields][kTilePanels];
+    __shared__ R sgf[2][kSymFields][kTilePanels];
     __shared__ std::uint32_t snode[2][3][kTilePanels];
     __shared__ std::uint8_t slocal[2][3][kTilePanels];
     __shared__ std::int32_t sfc[2][kTilePanels];
-    __shared__ float srs[2][kTileNodes];
+    __shared__ R srs[2][kTileNodes];
 
     const std::uint32_t ca = tiles[2 * blockIdx.x], cb = tiles[2 * blockIdx.x + 1];
     const bool diag = !FAR && ca == cb;
@@ -745,10 +565,10 @@ void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const d
     const int tid = threadIdx.x;
 
     for (int e = tid; e < kTileNodes * kStrideAB; e += kTileThreads)
-        tab[e] = 0.f;
+        tab[e] = R(0);
     if (!diag)
         for (int e = tid; e < kTileNodes * kStrideBA; e += kTileThreads)
-            tba[e] = 0.f;
+            tba[e] = R(0);
     for (int e = tid; e < 2 * G_COUNT * kTilePanels; e += kTileThreads) {
         const int c = e / (G_COUNT * kTilePanels), f = (e / kTilePanels) % G_COUNT, p = e % kTilePanels;
         if (p < static_cast<int>(c ? nS : nT)) {
@@ -756,7 +576,7 @@ void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const d
             if (f < G_E1)
                 sv0[c][f][p] = v;
             else
-                sgf[c][f - G_E1][p] = static_cast<float>(v);
+                sgf[c][f - G_E1][p] = static_cast<R>(v);
         }
     }
     for (int e = tid; e < 2 * kTilePanels; e += kTileThreads) {
@@ -770,13 +590,12 @@ void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const d
             }
         }
         if (p < static_cast<int>(c ? nB : nA))
-            srs[c][p] = static_cast<float>(args.row_scale[cl.nodes[(c ? b0 : a0) + p]]);
+            srs[c][p] = static_cast<R>(args.row_scale[cl.nodes[(c ? b0 : a0) + p]]);
     }
     __syncthreads();
 
     const double k = args.k;
-    const float kf = static_cast<float>(k), inv_k = static_cast<float>(1.0 / k),
-                k_rev = static_cast<float>(k * 0.15915494309189533577);
+    const R kf = static_cast<R>(k), inv_k = static_cast<R>(1.0 / k), k_rev = static_cast<R>(k * 0.15915494309189533577);
     const std::uint32_t pairs = nT * nS;
     unsigned long long counted = 0;
     bool bad = false;
@@ -798,14 +617,14 @@ void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const d
             if (offset < 0)
                 continue;
         }
-        SymPair c;
+        SymPair<R> c;
         {
             const double bd[3] = {sv0[0][0][lt] - sv0[1][0][ls], sv0[0][1][lt] - sv0[1][1][ls],
                                   sv0[0][2][lt] - sv0[1][2][ls]};
-            float tn[3], sn[3];
+            R tn[3], sn[3];
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                c.b[i] = static_cast<float>(bd[i]);
+                c.b[i] = static_cast<R>(bd[i]);
                 c.te1[i] = sgf[0][F_E1 + i][lt];
                 c.te2[i] = sgf[0][F_E2 + i][lt];
                 c.se1[i] = sgf[1][F_E1 + i][ls];
@@ -813,65 +632,65 @@ void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const d
                 tn[i] = sgf[0][F_N + i][lt];
                 sn[i] = sgf[1][F_N + i][ls];
             }
-            auto dot = [](const float *u, const float *v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+            auto dot = [](const R *u, const R *v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
             c.Q11 = dot(c.se1, c.se1);
-            c.Q12 = 2.f * dot(c.se1, c.se2);
+            c.Q12 = R(2) * dot(c.se1, c.se2);
             c.Q22 = dot(c.se2, c.se2);
             c.A1 = dot(c.se1, tn) * inv_k;
             c.A2 = dot(c.se2, tn) * inv_k;
-            c.C0 = static_cast<float>((bd[0] * tn[0] + bd[1] * tn[1] + bd[2] * tn[2]) * (1.0 / k));
+            c.C0 = static_cast<R>((bd[0] * tn[0] + bd[1] * tn[1] + bd[2] * tn[2]) * (1.0 / k));
             c.T1 = dot(c.te1, sn);
             c.T2 = dot(c.te2, sn);
-            c.Bn = static_cast<float>(bd[0] * sn[0] + bd[1] * sn[1] + bd[2] * sn[2]);
-            c.wscale = 4.f * sgf[0][F_AREA][lt] * sgf[1][F_AREA][ls] * 0.07957747154594766788f;
+            c.Bn = static_cast<R>(bd[0] * sn[0] + bd[1] * sn[1] + bd[2] * sn[2]);
+            c.wscale = R(4) * sgf[0][F_AREA][lt] * sgf[1][F_AREA][ls] * R(0.07957747154594766788);
         }
-        SymSums T;
+        SymSums<R> T;
         if constexpr (FAR) {
-            contract_sym<6>(T, c, kf, k_rev, fixed.d4, nullptr, 6);
+            contract_sym<R, 6>(T, c, kf, k_rev, fixed.d4, nullptr, 6);
         } else {
             const int first = __builtin_amdgcn_readfirstlane(offset);
             const bool uniform = __all(offset == first);
             if (uniform && first == kRule6)
-                contract_sym<12>(T, c, kf, k_rev, fixed.d6, nullptr, 12);
+                contract_sym<R, 12>(T, c, kf, k_rev, fixed.d6, nullptr, 12);
             else if (uniform && first == kRule4)
-                contract_sym<6>(T, c, kf, k_rev, fixed.d4, nullptr, 6);
+                contract_sym<R, 6>(T, c, kf, k_rev, fixed.d4, nullptr, 6);
             else if (uniform && first == kRule8)
-                contract_sym<16>(T, c, kf, k_rev, fixed.d8, nullptr, 16);
+                contract_sym<R, 16>(T, c, kf, k_rev, fixed.d8, nullptr, 16);
             else
-                contract_sym<0>(T, c, kf, k_rev, nullptr, rules + 4 * offset, count);
+                contract_sym<R, 0>(T, c, kf, k_rev, nullptr, rules + 4 * offset, count);
         }
         counted += 2ull * static_cast<unsigned long long>(count) * count;
-        CplxT<float> G = {0.f, 0.f};
+        CplxT<R> G = {R(0), R(0)};
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j)
                 G = cadd_t(G, T.g[i][j]);
         bad |= !isfinite(G.re) || !isfinite(G.im);
-        float tn[3], sn[3];
+        R tn[3], sn[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             tn[i] = sgf[0][F_N + i][lt];
             sn[i] = sgf[1][F_N + i][ls];
         }
-        const float knn = kf * (tn[0] * sn[0] + tn[1] * sn[1] + tn[2] * sn[2]);
-        const CplxT<float> Gk = {G.re * inv_k, G.im * inv_k};
+        const R knn = kf * (tn[0] * sn[0] + tn[1] * sn[1] + tn[2] * sn[2]);
+        const CplxT<R> Gk = {G.re * inv_k, G.im * inv_k};
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             const int a = slocal[0][i][lt];
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 const int b = slocal[1][j][ls];
-                const float cc = sgf[0][F_C0 + 3 * i][lt] * sgf[1][F_C0 + 3 * j][ls] +
+                const R cc = sgf[0][F_C0 + 3 * i][lt] * sgf[1][F_C0 + 3 * j][ls] +
                                  sgf[0][F_C0 + 3 * i + 1][lt] * sgf[1][F_C0 + 3 * j + 1][ls] +
                                  sgf[0][F_C0 + 3 * i + 2][lt] * sgf[1][F_C0 + 3 * j + 2][ls];
-                const CplxT<float> tg = T.g[i][j], tp = T.p[i][j], tq = T.q[i][j];
+                const CplxT<R> tg = T.g[i][j], tp = T.p[i][j], tq = T.q[i][j];
                 // (t, s): row = t's node i (A), column = s's node j (B)
-                float *ts = tab + b * kStrideAB + 2 * a;
+                R *ts = tab + b * kStrideAB + 2 * a;
                 unsafeAtomicAdd(ts, -tp.re - knn * tg.im + Gk.im * cc);
                 unsafeAtomicAdd(ts + 1, -tp.im + knn * tg.re - Gk.re * cc);
                 // (s, t): row = s's node j (B), column = t's node i (A)
-                float *st = diag ? tab + a * kStrideAB + 2 * b : tba + a * kStrideBA + 2 * b;
+                R *st = diag ? tab + a * kStrideAB + 2 * b : tba + a * kStrideBA + 2 * b;
                 unsafeAtomicAdd(st, -kf * tq.re - knn * tg.im + Gk.im * cc);
                 unsafeAtomicAdd(st + 1, -kf * tq.im + knn * tg.re - Gk.re * cc);
             }
@@ -880,7 +699,7 @@ void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const d
         if (fs >= 0) {
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                float re = 0.f, im = 0.f;
+                R re = 0, im = 0;
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
                     re -= T.g[i][j].re + T.q[i][j].im;
@@ -894,7 +713,7 @@ void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const d
         if (ft >= 0) {
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                float re = 0.f, im = 0.f;
+                R re = 0, im = 0;
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
                     re -= T.g[i][j].re + T.p[i][j].im * inv_k;
@@ -917,13 +736,13 @@ void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const d
             atomicAdd(args.samples, v);
     }
     __syncthreads();
-    float *A = args.A32;
+    R *A = matrix_of<R>(args);
     for (std::uint32_t e = tid; e < nA * nB; e += kTileThreads) {
         const std::uint32_t a = e % nA, b = e / nA;
-        const float re = tab[b * kStrideAB + 2 * a], im = tab[b * kStrideAB + 2 * a + 1];
-        if (re != 0.f || im != 0.f) {
+        const R re = tab[b * kStrideAB + 2 * a], im = tab[b * kStrideAB + 2 * a + 1];
+        if (re != R(0) || im != R(0)) {
             const std::uint32_t row = cl.nodes[a0 + a], col = cl.nodes[b0 + b];
-            const float rs = srs[0][a];
+            const R rs = srs[0][a];
             unsafeAtomicAdd(A + 2 * (static_cast<std::size_t>(col) * pv.nv + row), re * rs);
             unsafeAtomicAdd(A + 2 * (static_cast<std::size_t>(col) * pv.nv + row) + 1, im * rs);
         }
@@ -931,10 +750,10 @@ void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const d
     if (!diag)
         for (std::uint32_t e = tid; e < nA * nB; e += kTileThreads) {
             const std::uint32_t b = e % nB, a = e / nB;
-            const float re = tba[a * kStrideBA + 2 * b], im = tba[a * kStrideBA + 2 * b + 1];
-            if (re != 0.f || im != 0.f) {
+            const R re = tba[a * kStrideBA + 2 * b], im = tba[a * kStrideBA + 2 * b + 1];
+            if (re != R(0) || im != R(0)) {
                 const std::uint32_t row = cl.nodes[b0 + b], col = cl.nodes[a0 + a];
-                const float rs = srs[1][b];
+                const R rs = srs[1][b];
                 unsafeAtomicAdd(A + 2 * (static_cast<std::size_t>(col) * pv.nv + row), re * rs);
                 unsafeAtomicAdd(A + 2 * (static_cast<std::size_t>(col) * pv.nv + row) + 1, im * rs);
             }
@@ -1230,7 +1049,7 @@ __global__ void flux_rhs(const double *BD, std::uint32_t nv, std::uint32_t nd, c
 }
 
 // Regular (disjoint, not near) pairs of a list, one thread per pair: the same
-// rule selection as regular_tiles (all-pairs degree byte, else the qualified
+// rule selection as regular_tiles_sym (all-pairs degree byte, else the qualified
 // degree at the solve wavenumber) and FP64 contraction.
 template <class Sink>
 __global__ __launch_bounds__(256) void regular_pairs(AssemblyArgs args, NearView near, const std::uint32_t *pairs,
@@ -1395,19 +1214,19 @@ void launch_regular(const AssemblyLaunch &L, hipStream_t stream) {
     if (L.args.A32) {
         static const TileRules<float> fixed = make_tile_rules<float>();
         if (L.far_tiles)
-            hipLaunchKernelGGL((regular_tiles_sym<true>), dim3(L.far_tiles), dim3(kTileThreads), 0, stream, L.args,
-                               L.clusters, L.near, L.rules, L.tiles, fixed);
+            hipLaunchKernelGGL((regular_tiles_sym<float, true>), dim3(L.far_tiles), dim3(kTileThreads), 0, stream,
+                               L.args, L.clusters, L.near, L.rules, L.tiles, fixed);
         if (general)
-            hipLaunchKernelGGL((regular_tiles_sym<false>), dim3(general), dim3(kTileThreads), 0, stream, L.args,
-                               L.clusters, L.near, L.rules, general_tiles, fixed);
+            hipLaunchKernelGGL((regular_tiles_sym<float, false>), dim3(general), dim3(kTileThreads), 0, stream,
+                               L.args, L.clusters, L.near, L.rules, general_tiles, fixed);
     } else {
         static const TileRules<double> fixed = make_tile_rules<double>();
         if (L.far_tiles)
-            hipLaunchKernelGGL((regular_tiles<double, true>), dim3(2 * L.far_tiles), dim3(kTileThreads), 0, stream,
-                               L.args, L.clusters, L.near, L.rules, L.tiles, L.far_tiles, fixed);
+            hipLaunchKernelGGL((regular_tiles_sym<double, true>), dim3(L.far_tiles), dim3(kTileThreads), 0, stream,
+                               L.args, L.clusters, L.near, L.rules, L.tiles, fixed);
         if (general)
-            hipLaunchKernelGGL((regular_tiles<double, false>), dim3(2 * general), dim3(kTileThreads), 0, stream,
-                               L.args, L.clusters, L.near, L.rules, general_tiles, general, fixed);
+            hipLaunchKernelGGL((regular_tiles_sym<double, false>), dim3(general), dim3(kTileThreads), 0, stream,
+                               L.args, L.clusters, L.near, L.rules, general_tiles, fixed);
     }
 }
 
