@@ -382,8 +382,10 @@ template <> __device__ __forceinline__ float *matrix_of<float>(const AssemblyArg
 // orientations go to tile AB. Geometry: vertex 0 in FP64 (the origin difference is
 // formed in FP64 and rounded to R), edges, normals, curls and area in R.
 constexpr int kSymFields = G_COUNT - G_E1;     // fields held in R: e1, e2, n, curls, area
-constexpr int kStrideAB = 2 * kTileNodes;      // lanes run along A nodes (the column offset)
-constexpr int kStrideBA = 2 * kTileNodes + 1;  // lanes run along A nodes (the row index): odd stride
+// Node tiles: odd strides, so entries of one wavefront's update (8 test x 8 trial
+// panels, see tile_pair) fall in distinct banks whichever index varies.
+constexpr int kStrideAB = 2 * kTileNodes + 1;
+constexpr int kStrideBA = 2 * kTileNodes + 1;
 enum : int { F_E1 = 0, F_E2 = 3, F_N = 6, F_C0 = 9, F_AREA = 18 };
 
 template <class R> struct SymSums {
@@ -392,7 +394,7 @@ template <class R> struct SymSums {
 
 template <class R> struct SymPair {
     R b[3], te1[3], te2[3], se1[3], se2[3];
-    R Q11, Q12, Q22, A1, A2, C0, T1, T2, Bn, wscale;
+    R Q11, Q12, Q22, A1, A2, C0, T1, T2, Bn, wscale, knn;
 };
 
 // Shared part of a sample: g = wq exp(-ikr)/r and radial = g (1 + i k r), and 1/r^2.
@@ -539,6 +541,24 @@ __device__ __forceinline__ int general_pair_rule(const AssemblyArgs &args, const
     return d == 4 ? kRule4 : d == 6 ? kRule6 : kRule8;
 }
 
+// Pair p of an nT x nS tile: blocks of 8 consecutive trial panels, test-major inside a
+// block (the last block holds the remaining nS mod 8), so a wavefront spans 8 test x 8
+// trial panels and its node-tile updates hit fewer equal entries than with one trial
+// panel per wavefront. A bijection of [0, nT nS).
+__device__ __forceinline__ void tile_pair(std::uint32_t p, std::uint32_t nT, std::uint32_t nS, std::uint32_t &lt,
+                                          std::uint32_t &ls) {
+    const std::uint32_t full = nS / 8, span = 8 * nT, blk = p / span;
+    if (blk < full) {
+        const std::uint32_t rem = p - blk * span;
+        lt = rem / 8;
+        ls = 8 * blk + rem % 8;
+    } else {
+        const std::uint32_t r = nS - 8 * full, rem = p - full * span;
+        lt = rem / r;
+        ls = 8 * full + rem % r;
+    }
+}
+
 // FAR: the tile list holds far tiles only (A < B, every pair disjoint, not near,
 // degree 4 at this wavenumber). Occupancy: FP32, two 512-thread workgroups per CU
 // (LDS <= 80 KiB, VGPRs <= 128); FP64, one (LDS <= 160 KiB, VGPRs <= 256).
@@ -599,67 +619,38 @@ void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const d
     const std::uint32_t pairs = nT * nS;
     unsigned long long counted = 0;
     bool bad = false;
-    for (std::uint32_t p = tid; p < pairs; p += kTileThreads) {
-        const std::uint32_t lt = p % nT, ls = p / nT;
-        int offset = kRule4, count = 6;
-        if constexpr (!FAR) {
-            if (diag && lt >= ls)
-                continue;
-            bool touching = false;
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    touching |= snode[0][i][lt] == snode[1][j][ls];
-            if (touching)
-                continue;
-            offset = general_pair_rule(args, near, cl.panels[t0 + lt], cl.panels[s0 + ls], k, count);
-            if (offset < 0)
-                continue;
-        }
+    // Pair constants from the staged geometry (LDS reads only).
+    auto make_pair = [&](std::uint32_t lt, std::uint32_t ls) {
         SymPair<R> c;
-        {
-            const double bd[3] = {sv0[0][0][lt] - sv0[1][0][ls], sv0[0][1][lt] - sv0[1][1][ls],
-                                  sv0[0][2][lt] - sv0[1][2][ls]};
-            R tn[3], sn[3];
+        const double bd[3] = {sv0[0][0][lt] - sv0[1][0][ls], sv0[0][1][lt] - sv0[1][1][ls],
+                              sv0[0][2][lt] - sv0[1][2][ls]};
+        R tn[3], sn[3];
 #pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                c.b[i] = static_cast<R>(bd[i]);
-                c.te1[i] = sgf[0][F_E1 + i][lt];
-                c.te2[i] = sgf[0][F_E2 + i][lt];
-                c.se1[i] = sgf[1][F_E1 + i][ls];
-                c.se2[i] = sgf[1][F_E2 + i][ls];
-                tn[i] = sgf[0][F_N + i][lt];
-                sn[i] = sgf[1][F_N + i][ls];
-            }
-            auto dot = [](const R *u, const R *v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
-            c.Q11 = dot(c.se1, c.se1);
-            c.Q12 = R(2) * dot(c.se1, c.se2);
-            c.Q22 = dot(c.se2, c.se2);
-            c.A1 = dot(c.se1, tn) * inv_k;
-            c.A2 = dot(c.se2, tn) * inv_k;
-            c.C0 = static_cast<R>((bd[0] * tn[0] + bd[1] * tn[1] + bd[2] * tn[2]) * (1.0 / k));
-            c.T1 = dot(c.te1, sn);
-            c.T2 = dot(c.te2, sn);
-            c.Bn = static_cast<R>(bd[0] * sn[0] + bd[1] * sn[1] + bd[2] * sn[2]);
-            c.wscale = R(4) * sgf[0][F_AREA][lt] * sgf[1][F_AREA][ls] * R(0.07957747154594766788);
+        for (int i = 0; i < 3; ++i) {
+            c.b[i] = static_cast<R>(bd[i]);
+            c.te1[i] = sgf[0][F_E1 + i][lt];
+            c.te2[i] = sgf[0][F_E2 + i][lt];
+            c.se1[i] = sgf[1][F_E1 + i][ls];
+            c.se2[i] = sgf[1][F_E2 + i][ls];
+            tn[i] = sgf[0][F_N + i][lt];
+            sn[i] = sgf[1][F_N + i][ls];
         }
-        SymSums<R> T;
-        if constexpr (FAR) {
-            contract_sym<R, 6>(T, c, kf, k_rev, fixed.d4, nullptr, 6);
-        } else {
-            const int first = __builtin_amdgcn_readfirstlane(offset);
-            const bool uniform = __all(offset == first);
-            if (uniform && first == kRule6)
-                contract_sym<R, 12>(T, c, kf, k_rev, fixed.d6, nullptr, 12);
-            else if (uniform && first == kRule4)
-                contract_sym<R, 6>(T, c, kf, k_rev, fixed.d4, nullptr, 6);
-            else if (uniform && first == kRule8)
-                contract_sym<R, 16>(T, c, kf, k_rev, fixed.d8, nullptr, 16);
-            else
-                contract_sym<R, 0>(T, c, kf, k_rev, nullptr, rules + 4 * offset, count);
-        }
-        counted += 2ull * static_cast<unsigned long long>(count) * count;
+        auto dot = [](const R *u, const R *v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+        c.Q11 = dot(c.se1, c.se1);
+        c.Q12 = R(2) * dot(c.se1, c.se2);
+        c.Q22 = dot(c.se2, c.se2);
+        c.A1 = dot(c.se1, tn) * inv_k;
+        c.A2 = dot(c.se2, tn) * inv_k;
+        c.C0 = static_cast<R>((bd[0] * tn[0] + bd[1] * tn[1] + bd[2] * tn[2]) * (1.0 / k));
+        c.T1 = dot(c.te1, sn);
+        c.T2 = dot(c.te2, sn);
+        c.Bn = static_cast<R>(bd[0] * sn[0] + bd[1] * sn[1] + bd[2] * sn[2]);
+        c.wscale = R(4) * sgf[0][F_AREA][lt] * sgf[1][F_AREA][ls] * R(0.07957747154594766788);
+        c.knn = kf * dot(tn, sn);
+        return c;
+    };
+    // Both orientations' entries into the node tiles and, for driven panels, B_D.
+    auto scatter = [&](const SymSums<R> &T, R knn, std::uint32_t lt, std::uint32_t ls) {
         CplxT<R> G = {R(0), R(0)};
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -667,23 +658,33 @@ void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const d
             for (int j = 0; j < 3; ++j)
                 G = cadd_t(G, T.g[i][j]);
         bad |= !isfinite(G.re) || !isfinite(G.im);
-        R tn[3], sn[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            tn[i] = sgf[0][F_N + i][lt];
-            sn[i] = sgf[1][F_N + i][ls];
-        }
-        const R knn = kf * (tn[0] * sn[0] + tn[1] * sn[1] + tn[2] * sn[2]);
         const CplxT<R> Gk = {G.re * inv_k, G.im * inv_k};
+        // Every LDS read of the scatter first, then one wait, then the node-tile updates
+        // back to back: LDS operations complete in order, so a read issued between the
+        // updates would wait for every update queued before it.
+        int a_of[3], b_of[3];
+        R ct[3][3], cs[3][3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            const int a = slocal[0][i][lt];
+            a_of[i] = slocal[0][i][lt];
+            b_of[i] = slocal[1][i][ls];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                ct[i][c] = sgf[0][F_C0 + 3 * i + c][lt];
+                cs[i][c] = sgf[1][F_C0 + 3 * i + c][ls];
+            }
+        }
+        const std::int32_t fs = sfc[1][ls], ft = sfc[0][lt];
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(0xc07f); // lgkmcnt(0); vmcnt and expcnt left open
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int a = a_of[i];
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                const int b = slocal[1][j][ls];
-                const R cc = sgf[0][F_C0 + 3 * i][lt] * sgf[1][F_C0 + 3 * j][ls] +
-                                 sgf[0][F_C0 + 3 * i + 1][lt] * sgf[1][F_C0 + 3 * j + 1][ls] +
-                                 sgf[0][F_C0 + 3 * i + 2][lt] * sgf[1][F_C0 + 3 * j + 2][ls];
+                const int b = b_of[j];
+                const R cc = ct[i][0] * cs[j][0] + ct[i][1] * cs[j][1] + ct[i][2] * cs[j][2];
                 const CplxT<R> tg = T.g[i][j], tp = T.p[i][j], tq = T.q[i][j];
                 // (t, s): row = t's node i (A), column = s's node j (B)
                 R *ts = tab + b * kStrideAB + 2 * a;
@@ -695,7 +696,6 @@ void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const d
                 unsafeAtomicAdd(st + 1, -kf * tq.im + knn * tg.re - Gk.re * cc);
             }
         }
-        const std::int32_t fs = sfc[1][ls], ft = sfc[0][lt];
         if (fs >= 0) {
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
@@ -723,6 +723,64 @@ void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const d
                 const double rs = args.row_scale[row];
                 atomic_add_entry(args.BD + 2 * (static_cast<std::size_t>(ft) * pv.nv + row), re * rs, im * rs);
             }
+        }
+    };
+    if constexpr (FAR) {
+        // Every pair of a far tile is regular: the next pair's constants are read from
+        // LDS before this pair's node-tile updates are queued (the scatter's wait lets
+        // them land first), so the next contraction starts without an LDS wait.
+        std::uint32_t p = tid, lt = 0, ls = 0;
+        SymPair<R> c{};
+        if (p < pairs) {
+            tile_pair(p, nT, nS, lt, ls);
+            c = make_pair(lt, ls);
+        }
+        for (; p < pairs; p += kTileThreads) {
+            SymSums<R> T;
+            contract_sym<R, 6>(T, c, kf, k_rev, fixed.d4, nullptr, 6);
+            counted += 2ull * 36ull;
+            const R knn = c.knn;
+            const std::uint32_t cur_t = lt, cur_s = ls;
+            if (p + kTileThreads < pairs) {
+                tile_pair(p + kTileThreads, nT, nS, lt, ls);
+                c = make_pair(lt, ls);
+            }
+            scatter(T, knn, cur_t, cur_s);
+        }
+    } else {
+        for (std::uint32_t p = tid; p < pairs; p += kTileThreads) {
+            std::uint32_t lt, ls;
+            tile_pair(p, nT, nS, lt, ls);
+            if (diag && lt >= ls)
+                continue;
+            bool touching = false;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    touching |= snode[0][i][lt] == snode[1][j][ls];
+            if (touching)
+                continue;
+            int count = 0;
+            const int offset = general_pair_rule(args, near, cl.panels[t0 + lt], cl.panels[s0 + ls], k, count);
+            if (offset < 0)
+                continue;
+            const SymPair<R> c = make_pair(lt, ls);
+            SymSums<R> T;
+            // the unrolled fixed-rule forms when every active lane of the wavefront
+            // takes the same rule (a uniform branch), else the counted loop
+            const int first = __builtin_amdgcn_readfirstlane(offset);
+            const bool uniform = __all(offset == first);
+            if (uniform && first == kRule6)
+                contract_sym<R, 12>(T, c, kf, k_rev, fixed.d6, nullptr, 12);
+            else if (uniform && first == kRule4)
+                contract_sym<R, 6>(T, c, kf, k_rev, fixed.d4, nullptr, 6);
+            else if (uniform && first == kRule8)
+                contract_sym<R, 16>(T, c, kf, k_rev, fixed.d8, nullptr, 16);
+            else
+                contract_sym<R, 0>(T, c, kf, k_rev, nullptr, rules + 4 * offset, count);
+            counted += 2ull * static_cast<unsigned long long>(count) * count;
+            scatter(T, c.knn, lt, ls);
         }
     }
     if (bad)

@@ -21,6 +21,7 @@
 #include <mutex>
 #include <shared_mutex>
 #include <utility>
+#include <vector>
 
 namespace hsbem::dev {
 namespace {
@@ -107,6 +108,71 @@ __global__ __launch_bounds__(kGemvRows *kGemvWaves) void gemv_partial(const T *_
             part[2 * o] = sr;
             part[2 * o + 1] = si;
         }
+}
+
+// The same split-K partials for an FP32 matrix and one vector with two rows per lane:
+// one 16-byte load per column carries rows (r, r + 1), so a wavefront covers 128 rows
+// with half the load instructions. Needs n even (every column then starts on a
+// 16-byte boundary).
+template <int COLS>
+__global__ __launch_bounds__(kGemvRows *kGemvWaves) void gemv_partial_pairs(const float2 *__restrict__ A,
+                                                                           std::uint32_t n,
+                                                                           const double *__restrict__ x,
+                                                                           std::uint32_t chunks,
+                                                                           double *__restrict__ part, const int *skip) {
+    if (skip && *skip)
+        return;
+    __shared__ double sx[2 * COLS];
+    __shared__ double red[kGemvWaves - 1][4][kGemvRows];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
+    const std::uint32_t row = 2 * (blockIdx.x * kGemvRows + lane);
+    const std::uint32_t c0 = blockIdx.y * COLS;
+    const std::uint32_t cn = min(static_cast<std::uint32_t>(COLS), n - c0);
+    for (std::uint32_t c = threadIdx.x; c < 2 * COLS; c += kGemvRows * kGemvWaves)
+        sx[c] = c / 2 < cn ? x[2 * static_cast<std::size_t>(c0) + c] : 0.0;
+    __syncthreads();
+    double acc[4] = {0.0, 0.0, 0.0, 0.0}; // re, im of row, re, im of row + 1
+    if (row < n) {
+        const std::size_t stride = n / 2; // one column in float4 units
+        const float4 *col = reinterpret_cast<const float4 *>(A + static_cast<std::size_t>(c0) * n + row);
+        auto mac = [&](const float4 a, std::uint32_t cc) {
+            const double xr = sx[2 * cc], xi = sx[2 * cc + 1];
+            const double a0 = a.x, b0 = a.y, a1 = a.z, b1 = a.w;
+            acc[0] += a0 * xr - b0 * xi;
+            acc[1] += a0 * xi + b0 * xr;
+            acc[2] += a1 * xr - b1 * xi;
+            acc[3] += a1 * xi + b1 * xr;
+        };
+        std::uint32_t c = wave;
+        for (; c + kGemvWaves * (kGemvUnroll - 1) < cn; c += kGemvWaves * kGemvUnroll) {
+            float4 a[kGemvUnroll];
+#pragma unroll
+            for (int u = 0; u < kGemvUnroll; ++u)
+                a[u] = col[static_cast<std::size_t>(c + kGemvWaves * u) * stride];
+#pragma unroll
+            for (int u = 0; u < kGemvUnroll; ++u)
+                mac(a[u], c + kGemvWaves * u);
+        }
+        for (; c < cn; c += kGemvWaves)
+            mac(col[static_cast<std::size_t>(c) * stride], c);
+    }
+    if (wave > 0)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            red[wave - 1][q][lane] = acc[q];
+    __syncthreads();
+    if (wave == 0 && row < n) {
+#pragma unroll
+        for (int w = 0; w < kGemvWaves - 1; ++w)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                acc[q] += red[w][q][lane];
+        const std::size_t o = static_cast<std::size_t>(blockIdx.y) * n + row;
+        part[2 * o] = acc[0];
+        part[2 * o + 1] = acc[1];
+        part[2 * o + 2] = acc[2];
+        part[2 * o + 3] = acc[3];
+    }
 }
 
 __global__ void gemv_reduce(const double *part, std::uint32_t n, std::uint32_t chunks, std::uint32_t nvec, double *y,
@@ -993,10 +1059,14 @@ namespace {
 
 template <int COLS>
 void launch_gemv_one(const void *A, bool fp32, std::uint32_t n, const double *x, double *y, double *scratch,
-                     hipStream_t stream, const int *skip) {
+                     hipStream_t stream, const int *skip, bool pairs) {
     const std::uint32_t chunks = (n + COLS - 1) / COLS;
     const dim3 grid((n + kGemvRows - 1) / kGemvRows, chunks);
-    if (fp32)
+    if (fp32 && pairs && n % 2 == 0)
+        hipLaunchKernelGGL((gemv_partial_pairs<COLS>), dim3((n + 2 * kGemvRows - 1) / (2 * kGemvRows), chunks),
+                           dim3(kGemvRows * kGemvWaves), 0, stream, static_cast<const float2 *>(A), n, x, chunks,
+                           scratch, skip);
+    else if (fp32)
         hipLaunchKernelGGL((gemv_partial<1, float2, COLS>), grid, dim3(kGemvRows * kGemvWaves), 0, stream,
                            static_cast<const float2 *>(A), n, x, chunks, scratch, skip);
     else
@@ -1005,21 +1075,28 @@ void launch_gemv_one(const void *A, bool fp32, std::uint32_t n, const double *x,
     hipLaunchKernelGGL(gemv_reduce, dim3((n + 255) / 256), dim3(256), 0, stream, scratch, n, chunks, 1u, y, skip);
 }
 
-void launch_gemv_cols(std::uint32_t cols, const void *A, bool fp32, std::uint32_t n, const double *x, double *y,
+// A product shape: the column chunk (512 / 1024 / 2048) and, for an FP32 matrix of
+// even order, whether each lane takes two rows (kGemvPairs).
+constexpr std::uint32_t kGemvPairs = 1u << 16;
+
+void launch_gemv_cols(std::uint32_t shape, const void *A, bool fp32, std::uint32_t n, const double *x, double *y,
                       double *scratch, hipStream_t stream, const int *skip = nullptr) {
+    const std::uint32_t cols = shape & 0xffffu;
+    const bool pairs = (shape & kGemvPairs) != 0;
     if (cols == 2048)
-        launch_gemv_one<2048>(A, fp32, n, x, y, scratch, stream, skip);
+        launch_gemv_one<2048>(A, fp32, n, x, y, scratch, stream, skip, pairs);
     else if (cols == 1024)
-        launch_gemv_one<1024>(A, fp32, n, x, y, scratch, stream, skip);
+        launch_gemv_one<1024>(A, fp32, n, x, y, scratch, stream, skip, pairs);
     else
-        launch_gemv_one<kGemvCols>(A, fp32, n, x, y, scratch, stream, skip);
+        launch_gemv_one<kGemvCols>(A, fp32, n, x, y, scratch, stream, skip, pairs);
 }
 
-// Tile autotuner of the one-vector product: per (n, precision) the column chunk
-// with the least time over three timed launches (after one warm launch each) on
-// the caller's stream and data; the product it leaves in y is the last candidate's,
-// and the caller's own launch follows. Not tuned (the default chunk) while the
-// stream is being captured; HSBEM_GEMV_COLS = 512 / 1024 / 2048 pins the chunk.
+// Tile autotuner of the one-vector product: per (n, precision) the shape (column
+// chunk, and for an FP32 matrix of even order one or two rows per lane) with the
+// least time over three timed launches (after one warm launch each) on the caller's
+// stream and data; the product it leaves in y is the last candidate's, and the
+// caller's own launch follows. Not tuned (the default chunk) while the stream is
+// being captured; HSBEM_GEMV_COLS = 512 / 1024 / 2048 pins the chunk (one row per lane).
 std::uint32_t gemv_columns(const void *A, bool fp32, std::uint32_t n, const double *x, double *y, double *scratch,
                            hipStream_t stream) {
     if (const char *pin = std::getenv("HSBEM_GEMV_COLS")) {
@@ -1053,17 +1130,21 @@ std::uint32_t gemv_columns(const void *A, bool fp32, std::uint32_t n, const doub
     }
     std::uint32_t best = kGemvCols;
     float best_ms = 0;
-    for (const std::uint32_t cols : {512u, 1024u, 2048u}) {
-        launch_gemv_cols(cols, A, fp32, n, x, y, scratch, stream);
+    std::vector<std::uint32_t> shapes = {512u, 1024u, 2048u};
+    if (fp32 && n % 2 == 0)
+        for (const std::uint32_t cols : {512u, 1024u, 2048u})
+            shapes.push_back(cols | kGemvPairs);
+    for (const std::uint32_t shape : shapes) {
+        launch_gemv_cols(shape, A, fp32, n, x, y, scratch, stream);
         (void)hipEventRecord(start, stream);
         for (int r = 0; r < 3; ++r)
-            launch_gemv_cols(cols, A, fp32, n, x, y, scratch, stream);
+            launch_gemv_cols(shape, A, fp32, n, x, y, scratch, stream);
         (void)hipEventRecord(stop, stream);
         float ms = 0;
         if (hipEventSynchronize(stop) != hipSuccess || hipEventElapsedTime(&ms, start, stop) != hipSuccess)
             continue;
-        if (cols == 512u || ms < best_ms) {
-            best = cols;
+        if (shape == 512u || ms < best_ms) {
+            best = shape;
             best_ms = ms;
         }
     }

@@ -38,7 +38,9 @@ def oracle():
 
 
 def reference():
-    """The reference planner library, or None where it is not built."""
+    """The reference planner library, or None where it is not built. Loaded on the
+    first call only (test modules mark their tests by REF.exists(), so collecting
+    them maps nothing into the process)."""
     if not REF.exists():
         return None
     lib = ctypes.CDLL(str(REF))

@@ -10,7 +10,7 @@ from scipy.special import sph_harm
 import oracle_bind as ob
 from hornsculpt_amd import bem
 
-needs_ref = pytest.mark.skipif(ob.reference() is None, reason="reference planner (oracle/_ref) not built")
+needs_ref = pytest.mark.skipif(not ob.REF.exists(), reason="reference planner (oracle/_ref) not built")
 pytestmark = pytest.mark.filterwarnings("ignore::DeprecationWarning")
 AXES = [(0.3, -0.4, 0.86), (0.0, 0.0, 1.0), (0.0, 0.0, -2.0), (1.0, 0.0, 0.0), (-0.2, 0.9, -0.1)]
 

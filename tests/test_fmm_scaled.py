@@ -16,7 +16,7 @@ from scipy.special import factorial2, sph_harm, spherical_jn, spherical_yn
 import oracle_bind as ob
 from hornsculpt_amd import bem
 
-needs_ref = pytest.mark.skipif(ob.reference() is None, reason="reference planner (oracle/_ref) not built")
+needs_ref = pytest.mark.skipif(not ob.REF.exists(), reason="reference planner (oracle/_ref) not built")
 pytestmark = pytest.mark.filterwarnings("ignore::DeprecationWarning")
 
 POINTS = [(0.01, -0.02, 0.015), (0.0, 0.0, 0.03), (0.0, 0.0, -0.02), (0.1, 0.05, -0.2), (-0.003, 0.004, 0.0)]

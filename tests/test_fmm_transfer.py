@@ -17,7 +17,7 @@ from scipy.special import spherical_jn, sph_harm
 import oracle_bind as ob
 from hornsculpt_amd import bem
 
-needs_ref = pytest.mark.skipif(ob.reference() is None, reason="reference planner (oracle/_ref) not built")
+needs_ref = pytest.mark.skipif(not ob.REF.exists(), reason="reference planner (oracle/_ref) not built")
 pytestmark = pytest.mark.filterwarnings("ignore::DeprecationWarning")
 
 DIRECTIONS = [(0.0, 0.0, 1.0), (0.0, 0.0, -1.0), (1.0, 0.0, 0.0), (0.36, -0.48, 0.8),

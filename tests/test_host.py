@@ -12,7 +12,7 @@ import oracle_bind as ob
 from hornsculpt_amd import bem, meshes
 
 ROOT = Path(__file__).resolve().parents[1]
-needs_ref = pytest.mark.skipif(ob.reference() is None, reason="reference planner (oracle/_ref) not built")
+needs_ref = pytest.mark.skipif(not ob.REF.exists(), reason="reference planner (oracle/_ref) not built")
 
 
 def test_library_exports_every_declared_symbol():

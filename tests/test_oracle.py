@@ -18,7 +18,7 @@ from fixtures import (BEAT_FREQUENCIES, SPHERE_OBSERVERS, TETRA_EXCITATIONS, ana
                       oracle_observer_field)
 from hornsculpt_amd import bem, meshes
 
-needs_ref = pytest.mark.skipif(ob.reference() is None, reason="reference planner (oracle/_ref) not built")
+needs_ref = pytest.mark.skipif(not ob.REF.exists(), reason="reference planner (oracle/_ref) not built")
 
 
 def _factorial(n):

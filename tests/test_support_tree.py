@@ -7,7 +7,7 @@ import pytest
 import oracle_bind as ob
 from hornsculpt_amd import bem, meshes
 
-needs_ref = pytest.mark.skipif(ob.reference() is None, reason="reference planner (oracle/_ref) not built")
+needs_ref = pytest.mark.skipif(not ob.REF.exists(), reason="reference planner (oracle/_ref) not built")
 
 CASES = [
     ("ico3", lambda: meshes.icosphere(3), {}),
