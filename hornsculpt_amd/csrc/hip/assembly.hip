@@ -559,6 +559,53 @@ __device__ __forceinline__ void tile_pair(std::uint32_t p, std::uint32_t nT, std
     }
 }
 
+// Far tiles accumulate their node tiles in fixed point with integer LDS atomics
+// (ds_add_u32 / ds_add_u64; the floating-point LDS atomics are several times slower
+// on gfx950 and bound the far tiles otherwise). The scale is a power of two chosen per
+// tile from a bound on every contribution (see far_scale), so no sum can overflow;
+// the resolution is about 2^-30 (FP32 form) or 2^-51 (FP64 form) of that bound.
+template <class R> struct FixedOf;
+template <> struct FixedOf<float> {
+    using type = int;
+    static constexpr double headroom = 1073741824.0; // 2^30
+};
+template <> struct FixedOf<double> {
+    using type = long long;
+    static constexpr double headroom = 2251799813685248.0; // 2^51: rint by the 1.5 * 2^52 shift
+};
+
+// rint(x) as a 64-bit integer for |x| < 2^51: adding 1.5 * 2^52 leaves the rounded
+// integer in the low mantissa bits (round to nearest even), read back as an integer
+// difference of the two bit patterns.
+__device__ __forceinline__ long long rint_to_int64(double x) {
+    constexpr double shift = 6755399441055744.0; // 1.5 * 2^52
+    return __double_as_longlong(x + shift) - __double_as_longlong(shift);
+}
+
+// Scale of a far tile (clusters fa, fb: kClusterFar doubles each). Every sample
+// point pair of the tile is at least r = |c_a - c_b| - reach_a - reach_b apart, and
+// each entry contribution of a pair is bounded by
+//   Wq [ (1 + k r)/r^2  (P and k Q terms)  +  k/r  (knn G)  +  |curl|^2/(k r)  (G cc / k) ]
+// with Wq the sum of the pair's sample weights (4 A_t A_s/(4 pi) times the squared
+// rule weight sum); at most val_a * val_b pairs add to one entry. Returns false
+// when the bound is not usable (the caller raises ERR_RANGE).
+__device__ __forceinline__ bool far_scale(const double *fa, const double *fb, double k, double weight_sum,
+                                          double headroom, double &scale, double &limit) {
+    const double dx = fa[0] - fb[0], dy = fa[1] - fb[1], dz = fa[2] - fb[2];
+    const double r = (sqrt(dx * dx + dy * dy + dz * dz) - fa[3] - fb[3]) * (1 - 1e-9);
+    const double wq = 4.0 * fa[4] * fb[4] * 0.07957747154594766788 * weight_sum * weight_sum;
+    const double per = 1.0625 * wq * ((1 + k * r) / (r * r) + k / r + fa[5] * fb[5] / (k * r));
+    const double total = per * fa[6] * fb[6];
+    if (!(r > 0) || !(total > 0) || !isfinite(total))
+        return false;
+    int e = 0;
+    frexp(headroom / total, &e);
+    e = min(max(e - 1, -100), 100);
+    scale = ldexp(1.0, e);
+    limit = per * scale;
+    return true;
+}
+
 // FAR: the tile list holds far tiles only (A < B, every pair disjoint, not near,
 // degree 4 at this wavenumber). Occupancy: FP32, two 512-thread workgroups per CU
 // (LDS <= 80 KiB, VGPRs <= 128); FP64, one (LDS <= 160 KiB, VGPRs <= 256).
@@ -617,6 +664,38 @@ void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const d
     const double k = args.k;
     const R kf = static_cast<R>(k), inv_k = static_cast<R>(1.0 / k), k_rev = static_cast<R>(k * 0.15915494309189533577);
     const std::uint32_t pairs = nT * nS;
+    using Fixed = typename FixedOf<R>::type;
+    R fixed_scale = 1, fixed_limit = 0;
+    bool out_of_range = false;
+    if constexpr (FAR) {
+        double weight_sum = 0, scale = 1, limit = 0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            weight_sum += static_cast<double>(fixed.d4[q][3]);
+        if (far_scale(cl.far + kClusterFar * ca, cl.far + kClusterFar * cb, k, weight_sum, FixedOf<R>::headroom,
+                      scale, limit)) {
+            fixed_scale = static_cast<R>(scale);
+            fixed_limit = static_cast<R>(limit);
+        } else {
+            out_of_range = true;
+        }
+    }
+    // One node-tile update: fixed point on far tiles (|v| within the tile bound, else
+    // ERR_RANGE; a non-finite v is reported through `bad`), a floating-point LDS
+    // atomic otherwise.
+    auto tile_add = [&](R *slot, R v) {
+        if constexpr (FAR) {
+            const R x = v * fixed_scale;
+            out_of_range |= fabs(x) > fixed_limit;
+            if constexpr (sizeof(R) == 4)
+                atomicAdd(reinterpret_cast<int *>(slot), static_cast<int>(rintf(x)));
+            else
+                atomicAdd(reinterpret_cast<unsigned long long *>(slot),
+                          static_cast<unsigned long long>(rint_to_int64(x)));
+        } else {
+            unsafeAtomicAdd(slot, v);
+        }
+    };
     unsigned long long counted = 0;
     bool bad = false;
     // Pair constants from the staged geometry (LDS reads only).
@@ -688,12 +767,12 @@ void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const d
                 const CplxT<R> tg = T.g[i][j], tp = T.p[i][j], tq = T.q[i][j];
                 // (t, s): row = t's node i (A), column = s's node j (B)
                 R *ts = tab + b * kStrideAB + 2 * a;
-                unsafeAtomicAdd(ts, -tp.re - knn * tg.im + Gk.im * cc);
-                unsafeAtomicAdd(ts + 1, -tp.im + knn * tg.re - Gk.re * cc);
+                tile_add(ts, -tp.re - knn * tg.im + Gk.im * cc);
+                tile_add(ts + 1, -tp.im + knn * tg.re - Gk.re * cc);
                 // (s, t): row = s's node j (B), column = t's node i (A)
                 R *st = diag ? tab + a * kStrideAB + 2 * b : tba + a * kStrideBA + 2 * b;
-                unsafeAtomicAdd(st, -kf * tq.re - knn * tg.im + Gk.im * cc);
-                unsafeAtomicAdd(st + 1, -kf * tq.im + knn * tg.re - Gk.re * cc);
+                tile_add(st, -kf * tq.re - knn * tg.im + Gk.im * cc);
+                tile_add(st + 1, -kf * tq.im + knn * tg.re - Gk.re * cc);
             }
         }
         if (fs >= 0) {
@@ -726,26 +805,14 @@ void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const d
         }
     };
     if constexpr (FAR) {
-        // Every pair of a far tile is regular: the next pair's constants are read from
-        // LDS before this pair's node-tile updates are queued (the scatter's wait lets
-        // them land first), so the next contraction starts without an LDS wait.
-        std::uint32_t p = tid, lt = 0, ls = 0;
-        SymPair<R> c{};
-        if (p < pairs) {
+        for (std::uint32_t p = tid; p < pairs; p += kTileThreads) {
+            std::uint32_t lt, ls;
             tile_pair(p, nT, nS, lt, ls);
-            c = make_pair(lt, ls);
-        }
-        for (; p < pairs; p += kTileThreads) {
+            const SymPair<R> c = make_pair(lt, ls);
             SymSums<R> T;
             contract_sym<R, 6>(T, c, kf, k_rev, fixed.d4, nullptr, 6);
             counted += 2ull * 36ull;
-            const R knn = c.knn;
-            const std::uint32_t cur_t = lt, cur_s = ls;
-            if (p + kTileThreads < pairs) {
-                tile_pair(p + kTileThreads, nT, nS, lt, ls);
-                c = make_pair(lt, ls);
-            }
-            scatter(T, knn, cur_t, cur_s);
+            scatter(T, c.knn, lt, ls);
         }
     } else {
         for (std::uint32_t p = tid; p < pairs; p += kTileThreads) {
@@ -794,10 +861,20 @@ void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const d
             atomicAdd(args.samples, v);
     }
     __syncthreads();
+    if (out_of_range)
+        atomicExch(args.error, ERR_RANGE);
     R *A = matrix_of<R>(args);
+    // a far tile's entries back from fixed point (exact power-of-two scaling)
+    const R unscale = R(1) / fixed_scale;
+    auto entry = [&](const R *slot) {
+        if constexpr (FAR)
+            return static_cast<R>(*reinterpret_cast<const Fixed *>(slot)) * unscale;
+        else
+            return *slot;
+    };
     for (std::uint32_t e = tid; e < nA * nB; e += kTileThreads) {
         const std::uint32_t a = e % nA, b = e / nA;
-        const R re = tab[b * kStrideAB + 2 * a], im = tab[b * kStrideAB + 2 * a + 1];
+        const R re = entry(tab + b * kStrideAB + 2 * a), im = entry(tab + b * kStrideAB + 2 * a + 1);
         if (re != R(0) || im != R(0)) {
             const std::uint32_t row = cl.nodes[a0 + a], col = cl.nodes[b0 + b];
             const R rs = srs[0][a];
@@ -808,7 +885,7 @@ void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const d
     if (!diag)
         for (std::uint32_t e = tid; e < nA * nB; e += kTileThreads) {
             const std::uint32_t b = e % nB, a = e / nB;
-            const R re = tba[a * kStrideBA + 2 * b], im = tba[a * kStrideBA + 2 * b + 1];
+            const R re = entry(tba + a * kStrideBA + 2 * b), im = entry(tba + a * kStrideBA + 2 * b + 1);
             if (re != R(0) || im != R(0)) {
                 const std::uint32_t row = cl.nodes[b0 + b], col = cl.nodes[a0 + a];
                 const R rs = srs[1][b];

@@ -15,7 +15,7 @@ enum : int { G_V0 = 0, G_E1 = 3, G_E2 = 6, G_N = 9, G_C0 = 12, G_AREA = 21, G_CO
 enum : int { B_C = 0, B_R = 3, B_H = 4, B_COUNT = 5 };
 
 // Error codes of the reference kernels (cuda_solver.cu:705-715).
-enum : int { ERR_DISTANCE = 1, ERR_OVERFLOW = 2, ERR_FIELD = 3, ERR_ESCAPED = 4 };
+enum : int { ERR_DISTANCE = 1, ERR_OVERFLOW = 2, ERR_FIELD = 3, ERR_ESCAPED = 4, ERR_RANGE = 5 };
 
 // Rule table layout (TrianglePoint = bary[3], weight): degree 4, 6, 8, 14.
 enum : int { kRule4 = 0, kRule6 = 6, kRule8 = 18, kRule14 = 34, kRuleTableCount = 98 };
@@ -42,10 +42,15 @@ struct AssemblyArgs {
     unsigned long long *samples; // optional: regular pair samples of the launch (algorithmic count)
 };
 
+// Doubles per cluster in ClusterView::far: centre (3), reach, largest panel area,
+// largest |curl phi_i|, largest number of the cluster's panels sharing a node, pad.
+constexpr int kClusterFar = 8;
+
 struct ClusterView {
     const std::uint32_t *panel_offsets, *panels; // cluster -> panel ids
     const std::uint32_t *node_offsets, *nodes;   // cluster -> internal node ids
     const std::uint8_t *local;                   // [3][nt] slot of each panel vertex in its cluster node list
+    const double *far;                           // kClusterFar per cluster (far-tile fixed-point bounds)
     std::uint32_t count;
 };
 

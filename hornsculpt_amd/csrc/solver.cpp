@@ -56,6 +56,8 @@ void check_error_flag(const DeviceBuffer<int> &flag, hipStream_t s) {
         throw std::runtime_error("observer field: near-field refinement did not reach its tolerance");
     if (code == dev::ERR_ESCAPED)
         throw std::runtime_error("a panel pair needs a rule the interaction plan did not qualify");
+    if (code == dev::ERR_RANGE)
+        throw std::runtime_error("a far-tile entry exceeded its fixed-point bound (internal error)");
 }
 
 bool point_inside(const std::vector<Panel> &panels, Vec3 p) {
@@ -114,7 +116,7 @@ MeshSolver::MeshSolver(Context *ctx, SurfaceMesh mesh) : ctx_(ctx) {
                       lumped_.capacity_bytes() + rules_.capacity_bytes() + nodes_.capacity_bytes() +
                       shape_.capacity_bytes() + local_.capacity_bytes() + cluster_panel_offsets_.capacity_bytes() +
                       cluster_panels_.capacity_bytes() + cluster_node_offsets_.capacity_bytes() +
-                      cluster_nodes_.capacity_bytes();
+                      cluster_nodes_.capacity_bytes() + cluster_far_.capacity_bytes();
     workspace_ = std::make_unique<Workspace>(ctx_->stream);
 }
 
@@ -269,6 +271,31 @@ void MeshSolver::build_clusters() {
             cb.shapes_ok = cb.shapes_ok && shape[t];
         }
     }
+    // Far-tile fixed-point bounds per cluster (dev::kClusterFar doubles): the centre,
+    // the reach (spread + ball: every point of every panel lies within it), the
+    // largest panel area, the largest |curl phi_i| and the largest number of the
+    // cluster's panels sharing one of its nodes.
+    std::vector<double> far(static_cast<std::size_t>(dev::kClusterFar) * cluster_count_, 0.0);
+    for (std::uint32_t c = 0; c < cluster_count_; ++c) {
+        const ClusterBounds &cb = cluster_bounds_[c];
+        double *f = far.data() + static_cast<std::size_t>(dev::kClusterFar) * c;
+        f[0] = cb.center.x;
+        f[1] = cb.center.y;
+        f[2] = cb.center.z;
+        f[3] = cb.spread + cb.ball;
+        std::vector<std::uint32_t> share(node_offsets[c + 1] - node_offsets[c], 0);
+        for (std::uint32_t q = panel_offsets[c]; q < panel_offsets[c + 1]; ++q) {
+            const Panel &pn = space_.panels[panels[q]];
+            f[4] = std::max(f[4], pn.area);
+            for (int l = 0; l < 3; ++l) {
+                f[5] = std::max(f[5], std::sqrt(vnorm2(pn.curls[l])));
+                ++share[local[static_cast<std::size_t>(l) * nt + panels[q]]];
+            }
+        }
+        for (const auto v : share)
+            f[6] = std::max(f[6], static_cast<double>(v));
+    }
+    cluster_far_.upload(far, s);
     cluster_panel_offsets_.upload(panel_offsets, s);
     cluster_panels_.upload(panels, s);
     cluster_node_offsets_.upload(node_offsets, s);
@@ -484,7 +511,7 @@ dev::AssemblyLaunch MeshSolver::assembly_launch(const DevicePlan &p, const Numer
     L.args.regular_order = n.quadrature_order;
     L.args.error = error;
     L.clusters = {cluster_panel_offsets_.data(), cluster_panels_.data(), cluster_node_offsets_.data(),
-                  cluster_nodes_.data(), local_.data(), cluster_count_};
+                  cluster_nodes_.data(), local_.data(), cluster_far_.data(), cluster_count_};
     // far tiles on the uniform degree-4 path at this k: the prefix with k * edge <= 0.5
     // (the same product and comparison as the qualification of each pair)
     L.tiles = p.tiles.data();

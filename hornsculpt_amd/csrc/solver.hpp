@@ -379,6 +379,7 @@ class MeshSolver {
     DeviceBuffer<std::uint32_t> nodes_;
     DeviceBuffer<std::uint8_t> shape_, local_;
     DeviceBuffer<std::uint32_t> cluster_panel_offsets_, cluster_panels_, cluster_node_offsets_, cluster_nodes_;
+    DeviceBuffer<double> cluster_far_; // per cluster: the bounds of the far tiles' fixed-point scale
     std::uint32_t cluster_count_{};
     std::vector<ClusterBounds> cluster_bounds_;
     std::map<std::tuple<unsigned, unsigned, double, double, unsigned, double>, std::shared_ptr<DevicePlan>> plans_;
