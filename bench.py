@@ -46,10 +46,24 @@ FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 (vector and matrix)
 FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 vector (published; half of it without packed FP32 instructions)
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 FLOPS_PER_SAMPLE = 150.0     # FP64 arithmetic of one Burton-Miller pair sample (sqrt/sincos/div not counted)
+# The device counts ordered-pair samples (the Galerkin method's count); the tile kernels
+# integrate each unordered panel pair once for both orientations, so they evaluate
+# half as many sample points. Achieved rates count 150 flops per evaluated point.
+EVALUATED_PER_SAMPLE = 0.5
+
+
+def evaluated_flops(samples):
+    return FLOPS_PER_SAMPLE * EVALUATED_PER_SAMPLE * samples
+
+
+# HBM bytes per leg launch (one frequency point of the 121 360-triangle pool mesh, mean of
+# 2500 Hz and 20 kHz) of regular_tiles_sym<float>: rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE x 1
+# (1 GiB-copy calibration), profiles/hbm_traffic_leg_121k.md. Mostly the node-tile flush:
+# FP32 atomics into A, which the L2 forwards to memory as one request each.
+LEG_TRAFFIC_BYTES = {121360: (142.72 + 523.46) * 1e9}
 # HBM bytes per regular_tiles launch on the default horn mesh (edge 5.2 mm), from rocprofv3
 # FETCH_SIZE + WRITE_SIZE runs scaled by a 1 GiB copy calibration (fetch x2.000, write x1.000):
 # most of it is the node-tile flush through L2 atomics (VALU-bound kernel).
-REGULAR_TRAFFIC_BYTES = {"fp32": (3579.8 + 11537.6) * 1e6, "fp64": (2489.1 + 15827.3) * 1e6}
 
 
 def device_synchronize():
@@ -470,8 +484,9 @@ def roofline_leg(pool, precision, tolerance):
     samples = sum(fr.kernel_milliseconds["regular_samples"] for fr in res.frequencies)
     launches = len(res.frequencies)
     return {"mesh_triangles": e["triangles"], "frequencies_hz": freqs, "launch_ms": ms / launches,
-            "flops_per_launch": FLOPS_PER_SAMPLE * samples / launches,
-            "achieved_tflops": FLOPS_PER_SAMPLE * samples / (ms / 1e3) / 1e12}
+            "flops_per_launch": evaluated_flops(samples) / launches,
+            "achieved_tflops": evaluated_flops(samples) / (ms / 1e3) / 1e12,
+            "ordered_pair_tflops": FLOPS_PER_SAMPLE * samples / (ms / 1e3) / 1e12}
 
 
 def cpu_stream_baseline(pool, threads):
@@ -665,10 +680,12 @@ def main_stream(args):
         "cold": {"plans_s": cold_s},
         "roofline": {"kernel": "regular_tiles (FP32 pair quadrature of the mixed form, VALU; isolated leg)",
                      "bound": "valu", "achieved": leg["achieved_tflops"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": leg["achieved_tflops"] / FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": leg["achieved_tflops"] / FP32_PEAK_TFLOPS,
+                     "traffic": LEG_TRAFFIC_BYTES.get(leg["mesh_triangles"]),
                      "launch_ms": leg["launch_ms"], "flops_per_launch": leg["flops_per_launch"],
                      "leg": {"mesh_triangles": leg["mesh_triangles"], "frequencies_hz": leg["frequencies_hz"]},
-                     "served_achieved": (FLOPS_PER_SAMPLE * head["regular_samples_served"] /
+                     "ordered_pair_tflops": leg["ordered_pair_tflops"],
+                     "served_achieved": (evaluated_flops(head["regular_samples_served"]) /
                                          (head["regular_kernel_ms_served"] / 1e3) / 1e12
                                          if head["regular_kernel_ms_served"] > 0 else None)},
     }
@@ -871,7 +888,7 @@ def main():
     # Kernel roofline from the direct run of the same precision: in the served runs,
     # kernels of other slots overlap and inflate each launch's event time.
     roof = forms.get("mixed" if mixed_head else "fp64", head)
-    achieved = FLOPS_PER_SAMPLE * roof["samples"] / (roof["kernel_ms"] / 1e3) / 1e12 if roof["kernel_ms"] > 0 else None
+    achieved = evaluated_flops(roof["samples"]) / (roof["kernel_ms"] / 1e3) / 1e12 if roof["kernel_ms"] > 0 else None
     line = {
         "metric": "freq-points solved/s + p50/p99 latency, served mixed vs plain FP64 @<=1e-4 err",
         "value": head["value"], "unit": "freq-points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -902,9 +919,9 @@ def main():
         "roofline": {"kernel": f"regular_tiles ({'FP32' if mixed_head else 'FP64'} pair quadrature, VALU)",
                      "bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak if achieved else None,
-                     "traffic": REGULAR_TRAFFIC_BYTES["fp32" if mixed_head else "fp64"] if args.edge == 0.0052 else None,
+                     "traffic": None,
                      "launch_ms": roof["kernel_ms"] / roof["points"],
-                     "flops_per_launch": FLOPS_PER_SAMPLE * roof["samples"] / roof["points"]},
+                     "flops_per_launch": evaluated_flops(roof["samples"]) / roof["points"]},
     }
     if not args.no_fmm:
         try:

@@ -111,12 +111,25 @@ MeshSolver::MeshSolver(Context *ctx, SurfaceMesh mesh) : ctx_(ctx) {
     mesh_ = std::move(mesh);
     build_clusters();
     rules_.upload(rule_table(), ctx_->stream);
+    {
+        std::vector<double> table;
+        for (const unsigned degree : {4u, 6u, 8u, 12u, 14u}) {
+            const auto rule = triangle_rule(degree);
+            endpoint_rule_slots_[degree] = {static_cast<std::uint32_t>(table.size() / 4),
+                                            static_cast<std::uint32_t>(rule.size())};
+            for (const auto &p : rule)
+                table.insert(table.end(), {p.bary[0], p.bary[1], p.bary[2], p.weight});
+        }
+        endpoint_rules_.upload(table, ctx_->stream);
+        unit_.upload(std::vector<double>{1.0, 0.0}, ctx_->stream);
+    }
     hip_check(hipStreamSynchronize(ctx_->stream), "mesh upload");
     geometry_bytes_ = geo_.capacity_bytes() + bounds_.capacity_bytes() + row_scale_.capacity_bytes() +
                       lumped_.capacity_bytes() + rules_.capacity_bytes() + nodes_.capacity_bytes() +
                       shape_.capacity_bytes() + local_.capacity_bytes() + cluster_panel_offsets_.capacity_bytes() +
                       cluster_panels_.capacity_bytes() + cluster_node_offsets_.capacity_bytes() +
-                      cluster_nodes_.capacity_bytes() + cluster_far_.capacity_bytes();
+                      cluster_nodes_.capacity_bytes() + cluster_far_.capacity_bytes() +
+                      endpoint_rules_.capacity_bytes() + unit_.capacity_bytes();
     workspace_ = std::make_unique<Workspace>(ctx_->stream);
 }
 
@@ -1566,11 +1579,8 @@ FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_
             throw std::runtime_error(msg.str());
         }
         dev::lu_solve(ws.LU.data(), nv, ws.pivots.data(), ws.r.data(), ne, s);
-        const double one[2] = {1, 0};
-        DeviceBuffer<double> alpha;
-        alpha.upload(std::vector<double>(one, one + 2), s);
         for (std::uint32_t e = 0; e < ne; ++e)
-            dev::launch_axpy(nv, alpha.data(), ws.r.data() + 2 * static_cast<std::size_t>(e) * nv,
+            dev::launch_axpy(nv, unit_.data(), ws.r.data() + 2 * static_cast<std::size_t>(e) * nv,
                              ws.x.data() + 2 * static_cast<std::size_t>(e) * nv, s);
         hip_check(hipStreamSynchronize(s), "refinement sync");
         ++refinements;
@@ -1585,16 +1595,9 @@ FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_
     if (!observers.empty() && P.has_nonzero) {
         auto evaluate = [&](unsigned degree) {
             cancelled();
-            const auto rule = triangle_rule(degree);
-            std::vector<double> rt;
-            for (const auto &p : rule) {
-                rt.push_back(p.bary[0]);
-                rt.push_back(p.bary[1]);
-                rt.push_back(p.bary[2]);
-                rt.push_back(p.weight);
-            }
-            DeviceBuffer<double> drule;
-            drule.upload(rt, s);
+            const auto slot = endpoint_rule_slots_.find(degree);
+            if (slot == endpoint_rule_slots_.end())
+                throw std::logic_error("no resident observer rule of degree " + std::to_string(degree));
             dev::FieldArgs fa{};
             fa.panels = {geo_.data(), nodes_.data(), bounds_.data(), shape_.data(), nt, nv};
             fa.pressure = ws.x.data();
@@ -1602,8 +1605,8 @@ FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_
             fa.nrhs = ne;
             fa.points = ws.obs.data();
             fa.count = static_cast<std::uint32_t>(observers.size());
-            fa.rule = drule.data();
-            fa.rule_count = static_cast<std::uint32_t>(rule.size());
+            fa.rule = endpoint_rules_.data() + 4 * static_cast<std::size_t>(slot->second.first);
+            fa.rule_count = slot->second.second;
             fa.k = k;
             fa.near_tolerance = std::min(num.near_tolerance, .025 * num.residual_tolerance);
             fa.max_depth = num.near_depth;

@@ -376,6 +376,10 @@ class MeshSolver {
     std::vector<double> lumped_internal_;
     // device geometry
     DeviceBuffer<double> geo_, bounds_, row_scale_, lumped_, rules_;
+    // observer / far-field panel rules of the endpoint degrees (4, 6, 8, 12, 14), resident
+    // so that a served point allocates (and frees, which synchronises the device) nothing
+    DeviceBuffer<double> endpoint_rules_, unit_;
+    std::map<unsigned, std::pair<std::uint32_t, std::uint32_t>> endpoint_rule_slots_; // degree -> (first point, count)
     DeviceBuffer<std::uint32_t> nodes_;
     DeviceBuffer<std::uint8_t> shape_, local_;
     DeviceBuffer<std::uint32_t> cluster_panel_offsets_, cluster_panels_, cluster_node_offsets_, cluster_nodes_;
