@@ -1428,7 +1428,7 @@ int hsbem_dense_gmres(hsbem_context *c, uint32_t n, const double *A, const doubl
         double achieved = 0;
         std::uint32_t its = 0;
         hsbem::dense_gmres(s, dA.data(), false, n, db.data(), dx.data(), restart, maximum_iterations, tolerance,
-                           &achieved, &its, nullptr, ws, nullptr);
+                           &achieved, &its, nullptr, ws, hsbem::BlockJacobi{});
         hsbem::hip_check(hipMemcpyAsync(x, dx.data(), dx.bytes(), hipMemcpyDeviceToHost, s), "x download");
         hsbem::hip_check(hipStreamSynchronize(s), "GMRES sync");
         if (relative_residual)

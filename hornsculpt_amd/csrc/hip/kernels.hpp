@@ -205,6 +205,10 @@ void jacobi_factor_wide(double *blocks, std::uint32_t n, std::uint32_t width, do
                         hipStream_t stream);
 void jacobi_apply_wide(const double *inverses, std::uint32_t n, std::uint32_t width, const double *v, double *out,
                        hipStream_t stream, const std::uint32_t *order = nullptr);
+// The diagonal W x W blocks of a dense column-major complex matrix (FP32 or FP64) in
+// the gathered layout of jacobi_factor_wide (FP64); the tail block's padding is left.
+void jacobi_gather_dense(const void *A, bool fp32, std::uint32_t n, std::uint32_t width, double *blocks,
+                         hipStream_t stream);
 // Block-Jacobi preconditioner of 64 x 64 diagonal blocks (consecutive unknowns):
 // explicit FP64 inverses per block (LU with partial pivoting), and out = M^{-1} v.
 std::size_t jacobi_factor_doubles(std::uint32_t n);

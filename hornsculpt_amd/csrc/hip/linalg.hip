@@ -110,68 +110,77 @@ __global__ __launch_bounds__(kGemvRows *kGemvWaves) void gemv_partial(const T *_
         }
 }
 
-// The same split-K partials for an FP32 matrix and one vector with two rows per lane:
-// one 16-byte load per column carries rows (r, r + 1), so a wavefront covers 128 rows
-// with half the load instructions. Needs n even (every column then starts on a
-// 16-byte boundary).
-template <int COLS>
-__global__ __launch_bounds__(kGemvRows *kGemvWaves) void gemv_partial_pairs(const float2 *__restrict__ A,
-                                                                           std::uint32_t n,
-                                                                           const double *__restrict__ x,
-                                                                           std::uint32_t chunks,
-                                                                           double *__restrict__ part, const int *skip) {
+// The same split-K partials for an FP32 matrix and one vector with ROWS (2 or 4)
+// consecutive rows per lane: ROWS / 2 adjacent 16-byte loads per column, so a
+// wavefront covers 64 ROWS rows with fewer, wider load instructions. Needs n a
+// multiple of ROWS (every column then starts on a 16-byte boundary and no lane is
+// partly past the last row).
+template <int COLS, int ROWS>
+__global__ __launch_bounds__(kGemvRows *kGemvWaves) void gemv_partial_rows(const float2 *__restrict__ A,
+                                                                          std::uint32_t n,
+                                                                          const double *__restrict__ x,
+                                                                          std::uint32_t chunks,
+                                                                          double *__restrict__ part, const int *skip) {
+    static_assert(ROWS == 2 || ROWS == 4, "two or four rows per lane");
+    constexpr int kQuads = ROWS / 2;                 // float4 loads per column per lane
+    constexpr int kUnroll = kGemvUnroll * 2 / ROWS;  // the same bytes in flight for both widths
     if (skip && *skip)
         return;
     __shared__ double sx[2 * COLS];
-    __shared__ double red[kGemvWaves - 1][4][kGemvRows];
+    __shared__ double red[kGemvWaves - 1][2 * ROWS][kGemvRows];
     const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
-    const std::uint32_t row = 2 * (blockIdx.x * kGemvRows + lane);
+    const std::uint32_t row = ROWS * (blockIdx.x * kGemvRows + lane);
     const std::uint32_t c0 = blockIdx.y * COLS;
     const std::uint32_t cn = min(static_cast<std::uint32_t>(COLS), n - c0);
     for (std::uint32_t c = threadIdx.x; c < 2 * COLS; c += kGemvRows * kGemvWaves)
         sx[c] = c / 2 < cn ? x[2 * static_cast<std::size_t>(c0) + c] : 0.0;
     __syncthreads();
-    double acc[4] = {0.0, 0.0, 0.0, 0.0}; // re, im of row, re, im of row + 1
+    double acc[2 * ROWS] = {}; // re, im per row
     if (row < n) {
         const std::size_t stride = n / 2; // one column in float4 units
         const float4 *col = reinterpret_cast<const float4 *>(A + static_cast<std::size_t>(c0) * n + row);
-        auto mac = [&](const float4 a, std::uint32_t cc) {
+        auto mac = [&](const float4 a, int q, std::uint32_t cc) {
             const double xr = sx[2 * cc], xi = sx[2 * cc + 1];
             const double a0 = a.x, b0 = a.y, a1 = a.z, b1 = a.w;
-            acc[0] += a0 * xr - b0 * xi;
-            acc[1] += a0 * xi + b0 * xr;
-            acc[2] += a1 * xr - b1 * xi;
-            acc[3] += a1 * xi + b1 * xr;
+            acc[4 * q] += a0 * xr - b0 * xi;
+            acc[4 * q + 1] += a0 * xi + b0 * xr;
+            acc[4 * q + 2] += a1 * xr - b1 * xi;
+            acc[4 * q + 3] += a1 * xi + b1 * xr;
         };
         std::uint32_t c = wave;
-        for (; c + kGemvWaves * (kGemvUnroll - 1) < cn; c += kGemvWaves * kGemvUnroll) {
-            float4 a[kGemvUnroll];
+        for (; c + kGemvWaves * (kUnroll - 1) < cn; c += kGemvWaves * kUnroll) {
+            float4 a[kUnroll][kQuads];
 #pragma unroll
-            for (int u = 0; u < kGemvUnroll; ++u)
-                a[u] = col[static_cast<std::size_t>(c + kGemvWaves * u) * stride];
+            for (int u = 0; u < kUnroll; ++u)
 #pragma unroll
-            for (int u = 0; u < kGemvUnroll; ++u)
-                mac(a[u], c + kGemvWaves * u);
+                for (int q = 0; q < kQuads; ++q)
+                    a[u][q] = col[static_cast<std::size_t>(c + kGemvWaves * u) * stride + q];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u)
+#pragma unroll
+                for (int q = 0; q < kQuads; ++q)
+                    mac(a[u][q], q, c + kGemvWaves * u);
         }
         for (; c < cn; c += kGemvWaves)
-            mac(col[static_cast<std::size_t>(c) * stride], c);
+#pragma unroll
+            for (int q = 0; q < kQuads; ++q)
+                mac(col[static_cast<std::size_t>(c) * stride + q], q, c);
     }
     if (wave > 0)
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < 2 * ROWS; ++q)
             red[wave - 1][q][lane] = acc[q];
     __syncthreads();
     if (wave == 0 && row < n) {
 #pragma unroll
         for (int w = 0; w < kGemvWaves - 1; ++w)
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
+            for (int q = 0; q < 2 * ROWS; ++q)
                 acc[q] += red[w][q][lane];
         const std::size_t o = static_cast<std::size_t>(blockIdx.y) * n + row;
-        part[2 * o] = acc[0];
-        part[2 * o + 1] = acc[1];
-        part[2 * o + 2] = acc[2];
-        part[2 * o + 3] = acc[3];
+#pragma unroll
+        for (int q = 0; q < 2 * ROWS; ++q)
+            part[2 * o + q] = acc[q];
     }
 }
 
@@ -1032,7 +1041,37 @@ __global__ __launch_bounds__(256) void jacobi_apply_wide(const double *inverses,
     }
 }
 
+// One thread per entry of the gathered blocks: block b, column c, row r takes
+// A[(W b + c) n + W b + r] (complex, widened to FP64).
+template <class T>
+__global__ __launch_bounds__(256) void jacobi_gather_dense_kernel(const T *A, std::uint32_t n, std::uint32_t width,
+                                                                 double *blocks) {
+    const std::size_t e = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const std::size_t W = width, blocks_count = (n + W - 1) / W;
+    if (e >= blocks_count * W * W)
+        return;
+    const std::size_t b = e / (W * W), c = (e / W) % W, r = e % W;
+    const std::size_t row = b * W + r, col = b * W + c;
+    if (row >= n || col >= n)
+        return;
+    const T a = A[col * n + row];
+    blocks[2 * e] = static_cast<double>(a.x);
+    blocks[2 * e + 1] = static_cast<double>(a.y);
+}
+
 } // namespace
+
+void jacobi_gather_dense(const void *A, bool fp32, std::uint32_t n, std::uint32_t width, double *blocks,
+                         hipStream_t stream) {
+    const std::size_t count = static_cast<std::size_t>((n + width - 1) / width) * width * width;
+    const dim3 grid(static_cast<std::uint32_t>((count + 255) / 256));
+    if (fp32)
+        hipLaunchKernelGGL(jacobi_gather_dense_kernel<float2>, grid, dim3(256), 0, stream,
+                           static_cast<const float2 *>(A), n, width, blocks);
+    else
+        hipLaunchKernelGGL(jacobi_gather_dense_kernel<double2>, grid, dim3(256), 0, stream,
+                           static_cast<const double2 *>(A), n, width, blocks);
+}
 
 std::size_t jacobi_wide_doubles(std::uint32_t n, std::uint32_t width) {
     return static_cast<std::size_t>((n + width - 1) / width) * width * width * 2;
@@ -1059,11 +1098,15 @@ namespace {
 
 template <int COLS>
 void launch_gemv_one(const void *A, bool fp32, std::uint32_t n, const double *x, double *y, double *scratch,
-                     hipStream_t stream, const int *skip, bool pairs) {
+                     hipStream_t stream, const int *skip, std::uint32_t rows) {
     const std::uint32_t chunks = (n + COLS - 1) / COLS;
     const dim3 grid((n + kGemvRows - 1) / kGemvRows, chunks);
-    if (fp32 && pairs && n % 2 == 0)
-        hipLaunchKernelGGL((gemv_partial_pairs<COLS>), dim3((n + 2 * kGemvRows - 1) / (2 * kGemvRows), chunks),
+    if (fp32 && rows == 4 && n % 4 == 0)
+        hipLaunchKernelGGL((gemv_partial_rows<COLS, 4>), dim3((n + 4 * kGemvRows - 1) / (4 * kGemvRows), chunks),
+                           dim3(kGemvRows * kGemvWaves), 0, stream, static_cast<const float2 *>(A), n, x, chunks,
+                           scratch, skip);
+    else if (fp32 && rows == 2 && n % 2 == 0)
+        hipLaunchKernelGGL((gemv_partial_rows<COLS, 2>), dim3((n + 2 * kGemvRows - 1) / (2 * kGemvRows), chunks),
                            dim3(kGemvRows * kGemvWaves), 0, stream, static_cast<const float2 *>(A), n, x, chunks,
                            scratch, skip);
     else if (fp32)
@@ -1075,34 +1118,39 @@ void launch_gemv_one(const void *A, bool fp32, std::uint32_t n, const double *x,
     hipLaunchKernelGGL(gemv_reduce, dim3((n + 255) / 256), dim3(256), 0, stream, scratch, n, chunks, 1u, y, skip);
 }
 
-// A product shape: the column chunk (512 / 1024 / 2048) and, for an FP32 matrix of
-// even order, whether each lane takes two rows (kGemvPairs).
-constexpr std::uint32_t kGemvPairs = 1u << 16;
+// A product shape: the column chunk (512 / 1024 / 2048, low 16 bits) and, for an FP32
+// matrix whose order they divide, the rows per lane (1, 2 or 4; bits 16 and up).
+constexpr std::uint32_t gemv_shape(std::uint32_t cols, std::uint32_t rows) { return cols | rows << 16; }
 
 void launch_gemv_cols(std::uint32_t shape, const void *A, bool fp32, std::uint32_t n, const double *x, double *y,
                       double *scratch, hipStream_t stream, const int *skip = nullptr) {
-    const std::uint32_t cols = shape & 0xffffu;
-    const bool pairs = (shape & kGemvPairs) != 0;
+    const std::uint32_t cols = shape & 0xffffu, rows = std::max(shape >> 16, 1u);
     if (cols == 2048)
-        launch_gemv_one<2048>(A, fp32, n, x, y, scratch, stream, skip, pairs);
+        launch_gemv_one<2048>(A, fp32, n, x, y, scratch, stream, skip, rows);
     else if (cols == 1024)
-        launch_gemv_one<1024>(A, fp32, n, x, y, scratch, stream, skip, pairs);
+        launch_gemv_one<1024>(A, fp32, n, x, y, scratch, stream, skip, rows);
     else
-        launch_gemv_one<kGemvCols>(A, fp32, n, x, y, scratch, stream, skip, pairs);
+        launch_gemv_one<kGemvCols>(A, fp32, n, x, y, scratch, stream, skip, rows);
 }
 
 // Tile autotuner of the one-vector product: per (n, precision) the shape (column
-// chunk, and for an FP32 matrix of even order one or two rows per lane) with the
+// chunk, and for an FP32 matrix one, two or four rows per lane) with the
 // least time over three timed launches (after one warm launch each) on the caller's
 // stream and data; the product it leaves in y is the last candidate's, and the
 // caller's own launch follows. Not tuned (the default chunk) while the stream is
-// being captured; HSBEM_GEMV_COLS = 512 / 1024 / 2048 pins the chunk (one row per lane).
+// being captured; HSBEM_GEMV_COLS = 512 / 1024 / 2048 pins the chunk (one row per lane),
+// "COLSx2" / "COLSx4" the chunk and rows per lane (FP32 matrix whose order they divide).
 std::uint32_t gemv_columns(const void *A, bool fp32, std::uint32_t n, const double *x, double *y, double *scratch,
                            hipStream_t stream) {
-    if (const char *pin = std::getenv("HSBEM_GEMV_COLS")) {
-        const unsigned long v = std::strtoul(pin, nullptr, 10);
-        if (v == 512 || v == 1024 || v == 2048)
-            return static_cast<std::uint32_t>(v);
+    if (const char *pin = std::getenv("HSBEM_GEMV_COLS")) { // "COLS" or "COLSxROWS"
+        char *end = nullptr;
+        const unsigned long v = std::strtoul(pin, &end, 10);
+        unsigned long rows = 1;
+        if (end && *end == 'x')
+            rows = std::strtoul(end + 1, nullptr, 10);
+        const bool rows_ok = rows == 1 || ((rows == 2 || rows == 4) && fp32 && n % rows == 0);
+        if ((v == 512 || v == 1024 || v == 2048) && rows_ok)
+            return gemv_shape(static_cast<std::uint32_t>(v), static_cast<std::uint32_t>(rows));
     }
     // Lookups take a shared lock, so slots on other streams never wait on each
     // other once a size is tuned; only the first use of an (n, precision) pair
@@ -1130,10 +1178,11 @@ std::uint32_t gemv_columns(const void *A, bool fp32, std::uint32_t n, const doub
     }
     std::uint32_t best = kGemvCols;
     float best_ms = 0;
-    std::vector<std::uint32_t> shapes = {512u, 1024u, 2048u};
-    if (fp32 && n % 2 == 0)
-        for (const std::uint32_t cols : {512u, 1024u, 2048u})
-            shapes.push_back(cols | kGemvPairs);
+    std::vector<std::uint32_t> shapes;
+    for (const std::uint32_t rows : {1u, 2u, 4u})
+        if (rows == 1 || (fp32 && n % rows == 0))
+            for (const std::uint32_t cols : {512u, 1024u, 2048u})
+                shapes.push_back(gemv_shape(cols, rows));
     for (const std::uint32_t shape : shapes) {
         launch_gemv_cols(shape, A, fp32, n, x, y, scratch, stream);
         (void)hipEventRecord(start, stream);
@@ -1143,7 +1192,7 @@ std::uint32_t gemv_columns(const void *A, bool fp32, std::uint32_t n, const doub
         float ms = 0;
         if (hipEventSynchronize(stop) != hipSuccess || hipEventElapsedTime(&ms, start, stop) != hipSuccess)
             continue;
-        if (shape == 512u || ms < best_ms) {
+        if (shape == gemv_shape(512u, 1u) || ms < best_ms) {
             best = shape;
             best_ms = ms;
         }

@@ -158,6 +158,8 @@ void Workspace::release() {
     flux_column.release();
     pivots.release();
     jacobi.release();
+    jacobi_blocks.release();
+    jacobi_pivots.release();
     gmres.release();
     bound = 0;
 }
@@ -167,7 +169,8 @@ std::uint64_t Workspace::capacity_bytes() const {
     for (const auto *b : {&A, &LU, &BD, &q, &rhs, &x, &r, &flux, &scratch, &obs, &field, &partials, &fmm_b, &fmm_x,
                            &fmm_r, &fmm_q})
         total += b->capacity_bytes();
-    return total + flux_column.capacity_bytes() + pivots.capacity_bytes() + jacobi.capacity_bytes() + gmres.z.capacity_bytes() + gmres.basis.capacity_bytes() +
+    return total + flux_column.capacity_bytes() + pivots.capacity_bytes() + jacobi.capacity_bytes() +
+           jacobi_blocks.capacity_bytes() + jacobi_pivots.capacity_bytes() + gmres.z.capacity_bytes() + gmres.basis.capacity_bytes() +
            gmres.r.capacity_bytes() + gmres.scratch.capacity_bytes() + gmres.small.capacity_bytes() +
            gmres.multi_basis.capacity_bytes() + gmres.multi_in.capacity_bytes() + gmres.multi_out.capacity_bytes() +
            gmres.multi_small.capacity_bytes();
@@ -655,7 +658,7 @@ constexpr std::uint32_t kPoll = 4;
 void dense_gmres_graph(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const double *b, double *x,
                        std::uint32_t restart, std::uint32_t max_it, double tol, double *achieved,
                        std::uint32_t *iterations, const volatile std::int32_t *cancel, GmresWorkspace &ws,
-                       const double *jacobi) {
+                       const BlockJacobi &jacobi) {
     using C = std::complex<double>;
     const std::size_t vec = 2 * static_cast<std::size_t>(n), stride = restart + 1;
     if (ws.basis.size() < vec * stride)
@@ -701,7 +704,8 @@ void dense_gmres_graph(hipStream_t s, const void *A, bool fp32, std::uint32_t n,
         return std::sqrt(out);
     };
     // The captured step: its arguments (buffers, A, sizes) are fixed; recapture when they change.
-    const auto key = std::make_tuple(A, fp32, n, restart, jacobi, static_cast<const double *>(ws.basis.data()),
+    const auto key = std::make_tuple(A, fp32, n, restart, jacobi.inverses, jacobi.width,
+                                     static_cast<const double *>(ws.basis.data()),
                                      static_cast<const double *>(ws.scratch.data()));
     if (!ws.step_graph || ws.step_key != key) {
         if (ws.step_graph)
@@ -715,7 +719,7 @@ void dense_gmres_graph(hipStream_t s, const void *A, bool fp32, std::uint32_t n,
         dev::gmres_cycle_load(cyc, s);
         const double *in = ws.cycle_current.data();
         if (jacobi) {
-            dev::jacobi_apply(jacobi, n, in, ws.cycle_z.data(), s);
+            apply_block_jacobi(jacobi, n, in, ws.cycle_z.data(), s);
             in = ws.cycle_z.data();
         }
         dev::launch_gemv(A, fp32, n, in, ws.cycle_w.data(), 1, ws.scratch.data(), s, ws.cycle_state.data() + 1);
@@ -805,7 +809,7 @@ void dense_gmres_graph(hipStream_t s, const void *A, bool fp32, std::uint32_t n,
             if (jacobi) { // x += M^{-1} V y
                 hip_check(hipMemsetAsync(ws.r.data(), 0, vec * sizeof(double), s), "gmres clear");
                 dev::launch_subtract_combination(ws.basis.data(), n, used, ybuf, ws.r.data(), s);
-                dev::jacobi_apply(jacobi, n, ws.r.data(), ws.cycle_z.data(), s);
+                apply_block_jacobi(jacobi, n, ws.r.data(), ws.cycle_z.data(), s);
                 dev::launch_add(n, ws.cycle_z.data(), x, s);
             } else {
                 dev::launch_subtract_combination(ws.basis.data(), n, used, ybuf, x, s);
@@ -821,6 +825,26 @@ void dense_gmres_graph(hipStream_t s, const void *A, bool fp32, std::uint32_t n,
     throw std::runtime_error(msg.str());
 }
 
+} // namespace
+
+void apply_block_jacobi(const BlockJacobi &jacobi, std::uint32_t n, const double *v, double *out, hipStream_t s) {
+    if (jacobi.width == 64)
+        dev::jacobi_apply(jacobi.inverses, n, v, out, s);
+    else
+        dev::jacobi_apply_wide(jacobi.inverses, n, jacobi.width, v, out, s);
+}
+
+std::uint32_t dense_jacobi_width() {
+    if (const char *v = std::getenv("HSBEM_DENSE_JACOBI_WIDTH")) {
+        const unsigned long w = std::strtoul(v, nullptr, 10);
+        if (w >= 64 && w <= dev::kMaxJacobiWidth && (w & (w - 1)) == 0)
+            return static_cast<std::uint32_t>(w);
+    }
+    return 64;
+}
+
+namespace {
+
 bool gmres_graph_enabled() {
     const char *v = std::getenv("HSBEM_GMRES_GRAPH");
     return !(v && v[0] == '0');
@@ -830,7 +854,7 @@ bool gmres_graph_enabled() {
 
 void dense_gmres(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const double *b, double *x,
                  std::uint32_t restart, std::uint32_t max_it, double tol, double *achieved, std::uint32_t *iterations,
-                 const volatile std::int32_t *cancel, GmresWorkspace &ws, const double *jacobi) {
+                 const volatile std::int32_t *cancel, GmresWorkspace &ws, const BlockJacobi &jacobi) {
     if (gmres_graph_enabled()) {
         dense_gmres_graph(s, A, fp32, n, b, x, restart, max_it, tol, achieved, iterations, cancel, ws, jacobi);
         return;
@@ -846,14 +870,16 @@ void dense_gmres(hipStream_t s, const void *A, bool fp32, std::uint32_t n, const
         dev::launch_residual(A, fp32, n, bb, xx, 1, r, ws.scratch.data(), st);
     };
     if (jacobi)
-        op.precondition = [jacobi, n](const double *v, double *z, hipStream_t st) { dev::jacobi_apply(jacobi, n, v, z, st); };
+        op.precondition = [jacobi, n](const double *v, double *z, hipStream_t st) {
+            apply_block_jacobi(jacobi, n, v, z, st);
+        };
     gmres(s, op, n, b, x, restart, max_it, tol, achieved, iterations, cancel, ws);
 }
 
 void dense_gmres_multi(hipStream_t s, const void *A, bool fp32, std::uint32_t n, std::uint32_t nrhs, const double *b,
                        double *x, std::uint32_t restart, std::uint32_t max_it, double tol, double *achieved,
                        std::uint32_t *iterations, const volatile std::int32_t *cancel, GmresWorkspace &ws,
-                       const double *jacobi) {
+                       const BlockJacobi &jacobi) {
     using C = std::complex<double>;
     if (nrhs == 0 || nrhs > kGmresBatch)
         throw std::invalid_argument("dense_gmres_multi takes 1 to 8 right-hand sides");
@@ -954,7 +980,7 @@ void dense_gmres_multi(hipStream_t s, const void *A, bool fp32, std::uint32_t n,
             const auto c = static_cast<std::uint32_t>(active.size());
             for (std::uint32_t a = 0; a < c; ++a) { // pack M^{-1} v_j of the active systems
                 if (jacobi)
-                    dev::jacobi_apply(jacobi, n, basis(active[a], j), in + a * vec, s);
+                    apply_block_jacobi(jacobi, n, basis(active[a], j), in + a * vec, s);
                 else
                     hip_check(hipMemcpyAsync(in + a * vec, basis(active[a], j), sizeof(double) * vec,
                                              hipMemcpyDeviceToDevice, s), "pack");
@@ -1026,7 +1052,7 @@ void dense_gmres_multi(hipStream_t s, const void *A, bool fp32, std::uint32_t n,
             if (jacobi) {
                 hip_check(hipMemsetAsync(in, 0, sizeof(double) * vec, s), "gmres clear");
                 dev::launch_subtract_combination(basis(e, 0), n, m, h1(e), in, s);
-                dev::jacobi_apply(jacobi, n, in, in + vec * (nrhs > 1 ? 1 : 0), s);
+                apply_block_jacobi(jacobi, n, in, in + vec * (nrhs > 1 ? 1 : 0), s);
                 dev::launch_add(n, in + vec * (nrhs > 1 ? 1 : 0), xe(e), s);
             } else {
                 dev::launch_subtract_combination(basis(e, 0), n, m, h1(e), xe(e), s);
@@ -1314,7 +1340,11 @@ std::shared_ptr<PreparedRequest> MeshSolver::prepare(const Request &req) const {
                                8ull * std::min<std::uint32_t>(ne, kGmresBatch) * (4 * (num.gmres_restart + 2) + 16) +
                                8 * dev::gemv_scratch_doubles(nv, std::min<std::uint32_t>(ne, kGmresBatch))
                          : 0) +
-                 (num.preconditioner ? 8 * dev::jacobi_factor_doubles(nv) : 0);
+                 (!num.preconditioner ? 0
+                  : dense_jacobi_width() == 64
+                      ? 8 * dev::jacobi_factor_doubles(nv)
+                      : 16 * dev::jacobi_wide_doubles(nv, dense_jacobi_width()) +
+                            4 * dev::jacobi_wide_doubles(nv, dense_jacobi_width()) / (2 * dense_jacobi_width()));
     if (nobs)
         bytes += 24 * nobs + 16 * nobs * ne + 8 * dev::field_partial_doubles(nt, static_cast<std::uint32_t>(nobs), ne);
     P->footprint = bytes;
@@ -1527,11 +1557,23 @@ FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_
         hip_check(hipStreamSynchronize(s), "solve sync");
         fr.ms[3] = ms_since(t_t);
     } else {
-        const bool jacobi = num.preconditioner == 1;
-        if (jacobi) {
+        BlockJacobi jacobi;
+        if (num.preconditioner == 1) {
             const auto t_f = Clock::now();
-            ws.jacobi.resize(dev::jacobi_factor_doubles(nv));
-            dev::jacobi_factor(ws.A.data(), mixed, nv, ws.jacobi.data(), s);
+            jacobi.width = dense_jacobi_width();
+            if (jacobi.width == 64) {
+                ws.jacobi.resize(dev::jacobi_factor_doubles(nv));
+                dev::jacobi_factor(ws.A.data(), mixed, nv, ws.jacobi.data(), s);
+            } else {
+                const std::size_t doubles = dev::jacobi_wide_doubles(nv, jacobi.width);
+                ws.jacobi.resize(doubles);
+                ws.jacobi_blocks.resize(doubles);
+                ws.jacobi_pivots.resize(doubles / (2 * jacobi.width));
+                dev::jacobi_gather_dense(ws.A.data(), mixed, nv, jacobi.width, ws.jacobi_blocks.data(), s);
+                dev::jacobi_factor_wide(ws.jacobi_blocks.data(), nv, jacobi.width, ws.jacobi.data(),
+                                        ws.jacobi_pivots.data(), s);
+            }
+            jacobi.inverses = ws.jacobi.data();
             hip_check(hipGetLastError(), "block-Jacobi factor");
             hip_check(hipStreamSynchronize(s), "block-Jacobi sync");
             fr.ms[2] = ms_since(t_f);
@@ -1540,7 +1582,7 @@ FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_
             double achieved = 0;
             dense_gmres(s, ws.A.data(), mixed, nv, ws.rhs.data(), ws.x.data(), num.gmres_restart,
                         num.gmres_max_iterations, num.residual_tolerance * 0.1, &achieved, &iterations[0], cancel,
-                        ws.gmres, jacobi ? ws.jacobi.data() : nullptr);
+                        ws.gmres, jacobi);
         } else { // right-hand sides in lockstep batches sharing each product with A
             std::vector<double> achieved(kGmresBatch);
             for (std::uint32_t e0 = 0; e0 < ne; e0 += kGmresBatch) {
@@ -1548,7 +1590,7 @@ FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_
                 dense_gmres_multi(s, ws.A.data(), mixed, nv, m, ws.rhs.data() + 2 * static_cast<std::size_t>(e0) * nv,
                                   ws.x.data() + 2 * static_cast<std::size_t>(e0) * nv, num.gmres_restart,
                                   num.gmres_max_iterations, num.residual_tolerance * 0.1, achieved.data(),
-                                  &iterations[e0], cancel, ws.gmres, jacobi ? ws.jacobi.data() : nullptr);
+                                  &iterations[e0], cancel, ws.gmres, jacobi);
             }
         }
     }

@@ -172,7 +172,8 @@ struct GmresWorkspace {
     DeviceBuffer<double> cycle_current, cycle_w, cycle_z, cycle_small;
     DeviceBuffer<int> cycle_state;
     hipGraphExec_t step_graph{};
-    std::tuple<const void *, bool, std::uint32_t, std::uint32_t, const double *, const double *, const double *>
+    std::tuple<const void *, bool, std::uint32_t, std::uint32_t, const double *, std::uint32_t, const double *,
+               const double *>
         step_key{};
     int *cycle_pinned{};
     std::size_t cycle_pinned_ints{};
@@ -198,7 +199,8 @@ struct Workspace {
     DeviceBuffer<double> A, LU, BD, q, rhs, x, r, flux, scratch, obs, field, partials;
     DeviceBuffer<double> fmm_b, fmm_x, fmm_r, fmm_q; // FMM form, mesh vertex order
     DeviceBuffer<std::int32_t> flux_column, pivots;
-    DeviceBuffer<double> jacobi;
+    DeviceBuffer<double> jacobi, jacobi_blocks; // inverses; gathered blocks of a wide block-Jacobi
+    DeviceBuffer<std::int32_t> jacobi_pivots;
     DeviceBuffer<int> error;
     DeviceBuffer<unsigned long long> samples;
     GmresWorkspace gmres;
@@ -206,12 +208,24 @@ struct Workspace {
     std::uint64_t bound{}; // id of the PreparedRequest whose per-request data is uploaded
 };
 
+// Block-Jacobi inverses of a dense GMRES: 64 x 64 blocks (jacobi_factor layout) or
+// wider ones (jacobi_factor_wide layout); no preconditioner when inverses is null.
+struct BlockJacobi {
+    const double *inverses = nullptr;
+    std::uint32_t width = 64;
+    explicit operator bool() const { return inverses != nullptr; }
+};
+void apply_block_jacobi(const BlockJacobi &jacobi, std::uint32_t n, const double *v, double *out, hipStream_t stream);
+// Width of the dense forms' block-Jacobi: HSBEM_DENSE_JACOBI_WIDTH (a power of two in
+// 64 ... 2048), else 64.
+std::uint32_t dense_jacobi_width();
+
 // Restarted GMRES(restart) with CGS2 Arnoldi on a dense complex system (A n x n,
 // column-major, FP64 or FP32), right-preconditioned by the block-Jacobi inverses
 // when given; converged when ||b - A x|| <= tol ||b||.
 void dense_gmres(hipStream_t stream, const void *A, bool fp32, std::uint32_t n, const double *b, double *x,
                  std::uint32_t restart, std::uint32_t max_it, double tol, double *achieved, std::uint32_t *iterations,
-                 const volatile std::int32_t *cancel, GmresWorkspace &ws, const double *jacobi);
+                 const volatile std::int32_t *cancel, GmresWorkspace &ws, const BlockJacobi &jacobi);
 
 // dense_gmres for nrhs systems A x_e = b_e (b, x: nrhs vectors of n complex,
 // consecutive) in lockstep: every Arnoldi step applies A to the active systems'
@@ -221,7 +235,7 @@ void dense_gmres(hipStream_t stream, const void *A, bool fp32, std::uint32_t n, 
 void dense_gmres_multi(hipStream_t stream, const void *A, bool fp32, std::uint32_t n, std::uint32_t nrhs,
                        const double *b, double *x, std::uint32_t restart, std::uint32_t max_it, double tol,
                        double *achieved, std::uint32_t *iterations, const volatile std::int32_t *cancel,
-                       GmresWorkspace &ws, const double *jacobi);
+                       GmresWorkspace &ws, const BlockJacobi &jacobi);
 
 // A matrix-free operator for gmres(): y = A x and z = M^{-1} v (optional), both
 // enqueued on the given stream; residual (optional) r = b - A x in one pass.
