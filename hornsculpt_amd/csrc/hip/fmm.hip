@@ -19,6 +19,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -305,20 +306,30 @@ __device__ __forceinline__ std::size_t degree_block(int l) {
 //   back      per chunk again: W = e^{-i m phi} d Z;
 //   gather    one block per destination: the sum of its links' W.
 // Y and W share one workspace slot per link, Z uses a second.
-// Degree by degree: d_l and the chunk's degree-l slices staged in LDS; the next
-// degree's values are loaded into registers while the current one is computed.
+// Two degrees per stage, l and top - l: their Wigner blocks d_l and the chunk's
+// slices of both degrees are staged in LDS together, so every stage carries about
+// the same work ((2l+1)^2 + (2(top-l)+1)^2 multiply-adds per link) and the number of
+// barrier pairs halves; the next stage's values are loaded into registers while the
+// current one is computed. Workgroup (chunk, y) takes the stages h = y, y + Y, ... of
+// its chunk (Y = gridDim.y): the degrees are independent, so a level with few chunks
+// spreads its stages over more workgroups instead of running them one after another.
 // Instantiated for orders up to kP = 32, 40 and 48 (LDS and registers sized to kP, so
 // the common orders 24-32 keep three workgroups per CU).
-template <int kP> __global__ __launch_bounds__(256) void rotate_links(FmmTranslateLaunch L, bool back) {
-    constexpr int kStage = (2 * kP + 1) * (2 * kP + 1);
+template <int kP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kP <= 32 ? 3 : 1))) void rotate_links(FmmTranslateLaunch L, bool back) {
+    constexpr int kStage = (2 * kP + 1) * (2 * kP + 1) + 1; // >= n_a^2 + n_b^2 for n_a + n_b <= 2 kP + 2
     constexpr int kStagePerThread = (kStage + 255) / 256;
-    constexpr int kSlicePerThread = (kFmmChunk * (2 * kP + 1) + 255) / 256;
+    constexpr int kSliceWidth = 2 * kP + 2;                 // >= n_a + n_b
+    constexpr int kSlicePerThread = (kFmmChunk * kSliceWidth + 255) / 256;
     __shared__ double dl[kStage];
-    __shared__ C2 xs[kFmmChunk][2 * kP + 1]; // rows >= count: stale, their results are dropped
+    __shared__ C2 xs[kFmmChunk][kSliceWidth]; // rows >= count: stale, their results are dropped
     __shared__ C2 ph[2 * kP + 1];
     __shared__ std::uint64_t src_off[kFmmChunk], dst_off[kFmmChunk];
     __shared__ int ord[kFmmChunk];
     const FmmChunk ch = L.chunks[blockIdx.x];
+    const int split = static_cast<int>(blockIdx.y), splits = static_cast<int>(gridDim.y);
+    if (split > static_cast<int>(ch.max_order) / 2)
+        return; // this chunk has fewer stages than the launch has splits
     const FmmRotation rot = L.rotations[ch.rotation];
     const C2 *az = reinterpret_cast<const C2 *>(L.azimuth) + rot.azimuth;
     for (int m = threadIdx.x; m <= 2 * static_cast<int>(rot.order); m += blockDim.x)
@@ -333,66 +344,108 @@ template <int kP> __global__ __launch_bounds__(256) void rotate_links(FmmTransla
     const C2 *in = reinterpret_cast<const C2 *>(back ? L.work_z : L.source_expansions);
     C2 *out = reinterpret_cast<C2 *>(L.work_y);
     const int count = static_cast<int>(ch.count), P = static_cast<int>(rot.order), top = static_cast<int>(ch.max_order);
+    const int stages = top / 2 + 1; // stage h: degrees h and top - h (one degree when they coincide)
+    // degrees of stage h and their sizes; nb = 0 when the stage holds one degree
+    auto degrees = [&](int h, int &la, int &lb, int &na, int &nb) {
+        la = h;
+        lb = top - h;
+        na = 2 * la + 1;
+        nb = lb > la ? 2 * lb + 1 : 0;
+    };
     double pd[kStagePerThread];
     C2 px[kSlicePerThread];
-    auto fetch = [&](int l) {
-        const int n = 2 * l + 1;
-        const double *src = L.small_d + rot.small_d + static_cast<std::size_t>(l) * (2 * l - 1) * (2 * l + 1) / 3;
+    auto fetch = [&](int h) {
+        int la, lb, na, nb;
+        degrees(h, la, lb, na, nb);
+        const double *srca = L.small_d + rot.small_d + degree_block(la);
+        const double *srcb = L.small_d + rot.small_d + degree_block(lb);
 #pragma unroll
         for (int i = 0; i < kStagePerThread; ++i) {
             const int e = threadIdx.x + 256 * i;
-            pd[i] = e < n * n ? src[e] : 0.0;
+            pd[i] = e < na * na ? srca[e] : e < na * na + nb * nb ? srcb[e - na * na] : 0.0;
         }
+        const int w = na + nb;
 #pragma unroll
         for (int i = 0; i < kSlicePerThread; ++i) {
             const int e = threadIdx.x + 256 * i;
             C2 v{0, 0};
-            if (e < count * n) {
-                const int j = e / n, q = e % n;
+            if (e < count * w) {
+                const int j = e / w, q = e % w;
+                const int l = q < na ? la : lb, qq = q < na ? q : q - na;
                 if (l <= ord[j])
-                    v = in[src_off[j] + static_cast<std::size_t>(l) * l + q];
+                    v = in[src_off[j] + static_cast<std::size_t>(l) * l + qq];
             }
             px[i] = v;
         }
     };
-    fetch(0);
-    for (int l = 0; l <= top; ++l) {
-        const int n = 2 * l + 1;
-        const std::size_t base = static_cast<std::size_t>(l) * l;
+    fetch(split);
+    for (int h = split; h < stages; h += splits) {
+        int la, lb, na, nb;
+        degrees(h, la, lb, na, nb);
+        const int w = na + nb;
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < kStagePerThread; ++i) {
             const int e = threadIdx.x + 256 * i;
-            if (e < n * n)
+            if (e < na * na + nb * nb)
                 dl[e] = pd[i];
         }
 #pragma unroll
         for (int i = 0; i < kSlicePerThread; ++i) {
             const int e = threadIdx.x + 256 * i;
-            if (e < count * n) {
-                const int j = e / n, q = e % n;
-                xs[j][q] = back ? px[i] : ph[P + q - l] * px[i];
+            if (e < count * w) {
+                const int j = e / w, q = e % w;
+                const int l = q < na ? la : lb, qq = q < na ? q : q - na;
+                xs[j][q] = back ? px[i] : ph[P + qq - l] * px[i];
             }
         }
         __syncthreads();
-        if (l < top)
-            fetch(l + 1);
-        // thread = (output t, group of 4 links): one Wigner load feeds 4 accumulators
+        if (h + splits < stages)
+            fetch(h + splits);
+        // thread = (output t of one of the two degrees, group of 4 links): one Wigner
+        // load feeds 4 accumulators
         const int groups = (count + 3) / 4;
-        for (int e = threadIdx.x; e < groups * n; e += blockDim.x) {
-            const int g = e / n, t = e % n, j0 = 4 * g;
+        for (int e = threadIdx.x; e < groups * w; e += blockDim.x) {
+            const int g = e / w, tq = e % w, j0 = 4 * g;
+            const bool first = tq < na;
+            const int l = first ? la : lb, n = first ? na : nb, t = first ? tq : tq - na, x0 = first ? 0 : na;
+            const double *dbl = dl + (first ? 0 : na * na);
             C2 s0{0, 0}, s1{0, 0}, s2{0, 0}, s3{0, 0};
             const int stride = back ? 1 : n;
-            const double *drow = back ? dl + t * n : dl + t;
-            for (int q = 0; q < n; ++q) {
+            const double *drow = back ? dbl + t * n : dbl + t;
+            // three steps per pass: their 15 LDS reads are issued before the products
+            // (three, not four, keeps the order-32 kernel within the 168 registers of
+            // three waves per SIMD: measured 4.18 ms per horn apply against 4.57 ms)
+            constexpr int kSteps = 3;
+            int q = 0;
+            for (; q + kSteps <= n; q += kSteps) {
+                double d[kSteps];
+                C2 x[kSteps][4];
+#pragma unroll
+                for (int u = 0; u < kSteps; ++u) {
+                    d[u] = drow[(q + u) * stride];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        x[u][i] = xs[j0 + i][x0 + q + u];
+                }
+#pragma unroll
+                for (int u = 0; u < kSteps; ++u) {
+                    s0 = s0 + d[u] * x[u][0];
+                    s1 = s1 + d[u] * x[u][1];
+                    s2 = s2 + d[u] * x[u][2];
+                    s3 = s3 + d[u] * x[u][3];
+                }
+            }
+            for (; q < n; ++q) {
                 const double d = drow[q * stride];
-                s0 = s0 + d * xs[j0][q];
-                s1 = s1 + d * xs[j0 + 1][q];
-                s2 = s2 + d * xs[j0 + 2][q];
-                s3 = s3 + d * xs[j0 + 3][q];
+                s0 = s0 + d * xs[j0][x0 + q];
+                s1 = s1 + d * xs[j0 + 1][x0 + q];
+                s2 = s2 + d * xs[j0 + 2][x0 + q];
+                s3 = s3 + d * xs[j0 + 3][x0 + q];
             }
             const C2 sv[4] = {s0, s1, s2, s3};
             const C2 rot_back = back ? conj(ph[P + t - l]) : C2{1, 0};
+            const std::size_t base = static_cast<std::size_t>(l) * l;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int j = j0 + i;
@@ -403,19 +456,30 @@ template <int kP> __global__ __launch_bounds__(256) void rotate_links(FmmTransla
     }
 }
 
-// Z = T Y per |m| block for a chunk of links sharing one coaxial plan: the
-// block T_m and the chunk's order-m columns of Y are staged in LDS, the next
-// block prefetched into registers during the current product.
+// Z = T Y per |m| block for a chunk of links sharing one coaxial plan, two blocks per
+// stage (|m| = h and mm - h, which balances the rows x columns of the two): the blocks
+// T_m and the chunk's order-m columns of Y are staged in LDS, the next stage's values
+// prefetched into registers during the current products. Workgroup (chunk, y) takes
+// the stages y, y + Y, ... (Y = gridDim.y) of its chunk's stage sequence.
 // T_m rows are stored with an odd stride (cols | 1) of 16-byte values, so the
 // consecutive rows read by consecutive lanes fall in distinct bank groups.
-template <int kP> __global__ __launch_bounds__(256) void coaxial_links(FmmTranslateLaunch L) {
+template <int kP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kP <= 32 ? 4 : 1))) void coaxial_links(FmmTranslateLaunch L) {
     constexpr int kAxialRows = kP + 1;
-    constexpr int kBlockPerThread = (kAxialRows * kAxialRows + 255) / 256;
-    constexpr int kColumnPerThread = (kFmmChunk * 2 * kAxialRows + 255) / 256;
-    __shared__ C2 tm[kAxialRows * kAxialRows];
-    __shared__ C2 ys[kFmmChunk][2][kAxialRows];
+    // A stage pairs |m| = h with mm - h while their columns fit kColumns (always when
+    // ps <= pt: cols_a + cols_b = ps + 2), else holds one block. Its blocks hold at most
+    // (pt + 1)(ps + 2) <= kAxialRows^2 + kAxialRows values, stored with odd row strides
+    // (at most rows_a + rows_b <= 2 kAxialRows more).
+    constexpr int kBlockElems = kAxialRows * kAxialRows + kAxialRows;
+    constexpr int kBlock = kAxialRows * (kAxialRows + 3);
+    constexpr int kBlockPerThread = (kBlockElems + 255) / 256;
+    constexpr int kColumns = kAxialRows + 1;
+    constexpr int kColumnPerThread = (kFmmChunk * 2 * kColumns + 255) / 256;
+    __shared__ C2 tm[kBlock];
+    __shared__ C2 ys[kFmmChunk][2][kColumns];
     __shared__ std::uint64_t slot[kFmmChunk];
     const FmmChunk ch = L.axial_chunks[blockIdx.x];
+    const int split = static_cast<int>(blockIdx.y), splits = static_cast<int>(gridDim.y);
     const FmmLink first = L.links[L.axial_order[ch.first]];
     const int ps = static_cast<int>(first.source_order), pt = static_cast<int>(first.target_order), mm = min(ps, pt);
     const C2 *T = reinterpret_cast<const C2 *>(L.axial) + L.axial_offsets[first.axial];
@@ -427,70 +491,131 @@ template <int kP> __global__ __launch_bounds__(256) void coaxial_links(FmmTransl
     // degrees above min(ps, pt) in |m| have no source: those outputs are zero
     const int nct = (pt + 1) * (pt + 1);
     __syncthreads();
-    for (int e = threadIdx.x; e < count * nct; e += blockDim.x) {
-        const int j = e / nct, c = e % nct;
-        int l, t;
-        split_index(c, l, t);
-        if (abs(t - l) > mm)
-            z[slot[j] + c] = {0, 0};
-    }
+    if (split == 0)
+        for (int e = threadIdx.x; e < count * nct; e += blockDim.x) {
+            const int j = e / nct, c = e % nct;
+            int l, t;
+            split_index(c, l, t);
+            if (abs(t - l) > mm)
+                z[slot[j] + c] = {0, 0};
+        }
+    struct Pair {
+        int ma, mb, ra, ca, rb, cb; // rb = cb = 0 when the stage holds one block
+        int lo, hi;                 // the |m| still to do after this stage: lo .. hi
+    };
+    // the stage that starts with |m| in lo .. hi still to do
+    auto blocks = [&](int lo, int hi) {
+        Pair q;
+        q.ma = lo;
+        q.ra = pt - lo + 1;
+        q.ca = ps - lo + 1;
+        const bool two = hi > lo && q.ca + (ps - hi + 1) <= kColumns;
+        q.mb = two ? hi : lo;
+        q.rb = two ? pt - hi + 1 : 0;
+        q.cb = two ? ps - hi + 1 : 0;
+        q.lo = lo + 1;
+        q.hi = two ? hi - 1 : hi;
+        return q;
+    };
     C2 pt_[kBlockPerThread], py[kColumnPerThread];
-    std::size_t off = 0;
-    auto fetch = [&](int m, std::size_t o) {
-        const int rows = pt - m + 1, cols = ps - m + 1;
+    auto fetch = [&](const Pair &q) {
+        const C2 *Ta = T + axial_block(q.ma, ps, pt), *Tb = T + axial_block(q.mb, ps, pt);
+        const int sa = q.ra * q.ca, sb = q.rb * q.cb;
 #pragma unroll
         for (int i = 0; i < kBlockPerThread; ++i) {
             const int e = threadIdx.x + 256 * i;
-            pt_[i] = e < rows * cols ? T[o + e] : C2{0, 0};
+            pt_[i] = e < sa ? Ta[e] : e < sa + sb ? Tb[e - sa] : C2{0, 0};
         }
+        const int w = q.ca + q.cb;
 #pragma unroll
         for (int i = 0; i < kColumnPerThread; ++i) {
             const int e = threadIdx.x + 256 * i;
-            C2 v{0, 0};
-            if (e < count * 2 * cols) {
-                const int j = e / (2 * cols), r = e % (2 * cols), sgn = r / cols, n = m + r % cols;
-                v = y[slot[j] + m_major(ps, n, sgn ? -m : m)];
-            }
-            py[i] = v;
+            const bool live = e < count * 2 * w;
+            const int ee = live ? e : 0;
+            const int j = ee / (2 * w), r = ee % (2 * w), sgn = r / w, c = r % w;
+            const int m = c < q.ca ? q.ma : q.mb, n = m + (c < q.ca ? c : c - q.ca);
+            py[i] = live ? y[slot[j] + m_major(ps, n, sgn ? -m : m)] : C2{0, 0};
         }
     };
-    fetch(0, 0);
-    for (int m = 0; m <= mm; ++m) {
-        const int rows = pt - m + 1, cols = ps - m + 1;
+    // the stage `steps` stages after p; false when the sequence ends first
+    auto advance = [&](Pair &p, int steps) {
+        for (int i = 0; i < steps; ++i) {
+            if (p.lo > p.hi)
+                return false;
+            p = blocks(p.lo, p.hi);
+        }
+        return true;
+    };
+    Pair q = blocks(0, mm);
+    if (!advance(q, split))
+        return;
+    fetch(q);
+    for (;;) {
+        const int sa = q.ra * q.ca, w = q.ca + q.cb;
+        const int stride_a = q.ca | 1, stride_b = q.cb | 1, base_b = q.ra * stride_a;
         __syncthreads();
-        const int stride = cols | 1;
 #pragma unroll
         for (int i = 0; i < kBlockPerThread; ++i) {
             const int e = threadIdx.x + 256 * i;
-            if (e < rows * cols)
-                tm[(e / cols) * stride + e % cols] = pt_[i];
+            if (e < sa)
+                tm[(e / q.ca) * stride_a + e % q.ca] = pt_[i];
+            else if (e < sa + q.rb * q.cb)
+                tm[base_b + ((e - sa) / q.cb) * stride_b + (e - sa) % q.cb] = pt_[i];
         }
 #pragma unroll
         for (int i = 0; i < kColumnPerThread; ++i) {
             const int e = threadIdx.x + 256 * i;
-            if (e < count * 2 * cols) {
-                const int j = e / (2 * cols), r = e % (2 * cols), sgn = r / cols;
-                ys[j][sgn][r % cols] = py[i];
+            if (e < count * 2 * w) {
+                const int j = e / (2 * w), r = e % (2 * w), sgn = r / w;
+                ys[j][sgn][r % w] = py[i];
             }
         }
         __syncthreads();
-        off += static_cast<std::size_t>(rows) * cols;
-        if (m < mm)
-            fetch(m + 1, off);
-        // thread = (row, sign, group of 4 links): one T load feeds 4 accumulators
-        const int groups = (count + 3) / 4;
-        for (int e = threadIdx.x; e < groups * 2 * rows; e += blockDim.x) {
-            const int g = e / (2 * rows), r = e % (2 * rows), sgn = r / rows, l = m + r % rows, j0 = 4 * g;
+        Pair next = q;
+        const bool more = advance(next, splits);
+        if (more)
+            fetch(next);
+        // thread = (row of one of the two blocks, sign, group of 4 links): one T load
+        // feeds 4 accumulators
+        const int groups = (count + 3) / 4, rows2 = 2 * (q.ra + q.rb);
+        for (int e = threadIdx.x; e < groups * rows2; e += blockDim.x) {
+            const int g = e / rows2, r = e % rows2, j0 = 4 * g;
+            const int half = r < 2 * q.ra ? 0 : 1;
+            const int rr = half ? r - 2 * q.ra : r, rows = half ? q.rb : q.ra;
+            const int sgn = rr / rows, m = half ? q.mb : q.ma, l = m + rr % rows;
             if (sgn && !m)
                 continue;
+            const int cols = half ? q.cb : q.ca, c0 = half ? q.ca : 0;
+            const C2 *trow = tm + (half ? base_b + (l - m) * stride_b : (l - m) * stride_a);
             C2 s0{0, 0}, s1{0, 0}, s2{0, 0}, s3{0, 0};
-            const C2 *trow = tm + (l - m) * stride;
-            for (int c = 0; c < cols; ++c) {
+            // two columns per pass: their 10 LDS reads are issued before the products (two,
+            // not four, keeps the order-32 kernel within the 128 registers of four waves
+            // per SIMD, which its LDS allows; both measured alike on the horn apply)
+            constexpr int kSteps = 2;
+            int c = 0;
+            for (; c + kSteps <= cols; c += kSteps) {
+                C2 tv[kSteps], yv[kSteps][4];
+#pragma unroll
+                for (int u = 0; u < kSteps; ++u) {
+                    tv[u] = trow[c + u];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        yv[u][i] = ys[j0 + i][sgn][c0 + c + u];
+                }
+#pragma unroll
+                for (int u = 0; u < kSteps; ++u) {
+                    s0 = s0 + tv[u] * yv[u][0];
+                    s1 = s1 + tv[u] * yv[u][1];
+                    s2 = s2 + tv[u] * yv[u][2];
+                    s3 = s3 + tv[u] * yv[u][3];
+                }
+            }
+            for (; c < cols; ++c) {
                 const C2 tv = trow[c];
-                s0 = s0 + tv * ys[j0][sgn][c];
-                s1 = s1 + tv * ys[j0 + 1][sgn][c];
-                s2 = s2 + tv * ys[j0 + 2][sgn][c];
-                s3 = s3 + tv * ys[j0 + 3][sgn][c];
+                s0 = s0 + tv * ys[j0][sgn][c0 + c];
+                s1 = s1 + tv * ys[j0 + 1][sgn][c0 + c];
+                s2 = s2 + tv * ys[j0 + 2][sgn][c0 + c];
+                s3 = s3 + tv * ys[j0 + 3][sgn][c0 + c];
             }
             const C2 sv[4] = {s0, s1, s2, s3};
             const std::size_t idx = static_cast<std::size_t>(l) * l + l + (sgn ? -m : m);
@@ -499,6 +624,9 @@ template <int kP> __global__ __launch_bounds__(256) void coaxial_links(FmmTransl
                 if (j0 + i < count)
                     z[slot[j0 + i] + idx] = sv[i];
         }
+        if (!more)
+            break;
+        q = next;
     }
 }
 
@@ -838,10 +966,22 @@ __global__ __launch_bounds__(256) void gather_links(FmmTranslateLaunch L) {
     const int nct = static_cast<int>((dst.order + 1) * (dst.order + 1));
     const C2 *w = reinterpret_cast<const C2 *>(L.work_y);
     C2 *out = reinterpret_cast<C2 *>(L.target_expansions) + dst.target;
+    const FmmLink *links = L.links + dst.first_link;
+    const std::uint32_t n = dst.link_count;
     for (int e = threadIdx.x; e < nct; e += blockDim.x) {
+        // the sum runs in link order; the loads of four links are issued together
         C2 s{0, 0};
-        for (std::uint32_t i = 0; i < dst.link_count; ++i)
-            s = s + w[L.links[dst.first_link + i].slot + e];
+        std::uint32_t i = 0;
+        for (; i + 4 <= n; i += 4) {
+            const C2 a = w[links[i].slot + e], b = w[links[i + 1].slot + e];
+            const C2 c = w[links[i + 2].slot + e], d = w[links[i + 3].slot + e];
+            s = s + a;
+            s = s + b;
+            s = s + c;
+            s = s + d;
+        }
+        for (; i < n; ++i)
+            s = s + w[links[i].slot + e];
         out[e] = s;
     }
 }
@@ -1126,6 +1266,15 @@ void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destinatio
                       hipStream_t stream) {
     if (!destination_count)
         return;
+    // Levels with few chunks spread each chunk's stages over up to `splits` workgroups,
+    // so that a launch holds about kSplitTarget workgroups (a rotation chunk has
+    // max_order / 2 + 1 stages, a coaxial chunk at most as many).
+    constexpr std::uint32_t kSplitTarget = 2048;
+    auto splits = [&](std::uint32_t chunks) {
+        const std::uint32_t stages = max_order / 2 + 1;
+        return chunks ? std::max(1u, std::min(stages, (kSplitTarget + chunks - 1) / chunks)) : 1u;
+    };
+    const dim3 rot_grid(chunk_count, splits(chunk_count)), axial_grid(axial_chunk_count, splits(axial_chunk_count));
     if (link_count && max_order > kMaxWideOrder) {
         hipLaunchKernelGGL(rotate_links_big, dim3(chunk_count), dim3(256), 0, stream, launch, false);
         hipLaunchKernelGGL(coaxial_links_big, dim3(axial_chunk_count), dim3(256), 0, stream, launch);
@@ -1135,17 +1284,17 @@ void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destinatio
         hipLaunchKernelGGL(coaxial_links_wide, dim3(axial_chunk_count), dim3(256), 0, stream, launch);
         hipLaunchKernelGGL(rotate_links_wide, dim3(chunk_count), dim3(256), 0, stream, launch, true);
     } else if (link_count && max_order > kMaxFmmOrder) {
-        hipLaunchKernelGGL(rotate_links<kMidFmmOrder>, dim3(chunk_count), dim3(256), 0, stream, launch, false);
-        hipLaunchKernelGGL(coaxial_links<kMidFmmOrder>, dim3(axial_chunk_count), dim3(256), 0, stream, launch);
-        hipLaunchKernelGGL(rotate_links<kMidFmmOrder>, dim3(chunk_count), dim3(256), 0, stream, launch, true);
+        hipLaunchKernelGGL(rotate_links<kMidFmmOrder>, rot_grid, dim3(256), 0, stream, launch, false);
+        hipLaunchKernelGGL(coaxial_links<kMidFmmOrder>, axial_grid, dim3(256), 0, stream, launch);
+        hipLaunchKernelGGL(rotate_links<kMidFmmOrder>, rot_grid, dim3(256), 0, stream, launch, true);
     } else if (link_count && max_order > kSmallFmmOrder) {
-        hipLaunchKernelGGL(rotate_links<kMaxFmmOrder>, dim3(chunk_count), dim3(256), 0, stream, launch, false);
-        hipLaunchKernelGGL(coaxial_links<kMaxFmmOrder>, dim3(axial_chunk_count), dim3(256), 0, stream, launch);
-        hipLaunchKernelGGL(rotate_links<kMaxFmmOrder>, dim3(chunk_count), dim3(256), 0, stream, launch, true);
+        hipLaunchKernelGGL(rotate_links<kMaxFmmOrder>, rot_grid, dim3(256), 0, stream, launch, false);
+        hipLaunchKernelGGL(coaxial_links<kMaxFmmOrder>, axial_grid, dim3(256), 0, stream, launch);
+        hipLaunchKernelGGL(rotate_links<kMaxFmmOrder>, rot_grid, dim3(256), 0, stream, launch, true);
     } else if (link_count) {
-        hipLaunchKernelGGL(rotate_links<kSmallFmmOrder>, dim3(chunk_count), dim3(256), 0, stream, launch, false);
-        hipLaunchKernelGGL(coaxial_links<kSmallFmmOrder>, dim3(axial_chunk_count), dim3(256), 0, stream, launch);
-        hipLaunchKernelGGL(rotate_links<kSmallFmmOrder>, dim3(chunk_count), dim3(256), 0, stream, launch, true);
+        hipLaunchKernelGGL(rotate_links<kSmallFmmOrder>, rot_grid, dim3(256), 0, stream, launch, false);
+        hipLaunchKernelGGL(coaxial_links<kSmallFmmOrder>, axial_grid, dim3(256), 0, stream, launch);
+        hipLaunchKernelGGL(rotate_links<kSmallFmmOrder>, rot_grid, dim3(256), 0, stream, launch, true);
     }
     hipLaunchKernelGGL(gather_links, dim3(destination_count), dim3(256), 0, stream, launch);
 }
