@@ -472,6 +472,19 @@ def compare_forms(pool, draws, exact, mixed):
     return worst
 
 
+def tuning_summary():
+    """The autotuner's decisions so far: per tunable, the chosen candidates with their
+    counts over keys ("gemv_shape" decoded to "COLSxROWS"), and every decision."""
+    report = bem.tuning_report()
+    label = {"gemv_shape": lambda c: f"{c & 0xffff}x{max(c >> 16, 1)}"}
+    chosen = {}
+    for r in report:
+        name = label.get(r["tunable"], str)(int(r["chosen"]))
+        chosen.setdefault(r["tunable"], {}).setdefault(name, 0)
+        chosen[r["tunable"]][name] += 1
+    return {"chosen": chosen, "decisions": report}
+
+
 def roofline_leg(pool, precision, tolerance):
     """The dominant kernel alone on the GPU: one direct solve (no other slot) of the
     largest pool mesh at two frequencies of its raster (its limit / 8 and its limit);
@@ -613,7 +626,8 @@ def main_stream(args):
         leg = roofline_leg(pool, bem.PRECISION_MIXED, args.mixed_tolerance)
         print(json.dumps({"metric": "regular_tiles FP32 TFLOP/s (isolated leg of the served stream)",
                           "unit": "TFLOP/s", "value": leg["achieved_tflops"], "higher_is_better": True, "n_gpus": 1,
-                          "dtype": "f32", "data": "synthetic", "config": {"workload": "roofline leg", **leg}}),
+                          "dtype": "f32", "data": "synthetic", "config": {"workload": "roofline leg", **leg},
+                          "tuning": tuning_summary()}),
               flush=True)
         return
     # Cold: one interaction plan per resident mesh, at its frequency limit, through a
@@ -678,6 +692,7 @@ def main_stream(args):
                           "max_directivity_db": errors["directivity_db"],
                           "points_compared": errors["points_compared"]},
         "cold": {"plans_s": cold_s},
+        "tuning": tuning_summary()["chosen"],
         "roofline": {"kernel": "regular_tiles (FP32 pair quadrature of the mixed form, VALU; isolated leg)",
                      "bound": "valu", "achieved": leg["achieved_tflops"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": leg["achieved_tflops"] / FP32_PEAK_TFLOPS,
@@ -772,7 +787,7 @@ def main():
                                                  f"triangles / {fl['nodes']} P1 nodes, 500 Hz and 16 kHz, fast "
                                                  f"multipole form (leaf {args.fmm_leaf}, separation 1.5, GMRES restart "
                                                  f"50, rel tol 1e-6, accuracy seed 1e-7, block-Jacobi width {args.flange_block_width})"},
-                          "flange": fl}), flush=True)
+                          "flange": fl, "tuning": tuning_summary()}), flush=True)
         return
     if args.fixture:
         mesh, req, digest = bem.read_benchmark_fixture(args.fixture)
@@ -849,7 +864,8 @@ def main():
         print(json.dumps({"metric": "FMM pressure operator apply time", "unit": "ms", "value": fmm[-1]["apply_ms"],
                           "higher_is_better": False, "n_gpus": 1, "dtype": "f64", "data": "synthetic",
                           "config": {"workload": f"exponential horn {nt} triangles / {nv} P1 nodes, leaf "
-                                                 f"{args.fmm_leaf}, accuracy seed 5e-7"}, "fmm": fmm}), flush=True)
+                                                 f"{args.fmm_leaf}, accuracy seed 5e-7"}, "fmm": fmm,
+                          "tuning": tuning_summary()}), flush=True)
         return
     cross, cold_plan_ms = None, None
     if not args.no_cross_check:

@@ -14,6 +14,7 @@ The native library is required; there is no fallback path.
 from __future__ import annotations
 
 import ctypes
+import json
 import math
 import dataclasses
 import threading
@@ -114,6 +115,8 @@ def library() -> ctypes.CDLL:
                 "hsbem_numerics_default": (None, [ctypes.POINTER(_Numerics)]),
                 "hsbem_last_error": (ctypes.c_char_p, []),
                 "hsbem_device_description": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+                "hsbem_tuning_report": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t,
+                                                       ctypes.POINTER(ctypes.c_size_t)]),
                 "hsbem_context_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(P)]),
                 "hsbem_context_destroy": (ctypes.c_int, [P]),
                 "hsbem_mesh_create": (ctypes.c_int, [P, D, U32, ctypes.POINTER(U32), ctypes.POINTER(U32),
@@ -415,6 +418,19 @@ def device_description() -> str:
     buf = ctypes.create_string_buffer(256)
     _check(library().hsbem_device_description(buf, 256))
     return buf.value.decode()
+
+
+def tuning_report() -> list:
+    """Decisions of the launch-shape autotuner in this process: one dict per (tunable,
+    key) with the chosen candidate and the timed candidates as [candidate, ms] pairs
+    ("gemv_shape" candidates encode chunk | rows << 16). Pinned values are not listed."""
+    needed = ctypes.c_size_t(0)
+    buf = ctypes.create_string_buffer(1 << 16)
+    _check(library().hsbem_tuning_report(buf, len(buf), ctypes.byref(needed)))
+    if needed.value > len(buf):
+        buf = ctypes.create_string_buffer(needed.value)
+        _check(library().hsbem_tuning_report(buf, len(buf), ctypes.byref(needed)))
+    return json.loads(buf.value.decode())
 
 
 def _native_request(request: BemRequest):

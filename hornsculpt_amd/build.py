@@ -29,7 +29,7 @@ HOST_SOURCES = [
     "host/space.cpp", "host/rules.cpp", "host/adaptive.cpp", "host/plan.cpp", "host/mesh_check.cpp",
     "host/support_tree.cpp", "host/support_near_plan.cpp", "host/fmm_levels.cpp", "host/fmm_rotation.cpp", "host/fmm_transfer.cpp",
     "host/fmm_scaled.cpp", "host/acoustic_analysis.cpp", "host/translation_plan_cache.cpp", "host/frequency_schedule.cpp",
-    "solver.cpp", "near_operator.cpp", "fmm_operator.cpp", "server.cpp", "capi.cpp",
+    "autotune.cpp", "solver.cpp", "near_operator.cpp", "fmm_operator.cpp", "server.cpp", "capi.cpp",
 ]
 DEVICE_SOURCES = ["hip/assembly.hip", "hip/linalg.hip", "hip/field.hip", "hip/fmm.hip", "hip/fmm_hf.hip", "hip/noalloc.hip", "hip/gmres_graph.hip"]
 

@@ -2,6 +2,7 @@
 // way the reference throws them (std::invalid_argument / std::runtime_error).
 #include "../../include/hsbem.h"
 
+#include "autotune.hpp"
 #include "fmm_operator.hpp"
 #include "host/acoustic_analysis.hpp"
 #include "host/frequency_schedule.hpp"
@@ -265,6 +266,17 @@ int hsbem_device_description(char *buffer, size_t length) {
         const std::string s = std::string(prop.name) + " (" + prop.gcnArchName + ", " +
                               std::to_string(prop.multiProcessorCount) + " CUs, " +
                               std::to_string(prop.totalGlobalMem >> 20) + " MiB)";
+        std::strncpy(buffer, s.c_str(), length - 1);
+        buffer[length - 1] = 0;
+    });
+}
+
+int hsbem_tuning_report(char *buffer, size_t length, size_t *needed) {
+    return guarded([&] {
+        require(buffer && length, "null buffer");
+        const std::string s = hsbem::autotune::records_json();
+        if (needed)
+            *needed = s.size() + 1;
         std::strncpy(buffer, s.c_str(), length - 1);
         buffer[length - 1] = 0;
     });

@@ -1,5 +1,7 @@
 #include "fmm_operator.hpp"
 
+#include "autotune.hpp"
+
 #include "host/fmm_levels.hpp"
 #include "host/fmm_rotation.hpp"
 #include "host/fmm_scaled.hpp"
@@ -941,6 +943,28 @@ std::unique_ptr<FmmOperator> MeshSolver::build_fmm(const Numerics &n, double k_m
                       op->work_y_.bytes() + op->work_z_.bytes() + op->hf_legendre_.bytes() + op->diag_table_.bytes() +
                       op->hf_fourier_.bytes() + op->hf_harmonic_.bytes();
     st.shared_bytes = op->near_->values.bytes() + ns->columns.bytes() + op->jacobi_->bytes();
+    // Stage-split target of the translations, chosen by the autotuner per (mesh, number
+    // of translation launches: the tree's spherical levels) from whole applies on a zero
+    // vector; operators of other orders on the same tree (audits, other frequencies)
+    // take the stored choice. HSBEM_FMM_SPLIT pins it (0 = no split).
+    if (!op->translate_.empty()) {
+        std::ostringstream key;
+        key << "mesh " << std::hex << hash_ << std::dec << " (" << nt << " panels), " << op->translate_.size()
+            << " translation launches";
+        const auto pinned = autotune::pinned_from_env("HSBEM_FMM_SPLIT", {0, 1024, 2048, 4096});
+        op->graph_x_.resize(2 * static_cast<std::size_t>(nv));
+        op->graph_y_.resize(2 * static_cast<std::size_t>(nv));
+        hip_check(hipMemsetAsync(op->graph_x_.data(), 0, op->graph_x_.bytes(), s), "tuning input");
+        FmmOperator &o = *op;
+        o.split_target_ = autotune::choose(
+            "fmm_split_target", key.str(), {0, 1024, 2048, 4096}, 2048, s,
+            [&](std::uint32_t c) {
+                o.split_target_ = c;
+                o.apply(o.graph_x_.data(), o.graph_y_.data(), s);
+            },
+            1, pinned);
+        hip_check(hipGetLastError(), "translation tuning");
+    }
     return op;
 }
 
@@ -984,7 +1008,7 @@ void FmmOperator::apply(const double *x, double *y, hipStream_t s) const {
             T.axial_order = axial_order_.data() + L.first_link;
             T.axial_chunks = axial_chunks_.data() + L.first_axial_chunk;
             dev::launch_translate(T, L.destination_count, L.link_count, L.chunk_count, L.axial_chunk_count, L.max_order,
-                                  s);
+                                  split_target_, s);
         } else if (stage.kind == StageKind::angular) {
             const HfLaunch &A = angular_[stage.index];
             H.links = hf_links_.data() + A.first_link;

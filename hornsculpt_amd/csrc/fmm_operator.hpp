@@ -147,6 +147,7 @@ class FmmOperator {
     DeviceBuffer<dev::HfLeafItem> hf_leaf_items_;
     DeviceBuffer<double> leaf_scale_, leaf_rule_;
     std::uint32_t hf_leaf_count_{}, hf_leaf_item_count_{}, hf_max_samples_{}, block_width_{64};
+    std::uint32_t split_target_{2048}; // translations' stage-split target (autotuned per mesh and level orders)
     // direct audit: solver-owned geometry (alive while the operator is), leaf map, near matrix
     const double *geo_{}, *row_scale_{};
     const std::uint32_t *nodes_{};

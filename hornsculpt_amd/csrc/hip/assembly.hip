@@ -541,21 +541,22 @@ __device__ __forceinline__ int general_pair_rule(const AssemblyArgs &args, const
     return d == 4 ? kRule4 : d == 6 ? kRule6 : kRule8;
 }
 
-// Pair p of an nT x nS tile: blocks of 8 consecutive trial panels, test-major inside a
-// block (the last block holds the remaining nS mod 8), so a wavefront spans 8 test x 8
-// trial panels and its node-tile updates hit fewer equal entries than with one trial
-// panel per wavefront. A bijection of [0, nT nS).
-__device__ __forceinline__ void tile_pair(std::uint32_t p, std::uint32_t nT, std::uint32_t nS, std::uint32_t &lt,
-                                          std::uint32_t &ls) {
-    const std::uint32_t full = nS / 8, span = 8 * nT, blk = p / span;
+// Pair p of an nT x nS tile: blocks of W = 2^shift consecutive trial panels (W = 4, 8
+// or 16, chosen per mesh by the tile autotuner), test-major inside a block (the last
+// block holds the remaining nS mod W), so a wavefront spans 64 / W test x W trial
+// panels and its node-tile updates hit fewer equal entries than with one trial panel
+// per wavefront. A bijection of [0, nT nS).
+__device__ __forceinline__ void tile_pair(std::uint32_t p, std::uint32_t nT, std::uint32_t nS, std::uint32_t shift,
+                                          std::uint32_t &lt, std::uint32_t &ls) {
+    const std::uint32_t W = 1u << shift, full = nS >> shift, span = nT << shift, blk = p / span;
     if (blk < full) {
         const std::uint32_t rem = p - blk * span;
-        lt = rem / 8;
-        ls = 8 * blk + rem % 8;
+        lt = rem >> shift;
+        ls = (blk << shift) + (rem & (W - 1));
     } else {
-        const std::uint32_t r = nS - 8 * full, rem = p - full * span;
+        const std::uint32_t r = nS - (full << shift), rem = p - full * span;
         lt = rem / r;
-        ls = 8 * full + rem % r;
+        ls = (full << shift) + rem % r;
     }
 }
 
@@ -612,7 +613,7 @@ __device__ __forceinline__ bool far_scale(const double *fa, const double *fb, do
 template <class R, bool FAR>
 __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2)))
 void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const double *rules,
-                       const std::uint32_t *tiles, TileRules<R> fixed) {
+                       const std::uint32_t *tiles, TileRules<R> fixed, std::uint32_t trial_shift) {
     __shared__ R tab[kTileNodes * kStrideAB];
     __shared__ R tba[kTileNodes * kStrideBA];
     __shared__ double sv0[2][3][kTilePanels];
@@ -807,7 +808,7 @@ void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const d
     if constexpr (FAR) {
         for (std::uint32_t p = tid; p < pairs; p += kTileThreads) {
             std::uint32_t lt, ls;
-            tile_pair(p, nT, nS, lt, ls);
+            tile_pair(p, nT, nS, trial_shift, lt, ls);
             const SymPair<R> c = make_pair(lt, ls);
             SymSums<R> T;
             contract_sym<R, 6>(T, c, kf, k_rev, fixed.d4, nullptr, 6);
@@ -817,7 +818,7 @@ void regular_tiles_sym(AssemblyArgs args, ClusterView cl, NearView near, const d
     } else {
         for (std::uint32_t p = tid; p < pairs; p += kTileThreads) {
             std::uint32_t lt, ls;
-            tile_pair(p, nT, nS, lt, ls);
+            tile_pair(p, nT, nS, trial_shift, lt, ls);
             if (diag && lt >= ls)
                 continue;
             bool touching = false;
@@ -1345,23 +1346,29 @@ template <class R> TileRules<R> make_tile_rules() {
 // tile and its mirror.
 void launch_regular(const AssemblyLaunch &L, hipStream_t stream) {
     const std::uint32_t general = L.tile_count - L.far_tiles;
+    auto shift_of = [](std::uint32_t block) {
+        if (block != 4 && block != 8 && block != 16)
+            throw std::invalid_argument("regular tiles: the trial block must be 4, 8 or 16 panels");
+        return block == 4 ? 2u : block == 8 ? 3u : 4u;
+    };
+    const std::uint32_t shift = shift_of(L.trial_block), far_shift = shift_of(L.far_trial_block);
     const std::uint32_t *general_tiles = L.tiles + 2 * static_cast<std::size_t>(L.far_tiles);
     if (L.args.A32) {
         static const TileRules<float> fixed = make_tile_rules<float>();
         if (L.far_tiles)
             hipLaunchKernelGGL((regular_tiles_sym<float, true>), dim3(L.far_tiles), dim3(kTileThreads), 0, stream,
-                               L.args, L.clusters, L.near, L.rules, L.tiles, fixed);
+                               L.args, L.clusters, L.near, L.rules, L.tiles, fixed, far_shift);
         if (general)
             hipLaunchKernelGGL((regular_tiles_sym<float, false>), dim3(general), dim3(kTileThreads), 0, stream,
-                               L.args, L.clusters, L.near, L.rules, general_tiles, fixed);
+                               L.args, L.clusters, L.near, L.rules, general_tiles, fixed, shift);
     } else {
         static const TileRules<double> fixed = make_tile_rules<double>();
         if (L.far_tiles)
             hipLaunchKernelGGL((regular_tiles_sym<double, true>), dim3(L.far_tiles), dim3(kTileThreads), 0, stream,
-                               L.args, L.clusters, L.near, L.rules, L.tiles, fixed);
+                               L.args, L.clusters, L.near, L.rules, L.tiles, fixed, far_shift);
         if (general)
             hipLaunchKernelGGL((regular_tiles_sym<double, false>), dim3(general), dim3(kTileThreads), 0, stream,
-                               L.args, L.clusters, L.near, L.rules, general_tiles, fixed);
+                               L.args, L.clusters, L.near, L.rules, general_tiles, fixed, shift);
     }
 }
 

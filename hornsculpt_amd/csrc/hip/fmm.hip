@@ -1263,16 +1263,15 @@ void launch_leaf_test(const FmmLeaf *leaves, std::uint32_t leaf_count, const std
 
 void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destination_count, std::uint32_t link_count,
                       std::uint32_t chunk_count, std::uint32_t axial_chunk_count, std::uint32_t max_order,
-                      hipStream_t stream) {
+                      std::uint32_t split_target, hipStream_t stream) {
     if (!destination_count)
         return;
     // Levels with few chunks spread each chunk's stages over up to `splits` workgroups,
-    // so that a launch holds about kSplitTarget workgroups (a rotation chunk has
-    // max_order / 2 + 1 stages, a coaxial chunk at most as many).
-    constexpr std::uint32_t kSplitTarget = 2048;
+    // so that a launch holds about split_target workgroups (a rotation chunk has
+    // max_order / 2 + 1 stages, a coaxial chunk at most as many); 0 = no split.
     auto splits = [&](std::uint32_t chunks) {
         const std::uint32_t stages = max_order / 2 + 1;
-        return chunks ? std::max(1u, std::min(stages, (kSplitTarget + chunks - 1) / chunks)) : 1u;
+        return chunks && split_target ? std::max(1u, std::min(stages, (split_target + chunks - 1) / chunks)) : 1u;
     };
     const dim3 rot_grid(chunk_count, splits(chunk_count)), axial_grid(axial_chunk_count, splits(axial_chunk_count));
     if (link_count && max_order > kMaxWideOrder) {

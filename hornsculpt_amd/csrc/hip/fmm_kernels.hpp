@@ -255,9 +255,11 @@ void launch_leaf_test(const FmmLeaf *leaves, std::uint32_t leaf_count, const std
 // max_order: largest source or target order of the launch's links and rotation plans
 // (selects the tiled kernels above kMaxFmmOrder and the reduction-tiled ones above
 // kMaxWideOrder; at most kMaxBigOrder).
+// split_target: a level with few chunks spreads each chunk's degree / |m| stages over
+// more workgroups, up to about split_target workgroups per launch (0: one per chunk).
 void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destination_count, std::uint32_t link_count,
                       std::uint32_t chunk_count, std::uint32_t axial_chunk_count, std::uint32_t max_order,
-                      hipStream_t stream);
+                      std::uint32_t split_target, hipStream_t stream);
 // blocks (zeroed, n rounded up to width) from the near CSR in tree order;
 // position[dof] = index of dof in order.
 void launch_gather_blocks(const std::uint32_t *row_offsets, const std::uint32_t *columns, const double *values,

@@ -119,6 +119,9 @@ struct AssemblyLaunch {
     const NearPatchItem *near_items;       // tensor patches of the other jobs (per-patch kernel)
     std::uint32_t near_item_count;
     const PairPatch *patches;
+    // regular tiles: trial panels per wavefront block (4, 8 or 16) of the general and
+    // of the far tiles
+    std::uint32_t trial_block = 8, far_trial_block = 8;
 };
 
 void launch_assembly(const AssemblyLaunch &launch, hipStream_t stream);

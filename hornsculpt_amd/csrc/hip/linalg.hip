@@ -13,13 +13,13 @@
 //                   search, row interchanges, unit-lower TRSM, and trailing
 //                   updates on the FP64 matrix cores.
 //   lu_solve        blocked forward/backward substitution.
+#include "../autotune.hpp"
 #include "device.hpp"
 
 #include <algorithm>
 #include <cstdlib>
-#include <map>
-#include <mutex>
-#include <shared_mutex>
+#include <optional>
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -1133,15 +1133,17 @@ void launch_gemv_cols(std::uint32_t shape, const void *A, bool fp32, std::uint32
         launch_gemv_one<kGemvCols>(A, fp32, n, x, y, scratch, stream, skip, rows);
 }
 
-// Tile autotuner of the one-vector product: per (n, precision) the shape (column
-// chunk, and for an FP32 matrix one, two or four rows per lane) with the
-// least time over three timed launches (after one warm launch each) on the caller's
-// stream and data; the product it leaves in y is the last candidate's, and the
-// caller's own launch follows. Not tuned (the default chunk) while the stream is
-// being captured; HSBEM_GEMV_COLS = 512 / 1024 / 2048 pins the chunk (one row per lane),
-// "COLSx2" / "COLSx4" the chunk and rows per lane (FP32 matrix whose order they divide).
+// Shape of the one-vector product per (n, precision), chosen by the autotuner among
+// the column chunks 512 / 1024 / 2048 and, for an FP32 matrix whose order they
+// divide, one, two or four rows per lane: three timed launches (after one warm launch)
+// of each on the caller's stream and data; the product it leaves in y is the last
+// candidate's, and the caller's own launch follows. The default chunk while the
+// stream is being captured; HSBEM_GEMV_COLS = 512 / 1024 / 2048 pins the chunk (one
+// row per lane), "COLSx2" / "COLSx4" the chunk and rows per lane (FP32 matrix whose
+// order they divide).
 std::uint32_t gemv_columns(const void *A, bool fp32, std::uint32_t n, const double *x, double *y, double *scratch,
                            hipStream_t stream) {
+    std::optional<std::uint32_t> pinned;
     if (const char *pin = std::getenv("HSBEM_GEMV_COLS")) { // "COLS" or "COLSxROWS"
         char *end = nullptr;
         const unsigned long v = std::strtoul(pin, &end, 10);
@@ -1150,57 +1152,17 @@ std::uint32_t gemv_columns(const void *A, bool fp32, std::uint32_t n, const doub
             rows = std::strtoul(end + 1, nullptr, 10);
         const bool rows_ok = rows == 1 || ((rows == 2 || rows == 4) && fp32 && n % rows == 0);
         if ((v == 512 || v == 1024 || v == 2048) && rows_ok)
-            return gemv_shape(static_cast<std::uint32_t>(v), static_cast<std::uint32_t>(rows));
+            pinned = gemv_shape(static_cast<std::uint32_t>(v), static_cast<std::uint32_t>(rows));
     }
-    // Lookups take a shared lock, so slots on other streams never wait on each
-    // other once a size is tuned; only the first use of an (n, precision) pair
-    // tunes, under the exclusive lock (other slots needing that pair wait for it).
-    static std::shared_mutex mutex;
-    static std::map<std::pair<std::uint32_t, bool>, std::uint32_t> chosen;
-    const auto key = std::make_pair(n, fp32);
-    {
-        std::shared_lock lookup(mutex);
-        if (auto it = chosen.find(key); it != chosen.end())
-            return it->second;
-    }
-    std::unique_lock lock(mutex);
-    if (auto it = chosen.find(key); it != chosen.end())
-        return it->second;
-    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(stream, &capture) != hipSuccess || capture != hipStreamCaptureStatusNone)
-        return kGemvCols;
-    hipEvent_t start, stop;
-    if (hipEventCreate(&start) != hipSuccess)
-        return kGemvCols;
-    if (hipEventCreate(&stop) != hipSuccess) {
-        (void)hipEventDestroy(start);
-        return kGemvCols;
-    }
-    std::uint32_t best = kGemvCols;
-    float best_ms = 0;
     std::vector<std::uint32_t> shapes;
     for (const std::uint32_t rows : {1u, 2u, 4u})
         if (rows == 1 || (fp32 && n % rows == 0))
             for (const std::uint32_t cols : {512u, 1024u, 2048u})
                 shapes.push_back(gemv_shape(cols, rows));
-    for (const std::uint32_t shape : shapes) {
-        launch_gemv_cols(shape, A, fp32, n, x, y, scratch, stream);
-        (void)hipEventRecord(start, stream);
-        for (int r = 0; r < 3; ++r)
-            launch_gemv_cols(shape, A, fp32, n, x, y, scratch, stream);
-        (void)hipEventRecord(stop, stream);
-        float ms = 0;
-        if (hipEventSynchronize(stop) != hipSuccess || hipEventElapsedTime(&ms, start, stop) != hipSuccess)
-            continue;
-        if (shape == gemv_shape(512u, 1u) || ms < best_ms) {
-            best = shape;
-            best_ms = ms;
-        }
-    }
-    (void)hipEventDestroy(start);
-    (void)hipEventDestroy(stop);
-    chosen.emplace(key, best);
-    return best;
+    const std::string key = "n=" + std::to_string(n) + (fp32 ? " fp32" : " fp64");
+    return autotune::choose(
+        "gemv_shape", key, shapes, kGemvCols, stream,
+        [&](std::uint32_t shape) { launch_gemv_cols(shape, A, fp32, n, x, y, scratch, stream); }, 3, pinned);
 }
 
 } // namespace

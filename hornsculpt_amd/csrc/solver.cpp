@@ -8,6 +8,7 @@
 //   surface outputs and port impedances ................. :1800-1846
 #include "solver.hpp"
 
+#include "autotune.hpp"
 #include "host/mesh_check.hpp"
 #include "host/qualification.hpp"
 
@@ -109,6 +110,22 @@ MeshSolver::MeshSolver(Context *ctx, SurfaceMesh mesh) : ctx_(ctx) {
     validate_closed_exterior(mesh);
     space_ = build_space(mesh);
     mesh_ = std::move(mesh);
+    // FNV-1a over the recentred coordinates and the triangles: the mesh's key for the
+    // launch-shape autotuner
+    hash_ = 14695981039346656037ull;
+    auto mix = [this](const void *p, std::size_t n) {
+        const auto *b = static_cast<const unsigned char *>(p);
+        for (std::size_t i = 0; i < n; ++i)
+            hash_ = (hash_ ^ b[i]) * 1099511628211ull;
+    };
+    for (const auto &v : mesh_.vertices) {
+        const double c[3] = {v.x, v.y, v.z};
+        mix(c, sizeof(c));
+    }
+    for (const auto &t : mesh_.triangles) {
+        mix(t.vertices.data(), sizeof(std::uint32_t) * 3);
+        mix(&t.face_id, sizeof(t.face_id));
+    }
     build_clusters();
     rules_.upload(rule_table(), ctx_->stream);
     {
@@ -550,6 +567,40 @@ dev::AssemblyLaunch MeshSolver::assembly_launch(const DevicePlan &p, const Numer
     return L;
 }
 
+// Trial block of the regular tiles (4, 8 or 16 trial panels per wavefront block),
+// chosen per (mesh, precision, far or general tiles) by the autotuner on a prefix of
+// that tile list (max(512, count / 32) tiles, all when fewer): one warm and one timed
+// launch per candidate. The launches add into A, B, the error flag and the sample
+// counter, which the caller clears afterwards. HSBEM_TILE_BLOCK = 4 / 8 / 16 pins it.
+void MeshSolver::tune_regular_tiles(dev::AssemblyLaunch &L, bool mixed, hipStream_t s) const {
+    const auto pinned = autotune::pinned_from_env("HSBEM_TILE_BLOCK", {4, 8, 16});
+    std::ostringstream key;
+    key << "mesh " << std::hex << hash_ << std::dec << " (" << nt() << " panels) " << (mixed ? "fp32" : "fp64");
+    auto tune = [&](bool far) {
+        const std::uint32_t count = far ? L.far_tiles : L.tile_count - L.far_tiles;
+        if (!count)
+            return 8u;
+        dev::AssemblyLaunch T = L;
+        const std::uint32_t prefix = std::min(count, std::max(512u, count / 32));
+        if (far) {
+            T.tile_count = T.far_tiles = prefix;
+        } else {
+            T.tiles = L.tiles + 2 * static_cast<std::size_t>(L.far_tiles);
+            T.tile_count = prefix;
+            T.far_tiles = 0;
+        }
+        return autotune::choose(
+            "tile_trial_block", key.str() + (far ? " far" : " general"), {4, 8, 16}, 8, s,
+            [&](std::uint32_t w) {
+                T.trial_block = T.far_trial_block = w;
+                dev::launch_regular(T, s);
+            },
+            1, pinned);
+    };
+    L.far_trial_block = tune(true);
+    L.trial_block = tune(false);
+}
+
 void MeshSolver::assemble_dense(const Numerics &n, double k_max, double k, double *A_out, double *B_out) {
     std::lock_guard lock(mutex_);
     hipStream_t s = ctx_->stream;
@@ -563,10 +614,12 @@ void MeshSolver::assemble_dense(const Numerics &n, double k_max, double k, doubl
     std::iota(cols.begin(), cols.end(), 0);
     DeviceBuffer<std::int32_t> fc;
     fc.upload(cols, s);
+    auto L = assembly_launch(p, n, k, A.data(), BD.data(), nt, fc.data(), error.data());
+    tune_regular_tiles(L, fp32, s); // its launches add into A, B and the error flag: cleared next
     hip_check(hipMemsetAsync(A.data(), 0, A.bytes(), s), "clear A");
     hip_check(hipMemsetAsync(BD.data(), 0, BD.bytes(), s), "clear B");
     hip_check(hipMemsetAsync(error.data(), 0, sizeof(int), s), "clear error");
-    dev::launch_assembly(assembly_launch(p, n, k, A.data(), BD.data(), nt, fc.data(), error.data()), s);
+    dev::launch_assembly(L, s);
     hip_check(hipGetLastError(), "assembly launch");
     check_error_flag(error, s);
     std::vector<double> a(2 * static_cast<std::size_t>(nv) * nv), b(BD.size());
@@ -1489,12 +1542,13 @@ FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_
         fmm_point(P, ws, k, omega, relative, iterations, fr, cancel);
     } else {
     const auto t_asm = Clock::now();
+    auto L = assembly_launch(plan, num, k, ws.A.data(), ws.BD.data(), nd, ws.flux_column.data(), ws.error.data());
+    L.args.samples = ws.samples.data();
+    tune_regular_tiles(L, mixed, s); // its launches add into A, B, the error flag and the samples: cleared next
     hip_check(hipMemsetAsync(ws.A.data(), 0, ws.A.bytes(), s), "clear A");
     hip_check(hipMemsetAsync(ws.BD.data(), 0, ws.BD.bytes(), s), "clear B");
     hip_check(hipMemsetAsync(ws.error.data(), 0, sizeof(int), s), "clear error");
-    auto L = assembly_launch(plan, num, k, ws.A.data(), ws.BD.data(), nd, ws.flux_column.data(), ws.error.data());
     hip_check(hipMemsetAsync(ws.samples.data(), 0, sizeof(unsigned long long), s), "clear samples");
-    L.args.samples = ws.samples.data();
     hipEvent_t *ev = ws.events;
     hip_check(hipEventRecord(ev[0], s), "event");
     dev::launch_regular(L, s);

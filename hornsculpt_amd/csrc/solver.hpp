@@ -363,6 +363,7 @@ class MeshSolver {
     std::shared_ptr<DevicePlan> device_plan(const Numerics &n, double k_max, hipStream_t stream, bool *built);
     dev::AssemblyLaunch assembly_launch(const DevicePlan &p, const Numerics &n, double k, double *A, double *BD,
                                         std::uint32_t nd, const std::int32_t *flux_column, int *error);
+    void tune_regular_tiles(dev::AssemblyLaunch &launch, bool mixed, hipStream_t stream) const;
     void bind(const PreparedRequest &prepared, Workspace &ws);
     void build_clusters();
     // Near structure (tree, near plan, split jobs, CSR pattern; cached per settings)

@@ -122,6 +122,13 @@ typedef void (*hsbem_progress_fn)(void *user, uint32_t completed_frequencies, ui
 HSBEM_API void hsbem_numerics_default(hsbem_numerics *numerics);
 HSBEM_API const char *hsbem_last_error(void);
 HSBEM_API int hsbem_device_description(char *buffer, size_t length);
+/* The launch-shape autotuner's decisions so far in this process, as a JSON array of
+ * {"tunable", "key", "chosen", "timings": [[candidate, ms], ...]} (tunables:
+ * "tile_trial_block", "fmm_split_target", "gemv_shape" = column chunk | rows per lane << 16);
+ * values pinned by environment variables are not listed.
+ * Writes at most length - 1 characters and a terminating zero; *needed (optional)
+ * receives the full length + 1. No counterpart in the reference (it has no autotuner). */
+HSBEM_API int hsbem_tuning_report(char *buffer, size_t length, size_t *needed);
 
 HSBEM_API int hsbem_context_create(int32_t device, hsbem_context **context);
 HSBEM_API int hsbem_context_destroy(hsbem_context *context);
