@@ -31,7 +31,7 @@ HOST_SOURCES = [
     "host/fmm_scaled.cpp", "host/acoustic_analysis.cpp", "host/translation_plan_cache.cpp", "host/frequency_schedule.cpp",
     "autotune.cpp", "solver.cpp", "near_operator.cpp", "fmm_operator.cpp", "server.cpp", "capi.cpp",
 ]
-DEVICE_SOURCES = ["hip/assembly.hip", "hip/linalg.hip", "hip/field.hip", "hip/fmm.hip", "hip/fmm_hf.hip", "hip/noalloc.hip", "hip/gmres_graph.hip"]
+DEVICE_SOURCES = ["hip/assembly.hip", "hip/linalg.hip", "hip/field.hip", "hip/fmm.hip", "hip/fmm_hf.hip", "hip/noalloc.hip", "hip/gmres_graph.hip", "hip/estimator.hip"]
 
 
 def _compile(source: str, device: bool) -> Path:

@@ -17,6 +17,7 @@
 #include "host/mesh_check.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -577,7 +578,10 @@ int hsbem_plan_create(hsbem_mesh *m, const hsbem_numerics *n, double k_max, hsbe
     return guarded([&] {
         require(m && n && out, "invalid plan arguments");
         auto p = std::make_unique<hsbem_plan>();
-        p->plan = m->solver->plan(to_numerics(*n), k_max);
+        // HSBEM_PLAN_ESTIMATOR=device: the device patch estimates (the fast multipole
+        // form's near operator); else host threads (the dense form's plan)
+        const char *estimator = std::getenv("HSBEM_PLAN_ESTIMATOR");
+        p->plan = m->solver->plan(to_numerics(*n), k_max, nullptr, estimator && std::string(estimator) == "device");
         *out = p.release();
     });
 }
@@ -603,7 +607,15 @@ int hsbem_plan_build(const double *vertices, uint32_t nv, const uint32_t *triang
         q.near_tolerance = num.near_tolerance;
         q.near_max_depth = num.near_depth;
         auto p = std::make_unique<hsbem_plan>();
-        p->plan = hsbem::make_interaction_plan(space, q, k_max, num.plan_threads);
+        // HSBEM_PLAN_BATCHED=1: the batched refinement driver on the host estimator
+        // (the same plan; a check of the driver the device estimator feeds)
+        const char *batched = std::getenv("HSBEM_PLAN_BATCHED");
+        if (batched && std::string(batched) == "1") {
+            const auto host = hsbem::host_patch_estimator(space, k_max, num.plan_threads);
+            p->plan = hsbem::make_interaction_plan(space, q, k_max, num.plan_threads, &host);
+        } else {
+            p->plan = hsbem::make_interaction_plan(space, q, k_max, num.plan_threads);
+        }
         *out = p.release();
     });
 }

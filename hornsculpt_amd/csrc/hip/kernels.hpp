@@ -124,6 +124,32 @@ struct AssemblyLaunch {
     std::uint32_t trial_block = 8, far_trial_block = 8;
 };
 
+// ---- adaptive near-pair patch estimates (estimator.hip) ----
+
+struct EstimatorPanel {
+    FanPanel fan;    // vertices (recentred), unit normal, area
+    double max_edge;
+};
+struct EstimatorTask {
+    std::uint32_t test, trial;
+    PairPatch patch;
+};
+struct EstimatorResult {
+    double integral[72]; // the higher rule's 36 moments (re, im), moment m = c * 9 + i * 3 + j
+    double outer_error, inner_error;
+    int status;          // 0 ok, 1 coincident sample points (overlapping panels), 2 non-finite moment
+};
+struct EstimatorLaunch {
+    const EstimatorPanel *panels;
+    const EstimatorTask *tasks;
+    std::uint32_t count;
+    const TrianglePoint *rules; // triangle rules of degree 6 (12), 8 (16), 12 (49), 14 (64) in order
+    double k;
+    EstimatorResult *out;
+};
+// projection: every task of the launch is a projection patch, else none is.
+void launch_estimate_patches(const EstimatorLaunch &launch, bool projection, hipStream_t stream);
+
 void launch_assembly(const AssemblyLaunch &launch, hipStream_t stream);
 void launch_regular(const AssemblyLaunch &launch, hipStream_t stream);
 void launch_touching(const AssemblyLaunch &launch, hipStream_t stream);

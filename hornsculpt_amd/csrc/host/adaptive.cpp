@@ -4,9 +4,12 @@
 #include "adaptive.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <limits>
+#include <mutex>
 #include <stdexcept>
+#include <thread>
 
 namespace hsbem {
 namespace {
@@ -482,6 +485,42 @@ void AdaptivePair::commit(const std::vector<PatchEstimate> &estimates) {
         include(std::move(l));
     }
     advance();
+}
+
+PatchEstimator host_patch_estimator(const Space &space, double k, unsigned threads) {
+    return [&space, k, threads](const std::vector<PatchRequest> &requests, std::vector<PatchEstimate> &estimates) {
+        estimates.resize(requests.size());
+        const unsigned workers = static_cast<unsigned>(std::min<std::size_t>(
+            threads ? threads : std::max(1u, std::thread::hardware_concurrency()), requests.size()));
+        std::atomic<std::size_t> next{0};
+        std::exception_ptr failure;
+        std::mutex failure_mutex;
+        auto run = [&] {
+            for (;;) {
+                const std::size_t i = next.fetch_add(1, std::memory_order_relaxed);
+                if (i >= requests.size())
+                    return;
+                try {
+                    const PatchRequest &r = requests[i];
+                    estimates[i] = estimate_patch(space.panels[r.test], space.panels[r.trial], r.patch, k);
+                } catch (const std::exception &e) {
+                    std::lock_guard lock(failure_mutex);
+                    if (!failure)
+                        failure = std::make_exception_ptr(PatchEstimateError(i, e.what()));
+                    next = requests.size();
+                    return;
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (unsigned w = 1; w < workers; ++w)
+            pool.emplace_back(run);
+        run();
+        for (auto &t : pool)
+            t.join();
+        if (failure)
+            std::rethrow_exception(failure);
+    };
 }
 
 AdaptiveResult adaptive_pair_rule(const Panel &test, const Panel &trial, double tolerance, unsigned max_depth,

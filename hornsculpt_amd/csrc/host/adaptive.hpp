@@ -15,6 +15,10 @@
 
 #include <array>
 #include <complex>
+#include <cstdint>
+#include <functional>
+#include <stdexcept>
+#include <string>
 #include <vector>
 
 namespace hsbem {
@@ -28,6 +32,27 @@ struct PatchEstimate {
 };
 
 PatchEstimate estimate_patch(const Panel &test, const Panel &trial, const PairPatch &patch, double k);
+
+// One patch estimate of a batch: the panels (indices into the Space) and the patch.
+struct PatchRequest {
+    std::uint32_t test{}, trial{};
+    PairPatch patch;
+};
+
+// A batched estimator (AdaptivePairBatch's role, adaptive_patch_estimator.hpp:62-86):
+// estimates[i] receives the estimate of requests[i] at the wavenumber it was built
+// for, with estimate_patch's meaning and refusals (std::runtime_error).
+using PatchEstimator =
+    std::function<void(const std::vector<PatchRequest> &requests, std::vector<PatchEstimate> &estimates)>;
+
+// A refusal of one request of a batch (its index and estimate_patch's message).
+struct PatchEstimateError : std::runtime_error {
+    PatchEstimateError(std::size_t index, const std::string &what) : std::runtime_error(what), request(index) {}
+    std::size_t request;
+};
+
+// The host estimator as a batch: estimate_patch on up to `threads` threads (0: all).
+PatchEstimator host_patch_estimator(const Space &space, double k, unsigned threads = 0);
 
 struct AdaptiveResult {
     std::vector<PairPatch> patches;

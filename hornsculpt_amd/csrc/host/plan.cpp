@@ -5,6 +5,7 @@
 #include <cmath>
 #include <exception>
 #include <limits>
+#include <memory>
 #include <mutex>
 #include <numbers>
 #include <set>
@@ -132,8 +133,120 @@ double planning_wavenumber(const std::vector<double> &solved, const std::vector<
     return k;
 }
 
+namespace {
+
+// The message of a near pair's refusal, with the pair's identity.
+std::string near_failure(const Space &space, Pair pr, double k_max, unsigned depth, const char *what) {
+    const Panel &a = space.panels[pr.first], &b = space.panels[pr.second];
+    std::ostringstream msg;
+    msg << "near panels " << pr.first << '/' << pr.second << " (faces " << a.face_id << '/' << b.face_id
+        << ", k=" << k_max << ", gap=" << panel_distance(a, b) << " m, depth=" << depth << "): " << what;
+    return msg.str();
+}
+
+// Near pairs refined together (AdaptivePairBatch, adaptive_patch_estimator.hpp:62-86):
+// up to kBatchPairs pairs at a time; each round collects the pending patches of every
+// unfinished pair, estimates them in one call and commits each pair's estimates
+// (on up to `workers` threads: the pairs are independent). `done` receives each pair
+// and its rule.
+constexpr std::size_t kBatchPairs = std::size_t(1) << 15;
+
+// fn(i) for i < count on up to `workers` threads; the first exception thrown is rethrown
+// (the remaining items are skipped).
+template <class Fn> void parallel_items(std::size_t count, unsigned workers, Fn &&fn) {
+    std::atomic<std::size_t> next{0};
+    std::exception_ptr failure;
+    std::mutex failure_mutex;
+    auto run = [&] {
+        for (;;) {
+            const std::size_t i = next.fetch_add(1, std::memory_order_relaxed);
+            if (i >= count)
+                return;
+            try {
+                fn(i);
+            } catch (...) {
+                std::lock_guard lock(failure_mutex);
+                if (!failure)
+                    failure = std::current_exception();
+                next = count;
+                return;
+            }
+        }
+    };
+    const unsigned threads = static_cast<unsigned>(std::min<std::size_t>(std::max(workers, 1u), count));
+    std::vector<std::thread> pool;
+    for (unsigned w = 1; w < threads; ++w)
+        pool.emplace_back(run);
+    run();
+    for (auto &t : pool)
+        t.join();
+    if (failure)
+        std::rethrow_exception(failure);
+}
+
+template <class Done>
+void refine_batched(const Space &space, const QuadratureSettings &s, double k_max, std::vector<Pair> pairs,
+                    const PatchEstimator &estimator, unsigned workers, Done &&done) {
+    std::sort(pairs.begin(), pairs.end());
+    std::vector<PatchRequest> requests;
+    std::vector<PatchEstimate> estimates;
+    for (std::size_t first = 0; first < pairs.size(); first += kBatchPairs) {
+        const std::size_t count = std::min(kBatchPairs, pairs.size() - first);
+        std::vector<std::unique_ptr<AdaptivePair>> states(count);
+        parallel_items(count, workers, [&](std::size_t i) {
+            const Pair pr = pairs[first + i];
+            try {
+                states[i] = std::make_unique<AdaptivePair>(space.panels[pr.first], space.panels[pr.second],
+                                                           s.near_tolerance, s.near_max_depth, k_max);
+            } catch (const std::exception &e) {
+                throw std::runtime_error(near_failure(space, pr, k_max, s.near_max_depth, e.what()));
+            }
+        });
+        std::vector<std::size_t> active(count), offsets;
+        for (std::size_t i = 0; i < count; ++i)
+            active[i] = i;
+        while (!active.empty()) {
+            requests.clear();
+            offsets.assign(1, 0);
+            for (const std::size_t i : active) {
+                const Pair pr = pairs[first + i];
+                for (const auto &patch : states[i]->requests())
+                    requests.push_back({pr.first, pr.second, patch});
+                offsets.push_back(requests.size());
+            }
+            estimates.assign(requests.size(), PatchEstimate{});
+            try {
+                estimator(requests, estimates);
+            } catch (const PatchEstimateError &e) {
+                const auto at = std::upper_bound(offsets.begin(), offsets.end(), e.request) - offsets.begin() - 1;
+                throw std::runtime_error(
+                    near_failure(space, pairs[first + active[at]], k_max, s.near_max_depth, e.what()));
+            }
+            parallel_items(active.size(), workers, [&](std::size_t a) {
+                const std::size_t i = active[a];
+                const std::vector<PatchEstimate> mine(estimates.begin() + static_cast<std::ptrdiff_t>(offsets[a]),
+                                                      estimates.begin() + static_cast<std::ptrdiff_t>(offsets[a + 1]));
+                try {
+                    states[i]->commit(mine);
+                } catch (const std::exception &e) {
+                    throw std::runtime_error(near_failure(space, pairs[first + i], k_max, s.near_max_depth, e.what()));
+                }
+            });
+            std::vector<std::size_t> still;
+            for (const std::size_t i : active)
+                if (!states[i]->done())
+                    still.push_back(i);
+            active = std::move(still);
+        }
+        for (std::size_t i = 0; i < count; ++i)
+            done(pairs[first + i], states[i]->take());
+    }
+}
+
+} // namespace
+
 InteractionPlan make_interaction_plan(const Space &space, const QuadratureSettings &s, double k_max,
-                                      unsigned threads) {
+                                      unsigned threads, const PatchEstimator *estimator) {
     if (space.panels.empty() || space.vertices.empty() || !std::isfinite(k_max) || k_max < 0 ||
         !std::isfinite(s.near_distance_ratio) || s.near_distance_ratio < 0 || !std::isfinite(s.near_tolerance) ||
         s.near_tolerance <= 0 || s.near_max_depth > 16 || (s.regular_order != 2 && s.regular_order != 4) ||
@@ -180,6 +293,18 @@ InteractionPlan make_interaction_plan(const Space &space, const QuadratureSettin
     std::atomic<bool> failed{false};
     std::exception_ptr failure;
     std::mutex failure_mutex;
+    std::vector<Pair> batched;
+    auto record = [&](Pair pr, AdaptiveResult rule) {
+        std::lock_guard lock(out_mutex);
+        if (all_pairs && rule.maximum_depth == 0 && rule.patches.size() == 1 && !rule.patches.front().projection) {
+            plan.whole_panel_degrees[unordered_pair_index(pr.first, pr.second)] =
+                static_cast<std::uint8_t>(rule.patches.front().degree);
+            plan.maximum_estimated_relative_error =
+                std::max(plan.maximum_estimated_relative_error, rule.estimated_relative_error);
+            return;
+        }
+        planned.push_back({pr, std::move(rule)});
+    };
     auto process = [&](Pair pr) {
         const Panel &a = space.panels[pr.first], &b = space.panels[pr.second];
         if (classify_checked(a, b).topology != Topology::disjoint)
@@ -196,25 +321,18 @@ InteractionPlan make_interaction_plan(const Space &space, const QuadratureSettin
                                     is_near_pair(a, b, s.near_distance_ratio);
         if (degree && !geometric_near)
             return;
+        if (estimator) { // refined in batches below
+            std::lock_guard lock(out_mutex);
+            batched.push_back(pr);
+            return;
+        }
         AdaptiveResult rule;
         try {
             rule = adaptive_pair_rule(a, b, s.near_tolerance, s.near_max_depth, k_max);
         } catch (const std::exception &e) {
-            std::ostringstream msg;
-            msg << "near panels " << pr.first << '/' << pr.second << " (faces " << a.face_id << '/'
-                << b.face_id << ", k=" << k_max << ", gap=" << panel_distance(a, b)
-                << " m, depth=" << s.near_max_depth << "): " << e.what();
-            throw std::runtime_error(msg.str());
+            throw std::runtime_error(near_failure(space, pr, k_max, s.near_max_depth, e.what()));
         }
-        std::lock_guard lock(out_mutex);
-        if (all_pairs && rule.maximum_depth == 0 && rule.patches.size() == 1 && !rule.patches.front().projection) {
-            plan.whole_panel_degrees[unordered_pair_index(pr.first, pr.second)] =
-                static_cast<std::uint8_t>(rule.patches.front().degree);
-            plan.maximum_estimated_relative_error =
-                std::max(plan.maximum_estimated_relative_error, rule.estimated_relative_error);
-            return;
-        }
-        planned.push_back({pr, std::move(rule)});
+        record(pr, std::move(rule));
     };
     std::vector<std::thread> pool;
     for (unsigned w = 0; w < workers; ++w)
@@ -244,6 +362,8 @@ InteractionPlan make_interaction_plan(const Space &space, const QuadratureSettin
         t.join();
     if (failure)
         std::rethrow_exception(failure);
+    if (estimator)
+        refine_batched(space, s, k_max, std::move(batched), *estimator, workers, record);
     if (all_pairs)
         plan.candidate_pairs = streamed.load();
     std::sort(planned.begin(), planned.end(), [](const auto &x, const auto &y) { return x.pair < y.pair; });

@@ -51,9 +51,13 @@ HS_HD inline std::size_t unordered_pair_index(std::uint32_t a, std::uint32_t b) 
 }
 
 // Throws std::invalid_argument on invalid settings and std::runtime_error when a
-// near pair cannot reach its tolerance. threads=0 uses up to eight host threads.
+// near pair cannot reach its tolerance. threads=0 uses up to sixteen host threads.
+// With an estimator (built for k_max), the near pairs are refined together in
+// batches, each round estimating every pending patch of every unfinished pair in
+// one call; without one, each pair runs to completion on host estimates. Both give
+// the same plan from the same estimates.
 InteractionPlan make_interaction_plan(const Space &space, const QuadratureSettings &settings, double k_max,
-                                      unsigned threads = 0);
+                                      unsigned threads = 0, const PatchEstimator *estimator = nullptr);
 
 // Planning wavenumber: 2 pi max(f)/c over the planning raster when given, else
 // over the solved frequencies (planning_frequencies.hpp:14-38).

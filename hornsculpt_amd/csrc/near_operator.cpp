@@ -39,7 +39,7 @@ std::shared_ptr<const NearStructure> MeshSolver::near_structure(const Numerics &
     ns->tree = make_support_tree(space_, ts);
     ns->plan = make_support_near_plan(space_, ns->tree, faces);
     const std::vector<SupportNearJob> jobs = support_near_jobs(space_, ns->tree, ns->plan);
-    const std::shared_ptr<DevicePlan> dp_ref = device_plan(n, k_max, s, nullptr);
+    const std::shared_ptr<DevicePlan> dp_ref = device_plan(n, k_max, s, nullptr, device_estimates_);
     const InteractionPlan &ip = dp_ref->plan;
     ns->interaction = dp_ref;
     // split the jobs by integration route: contiguous job ranges per thread, each

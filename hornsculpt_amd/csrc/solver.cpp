@@ -112,6 +112,8 @@ MeshSolver::MeshSolver(Context *ctx, SurfaceMesh mesh) : ctx_(ctx) {
     mesh_ = std::move(mesh);
     // FNV-1a over the recentred coordinates and the triangles: the mesh's key for the
     // launch-shape autotuner
+    if (const char *e = std::getenv("HSBEM_PLAN_ESTIMATOR"))
+        device_estimates_ = std::string(e) != "host";
     hash_ = 14695981039346656037ull;
     auto mix = [this](const void *p, std::size_t n) {
         const auto *b = static_cast<const unsigned char *>(p);
@@ -335,19 +337,97 @@ void MeshSolver::build_clusters() {
     cluster_nodes_.upload(cluster_nodes, s);
 }
 
-InteractionPlan MeshSolver::plan(const Numerics &n, double k_max) {
+InteractionPlan MeshSolver::plan(const Numerics &n, double k_max, hipStream_t s, bool device) {
     QuadratureSettings q;
     q.regular_order = n.quadrature_order;
     q.singular_order = n.singular_order;
     q.near_distance_ratio = n.near_ratio;
     q.near_tolerance = n.near_tolerance;
     q.near_max_depth = n.near_depth;
-    return make_interaction_plan(space_, q, k_max, n.plan_threads);
+    if (!device)
+        return make_interaction_plan(space_, q, k_max, n.plan_threads);
+    if (!s)
+        s = ctx_->stream;
+    {
+        std::lock_guard lock(estimator_mutex_);
+        if (!estimator_panels_.size()) {
+            std::vector<dev::EstimatorPanel> panels(space_.panels.size());
+            for (std::size_t t = 0; t < panels.size(); ++t) {
+                const Panel &p = space_.panels[t];
+                panels[t].fan = {{p.vertices[0], p.vertices[1], p.vertices[2]}, p.normal, p.area};
+                panels[t].max_edge = p.max_edge;
+            }
+            std::vector<TrianglePoint> rules;
+            for (const unsigned degree : {6u, 8u, 12u, 14u}) {
+                const auto r = triangle_rule(degree);
+                rules.insert(rules.end(), r.begin(), r.end());
+            }
+            if (rules.size() != 12 + 16 + 49 + 64)
+                throw std::logic_error("patch estimator: unexpected triangle rule sizes");
+            estimator_panels_.upload(panels, s);
+            estimator_rules_.upload(rules, s);
+            hip_check(hipStreamSynchronize(s), "patch estimator upload");
+        }
+    }
+    // Device buffers of one call (tasks by kind, results); requests split into tensor
+    // and projection launches, each result put back at its request's index.
+    struct Buffers {
+        DeviceBuffer<dev::EstimatorTask> tasks;
+        DeviceBuffer<dev::EstimatorResult> out;
+    };
+    auto buffers = std::make_shared<Buffers>();
+    const PatchEstimator estimator = [this, k_max, s, buffers](const std::vector<PatchRequest> &requests,
+                                                               std::vector<PatchEstimate> &estimates) {
+        const std::size_t n = requests.size();
+        if (n > std::numeric_limits<std::uint32_t>::max())
+            throw std::runtime_error("patch estimator: too many requests in one batch");
+        std::vector<dev::EstimatorTask> tasks(n);
+        std::vector<std::size_t> where(n);
+        std::size_t tensor = 0;
+        for (std::size_t i = 0; i < n; ++i)
+            tensor += requests[i].patch.projection ? 0 : 1;
+        for (std::size_t i = 0, a = 0, b = tensor; i < n; ++i) {
+            const std::size_t at = requests[i].patch.projection ? b++ : a++;
+            tasks[at] = {requests[i].test, requests[i].trial, requests[i].patch};
+            where[i] = at;
+        }
+        buffers->tasks.upload(tasks, s);
+        buffers->out.resize(n);
+        dev::EstimatorLaunch L{estimator_panels_.data(), buffers->tasks.data(), static_cast<std::uint32_t>(tensor),
+                               estimator_rules_.data(), k_max, buffers->out.data()};
+        dev::launch_estimate_patches(L, false, s);
+        L.tasks += tensor;
+        L.out += tensor;
+        L.count = static_cast<std::uint32_t>(n - tensor);
+        dev::launch_estimate_patches(L, true, s);
+        hip_check(hipGetLastError(), "patch estimator launch");
+        std::vector<dev::EstimatorResult> out(n);
+        hip_check(hipMemcpyAsync(out.data(), buffers->out.data(), n * sizeof(dev::EstimatorResult),
+                                 hipMemcpyDeviceToHost, s),
+                  "patch estimates download");
+        hip_check(hipStreamSynchronize(s), "patch estimates");
+        estimates.resize(n);
+        for (std::size_t i = 0; i < n; ++i) {
+            const dev::EstimatorResult &r = out[where[i]];
+            if (r.status == 1)
+                throw PatchEstimateError(i, "near pair: the two panels overlap or repeat a vertex position");
+            if (r.status != 0)
+                throw PatchEstimateError(i, "near pair: a quadrature moment is not finite");
+            PatchEstimate &e = estimates[i];
+            for (unsigned m = 0; m < 36; ++m)
+                e.integral[m] = {r.integral[2 * m], r.integral[2 * m + 1]};
+            e.outer_error = r.outer_error;
+            e.inner_error = r.inner_error;
+            e.absolute_error = r.outer_error + r.inner_error;
+        }
+    };
+    return make_interaction_plan(space_, q, k_max, n.plan_threads, &estimator);
 }
 
-std::shared_ptr<DevicePlan> MeshSolver::device_plan(const Numerics &n, double k_max, hipStream_t s, bool *built) {
+std::shared_ptr<DevicePlan> MeshSolver::device_plan(const Numerics &n, double k_max, hipStream_t s, bool *built,
+                                                    bool device) {
     const auto key = std::make_tuple(n.quadrature_order, n.singular_order, n.near_ratio, n.near_tolerance, n.near_depth,
-                                     k_max);
+                                     k_max, device);
     // Held while building: other streams needing the same mesh's plan wait for it.
     std::lock_guard lock(plan_mutex_);
     {
@@ -362,7 +442,7 @@ std::shared_ptr<DevicePlan> MeshSolver::device_plan(const Numerics &n, double k_
     }
     const auto t0 = Clock::now();
     auto dp = std::make_shared<DevicePlan>();
-    dp->plan = plan(n, k_max);
+    dp->plan = plan(n, k_max, s, device);
     dp->touching.upload(dp->plan.touching, s);
     // Near jobs: tensor patches go to the per-patch kernel (degree 8 first, then
     // 14, for uniform trip counts); jobs holding a projection fan stay whole.
@@ -501,7 +581,7 @@ std::uint64_t MeshSolver::resident_bytes() { return geometry_bytes_ + plan_bytes
 std::uint64_t MeshSolver::pending_plan_bytes(const PreparedRequest &P) {
     const Numerics &n = P.request.numerics;
     const auto key = std::make_tuple(n.quadrature_order, n.singular_order, n.near_ratio, n.near_tolerance, n.near_depth,
-                                     P.k_max);
+                                     P.k_max, n.acceleration == 1 && device_estimates_);
     std::lock_guard index(plan_index_mutex_);
     return plans_.count(key) && (n.acceleration == 0 || near_bytes_.load() > 0) ? 0 : P.plan_estimate;
 }
@@ -1495,7 +1575,7 @@ FrequencyResult MeshSolver::solve_frequency(const PreparedRequest &P, std::size_
     cancelled();
     hipStream_t s = ws.stream;
     bool built = false;
-    const std::shared_ptr<DevicePlan> plan_ref = device_plan(num, P.k_max, s, &built);
+    const std::shared_ptr<DevicePlan> plan_ref = device_plan(num, P.k_max, s, &built, num.acceleration == 1 && device_estimates_);
     const DevicePlan &plan = *plan_ref;
     if (plan_built)
         *plan_built = built;

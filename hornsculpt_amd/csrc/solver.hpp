@@ -333,7 +333,12 @@ class MeshSolver {
                                     const volatile std::int32_t *cancel, bool *plan_built = nullptr,
                                     const std::function<void(std::int32_t)> &report = {});
     // Component seams.
-    InteractionPlan plan(const Numerics &numerics, double k_max);
+    // The Galerkin interaction plan; device: its near-pair patch estimates come from the
+    // device (estimate_patches, on `stream` or the mesh's stream), else from host threads.
+    InteractionPlan plan(const Numerics &numerics, double k_max, hipStream_t stream = nullptr, bool device = false);
+    // Whether the fast multipole form estimates its near pairs on the device (default;
+    // HSBEM_PLAN_ESTIMATOR=host at mesh construction takes host threads).
+    bool device_estimates() const { return device_estimates_; }
     NearOperator near_operator(const Numerics &numerics, double k_max, double k, const SupportTreeSettings &tree,
                                const std::vector<std::uint32_t> &active_faces, const double *x);
     void assemble_dense(const Numerics &numerics, double k_max, double k, double *A, double *B);
@@ -360,7 +365,11 @@ class MeshSolver {
     Context *context() const { return ctx_; }
 
   private:
-    std::shared_ptr<DevicePlan> device_plan(const Numerics &n, double k_max, hipStream_t stream, bool *built);
+    // device: near-pair patch estimates on the device (the fast multipole form's near
+    // operator, as the reference builds it), else on host threads (the dense form's plan,
+    // the reference's CPU plan)
+    std::shared_ptr<DevicePlan> device_plan(const Numerics &n, double k_max, hipStream_t stream, bool *built,
+                                            bool device = false);
     dev::AssemblyLaunch assembly_launch(const DevicePlan &p, const Numerics &n, double k, double *A, double *BD,
                                         std::uint32_t nd, const std::int32_t *flux_column, int *error);
     void tune_regular_tiles(dev::AssemblyLaunch &launch, bool mixed, hipStream_t stream) const;
@@ -401,8 +410,12 @@ class MeshSolver {
     DeviceBuffer<double> cluster_far_; // per cluster: the bounds of the far tiles' fixed-point scale
     std::uint32_t cluster_count_{};
     std::vector<ClusterBounds> cluster_bounds_;
-    std::map<std::tuple<unsigned, unsigned, double, double, unsigned, double>, std::shared_ptr<DevicePlan>> plans_;
+    std::map<std::tuple<unsigned, unsigned, double, double, unsigned, double, bool>, std::shared_ptr<DevicePlan>> plans_;
     std::mutex plan_mutex_;       // serialises plan and near-structure builds
+    bool device_estimates_{true}; // the FMM near operator's patch estimates on the device
+    std::mutex estimator_mutex_;
+    DeviceBuffer<dev::EstimatorPanel> estimator_panels_; // built with the first device-estimated plan
+    DeviceBuffer<TrianglePoint> estimator_rules_;
     std::mutex plan_index_mutex_; // guards plans_ lookups, insertion and eviction only
     std::uint64_t plan_clock_{};
     std::atomic<std::uint64_t> plan_bytes_{0}, near_bytes_{0};

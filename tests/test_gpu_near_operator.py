@@ -3,8 +3,12 @@
 Every planned near entry (i, j) of A_near receives the contributions of all
 panel pairs (t ∋ i, s ∋ j), so it must equal the dense A entry; B_near(i, face)
 sums the dense B columns of the face's panels in the near flux leaves of i's
-leaf. Tolerance: 1e-12 relative to max |A| (FP64, different summation order);
-the device SpMV matches a numpy CSR product to 1e-13.
+leaf. With the near pairs' patch estimates on host threads (the dense plan's), the
+tolerance is 1e-12 relative to max |A| (FP64, different summation order); the device
+SpMV matches a numpy CSR product to 1e-13. With the device estimates (the default for
+the fast multipole form), a pair whose leaves tie in error may refine another leaf
+first and end on another valid patch set, so entries agree to the near quadrature's
+accuracy (1e-8 of max |A| here).
 """
 import math
 
@@ -30,8 +34,10 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("estimator,bar", [("host", 1e-12), ("device", 1e-8)])
 @pytest.mark.parametrize("name,make,faces,f,leaf", CASES, ids=[c[0] for c in CASES])
-def test_near_operator_matches_dense_assembly(name, make, faces, f, leaf):
+def test_near_operator_matches_dense_assembly(monkeypatch, name, make, faces, f, leaf, estimator, bar):
+    monkeypatch.setenv("HSBEM_PLAN_ESTIMATOR", estimator)
     mesh = make()
     k = 2 * math.pi * f / 343.0
     num = bem.BemNumerics()
@@ -66,6 +72,6 @@ def test_near_operator_matches_dense_assembly(name, make, faces, f, leaf):
     y = np.zeros(len(mesh.vertices), complex)
     np.add.at(y, rows, va * x[co])
     err_y = np.max(np.abs(op["applied"] - y)) / np.max(np.abs(y))
-    print(f"{name}: nnz {len(co)} of {A.size}, jobs {op['counts']}, kernel {op['kernel_ms']:.2f} ms, "
-          f"A {err_a:.2e} B {err_b:.2e} SpMV {err_y:.2e}")
-    assert err_a <= 1e-12 and err_b <= 1e-12 and err_y <= 1e-13
+    print(f"{name} ({estimator} estimates): nnz {len(co)} of {A.size}, jobs {op['counts']}, kernel "
+          f"{op['kernel_ms']:.2f} ms, A {err_a:.2e} B {err_b:.2e} SpMV {err_y:.2e}")
+    assert err_a <= bar and err_b <= bar and err_y <= 1e-13
