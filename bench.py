@@ -551,6 +551,7 @@ def run_fmm(solver, frequencies, leaf_capacity, repetitions, check=True):
         build_ms = 1e3 * (time.perf_counter() - t0)
         st = op.statistics()
         apply_ms = op.time_apply(repetitions)
+        apply32_ms = op.time_apply(repetitions, fp32=True) if st["plane_wave_levels"] == 0 else None
         err = None
         if check:
             A, _ = solver.assemble(num, k, k)
@@ -559,11 +560,16 @@ def run_fmm(solver, frequencies, leaf_capacity, repetitions, check=True):
             err = float(np.linalg.norm(op.apply(x) - y) / np.linalg.norm(y))
             del A
         out.append({"frequency_hz": f, "build_ms": round(build_ms, 1), "apply_ms": round(apply_ms, 3),
+                    "apply_fp32_ms": round(apply32_ms, 3) if apply32_ms else None,
                     "apply_rel_err_vs_dense": err, "level_orders": st["level_orders"], "leaves": st["leaves"],
                     "m2l_links": st["m2l_links"], "near_nnz": st["near_nnz"],
-                    "device_mb": round(st["device_bytes"] / 1e6, 1)})
-        print(f"# FMM {f:.0f} Hz: build {build_ms:.0f} ms, apply {apply_ms:.3f} ms, vs dense {err}, orders "
-              f"{st['level_orders']}, {st['m2l_links']} M2L links", file=sys.stderr, flush=True)
+                    "device_mb": round(st["device_bytes"] / 1e6, 1),
+                    "build_phases_ms": {key: round(st[f"{key}_ms"], 1) for key in ("tree", "near", "plans", "moments")}})
+        print(f"# FMM {f:.0f} Hz: build {build_ms:.0f} ms (tree + expansion plan {st['tree_ms']:.0f}, near "
+              f"{st['near_ms']:.0f}, translation plans {st['plans_ms']:.0f}, moments {st['moments_ms']:.0f}), apply "
+              f"{apply_ms:.3f} ms (FP32-bulk {apply32_ms}), vs dense {err}, orders {st['level_orders']}, "
+              f"{st['m2l_links']} M2L links",
+              file=sys.stderr, flush=True)
         op.close()
     return out
 
@@ -572,7 +578,10 @@ def run_flange(edge, frequencies, leaf_capacity, tolerance, repeats, block_width
     """Horn on a 0.6 m square flange (5 mm slab in the mouth plane) through the fast
     multipole form of the request solve (FP64, GMRES restart 50, relative residual
     tolerance, accuracy seed tolerance/10, seed/10 audit): the first request builds
-    the cached near structure (cold), the repeats reuse it (warm)."""
+    the cached near structure (cold), the repeats reuse it (warm). Then the same
+    request in the FP32-bulk form (precision mixed: FP32 translations in the Krylov
+    iterations where the operator allows them, FP64 residuals, audits and corrections),
+    warm, with its surface-pressure difference from the plain form."""
     mesh = meshes.exponential_horn(edge, flange=0.6)
     nt, nv = len(mesh.triangles), len(mesh.vertices)
     print(f"# flanged horn: {nt} triangles, {nv} P1 nodes, frequencies {frequencies}", file=sys.stderr, flush=True)
@@ -586,12 +595,17 @@ def run_flange(edge, frequencies, leaf_capacity, tolerance, repeats, block_width
                                  fmm_separation_ratio=1.5, fmm_restart_length=50, fmm_maximum_iterations=500,
                                  fmm_block_width=block_width))
     runs = []
-    for r in range(1 + repeats):
+    traces = {}
+    for r in range(2 + repeats):
+        bulk = r == 1 + repeats
+        if bulk:
+            req.numerics.precision = bem.PRECISION_MIXED
         t0 = time.perf_counter()
         res = solver.solve(req, progress=lambda done, total, f, phase: print(
             f"# flange point {done}/{total} {f:.0f} Hz phase {phase} at {time.perf_counter() - t0:.1f} s",
             file=sys.stderr, flush=True))
         wall = time.perf_counter() - t0
+        traces["fp32_bulk" if bulk else "plain"] = [fr.excitations[0].boundary.pressure_trace for fr in res.frequencies]
         points = []
         for fr in res.frequencies:
             ex = fr.excitations[0]
@@ -599,18 +613,27 @@ def run_flange(edge, frequencies, leaf_capacity, tolerance, repeats, block_width
                            "solve_ms": round(fr.solve_milliseconds, 1), "iterations": ex.iterations,
                            "residual": ex.boundary_residual, **fr.fmm,
                            "on_axis_spl_db": round(bem.spl_db(complex(ex.observer_pressure[0])), 3)})
-            print(f"# flange {'cold' if r == 0 else 'warm'} {fr.frequency:.0f} Hz: build {fr.assembly_milliseconds:.0f} ms, "
+            print(f"# flange {'FP32-bulk' if bulk else 'cold' if r == 0 else 'warm'} {fr.frequency:.0f} Hz: build "
+                  f"{fr.assembly_milliseconds:.0f} ms, "
                   f"solve {fr.solve_milliseconds:.0f} ms, {ex.iterations} iterations, residual "
                   f"{ex.boundary_residual:.1e}, audit {fr.fmm['audit_discrepancy']:.1e}, apply "
                   f"{fr.fmm['apply_milliseconds']:.1f} ms, plane-wave levels {fr.fmm['plane_wave_levels']}, endpoint "
                   f"degree {fr.fmm['endpoint_degree']} (comparison {fr.fmm['quadrature_discrepancy']:.1e}, "
                   f"{fr.fmm['promotions']} promotions), direct audit {fr.fmm['direct_audit']}",
                   file=sys.stderr, flush=True)
-        runs.append({"wall_s": round(wall, 3), "points": points})
+        runs.append({"wall_s": round(wall, 3), "points": points, "form": "fp32_bulk" if bulk else "plain"})
+    bulk_run = runs.pop()
     warm = runs[1:] or runs
     wall = min(run["wall_s"] for run in warm)
+    diff = [float(np.linalg.norm(b - p) / np.linalg.norm(p)) for b, p in zip(traces["fp32_bulk"], traces["plain"])]
+    solve_plain = sum(p["solve_ms"] for p in runs[-1]["points"])
+    solve_bulk = sum(p["solve_ms"] for p in bulk_run["points"])
+    print(f"# flange FP32-bulk form (run warm): solves {solve_bulk / 1e3:.2f} s (plain {solve_plain / 1e3:.2f} s), "
+          f"wall {bulk_run['wall_s']:.1f} s, surface pressure vs plain {max(diff):.1e}", file=sys.stderr, flush=True)
     return {"triangles": nt, "nodes": nv, "block_width": block_width, "cold_wall_s": runs[0]["wall_s"], "warm_wall_s": wall,
-            "points_per_s_warm": len(frequencies) / wall, "runs": runs}
+            "points_per_s_warm": len(frequencies) / wall, "runs": runs,
+            "fp32_bulk": {"wall_s_warm": bulk_run["wall_s"], "solve_s": solve_bulk / 1e3, "plain_solve_s": solve_plain / 1e3,
+                          "rel_l2_surface_pressure_vs_plain": diff, "points": bulk_run["points"]}}
 
 
 def main_stream(args):

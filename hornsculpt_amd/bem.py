@@ -166,9 +166,11 @@ def library() -> ctypes.CDLL:
                 "hsbem_fmm_operator_apply": (ctypes.c_int, [P, D, D]),
                 "hsbem_fmm_operator_precondition": (ctypes.c_int, [P, D, D]),
                 "hsbem_fmm_operator_apply_direct": (ctypes.c_int, [P, D, D]),
+                "hsbem_fmm_operator_apply_fp32": (ctypes.c_int, [P, D, D]),
                 "hsbem_fmm_operator_solve": (ctypes.c_int, [P, D, D, U32, U32, ctypes.c_double, ctypes.c_int32, D,
                                                             ctypes.POINTER(U32)]),
                 "hsbem_fmm_operator_time_apply": (ctypes.c_int, [P, U32, D]),
+                "hsbem_fmm_operator_time_apply_fp32": (ctypes.c_int, [P, U32, D]),
                 "hsbem_fmm_operator_statistics": (ctypes.c_int, [P, ctypes.POINTER(U64), D, ctypes.POINTER(U32),
                                                                  ctypes.POINTER(U32)]),
                 "hsbem_fmm_operator_levels": (ctypes.c_int, [P, ctypes.POINTER(U64), ctypes.POINTER(U32),
@@ -630,13 +632,14 @@ def _collect(lib, handle, request: BemRequest) -> BemResult:
                                            refs.value, BemBoundaryData(surface, trace, deriv)))
         served = ctypes.c_double()
         _check(lib.hsbem_result_served_latency(handle, f, ctypes.byref(served)))
-        fm = np.zeros(12)
+        fm = np.zeros(13)
         _check(lib.hsbem_result_fmm(handle, f, _dptr(fm)))
         fmm = None if request.numerics.acceleration != ACCELERATION_FAST_MULTIPOLE else {
             "audit_discrepancy": float(fm[0]), "apply_milliseconds": float(fm[1]), "corrected": bool(fm[2]),
             "plane_wave_levels": int(fm[3]), "m2l_links": int(fm[4]), "near_nnz": int(fm[5]),
             "device_bytes": int(fm[6]), "direct_audit": bool(fm[7]), "quadrature_discrepancy": float(fm[8]),
-            "endpoint_degree": int(fm[9]), "promotions": int(fm[10]), "verification_seed": float(fm[11])}
+            "endpoint_degree": int(fm[9]), "promotions": int(fm[10]), "verification_seed": float(fm[11]),
+            "fp32_translations": bool(fm[12])}
         freqs.append(BemFrequencyResult(request.frequencies[f], exs, ms[0], ms[1], ms[2], ms[3], ms[4], ms[5],
                                         {"regular": ms[6], "touching": ms[7], "near": ms[8],
                                          "regular_samples": ms[9], "touching_samples": ms[10],
@@ -786,6 +789,13 @@ class HipFmmOperator:
         _check(self._lib.hsbem_fmm_operator_apply_direct(self.handle, _dptr(xv), _dptr(y)))
         return y.view(np.complex128)
 
+    def apply_fp32(self, x) -> np.ndarray:
+        """A x in the FP32-bulk form (FP32 expansions and translations; spherical
+        levels of order <= 48 only, else ValueError)."""
+        xv, y = self._vector(x), np.zeros(2 * self.size)
+        _check(self._lib.hsbem_fmm_operator_apply_fp32(self.handle, _dptr(xv), _dptr(y)))
+        return y.view(np.complex128)
+
     def precondition(self, v) -> np.ndarray:
         vv, z = self._vector(v), np.zeros(2 * self.size)
         _check(self._lib.hsbem_fmm_operator_precondition(self.handle, _dptr(vv), _dptr(z)))
@@ -805,9 +815,11 @@ class HipFmmOperator:
         """Replay the whole apply from one captured HIP graph (captured on first use)."""
         _check(self._lib.hsbem_fmm_operator_set_graph(self.handle, int(bool(enable))))
 
-    def time_apply(self, repetitions: int = 10) -> float:
+    def time_apply(self, repetitions: int = 10, fp32: bool = False) -> float:
+        """HIP-event milliseconds per apply (fp32: the FP32-bulk apply)."""
         ms = ctypes.c_double()
-        _check(self._lib.hsbem_fmm_operator_time_apply(self.handle, repetitions, ctypes.byref(ms)))
+        fn = self._lib.hsbem_fmm_operator_time_apply_fp32 if fp32 else self._lib.hsbem_fmm_operator_time_apply
+        _check(fn(self.handle, repetitions, ctypes.byref(ms)))
         return ms.value
 
     def statistics(self) -> dict:

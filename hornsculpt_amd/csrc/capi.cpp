@@ -564,6 +564,7 @@ int hsbem_result_fmm(const hsbem_result *r, uint32_t f, double *out) {
         out[9] = fr.fmm_endpoint_degree;
         out[10] = fr.fmm_promotions;
         out[11] = fr.fmm_verification_seed;
+        out[12] = fr.fmm_fp32 ? 1 : 0;
     });
 }
 
@@ -812,6 +813,13 @@ int hsbem_fmm_operator_apply(hsbem_fmm_operator *o, const double *x, double *y) 
     });
 }
 
+int hsbem_fmm_operator_apply_fp32(hsbem_fmm_operator *o, const double *x, double *y) {
+    return guarded([&] {
+        require(o && o->op->fp32_available(), "the FP32-bulk apply needs spherical levels of order at most 48");
+        fmm_host_call(o, x, y, [&](const double *dx, double *dy, hipStream_t s) { o->op->apply_fp32(dx, dy, s); });
+    });
+}
+
 int hsbem_fmm_operator_apply_direct(hsbem_fmm_operator *o, const double *x, double *y) {
     return guarded([&] {
         fmm_host_call(o, x, y, [&](const double *dx, double *dy, hipStream_t s) { o->op->apply_direct(dx, dy, s); });
@@ -842,6 +850,32 @@ int hsbem_fmm_operator_solve(hsbem_fmm_operator *o, const double *b, double *x, 
             hsbem::gmres(s, kop, op->size(), db, dx, restart, maximum_iterations, tolerance, achieved, iterations,
                          nullptr, o->gmres);
         });
+    });
+}
+
+int hsbem_fmm_operator_time_apply_fp32(hsbem_fmm_operator *o, uint32_t repetitions, double *milliseconds) {
+    return guarded([&] {
+        require(o && milliseconds && repetitions, "invalid FMM timing arguments");
+        require(o->op->fp32_available(), "the FP32-bulk apply needs spherical levels of order at most 48");
+        hsbem::hip_check(hipSetDevice(o->ctx->device), "set device");
+        const std::size_t n = 2 * static_cast<std::size_t>(o->op->size());
+        hipStream_t s = o->ctx->stream;
+        o->x.upload(std::vector<double>(n, 1.0), s);
+        o->y.resize(n);
+        o->op->apply_fp32(o->x.data(), o->y.data(), s); // warm (and the FP32 buffers)
+        hipEvent_t e0, e1;
+        hsbem::hip_check(hipEventCreate(&e0), "event");
+        hsbem::hip_check(hipEventCreate(&e1), "event");
+        hsbem::hip_check(hipEventRecord(e0, s), "event");
+        for (uint32_t r = 0; r < repetitions; ++r)
+            o->op->apply_fp32(o->x.data(), o->y.data(), s);
+        hsbem::hip_check(hipEventRecord(e1, s), "event");
+        hsbem::hip_check(hipEventSynchronize(e1), "event sync");
+        float ms = 0;
+        hsbem::hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *milliseconds = ms / repetitions;
     });
 }
 

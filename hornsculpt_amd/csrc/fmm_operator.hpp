@@ -78,6 +78,13 @@ class FmmOperator {
   public:
     // y = A x (device pointers, nv complex each, y may not alias x).
     void apply(const double *x, double *y, hipStream_t stream) const;
+    // The FP32-bulk form of the apply: expansions and translation workspaces in FP32
+    // (the rotations, coaxial products and gathers run in FP32, the Wigner and coaxial
+    // tables rounded when staged); the near part, the emission and testing sums and the
+    // vectors stay FP64. Available for operators without plane-wave levels and with
+    // orders up to 48 (fp32_available); its FP32 buffers are allocated on first use.
+    bool fp32_available() const;
+    void apply_fp32(const double *x, double *y, hipStream_t stream) const;
     // y = A x through a captured graph of the whole apply (M2L graph replay): the
     // graph runs on operator-owned input/output buffers, captured on first use on
     // the given (non-default) stream; x and y are copied in and out on the device.
@@ -157,6 +164,8 @@ class FmmOperator {
     double direct_work_{}, multipole_work_{};
     double k_{};
     std::uint32_t leaf_count_{};
+    mutable DeviceBuffer<float> pool32_, work_y32_, work_z32_; // the FP32-bulk form's buffers
+    void apply_in(const double *x, double *y, hipStream_t stream, bool fp32) const;
     mutable DeviceBuffer<double> graph_x_, graph_y_;
     mutable hipGraph_t graph_{};
     mutable hipGraphExec_t graph_exec_{};

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <stdexcept>
 
 namespace hsbem::dev {
 namespace {
@@ -38,6 +39,24 @@ __device__ __forceinline__ C2 operator-(C2 a, C2 b) { return {a.re - b.re, a.im 
 __device__ __forceinline__ C2 operator*(C2 a, C2 b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
 __device__ __forceinline__ C2 operator*(double s, C2 a) { return {s * a.re, s * a.im}; }
 __device__ __forceinline__ C2 conj(C2 a) { return {a.re, -a.im}; }
+
+// FP32 complex (the FP32-bulk form's expansions and translation workspaces)
+struct C2f {
+    float re, im;
+};
+__device__ __forceinline__ C2f operator+(C2f a, C2f b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ C2f operator*(C2f a, C2f b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ C2f operator*(float s, C2f a) { return {s * a.re, s * a.im}; }
+__device__ __forceinline__ C2f conj(C2f a) { return {a.re, -a.im}; }
+// complex type of an expansion scalar E (double: C2, float: C2f)
+template <class E> struct CplxOf {
+    using type = C2;
+};
+template <> struct CplxOf<float> {
+    using type = C2f;
+};
+template <class E> using Cx = typename CplxOf<E>::type;
+template <class E> __device__ __forceinline__ Cx<E> to_cx(C2 v) { return {static_cast<E>(v.re), static_cast<E>(v.im)}; }
 
 // value and gradient of one solid harmonic
 struct Wave {
@@ -213,9 +232,11 @@ template <int kGroup, int kTop> __global__ __launch_bounds__(64) void leaf_momen
     }
 }
 
-// M_leaf = E_leaf p: one block per leaf, thread per coefficient.
+// M_leaf = E_leaf p: one block per leaf, thread per coefficient; the expansions are
+// stored as S (the sums run in FP64).
+template <class S>
 __global__ __launch_bounds__(256) void leaf_emit(const FmmLeaf *leaves, const std::uint32_t *leaf_dofs,
-                                                 const double *emission, const double *p, double *expansions) {
+                                                 const double *emission, const double *p, S *expansions) {
     __shared__ double sp[2 * kMaxLeafDofs];
     const FmmLeaf leaf = leaves[blockIdx.x];
     for (std::uint32_t j = threadIdx.x; j < leaf.dof_count; j += blockDim.x) {
@@ -233,25 +254,27 @@ __global__ __launch_bounds__(256) void leaf_emit(const FmmLeaf *leaves, const st
             re += e.x * sp[2 * j] - e.y * sp[2 * j + 1];
             im += e.x * sp[2 * j + 1] + e.y * sp[2 * j];
         }
-        expansions[2 * (leaf.multipole + c)] = re;
-        expansions[2 * (leaf.multipole + c) + 1] = im;
+        expansions[2 * (leaf.multipole + c)] = static_cast<S>(re);
+        expansions[2 * (leaf.multipole + c) + 1] = static_cast<S>(im);
     }
 }
 
-// y_i += T_leaf[i] . L_leaf: one wavefront per DOF.
+// y_i += T_leaf[i] . L_leaf: one wavefront per DOF; the expansions are read as S.
+template <class S>
 __global__ __launch_bounds__(256) void leaf_test(const FmmLeaf *leaves, const std::uint32_t *leaf_dofs,
-                                                 const double *testing, const double *expansions, double *y) {
+                                                 const double *testing, const S *expansions, double *y) {
     const FmmLeaf leaf = leaves[blockIdx.x];
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const std::size_t nc = static_cast<std::size_t>(leaf.order + 1) * (leaf.order + 1);
     const double2 *T = reinterpret_cast<const double2 *>(testing) + leaf.emission;
-    const double2 *Lc = reinterpret_cast<const double2 *>(expansions) + leaf.local;
+    const Cx<S> *Lc = reinterpret_cast<const Cx<S> *>(expansions) + leaf.local;
     for (std::uint32_t j = wave; j < leaf.dof_count; j += blockDim.x / 64) {
         double re = 0, im = 0;
         for (std::size_t c = lane; c < nc; c += 64) {
-            const double2 t = T[j * nc + c], l = Lc[c];
-            re += t.x * l.x - t.y * l.y;
-            im += t.x * l.y + t.y * l.x;
+            const double2 t = T[j * nc + c];
+            const double lr = Lc[c].re, li = Lc[c].im;
+            re += t.x * lr - t.y * li;
+            im += t.x * li + t.y * lr;
         }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
@@ -315,15 +338,16 @@ __device__ __forceinline__ std::size_t degree_block(int l) {
 // spreads its stages over more workgroups instead of running them one after another.
 // Instantiated for orders up to kP = 32, 40 and 48 (LDS and registers sized to kP, so
 // the common orders 24-32 keep three workgroups per CU).
-template <int kP>
+template <int kP, class E>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kP <= 32 ? 3 : 1))) void rotate_links(FmmTranslateLaunch L, bool back) {
+    using CE = Cx<E>;
     constexpr int kStage = (2 * kP + 1) * (2 * kP + 1) + 1; // >= n_a^2 + n_b^2 for n_a + n_b <= 2 kP + 2
     constexpr int kStagePerThread = (kStage + 255) / 256;
     constexpr int kSliceWidth = 2 * kP + 2;                 // >= n_a + n_b
     constexpr int kSlicePerThread = (kFmmChunk * kSliceWidth + 255) / 256;
-    __shared__ double dl[kStage];
-    __shared__ C2 xs[kFmmChunk][kSliceWidth]; // rows >= count: stale, their results are dropped
-    __shared__ C2 ph[2 * kP + 1];
+    __shared__ E dl[kStage];
+    __shared__ CE xs[kFmmChunk][kSliceWidth]; // rows >= count: stale, their results are dropped
+    __shared__ CE ph[2 * kP + 1];
     __shared__ std::uint64_t src_off[kFmmChunk], dst_off[kFmmChunk];
     __shared__ int ord[kFmmChunk];
     const FmmChunk ch = L.chunks[blockIdx.x];
@@ -333,7 +357,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kP <= 32 ? 
     const FmmRotation rot = L.rotations[ch.rotation];
     const C2 *az = reinterpret_cast<const C2 *>(L.azimuth) + rot.azimuth;
     for (int m = threadIdx.x; m <= 2 * static_cast<int>(rot.order); m += blockDim.x)
-        ph[m] = az[m];
+        ph[m] = to_cx<E>(az[m]);
     if (threadIdx.x < ch.count) {
         const FmmLink lk = L.links[L.order[ch.first + threadIdx.x]];
         src_off[threadIdx.x] = back ? lk.slot : lk.source;
@@ -341,8 +365,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kP <= 32 ? 
         ord[threadIdx.x] = static_cast<int>(back ? lk.target_order : lk.source_order);
     }
     __syncthreads();
-    const C2 *in = reinterpret_cast<const C2 *>(back ? L.work_z : L.source_expansions);
-    C2 *out = reinterpret_cast<C2 *>(L.work_y);
+    const CE *in = reinterpret_cast<const CE *>(back ? L.work_z : L.source_expansions);
+    CE *out = reinterpret_cast<CE *>(L.work_y);
     const int count = static_cast<int>(ch.count), P = static_cast<int>(rot.order), top = static_cast<int>(ch.max_order);
     const int stages = top / 2 + 1; // stage h: degrees h and top - h (one degree when they coincide)
     // degrees of stage h and their sizes; nb = 0 when the stage holds one degree
@@ -352,8 +376,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kP <= 32 ? 
         na = 2 * la + 1;
         nb = lb > la ? 2 * lb + 1 : 0;
     };
-    double pd[kStagePerThread];
-    C2 px[kSlicePerThread];
+    E pd[kStagePerThread];
+    CE px[kSlicePerThread];
     auto fetch = [&](int h) {
         int la, lb, na, nb;
         degrees(h, la, lb, na, nb);
@@ -362,13 +386,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kP <= 32 ? 
 #pragma unroll
         for (int i = 0; i < kStagePerThread; ++i) {
             const int e = threadIdx.x + 256 * i;
-            pd[i] = e < na * na ? srca[e] : e < na * na + nb * nb ? srcb[e - na * na] : 0.0;
+            pd[i] = static_cast<E>(e < na * na ? srca[e] : e < na * na + nb * nb ? srcb[e - na * na] : 0.0);
         }
         const int w = na + nb;
 #pragma unroll
         for (int i = 0; i < kSlicePerThread; ++i) {
             const int e = threadIdx.x + 256 * i;
-            C2 v{0, 0};
+            CE v{0, 0};
             if (e < count * w) {
                 const int j = e / w, q = e % w;
                 const int l = q < na ? la : lb, qq = q < na ? q : q - na;
@@ -409,18 +433,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kP <= 32 ? 
             const int g = e / w, tq = e % w, j0 = 4 * g;
             const bool first = tq < na;
             const int l = first ? la : lb, n = first ? na : nb, t = first ? tq : tq - na, x0 = first ? 0 : na;
-            const double *dbl = dl + (first ? 0 : na * na);
-            C2 s0{0, 0}, s1{0, 0}, s2{0, 0}, s3{0, 0};
+            const E *dbl = dl + (first ? 0 : na * na);
+            CE s0{0, 0}, s1{0, 0}, s2{0, 0}, s3{0, 0};
             const int stride = back ? 1 : n;
-            const double *drow = back ? dbl + t * n : dbl + t;
+            const E *drow = back ? dbl + t * n : dbl + t;
             // three steps per pass: their 15 LDS reads are issued before the products
             // (three, not four, keeps the order-32 kernel within the 168 registers of
             // three waves per SIMD: measured 4.18 ms per horn apply against 4.57 ms)
             constexpr int kSteps = 3;
             int q = 0;
             for (; q + kSteps <= n; q += kSteps) {
-                double d[kSteps];
-                C2 x[kSteps][4];
+                E d[kSteps];
+                CE x[kSteps][4];
 #pragma unroll
                 for (int u = 0; u < kSteps; ++u) {
                     d[u] = drow[(q + u) * stride];
@@ -437,14 +461,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kP <= 32 ? 
                 }
             }
             for (; q < n; ++q) {
-                const double d = drow[q * stride];
+                const E d = drow[q * stride];
                 s0 = s0 + d * xs[j0][x0 + q];
                 s1 = s1 + d * xs[j0 + 1][x0 + q];
                 s2 = s2 + d * xs[j0 + 2][x0 + q];
                 s3 = s3 + d * xs[j0 + 3][x0 + q];
             }
-            const C2 sv[4] = {s0, s1, s2, s3};
-            const C2 rot_back = back ? conj(ph[P + t - l]) : C2{1, 0};
+            const CE sv[4] = {s0, s1, s2, s3};
+            const CE rot_back = back ? conj(ph[P + t - l]) : CE{1, 0};
             const std::size_t base = static_cast<std::size_t>(l) * l;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -463,8 +487,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kP <= 32 ? 
 // the stages y, y + Y, ... (Y = gridDim.y) of its chunk's stage sequence.
 // T_m rows are stored with an odd stride (cols | 1) of 16-byte values, so the
 // consecutive rows read by consecutive lanes fall in distinct bank groups.
-template <int kP>
+template <int kP, class E>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kP <= 32 ? 4 : 1))) void coaxial_links(FmmTranslateLaunch L) {
+    using CE = Cx<E>;
     constexpr int kAxialRows = kP + 1;
     // A stage pairs |m| = h with mm - h while their columns fit kColumns (always when
     // ps <= pt: cols_a + cols_b = ps + 2), else holds one block. Its blocks hold at most
@@ -475,16 +500,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kP <= 32 ? 
     constexpr int kBlockPerThread = (kBlockElems + 255) / 256;
     constexpr int kColumns = kAxialRows + 1;
     constexpr int kColumnPerThread = (kFmmChunk * 2 * kColumns + 255) / 256;
-    __shared__ C2 tm[kBlock];
-    __shared__ C2 ys[kFmmChunk][2][kColumns];
+    __shared__ CE tm[kBlock];
+    __shared__ CE ys[kFmmChunk][2][kColumns];
     __shared__ std::uint64_t slot[kFmmChunk];
     const FmmChunk ch = L.axial_chunks[blockIdx.x];
     const int split = static_cast<int>(blockIdx.y), splits = static_cast<int>(gridDim.y);
     const FmmLink first = L.links[L.axial_order[ch.first]];
     const int ps = static_cast<int>(first.source_order), pt = static_cast<int>(first.target_order), mm = min(ps, pt);
     const C2 *T = reinterpret_cast<const C2 *>(L.axial) + L.axial_offsets[first.axial];
-    const C2 *y = reinterpret_cast<const C2 *>(L.work_y);
-    C2 *z = reinterpret_cast<C2 *>(L.work_z);
+    const CE *y = reinterpret_cast<const CE *>(L.work_y);
+    CE *z = reinterpret_cast<CE *>(L.work_z);
     const int count = static_cast<int>(ch.count);
     if (threadIdx.x < count)
         slot[threadIdx.x] = L.links[L.axial_order[ch.first + threadIdx.x]].slot;
@@ -517,14 +542,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kP <= 32 ? 
         q.hi = two ? hi - 1 : hi;
         return q;
     };
-    C2 pt_[kBlockPerThread], py[kColumnPerThread];
+    CE pt_[kBlockPerThread], py[kColumnPerThread];
     auto fetch = [&](const Pair &q) {
         const C2 *Ta = T + axial_block(q.ma, ps, pt), *Tb = T + axial_block(q.mb, ps, pt);
         const int sa = q.ra * q.ca, sb = q.rb * q.cb;
 #pragma unroll
         for (int i = 0; i < kBlockPerThread; ++i) {
             const int e = threadIdx.x + 256 * i;
-            pt_[i] = e < sa ? Ta[e] : e < sa + sb ? Tb[e - sa] : C2{0, 0};
+            pt_[i] = to_cx<E>(e < sa ? Ta[e] : e < sa + sb ? Tb[e - sa] : C2{0, 0});
         }
         const int w = q.ca + q.cb;
 #pragma unroll
@@ -534,7 +559,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kP <= 32 ? 
             const int ee = live ? e : 0;
             const int j = ee / (2 * w), r = ee % (2 * w), sgn = r / w, c = r % w;
             const int m = c < q.ca ? q.ma : q.mb, n = m + (c < q.ca ? c : c - q.ca);
-            py[i] = live ? y[slot[j] + m_major(ps, n, sgn ? -m : m)] : C2{0, 0};
+            py[i] = live ? y[slot[j] + m_major(ps, n, sgn ? -m : m)] : CE{0, 0};
         }
     };
     // the stage `steps` stages after p; false when the sequence ends first
@@ -586,15 +611,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kP <= 32 ? 
             if (sgn && !m)
                 continue;
             const int cols = half ? q.cb : q.ca, c0 = half ? q.ca : 0;
-            const C2 *trow = tm + (half ? base_b + (l - m) * stride_b : (l - m) * stride_a);
-            C2 s0{0, 0}, s1{0, 0}, s2{0, 0}, s3{0, 0};
+            const CE *trow = tm + (half ? base_b + (l - m) * stride_b : (l - m) * stride_a);
+            CE s0{0, 0}, s1{0, 0}, s2{0, 0}, s3{0, 0};
             // two columns per pass: their 10 LDS reads are issued before the products (two,
             // not four, keeps the order-32 kernel within the 128 registers of four waves
             // per SIMD, which its LDS allows; both measured alike on the horn apply)
             constexpr int kSteps = 2;
             int c = 0;
             for (; c + kSteps <= cols; c += kSteps) {
-                C2 tv[kSteps], yv[kSteps][4];
+                CE tv[kSteps], yv[kSteps][4];
 #pragma unroll
                 for (int u = 0; u < kSteps; ++u) {
                     tv[u] = trow[c + u];
@@ -611,13 +636,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kP <= 32 ? 
                 }
             }
             for (; c < cols; ++c) {
-                const C2 tv = trow[c];
+                const CE tv = trow[c];
                 s0 = s0 + tv * ys[j0][sgn][c0 + c];
                 s1 = s1 + tv * ys[j0 + 1][sgn][c0 + c];
                 s2 = s2 + tv * ys[j0 + 2][sgn][c0 + c];
                 s3 = s3 + tv * ys[j0 + 3][sgn][c0 + c];
             }
-            const C2 sv[4] = {s0, s1, s2, s3};
+            const CE sv[4] = {s0, s1, s2, s3};
             const std::size_t idx = static_cast<std::size_t>(l) * l + l + (sgn ? -m : m);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
@@ -961,20 +986,21 @@ __global__ __launch_bounds__(256) void coaxial_links_big(FmmTranslateLaunch L) {
 }
 
 // target = sum of the destination's link results.
-__global__ __launch_bounds__(256) void gather_links(FmmTranslateLaunch L) {
+template <class E> __global__ __launch_bounds__(256) void gather_links(FmmTranslateLaunch L) {
+    using CE = Cx<E>;
     const FmmDestination dst = L.destinations[blockIdx.x];
     const int nct = static_cast<int>((dst.order + 1) * (dst.order + 1));
-    const C2 *w = reinterpret_cast<const C2 *>(L.work_y);
-    C2 *out = reinterpret_cast<C2 *>(L.target_expansions) + dst.target;
+    const CE *w = reinterpret_cast<const CE *>(L.work_y);
+    CE *out = reinterpret_cast<CE *>(L.target_expansions) + dst.target;
     const FmmLink *links = L.links + dst.first_link;
     const std::uint32_t n = dst.link_count;
     for (int e = threadIdx.x; e < nct; e += blockDim.x) {
         // the sum runs in link order; the loads of four links are issued together
-        C2 s{0, 0};
+        CE s{0, 0};
         std::uint32_t i = 0;
         for (; i + 4 <= n; i += 4) {
-            const C2 a = w[links[i].slot + e], b = w[links[i + 1].slot + e];
-            const C2 c = w[links[i + 2].slot + e], d = w[links[i + 3].slot + e];
+            const CE a = w[links[i].slot + e], b = w[links[i + 1].slot + e];
+            const CE c = w[links[i + 2].slot + e], d = w[links[i + 3].slot + e];
             s = s + a;
             s = s + b;
             s = s + c;
@@ -1250,15 +1276,46 @@ void launch_leaf_moments(const FmmLeafMomentsLaunch &launch, hipStream_t stream)
 }
 
 void launch_leaf_emit(const FmmLeaf *leaves, std::uint32_t leaf_count, const std::uint32_t *leaf_dofs,
-                      const double *emission, const double *p, double *expansions, hipStream_t stream) {
-    if (leaf_count)
-        hipLaunchKernelGGL(leaf_emit, dim3(leaf_count), dim3(256), 0, stream, leaves, leaf_dofs, emission, p, expansions);
+                      const double *emission, const double *p, double *expansions, hipStream_t stream, bool fp32) {
+    if (!leaf_count)
+        return;
+    if (fp32)
+        hipLaunchKernelGGL(leaf_emit<float>, dim3(leaf_count), dim3(256), 0, stream, leaves, leaf_dofs, emission, p,
+                           reinterpret_cast<float *>(expansions));
+    else
+        hipLaunchKernelGGL(leaf_emit<double>, dim3(leaf_count), dim3(256), 0, stream, leaves, leaf_dofs, emission, p,
+                           expansions);
 }
 
 void launch_leaf_test(const FmmLeaf *leaves, std::uint32_t leaf_count, const std::uint32_t *leaf_dofs,
-                      const double *testing, const double *expansions, double *y, hipStream_t stream) {
-    if (leaf_count)
-        hipLaunchKernelGGL(leaf_test, dim3(leaf_count), dim3(256), 0, stream, leaves, leaf_dofs, testing, expansions, y);
+                      const double *testing, const double *expansions, double *y, hipStream_t stream, bool fp32) {
+    if (!leaf_count)
+        return;
+    if (fp32)
+        hipLaunchKernelGGL(leaf_test<float>, dim3(leaf_count), dim3(256), 0, stream, leaves, leaf_dofs, testing,
+                           reinterpret_cast<const float *>(expansions), y);
+    else
+        hipLaunchKernelGGL(leaf_test<double>, dim3(leaf_count), dim3(256), 0, stream, leaves, leaf_dofs, testing,
+                           expansions, y);
+}
+
+// The order-templated translations of one launch (orders up to kMidFmmOrder) in scalar E.
+template <class E>
+void launch_links(const FmmTranslateLaunch &launch, std::uint32_t max_order, dim3 rot_grid, dim3 axial_grid,
+                  hipStream_t stream) {
+    if (max_order > kMaxFmmOrder) {
+        hipLaunchKernelGGL((rotate_links<kMidFmmOrder, E>), rot_grid, dim3(256), 0, stream, launch, false);
+        hipLaunchKernelGGL((coaxial_links<kMidFmmOrder, E>), axial_grid, dim3(256), 0, stream, launch);
+        hipLaunchKernelGGL((rotate_links<kMidFmmOrder, E>), rot_grid, dim3(256), 0, stream, launch, true);
+    } else if (max_order > kSmallFmmOrder) {
+        hipLaunchKernelGGL((rotate_links<kMaxFmmOrder, E>), rot_grid, dim3(256), 0, stream, launch, false);
+        hipLaunchKernelGGL((coaxial_links<kMaxFmmOrder, E>), axial_grid, dim3(256), 0, stream, launch);
+        hipLaunchKernelGGL((rotate_links<kMaxFmmOrder, E>), rot_grid, dim3(256), 0, stream, launch, true);
+    } else {
+        hipLaunchKernelGGL((rotate_links<kSmallFmmOrder, E>), rot_grid, dim3(256), 0, stream, launch, false);
+        hipLaunchKernelGGL((coaxial_links<kSmallFmmOrder, E>), axial_grid, dim3(256), 0, stream, launch);
+        hipLaunchKernelGGL((rotate_links<kSmallFmmOrder, E>), rot_grid, dim3(256), 0, stream, launch, true);
+    }
 }
 
 void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destination_count, std::uint32_t link_count,
@@ -1274,6 +1331,8 @@ void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destinatio
         return chunks && split_target ? std::max(1u, std::min(stages, (split_target + chunks - 1) / chunks)) : 1u;
     };
     const dim3 rot_grid(chunk_count, splits(chunk_count)), axial_grid(axial_chunk_count, splits(axial_chunk_count));
+    if (launch.fp32 && link_count && max_order > kMidFmmOrder)
+        throw std::invalid_argument("FP32 translations take orders up to 48 only");
     if (link_count && max_order > kMaxWideOrder) {
         hipLaunchKernelGGL(rotate_links_big, dim3(chunk_count), dim3(256), 0, stream, launch, false);
         hipLaunchKernelGGL(coaxial_links_big, dim3(axial_chunk_count), dim3(256), 0, stream, launch);
@@ -1282,20 +1341,15 @@ void launch_translate(const FmmTranslateLaunch &launch, std::uint32_t destinatio
         hipLaunchKernelGGL(rotate_links_wide, dim3(chunk_count), dim3(256), 0, stream, launch, false);
         hipLaunchKernelGGL(coaxial_links_wide, dim3(axial_chunk_count), dim3(256), 0, stream, launch);
         hipLaunchKernelGGL(rotate_links_wide, dim3(chunk_count), dim3(256), 0, stream, launch, true);
-    } else if (link_count && max_order > kMaxFmmOrder) {
-        hipLaunchKernelGGL(rotate_links<kMidFmmOrder>, rot_grid, dim3(256), 0, stream, launch, false);
-        hipLaunchKernelGGL(coaxial_links<kMidFmmOrder>, axial_grid, dim3(256), 0, stream, launch);
-        hipLaunchKernelGGL(rotate_links<kMidFmmOrder>, rot_grid, dim3(256), 0, stream, launch, true);
-    } else if (link_count && max_order > kSmallFmmOrder) {
-        hipLaunchKernelGGL(rotate_links<kMaxFmmOrder>, rot_grid, dim3(256), 0, stream, launch, false);
-        hipLaunchKernelGGL(coaxial_links<kMaxFmmOrder>, axial_grid, dim3(256), 0, stream, launch);
-        hipLaunchKernelGGL(rotate_links<kMaxFmmOrder>, rot_grid, dim3(256), 0, stream, launch, true);
+    } else if (link_count && launch.fp32) {
+        launch_links<float>(launch, max_order, rot_grid, axial_grid, stream);
     } else if (link_count) {
-        hipLaunchKernelGGL(rotate_links<kSmallFmmOrder>, rot_grid, dim3(256), 0, stream, launch, false);
-        hipLaunchKernelGGL(coaxial_links<kSmallFmmOrder>, axial_grid, dim3(256), 0, stream, launch);
-        hipLaunchKernelGGL(rotate_links<kSmallFmmOrder>, rot_grid, dim3(256), 0, stream, launch, true);
+        launch_links<double>(launch, max_order, rot_grid, axial_grid, stream);
     }
-    hipLaunchKernelGGL(gather_links, dim3(destination_count), dim3(256), 0, stream, launch);
+    if (launch.fp32)
+        hipLaunchKernelGGL(gather_links<float>, dim3(destination_count), dim3(256), 0, stream, launch);
+    else
+        hipLaunchKernelGGL(gather_links<double>, dim3(destination_count), dim3(256), 0, stream, launch);
 }
 
 void launch_gather_blocks(const std::uint32_t *row_offsets, const std::uint32_t *columns, const double *values,

@@ -95,6 +95,9 @@ struct FmmTranslateLaunch {
     const std::uint64_t *axial_offsets; // complex offset of each plan
     const double *source_expansions;
     double *target_expansions;
+    // FP32-bulk form: work_y, work_z and the expansions hold float pairs (orders up to
+    // 48; the tables stay FP64 and are rounded when staged)
+    bool fp32{};
 };
 
 // Far part of B q by direct pair quadrature over the flux panels (original numbering).
@@ -248,10 +251,13 @@ void launch_flux_csr_apply(const std::uint32_t *offsets, const std::uint32_t *co
 // Per vector of length n: out[map[i]] = in[i] (forward) or out[i] = in[map[i]].
 void launch_permute(const double *in, double *out, const std::uint32_t *map, std::uint32_t n, std::uint32_t count,
                     bool forward, hipStream_t stream);
+// fp32: the expansions hold float pairs (the FP32-bulk form); the sums run in FP64.
 void launch_leaf_emit(const FmmLeaf *leaves, std::uint32_t leaf_count, const std::uint32_t *leaf_dofs,
-                      const double *emission, const double *p, double *expansions, hipStream_t stream);
+                      const double *emission, const double *p, double *expansions, hipStream_t stream,
+                      bool fp32 = false);
 void launch_leaf_test(const FmmLeaf *leaves, std::uint32_t leaf_count, const std::uint32_t *leaf_dofs,
-                      const double *testing, const double *expansions, double *y, hipStream_t stream);
+                      const double *testing, const double *expansions, double *y, hipStream_t stream,
+                      bool fp32 = false);
 // max_order: largest source or target order of the launch's links and rotation plans
 // (selects the tiled kernels above kMaxFmmOrder and the reduction-tiled ones above
 // kMaxWideOrder; at most kMaxBigOrder).

@@ -1383,8 +1383,6 @@ std::shared_ptr<PreparedRequest> MeshSolver::prepare(const Request &req) const {
          num.fmm_block_width < 32 || num.fmm_block_width > 2048 ||
          (num.fmm_block_width & (num.fmm_block_width - 1)) != 0))
         throw std::invalid_argument("invalid Fast Multipole or GMRES settings (leaf capacity, separation, restart, iterations or block width)");
-    if (num.acceleration == 1 && num.precision != 0)
-        throw std::invalid_argument("The fast multipole form runs in FP64 (precision must be FP64).");
     if (req.excitations.size() > 1024 || req.observers.size() > 100000)
         throw std::invalid_argument("the request is too large for 32-bit indices");
     static std::atomic<std::uint64_t> next_id{1};

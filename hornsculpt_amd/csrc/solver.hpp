@@ -140,6 +140,7 @@ struct FrequencyResult {
     // whether the defect correction with the finer operator ran, and the operator's apply time
     double fmm_audit{}, fmm_apply_ms{};
     bool fmm_corrected{}, fmm_direct_audit{};
+    bool fmm_fp32{}; // the Krylov iterations applied the operator with FP32 translations
     double fmm_quadrature_discrepancy{}, fmm_verification_seed{};
     unsigned fmm_endpoint_degree{}, fmm_promotions{};
     std::uint64_t fmm_plane_wave_levels{}, fmm_m2l_links{}, fmm_near_nnz{}, fmm_device_bytes{};

@@ -242,7 +242,7 @@ inline std::complex<double> at(const std::vector<double> &v, std::size_t i) { re
 enum FmmEvidence : int {
     kAuditDiscrepancy = 0, kApplyMs = 1, kCorrected = 2, kPlaneWaveLevels = 3, kM2lLinks = 4, kNearEntries = 5,
     kDeviceBytes = 6, kDirectAudit = 7, kQuadratureDiscrepancy = 8, kEndpointDegree = 9, kPromotions = 10,
-    kVerificationSeed = 11, kFmmEvidenceCount = 12
+    kVerificationSeed = 11, kFp32Translations = 12, kFmmEvidenceCount = 13
 };
 } // namespace hsbem_detail
 

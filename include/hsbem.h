@@ -187,14 +187,16 @@ HSBEM_API int hsbem_result_residual(const hsbem_result *result, uint32_t frequen
 HSBEM_API int hsbem_result_timings(const hsbem_result *result, uint32_t frequency, double *milliseconds);
 HSBEM_API int hsbem_result_peak_device_bytes(const hsbem_result *result, uint64_t *bytes);
 /* Fast multipole evidence of a frequency point (BemFrequencyResult's FMM statistics,
- * bem.hpp:193-332; zeros for the dense form): out[12] = seed-audit discrepancy
+ * bem.hpp:193-332; zeros for the dense form): out[13] = seed-audit discrepancy
  * (relative; 0 under the direct audit), apply
  * milliseconds, defect correction applied (0/1), plane-wave levels, M2L links, near
  * CSR entries, operator device bytes, residual evaluated by the direct Galerkin
  * audit (0/1), endpoint-quadrature discrepancy against the next panel rule (A and B
  * actions, relative), endpoint rule degree, endpoint-rule promotions, and (out[11])
- * the accuracy seed of the verification operator (0 under the direct audit). The seed
- * audit discrepancy is that of the verification operator against the audit one. */
+ * the accuracy seed of the verification operator (0 under the direct audit), and
+ * (out[12]) whether the Krylov iterations applied FP32 translations (0/1: precision
+ * mixed, the FP32-bulk form, on operators with spherical levels of order <= 48 only).
+ * The seed audit discrepancy is that of the verification operator against the audit one. */
 HSBEM_API int hsbem_result_fmm(const hsbem_result *result, uint32_t frequency, double *out);
 
 /* ---- serving loop (no counterpart class in the reference: its callers run
@@ -354,6 +356,12 @@ HSBEM_API int hsbem_fmm_operator_apply(hsbem_fmm_operator *op, const double *x, 
  * (SupportDirectAudit::applyPressureDevice, support_direct_audit.hpp:11-22); meshes
  * of at most 20000 panels (HSBEM_RUNTIME_ERROR above). */
 HSBEM_API int hsbem_fmm_operator_apply_direct(hsbem_fmm_operator *op, const double *x, double *y);
+/* y = A x in the FP32-bulk form: expansions, rotations, coaxial products and gathers in
+ * FP32 (the near part and the emission and testing sums in FP64); operators with
+ * spherical levels of order <= 48 only (else HSBEM_INVALID_ARGUMENT). No counterpart
+ * entry point in the reference (its FP32-bulk form is internal to the request solve);
+ * for callers checking that form's action. */
+HSBEM_API int hsbem_fmm_operator_apply_fp32(hsbem_fmm_operator *op, const double *x, double *y);
 HSBEM_API int hsbem_fmm_operator_precondition(hsbem_fmm_operator *op, const double *v, double *z);
 /* Restarted GMRES on A x = b with the FMM operator (DeviceGmresSolver::solve,
  * modules/bem/cuda/device_gmres.hpp:11-35), right block-Jacobi when precondition != 0. */
@@ -362,6 +370,8 @@ HSBEM_API int hsbem_fmm_operator_solve(hsbem_fmm_operator *op, const double *b, 
                                        double *achieved, uint32_t *iterations);
 /* Average device time of one apply over repetitions (HIP events, context stream). */
 HSBEM_API int hsbem_fmm_operator_time_apply(hsbem_fmm_operator *op, uint32_t repetitions, double *milliseconds);
+/* The same timing of hsbem_fmm_operator_apply_fp32 (no counterpart in the reference). */
+HSBEM_API int hsbem_fmm_operator_time_apply_fp32(hsbem_fmm_operator *op, uint32_t repetitions, double *milliseconds);
 /* counts[16] = nodes, levels, leaves, top level, rule points, near nnz, far pairs,
  * M2M links, M2L links, L2L links, rotations, coaxial plans, expansion values,
  * moment values, device bytes, nv; times[5] = tree, near, plans, moments, near

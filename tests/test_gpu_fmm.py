@@ -228,14 +228,54 @@ def test_fmm_request_matches_dense_request():
         assert ff.fmm["direct_audit"] or (ff.fmm["verification_seed"] <= 1e-9 and ff.fmm["audit_discrepancy"] <= 0.25e-6)
 
 
+def test_fmm_request_fp32_bulk_form():
+    """precision=mixed with the fast multipole form: the Krylov iterations apply FP32
+    translations, the residual (FP64 operator), audits and corrections are those of the
+    plain form, so the solution holds the same tolerance and matches the plain FMM and
+    the dense solves to it."""
+    freqs = [400.0, 2500.0]
+    plain = bem.BemNumerics(solver=bem.SOLVER_GMRES, relative_residual_tolerance=1e-6,
+                            acceleration=bem.ACCELERATION_FAST_MULTIPOLE)
+    bulk = bem.BemNumerics(solver=bem.SOLVER_GMRES, relative_residual_tolerance=1e-6,
+                           acceleration=bem.ACCELERATION_FAST_MULTIPOLE, precision=bem.PRECISION_MIXED)
+    dense = bem.BemNumerics(solver=bem.SOLVER_GMRES, relative_residual_tolerance=1e-8)
+    mesh, rp = _horn_request(freqs, plain)
+    _, rb = _horn_request(freqs, bulk)
+    _, rd = _horn_request(freqs, dense)
+    solver = bem.HipBemSolver(mesh)
+    res_p, res_b, res_d = solver.solve(rp), solver.solve(rb), solver.solve(rd)
+    for fp, fb, fd in zip(res_p.frequencies, res_b.frequencies, res_d.frequencies):
+        ep, eb, ed = fp.excitations[0], fb.excitations[0], fd.excitations[0]
+        vs_plain = _rel(eb.boundary.pressure_trace, ep.boundary.pressure_trace)
+        vs_dense = _rel(eb.boundary.pressure_trace, ed.boundary.pressure_trace)
+        print(f"{fb.frequency:.0f} Hz: FP32-bulk vs plain FMM {vs_plain:.2e}, vs dense {vs_dense:.2e}, residual "
+              f"{eb.boundary_residual:.1e}, iterations {eb.iterations} (plain {ep.iterations}), corrected "
+              f"{fb.fmm['corrected']}, solve {fb.solve_milliseconds:.0f} ms (plain {fp.solve_milliseconds:.0f} ms)")
+        assert fb.fmm["fp32_translations"] and not fp.fmm["fp32_translations"]
+        assert eb.boundary_residual <= 1e-6
+        assert vs_plain <= 1e-5 and vs_dense <= 1e-5
+
+
+def test_fmm_fp32_apply_matches_fp64_apply():
+    """The FP32-bulk action agrees with the FP64 one to FP32 rounding of the
+    expansions (1e-5 relative here)."""
+    mesh = meshes.exponential_horn(0.012)
+    k = 2 * math.pi * 2000.0 / 343.0
+    solver = bem.HipBemSolver(mesh)
+    op = solver.fmm_operator(bem.BemNumerics(), k, k, leaf_capacity=32)
+    rng = np.random.default_rng(17)
+    x = rng.standard_normal(op.size) + 1j * rng.standard_normal(op.size)
+    y64, y32 = op.apply(x), op.apply_fp32(x)
+    err = _rel(y32, y64)
+    print(f"FP32-bulk apply vs FP64 apply {err:.2e}")
+    assert err <= 1e-5
+    op.close()
+
+
 def test_fmm_request_validation():
     mesh, req = _horn_request([500.0], bem.BemNumerics(acceleration=bem.ACCELERATION_FAST_MULTIPOLE,
-                                                       precision=bem.PRECISION_MIXED))
+                                                       fmm_leaf_capacity=2))
     solver = bem.HipBemSolver(mesh)
-    with pytest.raises(ValueError, match="FP64"):
-        solver.solve(req)
-    _, req = _horn_request([500.0], bem.BemNumerics(acceleration=bem.ACCELERATION_FAST_MULTIPOLE,
-                                                    fmm_leaf_capacity=2))
     with pytest.raises(ValueError, match="Fast Multipole"):
         solver.solve(req)
     for width in (16, 96, 4096):  # preconditionerBlockWidth: 32, 64, ..., 2048 (cuda_solver.cu:1063-1066)
