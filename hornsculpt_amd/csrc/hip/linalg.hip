@@ -480,21 +480,22 @@ __global__ __launch_bounds__(64) void lu_trsm(double *A, std::uint32_t n, std::u
     }
 }
 
-// A22 -= L21 U12 over rows [k0 + nb, n) and columns [k0 + nb, c_end), K = nb <= 32,
-// on the FP64 matrix cores: 256 threads per 64 x 64 complex output
-// tile, each wavefront a 32 x 32 quarter as 2 x 2 blocks of 16 x 16. The product is
-// formed transposed, D = U^T L^T (v_mfma_f64_16x16x4f64: A[i][k] from lane i + 16 k,
-// B[k][j] from lane j + 16 k, D[i][j] in lane j + 16 (i mod 4), register i / 4), so
-// D's index j runs over rows and each 16-lane group writes 16 consecutive rows of one
-// column. A complex MAC is four real MFMAs: Re += Ur Lr - Ui Li, Im += Ui Lr + Ur Li.
-// K is padded with zeros to a multiple of 4.
+// C -= L U over rows [r_begin, r_end) and columns [c_begin, c_end) with L = A[rows][k0 ..
+// k0 + nb) and U = A[k0 .. k0 + nb)[columns], K = nb <= 2 kPanel (staged through LDS in chunks
+// of kPanel, the accumulators carried across the chunks), on the FP64 matrix cores: 256
+// threads per 64 x 64 complex output tile, each wavefront a 32 x 32 quarter as 2 x 2 blocks
+// of 16 x 16. The product is formed transposed, D = U^T L^T (v_mfma_f64_16x16x4f64: A[i][k]
+// from lane i + 16 k, B[k][j] from lane j + 16 k, D[i][j] in lane j + 16 (i mod 4), register
+// i / 4), so D's index j runs over rows and each 16-lane group writes 16 consecutive rows of
+// one column. A complex MAC is four real MFMAs: Re += Ur Lr - Ui Li, Im += Ui Lr + Ur Li.
+// Each chunk's K is padded with zeros to a multiple of 4.
 typedef double lu_v4d __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void lu_update_mfma(double *A, std::uint32_t n, std::uint32_t k0, std::uint32_t nb,
-                                                      std::uint32_t c_end) {
-    __shared__ double2 Ls[kPanel][65]; // Ls[k][q] = A[r0 + q][k0 + k] (padded row: no bank conflicts across k)
-    __shared__ double2 Us[kPanel][65]; // Us[k][q] = A[k0 + k][c0 + q]
-    const std::uint32_t base = k0 + nb, r0 = base + blockIdx.x * 64, c0 = base + blockIdx.y * 64;
-    const std::uint32_t kpad = (nb + 3) & ~3u;
+                                                      std::uint32_t r_begin, std::uint32_t r_end,
+                                                      std::uint32_t c_begin, std::uint32_t c_end) {
+    __shared__ double2 Ls[kPanel][65]; // Ls[k][q] = A[r0 + q][k0 + kc + k] (padded row: no bank conflicts across k)
+    __shared__ double2 Us[kPanel][65]; // Us[k][q] = A[k0 + kc + k][c0 + q]
+    const std::uint32_t r0 = r_begin + blockIdx.x * 64, c0 = c_begin + blockIdx.y * 64;
     const double2 *A2 = reinterpret_cast<const double2 *>(A);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, i = lane & 15, kq = lane >> 4;
     const int rh = wave & 1, ch = wave >> 1;
@@ -507,50 +508,56 @@ __global__ __launch_bounds__(256) void lu_update_mfma(double *A, std::uint32_t n
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const std::uint32_t r = r0 + rh * 32 + a * 16 + i, c = c0 + ch * 32 + b * 16 + 4 * v + kq;
-                cold[a][b][v] = r < n && c < c_end ? A2[static_cast<std::size_t>(c) * n + r] : double2{0.0, 0.0};
+                cold[a][b][v] = r < r_end && c < c_end ? A2[static_cast<std::size_t>(c) * n + r] : double2{0.0, 0.0};
             }
-    for (std::uint32_t e = threadIdx.x; e < kpad * 64; e += 256) {
-        const std::uint32_t kk = e / 64, q = e % 64, r = r0 + q, c = c0 + q;
-        Ls[kk][q] = kk < nb && r < n ? A2[(static_cast<std::size_t>(k0) + kk) * n + r] : double2{0.0, 0.0};
-        Us[kk][q] = kk < nb && c < c_end ? A2[static_cast<std::size_t>(c) * n + k0 + kk] : double2{0.0, 0.0};
-    }
-    __syncthreads();
     lu_v4d accr[2][2], acci[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b)
             accr[a][b] = acci[a][b] = lu_v4d{0.0, 0.0, 0.0, 0.0};
-    for (std::uint32_t ks = 0; ks < kpad / 4; ++ks) {
-        const std::uint32_t kk = 4 * ks + kq;
-        double lr[2], li[2], ur[2], ui[2];
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            const double2 l = Ls[kk][rh * 32 + a * 16 + i];
-            lr[a] = l.x;
-            li[a] = l.y;
+    for (std::uint32_t kc = 0; kc < nb; kc += kPanel) {
+        const std::uint32_t kn = min(static_cast<std::uint32_t>(kPanel), nb - kc), kpad = (kn + 3) & ~3u;
+        if (kc)
+            __syncthreads(); // the previous chunk's reads are done
+        for (std::uint32_t e = threadIdx.x; e < kpad * 64; e += 256) {
+            const std::uint32_t kk = e / 64, q = e % 64, r = r0 + q, c = c0 + q;
+            const std::size_t k = static_cast<std::size_t>(k0) + kc + kk;
+            Ls[kk][q] = kk < kn && r < r_end ? A2[k * n + r] : double2{0.0, 0.0};
+            Us[kk][q] = kk < kn && c < c_end ? A2[static_cast<std::size_t>(c) * n + k] : double2{0.0, 0.0};
         }
+        __syncthreads();
+        for (std::uint32_t ks = 0; ks < kpad / 4; ++ks) {
+            const std::uint32_t kk = 4 * ks + kq;
+            double lr[2], li[2], ur[2], ui[2];
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const double2 u = Us[kk][ch * 32 + b * 16 + i];
-            ur[b] = u.x;
-            ui[b] = u.y;
-        }
-        // four passes over the eight independent accumulators: no MFMA waits on the one before it
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[b], lr[a], accr[a][b], 0, 0, 0);
-                acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ui[b], lr[a], acci[a][b], 0, 0, 0);
+            for (int a = 0; a < 2; ++a) {
+                const double2 l = Ls[kk][rh * 32 + a * 16 + i];
+                lr[a] = l.x;
+                li[a] = l.y;
             }
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
             for (int b = 0; b < 2; ++b) {
-                accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ui[b], li[a], accr[a][b], 0, 0, 0);
-                acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[b], li[a], acci[a][b], 0, 0, 0);
+                const double2 u = Us[kk][ch * 32 + b * 16 + i];
+                ur[b] = u.x;
+                ui[b] = u.y;
             }
+            // four passes over the eight independent accumulators: no MFMA waits on the one before it
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[b], lr[a], accr[a][b], 0, 0, 0);
+                    acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ui[b], lr[a], acci[a][b], 0, 0, 0);
+                }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ui[b], li[a], accr[a][b], 0, 0, 0);
+                    acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[b], li[a], acci[a][b], 0, 0, 0);
+                }
+        }
     }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -559,7 +566,7 @@ __global__ __launch_bounds__(256) void lu_update_mfma(double *A, std::uint32_t n
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const std::uint32_t r = r0 + rh * 32 + a * 16 + i, c = c0 + ch * 32 + b * 16 + 4 * v + kq;
-                if (r < n && c < c_end)
+                if (r < r_end && c < c_end)
                     reinterpret_cast<double2 *>(A)[static_cast<std::size_t>(c) * n + r] =
                         double2{cold[a][b][v].x - accr[a][b][v], cold[a][b][v].y - acci[a][b][v]};
             }
@@ -1257,30 +1264,64 @@ void launch_residual(const void *A, bool fp32, std::uint32_t n, const double *b,
 
 std::size_t lu_work_doubles(std::uint32_t) { return 0; }
 
+namespace {
+
+// The leaves of panel [k0, k0 + nb): pivoted 4-column leaves (row interchanges inside the
+// panel), each leaf's U rows of the panel and the rank-4 update of the panel below it.
+void factor_panel(double *A, std::uint32_t n, std::uint32_t k0, std::uint32_t nb, std::int32_t *piv, int *info,
+                  hipStream_t stream) {
+    const std::uint32_t pe = k0 + nb;
+    for (std::uint32_t j0 = k0; j0 < pe; j0 += kLeaf) {
+        const std::uint32_t w = std::min<std::uint32_t>(kLeaf, pe - j0);
+        hipLaunchKernelGGL(lu_leaf, dim3(1), dim3(kPanelThreads), 0, stream, A, n, k0, j0, w, pe, piv, info);
+        if (j0 + w >= pe)
+            continue;
+        hipLaunchKernelGGL(lu_trsm, dim3(1), dim3(64), 0, stream, A, n, j0, w, pe);
+        const std::uint32_t below = n - j0 - w;
+        hipLaunchKernelGGL(lu_update_mfma, dim3((below + 63) / 64, 1), dim3(256), 0, stream, A, n, j0, w, j0 + w, n,
+                           j0 + w, pe);
+    }
+}
+
+// C -= L U over rows [r0, r1) x columns [c0, c1) with K = A's columns / rows [k0, k0 + nb).
+void update(double *A, std::uint32_t n, std::uint32_t k0, std::uint32_t nb, std::uint32_t r0, std::uint32_t r1,
+            std::uint32_t c0, std::uint32_t c1, hipStream_t stream) {
+    if (r1 > r0 && c1 > c0)
+        hipLaunchKernelGGL(lu_update_mfma, dim3((r1 - r0 + 63) / 64, (c1 - c0 + 63) / 64), dim3(256), 0, stream, A, n,
+                           k0, nb, r0, r1, c0, c1);
+}
+
+} // namespace
+
 // Right-looking blocked LU with partial pivoting: panels of kPanel columns, each
 // factored in leaves of kLeaf columns (lu_leaf on one workgroup, then the leaf's
 // rows of U for the rest of the panel and a rank-kLeaf update of the panel below
-// it on every CU), then the panel's interchanges outside it, U12 and the
-// rank-kPanel trailing update on the matrix cores.
+// it on every CU), then the panel's interchanges outside it, U12 and the trailing
+// update on the matrix cores, two panels to one rank-2 kPanel trailing update.
 void lu_factor(double *A, std::uint32_t n, std::int32_t *piv, int *info, double *, hipStream_t stream) {
-    for (std::uint32_t k0 = 0; k0 < n; k0 += kPanel) {
-        const std::uint32_t nb = std::min<std::uint32_t>(kPanel, n - k0), pe = k0 + nb;
-        for (std::uint32_t j0 = k0; j0 < pe; j0 += kLeaf) {
-            const std::uint32_t w = std::min<std::uint32_t>(kLeaf, pe - j0);
-            hipLaunchKernelGGL(lu_leaf, dim3(1), dim3(kPanelThreads), 0, stream, A, n, k0, j0, w, pe, piv, info);
-            if (j0 + w >= pe)
-                continue;
-            hipLaunchKernelGGL(lu_trsm, dim3(1), dim3(64), 0, stream, A, n, j0, w, pe);
-            const std::uint32_t below = n - j0 - w;
-            hipLaunchKernelGGL(lu_update_mfma, dim3((below + 63) / 64, 1), dim3(256), 0, stream, A, n, j0, w, pe);
-        }
-        hipLaunchKernelGGL(lu_swap, dim3((n + 63) / 64), dim3(64), 0, stream, A, n, k0, nb, piv);
-        const std::uint32_t rest = n - pe;
-        if (!rest)
-            continue;
-        hipLaunchKernelGGL(lu_trsm, dim3((rest + 63) / 64), dim3(64), 0, stream, A, n, k0, nb, n);
-        const std::uint32_t tiles = (rest + 63) / 64;
-        hipLaunchKernelGGL(lu_update_mfma, dim3(tiles, tiles), dim3(256), 0, stream, A, n, k0, nb, n);
+    // Pairs of kPanel-column panels P1 = [k0, e1), P2 = [e1, e2) share one rank-2 kPanel
+    // trailing update (half the trailing matrix's HBM traffic per flop of rank-kPanel
+    // updates): P1 is factored, its interchanges applied and its U rows solved over every
+    // later column; P1's update reaches P2's columns before P2 is factored; after P2's
+    // interchanges (applied to the still un-updated trailing rows and to P1's L rows alike)
+    // P1's update reaches P2's U rows, P2's U rows are solved, and the rows below P2 take
+    // both panels' updates at once.
+    for (std::uint32_t k0 = 0; k0 < n; k0 += 2 * kPanel) {
+        const std::uint32_t n1 = std::min<std::uint32_t>(kPanel, n - k0), e1 = k0 + n1;
+        factor_panel(A, n, k0, n1, piv, info, stream);
+        hipLaunchKernelGGL(lu_swap, dim3((n + 63) / 64), dim3(64), 0, stream, A, n, k0, n1, piv);
+        if (e1 == n)
+            break;
+        hipLaunchKernelGGL(lu_trsm, dim3((n - e1 + 63) / 64), dim3(64), 0, stream, A, n, k0, n1, n);
+        const std::uint32_t n2 = std::min<std::uint32_t>(kPanel, n - e1), e2 = e1 + n2;
+        update(A, n, k0, n1, e1, n, e1, e2, stream);
+        factor_panel(A, n, e1, n2, piv, info, stream);
+        hipLaunchKernelGGL(lu_swap, dim3((n + 63) / 64), dim3(64), 0, stream, A, n, e1, n2, piv);
+        if (e2 == n)
+            break;
+        update(A, n, k0, n1, e1, e2, e2, n, stream);
+        hipLaunchKernelGGL(lu_trsm, dim3((n - e2 + 63) / 64), dim3(64), 0, stream, A, n, e1, n2, n);
+        update(A, n, k0, n1 + n2, e2, n, e2, n, stream);
     }
 }
 

@@ -204,7 +204,7 @@ def test_gmres_manufactured_system():
     assert 4 < its < 120 and err < 2e-8 and res < 1e-8
 
 
-@pytest.mark.parametrize("n,nrhs", [(1, 1), (31, 2), (64, 1), (97, 3), (300, 2), (1234, 2)])
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (31, 2), (64, 1), (65, 1), (97, 3), (129, 2), (300, 2), (1234, 2)])
 def test_dense_lu_matches_numpy(n, nrhs):
     rng = np.random.default_rng(n)
     A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
