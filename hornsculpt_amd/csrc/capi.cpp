@@ -608,8 +608,8 @@ int hsbem_plan_build(const double *vertices, uint32_t nv, const uint32_t *triang
         q.near_tolerance = num.near_tolerance;
         q.near_max_depth = num.near_depth;
         auto p = std::make_unique<hsbem_plan>();
-        // HSBEM_PLAN_BATCHED=1: the batched refinement driver on the host estimator
-        // (the same plan; a check of the driver the device estimator feeds)
+        // HSBEM_PLAN_BATCHED=1: the batched refinement loop on the host estimator (the same
+        // plan; a check of the loop the device estimator feeds)
         const char *batched = std::getenv("HSBEM_PLAN_BATCHED");
         if (batched && std::string(batched) == "1") {
             const auto host = hsbem::host_patch_estimator(space, k_max, num.plan_threads);

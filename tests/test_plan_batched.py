@@ -1,7 +1,7 @@
-"""The batched refinement driver of the interaction plan (the one the device patch
-estimator feeds: every round estimates the pending patches of all unfinished near
+"""The batched refinement loop of the interaction plan (the one the device patch
+estimator feeds: every pass estimates the pending patches of all unfinished near
 pairs in one call) gives the same plan as refining each pair on its own, when both
-take the host estimates (HSBEM_PLAN_BATCHED=1 switches the host plan to the driver).
+take the host estimates (HSBEM_PLAN_BATCHED=1 switches the host plan to the loop).
 Covers tensor refinement, degree promotion, projected fans and the all-pairs mode."""
 import math
 
@@ -13,7 +13,7 @@ from test_host import PLAN_CASES
 
 
 @pytest.mark.parametrize("name,make,frequency,opts", PLAN_CASES, ids=[c[0] for c in PLAN_CASES])
-def test_batched_driver_gives_the_same_plan(monkeypatch, name, make, frequency, opts):
+def test_batched_loop_gives_the_same_plan(monkeypatch, name, make, frequency, opts):
     mesh = make()
     k = 2 * math.pi * frequency / 343.0
     num = bem.BemNumerics(near_relative_tolerance=opts.get("tolerance", 1e-4),
@@ -29,7 +29,7 @@ def test_batched_driver_gives_the_same_plan(monkeypatch, name, make, frequency, 
         assert batched[key] == direct[key], key
 
 
-def test_batched_driver_reports_the_failing_pair(monkeypatch):
+def test_batched_loop_reports_the_failing_pair(monkeypatch):
     """A pair that cannot reach its tolerance within the depth limit is refused with
     the pair's identity, as in the per-pair path."""
     mesh = meshes.icosphere(2)
