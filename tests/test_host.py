@@ -221,3 +221,16 @@ def test_progressive_requests_share_the_sweep_raster():
         assert r.frequencies == [freqs[i]] and r.interaction_plan_frequencies == freqs
     req.interaction_plan_frequencies = freqs + [9000.0]
     assert all(r.interaction_plan_frequencies == freqs + [9000.0] for _, r in bem.progressive_requests(req))
+
+
+def test_tuning_report_before_any_tuning():
+    """No launch-shape decision without a device: the report is an empty JSON array; a
+    short buffer receives a truncated, terminated copy and `needed` the full size."""
+    import ctypes
+    assert bem.tuning_report() == []
+    lib = bem.library()
+    needed = ctypes.c_size_t(0)
+    buf = ctypes.create_string_buffer(2)
+    assert lib.hsbem_tuning_report(buf, 2, ctypes.byref(needed)) == 0
+    assert needed.value == 3 and buf.value == b"["
+    assert lib.hsbem_tuning_report(None, 0, None) != 0  # null buffer refused
